@@ -1,0 +1,166 @@
+"""Benchmark of the integrate -> assemble -> GMRES hot path on one MI355X.
+
+Workload (default): std-linear prism convection-diffusion on a 128^3-hex box
+split into 2 prisms per hex (4,194,304 prisms, 2,146,689 DOFs), interior
+nodes jittered by +-0.2h, per-element lognormal diffusion (mu=0, sigma=1) and
+uniform random velocity in [-1,1]^3, unit source, Dirichlet penalty on all
+boundary faces.  One step = FP64 integrate + assemble into CRS, Jacobi fill,
+and GMRES(50) with Jacobi to a 1e-8 relative preconditioned residual (the
+reference's stopping test).  Inputs are resident in device memory before the
+timed region.
+
+Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: the
+build uses one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
+FP64_PEAK_TFS = 78.6       # MI355X dense FP64, TFLOP/s
+METRIC = "elements integrated+assembled/s; solves/s & p99 latency at 1e-8 rel residual"
+
+
+def make_workload(n, seed):
+    import modfem_amd as mf
+    mesh = mf.Mesh.box_prism(n, n, n, seed=seed, jitter=0.2, bc=1)
+    rng = np.random.default_rng(seed + 1)
+    kappa = np.exp(rng.normal(0.0, 1.0, mesh.nel))
+    b = rng.uniform(-1.0, 1.0, (mesh.nel, 3))
+    cc = np.zeros(mf.NCOEF)
+    cc[mf.C_S] = 1.0
+    coef_el = np.concatenate([kappa, b[:, 0], b[:, 1], b[:, 2]])
+    return mesh, kappa, b, cc, coef_el
+
+
+def cpu_baseline(mesh, kappa, b, gpu_iters, krylbas, threads, sample_iters):
+    """oracle timed on the host: full integrate+assemble of the workload plus a
+    bounded GMRES sample, extrapolated to the GPU iteration count"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import modfem_amd as mf
+    from oracle_lib import OracleProblem, oracle
+    oracle().orc_set_threads(threads)
+    arr = mesh.arrays()
+    op = OracleProblem(arr)
+    coef = np.zeros((mesh.nel, 21))
+    coef[:, 0] = coef[:, 4] = coef[:, 8] = kappa
+    coef[:, 9:12] = b
+    coef[:, 19] = 1.0
+    t0 = time.perf_counter()
+    val, rhs = op.assemble(mf.IND_CONV_DIFF, coef)
+    t_ia = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, info = op.gmres(val, rhs, precon=1, krylbas=krylbas, max_iter=sample_iters, tol=1e-8)
+    t_it = (time.perf_counter() - t0) / max(info["iters"], 1)
+    t_step = t_ia + gpu_iters * t_it
+    return {
+        "value": mesh.nel / t_step, "unit": "elements/s", "cores": threads, "kind": "port",
+        "sample": (f"oracle (C, OpenMP) on the same mesh: full integrate+assemble "
+                   f"({t_ia:.2f} s) + {info['iters']} GMRES({krylbas}) iterations "
+                   f"({t_it * 1e3:.1f} ms/it) extrapolated to the GPU's {gpu_iters} iterations"),
+        "integrate_assemble_s": t_ia, "gmres_ms_per_iter": t_it * 1e3,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="hexes per box edge")
+    ap.add_argument("--krylbas", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=2017)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-iters", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import modfem_amd as mf
+    t_setup = time.perf_counter()
+    mesh, kappa, b, cc, coef_el = make_workload(args.n, args.seed)
+    p = mf.Problem.from_mesh(mesh)
+    p.set_coefficients(mf.IND_CONV_DIFF, mf.COEF_KAPPA_B, coef_const=cc, coef_el=coef_el)
+    t_setup = time.perf_counter() - t_setup
+
+    def step():
+        p.integrate_assemble(64)
+        x, info = p.gmres(precon=1, krylbas=args.krylbas, max_iter=20000, tol=1e-8)
+        return x, info
+
+    for _ in range(args.warmup):
+        step()
+    mf.gmres_kernel_stats(True)
+    lat = []
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        x, info = step()
+        lat.append(time.perf_counter() - ts)
+        infos.append(info)
+    t_total = time.perf_counter() - t0
+    ks = mf.gmres_kernel_stats(False)
+    true_rel = p.true_residual(x)
+    ms_int, ms_faces = p.bench_integrate(64, reps=5)
+    ms_spmv = p.bench_spmv(reps=20)
+
+    nel, n, nnz = p.nel, p.n, p.nnz
+    iters = infos[-1]["iters"]
+    n_op, t_op, b_op = ks["op_dots"]
+    n_up, t_up, b_up = ks["update"]
+    ach_op = b_op / (t_op * 1e-3) / 1e9 if t_op > 0 else 0.0
+    # integrate+assemble: R's FLOP model (aps_std_lin_intf.c:1847,2200,2266,2813):
+    # G*(366 + 38*ns^2 + 3*ns) with convection, G=6, ns=6
+    f_int = nel * 6 * (366 + 38 * 36 + 3 * 6)
+    b_int = 8.0 * nnz + 8.0 * n + 24.0 * n + 4.0 * 6 * nel + 8.0 * 4 * nel + 36.0 * nel
+    out = {
+        "metric": METRIC,
+        "value": nel * args.steps / t_total,
+        "unit": "elements/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_total / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "std-linear prism convection-diffusion, 128^3 hex box x 2 prisms "
+                               "(jitter 0.2h, lognormal kappa sigma=1, random b), FP64, "
+                               "GMRES(%d)+Jacobi to 1e-8" % args.krylbas if args.n == 128 else
+                               "std-linear prism convection-diffusion %d^3 hex box x 2 prisms" % args.n,
+                   "elements": nel, "dofs": n, "nnz": nnz, "boundary_faces": p.nbf},
+        "solves_per_s": args.steps / t_total,
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                       "p99": float(np.percentile(lat, 99) * 1e3)},
+        "gmres": {"iters": iters, "res_rel": infos[-1]["res_rel"], "true_rel_residual": true_rel},
+        "integrate_assemble": {"elements_per_s": nel / ((ms_int + ms_faces) * 1e-3),
+                               "ms_elements": ms_int, "ms_faces": ms_faces,
+                               "fp64_tflops_R_model": f_int / (ms_int * 1e-3) / 1e12,
+                               "frac_fp64_peak": f_int / (ms_int * 1e-3) / 1e12 / FP64_PEAK_TFS,
+                               "algorithmic_GBs": b_int / (ms_int * 1e-3) / 1e9},
+        "spmv_GBs": (12.0 * nnz + 4.0 * (n + 1) + 16.0 * n) / (ms_spmv * 1e-3) / 1e9,
+        "roofline": {"kernel": "k_op_dots (Jacobi-preconditioned SpMV + CGS dots)",
+                     "bound": "hbm", "achieved": ach_op, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_op / HBM_PEAK_GBS, "traffic": None,
+                     "launches": int(n_op), "avg_launch_ms": t_op / max(n_op, 1),
+                     "bytes_per_launch": b_op / max(n_op, 1)},
+        "roofline_update": {"kernel": "k_update (CGS update + norm + Givens)",
+                            "achieved": b_up / (t_up * 1e-3) / 1e9 if t_up > 0 else 0.0,
+                            "launches": int(n_up), "avg_launch_ms": t_up / max(n_up, 1)},
+        "setup_s": t_setup,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(mesh, kappa, b, iters, args.krylbas, threads,
+                                           args.cpu_iters)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+/*
+ * modfem_hip.h -- C ABI of the MI355X-native ModFEM hot path
+ * (element integration -> CRS assembly -> GMRES), libmodfem_hip.so.
+ *
+ * Plain C types only: int/long long/double pointers and sizes.  Every entry
+ * point returns >= 0 on success and < 0 on error (-1 bad argument, -2 HIP
+ * runtime failure, -3 unknown handle, -4 numerical failure); mf_last_error()
+ * gives the message.  The reference instead calls exit(-1) on most errors
+ * (e.g. sid_mkb/sis_mkb_intf.c:289-292).
+ *
+ * Objects are small integer handles, as in the reference's ID-indexed tables
+ * (sid_mkb/sis_mkb_intf.c:95-97, lsd_mkb/lad_crs/las_crs_intf.c:65-66).
+ * Paths are relative to the reference tree modfem2017/src/.
+ *
+ * Not re-entrant: like the reference's lar_/lsr_ layers, call from one host
+ * thread.  All device work runs on one non-blocking HIP stream per process.
+ */
+#ifndef MODFEM_HIP_H
+#define MODFEM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_PRISM 5 /* MMC_PRISM, include/mmh_intf.h */
+#define MF_TETRA 7 /* MMC_TETRA */
+
+/* coefficient storage models for mf_problem_set_coefficients */
+#define MF_COEF_UNIFORM 0 /* all 21 values from coef_const                  */
+#define MF_COEF_KAPPA 1   /* coef_el[nel]: diffusion a = kappa*I             */
+#define MF_COEF_KAPPA_B 2 /* coef_el[4*nel] SoA: kappa, bx, by, bz            */
+#define MF_COEF_FULL 3    /* coef_el[21*nel] SoA: a[9] b[3] t[3] c q[3] s m  */
+/* source / Dirichlet data models */
+#define MF_SRC_CONST 0
+#define MF_SRC_DIFF_IN_CUBE 1 /* pdd_conv_diff/main/pds_conv_diff_util.c, pdr_exact_sol */
+#define MF_SRC_DIFF_IN_BOX 2
+
+/* ---------------------------------------------------------------- library */
+/* select the HIP device and create the library stream (no reference
+   counterpart: the reference's device set-up is tmr_init_... in
+   tmd_opencl/tms_ocl_intf.c:105) */
+int mf_init(int device);
+const char* mf_last_error(void);
+int mf_version(void);
+
+/* ------------------------------------------------------------------- mesh
+ * Input the approximation module reads through mmr_get_next_elem_all,
+ * mmr_el_node_coor, mmr_fa_neig and mmr_fa_bc (include/mmh_intf.h;
+ * mmd_prism/mms_prism_intf.c:385, mms_prism_datstr.c:365,993).  Indices are
+ * 0-based, element nodes padded to 6 per element, boundary faces (bc>0)
+ * listed in face-ID order with their owning element and local side. */
+int mf_mesh_create(int nr_nodes, const double* node_coor, int nr_elems,
+                   const int* el_type, const int* el_nodes, int nr_bfaces,
+                   const int* bf_elem, const int* bf_side, const int* bf_bc);
+/* jk ("gradmesh") half-structured prism format: mmr_import_mesh_grad
+   (mmd_prism/mms_prism_input_gradmesh.c:29) */
+int mf_mesh_read_jk(const char* filename);
+int mf_mesh_write_jk(int mesh, const char* filename);
+/* seeded synthetic boxes (no reference counterpart: workload generators) */
+int mf_mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
+                      unsigned long long seed, double jitter, int bc);
+int mf_mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
+                      unsigned long long seed, double jitter, int bc);
+int mf_mesh_sizes(int mesh, int* nr_nodes, int* nr_elems, int* nr_bfaces);
+int mf_mesh_get(int mesh, double* node_coor, int* el_type, int* el_nodes,
+                int* bf_elem, int* bf_side, int* bf_bc);
+int mf_mesh_free(int mesh);
+/* host-only integer set-up of a mesh (DOF list, CRS pattern, tables) without
+   device work; same maps as mf_problem_create */
+int mf_mesh_setup(int mesh, int* n, long long* nnz);
+int mf_mesh_setup_get(int mesh, int* dof_node, int* row_ptr, int* col_ind);
+int mf_mesh_setup_table(int mesh, int int_ent, int* table);
+
+/* --------------------------------------------------------------- problem
+ * mf_problem_create replaces sir_create's integer set-up for a std-linear
+ * field with block size 1 (sid_mkb/sis_mkb_intf.c:396-1692):
+ * utr_get_list_int_ent / utr_get_list_dof_ent (utd_util/uts_ls_intf.c:102,182),
+ * DOF structures and sorted neighbour lists, lar_allocate_SM_and_LV_crs
+ * (lsd_mkb/lad_crs/las_crs_intf.c:72) and the assembly tables
+ * (las_crs_intf.c:354).  The matrix lives in device memory. */
+int mf_problem_create(int mesh);
+/* solver-only problem from a given CRS matrix (strictly ascending columns
+   with the diagonal present), e.g. for a caller that assembles itself */
+int mf_problem_create_crs(int n, const int* row_ptr, const int* col_ind,
+                          const double* val, const double* rhs);
+int mf_problem_sizes(int pid, int* n, long long* nnz, int* nr_elems,
+                     int* nr_bfaces, int* max_row);
+int mf_problem_get_dofs(int pid, int* dof_node);
+int mf_problem_get_pattern(int pid, int* row_ptr, int* col_ind);
+/* the reference-format table of one integration entity (elements first,
+   then boundary faces): (k+1)*k ints, lar_fill_assembly_table_int_ent_crs
+   (las_crs_intf.c:354-432) */
+int mf_problem_get_table(int pid, int int_ent, int* table);
+
+/* coefficients: coeff_ind[23] is the reference's coefficient indicator
+   vector (pdr_select_el_coeff_vect, include/pdh_intf.h:486); bf_g[nbf]
+   optional Dirichlet values for MF_SRC_CONST */
+int mf_problem_set_coefficients(int pid, const int* coeff_ind, int coef_model,
+                                const double* coef_const, const double* coef_el,
+                                int src_model, const double* bf_g);
+
+/* batched integrate + assemble on the device: replaces the create/assemble
+   loop of sir_solve (sis_mkb_intf.c:1843-2073) with apr_num_int_el
+   (apd_std_lin/aps_std_lin_intf.c:1537), pdr_conv_diff_comp_fa_stiff_mat_app
+   (pdd_conv_diff/weak_formulation/approx_std/pds_conv_diff_weakform_std.c:253)
+   and lar_assemble_SM_and_LV_with_table_crs (las_crs_intf.c:440); the
+   OpenCL counterpart is tmr_perform_create_assemble_accel
+   (utd_util/uts_accel_intf.c:70-103).  precision: 64 or 32 (FP32
+   shape-function products with FP64 geometry and FP64 global sums). */
+int mf_integrate_assemble(int pid, int precision);
+/* element / face matrices without assembly, column-major stiff[u*k+w] then
+   rhs[k], 42 doubles per entity (apr_num_int_el output layout,
+   aps_std_lin_intf.c:1558-1577) */
+int mf_element_matrices(int pid, int first, int count, int precision, double* out);
+int mf_face_matrices(int pid, int first, int count, double* out);
+int mf_get_system(int pid, double* val, double* rhs);
+int mf_set_system(int pid, const double* val, const double* rhs);
+
+/* ---------------------------------------------------------------- solver */
+/* y = A x (host arrays; lar_compute_residual_crs with Use_rhs=0 gives -y,
+   las_crs_intf.c:788) */
+int mf_spmv(int pid, const double* x, double* y);
+/* Jacobi fill 1/a_ii, lar_fill_preconditioner_crs (las_crs_intf.c:707) */
+int mf_fill_precon(int pid, int precon);
+/* restarted left-preconditioned GMRES(krylbas), lsr_mkb_core_gmres
+   (lsd_mkb/lsd_mkb_core/lss_mkb_core.c:2317); precon 0 none, 1 Jacobi.
+   Returns 1 on convergence, 0 otherwise, <0 on error.  x: host array of n
+   (input guess unless ini_zero, output solution). */
+int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol,
+             int ini_zero, double* x, int* iters, double* res_rel, double* resid);
+/* true relative residual ||b - A x|| / ||b|| in FP64 on the device */
+int mf_true_residual(int pid, const double* x, double* rel);
+
+/* ---------------------------------------------------------- measurement
+ * kernel times measured with HIP events on the library stream */
+int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems,
+                       double* ms_faces);
+int mf_bench_spmv(int pid, int reps, double* ms);
+int mf_last_solve_ms(int pid, double* ms);
+/* per-launch HIP-event timing of the two Arnoldi kernels inside mf_gmres:
+   out[6] = {launches, ms, algorithmic bytes} for the operator+dots kernel,
+   then for the update+norm kernel (accumulated since the last call); enable
+   switches the timing on or off for later solves */
+int mf_gmres_kernel_stats(int enable, double* out);
+
+int mf_problem_free(int pid);
+
+/* host copy of a storage-seam matrix (lar_get_SM_and_LV_crs_from_crs with
+   offset 0, lsd_mkb/lad_crs/las_crs_intf.c:1063) */
+int mf_lar_get_system(int matrix_id, int* row_ptr, int* col_ind, double* val,
+                      double* rhs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,346 @@
+"""modfem_amd -- MI355X-native ModFEM hot path (integrate -> assemble -> GMRES).
+
+Python mirror of the C ABI in ``include/modfem_hip.h`` (``libmodfem_hip.so``).
+It follows the reference's call sequence for one solve
+(sid_mkb/sis_mkb_intf.c: sir_create -> sir_solve): build the mesh input,
+create the problem (DOF lists, CRS pattern, assembly tables), bind the PDE
+coefficients, integrate + assemble on the device, then run GMRES.
+
+The library is loaded from this directory; there is no CPU fallback: if the
+HIP library is missing or fails to initialise, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["lib", "Mesh", "Problem", "MFError", "PRISM", "TETRA",
+           "COEF_UNIFORM", "COEF_KAPPA", "COEF_KAPPA_B", "COEF_FULL",
+           "SRC_CONST", "SRC_DIFF_IN_CUBE", "SRC_DIFF_IN_BOX", "IND",
+           "coeff_indicator", "init"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmodfem_hip.so")
+
+PRISM, TETRA = 5, 7
+COEF_UNIFORM, COEF_KAPPA, COEF_KAPPA_B, COEF_FULL = 0, 1, 2, 3
+SRC_CONST, SRC_DIFF_IN_CUBE, SRC_DIFF_IN_BOX = 0, 1, 2
+# coefficient slots: a[9] b[3] t[3] c q[3] s m
+NCOEF = 21
+C_A, C_B, C_T, C_C, C_Q, C_S, C_M = 0, 9, 12, 15, 16, 19, 20
+# reference coefficient-indicator slots (pds_conv_diff_weakform.c:87-276)
+IND = {"m": 1, "axx": 2, "axy": 3, "axz": 4, "ayx": 5, "ayy": 6, "ayz": 7,
+       "azx": 8, "azy": 9, "azz": 10, "bx": 11, "by": 12, "bz": 13, "tx": 14,
+       "ty": 15, "tz": 16, "c": 17, "l": 18, "qx": 19, "qy": 20, "qz": 21,
+       "s": 22}
+
+
+def coeff_indicator(*names):
+    """coeff_vect_ind[23] with slot 0 set and the named slots selected."""
+    ind = np.zeros(23, dtype=np.int32)
+    ind[0] = 1
+    for n in names:
+        ind[IND[n]] = 1
+    return ind
+
+
+# DIFF_IN_CUBE / DIFF_IN_BOX selection (pds_conv_diff_weakform.c:103-113)
+IND_DIFF = coeff_indicator("axx", "ayy", "azz", "s")
+# generic conv-diff selection used for convection-diffusion workloads
+IND_CONV_DIFF = coeff_indicator("axx", "axy", "axz", "ayx", "ayy", "ayz",
+                                "azx", "azy", "azz", "bx", "by", "bz", "s")
+# TEST_NUM_INT: every term (pds_conv_diff_weakform.c:114-160)
+IND_ALL = coeff_indicator("axx", "axy", "axz", "ayx", "ayy", "ayz", "azx",
+                          "azy", "azz", "bx", "by", "bz", "tx", "ty", "tz",
+                          "c", "qx", "qy", "qz", "s")
+
+
+class MFError(RuntimeError):
+    pass
+
+
+_lib = None
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+def _declare(L):
+    I, D, LL, S = ctypes.c_int, ctypes.c_double, ctypes.c_longlong, ctypes.c_char_p
+    V = ctypes.c_void_p
+    sig = {
+        "mf_init": (I, [I]),
+        "mf_last_error": (S, []),
+        "mf_version": (I, []),
+        "mf_mesh_create": (I, [I, V, I, V, V, I, V, V, V]),
+        "mf_mesh_read_jk": (I, [S]),
+        "mf_mesh_write_jk": (I, [I, S]),
+        "mf_mesh_box_prism": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
+        "mf_mesh_box_tetra": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
+        "mf_mesh_sizes": (I, [I, V, V, V]),
+        "mf_mesh_get": (I, [I, V, V, V, V, V, V]),
+        "mf_mesh_free": (I, [I]),
+        "mf_mesh_setup": (I, [I, V, V]),
+        "mf_mesh_setup_get": (I, [I, V, V, V]),
+        "mf_mesh_setup_table": (I, [I, I, V]),
+        "mf_problem_create": (I, [I]),
+        "mf_problem_create_crs": (I, [I, V, V, V, V]),
+        "mf_problem_sizes": (I, [I, V, V, V, V, V]),
+        "mf_problem_get_dofs": (I, [I, V]),
+        "mf_problem_get_pattern": (I, [I, V, V]),
+        "mf_problem_get_table": (I, [I, I, V]),
+        "mf_problem_set_coefficients": (I, [I, V, I, V, V, I, V]),
+        "mf_integrate_assemble": (I, [I, I]),
+        "mf_element_matrices": (I, [I, I, I, I, V]),
+        "mf_face_matrices": (I, [I, I, I, V]),
+        "mf_get_system": (I, [I, V, V]),
+        "mf_set_system": (I, [I, V, V]),
+        "mf_spmv": (I, [I, V, V]),
+        "mf_fill_precon": (I, [I, I]),
+        "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
+        "mf_true_residual": (I, [I, V, V]),
+        "mf_bench_integrate": (I, [I, I, I, V, V]),
+        "mf_bench_spmv": (I, [I, I, V]),
+        "mf_last_solve_ms": (I, [I, V]),
+        "mf_gmres_kernel_stats": (I, [I, V]),
+        "mf_problem_free": (I, [I]),
+        # storage seam with the reference's names (lsd_mkb/lah_intf.h)
+        "lar_allocate_SM_and_LV": (I, [I, I, I, I, I, I, V, V, V, V]),
+        "lar_initialize_SM_and_LV": (I, [I, I]),
+        "lar_get_storage": (D, [I]),
+        "lar_fill_assembly_table_int_ent": (I, [I, I, V, V, V]),
+        "lar_assemble_SM_and_LV_with_table": (I, [I, I, I, V, V, V, V]),
+        "lar_assemble_SM_and_LV": (I, [I, I, I, V, V, V, V, V]),
+        "lar_free_SM_and_LV": (I, [I]),
+        "lar_allocate_preconditioner": (I, [I, I, I]),
+        "lar_fill_preconditioner": (I, [I]),
+        "lar_free_preconditioner": (I, [I]),
+        "lar_compute_residual": (None, [I, I, I, I, V, V, V]),
+        "lar_perform_BJ_or_GS_iterations": (None, [I, I, I, I, I, V, V]),
+        "lar_perform_rhsub": (None, [I, I, V, V]),
+        "mf_lar_get_system": (I, [I, V, V, V, V]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def lib():
+    """The loaded HIP library (raises MFError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MFError("libmodfem_hip.so is not built (run __graft_entry__.build())")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(rc):
+    if rc < 0:
+        raise MFError(lib().mf_last_error().decode())
+    return rc
+
+
+_inited = False
+
+
+def init(device=0):
+    global _inited
+    if not _inited:
+        check(lib().mf_init(device))
+        _inited = True
+
+
+def gmres_kernel_stats(enable):
+    """returns accumulated {op:(launches, ms, bytes), update:(...)} and sets the
+    timing switch for later solves"""
+    out = np.zeros(6)
+    check(lib().mf_gmres_kernel_stats(int(enable), _p(out)))
+    return {"op_dots": tuple(out[0:3]), "update": tuple(out[3:6])}
+
+
+class Mesh:
+    """Mesh input (element/face lists as the approximation module sees them)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        nno, nel, nbf = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().mf_mesh_sizes(handle, ctypes.byref(nno), ctypes.byref(nel), ctypes.byref(nbf)))
+        self.nno, self.nel, self.nbf = nno.value, nel.value, nbf.value
+
+    @classmethod
+    def box_prism(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, seed=0, jitter=0.0, bc=1):
+        return cls(check(lib().mf_mesh_box_prism(nx, ny, nz, lx, ly, lz, seed, jitter, bc)))
+
+    @classmethod
+    def box_tetra(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, seed=0, jitter=0.0, bc=1):
+        return cls(check(lib().mf_mesh_box_tetra(nx, ny, nz, lx, ly, lz, seed, jitter, bc)))
+
+    @classmethod
+    def read_jk(cls, path):
+        return cls(check(lib().mf_mesh_read_jk(path.encode())))
+
+    @classmethod
+    def from_arrays(cls, coor, el_type, el_nodes, bf_elem, bf_side, bf_bc):
+        coor = np.ascontiguousarray(coor, dtype=np.float64)
+        el_type = np.ascontiguousarray(el_type, dtype=np.int32)
+        el_nodes = np.ascontiguousarray(el_nodes, dtype=np.int32)
+        bf = [np.ascontiguousarray(a, dtype=np.int32) for a in (bf_elem, bf_side, bf_bc)]
+        return cls(check(lib().mf_mesh_create(coor.shape[0], _p(coor), el_type.shape[0],
+                                              _p(el_type), _p(el_nodes), bf[0].shape[0],
+                                              _p(bf[0]), _p(bf[1]), _p(bf[2]))))
+
+    def write_jk(self, path):
+        check(lib().mf_mesh_write_jk(self.h, path.encode()))
+
+    def arrays(self):
+        coor = np.zeros((self.nno, 3))
+        et = np.zeros(self.nel, np.int32)
+        en = np.zeros((self.nel, 6), np.int32)
+        be = np.zeros(self.nbf, np.int32)
+        bs = np.zeros(self.nbf, np.int32)
+        bb = np.zeros(self.nbf, np.int32)
+        check(lib().mf_mesh_get(self.h, _p(coor), _p(et), _p(en), _p(be), _p(bs), _p(bb)))
+        return dict(coor=coor, el_type=et, el_nodes=en, bf_elem=be, bf_side=bs, bf_bc=bb)
+
+    def setup(self):
+        """host integer set-up: (dof_node, row_ptr, col_ind)"""
+        n, nnz = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().mf_mesh_setup(self.h, ctypes.byref(n), ctypes.byref(nnz)))
+        d = np.zeros(n.value, np.int32)
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        check(lib().mf_mesh_setup_get(self.h, _p(d), _p(rp), _p(ci)))
+        return d, rp, ci
+
+    def setup_table(self, int_ent):
+        t = np.zeros(42, np.int32)
+        k = check(lib().mf_mesh_setup_table(self.h, int_ent, _p(t)))
+        return t[:k]
+
+    def free(self):
+        if self.h:
+            lib().mf_mesh_free(self.h)
+            self.h = 0
+
+
+class Problem:
+    """One std-linear problem with its device-resident CRS system."""
+
+    def __init__(self, handle):
+        self.h = handle
+        n, nnz, nel, nbf, mr = (ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int(),
+                                ctypes.c_int(), ctypes.c_int())
+        check(lib().mf_problem_sizes(handle, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(nel),
+                                     ctypes.byref(nbf), ctypes.byref(mr)))
+        self.n, self.nnz, self.nel, self.nbf, self.max_row = (n.value, nnz.value, nel.value,
+                                                               nbf.value, mr.value)
+
+    @classmethod
+    def from_mesh(cls, mesh):
+        init()
+        return cls(check(lib().mf_problem_create(mesh.h)))
+
+    @classmethod
+    def from_crs(cls, row_ptr, col_ind, val, rhs=None):
+        init()
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        b = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
+        return cls(check(lib().mf_problem_create_crs(rp.shape[0] - 1, _p(rp), _p(ci), _p(v), _p(b))))
+
+    def dofs(self):
+        d = np.zeros(self.n, np.int32)
+        check(lib().mf_problem_get_dofs(self.h, _p(d)))
+        return d
+
+    def pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(self.nnz, np.int32)
+        check(lib().mf_problem_get_pattern(self.h, _p(rp), _p(ci)))
+        return rp, ci
+
+    def table(self, int_ent):
+        t = np.zeros(42, np.int32)
+        k = check(lib().mf_problem_get_table(self.h, int_ent, _p(t)))
+        return t[:k]
+
+    def set_coefficients(self, ind, coef_model=COEF_UNIFORM, coef_const=None, coef_el=None,
+                         src_model=SRC_CONST, bf_g=None):
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        cc = np.zeros(NCOEF) if coef_const is None else np.ascontiguousarray(coef_const, np.float64)
+        ce = None if coef_el is None else np.ascontiguousarray(coef_el, np.float64)
+        g = None if bf_g is None else np.ascontiguousarray(bf_g, np.float64)
+        check(lib().mf_problem_set_coefficients(self.h, _p(ind), coef_model, _p(cc), _p(ce),
+                                                src_model, _p(g)))
+
+    def integrate_assemble(self, precision=64):
+        check(lib().mf_integrate_assemble(self.h, precision))
+
+    def element_matrices(self, first=0, count=None, precision=64):
+        count = self.nel - first if count is None else count
+        out = np.zeros((count, 42))
+        check(lib().mf_element_matrices(self.h, first, count, precision, _p(out)))
+        return out
+
+    def face_matrices(self, first=0, count=None):
+        count = self.nbf - first if count is None else count
+        out = np.zeros((count, 42))
+        check(lib().mf_face_matrices(self.h, first, count, _p(out)))
+        return out
+
+    def system(self):
+        val = np.zeros(self.nnz)
+        rhs = np.zeros(self.n)
+        check(lib().mf_get_system(self.h, _p(val), _p(rhs)))
+        return val, rhs
+
+    def set_system(self, val=None, rhs=None):
+        v = None if val is None else np.ascontiguousarray(val, np.float64)
+        b = None if rhs is None else np.ascontiguousarray(rhs, np.float64)
+        check(lib().mf_set_system(self.h, _p(v), _p(b)))
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n)
+        check(lib().mf_spmv(self.h, _p(x), _p(y)))
+        return y
+
+    def gmres(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, x0=None):
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        conv = check(lib().mf_gmres(self.h, precon, krylbas, max_iter, tol, int(x0 is None),
+                                    _p(x), ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs)))
+        return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+    def true_residual(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        r = ctypes.c_double()
+        check(lib().mf_true_residual(self.h, _p(x), ctypes.byref(r)))
+        return r.value
+
+    def bench_integrate(self, precision=64, reps=5):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        check(lib().mf_bench_integrate(self.h, precision, reps, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def bench_spmv(self, reps=20):
+        a = ctypes.c_double()
+        check(lib().mf_bench_spmv(self.h, reps, ctypes.byref(a)))
+        return a.value
+
+    def last_solve_ms(self):
+        a = ctypes.c_double()
+        check(lib().mf_last_solve_ms(self.h, ctypes.byref(a)))
+        return a.value
+
+    def free(self):
+        if self.h:
+            lib().mf_problem_free(self.h)
+            self.h = 0

@@ -1,0 +1,56 @@
+"""Build recipe for libmodfem_hip.so (gfx950).
+
+Host-only C++ (mesh, setup) is compiled with g++/OpenMP; the HIP translation
+units with hipcc for gfx950 only.  The shared library is written in-tree next
+to this file so that it travels with the repository snapshot.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmodfem_hip.so")
+BUILD = os.path.join(HERE, "_build")
+
+HOST_SRCS = ["mf_mesh.cpp", "mf_setup.cpp"]
+HIP_SRCS = ["mf_device.hip", "mf_capi.hip", "mf_lar.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wno-unused-result",
+    # packed FP32 instructions stay off for every HIP translation unit
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return r.stdout
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for s in HOST_SRCS:
+        o = os.path.join(BUILD, s + ".o")
+        _run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in HIP_SRCS:
+        o = os.path.join(BUILD, s + ".o")
+        _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lgomp"])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

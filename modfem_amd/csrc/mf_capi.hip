@@ -1,0 +1,649 @@
+// C ABI of libmodfem_hip.so (declarations and reference citations in
+// include/modfem_hip.h).  Handles are small integers into process-wide
+// tables; device memory is owned by the library.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "../../include/modfem_hip.h"
+#include "mf_device.h"
+
+using namespace mf;
+
+namespace {
+
+std::map<int, std::unique_ptr<Mesh>> g_meshes;
+std::map<int, std::unique_ptr<Problem>> g_problems;
+std::map<int, std::unique_ptr<Setup>> g_setups;
+int g_next_mesh = 1, g_next_problem = 1;
+bool g_inited = false;
+
+Mesh* get_mesh(int id) {
+  auto it = g_meshes.find(id);
+  if (it == g_meshes.end()) {
+    set_error("unknown mesh handle " + std::to_string(id));
+    return nullptr;
+  }
+  return it->second.get();
+}
+
+Problem* get_problem(int id) {
+  auto it = g_problems.find(id);
+  if (it == g_problems.end()) {
+    set_error("unknown problem handle " + std::to_string(id));
+    return nullptr;
+  }
+  return it->second.get();
+}
+
+int need_init() {
+  if (!g_inited) {
+    set_error("mf_init has not been called");
+    return -1;
+  }
+  return 0;
+}
+
+int add_mesh(std::unique_ptr<Mesh> m) {
+  int id = g_next_mesh++;
+  g_meshes[id] = std::move(m);
+  return id;
+}
+
+void free_problem_device(Problem& P) {
+  void* ptrs[] = {P.d_coor, P.d_el_dof, P.d_el_off, P.d_bf_elem, P.d_bf_side, P.d_bf_off,
+                  P.d_bf_g, P.d_row_ptr, P.d_col, P.d_val, P.d_rhs, P.d_val32,
+                  P.d_coef, P.d_dinv, P.d_diag_ptr, P.d_dinv32, P.d_tile_row, P.d_V, P.d_H,
+                  P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (P.h_stat) hipHostFree(P.h_stat);
+}
+
+template <typename T>
+int upload(T** dst, const T* src, size_t count) {
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  MF_HIP(hipMalloc(dst, bytes));
+  if (src && count) MF_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mf_init(int device) {
+  int rc = device_init(device);
+  if (rc == 0) g_inited = true;
+  return rc;
+}
+
+const char* mf_last_error(void) { return last_error(); }
+int mf_version(void) { return 1; }
+
+int mf_mesh_create(int nno, const double* coor, int nel, const int* el_type,
+                   const int* el_nodes, int nbf, const int* bf_elem, const int* bf_side,
+                   const int* bf_bc) {
+  if (nno <= 0 || nel <= 0 || !coor || !el_type || !el_nodes || nbf < 0 ||
+      (nbf > 0 && (!bf_elem || !bf_side || !bf_bc))) {
+    set_error("mf_mesh_create: invalid arguments");
+    return -1;
+  }
+  auto m = std::make_unique<Mesh>();
+  m->nno = nno;
+  m->coor.assign(coor, coor + 3 * (size_t)nno);
+  m->nel = nel;
+  m->type.assign(el_type, el_type + nel);
+  m->nodes.assign(el_nodes, el_nodes + 6 * (size_t)nel);
+  for (int e = 0; e < nel; e++) {
+    if (m->type[e] != kPrism && m->type[e] != kTetra) {
+      set_error("mf_mesh_create: element type must be MF_PRISM or MF_TETRA");
+      return -1;
+    }
+    for (int i = 0; i < nodes_of(m->type[e]); i++) {
+      int v = m->nodes[6 * (size_t)e + i];
+      if (v < 0 || v >= nno) {
+        set_error("mf_mesh_create: element node index out of range");
+        return -1;
+      }
+    }
+  }
+  m->nbf = nbf;
+  if (nbf) {
+    m->bf_elem.assign(bf_elem, bf_elem + nbf);
+    m->bf_side.assign(bf_side, bf_side + nbf);
+    m->bf_bc.assign(bf_bc, bf_bc + nbf);
+    for (int f = 0; f < nbf; f++) {
+      int e = m->bf_elem[f];
+      if (e < 0 || e >= nel || m->bf_side[f] < 0 ||
+          m->bf_side[f] >= (m->type[e] == kPrism ? 5 : 4)) {
+        set_error("mf_mesh_create: boundary face owner/side out of range");
+        return -1;
+      }
+    }
+  }
+  return add_mesh(std::move(m));
+}
+
+int mf_mesh_read_jk(const char* filename) {
+  auto m = std::make_unique<Mesh>();
+  if (mesh_read_jk(filename, *m) < 0) return -1;
+  return add_mesh(std::move(m));
+}
+
+int mf_mesh_write_jk(int mesh, const char* filename) {
+  Mesh* m = get_mesh(mesh);
+  if (!m) return -3;
+  return mesh_write_jk(*m, filename);
+}
+
+int mf_mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
+                      unsigned long long seed, double jitter, int bc) {
+  auto m = std::make_unique<Mesh>();
+  if (mesh_box_prism(nx, ny, nz, lx, ly, lz, seed, jitter, bc, *m) < 0) return -1;
+  return add_mesh(std::move(m));
+}
+
+int mf_mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
+                      unsigned long long seed, double jitter, int bc) {
+  auto m = std::make_unique<Mesh>();
+  if (mesh_box_tetra(nx, ny, nz, lx, ly, lz, seed, jitter, bc, *m) < 0) return -1;
+  return add_mesh(std::move(m));
+}
+
+int mf_mesh_sizes(int mesh, int* nno, int* nel, int* nbf) {
+  Mesh* m = get_mesh(mesh);
+  if (!m) return -3;
+  if (nno) *nno = m->nno;
+  if (nel) *nel = m->nel;
+  if (nbf) *nbf = m->nbf;
+  return 0;
+}
+
+int mf_mesh_get(int mesh, double* coor, int* el_type, int* el_nodes, int* bf_elem,
+                int* bf_side, int* bf_bc) {
+  Mesh* m = get_mesh(mesh);
+  if (!m) return -3;
+  if (coor) std::memcpy(coor, m->coor.data(), m->coor.size() * sizeof(double));
+  if (el_type) std::memcpy(el_type, m->type.data(), m->type.size() * sizeof(int));
+  if (el_nodes) std::memcpy(el_nodes, m->nodes.data(), m->nodes.size() * sizeof(int));
+  if (m->nbf) {
+    if (bf_elem) std::memcpy(bf_elem, m->bf_elem.data(), m->nbf * sizeof(int));
+    if (bf_side) std::memcpy(bf_side, m->bf_side.data(), m->nbf * sizeof(int));
+    if (bf_bc) std::memcpy(bf_bc, m->bf_bc.data(), m->nbf * sizeof(int));
+  }
+  return 0;
+}
+
+int mf_mesh_free(int mesh) {
+  if (!get_mesh(mesh)) return -3;
+  g_setups.erase(mesh);
+  g_meshes.erase(mesh);
+  return 0;
+}
+
+// host-only set-up of a mesh (no device work): lets the integer maps be
+// checked without a GPU
+int mf_mesh_setup(int mesh, int* n, long long* nnz) {
+  Mesh* m = get_mesh(mesh);
+  if (!m) return -3;
+  auto s = std::make_unique<Setup>();
+  if (build_setup(*m, *s) < 0) return -1;
+  if (n) *n = s->n;
+  if (nnz) *nnz = s->nnz;
+  g_setups[mesh] = std::move(s);
+  return 0;
+}
+
+int mf_mesh_setup_get(int mesh, int* dof_node, int* row_ptr, int* col_ind) {
+  auto it = g_setups.find(mesh);
+  if (it == g_setups.end()) {
+    set_error("mf_mesh_setup has not been called for this mesh");
+    return -3;
+  }
+  const Setup& s = *it->second;
+  if (dof_node) std::memcpy(dof_node, s.dof_node.data(), s.dof_node.size() * sizeof(int));
+  if (row_ptr) std::memcpy(row_ptr, s.row_ptr.data(), s.row_ptr.size() * sizeof(int));
+  if (col_ind) std::memcpy(col_ind, s.col_ind.data(), s.col_ind.size() * sizeof(int));
+  return 0;
+}
+
+int mf_mesh_setup_table(int mesh, int int_ent, int* table) {
+  auto it = g_setups.find(mesh);
+  Mesh* m = get_mesh(mesh);
+  if (!m) return -3;
+  if (it == g_setups.end()) {
+    set_error("mf_mesh_setup has not been called for this mesh");
+    return -3;
+  }
+  return fill_reference_table(*m, *it->second, int_ent, table);
+}
+
+int mf_problem_create(int mesh) {
+  if (need_init()) return -1;
+  Mesh* m = get_mesh(mesh);
+  if (!m) return -3;
+  auto P = std::make_unique<Problem>();
+  P->mesh = *m;
+  if (build_setup(P->mesh, P->setup) < 0) return -1;
+  const Mesh& M = P->mesh;
+  const Setup& S = P->setup;
+  P->n = S.n;
+  P->nel = M.nel;
+  P->nbf = M.nbf;
+  P->nnz = S.nnz;
+  P->el_type = M.type[0];
+  for (int e = 0; e < M.nel; e++)
+    if (M.type[e] != P->el_type) {
+      set_error("mf_problem_create: mixed prism/tetra std meshes are not supported");
+      return -1;
+    }
+  std::vector<double> cd(3 * (size_t)S.n);
+  for (int d = 0; d < S.n; d++)
+    for (int k = 0; k < 3; k++) cd[3 * (size_t)d + k] = M.coor[3 * (size_t)S.dof_node[d] + k];
+  if (upload(&P->d_coor, cd.data(), cd.size())) return -2;
+  if (upload(&P->d_el_dof, S.el_dof.data(), S.el_dof.size())) return -2;
+  if (upload(&P->d_el_off, S.el_off.data(), S.el_off.size())) return -2;
+  if (upload(&P->d_bf_elem, M.bf_elem.data(), M.bf_elem.size())) return -2;
+  if (upload(&P->d_bf_side, M.bf_side.data(), M.bf_side.size())) return -2;
+  if (upload(&P->d_row_ptr, S.row_ptr.data(), S.row_ptr.size())) return -2;
+  if (upload(&P->d_col, S.col_ind.data(), S.col_ind.size())) return -2;
+  if (upload<double>(&P->d_val, nullptr, (size_t)S.nnz)) return -2;
+  if (upload<double>(&P->d_rhs, nullptr, (size_t)S.n)) return -2;
+  if (launch_zero(P->d_val, S.nnz) || launch_zero(P->d_rhs, S.n)) return -2;
+  if (build_tiles(*P) < 0) return -1;
+  // default coefficients: -Laplace u = 1 (DIFF_IN_CUBE-style selection)
+  P->mask = TermMask{0, 1, 0, 0, 0, 0, 0, 1};
+  P->coef_const[C_A + 0] = P->coef_const[C_A + 4] = P->coef_const[C_A + 8] = 1.0;
+  P->coef_const[C_S] = 1.0;
+  MF_HIP(hipStreamSynchronize(stream()));
+  int id = g_next_problem++;
+  g_problems[id] = std::move(P);
+  return id;
+}
+
+int mf_problem_create_crs(int n, const int* row_ptr, const int* col_ind, const double* val,
+                          const double* rhs) {
+  if (need_init()) return -1;
+  if (n <= 0 || !row_ptr || !col_ind || !val) {
+    set_error("mf_problem_create_crs: invalid arguments");
+    return -1;
+  }
+  auto P = std::make_unique<Problem>();
+  P->n = n;
+  P->setup.n = n;
+  P->setup.row_ptr.assign(row_ptr, row_ptr + n + 1);
+  long long nnz = row_ptr[n];
+  P->setup.col_ind.assign(col_ind, col_ind + nnz);
+  P->setup.nnz = nnz;
+  P->nnz = nnz;
+  for (int r = 0; r < n; r++) {
+    bool diag = false;
+    for (int k = row_ptr[r]; k < row_ptr[r + 1]; k++) {
+      if (col_ind[k] < 0 || col_ind[k] >= n || (k > row_ptr[r] && col_ind[k] <= col_ind[k - 1])) {
+        set_error("mf_problem_create_crs: columns must be in range and strictly ascending");
+        return -1;
+      }
+      if (col_ind[k] == r) diag = true;
+    }
+    if (!diag) {
+      set_error("mf_problem_create_crs: every row needs its diagonal entry");
+      return -1;
+    }
+  }
+  if (upload(&P->d_row_ptr, row_ptr, (size_t)n + 1)) return -2;
+  if (upload(&P->d_col, col_ind, (size_t)nnz)) return -2;
+  if (upload(&P->d_val, val, (size_t)nnz)) return -2;
+  if (upload<double>(&P->d_rhs, rhs, (size_t)n)) return -2;
+  if (!rhs && launch_zero(P->d_rhs, n)) return -2;
+  if (build_tiles(*P) < 0) return -1;
+  MF_HIP(hipStreamSynchronize(stream()));
+  int id = g_next_problem++;
+  g_problems[id] = std::move(P);
+  return id;
+}
+
+int mf_problem_sizes(int pid, int* n, long long* nnz, int* nel, int* nbf, int* max_row) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (n) *n = P->n;
+  if (nnz) *nnz = P->nnz;
+  if (nel) *nel = P->nel;
+  if (nbf) *nbf = P->nbf;
+  if (max_row) *max_row = P->setup.max_row;
+  return 0;
+}
+
+int mf_problem_get_dofs(int pid, int* dof_node) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  std::memcpy(dof_node, P->setup.dof_node.data(), P->setup.dof_node.size() * sizeof(int));
+  return 0;
+}
+
+int mf_problem_get_pattern(int pid, int* row_ptr, int* col_ind) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (row_ptr) std::memcpy(row_ptr, P->setup.row_ptr.data(), P->setup.row_ptr.size() * sizeof(int));
+  if (col_ind) std::memcpy(col_ind, P->setup.col_ind.data(), P->setup.col_ind.size() * sizeof(int));
+  return 0;
+}
+
+int mf_problem_get_table(int pid, int int_ent, int* table) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (P->nel == 0) {
+    set_error("problem has no mesh");
+    return -1;
+  }
+  return fill_reference_table(P->mesh, P->setup, int_ent, table);
+}
+
+int mf_problem_set_coefficients(int pid, const int* ind, int coef_model,
+                                const double* coef_const, const double* coef_el,
+                                int src_model, const double* bf_g) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (!ind || coef_model < CM_UNIFORM || coef_model > CM_FULL || src_model < SRC_CONST ||
+      src_model > SRC_DIFF_IN_BOX) {
+    set_error("mf_problem_set_coefficients: invalid arguments");
+    return -1;
+  }
+  if ((ind[11] || ind[12]) && !ind[13]) {
+    set_error("2D convection coefficient sets (bx,by without bz) are not supported");
+    return -1;
+  }
+  if (!ind[10] && (ind[2] || ind[6])) {
+    set_error("2D diffusion coefficient sets (no azz) are not supported");
+    return -1;
+  }
+  if ((ind[14] || ind[15]) && !ind[16]) {
+    set_error("2D t-coefficient sets (tx,ty without tz) are not supported");
+    return -1;
+  }
+  if ((ind[19] || ind[20]) && !ind[21]) {
+    set_error("2D q-coefficient sets (qx,qy without qz) are not supported");
+    return -1;
+  }
+  if (ind[18]) {
+    set_error("the obsolete l (time rhs) coefficient is not supported");
+    return -1;
+  }
+  TermMask M;
+  M.m = ind[1] != 0;
+  M.diff_full = ind[10] && ind[2];
+  M.diff_diag = ind[10] && !ind[2];
+  M.conv = ind[13] != 0;
+  M.tconv = ind[16] != 0;
+  M.reac = ind[17] != 0;
+  M.q = ind[21] != 0;
+  M.s = ind[22] != 0;
+  P->mask = M;
+  P->term_set = (M.m || M.tconv || M.reac || M.q) ? TS_ALL : (M.conv ? TS_DIFF_CONV : TS_DIFF);
+  for (int k = 0; k < NCOEF; k++) P->coef_const[k] = coef_const ? coef_const[k] : 0.0;
+  P->coef_model = coef_model;
+  P->src_model = src_model;
+  if (P->d_coef) {
+    hipFree(P->d_coef);
+    P->d_coef = nullptr;
+  }
+  size_t per = coef_model == CM_KAPPA ? 1 : coef_model == CM_KAPPA_B ? 4 : coef_model == CM_FULL ? NCOEF : 0;
+  if (per) {
+    if (!coef_el) {
+      set_error("mf_problem_set_coefficients: coef_el required for this model");
+      return -1;
+    }
+    if (upload(&P->d_coef, coef_el, per * (size_t)P->nel)) return -2;
+  }
+  if (P->d_bf_g) {
+    hipFree(P->d_bf_g);
+    P->d_bf_g = nullptr;
+  }
+  if (bf_g && P->nbf)
+    if (upload(&P->d_bf_g, bf_g, (size_t)P->nbf)) return -2;
+  return 0;
+}
+
+int mf_integrate_assemble(int pid, int precision) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (precision != 64 && precision != 32) {
+    set_error("precision must be 64 or 32");
+    return -1;
+  }
+  if (P->nel == 0) {
+    set_error("problem has no mesh");
+    return -1;
+  }
+  if (launch_zero(P->d_val, P->nnz) || launch_zero(P->d_rhs, P->n)) return -2;
+  if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
+  if (launch_faces(*P, 0, nullptr, 0, P->nbf)) return -2;
+  if (P->d_dinv) {
+    hipFree(P->d_dinv);
+    P->d_dinv = nullptr;
+  }
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_element_matrices(int pid, int first, int count, int precision, double* out) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (first < 0 || count < 0 || first + count > P->nel || !out) {
+    set_error("mf_element_matrices: range out of bounds");
+    return -1;
+  }
+  if (count == 0) return 0;
+  double* d = nullptr;
+  MF_HIP(hipMalloc(&d, (size_t)count * 42 * sizeof(double)));
+  MF_HIP(hipMemsetAsync(d, 0, (size_t)count * 42 * sizeof(double), stream()));
+  int rc = launch_integrate(*P, precision, 1, d, first, count);
+  if (rc == 0) {
+    hipError_t e = hipMemcpyAsync(out, d, (size_t)count * 42 * sizeof(double),
+                                  hipMemcpyDeviceToHost, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) {
+      set_error(hipGetErrorString(e));
+      rc = -2;
+    }
+  }
+  hipFree(d);
+  return rc;
+}
+
+int mf_face_matrices(int pid, int first, int count, double* out) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (first < 0 || count < 0 || first + count > P->nbf || !out) {
+    set_error("mf_face_matrices: range out of bounds");
+    return -1;
+  }
+  if (count == 0) return 0;
+  double* d = nullptr;
+  MF_HIP(hipMalloc(&d, (size_t)count * 42 * sizeof(double)));
+  MF_HIP(hipMemsetAsync(d, 0, (size_t)count * 42 * sizeof(double), stream()));
+  int rc = launch_faces(*P, 1, d, first, count);
+  if (rc == 0) {
+    hipError_t e = hipMemcpyAsync(out, d, (size_t)count * 42 * sizeof(double),
+                                  hipMemcpyDeviceToHost, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    if (e != hipSuccess) {
+      set_error(hipGetErrorString(e));
+      rc = -2;
+    }
+  }
+  hipFree(d);
+  return rc;
+}
+
+int mf_get_system(int pid, double* val, double* rhs) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  MF_HIP(hipStreamSynchronize(stream()));
+  if (val) MF_HIP(hipMemcpy(val, P->d_val, (size_t)P->nnz * sizeof(double), hipMemcpyDeviceToHost));
+  if (rhs) MF_HIP(hipMemcpy(rhs, P->d_rhs, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mf_set_system(int pid, const double* val, const double* rhs) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  MF_HIP(hipStreamSynchronize(stream()));
+  if (val) MF_HIP(hipMemcpy(P->d_val, val, (size_t)P->nnz * sizeof(double), hipMemcpyHostToDevice));
+  if (rhs) MF_HIP(hipMemcpy(P->d_rhs, rhs, (size_t)P->n * sizeof(double), hipMemcpyHostToDevice));
+  if (P->d_dinv) {
+    hipFree(P->d_dinv);
+    P->d_dinv = nullptr;
+  }
+  return 0;
+}
+
+int mf_spmv(int pid, const double* x, double* y) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  double *dx = nullptr, *dy = nullptr;
+  MF_HIP(hipMalloc(&dx, (size_t)P->n * sizeof(double)));
+  MF_HIP(hipMalloc(&dy, (size_t)P->n * sizeof(double)));
+  int rc = 0;
+  hipError_t e = hipMemcpyAsync(dx, x, (size_t)P->n * sizeof(double), hipMemcpyHostToDevice, stream());
+  if (e == hipSuccess) rc = launch_spmv(*P, dx, dy);
+  if (e == hipSuccess && rc == 0)
+    e = hipMemcpyAsync(y, dy, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost, stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  if (e != hipSuccess) {
+    set_error(hipGetErrorString(e));
+    rc = -2;
+  }
+  hipFree(dx);
+  hipFree(dy);
+  return rc;
+}
+
+int mf_fill_precon(int pid, int precon) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (precon == 0) return 0;
+  if (precon != 1) {
+    set_error("mf_fill_precon: only 0 (none) and 1 (Jacobi) are available");
+    return -1;
+  }
+  if (launch_fill_jacobi(*P)) return -2;
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol, int ini_zero, double* x,
+             int* iters, double* res_rel, double* resid) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (precon != 0 && precon != 1) {
+    set_error("mf_gmres: precon must be 0 (none) or 1 (Jacobi)");
+    return -1;
+  }
+  if (max_iter < 1 || tol < 0) {
+    set_error("mf_gmres: max_iter >= 1 and tol >= 0 required");
+    return -1;
+  }
+  return gmres_solve(*P, precon, krylbas, max_iter, tol, ini_zero, x, iters, res_rel, resid);
+}
+
+int mf_true_residual(int pid, const double* x, double* rel) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  std::vector<double> y(P->n), b(P->n);
+  int rc = mf_spmv(pid, x, y.data());
+  if (rc) return rc;
+  MF_HIP(hipMemcpy(b.data(), P->d_rhs, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost));
+  double nr = 0, nb = 0;
+  for (int i = 0; i < P->n; i++) {
+    double r = b[i] - y[i];
+    nr += r * r;
+    nb += b[i] * b[i];
+  }
+  *rel = nb > 0 ? std::sqrt(nr / nb) : std::sqrt(nr);
+  return 0;
+}
+
+int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems, double* ms_faces) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (reps < 1) reps = 1;
+  hipEvent_t a, b, c;
+  MF_HIP(hipEventCreate(&a));
+  MF_HIP(hipEventCreate(&b));
+  MF_HIP(hipEventCreate(&c));
+  float te = 0, tf = 0;
+  for (int r = 0; r < reps; r++) {
+    if (launch_zero(P->d_val, P->nnz) || launch_zero(P->d_rhs, P->n)) return -2;
+    MF_HIP(hipEventRecord(a, stream()));
+    if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
+    MF_HIP(hipEventRecord(b, stream()));
+    if (launch_faces(*P, 0, nullptr, 0, P->nbf)) return -2;
+    MF_HIP(hipEventRecord(c, stream()));
+    MF_HIP(hipEventSynchronize(c));
+    float x = 0, y = 0;
+    hipEventElapsedTime(&x, a, b);
+    hipEventElapsedTime(&y, b, c);
+    te += x;
+    tf += y;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  hipEventDestroy(c);
+  if (ms_elems) *ms_elems = te / reps;
+  if (ms_faces) *ms_faces = tf / reps;
+  return 0;
+}
+
+int mf_bench_spmv(int pid, int reps, double* ms) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  double *dx = nullptr, *dy = nullptr;
+  MF_HIP(hipMalloc(&dx, (size_t)P->n * sizeof(double)));
+  MF_HIP(hipMalloc(&dy, (size_t)P->n * sizeof(double)));
+  if (launch_zero(dx, P->n)) return -2;
+  hipEvent_t a, b;
+  MF_HIP(hipEventCreate(&a));
+  MF_HIP(hipEventCreate(&b));
+  if (launch_spmv(*P, dx, dy)) return -2;
+  MF_HIP(hipEventRecord(a, stream()));
+  for (int r = 0; r < reps; r++)
+    if (launch_spmv(*P, dx, dy)) return -2;
+  MF_HIP(hipEventRecord(b, stream()));
+  MF_HIP(hipEventSynchronize(b));
+  float t = 0;
+  hipEventElapsedTime(&t, a, b);
+  *ms = t / std::max(reps, 1);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  hipFree(dx);
+  hipFree(dy);
+  return 0;
+}
+
+int mf_last_solve_ms(int pid, double* ms) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  *ms = P->t_solve_ms;
+  return 0;
+}
+
+int mf_gmres_kernel_stats(int enable, double* out) {
+  if (out) kstats_get(out);
+  kstats_enable(enable);
+  return 0;
+}
+
+int mf_problem_free(int pid) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  hipStreamSynchronize(stream());
+  free_problem_device(*P);
+  g_problems.erase(pid);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Device-side data structures shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+#define MF_HIP(call)                                                       \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      ::mf::set_error(std::string("HIP error ") + hipGetErrorString(e_) + \
+                      " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+      return -2;                                                           \
+    }                                                                      \
+  } while (0)
+
+// coefficient storage models (per-element arrays are SoA [slot][nel])
+enum CoefModel {
+  CM_UNIFORM = 0,  // all 21 values from coef_const
+  CM_KAPPA = 1,    // a = kappa_e * I (diffusion branch), rest from coef_const
+  CM_KAPPA_B = 2,  // kappa_e and b_e[3]; rest from coef_const
+  CM_FULL = 3      // 21 values per element
+};
+// coefficient slots (same as the oracle): a[9] b[3] t[3] c q[3] s m
+constexpr int C_A = 0, C_B = 9, C_T = 12, C_C = 15, C_Q = 16, C_S = 19, C_M = 20,
+              NCOEF = 21;
+// term sets compiled into the integration kernel
+enum TermSet { TS_DIFF = 0, TS_DIFF_CONV = 1, TS_ALL = 2 };
+// source models
+enum SrcModel { SRC_CONST = 0, SRC_DIFF_IN_CUBE = 1, SRC_DIFF_IN_BOX = 2 };
+
+// selected-term bits (reference coefficient indicator slots 1..22)
+struct TermMask {
+  int m, diff_full, diff_diag, conv, tconv, reac, q, s;
+};
+
+struct GmresStat {
+  double beta0;     // norm of the first preconditioned residual (restart 0)
+  double rnrm;      // norm at the start of the current restart
+  double resid;     // current |g_{j+1}|
+  double res_rel;
+  double tol_rel, tol_res;
+  int j_done;       // Arnoldi steps completed in this restart
+  int gmiter;       // total iterations
+  int max_iter;
+  int stop;         // 1: stop this restart
+  int converged;
+  int pad[3];
+};
+
+struct Problem {
+  // host side
+  Mesh mesh;
+  Setup setup;
+  int n = 0, nel = 0, nbf = 0;
+  long long nnz = 0;
+  int el_type = kPrism;  // uniform element type (mixed handled per launch)
+  // device arrays
+  double* d_coor = nullptr;    // 3*n, DOF order
+  int* d_el_dof = nullptr;     // 6*nel SoA
+  uint8_t* d_el_off = nullptr; // 36*nel SoA
+  int* d_bf_elem = nullptr;
+  int* d_bf_side = nullptr;
+  uint8_t* d_bf_off = nullptr;
+  double* d_bf_g = nullptr;
+  int* d_row_ptr = nullptr;
+  int* d_col = nullptr;
+  double* d_val = nullptr;
+  double* d_rhs = nullptr;
+  float* d_val32 = nullptr;
+  // coefficients
+  int coef_model = CM_UNIFORM, src_model = SRC_CONST, term_set = TS_DIFF;
+  TermMask mask{};
+  double coef_const[NCOEF] = {0};
+  double* d_coef = nullptr;    // SoA per model
+  // preconditioner
+  double* d_dinv = nullptr;
+  int* d_diag_ptr = nullptr;
+  float* d_dinv32 = nullptr;
+  // solver tiles
+  std::vector<int> tile_row;   // ntiles+1
+  int* d_tile_row = nullptr;
+  int ntiles = 0;
+  // GMRES workspace
+  int kryl_alloc = 0;
+  double* d_V = nullptr;       // (m+1)*n
+  double* d_H = nullptr;       // (m+1)*(m+1) + 2*(m+1) givens + scale
+  double* d_partials = nullptr;
+  unsigned* d_counter = nullptr;
+  GmresStat* d_stat = nullptr;
+  GmresStat* h_stat = nullptr; // pinned
+  double* d_x = nullptr;
+  double* d_work = nullptr;    // n
+  // last measured times (ms)
+  double t_integrate_ms = 0, t_faces_ms = 0, t_solve_ms = 0;
+};
+
+int device_init(int device);
+hipStream_t stream();
+
+// kernels (mf_device.hip)
+int launch_zero(double* p, long long n);
+int launch_integrate(Problem& P, int precision, int write_elements,
+                     double* d_emat, int e0, int ecount);
+int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount);
+int launch_fill_jacobi(Problem& P);
+int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,
+                int ini_zero, double* h_x, int* iters, double* res_rel,
+                double* resid);
+int launch_spmv(Problem& P, const double* d_x, double* d_y);
+int build_tiles(Problem& P);
+int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,
+                 const int* d_diag_ptr, double* d_v);
+int launch_diag_ptr(Problem& P, int* d_dp);
+void kstats_enable(int on);
+void kstats_get(double* out);
+
+}  // namespace mf

@@ -1,0 +1,329 @@
+// Mesh construction: the jk ("gradmesh") half-structured prism format and
+// seeded box generators.  The 3D mesh built from a jk structure follows the
+// reference importer mmr_import_mesh_grad (mmd_prism/
+// mms_prism_input_gradmesh.c:29-600) and element node order of
+// mmr_el_node_coor (mmd_prism/mms_prism_datstr.c:365-540):
+//   node ID     = istr + layer*N2D
+//   element ID  = itri + layer*Ntri, nodes (n0,n1,n2) bottom then top
+//   face IDs    = horizontal faces itri + layer*Ntri (layer 0..L), then
+//                 vertical faces in creation order (triangle, edge), L each
+//   boundary    = faces with bc>0 in face-ID order; owner = face neighbour 0
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <random>
+#include <utility>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+static std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+int mesh_from_jk_struct(Mesh& m) {
+  const int N2 = m.jk_nno2d, Nt = m.jk_ntri, L = m.jk_nlay;
+  if (N2 <= 0 || Nt <= 0 || L <= 0) {
+    set_error("jk mesh: empty structure");
+    return -1;
+  }
+  const double dz = (m.jk_zt - m.jk_zb) / L;
+  m.nno = N2 * (L + 1);
+  m.coor.assign(3 * (size_t)m.nno, 0.0);
+  for (int istr = 0; istr < N2; istr++)
+    for (int lay = 0; lay <= L; lay++) {
+      size_t id = istr + (size_t)lay * N2;
+      m.coor[3 * id] = m.jk_xy[2 * istr];
+      m.coor[3 * id + 1] = m.jk_xy[2 * istr + 1];
+      m.coor[3 * id + 2] = m.jk_zb + lay * dz;
+    }
+  m.nel = Nt * L;
+  m.type.assign(m.nel, kPrism);
+  m.nodes.assign(6 * (size_t)m.nel, -1);
+  for (int t = 0; t < Nt; t++) {
+    const int* tr = &m.jk_tri[7 * (size_t)t];
+    if (tr[0] <= 0) {
+      set_error("jk mesh: only triangles (prisms) are supported");
+      return -1;
+    }
+    for (int lay = 0; lay < L; lay++) {
+      size_t e = t + (size_t)lay * Nt;
+      for (int i = 0; i < 3; i++) {
+        m.nodes[6 * e + i] = (tr[1 + i] - 1) + lay * N2;
+        m.nodes[6 * e + 3 + i] = (tr[1 + i] - 1) + (lay + 1) * N2;
+      }
+    }
+  }
+  // boundary faces in face-ID order
+  m.bf_elem.clear(); m.bf_side.clear(); m.bf_bc.clear();
+  if (m.jk_bottom_bc > 0)
+    for (int t = 0; t < Nt; t++) {
+      m.bf_elem.push_back(t); m.bf_side.push_back(0); m.bf_bc.push_back(m.jk_bottom_bc);
+    }
+  if (m.jk_top_bc > 0)
+    for (int t = 0; t < Nt; t++) {
+      m.bf_elem.push_back(t + (L - 1) * Nt); m.bf_side.push_back(1);
+      m.bf_bc.push_back(m.jk_top_bc);
+    }
+  // vertical faces: created by the first triangle (in jk order) that meets
+  // the edge; boundary edges (neig<0) always create one.
+  std::map<std::pair<int, int>, int> created;  // (min,max tri) -> 1
+  for (int t = 0; t < Nt; t++) {
+    const int* tr = &m.jk_tri[7 * (size_t)t];
+    for (int ino = 0; ino < 3; ino++) {
+      int neig = tr[4 + ino];
+      if (neig > 0) {
+        auto key = std::make_pair(std::min(t + 1, neig), std::max(t + 1, neig));
+        if (!created.count(key)) created[key] = 1;
+      } else if (neig < 0) {
+        for (int lay = 0; lay < L; lay++) {
+          m.bf_elem.push_back(t + lay * Nt);
+          m.bf_side.push_back(2 + ino);
+          m.bf_bc.push_back(-neig);
+        }
+      }
+    }
+  }
+  m.nbf = (int)m.bf_elem.size();
+  m.has_jk = true;
+  return 0;
+}
+
+int mesh_read_jk(const char* fname, Mesh& m) {
+  FILE* fp = fopen(fname, "r");
+  if (!fp) {
+    set_error(std::string("cannot open jk mesh file ") + fname);
+    return -1;
+  }
+  int mx[4];
+  int ok = fscanf(fp, "%d %d %d %d", &mx[0], &mx[1], &mx[2], &mx[3]) == 4;
+  ok = ok && fscanf(fp, "%d %lg %lg %d %d %d", &m.jk_nno2d, &m.jk_zb, &m.jk_zt,
+                    &m.jk_nlay, &m.jk_bottom_bc, &m.jk_top_bc) == 6;
+  if (ok && m.jk_nlay < 1) m.jk_nlay = 1;
+  if (ok) {
+    m.jk_xy.resize(2 * (size_t)m.jk_nno2d);
+    for (int i = 0; ok && i < m.jk_nno2d; i++)
+      ok = fscanf(fp, "%lg %lg", &m.jk_xy[2 * i], &m.jk_xy[2 * i + 1]) == 2;
+  }
+  ok = ok && fscanf(fp, "%d", &m.jk_ntri) == 1;
+  if (ok) {
+    m.jk_tri.resize(7 * (size_t)m.jk_ntri);
+    for (int t = 0; ok && t < m.jk_ntri; t++) {
+      int* tr = &m.jk_tri[7 * (size_t)t];
+      ok = fscanf(fp, "%d %d %d %d %d %d %d", &tr[0], &tr[1], &tr[2], &tr[3],
+                  &tr[4], &tr[5], &tr[6]) == 7;
+    }
+  }
+  fclose(fp);
+  if (!ok) {
+    set_error(std::string("malformed jk mesh file ") + fname);
+    return -1;
+  }
+  return mesh_from_jk_struct(m);
+}
+
+int mesh_write_jk(const Mesh& m, const char* fname) {
+  if (!m.has_jk) {
+    set_error("mesh has no jk structure");
+    return -1;
+  }
+  FILE* fp = fopen(fname, "w");
+  if (!fp) {
+    set_error(std::string("cannot write ") + fname);
+    return -1;
+  }
+  fprintf(fp, "%d %d %d %d\n", m.nno, 4 * m.nno, 5 * m.nel, m.nel);
+  fprintf(fp, "%d %.17g %.17g %d %d %d\n", m.jk_nno2d, m.jk_zb, m.jk_zt,
+          m.jk_nlay, m.jk_bottom_bc, m.jk_top_bc);
+  for (int i = 0; i < m.jk_nno2d; i++)
+    fprintf(fp, "%.17g %.17g\n", m.jk_xy[2 * i], m.jk_xy[2 * i + 1]);
+  fprintf(fp, "%d\n", m.jk_ntri);
+  for (int t = 0; t < m.jk_ntri; t++) {
+    const int* tr = &m.jk_tri[7 * (size_t)t];
+    fprintf(fp, "%d %d %d %d %d %d %d\n", tr[0], tr[1], tr[2], tr[3], tr[4],
+            tr[5], tr[6]);
+  }
+  fclose(fp);
+  return 0;
+}
+
+// Seeded permutation (Fisher-Yates over mt19937_64, uniform via modulo of
+// 64-bit draws; deterministic across platforms).
+static std::vector<int> seeded_perm(int n, std::mt19937_64& rng) {
+  std::vector<int> p(n);
+  for (int i = 0; i < n; i++) p[i] = i;
+  for (int i = n - 1; i > 0; i--) {
+    int j = (int)(rng() % (uint64_t)(i + 1));
+    std::swap(p[i], p[j]);
+  }
+  return p;
+}
+
+static double unit_draw(std::mt19937_64& rng) {
+  return (double)(rng() >> 11) * (1.0 / 9007199254740992.0);  // [0,1)
+}
+
+static void jitter_interior(Mesh& m, double lx, double ly, double lz, int nx,
+                            int ny, int nz, double jitter, uint64_t seed) {
+  if (jitter <= 0) return;
+  std::mt19937_64 rng(seed ^ 0x9e3779b97f4a7c15ULL);
+  const double h[3] = {lx / nx, ly / ny, lz / nz}, L[3] = {lx, ly, lz};
+  const double eps = 1e-9;
+  for (int v = 0; v < m.nno; v++) {
+    double* x = &m.coor[3 * (size_t)v];
+    bool interior = true;
+    for (int d = 0; d < 3; d++)
+      if (x[d] < eps * L[d] || x[d] > L[d] * (1 - eps)) interior = false;
+    double r[3] = {unit_draw(rng), unit_draw(rng), unit_draw(rng)};
+    if (!interior) continue;
+    for (int d = 0; d < 3; d++) x[d] += (2.0 * r[d] - 1.0) * jitter * h[d];
+  }
+}
+
+int mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
+                   uint64_t seed, double jitter, int bc, Mesh& m) {
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error("box mesh: nx, ny, nz must be >= 1");
+    return -1;
+  }
+  std::mt19937_64 rng(seed);
+  const int N2 = (nx + 1) * (ny + 1), Nt = 2 * nx * ny;
+  std::vector<int> pn = seeded_perm(N2, rng);  // grid node -> jk id-1
+  std::vector<int> pt = seeded_perm(Nt, rng);  // grid tri  -> jk id-1
+  std::vector<int> inv_t(Nt);
+  for (int i = 0; i < Nt; i++) inv_t[pt[i]] = i;
+  // grid triangles, counter-clockwise, split by the (0,0)-(1,1) diagonal
+  std::vector<int> tri(3 * (size_t)Nt);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      int a = i + j * (nx + 1), b = a + 1, c = a + (nx + 1) + 1, d = a + (nx + 1);
+      int t = 2 * (i + j * nx);
+      tri[3 * t] = a; tri[3 * t + 1] = b; tri[3 * t + 2] = c;
+      tri[3 * (t + 1)] = a; tri[3 * (t + 1) + 1] = c; tri[3 * (t + 1) + 2] = d;
+    }
+  // edge -> triangles
+  std::map<std::pair<int, int>, std::vector<int>> em;
+  for (int t = 0; t < Nt; t++)
+    for (int e = 0; e < 3; e++) {
+      int u = tri[3 * t + e], v = tri[3 * t + (e + 1) % 3];
+      em[std::make_pair(std::min(u, v), std::max(u, v))].push_back(t);
+    }
+  m = Mesh();
+  m.jk_nno2d = N2; m.jk_ntri = Nt; m.jk_nlay = nz;
+  m.jk_zb = 0.0; m.jk_zt = lz; m.jk_bottom_bc = bc; m.jk_top_bc = bc;
+  m.jk_xy.assign(2 * (size_t)N2, 0.0);
+  for (int j = 0; j <= ny; j++)
+    for (int i = 0; i <= nx; i++) {
+      int g = i + j * (nx + 1);
+      m.jk_xy[2 * (size_t)pn[g]] = lx * i / nx;
+      m.jk_xy[2 * (size_t)pn[g] + 1] = ly * j / ny;
+    }
+  m.jk_tri.assign(7 * (size_t)Nt, 0);
+  for (int jt = 0; jt < Nt; jt++) {
+    int t = inv_t[jt];
+    int* out = &m.jk_tri[7 * (size_t)jt];
+    out[0] = 1;
+    for (int e = 0; e < 3; e++) {
+      int u = tri[3 * t + e], v = tri[3 * t + (e + 1) % 3];
+      out[1 + e] = pn[u] + 1;
+      const auto& owners = em[std::make_pair(std::min(u, v), std::max(u, v))];
+      int nb = -bc;
+      for (int o : owners)
+        if (o != t) nb = pt[o] + 1;
+      out[4 + e] = nb;
+    }
+  }
+  int rc = mesh_from_jk_struct(m);
+  if (rc < 0) return rc;
+  jitter_interior(m, lx, ly, lz, nx, ny, nz, jitter, seed);
+  return 0;
+}
+
+// Tetrahedral box: every hex cell split into 6 tetrahedra around its main
+// diagonal (Kuhn split, conforming).  Node and element IDs are permuted by the
+// seed; boundary faces are listed element by element, local face order
+// (mmd_prism/mms_prism_intf.c:97 face-node table {0,1,2},{0,1,3},{0,2,3},{1,2,3}).
+int mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
+                   uint64_t seed, double jitter, int bc, Mesh& m) {
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error("box mesh: nx, ny, nz must be >= 1");
+    return -1;
+  }
+  std::mt19937_64 rng(seed);
+  const int NN = (nx + 1) * (ny + 1) * (nz + 1), NE = 6 * nx * ny * nz;
+  std::vector<int> pn = seeded_perm(NN, rng);
+  std::vector<int> pe = seeded_perm(NE, rng);
+  m = Mesh();
+  m.nno = NN;
+  m.coor.assign(3 * (size_t)NN, 0.0);
+  auto gid = [&](int i, int j, int k) { return i + (nx + 1) * (j + (ny + 1) * k); };
+  for (int k = 0; k <= nz; k++)
+    for (int j = 0; j <= ny; j++)
+      for (int i = 0; i <= nx; i++) {
+        size_t v = pn[gid(i, j, k)];
+        m.coor[3 * v] = lx * i / nx;
+        m.coor[3 * v + 1] = ly * j / ny;
+        m.coor[3 * v + 2] = lz * k / nz;
+      }
+  m.nel = NE;
+  m.type.assign(NE, kTetra);
+  m.nodes.assign(6 * (size_t)NE, -1);
+  // Kuhn paths from corner 0 to corner 7 through the 6 axis permutations
+  static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2},
+                                  {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++)
+        for (int p = 0; p < 6; p++) {
+          int c[3] = {i, j, k};
+          int vv[4];
+          vv[0] = gid(c[0], c[1], c[2]);
+          for (int s = 0; s < 3; s++) {
+            c[perms[p][s]]++;
+            vv[s + 1] = gid(c[0], c[1], c[2]);
+          }
+          // orient positively
+          double x[4][3];
+          for (int a = 0; a < 4; a++) {
+            int cc[3] = {vv[a] % (nx + 1), (vv[a] / (nx + 1)) % (ny + 1),
+                         vv[a] / ((nx + 1) * (ny + 1))};
+            x[a][0] = cc[0]; x[a][1] = cc[1]; x[a][2] = cc[2];
+          }
+          double a1[3], a2[3], a3[3];
+          for (int d = 0; d < 3; d++) {
+            a1[d] = x[1][d] - x[0][d]; a2[d] = x[2][d] - x[0][d]; a3[d] = x[3][d] - x[0][d];
+          }
+          double det = a1[0] * (a2[1] * a3[2] - a2[2] * a3[1]) -
+                       a1[1] * (a2[0] * a3[2] - a2[2] * a3[0]) +
+                       a1[2] * (a2[0] * a3[1] - a2[1] * a3[0]);
+          if (det < 0) std::swap(vv[2], vv[3]);
+          size_t e = pe[((k * ny + j) * nx + i) * 6 + p];
+          for (int a = 0; a < 4; a++) m.nodes[6 * e + a] = pn[vv[a]];
+        }
+  // boundary faces
+  static const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  const double L3[3] = {lx, ly, lz};
+  for (int e = 0; e < NE; e++)
+    for (int f = 0; f < 4; f++) {
+      for (int d = 0; d < 3; d++) {
+        bool lo = true, hi = true;
+        for (int a = 0; a < 3; a++) {
+          double xv = m.coor[3 * (size_t)m.nodes[6 * (size_t)e + fn[f][a]] + d];
+          if (std::fabs(xv) > 1e-12 * L3[d]) lo = false;
+          if (std::fabs(xv - L3[d]) > 1e-12 * L3[d]) hi = false;
+        }
+        if (lo || hi) {
+          m.bf_elem.push_back(e); m.bf_side.push_back(f); m.bf_bc.push_back(bc);
+          break;
+        }
+      }
+    }
+  m.nbf = (int)m.bf_elem.size();
+  jitter_interior(m, lx, ly, lz, nx, ny, nz, jitter, seed);
+  return 0;
+}
+
+}  // namespace mf

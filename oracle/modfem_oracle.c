@@ -1,0 +1,792 @@
+/*
+ * modfem_oracle.c -- CPU restatement of ModFEM's std-linear
+ * integrate -> assemble -> MKB GMRES path (test infrastructure only; see the
+ * header).  Paths cited as file:line are relative to modfem2017/src/.
+ *
+ * Parity status of each part (what pins it; tests/test_oracle_*.py):
+ *   quadrature, shape functions, Jacobians  -- compared with the reference's
+ *       own apr_set_quadr_3D / apr_elem_calc_3D compiled from its sources
+ *       (oracle/_ref) and with closed forms (partition of unity, volumes);
+ *   element integral                          -- closed-form P1 tet stiffness,
+ *       zero row sums, patch test, reference Gauss-point data;
+ *   penalty faces                             -- closed-form face mass;
+ *   DOF list, CRS pattern, assembly tables,
+ *   assembly, Jacobi, BJ sweep, residual      -- the reference's own
+ *       las_crs_intf.c compiled from its sources (oracle/_ref), bit-exact;
+ *   GMRES                                     -- dense LU solution on small
+ *       systems, Krylov invariance, the reference's field dump
+ *       (ctest/conv_diff/Laplace_std_in_box_opencl/comparison/field_box.dmp).
+ *   drotg/drot/dtrsv/dgemv                    -- the BLAS the reference links
+ *       is not in the tree (parity unpinned at that boundary); restated from
+ *       the netlib reference algorithms and checked by their identities.
+ */
+#include "modfem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+void orc_set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+/* ---------------------------------------------------------------------
+ * Quadrature.  Literals are the 15-digit values of
+ * apd_std_lin/aps_gauss_util_std.c:152-299; the point ordering follows
+ * apr_set_quadr_3D (aps_std_util.c:1400-1434: z outer, triangle inner,
+ * triangle coordinates taken from barycentric slots 1 and 2).
+ * The tetrahedral rule reads the 4-coordinate table with stride 3 exactly as
+ * aps_std_util.c:1377-1383 does, which yields the four points below.
+ * ------------------------------------------------------------------- */
+static const double G2 = 0.577350269189626;
+static const double T1 = 0.166666666666667, T2 = 0.666666666666667;
+static const double TW = 0.333333333333333;
+static const double TA = 0.585410196624969, TB = 0.138196601125011;
+
+int orc_quadr_3d(int el_type, double *xg, double *wg) {
+  if (el_type == ORC_TETRA) {
+    const double p[4][3] = {{TA, TB, TB}, {TB, TB, TA}, {TB, TB, TB}, {TB, TA, TB}};
+    for (int k = 0; k < 4; k++) {
+      xg[3 * k] = p[k][0]; xg[3 * k + 1] = p[k][1]; xg[3 * k + 2] = p[k][2];
+      wg[k] = 0.25;
+    }
+    return 4;
+  }
+  const double tri[3][2] = {{T1, T1}, {T1, T2}, {T2, T1}};
+  const double zg[2] = {-G2, G2};
+  for (int ki = 0; ki < 2; ki++)
+    for (int kj = 0; kj < 3; kj++) {
+      int g = ki * 3 + kj;
+      xg[3 * g] = tri[kj][0];
+      xg[3 * g + 1] = tri[kj][1];
+      xg[3 * g + 2] = zg[ki];
+      wg[g] = 0.5 * TW * 1.0;
+    }
+  return 6;
+}
+
+/* apr_set_quadr_2D (aps_std_util.c:1441-1530): triangles use the 3-point
+   rule with weights 0.5*w, quads 2x2 Gauss on [-1,1]^2 with weights 0.5*w*w */
+int orc_quadr_2d(int fa_is_quad, double *xg, double *wg) {
+  if (fa_is_quad) {
+    const double g[2] = {-G2, G2};
+    for (int ki = 0; ki < 2; ki++)
+      for (int kj = 0; kj < 2; kj++) {
+        xg[2 * (ki * 2 + kj)] = g[ki];
+        xg[2 * (ki * 2 + kj) + 1] = g[kj];
+        wg[ki * 2 + kj] = 0.5 * 1.0 * 1.0;
+      }
+    return 4;
+  }
+  const double tri[3][2] = {{T1, T1}, {T1, T2}, {T2, T1}};
+  for (int k = 0; k < 3; k++) {
+    xg[2 * k] = tri[k][0];
+    xg[2 * k + 1] = tri[k][1];
+    wg[k] = 0.5 * TW;
+  }
+  return 3;
+}
+
+/* ut_mat3_inv (aps_std_util.c:2770-2822): cofactor inverse, returns det */
+double orc_mat3_inv(const double *m, double *mi) {
+  double rjac = m[0] * (m[4] * m[8] - m[7] * m[5]) +
+                m[3] * (m[7] * m[2] - m[1] * m[8]) +
+                m[6] * (m[1] * m[5] - m[4] * m[2]);
+  double r = 1.0 / rjac;
+  double s0, s1, s2;
+  s0 = m[4] * m[8] - m[7] * m[5];
+  s1 = -m[3] * m[8] + m[6] * m[5];
+  s2 = m[3] * m[7] - m[6] * m[4];
+  mi[0] = s0 * r; mi[3] = s1 * r; mi[6] = s2 * r;
+  s0 = m[7] * m[2] - m[1] * m[8];
+  s1 = m[0] * m[8] - m[6] * m[2];
+  s2 = -m[0] * m[7] + m[6] * m[1];
+  mi[1] = s0 * r; mi[4] = s1 * r; mi[7] = s2 * r;
+  s0 = m[1] * m[5] - m[4] * m[2];
+  s1 = -m[0] * m[5] + m[3] * m[2];
+  s2 = m[0] * m[4] - m[3] * m[1];
+  mi[2] = s0 * r; mi[5] = s1 * r; mi[8] = s2 * r;
+  return rjac;
+}
+
+static double vec3_len(const double *v) {
+  return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+}
+
+/* Element calculations.  Prism: apr_shape_fun_3D_std_lin_prism
+   (aps_std_util.c:463-511) and apr_elem_calc_3D_prism (:520-820); tetra:
+   apr_shape_fun_3D_std_lin_tetra (:826-867), apr_elem_calc_3D_tetra
+   (:869-1226).  control==2: volume (returns |det|); control>=3: face
+   control-3 (returns the area element, vec_nor optional). */
+double orc_elem_calc_3d(int el_type, int control, const double *eta,
+                        const double *nc, double *phi, double *dphix,
+                        double *dphiy, double *dphiz, double *xcoor,
+                        double *vec_nor) {
+  double J[9], Ji[9], det;
+  int ns;
+  if (el_type == ORC_TETRA) {
+    ns = 4;
+    phi[0] = 1.0 - eta[0] - eta[1] - eta[2];
+    phi[1] = eta[0]; phi[2] = eta[1]; phi[3] = eta[2];
+    dphix[0] = -1.0; dphix[1] = 1.0; dphix[2] = 0.0; dphix[3] = 0.0;
+    dphiy[0] = -1.0; dphiy[1] = 0.0; dphiy[2] = 1.0; dphiy[3] = 0.0;
+    dphiz[0] = -1.0; dphiz[1] = 0.0; dphiz[2] = 0.0; dphiz[3] = 1.0;
+    J[0] = nc[3] - nc[0]; J[3] = nc[4] - nc[1]; J[6] = nc[5] - nc[2];
+    J[1] = nc[6] - nc[0]; J[4] = nc[7] - nc[1]; J[7] = nc[8] - nc[2];
+    J[2] = nc[9] - nc[0]; J[5] = nc[10] - nc[1]; J[8] = nc[11] - nc[2];
+    if (xcoor) {
+      xcoor[0] = nc[0] + eta[0] * J[0] + eta[1] * J[1] + eta[2] * J[2];
+      xcoor[1] = nc[1] + eta[0] * J[3] + eta[1] * J[4] + eta[2] * J[5];
+      xcoor[2] = nc[2] + eta[0] * J[6] + eta[1] * J[7] + eta[2] * J[8];
+    }
+    det = orc_mat3_inv(J, Ji) * 0.16666666666666666666666666666667;
+  } else {
+    ns = 6;
+    const double e0 = eta[0], e1 = eta[1], e2 = eta[2];
+    phi[0] = (-e2 + 1) / 2. * (1 - e0 - e1);
+    phi[1] = (-e2 + 1) / 2. * e0;
+    phi[2] = (-e2 + 1) / 2. * e1;
+    phi[3] = (e2 + 1) / 2. * (1 - e0 - e1);
+    phi[4] = (e2 + 1) / 2. * e0;
+    phi[5] = (e2 + 1) / 2. * e1;
+    dphix[0] = (e2 - 1) / 2.; dphix[1] = (-e2 + 1) / 2.; dphix[2] = 0.0;
+    dphix[3] = (-e2 - 1) / 2.; dphix[4] = (e2 + 1) / 2.; dphix[5] = 0.0;
+    dphiy[0] = (e2 - 1) / 2.; dphiy[1] = 0.0; dphiy[2] = (-e2 + 1) / 2.;
+    dphiy[3] = (-e2 - 1) / 2.; dphiy[4] = 0.0; dphiy[5] = (e2 + 1) / 2.;
+    dphiz[0] = -0.5 * (1 - e0 - e1); dphiz[1] = -0.5 * e0; dphiz[2] = -0.5 * e1;
+    dphiz[3] = 0.5 * (1 - e0 - e1); dphiz[4] = 0.5 * e0; dphiz[5] = 0.5 * e1;
+    double gp[6], gx[6], gy[6], gz[6];
+    gp[0] = (1.0 - e0 - e1) * (1.0 - e2) / 2.0;
+    gp[1] = e0 * (1.0 - e2) / 2.0;
+    gp[2] = e1 * (1.0 - e2) / 2.0;
+    gp[3] = (1.0 - e0 - e1) * (1.0 + e2) / 2.0;
+    gp[4] = e0 * (1.0 + e2) / 2.0;
+    gp[5] = e1 * (1.0 + e2) / 2.0;
+    if (xcoor) {
+      xcoor[0] = 0; xcoor[1] = 0; xcoor[2] = 0;
+      for (int i = 0; i < 6; i++) {
+        xcoor[0] += nc[3 * i] * gp[i];
+        xcoor[1] += nc[3 * i + 1] * gp[i];
+        xcoor[2] += nc[3 * i + 2] * gp[i];
+      }
+    }
+    gx[0] = -(1.0 - e2) / 2.0; gx[1] = (1.0 - e2) / 2.0; gx[2] = 0.0;
+    gx[3] = -(1.0 + e2) / 2.0; gx[4] = (1.0 + e2) / 2.0; gx[5] = 0.0;
+    gy[0] = -(1.0 - e2) / 2.0; gy[1] = 0.0; gy[2] = (1.0 - e2) / 2.0;
+    gy[3] = -(1.0 + e2) / 2.0; gy[4] = 0.0; gy[5] = (1.0 + e2) / 2.0;
+    gz[0] = -(1.0 - e0 - e1) / 2.0; gz[1] = -e0 / 2.0; gz[2] = -e1 / 2.0;
+    gz[3] = (1.0 - e0 - e1) / 2.0; gz[4] = e0 / 2.0; gz[5] = e1 / 2.0;
+    for (int k = 0; k < 9; k++) J[k] = 0.0;
+    for (int i = 0; i < 6; i++) {
+      J[0] += nc[3 * i] * gx[i];
+      J[1] += nc[3 * i] * gy[i];
+      J[2] += nc[3 * i] * gz[i];
+      J[3] += nc[3 * i + 1] * gx[i];
+      J[4] += nc[3 * i + 1] * gy[i];
+      J[5] += nc[3 * i + 1] * gz[i];
+      J[6] += nc[3 * i + 2] * gx[i];
+      J[7] += nc[3 * i + 2] * gy[i];
+      J[8] += nc[3 * i + 2] * gz[i];
+    }
+    det = orc_mat3_inv(J, Ji);
+  }
+  for (int i = 0; i < ns; i++) {
+    double daux = dphix[i] * Ji[0] + dphiy[i] * Ji[3] + dphiz[i] * Ji[6];
+    double faux = dphix[i] * Ji[1] + dphiy[i] * Ji[4] + dphiz[i] * Ji[7];
+    dphiz[i] = dphix[i] * Ji[2] + dphiy[i] * Ji[5] + dphiz[i] * Ji[8];
+    dphix[i] = daux;
+    dphiy[i] = faux;
+  }
+  if (control == 2) return fabs(det);
+  double ds[3];
+  if (el_type == ORC_TETRA) {
+    static const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    int f = control - 3;
+    const double *p0 = nc + 3 * fn[f][0], *p1 = nc + 3 * fn[f][1],
+                 *p2 = nc + 3 * fn[f][2];
+    double v1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    double v2[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    ds[0] = v1[1] * v2[2] - v1[2] * v2[1];
+    ds[1] = v1[2] * v2[0] - v1[0] * v2[2];
+    ds[2] = v1[0] * v2[1] - v1[1] * v2[0];
+  } else {
+    switch (control) {
+      case 3:
+        ds[0] = -J[3] * J[7] + J[6] * J[4];
+        ds[1] = -J[6] * J[1] + J[0] * J[7];
+        ds[2] = -J[0] * J[4] + J[3] * J[1];
+        break;
+      case 4:
+        ds[0] = J[3] * J[7] - J[6] * J[4];
+        ds[1] = J[6] * J[1] - J[0] * J[7];
+        ds[2] = J[0] * J[4] - J[3] * J[1];
+        break;
+      case 5:
+        ds[0] = J[3] * J[8] - J[6] * J[5];
+        ds[1] = J[6] * J[2] - J[0] * J[8];
+        ds[2] = J[0] * J[5] - J[3] * J[2];
+        break;
+      case 6:
+        ds[0] = (J[4] - J[3]) * J[8] - (J[7] - J[6]) * J[5];
+        ds[1] = (J[7] - J[6]) * J[2] - (J[1] - J[0]) * J[8];
+        ds[2] = (J[1] - J[0]) * J[5] - (J[4] - J[3]) * J[2];
+        break;
+      default:
+        ds[0] = J[5] * J[7] - J[8] * J[4];
+        ds[1] = J[8] * J[1] - J[2] * J[7];
+        ds[2] = J[2] * J[4] - J[5] * J[1];
+        break;
+    }
+  }
+  det = vec3_len(ds);
+  if (vec_nor) {
+    vec_nor[0] = ds[0] / det; vec_nor[1] = ds[1] / det; vec_nor[2] = ds[2] / det;
+  }
+  return det;
+}
+
+/* exact solutions of the DIFF_IN_CUBE / DIFF_IN_BOX test problems
+   (pdd_conv_diff/main/pds_conv_diff_util.c, pdr_exact_sol) */
+static void exact_sol(int model, const double *x, double *u, double *lapl) {
+  double a1 = (model == ORC_SRC_DIFF_IN_BOX) ? 100.0 : 1.0;
+  double X = x[0], Y = x[1], Z = x[2];
+  double e = exp((-X * X - Y * Y - Z * Z) / a1);
+  if (u) *u = e;
+  if (lapl)
+    *lapl = -(2.0 / a1) *
+            ((1 - 2 * X * X / a1) + (1 - 2 * Y * Y / a1) + (1 - 2 * Z * Z / a1)) * e;
+}
+
+/* apr_num_int_el, nreq==1 (aps_std_lin_intf.c:1537-2870).  Term order and
+   branch selection follow the reference: mass (ind[1]); diffusion full 9-term
+   branch when ind[10] && ind[2] (:2112-2136), diagonal 3-term branch when
+   ind[10] && !ind[2] (:2209-2224); convection b (ind[13], :2346-2360); t
+   terms (ind[16]); reaction c (ind[17], :2612-2625); rhs q (ind[21]) and s
+   (ind[22]) (:2672-2810).  Stiff is column-major stiff[u*ns+w]. */
+int orc_num_int_el(int el_type, const double *nc, const int *ind,
+                   const double *cf, int src_model, double *stiff,
+                   double *rhs) {
+  double xg[18], wg[6], phi[6], dx[6], dy[6], dz[6], xcoor[3];
+  int ng = orc_quadr_3d(el_type, xg, wg);
+  int ns = (el_type == ORC_TETRA) ? 4 : 6;
+  const double *a = cf + ORC_C_A, *b = cf + ORC_C_B, *t = cf + ORC_C_T;
+  const double *q = cf + ORC_C_Q;
+  for (int ki = 0; ki < ng; ki++) {
+    double determ = orc_elem_calc_3d(el_type, 2, &xg[3 * ki], nc, phi, dx, dy,
+                                     dz, xcoor, NULL);
+    double vol = determ * wg[ki];
+    double sval = cf[ORC_C_S];
+    if (src_model != ORC_SRC_CONST) {
+      double lapl;
+      exact_sol(src_model, xcoor, NULL, &lapl);
+      sval = -lapl;
+    }
+    int kk;
+    if (stiff) {
+      if (ind[ORC_IND_M]) {
+        kk = 0;
+        for (int u = 0; u < ns; u++) {
+          for (int w = 0; w < ns; w++)
+            stiff[kk + w] += cf[ORC_C_M] * phi[u] * phi[w] * vol;
+          kk += ns;
+        }
+      }
+      if (ind[ORC_IND_AZZ]) {
+        if (ind[ORC_IND_AXX]) {
+          kk = 0;
+          for (int u = 0; u < ns; u++) {
+            for (int w = 0; w < ns; w++)
+              stiff[kk + w] += (a[0] * dx[u] * dx[w] + a[1] * dy[u] * dx[w] +
+                                a[2] * dz[u] * dx[w] + a[3] * dx[u] * dy[w] +
+                                a[4] * dy[u] * dy[w] + a[5] * dz[u] * dy[w] +
+                                a[6] * dx[u] * dz[w] + a[7] * dy[u] * dz[w] +
+                                a[8] * dz[u] * dz[w]) * vol;
+            kk += ns;
+          }
+        } else {
+          kk = 0;
+          for (int u = 0; u < ns; u++) {
+            for (int w = 0; w < ns; w++)
+              stiff[kk + w] += (a[0] * dx[u] * dx[w] + a[4] * dy[u] * dy[w] +
+                                a[8] * dz[u] * dz[w]) * vol;
+            kk += ns;
+          }
+        }
+      }
+      if (ind[ORC_IND_BZ]) {
+        kk = 0;
+        for (int u = 0; u < ns; u++) {
+          for (int w = 0; w < ns; w++)
+            stiff[kk + w] += (b[0] * dx[u] * phi[w] + b[1] * dy[u] * phi[w] +
+                              b[2] * dz[u] * phi[w]) * vol;
+          kk += ns;
+        }
+      }
+      if (ind[ORC_IND_TZ]) {
+        kk = 0;
+        for (int u = 0; u < ns; u++) {
+          for (int w = 0; w < ns; w++)
+            stiff[kk + w] += (t[0] * phi[u] * dx[w] + t[1] * phi[u] * dy[w] +
+                              t[2] * phi[u] * dz[w]) * vol;
+          kk += ns;
+        }
+      }
+      if (ind[ORC_IND_C]) {
+        kk = 0;
+        for (int u = 0; u < ns; u++) {
+          for (int w = 0; w < ns; w++)
+            stiff[kk + w] += cf[ORC_C_C] * phi[u] * phi[w] * vol;
+          kk += ns;
+        }
+      }
+    }
+    if (rhs) {
+      if (ind[ORC_IND_QZ]) {
+        for (int w = 0; w < ns; w++)
+          rhs[w] += (q[0] * dx[w] + q[1] * dy[w] + q[2] * dz[w]) * vol;
+      }
+      if (ind[ORC_IND_S]) {
+        for (int w = 0; w < ns; w++) rhs[w] += sval * phi[w] * vol;
+      }
+    }
+  }
+  return ns;
+}
+
+/* face-local -> element-local coordinates for the first (owning) neighbour,
+   mmr_fa_elem_coor (mmd_prism/mms_prism_datstr.c:1562-1700); tetra faces
+   are parametrised over their MMC_FACE_NODES_FOR_TETRA vertices. */
+static void face_to_elem(int el_type, int side, const double *c, double *eta) {
+  if (el_type == ORC_TETRA) {
+    static const double v[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    static const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    const double *p0 = v[fn[side][0]], *p1 = v[fn[side][1]], *p2 = v[fn[side][2]];
+    for (int d = 0; d < 3; d++)
+      eta[d] = p0[d] + c[0] * (p1[d] - p0[d]) + c[1] * (p2[d] - p0[d]);
+    return;
+  }
+  switch (side) {
+    case 0: eta[0] = c[1]; eta[1] = c[0]; eta[2] = -1.0; break;
+    case 1: eta[0] = c[0]; eta[1] = c[1]; eta[2] = 1.0; break;
+    case 2: eta[0] = 0.5 * (c[0] + 1.0); eta[1] = 0.0; eta[2] = c[1]; break;
+    case 3:
+      eta[0] = 0.5 * (-c[0] + 1.0); eta[1] = 1.0 - eta[0]; eta[2] = c[1];
+      break;
+    default: eta[0] = 0.0; eta[1] = 0.5 * (-c[0] + 1.0); eta[2] = c[1]; break;
+  }
+}
+
+/* Dirichlet penalty on a boundary face (pds_conv_diff_weakform_std.c:
+   253-629): K += 1e7 phi_j phi_i dA, F += 1e7 g phi_i dA on the owning
+   element's full local matrix (column-major). */
+int orc_face_penalty(int el_type, int side, const double *nc, double g_const,
+                     int src_model, double *stiff, double *rhs) {
+  int quad = (el_type == ORC_PRISM && side >= 2);
+  double xg[8], wg[4], eta[3], phi[6], dx[6], dy[6], dz[6], xcoor[3], vn[3];
+  int ng = orc_quadr_2d(quad, xg, wg);
+  int ns = (el_type == ORC_TETRA) ? 4 : 6;
+  const double penalty = 1.0e7;
+  for (int ki = 0; ki < ng; ki++) {
+    face_to_elem(el_type, side, &xg[2 * ki], eta);
+    double determ = orc_elem_calc_3d(el_type, 3 + side, eta, nc, phi, dx, dy,
+                                     dz, xcoor, vn);
+    double area = determ * wg[ki];
+    double g = g_const;
+    if (src_model != ORC_SRC_CONST) exact_sol(src_model, xcoor, &g, NULL);
+    int kk = 0;
+    for (int j = 0; j < ns; j++) {
+      for (int i = 0; i < ns; i++)
+        stiff[kk + i] += penalty * phi[j] * phi[i] * area;
+      kk += ns;
+    }
+    for (int i = 0; i < ns; i++) rhs[i] += penalty * g * phi[i] * area;
+  }
+  return ns;
+}
+
+/* ---------------------------------------------------------------------
+ * Lists and DOF structures
+ * ------------------------------------------------------------------- */
+static int nodes_of(int t) { return t == ORC_TETRA ? 4 : 6; }
+
+/* utr_get_list_dof_ent, std-linear branch (uts_ls_intf.c:229-398): DOF
+   entities are vertices in first-appearance order over element int-ents in
+   list order, element-local node order within each element. */
+int orc_dof_list(int nel, const int *el_type, const int *el_nodes, int nno,
+                 int *dof_node, int *node_dof) {
+  for (int i = 0; i < nno; i++) node_dof[i] = -1;
+  int n = 0;
+  for (int e = 0; e < nel; e++) {
+    int k = nodes_of(el_type[e]);
+    for (int i = 0; i < k; i++) {
+      int v = el_nodes[6 * e + i];
+      if (node_dof[v] < 0) {
+        node_dof[v] = n;
+        dof_node[n] = v;
+        n++;
+      }
+    }
+  }
+  return n;
+}
+
+static int cmp_int(const void *a, const void *b) {
+  int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+/* neighbour block sets (sis_mkb_intf.c:762-984; boundary faces reuse the
+   owning element's DOF list, so elements suffice), sorted ascending
+   (:1107-1142), then the CRS layout of lar_allocate_SM_and_LV_crs
+   (las_crs_intf.c:184-250): strictly ascending columns with the diagonal
+   inserted in place. */
+long orc_crs_pattern(int nel, const int *el_type, const int *el_nodes,
+                     const int *node_dof, int n, int *row_ptr, int *col_ind) {
+  /* count candidate (row, col) pairs per row */
+  int *cnt = (int *)calloc((size_t)n + 1, sizeof(int));
+  for (int e = 0; e < nel; e++) {
+    int k = nodes_of(el_type[e]);
+    for (int i = 0; i < k; i++) cnt[node_dof[el_nodes[6 * e + i]]] += k;
+  }
+  long *off = (long *)malloc(((size_t)n + 1) * sizeof(long));
+  off[0] = 0;
+  for (int i = 0; i < n; i++) off[i + 1] = off[i] + cnt[i];
+  int *buf = (int *)malloc((size_t)off[n] * sizeof(int) + 1);
+  for (int i = 0; i < n; i++) cnt[i] = 0;
+  for (int e = 0; e < nel; e++) {
+    int k = nodes_of(el_type[e]);
+    for (int i = 0; i < k; i++) {
+      int r = node_dof[el_nodes[6 * e + i]];
+      for (int j = 0; j < k; j++) buf[off[r] + cnt[r]++] = node_dof[el_nodes[6 * e + j]];
+    }
+  }
+  long nnz = 0;
+  for (int r = 0; r < n; r++) {
+    int *p = buf + off[r];
+    int m = cnt[r];
+    qsort(p, (size_t)m, sizeof(int), cmp_int);
+    int u = 0;
+    for (int i = 0; i < m; i++)
+      if (u == 0 || p[i] != p[u - 1]) p[u++] = p[i];
+    /* u distinct columns including the diagonal itself */
+    if (row_ptr) row_ptr[r] = (int)nnz;
+    if (col_ind) memcpy(col_ind + nnz, p, (size_t)u * sizeof(int));
+    nnz += u;
+  }
+  if (row_ptr) row_ptr[n] = (int)nnz;
+  free(buf); free(off); free(cnt);
+  return nnz;
+}
+
+/* lar_fill_assembly_table_int_ent_crs (las_crs_intf.c:354-432) for every
+   int-ent in list order (elements, then boundary faces which reuse the
+   owning element's DOF list): table[j+i*k] = CRS position of
+   (row L[j], col L[i]); table[k*k+j] = L[j]. */
+long orc_assembly_tables(int nel, const int *el_type, const int *el_nodes,
+                         int nbf, const int *bf_elem, const int *node_dof,
+                         const int *row_ptr, const int *col_ind,
+                         long *table_ptr, int *table) {
+  long pos = 0;
+  int nint = nel + nbf;
+  for (int ie = 0; ie < nint; ie++) {
+    int e = ie < nel ? ie : bf_elem[ie - nel];
+    int k = nodes_of(el_type[e]);
+    if (table_ptr) table_ptr[ie] = pos;
+    if (table) {
+      int L[6];
+      for (int i = 0; i < k; i++) L[i] = node_dof[el_nodes[6 * e + i]];
+      for (int i = 0; i < k; i++) {
+        int col = L[i];
+        for (int j = 0; j < k; j++) {
+          int row = L[j];
+          int found = -1;
+          for (int c = row_ptr[row]; c < row_ptr[row + 1]; c++)
+            if (col_ind[c] == col) { found = c; break; }
+          table[pos + j + i * k] = found;
+        }
+      }
+      for (int j = 0; j < k; j++) table[pos + k * k + j] = L[j];
+    }
+    pos += (long)(k + 1) * k;
+  }
+  if (table_ptr) table_ptr[nint] = pos;
+  return pos;
+}
+
+static void gather_coor(const double *all, const int *nodes, int k, double *nc) {
+  for (int i = 0; i < k; i++)
+    for (int d = 0; d < 3; d++) nc[3 * i + d] = all[3 * nodes[i] + d];
+}
+
+/* sequential create+assemble (sis_mkb_intf.c:1843-1866): element int-ents in
+   ID order then faces; each entity's matrix is zeroed (pds_conv_diff_
+   weakform_std.c:161-173), integrated, and added through its table
+   (las_crs_intf.c:440-513).  Element integrals run in parallel into
+   per-element buffers; the scatter stays sequential so sums keep the
+   reference order. */
+int orc_integrate_assemble(int nel, const int *el_type, const int *el_nodes,
+                           const double *coor, const int *ind,
+                           const double *coef_el, int src_model, int nbf,
+                           const int *bf_elem, const int *bf_side,
+                           const double *bf_g, const long *table_ptr,
+                           const int *table, double *val, double *rhs) {
+  int n_rows = 0;
+  long nnz = 0;
+  (void)n_rows; (void)nnz;
+  int nint = nel + nbf;
+  double *em = (double *)malloc((size_t)nint * 42 * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int ie = 0; ie < nint; ie++) {
+    double *sm = em + (size_t)ie * 42;
+    for (int i = 0; i < 42; i++) sm[i] = 0.0;
+    double nc[18];
+    if (ie < nel) {
+      int k = nodes_of(el_type[ie]);
+      gather_coor(coor, el_nodes + 6 * ie, k, nc);
+      orc_num_int_el(el_type[ie], nc, ind, coef_el + (size_t)ie * ORC_NCOEF,
+                     src_model, sm, sm + k * k);
+    } else {
+      int f = ie - nel, e = bf_elem[f];
+      int k = nodes_of(el_type[e]);
+      gather_coor(coor, el_nodes + 6 * e, k, nc);
+      orc_face_penalty(el_type[e], bf_side[f], nc, bf_g ? bf_g[f] : 0.0,
+                       src_model, sm, sm + k * k);
+    }
+  }
+  for (int ie = 0; ie < nint; ie++) {
+    int e = ie < nel ? ie : bf_elem[ie - nel];
+    int k = nodes_of(el_type[e]);
+    const double *sm = em + (size_t)ie * 42;
+    const int *tb = table + table_ptr[ie];
+    for (int i = 0; i < k; i++)
+      for (int j = 0; j < k; j++) val[tb[j + i * k]] += sm[j + i * k];
+    for (int j = 0; j < k; j++) rhs[tb[k * k + j]] += sm[k * k + j];
+  }
+  free(em);
+  return nint;
+}
+
+/* ---------------------------------------------------------------------
+ * Storage operations (lad_crs)
+ * ------------------------------------------------------------------- */
+/* lar_allocate_preconditioner_crs + lar_fill_preconditioner_crs
+   (las_crs_intf.c:658-737) */
+void orc_fill_jacobi(int n, const int *row_ptr, const int *col_ind,
+                     const double *val, int *diag_ptr, double *diag_precon) {
+  for (int r = 0; r < n; r++)
+    for (int c = row_ptr[r]; c < row_ptr[r + 1]; c++)
+      if (col_ind[c] == r) diag_ptr[r] = c;
+  for (int r = 0; r < n; r++) diag_precon[r] = 1.0 / val[diag_ptr[r]];
+}
+
+/* lar_compute_residual_crs (las_crs_intf.c:788-849): v = 0; v -= A x; v += b */
+void orc_compute_residual(int n, const int *row_ptr, const int *col_ind,
+                          const double *val, int use_rhs, const double *b,
+                          const double *x, double *v) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i++) {
+    double s = 0.0;
+    for (int k = row_ptr[i]; k < row_ptr[i + 1]; k++)
+      s -= val[k] * x[col_ind[k]];
+    if (use_rhs) s += b[i];
+    v[i] = s;
+  }
+}
+
+/* lar_perform_BJ_or_GS_iterations_crs, BLOCK_JACOBI, one sweep
+   (las_crs_intf.c:857-1013): v_i <- d_i * (b_i - sum_{j != i} a_ij v_j),
+   entries left of the diagonal first, then right of it. */
+void orc_bj_iteration(int n, const int *row_ptr, const int *col_ind,
+                      const double *val, const int *diag_ptr,
+                      const double *diag_precon, int use_rhs,
+                      const double *b, double *v) {
+  double *vt = (double *)malloc((size_t)n * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < n; r++) {
+    double vloc = 0.0;
+    for (int k = row_ptr[r]; k < diag_ptr[r]; k++) vloc -= val[k] * v[col_ind[k]];
+    for (int k = diag_ptr[r] + 1; k < row_ptr[r + 1]; k++)
+      vloc -= val[k] * v[col_ind[k]];
+    if (use_rhs) vloc += b[r];
+    vloc *= diag_precon[r];
+    vt[r] = vloc;
+  }
+  memcpy(v, vt, (size_t)n * sizeof(double));
+  free(vt);
+}
+
+/* ---------------------------------------------------------------------
+ * BLAS restatements (netlib reference algorithms)
+ * ------------------------------------------------------------------- */
+void orc_drotg(double *a, double *b, double *c, double *s) {
+  double roe = *b;
+  if (fabs(*a) > fabs(*b)) roe = *a;
+  double scale = fabs(*a) + fabs(*b);
+  double r, z;
+  if (scale == 0.0) {
+    *c = 1.0; *s = 0.0; r = 0.0; z = 0.0;
+  } else {
+    double ra = *a / scale, rb = *b / scale;
+    r = scale * sqrt(ra * ra + rb * rb);
+    r = (roe >= 0 ? 1.0 : -1.0) * r;
+    *c = *a / r;
+    *s = *b / r;
+    z = 1.0;
+    if (fabs(*a) > fabs(*b)) z = *s;
+    if (fabs(*b) >= fabs(*a) && *c != 0.0) z = 1.0 / *c;
+  }
+  *a = r;
+  *b = z;
+}
+
+static void drot1(double *x, double *y, double c, double s) {
+  double t = c * (*x) + s * (*y);
+  *y = c * (*y) - s * (*x);
+  *x = t;
+}
+
+static double dot(int n, const double *x, const double *y) {
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int i = 0; i < n; i++) s += x[i] * y[i];
+  return s;
+}
+
+static double nrm2(int n, const double *x) { return sqrt(dot(n, x, x)); }
+
+/* lsr_mkb_core_compreres for NO_PRECON / BLOCK_JACOBI with one sweep
+   (lss_mkb_core.c:1861-2097):
+     control 1: V = X + M^-1 (b - A X); V -= X
+     control 0: V = X; one BJ sweep with no rhs; V -= X; V *= -1 */
+typedef struct {
+  int n;
+  const int *rp, *ci;
+  const double *val, *b;
+  int precon;
+  int *diag_ptr;
+  double *dinv;
+} orc_op;
+
+static void compreres(const orc_op *op, int control, const double *x, double *v) {
+  int n = op->n;
+  if (op->precon == 0) {
+    orc_compute_residual(n, op->rp, op->ci, op->val, control, op->b, x, v);
+  } else {
+    memcpy(v, x, (size_t)n * sizeof(double));
+    orc_bj_iteration(n, op->rp, op->ci, op->val, op->diag_ptr, op->dinv,
+                     control, op->b, v);
+    for (int i = 0; i < n; i++) v[i] += -1.0 * x[i];
+  }
+  if (control == 0)
+    for (int i = 0; i < n; i++) v[i] *= -1.0;
+}
+
+int orc_gmres(int n, const int *row_ptr, const int *col_ind, const double *val,
+              const double *b, int precon, int krylbas, int max_iter,
+              double tol_rel, int ini_zero, double *x, int *iters,
+              double *res_rel_out, double *res_out) {
+  const double SMALL = 1.0e-15;
+  const double MAX_CONVRATE = 0.999;
+  orc_op op = {n, row_ptr, col_ind, val, b, precon, NULL, NULL};
+  if (precon) {
+    op.diag_ptr = (int *)malloc((size_t)n * sizeof(int));
+    op.dinv = (double *)malloc((size_t)n * sizeof(double));
+    orc_fill_jacobi(n, row_ptr, col_ind, val, op.diag_ptr, op.dinv);
+  }
+  int m = krylbas;
+  double *v = (double *)calloc((size_t)n * (m + 1), sizeof(double));
+  double *h = (double *)calloc((size_t)(m + 1) * (m + 1), sizeof(double));
+  double *giv = (double *)calloc((size_t)2 * (m + 1), sizeof(double));
+  double tol_res = 1.0e-12; /* REL_RES_INI passes 0, floored (:2377-2378) */
+  if (tol_rel < 1.0e-12) tol_rel = 1.0e-12;
+  int restrt = max_iter / m + 1, restit = 0, gmiter = 0, conv = 0;
+  double beta0 = 0, beta1 = 0, rnrm2, bnrm2, convrate = 0, faux = 0;
+  double resid = 0, res_rel = 0;
+  if (ini_zero) memset(x, 0, (size_t)n * sizeof(double));
+  do {
+    compreres(&op, 1, x, v);
+    rnrm2 = nrm2(n, v);
+    bnrm2 = 0.0;
+    if (ini_zero && gmiter == 0) bnrm2 = rnrm2;
+    if (bnrm2 < SMALL * SMALL) bnrm2 = 1.0;
+    if (restit == 0) {
+      convrate = 0.0; faux = 0.0; beta0 = rnrm2; beta1 = beta0;
+    } else if (rnrm2 < SMALL * SMALL || gmiter == 0 || beta0 < SMALL * SMALL ||
+               beta1 < SMALL * SMALL) {
+      convrate = 0.0; faux = 0.0; beta1 = 0.0;
+    } else {
+      convrate = exp(log(rnrm2 / beta0) / gmiter);
+      faux = exp(log(rnrm2 / beta1) / m);
+      beta1 = rnrm2;
+    }
+    resid = rnrm2;
+    res_rel = beta0 != 0.0 ? rnrm2 / beta0 : 0.0;
+    if (rnrm2 < tol_res || res_rel < tol_rel || rnrm2 < SMALL * SMALL) {
+      conv = 1;
+      break;
+    }
+    if (convrate > MAX_CONVRATE || faux > MAX_CONVRATE) break;
+    double daux = 1.0 / rnrm2;
+    for (int i = 0; i < n; i++) v[i] *= daux;
+    h[0] = rnrm2;
+    for (int i = 1; i <= m; i++) h[i] = 0.0;
+    int done = 0;
+    for (int ik = 1; ik <= m; ik++) {
+      gmiter++;
+      double *w = v + (size_t)ik * n;
+      compreres(&op, 0, v + (size_t)(ik - 1) * n, w);
+      int k = (m + 1) * ik;
+      for (int i = 1; i <= ik; i++) h[k++] = dot(n, v + (size_t)(i - 1) * n, w);
+      k = (m + 1) * ik;
+      for (int i = 1; i <= ik; i++) {
+        double a = -h[k++];
+        const double *vi = v + (size_t)(i - 1) * n;
+        for (int j = 0; j < n; j++) w[j] += a * vi[j];
+      }
+      k = (m + 1) * ik + ik;
+      h[k] = nrm2(n, w);
+      daux = 1.0 / h[k];
+      for (int j = 0; j < n; j++) w[j] *= daux;
+      double *col = h + (size_t)ik * (m + 1);
+      for (int i = 1; i < ik; i++)
+        drot1(&col[i - 1], &col[i], giv[2 * (i - 1)], giv[2 * (i - 1) + 1]);
+      double ra = col[ik - 1], rb = col[ik];
+      orc_drotg(&ra, &rb, &giv[2 * (ik - 1)], &giv[2 * (ik - 1) + 1]);
+      drot1(&col[ik - 1], &col[ik], giv[2 * (ik - 1)], giv[2 * (ik - 1) + 1]);
+      drot1(&h[ik - 1], &h[ik], giv[2 * (ik - 1)], giv[2 * (ik - 1) + 1]);
+      resid = fabs(h[ik]);
+      res_rel = beta0 != 0.0 ? resid / beta0 : 0.0;
+      if (ik == m || gmiter == max_iter || resid < tol_res || res_rel < tol_rel) {
+        /* dtrsv("U","N","N") on h[1..ik][..] then x += V y (dgemv) */
+        for (int i = ik - 1; i >= 0; i--) {
+          double s = h[i];
+          for (int j = i + 1; j < ik; j++) s -= h[(size_t)(j + 1) * (m + 1) + i] * h[j];
+          h[i] = s / h[(size_t)(i + 1) * (m + 1) + i];
+        }
+        for (int j = 0; j < ik; j++) {
+          const double *vj = v + (size_t)j * n;
+          double y = h[j];
+          for (int r = 0; r < n; r++) x[r] += vj[r] * y;
+        }
+        if (resid < tol_res || res_rel < tol_rel) { conv = 1; done = 1; }
+        else if (gmiter == max_iter) done = 1;
+        break;
+      }
+    }
+    if (done) break;
+    restit++;
+  } while (restit <= restrt);
+  if (iters) *iters = gmiter;
+  if (res_rel_out) *res_rel_out = res_rel;
+  if (res_out) *res_out = resid;
+  free(v); free(h); free(giv);
+  free(op.diag_ptr); free(op.dinv);
+  return conv;
+}

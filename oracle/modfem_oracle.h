@@ -1,0 +1,127 @@
+/*
+ * modfem_oracle.h -- CPU restatement of ModFEM's std-linear
+ * integrate -> assemble -> MKB GMRES path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or as the timed CPU baseline.  The product (modfem_amd, libmodfem_hip.so)
+ * never links, loads or calls it.
+ *
+ * Every function names the reference routine (file:line, relative to
+ * modfem2017/src/) whose algorithm, data layout and summation order it
+ * follows.  Index conventions: element/node/face indices are 0-based here;
+ * the reference uses 1-based IDs in the same order.
+ */
+#ifndef MODFEM_ORACLE_H
+#define MODFEM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_PRISM 5 /* matches MMC_PRISM */
+#define ORC_TETRA 7 /* matches MMC_TETRA */
+
+/* coefficient-vector indicator slots (pds_conv_diff_weakform.c:87-276) */
+enum {
+  ORC_IND_M = 1, ORC_IND_AXX = 2, ORC_IND_AXY = 3, ORC_IND_AXZ = 4,
+  ORC_IND_AYX = 5, ORC_IND_AYY = 6, ORC_IND_AYZ = 7, ORC_IND_AZX = 8,
+  ORC_IND_AZY = 9, ORC_IND_AZZ = 10, ORC_IND_BX = 11, ORC_IND_BY = 12,
+  ORC_IND_BZ = 13, ORC_IND_TX = 14, ORC_IND_TY = 15, ORC_IND_TZ = 16,
+  ORC_IND_C = 17, ORC_IND_L = 18, ORC_IND_QX = 19, ORC_IND_QY = 20,
+  ORC_IND_QZ = 21, ORC_IND_S = 22
+};
+
+/* source/Dirichlet-data models */
+#define ORC_SRC_CONST 0        /* s and g constant per element / face      */
+#define ORC_SRC_DIFF_IN_CUBE 1 /* u=exp(-(x2+y2+z2)),     s=-lapl u, g=u   */
+#define ORC_SRC_DIFF_IN_BOX 2  /* u=exp(-(x2+y2+z2)/100), s=-lapl u, g=u   */
+
+/* per-element PDE coefficients, constant over the element:
+   a[9] = axx axy axz ayx ayy ayz azx azy azz, b[3], t[3], c, q[3], s, m */
+#define ORC_NCOEF 21
+#define ORC_C_A 0
+#define ORC_C_B 9
+#define ORC_C_T 12
+#define ORC_C_C 15
+#define ORC_C_Q 16
+#define ORC_C_S 19
+#define ORC_C_M 20
+
+/* quadrature (aps_std_util.c:1335-1436, aps_gauss_util_std.c:126-) */
+int orc_quadr_3d(int el_type, double *xg, double *wg);
+int orc_quadr_2d(int fa_is_quad, double *xg, double *wg);
+
+/* shape functions + element calc (aps_std_util.c:463-820, 826-1226) */
+double orc_elem_calc_3d(int el_type, int control, const double *eta,
+                        const double *node_coor, double *phi, double *dphix,
+                        double *dphiy, double *dphiz, double *xcoor,
+                        double *vec_nor);
+double orc_mat3_inv(const double *mat, double *mat_inv);
+
+/* element integral (aps_std_lin_intf.c:1537-2870), nreq=1 */
+int orc_num_int_el(int el_type, const double *node_coor, const int *ind,
+                   const double *coef, int src_model, double *stiff,
+                   double *rhs);
+/* Dirichlet penalty face (pds_conv_diff_weakform_std.c:253-629) */
+int orc_face_penalty(int el_type, int side, const double *node_coor,
+                     double g_const, int src_model, double *stiff,
+                     double *rhs);
+
+/* lists and DOF structures (uts_ls_intf.c:102-398, sis_mkb_intf.c:396-1142)
+   el_nodes: nel x 6 (tets use the first 4).  Returns number of DOFs.
+   dof_node[n]: node index of DOF n (first-appearance order).
+   node_dof[nno]: inverse map (-1 when the node carries no DOF). */
+int orc_dof_list(int nel, const int *el_type, const int *el_nodes, int nno,
+                 int *dof_node, int *node_dof);
+/* CRS pattern (las_crs_intf.c:72-294 fed by sis_mkb_intf.c:762-1142).
+   row_ptr[n+1]; col_ind sized by orc_crs_count (first call with col_ind=NULL
+   returns nnz). */
+long orc_crs_pattern(int nel, const int *el_type, const int *el_nodes,
+                     const int *node_dof, int n, int *row_ptr, int *col_ind);
+/* assembly tables (las_crs_intf.c:354-432): for every int-ent (elements,
+   then boundary faces) (k+1)*k ints; table_ptr[nint+1] offsets. */
+long orc_assembly_tables(int nel, const int *el_type, const int *el_nodes,
+                         int nbf, const int *bf_elem, const int *node_dof,
+                         const int *row_ptr, const int *col_ind,
+                         long *table_ptr, int *table);
+
+/* integrate + assemble sequentially in int-ent order (sis_mkb_intf.c:1843-1866,
+   las_crs_intf.c:440-513).  val[nnz], rhs[n] zeroed inside. */
+int orc_integrate_assemble(int nel, const int *el_type, const int *el_nodes,
+                           const double *node_coor_all, const int *ind,
+                           const double *coef_el, int src_model, int nbf,
+                           const int *bf_elem, const int *bf_side,
+                           const double *bf_g, const long *table_ptr,
+                           const int *table, double *val, double *rhs);
+
+/* linear algebra (las_crs_intf.c:707-1013) */
+void orc_fill_jacobi(int n, const int *row_ptr, const int *col_ind,
+                     const double *val, int *diag_ptr, double *diag_precon);
+void orc_compute_residual(int n, const int *row_ptr, const int *col_ind,
+                          const double *val, int use_rhs, const double *b,
+                          const double *x, double *v);
+void orc_bj_iteration(int n, const int *row_ptr, const int *col_ind,
+                      const double *val, const int *diag_ptr,
+                      const double *diag_precon, int use_rhs,
+                      const double *b, double *v);
+
+/* restarted left-preconditioned GMRES with CGS and Givens
+   (lss_mkb_core.c:2317-2668), Jacobi preconditioner applied as one BJ sweep
+   through compreres (lss_mkb_core.c:1861-2097).
+   ini_zero: x is taken as zero on entry.  Returns 1 on convergence. */
+int orc_gmres(int n, const int *row_ptr, const int *col_ind, const double *val,
+              const double *b, int precon, int krylbas, int max_iter,
+              double tol_rel, int ini_zero, double *x, int *iters,
+              double *res_rel_out, double *res_out);
+
+/* BLAS level-1/2 restatements (netlib reference algorithms) */
+void orc_drotg(double *a, double *b, double *c, double *s);
+
+/* threads used by the OpenMP parts (0 = all) */
+void orc_set_threads(int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

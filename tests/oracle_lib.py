@@ -1,0 +1,156 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the
+reference subsets compiled from the reference's own sources (oracle/_ref).
+Test infrastructure: used only as the checker."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+PRISM, TETRA = 5, 7
+V = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+
+
+def P(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+_orc = None
+
+
+def oracle():
+    global _orc
+    if _orc is None:
+        L = ctypes.CDLL(ORACLE)
+        sig = {
+            "orc_quadr_3d": (I, [I, V, V]),
+            "orc_quadr_2d": (I, [I, V, V]),
+            "orc_elem_calc_3d": (D, [I, I, V, V, V, V, V, V, V, V]),
+            "orc_num_int_el": (I, [I, V, V, V, I, V, V]),
+            "orc_face_penalty": (I, [I, I, V, D, I, V, V]),
+            "orc_dof_list": (I, [I, V, V, I, V, V]),
+            "orc_crs_pattern": (ctypes.c_long, [I, V, V, V, I, V, V]),
+            "orc_assembly_tables": (ctypes.c_long, [I, V, V, I, V, V, V, V, V, V]),
+            "orc_integrate_assemble": (I, [I, V, V, V, V, V, I, I, V, V, V, V, V, V, V]),
+            "orc_fill_jacobi": (None, [I, V, V, V, V, V]),
+            "orc_compute_residual": (None, [I, V, V, V, I, V, V, V]),
+            "orc_bj_iteration": (None, [I, V, V, V, V, V, I, V, V]),
+            "orc_gmres": (I, [I, V, V, V, V, I, I, I, D, I, V, V, V, V]),
+            "orc_drotg": (None, [V, V, V, V]),
+            "orc_set_threads": (None, [I]),
+        }
+        for k, (r, a) in sig.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _orc = L
+    return _orc
+
+
+def ref_available(name):
+    return os.path.exists(os.path.join(REF_DIR, name))
+
+
+_refs = {}
+
+
+def ref(name):
+    """a reference subset library, loaded lazily-bound: symbols of modules
+    that are not compiled in stay unresolved and are never called"""
+    if name not in _refs:
+        _refs[name] = ctypes.CDLL(os.path.join(REF_DIR, name),
+                                  mode=os.RTLD_LAZY | os.RTLD_GLOBAL)
+    return _refs[name]
+
+
+class OracleProblem:
+    """the oracle's view of a mesh: lists, pattern, tables, assembly"""
+
+    def __init__(self, arrays):
+        self.a = arrays
+        o = oracle()
+        self.nno = arrays["coor"].shape[0]
+        self.nel = arrays["el_type"].shape[0]
+        self.nbf = arrays["bf_elem"].shape[0]
+        self.et = np.ascontiguousarray(arrays["el_type"], np.int32)
+        self.en = np.ascontiguousarray(arrays["el_nodes"], np.int32)
+        self.coor = np.ascontiguousarray(arrays["coor"], np.float64)
+        self.bfe = np.ascontiguousarray(arrays["bf_elem"], np.int32)
+        self.bfs = np.ascontiguousarray(arrays["bf_side"], np.int32)
+        self.dof_node = np.zeros(self.nno, np.int32)
+        self.node_dof = np.zeros(self.nno, np.int32)
+        self.n = o.orc_dof_list(self.nel, P(self.et), P(self.en), self.nno,
+                                P(self.dof_node), P(self.node_dof))
+        self.dof_node = self.dof_node[: self.n].copy()
+        self.nnz = o.orc_crs_pattern(self.nel, P(self.et), P(self.en), P(self.node_dof),
+                                     self.n, None, None)
+        self.row_ptr = np.zeros(self.n + 1, np.int32)
+        self.col_ind = np.zeros(self.nnz, np.int32)
+        o.orc_crs_pattern(self.nel, P(self.et), P(self.en), P(self.node_dof), self.n,
+                          P(self.row_ptr), P(self.col_ind))
+        nint = self.nel + self.nbf
+        self.table_ptr = np.zeros(nint + 1, np.int64)
+        tot = o.orc_assembly_tables(self.nel, P(self.et), P(self.en), self.nbf, P(self.bfe),
+                                    P(self.node_dof), P(self.row_ptr), P(self.col_ind),
+                                    P(self.table_ptr), None)
+        self.table = np.zeros(tot, np.int32)
+        o.orc_assembly_tables(self.nel, P(self.et), P(self.en), self.nbf, P(self.bfe),
+                              P(self.node_dof), P(self.row_ptr), P(self.col_ind),
+                              P(self.table_ptr), P(self.table))
+
+    def assemble(self, ind, coef_el, src_model=0, bf_g=None):
+        o = oracle()
+        ind = np.ascontiguousarray(ind, np.int32)
+        coef_el = np.ascontiguousarray(coef_el, np.float64)
+        self.val = np.zeros(self.nnz)
+        self.rhs = np.zeros(self.n)
+        g = None if bf_g is None else np.ascontiguousarray(bf_g, np.float64)
+        o.orc_integrate_assemble(self.nel, P(self.et), P(self.en), P(self.coor), P(ind),
+                                 P(coef_el), src_model, self.nbf, P(self.bfe), P(self.bfs),
+                                 P(g), P(self.table_ptr), P(self.table), P(self.val),
+                                 P(self.rhs))
+        return self.val, self.rhs
+
+    def element(self, e, ind, coef, src_model=0):
+        o = oracle()
+        k = 4 if self.et[e] == TETRA else 6
+        nc = np.ascontiguousarray(self.coor[self.en[e, :k]].reshape(-1))
+        sm = np.zeros(36)
+        rhs = np.zeros(6)
+        o.orc_num_int_el(int(self.et[e]), P(nc), P(np.ascontiguousarray(ind, np.int32)),
+                         P(np.ascontiguousarray(coef, np.float64)), src_model, P(sm), P(rhs))
+        return sm[: k * k].reshape(k, k), rhs[:k]  # [u][w] (column-major stiff[u*k+w])
+
+    def face(self, f, g=0.0, src_model=0):
+        o = oracle()
+        e = self.bfe[f]
+        k = 4 if self.et[e] == TETRA else 6
+        nc = np.ascontiguousarray(self.coor[self.en[e, :k]].reshape(-1))
+        sm = np.zeros(36)
+        rhs = np.zeros(6)
+        o.orc_face_penalty(int(self.et[e]), int(self.bfs[f]), P(nc), g, src_model, P(sm), P(rhs))
+        return sm[: k * k].reshape(k, k), rhs[:k]
+
+    def gmres(self, val, rhs, precon=1, krylbas=30, max_iter=1000, tol=1e-8):
+        o = oracle()
+        x = np.zeros(self.n)
+        it, rr, rs = I(), D(), D()
+        conv = o.orc_gmres(self.n, P(self.row_ptr), P(self.col_ind), P(val), P(rhs), precon,
+                           krylbas, max_iter, tol, 1, P(x), ctypes.byref(it), ctypes.byref(rr),
+                           ctypes.byref(rs))
+        return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+
+def oracle_gmres_crs(row_ptr, col_ind, val, rhs, precon=1, krylbas=30, max_iter=1000, tol=1e-8):
+    o = oracle()
+    n = row_ptr.shape[0] - 1
+    x = np.zeros(n)
+    it, rr, rs = I(), D(), D()
+    conv = o.orc_gmres(n, P(row_ptr), P(col_ind), P(val), P(rhs), precon, krylbas, max_iter,
+                       tol, 1, P(x), ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs))
+    return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)

@@ -1,0 +1,245 @@
+"""GPU parity: the HIP path (libmodfem_hip.so through its C ABI) against the
+CPU oracle on identical seeded inputs.
+
+Bars (stated per test):
+  integer maps (DOF list, CRS pattern, tables)  bit-exact
+  element / face matrices (FP64)                max|K-K0| <= 1e-12 max|K0| per entity
+  assembled matrix and load (FP64)              max|A-A0| <= 1e-12 max|A0|
+  lar_ residual / Jacobi sweep                  bit-exact with the reference order
+  GMRES solution                                both converge to the 1e-8 preconditioned
+                                                residual; max|x-x0| <= 1e-6 max|x0|
+  FP32 integration                              max|K32-K64| <= 2e-6 max|K64|
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import modfem_amd as mf  # noqa: E402
+from oracle_lib import OracleProblem  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def coef_table(nel, kappa=None, b=None, full=None, s=1.0):
+    c = np.zeros((nel, 21))
+    if full is not None:
+        return full
+    k = np.ones(nel) if kappa is None else kappa
+    c[:, 0] = c[:, 4] = c[:, 8] = k
+    if b is not None:
+        c[:, 9:12] = b
+    c[:, 19] = s
+    return c
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def per_entity_err(G, O):
+    return max(np.abs(g - o).max() / max(np.abs(o).max(), 1e-300) for g, o in zip(G, O))
+
+
+CASES = {
+    "poisson_uniform": dict(ind=mf.IND_DIFF, model=mf.COEF_UNIFORM),
+    "poisson_kappa": dict(ind=mf.IND_DIFF, model=mf.COEF_KAPPA),
+    "convdiff_kappa_b": dict(ind=mf.IND_CONV_DIFF, model=mf.COEF_KAPPA_B),
+    "all_terms_full": dict(ind=mf.IND_ALL, model=mf.COEF_FULL),
+}
+
+
+def make_case(mesh, name, seed=5):
+    rng = np.random.default_rng(seed)
+    nel = mesh.nel
+    kappa = np.exp(rng.normal(0, 1, nel))
+    b = rng.uniform(-1, 1, (nel, 3))
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    cc[19] = 1.0
+    spec = CASES[name]
+    if spec["model"] == mf.COEF_UNIFORM:
+        return spec["ind"], spec["model"], cc, None, coef_table(nel)
+    if spec["model"] == mf.COEF_KAPPA:
+        return spec["ind"], spec["model"], cc, kappa, coef_table(nel, kappa)
+    if spec["model"] == mf.COEF_KAPPA_B:
+        ce = np.concatenate([kappa, b[:, 0], b[:, 1], b[:, 2]])
+        return spec["ind"], spec["model"], cc, ce, coef_table(nel, kappa, b)
+    full = rng.uniform(-0.3, 0.3, (nel, 21))
+    full[:, 0] += 2.0
+    full[:, 4] += 2.0
+    full[:, 8] += 2.0
+    return spec["ind"], spec["model"], cc, full.T.copy().reshape(-1), full
+
+
+@pytest.fixture(scope="module")
+def prism_mesh():
+    return mf.Mesh.box_prism(7, 6, 5, lx=1.0, ly=2.0, lz=0.5, seed=11, jitter=0.2)
+
+
+@pytest.fixture(scope="module")
+def tetra_mesh():
+    return mf.Mesh.box_tetra(5, 4, 4, seed=3, jitter=0.15)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("kind", ["prism", "tetra"])
+def test_element_matrices(case, kind, prism_mesh, tetra_mesh):
+    mesh = prism_mesh if kind == "prism" else tetra_mesh
+    p = mf.Problem.from_mesh(mesh)
+    op = OracleProblem(mesh.arrays())
+    ind, model, cc, ce, tab = make_case(mesh, case)
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    E = p.element_matrices()
+    k = 6 if kind == "prism" else 4
+    errs = []
+    for e in range(p.nel):
+        K0, F0 = op.element(e, ind, tab[e])
+        K = E[e, : k * k].reshape(k, k)
+        F = E[e, k * k: k * k + k]
+        errs.append(max(np.abs(K - K0).max() / np.abs(K0).max(),
+                        np.abs(F - F0).max() / max(np.abs(F0).max(), 1e-300)))
+    assert max(errs) <= 1e-12, (case, kind, max(errs))
+    p.free()
+
+
+@pytest.mark.parametrize("kind", ["prism", "tetra"])
+def test_face_matrices(kind, prism_mesh, tetra_mesh):
+    mesh = prism_mesh if kind == "prism" else tetra_mesh
+    p = mf.Problem.from_mesh(mesh)
+    op = OracleProblem(mesh.arrays())
+    g = np.random.default_rng(2).uniform(-1, 1, p.nbf)
+    p.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=np.r_[1, 0, 0, 0, 1, 0, 0, 0, 1,
+                                                                         np.zeros(10), 1, 0],
+                       bf_g=g)
+    Fm = p.face_matrices()
+    k = 6 if kind == "prism" else 4
+    for f in range(p.nbf):
+        K0, F0 = op.face(f, g[f])
+        K = Fm[f, : k * k].reshape(k, k)
+        F = Fm[f, k * k: k * k + k]
+        assert np.abs(K - K0).max() <= 1e-12 * np.abs(K0).max()
+        assert np.abs(F - F0).max() <= 1e-12 * max(np.abs(F0).max(), 1e-300) + 1e-300
+    p.free()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_assembled_system(case, prism_mesh):
+    p = mf.Problem.from_mesh(prism_mesh)
+    op = OracleProblem(prism_mesh.arrays())
+    rp, ci = p.pattern()
+    assert np.array_equal(rp, op.row_ptr) and np.array_equal(ci, op.col_ind)
+    assert np.array_equal(p.dofs(), op.dof_node)
+    for ie in range(0, p.nel + p.nbf, 7):
+        t0 = op.table[op.table_ptr[ie]: op.table_ptr[ie + 1]]
+        assert np.array_equal(p.table(ie), t0)
+    ind, model, cc, ce, tab = make_case(prism_mesh, case)
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    p.integrate_assemble(64)
+    val, rhs = p.system()
+    v0, r0 = op.assemble(ind, tab)
+    assert rel_err(val, v0) <= 1e-12
+    assert rel_err(rhs, r0) <= 1e-12
+    p.free()
+
+
+@pytest.mark.parametrize("src", [mf.SRC_DIFF_IN_CUBE, mf.SRC_DIFF_IN_BOX])
+def test_exact_solution_source(src, prism_mesh):
+    p = mf.Problem.from_mesh(prism_mesh)
+    op = OracleProblem(prism_mesh.arrays())
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    p.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc, src_model=src)
+    p.integrate_assemble(64)
+    val, rhs = p.system()
+    v0, r0 = op.assemble(mf.IND_DIFF, coef_table(p.nel, s=0.0), src_model=src)
+    assert rel_err(val, v0) <= 1e-12
+    assert rel_err(rhs, r0) <= 1e-12
+    p.free()
+
+
+@pytest.mark.parametrize("precon", [0, 1])
+def test_gmres_matches_oracle(precon):
+    mesh = mf.Mesh.box_prism(10, 9, 8, seed=21, jitter=0.2)
+    p = mf.Problem.from_mesh(mesh)
+    op = OracleProblem(mesh.arrays())
+    ind, model, cc, ce, tab = make_case(mesh, "convdiff_kappa_b", seed=9)
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    p.integrate_assemble(64)
+    val, rhs = p.system()
+    x, info = p.gmres(precon=precon, krylbas=30, max_iter=3000, tol=1e-8)
+    v0, r0 = op.assemble(ind, tab)
+    x0, info0 = op.gmres(v0, r0, precon=precon, krylbas=30, max_iter=3000, tol=1e-8)
+    assert info["converged"] and info0["converged"]
+    assert info["res_rel"] < 1e-8
+    # iteration counts agree up to rounding-driven drift
+    assert abs(info["iters"] - info0["iters"]) <= max(3, info0["iters"] // 20)
+    assert rel_err(x, x0) <= 1e-6
+
+
+def test_gmres_dense_reference():
+    """small nonsymmetric system: GMRES solution equals the dense LU solve"""
+    rng = np.random.default_rng(4)
+    n = 60
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in range(max(0, i - 3), min(n, i + 4)):
+            A[i, j] = rng.uniform(-1, 1)
+        A[i, i] = 8.0 + rng.uniform(0, 1)
+    rp = [0]
+    ci, v = [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        ci += list(nz)
+        v += list(A[i, nz])
+        rp.append(len(ci))
+    b = rng.uniform(-1, 1, n)
+    p = mf.Problem.from_crs(np.array(rp), np.array(ci), np.array(v), b)
+    x, info = p.gmres(precon=1, krylbas=n, max_iter=200, tol=1e-12)
+    xd = np.linalg.solve(A, b)
+    assert info["converged"]
+    assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()
+    # Krylov invariance: with m >= n GMRES finishes within n steps
+    assert info["iters"] <= n
+    p.free()
+
+
+def test_fp32_integration_error(prism_mesh):
+    p = mf.Problem.from_mesh(prism_mesh)
+    ind, model, cc, ce, tab = make_case(prism_mesh, "convdiff_kappa_b")
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    E64 = p.element_matrices(precision=64)
+    E32 = p.element_matrices(precision=32)
+    err = per_entity_err(E32, E64)
+    assert err <= 2e-6, err
+    p.integrate_assemble(32)
+    v32, r32 = p.system()
+    p.integrate_assemble(64)
+    v64, r64 = p.system()
+    assert rel_err(v32, v64) <= 2e-6
+    p.free()
+
+
+def test_large_mesh_invariants():
+    """size-independent properties on a 40^3 jittered box: zero row sums of the
+    pure-diffusion element part, total load equals the volume, symmetric
+    pattern"""
+    mesh = mf.Mesh.box_prism(40, 40, 40, seed=5, jitter=0.2, bc=0)
+    p = mf.Problem.from_mesh(mesh)
+    kappa = np.exp(np.random.default_rng(1).normal(0, 1, p.nel))
+    cc = np.zeros(21)
+    cc[19] = 1.0
+    p.set_coefficients(mf.IND_DIFF, mf.COEF_KAPPA, coef_const=cc, coef_el=kappa)
+    p.integrate_assemble(64)
+    val, rhs = p.system()
+    rp, ci = p.pattern()
+    rows = np.repeat(np.arange(p.n), np.diff(rp))
+    rowsum = np.bincount(rows, weights=val, minlength=p.n)
+    assert np.abs(rowsum).max() <= 1e-11 * np.abs(val).max()
+    assert abs(rhs.sum() - 1.0) <= 1e-12
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val, ci, rp), shape=(p.n, p.n))
+    assert abs(A - A.T).max() <= 1e-13 * np.abs(val).max()
+    p.free()
