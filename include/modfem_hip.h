@@ -112,6 +112,11 @@ int mf_integrate_assemble(int pid, int precision);
    rhs[k], 42 doubles per entity (apr_num_int_el output layout,
    aps_std_lin_intf.c:1558-1577) */
 int mf_element_matrices(int pid, int first, int count, int precision, double* out);
+/* assembly strategy of mf_integrate_assemble: 0 (default) one lane per
+   matrix row walking its elements in ascending order, deterministic and in
+   the reference's summation order; 1 one lane group per element with FP64
+   atomics (order-dependent rounding) */
+int mf_set_assembly_mode(int pid, int mode);
 int mf_face_matrices(int pid, int first, int count, double* out);
 int mf_get_system(int pid, double* val, double* rhs);
 int mf_set_system(int pid, const double* val, const double* rhs);

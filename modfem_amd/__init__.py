@@ -102,6 +102,7 @@ def _declare(L):
         "mf_bench_spmv": (I, [I, I, V]),
         "mf_last_solve_ms": (I, [I, V]),
         "mf_gmres_kernel_stats": (I, [I, V]),
+        "mf_set_assembly_mode": (I, [I, I]),
         "mf_problem_free": (I, [I]),
         # storage seam with the reference's names (lsd_mkb/lah_intf.h)
         "lar_allocate_SM_and_LV": (I, [I, I, I, I, I, I, V, V, V, V]),
@@ -279,6 +280,9 @@ class Problem:
         g = None if bf_g is None else np.ascontiguousarray(bf_g, np.float64)
         check(lib().mf_problem_set_coefficients(self.h, _p(ind), coef_model, _p(cc), _p(ce),
                                                 src_model, _p(g)))
+
+    def set_assembly_mode(self, mode):
+        check(lib().mf_set_assembly_mode(self.h, mode))
 
     def integrate_assemble(self, precision=64):
         check(lib().mf_integrate_assemble(self.h, precision))

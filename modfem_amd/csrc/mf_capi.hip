@@ -57,7 +57,9 @@ void free_problem_device(Problem& P) {
   void* ptrs[] = {P.d_coor, P.d_el_dof, P.d_el_off, P.d_bf_elem, P.d_bf_side, P.d_bf_off,
                   P.d_bf_g, P.d_row_ptr, P.d_col, P.d_val, P.d_rhs, P.d_val32,
                   P.d_coef, P.d_dinv, P.d_diag_ptr, P.d_dinv32, P.d_tile_row, P.d_V, P.d_H,
-                  P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work};
+                  P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work, P.d_sl_ptr,
+                  P.d_scol, P.d_diag_k, P.d_inc_ptr, P.d_inc_el, P.d_inc_loc,
+                  P.d_finc_ptr, P.d_finc_f, P.d_finc_loc, P.d_brows};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (P.h_stat) hipHostFree(P.h_stat);
@@ -69,6 +71,37 @@ int upload(T** dst, const T* src, size_t count) {
   MF_HIP(hipMalloc(dst, bytes));
   if (src && count) MF_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
   return 0;
+}
+
+// SELL-64 arrays of a problem whose host setup has row_ptr/col_ind
+int upload_sell(Problem& P) {
+  Setup& S = P.setup;
+  if (S.sl_ptr.empty()) build_sell(S);
+  P.sell = true;
+  P.nnz_sell = S.sl_ptr.back();
+  if (upload(&P.d_sl_ptr, S.sl_ptr.data(), S.sl_ptr.size())) return -2;
+  if (upload(&P.d_scol, S.sl_col.data(), S.sl_col.size())) return -2;
+  if (upload(&P.d_diag_k, S.diag_k.data(), S.diag_k.size())) return -2;
+  if (upload<double>(&P.d_val, nullptr, (size_t)P.nnz_sell)) return -2;
+  return 0;
+}
+
+// CRS-ordered values <-> SELL-64 positions (host)
+void crs_to_sell(const Setup& S, const double* crs, double* sell) {
+  std::fill(sell, sell + S.sl_ptr.back(), 0.0);
+  for (int r = 0; r < S.n; r++) {
+    const long long base = S.sl_ptr[r >> 6] + (r & 63);
+    for (int k = S.row_ptr[r]; k < S.row_ptr[r + 1]; k++)
+      sell[base + 64LL * (k - S.row_ptr[r])] = crs[k];
+  }
+}
+
+void sell_to_crs(const Setup& S, const double* sell, double* crs) {
+  for (int r = 0; r < S.n; r++) {
+    const long long base = S.sl_ptr[r >> 6] + (r & 63);
+    for (int k = S.row_ptr[r]; k < S.row_ptr[r + 1]; k++)
+      crs[k] = sell[base + 64LL * (k - S.row_ptr[r])];
+  }
 }
 
 }  // namespace
@@ -249,12 +282,22 @@ int mf_problem_create(int mesh) {
   if (upload(&P->d_el_off, S.el_off.data(), S.el_off.size())) return -2;
   if (upload(&P->d_bf_elem, M.bf_elem.data(), M.bf_elem.size())) return -2;
   if (upload(&P->d_bf_side, M.bf_side.data(), M.bf_side.size())) return -2;
-  if (upload(&P->d_row_ptr, S.row_ptr.data(), S.row_ptr.size())) return -2;
-  if (upload(&P->d_col, S.col_ind.data(), S.col_ind.size())) return -2;
-  if (upload<double>(&P->d_val, nullptr, (size_t)S.nnz)) return -2;
+  if (upload_sell(*P)) return -2;
   if (upload<double>(&P->d_rhs, nullptr, (size_t)S.n)) return -2;
-  if (launch_zero(P->d_val, S.nnz) || launch_zero(P->d_rhs, S.n)) return -2;
-  if (build_tiles(*P) < 0) return -1;
+  if (launch_zero(P->d_val, P->nnz_sell) || launch_zero(P->d_rhs, S.n)) return -2;
+  if (upload(&P->d_inc_ptr, S.inc_ptr.data(), S.inc_ptr.size())) return -2;
+  if (upload(&P->d_inc_el, S.inc_el.data(), S.inc_el.size())) return -2;
+  if (upload(&P->d_inc_loc, S.inc_loc.data(), S.inc_loc.size())) return -2;
+  if (upload(&P->d_finc_ptr, S.finc_ptr.data(), S.finc_ptr.size())) return -2;
+  if (upload(&P->d_finc_f, S.finc_f.data(), S.finc_f.size())) return -2;
+  if (upload(&P->d_finc_loc, S.finc_loc.data(), S.finc_loc.size())) return -2;
+  {
+    std::vector<int> brows;
+    for (int r = 0; r < S.n; r++)
+      if (S.finc_ptr[r + 1] > S.finc_ptr[r]) brows.push_back(r);
+    P->nbrows = (int)brows.size();
+    if (upload(&P->d_brows, brows.data(), brows.size())) return -2;
+  }
   // default coefficients: -Laplace u = 1 (DIFF_IN_CUBE-style selection)
   P->mask = TermMask{0, 1, 0, 0, 0, 0, 0, 1};
   P->coef_const[C_A + 0] = P->coef_const[C_A + 4] = P->coef_const[C_A + 8] = 1.0;
@@ -294,12 +337,23 @@ int mf_problem_create_crs(int n, const int* row_ptr, const int* col_ind, const d
       return -1;
     }
   }
-  if (upload(&P->d_row_ptr, row_ptr, (size_t)n + 1)) return -2;
-  if (upload(&P->d_col, col_ind, (size_t)nnz)) return -2;
-  if (upload(&P->d_val, val, (size_t)nnz)) return -2;
+  {
+    int mx = 0;
+    for (int r = 0; r < n; r++) mx = std::max(mx, row_ptr[r + 1] - row_ptr[r]);
+    if (mx > 255) {
+      set_error("mf_problem_create_crs: rows longer than 255 entries are not supported");
+      return -1;
+    }
+    P->setup.max_row = mx;
+  }
+  if (upload_sell(*P)) return -2;
+  {
+    std::vector<double> sv(P->nnz_sell);
+    crs_to_sell(P->setup, val, sv.data());
+    MF_HIP(hipMemcpy(P->d_val, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (upload<double>(&P->d_rhs, rhs, (size_t)n)) return -2;
   if (!rhs && launch_zero(P->d_rhs, n)) return -2;
-  if (build_tiles(*P) < 0) return -1;
   MF_HIP(hipStreamSynchronize(stream()));
   int id = g_next_problem++;
   g_problems[id] = std::move(P);
@@ -418,9 +472,13 @@ int mf_integrate_assemble(int pid, int precision) {
     set_error("problem has no mesh");
     return -1;
   }
-  if (launch_zero(P->d_val, P->nnz) || launch_zero(P->d_rhs, P->n)) return -2;
-  if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
-  if (launch_faces(*P, 0, nullptr, 0, P->nbf)) return -2;
+  if (P->assembly_mode == 0) {
+    if (launch_assemble_rows(*P, precision)) return -2;
+  } else {
+    if (launch_zero(P->d_val, P->nnz_sell) || launch_zero(P->d_rhs, P->n)) return -2;
+    if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
+    if (launch_faces(*P, 0, nullptr, 0, P->nbf)) return -2;
+  }
   if (P->d_dinv) {
     hipFree(P->d_dinv);
     P->d_dinv = nullptr;
@@ -483,7 +541,11 @@ int mf_get_system(int pid, double* val, double* rhs) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
   MF_HIP(hipStreamSynchronize(stream()));
-  if (val) MF_HIP(hipMemcpy(val, P->d_val, (size_t)P->nnz * sizeof(double), hipMemcpyDeviceToHost));
+  if (val) {
+    std::vector<double> sv(P->nnz_sell);
+    MF_HIP(hipMemcpy(sv.data(), P->d_val, sv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    sell_to_crs(P->setup, sv.data(), val);
+  }
   if (rhs) MF_HIP(hipMemcpy(rhs, P->d_rhs, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
@@ -492,7 +554,11 @@ int mf_set_system(int pid, const double* val, const double* rhs) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
   MF_HIP(hipStreamSynchronize(stream()));
-  if (val) MF_HIP(hipMemcpy(P->d_val, val, (size_t)P->nnz * sizeof(double), hipMemcpyHostToDevice));
+  if (val) {
+    std::vector<double> sv(P->nnz_sell);
+    crs_to_sell(P->setup, val, sv.data());
+    MF_HIP(hipMemcpy(P->d_val, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (rhs) MF_HIP(hipMemcpy(P->d_rhs, rhs, (size_t)P->n * sizeof(double), hipMemcpyHostToDevice));
   if (P->d_dinv) {
     hipFree(P->d_dinv);
@@ -577,12 +643,19 @@ int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems, doubl
   MF_HIP(hipEventCreate(&c));
   float te = 0, tf = 0;
   for (int r = 0; r < reps; r++) {
-    if (launch_zero(P->d_val, P->nnz) || launch_zero(P->d_rhs, P->n)) return -2;
-    MF_HIP(hipEventRecord(a, stream()));
-    if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
-    MF_HIP(hipEventRecord(b, stream()));
-    if (launch_faces(*P, 0, nullptr, 0, P->nbf)) return -2;
-    MF_HIP(hipEventRecord(c, stream()));
+    if (P->assembly_mode == 0) {
+      MF_HIP(hipEventRecord(a, stream()));
+      if (launch_assemble_rows(*P, precision)) return -2;
+      MF_HIP(hipEventRecord(b, stream()));
+      MF_HIP(hipEventRecord(c, stream()));
+    } else {
+      if (launch_zero(P->d_val, P->nnz_sell) || launch_zero(P->d_rhs, P->n)) return -2;
+      MF_HIP(hipEventRecord(a, stream()));
+      if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
+      MF_HIP(hipEventRecord(b, stream()));
+      if (launch_faces(*P, 0, nullptr, 0, P->nbf)) return -2;
+      MF_HIP(hipEventRecord(c, stream()));
+    }
     MF_HIP(hipEventSynchronize(c));
     float x = 0, y = 0;
     hipEventElapsedTime(&x, a, b);
@@ -628,6 +701,17 @@ int mf_last_solve_ms(int pid, double* ms) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
   *ms = P->t_solve_ms;
+  return 0;
+}
+
+int mf_set_assembly_mode(int pid, int mode) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (mode != 0 && mode != 1) {
+    set_error("assembly mode: 0 row-wise (deterministic), 1 element atomics");
+    return -1;
+  }
+  P->assembly_mode = mode;
   return 0;
 }
 

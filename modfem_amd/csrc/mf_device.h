@@ -72,7 +72,22 @@ struct Problem {
   double* d_bf_g = nullptr;
   int* d_row_ptr = nullptr;
   int* d_col = nullptr;
-  double* d_val = nullptr;
+  double* d_val = nullptr;     // SELL-64 values when sell, else CRS
+  // device-native SELL-64 storage (mf_ problems) and its incidence lists
+  bool sell = false;
+  long long nnz_sell = 0;
+  long long* d_sl_ptr = nullptr;
+  int* d_scol = nullptr;
+  uint8_t* d_diag_k = nullptr;
+  int* d_inc_ptr = nullptr;
+  int* d_inc_el = nullptr;
+  uint8_t* d_inc_loc = nullptr;
+  int* d_finc_ptr = nullptr;
+  int* d_finc_f = nullptr;
+  uint8_t* d_finc_loc = nullptr;
+  int* d_brows = nullptr;      // rows touched by boundary faces
+  int nbrows = 0;
+  int assembly_mode = 0;       // 0: row-wise (deterministic), 1: element atomics
   double* d_rhs = nullptr;
   float* d_val32 = nullptr;
   // coefficients
@@ -109,6 +124,7 @@ hipStream_t stream();
 int launch_zero(double* p, long long n);
 int launch_integrate(Problem& P, int precision, int write_elements,
                      double* d_emat, int e0, int ecount);
+int launch_assemble_rows(Problem& P, int precision);
 int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount);
 int launch_fill_jacobi(Problem& P);
 int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,

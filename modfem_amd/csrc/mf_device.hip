@@ -1,5 +1,5 @@
-// HIP kernels for gfx950: element integration + assembly, Dirichlet penalty
-// faces, CRS-stream SpMV and the fused GMRES Arnoldi step.
+// HIP kernels for gfx950: element integration + assembly and Dirichlet
+// penalty faces (the solver kernels are in mf_solver.hip).
 //
 // Reference behaviour followed (paths relative to modfem2017/src/):
 //   element integral  apd_std_lin/aps_std_lin_intf.c:1537-2870 (nreq=1)
@@ -9,18 +9,13 @@
 //   penalty faces     pdd_conv_diff/weak_formulation/approx_std/
 //                     pds_conv_diff_weakform_std.c:253-629
 //   assembly          lsd_mkb/lad_crs/las_crs_intf.c:440-513
-//   Jacobi            lsd_mkb/lad_crs/las_crs_intf.c:707-737
-//   GMRES             lsd_mkb/lsd_mkb_core/lss_mkb_core.c:2317-2668
 //
-// Design: one lane integrates one element (a 64-lane wave is a 64-element
-// batch); the Gauss loop is fully unrolled so reference shape values fold to
-// literals; geometry (Jacobian, determinant, inverse) is FP64; the element
-// matrix is scattered with hardware FP64 atomics through compact per-element
-// row-offset tables (1 byte per entry).  Solver kernels stream contiguous
-// row tiles: CRS values/columns are read coalesced into LDS products and each
-// lane then owns one row, which lets the Arnoldi dot products, the update and
-// the norm run in the same pass as the operator.  Blocks are remapped so
-// that each XCD works on one contiguous eighth of the rows.
+// Design: a group of k lanes integrates one element (k = 6 for prisms, 4 for
+// tetrahedra): geometry (Jacobian, determinant, inverse) is evaluated in
+// FP64, one Gauss point per lane, and staged in LDS; each lane then
+// accumulates one row of the element matrix and scatters it with hardware
+// FP64 atomics through compact per-element row-offset tables (1 byte per
+// entry).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,54 +29,6 @@ namespace mf {
 
 static hipStream_t g_stream = nullptr;
 static int g_device = -1;
-
-// optional per-launch timing of the Arnoldi kernels (HIP events on the
-// library stream) with the algorithmic bytes of every launch
-struct KStats {
-  bool on = false;
-  std::vector<hipEvent_t> ev;  // pairs (start, stop) per launch
-  std::vector<int> kind;       // 0: op_dots, 1: update
-  std::vector<double> bytes;
-  size_t used = 0;
-  double t[2] = {0, 0}, b[2] = {0, 0};
-  long long nl[2] = {0, 0};
-};
-static KStats g_ks;
-
-void kstats_enable(int on) {
-  g_ks.on = on != 0;
-  g_ks.t[0] = g_ks.t[1] = g_ks.b[0] = g_ks.b[1] = 0;
-  g_ks.nl[0] = g_ks.nl[1] = 0;
-}
-
-void kstats_get(double* out) {
-  out[0] = (double)g_ks.nl[0]; out[1] = g_ks.t[0]; out[2] = g_ks.b[0];
-  out[3] = (double)g_ks.nl[1]; out[4] = g_ks.t[1]; out[5] = g_ks.b[1];
-}
-
-static hipEvent_t ks_event() {
-  if (g_ks.used == g_ks.ev.size()) {
-    hipEvent_t e;
-    (void)hipEventCreate(&e);
-    g_ks.ev.push_back(e);
-  }
-  return g_ks.ev[g_ks.used++];
-}
-
-// collect the elapsed times of the recorded launches (after a sync)
-static void ks_collect() {
-  for (size_t q = 0; q < g_ks.kind.size() && 2 * q + 1 < g_ks.used; q++) {
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, g_ks.ev[2 * q], g_ks.ev[2 * q + 1]);
-    int k = g_ks.kind[q];
-    g_ks.t[k] += ms;
-    g_ks.b[k] += g_ks.bytes[q];
-    g_ks.nl[k]++;
-  }
-  g_ks.used = 0;
-  g_ks.kind.clear();
-  g_ks.bytes.clear();
-}
 
 int device_init(int device) {
   if (g_device == device && g_stream) return 0;
@@ -202,7 +149,7 @@ struct IntArgs {
   const double* coor;      // 3*n, DOF order
   const int* el_dof;       // [6][nel]
   const uint8_t* el_off;   // [36][nel]
-  const int* row_ptr;
+  const long long* sl_ptr;  // SELL-64 slice offsets
   double* val;
   double* rhs;
   int coef_model, src_model;
@@ -421,11 +368,11 @@ __global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
   }
   // phase 3
   const int dw = s_dof[le][w];
-  const int base = A.row_ptr[dw];
+  const long long base = A.sl_ptr[dw >> 6] + (dw & 63);
 #pragma unroll
   for (int u = 0; u < NS; u++) {
     const int off = A.el_off[(size_t)(w * 6 + u) * N + e];
-    unsafeAtomicAdd(&A.val[base + off], (double)K[u]);
+    unsafeAtomicAdd(&A.val[base + 64LL * off], (double)K[u]);
   }
   unsafeAtomicAdd(&A.rhs[dw], (double)F);
 }
@@ -442,7 +389,7 @@ struct FaceArgs {
   const int* bf_elem;
   const int* bf_side;
   const double* bf_g;
-  const int* row_ptr;
+  const long long* sl_ptr;  // SELL-64 slice offsets
   double* val;
   double* rhs;
   int src_model;
@@ -568,11 +515,11 @@ __global__ void __launch_bounds__(256) k_faces(FaceArgs A) {
   }
 #pragma unroll
   for (int w = 0; w < NS; w++) {
-    const int base = A.row_ptr[dof[w]];
+    const long long base = A.sl_ptr[dof[w] >> 6] + (dof[w] & 63);
 #pragma unroll
     for (int u = 0; u < NS; u++) {
       const int off = A.el_off[(size_t)(w * 6 + u) * N + e];
-      unsafeAtomicAdd(&A.val[base + off], K[w][u]);
+      unsafeAtomicAdd(&A.val[base + 64LL * off], K[w][u]);
     }
     unsafeAtomicAdd(&A.rhs[dof[w]], F[w]);
   }
@@ -597,7 +544,7 @@ static void fill_int_args(Problem& P, IntArgs& A) {
   A.coor = P.d_coor;
   A.el_dof = P.d_el_dof;
   A.el_off = P.d_el_off;
-  A.row_ptr = P.d_row_ptr;
+  A.sl_ptr = P.d_sl_ptr;
   A.val = P.d_val;
   A.rhs = P.d_rhs;
   A.coef_model = P.coef_model;
@@ -645,7 +592,7 @@ int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount
   FaceArgs A;
   A.nel = P.nel; A.nbf = P.nbf; A.coor = P.d_coor; A.el_dof = P.d_el_dof;
   A.el_off = P.d_el_off; A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side;
-  A.bf_g = P.d_bf_g; A.row_ptr = P.d_row_ptr; A.val = P.d_val; A.rhs = P.d_rhs;
+  A.bf_g = P.d_bf_g; A.sl_ptr = P.d_sl_ptr; A.val = P.d_val; A.rhs = P.d_rhs;
   A.src_model = P.src_model; A.write_faces = write_faces; A.fmat = d_fmat;
   A.f0 = f0; A.count = fcount;
   dim3 g((fcount + 255) / 256);
@@ -658,608 +605,347 @@ int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount
 }
 
 // ---------------------------------------------------------------------
-// Solver kernels
+// Row-wise integrate + assemble (default assembly path): one lane owns one
+// matrix row (DOF r) and walks the integration entities that contain r in
+// ascending order -- elements, then boundary faces -- exactly the order in
+// which the reference's sequential create/assemble loop adds into that row
+// (sid_mkb/sis_mkb_intf.c:1843-1866).  For each entity the lane evaluates
+// the element-matrix row of r (FP64 geometry at every Gauss point) and adds
+// it into register accumulators indexed by the row position from the
+// compact table.  A wavefront owns one 64-row SELL slice and writes it once
+// with coalesced stores: no atomics, no zero fill, bitwise reproducible.
 // ---------------------------------------------------------------------
-constexpr int SB = 256;         // threads per solver block
-constexpr int TILE_NNZ = 4096;  // CRS entries staged per tile (32 KB LDS)
-constexpr int MAX_BLOCKS = 2048;
-
-// blocks are dispatched round-robin over the 8 XCDs; give XCD x the x-th
-// contiguous range of logical blocks so its L2 sees neighbouring rows
-__device__ __forceinline__ int xcd_block(int bid, int nb) {
-  int xcd = bid & 7, k = bid >> 3;
-  int per = nb >> 3, rem = nb & 7;
-  return xcd * per + min(xcd, rem) + k;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// reduce NV per-thread values over the block; results in red[0..NV-1]
-template <int NV>
-__device__ __forceinline__ void block_sum_multi(double* acc, int nact, double* red) {
-  __shared__ double wsum[SB / 64][NV > 0 ? NV : 1];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    if (i < nact) {
-      double v = wave_sum(acc[i]);
-      if (lane == 0) wsum[wid][i] = v;
-    }
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < nact) {
-    double s = 0;
-#pragma unroll
-    for (int w = 0; w < SB / 64; w++) s += wsum[w][threadIdx.x];
-    red[threadIdx.x] = s;
-  }
-  __syncthreads();
-}
-
-// signal completion; returns true in the last block to finish
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-  __shared__ bool am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last) __threadfence();
-  return am_last;
-}
-
-struct SolveArgs {
-  int n, ntiles, m;
-  const int* tile_row;
-  const int* row_ptr;
-  const int* col;
-  const double* val;
-  const double* dinv;   // null: no preconditioner
-  const double* b;
-  double* x;
-  double* V;            // (m+1) x n
-  double* H;            // (m+1)x(m+1) + givens 2(m+1) + scale (m+1) + coef (m+1)
-  double* partials;
-  unsigned* counter;
-  GmresStat* stat;
-  GmresStat* hstat;     // pinned mirror
+struct RowArgs {
+  IntArgs I;              // element data + coefficients (val/rhs unused)
+  int n;
+  const int* inc_ptr;
+  const int* inc_el;
+  const uint8_t* inc_loc;
+  const int* finc_ptr;
+  const int* finc_f;
+  const uint8_t* finc_loc;
+  const int* bf_elem;
+  const int* bf_side;
+  const double* bf_g;
+  double* val;            // SELL-64
+  double* rhs;
 };
 
-__device__ __forceinline__ double* Hcol(const SolveArgs& A, int c) { return A.H + (size_t)(A.m + 1) * c; }
-__device__ __forceinline__ double* Giv(const SolveArgs& A) { return A.H + (size_t)(A.m + 1) * (A.m + 1); }
-__device__ __forceinline__ double* Scale(const SolveArgs& A) { return Giv(A) + 2 * (A.m + 1); }
-__device__ __forceinline__ double* Coef(const SolveArgs& A) { return Scale(A) + (A.m + 1); }
-
-// products of one tile into LDS and per-row sums; returns the row sum for
-// thread's row (valid when threadIdx.x < nrows)
-__device__ __forceinline__ double tile_rows(const SolveArgs& A, const double* x, int r0,
-                                            int r1, double* prod) {
-  const int k0 = A.row_ptr[r0], k1 = A.row_ptr[r1];
-  for (int k = k0 + threadIdx.x; k < k1; k += SB) prod[k - k0] = A.val[k] * x[A.col[k]];
-  __syncthreads();
-  double s = 0.0;
-  const int r = r0 + threadIdx.x;
-  if (r < r1) {
-    const int a = A.row_ptr[r] - k0, z = A.row_ptr[r + 1] - k0;
-    for (int k = a; k < z; k++) s += prod[k];
+template <int NS>
+__device__ __forceinline__ void gather_nodes(const IntArgs& I, int e, double (*xn)[3]) {
+  const size_t N = I.nel;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    const int d = I.el_dof[i * N + e];
+    xn[i][0] = I.coor[3 * (size_t)d];
+    xn[i][1] = I.coor[3 * (size_t)d + 1];
+    xn[i][2] = I.coor[3 * (size_t)d + 2];
   }
-  __syncthreads();
-  return s;
 }
 
-// v0 = M^-1 (b - A x); rnrm = ||v0||
-__global__ void __launch_bounds__(SB) k_residual(SolveArgs A) {
-  __shared__ double prod[TILE_NNZ];
-  __shared__ double red[1];
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t0 = (int)((long long)lb * A.ntiles / gridDim.x);
-  const int t1 = (int)((long long)(lb + 1) * A.ntiles / gridDim.x);
-  double acc = 0.0;
-  for (int t = t0; t < t1; t++) {
-    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
-    double s = tile_rows(A, A.x, r0, r1, prod);
-    const int r = r0 + threadIdx.x;
-    if (r < r1) {
-      double v = A.b[r] - s;
-      if (A.dinv) v *= A.dinv[r];
-      A.V[r] = v;
-      acc += v * v;
+// reference shape function i and its local derivatives at eta (i is a
+// compile-time index after unrolling, eta runtime)
+template <int NS>
+__device__ __forceinline__ void ref_one(int i, const double* eta, double& phi, double& dx,
+                                        double& dy, double& dz) {
+  double p[NS], x[NS], y[NS], z[NS];
+  ref_shape<NS>(eta[0], eta[1], eta[2], p, x, y, z);
+  phi = p[0]; dx = x[0]; dy = y[0]; dz = z[0];
+#pragma unroll
+  for (int k = 1; k < NS; k++)
+    if (k == i) { phi = p[k]; dx = x[k]; dy = y[k]; dz = z[k]; }
+}
+
+// row w of element e's matrix and load vector, Gauss points outer, kept to a
+// small live set: node coordinates, the row accumulators and one Gauss
+// point's Jacobian inverse
+template <int NS, class Q, int TS, typename R>
+__device__ __forceinline__ void element_row(const IntArgs& I, int e, int w, R* Krow, R& Fw) {
+  double xn[NS][3];
+  gather_nodes<NS>(I, e, xn);
+  Coefs C;
+  load_coefs(I, e, C);
+  const TermMask M = I.mask;
+  const R cm = (R)((M.reac ? C.c : 0.0) + (M.m ? C.m : 0.0));
+#pragma unroll
+  for (int u = 0; u < NS; u++) Krow[u] = 0;
+  Fw = 0;
+#pragma unroll 1
+  for (int g = 0; g < Q::NG; g++) {
+    double eta[3];
+    gauss_point<Q>(g, eta);
+    double phi[NS], dx[NS], dy[NS], dz[NS], J[9], Ji[9];
+    ref_shape<NS>(eta[0], eta[1], eta[2], phi, dx, dy, dz);
+    const double det = jacobian<NS>(xn, dx, dy, dz, J, Ji);
+    double wg = 0;
+#pragma unroll
+    for (int k = 0; k < Q::NG; k++)
+      if (k == g) wg = Q::w(k);
+    const R v = (R)(fabs(det) * wg);
+    double pw, dxw, dyw, dzw;
+    pw = phi[0]; dxw = dx[0]; dyw = dy[0]; dzw = dz[0];
+#pragma unroll
+    for (int k = 1; k < NS; k++)
+      if (k == w) { pw = phi[k]; dxw = dx[k]; dyw = dy[k]; dzw = dz[k]; }
+    const R gwx = (R)(dxw * Ji[0] + dyw * Ji[3] + dzw * Ji[6]);
+    const R gwy = (R)(dxw * Ji[1] + dyw * Ji[4] + dzw * Ji[7]);
+    const R gwz = (R)(dxw * Ji[2] + dyw * Ji[5] + dzw * Ji[8]);
+    const R phw = (R)pw;
+    R aw0, aw1, aw2;
+    if (M.diff_full) {
+      aw0 = (R)C.a[0] * gwx + (R)C.a[3] * gwy + (R)C.a[6] * gwz;
+      aw1 = (R)C.a[1] * gwx + (R)C.a[4] * gwy + (R)C.a[7] * gwz;
+      aw2 = (R)C.a[2] * gwx + (R)C.a[5] * gwy + (R)C.a[8] * gwz;
+    } else if (M.diff_diag) {
+      aw0 = (R)C.a[0] * gwx; aw1 = (R)C.a[4] * gwy; aw2 = (R)C.a[8] * gwz;
+    } else {
+      aw0 = aw1 = aw2 = 0;
     }
-  }
-  block_sum_multi<1>(&acc, 1, red);
-  if (threadIdx.x == 0) A.partials[lb] = red[0];
-  if (last_block(A.counter)) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += SB) s += A.partials[b];
-    block_sum_multi<1>(&s, 1, red);
-    if (threadIdx.x == 0) {
-      A.stat->rnrm = sqrt(red[0]);
-      *A.hstat = *A.stat;
-      *A.counter = 0;
+    R tw = 0;
+    if (TS >= TS_ALL && M.tconv) tw = (R)C.t[0] * gwx + (R)C.t[1] * gwy + (R)C.t[2] * gwz;
+    const R bphw0 = (R)C.b[0] * phw, bphw1 = (R)C.b[1] * phw, bphw2 = (R)C.b[2] * phw;
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+      const R gux = (R)(dx[u] * Ji[0] + dy[u] * Ji[3] + dz[u] * Ji[6]);
+      const R guy = (R)(dx[u] * Ji[1] + dy[u] * Ji[4] + dz[u] * Ji[7]);
+      const R guz = (R)(dx[u] * Ji[2] + dy[u] * Ji[5] + dz[u] * Ji[8]);
+      R k = gux * aw0 + guy * aw1 + guz * aw2;
+      if (TS >= TS_DIFF_CONV && M.conv) k += gux * bphw0 + guy * bphw1 + guz * bphw2;
+      if (TS >= TS_ALL) k += (R)phi[u] * (tw + cm * phw);
+      Krow[u] += k * v;
     }
-  }
-}
-
-// Arnoldi step j, part 1: w = s_j M^-1 A V_j (stored in V_{j+1});
-// h_i = s_i (V_i . w), i <= j
-template <int NV>
-__global__ void __launch_bounds__(SB) k_op_dots(SolveArgs A, int j) {
-  if (A.stat->stop) return;
-  __shared__ double prod[TILE_NNZ];
-  __shared__ double red[NV];
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t0 = (int)((long long)lb * A.ntiles / gridDim.x);
-  const int t1 = (int)((long long)(lb + 1) * A.ntiles / gridDim.x);
-  const size_t n = A.n;
-  const double* x = A.V + (size_t)j * n;
-  double* w = A.V + (size_t)(j + 1) * n;
-  const double sj = Scale(A)[j];
-  double acc[NV];
+    double sval = C.s;
+    if (I.src_model != SRC_CONST) {
+      double xc[3] = {0, 0, 0};
+      if constexpr (NS == 6) {
 #pragma unroll
-  for (int i = 0; i < NV; i++) acc[i] = 0.0;
-  for (int t = t0; t < t1; t++) {
-    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
-    double s = tile_rows(A, x, r0, r1, prod);
-    const int r = r0 + threadIdx.x;
-    if (r < r1) {
-      double wr = sj * s;
-      if (A.dinv) wr *= A.dinv[r];
-      w[r] = wr;
+        for (int i = 0; i < 6; i++) {
+          xc[0] += xn[i][0] * phi[i]; xc[1] += xn[i][1] * phi[i]; xc[2] += xn[i][2] * phi[i];
+        }
+      } else {
 #pragma unroll
-      for (int i = 0; i < NV; i++)
-        if (i <= j) acc[i] += A.V[(size_t)i * n + r] * wr;
-    }
-  }
-  block_sum_multi<NV>(acc, j + 1, red);
-  if ((int)threadIdx.x <= j) A.partials[(size_t)lb * NV + threadIdx.x] = red[threadIdx.x];
-  if (last_block(A.counter)) {
-    double acc2[NV];
-#pragma unroll
-    for (int i = 0; i < NV; i++) acc2[i] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += SB)
-#pragma unroll
-      for (int i = 0; i < NV; i++)
-        if (i <= j) acc2[i] += A.partials[(size_t)b * NV + i];
-    block_sum_multi<NV>(acc2, j + 1, red);
-    if ((int)threadIdx.x <= j) {
-      double h = Scale(A)[threadIdx.x] * red[threadIdx.x];
-      Hcol(A, j + 1)[threadIdx.x] = h;
-      Coef(A)[threadIdx.x] = h * Scale(A)[threadIdx.x];
-    }
-    if (threadIdx.x == 0) *A.counter = 0;
-  }
-}
-
-__device__ void drotg_dev(double* a, double* b, double* c, double* s) {
-  double roe = *b;
-  if (fabs(*a) > fabs(*b)) roe = *a;
-  double scale = fabs(*a) + fabs(*b);
-  double r, z;
-  if (scale == 0.0) {
-    *c = 1.0; *s = 0.0; r = 0.0; z = 0.0;
-  } else {
-    double ra = *a / scale, rb = *b / scale;
-    r = scale * sqrt(ra * ra + rb * rb);
-    r = (roe >= 0 ? 1.0 : -1.0) * r;
-    *c = *a / r;
-    *s = *b / r;
-    z = 1.0;
-    if (fabs(*a) > fabs(*b)) z = *s;
-    if (fabs(*b) >= fabs(*a) && *c != 0.0) z = 1.0 / *c;
-  }
-  *a = r;
-  *b = z;
-}
-
-__device__ __forceinline__ void rot(double* x, double* y, double c, double s) {
-  double t = c * (*x) + s * (*y);
-  *y = c * (*y) - s * (*x);
-  *x = t;
-}
-
-// Arnoldi step j, part 2: w -= sum_i c_i V_i; h_{j+1,j} = ||w||; Givens
-template <int NV>
-__global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
-  if (A.stat->stop) return;
-  __shared__ double red[1];
-  __shared__ double cs[NV];
-  if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
-  __syncthreads();
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const size_t n = A.n;
-  const size_t r0 = (size_t)lb * n / gridDim.x, r1 = (size_t)(lb + 1) * n / gridDim.x;
-  double* w = A.V + (size_t)(j + 1) * n;
-  double acc = 0.0;
-  for (size_t r = r0 + threadIdx.x; r < r1; r += SB) {
-    double wr = w[r];
-#pragma unroll
-    for (int i = 0; i < NV; i++)
-      if (i <= j) wr -= cs[i] * A.V[(size_t)i * n + r];
-    w[r] = wr;
-    acc += wr * wr;
-  }
-  block_sum_multi<1>(&acc, 1, red);
-  if (threadIdx.x == 0) A.partials[lb] = red[0];
-  if (last_block(A.counter)) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += SB) s += A.partials[b];
-    block_sum_multi<1>(&s, 1, red);
-    if (threadIdx.x == 0) {
-      *A.counter = 0;
-      const int m = A.m;
-      double hn = sqrt(red[0]);
-      double* col = Hcol(A, j + 1);
-      col[j + 1] = hn;
-      Scale(A)[j + 1] = 1.0 / hn;
-      double* giv = Giv(A);
-      for (int i = 1; i <= j; i++) rot(&col[i - 1], &col[i], giv[2 * (i - 1)], giv[2 * (i - 1) + 1]);
-      double ra = col[j], rb = col[j + 1];
-      drotg_dev(&ra, &rb, &giv[2 * j], &giv[2 * j + 1]);
-      rot(&col[j], &col[j + 1], giv[2 * j], giv[2 * j + 1]);
-      double* g = Hcol(A, 0);
-      rot(&g[j], &g[j + 1], giv[2 * j], giv[2 * j + 1]);
-      GmresStat* st = A.stat;
-      st->gmiter += 1;
-      st->j_done = j + 1;
-      st->resid = fabs(g[j + 1]);
-      st->res_rel = st->beta0 != 0.0 ? st->resid / st->beta0 : 0.0;
-      if (st->resid < st->tol_res || st->res_rel < st->tol_rel) {
-        st->converged = 1;
-        st->stop = 1;
+        for (int d = 0; d < 3; d++)
+          xc[d] = xn[0][d] + eta[0] * J[3 * d] + eta[1] * J[3 * d + 1] + eta[2] * J[3 * d + 2];
       }
-      if (j + 1 == m || st->gmiter == st->max_iter) st->stop = 1;
-      *A.hstat = *st;
+      double uu, lapl;
+      exact_sol(I.src_model, xc, &uu, &lapl);
+      sval = -lapl;
     }
+    R f = 0;
+    if (TS >= TS_ALL && M.q) f += (R)C.q[0] * gwx + (R)C.q[1] * gwy + (R)C.q[2] * gwz;
+    if (M.s) f += (R)sval * phw;
+    Fw += f * v;
   }
 }
 
-// x += sum_c y_c s_c V_c  (coefficients y_c s_c in Coef)
-__global__ void __launch_bounds__(SB) k_xupdate(SolveArgs A, int ik) {
-  __shared__ double cs[64];
-  if ((int)threadIdx.x < ik) cs[threadIdx.x] = Coef(A)[threadIdx.x];
-  __syncthreads();
-  const size_t n = A.n;
-  for (size_t r = (size_t)blockIdx.x * SB + threadIdx.x; r < n; r += (size_t)gridDim.x * SB) {
-    double xr = A.x[r];
-    for (int c = 0; c < ik; c++) xr += A.V[(size_t)c * n + r] * cs[c];
-    A.x[r] = xr;
-  }
-}
-
-__global__ void k_fill_jacobi(int n, const int* rp, const int* col, const double* val,
-                              double* dinv) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double d = 0.0;
-  for (int k = rp[r]; k < rp[r + 1]; k++)
-    if (col[k] == r) d = val[k];
-  dinv[r] = 1.0 / d;
-}
-
-// plain y = A x through the same tiles
-__global__ void __launch_bounds__(SB) k_spmv(SolveArgs A, const double* x, double* y) {
-  __shared__ double prod[TILE_NNZ];
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t0 = (int)((long long)lb * A.ntiles / gridDim.x);
-  const int t1 = (int)((long long)(lb + 1) * A.ntiles / gridDim.x);
-  for (int t = t0; t < t1; t++) {
-    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
-    double s = tile_rows(A, x, r0, r1, prod);
-    const int r = r0 + threadIdx.x;
-    if (r < r1) y[r] = s;
-  }
-}
-
-static void fill_solve_args(Problem& P, SolveArgs& A, int precon, int m);
-
-// Reference-order row operations for the storage seam (lar_*):
-//   mode 0: v = (use_rhs ? b : 0) - A x        lar_compute_residual_crs
-//                                              (las_crs_intf.c:788-849)
-//   mode 1: v = d .* ((use_rhs ? b : 0) - sum_{j!=i} a_ij x_j)
-//                                              lar_perform_BJ_or_GS_iterations_crs,
-//                                              BLOCK_JACOBI (las_crs_intf.c:857-1013)
-// Row sums run in column order over LDS-staged products without FMA
-// contraction, matching the reference's sequential "v -= a*x" exactly.
-__global__ void __launch_bounds__(SB) k_rowop(SolveArgs A, int mode, int use_rhs,
-                                              const double* x, const double* b,
-                                              const int* diag_ptr, double* v) {
-  __shared__ double prod[TILE_NNZ];
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int t0 = (int)((long long)lb * A.ntiles / gridDim.x);
-  const int t1 = (int)((long long)(lb + 1) * A.ntiles / gridDim.x);
-  for (int t = t0; t < t1; t++) {
-    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
-    const int k0 = A.row_ptr[r0], k1 = A.row_ptr[r1];
-    if (x)
-      for (int k = k0 + threadIdx.x; k < k1; k += SB) prod[k - k0] = A.val[k] * x[A.col[k]];
-    __syncthreads();
-    const int r = r0 + threadIdx.x;
-    if (r < r1) {
-      double s = 0.0;
-      if (x) {
-        const int a = A.row_ptr[r] - k0, z = A.row_ptr[r + 1] - k0;
-        const int dk = (mode == 1) ? diag_ptr[r] - k0 : -1;
-        for (int k = a; k < z; k++)
-          if (k != dk) s = __dsub_rn(s, prod[k]);
+// penalty-face row w of the owning element's local matrix
+template <int NS>
+__device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double* Krow, double& Fw) {
+  const int e = A.bf_elem[f], side = A.bf_side[f];
+  double xn[NS][3];
+  gather_nodes<NS>(A.I, e, xn);
+  const double gconst = A.bf_g ? A.bf_g[f] : 0.0;
+#pragma unroll
+  for (int u = 0; u < NS; u++) Krow[u] = 0;
+  Fw = 0;
+  const bool quad = (NS == 6 && side >= 2);
+  const int ng = quad ? 4 : 3;
+  for (int g = 0; g < ng; g++) {
+    double c0, c1, wgt;
+    if (quad) {
+      c0 = (g / 2 == 0) ? -G2 : G2;
+      c1 = (g % 2 == 0) ? -G2 : G2;
+      wgt = 0.5 * 1.0 * 1.0;
+    } else {
+      c0 = (g == 2) ? T2 : T1;
+      c1 = (g == 1) ? T2 : T1;
+      wgt = 0.5 * TW;
+    }
+    double eta[3];
+    if constexpr (NS == 6) {
+      switch (side) {
+        case 0: eta[0] = c1; eta[1] = c0; eta[2] = -1.0; break;
+        case 1: eta[0] = c0; eta[1] = c1; eta[2] = 1.0; break;
+        case 2: eta[0] = 0.5 * (c0 + 1.0); eta[1] = 0.0; eta[2] = c1; break;
+        case 3: eta[0] = 0.5 * (-c0 + 1.0); eta[1] = 1.0 - eta[0]; eta[2] = c1; break;
+        default: eta[0] = 0.0; eta[1] = 0.5 * (-c0 + 1.0); eta[2] = c1; break;
       }
-      if (use_rhs) s = __dadd_rn(s, b[r]);
-      if (mode == 1) s = __dmul_rn(s, A.dinv[r]);
-      v[r] = s;
+    } else {
+      const double vv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+      const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      const double *p0 = vv[fn[side][0]], *p1 = vv[fn[side][1]], *p2 = vv[fn[side][2]];
+      for (int d = 0; d < 3; d++) eta[d] = p0[d] + c0 * (p1[d] - p0[d]) + c1 * (p2[d] - p0[d]);
     }
-    __syncthreads();
-  }
-}
-
-__global__ void k_diag_ptr(int n, const int* rp, const int* col, int* dp) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  int p = -1;
-  for (int k = rp[r]; k < rp[r + 1]; k++)
-    if (col[k] == r) p = k;
-  dp[r] = p;
-}
-
-int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,
-                 const int* d_diag_ptr, double* d_v) {
-  SolveArgs A;
-  fill_solve_args(P, A, mode == 1, 0);
-  int G = std::min(P.ntiles, MAX_BLOCKS);
-  hipLaunchKernelGGL(k_rowop, dim3(G), dim3(SB), 0, g_stream, A, mode, use_rhs, d_x, d_b,
-                     d_diag_ptr, d_v);
-  MF_HIP(hipGetLastError());
-  return 0;
-}
-
-int launch_diag_ptr(Problem& P, int* d_dp) {
-  hipLaunchKernelGGL(k_diag_ptr, dim3((P.n + 255) / 256), dim3(256), 0, g_stream, P.n,
-                     P.d_row_ptr, P.d_col, d_dp);
-  MF_HIP(hipGetLastError());
-  return 0;
-}
-
-int build_tiles(Problem& P) {
-  const auto& rp = P.setup.row_ptr;
-  P.tile_row.clear();
-  P.tile_row.push_back(0);
-  int r = 0;
-  while (r < P.n) {
-    int start = r;
-    while (r < P.n && r - start < SB && rp[r + 1] - rp[start] <= TILE_NNZ) r++;
-    if (r == start) {
-      set_error("a CRS row exceeds the tile capacity");
-      return -1;
+    double phi[NS], dx[NS], dy[NS], dz[NS], J[9], Ji[9];
+    ref_shape<NS>(eta[0], eta[1], eta[2], phi, dx, dy, dz);
+    double ds[3];
+    if constexpr (NS == 6) {
+      jacobian<NS>(xn, dx, dy, dz, J, Ji);
+      switch (side) {
+        case 0:
+          ds[0] = -J[3] * J[7] + J[6] * J[4]; ds[1] = -J[6] * J[1] + J[0] * J[7];
+          ds[2] = -J[0] * J[4] + J[3] * J[1]; break;
+        case 1:
+          ds[0] = J[3] * J[7] - J[6] * J[4]; ds[1] = J[6] * J[1] - J[0] * J[7];
+          ds[2] = J[0] * J[4] - J[3] * J[1]; break;
+        case 2:
+          ds[0] = J[3] * J[8] - J[6] * J[5]; ds[1] = J[6] * J[2] - J[0] * J[8];
+          ds[2] = J[0] * J[5] - J[3] * J[2]; break;
+        case 3:
+          ds[0] = (J[4] - J[3]) * J[8] - (J[7] - J[6]) * J[5];
+          ds[1] = (J[7] - J[6]) * J[2] - (J[1] - J[0]) * J[8];
+          ds[2] = (J[1] - J[0]) * J[5] - (J[4] - J[3]) * J[2]; break;
+        default:
+          ds[0] = J[5] * J[7] - J[8] * J[4]; ds[1] = J[8] * J[1] - J[2] * J[7];
+          ds[2] = J[2] * J[4] - J[5] * J[1]; break;
+      }
+    } else {
+      const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      const double *p0 = xn[fn[side][0]], *p1 = xn[fn[side][1]], *p2 = xn[fn[side][2]];
+      double v1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+      double v2[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+      ds[0] = v1[1] * v2[2] - v1[2] * v2[1];
+      ds[1] = v1[2] * v2[0] - v1[0] * v2[2];
+      ds[2] = v1[0] * v2[1] - v1[1] * v2[0];
     }
-    P.tile_row.push_back(r);
+    const double area = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]) * wgt;
+    double gv = gconst;
+    if (A.I.src_model != SRC_CONST) {
+      double xc[3] = {0, 0, 0};
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        xc[0] += xn[i][0] * phi[i]; xc[1] += xn[i][1] * phi[i]; xc[2] += xn[i][2] * phi[i];
+      }
+      double lapl;
+      exact_sol(A.I.src_model, xc, &gv, &lapl);
+    }
+    double phw = 0;
+#pragma unroll
+    for (int i = 0; i < NS; i++)
+      if (i == w) phw = phi[i];
+    const double pen = 1.0e7;
+#pragma unroll
+    for (int u = 0; u < NS; u++) Krow[u] += pen * phi[u] * phw * area;
+    Fw += pen * gv * phw * area;
   }
-  P.ntiles = (int)P.tile_row.size() - 1;
-  if (P.d_tile_row) hipFree(P.d_tile_row);
-  MF_HIP(hipMalloc(&P.d_tile_row, P.tile_row.size() * sizeof(int)));
-  MF_HIP(hipMemcpy(P.d_tile_row, P.tile_row.data(), P.tile_row.size() * sizeof(int),
-                   hipMemcpyHostToDevice));
-  return 0;
 }
 
-int launch_fill_jacobi(Problem& P) {
-  if (!P.d_dinv) MF_HIP(hipMalloc(&P.d_dinv, (size_t)P.n * sizeof(double)));
-  hipLaunchKernelGGL(k_fill_jacobi, dim3((P.n + 255) / 256), dim3(256), 0, g_stream, P.n,
-                     P.d_row_ptr, P.d_col, P.d_val, P.d_dinv);
-  MF_HIP(hipGetLastError());
-  return 0;
-}
+constexpr int ROW_WAVES = 4;  // wavefronts (SELL slices) per block
 
-static int ensure_workspace(Problem& P, int m) {
-  if (P.kryl_alloc >= m && P.d_V) return 0;
-  if (P.d_V) hipFree(P.d_V);
-  if (P.d_H) hipFree(P.d_H);
-  MF_HIP(hipMalloc(&P.d_V, (size_t)(m + 1) * P.n * sizeof(double)));
-  MF_HIP(hipMalloc(&P.d_H, (size_t)((m + 1) * (m + 1) + 4 * (m + 1)) * sizeof(double)));
-  if (!P.d_partials) MF_HIP(hipMalloc(&P.d_partials, (size_t)MAX_BLOCKS * 64 * sizeof(double)));
-  if (!P.d_counter) {
-    MF_HIP(hipMalloc(&P.d_counter, sizeof(unsigned)));
-    MF_HIP(hipMemset(P.d_counter, 0, sizeof(unsigned)));
+// elements: one lane per row, accumulators in LDS (k-major per wave, so the
+// 64 lanes of a wave touch 64 consecutive words)
+template <int NS, class Q, int TS, int RMAX, typename R>
+__global__ void __launch_bounds__(64 * ROW_WAVES) k_assemble_rows(RowArgs A) {
+  __shared__ double s_acc[ROW_WAVES][RMAX][64];
+  const int nsl = (A.n + 63) >> 6;
+  const int wv = threadIdx.x >> 6;
+  const int sl = blockIdx.x * ROW_WAVES + wv;
+  if (sl >= nsl) return;  // wave-uniform, no block barriers below
+  const int lane = threadIdx.x & 63;
+  const int r = sl * 64 + lane;
+  double(*acc)[64] = s_acc[wv];
+#pragma unroll
+  for (int k = 0; k < RMAX; k++) acc[k][lane] = 0.0;
+  double F = 0.0;
+  if (r < A.n) {
+    const size_t N = A.I.nel;
+    for (int p = A.inc_ptr[r]; p < A.inc_ptr[r + 1]; p++) {
+      const int e = A.inc_el[p], w = A.inc_loc[p];
+      R Kr[NS], Fr;
+      element_row<NS, Q, TS, R>(A.I, e, w, Kr, Fr);
+#pragma unroll
+      for (int u = 0; u < NS; u++) {
+        const int o = A.I.el_off[(size_t)(w * 6 + u) * N + e];
+        acc[o][lane] += (double)Kr[u];
+      }
+      F += (double)Fr;
+    }
   }
-  if (!P.d_stat) MF_HIP(hipMalloc(&P.d_stat, sizeof(GmresStat)));
-  if (!P.h_stat) MF_HIP(hipHostMalloc(&P.h_stat, sizeof(GmresStat), hipHostMallocDefault));
-  if (!P.d_x) MF_HIP(hipMalloc(&P.d_x, (size_t)P.n * sizeof(double)));
-  P.kryl_alloc = m;
-  return 0;
+  const long long base = A.I.sl_ptr[sl];
+  const int len = (int)((A.I.sl_ptr[sl + 1] - base) >> 6);
+  for (int k = 0; k < len; k++) A.val[base + 64LL * k + lane] = acc[k][lane];
+  if (r < A.n) A.rhs[r] = F;
 }
 
-static void fill_solve_args(Problem& P, SolveArgs& A, int precon, int m) {
-  A.n = P.n; A.ntiles = P.ntiles; A.m = m;
-  A.tile_row = P.d_tile_row; A.row_ptr = P.d_row_ptr; A.col = P.d_col; A.val = P.d_val;
-  A.dinv = precon ? P.d_dinv : nullptr;
-  A.b = P.d_rhs; A.x = P.d_x; A.V = P.d_V; A.H = P.d_H; A.partials = P.d_partials;
-  A.counter = P.d_counter; A.stat = P.d_stat; A.hstat = P.h_stat;
-}
-
-template <int NV>
-static void launch_step(dim3 g_op, dim3 g_up, SolveArgs& A, int j) {
-  if (g_ks.on) {
-    // algorithmic bytes: CRS values+columns, row pointers, the input vector,
-    // the Jacobi diagonal, the output, and V_0..V_{j-1} for the dots
-    const double n = A.n;
-    hipEvent_t a = ks_event(), b = ks_event();
-    (void)hipEventRecord(a, g_stream);
-    hipLaunchKernelGGL(k_op_dots<NV>, g_op, dim3(SB), 0, g_stream, A, j);
-    (void)hipEventRecord(b, g_stream);
-    g_ks.kind.push_back(0);
-    g_ks.bytes.push_back(-1.0 - j);  // resolved in the solver (needs nnz)
-    hipEvent_t c = ks_event(), d = ks_event();
-    (void)hipEventRecord(c, g_stream);
-    hipLaunchKernelGGL(k_update<NV>, g_up, dim3(SB), 0, g_stream, A, j);
-    (void)hipEventRecord(d, g_stream);
-    g_ks.kind.push_back(1);
-    g_ks.bytes.push_back(8.0 * n * (j + 3));
-    return;
+// boundary faces, added afterwards to the rows they touch (the reference
+// assembles faces after all elements); one lane per such row
+template <int NS>
+__global__ void __launch_bounds__(256) k_assemble_face_rows(RowArgs A, const int* rows, int nrows) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int r = rows[t];
+  const long long base = A.I.sl_ptr[r >> 6] + (r & 63);
+  const size_t N = A.I.nel;
+  double F = 0.0;
+  for (int p = A.finc_ptr[r]; p < A.finc_ptr[r + 1]; p++) {
+    const int f = A.finc_f[p], w = A.finc_loc[p];
+    const int e = A.bf_elem[f];
+    double Kd[NS], Fd;
+    face_row<NS>(A, f, w, Kd, Fd);
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+      const int o = A.I.el_off[(size_t)(w * 6 + u) * N + e];
+      A.val[base + 64LL * o] += Kd[u];
+    }
+    F += Fd;
   }
-  hipLaunchKernelGGL(k_op_dots<NV>, g_op, dim3(SB), 0, g_stream, A, j);
-  hipLaunchKernelGGL(k_update<NV>, g_up, dim3(SB), 0, g_stream, A, j);
+  A.rhs[r] += F;
 }
 
-static void launch_arnoldi(dim3 g_op, dim3 g_up, SolveArgs& A, int j) {
-  int nv = j + 1;
-  if (nv <= 8) launch_step<8>(g_op, g_up, A, j);
-  else if (nv <= 16) launch_step<16>(g_op, g_up, A, j);
-  else if (nv <= 24) launch_step<24>(g_op, g_up, A, j);
-  else if (nv <= 32) launch_step<32>(g_op, g_up, A, j);
-  else if (nv <= 40) launch_step<40>(g_op, g_up, A, j);
-  else if (nv <= 48) launch_step<48>(g_op, g_up, A, j);
-  else launch_step<64>(g_op, g_up, A, j);
+template <int NS, class Q, int RMAX, typename R>
+static void launch_rows_ts(int ts, dim3 g, RowArgs& A) {
+  const dim3 b(64 * ROW_WAVES);
+  if (ts == TS_DIFF)
+    hipLaunchKernelGGL((k_assemble_rows<NS, Q, TS_DIFF, RMAX, R>), g, b, 0, g_stream, A);
+  else if (ts == TS_DIFF_CONV)
+    hipLaunchKernelGGL((k_assemble_rows<NS, Q, TS_DIFF_CONV, RMAX, R>), g, b, 0, g_stream, A);
+  else
+    hipLaunchKernelGGL((k_assemble_rows<NS, Q, TS_ALL, RMAX, R>), g, b, 0, g_stream, A);
 }
 
-int launch_spmv(Problem& P, const double* d_x, double* d_y) {
-  SolveArgs A;
-  fill_solve_args(P, A, 0, 0);
-  int G = std::min(P.ntiles, MAX_BLOCKS);
-  hipLaunchKernelGGL(k_spmv, dim3(G), dim3(SB), 0, g_stream, A, d_x, d_y);
-  MF_HIP(hipGetLastError());
-  return 0;
-}
-
-// Restarted left-preconditioned GMRES(m) with classical Gram-Schmidt and
-// Givens rotations, following lsr_mkb_core_gmres (lss_mkb_core.c:2317-2668):
-// same restart count, tolerance floors, stopping tests on the preconditioned
-// residual relative to the initial one, and convergence-rate abandonment.
-// The basis is kept unnormalised with per-vector scale factors so that the
-// normalisation costs no extra pass over memory.
-int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, int ini_zero,
-                double* h_x, int* iters, double* res_rel_out, double* resid_out) {
-  const int m = krylbas;
-  if (m < 1 || m > 63) {
-    set_error("GMRES: number of Krylov vectors must be in [1,63]");
+int launch_assemble_rows(Problem& P, int precision) {
+  RowArgs A;
+  fill_int_args(P, A.I);
+  A.n = P.n;
+  A.inc_ptr = P.d_inc_ptr; A.inc_el = P.d_inc_el; A.inc_loc = P.d_inc_loc;
+  A.finc_ptr = P.d_finc_ptr; A.finc_f = P.d_finc_f; A.finc_loc = P.d_finc_loc;
+  A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side; A.bf_g = P.d_bf_g;
+  A.val = P.d_val; A.rhs = P.d_rhs;
+  const int nsl = (P.n + 63) / 64;
+  dim3 g((nsl + ROW_WAVES - 1) / ROW_WAVES);
+  const int mr = P.setup.max_row;
+  const int ts = P.term_set;
+  if (mr > 32) {
+    set_error("row-wise assembly supports rows of at most 32 entries; use the atomic path");
     return -1;
   }
-  if (ensure_workspace(P, m)) return -2;
-  if (precon && !P.d_dinv) {
-    if (launch_fill_jacobi(P)) return -2;
-  }
-  SolveArgs A;
-  fill_solve_args(P, A, precon, m);
-  const int G_op = std::min(P.ntiles, MAX_BLOCKS);
-  const int G_up = (int)std::min<long long>(std::max<long long>(1, (P.n + 1023) / 1024), MAX_BLOCKS);
-  hipEvent_t e0, e1;
-  MF_HIP(hipEventCreate(&e0));
-  MF_HIP(hipEventCreate(&e1));
-  MF_HIP(hipEventRecord(e0, g_stream));
-  if (ini_zero || !h_x) {
-    MF_HIP(hipMemsetAsync(P.d_x, 0, (size_t)P.n * sizeof(double), g_stream));
-  } else {
-    MF_HIP(hipMemcpyAsync(P.d_x, h_x, (size_t)P.n * sizeof(double), hipMemcpyHostToDevice, g_stream));
-  }
-  const double SMALL = 1.0e-15, MAX_CONVRATE = 0.999;
-  double tol_rel = tol < 1e-12 ? 1e-12 : tol;
-  GmresStat st{};
-  st.tol_rel = tol_rel;
-  st.tol_res = 1.0e-12;
-  st.max_iter = max_iter;
-  int restrt = max_iter / m + 1, restit = 0, gmiter = 0, conv = 0;
-  double beta0 = 0, beta1 = 0, convrate = 0, faux = 0, resid = 0, res_rel = 0;
-  std::vector<double> H((size_t)(m + 1) * (m + 1) + 4 * (m + 1));
-  do {
-    st.gmiter = gmiter;
-    st.stop = 0;
-    st.j_done = 0;
-    MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, g_stream));
-    hipLaunchKernelGGL(k_residual, dim3(G_op), dim3(SB), 0, g_stream, A);
-    MF_HIP(hipGetLastError());
-    MF_HIP(hipStreamSynchronize(g_stream));
-    double rnrm2 = P.h_stat->rnrm;
-    if (restit == 0) {
-      convrate = 0.0; faux = 0.0; beta0 = rnrm2; beta1 = beta0;
-    } else if (rnrm2 < SMALL * SMALL || gmiter == 0 || beta0 < SMALL * SMALL ||
-               beta1 < SMALL * SMALL) {
-      convrate = 0.0; faux = 0.0; beta1 = 0.0;
+  if (P.el_type == kTetra) {
+    if (precision == 32) {
+      if (mr <= 16) launch_rows_ts<4, TetQ, 16, float>(ts, g, A);
+      else launch_rows_ts<4, TetQ, 32, float>(ts, g, A);
     } else {
-      convrate = exp(log(rnrm2 / beta0) / gmiter);
-      faux = exp(log(rnrm2 / beta1) / m);
-      beta1 = rnrm2;
+      if (mr <= 16) launch_rows_ts<4, TetQ, 16, double>(ts, g, A);
+      else launch_rows_ts<4, TetQ, 32, double>(ts, g, A);
     }
-    resid = rnrm2;
-    res_rel = beta0 != 0.0 ? rnrm2 / beta0 : 0.0;
-    if (!(rnrm2 == rnrm2)) {
-      set_error("GMRES: residual is not a number (singular matrix or zero diagonal?)");
-      hipEventDestroy(e0); hipEventDestroy(e1);
-      return -4;
+  } else {
+    if (precision == 32) {
+      if (mr <= 24) launch_rows_ts<6, PrismQ, 24, float>(ts, g, A);
+      else launch_rows_ts<6, PrismQ, 32, float>(ts, g, A);
+    } else {
+      if (mr <= 24) launch_rows_ts<6, PrismQ, 24, double>(ts, g, A);
+      else launch_rows_ts<6, PrismQ, 32, double>(ts, g, A);
     }
-    if (rnrm2 < st.tol_res || res_rel < tol_rel || rnrm2 < SMALL * SMALL) {
-      conv = 1;
-      break;
-    }
-    if (convrate > MAX_CONVRATE || faux > MAX_CONVRATE) break;
-    // first column: g = (rnrm, 0, ...), scale s_0 = 1/rnrm
-    std::fill(H.begin(), H.end(), 0.0);
-    H[0] = rnrm2;
-    double* scale = H.data() + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
-    scale[0] = 1.0 / rnrm2;
-    st.beta0 = beta0;
-    st.rnrm = rnrm2;
-    MF_HIP(hipMemcpyAsync(P.d_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, g_stream));
-    MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, g_stream));
-    for (int j = 0; j < m; j++) launch_arnoldi(dim3(G_op), dim3(G_up), A, j);
+  }
+  MF_HIP(hipGetLastError());
+  if (P.nbrows > 0) {
+    dim3 gf((P.nbrows + 255) / 256);
+    if (P.el_type == kTetra)
+      hipLaunchKernelGGL(k_assemble_face_rows<4>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
+    else
+      hipLaunchKernelGGL(k_assemble_face_rows<6>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
     MF_HIP(hipGetLastError());
-    MF_HIP(hipMemcpyAsync(H.data(), P.d_H, H.size() * sizeof(double), hipMemcpyDeviceToHost, g_stream));
-    MF_HIP(hipStreamSynchronize(g_stream));
-    GmresStat hs = *P.h_stat;
-    if (g_ks.on) {
-      // launches after the stop flag exit at once: count only executed steps
-      const double n = P.n, nnz = (double)P.nnz;
-      size_t keep = 0;
-      for (size_t i = 0; i < g_ks.kind.size(); i++) {
-        double bj = g_ks.bytes[i];
-        int jj = g_ks.kind[i] == 0 ? (int)(-bj - 1.0) : -1;
-        if (g_ks.kind[i] == 0) g_ks.bytes[i] = 12.0 * nnz + 4.0 * (n + 1) + 24.0 * n + 8.0 * n * jj;
-        if ((i / 2) < (size_t)hs.j_done) keep = i + 1;
-      }
-      g_ks.kind.resize(keep);
-      g_ks.bytes.resize(keep);
-      ks_collect();
-    }
-    int ik = hs.j_done;
-    gmiter = hs.gmiter;
-    resid = hs.resid;
-    res_rel = hs.res_rel;
-    // y = R^-1 g (dtrsv "U","N","N"), then x += V diag(s) y (dgemv)
-    std::vector<double> y(ik);
-    for (int i = ik - 1; i >= 0; i--) {
-      double s = H[i];
-      for (int c = i + 1; c < ik; c++) s -= H[(size_t)(c + 1) * (m + 1) + i] * y[c];
-      y[i] = s / H[(size_t)(i + 1) * (m + 1) + i];
-    }
-    double* sc = H.data() + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
-    std::vector<double> coef(ik);
-    for (int c = 0; c < ik; c++) coef[c] = y[c] * sc[c];
-    double* d_coef = P.d_H + (size_t)(m + 1) * (m + 1) + 3 * (m + 1);
-    MF_HIP(hipMemcpyAsync(d_coef, coef.data(), ik * sizeof(double), hipMemcpyHostToDevice, g_stream));
-    int gx = (int)std::min<long long>((P.n + SB - 1) / SB, 4096);
-    hipLaunchKernelGGL(k_xupdate, dim3(gx), dim3(SB), 0, g_stream, A, ik);
-    MF_HIP(hipGetLastError());
-    MF_HIP(hipStreamSynchronize(g_stream));
-    if (hs.converged) {
-      conv = 1;
-      break;
-    }
-    if (gmiter == max_iter) break;
-    restit++;
-  } while (restit <= restrt);
-  MF_HIP(hipEventRecord(e1, g_stream));
-  MF_HIP(hipEventSynchronize(e1));
-  float ms = 0;
-  hipEventElapsedTime(&ms, e0, e1);
-  P.t_solve_ms = ms;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (h_x) MF_HIP(hipMemcpy(h_x, P.d_x, (size_t)P.n * sizeof(double), hipMemcpyDeviceToHost));
-  if (iters) *iters = gmiter;
-  if (res_rel_out) *res_rel_out = res_rel;
-  if (resid_out) *resid_out = resid;
-  return conv;
+  }
+  return 0;
 }
 
 }  // namespace mf

@@ -69,7 +69,23 @@ struct Setup {
   // column dof(u) inside row dof(w): off[(w*6+u)*nel + e]
   std::vector<uint8_t> el_off;     // 36*nel
   std::vector<uint8_t> bf_off;     // 36*nbf (faces reuse owner's DOFs)
+  // device-native SELL-64 layout of the same matrix: slice s holds rows
+  // 64s..64s+63 column-major, padded to its longest row (padding: column =
+  // own row, value 0).  Entry k of row r lives at
+  //   sl_ptr[r/64] + 64*k + r%64.
+  std::vector<long long> sl_ptr;   // ns+1
+  std::vector<int> sl_col;         // sl_ptr[ns]
+  std::vector<uint8_t> diag_k;     // n: position of the diagonal in its row
+  // DOF -> integration-entity incidence in ascending entity order (the
+  // reference's summation order): elements, and boundary faces through their
+  // owning element; loc = local index of the DOF in that element
+  std::vector<int> inc_ptr, inc_el;
+  std::vector<uint8_t> inc_loc;
+  std::vector<int> finc_ptr, finc_f;
+  std::vector<uint8_t> finc_loc;
 };
+
+int build_sell(Setup& s);
 
 int build_setup(const Mesh& m, Setup& s);
 // full reference-format table for int-ent ie (elements then faces)

@@ -145,6 +145,69 @@ int build_setup(const Mesh& m, Setup& s) {
     set_error("assembly table lookup failed");
     return -1;
   }
+  // keep the element incidence (ascending element order by construction)
+  s.inc_ptr = inc_ptr;
+  s.inc_el = inc;
+  s.inc_loc.assign(inc.size(), 0);
+#pragma omp parallel for schedule(static)
+  for (int r = 0; r < n; r++)
+    for (int p = inc_ptr[r]; p < inc_ptr[r + 1]; p++) {
+      int e = inc[p];
+      for (int i = 0; i < nodes_of(m.type[e]); i++)
+        if (s.el_dof[(size_t)i * m.nel + e] == r) s.inc_loc[p] = (uint8_t)i;
+    }
+  // boundary-face incidence through the owning element's DOFs
+  s.finc_ptr.assign(n + 1, 0);
+  for (int f = 0; f < m.nbf; f++) {
+    int e = m.bf_elem[f];
+    for (int i = 0; i < nodes_of(m.type[e]); i++) s.finc_ptr[s.el_dof[(size_t)i * m.nel + e] + 1]++;
+  }
+  for (int r = 0; r < n; r++) s.finc_ptr[r + 1] += s.finc_ptr[r];
+  s.finc_f.assign(std::max(s.finc_ptr[n], 1), 0);
+  s.finc_loc.assign(std::max(s.finc_ptr[n], 1), 0);
+  {
+    std::vector<int> fill(s.finc_ptr.begin(), s.finc_ptr.end() - 1);
+    for (int f = 0; f < m.nbf; f++) {
+      int e = m.bf_elem[f];
+      for (int i = 0; i < nodes_of(m.type[e]); i++) {
+        int r = s.el_dof[(size_t)i * m.nel + e];
+        s.finc_f[fill[r]] = f;
+        s.finc_loc[fill[r]] = (uint8_t)i;
+        fill[r]++;
+      }
+    }
+  }
+  return build_sell(s);
+}
+
+int build_sell(Setup& s) {
+  const int n = s.n;
+  const int ns = (n + 63) / 64;
+  s.sl_ptr.assign(ns + 1, 0);
+  for (int sl = 0; sl < ns; sl++) {
+    int len = 0;
+    for (int r = sl * 64; r < std::min(n, sl * 64 + 64); r++)
+      len = std::max(len, s.row_ptr[r + 1] - s.row_ptr[r]);
+    s.sl_ptr[sl + 1] = s.sl_ptr[sl] + 64LL * len;
+  }
+  s.sl_col.assign(s.sl_ptr[ns], 0);
+  s.diag_k.assign(n, 0);
+#pragma omp parallel for schedule(static)
+  for (int sl = 0; sl < ns; sl++) {
+    const long long base = s.sl_ptr[sl];
+    const int len = (int)((s.sl_ptr[sl + 1] - base) / 64);
+    for (int lane = 0; lane < 64; lane++) {
+      const int r = sl * 64 + lane;
+      for (int k = 0; k < len; k++) {
+        int c = r < n ? r : n - 1;  // padding points at the own row
+        if (r < n && k < s.row_ptr[r + 1] - s.row_ptr[r]) c = s.col_ind[s.row_ptr[r] + k];
+        s.sl_col[base + 64LL * k + lane] = c;
+      }
+      if (r < n)
+        for (int k = s.row_ptr[r]; k < s.row_ptr[r + 1]; k++)
+          if (s.col_ind[k] == r) s.diag_k[r] = (uint8_t)(k - s.row_ptr[r]);
+    }
+  }
   return 0;
 }
 

@@ -1,0 +1,859 @@
+// Krylov solver kernels for gfx950: CRS-stream operator, fused Arnoldi
+// steps of restarted GMRES, and the reference-order row operations of the
+// storage seam.
+//
+// Reference behaviour followed (paths relative to modfem2017/src/):
+//   GMRES(m), CGS, Givens   lsd_mkb/lsd_mkb_core/lss_mkb_core.c:2317-2668
+//   preconditioned operator lss_mkb_core.c:1861-2097 (compreres)
+//   residual / BJ sweep     lsd_mkb/lad_crs/las_crs_intf.c:788-1013
+//   Jacobi fill             las_crs_intf.c:707-737
+//
+// Layout: CRS values (FP64), columns and row pointers (int32); the Krylov
+// basis V is (m+1) vectors with a padded stride ldv (multiple of 64 doubles)
+// so that every vector starts 512-byte aligned.
+//
+// One Arnoldi step j is four launches on the library stream:
+//   k_op_dots    w = s_j D^-1 A V_j written to V_{j+1}; per-block partial
+//                dot products V_i . w for i <= j.  Rows are processed in
+//                tiles: a tile's CRS entries are streamed coalesced into LDS
+//                products, then one lane owns one row for the row sum, the
+//                scaling and the dot products.
+//   k_fin_dots   one block: h_i = s_i sum(partials), c_i = h_i s_i
+//   k_update     w -= sum_i c_i V_i, per-block partial ||w||^2
+//   k_fin_update one block: h_{j+1,j} = ||w||, Givens rotations, stopping test
+// The basis vectors stay unnormalised (scale factors s_i = 1/||V_i||), so no
+// separate normalisation pass is needed.  Every kernel of a step returns at
+// once when the stopping flag is set, so a whole restart cycle is enqueued
+// without host synchronisation.  Blocks are remapped so that each XCD
+// streams one contiguous eighth of the rows.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "mf_device.h"
+
+namespace mf {
+
+constexpr int SB = 256;         // threads per solver block
+constexpr int TILE_NNZ = 4096;  // CRS entries staged per tile (32 KB LDS)
+constexpr int MAX_BLOCKS = 4096;
+constexpr int MAX_NV = 64;
+
+static int g_ncu = 0;
+
+static int num_cus() {
+  if (!g_ncu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return 256;
+    g_ncu = p.multiProcessorCount;
+  }
+  return g_ncu;
+}
+
+// ---------------------------------------------------------------------
+// optional per-launch timing of the Arnoldi kernels (HIP events on the
+// library stream) with the algorithmic bytes of every launch
+// ---------------------------------------------------------------------
+struct KStats {
+  bool on = false;
+  std::vector<hipEvent_t> ev;  // (start, stop) per recorded launch
+  std::vector<int> kind;       // 0: op_dots, 1: update
+  std::vector<double> bytes;
+  std::vector<int> step;       // Arnoldi step index of the launch
+  size_t used = 0;
+  double t[2] = {0, 0}, b[2] = {0, 0};
+  long long nl[2] = {0, 0};
+};
+static KStats g_ks;
+
+void kstats_enable(int on) {
+  g_ks.on = on != 0;
+  g_ks.t[0] = g_ks.t[1] = g_ks.b[0] = g_ks.b[1] = 0;
+  g_ks.nl[0] = g_ks.nl[1] = 0;
+}
+
+void kstats_get(double* out) {
+  out[0] = (double)g_ks.nl[0]; out[1] = g_ks.t[0]; out[2] = g_ks.b[0];
+  out[3] = (double)g_ks.nl[1]; out[4] = g_ks.t[1]; out[5] = g_ks.b[1];
+}
+
+static hipEvent_t ks_event() {
+  if (g_ks.used == g_ks.ev.size()) {
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    g_ks.ev.push_back(e);
+  }
+  return g_ks.ev[g_ks.used++];
+}
+
+// accumulate the launches of Arnoldi steps < steps_done (after a sync)
+static void ks_collect(int steps_done) {
+  for (size_t q = 0; q < g_ks.kind.size() && 2 * q + 1 < g_ks.used; q++) {
+    if (g_ks.step[q] >= steps_done) continue;  // returned at once (stopped)
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, g_ks.ev[2 * q], g_ks.ev[2 * q + 1]);
+    int k = g_ks.kind[q];
+    g_ks.t[k] += ms;
+    g_ks.b[k] += g_ks.bytes[q];
+    g_ks.nl[k]++;
+  }
+  g_ks.used = 0;
+  g_ks.kind.clear();
+  g_ks.bytes.clear();
+  g_ks.step.clear();
+}
+
+// ---------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------
+// blocks are dispatched round-robin over the 8 XCDs; give XCD x the x-th
+// contiguous range of logical blocks so its L2 sees neighbouring rows
+__device__ __forceinline__ int xcd_block(int bid, int nb) {
+  int xcd = bid & 7, k = bid >> 3;
+  int per = nb >> 3, rem = nb & 7;
+  return xcd * per + min(xcd, rem) + k;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// sum NV per-thread values over the block; results in red[0..nact-1]
+template <int NV>
+__device__ __forceinline__ void block_sum_multi(const double* acc, int nact, double* red) {
+  __shared__ double wsum[SB / 64][NV];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    if (i < nact) {
+      double v = wave_sum(acc[i]);
+      if (lane == 0) wsum[wid][i] = v;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nact) {
+    double s = 0;
+#pragma unroll
+    for (int w = 0; w < SB / 64; w++) s += wsum[w][threadIdx.x];
+    red[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+struct SolveArgs {
+  int n, ntiles, m;
+  long long ldv;        // stride of the Krylov vectors
+  const int* tile_row;  // CRS-tile path (storage seam)
+  const int* row_ptr;
+  const int* col;
+  const double* val;    // CRS values (tile path) or SELL-64 values
+  const long long* sl_ptr;  // SELL-64 path
+  const int* scol;
+  const double* dinv;   // null: no preconditioner
+  const double* b;
+  double* x;
+  double* V;            // (m+1) x ldv
+  double* H;            // (m+1)^2 | givens 2(m+1) | scale (m+1) | coef (m+1)
+  double* partials;
+  GmresStat* stat;
+  GmresStat* hstat;     // pinned host mirror
+};
+
+__device__ __forceinline__ double* Hcol(const SolveArgs& A, int c) { return A.H + (size_t)(A.m + 1) * c; }
+__device__ __forceinline__ double* Giv(const SolveArgs& A) { return A.H + (size_t)(A.m + 1) * (A.m + 1); }
+__device__ __forceinline__ double* Scale(const SolveArgs& A) { return Giv(A) + 2 * (A.m + 1); }
+__device__ __forceinline__ double* Coef(const SolveArgs& A) { return Scale(A) + (A.m + 1); }
+
+// tile range of a logical block
+__device__ __forceinline__ void tile_range(const SolveArgs& A, int lb, int& t0, int& t1) {
+  t0 = (int)((long long)lb * A.ntiles / gridDim.x);
+  t1 = (int)((long long)(lb + 1) * A.ntiles / gridDim.x);
+}
+
+// stream one tile's CRS entries into LDS products and return the row sum of
+// the calling lane's row (valid when r0 + threadIdx.x < r1)
+__device__ __forceinline__ double tile_rows(const SolveArgs& A, const double* x, int r0, int r1,
+                                            double* prod) {
+  const int k0 = A.row_ptr[r0], k1 = A.row_ptr[r1];
+#pragma unroll 4
+  for (int k = k0 + threadIdx.x; k < k1; k += SB) prod[k - k0] = A.val[k] * x[A.col[k]];
+  __syncthreads();
+  double s = 0.0;
+  const int r = r0 + threadIdx.x;
+  if (r < r1) {
+    const int a = A.row_ptr[r] - k0, z = A.row_ptr[r + 1] - k0;
+    for (int k = a; k < z; k++) s += prod[k];
+  }
+  __syncthreads();
+  return s;
+}
+
+// SELL-64: wavefront = slice of 64 rows, lane = row; entries column-major
+// inside the slice, so every load of values and columns is one contiguous
+// 512/256-byte access.  Logical block lb owns a contiguous range of slices
+// (one contiguous eighth of the matrix per XCD); its 4 wavefronts take the
+// range's slices round-robin.
+constexpr int SW = SB / 64;  // wavefronts per block
+
+__device__ __forceinline__ void slice_range(int nsl, int lb, int& s0, int& s1) {
+  s0 = (int)((long long)lb * nsl / gridDim.x);
+  s1 = (int)((long long)(lb + 1) * nsl / gridDim.x);
+}
+
+__device__ __forceinline__ double sell_row(const SolveArgs& A, const double* x, int sl, int lane) {
+  const long long base = A.sl_ptr[sl] + lane;
+  const int len = (int)((A.sl_ptr[sl + 1] - A.sl_ptr[sl]) >> 6);
+  double s = 0.0;
+  int k = 0;
+  for (; k + 4 <= len; k += 4) {
+    const long long q = base + 64LL * k;
+    const double v0 = A.val[q], v1 = A.val[q + 64], v2 = A.val[q + 128], v3 = A.val[q + 192];
+    const int c0 = A.scol[q], c1 = A.scol[q + 64], c2 = A.scol[q + 128], c3 = A.scol[q + 192];
+    const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    s += v0 * x0;
+    s += v1 * x1;
+    s += v2 * x2;
+    s += v3 * x3;
+  }
+  for (; k < len; k++) {
+    const long long q = base + 64LL * k;
+    s += A.val[q] * x[A.scol[q]];
+  }
+  return s;
+}
+
+__global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
+  __shared__ double red[1];
+  const int nsl = (A.n + 63) >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int s0, s1;
+  slice_range(nsl, lb, s0, s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int sl = s0 + wv; sl < s1; sl += SW) {
+    const double s = sell_row(A, A.x, sl, lane);
+    const int r = sl * 64 + lane;
+    if (r < A.n) {
+      double v = A.b[r] - s;
+      if (A.dinv) v *= A.dinv[r];
+      A.V[r] = v;
+      acc += v * v;
+    }
+  }
+  block_sum_multi<1>(&acc, 1, red);
+  if (threadIdx.x == 0) A.partials[lb] = red[0];
+}
+
+template <int NV>
+__global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  __shared__ double red[NV];
+  const int nsl = (A.n + 63) >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int s0, s1;
+  slice_range(nsl, lb, s0, s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double* x = A.V + (size_t)j * A.ldv;
+  double* w = A.V + (size_t)(j + 1) * A.ldv;
+  const double sj = Scale(A)[j];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) acc[i] = 0.0;
+  for (int sl = s0 + wv; sl < s1; sl += SW) {
+    const double s = sell_row(A, x, sl, lane);
+    const int r = sl * 64 + lane;
+    if (r < A.n) {
+      double wr = sj * s;
+      if (A.dinv) wr *= A.dinv[r];
+      w[r] = wr;
+#pragma unroll
+      for (int i = 0; i < NV; i++)
+        if (i <= j) acc[i] += A.V[(size_t)i * A.ldv + r] * wr;
+    }
+  }
+  block_sum_multi<NV>(acc, j + 1, red);
+  if ((int)threadIdx.x <= j) A.partials[(size_t)lb * NV + threadIdx.x] = red[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(SB) k_spmv_sell(SolveArgs A, const double* x, double* y) {
+  const int nsl = (A.n + 63) >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int s0, s1;
+  slice_range(nsl, lb, s0, s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int sl = s0 + wv; sl < s1; sl += SW) {
+    const double s = sell_row(A, x, sl, lane);
+    const int r = sl * 64 + lane;
+    if (r < A.n) y[r] = s;
+  }
+}
+
+__global__ void k_fill_jacobi_sell(int n, const long long* sl_ptr, const uint8_t* diag_k,
+                                   const double* val, double* dinv) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  dinv[r] = 1.0 / val[sl_ptr[r >> 6] + 64LL * diag_k[r] + (r & 63)];
+}
+
+// ---------------------------------------------------------------------
+// GMRES kernels
+// ---------------------------------------------------------------------
+// V_0 = D^-1 (b - A x); per-block partial ||V_0||^2
+__global__ void __launch_bounds__(SB) k_residual(SolveArgs A) {
+  __shared__ double prod[TILE_NNZ];
+  __shared__ double red[1];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int t0, t1;
+  tile_range(A, lb, t0, t1);
+  double acc = 0.0;
+  for (int t = t0; t < t1; t++) {
+    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
+    double s = tile_rows(A, A.x, r0, r1, prod);
+    const int r = r0 + threadIdx.x;
+    if (r < r1) {
+      double v = A.b[r] - s;
+      if (A.dinv) v *= A.dinv[r];
+      A.V[r] = v;
+      acc += v * v;
+    }
+  }
+  block_sum_multi<1>(&acc, 1, red);
+  if (threadIdx.x == 0) A.partials[lb] = red[0];
+}
+
+__global__ void __launch_bounds__(SB) k_fin_residual(SolveArgs A, int G) {
+  __shared__ double red[1];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += SB) s += A.partials[b];
+  block_sum_multi<1>(&s, 1, red);
+  if (threadIdx.x == 0) {
+    A.stat->rnrm = sqrt(red[0]);
+    *A.hstat = *A.stat;
+  }
+}
+
+// Arnoldi step j, part 1
+template <int NV>
+__global__ void __launch_bounds__(SB) k_op_dots(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  __shared__ double prod[TILE_NNZ];
+  __shared__ double red[NV];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int t0, t1;
+  tile_range(A, lb, t0, t1);
+  const double* x = A.V + (size_t)j * A.ldv;
+  double* w = A.V + (size_t)(j + 1) * A.ldv;
+  const double sj = Scale(A)[j];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) acc[i] = 0.0;
+  for (int t = t0; t < t1; t++) {
+    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
+    double s = tile_rows(A, x, r0, r1, prod);
+    const int r = r0 + threadIdx.x;
+    if (r < r1) {
+      double wr = sj * s;
+      if (A.dinv) wr *= A.dinv[r];
+      w[r] = wr;
+#pragma unroll
+      for (int i = 0; i < NV; i++)
+        if (i <= j) acc[i] += A.V[(size_t)i * A.ldv + r] * wr;
+    }
+  }
+  block_sum_multi<NV>(acc, j + 1, red);
+  if ((int)threadIdx.x <= j) A.partials[(size_t)lb * NV + threadIdx.x] = red[threadIdx.x];
+}
+
+template <int NV>
+__global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
+  if (A.stat->stop) return;
+  __shared__ double red[NV];
+  double acc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) acc[i] = 0.0;
+  for (int b = threadIdx.x; b < G; b += SB)
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+      if (i <= j) acc[i] += A.partials[(size_t)b * NV + i];
+  block_sum_multi<NV>(acc, j + 1, red);
+  if ((int)threadIdx.x <= j) {
+    const double si = Scale(A)[threadIdx.x];
+    double h = si * red[threadIdx.x];
+    Hcol(A, j + 1)[threadIdx.x] = h;
+    Coef(A)[threadIdx.x] = h * si;
+  }
+}
+
+// Arnoldi step j, part 2: w -= sum_i c_i V_i over row pairs (16-byte loads)
+template <int NV>
+__global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  __shared__ double red[1];
+  __shared__ double cs[NV];
+  if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
+  __syncthreads();
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long np = (A.n + 1) / 2;  // row pairs (ldv is even)
+  const long long p0 = lb * np / gridDim.x, p1 = (lb + 1) * np / gridDim.x;
+  double2* w = reinterpret_cast<double2*>(A.V + (size_t)(j + 1) * A.ldv);
+  double acc = 0.0;
+  for (long long p = p0 + threadIdx.x; p < p1; p += SB) {
+    double2 wr = w[p];
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+      if (i <= j) {
+        const double2 v = reinterpret_cast<const double2*>(A.V + (size_t)i * A.ldv)[p];
+        wr.x -= cs[i] * v.x;
+        wr.y -= cs[i] * v.y;
+      }
+    if (2 * p + 1 >= A.n) wr.y = 0.0;  // padding lane of an odd n
+    w[p] = wr;
+    acc += wr.x * wr.x + wr.y * wr.y;
+  }
+  block_sum_multi<1>(&acc, 1, red);
+  if (threadIdx.x == 0) A.partials[lb] = red[0];
+}
+
+__device__ void drotg_dev(double* a, double* b, double* c, double* s) {
+  double roe = *b;
+  if (fabs(*a) > fabs(*b)) roe = *a;
+  double scale = fabs(*a) + fabs(*b);
+  double r, z;
+  if (scale == 0.0) {
+    *c = 1.0; *s = 0.0; r = 0.0; z = 0.0;
+  } else {
+    double ra = *a / scale, rb = *b / scale;
+    r = scale * sqrt(ra * ra + rb * rb);
+    r = (roe >= 0 ? 1.0 : -1.0) * r;
+    *c = *a / r;
+    *s = *b / r;
+    z = 1.0;
+    if (fabs(*a) > fabs(*b)) z = *s;
+    if (fabs(*b) >= fabs(*a) && *c != 0.0) z = 1.0 / *c;
+  }
+  *a = r;
+  *b = z;
+}
+
+__device__ __forceinline__ void rot(double* x, double* y, double c, double s) {
+  double t = c * (*x) + s * (*y);
+  *y = c * (*y) - s * (*x);
+  *x = t;
+}
+
+// h_{j+1,j} = ||w||, Givens rotations (drot/drotg as lss_mkb_core.c:2539-2560),
+// stopping tests (:2562-2590)
+__global__ void __launch_bounds__(SB) k_fin_update(SolveArgs A, int j, int G) {
+  if (A.stat->stop) return;
+  __shared__ double red[1];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += SB) s += A.partials[b];
+  block_sum_multi<1>(&s, 1, red);
+  if (threadIdx.x != 0) return;
+  const int m = A.m;
+  double hn = sqrt(red[0]);
+  double* col = Hcol(A, j + 1);
+  col[j + 1] = hn;
+  Scale(A)[j + 1] = 1.0 / hn;
+  double* giv = Giv(A);
+  for (int i = 1; i <= j; i++) rot(&col[i - 1], &col[i], giv[2 * (i - 1)], giv[2 * (i - 1) + 1]);
+  double ra = col[j], rb = col[j + 1];
+  drotg_dev(&ra, &rb, &giv[2 * j], &giv[2 * j + 1]);
+  rot(&col[j], &col[j + 1], giv[2 * j], giv[2 * j + 1]);
+  double* g = Hcol(A, 0);
+  rot(&g[j], &g[j + 1], giv[2 * j], giv[2 * j + 1]);
+  GmresStat* st = A.stat;
+  st->gmiter += 1;
+  st->j_done = j + 1;
+  st->resid = fabs(g[j + 1]);
+  st->res_rel = st->beta0 != 0.0 ? st->resid / st->beta0 : 0.0;
+  if (st->resid < st->tol_res || st->res_rel < st->tol_rel) {
+    st->converged = 1;
+    st->stop = 1;
+  }
+  if (j + 1 == m || st->gmiter == st->max_iter) st->stop = 1;
+  *A.hstat = *st;
+}
+
+// x += sum_c (y_c s_c) V_c
+__global__ void __launch_bounds__(SB) k_xupdate(SolveArgs A, int ik) {
+  __shared__ double cs[MAX_NV];
+  if ((int)threadIdx.x < ik) cs[threadIdx.x] = Coef(A)[threadIdx.x];
+  __syncthreads();
+  const size_t n = A.n;
+  for (size_t r = (size_t)blockIdx.x * SB + threadIdx.x; r < n; r += (size_t)gridDim.x * SB) {
+    double xr = A.x[r];
+    for (int c = 0; c < ik; c++) xr += A.V[(size_t)c * A.ldv + r] * cs[c];
+    A.x[r] = xr;
+  }
+}
+
+__global__ void k_fill_jacobi(int n, const int* rp, const int* col, const double* val,
+                              double* dinv) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double d = 0.0;
+  for (int k = rp[r]; k < rp[r + 1]; k++)
+    if (col[k] == r) d = val[k];
+  dinv[r] = 1.0 / d;
+}
+
+// plain y = A x through the same tiles
+__global__ void __launch_bounds__(SB) k_spmv(SolveArgs A, const double* x, double* y) {
+  __shared__ double prod[TILE_NNZ];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int t0, t1;
+  tile_range(A, lb, t0, t1);
+  for (int t = t0; t < t1; t++) {
+    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
+    double s = tile_rows(A, x, r0, r1, prod);
+    const int r = r0 + threadIdx.x;
+    if (r < r1) y[r] = s;
+  }
+}
+
+// Reference-order row operations for the storage seam (lar_*):
+//   mode 0: v = (use_rhs ? b : 0) - A x        lar_compute_residual_crs
+//                                              (las_crs_intf.c:788-849)
+//   mode 1: v = d .* ((use_rhs ? b : 0) - sum_{j!=i} a_ij x_j)
+//                                              lar_perform_BJ_or_GS_iterations_crs,
+//                                              BLOCK_JACOBI (las_crs_intf.c:857-1013)
+// Row sums run in column order over LDS-staged products without FMA
+// contraction, matching the reference's sequential "v -= a*x" exactly.
+__global__ void __launch_bounds__(SB) k_rowop(SolveArgs A, int mode, int use_rhs,
+                                              const double* x, const double* b,
+                                              const int* diag_ptr, double* v) {
+  __shared__ double prod[TILE_NNZ];
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int t0, t1;
+  tile_range(A, lb, t0, t1);
+  for (int t = t0; t < t1; t++) {
+    const int r0 = A.tile_row[t], r1 = A.tile_row[t + 1];
+    const int k0 = A.row_ptr[r0], k1 = A.row_ptr[r1];
+    if (x)
+      for (int k = k0 + threadIdx.x; k < k1; k += SB) prod[k - k0] = __dmul_rn(A.val[k], x[A.col[k]]);
+    __syncthreads();
+    const int r = r0 + threadIdx.x;
+    if (r < r1) {
+      double s = 0.0;
+      if (x) {
+        const int a = A.row_ptr[r] - k0, z = A.row_ptr[r + 1] - k0;
+        const int dk = (mode == 1) ? diag_ptr[r] - k0 : -1;
+        for (int k = a; k < z; k++)
+          if (k != dk) s = __dsub_rn(s, prod[k]);
+      }
+      if (use_rhs) s = __dadd_rn(s, b[r]);
+      if (mode == 1) s = __dmul_rn(s, A.dinv[r]);
+      v[r] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_diag_ptr(int n, const int* rp, const int* col, int* dp) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int p = -1;
+  for (int k = rp[r]; k < rp[r + 1]; k++)
+    if (col[k] == r) p = k;
+  dp[r] = p;
+}
+
+// ---------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------
+static long long ldv_of(int n) { return ((long long)n + 63) / 64 * 64; }
+
+static void fill_solve_args(Problem& P, SolveArgs& A, int precon, int m) {
+  A.n = P.n; A.ntiles = P.ntiles; A.m = m; A.ldv = ldv_of(P.n);
+  A.tile_row = P.d_tile_row; A.row_ptr = P.d_row_ptr; A.col = P.d_col; A.val = P.d_val;
+  A.sl_ptr = P.d_sl_ptr; A.scol = P.d_scol;
+  A.dinv = precon ? P.d_dinv : nullptr;
+  A.b = P.d_rhs; A.x = P.d_x; A.V = P.d_V; A.H = P.d_H; A.partials = P.d_partials;
+  A.stat = P.d_stat; A.hstat = P.h_stat;
+}
+
+// grid sizes: one resident round over the CUs (LDS limits a tile kernel to 5
+// blocks per CU), never more blocks than work items
+static int grid_tiles(const Problem& P) {
+  return std::max(1, std::min(P.ntiles, std::min(MAX_BLOCKS, num_cus() * 5)));
+}
+static int grid_slices(const Problem& P) {
+  const int nsl = (P.n + 63) / 64;
+  return std::max(1, std::min((nsl + SW - 1) / SW, std::min(MAX_BLOCKS, num_cus() * 8)));
+}
+static int grid_rows(const Problem& P) {
+  long long pairs = (P.n + 1) / 2;
+  return (int)std::max(1LL, std::min<long long>((pairs + SB - 1) / SB, std::min(MAX_BLOCKS, num_cus() * 8)));
+}
+
+int build_tiles(Problem& P) {
+  const auto& rp = P.setup.row_ptr;
+  P.tile_row.clear();
+  P.tile_row.push_back(0);
+  int r = 0;
+  while (r < P.n) {
+    int start = r;
+    while (r < P.n && r - start < SB && rp[r + 1] - rp[start] <= TILE_NNZ) r++;
+    if (r == start) {
+      set_error("a CRS row exceeds the tile capacity");
+      return -1;
+    }
+    P.tile_row.push_back(r);
+  }
+  P.ntiles = (int)P.tile_row.size() - 1;
+  if (P.d_tile_row) (void)hipFree(P.d_tile_row);
+  MF_HIP(hipMalloc(&P.d_tile_row, P.tile_row.size() * sizeof(int)));
+  MF_HIP(hipMemcpy(P.d_tile_row, P.tile_row.data(), P.tile_row.size() * sizeof(int),
+                   hipMemcpyHostToDevice));
+  return 0;
+}
+
+int launch_fill_jacobi(Problem& P) {
+  if (!P.d_dinv) MF_HIP(hipMalloc(&P.d_dinv, (size_t)P.n * sizeof(double)));
+  if (P.sell)
+    hipLaunchKernelGGL(k_fill_jacobi_sell, dim3((P.n + 255) / 256), dim3(256), 0, stream(), P.n,
+                       P.d_sl_ptr, P.d_diag_k, P.d_val, P.d_dinv);
+  else
+    hipLaunchKernelGGL(k_fill_jacobi, dim3((P.n + 255) / 256), dim3(256), 0, stream(), P.n,
+                       P.d_row_ptr, P.d_col, P.d_val, P.d_dinv);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,
+                 const int* d_diag_ptr, double* d_v) {
+  SolveArgs A;
+  fill_solve_args(P, A, mode == 1, 0);
+  hipLaunchKernelGGL(k_rowop, dim3(grid_tiles(P)), dim3(SB), 0, stream(), A, mode, use_rhs,
+                     d_x, d_b, d_diag_ptr, d_v);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_diag_ptr(Problem& P, int* d_dp) {
+  hipLaunchKernelGGL(k_diag_ptr, dim3((P.n + 255) / 256), dim3(256), 0, stream(), P.n,
+                     P.d_row_ptr, P.d_col, d_dp);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_spmv(Problem& P, const double* d_x, double* d_y) {
+  SolveArgs A;
+  fill_solve_args(P, A, 0, 0);
+  if (P.sell)
+    hipLaunchKernelGGL(k_spmv_sell, dim3(grid_slices(P)), dim3(SB), 0, stream(), A, d_x, d_y);
+  else
+    hipLaunchKernelGGL(k_spmv, dim3(grid_tiles(P)), dim3(SB), 0, stream(), A, d_x, d_y);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+static int ensure_workspace(Problem& P, int m) {
+  const long long ldv = ldv_of(P.n);
+  if (P.kryl_alloc >= m && P.d_V) return 0;
+  if (P.d_V) (void)hipFree(P.d_V);
+  if (P.d_H) (void)hipFree(P.d_H);
+  MF_HIP(hipMalloc(&P.d_V, (size_t)(m + 1) * ldv * sizeof(double)));
+  MF_HIP(hipMemset(P.d_V, 0, (size_t)(m + 1) * ldv * sizeof(double)));
+  MF_HIP(hipMalloc(&P.d_H, (size_t)((m + 1) * (m + 1) + 4 * (m + 1)) * sizeof(double)));
+  if (!P.d_partials)
+    MF_HIP(hipMalloc(&P.d_partials, (size_t)MAX_BLOCKS * MAX_NV * sizeof(double)));
+  if (!P.d_stat) MF_HIP(hipMalloc(&P.d_stat, sizeof(GmresStat)));
+  if (!P.h_stat) MF_HIP(hipHostMalloc(&P.h_stat, sizeof(GmresStat), hipHostMallocDefault));
+  if (!P.d_x) MF_HIP(hipMalloc(&P.d_x, (size_t)P.n * sizeof(double)));
+  P.kryl_alloc = m;
+  return 0;
+}
+
+template <int NV>
+static void launch_step(int G_op, int G_up, SolveArgs& A, int j) {
+  hipStream_t s = stream();
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+  if (g_ks.on) {
+    e0 = ks_event(); e1 = ks_event();
+    (void)hipEventRecord(e0, s);
+  }
+  if (A.sl_ptr)
+    hipLaunchKernelGGL(k_op_dots_sell<NV>, dim3(G_op), dim3(SB), 0, s, A, j);
+  else
+    hipLaunchKernelGGL(k_op_dots<NV>, dim3(G_op), dim3(SB), 0, s, A, j);
+  if (g_ks.on) (void)hipEventRecord(e1, s);
+  hipLaunchKernelGGL(k_fin_dots<NV>, dim3(1), dim3(SB), 0, s, A, j, G_op);
+  if (g_ks.on) {
+    e2 = ks_event(); e3 = ks_event();
+    (void)hipEventRecord(e2, s);
+  }
+  hipLaunchKernelGGL(k_update<NV>, dim3(G_up), dim3(SB), 0, s, A, j);
+  if (g_ks.on) (void)hipEventRecord(e3, s);
+  hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(SB), 0, s, A, j, G_up);
+  if (g_ks.on) {
+    // algorithmic bytes: op = CRS values+columns, row pointers, input
+    // vector, Jacobi diagonal, output, and V_0..V_{j-1} for the dots;
+    // update = w read+write and V_0..V_j
+    const double n = A.n;
+    g_ks.kind.push_back(0);
+    g_ks.step.push_back(j);
+    g_ks.bytes.push_back(-1.0);  // set by the caller (needs nnz)
+    g_ks.kind.push_back(1);
+    g_ks.step.push_back(j);
+    g_ks.bytes.push_back(8.0 * n * (j + 3));
+  }
+}
+
+static void launch_arnoldi(int G_op, int G_up, SolveArgs& A, int j) {
+  int nv = j + 1;
+  if (nv <= 8) launch_step<8>(G_op, G_up, A, j);
+  else if (nv <= 16) launch_step<16>(G_op, G_up, A, j);
+  else if (nv <= 24) launch_step<24>(G_op, G_up, A, j);
+  else if (nv <= 32) launch_step<32>(G_op, G_up, A, j);
+  else if (nv <= 40) launch_step<40>(G_op, G_up, A, j);
+  else if (nv <= 48) launch_step<48>(G_op, G_up, A, j);
+  else launch_step<64>(G_op, G_up, A, j);
+}
+
+// Restarted left-preconditioned GMRES(m), classical Gram-Schmidt and Givens
+// rotations, following lsr_mkb_core_gmres (lss_mkb_core.c:2317-2668): same
+// restart count, tolerance floors, stopping tests on the preconditioned
+// residual relative to the initial one, and convergence-rate abandonment.
+int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, int ini_zero,
+                double* h_x, int* iters, double* res_rel_out, double* resid_out) {
+  const int m = krylbas;
+  if (m < 1 || m >= MAX_NV) {
+    set_error("GMRES: number of Krylov vectors must be in [1,63]");
+    return -1;
+  }
+  if (ensure_workspace(P, m)) return -2;
+  if (precon && !P.d_dinv) {
+    if (launch_fill_jacobi(P)) return -2;
+  }
+  SolveArgs A;
+  fill_solve_args(P, A, precon, m);
+  const int G_op = P.sell ? grid_slices(P) : grid_tiles(P), G_up = grid_rows(P);
+  if (!P.sell) A.sl_ptr = nullptr;
+  hipStream_t s = stream();
+  hipEvent_t e0, e1;
+  MF_HIP(hipEventCreate(&e0));
+  MF_HIP(hipEventCreate(&e1));
+  MF_HIP(hipEventRecord(e0, s));
+  if (ini_zero || !h_x) {
+    MF_HIP(hipMemsetAsync(P.d_x, 0, (size_t)P.n * sizeof(double), s));
+  } else {
+    MF_HIP(hipMemcpyAsync(P.d_x, h_x, (size_t)P.n * sizeof(double), hipMemcpyHostToDevice, s));
+  }
+  const double SMALL = 1.0e-15, MAX_CONVRATE = 0.999;
+  const double tol_rel = tol < 1e-12 ? 1e-12 : tol;
+  GmresStat st{};
+  st.tol_rel = tol_rel;
+  st.tol_res = 1.0e-12;
+  st.max_iter = max_iter;
+  int restrt = max_iter / m + 1, restit = 0, gmiter = 0, conv = 0;
+  double beta0 = 0, beta1 = 0, convrate = 0, faux = 0, resid = 0, res_rel = 0;
+  std::vector<double> H((size_t)(m + 1) * (m + 1) + 4 * (m + 1));
+  const double op_bytes_base =
+      12.0 * (double)P.nnz + 4.0 * (P.n + 1) + 24.0 * (double)P.n;
+  do {
+    st.gmiter = gmiter;
+    st.stop = 0;
+    st.j_done = 0;
+    MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
+    if (P.sell)
+      hipLaunchKernelGGL(k_residual_sell, dim3(G_op), dim3(SB), 0, s, A);
+    else
+      hipLaunchKernelGGL(k_residual, dim3(G_op), dim3(SB), 0, s, A);
+    hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(SB), 0, s, A, G_op);
+    MF_HIP(hipGetLastError());
+    MF_HIP(hipStreamSynchronize(s));
+    double rnrm2 = P.h_stat->rnrm;
+    if (restit == 0) {
+      convrate = 0.0; faux = 0.0; beta0 = rnrm2; beta1 = beta0;
+    } else if (rnrm2 < SMALL * SMALL || gmiter == 0 || beta0 < SMALL * SMALL ||
+               beta1 < SMALL * SMALL) {
+      convrate = 0.0; faux = 0.0; beta1 = 0.0;
+    } else {
+      convrate = exp(log(rnrm2 / beta0) / gmiter);
+      faux = exp(log(rnrm2 / beta1) / m);
+      beta1 = rnrm2;
+    }
+    resid = rnrm2;
+    res_rel = beta0 != 0.0 ? rnrm2 / beta0 : 0.0;
+    if (!(rnrm2 == rnrm2)) {
+      set_error("GMRES: residual is not a number (singular matrix or zero diagonal?)");
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return -4;
+    }
+    if (rnrm2 < st.tol_res || res_rel < tol_rel || rnrm2 < SMALL * SMALL) {
+      conv = 1;
+      break;
+    }
+    if (convrate > MAX_CONVRATE || faux > MAX_CONVRATE) break;
+    std::fill(H.begin(), H.end(), 0.0);
+    H[0] = rnrm2;
+    double* scale = H.data() + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
+    scale[0] = 1.0 / rnrm2;
+    st.beta0 = beta0;
+    st.rnrm = rnrm2;
+    MF_HIP(hipMemcpyAsync(P.d_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
+    for (int j = 0; j < m; j++) launch_arnoldi(G_op, G_up, A, j);
+    MF_HIP(hipGetLastError());
+    MF_HIP(hipMemcpyAsync(H.data(), P.d_H, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    MF_HIP(hipStreamSynchronize(s));
+    GmresStat hs = *P.h_stat;
+    if (g_ks.on) {
+      for (size_t q = 0; q < g_ks.kind.size(); q++)
+        if (g_ks.kind[q] == 0) g_ks.bytes[q] = op_bytes_base + 8.0 * P.n * g_ks.step[q];
+      ks_collect(hs.j_done);
+    }
+    const int ik = hs.j_done;
+    gmiter = hs.gmiter;
+    resid = hs.resid;
+    res_rel = hs.res_rel;
+    // y = R^-1 g (dtrsv "U","N","N"), then x += V diag(s) y (dgemv)
+    std::vector<double> y(ik);
+    for (int i = ik - 1; i >= 0; i--) {
+      double sum = H[i];
+      for (int c = i + 1; c < ik; c++) sum -= H[(size_t)(c + 1) * (m + 1) + i] * y[c];
+      y[i] = sum / H[(size_t)(i + 1) * (m + 1) + i];
+    }
+    const double* sc = H.data() + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
+    std::vector<double> coef(std::max(ik, 1));
+    for (int c = 0; c < ik; c++) coef[c] = y[c] * sc[c];
+    double* d_coef = P.d_H + (size_t)(m + 1) * (m + 1) + 3 * (m + 1);
+    if (ik > 0) {
+      MF_HIP(hipMemcpyAsync(d_coef, coef.data(), ik * sizeof(double), hipMemcpyHostToDevice, s));
+      int gx = (int)std::min<long long>((P.n + SB - 1) / SB, 4096);
+      hipLaunchKernelGGL(k_xupdate, dim3(gx), dim3(SB), 0, s, A, ik);
+      MF_HIP(hipGetLastError());
+    }
+    MF_HIP(hipStreamSynchronize(s));
+    if (hs.converged) {
+      conv = 1;
+      break;
+    }
+    if (gmiter == max_iter) break;
+    restit++;
+  } while (restit <= restrt);
+  MF_HIP(hipEventRecord(e1, s));
+  MF_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  P.t_solve_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (h_x) MF_HIP(hipMemcpy(h_x, P.d_x, (size_t)P.n * sizeof(double), hipMemcpyDeviceToHost));
+  if (iters) *iters = gmiter;
+  if (res_rel_out) *res_rel_out = res_rel;
+  if (resid_out) *resid_out = resid;
+  return conv;
+}
+
+}  // namespace mf

@@ -258,3 +258,42 @@ def test_large_mesh_invariants():
     A = sp.csr_matrix((val, ci, rp), shape=(p.n, p.n))
     assert abs(A - A.T).max() <= 1e-13 * np.abs(val).max()
     p.free()
+
+
+def test_gpu_reproduces_reference_field_dump():
+    """the reference's own known answer (ctest Laplace_std_in_box_opencl) on
+    the device path: jk mesh, DIFF_IN_BOX, penalty faces, GMRES"""
+    from test_golden import GOLD, load_dump
+    m = mf.Mesh.read_jk(os.path.join(GOLD, "laplace_box_mesh_jk.dat"))
+    p = mf.Problem.from_mesh(m)
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    p.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc, src_model=mf.SRC_DIFF_IN_BOX)
+    p.integrate_assemble(64)
+    x, info = p.gmres(precon=1, krylbas=20, max_iter=1000, tol=1e-12)
+    assert info["converged"]
+    u = np.zeros(m.nno)
+    u[p.dofs()] = x
+    ids, ref = load_dump(os.path.join(GOLD, "laplace_box_field.dmp"))
+    assert np.abs(u[ids - 1] - ref).max() <= 1e-9
+    p.free()
+
+
+@pytest.mark.parametrize("kind", ["prism", "tetra"])
+def test_assembly_modes_agree_and_rowwise_is_reproducible(kind, prism_mesh, tetra_mesh):
+    """row-wise (deterministic) and element-atomic assembly agree to 1e-12;
+    row-wise is bitwise identical across runs"""
+    mesh = prism_mesh if kind == "prism" else tetra_mesh
+    p = mf.Problem.from_mesh(mesh)
+    ind, model, cc, ce, tab = make_case(mesh, "all_terms_full")
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    p.integrate_assemble(64)
+    v0, r0 = p.system()
+    p.integrate_assemble(64)
+    v1, r1 = p.system()
+    assert np.array_equal(v0, v1) and np.array_equal(r0, r1)
+    p.set_assembly_mode(1)
+    p.integrate_assemble(64)
+    v2, r2 = p.system()
+    assert rel_err(v2, v0) <= 1e-12 and rel_err(r2, r0) <= 1e-12
+    p.free()

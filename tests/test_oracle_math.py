@@ -1,0 +1,194 @@
+"""Pin the oracle by what the mathematics fixes (for the parts the reference's
+compiled subset cannot run: the element integral needs the problem module,
+GMRES needs BLAS):
+  - P1 tetrahedron: stiffness V*G*G^T and mass V/20*(1+delta) in closed form;
+  - prisms: zero row sums, symmetry, patch test on affine meshes, total load
+    equals the volume;
+  - penalty faces: 1e7 x the closed-form triangle / rectangle face mass;
+  - GMRES: dense LU solution, Krylov invariance, non-convergence flag;
+  - drotg: rotation identities.
+Each check is chosen so that a dropped term, a sign, a wrong index or a
+transposed operand fails it.
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import modfem_amd as mf  # noqa: E402
+from oracle_lib import PRISM, TETRA, OracleProblem, P, oracle, oracle_gmres_crs  # noqa: E402
+
+
+def elem(el_type, nc, ind, coef, src=0):
+    o = oracle()
+    k = 4 if el_type == TETRA else 6
+    sm, rhs = np.zeros(36), np.zeros(6)
+    o.orc_num_int_el(el_type, P(np.ascontiguousarray(nc.reshape(-1))), P(np.asarray(ind, np.int32)),
+                     P(np.asarray(coef, float)), src, P(sm), P(rhs))
+    return sm[: k * k].reshape(k, k), rhs[:k]  # K[u][w]
+
+
+def coef_vec(a=None, b=None, c=0.0, s=0.0):
+    cf = np.zeros(21)
+    cf[0:9] = np.eye(3).reshape(-1) if a is None else np.asarray(a).reshape(-1)
+    if b is not None:
+        cf[9:12] = b
+    cf[15] = c
+    cf[19] = s
+    return cf
+
+
+def tet_closed_form(v):
+    T = np.array([v[1] - v[0], v[2] - v[0], v[3] - v[0]]).T
+    vol = abs(np.linalg.det(T)) / 6
+    G = np.zeros((4, 3))
+    Ti = np.linalg.inv(T)
+    G[1:] = Ti
+    G[0] = -Ti.sum(0)
+    return vol, G
+
+
+def test_tet_stiffness_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) + rng.uniform(-.3, .3, (4, 3))
+        A = rng.uniform(-0.2, 0.2, (3, 3)) + 2 * np.eye(3)
+        K, F = elem(TETRA, v, mf.IND_ALL, coef_vec(a=A, s=0.0))
+        vol, G = tet_closed_form(v)
+        # K[u][w] = vol * grad_w . A grad_u (reference index a_rc: row r = test direction)
+        Kc = vol * G @ A.T @ G.T
+        assert np.abs(K - Kc).max() <= 1e-13 * np.abs(Kc).max()
+
+
+def test_tet_mass_and_source_closed_form():
+    rng = np.random.default_rng(2)
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) + rng.uniform(-.3, .3, (4, 3))
+    ind = mf.coeff_indicator("c", "s")
+    K, F = elem(TETRA, v, ind, coef_vec(a=np.zeros((3, 3)), c=1.0, s=2.0))
+    vol, _ = tet_closed_form(v)
+    M = vol / 20 * (np.ones((4, 4)) + np.eye(4))
+    assert np.abs(K - M).max() <= 1e-13 * M.max()
+    assert np.abs(F - 2.0 * vol / 4).max() <= 1e-13
+
+
+def test_convection_term_orientation():
+    """K[u][w] = int (b . grad phi_u) phi_w: with constant gradients on a tet the
+    closed form is vol/4 * (b . g_u) for every w"""
+    rng = np.random.default_rng(3)
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) + rng.uniform(-.2, .2, (4, 3))
+    b = np.array([0.3, -1.1, 0.7])
+    ind = mf.coeff_indicator("bx", "by", "bz")
+    K, _ = elem(TETRA, v, ind, coef_vec(a=np.zeros((3, 3)), b=b))
+    vol, G = tet_closed_form(v)
+    Kc = np.outer(G @ b, np.ones(4)) * vol / 4
+    assert np.abs(K - Kc).max() <= 1e-13 * np.abs(Kc).max()
+
+
+def test_prism_row_sums_symmetry_volume():
+    rng = np.random.default_rng(4)
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]], float)
+    for _ in range(50):
+        v = base + rng.uniform(-.2, .2, (6, 3))
+        K, F = elem(PRISM, v, mf.IND_DIFF, coef_vec(s=1.0))
+        assert np.abs(K.sum(0)).max() <= 1e-13 * np.abs(K).max()  # K . 1 = 0
+        assert np.abs(K - K.T).max() <= 1e-14 * np.abs(K).max()
+        assert np.all(np.linalg.eigvalsh(0.5 * (K + K.T)) > -1e-12)
+
+
+def test_affine_patch_test_and_total_volume():
+    """a globally linear u makes K u vanish at interior nodes on an affine
+    (unjittered, linearly mapped) prism mesh; sum of the load = volume"""
+    m = mf.Mesh.box_prism(4, 5, 3, lx=1.0, ly=1.3, lz=0.7, seed=9, bc=0)
+    a = m.arrays()
+    L = np.array([[1.0, 0.2, 0.1], [0.0, 0.9, -0.3], [0.1, 0.0, 1.2]])
+    a["coor"] = a["coor"] @ L.T
+    op = OracleProblem(a)
+    coef = np.tile(coef_vec(s=1.0), (op.nel, 1))
+    val, rhs = op.assemble(mf.IND_DIFF, coef)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val, op.col_ind, op.row_ptr), shape=(op.n, op.n))
+    x = op.coor[op.dof_node]
+    u = 0.7 * x[:, 0] - 1.3 * x[:, 1] + 0.4 * x[:, 2] + 2.0
+    Ku = A @ u
+    lo, hi = op.coor.min(0), op.coor.max(0)
+    box0 = np.array([[0, 0, 0]]) @ L.T
+    # interior nodes: those not on the mapped box boundary (reference coords)
+    ref = op.coor[op.dof_node] @ np.linalg.inv(L).T
+    interior = np.all((ref > 1e-9) & (ref < np.array([1.0, 1.3, 0.7]) - 1e-9), axis=1)
+    assert interior.sum() > 10
+    assert np.abs(Ku[interior]).max() <= 1e-12 * np.abs(val).max()
+    assert abs(rhs.sum() - abs(np.linalg.det(L)) * 1.0 * 1.3 * 0.7) <= 1e-13
+
+
+def test_penalty_face_closed_forms():
+    m = mf.Mesh.box_prism(2, 2, 2, lx=1.0, ly=2.0, lz=3.0, seed=1)
+    op = OracleProblem(m.arrays())
+    for f in range(op.nbf):
+        e, side = op.bfe[f], op.bfs[f]
+        K, F = op.face(f, g=0.5)
+        nodes = {0: [0, 1, 2], 1: [3, 4, 5], 2: [0, 1, 3, 4], 3: [1, 2, 4, 5], 4: [2, 0, 5, 3]}[side]
+        X = op.coor[op.en[e, :6]]
+        off = [i for i in range(6) if i not in nodes]
+        assert np.abs(K[off]).max() == 0 and np.abs(K[:, off]).max() == 0
+        if side < 2:
+            A = 0.5 * np.linalg.norm(np.cross(X[nodes[1]] - X[nodes[0]], X[nodes[2]] - X[nodes[0]]))
+            Mc = A / 12 * (np.ones((3, 3)) + np.eye(3))
+        else:
+            p0, p1, p3 = X[nodes[0]], X[nodes[1]], X[nodes[2]]
+            A = np.linalg.norm(p1 - p0) * np.linalg.norm(p3 - p0)
+            # bilinear mass on a rectangle, node order (0,1) bottom edge, (2,3) top edge above them
+            Mc = A / 36 * np.array([[4, 2, 2, 1], [2, 4, 1, 2], [2, 1, 4, 2], [1, 2, 2, 4]], float)
+        Ks = K[np.ix_(nodes, nodes)]
+        assert np.abs(Ks - 1e7 * Mc).max() <= 1e-13 * 1e7 * Mc.max(), (f, side)
+        assert np.abs(F[nodes] - 1e7 * 0.5 * Mc.sum(1)).max() <= 1e-13 * 1e7 * Mc.max()
+
+
+def dense_system(n, seed, nonsym=True):
+    rng = np.random.default_rng(seed)
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in range(max(0, i - 2), min(n, i + 3)):
+            A[i, j] = rng.uniform(-1, 1) if nonsym else 0.0
+        A[i, i] = 6.0 + rng.uniform(0, 1)
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        ci += list(nz)
+        v += list(A[i, nz])
+        rp.append(len(ci))
+    return A, np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+
+
+@pytest.mark.parametrize("precon", [0, 1])
+def test_gmres_dense_solution(precon):
+    A, rp, ci, v = dense_system(80, 5)
+    b = np.random.default_rng(6).uniform(-1, 1, 80)
+    x, info = oracle_gmres_crs(rp, ci, v, b, precon=precon, krylbas=20, max_iter=500, tol=1e-12)
+    xd = np.linalg.solve(A, b)
+    assert info["converged"]
+    assert np.abs(x - xd).max() <= 1e-9 * np.abs(xd).max()
+
+
+def test_gmres_krylov_invariance_and_nonconvergence():
+    A, rp, ci, v = dense_system(30, 7)
+    b = np.random.default_rng(8).uniform(-1, 1, 30)
+    x, info = oracle_gmres_crs(rp, ci, v, b, precon=0, krylbas=30, max_iter=60, tol=1e-13)
+    assert info["converged"] and info["iters"] <= 30
+    x, info = oracle_gmres_crs(rp, ci, v, b, precon=0, krylbas=5, max_iter=3, tol=1e-13)
+    assert not info["converged"] and info["iters"] == 3
+
+
+def test_drotg_identities():
+    o = oracle()
+    rng = np.random.default_rng(9)
+    for a0, b0 in list(rng.normal(size=(50, 2))) + [(0.0, 0.0), (0.0, 2.0), (3.0, 0.0)]:
+        a, b, c, s = ctypes.c_double(a0), ctypes.c_double(b0), ctypes.c_double(), ctypes.c_double()
+        o.orc_drotg(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(s))
+        r = a.value
+        assert abs(c.value ** 2 + s.value ** 2 - 1) <= 1e-15 or (a0 == 0 and b0 == 0)
+        assert abs(abs(r) - np.hypot(a0, b0)) <= 1e-15 * max(1, abs(r))
+        assert abs(c.value * a0 + s.value * b0 - r) <= 1e-14 * max(1, abs(r))
+        assert abs(-s.value * a0 + c.value * b0) <= 1e-14 * max(1, abs(r))
