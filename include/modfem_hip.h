@@ -70,6 +70,11 @@ int mf_mesh_free(int mesh);
 int mf_mesh_setup(int mesh, int* n, long long* nnz);
 int mf_mesh_setup_get(int mesh, int* dof_node, int* row_ptr, int* col_ind);
 int mf_mesh_setup_table(int mesh, int int_ent, int* table);
+/* device-internal numbering chosen by the set-up (no reference counterpart:
+   the reference keeps first-appearance numbering, and so does every mf_
+   boundary): device row i holds DOF row_perm[i], device element slot e holds
+   element el_perm[e] */
+int mf_mesh_setup_order(int mesh, int* row_perm, int* el_perm);
 
 /* --------------------------------------------------------------- problem
  * mf_problem_create replaces sir_create's integer set-up for a std-linear
@@ -115,7 +120,9 @@ int mf_element_matrices(int pid, int first, int count, int precision, double* ou
 /* assembly strategy of mf_integrate_assemble: 0 (default) one lane per
    matrix row walking its elements in ascending order, deterministic and in
    the reference's summation order; 1 one lane group per element with FP64
-   atomics (order-dependent rounding) */
+   atomics (order-dependent rounding); 2 two passes: element rows written
+   once to a device buffer, then gathered per matrix row in ascending element
+   order (deterministic) */
 int mf_set_assembly_mode(int pid, int mode);
 int mf_face_matrices(int pid, int first, int count, double* out);
 int mf_get_system(int pid, double* val, double* rhs);

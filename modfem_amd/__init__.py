@@ -82,6 +82,7 @@ def _declare(L):
         "mf_mesh_setup": (I, [I, V, V]),
         "mf_mesh_setup_get": (I, [I, V, V, V]),
         "mf_mesh_setup_table": (I, [I, I, V]),
+        "mf_mesh_setup_order": (I, [I, V, V]),
         "mf_problem_create": (I, [I]),
         "mf_problem_create_crs": (I, [I, V, V, V, V]),
         "mf_problem_sizes": (I, [I, V, V, V, V, V]),
@@ -218,6 +219,15 @@ class Mesh:
         ci = np.zeros(nnz.value, np.int32)
         check(lib().mf_mesh_setup_get(self.h, _p(d), _p(rp), _p(ci)))
         return d, rp, ci
+
+    def setup_order(self):
+        """device-internal numbering after setup(): (row_perm, el_perm)"""
+        n, nnz = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().mf_mesh_setup(self.h, ctypes.byref(n), ctypes.byref(nnz)))
+        rp = np.zeros(n.value, np.int32)
+        ep = np.zeros(self.nel, np.int32)
+        check(lib().mf_mesh_setup_order(self.h, _p(rp), _p(ep)))
+        return rp, ep
 
     def setup_table(self, int_ent):
         t = np.zeros(42, np.int32)

@@ -59,7 +59,8 @@ void free_problem_device(Problem& P) {
                   P.d_coef, P.d_dinv, P.d_diag_ptr, P.d_dinv32, P.d_tile_row, P.d_V, P.d_H,
                   P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work, P.d_sl_ptr,
                   P.d_scol, P.d_diag_k, P.d_inc_ptr, P.d_inc_el, P.d_inc_loc,
-                  P.d_finc_ptr, P.d_finc_f, P.d_finc_loc, P.d_brows};
+                  P.d_finc_ptr, P.d_finc_f, P.d_finc_loc, P.d_brows, P.d_el_offT,
+                  P.d_erow};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (P.h_stat) hipHostFree(P.h_stat);
@@ -87,10 +88,12 @@ int upload_sell(Problem& P) {
 }
 
 // CRS-ordered values <-> SELL-64 positions (host)
+// (reference row r is device row iperm[r]; entry order inside a row is kept)
 void crs_to_sell(const Setup& S, const double* crs, double* sell) {
   std::fill(sell, sell + S.sl_ptr.back(), 0.0);
   for (int r = 0; r < S.n; r++) {
-    const long long base = S.sl_ptr[r >> 6] + (r & 63);
+    const int i = S.iperm[r];
+    const long long base = S.sl_ptr[i >> 6] + (i & 63);
     for (int k = S.row_ptr[r]; k < S.row_ptr[r + 1]; k++)
       sell[base + 64LL * (k - S.row_ptr[r])] = crs[k];
   }
@@ -98,10 +101,29 @@ void crs_to_sell(const Setup& S, const double* crs, double* sell) {
 
 void sell_to_crs(const Setup& S, const double* sell, double* crs) {
   for (int r = 0; r < S.n; r++) {
-    const long long base = S.sl_ptr[r >> 6] + (r & 63);
+    const int i = S.iperm[r];
+    const long long base = S.sl_ptr[i >> 6] + (i & 63);
     for (int k = S.row_ptr[r]; k < S.row_ptr[r + 1]; k++)
       crs[k] = sell[base + 64LL * (k - S.row_ptr[r])];
   }
+}
+
+// reference-order vector -> device order and back
+std::vector<double> to_device_order(const Setup& S, const double* x) {
+  std::vector<double> d(S.n);
+  for (int i = 0; i < S.n; i++) d[i] = x[S.perm[i]];
+  return d;
+}
+
+void to_reference_order(const Setup& S, const double* d, double* x) {
+  for (int i = 0; i < S.n; i++) x[S.perm[i]] = d[i];
+}
+
+int rhs_to_host(const Problem& P, double* rhs) {
+  std::vector<double> d(P.n);
+  MF_HIP(hipMemcpy(d.data(), P.d_rhs, (size_t)P.n * sizeof(double), hipMemcpyDeviceToHost));
+  to_reference_order(P.setup, d.data(), rhs);
+  return 0;
 }
 
 }  // namespace
@@ -244,6 +266,18 @@ int mf_mesh_setup_get(int mesh, int* dof_node, int* row_ptr, int* col_ind) {
   return 0;
 }
 
+int mf_mesh_setup_order(int mesh, int* row_perm, int* el_perm) {
+  auto it = g_setups.find(mesh);
+  if (it == g_setups.end()) {
+    set_error("mf_mesh_setup has not been called for this mesh");
+    return -3;
+  }
+  const Setup& s = *it->second;
+  if (row_perm) std::memcpy(row_perm, s.perm.data(), s.perm.size() * sizeof(int));
+  if (el_perm) std::memcpy(el_perm, s.eperm.data(), s.eperm.size() * sizeof(int));
+  return 0;
+}
+
 int mf_mesh_setup_table(int mesh, int int_ent, int* table) {
   auto it = g_setups.find(mesh);
   Mesh* m = get_mesh(mesh);
@@ -274,27 +308,77 @@ int mf_problem_create(int mesh) {
       set_error("mf_problem_create: mixed prism/tetra std meshes are not supported");
       return -1;
     }
-  std::vector<double> cd(3 * (size_t)S.n);
-  for (int d = 0; d < S.n; d++)
-    for (int k = 0; k < 3; k++) cd[3 * (size_t)d + k] = M.coor[3 * (size_t)S.dof_node[d] + k];
-  if (upload(&P->d_coor, cd.data(), cd.size())) return -2;
-  if (upload(&P->d_el_dof, S.el_dof.data(), S.el_dof.size())) return -2;
-  if (upload(&P->d_el_off, S.el_off.data(), S.el_off.size())) return -2;
-  if (upload(&P->d_bf_elem, M.bf_elem.data(), M.bf_elem.size())) return -2;
+  // device arrays in the internal locality numbering (rows perm, elements
+  // eperm); every list keeps the reference's order within a row, so the
+  // summation order of each matrix entry is unchanged
+  const int n = S.n, nel = M.nel;
+  const std::vector<int>&PR = S.perm, &IPR = S.iperm, &EP = S.eperm, &IEP = S.ieperm;
+  {
+    std::vector<double> cd(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+      for (int k = 0; k < 3; k++) cd[3 * (size_t)i + k] = M.coor[3 * (size_t)S.dof_node[PR[i]] + k];
+    if (upload(&P->d_coor, cd.data(), cd.size())) return -2;
+  }
+  {
+    std::vector<int> ed(6 * (size_t)nel);
+    std::vector<uint8_t> eo(36 * (size_t)nel), offT((size_t)nel * 36);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < nel; e++) {
+      const int r = EP[e];
+      for (int k = 0; k < 6; k++) {
+        const int d = S.el_dof[(size_t)k * nel + r];
+        ed[(size_t)k * nel + e] = d < 0 ? -1 : IPR[d];
+      }
+      for (int q = 0; q < 36; q++) {
+        const uint8_t o = S.el_off[(size_t)q * nel + r];
+        eo[(size_t)q * nel + e] = o;
+        offT[(size_t)e * 36 + q] = o;
+      }
+    }
+    if (upload(&P->d_el_dof, ed.data(), ed.size())) return -2;
+    if (upload(&P->d_el_off, eo.data(), eo.size())) return -2;
+    if (upload(&P->d_el_offT, offT.data(), offT.size())) return -2;
+  }
+  {
+    std::vector<int> bfe(M.nbf);
+    for (int f = 0; f < M.nbf; f++) bfe[f] = IEP[M.bf_elem[f]];
+    if (upload(&P->d_bf_elem, bfe.data(), bfe.size())) return -2;
+  }
   if (upload(&P->d_bf_side, M.bf_side.data(), M.bf_side.size())) return -2;
   if (upload_sell(*P)) return -2;
-  if (upload<double>(&P->d_rhs, nullptr, (size_t)S.n)) return -2;
-  if (launch_zero(P->d_val, P->nnz_sell) || launch_zero(P->d_rhs, S.n)) return -2;
-  if (upload(&P->d_inc_ptr, S.inc_ptr.data(), S.inc_ptr.size())) return -2;
-  if (upload(&P->d_inc_el, S.inc_el.data(), S.inc_el.size())) return -2;
-  if (upload(&P->d_inc_loc, S.inc_loc.data(), S.inc_loc.size())) return -2;
-  if (upload(&P->d_finc_ptr, S.finc_ptr.data(), S.finc_ptr.size())) return -2;
-  if (upload(&P->d_finc_f, S.finc_f.data(), S.finc_f.size())) return -2;
-  if (upload(&P->d_finc_loc, S.finc_loc.data(), S.finc_loc.size())) return -2;
+  if (upload<double>(&P->d_rhs, nullptr, (size_t)n)) return -2;
+  if (launch_zero(P->d_val, P->nnz_sell) || launch_zero(P->d_rhs, n)) return -2;
   {
+    // incidence lists per device row, entries still in ascending reference
+    // element / face order (the reference's summation order)
+    std::vector<int> ip(n + 1, 0), fp(n + 1, 0);
+    for (int i = 0; i < n; i++) {
+      ip[i + 1] = ip[i] + S.inc_ptr[PR[i] + 1] - S.inc_ptr[PR[i]];
+      fp[i + 1] = fp[i] + S.finc_ptr[PR[i] + 1] - S.finc_ptr[PR[i]];
+    }
+    std::vector<int> iel(S.inc_el.size()), ff(S.finc_f.size());
+    std::vector<uint8_t> il(S.inc_loc.size()), fl(S.finc_loc.size());
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+      const int r = PR[i];
+      for (int p = S.inc_ptr[r], q = ip[i]; p < S.inc_ptr[r + 1]; p++, q++) {
+        iel[q] = IEP[S.inc_el[p]];
+        il[q] = S.inc_loc[p];
+      }
+      for (int p = S.finc_ptr[r], q = fp[i]; p < S.finc_ptr[r + 1]; p++, q++) {
+        ff[q] = S.finc_f[p];
+        fl[q] = S.finc_loc[p];
+      }
+    }
+    if (upload(&P->d_inc_ptr, ip.data(), ip.size())) return -2;
+    if (upload(&P->d_inc_el, iel.data(), iel.size())) return -2;
+    if (upload(&P->d_inc_loc, il.data(), il.size())) return -2;
+    if (upload(&P->d_finc_ptr, fp.data(), fp.size())) return -2;
+    if (upload(&P->d_finc_f, ff.data(), ff.size())) return -2;
+    if (upload(&P->d_finc_loc, fl.data(), fl.size())) return -2;
     std::vector<int> brows;
-    for (int r = 0; r < S.n; r++)
-      if (S.finc_ptr[r + 1] > S.finc_ptr[r]) brows.push_back(r);
+    for (int i = 0; i < n; i++)
+      if (fp[i + 1] > fp[i]) brows.push_back(i);
     P->nbrows = (int)brows.size();
     if (upload(&P->d_brows, brows.data(), brows.size())) return -2;
   }
@@ -352,7 +436,11 @@ int mf_problem_create_crs(int n, const int* row_ptr, const int* col_ind, const d
     crs_to_sell(P->setup, val, sv.data());
     MF_HIP(hipMemcpy(P->d_val, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
   }
-  if (upload<double>(&P->d_rhs, rhs, (size_t)n)) return -2;
+  {
+    std::vector<double> d;
+    if (rhs) d = to_device_order(P->setup, rhs);
+    if (upload<double>(&P->d_rhs, rhs ? d.data() : nullptr, (size_t)n)) return -2;
+  }
   if (!rhs && launch_zero(P->d_rhs, n)) return -2;
   MF_HIP(hipStreamSynchronize(stream()));
   int id = g_next_problem++;
@@ -450,7 +538,12 @@ int mf_problem_set_coefficients(int pid, const int* ind, int coef_model,
       set_error("mf_problem_set_coefficients: coef_el required for this model");
       return -1;
     }
-    if (upload(&P->d_coef, coef_el, per * (size_t)P->nel)) return -2;
+    // SoA [slot][nel] in device element order
+    const size_t N = P->nel;
+    std::vector<double> cd(per * N);
+    for (size_t k = 0; k < per; k++)
+      for (size_t e = 0; e < N; e++) cd[k * N + e] = coef_el[k * N + P->setup.eperm[e]];
+    if (upload(&P->d_coef, cd.data(), cd.size())) return -2;
   }
   if (P->d_bf_g) {
     hipFree(P->d_bf_g);
@@ -474,6 +567,8 @@ int mf_integrate_assemble(int pid, int precision) {
   }
   if (P->assembly_mode == 0) {
     if (launch_assemble_rows(*P, precision)) return -2;
+  } else if (P->assembly_mode == 2) {
+    if (launch_assemble_two_pass(*P, precision)) return -2;
   } else {
     if (launch_zero(P->d_val, P->nnz_sell) || launch_zero(P->d_rhs, P->n)) return -2;
     if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
@@ -498,7 +593,17 @@ int mf_element_matrices(int pid, int first, int count, int precision, double* ou
   double* d = nullptr;
   MF_HIP(hipMalloc(&d, (size_t)count * 42 * sizeof(double)));
   MF_HIP(hipMemsetAsync(d, 0, (size_t)count * 42 * sizeof(double), stream()));
-  int rc = launch_integrate(*P, precision, 1, d, first, count);
+  // reference elements first..first+count-1 live at device slots ieperm[.]
+  int* dl = nullptr;
+  {
+    std::vector<int> el(count);
+    for (int i = 0; i < count; i++) el[i] = P->setup.ieperm[first + i];
+    if (upload(&dl, el.data(), el.size())) {
+      hipFree(d);
+      return -2;
+    }
+  }
+  int rc = launch_integrate(*P, precision, 1, d, 0, count, dl);
   if (rc == 0) {
     hipError_t e = hipMemcpyAsync(out, d, (size_t)count * 42 * sizeof(double),
                                   hipMemcpyDeviceToHost, stream());
@@ -509,6 +614,7 @@ int mf_element_matrices(int pid, int first, int count, int precision, double* ou
     }
   }
   hipFree(d);
+  hipFree(dl);
   return rc;
 }
 
@@ -546,7 +652,7 @@ int mf_get_system(int pid, double* val, double* rhs) {
     MF_HIP(hipMemcpy(sv.data(), P->d_val, sv.size() * sizeof(double), hipMemcpyDeviceToHost));
     sell_to_crs(P->setup, sv.data(), val);
   }
-  if (rhs) MF_HIP(hipMemcpy(rhs, P->d_rhs, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost));
+  if (rhs && rhs_to_host(*P, rhs)) return -2;
   return 0;
 }
 
@@ -559,7 +665,10 @@ int mf_set_system(int pid, const double* val, const double* rhs) {
     crs_to_sell(P->setup, val, sv.data());
     MF_HIP(hipMemcpy(P->d_val, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
   }
-  if (rhs) MF_HIP(hipMemcpy(P->d_rhs, rhs, (size_t)P->n * sizeof(double), hipMemcpyHostToDevice));
+  if (rhs) {
+    std::vector<double> d = to_device_order(P->setup, rhs);
+    MF_HIP(hipMemcpy(P->d_rhs, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (P->d_dinv) {
     hipFree(P->d_dinv);
     P->d_dinv = nullptr;
@@ -574,11 +683,13 @@ int mf_spmv(int pid, const double* x, double* y) {
   MF_HIP(hipMalloc(&dx, (size_t)P->n * sizeof(double)));
   MF_HIP(hipMalloc(&dy, (size_t)P->n * sizeof(double)));
   int rc = 0;
-  hipError_t e = hipMemcpyAsync(dx, x, (size_t)P->n * sizeof(double), hipMemcpyHostToDevice, stream());
+  std::vector<double> xd = to_device_order(P->setup, x), yd(P->n);
+  hipError_t e = hipMemcpyAsync(dx, xd.data(), (size_t)P->n * sizeof(double), hipMemcpyHostToDevice, stream());
   if (e == hipSuccess) rc = launch_spmv(*P, dx, dy);
   if (e == hipSuccess && rc == 0)
-    e = hipMemcpyAsync(y, dy, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost, stream());
+    e = hipMemcpyAsync(yd.data(), dy, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost, stream());
   if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  if (e == hipSuccess && rc == 0) to_reference_order(P->setup, yd.data(), y);
   if (e != hipSuccess) {
     set_error(hipGetErrorString(e));
     rc = -2;
@@ -613,7 +724,13 @@ int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol, int ini
     set_error("mf_gmres: max_iter >= 1 and tol >= 0 required");
     return -1;
   }
-  return gmres_solve(*P, precon, krylbas, max_iter, tol, ini_zero, x, iters, res_rel, resid);
+  std::vector<double> xd;
+  if (x) xd = ini_zero ? std::vector<double>(P->n, 0.0) : to_device_order(P->setup, x);
+  int rc = gmres_solve(*P, precon, krylbas, max_iter, tol, ini_zero, x ? xd.data() : nullptr,
+                       iters, res_rel, resid);
+  // rc: 1 converged, 0 not converged, <0 error
+  if (rc >= 0 && x) to_reference_order(P->setup, xd.data(), x);
+  return rc;
 }
 
 int mf_true_residual(int pid, const double* x, double* rel) {
@@ -622,7 +739,7 @@ int mf_true_residual(int pid, const double* x, double* rel) {
   std::vector<double> y(P->n), b(P->n);
   int rc = mf_spmv(pid, x, y.data());
   if (rc) return rc;
-  MF_HIP(hipMemcpy(b.data(), P->d_rhs, (size_t)P->n * sizeof(double), hipMemcpyDeviceToHost));
+  if (rhs_to_host(*P, b.data())) return -2;
   double nr = 0, nb = 0;
   for (int i = 0; i < P->n; i++) {
     double r = b[i] - y[i];
@@ -643,9 +760,11 @@ int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems, doubl
   MF_HIP(hipEventCreate(&c));
   float te = 0, tf = 0;
   for (int r = 0; r < reps; r++) {
-    if (P->assembly_mode == 0) {
+    if (P->assembly_mode == 0 || P->assembly_mode == 2) {
       MF_HIP(hipEventRecord(a, stream()));
-      if (launch_assemble_rows(*P, precision)) return -2;
+      if (P->assembly_mode == 0 ? launch_assemble_rows(*P, precision)
+                                : launch_assemble_two_pass(*P, precision))
+        return -2;
       MF_HIP(hipEventRecord(b, stream()));
       MF_HIP(hipEventRecord(c, stream()));
     } else {
@@ -707,8 +826,8 @@ int mf_last_solve_ms(int pid, double* ms) {
 int mf_set_assembly_mode(int pid, int mode) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
-  if (mode != 0 && mode != 1) {
-    set_error("assembly mode: 0 row-wise (deterministic), 1 element atomics");
+  if (mode < 0 || mode > 2) {
+    set_error("assembly mode: 0 row-wise, 1 element atomics, 2 two-pass (element rows + row gather)");
     return -1;
   }
   P->assembly_mode = mode;

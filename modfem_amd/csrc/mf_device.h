@@ -63,7 +63,7 @@ struct Problem {
   long long nnz = 0;
   int el_type = kPrism;  // uniform element type (mixed handled per launch)
   // device arrays
-  double* d_coor = nullptr;    // 3*n, DOF order
+  double* d_coor = nullptr;    // 3*n, device row order
   int* d_el_dof = nullptr;     // 6*nel SoA
   uint8_t* d_el_off = nullptr; // 36*nel SoA
   int* d_bf_elem = nullptr;
@@ -87,6 +87,9 @@ struct Problem {
   uint8_t* d_finc_loc = nullptr;
   int* d_brows = nullptr;      // rows touched by boundary faces
   int nbrows = 0;
+  uint8_t* d_el_offT = nullptr; // [nel][36] offsets (row-major per element)
+  void* d_erow = nullptr;       // element-row buffers of the two-pass assembly
+  size_t erow_bytes = 0;
   int assembly_mode = 0;       // 0: row-wise (deterministic), 1: element atomics
   double* d_rhs = nullptr;
   float* d_val32 = nullptr;
@@ -122,9 +125,11 @@ hipStream_t stream();
 
 // kernels (mf_device.hip)
 int launch_zero(double* p, long long n);
+// elements e0..e0+ecount-1 (device slots), or d_elist[0..ecount) when given
 int launch_integrate(Problem& P, int precision, int write_elements,
-                     double* d_emat, int e0, int ecount);
+                     double* d_emat, int e0, int ecount, const int* d_elist = nullptr);
 int launch_assemble_rows(Problem& P, int precision);
+int launch_assemble_two_pass(Problem& P, int precision);
 int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount);
 int launch_fill_jacobi(Problem& P);
 int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,

@@ -156,9 +156,12 @@ struct IntArgs {
   TermMask mask;
   double cc[NCOEF];        // uniform coefficients
   const double* coef;      // per-element SoA
-  int write_elements;
-  double* emat;            // [count][42] when write_elements
+  int write_elements;      // 1: [count][42] column-major; 2: row-major buffers below
+  double* emat;            // [count][42] when write_elements == 1
+  void* erow;              // [nel][6][6] rows (double or float) when == 2
+  void* ef;                // [nel][6] load vector entries
   int e0, count;
+  const int* elist;        // optional element list (write_elements == 1)
 };
 
 struct Coefs {
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
   const bool act_lane = (lane / NS) < EPW;
   const int t = blockIdx.x * EPB + le;
   const bool active = act_lane && t < A.count;
-  const int e = A.e0 + t;
+  const int e = active ? (A.elist ? A.elist[t] : A.e0 + t) : 0;
   const size_t N = A.nel;
   // phase 0
   if (active) {
@@ -359,11 +362,18 @@ __global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
     if (M.s) f += (R)rec[3 * NS + 1] * phw;
     F += f * v;
   }
-  if (A.write_elements) {
+  if (A.write_elements == 1) {
     double* o = A.emat + (size_t)t * 42;
 #pragma unroll
     for (int u = 0; u < NS; u++) o[u * NS + w] = (double)K[u];
     o[NS * NS + w] = (double)F;
+    return;
+  }
+  if (A.write_elements == 2) {
+    R* o = reinterpret_cast<R*>(A.erow) + (size_t)e * 36 + w * 6;
+#pragma unroll
+    for (int u = 0; u < NS; u++) o[u] = K[u];
+    reinterpret_cast<R*>(A.ef)[(size_t)e * 6 + w] = F;
     return;
   }
   // phase 3
@@ -552,6 +562,7 @@ static void fill_int_args(Problem& P, IntArgs& A) {
   A.mask = P.mask;
   for (int k = 0; k < NCOEF; k++) A.cc[k] = P.coef_const[k];
   A.coef = P.d_coef;
+  A.elist = nullptr;
 }
 
 template <int NS, class Q, typename R>
@@ -567,7 +578,7 @@ static void launch_int_ts(int ts, int count, IntArgs& A) {
 }
 
 int launch_integrate(Problem& P, int precision, int write_elements, double* d_emat,
-                     int e0, int ecount) {
+                     int e0, int ecount, const int* d_elist) {
   if (ecount <= 0) return 0;
   IntArgs A;
   fill_int_args(P, A);
@@ -575,6 +586,7 @@ int launch_integrate(Problem& P, int precision, int write_elements, double* d_em
   A.emat = d_emat;
   A.e0 = e0;
   A.count = ecount;
+  A.elist = d_elist;
   const int ts = P.term_set;
   if (P.el_type == kTetra) {
     if (precision == 32) launch_int_ts<4, TetQ, float>(ts, ecount, A);
@@ -935,6 +947,102 @@ int launch_assemble_rows(Problem& P, int precision) {
       if (mr <= 24) launch_rows_ts<6, PrismQ, 24, double>(ts, g, A);
       else launch_rows_ts<6, PrismQ, 32, double>(ts, g, A);
     }
+  }
+  MF_HIP(hipGetLastError());
+  if (P.nbrows > 0) {
+    dim3 gf((P.nbrows + 255) / 256);
+    if (P.el_type == kTetra)
+      hipLaunchKernelGGL(k_assemble_face_rows<4>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
+    else
+      hipLaunchKernelGGL(k_assemble_face_rows<6>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
+    MF_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+// Two-pass assembly, pass 2: one lane per matrix row gathers the rows its
+// DOF owns in the element-row buffers (ascending element order), adds them
+// into LDS accumulators through the per-element offset table and writes the
+// SELL slice once.  T is the buffer type (FP64, or FP32 for the
+// mixed-precision form; sums are FP64 either way).
+template <int RMAX, typename T>
+__global__ void __launch_bounds__(64 * ROW_WAVES) k_gather_rows(RowArgs A, const T* erow,
+                                                               const T* ef, const uint8_t* offT,
+                                                               int ns) {
+  __shared__ double s_acc[ROW_WAVES][RMAX][64];
+  const int nsl = (A.n + 63) >> 6;
+  const int wv = threadIdx.x >> 6;
+  const int sl = blockIdx.x * ROW_WAVES + wv;
+  if (sl >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const int r = sl * 64 + lane;
+  double(*acc)[64] = s_acc[wv];
+#pragma unroll
+  for (int k = 0; k < RMAX; k++) acc[k][lane] = 0.0;
+  double F = 0.0;
+  if (r < A.n) {
+    for (int p = A.inc_ptr[r]; p < A.inc_ptr[r + 1]; p++) {
+      const size_t q = (size_t)A.inc_el[p] * 36 + A.inc_loc[p] * 6;
+      for (int u = 0; u < ns; u++) acc[offT[q + u]][lane] += (double)erow[q + u];
+      F += (double)ef[(size_t)A.inc_el[p] * 6 + A.inc_loc[p]];
+    }
+  }
+  const long long base = A.I.sl_ptr[sl];
+  const int len = (int)((A.I.sl_ptr[sl + 1] - base) >> 6);
+  for (int k = 0; k < len; k++) A.val[base + 64LL * k + lane] = acc[k][lane];
+  if (r < A.n) A.rhs[r] = F;
+}
+
+int launch_assemble_two_pass(Problem& P, int precision) {
+  const size_t esz = precision == 32 ? sizeof(float) : sizeof(double);
+  if (!P.d_erow || P.erow_bytes < (size_t)P.nel * 42 * esz) {
+    if (P.d_erow) (void)hipFree(P.d_erow);
+    MF_HIP(hipMalloc(&P.d_erow, (size_t)P.nel * 42 * esz));
+    P.erow_bytes = (size_t)P.nel * 42 * esz;
+  }
+  // pass 1
+  IntArgs I;
+  fill_int_args(P, I);
+  I.write_elements = 2;
+  I.erow = P.d_erow;
+  I.ef = (char*)P.d_erow + (size_t)P.nel * 36 * esz;
+  I.emat = nullptr;
+  I.e0 = 0;
+  I.count = P.nel;
+  const int ts = P.term_set;
+  if (P.el_type == kTetra) {
+    if (precision == 32) launch_int_ts<4, TetQ, float>(ts, P.nel, I);
+    else launch_int_ts<4, TetQ, double>(ts, P.nel, I);
+  } else {
+    if (precision == 32) launch_int_ts<6, PrismQ, float>(ts, P.nel, I);
+    else launch_int_ts<6, PrismQ, double>(ts, P.nel, I);
+  }
+  MF_HIP(hipGetLastError());
+  // pass 2
+  RowArgs A;
+  A.I = I;
+  A.n = P.n;
+  A.inc_ptr = P.d_inc_ptr; A.inc_el = P.d_inc_el; A.inc_loc = P.d_inc_loc;
+  A.finc_ptr = P.d_finc_ptr; A.finc_f = P.d_finc_f; A.finc_loc = P.d_finc_loc;
+  A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side; A.bf_g = P.d_bf_g;
+  A.val = P.d_val; A.rhs = P.d_rhs;
+  const int nsl = (P.n + 63) / 64;
+  dim3 g((nsl + ROW_WAVES - 1) / ROW_WAVES), b(64 * ROW_WAVES);
+  const int mr = P.setup.max_row, ns = nodes_of(P.el_type);
+  if (mr > 32) {
+    set_error("two-pass assembly supports rows of at most 32 entries; use the atomic path");
+    return -1;
+  }
+  if (precision == 32) {
+    const float* er = (const float*)P.d_erow;
+    const float* ef = er + (size_t)P.nel * 36;
+    if (mr <= 24) hipLaunchKernelGGL((k_gather_rows<24, float>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
+    else hipLaunchKernelGGL((k_gather_rows<32, float>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
+  } else {
+    const double* er = (const double*)P.d_erow;
+    const double* ef = er + (size_t)P.nel * 36;
+    if (mr <= 24) hipLaunchKernelGGL((k_gather_rows<24, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
+    else hipLaunchKernelGGL((k_gather_rows<32, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
   }
   MF_HIP(hipGetLastError());
   if (P.nbrows > 0) {

@@ -69,10 +69,10 @@ struct Setup {
   // column dof(u) inside row dof(w): off[(w*6+u)*nel + e]
   std::vector<uint8_t> el_off;     // 36*nel
   std::vector<uint8_t> bf_off;     // 36*nbf (faces reuse owner's DOFs)
-  // device-native SELL-64 layout of the same matrix: slice s holds rows
-  // 64s..64s+63 column-major, padded to its longest row (padding: column =
-  // own row, value 0).  Entry k of row r lives at
-  //   sl_ptr[r/64] + 64*k + r%64.
+  // device-native SELL-64 layout of the same matrix in device row order
+  // (perm below): slice s holds device rows 64s..64s+63 column-major, padded
+  // to its longest row (padding: column = own row, value 0).  Entry k of
+  // device row i lives at sl_ptr[i/64] + 64*k + i%64; columns are device rows.
   std::vector<long long> sl_ptr;   // ns+1
   std::vector<int> sl_col;         // sl_ptr[ns]
   std::vector<uint8_t> diag_k;     // n: position of the diagonal in its row
@@ -83,9 +83,19 @@ struct Setup {
   std::vector<uint8_t> inc_loc;
   std::vector<int> finc_ptr, finc_f;
   std::vector<uint8_t> finc_loc;
+  // device-internal numbering (no counterpart in the reference, whose DOF
+  // and element numbering is kept at every host boundary): device row i holds
+  // reference DOF perm[i] and device element slot e holds reference element
+  // eperm[e]; iperm / ieperm are the inverses.  Rows keep the reference's
+  // entry order, so per-entry and per-row sums are unchanged by it.
+  std::vector<int> perm, iperm, eperm, ieperm;
 };
 
+// SELL-64 arrays (sl_ptr, sl_col, diag_k) in device row order
 int build_sell(Setup& s);
+// locality order: elements along a Morton curve of their centroids, DOFs by
+// first appearance over elements in that order; identity when !mesh
+int build_device_order(const Mesh* m, Setup& s);
 
 int build_setup(const Mesh& m, Setup& s);
 // full reference-format table for int-ent ie (elements then faces)

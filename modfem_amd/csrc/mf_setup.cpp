@@ -177,17 +177,98 @@ int build_setup(const Mesh& m, Setup& s) {
       }
     }
   }
+  if (build_device_order(&m, s)) return -1;
   return build_sell(s);
+}
+
+namespace {
+// 3 x 21-bit Morton interleave
+uint64_t spread21(uint64_t v) {
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+}  // namespace
+
+int build_device_order(const Mesh* m, Setup& s) {
+  const int n = s.n;
+  s.perm.resize(n);
+  s.iperm.resize(n);
+  if (!m) {
+    for (int i = 0; i < n; i++) s.perm[i] = s.iperm[i] = i;
+    s.eperm.clear();
+    s.ieperm.clear();
+    return 0;
+  }
+  const int nel = m->nel;
+  // element centroids -> Morton keys over the bounding box
+  std::vector<double> cen(3 * (size_t)nel);
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int e = 0; e < nel; e++) {
+    const int k = nodes_of(m->type[e]);
+    for (int d = 0; d < 3; d++) {
+      double c = 0;
+      for (int i = 0; i < k; i++) c += m->coor[3 * (size_t)m->nodes[6 * (size_t)e + i] + d];
+      c /= k;
+      cen[3 * (size_t)e + d] = c;
+      lo[d] = std::min(lo[d], c);
+      hi[d] = std::max(hi[d], c);
+    }
+  }
+  std::vector<std::pair<uint64_t, int>> key(nel);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < nel; e++) {
+    uint64_t code = 0;
+    for (int d = 0; d < 3; d++) {
+      const double span = hi[d] - lo[d];
+      double t = span > 0 ? (cen[3 * (size_t)e + d] - lo[d]) / span : 0.0;
+      uint64_t q = (uint64_t)(t * 2097151.0);
+      code |= spread21(q) << d;
+    }
+    key[e] = {code, e};
+  }
+  std::sort(key.begin(), key.end());
+  s.eperm.resize(nel);
+  s.ieperm.resize(nel);
+  for (int i = 0; i < nel; i++) {
+    s.eperm[i] = key[i].second;
+    s.ieperm[key[i].second] = i;
+  }
+  // DOFs by first appearance over elements in device order
+  std::fill(s.iperm.begin(), s.iperm.end(), -1);
+  int next = 0;
+  for (int i = 0; i < nel; i++) {
+    const int e = s.eperm[i];
+    for (int k = 0; k < nodes_of(m->type[e]); k++) {
+      const int d = s.el_dof[(size_t)k * nel + e];
+      if (s.iperm[d] < 0) {
+        s.iperm[d] = next;
+        s.perm[next++] = d;
+      }
+    }
+  }
+  if (next != n) {
+    set_error("device order: DOF without an element");
+    return -1;
+  }
+  return 0;
 }
 
 int build_sell(Setup& s) {
   const int n = s.n;
   const int ns = (n + 63) / 64;
+  if ((int)s.perm.size() != n) build_device_order(nullptr, s);
+  const std::vector<int>& P = s.perm;
+  const std::vector<int>& IP = s.iperm;
   s.sl_ptr.assign(ns + 1, 0);
   for (int sl = 0; sl < ns; sl++) {
     int len = 0;
-    for (int r = sl * 64; r < std::min(n, sl * 64 + 64); r++)
-      len = std::max(len, s.row_ptr[r + 1] - s.row_ptr[r]);
+    for (int i = sl * 64; i < std::min(n, sl * 64 + 64); i++)
+      len = std::max(len, s.row_ptr[P[i] + 1] - s.row_ptr[P[i]]);
     s.sl_ptr[sl + 1] = s.sl_ptr[sl] + 64LL * len;
   }
   s.sl_col.assign(s.sl_ptr[ns], 0);
@@ -197,15 +278,16 @@ int build_sell(Setup& s) {
     const long long base = s.sl_ptr[sl];
     const int len = (int)((s.sl_ptr[sl + 1] - base) / 64);
     for (int lane = 0; lane < 64; lane++) {
-      const int r = sl * 64 + lane;
+      const int i = sl * 64 + lane;  // device row
+      const int r = i < n ? P[i] : -1;
       for (int k = 0; k < len; k++) {
-        int c = r < n ? r : n - 1;  // padding points at the own row
-        if (r < n && k < s.row_ptr[r + 1] - s.row_ptr[r]) c = s.col_ind[s.row_ptr[r] + k];
+        int c = i < n ? i : n - 1;  // padding points at the own row
+        if (r >= 0 && k < s.row_ptr[r + 1] - s.row_ptr[r]) c = IP[s.col_ind[s.row_ptr[r] + k]];
         s.sl_col[base + 64LL * k + lane] = c;
       }
-      if (r < n)
+      if (r >= 0)
         for (int k = s.row_ptr[r]; k < s.row_ptr[r + 1]; k++)
-          if (s.col_ind[k] == r) s.diag_k[r] = (uint8_t)(k - s.row_ptr[r]);
+          if (s.col_ind[k] == r) s.diag_k[i] = (uint8_t)(k - s.row_ptr[r]);
     }
   }
   return 0;

@@ -207,24 +207,30 @@ __device__ __forceinline__ void slice_range(int nsl, int lb, int& s0, int& s1) {
   s1 = (int)((long long)(lb + 1) * nsl / gridDim.x);
 }
 
+// row sum of the calling lane's row: the slice's column indices and values
+// are loaded first (up to 24 entries in flight per lane), then the vector
+// gathers are issued together, so a slice costs about two memory round trips
+// instead of one per entry
 __device__ __forceinline__ double sell_row(const SolveArgs& A, const double* x, int sl, int lane) {
+  constexpr int CH = 12;
   const long long base = A.sl_ptr[sl] + lane;
   const int len = (int)((A.sl_ptr[sl + 1] - A.sl_ptr[sl]) >> 6);
   double s = 0.0;
-  int k = 0;
-  for (; k + 4 <= len; k += 4) {
-    const long long q = base + 64LL * k;
-    const double v0 = A.val[q], v1 = A.val[q + 64], v2 = A.val[q + 128], v3 = A.val[q + 192];
-    const int c0 = A.scol[q], c1 = A.scol[q + 64], c2 = A.scol[q + 128], c3 = A.scol[q + 192];
-    const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-    s += v0 * x0;
-    s += v1 * x1;
-    s += v2 * x2;
-    s += v3 * x3;
-  }
-  for (; k < len; k++) {
-    const long long q = base + 64LL * k;
-    s += A.val[q] * x[A.scol[q]];
+  for (int k0 = 0; k0 < len; k0 += CH) {
+    int c[CH];
+    double v[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      const bool ok = k0 + k < len;
+      const long long q = base + 64LL * (ok ? k0 + k : k0);
+      c[k] = A.scol[q];
+      v[k] = ok ? A.val[q] : 0.0;
+    }
+    double xv[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) xv[k] = x[c[k]];
+#pragma unroll
+    for (int k = 0; k < CH; k++) s += v[k] * xv[k];
   }
   return s;
 }
@@ -251,10 +257,14 @@ __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
   if (threadIdx.x == 0) A.partials[lb] = red[0];
 }
 
-template <int NV>
+// Arnoldi step j, part 1 (SELL-64).  Phase 1: w = s_j D^-1 A V_j for the
+// block's slices, written to V_{j+1} and staged in LDS.  Phase 2: wavefront q
+// forms V_i . w over the block's rows for i = q, q+SW, ..., streaming V_i with
+// four independent accumulators and one wave-wide sum per i, so the number of
+// basis vectors costs neither registers nor occupancy.
 __global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
   if (A.stat->stop) return;
-  __shared__ double red[NV];
+  extern __shared__ double ws[];  // (s1 - s0) * 64 values of w
   const int nsl = (A.n + 63) >> 6;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   int s0, s1;
@@ -263,23 +273,34 @@ __global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
   const double* x = A.V + (size_t)j * A.ldv;
   double* w = A.V + (size_t)(j + 1) * A.ldv;
   const double sj = Scale(A)[j];
-  double acc[NV];
-#pragma unroll
-  for (int i = 0; i < NV; i++) acc[i] = 0.0;
   for (int sl = s0 + wv; sl < s1; sl += SW) {
     const double s = sell_row(A, x, sl, lane);
     const int r = sl * 64 + lane;
+    double wr = 0.0;
     if (r < A.n) {
-      double wr = sj * s;
+      wr = sj * s;
       if (A.dinv) wr *= A.dinv[r];
       w[r] = wr;
-#pragma unroll
-      for (int i = 0; i < NV; i++)
-        if (i <= j) acc[i] += A.V[(size_t)i * A.ldv + r] * wr;
     }
+    ws[(sl - s0) * 64 + lane] = wr;
   }
-  block_sum_multi<NV>(acc, j + 1, red);
-  if ((int)threadIdx.x <= j) A.partials[(size_t)lb * NV + threadIdx.x] = red[threadIdx.x];
+  __syncthreads();
+  const long long r0 = (long long)s0 * 64;
+  const int nv = (int)min((long long)(s1 - s0) * 64, (long long)A.n - r0);
+  for (int i = wv; i <= j; i += SW) {
+    const double* v = A.V + (size_t)i * A.ldv + r0;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = lane;
+    for (; k + 192 < nv; k += 256) {
+      a0 += v[k] * ws[k];
+      a1 += v[k + 64] * ws[k + 64];
+      a2 += v[k + 128] * ws[k + 128];
+      a3 += v[k + 192] * ws[k + 192];
+    }
+    for (; k < nv; k += 64) a0 += v[k] * ws[k];
+    const double d = wave_sum((a0 + a1) + (a2 + a3));
+    if (lane == 0) A.partials[(size_t)i * gridDim.x + lb] = d;
+  }
 }
 
 __global__ void __launch_bounds__(SB) k_spmv_sell(SolveArgs A, const double* x, double* y) {
@@ -368,26 +389,23 @@ __global__ void __launch_bounds__(SB) k_op_dots(SolveArgs A, int j) {
     }
   }
   block_sum_multi<NV>(acc, j + 1, red);
-  if ((int)threadIdx.x <= j) A.partials[(size_t)lb * NV + threadIdx.x] = red[threadIdx.x];
+  if ((int)threadIdx.x <= j) A.partials[(size_t)threadIdx.x * gridDim.x + lb] = red[threadIdx.x];
 }
 
-template <int NV>
+// one block per dot product i <= j: h_i = s_i * sum_b partials[i][b]
 __global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
   if (A.stat->stop) return;
-  __shared__ double red[NV];
-  double acc[NV];
-#pragma unroll
-  for (int i = 0; i < NV; i++) acc[i] = 0.0;
-  for (int b = threadIdx.x; b < G; b += SB)
-#pragma unroll
-    for (int i = 0; i < NV; i++)
-      if (i <= j) acc[i] += A.partials[(size_t)b * NV + i];
-  block_sum_multi<NV>(acc, j + 1, red);
-  if ((int)threadIdx.x <= j) {
-    const double si = Scale(A)[threadIdx.x];
-    double h = si * red[threadIdx.x];
-    Hcol(A, j + 1)[threadIdx.x] = h;
-    Coef(A)[threadIdx.x] = h * si;
+  __shared__ double red[1];
+  const int i = blockIdx.x;
+  const double* p = A.partials + (size_t)i * G;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += SB) s += p[b];
+  block_sum_multi<1>(&s, 1, red);
+  if (threadIdx.x == 0) {
+    const double si = Scale(A)[i];
+    const double h = si * red[0];
+    Hcol(A, j + 1)[i] = h;
+    Coef(A)[i] = h * si;
   }
 }
 
@@ -453,33 +471,45 @@ __device__ __forceinline__ void rot(double* x, double* y, double c, double s) {
 __global__ void __launch_bounds__(SB) k_fin_update(SolveArgs A, int j, int G) {
   if (A.stat->stop) return;
   __shared__ double red[1];
+  __shared__ double col[MAX_NV + 1], giv[2 * MAX_NV], g2[2];
   double s = 0.0;
   for (int b = threadIdx.x; b < G; b += SB) s += A.partials[b];
+  double* hc = Hcol(A, j + 1);
+  double* gv = Giv(A);
+  if ((int)threadIdx.x <= j) col[threadIdx.x] = hc[threadIdx.x];
+  if ((int)threadIdx.x < 2 * j) giv[threadIdx.x] = gv[threadIdx.x];
   block_sum_multi<1>(&s, 1, red);
-  if (threadIdx.x != 0) return;
-  const int m = A.m;
-  double hn = sqrt(red[0]);
-  double* col = Hcol(A, j + 1);
-  col[j + 1] = hn;
-  Scale(A)[j + 1] = 1.0 / hn;
-  double* giv = Giv(A);
-  for (int i = 1; i <= j; i++) rot(&col[i - 1], &col[i], giv[2 * (i - 1)], giv[2 * (i - 1) + 1]);
-  double ra = col[j], rb = col[j + 1];
-  drotg_dev(&ra, &rb, &giv[2 * j], &giv[2 * j + 1]);
-  rot(&col[j], &col[j + 1], giv[2 * j], giv[2 * j + 1]);
-  double* g = Hcol(A, 0);
-  rot(&g[j], &g[j + 1], giv[2 * j], giv[2 * j + 1]);
-  GmresStat* st = A.stat;
-  st->gmiter += 1;
-  st->j_done = j + 1;
-  st->resid = fabs(g[j + 1]);
-  st->res_rel = st->beta0 != 0.0 ? st->resid / st->beta0 : 0.0;
-  if (st->resid < st->tol_res || st->res_rel < st->tol_rel) {
-    st->converged = 1;
-    st->stop = 1;
+  if (threadIdx.x == 0) {
+    const int m = A.m;
+    const double hn = sqrt(red[0]);
+    col[j + 1] = hn;
+    Scale(A)[j + 1] = 1.0 / hn;
+    for (int i = 1; i <= j; i++) rot(&col[i - 1], &col[i], giv[2 * (i - 1)], giv[2 * (i - 1) + 1]);
+    double ra = col[j], rb = col[j + 1], c, sn;
+    drotg_dev(&ra, &rb, &c, &sn);
+    gv[2 * j] = c;
+    gv[2 * j + 1] = sn;
+    rot(&col[j], &col[j + 1], c, sn);
+    double* g = Hcol(A, 0);
+    g2[0] = g[j];
+    g2[1] = g[j + 1];
+    rot(&g2[0], &g2[1], c, sn);
+    g[j] = g2[0];
+    g[j + 1] = g2[1];
+    GmresStat* st = A.stat;
+    st->gmiter += 1;
+    st->j_done = j + 1;
+    st->resid = fabs(g2[1]);
+    st->res_rel = st->beta0 != 0.0 ? st->resid / st->beta0 : 0.0;
+    if (st->resid < st->tol_res || st->res_rel < st->tol_rel) {
+      st->converged = 1;
+      st->stop = 1;
+    }
+    if (j + 1 == m || st->gmiter == st->max_iter) st->stop = 1;
+    *A.hstat = *st;
   }
-  if (j + 1 == m || st->gmiter == st->max_iter) st->stop = 1;
-  *A.hstat = *st;
+  __syncthreads();
+  if ((int)threadIdx.x <= j + 1) hc[threadIdx.x] = col[threadIdx.x];
 }
 
 // x += sum_c (y_c s_c) V_c
@@ -585,9 +615,16 @@ static void fill_solve_args(Problem& P, SolveArgs& A, int precon, int m) {
 static int grid_tiles(const Problem& P) {
   return std::max(1, std::min(P.ntiles, std::min(MAX_BLOCKS, num_cus() * 5)));
 }
+// at most MAX_SLICES slices per block: the op kernel stages their w in LDS
+constexpr int MAX_SLICES = 32;
 static int grid_slices(const Problem& P) {
   const int nsl = (P.n + 63) / 64;
-  return std::max(1, std::min((nsl + SW - 1) / SW, std::min(MAX_BLOCKS, num_cus() * 8)));
+  const int want = std::max(num_cus() * 8, (nsl + MAX_SLICES - 1) / MAX_SLICES);
+  return std::max(1, std::min((nsl + SW - 1) / SW, want));
+}
+static size_t op_lds_bytes(int n, int G) {
+  const int nsl = (n + 63) / 64;
+  return (size_t)((nsl + G - 1) / G + 1) * 64 * sizeof(double);
 }
 static int grid_rows(const Problem& P) {
   long long pairs = (P.n + 1) / 2;
@@ -664,8 +701,10 @@ static int ensure_workspace(Problem& P, int m) {
   MF_HIP(hipMalloc(&P.d_V, (size_t)(m + 1) * ldv * sizeof(double)));
   MF_HIP(hipMemset(P.d_V, 0, (size_t)(m + 1) * ldv * sizeof(double)));
   MF_HIP(hipMalloc(&P.d_H, (size_t)((m + 1) * (m + 1) + 4 * (m + 1)) * sizeof(double)));
-  if (!P.d_partials)
-    MF_HIP(hipMalloc(&P.d_partials, (size_t)MAX_BLOCKS * MAX_NV * sizeof(double)));
+  if (!P.d_partials) {
+    const size_t G = std::max<size_t>(MAX_BLOCKS, (size_t)grid_slices(P));
+    MF_HIP(hipMalloc(&P.d_partials, G * MAX_NV * sizeof(double)));
+  }
   if (!P.d_stat) MF_HIP(hipMalloc(&P.d_stat, sizeof(GmresStat)));
   if (!P.h_stat) MF_HIP(hipHostMalloc(&P.h_stat, sizeof(GmresStat), hipHostMallocDefault));
   if (!P.d_x) MF_HIP(hipMalloc(&P.d_x, (size_t)P.n * sizeof(double)));
@@ -682,11 +721,11 @@ static void launch_step(int G_op, int G_up, SolveArgs& A, int j) {
     (void)hipEventRecord(e0, s);
   }
   if (A.sl_ptr)
-    hipLaunchKernelGGL(k_op_dots_sell<NV>, dim3(G_op), dim3(SB), 0, s, A, j);
+    hipLaunchKernelGGL(k_op_dots_sell, dim3(G_op), dim3(SB), op_lds_bytes(A.n, G_op), s, A, j);
   else
     hipLaunchKernelGGL(k_op_dots<NV>, dim3(G_op), dim3(SB), 0, s, A, j);
   if (g_ks.on) (void)hipEventRecord(e1, s);
-  hipLaunchKernelGGL(k_fin_dots<NV>, dim3(1), dim3(SB), 0, s, A, j, G_op);
+  hipLaunchKernelGGL(k_fin_dots, dim3(j + 1), dim3(SB), 0, s, A, j, G_op);
   if (g_ks.on) {
     e2 = ks_event(); e3 = ks_event();
     (void)hipEventRecord(e2, s);

@@ -296,4 +296,15 @@ def test_assembly_modes_agree_and_rowwise_is_reproducible(kind, prism_mesh, tetr
     p.integrate_assemble(64)
     v2, r2 = p.system()
     assert rel_err(v2, v0) <= 1e-12 and rel_err(r2, r0) <= 1e-12
+    p.set_assembly_mode(2)
+    p.integrate_assemble(64)
+    v3, r3 = p.system()
+    p.integrate_assemble(64)
+    v4, r4 = p.system()
+    assert np.array_equal(v3, v4) and np.array_equal(r3, r4)
+    assert rel_err(v3, v0) <= 1e-12 and rel_err(r3, r0) <= 1e-12
+    # mixed-precision form: FP32 element rows, FP64 sums
+    p.integrate_assemble(32)
+    v5, r5 = p.system()
+    assert rel_err(v5, v0) <= 2e-6 and rel_err(r5, r0) <= 2e-6
     p.free()
