@@ -26,6 +26,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 FP64_PEAK_TFS = 78.6       # MI355X dense FP64, TFLOP/s
 ASSEMBLY_MODE = 0          # row-wise deterministic assembly (fastest measured)
+COPY_BYTES = 1 << 30        # stream-copy yardstick (read + write of 1 GiB)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_op_dots_sell_box128_pmc.txt")
 METRIC = "elements integrated+assembled/s; solves/s & p99 latency at 1e-8 rel residual"
 
 
@@ -70,6 +72,17 @@ def cpu_baseline(mesh, kappa, b, gpu_iters, krylbas, threads, sample_iters):
     }
 
 
+def measured_traffic(n, krylbas):
+    """HBM bytes per launch of the operator+dots kernel from the committed
+    FETCH_SIZE / WRITE_SIZE counter summary of this workload (None if absent)"""
+    if n != 128 or krylbas != 50 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +94,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-iters", type=int, default=50)
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware-counter passes: one integrate+assemble, one "
+                         "GMRES restart and a 1 GiB stream copy (the counters' yardstick)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -92,6 +108,13 @@ def main():
     p.set_coefficients(mf.IND_CONV_DIFF, mf.COEF_KAPPA_B, coef_const=cc, coef_el=coef_el)
     p.set_assembly_mode(ASSEMBLY_MODE)
     t_setup = time.perf_counter() - t_setup
+    if args.probe:
+        p.integrate_assemble(64)
+        _, info = p.gmres(precon=1, krylbas=args.krylbas, max_iter=args.krylbas, tol=1e-8)
+        ms = mf.bench_copy(COPY_BYTES, reps=3)
+        print(json.dumps({"probe": True, "gmres_iters": info["iters"], "copy_bytes": COPY_BYTES,
+                          "copy_launches": 4, "copy_GBs": 2 * COPY_BYTES / (ms * 1e-3) / 1e9}))
+        return
 
     def step():
         p.integrate_assemble(64)
@@ -120,6 +143,7 @@ def main():
     p.set_assembly_mode(ASSEMBLY_MODE)
     ms_int, ms_faces = p.bench_integrate(64, reps=5)
     ms_spmv = p.bench_spmv(reps=20)
+    ms_copy = mf.bench_copy(COPY_BYTES, reps=5)
 
     nel, n, nnz = p.nel, p.n, p.nnz
     iters = infos[-1]["iters"]
@@ -153,10 +177,12 @@ def main():
                                "frac_fp64_peak": f_int / (ms_int * 1e-3) / 1e12 / FP64_PEAK_TFS,
                                "algorithmic_GBs": b_int / (ms_int * 1e-3) / 1e9,
                                "modes_ms": modes},
+        "copy_GBs": 2 * COPY_BYTES / (ms_copy * 1e-3) / 1e9,
         "spmv_GBs": (12.0 * nnz + 4.0 * (n + 1) + 16.0 * n) / (ms_spmv * 1e-3) / 1e9,
         "roofline": {"kernel": "k_op_dots (Jacobi-preconditioned SpMV + CGS dots)",
                      "bound": "hbm", "achieved": ach_op, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_op / HBM_PEAK_GBS, "traffic": None,
+                     "frac": ach_op / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.n, args.krylbas),
                      "launches": int(n_op), "avg_launch_ms": t_op / max(n_op, 1),
                      "bytes_per_launch": b_op / max(n_op, 1)},
         "roofline_update": {"kernel": "k_update (CGS update + norm + Givens)",

@@ -149,6 +149,10 @@ int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems,
                        double* ms_faces);
 int mf_bench_spmv(int pid, int reps, double* ms);
 int mf_last_solve_ms(int pid, double* ms);
+/* plain device-to-device stream copy of nbytes (rounded down to 16 bytes):
+   the achievable-bandwidth yardstick and the known-size kernel that scales
+   the HBM byte counters (no reference counterpart) */
+int mf_bench_copy(long long nbytes, int reps, double* ms);
 /* per-launch HIP-event timing of the two Arnoldi kernels inside mf_gmres:
    out[6] = {launches, ms, algorithmic bytes} for the operator+dots kernel,
    then for the update+norm kernel (accumulated since the last call); enable

@@ -102,6 +102,7 @@ def _declare(L):
         "mf_bench_integrate": (I, [I, I, I, V, V]),
         "mf_bench_spmv": (I, [I, I, V]),
         "mf_last_solve_ms": (I, [I, V]),
+        "mf_bench_copy": (I, [ctypes.c_longlong, I, V]),
         "mf_gmres_kernel_stats": (I, [I, V]),
         "mf_set_assembly_mode": (I, [I, I]),
         "mf_problem_free": (I, [I]),
@@ -164,6 +165,14 @@ def gmres_kernel_stats(enable):
     out = np.zeros(6)
     check(lib().mf_gmres_kernel_stats(int(enable), _p(out)))
     return {"op_dots": tuple(out[0:3]), "update": tuple(out[3:6])}
+
+
+def bench_copy(nbytes, reps=5):
+    """ms per device-to-device copy of nbytes"""
+    init()
+    ms = ctypes.c_double()
+    check(lib().mf_bench_copy(int(nbytes), reps, ctypes.byref(ms)))
+    return ms.value
 
 
 class Mesh:

@@ -816,6 +816,15 @@ int mf_bench_spmv(int pid, int reps, double* ms) {
   return 0;
 }
 
+int mf_bench_copy(long long nbytes, int reps, double* ms) {
+  if (need_init()) return -1;
+  if (!ms) {
+    set_error("mf_bench_copy: ms is null");
+    return -1;
+  }
+  return bench_copy(nbytes, reps, ms);
+}
+
 int mf_last_solve_ms(int pid, double* ms) {
   Problem* P = get_problem(pid);
   if (!P) return -3;

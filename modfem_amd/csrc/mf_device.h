@@ -136,6 +136,7 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,
                 int ini_zero, double* h_x, int* iters, double* res_rel,
                 double* resid);
 int launch_spmv(Problem& P, const double* d_x, double* d_y);
+int bench_copy(long long nbytes, int reps, double* ms);
 int build_tiles(Problem& P);
 int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,
                  const int* d_diag_ptr, double* d_v);

@@ -409,28 +409,40 @@ __global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
   }
 }
 
-// Arnoldi step j, part 2: w -= sum_i c_i V_i over row pairs (16-byte loads)
-template <int NV>
+// Arnoldi step j, part 2: w -= sum_i c_i V_i over row pairs (16-byte loads),
+// per-block partial ||w||^2.  The vectors are read in groups of eight
+// independent loads; the subtraction order is i = 0, 1, ..., j.
 __global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
   if (A.stat->stop) return;
   __shared__ double red[1];
-  __shared__ double cs[NV];
+  __shared__ double cs[MAX_NV];
   if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
   __syncthreads();
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const long long np = (A.n + 1) / 2;  // row pairs (ldv is even)
   const long long p0 = lb * np / gridDim.x, p1 = (lb + 1) * np / gridDim.x;
+  const long long ldp = A.ldv / 2;
+  const double2* V2 = reinterpret_cast<const double2*>(A.V);
   double2* w = reinterpret_cast<double2*>(A.V + (size_t)(j + 1) * A.ldv);
   double acc = 0.0;
   for (long long p = p0 + threadIdx.x; p < p1; p += SB) {
     double2 wr = w[p];
+    int i = 0;
+    for (; i + 7 <= j; i += 8) {
+      double2 v[8];
 #pragma unroll
-    for (int i = 0; i < NV; i++)
-      if (i <= j) {
-        const double2 v = reinterpret_cast<const double2*>(A.V + (size_t)i * A.ldv)[p];
-        wr.x -= cs[i] * v.x;
-        wr.y -= cs[i] * v.y;
+      for (int q = 0; q < 8; q++) v[q] = V2[(size_t)(i + q) * ldp + p];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        wr.x -= cs[i + q] * v[q].x;
+        wr.y -= cs[i + q] * v[q].y;
       }
+    }
+    for (; i <= j; i++) {
+      const double2 v = V2[(size_t)i * ldp + p];
+      wr.x -= cs[i] * v.x;
+      wr.y -= cs[i] * v.y;
+    }
     if (2 * p + 1 >= A.n) wr.y = 0.0;  // padding lane of an odd n
     w[p] = wr;
     acc += wr.x * wr.x + wr.y * wr.y;
@@ -682,6 +694,42 @@ int launch_diag_ptr(Problem& P, int* d_dp) {
   return 0;
 }
 
+// plain stream copy (bandwidth yardstick, counter calibration)
+__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a, double2* __restrict__ b,
+                                              long long n2) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) b[i] = a[i];
+}
+
+int bench_copy(long long nbytes, int reps, double* ms) {
+  const long long n2 = nbytes / 16;
+  if (n2 <= 0 || reps < 1) {
+    set_error("mf_bench_copy: nbytes >= 16 and reps >= 1 required");
+    return -1;
+  }
+  double2 *a = nullptr, *b = nullptr;
+  MF_HIP(hipMalloc(&a, n2 * 16));
+  MF_HIP(hipMalloc(&b, n2 * 16));
+  MF_HIP(hipMemsetAsync(a, 0, n2 * 16, stream()));
+  const int G = (int)std::min<long long>((n2 + 255) / 256, num_cus() * 16LL);
+  hipEvent_t e0, e1;
+  MF_HIP(hipEventCreate(&e0));
+  MF_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_copy, dim3(G), dim3(256), 0, stream(), a, b, n2);
+  MF_HIP(hipEventRecord(e0, stream()));
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(G), dim3(256), 0, stream(), a, b, n2);
+  MF_HIP(hipEventRecord(e1, stream()));
+  MF_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+
 int launch_spmv(Problem& P, const double* d_x, double* d_y) {
   SolveArgs A;
   fill_solve_args(P, A, 0, 0);
@@ -730,7 +778,7 @@ static void launch_step(int G_op, int G_up, SolveArgs& A, int j) {
     e2 = ks_event(); e3 = ks_event();
     (void)hipEventRecord(e2, s);
   }
-  hipLaunchKernelGGL(k_update<NV>, dim3(G_up), dim3(SB), 0, s, A, j);
+  hipLaunchKernelGGL(k_update, dim3(G_up), dim3(SB), 0, s, A, j);
   if (g_ks.on) (void)hipEventRecord(e3, s);
   hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(SB), 0, s, A, j, G_up);
   if (g_ks.on) {
