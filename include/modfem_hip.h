@@ -137,9 +137,13 @@ int mf_fill_precon(int pid, int precon);
 /* restarted left-preconditioned GMRES(krylbas), lsr_mkb_core_gmres
    (lsd_mkb/lsd_mkb_core/lss_mkb_core.c:2317); precon 0 none, 1 Jacobi.
    Returns 1 on convergence, 0 otherwise, <0 on error.  x: host array of n
-   (input guess unless ini_zero, output solution). */
+   (input guess unless ini_zero, output solution), or NULL to start from zero
+   and keep the solution in device memory only (read it with
+   mf_get_solution). */
 int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol,
              int ini_zero, double* x, int* iters, double* res_rel, double* resid);
+/* host copy of the last mf_gmres solution (DOF order of mf_problem_get_dofs) */
+int mf_get_solution(int pid, double* x);
 /* true relative residual ||b - A x|| / ||b|| in FP64 on the device */
 int mf_true_residual(int pid, const double* x, double* rel);
 

@@ -99,6 +99,7 @@ def _declare(L):
         "mf_fill_precon": (I, [I, I]),
         "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
         "mf_true_residual": (I, [I, V, V]),
+        "mf_get_solution": (I, [I, V]),
         "mf_bench_integrate": (I, [I, I, I, V, V]),
         "mf_bench_spmv": (I, [I, I, V]),
         "mf_last_solve_ms": (I, [I, V]),
@@ -335,12 +336,24 @@ class Problem:
         check(lib().mf_spmv(self.h, _p(x), _p(y)))
         return y
 
-    def gmres(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, x0=None):
-        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+    def gmres(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, x0=None, fetch=True):
+        """fetch=False keeps the solution on the device (x is None; see
+        solution()) and starts from zero"""
+        if fetch:
+            x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+            xp = _p(x)
+        else:
+            x, xp = None, None
         it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         conv = check(lib().mf_gmres(self.h, precon, krylbas, max_iter, tol, int(x0 is None),
-                                    _p(x), ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs)))
+                                    xp, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs)))
         return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+    def solution(self):
+        """host copy of the last GMRES solution"""
+        x = np.zeros(self.n)
+        check(lib().mf_get_solution(self.h, _p(x)))
+        return x
 
     def true_residual(self, x):
         x = np.ascontiguousarray(x, np.float64)

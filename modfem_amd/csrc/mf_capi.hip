@@ -733,6 +733,20 @@ int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol, int ini
   return rc;
 }
 
+int mf_get_solution(int pid, double* x) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (!P->d_x || !x) {
+    set_error("mf_get_solution: no solution (run mf_gmres first) or x is null");
+    return -1;
+  }
+  std::vector<double> d(P->n);
+  MF_HIP(hipStreamSynchronize(stream()));
+  MF_HIP(hipMemcpy(d.data(), P->d_x, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+  to_reference_order(P->setup, d.data(), x);
+  return 0;
+}
+
 int mf_true_residual(int pid, const double* x, double* rel) {
   Problem* P = get_problem(pid);
   if (!P) return -3;

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -119,6 +120,11 @@ int lar_allocate_SM_and_LV(int Storage_type, int solver_type, int Nrdof_glob, in
   if (Block_size != 1 || Nrblocks != Nrdof_glob) {
     set_error("lar_allocate_SM_and_LV: block size must be 1");
     return -1;
+  }
+  // a ModFEM build calls no mf_init: open device MF_DEVICE (default 0) here
+  if (!stream()) {
+    const char* dev = std::getenv("MF_DEVICE");
+    if (device_init(dev ? std::atoi(dev) : 0)) return -2;
   }
   if ((int)g_mats.size() >= LAC_MAX_MATRICES) {
     set_error("lar_allocate_SM_and_LV: too many matrices");

@@ -191,6 +191,10 @@ def test_gmres_matches_oracle(precon):
     # iteration counts agree up to rounding-driven drift
     assert abs(info["iters"] - info0["iters"]) <= max(3, info0["iters"] // 20), (info, info0)
     assert rel_err(x, x0) <= 1e-6
+    # device-resident solve: same iterates, solution fetched afterwards
+    xn, info_n = p.gmres(precon=precon, krylbas=30, max_iter=3000, tol=1e-8, fetch=False)
+    assert xn is None and info_n["iters"] == info["iters"]
+    assert np.array_equal(p.solution(), x)
     p.free()
 
 
