@@ -142,6 +142,21 @@ int mf_fill_precon(int pid, int precon);
    mf_get_solution). */
 int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol,
              int ini_zero, double* x, int* iters, double* res_rel, double* resid);
+/* Solver control file (mkb.dat), lsr_mkb_core_init single-level branch
+   (lsd_mkb/lsd_mkb_core/lss_mkb_core.c:148-705).  params[13] receives
+   {solver, nr_level, krylbas, precon, nr_prec, block_type, storage_type,
+    ilu_k, max_iter, conv_type, monitor, nr_pre_smooth, nr_post_smooth};
+   arguments > 0 override the file as in the reference; an empty file name
+   gives the reference defaults (GMRES(50), block GS, 100 iterations,
+   REL_RES_INI 1e-6). */
+int mf_mkb_read(const char* filename, int max_iter, int error_type, double error_tol,
+                int monitoring, int* params, double* conv_meas);
+/* GMRES driven by mkb parameters, lsr_mkb_core_solve (lss_mkb_core.c:2200-2245):
+   conv_type 0 REL_RES_INI, 1 ABS_RES, 2 REL_RES_RHS (reset to 0 by the
+   reference, leaving the 1e-12 floors).  Available here: GMRES with no
+   preconditioner or one Jacobi sweep; other settings return -1. */
+int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, double* x,
+                 int* iters, double* res_rel, double* resid);
 /* host copy of the last mf_gmres solution (DOF order of mf_problem_get_dofs) */
 int mf_get_solution(int pid, double* x);
 /* true relative residual ||b - A x|| / ||b|| in FP64 on the device */

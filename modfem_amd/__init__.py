@@ -100,6 +100,8 @@ def _declare(L):
         "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
         "mf_true_residual": (I, [I, V, V]),
         "mf_get_solution": (I, [I, V]),
+        "mf_mkb_read": (I, [ctypes.c_char_p, I, I, ctypes.c_double, I, V, V]),
+        "mf_solve_mkb": (I, [I, V, ctypes.c_double, I, V, V, V, V]),
         "mf_bench_integrate": (I, [I, I, I, V, V]),
         "mf_bench_spmv": (I, [I, I, V]),
         "mf_last_solve_ms": (I, [I, V]),
@@ -166,6 +168,21 @@ def gmres_kernel_stats(enable):
     out = np.zeros(6)
     check(lib().mf_gmres_kernel_stats(int(enable), _p(out)))
     return {"op_dots": tuple(out[0:3]), "update": tuple(out[3:6])}
+
+
+MKB_KEYS = ("solver", "nr_level", "krylbas", "precon", "nr_prec", "block_type", "storage_type",
+            "ilu_k", "max_iter", "conv_type", "monitor", "nr_pre_smooth", "nr_post_smooth")
+
+
+def mkb_read(filename="", max_iter=-1, error_type=-1, error_tol=-1.0, monitoring=-1):
+    """solver control file -> dict (keys MKB_KEYS + conv_meas)"""
+    ip = np.zeros(13, np.int32)
+    cm = ctypes.c_double()
+    check(lib().mf_mkb_read(filename.encode(), max_iter, error_type, error_tol, monitoring,
+                            _p(ip), ctypes.byref(cm)))
+    d = dict(zip(MKB_KEYS, (int(v) for v in ip)))
+    d["conv_meas"] = cm.value
+    return d
 
 
 def bench_copy(nbytes, reps=5):
@@ -347,6 +364,16 @@ class Problem:
         it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         conv = check(lib().mf_gmres(self.h, precon, krylbas, max_iter, tol, int(x0 is None),
                                     xp, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs)))
+        return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+    def solve_mkb(self, params, fetch=True):
+        """GMRES as the solver control parameters (mkb_read) say"""
+        ip = np.array([params[k] for k in MKB_KEYS], np.int32)
+        x = np.zeros(self.n) if fetch else None
+        it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        conv = check(lib().mf_solve_mkb(self.h, _p(ip), params["conv_meas"], 1,
+                                        _p(x) if fetch else None, ctypes.byref(it),
+                                        ctypes.byref(rr), ctypes.byref(rs)))
         return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
 
     def solution(self):

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmodfem_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
-HOST_SRCS = ["mf_mesh.cpp", "mf_setup.cpp"]
+HOST_SRCS = ["mf_mesh.cpp", "mf_setup.cpp", "mf_mkb.cpp"]
 HIP_SRCS = ["mf_device.hip", "mf_solver.hip", "mf_capi.hip", "mf_lar.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -726,9 +726,65 @@ int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol, int ini
   }
   std::vector<double> xd;
   if (x) xd = ini_zero ? std::vector<double>(P->n, 0.0) : to_device_order(P->setup, x);
-  int rc = gmres_solve(*P, precon, krylbas, max_iter, tol, ini_zero, x ? xd.data() : nullptr,
+  int rc = gmres_solve(*P, precon, krylbas, max_iter, tol, 0.0, ini_zero, x ? xd.data() : nullptr,
                        iters, res_rel, resid);
   // rc: 1 converged, 0 not converged, <0 error
+  if (rc >= 0 && x) to_reference_order(P->setup, xd.data(), x);
+  return rc;
+}
+
+int mf_mkb_read(const char* filename, int max_iter, int error_type, double error_tol,
+                int monitoring, int* params, double* conv_meas) {
+  if (!params || !conv_meas) {
+    set_error("mf_mkb_read: params and conv_meas are required");
+    return -1;
+  }
+  MkbParams p;
+  if (mkb_read(filename, max_iter, error_type, error_tol, monitoring, p)) return -1;
+  const int v[13] = {p.solver, p.nr_level, p.krylbas, p.precon, p.nr_prec, p.block_type,
+                     p.storage_type, p.ilu_k, p.max_iter, p.conv_type, p.monitor,
+                     p.nr_pre_smooth, p.nr_post_smooth};
+  std::memcpy(params, v, sizeof(v));
+  *conv_meas = p.conv_meas;
+  return 0;
+}
+
+int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, double* x,
+                 int* iters, double* res_rel, double* resid) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (!params) {
+    set_error("mf_solve_mkb: params is null");
+    return -1;
+  }
+  const int solver = params[0], krylbas = params[2], precon = params[3], nr_prec = params[4];
+  const int storage = params[6], max_iter = params[8], conv_type = params[9];
+  if (solver != 1) {
+    set_error("mf_solve_mkb: only single-level GMRES (solver 1) is available");
+    return -1;
+  }
+  if (precon != 0 && !(precon == 1 && nr_prec == 1)) {
+    set_error("mf_solve_mkb: available preconditioners are none (0) and one Jacobi sweep (1 1)");
+    return -1;
+  }
+  if (storage != -1 && storage != 1 && storage != 2) {
+    set_error("mf_solve_mkb: storage must be undefined, block (size 1) or CRS");
+    return -1;
+  }
+  if (conv_type < 0 || conv_type > 2) {
+    set_error("mf_solve_mkb: conv_type must be 0, 1 or 2");
+    return -1;
+  }
+  const double tol_rel = conv_type == 0 ? conv_meas : 0.0;
+  const double tol_abs = conv_type == 1 ? conv_meas : 0.0;
+  if (max_iter < 1) {
+    set_error("mf_solve_mkb: max_iter >= 1 required");
+    return -1;
+  }
+  std::vector<double> xd;
+  if (x) xd = ini_zero ? std::vector<double>(P->n, 0.0) : to_device_order(P->setup, x);
+  int rc = gmres_solve(*P, precon, krylbas, max_iter, tol_rel, tol_abs, ini_zero,
+                       x ? xd.data() : nullptr, iters, res_rel, resid);
   if (rc >= 0 && x) to_reference_order(P->setup, xd.data(), x);
   return rc;
 }

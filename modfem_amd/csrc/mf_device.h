@@ -132,8 +132,9 @@ int launch_assemble_rows(Problem& P, int precision);
 int launch_assemble_two_pass(Problem& P, int precision);
 int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount);
 int launch_fill_jacobi(Problem& P);
+// tol: relative to the initial preconditioned residual; tol_abs: absolute
 int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,
-                int ini_zero, double* h_x, int* iters, double* res_rel,
+                double tol_abs, int ini_zero, double* h_x, int* iters, double* res_rel,
                 double* resid);
 int launch_spmv(Problem& P, const double* d_x, double* d_y);
 int bench_copy(long long nbytes, int reps, double* ms);

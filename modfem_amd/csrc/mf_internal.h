@@ -98,6 +98,18 @@ int build_sell(Setup& s);
 int build_device_order(const Mesh* m, Setup& s);
 
 int build_setup(const Mesh& m, Setup& s);
+
+// single-level solver control (lsd_mkb/lsd_mkb_core/lsh_mkb_core.h:114-153)
+struct MkbParams {
+  int solver = 0, nr_level = 0, krylbas = 0, precon = 0, nr_prec = 0, block_type = 0;
+  int storage_type = -1, ilu_k = 0, max_iter = 0, conv_type = 0, monitor = 0;
+  int nr_pre_smooth = 0, nr_post_smooth = 0;
+  double conv_meas = 0.0;
+};
+// mkb.dat reader (lss_mkb_core.c:148-705, single-level branch); arguments
+// > 0 override the file, as in the reference
+int mkb_read(const char* filename, int max_iter, int error_type, double error_tol,
+             int monitoring, MkbParams& p);
 // full reference-format table for int-ent ie (elements then faces)
 int fill_reference_table(const Mesh& m, const Setup& s, int ie, int* table);
 

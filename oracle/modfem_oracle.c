@@ -692,6 +692,14 @@ int orc_gmres(int n, const int *row_ptr, const int *col_ind, const double *val,
               const double *b, int precon, int krylbas, int max_iter,
               double tol_rel, int ini_zero, double *x, int *iters,
               double *res_rel_out, double *res_out) {
+  return orc_gmres_tol(n, row_ptr, col_ind, val, b, precon, krylbas, max_iter, tol_rel, 0.0,
+                       ini_zero, x, iters, res_rel_out, res_out);
+}
+
+int orc_gmres_tol(int n, const int *row_ptr, const int *col_ind, const double *val,
+                  const double *b, int precon, int krylbas, int max_iter,
+                  double tol_rel, double tol_abs, int ini_zero, double *x, int *iters,
+                  double *res_rel_out, double *res_out) {
   const double SMALL = 1.0e-15;
   const double MAX_CONVRATE = 0.999;
   orc_op op = {n, row_ptr, col_ind, val, b, precon, NULL, NULL};
@@ -704,7 +712,8 @@ int orc_gmres(int n, const int *row_ptr, const int *col_ind, const double *val,
   double *v = (double *)calloc((size_t)n * (m + 1), sizeof(double));
   double *h = (double *)calloc((size_t)(m + 1) * (m + 1), sizeof(double));
   double *giv = (double *)calloc((size_t)2 * (m + 1), sizeof(double));
-  double tol_res = 1.0e-12; /* REL_RES_INI passes 0, floored (:2377-2378) */
+  /* both tolerances floored at 1e-12 (:2377-2378) */
+  double tol_res = tol_abs < 1.0e-12 ? 1.0e-12 : tol_abs;
   if (tol_rel < 1.0e-12) tol_rel = 1.0e-12;
   int restrt = max_iter / m + 1, restit = 0, gmiter = 0, conv = 0;
   double beta0 = 0, beta1 = 0, rnrm2, bnrm2, convrate = 0, faux = 0;

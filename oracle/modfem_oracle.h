@@ -115,6 +115,13 @@ int orc_gmres(int n, const int *row_ptr, const int *col_ind, const double *val,
               double tol_rel, int ini_zero, double *x, int *iters,
               double *res_rel_out, double *res_out);
 
+/* the same with an absolute tolerance on the preconditioned residual
+   (ABS_RES; lss_mkb_core.c:2220-2225 passes one of the two, the other 0) */
+int orc_gmres_tol(int n, const int *row_ptr, const int *col_ind, const double *val,
+                  const double *b, int precon, int krylbas, int max_iter,
+                  double tol_rel, double tol_abs, int ini_zero, double *x,
+                  int *iters, double *res_rel_out, double *res_out);
+
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);
 

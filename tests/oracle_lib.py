@@ -41,6 +41,7 @@ def oracle():
             "orc_compute_residual": (None, [I, V, V, V, I, V, V, V]),
             "orc_bj_iteration": (None, [I, V, V, V, V, V, I, V, V]),
             "orc_gmres": (I, [I, V, V, V, V, I, I, I, D, I, V, V, V, V]),
+            "orc_gmres_tol": (I, [I, V, V, V, V, I, I, I, D, D, I, V, V, V, V]),
             "orc_drotg": (None, [V, V, V, V]),
             "orc_set_threads": (None, [I]),
         }
@@ -136,13 +137,13 @@ class OracleProblem:
         o.orc_face_penalty(int(self.et[e]), int(self.bfs[f]), P(nc), g, src_model, P(sm), P(rhs))
         return sm[: k * k].reshape(k, k), rhs[:k]
 
-    def gmres(self, val, rhs, precon=1, krylbas=30, max_iter=1000, tol=1e-8):
+    def gmres(self, val, rhs, precon=1, krylbas=30, max_iter=1000, tol=1e-8, tol_abs=0.0):
         o = oracle()
         x = np.zeros(self.n)
         it, rr, rs = I(), D(), D()
-        conv = o.orc_gmres(self.n, P(self.row_ptr), P(self.col_ind), P(val), P(rhs), precon,
-                           krylbas, max_iter, tol, 1, P(x), ctypes.byref(it), ctypes.byref(rr),
-                           ctypes.byref(rs))
+        conv = o.orc_gmres_tol(self.n, P(self.row_ptr), P(self.col_ind), P(val), P(rhs), precon,
+                               krylbas, max_iter, tol, tol_abs, 1, P(x), ctypes.byref(it),
+                               ctypes.byref(rr), ctypes.byref(rs))
         return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
 
 

@@ -312,3 +312,34 @@ def test_assembly_modes_agree_and_rowwise_is_reproducible(kind, prism_mesh, tetr
     v5, r5 = p.system()
     assert rel_err(v5, v0) <= 2e-6 and rel_err(r5, r0) <= 2e-6
     p.free()
+
+
+@pytest.mark.parametrize("conv_type", [0, 1])
+def test_solve_with_reference_control_file(conv_type):
+    """GMRES driven by the reference's own Jacobi control file
+    (ctest Laplace_std_in_cube/mkb.dat): REL_RES_INI as written, and ABS_RES
+    through the override argument, against the oracle with the same
+    tolerances (lss_mkb_core.c:2220-2225, 2377-2378)"""
+    G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mkb")
+    tol = 1e-9 if conv_type == 0 else 1e-7
+    prm = mf.mkb_read(os.path.join(G, "laplace_std_in_cube.dat"), -1, conv_type, tol, -1)
+    assert prm["conv_type"] == conv_type and prm["precon"] == 1
+    mesh = mf.Mesh.box_prism(8, 7, 6, seed=31, jitter=0.2)
+    ind, model, cc, ce, tab = make_case(mesh, "convdiff_kappa_b", seed=5)
+    p = mf.Problem.from_mesh(mesh)
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    p.integrate_assemble(64)
+    x, info = p.solve_mkb(prm)
+    op = OracleProblem(mesh.arrays())
+    v0, r0 = op.assemble(ind, tab)
+    x0, info0 = op.gmres(v0, r0, precon=1, krylbas=prm["krylbas"], max_iter=prm["max_iter"],
+                         tol=tol if conv_type == 0 else 0.0, tol_abs=tol if conv_type == 1 else 0.0)
+    assert info0["converged"] and info["converged"]
+    assert abs(info["iters"] - info0["iters"]) <= max(3, info0["iters"] // 20), (info, info0)
+    if conv_type == 1:
+        assert info["resid"] < tol
+    assert rel_err(x, x0) <= 1e-5
+    prm_gs = dict(prm, precon=2)
+    with pytest.raises(RuntimeError):
+        p.solve_mkb(prm_gs)
+    p.free()
