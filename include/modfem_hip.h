@@ -157,6 +157,16 @@ int mf_mkb_read(const char* filename, int max_iter, int error_type, double error
    preconditioner or one Jacobi sweep; other settings return -1. */
 int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, double* x,
                  int* iters, double* res_rel, double* resid);
+/* Mixed-precision solve (no reference counterpart; the reference is FP64
+   throughout): FP64 iterative refinement r = b - A x, x += d, around
+   GMRES(krylbas) whose operator reads FP32 matrix values while the Krylov
+   basis, Gram-Schmidt, Givens rotations, corrections and residuals stay FP64.
+   Stops when ||D^-1 (b - A x)|| <= tol ||D^-1 b|| in FP64 (D = I when precon
+   is 0): the measure of mf_gmres from a zero guess.  iters = total inner
+   iterations, refinements = outer steps; x as in mf_gmres (NULL keeps the
+   solution on the device). */
+int mf_gmres_mixed(int pid, int precon, int krylbas, int max_iter, double tol, double* x,
+                   int* iters, double* res_rel, double* resid, int* refinements);
 /* host copy of the last mf_gmres solution (DOF order of mf_problem_get_dofs) */
 int mf_get_solution(int pid, double* x);
 /* true relative residual ||b - A x|| / ||b|| in FP64 on the device */

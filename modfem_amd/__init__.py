@@ -100,6 +100,7 @@ def _declare(L):
         "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
         "mf_true_residual": (I, [I, V, V]),
         "mf_get_solution": (I, [I, V]),
+        "mf_gmres_mixed": (I, [I, I, I, I, ctypes.c_double, V, V, V, V, V]),
         "mf_mkb_read": (I, [ctypes.c_char_p, I, I, ctypes.c_double, I, V, V]),
         "mf_solve_mkb": (I, [I, V, ctypes.c_double, I, V, V, V, V]),
         "mf_bench_integrate": (I, [I, I, I, V, V]),
@@ -365,6 +366,16 @@ class Problem:
         conv = check(lib().mf_gmres(self.h, precon, krylbas, max_iter, tol, int(x0 is None),
                                     xp, ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs)))
         return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+    def gmres_mixed(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, fetch=True):
+        """FP64 refinement around GMRES with an FP32-valued operator"""
+        x = np.zeros(self.n) if fetch else None
+        it, rr, rs, nref = ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        conv = check(lib().mf_gmres_mixed(self.h, precon, krylbas, max_iter, tol,
+                                          _p(x) if fetch else None, ctypes.byref(it),
+                                          ctypes.byref(rr), ctypes.byref(rs), ctypes.byref(nref)))
+        return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value,
+                       refinements=nref.value)
 
     def solve_mkb(self, params, fetch=True):
         """GMRES as the solver control parameters (mkb_read) say"""

@@ -789,6 +789,25 @@ int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, dou
   return rc;
 }
 
+int mf_gmres_mixed(int pid, int precon, int krylbas, int max_iter, double tol, double* x,
+                   int* iters, double* res_rel, double* resid, int* refinements) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (precon != 0 && precon != 1) {
+    set_error("mf_gmres_mixed: precon must be 0 (none) or 1 (Jacobi)");
+    return -1;
+  }
+  if (tol <= 0) {
+    set_error("mf_gmres_mixed: tol > 0 required");
+    return -1;
+  }
+  std::vector<double> xd(x ? P->n : 0);
+  int rc = gmres_mixed(*P, precon, krylbas, max_iter, tol, x ? xd.data() : nullptr, iters,
+                       res_rel, resid, refinements);
+  if (rc >= 0 && x) to_reference_order(P->setup, xd.data(), x);
+  return rc;
+}
+
 int mf_get_solution(int pid, double* x) {
   Problem* P = get_problem(pid);
   if (!P) return -3;

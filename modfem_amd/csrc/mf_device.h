@@ -136,6 +136,9 @@ int launch_fill_jacobi(Problem& P);
 int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,
                 double tol_abs, int ini_zero, double* h_x, int* iters, double* res_rel,
                 double* resid);
+// FP64 refinement around GMRES with an FP32-valued operator
+int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, double* h_x,
+                int* iters, double* res_rel, double* resid, int* refinements);
 int launch_spmv(Problem& P, const double* d_x, double* d_y);
 int bench_copy(long long nbytes, int reps, double* ms);
 int build_tiles(Problem& P);

@@ -155,6 +155,8 @@ struct SolveArgs {
   const int* row_ptr;
   const int* col;
   const double* val;    // CRS values (tile path) or SELL-64 values
+  const float* val32;   // SELL-64 values rounded to FP32 (mixed-precision inner solves)
+  int op32;             // 1: the operator kernels read val32
   const long long* sl_ptr;  // SELL-64 path
   const int* scol;
   const double* dinv;   // null: no preconditioner
@@ -240,7 +242,9 @@ __device__ __forceinline__ SliceSet block_slices(int nsl) {
 // are loaded first (up to 24 entries in flight per lane), then the vector
 // gathers are issued together, so a slice costs about two memory round trips
 // instead of one per entry
-__device__ __forceinline__ double sell_row(const SolveArgs& A, const double* x, int sl, int lane) {
+template <typename VT>
+__device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, const double* x,
+                                           int sl, int lane) {
   constexpr int CH = 12;
   const long long base = A.sl_ptr[sl] + lane;
   const int len = (int)((A.sl_ptr[sl + 1] - A.sl_ptr[sl]) >> 6);
@@ -253,7 +257,7 @@ __device__ __forceinline__ double sell_row(const SolveArgs& A, const double* x, 
       const bool ok = k0 + k < len;
       const long long q = base + 64LL * (ok ? k0 + k : k0);
       c[k] = A.scol[q];
-      v[k] = ok ? A.val[q] : 0.0;
+      v[k] = ok ? (double)val[q] : 0.0;
     }
     double xv[CH];
 #pragma unroll
@@ -262,6 +266,11 @@ __device__ __forceinline__ double sell_row(const SolveArgs& A, const double* x, 
     for (int k = 0; k < CH; k++) s += v[k] * xv[k];
   }
   return s;
+}
+
+// row sum with the FP64 or the FP32 copy of the values (accumulated in FP64)
+__device__ __forceinline__ double sell_row_op(const SolveArgs& A, const double* x, int sl, int lane) {
+  return A.op32 ? sell_row(A, A.val32, x, sl, lane) : sell_row(A, A.val, x, sl, lane);
 }
 
 __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
@@ -273,7 +282,7 @@ __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double acc = 0.0;
   for (int sl = s0 + wv; sl < s1; sl += SW) {
-    const double s = sell_row(A, A.x, sl, lane);
+    const double s = sell_row_op(A, A.x, sl, lane);
     const int r = sl * 64 + lane;
     if (r < A.n) {
       double v = A.b[r] - s;
@@ -303,7 +312,7 @@ __global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
   const double sj = Scale(A)[j];
   for (int t = wv; t < ns; t += SW) {
     const int sl = S.slice(t);
-    const double s = sell_row(A, x, sl, lane);
+    const double s = sell_row_op(A, x, sl, lane);
     const int r = sl * 64 + lane;
     double wr = 0.0;
     if (r < A.n) {
@@ -338,10 +347,53 @@ __global__ void __launch_bounds__(SB) k_spmv_sell(SolveArgs A, const double* x, 
   slice_range(nsl, lb, s0, s1);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int sl = s0 + wv; sl < s1; sl += SW) {
-    const double s = sell_row(A, x, sl, lane);
+    const double s = sell_row(A, A.val, x, sl, lane);
     const int r = sl * 64 + lane;
     if (r < A.n) y[r] = s;
   }
+}
+
+// r = b - A x in FP64 and per-block partial ||D^-1 r||^2 (refinement
+// residual of the mixed-precision solve)
+__global__ void __launch_bounds__(SB) k_refine_residual(SolveArgs A, const double* b,
+                                                        const double* x, double* r) {
+  __shared__ double red[1];
+  const int nsl = (A.n + 63) >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int s0, s1;
+  slice_range(nsl, lb, s0, s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int sl = s0 + wv; sl < s1; sl += SW) {
+    const double s = sell_row(A, A.val, x, sl, lane);
+    const int i = sl * 64 + lane;
+    if (i < A.n) {
+      const double v = b[i] - s;
+      r[i] = v;
+      const double pv = A.dinv ? v * A.dinv[i] : v;
+      acc += pv * pv;
+    }
+  }
+  block_sum_multi<1>(&acc, 1, red);
+  if (threadIdx.x == 0) A.partials[lb] = red[0];
+}
+
+__global__ void __launch_bounds__(SB) k_sum_partials(const double* partials, int G, double* out) {
+  __shared__ double red[1];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += SB) s += partials[b];
+  block_sum_multi<1>(&s, 1, red);
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void k_axpy(long long n, const double* d, double* x) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+__global__ void k_round_f32(long long n, const double* a, float* b) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 __global__ void k_fill_jacobi_sell(int n, const long long* sl_ptr, const uint8_t* diag_k,
@@ -645,6 +697,7 @@ static void fill_solve_args(Problem& P, SolveArgs& A, int precon, int m) {
   A.n = P.n; A.ntiles = P.ntiles; A.m = m; A.ldv = ldv_of(P.n);
   A.tile_row = P.d_tile_row; A.row_ptr = P.d_row_ptr; A.col = P.d_col; A.val = P.d_val;
   A.sl_ptr = P.d_sl_ptr; A.scol = P.d_scol;
+  A.val32 = P.d_val32; A.op32 = 0;
   A.dinv = precon ? P.d_dinv : nullptr;
   A.b = P.d_rhs; A.x = P.d_x; A.V = P.d_V; A.H = P.d_H; A.partials = P.d_partials;
   A.stat = P.d_stat; A.hstat = P.h_stat;
@@ -844,9 +897,12 @@ static void launch_arnoldi(int G_op, int G_up, SolveArgs& A, int j) {
 // rotations, following lsr_mkb_core_gmres (lss_mkb_core.c:2317-2668): same
 // restart count, tolerance floors, stopping tests on the preconditioned
 // residual relative to the initial one, and convergence-rate abandonment.
-int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, double tol_abs,
-                int ini_zero,
-                double* h_x, int* iters, double* res_rel_out, double* resid_out) {
+// d_b / d_x: right-hand side and iterate in device memory; op32: the
+// operator reads the FP32 copy of the values (Gram-Schmidt, Givens and the
+// iterate stay FP64)
+static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double tol,
+                      double tol_abs, int ini_zero, double* h_x, const double* d_b, double* d_x,
+                      int op32, int* iters, double* res_rel_out, double* resid_out) {
   const int m = krylbas;
   if (m < 1 || m >= MAX_NV) {
     set_error("GMRES: number of Krylov vectors must be in [1,63]");
@@ -858,6 +914,13 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   }
   SolveArgs A;
   fill_solve_args(P, A, precon, m);
+  A.b = d_b;
+  A.x = d_x;
+  A.op32 = op32;
+  if (op32 && (!P.sell || !P.d_val32)) {
+    set_error("GMRES: the FP32 operator needs SELL storage and its FP32 values");
+    return -1;
+  }
   const int G_op = P.sell ? grid_slices(P) : grid_tiles(P), G_up = grid_rows(P);
   if (!P.sell) A.sl_ptr = nullptr;
   hipStream_t s = stream();
@@ -866,9 +929,9 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   MF_HIP(hipEventCreate(&e1));
   MF_HIP(hipEventRecord(e0, s));
   if (ini_zero || !h_x) {
-    MF_HIP(hipMemsetAsync(P.d_x, 0, (size_t)P.n * sizeof(double), s));
+    MF_HIP(hipMemsetAsync(d_x, 0, (size_t)P.n * sizeof(double), s));
   } else {
-    MF_HIP(hipMemcpyAsync(P.d_x, h_x, (size_t)P.n * sizeof(double), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(d_x, h_x, (size_t)P.n * sizeof(double), hipMemcpyHostToDevice, s));
   }
   const double SMALL = 1.0e-15, MAX_CONVRATE = 0.999;
   // both tolerances floored at 1e-12 (lss_mkb_core.c:2377-2378)
@@ -881,7 +944,7 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   double beta0 = 0, beta1 = 0, convrate = 0, faux = 0, resid = 0, res_rel = 0;
   std::vector<double> H((size_t)(m + 1) * (m + 1) + 4 * (m + 1));
   const double op_bytes_base =
-      12.0 * (double)P.nnz + 4.0 * (P.n + 1) + 24.0 * (double)P.n;
+      (op32 ? 8.0 : 12.0) * (double)P.nnz + 4.0 * (P.n + 1) + 24.0 * (double)P.n;
   do {
     st.gmiter = gmiter;
     st.stop = 0;
@@ -972,10 +1035,101 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   P.t_solve_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (h_x) MF_HIP(hipMemcpy(h_x, P.d_x, (size_t)P.n * sizeof(double), hipMemcpyDeviceToHost));
+  if (h_x) MF_HIP(hipMemcpy(h_x, d_x, (size_t)P.n * sizeof(double), hipMemcpyDeviceToHost));
   if (iters) *iters = gmiter;
   if (res_rel_out) *res_rel_out = res_rel;
   if (resid_out) *resid_out = resid;
+  return conv;
+}
+
+int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, double tol_abs,
+                int ini_zero, double* h_x, int* iters, double* res_rel_out, double* resid_out) {
+  if (!P.d_x) MF_HIP(hipMalloc(&P.d_x, (size_t)P.n * sizeof(double)));
+  return gmres_core(P, precon, krylbas, max_iter, tol, tol_abs, ini_zero, h_x, P.d_rhs, P.d_x, 0,
+                    iters, res_rel_out, resid_out);
+}
+
+// Mixed-precision solve: FP64 iterative refinement around GMRES(m) whose
+// operator reads FP32 matrix values (Gram-Schmidt, Givens rotations, Krylov
+// basis, corrections and residuals FP64).  Step k: r_k = b - A x_k and
+// rho_k = ||D^-1 r_k|| in FP64; stop when rho_k <= tol * rho_0 (the same
+// preconditioned relative measure as the plain solve from x_0 = 0);
+// otherwise solve A d = r_k with the FP32 operator to the relative tolerance
+// max(tol rho_0 / rho_k, 1e-6) and set x_{k+1} = x_k + d.
+int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, double* h_x,
+                int* iters, double* res_rel_out, double* resid_out, int* refinements) {
+  if (!P.sell) {
+    set_error("mixed-precision GMRES needs a SELL problem");
+    return -1;
+  }
+  if (krylbas < 1 || krylbas >= MAX_NV || max_iter < 1) {
+    set_error("GMRES: number of Krylov vectors must be in [1,63], max_iter >= 1");
+    return -1;
+  }
+  if (ensure_workspace(P, krylbas)) return -2;
+  if (precon && !P.d_dinv && launch_fill_jacobi(P)) return -2;
+  hipStream_t s = stream();
+  const long long n = P.n;
+  if (!P.d_val32) MF_HIP(hipMalloc(&P.d_val32, (size_t)P.nnz_sell * sizeof(float)));
+  hipLaunchKernelGGL(k_round_f32, dim3((unsigned)((P.nnz_sell + 255) / 256)), dim3(256), 0, s,
+                     P.nnz_sell, P.d_val, P.d_val32);
+  if (!P.d_work) MF_HIP(hipMalloc(&P.d_work, (size_t)n * sizeof(double)));
+  double *d_r = P.d_work, *d_d = nullptr, *d_sum = nullptr;
+  MF_HIP(hipMalloc(&d_d, (size_t)n * sizeof(double)));
+  MF_HIP(hipMalloc(&d_sum, sizeof(double)));
+  MF_HIP(hipMemsetAsync(P.d_x, 0, (size_t)n * sizeof(double), s));
+  SolveArgs A;
+  fill_solve_args(P, A, precon, krylbas);
+  const int G = grid_slices(P);
+  const int gx = (int)((n + 255) / 256);
+  int total = 0, conv = 0, k = 0, rc = 0;
+  double rho0 = 0.0, rho = 0.0;
+  hipEvent_t e0, e1;
+  MF_HIP(hipEventCreate(&e0));
+  MF_HIP(hipEventCreate(&e1));
+  MF_HIP(hipEventRecord(e0, s));
+  for (k = 0; k < 30; k++) {
+    hipLaunchKernelGGL(k_refine_residual, dim3(G), dim3(SB), 0, s, A, P.d_rhs, P.d_x, d_r);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(SB), 0, s, P.d_partials, G, d_sum);
+    double h_sum = 0.0;
+    MF_HIP(hipMemcpyAsync(&h_sum, d_sum, sizeof(double), hipMemcpyDeviceToHost, s));
+    MF_HIP(hipStreamSynchronize(s));
+    rho = std::sqrt(h_sum);
+    if (k == 0) rho0 = rho;
+    if (!(rho == rho)) {
+      set_error("mixed GMRES: residual is not a number");
+      rc = -4;
+      break;
+    }
+    if (rho <= tol * rho0 || rho < 1e-300 || total >= max_iter) {
+      conv = rho <= tol * rho0 || rho < 1e-300;
+      break;
+    }
+    const double tol_in = std::max(tol * rho0 / rho, 1e-6);
+    int it = 0;
+    double rr = 0, rs = 0;
+    rc = gmres_core(P, precon, krylbas, max_iter - total, tol_in, 0.0, 1, nullptr, d_r, d_d, 1,
+                    &it, &rr, &rs);
+    if (rc < 0) break;
+    total += it;
+    hipLaunchKernelGGL(k_axpy, dim3(gx), dim3(256), 0, s, n, d_d, P.d_x);
+    MF_HIP(hipGetLastError());
+  }
+  MF_HIP(hipEventRecord(e1, s));
+  MF_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  P.t_solve_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_d);
+  (void)hipFree(d_sum);
+  if (rc < 0) return rc;
+  if (h_x) MF_HIP(hipMemcpy(h_x, P.d_x, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  if (iters) *iters = total;
+  if (res_rel_out) *res_rel_out = rho0 > 0 ? rho / rho0 : 0.0;
+  if (resid_out) *resid_out = rho;
+  if (refinements) *refinements = k;
   return conv;
 }
 

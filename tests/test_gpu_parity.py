@@ -343,3 +343,32 @@ def test_solve_with_reference_control_file(conv_type):
     with pytest.raises(RuntimeError):
         p.solve_mkb(prm_gs)
     p.free()
+
+
+@pytest.mark.parametrize("precon", [1])
+def test_mixed_precision_gmres(precon):
+    """FP64 refinement around the FP32-valued operator reaches the plain
+    solve's stopping measure, checked independently in FP64 on the host:
+    ||D^-1 (b - A x)|| <= tol ||D^-1 b|| with the oracle-assembled system"""
+    import scipy.sparse as sp
+    mesh = mf.Mesh.box_prism(12, 11, 10, seed=41, jitter=0.2)
+    ind, model, cc, ce, tab = make_case(mesh, "convdiff_kappa_b", seed=13)
+    p = mf.Problem.from_mesh(mesh)
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    p.integrate_assemble(64)
+    tol = 1e-8
+    x, info = p.gmres_mixed(precon=precon, krylbas=30, max_iter=5000, tol=tol)
+    assert info["converged"] and info["refinements"] >= 1, info
+    op = OracleProblem(mesh.arrays())
+    v0, r0 = op.assemble(ind, tab)
+    A = sp.csr_matrix((v0, op.col_ind, op.row_ptr), shape=(op.n, op.n))
+    d = 1.0 / A.diagonal()
+    rel = np.linalg.norm(d * (r0 - A @ x)) / np.linalg.norm(d * r0)
+    assert rel <= tol * 1.0001, (rel, info)
+    assert abs(rel - info["res_rel"]) <= 1e-3 * tol
+    xp, infop = p.gmres(precon=precon, krylbas=30, max_iter=5000, tol=tol)
+    # both solutions satisfy the same 1e-8 preconditioned-residual test
+    assert rel_err(x, xp) <= 1e-5
+    xn, info_n = p.gmres_mixed(precon=precon, krylbas=30, max_iter=5000, tol=tol, fetch=False)
+    assert xn is None and np.array_equal(p.solution(), x)
+    p.free()
