@@ -190,6 +190,28 @@ int mf_gmres_kernel_stats(int enable, double* out);
 
 int mf_problem_free(int pid);
 
+/* ------------------------------------------------------------- batches
+ * Many independent problems solved together (no reference counterpart: the
+ * reference solves one system at a time).  Members keep their own device
+ * matrices, vectors and Krylov bases; every kernel launch of a restart cycle
+ * serves all members, and the restart logic of lsr_mkb_core_gmres runs on
+ * the device, so a cycle can be replayed as one hipGraph (use_graph).
+ * Members may join or leave between cycles (continuous batching); solutions
+ * stay in device memory (mf_get_solution).  A member must not be freed, and
+ * should not be re-assembled, while it is in a batch. */
+int mf_batch_create(int krylbas);
+/* start a member: zero initial guess, GMRES(krylbas) with precon 0/1 to the
+   REL_RES_INI tolerance tol within max_iter iterations */
+int mf_batch_add(int bid, int pid, int precon, int max_iter, double tol);
+int mf_batch_remove(int bid, int pid);
+/* integrate + assemble every member (mf_integrate_assemble of each) */
+int mf_batch_integrate_assemble(int bid, int precision);
+/* run up to max_cycles restart cycles; *active = members still iterating */
+int mf_batch_run(int bid, int max_cycles, int use_graph, int* active);
+int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters,
+                    double* res_rel);
+int mf_batch_free(int bid);
+
 /* host copy of a storage-seam matrix (lar_get_SM_and_LV_crs_from_crs with
    offset 0, lsd_mkb/lad_crs/las_crs_intf.c:1063) */
 int mf_lar_get_system(int matrix_id, int* row_ptr, int* col_ind, double* val,

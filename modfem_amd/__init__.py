@@ -100,6 +100,13 @@ def _declare(L):
         "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
         "mf_true_residual": (I, [I, V, V]),
         "mf_get_solution": (I, [I, V]),
+        "mf_batch_create": (I, [I]),
+        "mf_batch_add": (I, [I, I, I, I, ctypes.c_double]),
+        "mf_batch_remove": (I, [I, I]),
+        "mf_batch_integrate_assemble": (I, [I, I]),
+        "mf_batch_run": (I, [I, I, I, V]),
+        "mf_batch_status": (I, [I, I, V, V, V, V]),
+        "mf_batch_free": (I, [I]),
         "mf_gmres_mixed": (I, [I, I, I, I, ctypes.c_double, V, V, V, V, V]),
         "mf_mkb_read": (I, [ctypes.c_char_p, I, I, ctypes.c_double, I, V, V]),
         "mf_solve_mkb": (I, [I, V, ctypes.c_double, I, V, V, V, V]),
@@ -192,6 +199,40 @@ def bench_copy(nbytes, reps=5):
     ms = ctypes.c_double()
     check(lib().mf_bench_copy(int(nbytes), reps, ctypes.byref(ms)))
     return ms.value
+
+
+class Batch:
+    """Batched GMRES over resident problems (continuous batching)."""
+
+    def __init__(self, krylbas=30):
+        init()
+        self.h = check(lib().mf_batch_create(krylbas))
+
+    def add(self, problem, precon=1, max_iter=1000, tol=1e-8):
+        return check(lib().mf_batch_add(self.h, problem.h, precon, max_iter, tol))
+
+    def remove(self, problem):
+        check(lib().mf_batch_remove(self.h, problem.h))
+
+    def integrate_assemble(self, precision=64):
+        check(lib().mf_batch_integrate_assemble(self.h, precision))
+
+    def run(self, max_cycles=1000000, use_graph=True):
+        """run restart cycles; returns the number of members still iterating"""
+        a = ctypes.c_int()
+        check(lib().mf_batch_run(self.h, max_cycles, int(use_graph), ctypes.byref(a)))
+        return a.value
+
+    def status(self, problem):
+        d, c, it, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib().mf_batch_status(self.h, problem.h, ctypes.byref(d), ctypes.byref(c),
+                                    ctypes.byref(it), ctypes.byref(rr)))
+        return dict(done=bool(d.value), converged=bool(c.value), iters=it.value, res_rel=rr.value)
+
+    def free(self):
+        if self.h:
+            lib().mf_batch_free(self.h)
+            self.h = 0
 
 
 class Mesh:

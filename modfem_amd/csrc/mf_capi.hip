@@ -18,7 +18,8 @@ namespace {
 std::map<int, std::unique_ptr<Mesh>> g_meshes;
 std::map<int, std::unique_ptr<Problem>> g_problems;
 std::map<int, std::unique_ptr<Setup>> g_setups;
-int g_next_mesh = 1, g_next_problem = 1;
+std::map<int, std::unique_ptr<Batch>> g_batches;
+int g_next_mesh = 1, g_next_problem = 1, g_next_batch = 1;
 bool g_inited = false;
 
 Mesh* get_mesh(int id) {
@@ -117,6 +118,21 @@ std::vector<double> to_device_order(const Setup& S, const double* x) {
 
 void to_reference_order(const Setup& S, const double* d, double* x) {
   for (int i = 0; i < S.n; i++) x[S.perm[i]] = d[i];
+}
+
+// integrate + assemble in the problem's assembly mode
+int assemble_problem(Problem& P, int precision) {
+  if (P.assembly_mode == 0) {
+    if (launch_assemble_rows(P, precision)) return -2;
+  } else if (P.assembly_mode == 2) {
+    if (launch_assemble_two_pass(P, precision)) return -2;
+  } else {
+    if (launch_zero(P.d_val, P.nnz_sell) || launch_zero(P.d_rhs, P.n)) return -2;
+    if (launch_integrate(P, precision, 0, nullptr, 0, P.nel)) return -2;
+    if (launch_faces(P, 0, nullptr, 0, P.nbf)) return -2;
+  }
+  P.dinv_valid = false;
+  return 0;
 }
 
 int rhs_to_host(const Problem& P, double* rhs) {
@@ -565,19 +581,8 @@ int mf_integrate_assemble(int pid, int precision) {
     set_error("problem has no mesh");
     return -1;
   }
-  if (P->assembly_mode == 0) {
-    if (launch_assemble_rows(*P, precision)) return -2;
-  } else if (P->assembly_mode == 2) {
-    if (launch_assemble_two_pass(*P, precision)) return -2;
-  } else {
-    if (launch_zero(P->d_val, P->nnz_sell) || launch_zero(P->d_rhs, P->n)) return -2;
-    if (launch_integrate(*P, precision, 0, nullptr, 0, P->nel)) return -2;
-    if (launch_faces(*P, 0, nullptr, 0, P->nbf)) return -2;
-  }
-  if (P->d_dinv) {
-    hipFree(P->d_dinv);
-    P->d_dinv = nullptr;
-  }
+  if (assemble_problem(*P, precision)) return -2;
+  P->dinv_valid = false;  // refilled before the next preconditioned solve
   MF_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
@@ -669,10 +674,7 @@ int mf_set_system(int pid, const double* val, const double* rhs) {
     std::vector<double> d = to_device_order(P->setup, rhs);
     MF_HIP(hipMemcpy(P->d_rhs, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
   }
-  if (P->d_dinv) {
-    hipFree(P->d_dinv);
-    P->d_dinv = nullptr;
-  }
+  P->dinv_valid = false;  // refilled before the next preconditioned solve
   return 0;
 }
 
@@ -938,9 +940,95 @@ int mf_gmres_kernel_stats(int enable, double* out) {
   return 0;
 }
 
+int mf_batch_create(int krylbas) {
+  if (need_init()) return -1;
+  if (krylbas < 1 || krylbas > 63) {
+    set_error("mf_batch_create: krylbas must be in [1,63]");
+    return -1;
+  }
+  auto B = std::make_unique<Batch>();
+  B->m = krylbas;
+  int id = g_next_batch++;
+  g_batches[id] = std::move(B);
+  return id;
+}
+
+static Batch* get_batch(int bid) {
+  auto it = g_batches.find(bid);
+  if (it == g_batches.end()) {
+    set_error("unknown batch handle " + std::to_string(bid));
+    return nullptr;
+  }
+  return it->second.get();
+}
+
+int mf_batch_add(int bid, int pid, int precon, int max_iter, double tol) {
+  Batch* B = get_batch(bid);
+  Problem* P = get_problem(pid);
+  if (!B || !P) return -3;
+  return batch_add(*B, P, pid, precon, max_iter, tol);
+}
+
+int mf_batch_remove(int bid, int pid) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  return batch_remove(*B, pid);
+}
+
+int mf_batch_integrate_assemble(int bid, int precision) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  if (precision != 64 && precision != 32) {
+    set_error("precision must be 64 or 32");
+    return -1;
+  }
+  for (auto& M : B->mem)
+    if (M.P->nel > 0 && assemble_problem(*M.P, precision)) return -2;
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_batch_run(int bid, int max_cycles, int use_graph, int* active) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  if (max_cycles < 0) {
+    set_error("mf_batch_run: max_cycles >= 0 required");
+    return -1;
+  }
+  return batch_run(*B, max_cycles, use_graph, active);
+}
+
+int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters, double* res_rel) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  for (size_t k = 0; k < B->mem.size(); k++)
+    if (B->mem[k].pid == pid) {
+      const GmresStat& st = B->h_stat[k];
+      if (done) *done = st.done;
+      if (converged) *converged = st.converged;
+      if (iters) *iters = st.gmiter;
+      if (res_rel) *res_rel = st.res_rel;
+      return st.nan ? -4 : 0;
+    }
+  set_error("mf_batch_status: problem is not a member");
+  return -1;
+}
+
+int mf_batch_free(int bid) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  batch_free(*B);
+  g_batches.erase(bid);
+  return 0;
+}
+
 int mf_problem_free(int pid) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
+  if (P->batch_refs > 0) {
+    set_error("mf_problem_free: the problem is a member of a batch (mf_batch_remove first)");
+    return -1;
+  }
   hipStreamSynchronize(stream());
   free_problem_device(*P);
   g_problems.erase(pid);

@@ -52,7 +52,12 @@ struct GmresStat {
   int max_iter;
   int stop;         // 1: stop this restart
   int converged;
-  int pad[3];
+  // restart bookkeeping kept on the device by the batched solver
+  int restit, restrt, m;
+  int done;         // 1: this problem's solve has finished
+  int nan;          // 1: residual was not a number
+  int ik_pending;   // coefficients computed this cycle, not yet applied to x
+  double beta1;
 };
 
 struct Problem {
@@ -100,6 +105,8 @@ struct Problem {
   double* d_coef = nullptr;    // SoA per model
   // preconditioner
   double* d_dinv = nullptr;
+  bool dinv_valid = false;     // d_dinv matches the current values
+  int batch_refs = 0;          // batches this problem is a member of
   int* d_diag_ptr = nullptr;
   float* d_dinv32 = nullptr;
   // solver tiles
@@ -119,6 +126,38 @@ struct Problem {
   // last measured times (ms)
   double t_integrate_ms = 0, t_faces_ms = 0, t_solve_ms = 0;
 };
+
+// Batched restarted GMRES over resident problems (mf_solver.hip).  Members
+// keep their own matrices, vectors and Krylov bases; one launch of each
+// kernel serves all members through a block table (block -> member, rank).
+struct BatchMember {
+  Problem* P = nullptr;
+  int pid = -1;
+  int precon = 1, max_iter = 1000;
+  double tol = 1e-8;
+};
+struct Batch {
+  int m = 30;
+  std::vector<BatchMember> mem;
+  std::vector<GmresStat> h_stat;   // host copy, current after every cycle
+  bool dirty = true;               // tables must be rebuilt
+  // device tables (valid when !dirty)
+  void* d_args = nullptr;          // SolveArgs per member
+  int* d_blk_prob = nullptr;
+  int* d_blk_rank = nullptr;
+  int* d_nblk = nullptr;
+  GmresStat* d_stat = nullptr;
+  GmresStat* h_pinned = nullptr;
+  int nblk = 0, cap = 0;
+  size_t lds = 0;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int cycles = 0;
+};
+int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double tol);
+int batch_remove(Batch& B, int pid);
+int batch_run(Batch& B, int max_cycles, int use_graph, int* active);
+void batch_free(Batch& B);
 
 int device_init(int device);
 hipStream_t stream();

@@ -35,13 +35,10 @@
 #include <vector>
 
 #include "mf_device.h"
+#include "mf_solver_dev.h"
 
 namespace mf {
 
-constexpr int SB = 256;         // threads per solver block
-constexpr int TILE_NNZ = 4096;  // CRS entries staged per tile (32 KB LDS)
-constexpr int MAX_BLOCKS = 4096;
-constexpr int MAX_NV = 64;
 
 static int g_ncu = 0;
 
@@ -109,176 +106,9 @@ static void ks_collect(int steps_done) {
   g_ks.step.clear();
 }
 
-// ---------------------------------------------------------------------
-// device helpers
-// ---------------------------------------------------------------------
-// blocks are dispatched round-robin over the 8 XCDs; give XCD x the x-th
-// contiguous range of logical blocks so its L2 sees neighbouring rows
-__device__ __forceinline__ int xcd_block(int bid, int nb) {
-  int xcd = bid & 7, k = bid >> 3;
-  int per = nb >> 3, rem = nb & 7;
-  return xcd * per + min(xcd, rem) + k;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// sum NV per-thread values over the block; results in red[0..nact-1]
-template <int NV>
-__device__ __forceinline__ void block_sum_multi(const double* acc, int nact, double* red) {
-  __shared__ double wsum[SB / 64][NV];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    if (i < nact) {
-      double v = wave_sum(acc[i]);
-      if (lane == 0) wsum[wid][i] = v;
-    }
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < nact) {
-    double s = 0;
-#pragma unroll
-    for (int w = 0; w < SB / 64; w++) s += wsum[w][threadIdx.x];
-    red[threadIdx.x] = s;
-  }
-  __syncthreads();
-}
-
-struct SolveArgs {
-  int n, ntiles, m;
-  long long ldv;        // stride of the Krylov vectors
-  const int* tile_row;  // CRS-tile path (storage seam)
-  const int* row_ptr;
-  const int* col;
-  const double* val;    // CRS values (tile path) or SELL-64 values
-  const float* val32;   // SELL-64 values rounded to FP32 (mixed-precision inner solves)
-  int op32;             // 1: the operator kernels read val32
-  const long long* sl_ptr;  // SELL-64 path
-  const int* scol;
-  const double* dinv;   // null: no preconditioner
-  const double* b;
-  double* x;
-  double* V;            // (m+1) x ldv
-  double* H;            // (m+1)^2 | givens 2(m+1) | scale (m+1) | coef (m+1)
-  double* partials;
-  GmresStat* stat;
-  GmresStat* hstat;     // pinned host mirror
-};
-
-__device__ __forceinline__ double* Hcol(const SolveArgs& A, int c) { return A.H + (size_t)(A.m + 1) * c; }
-__device__ __forceinline__ double* Giv(const SolveArgs& A) { return A.H + (size_t)(A.m + 1) * (A.m + 1); }
-__device__ __forceinline__ double* Scale(const SolveArgs& A) { return Giv(A) + 2 * (A.m + 1); }
-__device__ __forceinline__ double* Coef(const SolveArgs& A) { return Scale(A) + (A.m + 1); }
-
-// tile range of a logical block
-__device__ __forceinline__ void tile_range(const SolveArgs& A, int lb, int& t0, int& t1) {
-  t0 = (int)((long long)lb * A.ntiles / gridDim.x);
-  t1 = (int)((long long)(lb + 1) * A.ntiles / gridDim.x);
-}
-
-// stream one tile's CRS entries into LDS products and return the row sum of
-// the calling lane's row (valid when r0 + threadIdx.x < r1)
-__device__ __forceinline__ double tile_rows(const SolveArgs& A, const double* x, int r0, int r1,
-                                            double* prod) {
-  const int k0 = A.row_ptr[r0], k1 = A.row_ptr[r1];
-#pragma unroll 4
-  for (int k = k0 + threadIdx.x; k < k1; k += SB) prod[k - k0] = A.val[k] * x[A.col[k]];
-  __syncthreads();
-  double s = 0.0;
-  const int r = r0 + threadIdx.x;
-  if (r < r1) {
-    const int a = A.row_ptr[r] - k0, z = A.row_ptr[r + 1] - k0;
-    for (int k = a; k < z; k++) s += prod[k];
-  }
-  __syncthreads();
-  return s;
-}
-
-// SELL-64: wavefront = slice of 64 rows, lane = row; entries column-major
-// inside the slice, so every load of values and columns is one contiguous
-// 512/256-byte access.  Logical block lb owns a contiguous range of slices
-// (one contiguous eighth of the matrix per XCD); its 4 wavefronts take the
-// range's slices round-robin.
-constexpr int SW = SB / 64;  // wavefronts per block
-
-__device__ __forceinline__ void slice_range(int nsl, int lb, int& s0, int& s1) {
-  s0 = (int)((long long)lb * nsl / gridDim.x);
-  s1 = (int)((long long)(lb + 1) * nsl / gridDim.x);
-}
-
-// The slices of one operator block, in order t = 0, 1, ...: XCD x owns the
-// x-th contiguous part of the slices and its blocks take steps of SW
-// consecutive slices cyclically, so each XCD sweeps one contiguous window of
-// the matrix at a time (measured 3% faster than a contiguous range per block,
-// which leaves ~2k streams open at scattered addresses).
-struct SliceSet {
-  int base, step, end;
-  __device__ __forceinline__ int slice(int t) const { return base + (t / SW) * step + (t % SW); }
-  __device__ __forceinline__ int count() const {
-    if (end <= base) return 0;
-    const int full = (end - base) / step, rem = (end - base) % step;
-    return full * SW + min(rem, SW);
-  }
-};
-
-__device__ __forceinline__ SliceSet block_slices(int nsl) {
-  SliceSet S;
-  const int G = gridDim.x;
-  const int nx = G < 8 ? G : 8;
-  const int x = blockIdx.x % nx, q = blockIdx.x / nx;
-  const int nq = G / nx + (x < G % nx ? 1 : 0);
-  const int r0 = (int)((long long)x * nsl / nx), r1 = (int)((long long)(x + 1) * nsl / nx);
-  S.base = r0 + q * SW;
-  S.step = nq * SW;
-  S.end = r1;
-  return S;
-}
-
-// row sum of the calling lane's row: the slice's column indices and values
-// are loaded first (up to 24 entries in flight per lane), then the vector
-// gathers are issued together, so a slice costs about two memory round trips
-// instead of one per entry
-template <typename VT>
-__device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, const double* x,
-                                           int sl, int lane) {
-  constexpr int CH = 12;
-  const long long base = A.sl_ptr[sl] + lane;
-  const int len = (int)((A.sl_ptr[sl + 1] - A.sl_ptr[sl]) >> 6);
-  double s = 0.0;
-  for (int k0 = 0; k0 < len; k0 += CH) {
-    int c[CH];
-    double v[CH];
-#pragma unroll
-    for (int k = 0; k < CH; k++) {
-      const bool ok = k0 + k < len;
-      const long long q = base + 64LL * (ok ? k0 + k : k0);
-      c[k] = A.scol[q];
-      v[k] = ok ? (double)val[q] : 0.0;
-    }
-    double xv[CH];
-#pragma unroll
-    for (int k = 0; k < CH; k++) xv[k] = x[c[k]];
-#pragma unroll
-    for (int k = 0; k < CH; k++) s += v[k] * xv[k];
-  }
-  return s;
-}
-
-// row sum with the FP64 or the FP32 copy of the values (accumulated in FP64)
-__device__ __forceinline__ double sell_row_op(const SolveArgs& A, const double* x, int sl, int lane) {
-  return A.op32 ? sell_row(A, A.val32, x, sl, lane) : sell_row(A, A.val, x, sl, lane);
-}
-
-__global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
+// V_0 = D^-1 (b - A x) over slices [s0, s1); partial ||V_0||^2 to slot pb
+__device__ __forceinline__ void residual_body(const SolveArgs& A, int s0, int s1, int pb) {
   __shared__ double red[1];
-  const int nsl = (A.n + 63) >> 6;
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  int s0, s1;
-  slice_range(nsl, lb, s0, s1);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double acc = 0.0;
   for (int sl = s0 + wv; sl < s1; sl += SW) {
@@ -292,7 +122,15 @@ __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
     }
   }
   block_sum_multi<1>(&acc, 1, red);
-  if (threadIdx.x == 0) A.partials[lb] = red[0];
+  if (threadIdx.x == 0) A.partials[pb] = red[0];
+}
+
+__global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
+  const int nsl = (A.n + 63) >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int s0, s1;
+  slice_range(nsl, lb, s0, s1);
+  residual_body(A, s0, s1, lb);
 }
 
 // Arnoldi step j, part 1 (SELL-64).  Phase 1: w = s_j D^-1 A V_j for the
@@ -300,11 +138,8 @@ __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
 // forms V_i . w over the block's rows for i = q, q+SW, ..., streaming V_i with
 // four independent accumulators and one wave-wide sum per i, so the number of
 // basis vectors costs neither registers nor occupancy.
-__global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
-  if (A.stat->stop) return;
-  extern __shared__ double ws[];  // w of the block's slices, 64 per slice
-  const int nsl = (A.n + 63) >> 6;
-  const SliceSet S = block_slices(nsl);
+__device__ __forceinline__ void op_dots_body(const SolveArgs& A, int j, const SliceSet& S, int pb,
+                                             int G, double* ws) {
   const int ns = S.count();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const double* x = A.V + (size_t)j * A.ldv;
@@ -336,8 +171,15 @@ __global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
     }
     for (; t < ns; t++) a0 += v[(size_t)S.slice(t) * 64] * ws[t * 64 + lane];
     const double d = wave_sum((a0 + a1) + (a2 + a3));
-    if (lane == 0) A.partials[(size_t)i * gridDim.x + blockIdx.x] = d;
+    if (lane == 0) A.partials[(size_t)i * G + pb] = d;
   }
+}
+
+__global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  extern __shared__ double ws[];  // w of the block's slices, 64 per slice
+  const int nsl = (A.n + 63) >> 6;
+  op_dots_body(A, j, block_slices(nsl), blockIdx.x, gridDim.x, ws);
 }
 
 __global__ void __launch_bounds__(SB) k_spmv_sell(SolveArgs A, const double* x, double* y) {
@@ -436,7 +278,7 @@ __global__ void __launch_bounds__(SB) k_fin_residual(SolveArgs A, int G) {
   block_sum_multi<1>(&s, 1, red);
   if (threadIdx.x == 0) {
     A.stat->rnrm = sqrt(red[0]);
-    *A.hstat = *A.stat;
+    if (A.hstat) *A.hstat = *A.stat;
   }
 }
 
@@ -473,10 +315,8 @@ __global__ void __launch_bounds__(SB) k_op_dots(SolveArgs A, int j) {
 }
 
 // one block per dot product i <= j: h_i = s_i * sum_b partials[i][b]
-__global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
-  if (A.stat->stop) return;
+__device__ __forceinline__ void fin_dots_body(const SolveArgs& A, int j, int i, int G) {
   __shared__ double red[1];
-  const int i = blockIdx.x;
   const double* p = A.partials + (size_t)i * G;
   double s = 0.0;
   for (int b = threadIdx.x; b < G; b += SB) s += p[b];
@@ -489,18 +329,20 @@ __global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
   }
 }
 
+__global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
+  if (A.stat->stop) return;
+  fin_dots_body(A, j, blockIdx.x, G);
+}
+
 // Arnoldi step j, part 2: w -= sum_i c_i V_i over row pairs (16-byte loads),
 // per-block partial ||w||^2.  The vectors are read in groups of eight
 // independent loads; the subtraction order is i = 0, 1, ..., j.
-__global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
-  if (A.stat->stop) return;
+__device__ __forceinline__ void update_body(const SolveArgs& A, int j, long long p0, long long p1,
+                                            int pb) {
   __shared__ double red[1];
   __shared__ double cs[MAX_NV];
   if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
   __syncthreads();
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const long long np = (A.n + 1) / 2;  // row pairs (ldv is even)
-  const long long p0 = lb * np / gridDim.x, p1 = (lb + 1) * np / gridDim.x;
   const long long ldp = A.ldv / 2;
   const double2* V2 = reinterpret_cast<const double2*>(A.V);
   double2* w = reinterpret_cast<double2*>(A.V + (size_t)(j + 1) * A.ldv);
@@ -528,40 +370,19 @@ __global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
     acc += wr.x * wr.x + wr.y * wr.y;
   }
   block_sum_multi<1>(&acc, 1, red);
-  if (threadIdx.x == 0) A.partials[lb] = red[0];
+  if (threadIdx.x == 0) A.partials[pb] = red[0];
 }
 
-__device__ void drotg_dev(double* a, double* b, double* c, double* s) {
-  double roe = *b;
-  if (fabs(*a) > fabs(*b)) roe = *a;
-  double scale = fabs(*a) + fabs(*b);
-  double r, z;
-  if (scale == 0.0) {
-    *c = 1.0; *s = 0.0; r = 0.0; z = 0.0;
-  } else {
-    double ra = *a / scale, rb = *b / scale;
-    r = scale * sqrt(ra * ra + rb * rb);
-    r = (roe >= 0 ? 1.0 : -1.0) * r;
-    *c = *a / r;
-    *s = *b / r;
-    z = 1.0;
-    if (fabs(*a) > fabs(*b)) z = *s;
-    if (fabs(*b) >= fabs(*a) && *c != 0.0) z = 1.0 / *c;
-  }
-  *a = r;
-  *b = z;
-}
-
-__device__ __forceinline__ void rot(double* x, double* y, double c, double s) {
-  double t = c * (*x) + s * (*y);
-  *y = c * (*y) - s * (*x);
-  *x = t;
+__global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const long long np = (A.n + 1) / 2;  // row pairs (ldv is even)
+  update_body(A, j, lb * np / gridDim.x, (lb + 1) * np / gridDim.x, lb);
 }
 
 // h_{j+1,j} = ||w||, Givens rotations (drot/drotg as lss_mkb_core.c:2539-2560),
 // stopping tests (:2562-2590)
-__global__ void __launch_bounds__(SB) k_fin_update(SolveArgs A, int j, int G) {
-  if (A.stat->stop) return;
+__device__ __forceinline__ void fin_update_body(const SolveArgs& A, int j, int G) {
   __shared__ double red[1];
   __shared__ double col[MAX_NV + 1], giv[2 * MAX_NV], g2[2];
   double s = 0.0;
@@ -598,10 +419,15 @@ __global__ void __launch_bounds__(SB) k_fin_update(SolveArgs A, int j, int G) {
       st->stop = 1;
     }
     if (j + 1 == m || st->gmiter == st->max_iter) st->stop = 1;
-    *A.hstat = *st;
+    if (A.hstat) *A.hstat = *st;
   }
   __syncthreads();
   if ((int)threadIdx.x <= j + 1) hc[threadIdx.x] = col[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(SB) k_fin_update(SolveArgs A, int j, int G) {
+  if (A.stat->stop) return;
+  fin_update_body(A, j, G);
 }
 
 // x += sum_c (y_c s_c) V_c
@@ -752,6 +578,7 @@ int build_tiles(Problem& P) {
 
 int launch_fill_jacobi(Problem& P) {
   if (!P.d_dinv) MF_HIP(hipMalloc(&P.d_dinv, (size_t)P.n * sizeof(double)));
+  P.dinv_valid = true;
   if (P.sell)
     hipLaunchKernelGGL(k_fill_jacobi_sell, dim3((P.n + 255) / 256), dim3(256), 0, stream(), P.n,
                        P.d_sl_ptr, P.d_diag_k, P.d_val, P.d_dinv);
@@ -909,7 +736,7 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     return -1;
   }
   if (ensure_workspace(P, m)) return -2;
-  if (precon && !P.d_dinv) {
+  if (precon && (!P.d_dinv || !P.dinv_valid)) {
     if (launch_fill_jacobi(P)) return -2;
   }
   SolveArgs A;
@@ -1067,7 +894,7 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
     return -1;
   }
   if (ensure_workspace(P, krylbas)) return -2;
-  if (precon && !P.d_dinv && launch_fill_jacobi(P)) return -2;
+  if (precon && (!P.d_dinv || !P.dinv_valid) && launch_fill_jacobi(P)) return -2;
   hipStream_t s = stream();
   const long long n = P.n;
   if (!P.d_val32) MF_HIP(hipMalloc(&P.d_val32, (size_t)P.nnz_sell * sizeof(float)));
@@ -1131,6 +958,371 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   if (resid_out) *resid_out = rho;
   if (refinements) *refinements = k;
   return conv;
+}
+
+
+// ---------------------------------------------------------------------
+// Batched GMRES.  All members advance one restart cycle per round of
+// launches: residual, restart test, m Arnoldi steps (operator+dots, finish
+// dots, update, finish update), y = R^-1 g, x update, bookkeeping.  The
+// restart logic of gmres_core runs on the device (kb_restart, kb_advance),
+// so a cycle needs no host decision and can be replayed as one hipGraph.
+// Members that have stopped leave every later launch at once.
+// ---------------------------------------------------------------------
+struct BatchArgs {
+  const SolveArgs* args;
+  const int* blk_prob;
+  const int* blk_rank;
+  const int* nblk;
+};
+
+constexpr int BATCH_SPB = 8;  // target slices per block
+
+__global__ void __launch_bounds__(SB) kb_residual(BatchArgs B) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs A = B.args[p];
+  if (A.stat->done) return;
+  const int nsl = (A.n + 63) >> 6;
+  residual_body(A, (int)((long long)r * nsl / G), (int)((long long)(r + 1) * nsl / G), r);
+}
+
+// restart test of lsr_mkb_core_gmres (lss_mkb_core.c:2404-2470) per member
+__global__ void __launch_bounds__(SB) kb_restart(BatchArgs B) {
+  const SolveArgs A = B.args[blockIdx.x];
+  GmresStat* st = A.stat;
+  if (st->done) return;
+  __shared__ double red[1];
+  __shared__ int go;
+  const int G = B.nblk[blockIdx.x];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += SB) s += A.partials[b];
+  block_sum_multi<1>(&s, 1, red);
+  if (threadIdx.x == 0) {
+    const double SMALL = 1.0e-15, MAX_CONVRATE = 0.999;
+    const double rnrm2 = sqrt(red[0]);
+    double convrate = 0.0, faux = 0.0;
+    st->stop = 0;
+    st->j_done = 0;
+    if (st->restit == 0) {
+      st->beta0 = rnrm2;
+      st->beta1 = rnrm2;
+    } else if (rnrm2 < SMALL * SMALL || st->gmiter == 0 || st->beta0 < SMALL * SMALL ||
+               st->beta1 < SMALL * SMALL) {
+      st->beta1 = 0.0;
+    } else {
+      convrate = exp(log(rnrm2 / st->beta0) / st->gmiter);
+      faux = exp(log(rnrm2 / st->beta1) / st->m);
+      st->beta1 = rnrm2;
+    }
+    st->resid = rnrm2;
+    st->res_rel = st->beta0 != 0.0 ? rnrm2 / st->beta0 : 0.0;
+    st->rnrm = rnrm2;
+    go = 0;
+    if (!(rnrm2 == rnrm2)) {
+      st->nan = 1;
+      st->done = 1;
+      st->stop = 1;
+    } else if (rnrm2 < st->tol_res || st->res_rel < st->tol_rel || rnrm2 < SMALL * SMALL) {
+      st->converged = 1;
+      st->done = 1;
+      st->stop = 1;
+    } else if (convrate > MAX_CONVRATE || faux > MAX_CONVRATE) {
+      st->done = 1;
+      st->stop = 1;
+    } else {
+      go = 1;
+    }
+  }
+  __syncthreads();
+  if (!go) return;
+  const int nh = (A.m + 1) * (A.m + 1) + 4 * (A.m + 1);
+  for (int k = threadIdx.x; k < nh; k += SB) A.H[k] = 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    A.H[0] = st->rnrm;
+    Scale(A)[0] = 1.0 / st->rnrm;
+  }
+}
+
+__global__ void __launch_bounds__(SB) kb_op_dots(BatchArgs B, int j) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs A = B.args[p];
+  if (A.stat->stop) return;
+  extern __shared__ double ws[];
+  const int nsl = (A.n + 63) >> 6;
+  SliceSet S;
+  S.base = (int)((long long)r * nsl / G);
+  S.end = (int)((long long)(r + 1) * nsl / G);
+  S.step = SW;  // contiguous
+  op_dots_body(A, j, S, r, G, ws);
+}
+
+__global__ void __launch_bounds__(SB) kb_fin_dots(BatchArgs B, int j) {
+  const SolveArgs A = B.args[blockIdx.y];
+  if (A.stat->stop) return;
+  fin_dots_body(A, j, blockIdx.x, B.nblk[blockIdx.y]);
+}
+
+__global__ void __launch_bounds__(SB) kb_update(BatchArgs B, int j) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs A = B.args[p];
+  if (A.stat->stop) return;
+  const long long np = (A.n + 1) / 2;
+  update_body(A, j, r * np / G, (r + 1) * np / G, r);
+}
+
+__global__ void __launch_bounds__(SB) kb_fin_update(BatchArgs B, int j) {
+  const SolveArgs A = B.args[blockIdx.x];
+  if (A.stat->stop) return;
+  fin_update_body(A, j, B.nblk[blockIdx.x]);
+}
+
+// y = R^-1 g (dtrsv "U","N","N", lss_mkb_core.c:2591-2593), coefficients y_c s_c
+__global__ void __launch_bounds__(64) kb_ysolve(BatchArgs B) {
+  const SolveArgs A = B.args[blockIdx.x];
+  GmresStat* st = A.stat;
+  if (st->done || threadIdx.x != 0) return;
+  __shared__ double y[MAX_NV];
+  const int m = A.m, ik = st->j_done;
+  for (int i = ik - 1; i >= 0; i--) {
+    double sum = A.H[i];
+    for (int c = i + 1; c < ik; c++) sum -= A.H[(size_t)(c + 1) * (m + 1) + i] * y[c];
+    y[i] = sum / A.H[(size_t)(i + 1) * (m + 1) + i];
+  }
+  for (int c = 0; c < ik; c++) Coef(A)[c] = y[c] * Scale(A)[c];
+  st->ik_pending = ik;
+}
+
+// x += sum_c coef_c V_c over the block's rows (dgemv, lss_mkb_core.c:2602)
+__global__ void __launch_bounds__(SB) kb_xupdate(BatchArgs B) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs A = B.args[p];
+  const int ik = A.stat->ik_pending;
+  if (ik == 0) return;
+  __shared__ double cs[MAX_NV];
+  if ((int)threadIdx.x < ik) cs[threadIdx.x] = Coef(A)[threadIdx.x];
+  __syncthreads();
+  const long long r0 = (long long)r * A.n / G, r1 = (long long)(r + 1) * A.n / G;
+  for (long long i = r0 + threadIdx.x; i < r1; i += SB) {
+    double xr = A.x[i];
+    for (int c = 0; c < ik; c++) xr += A.V[(size_t)c * A.ldv + i] * cs[c];
+    A.x[i] = xr;
+  }
+}
+
+// end of a restart (lss_mkb_core.c:2606-2668): converged, out of
+// iterations, or the next restart
+__global__ void kb_advance(BatchArgs B, int nprob) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nprob) return;
+  GmresStat* st = B.args[p].stat;
+  if (st->ik_pending == 0) return;
+  st->ik_pending = 0;
+  if (st->converged) {
+    st->done = 1;
+  } else if (st->gmiter == st->max_iter) {
+    st->done = 1;
+  } else {
+    st->restit += 1;
+    if (st->restit > st->restrt) st->done = 1;
+  }
+  if (st->done) st->stop = 1;
+}
+
+static void batch_drop_graph(Batch& B) {
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B.exec = nullptr;
+  B.graph = nullptr;
+}
+
+static void batch_drop_tables(Batch& B) {
+  batch_drop_graph(B);
+  void* ptrs[] = {B.d_args, B.d_blk_prob, B.d_blk_rank, B.d_nblk, B.d_stat};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (B.h_pinned) (void)hipHostFree(B.h_pinned);
+  B.d_args = nullptr; B.d_blk_prob = B.d_blk_rank = B.d_nblk = nullptr;
+  B.d_stat = nullptr; B.h_pinned = nullptr;
+  B.cap = 0;
+}
+
+// initial state of a member: tolerances floored as lss_mkb_core.c:2377-2378
+static GmresStat member_init(const BatchMember& M, int m) {
+  GmresStat st{};
+  st.tol_rel = M.tol < 1e-12 ? 1e-12 : M.tol;
+  st.tol_res = 1e-12;
+  st.max_iter = M.max_iter;
+  st.m = m;
+  st.restrt = M.max_iter / m + 1;
+  return st;
+}
+
+int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double tol) {
+  if (!P->sell) {
+    set_error("batch: member needs SELL storage");
+    return -1;
+  }
+  if (max_iter < 1 || tol < 0 || (precon != 0 && precon != 1)) {
+    set_error("batch: precon 0/1, max_iter >= 1 and tol >= 0 required");
+    return -1;
+  }
+  for (auto& M : B.mem)
+    if (M.pid == pid) {
+      set_error("batch: problem is already a member");
+      return -1;
+    }
+  if (ensure_workspace(*P, B.m)) return -2;
+  if (precon && (!P->d_dinv || !P->dinv_valid) && launch_fill_jacobi(*P)) return -2;
+  if (!P->d_x) MF_HIP(hipMalloc(&P->d_x, (size_t)P->n * sizeof(double)));
+  MF_HIP(hipMemsetAsync(P->d_x, 0, (size_t)P->n * sizeof(double), stream()));
+  BatchMember M;
+  M.P = P; M.pid = pid; M.precon = precon; M.max_iter = max_iter; M.tol = tol;
+  P->batch_refs++;
+  B.mem.push_back(M);
+  B.h_stat.push_back(member_init(M, B.m));
+  B.dirty = true;
+  return (int)B.mem.size() - 1;
+}
+
+int batch_remove(Batch& B, int pid) {
+  for (size_t k = 0; k < B.mem.size(); k++)
+    if (B.mem[k].pid == pid) {
+      B.mem[k].P->batch_refs--;
+      B.mem.erase(B.mem.begin() + k);
+      B.h_stat.erase(B.h_stat.begin() + k);
+      B.dirty = true;
+      return 0;
+    }
+  set_error("batch: problem is not a member");
+  return -1;
+}
+
+static int batch_build(Batch& B) {
+  const int np = (int)B.mem.size();
+  batch_drop_graph(B);
+  if (np > B.cap) {
+    batch_drop_tables(B);
+    const int cap = std::max(np, 16);
+    MF_HIP(hipMalloc(&B.d_args, (size_t)cap * sizeof(SolveArgs)));
+    MF_HIP(hipMalloc(&B.d_nblk, (size_t)cap * sizeof(int)));
+    MF_HIP(hipMalloc(&B.d_stat, (size_t)cap * sizeof(GmresStat)));
+    MF_HIP(hipHostMalloc(&B.h_pinned, (size_t)cap * sizeof(GmresStat), hipHostMallocDefault));
+    B.cap = cap;
+  }
+  std::vector<SolveArgs> args(np);
+  std::vector<int> nblk(np), bp, br;
+  int max_sl = 1;
+  for (int k = 0; k < np; k++) {
+    Problem& P = *B.mem[k].P;
+    SolveArgs& A = args[k];
+    fill_solve_args(P, A, B.mem[k].precon, B.m);
+    A.b = P.d_rhs;
+    A.x = P.d_x;
+    A.stat = B.d_stat + k;
+    A.hstat = nullptr;
+    const int nsl = (P.n + 63) / 64;
+    const int G = std::max(1, std::min(MAX_BLOCKS, (nsl + BATCH_SPB - 1) / BATCH_SPB));
+    nblk[k] = G;
+    max_sl = std::max(max_sl, (nsl + G - 1) / G + 1);
+    for (int r = 0; r < G; r++) {
+      bp.push_back(k);
+      br.push_back(r);
+    }
+  }
+  B.nblk = (int)bp.size();
+  B.lds = (size_t)max_sl * 64 * sizeof(double);
+  if (B.lds > 64 * 1024) {
+    set_error("batch: a member is too large for the batched operator (> 33M rows)");
+    return -1;
+  }
+  if (B.d_blk_prob) (void)hipFree(B.d_blk_prob);
+  if (B.d_blk_rank) (void)hipFree(B.d_blk_rank);
+  B.d_blk_prob = B.d_blk_rank = nullptr;
+  if (np > 0) {
+    MF_HIP(hipMalloc(&B.d_blk_prob, bp.size() * sizeof(int)));
+    MF_HIP(hipMalloc(&B.d_blk_rank, br.size() * sizeof(int)));
+    hipStream_t s = stream();
+    MF_HIP(hipMemcpyAsync(B.d_blk_prob, bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(B.d_blk_rank, br.data(), br.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(B.d_args, args.data(), np * sizeof(SolveArgs), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(B.d_nblk, nblk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(B.d_stat, B.h_stat.data(), np * sizeof(GmresStat), hipMemcpyHostToDevice, s));
+    MF_HIP(hipStreamSynchronize(s));
+  }
+  B.dirty = false;
+  return 0;
+}
+
+static void batch_enqueue_cycle(Batch& B, hipStream_t s) {
+  BatchArgs a;
+  a.args = reinterpret_cast<const SolveArgs*>(B.d_args);
+  a.blk_prob = B.d_blk_prob;
+  a.blk_rank = B.d_blk_rank;
+  a.nblk = B.d_nblk;
+  const int np = (int)B.mem.size();
+  hipLaunchKernelGGL(kb_residual, dim3(B.nblk), dim3(SB), 0, s, a);
+  hipLaunchKernelGGL(kb_restart, dim3(np), dim3(SB), 0, s, a);
+  for (int j = 0; j < B.m; j++) {
+    hipLaunchKernelGGL(kb_op_dots, dim3(B.nblk), dim3(SB), B.lds, s, a, j);
+    hipLaunchKernelGGL(kb_fin_dots, dim3(j + 1, np), dim3(SB), 0, s, a, j);
+    hipLaunchKernelGGL(kb_update, dim3(B.nblk), dim3(SB), 0, s, a, j);
+    hipLaunchKernelGGL(kb_fin_update, dim3(np), dim3(SB), 0, s, a, j);
+  }
+  hipLaunchKernelGGL(kb_ysolve, dim3(np), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(kb_xupdate, dim3(B.nblk), dim3(SB), 0, s, a);
+  hipLaunchKernelGGL(kb_advance, dim3((np + 255) / 256), dim3(256), 0, s, a, np);
+}
+
+int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
+  const int np = (int)B.mem.size();
+  if (B.m < 1 || B.m >= MAX_NV) {
+    set_error("batch: number of Krylov vectors must be in [1,63]");
+    return -1;
+  }
+  if (B.dirty) {
+    const int rc = batch_build(B);
+    if (rc) return rc;
+  }
+  hipStream_t s = stream();
+  // members re-assembled since they joined get their Jacobi refilled in place
+  for (auto& M : B.mem)
+    if (M.precon && !M.P->dinv_valid && launch_fill_jacobi(*M.P)) return -2;
+  int act = 0;
+  for (int k = 0; k < np; k++) act += !B.h_stat[k].done;
+  for (int c = 0; c < max_cycles && act > 0; c++) {
+    if (use_graph) {
+      if (!B.exec) {
+        MF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        batch_enqueue_cycle(B, s);
+        MF_HIP(hipStreamEndCapture(s, &B.graph));
+        MF_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+      }
+      MF_HIP(hipGraphLaunch(B.exec, s));
+    } else {
+      batch_enqueue_cycle(B, s);
+    }
+    MF_HIP(hipGetLastError());
+    MF_HIP(hipMemcpyAsync(B.h_pinned, B.d_stat, np * sizeof(GmresStat), hipMemcpyDeviceToHost, s));
+    MF_HIP(hipStreamSynchronize(s));
+    std::copy(B.h_pinned, B.h_pinned + np, B.h_stat.begin());
+    B.cycles++;
+    act = 0;
+    for (int k = 0; k < np; k++) act += !B.h_stat[k].done;
+  }
+  if (active) *active = act;
+  return 0;
+}
+
+void batch_free(Batch& B) {
+  (void)hipStreamSynchronize(stream());
+  batch_drop_tables(B);
+  if (B.d_blk_prob) (void)hipFree(B.d_blk_prob);
+  if (B.d_blk_rank) (void)hipFree(B.d_blk_rank);
+  B.d_blk_prob = B.d_blk_rank = nullptr;
+  for (auto& M : B.mem) M.P->batch_refs--;
+  B.mem.clear();
+  B.h_stat.clear();
 }
 
 }  // namespace mf

@@ -1,0 +1,131 @@
+"""Batched GMRES over resident problems (no reference counterpart: the
+reference solves one system at a time, lss_mkb_core.c:2317).  Each member of
+a batch must give the result it gives alone: same convergence, iteration
+count within rounding drift of the reduction order, and solution within the
+solver tolerance, whether members join at once or while others are running
+(continuous batching), eagerly launched or replayed as one hipGraph."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import modfem_amd as mf  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make_problems(sizes, seed=0):
+    probs = []
+    rng = np.random.default_rng(seed)
+    for k, (nx, ny, nz) in enumerate(sizes):
+        mesh = mf.Mesh.box_prism(nx, ny, nz, seed=100 + k, jitter=0.2,
+                                 lx=1.0, ly=rng.uniform(1, 3), lz=rng.uniform(1, 2))
+        p = mf.Problem.from_mesh(mesh)
+        kappa = np.exp(rng.normal(0.0, 0.5, mesh.nel))
+        cc = np.zeros(mf.NCOEF)
+        cc[mf.C_S] = 1.0
+        p.set_coefficients(mf.IND_DIFF, mf.COEF_KAPPA, coef_const=cc, coef_el=kappa)
+        p.integrate_assemble(64)
+        probs.append((mesh, p))
+    return probs
+
+
+SIZES = [(6, 5, 4), (10, 9, 8), (3, 3, 3), (14, 12, 10), (8, 8, 8)]
+
+
+def alone(p, m=30, tol=1e-8):
+    x, info = p.gmres(precon=1, krylbas=m, max_iter=4000, tol=tol)
+    return x, info
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_single_solves(use_graph):
+    probs = make_problems(SIZES)
+    ref = [alone(p) for _, p in probs]
+    b = mf.Batch(krylbas=30)
+    for _, p in probs:
+        b.add(p, precon=1, max_iter=4000, tol=1e-8)
+    assert b.run(use_graph=use_graph) == 0
+    for (x0, i0), (_, p) in zip(ref, probs):
+        st = b.status(p)
+        assert st["done"] and st["converged"], st
+        assert abs(st["iters"] - i0["iters"]) <= max(2, i0["iters"] // 50), (st, i0)
+        assert st["res_rel"] < 1e-8
+        assert rel_err(p.solution(), x0) <= 1e-6
+    b.free()
+    for _, p in probs:
+        p.free()
+
+
+def test_graph_replays_match_eager_launches():
+    """the same cycle replayed from a graph and launched eagerly gives
+    bitwise the same iterates (same kernels, same order)"""
+    probs = make_problems(SIZES[:3], seed=3)
+    out = []
+    for use_graph in (False, True):
+        b = mf.Batch(krylbas=20)
+        for _, p in probs:
+            b.add(p, precon=1, max_iter=2000, tol=1e-9)
+        while b.run(max_cycles=1, use_graph=use_graph) > 0:
+            pass
+        out.append([(b.status(p)["iters"], p.solution()) for _, p in probs])
+        b.free()
+    for (i0, x0), (i1, x1) in zip(out[0], out[1]):
+        assert i0 == i1 and np.array_equal(x0, x1)
+    for _, p in probs:
+        p.free()
+
+
+def test_continuous_batching_same_alone_and_in_the_mix():
+    probs = make_problems(SIZES, seed=7)
+    ref = [alone(p) for _, p in probs]
+    b = mf.Batch(krylbas=30)
+    for _, p in probs[:2]:
+        b.add(p, precon=1, max_iter=4000, tol=1e-8)
+    b.run(max_cycles=1)
+    b.add(probs[2][1], precon=1, max_iter=4000, tol=1e-8)  # joins while others run
+    b.run(max_cycles=2)
+    for _, p in probs[3:]:
+        b.add(p, precon=1, max_iter=4000, tol=1e-8)
+    finished = set()
+    while True:
+        active = b.run(max_cycles=1)
+        for k, (_, p) in enumerate(probs[:3] + probs[3:]):
+            if k not in finished:
+                try:
+                    st = b.status(p)
+                except RuntimeError:
+                    continue
+                if st["done"]:
+                    finished.add(k)
+                    x = p.solution()
+                    assert st["converged"]
+                    assert abs(st["iters"] - ref[k][1]["iters"]) <= max(2, ref[k][1]["iters"] // 50)
+                    assert rel_err(x, ref[k][0]) <= 1e-6
+                    b.remove(p)  # slot freed while others still run
+        if active == 0:
+            break
+    assert len(finished) == len(probs)
+    with pytest.raises(RuntimeError):
+        b.status(probs[0][1])
+    b.free()
+    for _, p in probs:
+        p.free()
+
+
+def test_member_cannot_be_freed():
+    (_, p), = make_problems([(4, 4, 4)])
+    b = mf.Batch(krylbas=10)
+    b.add(p)
+    with pytest.raises(RuntimeError):
+        mf.check(mf.lib().mf_problem_free(p.h))
+    b.run()
+    b.remove(p)
+    b.free()
+    p.free()
