@@ -83,6 +83,182 @@ def measured_traffic(n, krylbas):
     return None
 
 
+# ---------------------------------------------------------------------------
+# many-problem workloads: a batch of independent problems (one pipeline versus
+# a loop of plain solves) and a served request stream (continuous batching)
+# ---------------------------------------------------------------------------
+def make_box_problem(mf, n_el, rng, seed):
+    """std-linear prism Poisson on a box of about n_el prisms with random
+    aspect ratios in [1,4], kappa lognormal(0, 0.5) and source U(-1,1) per
+    element, Dirichlet penalty on the boundary"""
+    asp = rng.uniform(1.0, 4.0, 3)
+    sc = (n_el / 2.0 / asp.prod()) ** (1.0 / 3.0)
+    nx, ny, nz = (max(1, int(round(a * sc))) for a in asp)
+    mesh = mf.Mesh.box_prism(nx, ny, nz, lx=asp[0], ly=asp[1], lz=asp[2], seed=seed, jitter=0.2)
+    p = mf.Problem.from_mesh(mesh)
+    ce = np.concatenate([np.exp(rng.normal(0.0, 0.5, mesh.nel)), rng.uniform(-1.0, 1.0, mesh.nel)])
+    p.set_coefficients(mf.IND_DIFF, mf.COEF_KAPPA_S, coef_el=ce)
+    mesh.free()
+    return p
+
+
+def bench_batch(args):
+    """independent std-linear Poisson problems, element counts log-uniform in
+    [el_min, el_max]: integrate+assemble+GMRES(30)+Jacobi to 1e-8 for all of
+    them as one batch (device restart logic, hipGraph cycles) versus a loop of
+    plain solves of the same problems"""
+    import modfem_amd as mf
+    rng = np.random.default_rng(args.seed)
+    t0 = time.perf_counter()
+    probs = [make_box_problem(mf, float(np.exp(rng.uniform(np.log(args.el_min), np.log(args.el_max)))),
+                              rng, args.seed * 1000 + k) for k in range(args.count)]
+    t_setup = time.perf_counter() - t0
+    m, tol, mi = 30, 1e-8, 20000
+    nel = sum(p.nel for p in probs)
+    ndof = sum(p.n for p in probs)
+    # warm-up of both paths on the first problems
+    for p in probs[:4]:
+        p.integrate_assemble(64)
+        p.gmres(precon=1, krylbas=m, max_iter=mi, tol=tol, fetch=False)
+    wb = mf.Batch(m)
+    for p in probs[:4]:
+        wb.add(p, 1, mi, tol)
+    wb.integrate_assemble(64)
+    wb.run(use_graph=True)
+    wb.free()
+    # loop of plain solves
+    t0 = time.perf_counter()
+    loop_iters = []
+    for p in probs:
+        p.integrate_assemble(64)
+        _, info = p.gmres(precon=1, krylbas=m, max_iter=mi, tol=tol, fetch=False)
+        loop_iters.append(info["iters"])
+    t_loop = time.perf_counter() - t0
+    x_loop = [p.solution() for p in probs]
+    # one batch
+    b = mf.Batch(m)
+    for p in probs:
+        b.add(p, 1, mi, tol)
+    t0 = time.perf_counter()
+    b.integrate_assemble(64)
+    b.run(use_graph=True)
+    t_batch = time.perf_counter() - t0
+    st = [b.status(p) for p in probs]
+    diff = max(float(np.linalg.norm(p.solution() - x) / np.linalg.norm(x)) for p, x in zip(probs, x_loop))
+    b.free()
+    out = {
+        "metric": "solves/s of a batch of independent problems at 1e-8 rel residual",
+        "value": len(probs) / t_batch, "unit": "solves/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d std-linear prism Poisson problems, %d-%d prisms log-uniform, "
+                               "aspect 1-4, kappa lognormal(0,0.5), source U(-1,1), "
+                               "GMRES(30)+Jacobi 1e-8" % (args.count, args.el_min, args.el_max),
+                   "elements": nel, "dofs": ndof},
+        "batch_s": t_batch, "loop_s": t_loop, "loop_solves_per_s": len(probs) / t_loop,
+        "speedup_vs_loop": t_loop / t_batch, "elements_per_s": nel / t_batch,
+        "all_converged": all(x["converged"] for x in st),
+        "iters": {"min": min(x["iters"] for x in st), "median": float(np.median([x["iters"] for x in st])),
+                  "max": max(x["iters"] for x in st)},
+        "max_iter_diff_vs_loop": max(abs(x["iters"] - i) for x, i in zip(st, loop_iters)),
+        "max_rel_diff_vs_loop_solution": diff, "setup_s": t_setup,
+    }
+    for p in probs:
+        p.free()
+    print(json.dumps(out))
+
+
+def footprint(p, m):
+    """device bytes a resident problem holds: SELL values+columns, element
+    tables, vectors and the Krylov basis"""
+    return 12.0 * p.nnz * 1.04 + 8.0 * (m + 8) * p.n + 120.0 * p.nel
+
+
+def bench_serve(args):
+    """Poisson-arrival stream of std-linear Poisson requests (prisms,
+    log-uniform sizes): admission by device-memory footprint and slot count,
+    continuous batching between restart cycles, latency p50/p99 and solves/s;
+    versus serving the same stream one plain FP64 solve at a time (FIFO,
+    with each request's measured solve time)"""
+    import modfem_amd as mf
+    rng = np.random.default_rng(args.seed)
+    m, tol, mi = 30, 1e-8, 20000
+    probs = [make_box_problem(mf, float(np.exp(rng.uniform(np.log(args.el_min), np.log(args.el_max)))),
+                              rng, args.seed * 1000 + k) for k in range(args.count)]
+    # sequential plain solves: service time of every request alone
+    for p in probs[:3]:
+        p.integrate_assemble(64)
+        p.gmres(precon=1, krylbas=m, max_iter=mi, tol=tol, fetch=False)
+    service = []
+    for p in probs:
+        t0 = time.perf_counter()
+        p.integrate_assemble(64)
+        p.gmres(precon=1, krylbas=m, max_iter=mi, tol=tol, fetch=False)
+        service.append(time.perf_counter() - t0)
+    results = []
+    budget = args.hbm_gb * 1e9
+    if max(footprint(p, m) for p in probs) > budget:
+        raise SystemExit("a request is larger than the admission budget")
+    for lam in args.rates:
+        arr = np.cumsum(rng.exponential(1.0 / lam, len(probs)))
+        # sequential FIFO server
+        fin, seq_lat = 0.0, []
+        for a, sv in zip(arr, service):
+            fin = max(fin, a) + sv
+            seq_lat.append(fin - a)
+        seq_span = fin - arr[0]
+        # continuous batching server
+        b = mf.Batch(m)
+        nxt, live, used, lat = 0, {}, 0.0, []
+        t0 = time.perf_counter()
+        while len(lat) < len(probs):
+            now = time.perf_counter() - t0
+            while (nxt < len(probs) and arr[nxt] <= now and len(live) < args.slots and
+                   used + footprint(probs[nxt], m) <= budget):
+                p = probs[nxt]
+                p.integrate_assemble(64)
+                b.add(p, 1, mi, tol)
+                live[nxt] = p
+                used += footprint(p, m)
+                nxt += 1
+            if not live:
+                time.sleep(max(0.0, min(arr[nxt] - now, 0.01)))
+                continue
+            b.run(max_cycles=1, use_graph=False)
+            now = time.perf_counter() - t0
+            for k in list(live):
+                st = b.status(live[k])
+                if st["done"]:
+                    if not st["converged"]:
+                        raise RuntimeError("request %d did not converge" % k)
+                    lat.append(now - arr[k])
+                    used -= footprint(live[k], m)
+                    b.remove(live[k])
+                    del live[k]
+        span = time.perf_counter() - t0 - arr[0]
+        b.free()
+        results.append({"rate_per_s": lam, "solves_per_s": len(probs) / span,
+                        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                                       "p99": float(np.percentile(lat, 99) * 1e3)},
+                        "sequential_solves_per_s": len(probs) / seq_span,
+                        "sequential_latency_ms": {"p50": float(np.percentile(seq_lat, 50) * 1e3),
+                                                  "p99": float(np.percentile(seq_lat, 99) * 1e3)},
+                        "speedup_solves_per_s": seq_span / span})
+    best = max(results, key=lambda r: r["solves_per_s"])
+    out = {
+        "metric": "served solves/s and p50/p99 latency at 1e-8 rel residual",
+        "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "Poisson arrivals of %d std-linear prism Poisson requests, %d-%d prisms "
+                               "log-uniform, GMRES(30)+Jacobi 1e-8, <= %d slots, %.0f GB admission "
+                               "budget" % (args.count, args.el_min, args.el_max, args.slots, args.hbm_gb)},
+        "rates": results,
+        "sequential_mean_service_ms": float(np.mean(service) * 1e3),
+    }
+    for p in probs:
+        p.free()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,10 +273,23 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: one integrate+assemble, one "
                          "GMRES restart and a 1 GiB stream copy (the counters' yardstick)")
+    ap.add_argument("--workload", choices=["box128", "batch", "serve"], default="box128",
+                    help="box128: the single large problem (default line); batch: many "
+                         "independent problems in one pipeline; serve: a request stream")
+    ap.add_argument("--count", type=int, default=256, help="problems (batch) or requests (serve)")
+    ap.add_argument("--el-min", type=int, default=2000)
+    ap.add_argument("--el-max", type=int, default=200000)
+    ap.add_argument("--rates", type=float, nargs="+", default=[10.0, 100.0, 1000.0, 2000.0])
+    ap.add_argument("--slots", type=int, default=256)
+    ap.add_argument("--hbm-gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     import modfem_amd as mf
     t_setup = time.perf_counter()
     mesh, kappa, b, cc, coef_el = make_workload(args.n, args.seed)

@@ -30,6 +30,7 @@ extern "C" {
 #define MF_COEF_KAPPA 1   /* coef_el[nel]: diffusion a = kappa*I             */
 #define MF_COEF_KAPPA_B 2 /* coef_el[4*nel] SoA: kappa, bx, by, bz            */
 #define MF_COEF_FULL 3    /* coef_el[21*nel] SoA: a[9] b[3] t[3] c q[3] s m  */
+#define MF_COEF_KAPPA_S 4 /* coef_el[2*nel] SoA: kappa, source s              */
 /* source / Dirichlet data models */
 #define MF_SRC_CONST 0
 #define MF_SRC_DIFF_IN_CUBE 1 /* pdd_conv_diff/main/pds_conv_diff_util.c, pdr_exact_sol */

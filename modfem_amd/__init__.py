@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 __all__ = ["lib", "Mesh", "Problem", "MFError", "PRISM", "TETRA",
-           "COEF_UNIFORM", "COEF_KAPPA", "COEF_KAPPA_B", "COEF_FULL",
+           "COEF_UNIFORM", "COEF_KAPPA", "COEF_KAPPA_B", "COEF_FULL", "COEF_KAPPA_S",
            "SRC_CONST", "SRC_DIFF_IN_CUBE", "SRC_DIFF_IN_BOX", "IND",
            "coeff_indicator", "init"]
 
@@ -23,7 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmodfem_hip.so")
 
 PRISM, TETRA = 5, 7
-COEF_UNIFORM, COEF_KAPPA, COEF_KAPPA_B, COEF_FULL = 0, 1, 2, 3
+COEF_UNIFORM, COEF_KAPPA, COEF_KAPPA_B, COEF_FULL, COEF_KAPPA_S = 0, 1, 2, 3, 4
 SRC_CONST, SRC_DIFF_IN_CUBE, SRC_DIFF_IN_BOX = 0, 1, 2
 # coefficient slots: a[9] b[3] t[3] c q[3] s m
 NCOEF = 21

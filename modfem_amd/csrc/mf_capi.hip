@@ -505,7 +505,7 @@ int mf_problem_set_coefficients(int pid, const int* ind, int coef_model,
                                 int src_model, const double* bf_g) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
-  if (!ind || coef_model < CM_UNIFORM || coef_model > CM_FULL || src_model < SRC_CONST ||
+  if (!ind || coef_model < CM_UNIFORM || coef_model > CM_KAPPA_S || src_model < SRC_CONST ||
       src_model > SRC_DIFF_IN_BOX) {
     set_error("mf_problem_set_coefficients: invalid arguments");
     return -1;
@@ -548,7 +548,8 @@ int mf_problem_set_coefficients(int pid, const int* ind, int coef_model,
     hipFree(P->d_coef);
     P->d_coef = nullptr;
   }
-  size_t per = coef_model == CM_KAPPA ? 1 : coef_model == CM_KAPPA_B ? 4 : coef_model == CM_FULL ? NCOEF : 0;
+  size_t per = coef_model == CM_KAPPA ? 1 : coef_model == CM_KAPPA_B ? 4
+             : coef_model == CM_FULL ? NCOEF : coef_model == CM_KAPPA_S ? 2 : 0;
   if (per) {
     if (!coef_el) {
       set_error("mf_problem_set_coefficients: coef_el required for this model");

@@ -26,7 +26,8 @@ enum CoefModel {
   CM_UNIFORM = 0,  // all 21 values from coef_const
   CM_KAPPA = 1,    // a = kappa_e * I (diffusion branch), rest from coef_const
   CM_KAPPA_B = 2,  // kappa_e and b_e[3]; rest from coef_const
-  CM_FULL = 3      // 21 values per element
+  CM_FULL = 3,     // 21 values per element
+  CM_KAPPA_S = 4   // kappa_e and source s_e; rest from coef_const
 };
 // coefficient slots (same as the oracle): a[9] b[3] t[3] c q[3] s m
 constexpr int C_A = 0, C_B = 9, C_T = 12, C_C = 15, C_Q = 16, C_S = 19, C_M = 20,

@@ -177,13 +177,14 @@ __device__ __forceinline__ void load_coefs(const IntArgs& A, int e, Coefs& C) {
   }
   C.c = A.cc[C_C]; C.s = A.cc[C_S]; C.m = A.cc[C_M];
   const size_t N = A.nel;
-  if (A.coef_model == CM_KAPPA || A.coef_model == CM_KAPPA_B) {
+  if (A.coef_model == CM_KAPPA || A.coef_model == CM_KAPPA_B || A.coef_model == CM_KAPPA_S) {
     double kap = A.coef[e];
     C.a[0] = kap; C.a[4] = kap; C.a[8] = kap;
     C.a[1] = C.a[2] = C.a[3] = C.a[5] = C.a[6] = C.a[7] = 0.0;
     if (A.coef_model == CM_KAPPA_B) {
       C.b[0] = A.coef[N + e]; C.b[1] = A.coef[2 * N + e]; C.b[2] = A.coef[3 * N + e];
     }
+    if (A.coef_model == CM_KAPPA_S) C.s = A.coef[N + e];
   } else if (A.coef_model == CM_FULL) {
 #pragma unroll
     for (int k = 0; k < 9; k++) C.a[k] = A.coef[(C_A + k) * N + e];

@@ -48,6 +48,7 @@ CASES = {
     "poisson_kappa": dict(ind=mf.IND_DIFF, model=mf.COEF_KAPPA),
     "convdiff_kappa_b": dict(ind=mf.IND_CONV_DIFF, model=mf.COEF_KAPPA_B),
     "all_terms_full": dict(ind=mf.IND_ALL, model=mf.COEF_FULL),
+    "poisson_kappa_source": dict(ind=mf.IND_DIFF, model=mf.COEF_KAPPA_S),
 }
 
 
@@ -67,6 +68,9 @@ def make_case(mesh, name, seed=5):
     if spec["model"] == mf.COEF_KAPPA_B:
         ce = np.concatenate([kappa, b[:, 0], b[:, 1], b[:, 2]])
         return spec["ind"], spec["model"], cc, ce, coef_table(nel, kappa, b)
+    if spec["model"] == mf.COEF_KAPPA_S:
+        src = rng.uniform(-1, 1, nel)
+        return spec["ind"], spec["model"], cc, np.concatenate([kappa, src]), coef_table(nel, kappa, s=src)
     full = rng.uniform(-0.3, 0.3, (nel, 21))
     full[:, 0] += 2.0
     full[:, 4] += 2.0
