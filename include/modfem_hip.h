@@ -133,10 +133,22 @@ int mf_set_system(int pid, const double* val, const double* rhs);
 /* y = A x (host arrays; lar_compute_residual_crs with Use_rhs=0 gives -y,
    las_crs_intf.c:788) */
 int mf_spmv(int pid, const double* x, double* y);
-/* Jacobi fill 1/a_ii, lar_fill_preconditioner_crs (las_crs_intf.c:707) */
+/* preconditioner fill: 1 Jacobi 1/a_ii, lar_fill_preconditioner_crs
+   (las_crs_intf.c:707); 2 ILU(0), the block ILU(0) of
+   lar_fill_preconditioner_block (lad_block/las_block_intf.c:1409-2143) for
+   1x1 blocks */
 int mf_fill_precon(int pid, int precon);
+/* ILU(0) elimination order: 0 the reference's DOF order (the reference's
+   factors; as many levels as its dependency chains), 1 multicolour (greedy
+   colouring of the matrix graph; a different ILU(0), few levels) */
+int mf_set_ilu_order(int pid, int order);
+/* the ILU(0) factors in reference CRS positions (strict L, strict U, 1/u_ii
+   on the diagonal), the elimination position of every reference row, and
+   levels[3] = {forward levels, backward levels, colours} */
+int mf_ilu_get(int pid, double* lu, int* elim_pos, int* levels);
 /* restarted left-preconditioned GMRES(krylbas), lsr_mkb_core_gmres
-   (lsd_mkb/lsd_mkb_core/lss_mkb_core.c:2317); precon 0 none, 1 Jacobi.
+   (lsd_mkb/lsd_mkb_core/lss_mkb_core.c:2317); precon 0 none, 1 Jacobi,
+   2 ILU(0) (compreres' BLOCK_ILU branch, lss_mkb_core.c:1947-1997).
    Returns 1 on convergence, 0 otherwise, <0 on error.  x: host array of n
    (input guess unless ini_zero, output solution), or NULL to start from zero
    and keep the solution in device memory only (read it with

@@ -100,6 +100,8 @@ def _declare(L):
         "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
         "mf_true_residual": (I, [I, V, V]),
         "mf_get_solution": (I, [I, V]),
+        "mf_set_ilu_order": (I, [I, I]),
+        "mf_ilu_get": (I, [I, V, V, V]),
         "mf_batch_create": (I, [I]),
         "mf_batch_add": (I, [I, I, I, I, ctypes.c_double]),
         "mf_batch_remove": (I, [I, I]),
@@ -427,6 +429,18 @@ class Problem:
                                         _p(x) if fetch else None, ctypes.byref(it),
                                         ctypes.byref(rr), ctypes.byref(rs)))
         return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+    def set_ilu_order(self, order):
+        check(lib().mf_set_ilu_order(self.h, order))
+
+    def ilu(self):
+        """(factors in reference CRS positions, elimination position per row,
+        (forward levels, backward levels, colours))"""
+        lu = np.zeros(self.nnz)
+        pos = np.zeros(self.n, np.int32)
+        lev = np.zeros(3, np.int32)
+        check(lib().mf_ilu_get(self.h, _p(lu), _p(pos), _p(lev)))
+        return lu, pos, tuple(int(v) for v in lev)
 
     def solution(self):
         """host copy of the last GMRES solution"""

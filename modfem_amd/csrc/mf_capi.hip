@@ -65,6 +65,7 @@ void free_problem_device(Problem& P) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (P.h_stat) hipHostFree(P.h_stat);
+  ilu_free(P);
 }
 
 template <typename T>
@@ -132,6 +133,7 @@ int assemble_problem(Problem& P, int precision) {
     if (launch_faces(P, 0, nullptr, 0, P.nbf)) return -2;
   }
   P.dinv_valid = false;
+  if (P.ilu) P.ilu->valid = false;
   return 0;
 }
 
@@ -676,6 +678,7 @@ int mf_set_system(int pid, const double* val, const double* rhs) {
     MF_HIP(hipMemcpy(P->d_rhs, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   P->dinv_valid = false;  // refilled before the next preconditioned solve
+  if (P->ilu) P->ilu->valid = false;
   return 0;
 }
 
@@ -706,11 +709,16 @@ int mf_fill_precon(int pid, int precon) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
   if (precon == 0) return 0;
-  if (precon != 1) {
-    set_error("mf_fill_precon: only 0 (none) and 1 (Jacobi) are available");
+  if (precon != 1 && precon != 2) {
+    set_error("mf_fill_precon: 0 (none), 1 (Jacobi) and 2 (ILU(0)) are available");
     return -1;
   }
-  if (launch_fill_jacobi(*P)) return -2;
+  if (precon == 2) {
+    if (!P->ilu && ilu_build(*P, P->ilu_order)) return -2;
+    if (ilu_factor(*P)) return -2;
+  } else if (launch_fill_jacobi(*P)) {
+    return -2;
+  }
   MF_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
@@ -719,8 +727,8 @@ int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol, int ini
              int* iters, double* res_rel, double* resid) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
-  if (precon != 0 && precon != 1) {
-    set_error("mf_gmres: precon must be 0 (none) or 1 (Jacobi)");
+  if (precon < 0 || precon > 2) {
+    set_error("mf_gmres: precon must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     return -1;
   }
   if (max_iter < 1 || tol < 0) {
@@ -760,14 +768,21 @@ int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, dou
     set_error("mf_solve_mkb: params is null");
     return -1;
   }
-  const int solver = params[0], krylbas = params[2], precon = params[3], nr_prec = params[4];
+  const int solver = params[0], krylbas = params[2], nr_prec = params[4], ilu_k = params[7];
   const int storage = params[6], max_iter = params[8], conv_type = params[9];
   if (solver != 1) {
     set_error("mf_solve_mkb: only single-level GMRES (solver 1) is available");
     return -1;
   }
-  if (precon != 0 && !(precon == 1 && nr_prec == 1)) {
-    set_error("mf_solve_mkb: available preconditioners are none (0) and one Jacobi sweep (1 1)");
+  // reference codes (lah_intf.h:45-50) -> this library's: NO_PRECON -> 0,
+  // BLOCK_JACOBI with one sweep -> 1, BLOCK_ILU with k = 0 -> 2
+  int precon = -1;
+  if (params[3] == 0) precon = 0;
+  else if (params[3] == 1 && nr_prec == 1) precon = 1;
+  else if (params[3] == 5 && ilu_k == 0) precon = 2;
+  if (precon < 0) {
+    set_error("mf_solve_mkb: available preconditioners are none (0), one Jacobi sweep (1 1) "
+              "and ILU(0) as preconditioner (5 with k = 0)");
     return -1;
   }
   if (storage != -1 && storage != 1 && storage != 2) {
@@ -809,6 +824,40 @@ int mf_gmres_mixed(int pid, int precon, int krylbas, int max_iter, double tol, d
                        res_rel, resid, refinements);
   if (rc >= 0 && x) to_reference_order(P->setup, xd.data(), x);
   return rc;
+}
+
+int mf_set_ilu_order(int pid, int order) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (order != 0 && order != 1) {
+    set_error("mf_set_ilu_order: 0 (reference DOF order) or 1 (multicolour)");
+    return -1;
+  }
+  P->ilu_order = order;
+  ilu_free(*P);
+  return 0;
+}
+
+int mf_ilu_get(int pid, double* lu, int* elim_pos, int* levels) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (!P->ilu && ilu_build(*P, P->ilu_order)) return -2;
+  if (!P->ilu->valid && ilu_factor(*P)) return -2;
+  IluData& L = *P->ilu;
+  MF_HIP(hipStreamSynchronize(stream()));
+  if (lu) {
+    std::vector<double> h(L.nnz);
+    MF_HIP(hipMemcpy(h.data(), L.d_lu, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (long long q = 0; q < L.nnz; q++) lu[L.ref_pos[q]] = h[q];
+  }
+  if (elim_pos)
+    for (int r = 0; r < P->n; r++) elim_pos[r] = L.pos_of_dev[P->setup.iperm[r]];
+  if (levels) {
+    levels[0] = L.nlev_l;
+    levels[1] = L.nlev_u;
+    levels[2] = L.ncolors;
+  }
+  return 0;
 }
 
 int mf_get_solution(int pid, double* x) {

@@ -61,6 +61,24 @@ struct GmresStat {
   double beta1;
 };
 
+// ILU(0) preconditioner on the device (mf_ilu.hip): the pattern in an
+// elimination order (0: the reference's DOF order, 1: multicolour), rows of
+// each forward / backward level, and the factors (strict L, strict U, and
+// 1/u_ii on the diagonal as the reference keeps it).
+struct IluData {
+  int order = 0, n = 0;
+  long long nnz = 0;
+  int nlev_l = 0, nlev_u = 0, ncolors = 0;
+  std::vector<int> lev_ptr_l, lev_ptr_u;   // host: level offsets
+  std::vector<long long> ref_pos;          // host: reference CRS position of each entry
+  std::vector<int> pos_of_dev;             // host: elimination position of a device row
+  int *d_rp = nullptr, *d_ci = nullptr, *d_dg = nullptr, *d_ord = nullptr;
+  int *d_lev_l = nullptr, *d_lev_u = nullptr;
+  long long* d_src = nullptr;              // SELL position of every entry
+  double *d_lu = nullptr, *d_t = nullptr;
+  bool valid = false;                      // factors match the current values
+};
+
 struct Problem {
   // host side
   Mesh mesh;
@@ -107,6 +125,8 @@ struct Problem {
   // preconditioner
   double* d_dinv = nullptr;
   bool dinv_valid = false;     // d_dinv matches the current values
+  IluData* ilu = nullptr;      // precon 2
+  int ilu_order = 0;           // elimination order for ILU(0)
   int batch_refs = 0;          // batches this problem is a member of
   int* d_diag_ptr = nullptr;
   float* d_dinv32 = nullptr;
@@ -180,6 +200,13 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,
 int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, double* h_x,
                 int* iters, double* res_rel, double* resid, int* refinements);
 int launch_spmv(Problem& P, const double* d_x, double* d_y);
+// ILU(0) (mf_ilu.hip): structure for an elimination order, factorization
+// from the current values, and out = (LU)^-1 in (device-order vectors; out
+// may alias in); stop: skip when *stop is set (GMRES stop flag) or null
+int ilu_build(Problem& P, int order);
+int ilu_factor(Problem& P);
+int ilu_apply(Problem& P, const double* in, double* out, const int* stop);
+void ilu_free(Problem& P);
 int bench_copy(long long nbytes, int reps, double* ms);
 int build_tiles(Problem& P);
 int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,

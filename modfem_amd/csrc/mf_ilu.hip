@@ -1,0 +1,249 @@
+// ILU(0) preconditioner on the device, as the reference's block ILU(0) does
+// it for 1x1 blocks (lsd_mkb/lad_block/las_block_intf.c:1409-2143 factor,
+// :3082-3490 forward/backward substitution; used by compreres' BLOCK_ILU
+// branch, lsd_mkb/lsd_mkb_core/lss_mkb_core.c:1947-1997):
+//   for each row i, lower k ascending: a_ik *= d_k (d_k = 1/u_kk), then
+//   a_ij -= a_ik a_kj for j > k in both rows; finally d_i = 1/u_ii.
+// The elimination order is either the reference's DOF order (factors equal
+// the reference's) or a multicolour order (greedy colouring of the matrix
+// graph), which has as many forward levels as colours.  Rows of one level
+// depend only on rows of earlier levels, so each level is one launch with a
+// lane per row; the factorization uses the forward levels, the backward
+// substitution its own levels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "mf_device.h"
+
+namespace mf {
+
+__global__ void k_ilu_load(long long nnz, const long long* src, const double* val, double* lu) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nnz) lu[e] = val[src[e]];
+}
+
+__global__ void k_ilu_factor_level(const int* rows, int count, const int* rp, const int* ci,
+                                   const int* dg, double* lu) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int i = rows[t];
+  const int e_end = rp[i + 1];
+  for (int p = rp[i]; p < dg[i]; p++) {
+    const int k = ci[p];
+    const double aik = lu[p] * lu[dg[k]];
+    lu[p] = aik;
+    int r = p + 1, q = dg[k] + 1;
+    const int q_end = rp[k + 1];
+    while (r < e_end && q < q_end) {
+      const int cr = ci[r], cq = ci[q];
+      if (cr == cq) {
+        lu[r] -= aik * lu[q];
+        r++;
+        q++;
+      } else if (cr < cq) {
+        r++;
+      } else {
+        q++;
+      }
+    }
+  }
+  lu[dg[i]] = 1.0 / lu[dg[i]];
+}
+
+__global__ void k_ilu_fwd_level(const int* rows, int count, const int* rp, const int* ci,
+                                const int* dg, const double* lu, const int* ord,
+                                const double* in, double* t, const int* stop) {
+  if (stop && *stop) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  const int i = rows[q];
+  double s = in[ord[i]];
+  for (int p = rp[i]; p < dg[i]; p++) s -= lu[p] * t[ci[p]];
+  t[i] = s;
+}
+
+__global__ void k_ilu_bwd_level(const int* rows, int count, const int* rp, const int* ci,
+                                const int* dg, const double* lu, const int* ord, double* t,
+                                double* out, const int* stop) {
+  if (stop && *stop) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  const int i = rows[q];
+  double s = t[i];
+  for (int p = dg[i] + 1; p < rp[i + 1]; p++) s -= lu[p] * t[ci[p]];
+  const double z = s * lu[dg[i]];
+  t[i] = z;
+  out[ord[i]] = z;
+}
+
+void ilu_free(Problem& P) {
+  if (!P.ilu) return;
+  IluData& L = *P.ilu;
+  void* ptrs[] = {L.d_rp, L.d_ci, L.d_dg, L.d_ord, L.d_lev_l, L.d_lev_u, L.d_src, L.d_lu, L.d_t};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  delete P.ilu;
+  P.ilu = nullptr;
+}
+
+template <typename T>
+static int up(T** d, const std::vector<T>& h) {
+  MF_HIP(hipMalloc(d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+  if (!h.empty()) MF_HIP(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// rows grouped by level; lev[] per row
+static void group_levels(const std::vector<int>& lev, int nlev, std::vector<int>& ptr,
+                         std::vector<int>& rows) {
+  ptr.assign(nlev + 1, 0);
+  for (int l : lev) ptr[l + 1]++;
+  for (int k = 0; k < nlev; k++) ptr[k + 1] += ptr[k];
+  rows.resize(lev.size());
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (int i = 0; i < (int)lev.size(); i++) rows[fill[lev[i]]++] = i;
+}
+
+int ilu_build(Problem& P, int order) {
+  if (!P.sell || P.setup.perm.empty()) {
+    set_error("ILU(0): needs a SELL problem with its set-up");
+    return -1;
+  }
+  if (order != 0 && order != 1) {
+    set_error("ILU(0): order must be 0 (reference DOF order) or 1 (multicolour)");
+    return -1;
+  }
+  const Setup& S = P.setup;
+  const int n = S.n;
+  // elimination order: ord[k] = device row of ILU position k, pos[device row]
+  std::vector<int> ord(n), pos(n);
+  int ncolors = 0;
+  if (order == 0) {
+    for (int k = 0; k < n; k++) ord[k] = S.iperm[k];  // reference row k
+  } else {
+    // greedy colouring of the (symmetric) pattern in device row order
+    std::vector<int> color(n, -1), mark;
+    for (int i = 0; i < n; i++) {
+      const int r = S.perm[i];
+      mark.assign(ncolors + 1, 0);
+      for (int e = S.row_ptr[r]; e < S.row_ptr[r + 1]; e++) {
+        const int c = color[S.iperm[S.col_ind[e]]];
+        if (c >= 0) mark[c] = 1;
+      }
+      int c = 0;
+      while (c < ncolors && mark[c]) c++;
+      color[i] = c;
+      ncolors = std::max(ncolors, c + 1);
+    }
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return color[a] < color[b]; });
+  }
+  for (int k = 0; k < n; k++) pos[ord[k]] = k;
+  // pattern in ILU positions, columns ascending, with the SELL position of
+  // every entry
+  std::vector<int> rp(n + 1, 0), ci(S.nnz), dg(n, -1);
+  std::vector<long long> src(S.nnz), refp(S.nnz);
+  for (int k = 0; k < n; k++) {
+    const int r = S.perm[ord[k]];
+    rp[k + 1] = rp[k] + (S.row_ptr[r + 1] - S.row_ptr[r]);
+  }
+  std::vector<std::pair<int, int>> buf;  // (column position, reference entry)
+  for (int k = 0; k < n; k++) {
+    const int i = ord[k], r = S.perm[i];
+    const long long base = S.sl_ptr[i >> 6] + (i & 63);
+    buf.clear();
+    for (int e = S.row_ptr[r]; e < S.row_ptr[r + 1]; e++)
+      buf.push_back({pos[S.iperm[S.col_ind[e]]], e});
+    std::sort(buf.begin(), buf.end());
+    for (size_t q = 0; q < buf.size(); q++) {
+      const int e = buf[q].second;
+      ci[rp[k] + q] = buf[q].first;
+      src[rp[k] + q] = base + 64LL * (e - S.row_ptr[r]);
+      refp[rp[k] + q] = e;
+      if (buf[q].first == k) dg[k] = rp[k] + (int)q;
+    }
+    if (dg[k] < 0) {
+      set_error("ILU(0): a row has no diagonal entry");
+      return -1;
+    }
+  }
+  // forward levels (lower dependencies) and backward levels (upper)
+  std::vector<int> levl(n, 0), levu(n, 0);
+  int nl = 0, nu = 0;
+  for (int k = 0; k < n; k++) {
+    int l = 0;
+    for (int p = rp[k]; p < dg[k]; p++) l = std::max(l, levl[ci[p]] + 1);
+    levl[k] = l;
+    nl = std::max(nl, l + 1);
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    int l = 0;
+    for (int p = dg[k] + 1; p < rp[k + 1]; p++) l = std::max(l, levu[ci[p]] + 1);
+    levu[k] = l;
+    nu = std::max(nu, l + 1);
+  }
+  std::vector<int> rows_l, rows_u;
+  ilu_free(P);
+  P.ilu = new IluData();
+  IluData& L = *P.ilu;
+  L.order = order;
+  L.n = n;
+  L.nnz = S.nnz;
+  L.ncolors = ncolors;
+  L.nlev_l = nl;
+  L.nlev_u = nu;
+  L.ref_pos = refp;
+  L.pos_of_dev = pos;
+  group_levels(levl, nl, L.lev_ptr_l, rows_l);
+  group_levels(levu, nu, L.lev_ptr_u, rows_u);
+  if (up(&L.d_rp, rp) || up(&L.d_ci, ci) || up(&L.d_dg, dg) || up(&L.d_ord, ord) ||
+      up(&L.d_lev_l, rows_l) || up(&L.d_lev_u, rows_u) || up(&L.d_src, src))
+    return -2;
+  MF_HIP(hipMalloc(&L.d_lu, std::max<long long>(S.nnz, 1) * sizeof(double)));
+  MF_HIP(hipMalloc(&L.d_t, std::max(n, 1) * sizeof(double)));
+  L.valid = false;
+  return 0;
+}
+
+int ilu_factor(Problem& P) {
+  if (!P.ilu) {
+    set_error("ILU(0): structure not built");
+    return -1;
+  }
+  IluData& L = *P.ilu;
+  hipStream_t s = stream();
+  hipLaunchKernelGGL(k_ilu_load, dim3((unsigned)((L.nnz + 255) / 256)), dim3(256), 0, s, L.nnz,
+                     L.d_src, P.d_val, L.d_lu);
+  for (int l = 0; l < L.nlev_l; l++) {
+    const int c = L.lev_ptr_l[l + 1] - L.lev_ptr_l[l];
+    hipLaunchKernelGGL(k_ilu_factor_level, dim3((c + 255) / 256), dim3(256), 0, s,
+                       L.d_lev_l + L.lev_ptr_l[l], c, L.d_rp, L.d_ci, L.d_dg, L.d_lu);
+  }
+  MF_HIP(hipGetLastError());
+  L.valid = true;
+  return 0;
+}
+
+int ilu_apply(Problem& P, const double* in, double* out, const int* stop) {
+  IluData& L = *P.ilu;
+  hipStream_t s = stream();
+  for (int l = 0; l < L.nlev_l; l++) {
+    const int c = L.lev_ptr_l[l + 1] - L.lev_ptr_l[l];
+    hipLaunchKernelGGL(k_ilu_fwd_level, dim3((c + 255) / 256), dim3(256), 0, s,
+                       L.d_lev_l + L.lev_ptr_l[l], c, L.d_rp, L.d_ci, L.d_dg, L.d_lu, L.d_ord, in,
+                       L.d_t, stop);
+  }
+  for (int l = 0; l < L.nlev_u; l++) {
+    const int c = L.lev_ptr_u[l + 1] - L.lev_ptr_u[l];
+    hipLaunchKernelGGL(k_ilu_bwd_level, dim3((c + 255) / 256), dim3(256), 0, s,
+                       L.d_lev_u + L.lev_ptr_u[l], c, L.d_rp, L.d_ci, L.d_dg, L.d_lu, L.d_ord,
+                       L.d_t, out, stop);
+  }
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mf

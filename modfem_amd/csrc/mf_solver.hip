@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstdlib>
 #include <vector>
 
@@ -180,6 +181,65 @@ __global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
   extern __shared__ double ws[];  // w of the block's slices, 64 per slice
   const int nsl = (A.n + 63) >> 6;
   op_dots_body(A, j, block_slices(nsl), blockIdx.x, gridDim.x, ws);
+}
+
+// ILU-preconditioned Arnoldi step (precon 2), part 1a: w = s_j A V_j into
+// V_{j+1}; the ILU solve then runs in place on w, and k_dots_sell forms the
+// CGS dots with the preconditioned w
+__global__ void __launch_bounds__(SB) k_op_sell(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  const int nsl = (A.n + 63) >> 6;
+  const SliceSet S = block_slices(nsl);
+  const int ns = S.count();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double* x = A.V + (size_t)j * A.ldv;
+  double* w = A.V + (size_t)(j + 1) * A.ldv;
+  const double sj = Scale(A)[j];
+  for (int t = wv; t < ns; t += SW) {
+    const int sl = S.slice(t);
+    const double sum = sell_row_op(A, x, sl, lane);
+    const int r = sl * 64 + lane;
+    if (r < A.n) w[r] = sj * sum;
+  }
+}
+
+__global__ void __launch_bounds__(SB) k_dots_sell(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  const int nsl = (A.n + 63) >> 6;
+  const SliceSet S = block_slices(nsl);
+  const int ns = S.count();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const double* w = A.V + (size_t)(j + 1) * A.ldv + lane;
+  for (int i = wv; i <= j; i += SW) {
+    const double* v = A.V + (size_t)i * A.ldv + lane;
+    double a0 = 0.0, a1 = 0.0;
+    int t = 0;
+    for (; t + 1 < ns; t += 2) {
+      const size_t o0 = (size_t)S.slice(t) * 64, o1 = (size_t)S.slice(t + 1) * 64;
+      a0 += v[o0] * w[o0];
+      a1 += v[o1] * w[o1];
+    }
+    for (; t < ns; t++) {
+      const size_t o = (size_t)S.slice(t) * 64;
+      a0 += v[o] * w[o];
+    }
+    const double d = wave_sum(a0 + a1);
+    if (lane == 0) A.partials[(size_t)i * gridDim.x + blockIdx.x] = d;
+  }
+}
+
+// per-block partial ||V_0||^2 over contiguous slice ranges (ILU restart)
+__global__ void __launch_bounds__(SB) k_sumsq_v0(SolveArgs A) {
+  __shared__ double red[1];
+  const int nsl = (A.n + 63) >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int s0, s1;
+  slice_range(nsl, lb, s0, s1);
+  double acc = 0.0;
+  for (long long r = (long long)s0 * 64 + threadIdx.x; r < (long long)s1 * 64 && r < A.n; r += SB)
+    acc += A.V[r] * A.V[r];
+  block_sum_multi<1>(&acc, 1, red);
+  if (threadIdx.x == 0) A.partials[lb] = red[0];
 }
 
 __global__ void __launch_bounds__(SB) k_spmv_sell(SolveArgs A, const double* x, double* y) {
@@ -736,14 +796,24 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     return -1;
   }
   if (ensure_workspace(P, m)) return -2;
-  if (precon && (!P.d_dinv || !P.dinv_valid)) {
+  const bool ilu = precon == 2;
+  if (ilu) {
+    if (!P.sell) {
+      set_error("GMRES: ILU(0) needs SELL storage");
+      return -1;
+    }
+    if (!P.ilu && ilu_build(P, P.ilu_order)) return -2;
+    if (!P.ilu->valid && ilu_factor(P)) return -2;
+  } else if (precon && (!P.d_dinv || !P.dinv_valid)) {
     if (launch_fill_jacobi(P)) return -2;
   }
   SolveArgs A;
-  fill_solve_args(P, A, precon, m);
+  fill_solve_args(P, A, ilu ? 0 : precon, m);
   A.b = d_b;
   A.x = d_x;
   A.op32 = op32;
+  const int* d_stop = reinterpret_cast<const int*>(reinterpret_cast<const char*>(P.d_stat) +
+                                                   offsetof(GmresStat, stop));
   if (op32 && (!P.sell || !P.d_val32)) {
     set_error("GMRES: the FP32 operator needs SELL storage and its FP32 values");
     return -1;
@@ -781,6 +851,11 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
       hipLaunchKernelGGL(k_residual_sell, dim3(G_op), dim3(SB), 0, s, A);
     else
       hipLaunchKernelGGL(k_residual, dim3(G_op), dim3(SB), 0, s, A);
+    if (ilu) {
+      // V_0 = (LU)^-1 (b - A x), lss_mkb_core.c:1953-1977
+      if (ilu_apply(P, A.V, A.V, nullptr)) return -2;
+      hipLaunchKernelGGL(k_sumsq_v0, dim3(G_op), dim3(SB), 0, s, A);
+    }
     hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(SB), 0, s, A, G_op);
     MF_HIP(hipGetLastError());
     MF_HIP(hipStreamSynchronize(s));
@@ -816,7 +891,19 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     st.rnrm = rnrm2;
     MF_HIP(hipMemcpyAsync(P.d_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
-    for (int j = 0; j < m; j++) launch_arnoldi(G_op, G_up, A, j);
+    if (ilu) {
+      for (int j = 0; j < m; j++) {
+        hipLaunchKernelGGL(k_op_sell, dim3(G_op), dim3(SB), 0, s, A, j);
+        double* w = A.V + (size_t)(j + 1) * A.ldv;
+        if (ilu_apply(P, w, w, d_stop)) return -2;
+        hipLaunchKernelGGL(k_dots_sell, dim3(G_op), dim3(SB), 0, s, A, j);
+        hipLaunchKernelGGL(k_fin_dots, dim3(j + 1), dim3(SB), 0, s, A, j, G_op);
+        hipLaunchKernelGGL(k_update, dim3(G_up), dim3(SB), 0, s, A, j);
+        hipLaunchKernelGGL(k_fin_update, dim3(1), dim3(SB), 0, s, A, j, G_up);
+      }
+    } else {
+      for (int j = 0; j < m; j++) launch_arnoldi(G_op, G_up, A, j);
+    }
     MF_HIP(hipGetLastError());
     MF_HIP(hipMemcpyAsync(H.data(), P.d_H, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     MF_HIP(hipStreamSynchronize(s));

@@ -672,12 +672,73 @@ typedef struct {
   int precon;
   int *diag_ptr;
   double *dinv;
+  double *lu, *tmp; /* precon 2: ILU(0) factors on the pattern, work vector */
 } orc_op;
+
+/* ILU(0) on the CRS pattern in row order, IKJ form (Saad, Iterative Methods
+   for Sparse Linear Systems, 2nd ed., Alg. 10.4) as the reference's block
+   ILU(0) does it for 1x1 blocks (lad_block/las_block_intf.c:1830-2143): for
+   each row i and each k < i of its pattern in ascending order,
+   a_ik = a_ik * d_k with d_k = 1/u_kk, then a_ij -= a_ik a_kj for every
+   j > k present in both rows i and k (the diagonal included); when row i is
+   done its diagonal is replaced by d_i = 1/u_ii.  lu holds the strict lower
+   part of the unit-lower L, the strict upper part of U and d on the
+   diagonal. */
+int orc_ilu0(int n, const int *rp, const int *ci, const double *val, double *lu) {
+  memcpy(lu, val, (size_t)rp[n] * sizeof(double));
+  int *dg = (int *)malloc((size_t)n * sizeof(int));
+  for (int i = 0; i < n; i++) {
+    dg[i] = -1;
+    for (int p = rp[i]; p < rp[i + 1]; p++)
+      if (ci[p] == i) dg[i] = p;
+    if (dg[i] < 0) { free(dg); return -1; }
+  }
+  for (int i = 0; i < n; i++) {
+    for (int p = rp[i]; p < dg[i]; p++) {
+      const int k = ci[p];
+      const double aik = lu[p] * lu[dg[k]];
+      lu[p] = aik;
+      int r = p + 1, q = dg[k] + 1;
+      while (r < rp[i + 1] && q < rp[k + 1]) {
+        if (ci[r] == ci[q]) { lu[r] -= aik * lu[q]; r++; q++; }
+        else if (ci[r] < ci[q]) r++;
+        else q++;
+      }
+    }
+    lu[dg[i]] = 1.0 / lu[dg[i]];
+  }
+  free(dg);
+  return 0;
+}
+
+/* out = U^-1 L^-1 in with the factors of orc_ilu0: forward substitution in
+   row order, then backward with the stored reciprocal diagonal (the
+   reference's lar_perform_rhsub for 1x1 blocks, las_block_intf.c:3082-3490) */
+void orc_ilu_solve(int n, const int *rp, const int *ci, const double *lu, const double *in,
+                   double *out) {
+  for (int i = 0; i < n; i++) {
+    double s = in[i];
+    for (int p = rp[i]; p < rp[i + 1] && ci[p] < i; p++) s -= lu[p] * out[ci[p]];
+    out[i] = s;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    double s = out[i], d = 0.0;
+    for (int p = rp[i]; p < rp[i + 1]; p++) {
+      if (ci[p] > i) s -= lu[p] * out[ci[p]];
+      else if (ci[p] == i) d = lu[p];
+    }
+    out[i] = s * d;
+  }
+}
 
 static void compreres(const orc_op *op, int control, const double *x, double *v) {
   int n = op->n;
   if (op->precon == 0) {
     orc_compute_residual(n, op->rp, op->ci, op->val, control, op->b, x, v);
+  } else if (op->precon == 2) {
+    /* BLOCK_ILU branch (lss_mkb_core.c:1947-1997): V = M^-1 (B - A X) */
+    orc_compute_residual(n, op->rp, op->ci, op->val, control, op->b, x, op->tmp);
+    orc_ilu_solve(n, op->rp, op->ci, op->lu, op->tmp, v);
   } else {
     memcpy(v, x, (size_t)n * sizeof(double));
     orc_bj_iteration(n, op->rp, op->ci, op->val, op->diag_ptr, op->dinv,
@@ -702,8 +763,12 @@ int orc_gmres_tol(int n, const int *row_ptr, const int *col_ind, const double *v
                   double *res_rel_out, double *res_out) {
   const double SMALL = 1.0e-15;
   const double MAX_CONVRATE = 0.999;
-  orc_op op = {n, row_ptr, col_ind, val, b, precon, NULL, NULL};
-  if (precon) {
+  orc_op op = {n, row_ptr, col_ind, val, b, precon, NULL, NULL, NULL, NULL};
+  if (precon == 2) {
+    op.lu = (double *)malloc((size_t)row_ptr[n] * sizeof(double));
+    op.tmp = (double *)malloc((size_t)n * sizeof(double));
+    if (orc_ilu0(n, row_ptr, col_ind, val, op.lu)) { free(op.lu); free(op.tmp); return -1; }
+  } else if (precon) {
     op.diag_ptr = (int *)malloc((size_t)n * sizeof(int));
     op.dinv = (double *)malloc((size_t)n * sizeof(double));
     orc_fill_jacobi(n, row_ptr, col_ind, val, op.diag_ptr, op.dinv);
@@ -796,6 +861,6 @@ int orc_gmres_tol(int n, const int *row_ptr, const int *col_ind, const double *v
   if (res_rel_out) *res_rel_out = res_rel;
   if (res_out) *res_out = resid;
   free(v); free(h); free(giv);
-  free(op.diag_ptr); free(op.dinv);
+  free(op.diag_ptr); free(op.dinv); free(op.lu); free(op.tmp);
   return conv;
 }

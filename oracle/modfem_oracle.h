@@ -122,6 +122,13 @@ int orc_gmres_tol(int n, const int *row_ptr, const int *col_ind, const double *v
                   double tol_rel, double tol_abs, int ini_zero, double *x,
                   int *iters, double *res_rel_out, double *res_out);
 
+/* ILU(0) on the CRS pattern (Saad Alg. 10.4, IKJ; the reference's block
+   ILU(0) with 1x1 blocks) and its forward/backward substitution; precon 2
+   in orc_gmres / orc_gmres_tol selects it */
+int orc_ilu0(int n, const int *rp, const int *ci, const double *val, double *lu);
+void orc_ilu_solve(int n, const int *rp, const int *ci, const double *lu, const double *in,
+                   double *out);
+
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);
 

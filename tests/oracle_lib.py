@@ -42,6 +42,8 @@ def oracle():
             "orc_bj_iteration": (None, [I, V, V, V, V, V, I, V, V]),
             "orc_gmres": (I, [I, V, V, V, V, I, I, I, D, I, V, V, V, V]),
             "orc_gmres_tol": (I, [I, V, V, V, V, I, I, I, D, D, I, V, V, V, V]),
+            "orc_ilu0": (I, [I, V, V, V, V]),
+            "orc_ilu_solve": (None, [I, V, V, V, V, V]),
             "orc_drotg": (None, [V, V, V, V]),
             "orc_set_threads": (None, [I]),
         }
@@ -155,3 +157,30 @@ def oracle_gmres_crs(row_ptr, col_ind, val, rhs, precon=1, krylbas=30, max_iter=
     conv = o.orc_gmres(n, P(row_ptr), P(col_ind), P(val), P(rhs), precon, krylbas, max_iter,
                        tol, 1, P(x), ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs))
     return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+
+def oracle_ilu0(row_ptr, col_ind, val):
+    """ILU(0) factors on the pattern (strict L, strict U, 1/u_ii on the diagonal)"""
+    lu = np.zeros_like(val)
+    rc = oracle().orc_ilu0(row_ptr.shape[0] - 1, P(row_ptr), P(col_ind), P(np.ascontiguousarray(val)), P(lu))
+    assert rc == 0
+    return lu
+
+
+def ilu_product(row_ptr, col_ind, lu, pos=None):
+    """L U (dense) from factors in CRS positions; pos = elimination position
+    of every row (identity when None); rows/columns in elimination order"""
+    n = row_ptr.shape[0] - 1
+    pos = np.arange(n) if pos is None else pos
+    L, U = np.eye(n), np.zeros((n, n))
+    for r in range(n):
+        for p in range(row_ptr[r], row_ptr[r + 1]):
+            i, j = pos[r], pos[col_ind[p]]
+            if j < i:
+                L[i, j] = lu[p]
+            elif j > i:
+                U[i, j] = lu[p]
+            else:
+                U[i, i] = 1.0 / lu[p]
+    return L @ U
+

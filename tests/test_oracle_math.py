@@ -192,3 +192,57 @@ def test_drotg_identities():
         assert abs(abs(r) - np.hypot(a0, b0)) <= 1e-15 * max(1, abs(r))
         assert abs(c.value * a0 + s.value * b0 - r) <= 1e-14 * max(1, abs(r))
         assert abs(-s.value * a0 + c.value * b0) <= 1e-14 * max(1, abs(r))
+
+
+def _tridiagonal(n, seed):
+    rng = np.random.default_rng(seed)
+    A = np.diag(4.0 + rng.uniform(0, 1, n)) + np.diag(rng.uniform(-1, 1, n - 1), 1) + \
+        np.diag(rng.uniform(-1, 1, n - 1), -1)
+    rp, ci, v = [0], [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        ci += list(nz)
+        v += list(A[i, nz])
+        rp.append(len(ci))
+    return A, np.array(rp, np.int32), np.array(ci, np.int32), np.array(v)
+
+
+def test_ilu0_of_tridiagonal_is_the_exact_lu():
+    """no fill-in: ILU(0) = LU, and the ILU solve is the exact solve
+    (Saad, Iterative Methods, Sect. 10.3)"""
+    from oracle_lib import ilu_product, oracle_ilu0
+    A, rp, ci, v = _tridiagonal(60, 3)
+    lu = oracle_ilu0(rp, ci, v)
+    assert np.abs(ilu_product(rp, ci, lu) - A).max() <= 1e-14 * np.abs(A).max()
+    b = np.random.default_rng(4).uniform(-1, 1, 60)
+    x = np.zeros(60)
+    oracle().orc_ilu_solve(60, P(rp), P(ci), P(lu), P(b), P(x))
+    assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-13
+
+
+def test_ilu0_reproduces_the_matrix_on_its_pattern():
+    """defining property of ILU(0): (L U)_ij = a_ij for (i,j) in the pattern,
+    on an assembled convection-diffusion matrix of a jittered prism mesh"""
+    from oracle_lib import ilu_product, oracle_ilu0
+    m = mf.Mesh.box_prism(3, 3, 2, seed=5, jitter=0.2)
+    op = OracleProblem(m.arrays())
+    rng = np.random.default_rng(6)
+    coef = np.zeros((op.nel, 21))
+    coef[:, 0] = coef[:, 4] = coef[:, 8] = np.exp(rng.normal(0, 1, op.nel))
+    coef[:, 9:12] = rng.uniform(-1, 1, (op.nel, 3))
+    coef[:, 19] = 1.0
+    val, _ = op.assemble(mf.IND_CONV_DIFF, coef)
+    lu = oracle_ilu0(op.row_ptr, op.col_ind, val)
+    LU = ilu_product(op.row_ptr, op.col_ind, lu)
+    rows = np.repeat(np.arange(op.n), np.diff(op.row_ptr))
+    err = np.abs(LU[rows, op.col_ind] - val).max()
+    assert err <= 1e-12 * np.abs(val).max(), err
+    m.free()
+
+
+def test_gmres_ilu_dense_solution():
+    A, rp, ci, v = dense_system(80, 11)
+    b = np.random.default_rng(12).uniform(-1, 1, 80)
+    x, info = oracle_gmres_crs(rp, ci, v, b, precon=2, krylbas=20, max_iter=500, tol=1e-12)
+    assert info["converged"]
+    assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-9
