@@ -174,8 +174,9 @@ int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, dou
    throughout): FP64 iterative refinement r = b - A x, x += d, around
    GMRES(krylbas) whose operator reads FP32 matrix values while the Krylov
    basis, Gram-Schmidt, Givens rotations, corrections and residuals stay FP64.
-   Stops when ||D^-1 (b - A x)|| <= tol ||D^-1 b|| in FP64 (D = I when precon
-   is 0): the measure of mf_gmres from a zero guess.  iters = total inner
+   Stops when ||M^-1 (b - A x)|| <= tol ||M^-1 b|| in FP64 (M = I, D or the
+   ILU(0) factors for precon 0, 1, 2): the measure of mf_gmres from a zero
+   guess.  iters = total inner
    iterations, refinements = outer steps; x as in mf_gmres (NULL keeps the
    solution on the device). */
 int mf_gmres_mixed(int pid, int precon, int krylbas, int max_iter, double tol, double* x,

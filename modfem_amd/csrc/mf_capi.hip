@@ -811,8 +811,8 @@ int mf_gmres_mixed(int pid, int precon, int krylbas, int max_iter, double tol, d
                    int* iters, double* res_rel, double* resid, int* refinements) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
-  if (precon != 0 && precon != 1) {
-    set_error("mf_gmres_mixed: precon must be 0 (none) or 1 (Jacobi)");
+  if (precon < 0 || precon > 2) {
+    set_error("mf_gmres_mixed: precon must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
     return -1;
   }
   if (tol <= 0) {

@@ -77,6 +77,22 @@ struct IluData {
   long long* d_src = nullptr;              // SELL position of every entry
   double *d_lu = nullptr, *d_t = nullptr;
   bool valid = false;                      // factors match the current values
+  // substitution copies of the factors, SELL-64 packed per level: forward
+  // (strict L) and backward (strict U, 1/u_ii); slice s of a sweep holds 64
+  // rows of one level, entry k of its row r at ptr[s] + 64 k + r % 64
+  struct Sweep {
+    std::vector<int> lev_sl;               // host: first slice of every level
+    long long* d_ptr = nullptr;            // slices + 1
+    int* d_col = nullptr;                  // elimination positions
+    int* d_src = nullptr;                  // factor entry, -1 padding
+    double* d_val = nullptr;
+    int* d_row = nullptr;                  // elimination position, -1 padding
+    int* d_len = nullptr;
+    double* d_diag = nullptr;              // backward: 1/u_ii per packed row
+    int* d_dpos = nullptr;                 // backward: factor entry of u_ii
+    long long nent = 0;
+    int nrows = 0;
+  } fwd, bwd;
 };
 
 struct Problem {

@@ -9,7 +9,8 @@
 // graph), which has as many forward levels as colours.  Rows of one level
 // depend only on rows of earlier levels, so each level is one launch with a
 // lane per row; the factorization uses the forward levels, the backward
-// substitution its own levels.
+// substitution its own levels.  The substitutions read copies of the
+// factors packed per level in SELL-64 slices, so their loads coalesce.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -53,30 +54,78 @@ __global__ void k_ilu_factor_level(const int* rows, int count, const int* rp, co
   lu[dg[i]] = 1.0 / lu[dg[i]];
 }
 
-__global__ void k_ilu_fwd_level(const int* rows, int count, const int* rp, const int* ci,
-                                const int* dg, const double* lu, const int* ord,
-                                const double* in, double* t, const int* stop) {
-  if (stop && *stop) return;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= count) return;
-  const int i = rows[q];
-  double s = in[ord[i]];
-  for (int p = rp[i]; p < dg[i]; p++) s -= lu[p] * t[ci[p]];
-  t[i] = s;
+// substitution over SELL-packed levels: a wavefront owns a slice of 64
+// rows of the level, lane = row; the factor entries of the slice are read
+// column-major (coalesced)
+// sum_k val_k t[col_k] over the L entries of one packed row, loads issued
+// in groups of 8 so that a group costs about two memory round trips
+__device__ __forceinline__ double packed_row_sum(const int* col, const double* val,
+                                                 long long base, int L, const double* t) {
+  constexpr int CH = 8;
+  double s = 0.0;
+  for (int k0 = 0; k0 < L; k0 += CH) {
+    int c[CH];
+    double v[CH], x[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      const bool ok = k0 + k < L;
+      const long long q = base + 64LL * (ok ? k0 + k : k0);
+      c[k] = col[q];
+      v[k] = ok ? val[q] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++) x[k] = t[c[k]];
+#pragma unroll
+    for (int k = 0; k < CH; k++) s += v[k] * x[k];
+  }
+  return s;
 }
 
-__global__ void k_ilu_bwd_level(const int* rows, int count, const int* rp, const int* ci,
-                                const int* dg, const double* lu, const int* ord, double* t,
-                                double* out, const int* stop) {
+__global__ void __launch_bounds__(256) k_ilu_fwd_sell(int s0, int s1, const long long* ptr,
+                                                      const int* col, const double* val,
+                                                      const int* row, const int* len,
+                                                      const int* ord, const double* in,
+                                                      double* t, const int* stop) {
   if (stop && *stop) return;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= count) return;
-  const int i = rows[q];
-  double s = t[i];
-  for (int p = dg[i] + 1; p < rp[i + 1]; p++) s -= lu[p] * t[ci[p]];
-  const double z = s * lu[dg[i]];
+  const int sl = s0 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (sl >= s1) return;
+  const int pr = sl * 64 + lane;
+  const int i = row[pr];
+  if (i < 0) return;
+  t[i] = in[ord[i]] - packed_row_sum(col, val, ptr[sl] + lane, len[pr], t);
+}
+
+__global__ void __launch_bounds__(256) k_ilu_bwd_sell(int s0, int s1, const long long* ptr,
+                                                      const int* col, const double* val,
+                                                      const int* row, const int* len,
+                                                      const double* diag, const int* ord,
+                                                      double* t, double* out, const int* stop) {
+  if (stop && *stop) return;
+  const int sl = s0 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (sl >= s1) return;
+  const int pr = sl * 64 + lane;
+  const int i = row[pr];
+  if (i < 0) return;
+  const double z = (t[i] - packed_row_sum(col, val, ptr[sl] + lane, len[pr], t)) * diag[pr];
   t[i] = z;
   out[ord[i]] = z;
+}
+
+__global__ void k_ilu_pack(long long nent, const int* src, const double* lu, double* val) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nent) val[q] = src[q] >= 0 ? lu[src[q]] : 0.0;
+}
+
+__global__ void k_ilu_pack_diag(int nrows, const int* dpos, const double* lu, double* diag) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nrows) diag[q] = dpos[q] >= 0 ? lu[dpos[q]] : 0.0;
+}
+
+static void free_sweep(IluData::Sweep& W) {
+  void* ptrs[] = {W.d_ptr, W.d_col, W.d_src, W.d_val, W.d_row, W.d_len, W.d_diag, W.d_dpos};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  W = IluData::Sweep();
 }
 
 void ilu_free(Problem& P) {
@@ -85,6 +134,8 @@ void ilu_free(Problem& P) {
   void* ptrs[] = {L.d_rp, L.d_ci, L.d_dg, L.d_ord, L.d_lev_l, L.d_lev_u, L.d_src, L.d_lu, L.d_t};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  free_sweep(L.fwd);
+  free_sweep(L.bwd);
   delete P.ilu;
   P.ilu = nullptr;
 }
@@ -105,6 +156,55 @@ static void group_levels(const std::vector<int>& lev, int nlev, std::vector<int>
   rows.resize(lev.size());
   std::vector<int> fill(ptr.begin(), ptr.end() - 1);
   for (int i = 0; i < (int)lev.size(); i++) rows[fill[lev[i]]++] = i;
+}
+
+// pack the rows of every level into SELL-64 slices: lower (fwd) or upper
+// (bwd) entries of each row, in ascending column order
+// Rows and columns are elimination positions; the work vector is in
+// elimination order (rows of one colour are contiguous there).
+static int pack_sweep(IluData::Sweep& W, int nlev, const std::vector<int>& lev_ptr,
+                      const std::vector<int>& rows, const std::vector<int>& rp,
+                      const std::vector<int>& ci, const std::vector<int>& dg, bool upper) {
+  std::vector<long long> ptr(1, 0);
+  std::vector<int> prow, plen, pdiag;
+  W.lev_sl.assign(nlev + 1, 0);
+  for (int l = 0; l < nlev; l++) {
+    W.lev_sl[l] = (int)ptr.size() - 1;
+    for (int a = lev_ptr[l]; a < lev_ptr[l + 1]; a += 64) {
+      int mx = 0;
+      for (int q = 0; q < 64; q++) {
+        const int k = a + q < lev_ptr[l + 1] ? rows[a + q] : -1;
+        const int len = k < 0 ? 0 : (upper ? rp[k + 1] - dg[k] - 1 : dg[k] - rp[k]);
+        prow.push_back(k);
+        plen.push_back(len);
+        pdiag.push_back(k < 0 ? -1 : dg[k]);
+        mx = std::max(mx, len);
+      }
+      ptr.push_back(ptr.back() + 64LL * mx);
+    }
+  }
+  W.lev_sl[nlev] = (int)ptr.size() - 1;
+  const long long nent = ptr.back();
+  std::vector<int> col(nent, 0), src(nent, -1);
+  const int nsl = (int)ptr.size() - 1;
+  for (int sl = 0; sl < nsl; sl++)
+    for (int q = 0; q < 64; q++) {
+      const int k = prow[(size_t)sl * 64 + q];
+      if (k < 0) continue;
+      const int first = upper ? dg[k] + 1 : rp[k];
+      for (int e = 0; e < plen[(size_t)sl * 64 + q]; e++) {
+        col[ptr[sl] + 64LL * e + q] = ci[first + e];
+        src[ptr[sl] + 64LL * e + q] = first + e;
+      }
+    }
+  W.nent = nent;
+  W.nrows = (int)prow.size();
+  if (up(&W.d_ptr, ptr) || up(&W.d_col, col) || up(&W.d_src, src) || up(&W.d_row, prow) ||
+      up(&W.d_len, plen) || (upper && up(&W.d_dpos, pdiag)))
+    return -2;
+  MF_HIP(hipMalloc(&W.d_val, std::max<long long>(nent, 1) * sizeof(double)));
+  if (upper) MF_HIP(hipMalloc(&W.d_diag, std::max(W.nrows, 1) * sizeof(double)));
+  return 0;
 }
 
 int ilu_build(Problem& P, int order) {
@@ -204,6 +304,9 @@ int ilu_build(Problem& P, int order) {
     return -2;
   MF_HIP(hipMalloc(&L.d_lu, std::max<long long>(S.nnz, 1) * sizeof(double)));
   MF_HIP(hipMalloc(&L.d_t, std::max(n, 1) * sizeof(double)));
+  if (pack_sweep(L.fwd, nl, L.lev_ptr_l, rows_l, rp, ci, dg, false) ||
+      pack_sweep(L.bwd, nu, L.lev_ptr_u, rows_u, rp, ci, dg, true))
+    return -2;
   L.valid = false;
   return 0;
 }
@@ -222,6 +325,12 @@ int ilu_factor(Problem& P) {
     hipLaunchKernelGGL(k_ilu_factor_level, dim3((c + 255) / 256), dim3(256), 0, s,
                        L.d_lev_l + L.lev_ptr_l[l], c, L.d_rp, L.d_ci, L.d_dg, L.d_lu);
   }
+  for (IluData::Sweep* W : {&L.fwd, &L.bwd})
+    if (W->nent > 0)
+      hipLaunchKernelGGL(k_ilu_pack, dim3((unsigned)((W->nent + 255) / 256)), dim3(256), 0, s,
+                         W->nent, W->d_src, L.d_lu, W->d_val);
+  hipLaunchKernelGGL(k_ilu_pack_diag, dim3((L.bwd.nrows + 255) / 256), dim3(256), 0, s,
+                     L.bwd.nrows, L.bwd.d_dpos, L.d_lu, L.bwd.d_diag);
   MF_HIP(hipGetLastError());
   L.valid = true;
   return 0;
@@ -231,16 +340,16 @@ int ilu_apply(Problem& P, const double* in, double* out, const int* stop) {
   IluData& L = *P.ilu;
   hipStream_t s = stream();
   for (int l = 0; l < L.nlev_l; l++) {
-    const int c = L.lev_ptr_l[l + 1] - L.lev_ptr_l[l];
-    hipLaunchKernelGGL(k_ilu_fwd_level, dim3((c + 255) / 256), dim3(256), 0, s,
-                       L.d_lev_l + L.lev_ptr_l[l], c, L.d_rp, L.d_ci, L.d_dg, L.d_lu, L.d_ord, in,
-                       L.d_t, stop);
+    const int s0 = L.fwd.lev_sl[l], s1 = L.fwd.lev_sl[l + 1];
+    hipLaunchKernelGGL(k_ilu_fwd_sell, dim3((s1 - s0 + 3) / 4), dim3(256), 0, s, s0, s1,
+                       L.fwd.d_ptr, L.fwd.d_col, L.fwd.d_val, L.fwd.d_row, L.fwd.d_len, L.d_ord,
+                       in, L.d_t, stop);
   }
   for (int l = 0; l < L.nlev_u; l++) {
-    const int c = L.lev_ptr_u[l + 1] - L.lev_ptr_u[l];
-    hipLaunchKernelGGL(k_ilu_bwd_level, dim3((c + 255) / 256), dim3(256), 0, s,
-                       L.d_lev_u + L.lev_ptr_u[l], c, L.d_rp, L.d_ci, L.d_dg, L.d_lu, L.d_ord,
-                       L.d_t, out, stop);
+    const int s0 = L.bwd.lev_sl[l], s1 = L.bwd.lev_sl[l + 1];
+    hipLaunchKernelGGL(k_ilu_bwd_sell, dim3((s1 - s0 + 3) / 4), dim3(256), 0, s, s0, s1,
+                       L.bwd.d_ptr, L.bwd.d_col, L.bwd.d_val, L.bwd.d_row, L.bwd.d_len,
+                       L.bwd.d_diag, L.d_ord, L.d_t, out, stop);
   }
   MF_HIP(hipGetLastError());
   return 0;

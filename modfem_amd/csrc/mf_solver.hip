@@ -981,7 +981,13 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
     return -1;
   }
   if (ensure_workspace(P, krylbas)) return -2;
-  if (precon && (!P.d_dinv || !P.dinv_valid) && launch_fill_jacobi(P)) return -2;
+  const bool ilu = precon == 2;
+  if (ilu) {
+    if (!P.ilu && ilu_build(P, P.ilu_order)) return -2;
+    if (!P.ilu->valid && ilu_factor(P)) return -2;
+  } else if (precon && (!P.d_dinv || !P.dinv_valid) && launch_fill_jacobi(P)) {
+    return -2;
+  }
   hipStream_t s = stream();
   const long long n = P.n;
   if (!P.d_val32) MF_HIP(hipMalloc(&P.d_val32, (size_t)P.nnz_sell * sizeof(float)));
@@ -993,10 +999,13 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   MF_HIP(hipMalloc(&d_sum, sizeof(double)));
   MF_HIP(hipMemsetAsync(P.d_x, 0, (size_t)n * sizeof(double), s));
   SolveArgs A;
-  fill_solve_args(P, A, precon, krylbas);
+  fill_solve_args(P, A, ilu ? 0 : precon, krylbas);
   const int G = grid_slices(P);
   const int gx = (int)((n + 255) / 256);
   int total = 0, conv = 0, k = 0, rc = 0;
+  // ILU: the measure is ||(LU)^-1 r||, formed in d_d before each inner solve
+  SolveArgs An = A;
+  An.V = d_d;
   double rho0 = 0.0, rho = 0.0;
   hipEvent_t e0, e1;
   MF_HIP(hipEventCreate(&e0));
@@ -1004,6 +1013,10 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   MF_HIP(hipEventRecord(e0, s));
   for (k = 0; k < 30; k++) {
     hipLaunchKernelGGL(k_refine_residual, dim3(G), dim3(SB), 0, s, A, P.d_rhs, P.d_x, d_r);
+    if (ilu) {
+      if (ilu_apply(P, d_r, d_d, nullptr)) return -2;
+      hipLaunchKernelGGL(k_sumsq_v0, dim3(G), dim3(SB), 0, s, An);
+    }
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(SB), 0, s, P.d_partials, G, d_sum);
     double h_sum = 0.0;
     MF_HIP(hipMemcpyAsync(&h_sum, d_sum, sizeof(double), hipMemcpyDeviceToHost, s));

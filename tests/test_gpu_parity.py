@@ -18,7 +18,7 @@ import pytest
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import modfem_amd as mf  # noqa: E402
-from oracle_lib import OracleProblem  # noqa: E402
+from oracle_lib import OracleProblem, P  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -349,7 +349,7 @@ def test_solve_with_reference_control_file(conv_type):
     p.free()
 
 
-@pytest.mark.parametrize("precon", [1])
+@pytest.mark.parametrize("precon", [1, 2])
 def test_mixed_precision_gmres(precon):
     """FP64 refinement around the FP32-valued operator reaches the plain
     solve's stopping measure, checked independently in FP64 on the host:
@@ -366,8 +366,19 @@ def test_mixed_precision_gmres(precon):
     op = OracleProblem(mesh.arrays())
     v0, r0 = op.assemble(ind, tab)
     A = sp.csr_matrix((v0, op.col_ind, op.row_ptr), shape=(op.n, op.n))
-    d = 1.0 / A.diagonal()
-    rel = np.linalg.norm(d * (r0 - A @ x)) / np.linalg.norm(d * r0)
+    if precon == 1:
+        d = 1.0 / A.diagonal()
+        prec = lambda v: d * v  # noqa: E731
+    else:
+        from oracle_lib import oracle_ilu0, oracle
+        lu = oracle_ilu0(op.row_ptr, op.col_ind, v0)
+
+        def prec(v):
+            out = np.zeros(op.n)
+            oracle().orc_ilu_solve(op.n, P(op.row_ptr), P(op.col_ind), P(lu),
+                                   P(np.ascontiguousarray(v)), P(out))
+            return out
+    rel = np.linalg.norm(prec(r0 - A @ x)) / np.linalg.norm(prec(r0))
     assert rel <= tol * 1.0001, (rel, info)
     assert abs(rel - info["res_rel"]) <= 1e-3 * tol
     xp, infop = p.gmres(precon=precon, krylbas=30, max_iter=5000, tol=tol)
