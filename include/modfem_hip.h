@@ -114,6 +114,11 @@ int mf_problem_set_coefficients(int pid, const int* coeff_ind, int coef_model,
    (utd_util/uts_accel_intf.c:70-103).  precision: 64 or 32 (FP32
    shape-function products with FP64 geometry and FP64 global sums). */
 int mf_integrate_assemble(int pid, int precision);
+/* per boundary face (face-ID order of the mesh): type 1 Dirichlet penalty
+   K += 1e7 phi_j phi_i dA, F += 1e7 g phi_i dA; type 2 Neumann F += g phi_i dA
+   with constant flux g (pds_conv_diff_weakform_std.c:480-522; the type is
+   what pdr_get_bc_type returns for the face's bc flag) */
+int mf_problem_set_face_bc(int pid, const int* bc_type, const double* g);
 /* element / face matrices without assembly, column-major stiff[u*k+w] then
    rhs[k], 42 doubles per entity (apr_num_int_el output layout,
    aps_std_lin_intf.c:1558-1577) */

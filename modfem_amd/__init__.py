@@ -100,6 +100,7 @@ def _declare(L):
         "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
         "mf_true_residual": (I, [I, V, V]),
         "mf_get_solution": (I, [I, V]),
+        "mf_problem_set_face_bc": (I, [I, V, V]),
         "mf_set_ilu_order": (I, [I, I]),
         "mf_ilu_get": (I, [I, V, V, V]),
         "mf_batch_create": (I, [I]),
@@ -429,6 +430,12 @@ class Problem:
                                         _p(x) if fetch else None, ctypes.byref(it),
                                         ctypes.byref(rr), ctypes.byref(rs)))
         return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+    def set_face_bc(self, bc_type, g):
+        """per-face type (1 Dirichlet penalty, 2 Neumann flux) and data"""
+        t = np.ascontiguousarray(bc_type, np.int32)
+        gv = np.ascontiguousarray(g, np.float64)
+        check(lib().mf_problem_set_face_bc(self.h, _p(t), _p(gv)))
 
     def set_ilu_order(self, order):
         check(lib().mf_set_ilu_order(self.h, order))

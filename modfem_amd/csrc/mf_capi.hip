@@ -61,7 +61,7 @@ void free_problem_device(Problem& P) {
                   P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work, P.d_sl_ptr,
                   P.d_scol, P.d_diag_k, P.d_inc_ptr, P.d_inc_el, P.d_inc_loc,
                   P.d_finc_ptr, P.d_finc_f, P.d_finc_loc, P.d_brows, P.d_el_offT,
-                  P.d_erow};
+                  P.d_erow, P.d_bf_type};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (P.h_stat) hipHostFree(P.h_stat);
@@ -570,6 +570,28 @@ int mf_problem_set_coefficients(int pid, const int* ind, int coef_model,
   }
   if (bf_g && P->nbf)
     if (upload(&P->d_bf_g, bf_g, (size_t)P->nbf)) return -2;
+  return 0;
+}
+
+int mf_problem_set_face_bc(int pid, const int* bc_type, const double* g) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (P->nbf == 0) return 0;
+  if (!bc_type || !g) {
+    set_error("mf_problem_set_face_bc: bc_type and g are required");
+    return -1;
+  }
+  for (int f = 0; f < P->nbf; f++)
+    if (bc_type[f] != BC_DIRICHLET && bc_type[f] != BC_NEUMANN) {
+      set_error("mf_problem_set_face_bc: types are 1 (Dirichlet penalty) or 2 (Neumann)");
+      return -1;
+    }
+  if (P->d_bf_type) hipFree(P->d_bf_type);
+  if (P->d_bf_g) hipFree(P->d_bf_g);
+  P->d_bf_type = nullptr;
+  P->d_bf_g = nullptr;
+  if (upload(&P->d_bf_type, bc_type, (size_t)P->nbf)) return -2;
+  if (upload(&P->d_bf_g, g, (size_t)P->nbf)) return -2;
   return 0;
 }
 

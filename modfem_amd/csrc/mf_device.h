@@ -36,6 +36,8 @@ constexpr int C_A = 0, C_B = 9, C_T = 12, C_C = 15, C_Q = 16, C_S = 19, C_M = 20
 enum TermSet { TS_DIFF = 0, TS_DIFF_CONV = 1, TS_ALL = 2 };
 // source models
 enum SrcModel { SRC_CONST = 0, SRC_DIFF_IN_CUBE = 1, SRC_DIFF_IN_BOX = 2 };
+// boundary-face types (pdr_get_bc_type; PDC_BC_DIRI / PDC_BC_NEUM)
+enum BcType { BC_DIRICHLET = 1, BC_NEUMANN = 2 };
 
 // selected-term bits (reference coefficient indicator slots 1..22)
 struct TermMask {
@@ -110,6 +112,7 @@ struct Problem {
   int* d_bf_side = nullptr;
   uint8_t* d_bf_off = nullptr;
   double* d_bf_g = nullptr;
+  int* d_bf_type = nullptr;    // per face BcType (null: all Dirichlet)
   int* d_row_ptr = nullptr;
   int* d_col = nullptr;
   double* d_val = nullptr;     // SELL-64 values when sell, else CRS

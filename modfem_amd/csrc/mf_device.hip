@@ -400,6 +400,7 @@ struct FaceArgs {
   const int* bf_elem;
   const int* bf_side;
   const double* bf_g;
+  const int* bf_type;       // per face: 2 Neumann flux, else Dirichlet penalty
   const long long* sl_ptr;  // SELL-64 slice offsets
   double* val;
   double* rhs;
@@ -426,6 +427,7 @@ __global__ void __launch_bounds__(256) k_faces(FaceArgs A) {
     xn[i][2] = A.coor[3 * (size_t)dof[i] + 2];
   }
   const double gconst = A.bf_g ? A.bf_g[f] : 0.0;
+  const bool neu = A.bf_type && A.bf_type[f] == BC_NEUMANN;
   double K[NS][NS], F[NS];
 #pragma unroll
   for (int i = 0; i < NS; i++) {
@@ -495,6 +497,11 @@ __global__ void __launch_bounds__(256) k_faces(FaceArgs A) {
     }
     double area = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]) * wgt;
     double gv = gconst;
+    if (neu) {  // flux only (pds_conv_diff_weakform_std.c:513-522)
+#pragma unroll
+      for (int i = 0; i < NS; i++) F[i] += gv * phi[i] * area;
+      continue;
+    }
     if (A.src_model != SRC_CONST) {
       double xc[3] = {0, 0, 0};
       if constexpr (NS == 6) {
@@ -605,7 +612,7 @@ int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount
   FaceArgs A;
   A.nel = P.nel; A.nbf = P.nbf; A.coor = P.d_coor; A.el_dof = P.d_el_dof;
   A.el_off = P.d_el_off; A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side;
-  A.bf_g = P.d_bf_g; A.sl_ptr = P.d_sl_ptr; A.val = P.d_val; A.rhs = P.d_rhs;
+  A.bf_g = P.d_bf_g; A.bf_type = P.d_bf_type; A.sl_ptr = P.d_sl_ptr; A.val = P.d_val; A.rhs = P.d_rhs;
   A.src_model = P.src_model; A.write_faces = write_faces; A.fmat = d_fmat;
   A.f0 = f0; A.count = fcount;
   dim3 g((fcount + 255) / 256);
@@ -640,6 +647,7 @@ struct RowArgs {
   const int* bf_elem;
   const int* bf_side;
   const double* bf_g;
+  const int* bf_type;     // per face: 2 Neumann flux, else Dirichlet penalty
   double* val;            // SELL-64
   double* rhs;
 };
@@ -758,6 +766,7 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
   double xn[NS][3];
   gather_nodes<NS>(A.I, e, xn);
   const double gconst = A.bf_g ? A.bf_g[f] : 0.0;
+  const bool neu = A.bf_type && A.bf_type[f] == BC_NEUMANN;
 #pragma unroll
   for (int u = 0; u < NS; u++) Krow[u] = 0;
   Fw = 0;
@@ -823,7 +832,7 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
     }
     const double area = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]) * wgt;
     double gv = gconst;
-    if (A.I.src_model != SRC_CONST) {
+    if (A.I.src_model != SRC_CONST && !neu) {
       double xc[3] = {0, 0, 0};
 #pragma unroll
       for (int i = 0; i < NS; i++) {
@@ -836,6 +845,10 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
 #pragma unroll
     for (int i = 0; i < NS; i++)
       if (i == w) phw = phi[i];
+    if (neu) {
+      Fw += gv * phw * area;
+      continue;
+    }
     const double pen = 1.0e7;
 #pragma unroll
     for (int u = 0; u < NS; u++) Krow[u] += pen * phi[u] * phw * area;
@@ -922,7 +935,7 @@ int launch_assemble_rows(Problem& P, int precision) {
   A.n = P.n;
   A.inc_ptr = P.d_inc_ptr; A.inc_el = P.d_inc_el; A.inc_loc = P.d_inc_loc;
   A.finc_ptr = P.d_finc_ptr; A.finc_f = P.d_finc_f; A.finc_loc = P.d_finc_loc;
-  A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side; A.bf_g = P.d_bf_g;
+  A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side; A.bf_g = P.d_bf_g; A.bf_type = P.d_bf_type;
   A.val = P.d_val; A.rhs = P.d_rhs;
   const int nsl = (P.n + 63) / 64;
   dim3 g((nsl + ROW_WAVES - 1) / ROW_WAVES);
@@ -1025,7 +1038,7 @@ int launch_assemble_two_pass(Problem& P, int precision) {
   A.n = P.n;
   A.inc_ptr = P.d_inc_ptr; A.inc_el = P.d_inc_el; A.inc_loc = P.d_inc_loc;
   A.finc_ptr = P.d_finc_ptr; A.finc_f = P.d_finc_f; A.finc_loc = P.d_finc_loc;
-  A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side; A.bf_g = P.d_bf_g;
+  A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side; A.bf_g = P.d_bf_g; A.bf_type = P.d_bf_type;
   A.val = P.d_val; A.rhs = P.d_rhs;
   const int nsl = (P.n + 63) / 64;
   dim3 g((nsl + ROW_WAVES - 1) / ROW_WAVES), b(64 * ROW_WAVES);

@@ -411,6 +411,23 @@ int orc_face_penalty(int el_type, int side, const double *nc, double g_const,
   return ns;
 }
 
+/* Neumann face (pds_conv_diff_weakform_std.c:513-522): F += g phi_i dA with a
+   constant flux g, no stiffness contribution. */
+int orc_face_neumann(int el_type, int side, const double *nc, double g, double *rhs) {
+  int quad = (el_type == ORC_PRISM && side >= 2);
+  double xg[8], wg[4], eta[3], phi[6], dx[6], dy[6], dz[6], xcoor[3], vn[3];
+  int ng = orc_quadr_2d(quad, xg, wg);
+  int ns = (el_type == ORC_TETRA) ? 4 : 6;
+  for (int ki = 0; ki < ng; ki++) {
+    face_to_elem(el_type, side, &xg[2 * ki], eta);
+    double determ = orc_elem_calc_3d(el_type, 3 + side, eta, nc, phi, dx, dy,
+                                     dz, xcoor, vn);
+    double area = determ * wg[ki];
+    for (int i = 0; i < ns; i++) rhs[i] += g * phi[i] * area;
+  }
+  return ns;
+}
+
 /* ---------------------------------------------------------------------
  * Lists and DOF structures
  * ------------------------------------------------------------------- */

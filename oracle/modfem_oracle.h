@@ -68,6 +68,10 @@ int orc_face_penalty(int el_type, int side, const double *node_coor,
                      double g_const, int src_model, double *stiff,
                      double *rhs);
 
+/* Neumann face with constant flux g (pds_conv_diff_weakform_std.c:513-522) */
+int orc_face_neumann(int el_type, int side, const double *node_coor, double g,
+                     double *rhs);
+
 /* lists and DOF structures (uts_ls_intf.c:102-398, sis_mkb_intf.c:396-1142)
    el_nodes: nel x 6 (tets use the first 4).  Returns number of DOFs.
    dof_node[n]: node index of DOF n (first-appearance order).

@@ -33,6 +33,7 @@ def oracle():
             "orc_elem_calc_3d": (D, [I, I, V, V, V, V, V, V, V, V]),
             "orc_num_int_el": (I, [I, V, V, V, I, V, V]),
             "orc_face_penalty": (I, [I, I, V, D, I, V, V]),
+            "orc_face_neumann": (I, [I, I, V, D, V]),
             "orc_dof_list": (I, [I, V, V, I, V, V]),
             "orc_crs_pattern": (ctypes.c_long, [I, V, V, V, I, V, V]),
             "orc_assembly_tables": (ctypes.c_long, [I, V, V, I, V, V, V, V, V, V]),
@@ -138,6 +139,14 @@ class OracleProblem:
         rhs = np.zeros(6)
         o.orc_face_penalty(int(self.et[e]), int(self.bfs[f]), P(nc), g, src_model, P(sm), P(rhs))
         return sm[: k * k].reshape(k, k), rhs[:k]
+
+    def face_neumann(self, f, g):
+        e = self.bfe[f]
+        k = 4 if self.et[e] == TETRA else 6
+        nc = np.ascontiguousarray(self.coor[self.en[e, :k]].reshape(-1))
+        rhs = np.zeros(6)
+        oracle().orc_face_neumann(int(self.et[e]), int(self.bfs[f]), P(nc), g, P(rhs))
+        return rhs[:k]
 
     def gmres(self, val, rhs, precon=1, krylbas=30, max_iter=1000, tol=1e-8, tol_abs=0.0):
         o = oracle()

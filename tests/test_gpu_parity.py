@@ -387,3 +387,41 @@ def test_mixed_precision_gmres(precon):
     xn, info_n = p.gmres_mixed(precon=precon, krylbas=30, max_iter=5000, tol=tol, fetch=False)
     assert xn is None and np.array_equal(p.solution(), x)
     p.free()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_dirichlet_and_neumann_faces(mode):
+    """per-face types: Dirichlet penalty faces with data g and Neumann faces
+    with flux g (pds_conv_diff_weakform_std.c:480-522), in every assembly
+    mode; the oracle system is built with penalty on every face and the
+    Neumann faces' penalty replaced by their flux through the tables"""
+    mesh = mf.Mesh.box_prism(5, 4, 3, seed=7, jitter=0.2)
+    op = OracleProblem(mesh.arrays())
+    ind, model, cc, ce, tab = make_case(mesh, "poisson_kappa", seed=3)
+    rng = np.random.default_rng(8)
+    types = rng.choice([1, 2], op.nbf).astype(np.int32)
+    g = rng.uniform(-1, 1, op.nbf)
+    p = mf.Problem.from_mesh(mesh)
+    p.set_coefficients(ind, model, coef_const=cc, coef_el=ce)
+    p.set_face_bc(types, g)
+    p.set_assembly_mode(mode)
+    p.integrate_assemble(64)
+    val, rhs = p.system()
+    v0, r0 = op.assemble(ind, tab, bf_g=g)
+    v0, r0 = v0.copy(), r0.copy()
+    k = 6
+    for f in np.nonzero(types == 2)[0]:
+        K, F = op.face(int(f), g=float(g[f]))
+        t = op.table[op.table_ptr[op.nel + f]:op.table_ptr[op.nel + f + 1]]
+        for w in range(k):
+            for u in range(k):
+                v0[t[w + u * k]] -= K[u][w]
+            r0[t[k * k + w]] += op.face_neumann(int(f), float(g[f]))[w] - F[w]
+    assert rel_err(val, v0) <= 1e-12 and rel_err(rhs, r0) <= 1e-12
+    # face matrices: Neumann faces have no stiffness
+    Fm = p.face_matrices()
+    for f in range(op.nbf):
+        if types[f] == 2:
+            assert np.abs(Fm[f, :36]).max() == 0
+            assert np.abs(Fm[f, 36:42] - op.face_neumann(f, float(g[f]))).max() <= 1e-14
+    p.free()

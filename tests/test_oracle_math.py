@@ -246,3 +246,23 @@ def test_gmres_ilu_dense_solution():
     x, info = oracle_gmres_crs(rp, ci, v, b, precon=2, krylbas=20, max_iter=500, tol=1e-12)
     assert info["converged"]
     assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-9
+
+
+def test_neumann_face_closed_form():
+    """F_i = g int phi_i dA: g A/3 per vertex of a flat triangle, g A/4 per
+    vertex of a rectangle, zero off the face"""
+    m = mf.Mesh.box_prism(2, 2, 2, lx=1.0, ly=2.0, lz=3.0, seed=1)
+    op = OracleProblem(m.arrays())
+    for f in range(op.nbf):
+        e, side = op.bfe[f], op.bfs[f]
+        F = op.face_neumann(f, 0.7)
+        nodes = {0: [0, 1, 2], 1: [3, 4, 5], 2: [0, 1, 3, 4], 3: [1, 2, 4, 5], 4: [2, 0, 5, 3]}[side]
+        X = op.coor[op.en[e, :6]]
+        if side < 2:
+            A = 0.5 * np.linalg.norm(np.cross(X[nodes[1]] - X[nodes[0]], X[nodes[2]] - X[nodes[0]]))
+        else:
+            A = np.linalg.norm(X[nodes[1]] - X[nodes[0]]) * np.linalg.norm(X[nodes[2]] - X[nodes[0]])
+        off = [i for i in range(6) if i not in nodes]
+        assert np.abs(F[off]).max() == 0
+        assert np.abs(F[nodes] - 0.7 * A / len(nodes)).max() <= 1e-14 * max(1.0, A), (f, side)
+    m.free()
