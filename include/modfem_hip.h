@@ -147,13 +147,22 @@ int mf_fill_precon(int pid, int precon);
    factors; as many levels as its dependency chains), 1 multicolour (greedy
    colouring of the matrix graph; a different ILU(0), few levels) */
 int mf_set_ilu_order(int pid, int order);
+/* precon 3: block Gauss-Seidel smoothing (BLOCK_GS, lar_perform_BJ_or_GS_
+   iterations_crs, las_crs_intf.c:857-1013, in compreres' smoothing branch,
+   lss_mkb_core.c:2001-2054) with nsweeps sweeps per application, rows
+   updated in a multicolour order (the reference's order is the DOF order
+   within each thread's chunk) */
+int mf_set_gs(int pid, int nsweeps);
+/* the Gauss-Seidel order: position of every reference row, and the colours */
+int mf_gs_get(int pid, int* elim_pos, int* ncolors);
 /* the ILU(0) factors in reference CRS positions (strict L, strict U, 1/u_ii
    on the diagonal), the elimination position of every reference row, and
    levels[3] = {forward levels, backward levels, colours} */
 int mf_ilu_get(int pid, double* lu, int* elim_pos, int* levels);
 /* restarted left-preconditioned GMRES(krylbas), lsr_mkb_core_gmres
    (lsd_mkb/lsd_mkb_core/lss_mkb_core.c:2317); precon 0 none, 1 Jacobi,
-   2 ILU(0) (compreres' BLOCK_ILU branch, lss_mkb_core.c:1947-1997).
+   2 ILU(0) (compreres' BLOCK_ILU branch, lss_mkb_core.c:1947-1997),
+   3 multicolour block Gauss-Seidel (mf_set_gs).
    Returns 1 on convergence, 0 otherwise, <0 on error.  x: host array of n
    (input guess unless ini_zero, output solution), or NULL to start from zero
    and keep the solution in device memory only (read it with

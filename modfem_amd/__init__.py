@@ -102,6 +102,8 @@ def _declare(L):
         "mf_get_solution": (I, [I, V]),
         "mf_problem_set_face_bc": (I, [I, V, V]),
         "mf_set_ilu_order": (I, [I, I]),
+        "mf_set_gs": (I, [I, I]),
+        "mf_gs_get": (I, [I, V, V]),
         "mf_ilu_get": (I, [I, V, V, V]),
         "mf_batch_create": (I, [I]),
         "mf_batch_add": (I, [I, I, I, I, ctypes.c_double]),
@@ -436,6 +438,16 @@ class Problem:
         t = np.ascontiguousarray(bc_type, np.int32)
         gv = np.ascontiguousarray(g, np.float64)
         check(lib().mf_problem_set_face_bc(self.h, _p(t), _p(gv)))
+
+    def set_gs(self, nsweeps):
+        check(lib().mf_set_gs(self.h, nsweeps))
+
+    def gs_order(self):
+        """(position of every reference row in the GS order, colours)"""
+        pos = np.zeros(self.n, np.int32)
+        nc = ctypes.c_int()
+        check(lib().mf_gs_get(self.h, _p(pos), ctypes.byref(nc)))
+        return pos, nc.value
 
     def set_ilu_order(self, order):
         check(lib().mf_set_ilu_order(self.h, order))

@@ -66,6 +66,7 @@ void free_problem_device(Problem& P) {
     if (p) hipFree(p);
   if (P.h_stat) hipHostFree(P.h_stat);
   ilu_free(P);
+  gs_free(P);
 }
 
 template <typename T>
@@ -134,6 +135,7 @@ int assemble_problem(Problem& P, int precision) {
   }
   P.dinv_valid = false;
   if (P.ilu) P.ilu->valid = false;
+  if (P.gs) P.gs->valid = false;
   return 0;
 }
 
@@ -701,6 +703,7 @@ int mf_set_system(int pid, const double* val, const double* rhs) {
   }
   P->dinv_valid = false;  // refilled before the next preconditioned solve
   if (P->ilu) P->ilu->valid = false;
+  if (P->gs) P->gs->valid = false;
   return 0;
 }
 
@@ -731,11 +734,15 @@ int mf_fill_precon(int pid, int precon) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
   if (precon == 0) return 0;
-  if (precon != 1 && precon != 2) {
-    set_error("mf_fill_precon: 0 (none), 1 (Jacobi) and 2 (ILU(0)) are available");
+  if (precon < 1 || precon > 3) {
+    set_error("mf_fill_precon: 0 (none), 1 (Jacobi), 2 (ILU(0)) and 3 (Gauss-Seidel) are available");
     return -1;
   }
-  if (precon == 2) {
+  if (precon == 3) {
+    if (launch_fill_jacobi(*P)) return -2;
+    if (!P->gs && gs_build(*P)) return -2;
+    if (gs_refresh(*P)) return -2;
+  } else if (precon == 2) {
     if (!P->ilu && ilu_build(*P, P->ilu_order)) return -2;
     if (ilu_factor(*P)) return -2;
   } else if (launch_fill_jacobi(*P)) {
@@ -749,8 +756,8 @@ int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol, int ini
              int* iters, double* res_rel, double* resid) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
-  if (precon < 0 || precon > 2) {
-    set_error("mf_gmres: precon must be 0 (none), 1 (Jacobi) or 2 (ILU(0))");
+  if (precon < 0 || precon > 3) {
+    set_error("mf_gmres: precon must be 0 (none), 1 (Jacobi), 2 (ILU(0)) or 3 (Gauss-Seidel)");
     return -1;
   }
   if (max_iter < 1 || tol < 0) {
@@ -802,11 +809,13 @@ int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, dou
   if (params[3] == 0) precon = 0;
   else if (params[3] == 1 && nr_prec == 1) precon = 1;
   else if (params[3] == 5 && ilu_k == 0) precon = 2;
+  else if (params[3] == 2 && nr_prec >= 1) precon = 3;  // BLOCK_GS, nr_prec sweeps
   if (precon < 0) {
-    set_error("mf_solve_mkb: available preconditioners are none (0), one Jacobi sweep (1 1) "
-              "and ILU(0) as preconditioner (5 with k = 0)");
+    set_error("mf_solve_mkb: available preconditioners are none (0), one Jacobi sweep (1 1), "
+              "block Gauss-Seidel (2 k) and ILU(0) as preconditioner (5 with k = 0)");
     return -1;
   }
+  if (precon == 3) P->gs_sweeps = nr_prec;
   if (storage != -1 && storage != 1 && storage != 2) {
     set_error("mf_solve_mkb: storage must be undefined, block (size 1) or CRS");
     return -1;
@@ -846,6 +855,27 @@ int mf_gmres_mixed(int pid, int precon, int krylbas, int max_iter, double tol, d
                        res_rel, resid, refinements);
   if (rc >= 0 && x) to_reference_order(P->setup, xd.data(), x);
   return rc;
+}
+
+int mf_set_gs(int pid, int nsweeps) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (nsweeps < 1) {
+    set_error("mf_set_gs: nsweeps >= 1 required");
+    return -1;
+  }
+  P->gs_sweeps = nsweeps;
+  return 0;
+}
+
+int mf_gs_get(int pid, int* elim_pos, int* ncolors) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (!P->gs && gs_build(*P)) return -2;
+  if (elim_pos)
+    for (int r = 0; r < P->n; r++) elim_pos[r] = P->gs->elim_pos[P->setup.iperm[r]];
+  if (ncolors) *ncolors = P->gs->ncolors;
+  return 0;
 }
 
 int mf_set_ilu_order(int pid, int order) {

@@ -97,6 +97,24 @@ struct IluData {
   } fwd, bwd;
 };
 
+// Multicolour Gauss-Seidel (mf_gs.hip): off-diagonal rows of every colour
+// packed in SELL-64 slices (rows and columns as device rows, values copied
+// from the matrix), swept colour by colour in place
+struct GsData {
+  int ncolors = 0, n = 0;
+  std::vector<int> col_sl;                 // host: first slice of every colour
+  std::vector<int> elim_pos;               // host: position of a device row in the order
+  long long* d_ptr = nullptr;
+  int* d_col = nullptr;
+  long long* d_src = nullptr;              // SELL position, -1 padding
+  double* d_val = nullptr;
+  int* d_row = nullptr;
+  int* d_len = nullptr;
+  long long nent = 0;
+  int nrows = 0;
+  bool valid = false;
+};
+
 struct Problem {
   // host side
   Mesh mesh;
@@ -146,6 +164,8 @@ struct Problem {
   bool dinv_valid = false;     // d_dinv matches the current values
   IluData* ilu = nullptr;      // precon 2
   int ilu_order = 0;           // elimination order for ILU(0)
+  GsData* gs = nullptr;        // precon 3
+  int gs_sweeps = 1;
   int batch_refs = 0;          // batches this problem is a member of
   int* d_diag_ptr = nullptr;
   float* d_dinv32 = nullptr;
@@ -226,6 +246,13 @@ int ilu_build(Problem& P, int order);
 int ilu_factor(Problem& P);
 int ilu_apply(Problem& P, const double* in, double* out, const int* stop);
 void ilu_free(Problem& P);
+int greedy_colors(const Setup& S, std::vector<int>& color);
+// multicolour Gauss-Seidel (mf_gs.hip): structure, value refresh, and
+// nsweeps sweeps on v (device order) with rhs b (null: zero rhs)
+int gs_build(Problem& P);
+int gs_refresh(Problem& P);
+int gs_sweeps(Problem& P, int nsweeps, const double* b, double* v, const int* stop);
+void gs_free(Problem& P);
 int bench_copy(long long nbytes, int reps, double* ms);
 int build_tiles(Problem& P);
 int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,

@@ -158,6 +158,27 @@ static void group_levels(const std::vector<int>& lev, int nlev, std::vector<int>
   for (int i = 0; i < (int)lev.size(); i++) rows[fill[lev[i]]++] = i;
 }
 
+// greedy colouring of the (symmetric) pattern, rows visited in device order
+int greedy_colors(const Setup& S, std::vector<int>& color) {
+  const int n = S.n;
+  int ncolors = 0;
+  color.assign(n, -1);
+  std::vector<int> mark;
+  for (int i = 0; i < n; i++) {
+    const int r = S.perm[i];
+    mark.assign(ncolors + 1, 0);
+    for (int e = S.row_ptr[r]; e < S.row_ptr[r + 1]; e++) {
+      const int c = color[S.iperm[S.col_ind[e]]];
+      if (c >= 0) mark[c] = 1;
+    }
+    int c = 0;
+    while (c < ncolors && mark[c]) c++;
+    color[i] = c;
+    ncolors = std::max(ncolors, c + 1);
+  }
+  return ncolors;
+}
+
 // pack the rows of every level into SELL-64 slices: lower (fwd) or upper
 // (bwd) entries of each row, in ascending column order
 // Rows and columns are elimination positions; the work vector is in
@@ -224,20 +245,8 @@ int ilu_build(Problem& P, int order) {
   if (order == 0) {
     for (int k = 0; k < n; k++) ord[k] = S.iperm[k];  // reference row k
   } else {
-    // greedy colouring of the (symmetric) pattern in device row order
-    std::vector<int> color(n, -1), mark;
-    for (int i = 0; i < n; i++) {
-      const int r = S.perm[i];
-      mark.assign(ncolors + 1, 0);
-      for (int e = S.row_ptr[r]; e < S.row_ptr[r + 1]; e++) {
-        const int c = color[S.iperm[S.col_ind[e]]];
-        if (c >= 0) mark[c] = 1;
-      }
-      int c = 0;
-      while (c < ncolors && mark[c]) c++;
-      color[i] = c;
-      ncolors = std::max(ncolors, c + 1);
-    }
+    std::vector<int> color;
+    ncolors = greedy_colors(S, color);
     std::iota(ord.begin(), ord.end(), 0);
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return color[a] < color[b]; });
   }

@@ -228,6 +228,29 @@ __global__ void __launch_bounds__(SB) k_dots_sell(SolveArgs A, int j) {
   }
 }
 
+// GS-smoothed operator (precon 3, lss_mkb_core.c:2001-2054): with
+// X = s_j V_j, w = X - GS(X; b = 0); the sweeps run in place on w
+__global__ void k_gs_op_init(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  A.V[(size_t)(j + 1) * A.ldv + i] = Scale(A)[j] * A.V[(size_t)j * A.ldv + i];
+}
+
+__global__ void k_gs_op_finish(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  double* w = A.V + (size_t)(j + 1) * A.ldv;
+  w[i] = Scale(A)[j] * A.V[(size_t)j * A.ldv + i] - w[i];
+}
+
+// restart: V_0 = GS(x; b) - x, starting from V_0 = x
+__global__ void k_gs_restart_finish(SolveArgs A) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < A.n) A.V[i] += -1.0 * A.x[i];
+}
+
 // per-block partial ||V_0||^2 over contiguous slice ranges (ILU restart)
 __global__ void __launch_bounds__(SB) k_sumsq_v0(SolveArgs A) {
   __shared__ double red[1];
@@ -796,8 +819,16 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     return -1;
   }
   if (ensure_workspace(P, m)) return -2;
-  const bool ilu = precon == 2;
-  if (ilu) {
+  const bool ilu = precon == 2, gs = precon == 3;
+  if (gs) {
+    if (!P.sell) {
+      set_error("GMRES: Gauss-Seidel needs SELL storage");
+      return -1;
+    }
+    if ((!P.d_dinv || !P.dinv_valid) && launch_fill_jacobi(P)) return -2;
+    if (!P.gs && gs_build(P)) return -2;
+    if (!P.gs->valid && gs_refresh(P)) return -2;
+  } else if (ilu) {
     if (!P.sell) {
       set_error("GMRES: ILU(0) needs SELL storage");
       return -1;
@@ -808,10 +839,11 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     if (launch_fill_jacobi(P)) return -2;
   }
   SolveArgs A;
-  fill_solve_args(P, A, ilu ? 0 : precon, m);
+  fill_solve_args(P, A, (ilu || gs) ? 0 : precon, m);
   A.b = d_b;
   A.x = d_x;
   A.op32 = op32;
+  const int gn = (P.n + 255) / 256;
   const int* d_stop = reinterpret_cast<const int*>(reinterpret_cast<const char*>(P.d_stat) +
                                                    offsetof(GmresStat, stop));
   if (op32 && (!P.sell || !P.d_val32)) {
@@ -847,7 +879,13 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     st.stop = 0;
     st.j_done = 0;
     MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
-    if (P.sell)
+    if (gs) {
+      // V_0 = GS_k(x; b) - x (the smoothing branch, lss_mkb_core.c:2001-2054)
+      MF_HIP(hipMemcpyAsync(A.V, A.x, (size_t)P.n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      if (gs_sweeps(P, P.gs_sweeps, A.b, A.V, nullptr)) return -2;
+      hipLaunchKernelGGL(k_gs_restart_finish, dim3(gn), dim3(256), 0, s, A);
+      hipLaunchKernelGGL(k_sumsq_v0, dim3(G_op), dim3(SB), 0, s, A);
+    } else if (P.sell)
       hipLaunchKernelGGL(k_residual_sell, dim3(G_op), dim3(SB), 0, s, A);
     else
       hipLaunchKernelGGL(k_residual, dim3(G_op), dim3(SB), 0, s, A);
@@ -891,11 +929,17 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     st.rnrm = rnrm2;
     MF_HIP(hipMemcpyAsync(P.d_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
-    if (ilu) {
+    if (ilu || gs) {
       for (int j = 0; j < m; j++) {
-        hipLaunchKernelGGL(k_op_sell, dim3(G_op), dim3(SB), 0, s, A, j);
         double* w = A.V + (size_t)(j + 1) * A.ldv;
-        if (ilu_apply(P, w, w, d_stop)) return -2;
+        if (gs) {
+          hipLaunchKernelGGL(k_gs_op_init, dim3(gn), dim3(256), 0, s, A, j);
+          if (gs_sweeps(P, P.gs_sweeps, nullptr, w, d_stop)) return -2;
+          hipLaunchKernelGGL(k_gs_op_finish, dim3(gn), dim3(256), 0, s, A, j);
+        } else {
+          hipLaunchKernelGGL(k_op_sell, dim3(G_op), dim3(SB), 0, s, A, j);
+          if (ilu_apply(P, w, w, d_stop)) return -2;
+        }
         hipLaunchKernelGGL(k_dots_sell, dim3(G_op), dim3(SB), 0, s, A, j);
         hipLaunchKernelGGL(k_fin_dots, dim3(j + 1), dim3(SB), 0, s, A, j, G_op);
         hipLaunchKernelGGL(k_update, dim3(G_up), dim3(SB), 0, s, A, j);

@@ -748,6 +748,35 @@ void orc_ilu_solve(int n, const int *rp, const int *ci, const double *lu, const 
   }
 }
 
+/* Gauss-Seidel sweeps in a given row order (NULL: natural order): every
+   row takes the newest values of the rows before it in the order and the
+   old values of the rows after it, v_r = d_r (b_r - sum_{j != r} a_rj v_j)
+   with the sum taken as the reference does (vloc -= a v over the row in
+   column order, then + b, then * d; las_crs_intf.c:905-998).  In natural
+   order this is the reference's GS with one thread; the device runs it in a
+   multicolour order, which the order argument reproduces. */
+static const int *g_gs_order = NULL;
+static int g_gs_sweeps = 1;
+
+void orc_set_gs(const int *order, int nsweeps) {
+  g_gs_order = order;
+  g_gs_sweeps = nsweeps;
+}
+
+void orc_gs_sweeps(int n, const int *row_ptr, const int *col_ind, const double *val,
+                   const int *diag_ptr, const double *diag_precon, const int *order,
+                   int nsweeps, int use_rhs, const double *b, double *v) {
+  for (int it = 0; it < nsweeps; it++)
+    for (int q = 0; q < n; q++) {
+      const int r = order ? order[q] : q;
+      double vloc = 0.0;
+      for (int k = row_ptr[r]; k < row_ptr[r + 1]; k++)
+        if (k != diag_ptr[r]) vloc -= val[k] * v[col_ind[k]];
+      if (use_rhs) vloc += b[r];
+      v[r] = vloc * diag_precon[r];
+    }
+}
+
 static void compreres(const orc_op *op, int control, const double *x, double *v) {
   int n = op->n;
   if (op->precon == 0) {
@@ -756,6 +785,12 @@ static void compreres(const orc_op *op, int control, const double *x, double *v)
     /* BLOCK_ILU branch (lss_mkb_core.c:1947-1997): V = M^-1 (B - A X) */
     orc_compute_residual(n, op->rp, op->ci, op->val, control, op->b, x, op->tmp);
     orc_ilu_solve(n, op->rp, op->ci, op->lu, op->tmp, v);
+  } else if (op->precon == 3) {
+    /* smoothing branch with GS sweeps (lss_mkb_core.c:2001-2054) */
+    memcpy(v, x, (size_t)n * sizeof(double));
+    orc_gs_sweeps(n, op->rp, op->ci, op->val, op->diag_ptr, op->dinv, g_gs_order, g_gs_sweeps,
+                  control, op->b, v);
+    for (int i = 0; i < n; i++) v[i] += -1.0 * x[i];
   } else {
     memcpy(v, x, (size_t)n * sizeof(double));
     orc_bj_iteration(n, op->rp, op->ci, op->val, op->diag_ptr, op->dinv,

@@ -133,6 +133,13 @@ int orc_ilu0(int n, const int *rp, const int *ci, const double *val, double *lu)
 void orc_ilu_solve(int n, const int *rp, const int *ci, const double *lu, const double *in,
                    double *out);
 
+/* Gauss-Seidel sweeps in a row order (NULL: natural, the reference's GS
+   with one thread); orc_set_gs selects order and sweeps for precon 3 */
+void orc_gs_sweeps(int n, const int *row_ptr, const int *col_ind, const double *val,
+                   const int *diag_ptr, const double *diag_precon, const int *order,
+                   int nsweeps, int use_rhs, const double *b, double *v);
+void orc_set_gs(const int *order, int nsweeps);
+
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);
 

@@ -44,6 +44,8 @@ def oracle():
             "orc_gmres": (I, [I, V, V, V, V, I, I, I, D, I, V, V, V, V]),
             "orc_gmres_tol": (I, [I, V, V, V, V, I, I, I, D, D, I, V, V, V, V]),
             "orc_ilu0": (I, [I, V, V, V, V]),
+            "orc_gs_sweeps": (None, [I, V, V, V, V, V, V, I, I, V, V]),
+            "orc_set_gs": (None, [V, I]),
             "orc_ilu_solve": (None, [I, V, V, V, V, V]),
             "orc_drotg": (None, [V, V, V, V]),
             "orc_set_threads": (None, [I]),

@@ -95,3 +95,42 @@ def test_control_file_block_ilu():
     x2, info2 = p.gmres(precon=2, krylbas=prm["krylbas"], max_iter=prm["max_iter"], tol=prm["conv_meas"])
     assert info["converged"] and info["iters"] == info2["iters"] and np.array_equal(x, x2)
     p.free()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_gmres_gauss_seidel_matches_oracle_in_the_same_order(sweeps):
+    """precon 3: GS smoothing (BLOCK_GS) in the device's multicolour order;
+    the oracle runs sequential GS in that same order"""
+    from oracle_lib import oracle
+    mesh, p, tab = conv_diff_problem(8, 7, 6, 17)
+    p.set_gs(sweeps)
+    pos, ncol = p.gs_order()
+    assert sorted(pos) == list(range(p.n)) and 2 <= ncol <= 40
+    x, info = p.gmres(precon=3, krylbas=30, max_iter=3000, tol=1e-8)
+    op = OracleProblem(mesh.arrays())
+    v0, r0 = op.assemble(mf.IND_CONV_DIFF, tab)
+    order = np.empty(p.n, np.int32)
+    order[pos] = np.arange(p.n, dtype=np.int32)
+    oracle().orc_set_gs(order.ctypes.data_as(__import__("ctypes").c_void_p), sweeps)
+    try:
+        x0, info0 = op.gmres(v0, r0, precon=3, krylbas=30, max_iter=3000, tol=1e-8)
+    finally:
+        oracle().orc_set_gs(None, 1)
+    assert info0["converged"] and info["converged"]
+    assert abs(info["iters"] - info0["iters"]) <= max(2, info0["iters"] // 20), (info, info0)
+    assert np.abs(x - x0).max() <= 1e-6 * np.abs(x0).max()
+    p.free()
+
+
+def test_control_file_block_gs():
+    """the reference's own GS control file (ctest Laplace_std/mkb.dat: block
+    GS, 2 sweeps) drives the solve"""
+    G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mkb")
+    prm = mf.mkb_read(os.path.join(G, "laplace_std.dat"))
+    assert (prm["precon"], prm["nr_prec"]) == (2, 2)
+    mesh, p, _ = conv_diff_problem(6, 6, 6, 19)
+    x, info = p.solve_mkb(prm)
+    p.set_gs(2)
+    x2, info2 = p.gmres(precon=3, krylbas=prm["krylbas"], max_iter=prm["max_iter"], tol=prm["conv_meas"])
+    assert info["converged"] and info["iters"] == info2["iters"] and np.array_equal(x, x2)
+    p.free()

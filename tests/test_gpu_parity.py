@@ -343,9 +343,8 @@ def test_solve_with_reference_control_file(conv_type):
     if conv_type == 1:
         assert info["resid"] < tol
     assert rel_err(x, x0) <= 1e-5
-    prm_gs = dict(prm, precon=2)
-    with pytest.raises(RuntimeError):
-        p.solve_mkb(prm_gs)
+    with pytest.raises(RuntimeError):  # additive Schwarz is not available
+        p.solve_mkb(dict(prm, precon=3))
     p.free()
 
 

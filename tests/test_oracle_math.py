@@ -266,3 +266,58 @@ def test_neumann_face_closed_form():
         assert np.abs(F[off]).max() == 0
         assert np.abs(F[nodes] - 0.7 * A / len(nodes)).max() <= 1e-14 * max(1.0, A), (f, side)
     m.free()
+
+
+def _gs(A, rp, ci, v, b, x0, order=None, sweeps=1):
+    n = A.shape[0]
+    dp = np.array([rp[i] + list(ci[rp[i]:rp[i + 1]]).index(i) for i in range(n)], np.int32)
+    d = 1.0 / np.diag(A)
+    x = np.array(x0, float)
+    o = None if order is None else np.ascontiguousarray(order, np.int32)
+    oracle().orc_gs_sweeps(n, P(rp), P(ci), P(v), P(dp), P(d), P(o), sweeps, 1, P(b), P(x))
+    return x
+
+
+def test_gauss_seidel_sweeps():
+    """one sweep solves a lower-triangular system exactly (the updated values
+    are used at once), many sweeps converge to A^-1 b on a diagonally
+    dominant system in any row order, and a sweep in a given order equals
+    the sweep in natural order of the permuted system"""
+    A, rp, ci, v = dense_system(40, 21)
+    b = np.random.default_rng(22).uniform(-1, 1, 40)
+    L = np.tril(A)
+    Lrp, Lci, Lv = [0], [], []
+    for i in range(40):
+        nz = np.nonzero(L[i])[0]
+        Lci += list(nz)
+        Lv += list(L[i, nz])
+        Lrp.append(len(Lci))
+    Lrp, Lci, Lv = np.array(Lrp, np.int32), np.array(Lci, np.int32), np.array(Lv)
+    x = _gs(L, Lrp, Lci, Lv, b, np.zeros(40))
+    assert np.abs(x - np.linalg.solve(L, b)).max() <= 1e-13
+    order = np.random.default_rng(23).permutation(40)
+    for o in (None, order):
+        x = _gs(A, rp, ci, v, b, np.zeros(40), o, sweeps=200)
+        assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-12
+    x0 = np.random.default_rng(24).uniform(-1, 1, 40)
+    x1 = _gs(A, rp, ci, v, b, x0, order, sweeps=1)
+    Ap = A[np.ix_(order, order)]
+    rpp, cip, vp = [0], [], []
+    for i in range(40):
+        nz = np.nonzero(Ap[i])[0]
+        cip += list(nz)
+        vp += list(Ap[i, nz])
+        rpp.append(len(cip))
+    # same values up to the order of the terms inside each row sum
+    x2 = _gs(Ap, np.array(rpp, np.int32), np.array(cip, np.int32), np.array(vp), b[order], x0[order])
+    assert np.abs(x1[order] - x2).max() <= 1e-14
+
+
+def test_gmres_gs_dense_solution():
+    A, rp, ci, v = dense_system(60, 31)
+    b = np.random.default_rng(32).uniform(-1, 1, 60)
+    oracle().orc_set_gs(None, 2)
+    x, info = oracle_gmres_crs(rp, ci, v, b, precon=3, krylbas=20, max_iter=500, tol=1e-12)
+    oracle().orc_set_gs(None, 1)
+    assert info["converged"]
+    assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-9
