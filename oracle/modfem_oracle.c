@@ -916,3 +916,245 @@ int orc_gmres_tol(int n, const int *row_ptr, const int *col_ind, const double *v
   free(op.diag_ptr); free(op.dinv); free(op.lu); free(op.tmp);
   return conv;
 }
+
+/* ---------------------------------------------------------------------
+ * DG prism approximation (apd_dg_prism), tensor-product base
+ * ------------------------------------------------------------------- */
+
+/* 1D monomials eta^i, i <= p, and their derivatives
+   (aps_dg_util.c:1371-1410: derivative built as eta*d_{i-1} then times i) */
+static void dg_mono1d(int p, double eta, double *f, double *df) {
+  f[0] = 1.0;
+  if (p > 0) f[1] = eta;
+  for (int i = 2; i <= p; i++) f[i] = eta * f[i - 1];
+  if (!df) return;
+  df[0] = 0.0;
+  if (p > 0) df[1] = 1.0;
+  for (int i = 2; i <= p; i++) df[i] = eta * df[i - 1];
+  for (int i = 2; i <= p; i++) df[i] *= i;
+}
+
+/* complete 2D base of degree p on the triangle in the reference's order
+   (aps_dg_util.c:997-1365): 1, e0, e1, e0 e1, then e0^i e1^j for
+   j = 0,1 (i >= 2, i + j <= p) and j >= 2 (i + j <= p) */
+static int dg_base2d(int p, const double *eta, double *f, double *fx, double *fy) {
+  f[0] = 1.0; f[1] = eta[0]; f[2] = eta[1];
+  fx[0] = 0.0; fx[1] = 1.0; fx[2] = 0.0;
+  fy[0] = 0.0; fy[1] = 0.0; fy[2] = 1.0;
+  if (p == 1) return 3;
+  f[3] = eta[0] * eta[1];
+  fx[3] = eta[1];
+  fy[3] = eta[0];
+  double s[16], ds[16], t[16], dt[16];
+  dg_mono1d(p, eta[0], s, ds);
+  dg_mono1d(p, eta[1], t, dt);
+  int k = 4;
+  for (int j = 0; j < 2; j++)
+    for (int i = 2; i <= p - j; i++) {
+      f[k] = s[i] * t[j]; fx[k] = ds[i] * t[j]; fy[k] = s[i] * dt[j]; k++;
+    }
+  for (int j = 2; j <= p; j++)
+    for (int i = 0; i <= p - j; i++) {
+      f[k] = s[i] * t[j]; fx[k] = ds[i] * t[j]; fy[k] = s[i] * dt[j]; k++;
+    }
+  return k;
+}
+
+/* tensor-product prism base for pdeg = 100*pz + px, px, pz >= 1
+   (apr_shape_fun_3D, aps_dg_util.c:408-992): 1, e0, e1, z, z e0, z e1,
+   then the 2D functions i >= 3 times z^0 and z^1, then all 2D functions
+   times z^j, j >= 2 */
+int orc_dg_shape(int pdeg, const double *eta, double *phi, double *dx, double *dy,
+                 double *dz) {
+  const int px = pdeg % 100, pz = pdeg / 100;
+  if (px < 1 || pz < 1) return -1;
+  double f[64], fx[64], fy[64], u[16], du[16];
+  int n2 = dg_base2d(px, eta, f, fx, fy);
+  dg_mono1d(pz, eta[2], u, du);
+  phi[0] = 1.0; phi[1] = eta[0]; phi[2] = eta[1]; phi[3] = eta[2];
+  phi[4] = eta[2] * eta[0]; phi[5] = eta[2] * eta[1];
+  dx[0] = 0.0; dx[1] = 1.0; dx[2] = 0.0; dx[3] = 0.0; dx[4] = eta[2]; dx[5] = 0.0;
+  dy[0] = 0.0; dy[1] = 0.0; dy[2] = 1.0; dy[3] = 0.0; dy[4] = 0.0; dy[5] = eta[2];
+  dz[0] = 0.0; dz[1] = 0.0; dz[2] = 0.0; dz[3] = 1.0; dz[4] = eta[0]; dz[5] = eta[1];
+  int k = 6;
+  for (int j = 0; j < 2; j++)
+    for (int i = 3; i < n2; i++) {
+      phi[k] = f[i] * u[j]; dx[k] = fx[i] * u[j]; dy[k] = fy[i] * u[j]; dz[k] = f[i] * du[j];
+      k++;
+    }
+  for (int j = 2; j <= pz; j++)
+    for (int i = 0; i < n2; i++) {
+      phi[k] = f[i] * u[j]; dx[k] = fx[i] * u[j]; dy[k] = fy[i] * u[j]; dz[k] = f[i] * du[j];
+      k++;
+    }
+  return k;
+}
+
+/* Gauss-Legendre points and weights on [-1,1] (Newton on P_n) */
+static void gauss_legendre(int n, double *x, double *w) {
+  for (int i = 0; i < n; i++) {
+    double z = cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5)), dp = 0.0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; k++) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1; p1 = p2;
+      }
+      if (n == 1) { p1 = z; p0 = 1.0; }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (fabs(dz) < 1e-17) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; k++) {
+      double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+      p0 = p1; p1 = p2;
+    }
+    if (n == 1) { p1 = z; p0 = 1.0; }
+    dp = n * (z * p1 - p0) / (z * z - 1.0);
+    x[n - 1 - i] = z;
+    w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+  if (n == 1) { x[0] = 0.0; w[0] = 2.0; }
+}
+
+/* symmetric triangle rules of degree 1, 2, 4, 6 (Dunavant, Int. J. Numer.
+   Meth. Engng. 21 (1985) 1129-1148; 1, 3, 6, 12 points), barycentric
+   points (l1, l2, l3), weights summing to 1; the reference selects them by
+   degree (aps_gauss_util.c:27-150) */
+static int dunavant(int order, double *l, double *w) {
+  int n = 0;
+#define ORB3(a, b, ww)                                                    \
+  do {                                                                    \
+    const double A_ = (a), B_ = (b);                                      \
+    l[3 * n] = A_; l[3 * n + 1] = B_; l[3 * n + 2] = B_; w[n++] = (ww);   \
+    l[3 * n] = B_; l[3 * n + 1] = B_; l[3 * n + 2] = A_; w[n++] = (ww);   \
+    l[3 * n] = B_; l[3 * n + 1] = A_; l[3 * n + 2] = B_; w[n++] = (ww);   \
+  } while (0)
+  if (order <= 1) {
+    l[0] = l[1] = l[2] = 1.0 / 3.0; w[0] = 1.0; return 1;
+  } else if (order == 2) {
+    ORB3(0.666666666666667, 0.166666666666667, 0.333333333333333);
+  } else if (order == 4) {
+    ORB3(0.108103018168070, 0.445948490915965, 0.223381589678011);
+    ORB3(0.816847572980459, 0.091576213509771, 0.109951743655322);
+  } else if (order == 6) {
+    ORB3(0.501426509658179, 0.249286745170910, 0.116786275726379);
+    ORB3(0.873821971016996, 0.063089014491502, 0.050844906370207);
+    const double a = 0.053145049844817, b = 0.310352451033784, c = 0.636502499121399;
+    const double p6[6][3] = {{a, b, c}, {b, c, a}, {c, a, b}, {c, b, a}, {a, c, b}, {b, a, c}};
+    for (int q = 0; q < 6; q++) {
+      l[3 * n] = p6[q][0]; l[3 * n + 1] = p6[q][1]; l[3 * n + 2] = p6[q][2];
+      w[n++] = 0.082851075618374;
+    }
+  } else {
+    return -1;
+  }
+#undef ORB3
+  return n;
+}
+
+/* prism quadrature of the DG module (apr_set_quadr_3D, aps_dg_util.c:
+   1415-1496): degree 2 px on the triangle x degree 2 pz on the line; point
+   (l2, l3, z), weight 1/2 w_tri w_line */
+int orc_dg_quadr(int pdeg, double *xg, double *wg) {
+  int ox = 2 * (pdeg % 100), oz = 2 * (pdeg / 100);
+  if (ox == 0) ox = 1;
+  if (oz == 0) oz = 1;
+  double l[3 * 16], wt[16], xz[12], wz[12];
+  int n2 = dunavant(ox, l, wt);
+  if (n2 < 0) return -1;
+  int nz = oz / 2 + 1;
+  gauss_legendre(nz, xz, wz);
+  for (int ki = 0; ki < nz; ki++)
+    for (int kj = 0; kj < n2; kj++) {
+      const int g = ki * n2 + kj;
+      xg[3 * g] = l[3 * kj + 1];
+      xg[3 * g + 1] = l[3 * kj + 2];
+      xg[3 * g + 2] = xz[ki];
+      wg[g] = 0.5 * wt[kj] * wz[ki];
+    }
+  return nz * n2;
+}
+
+/* DG element integral (apr_num_int_el, aps_dg_prism_intf.c:1239-1728),
+   nreq = 1, column-major stiff[j*ns + i] (i: test function):
+     m phi_j phi_i + sum_rc a_rc d_c phi_j d_r phi_i (full branch when
+     ind[10], diagonal-free 2D branch when only ind[2]) + sum_k b_k phi_j
+     d_k phi_i + c phi_j phi_i;  rhs: q . grad phi_i + s phi_i.
+   The convection term carries the derivative on the test function (the
+   reference's DG form), unlike the std-linear module. */
+int orc_dg_num_int_el(int pdeg, const double *nc, const int *ind, const double *coef,
+                      int src_model, double *stiff, double *rhs) {
+  double xg[3 * 64 * 12], wg[64 * 12];
+  const int ng = orc_dg_quadr(pdeg, xg, wg);
+  if (ng < 0) return -1;
+  double phi[128], dx[128], dy[128], dz[128];
+  int ns = 0;
+  const double *a = coef + ORC_C_A, *b = coef + ORC_C_B, *q = coef + ORC_C_Q;
+  for (int ki = 0; ki < ng; ki++) {
+    const double *eta = &xg[3 * ki];
+    ns = orc_dg_shape(pdeg, eta, phi, dx, dy, dz);
+    double gp[6], gdx[6], gdy[6], gdz[6];
+    gp[0] = (1.0 - eta[0] - eta[1]) * (1.0 - eta[2]) / 2.0;
+    gp[1] = eta[0] * (1.0 - eta[2]) / 2.0;
+    gp[2] = eta[1] * (1.0 - eta[2]) / 2.0;
+    gp[3] = (1.0 - eta[0] - eta[1]) * (1.0 + eta[2]) / 2.0;
+    gp[4] = eta[0] * (1.0 + eta[2]) / 2.0;
+    gp[5] = eta[1] * (1.0 + eta[2]) / 2.0;
+    gdx[0] = -(1.0 - eta[2]) / 2.0; gdx[1] = (1.0 - eta[2]) / 2.0; gdx[2] = 0.0;
+    gdx[3] = -(1.0 + eta[2]) / 2.0; gdx[4] = (1.0 + eta[2]) / 2.0; gdx[5] = 0.0;
+    gdy[0] = -(1.0 - eta[2]) / 2.0; gdy[1] = 0.0; gdy[2] = (1.0 - eta[2]) / 2.0;
+    gdy[3] = -(1.0 + eta[2]) / 2.0; gdy[4] = 0.0; gdy[5] = (1.0 + eta[2]) / 2.0;
+    gdz[0] = -(1.0 - eta[0] - eta[1]) / 2.0; gdz[1] = -eta[0] / 2.0; gdz[2] = -eta[1] / 2.0;
+    gdz[3] = (1.0 - eta[0] - eta[1]) / 2.0; gdz[4] = eta[0] / 2.0; gdz[5] = eta[1] / 2.0;
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Ji[9], xc[3] = {0, 0, 0};
+    for (int i = 0; i < 6; i++) {
+      J[0] += nc[3 * i] * gdx[i]; J[1] += nc[3 * i] * gdy[i]; J[2] += nc[3 * i] * gdz[i];
+      J[3] += nc[3 * i + 1] * gdx[i]; J[4] += nc[3 * i + 1] * gdy[i]; J[5] += nc[3 * i + 1] * gdz[i];
+      J[6] += nc[3 * i + 2] * gdx[i]; J[7] += nc[3 * i + 2] * gdy[i]; J[8] += nc[3 * i + 2] * gdz[i];
+      xc[0] += nc[3 * i] * gp[i]; xc[1] += nc[3 * i + 1] * gp[i]; xc[2] += nc[3 * i + 2] * gp[i];
+    }
+    const double det = orc_mat3_inv(J, Ji);
+    for (int i = 0; i < ns; i++) {
+      const double ax = dx[i] * Ji[0] + dy[i] * Ji[3] + dz[i] * Ji[6];
+      const double ay = dx[i] * Ji[1] + dy[i] * Ji[4] + dz[i] * Ji[7];
+      const double az = dx[i] * Ji[2] + dy[i] * Ji[5] + dz[i] * Ji[8];
+      dx[i] = ax; dy[i] = ay; dz[i] = az;
+    }
+    const double vol = fabs(det) * wg[ki];
+    double s = coef[ORC_C_S];
+    if (src_model != ORC_SRC_CONST) {
+      double u, lapl;
+      exact_sol(src_model, xc, &u, &lapl);
+      s = -lapl;
+    }
+    for (int j = 0; j < ns; j++)
+      for (int i = 0; i < ns; i++) {
+        double k = 0.0;
+        if (ind[ORC_IND_M]) k += coef[ORC_C_M] * phi[j] * phi[i] * vol;
+        if (ind[ORC_IND_AZZ])
+          k += (a[0] * dx[j] * dx[i] + a[1] * dy[j] * dx[i] + a[2] * dz[j] * dx[i] +
+                a[3] * dx[j] * dy[i] + a[4] * dy[j] * dy[i] + a[5] * dz[j] * dy[i] +
+                a[6] * dx[j] * dz[i] + a[7] * dy[j] * dz[i] + a[8] * dz[j] * dz[i]) * vol;
+        else if (ind[ORC_IND_AXX])
+          k += (a[0] * dx[j] * dx[i] + a[1] * dy[j] * dx[i] + a[3] * dx[j] * dy[i] +
+                a[4] * dy[j] * dy[i]) * vol;
+        if (ind[ORC_IND_BZ])
+          k += (b[0] * phi[j] * dx[i] + b[1] * phi[j] * dy[i] + b[2] * phi[j] * dz[i]) * vol;
+        else if (ind[ORC_IND_BX])
+          k += (b[0] * phi[j] * dx[i] + b[1] * phi[j] * dy[i]) * vol;
+        if (ind[ORC_IND_C]) k += coef[ORC_C_C] * phi[j] * phi[i] * vol;
+        stiff[j * ns + i] += k;
+      }
+    for (int i = 0; i < ns; i++) {
+      double f = 0.0;
+      if (ind[ORC_IND_QZ]) f += (q[0] * dx[i] + q[1] * dy[i] + q[2] * dz[i]) * vol;
+      else if (ind[ORC_IND_QX]) f += (q[0] * dx[i] + q[1] * dy[i]) * vol;
+      if (ind[ORC_IND_S]) f += s * phi[i] * vol;
+      rhs[i] += f;
+    }
+  }
+  return ns;
+}

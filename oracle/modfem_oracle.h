@@ -140,6 +140,15 @@ void orc_gs_sweeps(int n, const int *row_ptr, const int *col_ind, const double *
                    int nsweeps, int use_rhs, const double *b, double *v);
 void orc_set_gs(const int *order, int nsweeps);
 
+/* DG prism approximation, tensor base pdeg = 100 pz + px (apd_dg_prism):
+   shape functions, prism quadrature, element integral (nreq = 1; column-
+   major stiff[j*ns + i], i = test function; arrays sized for ns <= 128) */
+int orc_dg_shape(int pdeg, const double *eta, double *phi, double *dx, double *dy,
+                 double *dz);
+int orc_dg_quadr(int pdeg, double *xg, double *wg);
+int orc_dg_num_int_el(int pdeg, const double *node_coor, const int *ind, const double *coef,
+                      int src_model, double *stiff, double *rhs);
+
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);
 

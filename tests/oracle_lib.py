@@ -48,6 +48,9 @@ def oracle():
             "orc_set_gs": (None, [V, I]),
             "orc_ilu_solve": (None, [I, V, V, V, V, V]),
             "orc_drotg": (None, [V, V, V, V]),
+            "orc_dg_shape": (I, [I, V, V, V, V, V]),
+            "orc_dg_quadr": (I, [I, V, V]),
+            "orc_dg_num_int_el": (I, [I, V, V, V, I, V, V]),
             "orc_set_threads": (None, [I]),
         }
         for k, (r, a) in sig.items():
@@ -69,8 +72,12 @@ def ref(name):
     """a reference subset library, loaded lazily-bound: symbols of modules
     that are not compiled in stay unresolved and are never called"""
     if name not in _refs:
-        _refs[name] = ctypes.CDLL(os.path.join(REF_DIR, name),
-                                  mode=os.RTLD_LAZY | os.RTLD_GLOBAL)
+        mode = os.RTLD_LAZY | os.RTLD_GLOBAL
+        if name == "libref_dg.so":
+            # the DG module defines the same apr_* names as the std-linear
+            # one: bind its own calls to its own definitions
+            mode = os.RTLD_LAZY | os.RTLD_LOCAL | os.RTLD_DEEPBIND
+        _refs[name] = ctypes.CDLL(os.path.join(REF_DIR, name), mode=mode)
     return _refs[name]
 
 
@@ -195,3 +202,31 @@ def ilu_product(row_ptr, col_ind, lu, pos=None):
                 U[i, i] = 1.0 / lu[p]
     return L @ U
 
+
+
+def dg_shape(pdeg, eta):
+    """oracle DG prism shape functions at one point: (phi, dx, dy, dz)"""
+    out = [np.zeros(128) for _ in range(4)]
+    e = np.ascontiguousarray(eta, np.float64)
+    ns = oracle().orc_dg_shape(pdeg, P(e), *[P(a) for a in out])
+    assert ns > 0
+    return [a[:ns] for a in out]
+
+
+def dg_quadr(pdeg):
+    xg, wg = np.zeros(3 * 768), np.zeros(768)
+    ng = oracle().orc_dg_quadr(pdeg, P(xg), P(wg))
+    assert ng > 0
+    return xg[:3 * ng].reshape(ng, 3), wg[:ng]
+
+
+def dg_num_int_el(pdeg, nc, ind, coef, src_model=0):
+    """oracle DG element integral: (stiff[ns, ns] column-major as written,
+    i.e. stiff[j, i] = entry for trial j, test i; rhs[ns])"""
+    st, rhs = np.zeros(128 * 128), np.zeros(128)
+    ns = oracle().orc_dg_num_int_el(pdeg, P(np.ascontiguousarray(nc, np.float64)),
+                                    P(np.ascontiguousarray(ind, np.int32)),
+                                    P(np.ascontiguousarray(coef, np.float64)), src_model,
+                                    P(st), P(rhs))
+    assert ns > 0
+    return st[:ns * ns].reshape(ns, ns), rhs[:ns]

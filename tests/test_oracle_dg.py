@@ -1,0 +1,225 @@
+"""The oracle's DG prism element integral (apd_dg_prism), pinned three ways:
+- shape functions and the prism quadrature against the reference's own
+  apr_shape_fun_3D / apr_set_quadr_3D compiled into oracle/_ref
+  (aps_dg_util.c:408-1496, aps_gauss_util.c:27-150), bit-exact for the
+  monomials;
+- the element matrices on affine prisms against closed-form integrals of
+  monomials over the reference prism (exact: the rule's degree covers every
+  integrand), with exponents read off the shape functions themselves;
+- the Jacobian, inverse and derivative transformation on distorted prisms
+  against the reference's apr_elem_calc_3D (aps_dg_util.c:112-368).
+"""
+import ctypes
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from oracle_lib import P, dg_num_int_el, dg_quadr, dg_shape, ref, ref_available  # noqa: E402
+
+PDEGS = [101, 102, 103, 201, 202, 203, 301, 302, 303]
+REF_NODES = np.array([[0, 0, -1], [1, 0, -1], [0, 1, -1], [0, 0, 1], [1, 0, 1], [0, 1, 1]], float)
+need_dg = pytest.mark.skipif(not ref_available("libref_dg.so"), reason="oracle/_ref not built")
+TENSOR = 1  # APC_BASE_TENSOR_DG, include/aph_intf.h:122
+
+# slots of the coefficient-presence vector and of the coefficient vector
+IND_M, IND_AXX, IND_AZZ, IND_BX, IND_BZ, IND_C, IND_QX, IND_QZ, IND_S = 1, 2, 10, 11, 13, 17, 19, 21, 22
+C_A, C_B, C_C, C_Q, C_S, C_M = 0, 9, 15, 16, 19, 20
+
+
+def ref_dg():
+    L = ref("libref_dg.so")
+    L.apr_shape_fun_3D.restype = ctypes.c_int
+    L.apr_shape_fun_3D.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
+    L.apr_set_quadr_3D.restype = ctypes.c_int
+    L.apr_set_quadr_3D.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 4
+    L.apr_elem_calc_3D.restype = ctypes.c_double
+    L.apr_elem_calc_3D.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int] + \
+        [ctypes.c_void_p] * 13
+    L.apr_gauss_init.restype = None
+    L.apr_gauss_init()
+    return L
+
+
+def ref_shape(L, pdeg, eta):
+    out = [np.zeros(200) for _ in range(4)]
+    e = np.ascontiguousarray(eta, float)
+    ns = L.apr_shape_fun_3D(TENSOR, pdeg, P(e), *[P(a) for a in out])
+    return [a[:ns] for a in out]
+
+
+def exponents(pdeg):
+    """(a, b, c) of each shape function e0^a e1^b z^c, read off its value at
+    (2, 3, 5), where every monomial takes a distinct value"""
+    phi = dg_shape(pdeg, [2.0, 3.0, 5.0])[0]
+    ex = []
+    for v in phi:
+        v = int(round(v))
+        abc = []
+        for pr in (2, 3, 5):
+            k = 0
+            while v % pr == 0:
+                v //= pr
+                k += 1
+            abc.append(k)
+        assert v == 1
+        ex.append(tuple(abc))
+    return ex
+
+
+def mono_int(a, b, c):
+    """integral of e0^a e1^b z^c over {e0, e1 >= 0, e0 + e1 <= 1} x [-1, 1]"""
+    if c % 2:
+        return 0.0
+    return math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2) * 2.0 / (c + 1)
+
+
+@need_dg
+@pytest.mark.parametrize("pdeg", PDEGS)
+def test_shape_functions_match_reference(pdeg):
+    L = ref_dg()
+    rng = np.random.default_rng(pdeg)
+    for _ in range(8):
+        eta = np.array([rng.uniform(0, 0.6), rng.uniform(0, 0.4), rng.uniform(-1, 1)])
+        mine, theirs = dg_shape(pdeg, eta), ref_shape(L, pdeg, eta)
+        assert len(mine[0]) == len(theirs[0])
+        for m, t in zip(mine, theirs):
+            assert np.array_equal(m, t)
+    px, pz = pdeg % 100, pdeg // 100
+    assert len(mine[0]) == (px + 1) * (px + 2) // 2 * (pz + 1)
+
+
+@need_dg
+@pytest.mark.parametrize("pdeg", PDEGS)
+def test_quadrature_matches_reference(pdeg):
+    L = ref_dg()
+    xg, wg = np.zeros(3000), np.zeros(1000)
+    ng = ctypes.c_int(0)
+    pv = np.array([pdeg], np.int32)
+    L.apr_set_quadr_3D(TENSOR, P(pv), ctypes.byref(ng), P(xg), P(wg))
+    x, w = dg_quadr(pdeg)
+    assert len(w) == ng.value
+    # 15-digit literals in the reference, Newton-computed Gauss-Legendre here
+    assert np.abs(x - xg[:3 * ng.value].reshape(-1, 3)).max() <= 2e-15
+    assert np.abs(w - wg[:ng.value]).max() <= 5e-15
+
+
+@pytest.mark.parametrize("pdeg", PDEGS)
+def test_quadrature_is_exact_to_its_degree(pdeg):
+    x, w = dg_quadr(pdeg)
+    px, pz = pdeg % 100, pdeg // 100
+    for a in range(2 * px + 1):
+        for b in range(2 * px + 1 - a):
+            for c in range(2 * pz + 1):
+                q = np.sum(w * x[:, 0] ** a * x[:, 1] ** b * x[:, 2] ** c)
+                assert abs(q - mono_int(a, b, c)) <= 1e-13, (a, b, c)
+
+
+def all_terms(rng, full=True):
+    ind = np.zeros(23, np.int32)
+    ind[[IND_M, IND_AXX, IND_BX, IND_C, IND_QX, IND_S]] = 1
+    if full:
+        ind[[IND_AZZ, IND_BZ, IND_QZ]] = 1
+    coef = np.zeros(21)
+    coef[C_A:C_A + 9] = rng.uniform(-0.3, 0.3, 9) + np.eye(3).ravel() * 2.0
+    coef[C_B:C_B + 3] = rng.uniform(-1, 1, 3)
+    coef[C_C], coef[C_M], coef[C_S] = rng.uniform(0.5, 1.5, 3)
+    coef[C_Q:C_Q + 3] = rng.uniform(-1, 1, 3)
+    return ind, coef
+
+
+@pytest.mark.parametrize("pdeg", PDEGS)
+@pytest.mark.parametrize("full", [True, False])
+def test_affine_prism_matches_closed_form(pdeg, full):
+    rng = np.random.default_rng(pdeg + 7 * full)
+    A = np.eye(3) * rng.uniform(0.5, 2.0, 3) + rng.uniform(-0.3, 0.3, (3, 3))
+    x0 = rng.uniform(-1, 1, 3)
+    nc = x0 + REF_NODES @ A.T
+    ind, coef = all_terms(rng, full)
+    K, F = dg_num_int_el(pdeg, nc, ind, coef)
+    ex = exponents(pdeg)
+    ns = len(ex)
+    det = abs(np.linalg.det(A))
+    Ai = np.linalg.inv(A)  # Ai[k, c] = d eta_k / d x_c
+    a = coef[C_A:C_A + 9].reshape(3, 3)  # a[r, c]: d_c on trial, d_r on test
+    b, q = coef[C_B:C_B + 3], coef[C_Q:C_Q + 3]
+    if not full:  # the 2D branches: a_xx, a_xy, a_yx, a_yy and b_x, b_y, q_x, q_y only
+        a = a.copy()
+        a[2, :] = a[:, 2] = 0.0
+        b, q = b.copy(), q.copy()
+        b[2] = q[2] = 0.0
+    G = Ai @ a.T @ Ai.T  # G[k, l] multiplies d_k phi_j d_l phi_i
+
+    def deriv(e, k):  # d/d eta_k of a monomial: (coefficient, exponents)
+        e = list(e)
+        c = e[k]
+        e[k] = max(c - 1, 0)
+        return c, tuple(e)
+
+    def integ(c1, e1, c2, e2):
+        if c1 == 0 or c2 == 0:
+            return 0.0
+        return c1 * c2 * mono_int(e1[0] + e2[0], e1[1] + e2[1], e1[2] + e2[2])
+
+    Kx = np.zeros((ns, ns))
+    Fx = np.zeros(ns)
+    for j in range(ns):
+        for i in range(ns):
+            mij = mono_int(*[ex[i][t] + ex[j][t] for t in range(3)])
+            v = (coef[C_M] + coef[C_C]) * mij
+            for k in range(3):
+                ck, ek = deriv(ex[j], k)
+                for l in range(3):
+                    cl, el = deriv(ex[i], l)
+                    v += G[k, l] * integ(ck, ek, cl, el)
+            bt = Ai @ b  # sum_k b_k d_k = sum_l (Ai b)_l d eta_l
+            for l in range(3):
+                cl, el = deriv(ex[i], l)
+                v += bt[l] * integ(1, ex[j], cl, el)
+            Kx[j, i] = v * det
+    qt = Ai @ q
+    for i in range(ns):
+        f = coef[C_S] * mono_int(*ex[i])
+        for l in range(3):
+            cl, el = deriv(ex[i], l)
+            f += qt[l] * (cl * mono_int(*el) if cl else 0.0)
+        Fx[i] = f * det
+    assert np.abs(K - Kx).max() <= 1e-12 * np.abs(Kx).max()
+    assert np.abs(F - Fx).max() <= 1e-12 * np.abs(Fx).max()
+
+
+@need_dg
+@pytest.mark.parametrize("pdeg", [101, 202, 303, 102])
+def test_distorted_prism_geometry_matches_reference(pdeg):
+    """on a prism whose quads are not planar the Jacobian varies; the
+    oracle's matrix equals the one built from the reference's own
+    apr_elem_calc_3D values at the reference's quadrature points"""
+    L = ref_dg()
+    rng = np.random.default_rng(pdeg + 1)
+    nc = REF_NODES + rng.uniform(-0.15, 0.15, (6, 3))
+    ind, coef = all_terms(rng, True)
+    K, F = dg_num_int_el(pdeg, nc, ind, coef)
+    xg, wg = np.zeros(3000), np.zeros(1000)
+    ng = ctypes.c_int(0)
+    pv = np.array([pdeg], np.int32)
+    L.apr_set_quadr_3D(TENSOR, P(pv), ctypes.byref(ng), P(xg), P(wg))
+    ncc = np.ascontiguousarray(nc.ravel())
+    ns = K.shape[0]
+    Kr, Fr = np.zeros((ns, ns)), np.zeros(ns)
+    a = coef[C_A:C_A + 9].reshape(3, 3)
+    for g in range(ng.value):
+        phi, dx, dy, dz, xc = [np.zeros(200) for _ in range(4)] + [np.zeros(3)]
+        eta = np.ascontiguousarray(xg[3 * g:3 * g + 3])
+        det = L.apr_elem_calc_3D(2, 1, P(pv), TENSOR, P(eta), P(ncc), None, P(phi), P(dx), P(dy),
+                                 P(dz), P(xc), None, None, None, None, None)
+        vol = det * wg[g]
+        grad = np.stack([dx[:ns], dy[:ns], dz[:ns]])  # grad[c, j]
+        ph = phi[:ns]
+        Kr += (np.einsum("rc,cj,ri->ji", a, grad, grad) + (coef[C_M] + coef[C_C]) * np.outer(ph, ph)
+               + np.outer(ph, coef[C_B:C_B + 3] @ grad)) * vol
+        Fr += (coef[C_Q:C_Q + 3] @ grad + coef[C_S] * ph) * vol
+    assert np.abs(K - Kr).max() <= 1e-12 * np.abs(Kr).max()
+    assert np.abs(F - Fr).max() <= 1e-12 * np.abs(Fr).max()
