@@ -240,6 +240,33 @@ int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters,
                     double* res_rel);
 int mf_batch_free(int bid);
 
+/* ------------------------------------------------------------ DG prisms
+ * Discontinuous Galerkin approximation on prism meshes (apd_dg_prism):
+ * tensor-product monomial base of degree pdeg = 100*pz + px, 1 <= px, pz <= 3
+ * (apr_shape_fun_3D, apd_dg_prism/aps_dg_util.c:408; 6 / 18 / 40 shape
+ * functions for p = 1 / 2 / 3, apr_get_el_pdeg_numshap,
+ * aps_dg_prism_intf.c:725), uniform over the mesh.  Element DOF blocks
+ * follow the element order of the mesh. */
+int mf_dg_create(int mesh, int pdeg);
+int mf_dg_sizes(int did, int* nel, int* nshape, int* ngauss);
+/* coefficients as pdr_select_el_coeff_vect / pdr_el_coeff provide them to
+   apr_num_int_el (same ind[23], coefficient slots and storage models as
+   mf_problem_set_coefficients); the DG form has no t (tx, ty, tz) terms
+   (aps_dg_prism_intf.c:1653) and takes the 2D diffusion / convection / q
+   branches when azz / bz / qz are absent */
+int mf_dg_set_coefficients(int did, const int* ind, int coef_model, const double* coef_const,
+                           const double* coef_el, int src_model);
+/* element matrices of elements first..first+count-1 into host arrays:
+   stiff[count][ns*ns] column-major (stiff[j*ns + i], j trial, i test) and
+   rhs[count][ns], as apr_num_int_el (aps_dg_prism_intf.c:1239-1728) writes
+   them for nreq = 1 */
+int mf_dg_element_matrices(int did, int first, int count, double* stiff, double* rhs);
+/* the same for every element into device-resident buffers, then copied out
+   on request (mf_dg_get_element_matrices) */
+int mf_dg_integrate(int did);
+int mf_dg_get_element_matrices(int did, int first, int count, double* stiff, double* rhs);
+int mf_dg_free(int did);
+
 /* host copy of a storage-seam matrix (lar_get_SM_and_LV_crs_from_crs with
    offset 0, lsd_mkb/lad_crs/las_crs_intf.c:1063) */
 int mf_lar_get_system(int matrix_id, int* row_ptr, int* col_ind, double* val,

@@ -15,7 +15,7 @@ import os
 import numpy as np
 
 __all__ = ["lib", "Mesh", "Problem", "MFError", "PRISM", "TETRA",
-           "COEF_UNIFORM", "COEF_KAPPA", "COEF_KAPPA_B", "COEF_FULL", "COEF_KAPPA_S",
+           "COEF_UNIFORM", "COEF_KAPPA", "COEF_KAPPA_B", "COEF_FULL", "COEF_KAPPA_S", "DgProblem",
            "SRC_CONST", "SRC_DIFF_IN_CUBE", "SRC_DIFF_IN_BOX", "IND",
            "coeff_indicator", "init"]
 
@@ -122,6 +122,13 @@ def _declare(L):
         "mf_gmres_kernel_stats": (I, [I, V]),
         "mf_set_assembly_mode": (I, [I, I]),
         "mf_problem_free": (I, [I]),
+        "mf_dg_create": (I, [I, I]),
+        "mf_dg_sizes": (I, [I, V, V, V]),
+        "mf_dg_set_coefficients": (I, [I, V, I, V, V, I]),
+        "mf_dg_element_matrices": (I, [I, I, I, V, V]),
+        "mf_dg_integrate": (I, [I]),
+        "mf_dg_get_element_matrices": (I, [I, I, I, V, V]),
+        "mf_dg_free": (I, [I]),
         # storage seam with the reference's names (lsd_mkb/lah_intf.h)
         "lar_allocate_SM_and_LV": (I, [I, I, I, I, I, I, V, V, V, V]),
         "lar_initialize_SM_and_LV": (I, [I, I]),
@@ -311,6 +318,49 @@ class Mesh:
     def free(self):
         if self.h:
             lib().mf_mesh_free(self.h)
+            self.h = 0
+
+
+class DgProblem:
+    """A DG prism field of degree pdeg = 100*pz + px on a prism mesh
+    (apd_dg_prism); element DOF blocks in mesh element order."""
+
+    def __init__(self, mesh, pdeg):
+        init()
+        self.h = check(lib().mf_dg_create(mesh.h, int(pdeg)))
+        self.pdeg = pdeg
+        nel, ns, ng = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(lib().mf_dg_sizes(self.h, ctypes.byref(nel), ctypes.byref(ns), ctypes.byref(ng)))
+        self.nel, self.ns, self.ng = nel.value, ns.value, ng.value
+
+    def set_coefficients(self, ind, coef_model=COEF_UNIFORM, coef_const=None, coef_el=None,
+                         src_model=SRC_CONST):
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        cc = np.zeros(NCOEF) if coef_const is None else np.ascontiguousarray(coef_const, np.float64)
+        ce = None if coef_el is None else np.ascontiguousarray(coef_el, np.float64)
+        check(lib().mf_dg_set_coefficients(self.h, _p(ind), coef_model, _p(cc), _p(ce), src_model))
+
+    def element_matrices(self, first=0, count=None):
+        """(stiff[count, ns, ns], rhs[count, ns]); stiff[e, j, i]: trial j, test i"""
+        count = self.nel - first if count is None else count
+        st = np.zeros((count, self.ns, self.ns))
+        rhs = np.zeros((count, self.ns))
+        check(lib().mf_dg_element_matrices(self.h, first, count, _p(st), _p(rhs)))
+        return st, rhs
+
+    def integrate(self):
+        check(lib().mf_dg_integrate(self.h))
+
+    def get_element_matrices(self, first=0, count=None):
+        count = self.nel - first if count is None else count
+        st = np.zeros((count, self.ns, self.ns))
+        rhs = np.zeros((count, self.ns))
+        check(lib().mf_dg_get_element_matrices(self.h, first, count, _p(st), _p(rhs)))
+        return st, rhs
+
+    def free(self):
+        if self.h:
+            lib().mf_dg_free(self.h)
             self.h = 0
 
 

@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 
 HOST_SRCS = ["mf_mesh.cpp", "mf_setup.cpp", "mf_mkb.cpp"]
 HIP_SRCS = ["mf_device.hip", "mf_solver.hip", "mf_capi.hip", "mf_lar.hip", "mf_ilu.hip",
-            "mf_gs.hip"]
+            "mf_gs.hip", "mf_dg.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -40,6 +40,12 @@ Problem* get_problem(int id) {
   return it->second.get();
 }
 
+}  // namespace
+
+Mesh* mf::mesh_by_id(int id) { return get_mesh(id); }
+
+namespace {
+
 int need_init() {
   if (!g_inited) {
     set_error("mf_init has not been called");

@@ -39,6 +39,19 @@ enum SrcModel { SRC_CONST = 0, SRC_DIFF_IN_CUBE = 1, SRC_DIFF_IN_BOX = 2 };
 // boundary-face types (pdr_get_bc_type; PDC_BC_DIRI / PDC_BC_NEUM)
 enum BcType { BC_DIRICHLET = 1, BC_NEUMANN = 2 };
 
+// exact solution u = exp(-|x|^2 / a1) of the DIFF_IN_CUBE (a1 = 1) and
+// DIFF_IN_BOX (a1 = 100) problems and its Laplacian; the source is -lapl
+// (pdr_exact_sol, pdd_conv_diff/main/pds_conv_diff_util.c)
+__host__ __device__ inline void exact_sol(int model, const double* x, double* u,
+                                          double* lapl) {
+  double a1 = (model == SRC_DIFF_IN_BOX) ? 100.0 : 1.0;
+  double X = x[0], Y = x[1], Z = x[2];
+  double e = exp((-X * X - Y * Y - Z * Z) / a1);
+  *u = e;
+  *lapl = -(2.0 / a1) *
+          ((1 - 2 * X * X / a1) + (1 - 2 * Y * Y / a1) + (1 - 2 * Z * Z / a1)) * e;
+}
+
 // selected-term bits (reference coefficient indicator slots 1..22)
 struct TermMask {
   int m, diff_full, diff_diag, conv, tconv, reac, q, s;
@@ -221,6 +234,8 @@ void batch_free(Batch& B);
 
 int device_init(int device);
 hipStream_t stream();
+// mesh object behind a C-ABI mesh handle (null and an error if unknown)
+Mesh* mesh_by_id(int id);
 
 // kernels (mf_device.hip)
 int launch_zero(double* p, long long n);

@@ -134,16 +134,6 @@ __device__ __forceinline__ double jacobian(const double (*xn)[3], const double* 
   return rjac;
 }
 
-__device__ __forceinline__ void exact_sol(int model, const double* x, double* u,
-                                          double* lapl) {
-  double a1 = (model == SRC_DIFF_IN_BOX) ? 100.0 : 1.0;
-  double X = x[0], Y = x[1], Z = x[2];
-  double e = exp((-X * X - Y * Y - Z * Z) / a1);
-  *u = e;
-  *lapl = -(2.0 / a1) *
-          ((1 - 2 * X * X / a1) + (1 - 2 * Y * Y / a1) + (1 - 2 * Z * Z / a1)) * e;
-}
-
 struct IntArgs {
   int nel;
   const double* coor;      // 3*n, DOF order
