@@ -1158,3 +1158,316 @@ int orc_dg_num_int_el(int pdeg, const double *nc, const int *ind, const double *
   }
   return ns;
 }
+
+/* ---------------------------------------------------------------------
+ * DG prism faces (pdr_conv_diff_comp_fa_stiff_mat_app, pdd_conv_diff/
+ * weak_formulation/approx_dg/pds_conv_diff_weakform_dg.c:232-1400)
+ * ------------------------------------------------------------------- */
+
+/* prism face vertices, quads in cyclic order (bottom edge first), from the
+   face-node table mmd_prism/mms_prism_intf.c:98 */
+static const int DG_FACE_V[5][4] = {{0, 1, 2, -1}, {3, 4, 5, -1}, {0, 1, 4, 3}, {1, 2, 5, 4},
+                                    {2, 0, 3, 5}};
+
+/* linear prism map at eta: J[3r+c] = dx_r/deta_c, point x; returns det */
+static double dg_geom(const double *nc, const double *eta, double *J, double *x) {
+  const double e0 = eta[0], e1 = eta[1], z = eta[2];
+  double gp[6], gx[6], gy[6], gz[6];
+  gp[0] = (1.0 - e0 - e1) * (1.0 - z) / 2.0; gp[1] = e0 * (1.0 - z) / 2.0;
+  gp[2] = e1 * (1.0 - z) / 2.0; gp[3] = (1.0 - e0 - e1) * (1.0 + z) / 2.0;
+  gp[4] = e0 * (1.0 + z) / 2.0; gp[5] = e1 * (1.0 + z) / 2.0;
+  gx[0] = -(1.0 - z) / 2.0; gx[1] = (1.0 - z) / 2.0; gx[2] = 0.0;
+  gx[3] = -(1.0 + z) / 2.0; gx[4] = (1.0 + z) / 2.0; gx[5] = 0.0;
+  gy[0] = -(1.0 - z) / 2.0; gy[1] = 0.0; gy[2] = (1.0 - z) / 2.0;
+  gy[3] = -(1.0 + z) / 2.0; gy[4] = 0.0; gy[5] = (1.0 + z) / 2.0;
+  gz[0] = -(1.0 - e0 - e1) / 2.0; gz[1] = -e0 / 2.0; gz[2] = -e1 / 2.0;
+  gz[3] = (1.0 - e0 - e1) / 2.0; gz[4] = e0 / 2.0; gz[5] = e1 / 2.0;
+  for (int k = 0; k < 9; k++) J[k] = 0.0;
+  x[0] = x[1] = x[2] = 0.0;
+  for (int i = 0; i < 6; i++)
+    for (int r = 0; r < 3; r++) {
+      J[3 * r] += nc[3 * i + r] * gx[i];
+      J[3 * r + 1] += nc[3 * i + r] * gy[i];
+      J[3 * r + 2] += nc[3 * i + r] * gz[i];
+      x[r] += nc[3 * i + r] * gp[i];
+    }
+  return J[0] * (J[4] * J[8] - J[7] * J[5]) + J[3] * (J[7] * J[2] - J[1] * J[8]) +
+         J[6] * (J[1] * J[5] - J[4] * J[2]);
+}
+
+/* element-local coordinates of a face quadrature point seen from the face's
+   first neighbour (mmr_fa_elem_coor, mmd_prism/mms_prism_datstr.c:1541-1700):
+   triangles take (x0, x1) in [0,1]^2, quads (x0, x1) in [-1,1]^2 */
+void orc_dg_face_eta(int side, const double *xf, double *eta) {
+  switch (side) {
+    case 0: eta[0] = xf[1]; eta[1] = xf[0]; eta[2] = -1.0; break;
+    case 1: eta[0] = xf[0]; eta[1] = xf[1]; eta[2] = 1.0; break;
+    case 2: eta[0] = 0.5 * (xf[0] + 1.0); eta[1] = 0.0; eta[2] = xf[1]; break;
+    case 3: eta[0] = 0.5 * (-xf[0] + 1.0); eta[1] = 1.0 - eta[0]; eta[2] = xf[1]; break;
+    default: eta[0] = 0.0; eta[1] = 0.5 * (-xf[0] + 1.0); eta[2] = xf[1]; break;
+  }
+}
+
+/* area element and outward unit normal of an element side at eta
+   (apr_elem_calc_3D with Control = 3 + side, aps_dg_util.c:334-367) */
+double orc_dg_face_ds(int side, const double *nc, const double *eta, double *n, double *x) {
+  double J[9], ds[3];
+  dg_geom(nc, eta, J, x);
+  const double t0[3] = {J[0], J[3], J[6]}, t1[3] = {J[1], J[4], J[7]}, t2[3] = {J[2], J[5], J[8]};
+  const double *a, *b;
+  double d[3];
+  int neg = 0;
+  switch (side) {
+    case 0: a = t0; b = t1; neg = 1; break;
+    case 1: a = t0; b = t1; break;
+    case 2: a = t0; b = t2; break;
+    case 3:
+      d[0] = t1[0] - t0[0]; d[1] = t1[1] - t0[1]; d[2] = t1[2] - t0[2];
+      a = d; b = t2; break;
+    default: a = t2; b = t1; break;
+  }
+  ds[0] = a[1] * b[2] - a[2] * b[1];
+  ds[1] = a[2] * b[0] - a[0] * b[2];
+  ds[2] = a[0] * b[1] - a[1] * b[0];
+  if (neg) { ds[0] = -ds[0]; ds[1] = -ds[1]; ds[2] = -ds[2]; }
+  const double det = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  n[0] = ds[0] / det; n[1] = ds[1] / det; n[2] = ds[2] / det;
+  return det;
+}
+
+/* face quadrature (apr_set_quadr_2D, aps_dg_util.c:1499-1555): quads are
+   Gauss-Legendre of degree 2 px (first coordinate) x 2 pz on [-1,1]^2 with
+   weights 1/2 wx wz, triangles the degree-2 px rule with weights 1/2 w */
+int orc_dg_face_quadr(int pdeg, int quad, double *xg, double *wg) {
+  const int px = pdeg % 100, pz = pdeg / 100;
+  if (quad) {
+    double x1[8], w1[8], x2[8], w2[8];
+    const int n1 = px + 1, n2 = pz + 1;
+    gauss_legendre(n1, x1, w1);
+    gauss_legendre(n2, x2, w2);
+    for (int ki = 0; ki < n1; ki++)
+      for (int kj = 0; kj < n2; kj++) {
+        xg[2 * (ki * n2 + kj)] = x1[ki];
+        xg[2 * (ki * n2 + kj) + 1] = x2[kj];
+        wg[ki * n2 + kj] = 0.5 * w1[ki] * w2[kj];
+      }
+    return n1 * n2;
+  }
+  double l[3 * 16], w[16];
+  const int n = dunavant(2 * px, l, w);
+  for (int k = 0; k < n; k++) {
+    xg[2 * k] = l[3 * k + 1];
+    xg[2 * k + 1] = l[3 * k + 2];
+    wg[k] = 0.5 * w[k];
+  }
+  return n;
+}
+
+/* local coordinates in the second neighbour of a point given in the first
+   one: the point's barycentric (triangle) or edge/height (quad) coordinates
+   on the shared face carried over through the shared vertices (g1, g2 are
+   the global node numbers of the two elements' nodes) */
+int orc_dg_map_point(int side1, const int *g1, int side2, const int *g2, const double *eta,
+                     double *eta2) {
+  if (side1 < 2) {
+    const double lam[3] = {1.0 - eta[0] - eta[1], eta[0], eta[1]};
+    double mu[3];
+    for (int k2 = 0; k2 < 3; k2++) {
+      int found = 0;
+      for (int k1 = 0; k1 < 3; k1++)
+        if (g2[DG_FACE_V[side2][k2]] == g1[DG_FACE_V[side1][k1]]) {
+          mu[k2] = lam[k1];
+          found = 1;
+        }
+      if (!found) return -1;
+    }
+    eta2[0] = mu[1]; eta2[1] = mu[2]; eta2[2] = side2 == 0 ? -1.0 : 1.0;
+    return 0;
+  }
+  /* t runs along the bottom edge from face vertex 0 to face vertex 1 */
+  double t = side1 == 2 ? eta[0] : side1 == 3 ? eta[1] : 1.0 - eta[1];
+  const int a1 = g1[DG_FACE_V[side1][0]], b1 = g1[DG_FACE_V[side1][1]];
+  const int a2 = g2[DG_FACE_V[side2][0]], b2 = g2[DG_FACE_V[side2][1]];
+  if (a2 == a1 && b2 == b1) { /* same direction */
+  } else if (a2 == b1 && b2 == a1) {
+    t = 1.0 - t;
+  } else {
+    return -1;
+  }
+  eta2[2] = eta[2];
+  if (side2 == 2) { eta2[0] = t; eta2[1] = 0.0; }
+  else if (side2 == 3) { eta2[0] = 1.0 - t; eta2[1] = t; }
+  else { eta2[0] = 0.0; eta2[1] = 1.0 - t; }
+  return 0;
+}
+
+/* face area (mmr_fa_area, mms_prism_datstr.c:1732-1790): triangles 1/2|a x b|,
+   quads 1/2(|a x b| + |c x d|) around the face's vertex cycle */
+double orc_dg_face_area(const double *nc, int side) {
+  const int nv = side < 2 ? 3 : 4;
+  double p[4][3], e[4][3];
+  for (int k = 0; k < nv; k++)
+    for (int r = 0; r < 3; r++) p[k][r] = nc[3 * DG_FACE_V[side][k] + r];
+  for (int k = 0; k < nv; k++)
+    for (int r = 0; r < 3; r++) e[k][r] = p[(k + 1) % nv][r] - p[k][r];
+  double c1[3], c2[3];
+  c1[0] = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+  c1[1] = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+  c1[2] = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+  const double l1 = sqrt(c1[0] * c1[0] + c1[1] * c1[1] + c1[2] * c1[2]);
+  if (nv == 3) return 0.5 * l1;
+  c2[0] = e[2][1] * e[3][2] - e[2][2] * e[3][1];
+  c2[1] = e[2][2] * e[3][0] - e[2][0] * e[3][2];
+  c2[2] = e[2][0] * e[3][1] - e[2][1] * e[3][0];
+  return 0.5 * (l1 + sqrt(c2[0] * c2[0] + c2[1] * c2[1] + c2[2] * c2[2]));
+}
+
+/* shape functions with physical gradients at eta */
+static int dg_phys(int pdeg, const double *nc, const double *eta, double *phi, double *dx,
+                   double *dy, double *dz) {
+  const int ns = orc_dg_shape(pdeg, eta, phi, dx, dy, dz);
+  double J[9], Ji[9], x[3];
+  dg_geom(nc, eta, J, x);
+  orc_mat3_inv(J, Ji);
+  for (int i = 0; i < ns; i++) {
+    const double ax = dx[i] * Ji[0] + dy[i] * Ji[3] + dz[i] * Ji[6];
+    const double ay = dx[i] * Ji[1] + dy[i] * Ji[4] + dz[i] * Ji[7];
+    const double az = dx[i] * Ji[2] + dy[i] * Ji[5] + dz[i] * Ji[8];
+    dx[i] = ax; dy[i] = ay; dz[i] = az;
+  }
+  return ns;
+}
+
+/* one face's stiffness matrix and load vector for nreq = 1, steady
+   (implicitness 1), as pdr_conv_diff_comp_fa_stiff_mat_app writes them:
+   ndofs_fa = ns (boundary) or 2 ns (interior), column-major
+   stiff[j*ndofs_fa + i] with the first neighbour's functions first.
+   bc: 0 interior, 1 Dirichlet, 2 Neumann.  Normal fluxes An = a^T n and
+   Bn = b . n of each neighbour as pdr_fa_coeff forms them
+   (pds_conv_diff_weakform.c:577-740, Qn = 0); penalty = min(px, pz) times
+   the penalty constant (weakform_dg.c:516-531); hf = sqrt(face area);
+   Dirichlet data f: the exact solution when src_model selects one
+   (pdr_bc_diri_coeff), else bval; Neumann data g: the exact normal
+   derivative when src_model selects one, else bval. */
+int orc_dg_face_int(int pdeg, int bc, const double *nc1, const int *g1, int side1,
+                    const double *coef1, const double *nc2, const int *g2, int side2,
+                    const double *coef2, double penalty, double hf, int src_model, double bval,
+                    double *stiff, double *rhs) {
+  double xg[2 * 64], wg[64];
+  const int ng = orc_dg_face_quadr(pdeg, side1 >= 2, xg, wg);
+  double p1[64], x1[64], y1[64], z1[64], p2[64], x2[64], y2[64], z2[64];
+  int ns = 0, nf = 0;
+  for (int ki = 0; ki < ng; ki++) {
+    double eta[3], eta2[3], n[3], xc[3];
+    orc_dg_face_eta(side1, &xg[2 * ki], eta);
+    const double area = orc_dg_face_ds(side1, nc1, eta, n, xc) * wg[ki];
+    ns = dg_phys(pdeg, nc1, eta, p1, x1, y1, z1);
+    nf = bc == 0 ? 2 * ns : ns;
+    const double *a = coef1 + ORC_C_A, *b = coef1 + ORC_C_B;
+    double an1[3], an2[3] = {0, 0, 0};
+    for (int c = 0; c < 3; c++) an1[c] = a[c] * n[0] + a[3 + c] * n[1] + a[6 + c] * n[2];
+    const double bn1 = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+    double bn2 = 0.0, nv = bn1;
+    if (bc == 0) {
+      if (orc_dg_map_point(side1, g1, side2, g2, eta, eta2)) return -1;
+      dg_phys(pdeg, nc2, eta2, p2, x2, y2, z2);
+      const double *a2 = coef2 + ORC_C_A, *b2 = coef2 + ORC_C_B;
+      for (int c = 0; c < 3; c++) an2[c] = a2[c] * n[0] + a2[3 + c] * n[1] + a2[6 + c] * n[2];
+      bn2 = b2[0] * n[0] + b2[1] * n[1] + b2[2] * n[2];
+      nv = 0.5 * (bn1 + bn2);
+    }
+    if (bc == 0) {
+      for (int j = 0; j < ns; j++)
+        for (int i = 0; i < ns; i++) {
+          const double g1i = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
+          const double g1j = an1[0] * x1[j] + an1[1] * y1[j] + an1[2] * z1[j];
+          const double g2i = an2[0] * x2[i] + an2[1] * y2[i] + an2[2] * z2[i];
+          const double g2j = an2[0] * x2[j] + an2[1] * y2[j] + an2[2] * z2[j];
+          /* (1,1): test 1, trial 1 */
+          stiff[j * nf + i] += ((p1[j] * g1i - p1[i] * g1j) * 0.5 +
+                                penalty * p1[j] * p1[i] / hf) * area;
+          /* (1,2): test 1, trial 2 */
+          stiff[(ns + j) * nf + i] += ((-p2[j] * g1i - p1[i] * g2j) * 0.5 -
+                                       penalty * p2[j] * p1[i] / hf) * area;
+          /* (2,1): test 2, trial 1 */
+          stiff[j * nf + ns + i] += ((p1[j] * g2i + p2[i] * g1j) * 0.5 -
+                                     penalty * p1[j] * p2[i] / hf) * area;
+          /* (2,2) */
+          stiff[(ns + j) * nf + ns + i] += ((-p2[j] * g2i + p2[i] * g2j) * 0.5 +
+                                            penalty * p2[j] * p2[i] / hf) * area;
+        }
+      /* upwinding on the averaged normal velocity */
+      for (int j = 0; j < ns; j++)
+        for (int i = 0; i < ns; i++) {
+          if (nv > 0) {
+            stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+            stiff[j * nf + ns + i] -= area * bn2 * p2[i] * p1[j];
+          } else if (nv < 0) {
+            stiff[(ns + j) * nf + i] += area * bn1 * p1[i] * p2[j];
+            stiff[(ns + j) * nf + ns + i] -= area * bn2 * p2[i] * p2[j];
+          }
+        }
+    } else if (bc == 1) {
+      double f = bval;
+      if (src_model != ORC_SRC_CONST) exact_sol(src_model, xc, &f, NULL);
+      for (int j = 0; j < ns; j++)
+        for (int i = 0; i < ns; i++) {
+          const double gi = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
+          const double gj = an1[0] * x1[j] + an1[1] * y1[j] + an1[2] * z1[j];
+          stiff[j * nf + i] += (gi * p1[j] - gj * p1[i] + penalty * p1[j] * p1[i] / hf) * area;
+        }
+      for (int i = 0; i < ns; i++) {
+        const double gi = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
+        rhs[i] += (f * gi + penalty * f * p1[i] / hf) * area;
+      }
+      if (nv < 0) {
+        for (int i = 0; i < ns; i++) rhs[i] += -bn1 * f * p1[i] * area;
+      } else {
+        for (int j = 0; j < ns; j++)
+          for (int i = 0; i < ns; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+      }
+    } else {
+      /* Neumann data (pdr_bc_neum_coeff, pds_conv_diff_weakform.c:1100-1140):
+         the exact solution's normal derivative when src_model selects one */
+      double g = bval;
+      if (src_model != ORC_SRC_CONST) {
+        const double a1 = src_model == ORC_SRC_DIFF_IN_BOX ? 100.0 : 1.0;
+        const double e = exp((-xc[0] * xc[0] - xc[1] * xc[1] - xc[2] * xc[2]) / a1);
+        g = (-2.0 * xc[0] * n[0] - 2.0 * xc[1] * n[1] - 2.0 * xc[2] * n[2]) * e / a1;
+      }
+      if (nv > 0)
+        for (int j = 0; j < ns; j++)
+          for (int i = 0; i < ns; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+      for (int i = 0; i < ns; i++) rhs[i] += g * p1[i] * area;
+    }
+  }
+  return nf;
+}
+
+/* squared L2 and H1-seminorm errors of a DG solution against the exact
+   solution over one element (pdr_error_test_exact,
+   pdd_conv_diff/main/pds_conv_diff_util.c:749-965, element volume rule) */
+int orc_dg_errors(int pdeg, const double *nc, const double *dofs, int src_model, double *l2sq,
+                  double *h1sq) {
+  double xg[3 * 64 * 12], wg[64 * 12];
+  const int ng = orc_dg_quadr(pdeg, xg, wg);
+  double phi[128], dx[128], dy[128], dz[128];
+  *l2sq = 0.0; *h1sq = 0.0;
+  for (int ki = 0; ki < ng; ki++) {
+    double J[9], x[3];
+    const double det = fabs(dg_geom(nc, &xg[3 * ki], J, x));
+    const int ns = dg_phys(pdeg, nc, &xg[3 * ki], phi, dx, dy, dz);
+    double u = 0, ux = 0, uy = 0, uz = 0;
+    for (int i = 0; i < ns; i++) {
+      u += dofs[i] * phi[i]; ux += dofs[i] * dx[i]; uy += dofs[i] * dy[i]; uz += dofs[i] * dz[i];
+    }
+    const double a1 = src_model == ORC_SRC_DIFF_IN_BOX ? 100.0 : 1.0;
+    const double e = exp((-x[0] * x[0] - x[1] * x[1] - x[2] * x[2]) / a1);
+    const double f = e, fx = -2.0 * x[0] * e / a1, fy = -2.0 * x[1] * e / a1,
+                 fz = -2.0 * x[2] * e / a1;
+    const double vol = det * wg[ki];
+    *l2sq += (f - u) * (f - u) * vol;
+    *h1sq += ((fx - ux) * (fx - ux) + (fy - uy) * (fy - uy) + (fz - uz) * (fz - uz)) * vol;
+  }
+  return ng;
+}

@@ -149,6 +149,25 @@ int orc_dg_quadr(int pdeg, double *xg, double *wg);
 int orc_dg_num_int_el(int pdeg, const double *node_coor, const int *ind, const double *coef,
                       int src_model, double *stiff, double *rhs);
 
+/* DG faces: quadrature (quad != 0: side faces), the second neighbour's
+   local coordinates of a first-neighbour point (g1, g2: global node numbers
+   of the two elements), face area, one face's matrices (bc 0 interior,
+   1 Dirichlet, 2 Neumann; stiff (2 ns)^2 or ns^2, column-major), and the
+   element's squared L2 / H1-seminorm errors against the exact solution */
+int orc_dg_face_quadr(int pdeg, int quad, double *xg, double *wg);
+void orc_dg_face_eta(int side, const double *xf, double *eta);
+double orc_dg_face_ds(int side, const double *node_coor, const double *eta, double *normal,
+                      double *x);
+int orc_dg_map_point(int side1, const int *g1, int side2, const int *g2, const double *eta,
+                     double *eta2);
+double orc_dg_face_area(const double *node_coor, int side);
+int orc_dg_face_int(int pdeg, int bc, const double *nc1, const int *g1, int side1,
+                    const double *coef1, const double *nc2, const int *g2, int side2,
+                    const double *coef2, double penalty, double hf, int src_model, double bval,
+                    double *stiff, double *rhs);
+int orc_dg_errors(int pdeg, const double *node_coor, const double *dofs, int src_model,
+                  double *l2sq, double *h1sq);
+
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);
 

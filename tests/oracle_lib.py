@@ -51,6 +51,13 @@ def oracle():
             "orc_dg_shape": (I, [I, V, V, V, V, V]),
             "orc_dg_quadr": (I, [I, V, V]),
             "orc_dg_num_int_el": (I, [I, V, V, V, I, V, V]),
+            "orc_dg_face_quadr": (I, [I, I, V, V]),
+            "orc_dg_face_eta": (None, [I, V, V]),
+            "orc_dg_face_ds": (D, [I, V, V, V, V]),
+            "orc_dg_map_point": (I, [I, V, I, V, V, V]),
+            "orc_dg_face_area": (D, [V, I]),
+            "orc_dg_face_int": (I, [I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
+            "orc_dg_errors": (I, [I, V, V, I, V, V]),
             "orc_set_threads": (None, [I]),
         }
         for k, (r, a) in sig.items():
@@ -230,3 +237,153 @@ def dg_num_int_el(pdeg, nc, ind, coef, src_model=0):
                                     P(st), P(rhs))
     assert ns > 0
     return st[:ns * ns].reshape(ns, ns), rhs[:ns]
+
+
+# prism face vertices (mmd_prism/mms_prism_intf.c:98), quads cyclic
+DG_FACE_V = [[0, 1, 2], [3, 4, 5], [0, 1, 4, 3], [1, 2, 5, 4], [2, 0, 3, 5]]
+
+
+def dg_faces(el_nodes):
+    """faces of a prism mesh, each once: (e1, s1, e2, s2) with e1 < e2, or
+    (e, s, -1, -1) for a boundary face"""
+    seen = {}
+    faces = []
+    for e, nodes in enumerate(el_nodes):
+        for s in range(5):
+            key = tuple(sorted(int(nodes[v]) for v in DG_FACE_V[s]))
+            if key in seen:
+                e1, s1 = seen.pop(key)
+                faces.append((e1, s1, e, s))
+            else:
+                seen[key] = (e, s)
+    for (e, s) in seen.values():
+        faces.append((e, s, -1, -1))
+    return faces
+
+
+def dg_system_dense(arrays, pdeg, ind, coef_of, penalty_const=1.0, src_model=0, bval=0.0,
+                    sparse=False):
+    """the oracle's DG system: element integrals plus every face's matrices
+    (each neighbour pair once, first neighbour = lower element), boundary
+    faces Dirichlet; dense, or scipy CSR with sparse=True"""
+    o = oracle()
+    if sparse:
+        return _dg_system_sparse(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval)
+    coor, en = arrays["coor"], arrays["el_nodes"]
+    nel = en.shape[0]
+    nodes = coor[en]
+    px, pz = pdeg % 100, pdeg // 100
+    ns = (px + 1) * (px + 2) // 2 * (pz + 1)
+    n = nel * ns
+    A = np.zeros((n, n))
+    b = np.zeros(n)
+    for e in range(nel):
+        K, F = dg_num_int_el(pdeg, nodes[e], ind, coef_of(e), src_model)
+        A[e * ns:(e + 1) * ns, e * ns:(e + 1) * ns] += K.T
+        b[e * ns:(e + 1) * ns] += F
+    pen = min(px, pz) * penalty_const
+    bc_of = _dg_bc_types(arrays)
+    st = np.zeros(4 * ns * ns)
+    rhs = np.zeros(2 * ns)
+    for (e1, s1, e2, s2) in dg_faces(en):
+        st[:] = 0.0
+        rhs[:] = 0.0
+        nc1 = np.ascontiguousarray(nodes[e1].ravel())
+        g1 = np.ascontiguousarray(en[e1], np.int32)
+        hf = np.sqrt(o.orc_dg_face_area(P(nc1), s1))
+        c1 = np.ascontiguousarray(coef_of(e1), np.float64)
+        if e2 < 0:
+            nf = o.orc_dg_face_int(pdeg, bc_of(e1, s1), P(nc1), P(g1), s1, P(c1), None, None, 0,
+                                   None, pen, hf, src_model, bval, P(st), P(rhs))
+            assert nf == ns
+            A[e1 * ns:(e1 + 1) * ns, e1 * ns:(e1 + 1) * ns] += st[:ns * ns].reshape(ns, ns).T
+            b[e1 * ns:(e1 + 1) * ns] += rhs[:ns]
+            continue
+        nc2 = np.ascontiguousarray(nodes[e2].ravel())
+        g2 = np.ascontiguousarray(en[e2], np.int32)
+        c2 = np.ascontiguousarray(coef_of(e2), np.float64)
+        nf = o.orc_dg_face_int(pdeg, 0, P(nc1), P(g1), s1, P(c1), P(nc2), P(g2), s2, P(c2), pen,
+                               hf, src_model, bval, P(st), P(rhs))
+        assert nf == 2 * ns
+        M = st[:nf * nf].reshape(nf, nf).T  # M[i, j]: test i, trial j
+        idx = np.r_[e1 * ns:(e1 + 1) * ns, e2 * ns:(e2 + 1) * ns]
+        A[np.ix_(idx, idx)] += M
+        b[idx] += rhs[:nf]
+    return A, b
+
+
+def _dg_bc_types(arrays):
+    """boundary face (element, side) -> 1 Dirichlet / 2 Neumann from the mesh
+    bc flags (pdr_get_bc_type, pds_conv_diff_weakform.c:1240-1259: flags
+    1..9 Dirichlet, 10..19 Neumann); faces without a flag are Dirichlet"""
+    t = {}
+    for e, s, bc in zip(arrays["bf_elem"], arrays["bf_side"], arrays["bf_bc"]):
+        t[(int(e), int(s))] = 1 if bc < 10 else 2 if bc < 20 else 3
+    return lambda e, s: t.get((e, s), 1)
+
+
+def dg_errors(arrays, pdeg, x, src_model=1):
+    """(L2 error, H1-seminorm error) of a DG solution (element blocks in
+    element order) against the exact solution"""
+    o = oracle()
+    nodes = arrays["coor"][arrays["el_nodes"]]
+    nel = nodes.shape[0]
+    ns = len(x) // nel
+    l2 = h1 = 0.0
+    a, c = ctypes.c_double(), ctypes.c_double()
+    for e in range(nel):
+        o.orc_dg_errors(pdeg, P(np.ascontiguousarray(nodes[e].ravel())),
+                        P(np.ascontiguousarray(x[e * ns:(e + 1) * ns])), src_model,
+                        ctypes.byref(a), ctypes.byref(c))
+        l2 += a.value
+        h1 += c.value
+    return np.sqrt(l2), np.sqrt(h1)
+
+
+def _dg_system_sparse(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval):
+    import scipy.sparse as sp
+    o = oracle()
+    coor, en = arrays["coor"], arrays["el_nodes"]
+    nel = en.shape[0]
+    nodes = coor[en]
+    px, pz = pdeg % 100, pdeg // 100
+    ns = (px + 1) * (px + 2) // 2 * (pz + 1)
+    n = nel * ns
+    rows, cols, vals = [], [], []
+    b = np.zeros(n)
+    loc = np.arange(ns)
+    for e in range(nel):
+        K, F = dg_num_int_el(pdeg, nodes[e], ind, coef_of(e), src_model)
+        idx = e * ns + loc
+        rows.append(np.repeat(idx, ns)); cols.append(np.tile(idx, ns)); vals.append(K.T.ravel())
+        b[idx] += F
+    pen = min(px, pz) * penalty_const
+    bc_of = _dg_bc_types(arrays)
+    st = np.zeros(4 * ns * ns)
+    rhs = np.zeros(2 * ns)
+    for (e1, s1, e2, s2) in dg_faces(en):
+        st[:] = 0.0
+        rhs[:] = 0.0
+        nc1 = np.ascontiguousarray(nodes[e1].ravel())
+        g1 = np.ascontiguousarray(en[e1], np.int32)
+        hf = np.sqrt(o.orc_dg_face_area(P(nc1), s1))
+        c1 = np.ascontiguousarray(coef_of(e1), np.float64)
+        if e2 < 0:
+            o.orc_dg_face_int(pdeg, bc_of(e1, s1), P(nc1), P(g1), s1, P(c1), None, None, 0, None,
+                              pen, hf, src_model, bval, P(st), P(rhs))
+            idx = e1 * ns + loc
+            M = st[:ns * ns].reshape(ns, ns).T
+            nf = ns
+        else:
+            nc2 = np.ascontiguousarray(nodes[e2].ravel())
+            g2 = np.ascontiguousarray(en[e2], np.int32)
+            c2 = np.ascontiguousarray(coef_of(e2), np.float64)
+            nf = o.orc_dg_face_int(pdeg, 0, P(nc1), P(g1), s1, P(c1), P(nc2), P(g2), s2, P(c2),
+                                   pen, hf, src_model, bval, P(st), P(rhs))
+            idx = np.r_[e1 * ns + loc, e2 * ns + loc]
+            M = st[:nf * nf].reshape(nf, nf).T
+        rows.append(np.repeat(idx, nf)); cols.append(np.tile(idx, nf)); vals.append(M.ravel())
+        b[idx] += rhs[:nf]
+    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n, n)).tocsr()
+    return A, b

@@ -223,3 +223,131 @@ def test_distorted_prism_geometry_matches_reference(pdeg):
         Fr += (coef[C_Q:C_Q + 3] @ grad + coef[C_S] * ph) * vol
     assert np.abs(K - Kr).max() <= 1e-12 * np.abs(Kr).max()
     assert np.abs(F - Fr).max() <= 1e-12 * np.abs(Fr).max()
+
+
+# ---------------------------------------------------------------- faces
+GOLDEN_DG = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dg")
+
+
+def read_error_table():
+    """the reference's NIPG convergence table for DIFF_IN_CUBE
+    (ctest/conv_diff/Laplace_dg_in_cube/comparison_data/diff_nipg_h_conv.txt):
+    {(n, pdeg): (ndof, L2 error, H1-seminorm error)} with h = 1/n"""
+    out = {}
+    with open(os.path.join(GOLDEN_DG, "diff_nipg_h_conv.txt")) as f:
+        for line in f:
+            t = line.replace(",", ".").split()
+            try:
+                vals = [float(v) for v in t[:5]]
+            except ValueError:
+                continue
+            if len(vals) < 5:
+                continue
+            n = int(round(1.0 / float(t[0])))
+            out[(n, int(t[1]))] = (int(t[2]), float(t[3]), float(t[4]))
+    return out
+
+
+def cube_arrays(n, jitter=0.0, seed=0):
+    """unit cube, n^3 hexes split into prisms along the (0,0)-(1,1) diagonal
+    (the reference's two-prism cube mesh refined log2 n times), bottom and
+    top faces Neumann (bc flag 11), sides Dirichlet (flag 1), as in the
+    reference's mesh file for this test"""
+    import modfem_amd as mf
+    a = mf.Mesh.box_prism(n, n, n, seed=seed, jitter=jitter).arrays()
+    bc = a["bf_bc"].copy()
+    bc[a["bf_side"] < 2] = 11
+    a["bf_bc"] = bc
+    return a
+
+
+def test_face_points_coincide_across_neighbours():
+    """a face point mapped into the second neighbour is the same physical
+    point (the check the reference itself makes, weakform_dg.c:605-612)"""
+    from oracle_lib import dg_faces, oracle
+    a = cube_arrays(3, jitter=0.2, seed=4)
+    nodes = a["coor"][a["el_nodes"]]
+    o = oracle()
+    nfaces = 0
+    for (e1, s1, e2, s2) in dg_faces(a["el_nodes"]):
+        if e2 < 0:
+            continue
+        nfaces += 1
+        for q in ([0.2, 0.3], [-0.5, 0.7], [0.1, 0.1]):
+            eta, eta2, n, x1, x2 = [np.zeros(3) for _ in range(5)]
+            o.orc_dg_face_eta(s1, P(np.array(q)), P(eta))
+            rc = o.orc_dg_map_point(s1, P(np.ascontiguousarray(a["el_nodes"][e1], np.int32)), s2,
+                                    P(np.ascontiguousarray(a["el_nodes"][e2], np.int32)),
+                                    P(eta), P(eta2))
+            assert rc == 0
+            o.orc_dg_face_ds(s1, P(np.ascontiguousarray(nodes[e1].ravel())), P(eta), P(n), P(x1))
+            from oracle_lib import dg_shape  # noqa: F401
+            n2 = np.zeros(3)
+            o.orc_dg_face_ds(s2, P(np.ascontiguousarray(nodes[e2].ravel())), P(eta2), P(n2), P(x2))
+            assert np.abs(x1 - x2).max() <= 1e-14
+            if s1 < 2:  # planar faces: opposite outward normals
+                assert np.abs(n + n2).max() <= 1e-13
+    assert nfaces > 50
+
+
+@need_dg
+@pytest.mark.parametrize("pdeg", PDEGS)
+def test_face_quadrature_matches_reference(pdeg):
+    L = ref_dg()
+    L.apr_set_quadr_2D.restype = ctypes.c_int
+    L.apr_set_quadr_2D.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4
+    from oracle_lib import oracle
+    for fa_type, quad in ((3, 0), (4, 1)):  # MMC_TRIA, MMC_QUAD (include/mmh_intf.h:224)
+        xg, wg = np.zeros(450), np.zeros(150)
+        ng = ctypes.c_int(0)
+        L.apr_set_quadr_2D(fa_type, TENSOR, P(np.array([pdeg], np.int32)), ctypes.byref(ng),
+                           P(xg), P(wg))
+        mx, mw = np.zeros(128), np.zeros(64)
+        n = oracle().orc_dg_face_quadr(pdeg, quad, P(mx), P(mw))
+        assert n == ng.value
+        assert np.abs(mx[:2 * n] - xg[:2 * n]).max() <= 2e-15
+        assert np.abs(mw[:n] - wg[:n]).max() <= 5e-15
+
+
+@need_dg
+def test_face_area_element_and_normal_match_reference():
+    """apr_elem_calc_3D with Control = 3 + side returns |ds| and the unit
+    normal (aps_dg_util.c:334-367)"""
+    from oracle_lib import oracle
+    L = ref_dg()
+    rng = np.random.default_rng(5)
+    nc = np.ascontiguousarray((REF_NODES + rng.uniform(-0.15, 0.15, (6, 3))).ravel())
+    pv = np.array([202], np.int32)
+    for side in range(5):
+        for q in ([0.2, 0.3], [-0.4, 0.6]):
+            eta = np.zeros(3)
+            oracle().orc_dg_face_eta(side, P(np.array(q)), P(eta))
+            n, x = np.zeros(3), np.zeros(3)
+            d = oracle().orc_dg_face_ds(side, P(nc), P(eta), P(n), P(x))
+            phi, dx, dy, dz = [np.zeros(200) for _ in range(4)]
+            nr, xr = np.zeros(3), np.zeros(3)
+            dr = L.apr_elem_calc_3D(3 + side, 1, P(pv), TENSOR, P(eta), P(nc), None, P(phi),
+                                    P(dx), P(dy), P(dz), P(xr), None, None, None, None, P(nr))
+            assert abs(d - dr) <= 1e-15 * abs(dr) and np.array_equal(n, nr)
+            assert np.abs(x - xr).max() <= 1e-15
+
+
+@pytest.mark.parametrize("n,pdeg", [(1, 101), (1, 202), (1, 303), (2, 101), (2, 202), (2, 303),
+                                    (4, 101), (4, 202), (4, 303), (8, 101)])
+def test_reference_error_table_reproduced(n, pdeg):
+    """element integrals + NIPG faces (Dirichlet sides, Neumann top and
+    bottom, penalty constant 1) solved directly reproduce every printed digit
+    of the reference's DIFF_IN_CUBE convergence table"""
+    import scipy.sparse.linalg as spl
+    import modfem_amd as mf
+    from oracle_lib import dg_errors, dg_system_dense
+    table = read_error_table()
+    ndof, l2_ref, h1_ref = table[(n, pdeg)]
+    a = cube_arrays(n)
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    A, b = dg_system_dense(a, pdeg, mf.IND_DIFF, lambda e: cc, 1.0, 1, sparse=True)
+    assert len(b) == ndof
+    x = spl.spsolve(A.tocsc(), b)
+    l2, h1 = dg_errors(a, pdeg, x, 1)
+    assert abs(l2 - l2_ref) <= 6e-10 and abs(h1 - h1_ref) <= 6e-10, (l2, h1, l2_ref, h1_ref)
