@@ -265,6 +265,36 @@ int mf_dg_element_matrices(int did, int first, int count, double* stiff, double*
    on request (mf_dg_get_element_matrices) */
 int mf_dg_integrate(int did);
 int mf_dg_get_element_matrices(int did, int first, int count, double* stiff, double* rhs);
+/* NIPG face terms (pdr_conv_diff_comp_fa_stiff_mat_app, pdd_conv_diff/
+   weak_formulation/approx_dg/pds_conv_diff_weakform_dg.c:232-1400): penalty
+   min(px, pz) * penalty_const / sqrt(face area) (PENALTY_CONSTANT of the
+   problem file, default 1), steady form (implicitness 1).  Boundary faces
+   take the mesh's bc flags as pdr_get_bc_type does (1..9 Dirichlet, 10..19
+   Neumann; unflagged faces Dirichlet); their data is the exact solution
+   (its normal derivative) when a source model selects one, else the value
+   given per (element, side) by mf_dg_set_boundary_values (default 0) */
+int mf_dg_set_penalty(int did, double penalty_const);
+int mf_dg_set_boundary_values(int did, const double* val);
+/* element-centric face table, 5 entries per element: neighbour element or
+   -1 (Dirichlet) / -2 (Neumann), orientation code, sqrt(face area) */
+int mf_dg_get_faces(int did, int* nb, int* code, double* hf);
+/* the block system of lad_block storage (lsd_mkb/lad_block/lah_block.h:
+   22-83, las_block_intf.c:362): per element a row of six ns x ns blocks,
+   column-major (slot 0 the element's own block with its volume and face
+   terms, slot 1 + s the coupling to the neighbour across side s) and its
+   load vector, built on the device */
+int mf_dg_assemble(int did);
+int mf_dg_get_block_row(int did, int e, double* blocks, double* rhs);
+/* block Jacobi: inverses of the own blocks (lar_fill_preconditioner_block,
+   las_block_intf.c:1057, which factors them with LAPACK) */
+int mf_dg_fill_precon(int did);
+/* GMRES (lsr_mkb_core_gmres, lss_mkb_core.c:2317) on the block system with
+   precon 0 (none) or 1 (block Jacobi, left), zero initial guess, stopping on
+   the preconditioned residual relative to its initial value; x may be null
+   (the solution stays on the device, mf_dg_get_solution) */
+int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, double* x,
+                int* iters, double* res_rel, double* resid);
+int mf_dg_get_solution(int did, double* x);
 int mf_dg_free(int did);
 
 /* host copy of a storage-seam matrix (lar_get_SM_and_LV_crs_from_crs with

@@ -129,6 +129,14 @@ def _declare(L):
         "mf_dg_integrate": (I, [I]),
         "mf_dg_get_element_matrices": (I, [I, I, I, V, V]),
         "mf_dg_free": (I, [I]),
+        "mf_dg_set_penalty": (I, [I, D]),
+        "mf_dg_set_boundary_values": (I, [I, V]),
+        "mf_dg_get_faces": (I, [I, V, V, V]),
+        "mf_dg_assemble": (I, [I]),
+        "mf_dg_get_block_row": (I, [I, I, V, V]),
+        "mf_dg_fill_precon": (I, [I]),
+        "mf_dg_gmres": (I, [I, I, I, I, D, V, V, V, V]),
+        "mf_dg_get_solution": (I, [I, V]),
         # storage seam with the reference's names (lsd_mkb/lah_intf.h)
         "lar_allocate_SM_and_LV": (I, [I, I, I, I, I, I, V, V, V, V]),
         "lar_initialize_SM_and_LV": (I, [I, I]),
@@ -357,6 +365,51 @@ class DgProblem:
         rhs = np.zeros((count, self.ns))
         check(lib().mf_dg_get_element_matrices(self.h, first, count, _p(st), _p(rhs)))
         return st, rhs
+
+    def set_penalty(self, penalty_const):
+        check(lib().mf_dg_set_penalty(self.h, float(penalty_const)))
+
+    def set_boundary_values(self, val=None):
+        """per (element, side) boundary data used when no exact solution is selected"""
+        v = None if val is None else np.ascontiguousarray(val, np.float64).reshape(-1)
+        if v is not None and v.shape[0] != 5 * self.nel:
+            raise ValueError("need 5 values per element")
+        check(lib().mf_dg_set_boundary_values(self.h, _p(v)))
+
+    def faces(self):
+        """(neighbour or -1 Dirichlet / -2 Neumann, orientation code, face size) per (element, side)"""
+        nb = np.zeros((self.nel, 5), np.int32)
+        code = np.zeros((self.nel, 5), np.int32)
+        hf = np.zeros((self.nel, 5))
+        check(lib().mf_dg_get_faces(self.h, _p(nb), _p(code), _p(hf)))
+        return nb, code, hf
+
+    def assemble(self):
+        check(lib().mf_dg_assemble(self.h))
+
+    def block_row(self, e):
+        """blocks[6, ns, ns] with blocks[slot, j, i] = row i, column j of the
+        block (slot 0 own, 1 + s the neighbour across side s), rhs[ns]"""
+        blk = np.zeros((6, self.ns, self.ns))
+        rhs = np.zeros(self.ns)
+        check(lib().mf_dg_get_block_row(self.h, int(e), _p(blk), _p(rhs)))
+        return blk, rhs
+
+    def fill_precon(self):
+        check(lib().mf_dg_fill_precon(self.h))
+
+    def gmres(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, fetch=True):
+        x = np.zeros(self.nel * self.ns) if fetch else None
+        it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        conv = check(lib().mf_dg_gmres(self.h, precon, krylbas, max_iter, tol, _p(x),
+                                       ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs)))
+        return x, {"converged": bool(conv), "iters": it.value, "res_rel": rr.value,
+                   "resid": rs.value}
+
+    def solution(self):
+        x = np.zeros(self.nel * self.ns)
+        check(lib().mf_dg_get_solution(self.h, _p(x)))
+        return x
 
     def free(self):
         if self.h:

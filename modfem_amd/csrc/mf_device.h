@@ -198,11 +198,24 @@ struct Problem {
   double* d_work = nullptr;    // n
   // last measured times (ms)
   double t_integrate_ms = 0, t_faces_ms = 0, t_solve_ms = 0;
+  // operator of another discretisation (not owned); replaces the CRS/SELL
+  // operator and preconditioner inside gmres_core
+  struct ExtOp* ext = nullptr;
 };
 
 // Batched restarted GMRES over resident problems (mf_solver.hip).  Members
 // keep their own matrices, vectors and Krylov bases; one launch of each
 // kernel serves all members through a block table (block -> member, rank).
+// An operator supplied by another discretisation (the DG block system of
+// mf_dg.hip): GMRES calls residual() for V_0 = M^-1 (b - A x) and apply()
+// for w = s_j M^-1 A V_j (s_j = scale[j] read on the device; nothing to do
+// once *stop is set), then forms the CGS dots, updates and restarts itself.
+struct ExtOp {
+  virtual ~ExtOp() {}
+  virtual int residual(const double* b, const double* x, double* r) = 0;
+  virtual int apply(const double* v, const double* scale, int j, double* w, const int* stop) = 0;
+};
+
 struct BatchMember {
   Problem* P = nullptr;
   int pid = -1;

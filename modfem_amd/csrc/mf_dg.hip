@@ -409,6 +409,8 @@ __global__ void __launch_bounds__(dg_block_threads(PX, PZ)) k_dg_elem(DgElemArgs
 // ---------------------------------------------------------------------
 // DG problem objects
 // ---------------------------------------------------------------------
+struct DgOp;
+
 struct DgProblem {
   int mesh = 0, nel = 0, px = 1, pz = 1, ns = 0, ng = 0, nsp = 0;
   double* d_nc = nullptr;    // [18][nel]
@@ -421,6 +423,24 @@ struct DgProblem {
   double* d_stiff = nullptr;  // device-resident element matrices (mf_dg_integrate)
   double* d_rhs = nullptr;
   int ncu = 0;
+  // faces, element-centric: for element e and side s (index 5 e + s)
+  int* d_fnb = nullptr;       // neighbour element, or -1 Dirichlet / -2 Neumann
+  int* d_fcode = nullptr;     // neighbour side | orientation << 4
+  double* d_fhf = nullptr;    // sqrt(face area)
+  double* d_fval = nullptr;   // boundary data when no exact solution is selected
+  double* d_qt = nullptr;     // triangle-face rule [ngt][3]
+  double* d_qq = nullptr;     // quad-face rule [ngq][3]
+  int ngt = 0, ngq = 0;
+  int ex[40] = {0};           // exponents of the shape functions, a | b << 2 | c << 4
+  double penalty_const = 1.0;
+  // block system: [nel][6][ns*ns] column-major blocks (slot 0 the element's
+  // own block, slot 1 + s the coupling across side s), rhs [nel][ns]
+  double* d_blk = nullptr;
+  double* d_brhs = nullptr;
+  double* d_dinv = nullptr;   // inverses of the slot-0 blocks
+  bool assembled = false, dinv_valid = false;
+  Problem* P = nullptr;       // GMRES workspace over the block system
+  DgOp* op = nullptr;
 };
 
 namespace {
@@ -471,11 +491,812 @@ int launch_elem(DgProblem& D, int e0, int count, double* d_stiff, double* d_rhs)
   return -1;
 }
 
+void free_dg_solver(DgProblem& D);
+
 void free_dg(DgProblem& D) {
-  void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs};
+  free_dg_solver(D);
+  void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs, D.d_fnb, D.d_fcode,
+                  D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
+}  // namespace
+
+// ---------------------------------------------------------------------
+// DG block system: faces, assembly of block rows, block Jacobi, operator
+// ---------------------------------------------------------------------
+namespace {
+
+// prism face vertices, quads cyclic with the bottom edge first (face-node
+// table mmd_prism/mms_prism_intf.c:98)
+constexpr int kFaceV[5][4] = {{0, 1, 2, -1}, {3, 4, 5, -1}, {0, 1, 4, 3}, {1, 2, 5, 4},
+                              {2, 0, 3, 5}};
+
+// exponents (a, b, c) of e0^a e1^b z^c for every shape function, in the
+// order apr_shape_fun_3D lists them
+void shape_exponents(int px, int pz, int* ex) {
+  std::vector<std::pair<int, int>> b2 = {{0, 0}, {1, 0}, {0, 1}};
+  if (px > 1) {
+    b2.push_back({1, 1});
+    for (int j = 0; j < 2; j++)
+      for (int i = 2; i <= px - j; i++) b2.push_back({i, j});
+    for (int j = 2; j <= px; j++)
+      for (int i = 0; i <= px - j; i++) b2.push_back({i, j});
+  }
+  const int first[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}};
+  int k = 0;
+  for (auto& f : first) ex[k++] = f[0] | f[1] << 2 | f[2] << 4;
+  for (int j = 0; j < 2; j++)
+    for (size_t i = 3; i < b2.size(); i++) ex[k++] = b2[i].first | b2[i].second << 2 | j << 4;
+  for (int j = 2; j <= pz; j++)
+    for (size_t i = 0; i < b2.size(); i++) ex[k++] = b2[i].first | b2[i].second << 2 | j << 4;
+}
+
+// face rules (apr_set_quadr_2D, aps_dg_util.c:1499-1555): [n][3] = (x0, x1, w)
+void face_rules(int px, int pz, std::vector<double>& qt, std::vector<double>& qq) {
+  std::vector<double> l, w;
+  triangle_rule(2 * px, l, w);
+  qt.clear();
+  for (size_t k = 0; k < w.size(); k++) {
+    qt.push_back(l[3 * k + 1]); qt.push_back(l[3 * k + 2]); qt.push_back(0.5 * w[k]);
+  }
+  double x1[8], w1[8], x2[8], w2[8];
+  gauss_legendre(px + 1, x1, w1);
+  gauss_legendre(pz + 1, x2, w2);
+  qq.clear();
+  for (int a = 0; a <= px; a++)
+    for (int b = 0; b <= pz; b++) {
+      qq.push_back(x1[a]); qq.push_back(x2[b]); qq.push_back(0.5 * w1[a] * w2[b]);
+    }
+}
+
+double face_area(const Mesh& m, int e, int s) {
+  const int nv = s < 2 ? 3 : 4;
+  double p[4][3], d[4][3];
+  for (int k = 0; k < nv; k++)
+    for (int r = 0; r < 3; r++) p[k][r] = m.coor[3 * (size_t)m.nodes[6 * (size_t)e + kFaceV[s][k]] + r];
+  for (int k = 0; k < nv; k++)
+    for (int r = 0; r < 3; r++) d[k][r] = p[(k + 1) % nv][r] - p[k][r];
+  auto cross_len = [](const double* a, const double* b) {
+    const double c0 = a[1] * b[2] - a[2] * b[1], c1 = a[2] * b[0] - a[0] * b[2],
+                 c2 = a[0] * b[1] - a[1] * b[0];
+    return std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  };
+  const double l1 = cross_len(d[0], d[1]);
+  return nv == 3 ? 0.5 * l1 : 0.5 * (l1 + cross_len(d[2], d[3]));
+}
+
+// element-centric face table.  Pairs are matched by their sorted vertex
+// numbers; the orientation code lets an element carry a point of its own
+// face into the neighbour: triangles store, for each neighbour face vertex,
+// the matching vertex of the own face (2 bits each), quads whether the
+// bottom edge runs the other way.  The face size comes from the
+// lower-numbered element's vertex cycle (both sides use the same value).
+int build_faces(const Mesh& m, std::vector<int>& fnb, std::vector<int>& fcode,
+                std::vector<double>& fhf) {
+  const size_t nel = m.nel, nf = 5 * nel;
+  fnb.assign(nf, -1);
+  fcode.assign(nf, 0);
+  fhf.assign(nf, 0.0);
+  struct Key {
+    int v[4];
+    int es;
+  };
+  std::vector<Key> keys(nf);
+  for (size_t e = 0; e < nel; e++)
+    for (int s = 0; s < 5; s++) {
+      Key& k = keys[5 * e + s];
+      const int nv = s < 2 ? 3 : 4;
+      for (int t = 0; t < 4; t++) k.v[t] = t < nv ? m.nodes[6 * e + kFaceV[s][t]] : -1;
+      std::sort(k.v, k.v + nv);
+      k.es = (int)(5 * e + s);
+    }
+  std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+    for (int t = 0; t < 4; t++)
+      if (a.v[t] != b.v[t]) return a.v[t] < b.v[t];
+    return a.es < b.es;
+  });
+  std::map<std::pair<int, int>, int> bc;
+  for (int f = 0; f < m.nbf; f++) bc[{m.bf_elem[f], m.bf_side[f]}] = m.bf_bc[f];
+  size_t i = 0;
+  while (i < nf) {
+    size_t j = i + 1;
+    while (j < nf && std::equal(keys[j].v, keys[j].v + 4, keys[i].v)) j++;
+    if (j - i > 2) {
+      set_error("DG faces: a face is shared by more than two elements");
+      return -1;
+    }
+    const int e1 = keys[i].es / 5, s1 = keys[i].es % 5;
+    if (j - i == 1) {
+      auto it = bc.find({e1, s1});
+      const int flag = it == bc.end() ? 1 : it->second;
+      if (flag <= 0 || flag >= 20) {
+        set_error("DG faces: boundary flags must select Dirichlet (1..9) or Neumann (10..19)");
+        return -1;
+      }
+      fnb[keys[i].es] = flag < 10 ? -1 : -2;
+      fhf[keys[i].es] = std::sqrt(face_area(m, e1, s1));
+    } else {
+      const int e2 = keys[i + 1].es / 5, s2 = keys[i + 1].es % 5;
+      if ((s1 < 2) != (s2 < 2)) {
+        set_error("DG faces: a triangle face meets a quad face");
+        return -1;
+      }
+      const double hf = std::sqrt(face_area(m, std::min(e1, e2), e1 < e2 ? s1 : s2));
+      for (int side = 0; side < 2; side++) {
+        const int ea = side ? e2 : e1, sa = side ? s2 : s1, eb = side ? e1 : e2,
+                  sb = side ? s1 : s2;
+        const int* ga = &m.nodes[6 * (size_t)ea];
+        const int* gb = &m.nodes[6 * (size_t)eb];
+        int code = sb;
+        if (sa < 2) {
+          for (int kb = 0; kb < 3; kb++) {
+            int match = -1;
+            for (int ka = 0; ka < 3; ka++)
+              if (gb[kFaceV[sb][kb]] == ga[kFaceV[sa][ka]]) match = ka;
+            if (match < 0) {
+              set_error("DG faces: inconsistent triangle face");
+              return -1;
+            }
+            code |= match << (4 + 2 * kb);
+          }
+        } else {
+          const int A = ga[kFaceV[sa][0]], B = ga[kFaceV[sa][1]];
+          const int A2 = gb[kFaceV[sb][0]], B2 = gb[kFaceV[sb][1]];
+          if (A2 == B && B2 == A)
+            code |= 1 << 4;
+          else if (!(A2 == A && B2 == B)) {
+            set_error("DG faces: neighbouring prisms are not stacked the same way up");
+            return -1;
+          }
+        }
+        fnb[5 * (size_t)ea + sa] = eb;
+        fcode[5 * (size_t)ea + sa] = code;
+        fhf[5 * (size_t)ea + sa] = hf;
+      }
+    }
+    i = j;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ double sel4(int p, double a0, double a1, double a2, double a3) {
+  return p == 0 ? a0 : p == 1 ? a1 : p == 2 ? a2 : a3;
+}
+
+__device__ __forceinline__ double pw(double x, int a) {
+  return a == 0 ? 1.0 : a == 1 ? x : a == 2 ? x * x : x * x * x;
+}
+
+// shape functions t*16 + c16 (t < NT) and their reference derivatives at eta
+template <int NS, int NT>
+__device__ __forceinline__ void mono_eval(const int* ex, int c16, const double* eta,
+                                          double (&u)[4][NT]) {
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int i = 16 * t + c16;
+    if (i < NS) {
+      const int code = ex[i], a = code & 3, b = (code >> 2) & 3, c = (code >> 4) & 3;
+      const double pa = pw(eta[0], a), pb = pw(eta[1], b), pc = pw(eta[2], c);
+      const double da = a ? a * pw(eta[0], a - 1) : 0.0;
+      const double db = b ? b * pw(eta[1], b - 1) : 0.0;
+      const double dc = c ? c * pw(eta[2], c - 1) : 0.0;
+      u[0][t] = pa * pb * pc;
+      u[1][t] = da * pb * pc;
+      u[2][t] = pa * db * pc;
+      u[3][t] = pa * pb * dc;
+    } else {
+      u[0][t] = u[1][t] = u[2][t] = u[3][t] = 0.0;
+    }
+  }
+}
+
+// element-local coordinates of face point xf on side s (the first-neighbour
+// branch of mmr_fa_elem_coor, mms_prism_datstr.c:1541-1700)
+__device__ __forceinline__ void face_eta(int s, const double* xf, double* eta) {
+  if (s == 0) { eta[0] = xf[1]; eta[1] = xf[0]; eta[2] = -1.0; }
+  else if (s == 1) { eta[0] = xf[0]; eta[1] = xf[1]; eta[2] = 1.0; }
+  else if (s == 2) { eta[0] = 0.5 * (xf[0] + 1.0); eta[1] = 0.0; eta[2] = xf[1]; }
+  else if (s == 3) { eta[0] = 0.5 * (-xf[0] + 1.0); eta[1] = 1.0 - eta[0]; eta[2] = xf[1]; }
+  else { eta[0] = 0.0; eta[1] = 0.5 * (-xf[0] + 1.0); eta[2] = xf[1]; }
+}
+
+// the same point in the neighbour across side s (code from build_faces)
+__device__ __forceinline__ void neighbour_eta(int s, int code, const double* eta, double* eta2) {
+  const int s2 = code & 15;
+  if (s < 2) {
+    const double lam[3] = {1.0 - eta[0] - eta[1], eta[0], eta[1]};
+    const int m1 = (code >> 6) & 3, m2 = (code >> 8) & 3;
+    eta2[0] = m1 == 0 ? lam[0] : m1 == 1 ? lam[1] : lam[2];
+    eta2[1] = m2 == 0 ? lam[0] : m2 == 1 ? lam[1] : lam[2];
+    eta2[2] = s2 == 0 ? -1.0 : 1.0;
+  } else {
+    double t = s == 2 ? eta[0] : s == 3 ? eta[1] : 1.0 - eta[1];
+    if ((code >> 4) & 1) t = 1.0 - t;
+    if (s2 == 2) { eta2[0] = t; eta2[1] = 0.0; }
+    else if (s2 == 3) { eta2[0] = 1.0 - t; eta2[1] = t; }
+    else { eta2[0] = 0.0; eta2[1] = 1.0 - t; }
+    eta2[2] = eta[2];
+  }
+}
+
+// Jacobian columns, inverse and point of the linear prism map at eta
+__device__ __forceinline__ double prism_jac(const double* xn, const double* eta, double* J,
+                                            double* Ji, double* x) {
+  const double e0 = eta[0], e1 = eta[1], z = eta[2];
+  const double gp[6] = {(1.0 - e0 - e1) * (1.0 - z) / 2.0, e0 * (1.0 - z) / 2.0,
+                        e1 * (1.0 - z) / 2.0, (1.0 - e0 - e1) * (1.0 + z) / 2.0,
+                        e0 * (1.0 + z) / 2.0, e1 * (1.0 + z) / 2.0};
+  const double gx[6] = {-(1.0 - z) / 2.0, (1.0 - z) / 2.0, 0.0,
+                        -(1.0 + z) / 2.0, (1.0 + z) / 2.0, 0.0};
+  const double gy[6] = {-(1.0 - z) / 2.0, 0.0, (1.0 - z) / 2.0,
+                        -(1.0 + z) / 2.0, 0.0, (1.0 + z) / 2.0};
+  const double gz[6] = {-(1.0 - e0 - e1) / 2.0, -e0 / 2.0, -e1 / 2.0,
+                        (1.0 - e0 - e1) / 2.0, e0 / 2.0, e1 / 2.0};
+#pragma unroll
+  for (int k = 0; k < 9; k++) J[k] = 0.0;
+  x[0] = x[1] = x[2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      const double c = xn[3 * i + r];
+      J[3 * r] += c * gx[i]; J[3 * r + 1] += c * gy[i]; J[3 * r + 2] += c * gz[i];
+      x[r] += c * gp[i];
+    }
+  const double* m = J;
+  const double det = m[0] * (m[4] * m[8] - m[7] * m[5]) + m[3] * (m[7] * m[2] - m[1] * m[8]) +
+                     m[6] * (m[1] * m[5] - m[4] * m[2]);
+  const double r = 1.0 / det;
+  Ji[0] = (m[4] * m[8] - m[7] * m[5]) * r;
+  Ji[3] = (-m[3] * m[8] + m[6] * m[5]) * r;
+  Ji[6] = (m[3] * m[7] - m[6] * m[4]) * r;
+  Ji[1] = (m[7] * m[2] - m[1] * m[8]) * r;
+  Ji[4] = (m[0] * m[8] - m[6] * m[2]) * r;
+  Ji[7] = (-m[0] * m[7] + m[6] * m[1]) * r;
+  Ji[2] = (m[1] * m[5] - m[4] * m[2]) * r;
+  Ji[5] = (-m[0] * m[5] + m[3] * m[2]) * r;
+  Ji[8] = (m[0] * m[4] - m[3] * m[1]) * r;
+  return det;
+}
+
+// area element |ds| and unit outward normal of side s (apr_elem_calc_3D
+// with Control = 3 + s, aps_dg_util.c:334-367)
+__device__ __forceinline__ double face_ds(int s, const double* J, double* n) {
+  double a[3], b[3];
+  const double t0[3] = {J[0], J[3], J[6]}, t1[3] = {J[1], J[4], J[7]}, t2[3] = {J[2], J[5], J[8]};
+  double sg = 1.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    if (s == 0) { a[r] = t0[r]; b[r] = t1[r]; }
+    else if (s == 1) { a[r] = t0[r]; b[r] = t1[r]; }
+    else if (s == 2) { a[r] = t0[r]; b[r] = t2[r]; }
+    else if (s == 3) { a[r] = t1[r] - t0[r]; b[r] = t2[r]; }
+    else { a[r] = t2[r]; b[r] = t1[r]; }
+  }
+  if (s == 0) sg = -1.0;
+  double ds[3] = {sg * (a[1] * b[2] - a[2] * b[1]), sg * (a[2] * b[0] - a[0] * b[2]),
+                  sg * (a[0] * b[1] - a[1] * b[0])};
+  const double d = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  n[0] = ds[0] / d; n[1] = ds[1] / d; n[2] = ds[2] / d;
+  return d;
+}
+
+struct DgRowArgs {
+  DgElemArgs E;        // element data, coefficients, volume rule
+  const int* fnb;
+  const int* fcode;
+  const double* fhf;
+  const double* fval;
+  const double* qt;    // [ngt][3]
+  const double* qq;    // [ngq][3]
+  double penalty;      // min(px, pz) * penalty constant
+  int ex[40];
+  double* blk;         // [nel][6][NS*NS]
+  double* brhs;        // [nel][NS]
+};
+
+// coefficients of element e, full 3x3 a, b (diffusion/convection masks applied)
+__device__ __forceinline__ void row_coefs(const DgElemArgs& A, int e, double* a, double* b,
+                                          double& mc, double* q, double& s) {
+  dg_coefs(A, e, a, b, mc, q[0], q[1], q[2], s);
+}
+
+// One wave per element builds the element's whole block row: the volume
+// integral and the element's side of each of its five faces go into the
+// slot-0 block, the coupling to the neighbour across side s into slot 1+s;
+// every face term is a 4x4 point matrix between (phi, d/deta phi) of the two
+// functions, so each face point is one K-step of v_mfma_f64_16x16x4 as in
+// the volume integral.  Shape functions are evaluated on the fly from their
+// exponents.
+template <int PX, int PZ>
+__global__ void __launch_bounds__(256) k_dg_rows(DgRowArgs R) {
+  using S = DgShape<PX, PZ>;
+  constexpr int NS = S::NS, NG = S::NG, NT = S::NT;
+  constexpr int NGT = tri_points(PX), NGQ = (PX + 1) * (PZ + 1);
+  constexpr int WREC = (NG * 20 > 16 * 42 ? NG * 20 : 16 * 42) + 40;
+  __shared__ double lds[4 * WREC];
+  __shared__ int sex[40];
+  const DgElemArgs& A = R.E;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* rec = lds + wave * WREC;  // point records
+  double* sX = rec + WREC - 40;     // own nodes [18]
+  double* sN = sX + 18;             // neighbour nodes [18]
+  if (threadIdx.x < 40) sex[threadIdx.x] = R.ex[threadIdx.x];
+  const int p = lane >> 4, c16 = lane & 15;
+  for (int base = blockIdx.x * 4; base < A.nel; base += gridDim.x * 4) {
+    const int e = base + wave;
+    const bool valid = e < A.nel;
+    if (valid && lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
+    __syncthreads();
+    double a[9], b[3], mc, q[3], sv;
+    row_coefs(A, valid ? e : 0, a, b, mc, q, sv);
+    // volume point records: Mhat (16) and fhat (4) per point
+    if (valid && lane < NG) {
+      const double* qg = A.qv + 4 * lane;
+      double J[9], Ji[9], x[3];
+      const double vol = fabs(prism_jac(sX, qg, J, Ji, x)) * qg[3];
+      double s = sv;
+      if (A.src_model != SRC_CONST && A.mask.s) {
+        double uu, lapl;
+        exact_sol(A.src_model, x, &uu, &lapl);
+        s = -lapl;
+      }
+      double* M = rec + 20 * lane;
+      M[0] = vol * mc;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        M[1 + k] = vol * (b[0] * Ji[3 * k] + b[1] * Ji[3 * k + 1] + b[2] * Ji[3 * k + 2]);
+        M[4 * (1 + k)] = 0.0;
+        double t[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+          t[r] = Ji[3 * k] * a[3 * r] + Ji[3 * k + 1] * a[3 * r + 1] + Ji[3 * k + 2] * a[3 * r + 2];
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+          M[4 * (1 + k) + 1 + l] = vol * (t[0] * Ji[3 * l] + t[1] * Ji[3 * l + 1] + t[2] * Ji[3 * l + 2]);
+      }
+      M[16] = vol * s;
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        M[17 + k] = vol * (q[0] * Ji[3 * k] + q[1] * Ji[3 * k + 1] + q[2] * Ji[3 * k + 2]);
+    }
+    __syncthreads();
+    dbl4 accD[NT][NT], accB[NT][NT];
+    double f[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) f[t] = 0.0;
+#pragma unroll
+    for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+      for (int it = 0; it < NT; it++) accD[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+      for (int g = 0; g < NG; g++) {
+        double u[4][NT];
+        mono_eval<NS, NT>(sex, c16, A.qv + 4 * g, u);
+        const double* M = rec + 20 * g;
+        double av[NT], bv[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+          av[t] = sel4(p, u[0][t], u[1][t], u[2][t], u[3][t]);
+          bv[t] = M[4 * p] * u[0][t] + M[4 * p + 1] * u[1][t] + M[4 * p + 2] * u[2][t] +
+                  M[4 * p + 3] * u[3][t];
+          f[t] += M[16] * u[0][t] + M[17] * u[1][t] + M[18] * u[2][t] + M[19] * u[3][t];
+        }
+#pragma unroll
+        for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+          for (int it = 0; it < NT; it++)
+            accD[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], accD[jt][it], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    // faces
+    for (int s = 0; s < 5; s++) {
+      const int fi = 5 * (valid ? e : 0) + s;
+      const int nb = R.fnb[fi], code = R.fcode[fi];
+      const double hf = R.fhf[fi], bval = R.fval[fi];
+      const bool tri = s < 2, inner = nb >= 0;
+      const int npf = tri ? NGT : NGQ;
+      const double* rule = tri ? R.qt : R.qq;
+      if (valid && inner && lane < 18) sN[lane] = A.nc[(size_t)lane * A.nel + nb];
+      __syncthreads();
+      double an[9], bn_[3], mcn, qn[3], svn;
+      if (inner) row_coefs(A, valid ? nb : 0, an, bn_, mcn, qn, svn);
+      if (valid && lane < npf) {
+        double* Rg = rec + 42 * lane;
+        double eta[3], J[9], Ji[9], x[3], n[3];
+        face_eta(s, rule + 3 * lane, eta);
+        prism_jac(sX, eta, J, Ji, x);
+        const double area = face_ds(s, J, n) * rule[3 * lane + 2];
+        double Ane[3], gh[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) Ane[c] = a[c] * n[0] + a[3 + c] * n[1] + a[6 + c] * n[2];
+#pragma unroll
+        for (int k = 0; k < 3; k++) gh[k] = Ji[3 * k] * Ane[0] + Ji[3 * k + 1] * Ane[1] + Ji[3 * k + 2] * Ane[2];
+        const double bne = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+        const double pen = R.penalty / hf;
+#pragma unroll
+        for (int k = 0; k < 36; k++) Rg[k] = 0.0;
+        Rg[36] = eta[0]; Rg[37] = eta[1]; Rg[38] = eta[2];
+        if (inner) {
+          double eta2[3], J2[9], Ji2[9], x2[3];
+          neighbour_eta(s, code, eta, eta2);
+          prism_jac(sN, eta2, J2, Ji2, x2);
+          double Ann[3], hh[3];
+#pragma unroll
+          for (int c = 0; c < 3; c++) Ann[c] = an[c] * n[0] + an[3 + c] * n[1] + an[6 + c] * n[2];
+#pragma unroll
+          for (int k = 0; k < 3; k++)
+            hh[k] = Ji2[3 * k] * Ann[0] + Ji2[3 * k + 1] * Ann[1] + Ji2[3 * k + 2] * Ann[2];
+          const double bnn = bn_[0] * n[0] + bn_[1] * n[1] + bn_[2] * n[2];
+          const double nv = 0.5 * (bne + bnn);
+          // own block: 0.5 (phi_j g_i - g_j phi_i) + (pen + upwind) phi_j phi_i
+          Rg[0] = area * (pen + (nv > 0 ? bne : 0.0));
+          // coupling: -0.5 (psi_j g_i + h_j phi_i) + (-pen + upwind) psi_j phi_i
+          Rg[16] = area * (-pen + (nv < 0 ? bne : 0.0));
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            Rg[1 + k] = 0.5 * area * gh[k];
+            Rg[4 * (1 + k)] = -0.5 * area * gh[k];
+            Rg[17 + k] = -0.5 * area * gh[k];
+            Rg[16 + 4 * (1 + k)] = -0.5 * area * hh[k];
+          }
+          Rg[39] = eta2[0]; Rg[40] = eta2[1]; Rg[41] = eta2[2];
+        } else if (nb == -1) {
+          // Dirichlet (weakform_dg.c:1036-1120)
+          double fd = bval;
+          if (A.src_model != SRC_CONST) {
+            double lapl;
+            exact_sol(A.src_model, x, &fd, &lapl);
+          }
+          Rg[0] = area * (pen + (bne >= 0 ? bne : 0.0));
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            Rg[1 + k] = area * gh[k];
+            Rg[4 * (1 + k)] = -area * gh[k];
+            Rg[33 + k] = area * fd * gh[k];
+          }
+          Rg[32] = area * (pen * fd - (bne < 0 ? bne * fd : 0.0));
+        } else {
+          // Neumann (weakform_dg.c:1148-1185); data: the exact normal derivative
+          double gd = bval;
+          if (A.src_model != SRC_CONST) {
+            const double a1 = A.src_model == SRC_DIFF_IN_BOX ? 100.0 : 1.0;
+            const double ex = exp((-x[0] * x[0] - x[1] * x[1] - x[2] * x[2]) / a1);
+            gd = (-2.0 * x[0] * n[0] - 2.0 * x[1] * n[1] - 2.0 * x[2] * n[2]) * ex / a1;
+          }
+          Rg[0] = area * (bne > 0 ? bne : 0.0);
+          Rg[32] = area * gd;
+        }
+      }
+      __syncthreads();
+      if (valid) {
+#pragma unroll
+        for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+          for (int it = 0; it < NT; it++) accB[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int g = 0; g < npf; g++) {
+          const double* Rg = rec + 42 * g;
+          double u[4][NT];
+          mono_eval<NS, NT>(sex, c16, Rg + 36, u);
+          double av[NT], bv[NT];
+#pragma unroll
+          for (int t = 0; t < NT; t++) {
+            av[t] = sel4(p, u[0][t], u[1][t], u[2][t], u[3][t]);
+            bv[t] = Rg[4 * p] * u[0][t] + Rg[4 * p + 1] * u[1][t] + Rg[4 * p + 2] * u[2][t] +
+                    Rg[4 * p + 3] * u[3][t];
+            f[t] += Rg[32] * u[0][t] + Rg[33] * u[1][t] + Rg[34] * u[2][t] + Rg[35] * u[3][t];
+          }
+#pragma unroll
+          for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+            for (int it = 0; it < NT; it++)
+              accD[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], accD[jt][it], 0, 0, 0);
+          if (inner) {
+            double w2[4][NT];
+            mono_eval<NS, NT>(sex, c16, Rg + 39, w2);
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+              av[t] = sel4(p, w2[0][t], w2[1][t], w2[2][t], w2[3][t]);
+              bv[t] = Rg[16 + 4 * p] * u[0][t] + Rg[17 + 4 * p] * u[1][t] +
+                      Rg[18 + 4 * p] * u[2][t] + Rg[19 + 4 * p] * u[3][t];
+            }
+#pragma unroll
+            for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+              for (int it = 0; it < NT; it++)
+                accB[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], accB[jt][it], 0, 0, 0);
+          }
+        }
+        if (inner) {
+          double* out = R.blk + ((size_t)e * 6 + 1 + s) * NS * NS;
+#pragma unroll
+          for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+            for (int it = 0; it < NT; it++)
+#pragma unroll
+              for (int r = 0; r < 4; r++) {
+                const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
+                if (j < NS && i < NS) out[j * NS + i] = accB[jt][it][r];
+              }
+        }
+      }
+      __syncthreads();
+    }
+    if (valid) {
+      double* out = R.blk + (size_t)e * 6 * NS * NS;
+#pragma unroll
+      for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+        for (int it = 0; it < NT; it++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
+            if (j < NS && i < NS) out[j * NS + i] = accD[jt][it][r];
+          }
+      if (p == 0) {
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+          if (16 * t + c16 < NS) R.brhs[(size_t)e * NS + 16 * t + c16] = f[t];
+      }
+    }
+  }
+}
+
+// inverse of each element's own block by Gauss-Jordan elimination with
+// partial pivoting (the reference factors these blocks with LAPACK dgetrf for
+// its block Jacobi, lad_block/las_block_intf.c:1057); one wave per element,
+// lane r owns row r of [A | I] in LDS.  A zero pivot sets *bad.
+constexpr int inv_waves(int ns) { return ns >= 30 ? 2 : 4; }
+
+template <int NS>
+__global__ void __launch_bounds__(64 * inv_waves(NS)) k_dg_inv(int nel, const double* blk,
+                                                             double* dinv, int* bad) {
+  constexpr int IW = inv_waves(NS);
+  __shared__ double W[IW][NS][2 * NS + 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double (*M)[2 * NS + 1] = W[wave];
+  for (int e = blockIdx.x * IW + wave; e < nel; e += gridDim.x * IW) {
+    const double* Ae = blk + (size_t)e * 6 * NS * NS;
+    if (lane < NS)
+      for (int j = 0; j < NS; j++) {
+        M[lane][j] = Ae[(size_t)j * NS + lane];
+        M[lane][NS + j] = lane == j ? 1.0 : 0.0;
+      }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int c = 0; c < NS; c++) {
+      // pivot: the largest |M[r][c]| over r >= c (first one on ties)
+      double v = (lane >= c && lane < NS) ? fabs(M[lane][c]) : -1.0;
+      int r = lane;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double v2 = __shfl_xor(v, o);
+        const int r2 = __shfl_xor(r, o);
+        if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
+      }
+      if (v == 0.0) {
+        if (lane == 0) bad[0] = 1;
+        break;
+      }
+      if (r != c && lane < 2 * NS) {
+        const double t = M[c][lane];
+        M[c][lane] = M[r][lane];
+        M[r][lane] = t;
+      }
+      if (r != c && lane + 64 < 2 * NS) {
+        const double t = M[c][lane + 64];
+        M[c][lane + 64] = M[r][lane + 64];
+        M[r][lane + 64] = t;
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const double piv = M[c][c];
+      double fac = 0.0;
+      if (lane < NS && lane != c) fac = M[lane][c] / piv;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (lane < NS && lane != c)
+        for (int k = 0; k < 2 * NS; k++) M[lane][k] -= fac * M[c][k];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (lane < NS && lane == c)
+        for (int k = 0; k < 2 * NS; k++) M[c][k] /= piv;
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    double* Di = dinv + (size_t)e * NS * NS;
+    if (lane < NS)
+      for (int j = 0; j < NS; j++) Di[(size_t)j * NS + lane] = M[lane][NS + j];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+}
+
+// w = scale * Dinv (A x) for the block rows (mode 0, scale = sc[j] when sc,
+// else 1; Dinv skipped when dinv is null), or r = Dinv (b - A x) (mode 1).
+// 64 / NS elements per wave, lane = (element, row).
+template <int NS>
+__global__ void __launch_bounds__(256) k_dg_op(int nel, const double* blk, const int* fnb,
+                                               const double* dinv, const double* x,
+                                               const double* b, const double* sc, int j,
+                                               const int* stop, double* w, int mode) {
+  if (stop && *stop) return;
+  constexpr int EPW = 64 / NS;
+  __shared__ double T[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int el = lane / NS, i = lane - el * NS;
+  const double s = sc ? sc[j] : 1.0;
+  for (int e0 = (blockIdx.x * 4 + wave) * EPW; e0 < nel; e0 += gridDim.x * 4 * EPW) {
+    const int e = e0 + el;
+    const bool act = el < EPW && e < nel;
+    double acc = 0.0;
+    if (act) {
+      const double* Ae = blk + (size_t)e * 6 * NS * NS;
+      const double* xe = x + (size_t)e * NS;
+      for (int k = 0; k < NS; k++) acc += Ae[(size_t)k * NS + i] * xe[k];
+      for (int sd = 0; sd < 5; sd++) {
+        const int nb = fnb[5 * (size_t)e + sd];
+        if (nb < 0) continue;
+        const double* B = Ae + (size_t)(1 + sd) * NS * NS;
+        const double* xn = x + (size_t)nb * NS;
+        for (int k = 0; k < NS; k++) acc += B[(size_t)k * NS + i] * xn[k];
+      }
+      if (mode == 1) acc = b[(size_t)e * NS + i] - acc;
+    }
+    if (!dinv) {
+      if (act) w[(size_t)e * NS + i] = (mode == 1 ? 1.0 : s) * acc;
+      continue;
+    }
+    T[wave][lane] = acc;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    double r = 0.0;
+    if (act) {
+      const double* Di = dinv + (size_t)e * NS * NS;
+      const double* t = &T[wave][el * NS];
+      for (int k = 0; k < NS; k++) r += Di[(size_t)k * NS + i] * t[k];
+      w[(size_t)e * NS + i] = (mode == 1 ? 1.0 : s) * r;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+}
+
+}  // namespace
+
+// GMRES operator over the DG block system (precon 0: none, 1: block Jacobi)
+struct DgOp : ExtOp {
+  DgProblem* D = nullptr;
+  int precon = 1;
+  int grid() const {
+    const int epw = 64 / D->ns;
+    const long long waves = (D->nel + epw - 1) / epw;
+    return (int)std::max<long long>(1, std::min<long long>((waves + 3) / 4, (long long)D->ncu * 16));
+  }
+  template <int NS>
+  int launch(const double* x, const double* b, const double* sc, int j, const int* stop,
+             double* w, int mode) {
+    hipLaunchKernelGGL((k_dg_op<NS>), dim3(grid()), dim3(256), 0, stream(), D->nel, D->d_blk,
+                       D->d_fnb, precon ? D->d_dinv : nullptr, x, b, sc, j, stop, w, mode);
+    MF_HIP(hipGetLastError());
+    return 0;
+  }
+  int run(const double* x, const double* b, const double* sc, int j, const int* stop, double* w,
+          int mode) {
+    switch (D->ns) {
+      case 6: return launch<6>(x, b, sc, j, stop, w, mode);
+      case 9: return launch<9>(x, b, sc, j, stop, w, mode);
+      case 10: return launch<10>(x, b, sc, j, stop, w, mode);
+      case 12: return launch<12>(x, b, sc, j, stop, w, mode);
+      case 18: return launch<18>(x, b, sc, j, stop, w, mode);
+      case 20: return launch<20>(x, b, sc, j, stop, w, mode);
+      case 24: return launch<24>(x, b, sc, j, stop, w, mode);
+      case 30: return launch<30>(x, b, sc, j, stop, w, mode);
+      case 40: return launch<40>(x, b, sc, j, stop, w, mode);
+    }
+    set_error("DG operator: unsupported block size");
+    return -1;
+  }
+  int residual(const double* b, const double* x, double* r) override {
+    return run(x, b, nullptr, 0, nullptr, r, 1);
+  }
+  int apply(const double* v, const double* scale, int j, double* w, const int* stop) override {
+    return run(v, nullptr, scale, j, stop, w, 0);
+  }
+};
+
+namespace {
+
+void free_dg_solver(DgProblem& D) {
+  if (D.P) {
+    Problem& P = *D.P;
+    void* ptrs[] = {P.d_V, P.d_H, P.d_partials, P.d_stat, P.d_x};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+    if (P.h_stat) (void)hipHostFree(P.h_stat);
+    delete D.P;
+    D.P = nullptr;
+  }
+  delete D.op;
+  D.op = nullptr;
+}
+
+template <int PX, int PZ>
+int launch_rows_t(DgProblem& D) {
+  DgRowArgs R;
+  DgElemArgs& A = R.E;
+  A.nel = D.nel; A.e0 = 0; A.count = D.nel;
+  A.nc = D.d_nc; A.qv = D.d_qv; A.U = D.d_U;
+  A.coef_model = D.coef_model; A.src_model = D.src_model; A.mask = D.mask;
+  for (int k = 0; k < NCOEF; k++) A.cc[k] = D.cc[k];
+  A.coef = D.d_coef; A.stiff = nullptr; A.rhs = nullptr;
+  R.fnb = D.d_fnb; R.fcode = D.d_fcode; R.fhf = D.d_fhf; R.fval = D.d_fval;
+  R.qt = D.d_qt; R.qq = D.d_qq;
+  R.penalty = std::min(PX, PZ) * D.penalty_const;
+  for (int k = 0; k < 40; k++) R.ex[k] = D.ex[k];
+  R.blk = D.d_blk; R.brhs = D.d_brhs;
+  const int blocks = std::max(1, std::min((D.nel + 3) / 4, D.ncu * 8));
+  hipLaunchKernelGGL((k_dg_rows<PX, PZ>), dim3(blocks), dim3(256), 0, stream(), R);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_rows(DgProblem& D) {
+#define MF_DG_CASE(X, Z) \
+  if (D.px == X && D.pz == Z) return launch_rows_t<X, Z>(D);
+  MF_DG_CASE(1, 1) MF_DG_CASE(2, 1) MF_DG_CASE(3, 1)
+  MF_DG_CASE(1, 2) MF_DG_CASE(2, 2) MF_DG_CASE(3, 2)
+  MF_DG_CASE(1, 3) MF_DG_CASE(2, 3) MF_DG_CASE(3, 3)
+#undef MF_DG_CASE
+  set_error("DG degree out of range");
+  return -1;
+}
+
+template <int NS>
+int launch_inv_t(DgProblem& D, int* d_bad) {
+  constexpr int IW = inv_waves(NS);
+  const int blocks = std::max(1, std::min((D.nel + IW - 1) / IW, D.ncu * 8));
+  hipLaunchKernelGGL((k_dg_inv<NS>), dim3(blocks), dim3(64 * IW), 0, stream(), D.nel, D.d_blk,
+                     D.d_dinv, d_bad);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+int fill_block_jacobi(DgProblem& D) {
+  if (!D.d_dinv) MF_HIP(hipMalloc(&D.d_dinv, (size_t)D.nel * D.ns * D.ns * sizeof(double)));
+  int* d_bad = nullptr;
+  MF_HIP(hipMalloc(&d_bad, sizeof(int)));
+  MF_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), stream()));
+  int rc = -1;
+  switch (D.ns) {
+    case 6: rc = launch_inv_t<6>(D, d_bad); break;
+    case 9: rc = launch_inv_t<9>(D, d_bad); break;
+    case 10: rc = launch_inv_t<10>(D, d_bad); break;
+    case 12: rc = launch_inv_t<12>(D, d_bad); break;
+    case 18: rc = launch_inv_t<18>(D, d_bad); break;
+    case 20: rc = launch_inv_t<20>(D, d_bad); break;
+    case 24: rc = launch_inv_t<24>(D, d_bad); break;
+    case 30: rc = launch_inv_t<30>(D, d_bad); break;
+    case 40: rc = launch_inv_t<40>(D, d_bad); break;
+    default: set_error("DG block Jacobi: unsupported block size");
+  }
+  int bad = 0;
+  if (rc == 0) {
+    MF_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    MF_HIP(hipStreamSynchronize(stream()));
+  }
+  (void)hipFree(d_bad);
+  if (rc) return rc;
+  if (bad) {
+    set_error("DG block Jacobi: a diagonal block is singular");
+    return -1;
+  }
+  D.dinv_valid = true;
+  return 0;
+}
+
 }  // namespace
 
 }  // namespace mf
@@ -528,6 +1349,34 @@ int mf_dg_create(int mesh, int pdeg) {
   hipLaunchKernelGGL(k_dg_table, dim3(1), dim3(64), 0, stream(), px, pz, D->ng, D->nsp,
                      D->d_qv, D->d_U);
   MF_HIP(hipGetLastError());
+  // faces, face rules, shape-function exponents
+  std::vector<int> fnb, fcode;
+  std::vector<double> fhf, qt, qq;
+  if (build_faces(*m, fnb, fcode, fhf)) {
+    free_dg(*D);
+    return -1;
+  }
+  face_rules(px, pz, qt, qq);
+  D->ngt = (int)qt.size() / 3;
+  D->ngq = (int)qq.size() / 3;
+  shape_exponents(px, pz, D->ex);
+  const size_t nf = 5 * nel;
+  MF_HIP(hipMalloc(&D->d_fnb, nf * sizeof(int)));
+  MF_HIP(hipMalloc(&D->d_fcode, nf * sizeof(int)));
+  MF_HIP(hipMalloc(&D->d_fhf, nf * sizeof(double)));
+  MF_HIP(hipMalloc(&D->d_fval, nf * sizeof(double)));
+  MF_HIP(hipMalloc(&D->d_qt, qt.size() * sizeof(double)));
+  MF_HIP(hipMalloc(&D->d_qq, qq.size() * sizeof(double)));
+  MF_HIP(hipMemcpyAsync(D->d_fnb, fnb.data(), nf * sizeof(int), hipMemcpyHostToDevice, stream()));
+  MF_HIP(hipMemcpyAsync(D->d_fcode, fcode.data(), nf * sizeof(int), hipMemcpyHostToDevice,
+                        stream()));
+  MF_HIP(hipMemcpyAsync(D->d_fhf, fhf.data(), nf * sizeof(double), hipMemcpyHostToDevice,
+                        stream()));
+  MF_HIP(hipMemsetAsync(D->d_fval, 0, nf * sizeof(double), stream()));
+  MF_HIP(hipMemcpyAsync(D->d_qt, qt.data(), qt.size() * sizeof(double), hipMemcpyHostToDevice,
+                        stream()));
+  MF_HIP(hipMemcpyAsync(D->d_qq, qq.data(), qq.size() * sizeof(double), hipMemcpyHostToDevice,
+                        stream()));
   MF_HIP(hipStreamSynchronize(stream()));
   const int id = g_next_dg++;
   g_dg[id] = std::move(D);
@@ -564,6 +1413,7 @@ int mf_dg_set_coefficients(int did, const int* ind, int coef_model, const double
   M.q = ind[21] ? 2 : (ind[19] ? 1 : 0);
   M.s = ind[22] != 0;
   D->mask = M;
+  D->assembled = D->dinv_valid = false;
   for (int k = 0; k < NCOEF; k++) D->cc[k] = coef_const ? coef_const[k] : 0.0;
   D->coef_model = coef_model;
   D->src_model = src_model;
@@ -643,6 +1493,129 @@ int mf_dg_get_element_matrices(int did, int first, int count, double* stiff, dou
   if (rhs)
     MF_HIP(hipMemcpyAsync(rhs, D->d_rhs + first * ns, count * ns * sizeof(double),
                           hipMemcpyDeviceToHost, stream()));
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_dg_set_penalty(int did, double penalty_const) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!(penalty_const >= 0.0)) {
+    set_error("mf_dg_set_penalty: the penalty constant must be >= 0");
+    return -1;
+  }
+  D->penalty_const = penalty_const;
+  D->assembled = D->dinv_valid = false;
+  return 0;
+}
+
+int mf_dg_set_boundary_values(int did, const double* val) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  const size_t nf = 5 * (size_t)D->nel;
+  if (val)
+    MF_HIP(hipMemcpyAsync(D->d_fval, val, nf * sizeof(double), hipMemcpyHostToDevice, stream()));
+  else
+    MF_HIP(hipMemsetAsync(D->d_fval, 0, nf * sizeof(double), stream()));
+  MF_HIP(hipStreamSynchronize(stream()));
+  D->assembled = D->dinv_valid = false;
+  return 0;
+}
+
+int mf_dg_get_faces(int did, int* nb, int* code, double* hf) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  const size_t nf = 5 * (size_t)D->nel;
+  if (nb) MF_HIP(hipMemcpyAsync(nb, D->d_fnb, nf * sizeof(int), hipMemcpyDeviceToHost, stream()));
+  if (code)
+    MF_HIP(hipMemcpyAsync(code, D->d_fcode, nf * sizeof(int), hipMemcpyDeviceToHost, stream()));
+  if (hf) MF_HIP(hipMemcpyAsync(hf, D->d_fhf, nf * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_dg_assemble(int did) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  const size_t ns = D->ns, nel = D->nel;
+  if (!D->d_blk) {
+    MF_HIP(hipMalloc(&D->d_blk, nel * 6 * ns * ns * sizeof(double)));
+    MF_HIP(hipMalloc(&D->d_brhs, nel * ns * sizeof(double)));
+    // boundary slots are never written: keep them zero
+    MF_HIP(hipMemsetAsync(D->d_blk, 0, nel * 6 * ns * ns * sizeof(double), stream()));
+  }
+  const int rc = launch_rows(*D);
+  if (rc) return rc;
+  MF_HIP(hipStreamSynchronize(stream()));
+  D->assembled = true;
+  D->dinv_valid = false;
+  return 0;
+}
+
+int mf_dg_get_block_row(int did, int e, double* blocks, double* rhs) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->assembled || e < 0 || e >= D->nel) {
+    set_error("mf_dg_get_block_row: not assembled or element out of range");
+    return -1;
+  }
+  const size_t ns = D->ns;
+  if (blocks)
+    MF_HIP(hipMemcpyAsync(blocks, D->d_blk + (size_t)e * 6 * ns * ns, 6 * ns * ns * sizeof(double),
+                          hipMemcpyDeviceToHost, stream()));
+  if (rhs)
+    MF_HIP(hipMemcpyAsync(rhs, D->d_brhs + (size_t)e * ns, ns * sizeof(double),
+                          hipMemcpyDeviceToHost, stream()));
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_dg_fill_precon(int did) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->assembled) {
+    set_error("mf_dg_fill_precon: the system is not assembled");
+    return -1;
+  }
+  return fill_block_jacobi(*D);
+}
+
+int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, double* x,
+                int* iters, double* res_rel, double* resid) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->assembled) {
+    set_error("mf_dg_gmres: the system is not assembled");
+    return -1;
+  }
+  if (precon != 0 && precon != 1) {
+    set_error("mf_dg_gmres: precon must be 0 (none) or 1 (block Jacobi)");
+    return -1;
+  }
+  if (precon == 1 && !D->dinv_valid && fill_block_jacobi(*D)) return -2;
+  if (!D->P) {
+    D->P = new Problem();
+    D->P->n = D->nel * D->ns;
+    D->P->el_type = kPrism;
+    D->op = new DgOp();
+    D->op->D = D;
+    D->P->ext = D->op;
+  }
+  D->op->precon = precon;
+  D->P->d_rhs = D->d_brhs;
+  if (!D->P->d_x) MF_HIP(hipMalloc(&D->P->d_x, (size_t)D->P->n * sizeof(double)));
+  return gmres_solve(*D->P, precon, krylbas, max_iter, tol, 0.0, 1, x, iters, res_rel, resid);
+}
+
+int mf_dg_get_solution(int did, double* x) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->P || !D->P->d_x) {
+    set_error("mf_dg_get_solution: no solve has run");
+    return -1;
+  }
+  MF_HIP(hipMemcpyAsync(x, D->P->d_x, (size_t)D->P->n * sizeof(double), hipMemcpyDeviceToHost,
+                        stream()));
   MF_HIP(hipStreamSynchronize(stream()));
   return 0;
 }

@@ -819,8 +819,14 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     return -1;
   }
   if (ensure_workspace(P, m)) return -2;
-  const bool ilu = precon == 2, gs = precon == 3;
-  if (gs) {
+  const bool ext = P.ext != nullptr;
+  const bool ilu = !ext && precon == 2, gs = !ext && precon == 3;
+  if (ext) {
+    if (op32) {
+      set_error("GMRES: no FP32 operator for this system");
+      return -1;
+    }
+  } else if (gs) {
     if (!P.sell) {
       set_error("GMRES: Gauss-Seidel needs SELL storage");
       return -1;
@@ -839,7 +845,7 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     if (launch_fill_jacobi(P)) return -2;
   }
   SolveArgs A;
-  fill_solve_args(P, A, (ilu || gs) ? 0 : precon, m);
+  fill_solve_args(P, A, (ilu || gs || ext) ? 0 : precon, m);
   A.b = d_b;
   A.x = d_x;
   A.op32 = op32;
@@ -850,7 +856,8 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     set_error("GMRES: the FP32 operator needs SELL storage and its FP32 values");
     return -1;
   }
-  const int G_op = P.sell ? grid_slices(P) : grid_tiles(P), G_up = grid_rows(P);
+  const int G_op = (P.sell || ext) ? grid_slices(P) : grid_tiles(P), G_up = grid_rows(P);
+  const double* d_scale = P.d_H + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
   if (!P.sell) A.sl_ptr = nullptr;
   hipStream_t s = stream();
   hipEvent_t e0, e1;
@@ -879,7 +886,10 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     st.stop = 0;
     st.j_done = 0;
     MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
-    if (gs) {
+    if (ext) {
+      if (P.ext->residual(A.b, A.x, A.V)) return -2;
+      hipLaunchKernelGGL(k_sumsq_v0, dim3(G_op), dim3(SB), 0, s, A);
+    } else if (gs) {
       // V_0 = GS_k(x; b) - x (the smoothing branch, lss_mkb_core.c:2001-2054)
       MF_HIP(hipMemcpyAsync(A.V, A.x, (size_t)P.n * sizeof(double), hipMemcpyDeviceToDevice, s));
       if (gs_sweeps(P, P.gs_sweeps, A.b, A.V, nullptr)) return -2;
@@ -929,10 +939,12 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     st.rnrm = rnrm2;
     MF_HIP(hipMemcpyAsync(P.d_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
-    if (ilu || gs) {
+    if (ilu || gs || ext) {
       for (int j = 0; j < m; j++) {
         double* w = A.V + (size_t)(j + 1) * A.ldv;
-        if (gs) {
+        if (ext) {
+          if (P.ext->apply(A.V + (size_t)j * A.ldv, d_scale, j, w, d_stop)) return -2;
+        } else if (gs) {
           hipLaunchKernelGGL(k_gs_op_init, dim3(gn), dim3(256), 0, s, A, j);
           if (gs_sweeps(P, P.gs_sweeps, nullptr, w, d_stop)) return -2;
           hipLaunchKernelGGL(k_gs_op_finish, dim3(gn), dim3(256), 0, s, A, j);

@@ -104,3 +104,95 @@ def test_dg_rejects_bad_input():
     tet = mf.Mesh.box_tetra(2, 2, 1, seed=1)
     with pytest.raises(mf.MFError):
         mf.DgProblem(tet, 101)
+
+
+# ------------------------------------------------ block system and solve
+def cube_mesh(n, jitter=0.0, seed=0):
+    """unit cube, n^3 hexes split into prisms along the (0,0)-(1,1) diagonal,
+    bottom/top faces Neumann (flag 11), sides Dirichlet (flag 1): the
+    reference's DIFF_IN_CUBE DG mesh refined log2 n times"""
+    a = mf.Mesh.box_prism(n, n, n, seed=seed, jitter=jitter).arrays()
+    bc = a["bf_bc"].copy()
+    bc[a["bf_side"] < 2] = 11
+    return mf.Mesh.from_arrays(a["coor"], a["el_type"], a["el_nodes"], a["bf_elem"],
+                               a["bf_side"], bc)
+
+
+def block_of(dg, A):
+    """compare every device block row with the oracle's assembled matrix"""
+    nb, _, _ = dg.faces()
+    ns = dg.ns
+    worst = 0.0
+    for e in range(dg.nel):
+        blk, rhs = dg.block_row(e)
+        rows = slice(e * ns, (e + 1) * ns)
+        ref_row = A[rows].toarray()
+        got = np.zeros_like(ref_row)
+        got[:, e * ns:(e + 1) * ns] += blk[0].T
+        for s in range(5):
+            if nb[e, s] >= 0:
+                got[:, nb[e, s] * ns:(nb[e, s] + 1) * ns] += blk[1 + s].T
+        worst = max(worst, np.abs(got - ref_row).max() / np.abs(ref_row).max())
+    return worst
+
+
+@pytest.mark.parametrize("pdeg", [101, 202, 303, 102, 201])
+def test_block_system_matches_oracle(pdeg):
+    """volume and NIPG face terms with per-element anisotropic diffusion and
+    convection (upwinding on both sides of interior faces), Dirichlet and
+    Neumann data from the exact solution, on a distorted mesh"""
+    from oracle_lib import dg_system_dense
+    mesh = cube_mesh(3, jitter=0.2, seed=pdeg)
+    a = mesh.arrays()
+    rng = np.random.default_rng(pdeg + 1)
+    ce = random_full(rng, mesh.nel)
+    ind = mf.coeff_indicator("m", "axx", "axy", "axz", "ayx", "ayy", "ayz", "azx", "azy", "azz",
+                             "bx", "by", "bz", "c", "s")
+    dg = mf.DgProblem(mesh, pdeg)
+    dg.set_coefficients(ind, mf.COEF_FULL, coef_el=ce.ravel(), src_model=mf.SRC_DIFF_IN_CUBE)
+    dg.set_penalty(1.5)
+    dg.assemble()
+    A, b = dg_system_dense(a, pdeg, ind, lambda e: ce[:, e], 1.5, mf.SRC_DIFF_IN_CUBE,
+                           sparse=True)
+    assert block_of(dg, A) <= 1e-12
+    rhs = np.concatenate([dg.block_row(e)[1] for e in range(dg.nel)])
+    assert np.abs(rhs - b).max() <= 1e-12 * np.abs(b).max()
+    dg.free()
+
+
+def read_error_table():
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dg",
+                        "diff_nipg_h_conv.txt")
+    out = {}
+    with open(path) as f:
+        for line in f:
+            t = line.replace(",", ".").split()
+            try:
+                vals = [float(v) for v in t[:5]]
+            except ValueError:
+                continue
+            if len(vals) == 5:
+                out[(int(round(1.0 / vals[0])), int(vals[1]))] = (int(vals[2]), vals[3], vals[4])
+    return out
+
+
+@pytest.mark.parametrize("n,pdeg", [(2, 303), (4, 101), (4, 202), (8, 101), (8, 202), (8, 303),
+                                    (16, 101), (16, 202)])
+def test_gmres_solution_reproduces_reference_error_table(n, pdeg):
+    """the device-assembled DIFF_IN_CUBE system solved by GMRES with block
+    Jacobi gives the reference's published L2 and H1 errors to every printed
+    digit (comparison_data/diff_nipg_h_conv.txt)"""
+    from oracle_lib import dg_errors
+    ndof, l2_ref, h1_ref = read_error_table()[(n, pdeg)]
+    mesh = cube_mesh(n)
+    dg = mf.DgProblem(mesh, pdeg)
+    dg.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM,
+                        coef_const=np.r_[np.eye(3).ravel(), np.zeros(12)],
+                        src_model=mf.SRC_DIFF_IN_CUBE)
+    dg.assemble()
+    assert dg.nel * dg.ns == ndof
+    x, info = dg.gmres(precon=1, krylbas=50, max_iter=20000, tol=1e-13)
+    assert info["converged"], info
+    l2, h1 = dg_errors(mesh.arrays(), pdeg, x, 1)
+    assert abs(l2 - l2_ref) <= 6e-10 and abs(h1 - h1_ref) <= 6e-10, (l2, h1, l2_ref, h1_ref, info)
+    dg.free()
