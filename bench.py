@@ -259,6 +259,142 @@ def bench_serve(args):
     print(json.dumps(out))
 
 
+# ---------------------------------------------------------------------------
+# DG workload: NIPG Poisson on a jittered prism box, p = 1, 2, 3
+# ---------------------------------------------------------------------------
+def dg_mesh(mf, n, seed):
+    """jittered unit-cube prism box, bottom/top faces Neumann (flag 11), sides
+    Dirichlet (flag 1), as the reference's DG cube test"""
+    a = mf.Mesh.box_prism(n, n, n, seed=seed, jitter=0.2).arrays()
+    bc = a["bf_bc"].copy()
+    bc[a["bf_side"] < 2] = 11
+    return mf.Mesh.from_arrays(a["coor"], a["el_type"], a["el_nodes"], a["bf_elem"], a["bf_side"],
+                               bc), a
+
+
+def dg_counts(pdeg):
+    px, pz = pdeg % 100, pdeg // 100
+    ns = (px + 1) * (px + 2) // 2 * (pz + 1)
+    ngt = {1: 3, 2: 6, 3: 12}[px]
+    ngq = (px + 1) * (pz + 1)
+    ng = ngt * (pz + 1)
+    return ns, ng, ngt, ngq
+
+
+def dg_cpu_sample(a, pdeg, sample):
+    """oracle (C, one core): element and face integrals of the first `sample`
+    elements and of their faces"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes
+    from oracle_lib import dg_faces, oracle
+    en = a["el_nodes"][:sample]
+    nodes = np.ascontiguousarray(a["coor"][en].reshape(sample, 18))
+    gids = np.ascontiguousarray(en, np.int32)
+    faces = []
+    for (e1, s1, e2, s2) in dg_faces(en):
+        faces.append((e1, s1, e2, s2) if e2 >= 0 else (e1, s1, -1, -1))
+    faces = np.ascontiguousarray(np.array(faces, np.int32))
+    ind = np.zeros(23, np.int32)
+    ind[[2, 6, 10, 22]] = 1
+    coef = np.zeros(21)
+    coef[0] = coef[4] = coef[8] = 1.0
+    pen = min(pdeg % 100, pdeg // 100) * 1.0
+    P = lambda v: v.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    t0 = time.perf_counter()
+    oracle().orc_dg_assemble_sample(pdeg, sample, P(nodes), P(gids), len(faces), P(faces), P(ind),
+                                    P(coef), pen, 1)
+    return sample / (time.perf_counter() - t0), len(faces)
+
+
+def bench_dg(args):
+    """NIPG DG Poisson (DIFF_IN_CUBE: unit diffusion, manufactured source,
+    Dirichlet sides, Neumann top/bottom, penalty constant 1) on a jittered
+    n^3 prism box for p = 1, 2, 3: device assembly of the block system (own +
+    coupling blocks), block-Jacobi fill and GMRES(30) with block Jacobi"""
+    import modfem_amd as mf
+    mf.init()
+    mesh, a = dg_mesh(mf, args.dg_n, args.seed)
+    nel = mesh.nel
+    res = {}
+    cc = np.r_[np.eye(3).ravel(), np.zeros(12)]
+    for pdeg in args.dg_pdegs:
+        ns, ng, ngt, ngq = dg_counts(pdeg)
+        dg = mf.DgProblem(mesh, pdeg)
+        dg.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc, src_model=mf.SRC_DIFF_IN_CUBE)
+        t0 = time.perf_counter()
+        dg.assemble()
+        t_first = time.perf_counter() - t0
+        for _ in range(args.warmup):
+            dg.assemble()
+        t0 = time.perf_counter()
+        dg.fill_precon()
+        t_prec = time.perf_counter() - t0
+        ms_asm, ms_op = dg.bench(max(args.steps, 1))
+        nb, _, _ = dg.faces()
+        inner = nb >= 0
+        n_in_t = int(inner[:, :2].sum())
+        n_in_q = int(inner[:, 2:].sum())
+        pts_diag = nel * (ng + 2 * ngt + 3 * ngq)
+        pts_couple = n_in_t * ngt + n_in_q * ngq
+        flops = (pts_diag + pts_couple) * (8.0 * ns * ns + 32.0 * ns)
+        blocks = nel * 2 + int(inner.sum())  # own + Dinv + couplings
+        op_bytes = 8.0 * ns * ns * blocks + 8.0 * ns * (2 * nel + int(inner.sum()))
+        t0 = time.perf_counter()
+        full = pdeg in args.dg_solve
+        x, info = dg.gmres(precon=1, krylbas=30, max_iter=args.dg_max_iter if full else 0,
+                           tol=1e-8, fetch=False) if full else (None, {"iters": 0,
+                                                                       "converged": False,
+                                                                       "res_rel": None})
+        solve_s = time.perf_counter() - t0 if full else None
+        rec = {"pdeg": pdeg, "ns": ns, "dofs": nel * ns, "assemble_ms": ms_asm,
+               "elements_per_s": nel / (ms_asm * 1e-3), "assemble_tflops": flops / (ms_asm * 1e-3) / 1e12,
+               "first_assemble_s": t_first, "block_jacobi_ms": t_prec * 1e3,
+               "op_ms": ms_op, "op_GBs": op_bytes / (ms_op * 1e-3) / 1e9,
+               "gmres_iters": info["iters"], "gmres_converged": info["converged"],
+               "gmres_s": solve_s,
+               "gmres_res_rel": info["res_rel"]}
+        t0 = time.perf_counter()
+        x, info2 = dg.gmres(precon=1, krylbas=30, max_iter=min(args.dg_max_iter, 200), tol=1e-30,
+                            fetch=False)
+        rec["gmres_ms_per_iter"] = (time.perf_counter() - t0) * 1e3 / max(info2["iters"], 1)
+        if not args.no_cpu:
+            sample = min(nel, {101: 20000, 202: 4000}.get(pdeg, 800))
+            rate, _ = dg_cpu_sample(a, pdeg, sample)
+            rec["cpu_elements_per_s"] = rate
+        res[pdeg] = rec
+        dg.free()
+        print(json.dumps({"dg_level": rec}), file=sys.stderr, flush=True)
+    # elements with p drawn uniformly from the measured levels
+    rate = len(res) / sum(1.0 / r["elements_per_s"] for r in res.values())
+    top = max(res)
+    r3 = res[top]
+    out = {
+        "metric": METRIC, "value": rate, "unit": "elements integrated+assembled/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sum(r["assemble_ms"] for r in res.values()) / len(res),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered box, manufactured exact solution)",
+        "config": {"workload": "DG NIPG Poisson, %d^3 hex box x 2 prisms (%d prisms, jitter 0.2h), "
+                               "tensor p in %s, each element's p drawn uniformly (rate combined from "
+                               "uniform-p runs)" % (args.dg_n, nel, sorted(res)),
+                   "levels": {str(k): v for k, v in res.items()}},
+        "roofline": {"bound": "mfma", "achieved": r3["assemble_tflops"], "peak": FP64_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": r3["assemble_tflops"] / FP64_PEAK_TFS,
+                     "traffic": None,
+                     "kernel": "k_dg_diag + k_dg_couple, p=%d%d" % (top // 100, top % 100)},
+        "roofline_op": {"bound": "hbm", "achieved": r3["op_GBs"], "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": r3["op_GBs"] / HBM_PEAK_GBS,
+                        "kernel": "k_dg_op, p=%d%d" % (top // 100, top % 100)},
+    }
+    if not args.no_cpu:
+        cr = len(res) / sum(1.0 / r["cpu_elements_per_s"] for r in res.values())
+        out["cpu_baseline"] = {"value": cr, "unit": "elements integrated+assembled/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle (C, one core) element + face integrals of the first "
+                                         "20000/4000/800 elements (p=1/2/3) and their faces"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,9 +409,15 @@ def main():
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: one integrate+assemble, one "
                          "GMRES restart and a 1 GiB stream copy (the counters' yardstick)")
-    ap.add_argument("--workload", choices=["box128", "batch", "serve"], default="box128",
+    ap.add_argument("--dg-n", type=int, default=96, help="DG workload: hexes per box edge")
+    ap.add_argument("--dg-pdegs", type=int, nargs="+", default=[101, 202, 303])
+    ap.add_argument("--dg-max-iter", type=int, default=6000)
+    ap.add_argument("--dg-solve", type=int, nargs="*", default=[101],
+                    help="DG levels solved to 1e-8 (the others time 200 iterations)")
+    ap.add_argument("--workload", choices=["box128", "batch", "serve", "dg"], default="box128",
                     help="box128: the single large problem (default line); batch: many "
-                         "independent problems in one pipeline; serve: a request stream")
+                         "independent problems in one pipeline; serve: a request stream; "
+                         "dg: the DG NIPG Poisson case, p = 1, 2, 3")
     ap.add_argument("--count", type=int, default=256, help="problems (batch) or requests (serve)")
     ap.add_argument("--el-min", type=int, default=2000)
     ap.add_argument("--el-max", type=int, default=200000)
@@ -288,6 +430,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "dg":
+        return bench_dg(args)
     if args.workload == "serve":
         return bench_serve(args)
     import modfem_amd as mf

@@ -295,6 +295,11 @@ int mf_dg_fill_precon(int did);
 int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, double* x,
                 int* iters, double* res_rel, double* resid);
 int mf_dg_get_solution(int did, double* x);
+/* HIP-event times on the library stream, averaged over reps: out[0] ms per
+   assembly of the block system (own blocks + coupling blocks), out[1] ms per
+   block-Jacobi-preconditioned operator application (no reference
+   counterpart; measurement only) */
+int mf_dg_bench(int did, int reps, double* out);
 int mf_dg_free(int did);
 
 /* host copy of a storage-seam matrix (lar_get_SM_and_LV_crs_from_crs with

@@ -137,6 +137,7 @@ def _declare(L):
         "mf_dg_fill_precon": (I, [I]),
         "mf_dg_gmres": (I, [I, I, I, I, D, V, V, V, V]),
         "mf_dg_get_solution": (I, [I, V]),
+        "mf_dg_bench": (I, [I, I, V]),
         # storage seam with the reference's names (lsd_mkb/lah_intf.h)
         "lar_allocate_SM_and_LV": (I, [I, I, I, I, I, I, V, V, V, V]),
         "lar_initialize_SM_and_LV": (I, [I, I]),
@@ -410,6 +411,12 @@ class DgProblem:
         x = np.zeros(self.nel * self.ns)
         check(lib().mf_dg_get_solution(self.h, _p(x)))
         return x
+
+    def bench(self, reps=3):
+        """(ms per assembly, ms per preconditioned operator application)"""
+        out = np.zeros(2)
+        check(lib().mf_dg_bench(self.h, reps, _p(out)))
+        return float(out[0]), float(out[1])
 
     def free(self):
         if self.h:

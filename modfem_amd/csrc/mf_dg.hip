@@ -796,235 +796,183 @@ struct DgRowArgs {
   double* brhs;        // [nel][NS]
 };
 
-// coefficients of element e, full 3x3 a, b (diffusion/convection masks applied)
-__device__ __forceinline__ void row_coefs(const DgElemArgs& A, int e, double* a, double* b,
-                                          double& mc, double* q, double& s) {
-  dg_coefs(A, e, a, b, mc, q[0], q[1], q[2], s);
+// own side of a face point: local coordinates, area element times weight,
+// outward normal, global point, Ji a^T n (the flux a grad(.) . n in
+// reference derivatives) and b . n of element e
+struct FacePt {
+  double eta[3], area, n[3], x[3], gh[3], bn;
+};
+
+__device__ __forceinline__ void face_point(const DgElemArgs& A, const double* xe, int e, int s,
+                                           const double* rule, FacePt& F) {
+  double J[9], Ji[9];
+  face_eta(s, rule, F.eta);
+  prism_jac(xe, F.eta, J, Ji, F.x);
+  F.area = face_ds(s, J, F.n) * rule[2];
+  double a[9], b[3], mc, q0, q1, q2, sv;
+  dg_coefs(A, e, a, b, mc, q0, q1, q2, sv);
+  double An[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) An[c] = a[c] * F.n[0] + a[3 + c] * F.n[1] + a[6 + c] * F.n[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++) F.gh[k] = Ji[3 * k] * An[0] + Ji[3 * k + 1] * An[1] + Ji[3 * k + 2] * An[2];
+  F.bn = b[0] * F.n[0] + b[1] * F.n[1] + b[2] * F.n[2];
 }
 
-// One wave per element builds the element's whole block row: the volume
-// integral and the element's side of each of its five faces go into the
-// slot-0 block, the coupling to the neighbour across side s into slot 1+s;
-// every face term is a 4x4 point matrix between (phi, d/deta phi) of the two
-// functions, so each face point is one K-step of v_mfma_f64_16x16x4 as in
-// the volume integral.  Shape functions are evaluated on the fly from their
-// exponents.
+__device__ __forceinline__ double neighbour_bn(const DgElemArgs& A, int nb, const double* n) {
+  double a[9], b[3], mc, q0, q1, q2, sv;
+  dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
+  return b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+}
+
 template <int PX, int PZ>
-__global__ void __launch_bounds__(256) k_dg_rows(DgRowArgs R) {
+struct DgRowShape {
+  static constexpr int NGT = tri_points(PX), NGQ = (PX + 1) * (PZ + 1);
+  static constexpr int NPTS = DgShape<PX, PZ>::NG + 2 * NGT + 3 * NGQ;  // volume + 5 faces
+  static constexpr int REC = 24;  // eta[3], Mhat[16], fhat[4], pad
+  static constexpr int WAVES = 2;  // k_dg_diag launch bounds: 128 threads
+};
+
+// Own blocks and load vectors.  One wave per element: lanes first write one
+// record per integration point -- the volume points, then the points of the
+// five faces with the element's own terms of each face (its test and trial
+// functions) -- then the wave runs all records through one MFMA loop: every
+// record is a 4x4 matrix between (phi, d/deta phi) of trial and test
+// function, one K-step of v_mfma_f64_16x16x4.  Shape functions are
+// evaluated on the fly from their exponents.
+template <int PX, int PZ>
+__global__ void __launch_bounds__(128) k_dg_diag(DgRowArgs R) {
   using S = DgShape<PX, PZ>;
-  constexpr int NS = S::NS, NG = S::NG, NT = S::NT;
-  constexpr int NGT = tri_points(PX), NGQ = (PX + 1) * (PZ + 1);
-  constexpr int WREC = (NG * 20 > 16 * 42 ? NG * 20 : 16 * 42) + 40;
-  __shared__ double lds[4 * WREC];
+  using Q = DgRowShape<PX, PZ>;
+  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, NPTS = Q::NPTS, REC = Q::REC, W = Q::WAVES;
+  __shared__ double lds[W][NPTS * REC + 18];
   __shared__ int sex[40];
   const DgElemArgs& A = R.E;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double* rec = lds + wave * WREC;  // point records
-  double* sX = rec + WREC - 40;     // own nodes [18]
-  double* sN = sX + 18;             // neighbour nodes [18]
+  double* rec = lds[wave];
+  double* sX = rec + NPTS * REC;
   if (threadIdx.x < 40) sex[threadIdx.x] = R.ex[threadIdx.x];
   const int p = lane >> 4, c16 = lane & 15;
-  for (int base = blockIdx.x * 4; base < A.nel; base += gridDim.x * 4) {
+  for (int base = blockIdx.x * W; base < A.nel; base += gridDim.x * W) {
     const int e = base + wave;
     const bool valid = e < A.nel;
     if (valid && lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
     __syncthreads();
-    double a[9], b[3], mc, q[3], sv;
-    row_coefs(A, valid ? e : 0, a, b, mc, q, sv);
-    // volume point records: Mhat (16) and fhat (4) per point
-    if (valid && lane < NG) {
-      const double* qg = A.qv + 4 * lane;
-      double J[9], Ji[9], x[3];
-      const double vol = fabs(prism_jac(sX, qg, J, Ji, x)) * qg[3];
-      double s = sv;
-      if (A.src_model != SRC_CONST && A.mask.s) {
-        double uu, lapl;
-        exact_sol(A.src_model, x, &uu, &lapl);
-        s = -lapl;
+    for (int k = lane; valid && k < NPTS; k += 64) {
+      double* Rg = rec + REC * k;
+#pragma unroll
+      for (int t = 0; t < 23; t++) Rg[t] = 0.0;
+      if (k < NG) {
+        const double* qg = A.qv + 4 * k;
+        double a[9], b[3], mc, q0, q1, q2, sv, J[9], Ji[9], x[3];
+        dg_coefs(A, e, a, b, mc, q0, q1, q2, sv);
+        const double vol = fabs(prism_jac(sX, qg, J, Ji, x)) * qg[3];
+        if (A.src_model != SRC_CONST && A.mask.s) {
+          double uu, lapl;
+          exact_sol(A.src_model, x, &uu, &lapl);
+          sv = -lapl;
+        }
+        Rg[0] = qg[0]; Rg[1] = qg[1]; Rg[2] = qg[2];
+        double* M = Rg + 3;
+        M[0] = vol * mc;
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) {
+          M[1 + kk] = vol * (b[0] * Ji[3 * kk] + b[1] * Ji[3 * kk + 1] + b[2] * Ji[3 * kk + 2]);
+          double t[3];
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+            t[r] = Ji[3 * kk] * a[3 * r] + Ji[3 * kk + 1] * a[3 * r + 1] + Ji[3 * kk + 2] * a[3 * r + 2];
+#pragma unroll
+          for (int l = 0; l < 3; l++)
+            M[4 * (1 + kk) + 1 + l] = vol * (t[0] * Ji[3 * l] + t[1] * Ji[3 * l + 1] + t[2] * Ji[3 * l + 2]);
+        }
+        Rg[19] = vol * sv;
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++)
+          Rg[20 + kk] = vol * (q0 * Ji[3 * kk] + q1 * Ji[3 * kk + 1] + q2 * Ji[3 * kk + 2]);
+      } else {
+        int f = k - NG, s;
+        const double* rule;
+        if (f < 2 * Q::NGT) {
+          s = f / Q::NGT;
+          rule = R.qt + 3 * (f - s * Q::NGT);
+        } else {
+          f -= 2 * Q::NGT;
+          s = 2 + f / Q::NGQ;
+          rule = R.qq + 3 * (f - (s - 2) * Q::NGQ);
+        }
+        const int fi = 5 * e + s, nb = R.fnb[fi];
+        const double pen = R.penalty / R.fhf[fi];
+        FacePt F;
+        face_point(A, sX, e, s, rule, F);
+        Rg[0] = F.eta[0]; Rg[1] = F.eta[1]; Rg[2] = F.eta[2];
+        double* M = Rg + 3;
+        if (nb >= 0) {
+          // interior: 0.5 (phi_j g_i - g_j phi_i) + (pen + upwind) phi_j phi_i
+          const double nv = 0.5 * (F.bn + neighbour_bn(A, nb, F.n));
+          M[0] = F.area * (pen + (nv > 0 ? F.bn : 0.0));
+#pragma unroll
+          for (int kk = 0; kk < 3; kk++) {
+            M[1 + kk] = 0.5 * F.area * F.gh[kk];
+            M[4 * (1 + kk)] = -0.5 * F.area * F.gh[kk];
+          }
+        } else if (nb == -1) {
+          // Dirichlet (weakform_dg.c:1036-1120)
+          double fd = R.fval[fi];
+          if (A.src_model != SRC_CONST) {
+            double lapl;
+            exact_sol(A.src_model, F.x, &fd, &lapl);
+          }
+          M[0] = F.area * (pen + (F.bn >= 0 ? F.bn : 0.0));
+#pragma unroll
+          for (int kk = 0; kk < 3; kk++) {
+            M[1 + kk] = F.area * F.gh[kk];
+            M[4 * (1 + kk)] = -F.area * F.gh[kk];
+            Rg[20 + kk] = F.area * fd * F.gh[kk];
+          }
+          Rg[19] = F.area * (pen * fd - (F.bn < 0 ? F.bn * fd : 0.0));
+        } else {
+          // Neumann (weakform_dg.c:1148-1185): the exact normal derivative
+          double gd = R.fval[fi];
+          if (A.src_model != SRC_CONST) {
+            const double a1 = A.src_model == SRC_DIFF_IN_BOX ? 100.0 : 1.0;
+            const double* x = F.x;
+            const double ex = exp((-x[0] * x[0] - x[1] * x[1] - x[2] * x[2]) / a1);
+            gd = (-2.0 * x[0] * F.n[0] - 2.0 * x[1] * F.n[1] - 2.0 * x[2] * F.n[2]) * ex / a1;
+          }
+          M[0] = F.area * (F.bn > 0 ? F.bn : 0.0);
+          Rg[19] = F.area * gd;
+        }
       }
-      double* M = rec + 20 * lane;
-      M[0] = vol * mc;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        M[1 + k] = vol * (b[0] * Ji[3 * k] + b[1] * Ji[3 * k + 1] + b[2] * Ji[3 * k + 2]);
-        M[4 * (1 + k)] = 0.0;
-        double t[3];
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-          t[r] = Ji[3 * k] * a[3 * r] + Ji[3 * k + 1] * a[3 * r + 1] + Ji[3 * k + 2] * a[3 * r + 2];
-#pragma unroll
-        for (int l = 0; l < 3; l++)
-          M[4 * (1 + k) + 1 + l] = vol * (t[0] * Ji[3 * l] + t[1] * Ji[3 * l + 1] + t[2] * Ji[3 * l + 2]);
-      }
-      M[16] = vol * s;
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-        M[17 + k] = vol * (q[0] * Ji[3 * k] + q[1] * Ji[3 * k + 1] + q[2] * Ji[3 * k + 2]);
     }
     __syncthreads();
-    dbl4 accD[NT][NT], accB[NT][NT];
-    double f[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) f[t] = 0.0;
-#pragma unroll
-    for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-      for (int it = 0; it < NT; it++) accD[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
     if (valid) {
-      for (int g = 0; g < NG; g++) {
+      dbl4 acc[NT][NT];
+      double f[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++) f[t] = 0.0;
+#pragma unroll
+      for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+        for (int it = 0; it < NT; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int g = 0; g < NPTS; g++) {
+        const double* Rg = rec + REC * g;
         double u[4][NT];
-        mono_eval<NS, NT>(sex, c16, A.qv + 4 * g, u);
-        const double* M = rec + 20 * g;
+        mono_eval<NS, NT>(sex, c16, Rg, u);
+        const double* M = Rg + 3 + 4 * p;
         double av[NT], bv[NT];
 #pragma unroll
         for (int t = 0; t < NT; t++) {
           av[t] = sel4(p, u[0][t], u[1][t], u[2][t], u[3][t]);
-          bv[t] = M[4 * p] * u[0][t] + M[4 * p + 1] * u[1][t] + M[4 * p + 2] * u[2][t] +
-                  M[4 * p + 3] * u[3][t];
-          f[t] += M[16] * u[0][t] + M[17] * u[1][t] + M[18] * u[2][t] + M[19] * u[3][t];
+          bv[t] = M[0] * u[0][t] + M[1] * u[1][t] + M[2] * u[2][t] + M[3] * u[3][t];
+          f[t] += Rg[19] * u[0][t] + Rg[20] * u[1][t] + Rg[21] * u[2][t] + Rg[22] * u[3][t];
         }
 #pragma unroll
         for (int jt = 0; jt < NT; jt++)
 #pragma unroll
           for (int it = 0; it < NT; it++)
-            accD[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], accD[jt][it], 0, 0, 0);
+            acc[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], acc[jt][it], 0, 0, 0);
       }
-    }
-    __syncthreads();
-    // faces
-    for (int s = 0; s < 5; s++) {
-      const int fi = 5 * (valid ? e : 0) + s;
-      const int nb = R.fnb[fi], code = R.fcode[fi];
-      const double hf = R.fhf[fi], bval = R.fval[fi];
-      const bool tri = s < 2, inner = nb >= 0;
-      const int npf = tri ? NGT : NGQ;
-      const double* rule = tri ? R.qt : R.qq;
-      if (valid && inner && lane < 18) sN[lane] = A.nc[(size_t)lane * A.nel + nb];
-      __syncthreads();
-      double an[9], bn_[3], mcn, qn[3], svn;
-      if (inner) row_coefs(A, valid ? nb : 0, an, bn_, mcn, qn, svn);
-      if (valid && lane < npf) {
-        double* Rg = rec + 42 * lane;
-        double eta[3], J[9], Ji[9], x[3], n[3];
-        face_eta(s, rule + 3 * lane, eta);
-        prism_jac(sX, eta, J, Ji, x);
-        const double area = face_ds(s, J, n) * rule[3 * lane + 2];
-        double Ane[3], gh[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) Ane[c] = a[c] * n[0] + a[3 + c] * n[1] + a[6 + c] * n[2];
-#pragma unroll
-        for (int k = 0; k < 3; k++) gh[k] = Ji[3 * k] * Ane[0] + Ji[3 * k + 1] * Ane[1] + Ji[3 * k + 2] * Ane[2];
-        const double bne = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
-        const double pen = R.penalty / hf;
-#pragma unroll
-        for (int k = 0; k < 36; k++) Rg[k] = 0.0;
-        Rg[36] = eta[0]; Rg[37] = eta[1]; Rg[38] = eta[2];
-        if (inner) {
-          double eta2[3], J2[9], Ji2[9], x2[3];
-          neighbour_eta(s, code, eta, eta2);
-          prism_jac(sN, eta2, J2, Ji2, x2);
-          double Ann[3], hh[3];
-#pragma unroll
-          for (int c = 0; c < 3; c++) Ann[c] = an[c] * n[0] + an[3 + c] * n[1] + an[6 + c] * n[2];
-#pragma unroll
-          for (int k = 0; k < 3; k++)
-            hh[k] = Ji2[3 * k] * Ann[0] + Ji2[3 * k + 1] * Ann[1] + Ji2[3 * k + 2] * Ann[2];
-          const double bnn = bn_[0] * n[0] + bn_[1] * n[1] + bn_[2] * n[2];
-          const double nv = 0.5 * (bne + bnn);
-          // own block: 0.5 (phi_j g_i - g_j phi_i) + (pen + upwind) phi_j phi_i
-          Rg[0] = area * (pen + (nv > 0 ? bne : 0.0));
-          // coupling: -0.5 (psi_j g_i + h_j phi_i) + (-pen + upwind) psi_j phi_i
-          Rg[16] = area * (-pen + (nv < 0 ? bne : 0.0));
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            Rg[1 + k] = 0.5 * area * gh[k];
-            Rg[4 * (1 + k)] = -0.5 * area * gh[k];
-            Rg[17 + k] = -0.5 * area * gh[k];
-            Rg[16 + 4 * (1 + k)] = -0.5 * area * hh[k];
-          }
-          Rg[39] = eta2[0]; Rg[40] = eta2[1]; Rg[41] = eta2[2];
-        } else if (nb == -1) {
-          // Dirichlet (weakform_dg.c:1036-1120)
-          double fd = bval;
-          if (A.src_model != SRC_CONST) {
-            double lapl;
-            exact_sol(A.src_model, x, &fd, &lapl);
-          }
-          Rg[0] = area * (pen + (bne >= 0 ? bne : 0.0));
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            Rg[1 + k] = area * gh[k];
-            Rg[4 * (1 + k)] = -area * gh[k];
-            Rg[33 + k] = area * fd * gh[k];
-          }
-          Rg[32] = area * (pen * fd - (bne < 0 ? bne * fd : 0.0));
-        } else {
-          // Neumann (weakform_dg.c:1148-1185); data: the exact normal derivative
-          double gd = bval;
-          if (A.src_model != SRC_CONST) {
-            const double a1 = A.src_model == SRC_DIFF_IN_BOX ? 100.0 : 1.0;
-            const double ex = exp((-x[0] * x[0] - x[1] * x[1] - x[2] * x[2]) / a1);
-            gd = (-2.0 * x[0] * n[0] - 2.0 * x[1] * n[1] - 2.0 * x[2] * n[2]) * ex / a1;
-          }
-          Rg[0] = area * (bne > 0 ? bne : 0.0);
-          Rg[32] = area * gd;
-        }
-      }
-      __syncthreads();
-      if (valid) {
-#pragma unroll
-        for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-          for (int it = 0; it < NT; it++) accB[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
-        for (int g = 0; g < npf; g++) {
-          const double* Rg = rec + 42 * g;
-          double u[4][NT];
-          mono_eval<NS, NT>(sex, c16, Rg + 36, u);
-          double av[NT], bv[NT];
-#pragma unroll
-          for (int t = 0; t < NT; t++) {
-            av[t] = sel4(p, u[0][t], u[1][t], u[2][t], u[3][t]);
-            bv[t] = Rg[4 * p] * u[0][t] + Rg[4 * p + 1] * u[1][t] + Rg[4 * p + 2] * u[2][t] +
-                    Rg[4 * p + 3] * u[3][t];
-            f[t] += Rg[32] * u[0][t] + Rg[33] * u[1][t] + Rg[34] * u[2][t] + Rg[35] * u[3][t];
-          }
-#pragma unroll
-          for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-            for (int it = 0; it < NT; it++)
-              accD[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], accD[jt][it], 0, 0, 0);
-          if (inner) {
-            double w2[4][NT];
-            mono_eval<NS, NT>(sex, c16, Rg + 39, w2);
-#pragma unroll
-            for (int t = 0; t < NT; t++) {
-              av[t] = sel4(p, w2[0][t], w2[1][t], w2[2][t], w2[3][t]);
-              bv[t] = Rg[16 + 4 * p] * u[0][t] + Rg[17 + 4 * p] * u[1][t] +
-                      Rg[18 + 4 * p] * u[2][t] + Rg[19 + 4 * p] * u[3][t];
-            }
-#pragma unroll
-            for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-              for (int it = 0; it < NT; it++)
-                accB[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], accB[jt][it], 0, 0, 0);
-          }
-        }
-        if (inner) {
-          double* out = R.blk + ((size_t)e * 6 + 1 + s) * NS * NS;
-#pragma unroll
-          for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-            for (int it = 0; it < NT; it++)
-#pragma unroll
-              for (int r = 0; r < 4; r++) {
-                const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
-                if (j < NS && i < NS) out[j * NS + i] = accB[jt][it][r];
-              }
-        }
-      }
-      __syncthreads();
-    }
-    if (valid) {
       double* out = R.blk + (size_t)e * 6 * NS * NS;
 #pragma unroll
       for (int jt = 0; jt < NT; jt++)
@@ -1033,7 +981,7 @@ __global__ void __launch_bounds__(256) k_dg_rows(DgRowArgs R) {
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
-            if (j < NS && i < NS) out[j * NS + i] = accD[jt][it][r];
+            if (j < NS && i < NS) out[j * NS + i] = acc[jt][it][r];
           }
       if (p == 0) {
 #pragma unroll
@@ -1041,6 +989,109 @@ __global__ void __launch_bounds__(256) k_dg_rows(DgRowArgs R) {
           if (16 * t + c16 < NS) R.brhs[(size_t)e * NS + 16 * t + c16] = f[t];
       }
     }
+    __syncthreads();
+  }
+}
+
+// Coupling blocks: one wave per interior (element, side): the block of the
+// element's test functions against the neighbour's trial functions,
+// -0.5 (psi_j g_i + h_j phi_i) + (-pen + upwind) psi_j phi_i over the face
+// (the (1,2) block of pdr_conv_diff_comp_fa_stiff_mat_app seen from either
+// side).
+template <int PX, int PZ>
+__global__ void __launch_bounds__(256) k_dg_couple(DgRowArgs R) {
+  using S = DgShape<PX, PZ>;
+  using Q = DgRowShape<PX, PZ>;
+  constexpr int NS = S::NS, NT = S::NT, NGQ = Q::NGQ, REC = Q::REC;
+  __shared__ double lds[4][NGQ * REC + 36];
+  __shared__ int sex[40];
+  const DgElemArgs& A = R.E;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double* rec = lds[wave];
+  double* sX = rec + NGQ * REC;
+  double* sN = sX + 18;
+  if (threadIdx.x < 40) sex[threadIdx.x] = R.ex[threadIdx.x];
+  __syncthreads();
+  const int p = lane >> 4, c16 = lane & 15;
+  const long long nslot = 5LL * A.nel;
+  for (long long slot = (long long)blockIdx.x * 4 + wave; slot < nslot;
+       slot += (long long)gridDim.x * 4) {
+    const int e = (int)(slot / 5), s = (int)(slot % 5);
+    const int nb = R.fnb[slot];
+    if (nb < 0) continue;  // wave-uniform
+    const int code = R.fcode[slot];
+    const int npf = s < 2 ? Q::NGT : NGQ;
+    if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
+    else if (lane < 36) sN[lane - 18] = A.nc[(size_t)(lane - 18) * A.nel + nb];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < npf) {
+      const double* rule = (s < 2 ? R.qt : R.qq) + 3 * lane;
+      FacePt F;
+      face_point(A, sX, e, s, rule, F);
+      double eta2[3], J2[9], Ji2[9], x2[3];
+      neighbour_eta(s, code, F.eta, eta2);
+      prism_jac(sN, eta2, J2, Ji2, x2);
+      double a[9], b[3], mc, q0, q1, q2, sv;
+      dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
+      double An[3], hh[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) An[c] = a[c] * F.n[0] + a[3 + c] * F.n[1] + a[6 + c] * F.n[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) hh[k] = Ji2[3 * k] * An[0] + Ji2[3 * k + 1] * An[1] + Ji2[3 * k + 2] * An[2];
+      const double bnn = b[0] * F.n[0] + b[1] * F.n[1] + b[2] * F.n[2];
+      const double nv = 0.5 * (F.bn + bnn);
+      const double pen = R.penalty / R.fhf[slot];
+      double* Rg = rec + REC * lane;
+      Rg[0] = F.eta[0]; Rg[1] = F.eta[1]; Rg[2] = F.eta[2];
+      Rg[3] = eta2[0]; Rg[4] = eta2[1]; Rg[5] = eta2[2];
+      double* M = Rg + 6;
+#pragma unroll
+      for (int t = 0; t < 16; t++) M[t] = 0.0;
+      M[0] = F.area * (-pen + (nv < 0 ? F.bn : 0.0));
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        M[1 + k] = -0.5 * F.area * F.gh[k];
+        M[4 * (1 + k)] = -0.5 * F.area * hh[k];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    dbl4 acc[NT][NT];
+#pragma unroll
+    for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+      for (int it = 0; it < NT; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int g = 0; g < npf; g++) {
+      const double* Rg = rec + REC * g;
+      double u[4][NT], w[4][NT];
+      mono_eval<NS, NT>(sex, c16, Rg, u);
+      mono_eval<NS, NT>(sex, c16, Rg + 3, w);
+      const double* M = Rg + 6 + 4 * p;
+      double av[NT], bv[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        av[t] = sel4(p, w[0][t], w[1][t], w[2][t], w[3][t]);
+        bv[t] = M[0] * u[0][t] + M[1] * u[1][t] + M[2] * u[2][t] + M[3] * u[3][t];
+      }
+#pragma unroll
+      for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+        for (int it = 0; it < NT; it++)
+          acc[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], acc[jt][it], 0, 0, 0);
+    }
+    double* out = R.blk + ((size_t)e * 6 + 1 + s) * NS * NS;
+#pragma unroll
+    for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+      for (int it = 0; it < NT; it++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
+          if (j < NS && i < NS) out[j * NS + i] = acc[jt][it][r];
+        }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -1237,8 +1288,12 @@ int launch_rows_t(DgProblem& D) {
   R.penalty = std::min(PX, PZ) * D.penalty_const;
   for (int k = 0; k < 40; k++) R.ex[k] = D.ex[k];
   R.blk = D.d_blk; R.brhs = D.d_brhs;
-  const int blocks = std::max(1, std::min((D.nel + 3) / 4, D.ncu * 8));
-  hipLaunchKernelGGL((k_dg_rows<PX, PZ>), dim3(blocks), dim3(256), 0, stream(), R);
+  constexpr int W = DgRowShape<PX, PZ>::WAVES;
+  const int blocks = std::max(1, std::min((D.nel + W - 1) / W, D.ncu * 16));
+  hipLaunchKernelGGL((k_dg_diag<PX, PZ>), dim3(blocks), dim3(64 * W), 0, stream(), R);
+  MF_HIP(hipGetLastError());
+  const int cblocks = (int)std::max(1LL, std::min((5LL * D.nel + 3) / 4, (long long)D.ncu * 16));
+  hipLaunchKernelGGL((k_dg_couple<PX, PZ>), dim3(cblocks), dim3(256), 0, stream(), R);
   MF_HIP(hipGetLastError());
   return 0;
 }
@@ -1617,6 +1672,49 @@ int mf_dg_get_solution(int did, double* x) {
   MF_HIP(hipMemcpyAsync(x, D->P->d_x, (size_t)D->P->n * sizeof(double), hipMemcpyDeviceToHost,
                         stream()));
   MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_dg_bench(int did, int reps, double* out) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->assembled || reps < 1 || !out) {
+    set_error("mf_dg_bench: assemble first; reps >= 1");
+    return -1;
+  }
+  if (!D->dinv_valid && fill_block_jacobi(*D)) return -2;
+  hipStream_t s = stream();
+  hipEvent_t ev[4];
+  for (auto& e : ev) MF_HIP(hipEventCreate(&e));
+  double t_asm = 0, t_op = 0;
+  const size_t n = (size_t)D->nel * D->ns;
+  double *x = nullptr, *w = nullptr;
+  MF_HIP(hipMalloc(&x, n * sizeof(double)));
+  MF_HIP(hipMalloc(&w, n * sizeof(double)));
+  MF_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
+  DgOp op;
+  op.D = D;
+  op.precon = 1;
+  int rc = 0;
+  for (int r = 0; r < reps && rc == 0; r++) {
+    float ms = 0;
+    MF_HIP(hipEventRecord(ev[0], s));
+    rc = launch_rows(*D);
+    MF_HIP(hipEventRecord(ev[1], s));
+    if (rc == 0) rc = op.run(x, nullptr, nullptr, 0, nullptr, w, 0);
+    MF_HIP(hipEventRecord(ev[2], s));
+    MF_HIP(hipEventSynchronize(ev[2]));
+    MF_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    t_asm += ms;
+    MF_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    t_op += ms;
+  }
+  (void)hipFree(x);
+  (void)hipFree(w);
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  out[0] = t_asm / reps;
+  out[1] = t_op / reps;
   return 0;
 }
 

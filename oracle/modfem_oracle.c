@@ -1471,3 +1471,37 @@ int orc_dg_errors(int pdeg, const double *nc, const double *dofs, int src_model,
   }
   return ng;
 }
+
+/* CPU timing driver for the DG path: element integrals of elements
+   0..nel-1 and the face integrals of the listed faces (e1, s1, e2, s2 per
+   face, e2 < 0 for a boundary face with type s2 = -1 Dirichlet / -2
+   Neumann), every coefficient uniform; returns a checksum of the matrices */
+double orc_dg_assemble_sample(int pdeg, int nel, const double *nodes, const int *gids, int nfaces,
+                              const int *faces, const int *ind, const double *coef,
+                              double penalty, int src_model) {
+  double st[80 * 80], rhs[80], sum = 0.0;
+  int ns = 0;
+  for (int e = 0; e < nel; e++) {
+    memset(st, 0, sizeof(st));
+    memset(rhs, 0, sizeof(rhs));
+    ns = orc_dg_num_int_el(pdeg, nodes + 18 * (size_t)e, ind, coef, src_model, st, rhs);
+    sum += st[0] + rhs[0];
+  }
+  for (int f = 0; f < nfaces; f++) {
+    const int *fc = faces + 4 * (size_t)f;
+    const int e1 = fc[0], s1 = fc[1], e2 = fc[2], s2 = fc[3];
+    const double *nc1 = nodes + 18 * (size_t)e1;
+    const double hf = sqrt(orc_dg_face_area(nc1, s1));
+    memset(st, 0, sizeof(st));
+    memset(rhs, 0, sizeof(rhs));
+    if (e2 >= 0)
+      orc_dg_face_int(pdeg, 0, nc1, gids + 6 * (size_t)e1, s1, coef, nodes + 18 * (size_t)e2,
+                      gids + 6 * (size_t)e2, s2, coef, penalty, hf, src_model, 0.0, st, rhs);
+    else
+      orc_dg_face_int(pdeg, s2 == -1 ? 1 : 2, nc1, gids + 6 * (size_t)e1, s1, coef, NULL, NULL, 0,
+                      NULL, penalty, hf, src_model, 0.0, st, rhs);
+    sum += st[0] + rhs[0];
+  }
+  (void)ns;
+  return sum;
+}

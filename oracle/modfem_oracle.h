@@ -168,6 +168,10 @@ int orc_dg_face_int(int pdeg, int bc, const double *nc1, const int *g1, int side
 int orc_dg_errors(int pdeg, const double *node_coor, const double *dofs, int src_model,
                   double *l2sq, double *h1sq);
 
+double orc_dg_assemble_sample(int pdeg, int nel, const double *nodes, const int *gids, int nfaces,
+                              const int *faces, const int *ind, const double *coef,
+                              double penalty, int src_model);
+
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);
 

@@ -58,6 +58,7 @@ def oracle():
             "orc_dg_face_area": (D, [V, I]),
             "orc_dg_face_int": (I, [I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
             "orc_dg_errors": (I, [I, V, V, I, V, V]),
+            "orc_dg_assemble_sample": (D, [I, I, V, V, I, V, V, V, D, I]),
             "orc_set_threads": (None, [I]),
         }
         for k, (r, a) in sig.items():
