@@ -288,6 +288,8 @@ int mf_dg_get_block_row(int did, int e, double* blocks, double* rhs);
 /* block Jacobi: inverses of the own blocks (lar_fill_preconditioner_block,
    las_block_intf.c:1057, which factors them with LAPACK) */
 int mf_dg_fill_precon(int did);
+/* the inverse of element e's own block, column-major */
+int mf_dg_get_precon_block(int did, int e, double* inv);
 /* GMRES (lsr_mkb_core_gmres, lss_mkb_core.c:2317) on the block system with
    precon 0 (none) or 1 (block Jacobi, left), zero initial guess, stopping on
    the preconditioned residual relative to its initial value; x may be null

@@ -135,6 +135,7 @@ def _declare(L):
         "mf_dg_assemble": (I, [I]),
         "mf_dg_get_block_row": (I, [I, I, V, V]),
         "mf_dg_fill_precon": (I, [I]),
+        "mf_dg_get_precon_block": (I, [I, I, V]),
         "mf_dg_gmres": (I, [I, I, I, I, D, V, V, V, V]),
         "mf_dg_get_solution": (I, [I, V]),
         "mf_dg_bench": (I, [I, I, V]),
@@ -398,6 +399,12 @@ class DgProblem:
 
     def fill_precon(self):
         check(lib().mf_dg_fill_precon(self.h))
+
+    def precon_block(self, e):
+        """inverse of element e's own block (inv[j, i]: row i, column j)"""
+        inv = np.zeros((self.ns, self.ns))
+        check(lib().mf_dg_get_precon_block(self.h, int(e), _p(inv)))
+        return inv
 
     def gmres(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, fetch=True):
         x = np.zeros(self.nel * self.ns) if fetch else None

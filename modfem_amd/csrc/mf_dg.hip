@@ -428,6 +428,8 @@ struct DgProblem {
   int* d_fcode = nullptr;     // neighbour side | orientation << 4
   double* d_fhf = nullptr;    // sqrt(face area)
   double* d_fval = nullptr;   // boundary data when no exact solution is selected
+  int* d_slots = nullptr;     // interior (element, side) slots 5 e + s
+  int nslots = 0;
   double* d_qt = nullptr;     // triangle-face rule [ngt][3]
   double* d_qq = nullptr;     // quad-face rule [ngq][3]
   int ngt = 0, ngq = 0;
@@ -496,7 +498,7 @@ void free_dg_solver(DgProblem& D);
 void free_dg(DgProblem& D) {
   free_dg_solver(D);
   void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs, D.d_fnb, D.d_fcode,
-                  D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv};
+                  D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv, D.d_slots};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -668,26 +670,34 @@ __device__ __forceinline__ double pw(double x, int a) {
   return a == 0 ? 1.0 : a == 1 ? x : a == 2 ? x * x : x * x * x;
 }
 
-// shape functions t*16 + c16 (t < NT) and their reference derivatives at eta
+// shape functions t*16 + c16 (t < NT) and their reference derivatives at eta;
+// powers and derivatives are selected from x^0..x^3 without branches
 template <int NS, int NT>
 __device__ __forceinline__ void mono_eval(const int* ex, int c16, const double* eta,
                                           double (&u)[4][NT]) {
+  double P[3][4], dP[3][4];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double x = eta[d], x2 = x * x;
+    P[d][0] = 1.0; P[d][1] = x; P[d][2] = x2; P[d][3] = x2 * x;
+    dP[d][0] = 0.0; dP[d][1] = 1.0; dP[d][2] = 2.0 * x; dP[d][3] = 3.0 * x2;
+  }
 #pragma unroll
   for (int t = 0; t < NT; t++) {
     const int i = 16 * t + c16;
-    if (i < NS) {
-      const int code = ex[i], a = code & 3, b = (code >> 2) & 3, c = (code >> 4) & 3;
-      const double pa = pw(eta[0], a), pb = pw(eta[1], b), pc = pw(eta[2], c);
-      const double da = a ? a * pw(eta[0], a - 1) : 0.0;
-      const double db = b ? b * pw(eta[1], b - 1) : 0.0;
-      const double dc = c ? c * pw(eta[2], c - 1) : 0.0;
-      u[0][t] = pa * pb * pc;
-      u[1][t] = da * pb * pc;
-      u[2][t] = pa * db * pc;
-      u[3][t] = pa * pb * dc;
-    } else {
-      u[0][t] = u[1][t] = u[2][t] = u[3][t] = 0.0;
-    }
+    const int code = (i < NS) ? ex[i < NS ? i : 0] : 0;
+    const int a = code & 3, b = (code >> 2) & 3, c = (code >> 4) & 3;
+    const double pa = sel4(a, P[0][0], P[0][1], P[0][2], P[0][3]);
+    const double pb = sel4(b, P[1][0], P[1][1], P[1][2], P[1][3]);
+    const double pc = sel4(c, P[2][0], P[2][1], P[2][2], P[2][3]);
+    const double da = sel4(a, dP[0][0], dP[0][1], dP[0][2], dP[0][3]);
+    const double db = sel4(b, dP[1][0], dP[1][1], dP[1][2], dP[1][3]);
+    const double dc = sel4(c, dP[2][0], dP[2][1], dP[2][2], dP[2][3]);
+    const double m = (i < NS) ? 1.0 : 0.0;
+    u[0][t] = m * (pa * pb * pc);
+    u[1][t] = m * (da * pb * pc);
+    u[2][t] = m * (pa * db * pc);
+    u[3][t] = m * (pa * pb * dc);
   }
 }
 
@@ -829,39 +839,91 @@ template <int PX, int PZ>
 struct DgRowShape {
   static constexpr int NGT = tri_points(PX), NGQ = (PX + 1) * (PZ + 1);
   static constexpr int NPTS = DgShape<PX, PZ>::NG + 2 * NGT + 3 * NGQ;  // volume + 5 faces
-  static constexpr int REC = 24;  // eta[3], Mhat[16], fhat[4], pad
-  static constexpr int WAVES = 2;  // k_dg_diag launch bounds: 128 threads
 };
 
-// Own blocks and load vectors.  One wave per element: lanes first write one
-// record per integration point -- the volume points, then the points of the
-// five faces with the element's own terms of each face (its test and trial
-// functions) -- then the wave runs all records through one MFMA loop: every
-// record is a 4x4 matrix between (phi, d/deta phi) of trial and test
-// function, one K-step of v_mfma_f64_16x16x4.  Shape functions are
-// evaluated on the fly from their exponents.
+// powers x^0..x^3 of the three local coordinates of a point (the monomial
+// base needs no more), stored with a point's record
+__device__ __forceinline__ void put_powers(const double* eta, double* P) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double x = eta[d], x2 = x * x;
+    P[4 * d] = 1.0; P[4 * d + 1] = x; P[4 * d + 2] = x2; P[4 * d + 3] = x2 * x;
+  }
+}
+
+// per-lane constants for the lane's shape functions i = 16 t + c16: exponent
+// offsets into a power table and the exponents as factors of the derivative
+template <int NS, int NT>
+struct LaneShapes {
+  int oa[NT], ob[NT], oc[NT], da[NT], db[NT], dc[NT];
+  double fa[NT], fb[NT], fc[NT], live[NT];
+  __device__ __forceinline__ void init(const int* ex, int c16) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int i = 16 * t + c16;
+      const int code = i < NS ? ex[i < NS ? i : 0] : 0;
+      const int a = code & 3, b = (code >> 2) & 3, c = (code >> 4) & 3;
+      oa[t] = a; ob[t] = 4 + b; oc[t] = 8 + c;
+      da[t] = a > 0 ? a - 1 : 0; db[t] = 4 + (b > 0 ? b - 1 : 0); dc[t] = 8 + (c > 0 ? c - 1 : 0);
+      fa[t] = a; fb[t] = b; fc[t] = c;
+      live[t] = i < NS ? 1.0 : 0.0;
+    }
+  }
+  // phi and its three reference derivatives from a power table
+  __device__ __forceinline__ void eval(const double* P, double (&u)[4][NT]) const {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const double pa = P[oa[t]], pb = P[ob[t]], pc = P[oc[t]];
+      const double qa = fa[t] * P[da[t]], qb = fb[t] * P[db[t]], qc = fc[t] * P[dc[t]];
+      u[0][t] = live[t] * (pa * pb * pc);
+      u[1][t] = live[t] * (qa * pb * pc);
+      u[2][t] = live[t] * (pa * qb * pc);
+      u[3][t] = live[t] * (pa * pb * qc);
+    }
+  }
+};
+
+// Own blocks and load vectors.  One wave per element.  Points come in
+// chunks of up to 64, one lane each: the volume points, then the points of
+// the five faces with the element's own terms of each face; a lane writes its
+// point's record (powers of the local coordinates, the 4x4 matrix between
+// (phi, d/deta phi) of trial and test function, the load-vector row) to
+// LDS, and the wave runs the chunk through v_mfma_f64_16x16x4, one K-step
+// per point.
 template <int PX, int PZ>
-__global__ void __launch_bounds__(128) k_dg_diag(DgRowArgs R) {
+__global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   using S = DgShape<PX, PZ>;
   using Q = DgRowShape<PX, PZ>;
-  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, NPTS = Q::NPTS, REC = Q::REC, W = Q::WAVES;
-  __shared__ double lds[W][NPTS * REC + 18];
+  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, NPTS = Q::NPTS, REC = 32;
+  __shared__ double rec[64 * REC];
+  __shared__ double sX[18];
   __shared__ int sex[40];
   const DgElemArgs& A = R.E;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double* rec = lds[wave];
-  double* sX = rec + NPTS * REC;
-  if (threadIdx.x < 40) sex[threadIdx.x] = R.ex[threadIdx.x];
+  const int lane = threadIdx.x;
+  const int e = blockIdx.x;
+  if (lane < 40) sex[lane] = R.ex[lane];
+  if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
+  __syncthreads();
   const int p = lane >> 4, c16 = lane & 15;
-  for (int base = blockIdx.x * W; base < A.nel; base += gridDim.x * W) {
-    const int e = base + wave;
-    const bool valid = e < A.nel;
-    if (valid && lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
-    __syncthreads();
-    for (int k = lane; valid && k < NPTS; k += 64) {
-      double* Rg = rec + REC * k;
+  LaneShapes<NS, NT> L;
+  L.init(sex, c16);
+  const double w0 = p == 0 ? 1.0 : 0.0, w1 = p == 1 ? 1.0 : 0.0, w2 = p == 2 ? 1.0 : 0.0,
+               w3 = p == 3 ? 1.0 : 0.0;
+  dbl4 acc[NT][NT];
+  double f[NT];
 #pragma unroll
-      for (int t = 0; t < 23; t++) Rg[t] = 0.0;
+  for (int t = 0; t < NT; t++) f[t] = 0.0;
+#pragma unroll
+  for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+    for (int it = 0; it < NT; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int c0 = 0; c0 < NPTS; c0 += 64) {
+    const int k = c0 + lane;
+    if (k < NPTS) {
+      double* Rg = rec + REC * lane;
+      double* M = Rg + 12;
+#pragma unroll
+      for (int t = 0; t < 20; t++) M[t] = 0.0;
       if (k < NG) {
         const double* qg = A.qv + 4 * k;
         double a[9], b[3], mc, q0, q1, q2, sv, J[9], Ji[9], x[3];
@@ -872,8 +934,7 @@ __global__ void __launch_bounds__(128) k_dg_diag(DgRowArgs R) {
           exact_sol(A.src_model, x, &uu, &lapl);
           sv = -lapl;
         }
-        Rg[0] = qg[0]; Rg[1] = qg[1]; Rg[2] = qg[2];
-        double* M = Rg + 3;
+        put_powers(qg, Rg);
         M[0] = vol * mc;
 #pragma unroll
         for (int kk = 0; kk < 3; kk++) {
@@ -886,27 +947,26 @@ __global__ void __launch_bounds__(128) k_dg_diag(DgRowArgs R) {
           for (int l = 0; l < 3; l++)
             M[4 * (1 + kk) + 1 + l] = vol * (t[0] * Ji[3 * l] + t[1] * Ji[3 * l + 1] + t[2] * Ji[3 * l + 2]);
         }
-        Rg[19] = vol * sv;
+        M[16] = vol * sv;
 #pragma unroll
         for (int kk = 0; kk < 3; kk++)
-          Rg[20 + kk] = vol * (q0 * Ji[3 * kk] + q1 * Ji[3 * kk + 1] + q2 * Ji[3 * kk + 2]);
+          M[17 + kk] = vol * (q0 * Ji[3 * kk] + q1 * Ji[3 * kk + 1] + q2 * Ji[3 * kk + 2]);
       } else {
-        int f = k - NG, s;
+        int fq = k - NG, sd;
         const double* rule;
-        if (f < 2 * Q::NGT) {
-          s = f / Q::NGT;
-          rule = R.qt + 3 * (f - s * Q::NGT);
+        if (fq < 2 * Q::NGT) {
+          sd = fq / Q::NGT;
+          rule = R.qt + 3 * (fq - sd * Q::NGT);
         } else {
-          f -= 2 * Q::NGT;
-          s = 2 + f / Q::NGQ;
-          rule = R.qq + 3 * (f - (s - 2) * Q::NGQ);
+          fq -= 2 * Q::NGT;
+          sd = 2 + fq / Q::NGQ;
+          rule = R.qq + 3 * (fq - (sd - 2) * Q::NGQ);
         }
-        const int fi = 5 * e + s, nb = R.fnb[fi];
+        const int fi = 5 * e + sd, nb = R.fnb[fi];
         const double pen = R.penalty / R.fhf[fi];
         FacePt F;
-        face_point(A, sX, e, s, rule, F);
-        Rg[0] = F.eta[0]; Rg[1] = F.eta[1]; Rg[2] = F.eta[2];
-        double* M = Rg + 3;
+        face_point(A, sX, e, sd, rule, F);
+        put_powers(F.eta, Rg);
         if (nb >= 0) {
           // interior: 0.5 (phi_j g_i - g_j phi_i) + (pen + upwind) phi_j phi_i
           const double nv = 0.5 * (F.bn + neighbour_bn(A, nb, F.n));
@@ -928,9 +988,9 @@ __global__ void __launch_bounds__(128) k_dg_diag(DgRowArgs R) {
           for (int kk = 0; kk < 3; kk++) {
             M[1 + kk] = F.area * F.gh[kk];
             M[4 * (1 + kk)] = -F.area * F.gh[kk];
-            Rg[20 + kk] = F.area * fd * F.gh[kk];
+            M[17 + kk] = F.area * fd * F.gh[kk];
           }
-          Rg[19] = F.area * (pen * fd - (F.bn < 0 ? F.bn * fd : 0.0));
+          M[16] = F.area * (pen * fd - (F.bn < 0 ? F.bn * fd : 0.0));
         } else {
           // Neumann (weakform_dg.c:1148-1185): the exact normal derivative
           double gd = R.fval[fi];
@@ -941,138 +1001,25 @@ __global__ void __launch_bounds__(128) k_dg_diag(DgRowArgs R) {
             gd = (-2.0 * x[0] * F.n[0] - 2.0 * x[1] * F.n[1] - 2.0 * x[2] * F.n[2]) * ex / a1;
           }
           M[0] = F.area * (F.bn > 0 ? F.bn : 0.0);
-          Rg[19] = F.area * gd;
+          M[16] = F.area * gd;
         }
       }
     }
     __syncthreads();
-    if (valid) {
-      dbl4 acc[NT][NT];
-      double f[NT];
-#pragma unroll
-      for (int t = 0; t < NT; t++) f[t] = 0.0;
-#pragma unroll
-      for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-        for (int it = 0; it < NT; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
-      for (int g = 0; g < NPTS; g++) {
-        const double* Rg = rec + REC * g;
-        double u[4][NT];
-        mono_eval<NS, NT>(sex, c16, Rg, u);
-        const double* M = Rg + 3 + 4 * p;
-        double av[NT], bv[NT];
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-          av[t] = sel4(p, u[0][t], u[1][t], u[2][t], u[3][t]);
-          bv[t] = M[0] * u[0][t] + M[1] * u[1][t] + M[2] * u[2][t] + M[3] * u[3][t];
-          f[t] += Rg[19] * u[0][t] + Rg[20] * u[1][t] + Rg[21] * u[2][t] + Rg[22] * u[3][t];
-        }
-#pragma unroll
-        for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-          for (int it = 0; it < NT; it++)
-            acc[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], acc[jt][it], 0, 0, 0);
-      }
-      double* out = R.blk + (size_t)e * 6 * NS * NS;
-#pragma unroll
-      for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-        for (int it = 0; it < NT; it++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
-            if (j < NS && i < NS) out[j * NS + i] = acc[jt][it][r];
-          }
-      if (p == 0) {
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-          if (16 * t + c16 < NS) R.brhs[(size_t)e * NS + 16 * t + c16] = f[t];
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Coupling blocks: one wave per interior (element, side): the block of the
-// element's test functions against the neighbour's trial functions,
-// -0.5 (psi_j g_i + h_j phi_i) + (-pen + upwind) psi_j phi_i over the face
-// (the (1,2) block of pdr_conv_diff_comp_fa_stiff_mat_app seen from either
-// side).
-template <int PX, int PZ>
-__global__ void __launch_bounds__(256) k_dg_couple(DgRowArgs R) {
-  using S = DgShape<PX, PZ>;
-  using Q = DgRowShape<PX, PZ>;
-  constexpr int NS = S::NS, NT = S::NT, NGQ = Q::NGQ, REC = Q::REC;
-  __shared__ double lds[4][NGQ * REC + 36];
-  __shared__ int sex[40];
-  const DgElemArgs& A = R.E;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double* rec = lds[wave];
-  double* sX = rec + NGQ * REC;
-  double* sN = sX + 18;
-  if (threadIdx.x < 40) sex[threadIdx.x] = R.ex[threadIdx.x];
-  __syncthreads();
-  const int p = lane >> 4, c16 = lane & 15;
-  const long long nslot = 5LL * A.nel;
-  for (long long slot = (long long)blockIdx.x * 4 + wave; slot < nslot;
-       slot += (long long)gridDim.x * 4) {
-    const int e = (int)(slot / 5), s = (int)(slot % 5);
-    const int nb = R.fnb[slot];
-    if (nb < 0) continue;  // wave-uniform
-    const int code = R.fcode[slot];
-    const int npf = s < 2 ? Q::NGT : NGQ;
-    if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
-    else if (lane < 36) sN[lane - 18] = A.nc[(size_t)(lane - 18) * A.nel + nb];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane < npf) {
-      const double* rule = (s < 2 ? R.qt : R.qq) + 3 * lane;
-      FacePt F;
-      face_point(A, sX, e, s, rule, F);
-      double eta2[3], J2[9], Ji2[9], x2[3];
-      neighbour_eta(s, code, F.eta, eta2);
-      prism_jac(sN, eta2, J2, Ji2, x2);
-      double a[9], b[3], mc, q0, q1, q2, sv;
-      dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
-      double An[3], hh[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++) An[c] = a[c] * F.n[0] + a[3 + c] * F.n[1] + a[6 + c] * F.n[2];
-#pragma unroll
-      for (int k = 0; k < 3; k++) hh[k] = Ji2[3 * k] * An[0] + Ji2[3 * k + 1] * An[1] + Ji2[3 * k + 2] * An[2];
-      const double bnn = b[0] * F.n[0] + b[1] * F.n[1] + b[2] * F.n[2];
-      const double nv = 0.5 * (F.bn + bnn);
-      const double pen = R.penalty / R.fhf[slot];
-      double* Rg = rec + REC * lane;
-      Rg[0] = F.eta[0]; Rg[1] = F.eta[1]; Rg[2] = F.eta[2];
-      Rg[3] = eta2[0]; Rg[4] = eta2[1]; Rg[5] = eta2[2];
-      double* M = Rg + 6;
-#pragma unroll
-      for (int t = 0; t < 16; t++) M[t] = 0.0;
-      M[0] = F.area * (-pen + (nv < 0 ? F.bn : 0.0));
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        M[1 + k] = -0.5 * F.area * F.gh[k];
-        M[4 * (1 + k)] = -0.5 * F.area * hh[k];
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    dbl4 acc[NT][NT];
-#pragma unroll
-    for (int jt = 0; jt < NT; jt++)
-#pragma unroll
-      for (int it = 0; it < NT; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
-    for (int g = 0; g < npf; g++) {
+    const int nk = NPTS - c0 < 64 ? NPTS - c0 : 64;
+    for (int g = 0; g < nk; g++) {
       const double* Rg = rec + REC * g;
-      double u[4][NT], w[4][NT];
-      mono_eval<NS, NT>(sex, c16, Rg, u);
-      mono_eval<NS, NT>(sex, c16, Rg + 3, w);
-      const double* M = Rg + 6 + 4 * p;
+      double u[4][NT];
+      L.eval(Rg, u);
+      const double* M = Rg + 12 + 4 * p;
+      const double m0 = M[0], m1 = M[1], m2 = M[2], m3 = M[3];
+      const double f0 = Rg[28], f1 = Rg[29], f2 = Rg[30], f3 = Rg[31];
       double av[NT], bv[NT];
 #pragma unroll
       for (int t = 0; t < NT; t++) {
-        av[t] = sel4(p, w[0][t], w[1][t], w[2][t], w[3][t]);
-        bv[t] = M[0] * u[0][t] + M[1] * u[1][t] + M[2] * u[2][t] + M[3] * u[3][t];
+        av[t] = w0 * u[0][t] + w1 * u[1][t] + w2 * u[2][t] + w3 * u[3][t];
+        bv[t] = m0 * u[0][t] + m1 * u[1][t] + m2 * u[2][t] + m3 * u[3][t];
+        f[t] += f0 * u[0][t] + f1 * u[1][t] + f2 * u[2][t] + f3 * u[3][t];
       }
 #pragma unroll
       for (int jt = 0; jt < NT; jt++)
@@ -1080,88 +1027,188 @@ __global__ void __launch_bounds__(256) k_dg_couple(DgRowArgs R) {
         for (int it = 0; it < NT; it++)
           acc[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], acc[jt][it], 0, 0, 0);
     }
-    double* out = R.blk + ((size_t)e * 6 + 1 + s) * NS * NS;
+    __syncthreads();
+  }
+  double* out = R.blk + (size_t)e * 6 * NS * NS;
+#pragma unroll
+  for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+    for (int it = 0; it < NT; it++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
+        if (j < NS && i < NS) out[j * NS + i] = acc[jt][it][r];
+      }
+  if (p == 0) {
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+      if (16 * t + c16 < NS) R.brhs[(size_t)e * NS + 16 * t + c16] = f[t];
+  }
+}
+
+// Coupling blocks: one wave per interior (element, side): the block of the
+// element's test functions against the neighbour's trial functions,
+// -0.5 (psi_j g_i + h_j phi_i) + (-pen + upwind) psi_j phi_i over the face
+// (the (1,2) block of pdr_conv_diff_comp_fa_stiff_mat_app seen from either
+// side).  Records hold the powers of both elements' local coordinates.
+template <int PX, int PZ>
+__global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots) {
+  using S = DgShape<PX, PZ>;
+  using Q = DgRowShape<PX, PZ>;
+  constexpr int NS = S::NS, NT = S::NT, NGQ = Q::NGQ, REC = 40;
+  __shared__ double rec[NGQ * REC];
+  __shared__ double sX[36];
+  __shared__ int sex[40];
+  const DgElemArgs& A = R.E;
+  const int lane = threadIdx.x;
+  const int slot = slots[blockIdx.x];
+  const int e = slot / 5, s = slot % 5;
+  const int nb = R.fnb[slot], code = R.fcode[slot];
+  const int npf = s < 2 ? Q::NGT : NGQ;
+  if (lane < 40) sex[lane] = R.ex[lane];
+  if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
+  else if (lane < 36) sX[lane] = A.nc[(size_t)(lane - 18) * A.nel + nb];
+  __syncthreads();
+  if (lane < npf) {
+    const double* rule = (s < 2 ? R.qt : R.qq) + 3 * lane;
+    FacePt F;
+    face_point(A, sX, e, s, rule, F);
+    double eta2[3], J2[9], Ji2[9], x2[3];
+    neighbour_eta(s, code, F.eta, eta2);
+    prism_jac(sX + 18, eta2, J2, Ji2, x2);
+    double a[9], b[3], mc, q0, q1, q2, sv;
+    dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
+    double An[3], hh[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) An[c] = a[c] * F.n[0] + a[3 + c] * F.n[1] + a[6 + c] * F.n[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) hh[k] = Ji2[3 * k] * An[0] + Ji2[3 * k + 1] * An[1] + Ji2[3 * k + 2] * An[2];
+    const double bnn = b[0] * F.n[0] + b[1] * F.n[1] + b[2] * F.n[2];
+    const double nv = 0.5 * (F.bn + bnn);
+    const double pen = R.penalty / R.fhf[slot];
+    double* Rg = rec + REC * lane;
+    put_powers(F.eta, Rg);
+    put_powers(eta2, Rg + 12);
+    double* M = Rg + 24;
+#pragma unroll
+    for (int t = 0; t < 16; t++) M[t] = 0.0;
+    M[0] = F.area * (-pen + (nv < 0 ? F.bn : 0.0));
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      M[1 + k] = -0.5 * F.area * F.gh[k];
+      M[4 * (1 + k)] = -0.5 * F.area * hh[k];
+    }
+  }
+  __syncthreads();
+  const int p = lane >> 4, c16 = lane & 15;
+  LaneShapes<NS, NT> L;
+  L.init(sex, c16);
+  const double w0 = p == 0 ? 1.0 : 0.0, w1 = p == 1 ? 1.0 : 0.0, w2 = p == 2 ? 1.0 : 0.0,
+               w3 = p == 3 ? 1.0 : 0.0;
+  dbl4 acc[NT][NT];
+#pragma unroll
+  for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+    for (int it = 0; it < NT; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int g = 0; g < npf; g++) {
+    const double* Rg = rec + REC * g;
+    double u[4][NT], w[4][NT];
+    L.eval(Rg, u);
+    L.eval(Rg + 12, w);
+    const double* M = Rg + 24 + 4 * p;
+    const double m0 = M[0], m1 = M[1], m2 = M[2], m3 = M[3];
+    double av[NT], bv[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      av[t] = w0 * w[0][t] + w1 * w[1][t] + w2 * w[2][t] + w3 * w[3][t];
+      bv[t] = m0 * u[0][t] + m1 * u[1][t] + m2 * u[2][t] + m3 * u[3][t];
+    }
 #pragma unroll
     for (int jt = 0; jt < NT; jt++)
 #pragma unroll
       for (int it = 0; it < NT; it++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
-          if (j < NS && i < NS) out[j * NS + i] = acc[jt][it][r];
-        }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        acc[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], acc[jt][it], 0, 0, 0);
   }
+  double* out = R.blk + ((size_t)e * 6 + 1 + s) * NS * NS;
+#pragma unroll
+  for (int jt = 0; jt < NT; jt++)
+#pragma unroll
+    for (int it = 0; it < NT; it++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
+        if (j < NS && i < NS) out[j * NS + i] = acc[jt][it][r];
+      }
 }
 
 // inverse of each element's own block by Gauss-Jordan elimination with
 // partial pivoting (the reference factors these blocks with LAPACK dgetrf for
-// its block Jacobi, lad_block/las_block_intf.c:1057); one wave per element,
-// lane r owns row r of [A | I] in LDS.  A zero pivot sets *bad.
-constexpr int inv_waves(int ns) { return ns >= 30 ? 2 : 4; }
-
+// its block Jacobi, lad_block/las_block_intf.c:1057).  One wave per element;
+// lane r keeps row r of [A | I] in registers.  At step c the unused row with
+// the largest |a_rc| (lowest lane on ties) is the pivot: it is scaled and
+// broadcast through LDS, every other row subtracts its multiple, and the
+// eliminated column is shifted out so that the step's column is always
+// element 0 of the left part.  The pivot row of step c is row c of the
+// inverse.  A zero pivot sets *bad.
 template <int NS>
-__global__ void __launch_bounds__(64 * inv_waves(NS)) k_dg_inv(int nel, const double* blk,
-                                                             double* dinv, int* bad) {
-  constexpr int IW = inv_waves(NS);
-  __shared__ double W[IW][NS][2 * NS + 1];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double (*M)[2 * NS + 1] = W[wave];
-  for (int e = blockIdx.x * IW + wave; e < nel; e += gridDim.x * IW) {
-    const double* Ae = blk + (size_t)e * 6 * NS * NS;
-    if (lane < NS)
-      for (int j = 0; j < NS; j++) {
-        M[lane][j] = Ae[(size_t)j * NS + lane];
-        M[lane][NS + j] = lane == j ? 1.0 : 0.0;
-      }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (int c = 0; c < NS; c++) {
-      // pivot: the largest |M[r][c]| over r >= c (first one on ties)
-      double v = (lane >= c && lane < NS) ? fabs(M[lane][c]) : -1.0;
-      int r = lane;
+__global__ void __launch_bounds__(64) k_dg_inv(int nel, const double* blk, double* dinv, int* bad) {
+  __shared__ double prow[2 * NS];
+  const int lane = threadIdx.x, e = blockIdx.x;
+  const bool row = lane < NS;
+  const double* Ae = blk + (size_t)e * 6 * NS * NS;
+  double L[NS], Rr[NS];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double v2 = __shfl_xor(v, o);
-        const int r2 = __shfl_xor(r, o);
-        if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
-      }
-      if (v == 0.0) {
-        if (lane == 0) bad[0] = 1;
-        break;
-      }
-      if (r != c && lane < 2 * NS) {
-        const double t = M[c][lane];
-        M[c][lane] = M[r][lane];
-        M[r][lane] = t;
-      }
-      if (r != c && lane + 64 < 2 * NS) {
-        const double t = M[c][lane + 64];
-        M[c][lane + 64] = M[r][lane + 64];
-        M[r][lane + 64] = t;
-      }
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      const double piv = M[c][c];
-      double fac = 0.0;
-      if (lane < NS && lane != c) fac = M[lane][c] / piv;
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (lane < NS && lane != c)
-        for (int k = 0; k < 2 * NS; k++) M[lane][k] -= fac * M[c][k];
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      if (lane < NS && lane == c)
-        for (int k = 0; k < 2 * NS; k++) M[c][k] /= piv;
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int j = 0; j < NS; j++) {
+    L[j] = row ? Ae[(size_t)j * NS + lane] : 0.0;
+    Rr[j] = (lane == j) ? 1.0 : 0.0;
+  }
+  bool used = !row;
+  int mycol = -1;
+  for (int c = 0; c < NS; c++) {
+    double v = used ? -1.0 : fabs(L[0]);
+    int r = lane;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v2 = __shfl_xor(v, o);
+      const int r2 = __shfl_xor(r, o);
+      if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
     }
+    if (!(v > 0.0)) {
+      if (lane == 0) bad[0] = 1;
+      return;
+    }
+    if (lane == r) {
+      const double inv = 1.0 / L[0];
+#pragma unroll
+      for (int j = 1; j < NS; j++) prow[j] = L[j] * inv;
+#pragma unroll
+      for (int j = 0; j < NS; j++) prow[NS + j] = Rr[j] * inv;
+      used = true;
+      mycol = c;
+    }
+    __syncthreads();
+    if (lane == r) {
+#pragma unroll
+      for (int j = 1; j < NS; j++) L[j] = prow[j];
+#pragma unroll
+      for (int j = 0; j < NS; j++) Rr[j] = prow[NS + j];
+    } else {
+      const double fac = L[0];
+#pragma unroll
+      for (int j = 1; j < NS; j++) L[j] -= fac * prow[j];
+#pragma unroll
+      for (int j = 0; j < NS; j++) Rr[j] -= fac * prow[NS + j];
+    }
+    // drop the eliminated column
+#pragma unroll
+    for (int j = 0; j + 1 < NS; j++) L[j] = L[j + 1];
+    L[NS - 1] = 0.0;
+    __syncthreads();
+  }
+  if (row) {
     double* Di = dinv + (size_t)e * NS * NS;
-    if (lane < NS)
-      for (int j = 0; j < NS; j++) Di[(size_t)j * NS + lane] = M[lane][NS + j];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+    for (int j = 0; j < NS; j++) Di[(size_t)j * NS + mycol] = Rr[j];
   }
 }
 
@@ -1288,12 +1335,11 @@ int launch_rows_t(DgProblem& D) {
   R.penalty = std::min(PX, PZ) * D.penalty_const;
   for (int k = 0; k < 40; k++) R.ex[k] = D.ex[k];
   R.blk = D.d_blk; R.brhs = D.d_brhs;
-  constexpr int W = DgRowShape<PX, PZ>::WAVES;
-  const int blocks = std::max(1, std::min((D.nel + W - 1) / W, D.ncu * 16));
-  hipLaunchKernelGGL((k_dg_diag<PX, PZ>), dim3(blocks), dim3(64 * W), 0, stream(), R);
+  hipLaunchKernelGGL((k_dg_diag<PX, PZ>), dim3(D.nel), dim3(64), 0, stream(), R);
   MF_HIP(hipGetLastError());
-  const int cblocks = (int)std::max(1LL, std::min((5LL * D.nel + 3) / 4, (long long)D.ncu * 16));
-  hipLaunchKernelGGL((k_dg_couple<PX, PZ>), dim3(cblocks), dim3(256), 0, stream(), R);
+  if (D.nslots > 0)
+    hipLaunchKernelGGL((k_dg_couple<PX, PZ>), dim3(D.nslots), dim3(64), 0, stream(), R,
+                       (const int*)D.d_slots);
   MF_HIP(hipGetLastError());
   return 0;
 }
@@ -1311,10 +1357,8 @@ int launch_rows(DgProblem& D) {
 
 template <int NS>
 int launch_inv_t(DgProblem& D, int* d_bad) {
-  constexpr int IW = inv_waves(NS);
-  const int blocks = std::max(1, std::min((D.nel + IW - 1) / IW, D.ncu * 8));
-  hipLaunchKernelGGL((k_dg_inv<NS>), dim3(blocks), dim3(64 * IW), 0, stream(), D.nel, D.d_blk,
-                     D.d_dinv, d_bad);
+  hipLaunchKernelGGL((k_dg_inv<NS>), dim3(D.nel), dim3(64), 0, stream(), D.nel, D.d_blk, D.d_dinv,
+                     d_bad);
   MF_HIP(hipGetLastError());
   return 0;
 }
@@ -1428,6 +1472,15 @@ int mf_dg_create(int mesh, int pdeg) {
   MF_HIP(hipMemcpyAsync(D->d_fhf, fhf.data(), nf * sizeof(double), hipMemcpyHostToDevice,
                         stream()));
   MF_HIP(hipMemsetAsync(D->d_fval, 0, nf * sizeof(double), stream()));
+  std::vector<int> slots;
+  for (size_t k = 0; k < nf; k++)
+    if (fnb[k] >= 0) slots.push_back((int)k);
+  D->nslots = (int)slots.size();
+  if (D->nslots) {
+    MF_HIP(hipMalloc(&D->d_slots, slots.size() * sizeof(int)));
+    MF_HIP(hipMemcpyAsync(D->d_slots, slots.data(), slots.size() * sizeof(int),
+                          hipMemcpyHostToDevice, stream()));
+  }
   MF_HIP(hipMemcpyAsync(D->d_qt, qt.data(), qt.size() * sizeof(double), hipMemcpyHostToDevice,
                         stream()));
   MF_HIP(hipMemcpyAsync(D->d_qq, qq.data(), qq.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -1633,6 +1686,20 @@ int mf_dg_fill_precon(int did) {
     return -1;
   }
   return fill_block_jacobi(*D);
+}
+
+int mf_dg_get_precon_block(int did, int e, double* inv) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->dinv_valid || e < 0 || e >= D->nel || !inv) {
+    set_error("mf_dg_get_precon_block: no block Jacobi filled or element out of range");
+    return -1;
+  }
+  const size_t nn = (size_t)D->ns * D->ns;
+  MF_HIP(hipMemcpyAsync(inv, D->d_dinv + (size_t)e * nn, nn * sizeof(double), hipMemcpyDeviceToHost,
+                        stream()));
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
 }
 
 int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, double* x,

@@ -196,3 +196,25 @@ def test_gmres_solution_reproduces_reference_error_table(n, pdeg):
     l2, h1 = dg_errors(mesh.arrays(), pdeg, x, 1)
     assert abs(l2 - l2_ref) <= 6e-10 and abs(h1 - h1_ref) <= 6e-10, (l2, h1, l2_ref, h1_ref, info)
     dg.free()
+
+
+@pytest.mark.parametrize("pdeg", [101, 202, 303])
+def test_block_jacobi_inverts_the_own_blocks(pdeg):
+    """lar_fill_preconditioner_block's block inverses: D_e Dinv_e = I"""
+    mesh = cube_mesh(3, jitter=0.2, seed=3)
+    rng = np.random.default_rng(pdeg)
+    ce = random_full(rng, mesh.nel)
+    ind = mf.coeff_indicator("m", "axx", "axy", "axz", "ayx", "ayy", "ayz", "azx", "azy", "azz",
+                             "bx", "by", "bz", "c", "s")
+    dg = mf.DgProblem(mesh, pdeg)
+    dg.set_coefficients(ind, mf.COEF_FULL, coef_el=ce.ravel())
+    dg.assemble()
+    dg.fill_precon()
+    for e in range(0, mesh.nel, 7):
+        D = dg.block_row(e)[0][0].T        # row i, column j
+        Dinv = dg.precon_block(e).T
+        # backward-stable bound: the monomial base makes p = 3 blocks
+        # ill-conditioned, so the residual scales with |D| |Dinv|
+        bound = 20 * dg.ns * np.finfo(float).eps * np.abs(D).sum(1).max() * np.abs(Dinv).sum(1).max()
+        assert np.abs(D @ Dinv - np.eye(dg.ns)).max() <= bound
+    dg.free()
