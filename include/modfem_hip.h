@@ -290,10 +290,17 @@ int mf_dg_get_block_row(int did, int e, double* blocks, double* rhs);
 int mf_dg_fill_precon(int did);
 /* the inverse of element e's own block, column-major */
 int mf_dg_get_precon_block(int did, int e, double* inv);
+/* block Gauss-Seidel smoothing for precon 2 (BLOCK_GS, lar_perform_BJ_or_GS_
+   iterations for block storage, las_block_intf.c:2269-2460): nsweeps sweeps
+   per application, elements updated colour by colour (a greedy colouring of
+   the element graph; the reference sweeps the blocks in their order) */
+int mf_dg_set_gs(int did, int nsweeps, int* ncolors);
 /* GMRES (lsr_mkb_core_gmres, lss_mkb_core.c:2317) on the block system with
-   precon 0 (none) or 1 (block Jacobi, left), zero initial guess, stopping on
-   the preconditioned residual relative to its initial value; x may be null
-   (the solution stays on the device, mf_dg_get_solution) */
+   precon 0 (none), 1 (block Jacobi, left) or 2 (block Gauss-Seidel
+   smoothing, the smoothing branch of compreres, lss_mkb_core.c:2001-2054),
+   zero initial guess, stopping on the preconditioned residual relative to its
+   initial value; x may be null (the solution stays on the device,
+   mf_dg_get_solution) */
 int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, double* x,
                 int* iters, double* res_rel, double* resid);
 int mf_dg_get_solution(int did, double* x);

@@ -136,6 +136,7 @@ def _declare(L):
         "mf_dg_get_block_row": (I, [I, I, V, V]),
         "mf_dg_fill_precon": (I, [I]),
         "mf_dg_get_precon_block": (I, [I, I, V]),
+        "mf_dg_set_gs": (I, [I, I, V]),
         "mf_dg_gmres": (I, [I, I, I, I, D, V, V, V, V]),
         "mf_dg_get_solution": (I, [I, V]),
         "mf_dg_bench": (I, [I, I, V]),
@@ -405,6 +406,12 @@ class DgProblem:
         inv = np.zeros((self.ns, self.ns))
         check(lib().mf_dg_get_precon_block(self.h, int(e), _p(inv)))
         return inv
+
+    def set_gs(self, nsweeps):
+        """block Gauss-Seidel sweeps for precon 2; returns the number of colours"""
+        nc = ctypes.c_int()
+        check(lib().mf_dg_set_gs(self.h, int(nsweeps), ctypes.byref(nc)))
+        return nc.value
 
     def gmres(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, fetch=True):
         x = np.zeros(self.nel * self.ns) if fetch else None

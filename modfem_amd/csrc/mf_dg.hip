@@ -441,6 +441,12 @@ struct DgProblem {
   double* d_brhs = nullptr;
   double* d_dinv = nullptr;   // inverses of the slot-0 blocks
   bool assembled = false, dinv_valid = false;
+  // multicolour block Gauss-Seidel: elements grouped by colour (no two
+  // neighbours share a colour), sweeps per application
+  std::vector<int> gs_color_ptr;
+  int* d_gs_elems = nullptr;
+  int gs_sweeps = 2;
+  double* d_work = nullptr;   // n, GS work vector
   Problem* P = nullptr;       // GMRES workspace over the block system
   DgOp* op = nullptr;
 };
@@ -498,7 +504,8 @@ void free_dg_solver(DgProblem& D);
 void free_dg(DgProblem& D) {
   free_dg_solver(D);
   void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs, D.d_fnb, D.d_fcode,
-                  D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv, D.d_slots};
+                  D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv, D.d_slots,
+                  D.d_gs_elems, D.d_work};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1262,9 +1269,73 @@ __global__ void __launch_bounds__(256) k_dg_op(int nel, const double* blk, const
   }
 }
 
+// one colour of a block Gauss-Seidel sweep (lar_perform_BJ_or_GS_iterations
+// for block storage, las_block_intf.c:2269-2460, in colour order):
+// x_e = Dinv_e (b_e - sum_s B_{e,s} x_nb(s)) for the colour's elements (b
+// null: zero right-hand side)
+template <int NS>
+__global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, const double* blk,
+                                               const int* fnb, const double* dinv,
+                                               const double* b, double* x, const int* stop) {
+  if (stop && *stop) return;
+  constexpr int EPW = 64 / NS;
+  __shared__ double T[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int el = lane / NS, i = lane - el * NS;
+  for (int g0 = (blockIdx.x * 4 + wave) * EPW; g0 < cnt; g0 += gridDim.x * 4 * EPW) {
+    const int gi = g0 + el;
+    const bool act = el < EPW && gi < cnt;
+    const int e = act ? elems[gi] : 0;
+    double acc = 0.0;
+    if (act) {
+      acc = b ? b[(size_t)e * NS + i] : 0.0;
+      const double* Ae = blk + (size_t)e * 6 * NS * NS;
+      for (int sd = 0; sd < 5; sd++) {
+        const int nb = fnb[5 * (size_t)e + sd];
+        if (nb < 0) continue;
+        const double* B = Ae + (size_t)(1 + sd) * NS * NS;
+        const double* xn = x + (size_t)nb * NS;
+        for (int k = 0; k < NS; k++) acc -= B[(size_t)k * NS + i] * xn[k];
+      }
+    }
+    T[wave][lane] = acc;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    double r = 0.0;
+    if (act) {
+      const double* Di = dinv + (size_t)e * NS * NS;
+      const double* t = &T[wave][el * NS];
+      for (int k = 0; k < NS; k++) r += Di[(size_t)k * NS + i] * t[k];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (act) x[(size_t)e * NS + i] = r;
+  }
+}
+
+// w = s_j v and y = w (start of a GS-smoothed operator application)
+__global__ void k_dg_scale2(long long n, const double* v, const double* sc, int j, double* w,
+                            double* y, const int* stop) {
+  if (stop && *stop) return;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double t = sc[j] * v[k];
+  w[k] = t;
+  y[k] = t;
+}
+
+// w -= y
+__global__ void k_dg_sub(long long n, const double* y, double* w, const int* stop) {
+  if (stop && *stop) return;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) w[k] -= y[k];
+}
+
 }  // namespace
 
-// GMRES operator over the DG block system (precon 0: none, 1: block Jacobi)
+// GMRES operator over the DG block system (precon 0: none, 1: block Jacobi,
+// 2: block Gauss-Seidel smoothing with gs_sweeps multicolour sweeps, the
+// smoothing branch of compreres, lss_mkb_core.c:2001-2054)
 struct DgOp : ExtOp {
   DgProblem* D = nullptr;
   int precon = 1;
@@ -1281,18 +1352,54 @@ struct DgOp : ExtOp {
     MF_HIP(hipGetLastError());
     return 0;
   }
+  template <int NS>
+  int sweeps(const double* b, double* y, const int* stop) {
+    const int ncol = (int)D->gs_color_ptr.size() - 1;
+    for (int k = 0; k < D->gs_sweeps; k++)
+      for (int c = 0; c < ncol; c++) {
+        const int c0 = D->gs_color_ptr[c], cnt = D->gs_color_ptr[c + 1] - c0;
+        const int epw = 64 / NS;
+        const int g = std::max(1, std::min((cnt + 4 * epw - 1) / (4 * epw), D->ncu * 16));
+        hipLaunchKernelGGL((k_dg_gs<NS>), dim3(g), dim3(256), 0, stream(), cnt,
+                           D->d_gs_elems + c0, D->d_blk, D->d_fnb, D->d_dinv, b, y, stop);
+      }
+    MF_HIP(hipGetLastError());
+    return 0;
+  }
+  template <int NS>
+  int run_t(const double* x, const double* b, const double* sc, int j, const int* stop, double* w,
+            int mode) {
+    if (precon != 2) return launch<NS>(x, b, sc, j, stop, w, mode);
+    const long long n = (long long)D->nel * NS;
+    const int g = (int)((n + 255) / 256);
+    if (mode == 1) {
+      // r = GS_k(x; b) - x
+      MF_HIP(hipMemcpyAsync(w, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+      if (sweeps<NS>(b, w, stop)) return -2;
+      hipLaunchKernelGGL(k_dg_sub, dim3(g), dim3(256), 0, stream(), n, x, w, stop);
+      MF_HIP(hipGetLastError());
+      return 0;
+    }
+    // w = X - GS_k(X; 0), X = s_j v
+    hipLaunchKernelGGL(k_dg_scale2, dim3(g), dim3(256), 0, stream(), n, x, sc, j, w, D->d_work,
+                       stop);
+    if (sweeps<NS>(nullptr, D->d_work, stop)) return -2;
+    hipLaunchKernelGGL(k_dg_sub, dim3(g), dim3(256), 0, stream(), n, D->d_work, w, stop);
+    MF_HIP(hipGetLastError());
+    return 0;
+  }
   int run(const double* x, const double* b, const double* sc, int j, const int* stop, double* w,
           int mode) {
     switch (D->ns) {
-      case 6: return launch<6>(x, b, sc, j, stop, w, mode);
-      case 9: return launch<9>(x, b, sc, j, stop, w, mode);
-      case 10: return launch<10>(x, b, sc, j, stop, w, mode);
-      case 12: return launch<12>(x, b, sc, j, stop, w, mode);
-      case 18: return launch<18>(x, b, sc, j, stop, w, mode);
-      case 20: return launch<20>(x, b, sc, j, stop, w, mode);
-      case 24: return launch<24>(x, b, sc, j, stop, w, mode);
-      case 30: return launch<30>(x, b, sc, j, stop, w, mode);
-      case 40: return launch<40>(x, b, sc, j, stop, w, mode);
+      case 6: return run_t<6>(x, b, sc, j, stop, w, mode);
+      case 9: return run_t<9>(x, b, sc, j, stop, w, mode);
+      case 10: return run_t<10>(x, b, sc, j, stop, w, mode);
+      case 12: return run_t<12>(x, b, sc, j, stop, w, mode);
+      case 18: return run_t<18>(x, b, sc, j, stop, w, mode);
+      case 20: return run_t<20>(x, b, sc, j, stop, w, mode);
+      case 24: return run_t<24>(x, b, sc, j, stop, w, mode);
+      case 30: return run_t<30>(x, b, sc, j, stop, w, mode);
+      case 40: return run_t<40>(x, b, sc, j, stop, w, mode);
     }
     set_error("DG operator: unsupported block size");
     return -1;
@@ -1360,6 +1467,37 @@ int launch_inv_t(DgProblem& D, int* d_bad) {
   hipLaunchKernelGGL((k_dg_inv<NS>), dim3(D.nel), dim3(64), 0, stream(), D.nel, D.d_blk, D.d_dinv,
                      d_bad);
   MF_HIP(hipGetLastError());
+  return 0;
+}
+
+// greedy colouring of the element graph (elements sharing a face get
+// different colours), elements listed colour by colour
+int build_gs_colors(DgProblem& D) {
+  if (D.d_gs_elems) return 0;
+  const size_t nf = 5 * (size_t)D.nel;
+  std::vector<int> fnb(nf);
+  MF_HIP(hipMemcpy(fnb.data(), D.d_fnb, nf * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> col(D.nel, -1);
+  int ncol = 0;
+  for (int e = 0; e < D.nel; e++) {
+    unsigned used = 0;
+    for (int s = 0; s < 5; s++) {
+      const int nb = fnb[5 * (size_t)e + s];
+      if (nb >= 0 && col[nb] >= 0 && col[nb] < 32) used |= 1u << col[nb];
+    }
+    int c = 0;
+    while (c < 32 && (used >> c & 1u)) c++;
+    col[e] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  D.gs_color_ptr.assign(ncol + 1, 0);
+  for (int e = 0; e < D.nel; e++) D.gs_color_ptr[col[e] + 1]++;
+  for (int c = 0; c < ncol; c++) D.gs_color_ptr[c + 1] += D.gs_color_ptr[c];
+  std::vector<int> pos(D.gs_color_ptr.begin(), D.gs_color_ptr.end() - 1), elems(D.nel);
+  for (int e = 0; e < D.nel; e++) elems[pos[col[e]]++] = e;
+  MF_HIP(hipMalloc(&D.d_gs_elems, (size_t)D.nel * sizeof(int)));
+  MF_HIP(hipMemcpy(D.d_gs_elems, elems.data(), (size_t)D.nel * sizeof(int), hipMemcpyHostToDevice));
+  if (!D.d_work) MF_HIP(hipMalloc(&D.d_work, (size_t)D.nel * D.ns * sizeof(double)));
   return 0;
 }
 
@@ -1710,11 +1848,12 @@ int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, doub
     set_error("mf_dg_gmres: the system is not assembled");
     return -1;
   }
-  if (precon != 0 && precon != 1) {
-    set_error("mf_dg_gmres: precon must be 0 (none) or 1 (block Jacobi)");
+  if (precon < 0 || precon > 2) {
+    set_error("mf_dg_gmres: precon must be 0 (none), 1 (block Jacobi) or 2 (block Gauss-Seidel)");
     return -1;
   }
-  if (precon == 1 && !D->dinv_valid && fill_block_jacobi(*D)) return -2;
+  if (precon >= 1 && !D->dinv_valid && fill_block_jacobi(*D)) return -2;
+  if (precon == 2 && build_gs_colors(*D)) return -2;
   if (!D->P) {
     D->P = new Problem();
     D->P->n = D->nel * D->ns;
@@ -1727,6 +1866,19 @@ int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, doub
   D->P->d_rhs = D->d_brhs;
   if (!D->P->d_x) MF_HIP(hipMalloc(&D->P->d_x, (size_t)D->P->n * sizeof(double)));
   return gmres_solve(*D->P, precon, krylbas, max_iter, tol, 0.0, 1, x, iters, res_rel, resid);
+}
+
+int mf_dg_set_gs(int did, int nsweeps, int* ncolors) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (nsweeps < 1) {
+    set_error("mf_dg_set_gs: nsweeps must be >= 1");
+    return -1;
+  }
+  D->gs_sweeps = nsweeps;
+  if (build_gs_colors(*D)) return -2;
+  if (ncolors) *ncolors = (int)D->gs_color_ptr.size() - 1;
+  return 0;
 }
 
 int mf_dg_get_solution(int did, double* x) {
