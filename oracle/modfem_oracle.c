@@ -1505,3 +1505,167 @@ double orc_dg_assemble_sample(int pdeg, int nel, const double *nodes, const int 
   (void)ns;
   return sum;
 }
+
+/* ---------------------------------------------------------------------
+ * BCRS storage with 4x4 blocks (lsd_mkb/lad_bcrs/las_bcrs_intf.c)
+ * ------------------------------------------------------------------- */
+
+/* pattern (lar_allocate_SM_and_LV_bcrs, las_bcrs_intf.c:109-277): block
+   rows in order, the sorted neighbour blocks with the diagonal inserted in
+   place; neighbour lists flattened as nb_ptr / nb_ind.  Returns the number
+   of stored blocks (col_ind may be NULL to count) */
+int orc_bcrs_pattern(int nb, const int *nb_ptr, const int *nb_ind, int *row_ptr, int *col_ind) {
+  int idx = 0;
+  for (int r = 0; r < nb; r++) {
+    if (row_ptr) row_ptr[r] = idx;
+    const int cnt = nb_ptr[r + 1] - nb_ptr[r];
+    int flag = 0;
+    for (int i = 0; i < cnt; i++) {
+      const int c = nb_ind[nb_ptr[r] + i];
+      if (!flag && c > r) {
+        if (col_ind) col_ind[idx] = r;
+        idx++;
+        flag = 1;
+      }
+      if (col_ind) col_ind[idx] = c;
+      idx++;
+    }
+    if (!flag && cnt > 0) {
+      if (col_ind) col_ind[idx] = r;
+      idx++;
+    }
+  }
+  if (row_ptr) row_ptr[nb] = idx;
+  return idx;
+}
+
+/* assembly of one element (lar_assemble_SM_and_LV_bcrs, :530-623): block
+   (row l_bl[r], col l_bl[c]) += the 4x4 sub-block of the column-major
+   element matrix (4 k rows) at rows 4r.., columns 4c.. */
+void orc_bcrs_assemble(const int *row_ptr, const int *col_ind, int k, const int *l_bl,
+                       const double *stiff, const double *rhs_e, double *val, double *rhs) {
+  const int nrdof = 4 * k;
+  for (int r = 0; r < k; r++) {
+    const int rg = l_bl[r];
+    for (int c = 0; c < k; c++) {
+      const int cg = l_bl[c];
+      int ind = -1;
+      for (int s = row_ptr[rg]; s < row_ptr[rg + 1]; s++)
+        if (col_ind[s] == cg) { ind = s * 16; break; }
+      if (ind < 0) continue;
+      for (int kk = 0; kk < 4; kk++)
+        for (int l = 0; l < 4; l++) val[ind + kk * 4 + l] += stiff[(c * nrdof + r) * 4 + kk * nrdof + l];
+    }
+    for (int l = 0; l < 4; l++) rhs[rg * 4 + l] += rhs_e[r * 4 + l];
+  }
+}
+
+/* V = (use_rhs ? b : 0) - A x (lar_compute_residual_bcrs, :2787-2930):
+   V = 0, then for each stored block in row order and l = 0..3, k = 0..3:
+   V[4r+k] -= a_kl x[4c+l]; then V += b */
+void orc_bcrs_residual(int nb, const int *row_ptr, const int *col_ind, const double *val,
+                       int use_rhs, const double *x, const double *b, double *v) {
+  for (int i = 0; i < 4 * nb; i++) v[i] = 0.0;
+  if (x)
+    for (int r = 0; r < nb; r++)
+      for (int s = row_ptr[r]; s < row_ptr[r + 1]; s++) {
+        const int c = col_ind[s];
+        for (int l = 0; l < 4; l++)
+          for (int k = 0; k < 4; k++) v[4 * r + k] -= val[16 * s + k + 4 * l] * x[4 * c + l];
+      }
+  if (use_rhs)
+    for (int i = 0; i < 4 * nb; i++) v[i] += b[i];
+}
+
+/* LU factorisation of a column-major 4x4 block with partial pivoting, the
+   unblocked right-looking algorithm of LAPACK's dgetf2 (pivot = first
+   largest |a| in the column, rows swapped whole, column scaled by the
+   reciprocal pivot, rank-1 update); ipiv 0-based.  Returns 0, or j+1 for a
+   zero pivot in column j */
+int orc_lu4(double *a, int *ipiv) {
+  int info = 0;
+  for (int j = 0; j < 4; j++) {
+    int jp = j;
+    double amax = fabs(a[j + 4 * j]);
+    for (int i = j + 1; i < 4; i++)
+      if (fabs(a[i + 4 * j]) > amax) { amax = fabs(a[i + 4 * j]); jp = i; }
+    ipiv[j] = jp;
+    if (a[jp + 4 * j] != 0.0) {
+      if (jp != j)
+        for (int c = 0; c < 4; c++) {
+          const double t = a[j + 4 * c]; a[j + 4 * c] = a[jp + 4 * c]; a[jp + 4 * c] = t;
+        }
+      const double r = 1.0 / a[j + 4 * j];
+      for (int i = j + 1; i < 4; i++) a[i + 4 * j] *= r;
+    } else if (!info) {
+      info = j + 1;
+    }
+    for (int c = j + 1; c < 4; c++) {
+      const double t = -a[j + 4 * c];
+      for (int i = j + 1; i < 4; i++) a[i + 4 * c] += a[i + 4 * j] * t;
+    }
+  }
+  return info;
+}
+
+/* solve with the factors of orc_lu4 as LAPACK's dgetrs ('N'): row
+   interchanges, unit-lower forward and upper backward column sweeps
+   (dtrsm, skipping zero entries, dividing by the diagonal) */
+void orc_lu4_solve(const double *lu, const int *ipiv, double *b) {
+  for (int j = 0; j < 4; j++)
+    if (ipiv[j] != j) { const double t = b[j]; b[j] = b[ipiv[j]]; b[ipiv[j]] = t; }
+  for (int k = 0; k < 4; k++)
+    if (b[k] != 0.0)
+      for (int i = k + 1; i < 4; i++) b[i] -= b[k] * lu[i + 4 * k];
+  for (int k = 3; k >= 0; k--)
+    if (b[k] != 0.0) {
+      b[k] /= lu[k + 4 * k];
+      for (int i = 0; i < k; i++) b[i] -= b[k] * lu[i + 4 * k];
+    }
+}
+
+/* block Jacobi / block Gauss-Seidel iterations (lar_perform_BJ_or_GS_
+   iterations_bcrs, :2938-3300) as one thread runs them: Nr_prec even gives
+   Nr_prec/2 loops of a forward and a backward sweep, odd gives Nr_prec/2+1
+   forward sweeps; a sweep forms vloc = -(lower blocks) - (upper blocks)
+   (+ b), solves with the diagonal block's factors and stores into a copy
+   that is written back after the sweep.  Jacobi reads the old V everywhere;
+   Gauss-Seidel reads the copy (already updated rows) for the blocks on the
+   side the sweep has passed.  lu/ipiv: factors of the diagonal blocks */
+void orc_bcrs_bj_gs(int nb, const int *row_ptr, const int *col_ind, const int *diag_ptr,
+                    const double *val, const double *lu, const int *ipiv, int gs, int use_rhs,
+                    int nr_prec, const double *b, double *v) {
+  const int loops = nr_prec % 2 == 0 ? nr_prec / 2 : nr_prec / 2 + 1;
+  double *tmp = (double *)malloc(sizeof(double) * 4 * (size_t)nb);
+  for (int it = 0; it < loops; it++) {
+    for (int dir = 0; dir < (nr_prec % 2 == 0 ? 2 : 1); dir++) {
+      memcpy(tmp, v, sizeof(double) * 4 * (size_t)nb);
+      for (int q = 0; q < nb; q++) {
+        const int r = dir == 0 ? q : nb - 1 - q;
+        double vloc[4] = {0, 0, 0, 0};
+        /* blocks the sweep has passed: lower for forward, upper for backward */
+        const int p0 = dir == 0 ? row_ptr[r] : diag_ptr[r] + 1;
+        const int p1 = dir == 0 ? diag_ptr[r] : row_ptr[r + 1];
+        const int o0 = dir == 0 ? diag_ptr[r] + 1 : row_ptr[r];
+        const int o1 = dir == 0 ? row_ptr[r + 1] : diag_ptr[r];
+        for (int s = p0; s < p1; s++) {
+          const double *xs = gs ? tmp : v;
+          const int c = col_ind[s];
+          for (int l = 0; l < 4; l++)
+            for (int k = 0; k < 4; k++) vloc[k] -= val[16 * s + k + 4 * l] * xs[4 * c + l];
+        }
+        for (int s = o0; s < o1; s++) {
+          const int c = col_ind[s];
+          for (int k = 0; k < 4; k++)
+            for (int l = 0; l < 4; l++) vloc[k] -= val[16 * s + k + 4 * l] * v[4 * c + l];
+        }
+        if (use_rhs)
+          for (int k = 0; k < 4; k++) vloc[k] = 1.0 * b[4 * r + k] + vloc[k];
+        orc_lu4_solve(lu + 16 * (size_t)r, ipiv + 4 * (size_t)r, vloc);
+        for (int k = 0; k < 4; k++) tmp[4 * r + k] = vloc[k];
+      }
+      memcpy(v, tmp, sizeof(double) * 4 * (size_t)nb);
+    }
+  }
+  free(tmp);
+}

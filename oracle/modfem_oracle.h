@@ -172,6 +172,20 @@ double orc_dg_assemble_sample(int pdeg, int nel, const double *nodes, const int 
                               const int *faces, const int *ind, const double *coef,
                               double penalty, int src_model);
 
+/* BCRS storage with 4x4 column-major blocks (lad_bcrs): pattern, element
+   assembly, residual, LAPACK-style 4x4 LU (dgetf2 / dgetrs), block Jacobi /
+   Gauss-Seidel iterations in the one-thread order */
+int orc_bcrs_pattern(int nb, const int *nb_ptr, const int *nb_ind, int *row_ptr, int *col_ind);
+void orc_bcrs_assemble(const int *row_ptr, const int *col_ind, int k, const int *l_bl,
+                       const double *stiff, const double *rhs_e, double *val, double *rhs);
+void orc_bcrs_residual(int nb, const int *row_ptr, const int *col_ind, const double *val,
+                       int use_rhs, const double *x, const double *b, double *v);
+int orc_lu4(double *a, int *ipiv);
+void orc_lu4_solve(const double *lu, const int *ipiv, double *b);
+void orc_bcrs_bj_gs(int nb, const int *row_ptr, const int *col_ind, const int *diag_ptr,
+                    const double *val, const double *lu, const int *ipiv, int gs, int use_rhs,
+                    int nr_prec, const double *b, double *v);
+
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);
 

@@ -59,6 +59,12 @@ def oracle():
             "orc_dg_face_int": (I, [I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
             "orc_dg_errors": (I, [I, V, V, I, V, V]),
             "orc_dg_assemble_sample": (D, [I, I, V, V, I, V, V, V, D, I]),
+            "orc_bcrs_pattern": (I, [I, V, V, V, V]),
+            "orc_bcrs_assemble": (None, [V, V, I, V, V, V, V, V]),
+            "orc_bcrs_residual": (None, [I, V, V, V, I, V, V, V]),
+            "orc_lu4": (I, [V, V]),
+            "orc_lu4_solve": (None, [V, V, V]),
+            "orc_bcrs_bj_gs": (None, [I, V, V, V, V, V, V, I, I, I, V, V]),
             "orc_set_threads": (None, [I]),
         }
         for k, (r, a) in sig.items():

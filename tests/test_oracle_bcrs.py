@@ -1,0 +1,232 @@
+"""The oracle's BCRS storage with 4x4 blocks (lsd_mkb/lad_bcrs), pinned
+against the reference's own las_bcrs_intf.c compiled into oracle/_ref:
+pattern, element assembly (with and without assembly tables) and residual
+bit-exact; the 4x4 LU (the reference calls LAPACK dgetrf/dgetrs, absent
+here) by factor reconstruction and by solving; block Jacobi / Gauss-Seidel by
+convergence to the dense solution and by the reference's sweep-count rule.
+Skipped parts when oracle/_ref was not built."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import modfem_amd as mf  # noqa: E402
+from oracle_lib import P, oracle, ref, ref_available  # noqa: E402
+
+need_bcrs = pytest.mark.skipif(not ref_available("libref_bcrs.so"), reason="oracle/_ref not built")
+
+
+class BcrsStruct(ctypes.Structure):
+    """itt_bcrs_matrices (lsd_mkb/lad_bcrs/lah_bcrs.h)"""
+    _fields_ = [(n, ctypes.c_int) for n in ("Max_SM_size", "Nrblocks", "Nrdofgl", "Nr_sm_blocks",
+                                            "Half_bandwidth", "Max_nrneig", "Precon", "ILU_k",
+                                            "ILU_nr_sm_blocks", "ILU_half_bandwidth")] + \
+              [("rhs", ctypes.POINTER(ctypes.c_double)),
+               ("block_val", ctypes.POINTER(ctypes.c_double)),
+               ("block_col_ind", ctypes.POINTER(ctypes.c_int)),
+               ("block_row_ptr", ctypes.POINTER(ctypes.c_int)),
+               ("block_diag_ptr", ctypes.POINTER(ctypes.c_int)),
+               ("block_diag_precon", ctypes.POINTER(ctypes.c_double)),
+               ("block_diag_ips", ctypes.POINTER(ctypes.c_int)),
+               ("ilu_diag_ptr", ctypes.POINTER(ctypes.c_int)),
+               ("ilu_val", ctypes.POINTER(ctypes.c_double)),
+               ("ilu_col_ind", ctypes.POINTER(ctypes.c_int)),
+               ("ilu_row_ptr", ctypes.POINTER(ctypes.c_int)),
+               ("pivots", ctypes.POINTER(ctypes.c_double))]
+
+
+def vector_problem(nx, ny, nz, seed):
+    """a 4-component problem on a prism mesh: node blocks, neighbour lists
+    (nodes sharing an element), 24x24 element matrices (column-major) and
+    24-vectors"""
+    mesh = mf.Mesh.box_prism(nx, ny, nz, seed=seed, jitter=0.2)
+    a = mesh.arrays()
+    en = a["el_nodes"]
+    nn = a["coor"].shape[0]
+    nbs = [set() for _ in range(nn)]
+    for e in en:
+        for u in e:
+            nbs[u].update(int(v) for v in e if v != u)
+    nb_lists = [sorted(s) for s in nbs]
+    rng = np.random.default_rng(seed)
+    mats = []
+    for _ in range(en.shape[0]):
+        K = rng.uniform(-1, 1, (24, 24))
+        K += np.eye(24) * 6.0
+        mats.append((np.ascontiguousarray(K.T.ravel()), rng.uniform(-1, 1, 24)))
+    return mesh, en, nn, nb_lists, mats
+
+
+def flat(nb_lists):
+    ptr = np.zeros(len(nb_lists) + 1, np.int32)
+    ptr[1:] = np.cumsum([len(x) for x in nb_lists])
+    ind = np.array([v for x in nb_lists for v in x], np.int32)
+    return ptr, ind
+
+
+def oracle_assemble(nn, nb_lists, en, mats):
+    o = oracle()
+    ptr, ind = flat(nb_lists)
+    rp = np.zeros(nn + 1, np.int32)
+    nsm = o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), None)
+    ci = np.zeros(nsm, np.int32)
+    o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), P(ci))
+    val = np.zeros(16 * nsm)
+    rhs = np.zeros(4 * nn)
+    for e, (K, F) in zip(en, mats):
+        l_bl = np.ascontiguousarray(e, np.int32)
+        o.orc_bcrs_assemble(P(rp), P(ci), 6, P(l_bl), P(K), P(np.ascontiguousarray(F)), P(val), P(rhs))
+    return rp, ci, val, rhs
+
+
+def ref_bcrs():
+    L = ref("libref_bcrs.so")
+    L.lar_allocate_SM_and_LV_bcrs.restype = ctypes.c_int
+    L.lar_allocate_SM_and_LV_bcrs.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 2
+    L.lar_assemble_SM_and_LV_bcrs.restype = ctypes.c_int
+    L.lar_assemble_SM_and_LV_bcrs.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 5
+    L.lar_fill_assembly_table_int_ent_bcrs.restype = ctypes.c_int
+    L.lar_fill_assembly_table_int_ent_bcrs.argtypes = [ctypes.c_int] * 2 + [ctypes.c_void_p] * 3
+    L.lar_assemble_SM_and_LV_with_table_bcrs.restype = ctypes.c_int
+    L.lar_assemble_SM_and_LV_with_table_bcrs.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 4
+    L.lar_compute_residual_bcrs.restype = None
+    L.lar_compute_residual_bcrs.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3
+    L.lar_free_SM_and_LV_bcrs.restype = ctypes.c_int
+    L.lar_free_SM_and_LV_bcrs.argtypes = [ctypes.c_int]
+    return L
+
+
+def ref_allocate(L, nn, nb_lists):
+    arrs = [np.ascontiguousarray(x, np.int32) for x in nb_lists]
+    ptrs = (ctypes.c_void_p * nn)(*[P(x) for x in arrs])
+    nro = np.array([len(x) for x in nb_lists], np.int32)
+    mid = L.lar_allocate_SM_and_LV_bcrs(4 * nn, 24, 4, P(nro), ctypes.cast(ptrs, ctypes.c_void_p))
+    return mid, arrs
+
+
+def ref_arrays(L, mid, nn):
+    st = (BcrsStruct * 20).in_dll(L, "itv_bcrs_matrices")[mid]
+    nsm = st.Nr_sm_blocks
+    rp = np.ctypeslib.as_array(st.block_row_ptr, (nn + 1,)).copy()
+    ci = np.ctypeslib.as_array(st.block_col_ind, (nsm,)).copy()
+    val = np.ctypeslib.as_array(st.block_val, (16 * nsm,)).copy()
+    rhs = np.ctypeslib.as_array(st.rhs, (4 * nn,)).copy()
+    return rp, ci, val, rhs
+
+
+@need_bcrs
+def test_pattern_assembly_and_residual_bitexact_with_reference():
+    L = ref_bcrs()
+    mesh, en, nn, nb_lists, mats = vector_problem(3, 3, 2, 5)
+    mid, keep = ref_allocate(L, nn, nb_lists)
+    assert mid >= 0
+    for e, (K, F) in zip(en, mats):
+        l_bl = np.ascontiguousarray(e, np.int32)
+        L.lar_assemble_SM_and_LV_bcrs(mid, 0, 6, P(l_bl), None, P(K), P(np.ascontiguousarray(F)),
+                                      None)
+    rp, ci, val, rhs = ref_arrays(L, mid, nn)
+    orp, oci, oval, orhs = oracle_assemble(nn, nb_lists, en, mats)
+    assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
+    assert np.array_equal(val, oval) and np.array_equal(rhs, orhs)
+    # residual V = b - A x
+    x = np.random.default_rng(1).uniform(-1, 1, 4 * nn)
+    v = np.zeros(4 * nn)
+    L.lar_compute_residual_bcrs(mid, 1, 0, 4 * nn, P(x), None, P(v))
+    vo = np.zeros(4 * nn)
+    oracle().orc_bcrs_residual(nn, P(orp), P(oci), P(oval), 1, P(x), P(orhs), P(vo))
+    assert np.array_equal(v, vo)
+    L.lar_free_SM_and_LV_bcrs(mid)
+
+
+@need_bcrs
+def test_assembly_with_tables_equals_table_less():
+    L = ref_bcrs()
+    mesh, en, nn, nb_lists, mats = vector_problem(2, 3, 2, 7)
+    mid, keep = ref_allocate(L, nn, nb_lists)
+    tab = np.zeros(6 * 6 + 6, np.int32)
+    for e, (K, F) in zip(en, mats):
+        l_bl = np.ascontiguousarray(e, np.int32)
+        L.lar_fill_assembly_table_int_ent_bcrs(mid, 6, P(l_bl), None, P(tab))
+        L.lar_assemble_SM_and_LV_with_table_bcrs(mid, 0, 6, P(tab), P(K), P(np.ascontiguousarray(F)),
+                                                 None)
+    rp, ci, val, rhs = ref_arrays(L, mid, nn)
+    orp, oci, oval, orhs = oracle_assemble(nn, nb_lists, en, mats)
+    assert np.array_equal(val, oval) and np.array_equal(rhs, orhs)
+    L.lar_free_SM_and_LV_bcrs(mid)
+
+
+def test_lu4_factors_and_solves():
+    o = oracle()
+    rng = np.random.default_rng(3)
+    for t in range(200):
+        A = rng.uniform(-1, 1, (4, 4))
+        if t % 3 == 0:
+            A[0, 0] = 0.0  # forces a row interchange
+        a = np.ascontiguousarray(A.T.ravel())  # column-major
+        piv = np.zeros(4, np.int32)
+        assert o.orc_lu4(P(a), P(piv)) == 0
+        LU = a.reshape(4, 4).T
+        Lm = np.tril(LU, -1) + np.eye(4)
+        U = np.triu(LU)
+        PA = A.copy()
+        for j in range(4):  # the interchanges, in order
+            assert piv[j] >= j
+            PA[[j, piv[j]]] = PA[[piv[j], j]]
+        # partial pivoting: every multiplier is at most 1 in magnitude
+        assert np.abs(np.tril(LU, -1)).max() <= 1.0
+        assert np.abs(Lm @ U - PA).max() <= 1e-14 * np.abs(A).max() * 4
+        b = rng.uniform(-1, 1, 4)
+        xb = b.copy()
+        o.orc_lu4_solve(P(a), P(piv), P(xb))
+        assert np.abs(xb - np.linalg.solve(A, b)).max() <= 1e-11 * max(1.0, np.abs(np.linalg.solve(A, b)).max())
+
+
+def bj_gs_setup(seed):
+    mesh, en, nn, nb_lists, mats = vector_problem(3, 2, 2, seed)
+    rp, ci, val, rhs = oracle_assemble(nn, nb_lists, en, mats)
+    # make the diagonal blocks dominant so that both iterations converge
+    diag = np.array([next(s for s in range(rp[r], rp[r + 1]) if ci[s] == r) for r in range(nn)], np.int32)
+    for r in range(nn):
+        val[16 * diag[r]:16 * diag[r] + 16] += (np.eye(4) * 400.0).ravel()
+    lu = np.zeros(16 * nn)
+    piv = np.zeros(4 * nn, np.int32)
+    for r in range(nn):
+        lu[16 * r:16 * r + 16] = val[16 * diag[r]:16 * diag[r] + 16]
+        blk = np.ascontiguousarray(lu[16 * r:16 * r + 16])
+        pv = np.zeros(4, np.int32)
+        oracle().orc_lu4(P(blk), P(pv))
+        lu[16 * r:16 * r + 16] = blk
+        piv[4 * r:4 * r + 4] = pv
+    A = np.zeros((4 * nn, 4 * nn))
+    for r in range(nn):
+        for s in range(rp[r], rp[r + 1]):
+            A[4 * r:4 * r + 4, 4 * ci[s]:4 * ci[s] + 4] = val[16 * s:16 * s + 16].reshape(4, 4).T
+    return nn, rp, ci, diag, val, lu, piv, rhs, A
+
+
+@pytest.mark.parametrize("gs", [0, 1])
+def test_block_jacobi_and_gauss_seidel_converge_to_the_solution(gs):
+    nn, rp, ci, diag, val, lu, piv, rhs, A = bj_gs_setup(11)
+    x = np.zeros(4 * nn)
+    oracle().orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu), P(piv), gs, 1, 200, P(rhs), P(x))
+    xs = np.linalg.solve(A, rhs)
+    assert np.abs(x - xs).max() <= 1e-12 * np.abs(xs).max()
+    # one Jacobi sweep from zero is D^-1 b; Gauss-Seidel's first row too
+    x1 = np.zeros(4 * nn)
+    oracle().orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu), P(piv), gs, 1, 1, P(rhs), P(x1))
+    assert np.abs(x1[:4] - np.linalg.solve(A[:4, :4], rhs[:4])).max() <= 1e-14
+
+
+def test_sweep_count_rule():
+    """Nr_prec odd gives Nr_prec/2 + 1 forward sweeps (las_bcrs_intf.c:2960)"""
+    nn, rp, ci, diag, val, lu, piv, rhs, A = bj_gs_setup(13)
+    o = oracle()
+    x3 = np.zeros(4 * nn)
+    o.orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu), P(piv), 1, 1, 3, P(rhs), P(x3))
+    x11 = np.zeros(4 * nn)
+    for _ in range(2):
+        o.orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu), P(piv), 1, 1, 1, P(rhs), P(x11))
+    assert np.array_equal(x3, x11)
