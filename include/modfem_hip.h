@@ -311,10 +311,8 @@ int mf_dg_get_solution(int did, double* x);
 int mf_dg_bench(int did, int reps, double* out);
 int mf_dg_free(int did);
 
-/* host copy of a storage-seam matrix (lar_get_SM_and_LV_crs_from_crs with
-   offset 0, lsd_mkb/lad_crs/las_crs_intf.c:1063) */
-int mf_lar_get_system(int matrix_id, int* row_ptr, int* col_ind, double* val,
-                      double* rhs);
+/* storage-seam inspection (mf_lar_get_sizes, mf_lar_get_system,
+   mf_lar_get_block_precon) is declared in modfem_lar.h */
 
 #ifdef __cplusplus
 }

@@ -1,11 +1,12 @@
 /*
  * modfem_lar.h -- the storage/operations seam of the reference's MKB solver
  * with its own names and signatures, exported by libmodfem_hip.so for block
- * size 1 (CRS).  Each declaration replaces the reference function of the same
- * name in lsd_mkb/lah_intf.h (line numbers below, relative to
- * modfem2017/src/); the dispatcher lsd_mkb/las_intf.c and the CRS module
- * lsd_mkb/lad_crs/las_crs_intf.c are what a ModFEM build drops when linking
- * this library instead.
+ * size 1 (CRS) and block size 4 (BCRS, column-major 4x4 blocks).  Each
+ * declaration replaces the reference function of the same name in
+ * lsd_mkb/lah_intf.h (line numbers below, relative to modfem2017/src/); the
+ * dispatcher lsd_mkb/las_intf.c and the modules lsd_mkb/lad_crs/las_crs_intf.c
+ * and lsd_mkb/lad_bcrs/las_bcrs_intf.c are what a ModFEM build drops when
+ * linking this library instead.
  *
  * Matrices and vectors live in device memory.  Vector arguments (X, B, V)
  * may be host or device pointers.  Matrix ids are indices into a LIFO table
@@ -19,9 +20,11 @@
 extern "C" {
 #endif
 
-/* lah_intf.h:90 -- CRS pattern from sorted neighbour-block lists; returns
-   the matrix id (<0 on error; only LAC_STORAGE_CRS / _UNDEFINED with
-   Block_size 1 are accepted) */
+/* lah_intf.h:90 -- block pattern from sorted neighbour-block lists; returns
+   the matrix id (<0 on error).  LAC_STORAGE_CRS (2) with Block_size 1,
+   LAC_STORAGE_BCRS (3) with Block_size 4 and Nrdof_glob = 4 Nrblocks, or
+   LAC_STORAGE_UNDEFINED (-1), which picks by Block_size (las_intf.c:90-97);
+   other storage types are refused */
 int lar_allocate_SM_and_LV(int Storage_type, int solver_type, int Nrdof_glob,
                            int Max_SM_size, int Nrblocks, int Block_size,
                            int* Nrdofbl, int* Posglob, int* Nroffbl,
@@ -44,21 +47,40 @@ int lar_assemble_SM_and_LV(int Matrix_id, int Scope, int Nr_dof_bl,
                            double* Rhs_vect, char* Rewr_dofs);
 /* lah_intf.h:184 */
 int lar_free_SM_and_LV(int Matrix_id);
-/* lah_intf.h:192 -- NO_PRECON (0) and BLOCK_JACOBI (1) */
+/* lah_intf.h:192 -- CRS: NO_PRECON (0) and BLOCK_JACOBI (1); BCRS also
+   BLOCK_GS (2).  Returns the number of diagonal blocks */
 int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg);
-/* lah_intf.h:203 */
+/* lah_intf.h:203 -- CRS: 1/a_ii; BCRS: LU with partial pivoting of every
+   diagonal block (LAPACK dgetrf's algorithm); <0 for a singular block */
 int lar_fill_preconditioner(int Matrix_id);
 /* lah_intf.h:210 */
 int lar_free_preconditioner(int Matrix_id);
 /* lah_intf.h:219 -- V = (Use_rhs ? B (or stored rhs) : 0) - A X */
 void lar_compute_residual(int Matrix_id, int Use_rhs, int Ini_zero, int Ndof,
                           double* X, double* B, double* V);
-/* lah_intf.h:252 -- BLOCK_JACOBI: V = D^-1 (B - (A-D) V), Nr_prec sweeps */
+/* lah_intf.h:252 -- CRS BLOCK_JACOBI: V = D^-1 (B - (A-D) V), Nr_prec
+   sweeps.  BCRS (las_bcrs_intf.c:2938-3300): block Jacobi or block
+   Gauss-Seidel in the reference's row order, bitwise its single-thread
+   result; even Nr_prec: Nr_prec/2 forward+backward pairs, odd: Nr_prec/2+1
+   forward sweeps */
 void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero,
                                      int Nr_prec, int Ndof, double* V, double* B);
 /* lah_intf.h:269 -- empty for CRS storage in the reference
-   (las_crs_intf.c:1019-1057); records an error */
+   (las_crs_intf.c:1019-1057); records an error (no ILU is offered) */
 void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B);
+
+/* Inspection (no reference counterpart; lar_get_SM_and_LV_crs_from_crs /
+   _from_bcrs, las_crs_intf.c:1063, las_bcrs_intf.c:3812, are the nearest):
+   storage type, block size, rows of blocks and stored entries/blocks */
+int mf_lar_get_sizes(int Matrix_id, int* storage, int* block_size,
+                     int* nrblocks, long long* nr_stored);
+/* row pointers, column indices, values (BCRS: 16 per block, column-major)
+   and rhs; any pointer may be NULL */
+int mf_lar_get_system(int Matrix_id, int* row_ptr, int* col_ind, double* val,
+                      double* rhs);
+/* BCRS: LU factors (16 per block row, column-major) and 0-based pivots
+   (4 per block row) of the filled diagonal-block preconditioner */
+int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,8 @@ def _declare(L):
         "lar_perform_BJ_or_GS_iterations": (None, [I, I, I, I, I, V, V]),
         "lar_perform_rhsub": (None, [I, I, V, V]),
         "mf_lar_get_system": (I, [I, V, V, V, V]),
+        "mf_lar_get_sizes": (I, [I, V, V, V, V]),
+        "mf_lar_get_block_precon": (I, [I, V, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -281,6 +281,30 @@ int gs_build(Problem& P);
 int gs_refresh(Problem& P);
 int gs_sweeps(Problem& P, int nsweeps, const double* b, double* v, const int* stop);
 void gs_free(Problem& P);
+// BCRS storage with 4x4 column-major blocks (mf_bcrs.hip): block rows in
+// CRS order with the diagonal in place, rhs, and for the diagonal-block
+// preconditioners the LU factors and pivots of every diagonal block and the
+// Gauss-Seidel level schedules of the forward (0) and backward (1) sweeps
+struct BcrsDev {
+  int nb = 0;                     // block rows
+  long long nblk = 0;             // stored blocks
+  int *d_rp = nullptr, *d_ci = nullptr, *d_diag = nullptr;
+  double *d_val = nullptr, *d_rhs = nullptr, *d_lu = nullptr;
+  int* d_piv = nullptr;
+  std::vector<int> lev_ptr[2];
+  int* d_lev_rows[2] = {nullptr, nullptr};
+};
+// level schedule of a sweep in direction dir; returns the number of levels
+int bcrs_levels(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
+                const std::vector<int>& diag, int dir, std::vector<int>& lev_ptr,
+                std::vector<int>& rows);
+// v = (b ? b : 0) - A x (x null: zero)
+int bcrs_residual(const BcrsDev& M, const double* x, const double* b, double* v);
+// LU factors of the diagonal blocks (d_lu, d_piv allocated)
+int bcrs_fill_lu(BcrsDev& M);
+// block Jacobi (gs = 0) or Gauss-Seidel sweeps with the sweep-count rule of
+// the reference; tmp: 4 nb doubles
+int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, double* tmp);
 int bench_copy(long long nbytes, int reps, double* ms);
 int build_tiles(Problem& P);
 int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,

@@ -1,16 +1,17 @@
 // Storage/operations seam with the reference's own names and signatures
-// (lsd_mkb/lah_intf.h:90-289, dispatcher lsd_mkb/las_intf.c), block size 1
-// (LAC_STORAGE_CRS, las_intf.c:95-97).  A ModFEM build can link these in
-// place of las_intf.c + lad_crs: the matrix and vectors live in device memory
-// and every operation runs on the GPU; vectors passed in may be host or
-// device pointers.
+// (lsd_mkb/lah_intf.h:90-289, dispatcher lsd_mkb/las_intf.c): block size 1
+// (LAC_STORAGE_CRS) and 4x4 blocks (LAC_STORAGE_BCRS, chosen as in
+// las_intf.c:90-97).  A ModFEM build can link these in place of las_intf.c +
+// lad_crs + lad_bcrs: the matrix and vectors live in device memory and every
+// operation runs on the GPU; vectors passed in may be host or device pointers.
 //
 // Per-element assembly calls (lar_assemble_SM_and_LV[_with_table]) come one
 // element at a time from the host, so they accumulate into a host mirror of
 // the values in call order -- the same order and rounding as
-// lar_assemble_SM_and_LV_with_table_crs (lad_crs/las_crs_intf.c:440-513) --
-// and the mirror is uploaded before the next device operation.  Device-side
-// batched assembly is mf_integrate_assemble (include/modfem_hip.h).
+// lar_assemble_SM_and_LV_with_table_crs (lad_crs/las_crs_intf.c:440-513) and
+// of the BCRS variants (lad_bcrs/las_bcrs_intf.c:407-623) -- and the mirror
+// is uploaded before the next device operation.  Device-side batched assembly
+// is mf_integrate_assemble (include/modfem_hip.h).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -28,23 +29,30 @@ using namespace mf;
 #define LAC_SCOPE_SM_AND_LV 0
 #define NO_PRECON 0
 #define BLOCK_JACOBI 1
+#define BLOCK_GS 2
 #define LAC_STORAGE_UNDEFINED -1
 #define LAC_STORAGE_CRS 2
+#define LAC_STORAGE_BCRS 3
 #define LAC_MAX_MATRICES 20
+#define BLS 4
 
 namespace {
 
 struct LarMatrix {
-  std::unique_ptr<Problem> P;
+  std::unique_ptr<Problem> P;   // CRS, block size 1
+  std::unique_ptr<BcrsDev> B;   // BCRS, 4x4 blocks
+  std::vector<int> rp, ci, diag;  // BCRS block pattern on the host
   std::vector<double> h_val, h_rhs;  // host mirror for per-element assembly
   bool dirty = false;
+  bool filled = false;  // BCRS: diagonal-block factors current
   int precon = -1;
+  int ndof() const { return B ? BLS * B->nb : P->n; }
 };
 
 std::vector<LarMatrix> g_mats;  // LIFO like itv_crs_matrices
 
 LarMatrix* lar_get(int id) {
-  if (id < 0 || id >= (int)g_mats.size() || !g_mats[id].P) {
+  if (id < 0 || id >= (int)g_mats.size() || !(g_mats[id].P || g_mats[id].B)) {
     set_error("lar_: unknown matrix id " + std::to_string(id));
     return nullptr;
   }
@@ -53,10 +61,11 @@ LarMatrix* lar_get(int id) {
 
 int flush(LarMatrix& M) {
   if (!M.dirty) return 0;
-  Problem& P = *M.P;
-  MF_HIP(hipMemcpyAsync(P.d_val, M.h_val.data(), (size_t)P.nnz * sizeof(double),
+  double* d_val = M.B ? M.B->d_val : M.P->d_val;
+  double* d_rhs = M.B ? M.B->d_rhs : M.P->d_rhs;
+  MF_HIP(hipMemcpyAsync(d_val, M.h_val.data(), M.h_val.size() * sizeof(double),
                         hipMemcpyHostToDevice, stream()));
-  MF_HIP(hipMemcpyAsync(P.d_rhs, M.h_rhs.data(), (size_t)P.n * sizeof(double),
+  MF_HIP(hipMemcpyAsync(d_rhs, M.h_rhs.data(), M.h_rhs.size() * sizeof(double),
                         hipMemcpyHostToDevice, stream()));
   MF_HIP(hipStreamSynchronize(stream()));
   M.dirty = false;
@@ -102,44 +111,23 @@ int stage_out(double* host, const Staged& s, int n) {
   return 0;
 }
 
-}  // namespace
-
-extern "C" {
-
-// lar_allocate_SM_and_LV (lah_intf.h:90; CRS branch las_crs_intf.c:72-294):
-// rows = blocks, columns = sorted off-diagonal blocks with the diagonal
-// inserted in place.  Returns the matrix id.
-int lar_allocate_SM_and_LV(int Storage_type, int solver_type, int Nrdof_glob, int Max_SM_size,
-                           int Nrblocks, int Block_size, int* Nrdofbl, int* Posglob,
-                           int* Nroffbl, int** L_offbl) {
-  (void)solver_type; (void)Max_SM_size; (void)Nrdofbl; (void)Posglob;
-  if (Storage_type != LAC_STORAGE_UNDEFINED && Storage_type != LAC_STORAGE_CRS) {
-    set_error("lar_allocate_SM_and_LV: only CRS storage (block size 1) is available");
-    return -1;
-  }
-  if (Block_size != 1 || Nrblocks != Nrdof_glob) {
-    set_error("lar_allocate_SM_and_LV: block size must be 1");
-    return -1;
-  }
-  // a ModFEM build calls no mf_init: open device MF_DEVICE (default 0) here
-  if (!stream()) {
-    const char* dev = std::getenv("MF_DEVICE");
-    if (device_init(dev ? std::atoi(dev) : 0)) return -2;
-  }
-  if ((int)g_mats.size() >= LAC_MAX_MATRICES) {
-    set_error("lar_allocate_SM_and_LV: too many matrices");
-    return -1;
-  }
-  const int n = Nrdof_glob;
-  std::vector<int> rp(n + 1, 0), ci;
+// block pattern from neighbour lists (las_crs_intf.c:184-250,
+// las_bcrs_intf.c:210-262): rows = blocks, columns = the sorted off-diagonal
+// blocks with the diagonal inserted in place; ghost blocks (Nroffbl <= 0)
+// have no stored row
+int block_pattern(int n, const int* Nroffbl, int* const* L_offbl, std::vector<int>& rp,
+                  std::vector<int>& ci) {
+  rp.assign(n + 1, 0);
+  ci.clear();
   for (int ib = 0; ib < n; ib++) {
     rp[ib] = (int)ci.size();
-    if (Nroffbl[ib] <= 0) continue;  // ghost block: no stored row
+    if (Nroffbl[ib] <= 0) continue;
     bool inserted = false;
     for (int j = 0; j < Nroffbl[ib]; j++) {
       int c = L_offbl[ib][j];
-      if (j > 0 && L_offbl[ib][j - 1] >= c) {
-        set_error("lar_allocate_SM_and_LV: neighbour lists must be strictly ascending");
+      if (c < 0 || c >= n || c == ib || (j > 0 && L_offbl[ib][j - 1] >= c)) {
+        set_error("lar_allocate_SM_and_LV: neighbour lists must be strictly ascending block "
+                  "ids in range, without the block itself");
         return -1;
       }
       if (!inserted && c > ib) {
@@ -151,6 +139,84 @@ int lar_allocate_SM_and_LV(int Storage_type, int solver_type, int Nrdof_glob, in
     if (!inserted) ci.push_back(ib);
   }
   rp[n] = (int)ci.size();
+  return 0;
+}
+
+int allocate_bcrs(int nb, const int* Nroffbl, int* const* L_offbl) {
+  LarMatrix M;
+  if (block_pattern(nb, Nroffbl, L_offbl, M.rp, M.ci)) return -1;
+  M.B = std::make_unique<BcrsDev>();
+  BcrsDev& B = *M.B;
+  B.nb = nb;
+  B.nblk = (long long)M.ci.size();
+  MF_HIP(hipMalloc(&B.d_rp, M.rp.size() * sizeof(int)));
+  MF_HIP(hipMalloc(&B.d_ci, std::max<size_t>(M.ci.size(), 1) * sizeof(int)));
+  MF_HIP(hipMalloc(&B.d_val, std::max<size_t>(M.ci.size(), 1) * BLS * BLS * sizeof(double)));
+  MF_HIP(hipMalloc(&B.d_rhs, std::max(nb, 1) * BLS * sizeof(double)));
+  MF_HIP(hipMemcpy(B.d_rp, M.rp.data(), M.rp.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!M.ci.empty())
+    MF_HIP(hipMemcpy(B.d_ci, M.ci.data(), M.ci.size() * sizeof(int), hipMemcpyHostToDevice));
+  M.h_val.assign(M.ci.size() * BLS * BLS, 0.0);
+  M.h_rhs.assign((size_t)nb * BLS, 0.0);
+  M.dirty = true;
+  g_mats.push_back(std::move(M));
+  if (flush(g_mats.back())) return -2;
+  return (int)g_mats.size() - 1;
+}
+
+void free_bcrs_precon(BcrsDev& B) {
+  void* ptrs[] = {B.d_diag, B.d_lu, B.d_piv, B.d_lev_rows[0], B.d_lev_rows[1]};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  B.d_diag = nullptr;
+  B.d_lu = nullptr;
+  B.d_piv = nullptr;
+  for (int d = 0; d < 2; d++) {
+    B.d_lev_rows[d] = nullptr;
+    B.lev_ptr[d].clear();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// lar_allocate_SM_and_LV (lah_intf.h:90; storage choice las_intf.c:90-97;
+// CRS branch las_crs_intf.c:72-294, BCRS branch las_bcrs_intf.c:109-277).
+// Returns the matrix id.
+int lar_allocate_SM_and_LV(int Storage_type, int solver_type, int Nrdof_glob, int Max_SM_size,
+                           int Nrblocks, int Block_size, int* Nrdofbl, int* Posglob,
+                           int* Nroffbl, int** L_offbl) {
+  (void)solver_type; (void)Max_SM_size; (void)Nrdofbl; (void)Posglob;
+  int storage = Storage_type;
+  if (storage == LAC_STORAGE_UNDEFINED)
+    storage = Block_size == 1 ? LAC_STORAGE_CRS : Block_size > 1 ? LAC_STORAGE_BCRS : -1;
+  if (storage != LAC_STORAGE_CRS && storage != LAC_STORAGE_BCRS) {
+    set_error("lar_allocate_SM_and_LV: CRS (block size 1) and BCRS (block size 4) storage are "
+              "available");
+    return -1;
+  }
+  if (storage == LAC_STORAGE_CRS && (Block_size != 1 || Nrblocks != Nrdof_glob)) {
+    set_error("lar_allocate_SM_and_LV: CRS storage needs block size 1");
+    return -1;
+  }
+  if (storage == LAC_STORAGE_BCRS && (Block_size != BLS || Nrdof_glob != Nrblocks * BLS)) {
+    set_error("lar_allocate_SM_and_LV: BCRS storage needs block size 4 and Nrdof_glob = 4 Nrblocks");
+    return -1;
+  }
+  // a ModFEM build calls no mf_init: open device MF_DEVICE (default 0) here
+  if (!stream()) {
+    const char* dev = std::getenv("MF_DEVICE");
+    if (device_init(dev ? std::atoi(dev) : 0)) return -2;
+  }
+  if ((int)g_mats.size() >= LAC_MAX_MATRICES) {
+    set_error("lar_allocate_SM_and_LV: too many matrices");
+    return -1;
+  }
+  if (storage == LAC_STORAGE_BCRS) return allocate_bcrs(Nrblocks, Nroffbl, L_offbl);
+  const int n = Nrdof_glob;
+  std::vector<int> rp, ci;
+  if (block_pattern(n, Nroffbl, L_offbl, rp, ci)) return -1;
   LarMatrix M;
   M.P = std::make_unique<Problem>();
   Problem& P = *M.P;
@@ -188,9 +254,33 @@ int lar_initialize_SM_and_LV(int Matrix_id, int Scope) {
 double lar_get_storage(int Matrix_id) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -1;
+  if (M->B)  // values, column indices, row pointers, rhs (the reference returns -1)
+    return ((double)M->B->nblk * (BLS * BLS * 8 + 4) + (double)M->B->nb * (BLS * 8 + 4)) /
+           1024.0 / 1024.0;
   const Problem& P = *M->P;
   return ((double)P.nnz * 12 + (double)P.n * 20) / 1024.0 / 1024.0;
 }
+
+namespace {
+
+// position of block (row, col) in the BCRS pattern, -1 when absent
+int bcrs_find(const LarMatrix& M, int row, int col) {
+  if (row < 0 || row >= M.B->nb) return -1;
+  for (int s = M.rp[row]; s < M.rp[row + 1]; s++)
+    if (M.ci[s] == col) return s;
+  return -1;
+}
+
+bool blocks_in_range(const LarMatrix& M, int k, const int* L_bl_id) {
+  for (int i = 0; i < k; i++)
+    if (L_bl_id[i] < 0 || L_bl_id[i] >= M.B->nb) {
+      set_error("lar_: block id out of range");
+      return false;
+    }
+  return true;
+}
+
+}  // namespace
 
 // lar_fill_assembly_table_int_ent (las_crs_intf.c:354-432)
 int lar_fill_assembly_table_int_ent(int Matrix_id, int Nr_dof_bl, int* L_bl_id, int* L_bl_nrdof,
@@ -198,6 +288,22 @@ int lar_fill_assembly_table_int_ent(int Matrix_id, int Nr_dof_bl, int* L_bl_id, 
   (void)L_bl_nrdof;
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
+  if (M->B) {  // las_bcrs_intf.c:325-399: entry jb + ib k = 16 * (position of
+               // block (L_bl_id[ib], L_bl_id[jb])), then 4 * L_bl_id[jb]
+    const int k = Nr_dof_bl;
+    if (!blocks_in_range(*M, k, L_bl_id)) return -1;
+    for (int ib = 0; ib < k; ib++)
+      for (int jb = 0; jb < k; jb++) {
+        const int s = bcrs_find(*M, L_bl_id[ib], L_bl_id[jb]);
+        if (s < 0) {
+          set_error("lar_fill_assembly_table_int_ent: block pair not in the pattern");
+          return -1;
+        }
+        Assembly_table_int_ent[jb + ib * k] = s * BLS * BLS;
+      }
+    for (int jb = 0; jb < k; jb++) Assembly_table_int_ent[k * k + jb] = L_bl_id[jb] * BLS;
+    return 1;
+  }
   const Setup& S = M->P->setup;
   for (int i = 0; i < Nr_dof_bl; i++) {
     int col = L_bl_id[i];
@@ -228,6 +334,32 @@ int lar_assemble_SM_and_LV_with_table(int Matrix_id, int Scope, int Nr_dof_bl,
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
   const int k = Nr_dof_bl;
+  if (M->B) {  // las_bcrs_intf.c:407-523: block (ib, jb) of the column-major
+               // 4k x 4k element matrix added to the stored block
+    const int ld = k * BLS;
+    const long long nval = (long long)M->h_val.size(), nrhs = (long long)M->h_rhs.size();
+    for (int ib = 0; ib < k; ib++)
+      for (int jb = 0; jb < k; jb++) {
+        const int pos = Assembly_table_int_ent[jb + ib * k];
+        if (pos < 0 || pos + BLS * BLS > nval) {
+          set_error("lar_assemble_SM_and_LV_with_table: table entry out of range");
+          return -1;
+        }
+        const double* src = Stiff_mat + (jb * ld + ib) * BLS;
+        for (int c = 0; c < BLS; c++)
+          for (int r = 0; r < BLS; r++) M->h_val[pos + c * BLS + r] += src[c * ld + r];
+      }
+    for (int ib = 0; ib < k; ib++) {
+      const int pos = Assembly_table_int_ent[k * k + ib];
+      if (pos < 0 || pos + BLS > nrhs) {
+        set_error("lar_assemble_SM_and_LV_with_table: table entry out of range");
+        return -1;
+      }
+      for (int r = 0; r < BLS; r++) M->h_rhs[pos + r] += Rhs_vect[ib * BLS + r];
+    }
+    M->dirty = true;
+    return 1;
+  }
   for (int i = 0; i < k; i++)
     for (int j = 0; j < k; j++) M->h_val[Assembly_table_int_ent[j + i * k]] += Stiff_mat[j + i * k];
   for (int i = 0; i < k; i++) M->h_rhs[Assembly_table_int_ent[k * k + i]] += Rhs_vect[i];
@@ -243,8 +375,26 @@ int lar_assemble_SM_and_LV(int Matrix_id, int Scope, int Nr_dof_bl, int* L_bl_id
   (void)Scope; (void)Rewr_dofs; (void)L_bl_nrdof;
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
-  const Setup& S = M->P->setup;
   const int k = Nr_dof_bl;
+  if (M->B) {  // las_bcrs_intf.c:530-623: blocks missing from the pattern
+               // (ghost rows) are skipped, the rhs is added for every block
+    if (!blocks_in_range(*M, k, L_bl_id)) return -1;
+    const int ld = k * BLS;
+    for (int ib = 0; ib < k; ib++) {
+      for (int jb = 0; jb < k; jb++) {
+        const int s = bcrs_find(*M, L_bl_id[ib], L_bl_id[jb]);
+        if (s < 0) continue;
+        const double* src = Stiff_mat + (jb * ld + ib) * BLS;
+        double* dst = M->h_val.data() + (size_t)s * BLS * BLS;
+        for (int c = 0; c < BLS; c++)
+          for (int r = 0; r < BLS; r++) dst[c * BLS + r] += src[c * ld + r];
+      }
+      for (int r = 0; r < BLS; r++) M->h_rhs[L_bl_id[ib] * BLS + r] += Rhs_vect[ib * BLS + r];
+    }
+    M->dirty = true;
+    return 1;
+  }
+  const Setup& S = M->P->setup;
   for (int i = 0; i < k; i++) {
     int col = L_bl_id[i];
     for (int j = 0; j < k; j++) {
@@ -268,6 +418,15 @@ int lar_free_SM_and_LV(int Matrix_id) {
   }
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
+  if (M->B) {
+    BcrsDev& B = *M->B;
+    free_bcrs_precon(B);
+    void* ptrs[] = {B.d_rp, B.d_ci, B.d_val, B.d_rhs};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    g_mats.pop_back();
+    return 0;
+  }
   Problem& P = *M->P;
   void* ptrs[] = {P.d_row_ptr, P.d_col, P.d_val, P.d_rhs, P.d_dinv, P.d_diag_ptr, P.d_tile_row};
   for (void* p : ptrs)
@@ -276,11 +435,48 @@ int lar_free_SM_and_LV(int Matrix_id) {
   return 0;
 }
 
-// lar_allocate_preconditioner (las_crs_intf.c:658-700): Jacobi or none
+// lar_allocate_preconditioner (las_crs_intf.c:658-700): Jacobi or none;
+// BCRS (las_bcrs_intf.c:630-711): block Jacobi or block Gauss-Seidel with the
+// diagonal blocks' LU factors, the Gauss-Seidel level schedules built here
 int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
   (void)Second_arg;
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
+  if (M->B) {
+    if (Precon != NO_PRECON && Precon != BLOCK_JACOBI && Precon != BLOCK_GS) {
+      set_error("lar_allocate_preconditioner: BCRS storage offers NO_PRECON, BLOCK_JACOBI and "
+                "BLOCK_GS");
+      return -1;
+    }
+    BcrsDev& B = *M->B;
+    free_bcrs_precon(B);
+    M->filled = false;
+    M->precon = Precon;
+    if (Precon == NO_PRECON) return B.nb;
+    M->diag.assign(B.nb, -1);
+    for (int r = 0; r < B.nb; r++) {
+      for (int s = M->rp[r]; s < M->rp[r + 1]; s++)
+        if (M->ci[s] == r) M->diag[r] = s;
+      if (M->diag[r] < 0) {
+        set_error("lar_allocate_preconditioner: block row " + std::to_string(r) +
+                  " has no stored diagonal block");
+        return -1;
+      }
+    }
+    MF_HIP(hipMalloc(&B.d_diag, std::max(B.nb, 1) * sizeof(int)));
+    MF_HIP(hipMalloc(&B.d_lu, std::max(B.nb, 1) * BLS * BLS * sizeof(double)));
+    MF_HIP(hipMalloc(&B.d_piv, std::max(B.nb, 1) * BLS * sizeof(int)));
+    MF_HIP(hipMemcpy(B.d_diag, M->diag.data(), B.nb * sizeof(int), hipMemcpyHostToDevice));
+    if (Precon == BLOCK_GS)
+      for (int dir = 0; dir < 2; dir++) {
+        std::vector<int> rows;
+        bcrs_levels(B.nb, M->rp, M->ci, M->diag, dir, B.lev_ptr[dir], rows);
+        MF_HIP(hipMalloc(&B.d_lev_rows[dir], std::max(B.nb, 1) * sizeof(int)));
+        MF_HIP(hipMemcpy(B.d_lev_rows[dir], rows.data(), B.nb * sizeof(int),
+                         hipMemcpyHostToDevice));
+      }
+    return B.nb;
+  }
   if (Precon != NO_PRECON && Precon != BLOCK_JACOBI) {
     set_error("lar_allocate_preconditioner: CRS storage offers NO_PRECON and BLOCK_JACOBI");
     return -1;
@@ -292,11 +488,23 @@ int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
   return P.n;
 }
 
-// lar_fill_preconditioner (las_crs_intf.c:707-737): diag_precon = 1/a_ii
+// lar_fill_preconditioner (las_crs_intf.c:707-737): diag_precon = 1/a_ii;
+// BCRS (las_bcrs_intf.c:1072-1148): LU of every diagonal block as LAPACK
+// dgetrf computes it (an error, where dgetrf would report a zero pivot)
 int lar_fill_preconditioner(int Matrix_id) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
   if (flush(*M)) return -2;
+  if (M->B) {
+    if (M->precon == NO_PRECON) return 0;
+    if (M->precon < 0 || !M->B->d_lu) {
+      set_error("lar_fill_preconditioner: preconditioner not allocated");
+      return -1;
+    }
+    if (bcrs_fill_lu(*M->B)) return -1;
+    M->filled = true;
+    return 0;
+  }
   Problem& P = *M->P;
   if (!P.d_dinv) {
     set_error("lar_fill_preconditioner: preconditioner not allocated");
@@ -310,6 +518,12 @@ int lar_fill_preconditioner(int Matrix_id) {
 int lar_free_preconditioner(int Matrix_id) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
+  if (M->B) {
+    free_bcrs_precon(*M->B);
+    M->filled = false;
+    M->precon = -1;
+    return 0;
+  }
   Problem& P = *M->P;
   if (P.d_dinv) (void)hipFree(P.d_dinv);
   if (P.d_diag_ptr) (void)hipFree(P.d_diag_ptr);
@@ -318,15 +532,26 @@ int lar_free_preconditioner(int Matrix_id) {
   return 0;
 }
 
-// lar_compute_residual (las_crs_intf.c:788-849): V = (Use_rhs ? B|rhs : 0) - A X
+// lar_compute_residual (las_crs_intf.c:788-849, las_bcrs_intf.c:2787-2930):
+// V = (Use_rhs ? B|rhs : 0) - A X
 void lar_compute_residual(int Matrix_id, int Use_rhs, int Ini_zero, int Ndof, double* X,
                           double* B, double* V) {
   LarMatrix* M = lar_get(Matrix_id);
-  if (!M || Ndof != M->P->n) {
+  if (!M || Ndof != M->ndof()) {
     fprintf(stderr, "lar_compute_residual: bad matrix id or size\n");
     return;
   }
   if (flush(*M)) return;
+  if (M->B) {
+    Staged sx, sb, sv;
+    if (!Ini_zero && stage_in(X, Ndof, sx, true)) return;
+    if (Use_rhs == 1 && B && stage_in(B, Ndof, sb, true)) return;
+    if (stage_in(V, Ndof, sv, false)) return;
+    const double* db = (Use_rhs == 1) ? (B ? sb.d : M->B->d_rhs) : nullptr;
+    if (bcrs_residual(*M->B, Ini_zero ? nullptr : sx.d, db, sv.d)) return;
+    stage_out(V, sv, Ndof);
+    return;
+  }
   Problem& P = *M->P;
   Staged sx, sb, sv;
   if (!Ini_zero && stage_in(X, Ndof, sx, true)) return;
@@ -338,11 +563,33 @@ void lar_compute_residual(int Matrix_id, int Use_rhs, int Ini_zero, int Ndof, do
 }
 
 // lar_perform_BJ_or_GS_iterations (las_crs_intf.c:857-1013), BLOCK_JACOBI:
-// V <- D^-1 (B - (A - D) V), Nr_prec sweeps
+// V <- D^-1 (B - (A - D) V), Nr_prec sweeps.  BCRS (las_bcrs_intf.c:
+// 2938-3300): block Jacobi or block Gauss-Seidel in the reference's row order;
+// even Nr_prec gives Nr_prec/2 forward + backward sweep pairs, odd gives
+// Nr_prec/2 + 1 forward sweeps
 void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero, int Nr_prec,
                                      int Ndof, double* V, double* B) {
   (void)Ini_zero;
   LarMatrix* M = lar_get(Matrix_id);
+  if (M && M->B) {
+    if (Ndof != M->ndof() || (M->precon != BLOCK_JACOBI && M->precon != BLOCK_GS) ||
+        !M->filled) {
+      fprintf(stderr, "lar_perform_BJ_or_GS_iterations: needs a filled BLOCK_JACOBI or "
+                      "BLOCK_GS preconditioner\n");
+      return;
+    }
+    if (flush(*M)) return;
+    Staged sv, sb;
+    if (stage_in(V, Ndof, sv, true)) return;
+    if (Use_rhs == 1 && B && stage_in(B, Ndof, sb, true)) return;
+    double* tmp = nullptr;
+    if (hipMalloc(&tmp, (size_t)Ndof * sizeof(double)) != hipSuccess) return;
+    const double* db = (Use_rhs == 1) ? (B ? sb.d : M->B->d_rhs) : nullptr;
+    if (bcrs_sweeps(*M->B, M->precon == BLOCK_GS, Nr_prec, db, sv.d, tmp) == 0)
+      stage_out(V, sv, Ndof);
+    (void)hipFree(tmp);
+    return;
+  }
   if (!M || Ndof != M->P->n || M->precon != BLOCK_JACOBI || !M->P->d_dinv) {
     fprintf(stderr, "lar_perform_BJ_or_GS_iterations: needs a filled BLOCK_JACOBI preconditioner\n");
     return;
@@ -369,15 +616,55 @@ void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero, i
 // (las_crs_intf.c:1019-1057); kept as a no-op with a diagnostic
 void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B) {
   (void)Matrix_id; (void)Ndof; (void)V; (void)B;
-  set_error("lar_perform_rhsub: no ILU for CRS storage (the reference body is empty)");
+  set_error("lar_perform_rhsub: no ILU for this storage");
+}
+
+// storage type (2 CRS, 3 BCRS), block size, rows of blocks and stored blocks
+// of a matrix, to size the buffers of mf_lar_get_system
+int mf_lar_get_sizes(int Matrix_id, int* storage, int* block_size, int* nrblocks,
+                     long long* nr_stored) {
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M) return -3;
+  if (storage) *storage = M->B ? LAC_STORAGE_BCRS : LAC_STORAGE_CRS;
+  if (block_size) *block_size = M->B ? BLS : 1;
+  if (nrblocks) *nrblocks = M->B ? M->B->nb : M->P->n;
+  if (nr_stored) *nr_stored = M->B ? M->B->nblk : M->P->nnz;
+  return 0;
+}
+
+// host copies of the BCRS diagonal-block LU factors (16 per block row,
+// column-major, unit lower part below the diagonal) and 0-based pivots (4 per
+// block row: row j was interchanged with row piv[j])
+int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv) {
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M) return -3;
+  if (!M->B || !M->filled) {
+    set_error("mf_lar_get_block_precon: needs a BCRS matrix with a filled block preconditioner");
+    return -1;
+  }
+  const BcrsDev& B = *M->B;
+  if (lu) MF_HIP(hipMemcpy(lu, B.d_lu, (size_t)B.nb * BLS * BLS * sizeof(double), hipMemcpyDeviceToHost));
+  if (piv) MF_HIP(hipMemcpy(piv, B.d_piv, (size_t)B.nb * BLS * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 // host copies of the stored system (lar_get_SM_and_LV_crs_from_crs,
-// las_crs_intf.c:1063-1120 with offset 0)
+// las_crs_intf.c:1063-1120 with offset 0); BCRS: block row pointers, block
+// column indices, 16 values per block (column-major) and the rhs
 int mf_lar_get_system(int Matrix_id, int* row_ptr, int* col_ind, double* val, double* rhs) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
   if (flush(*M)) return -2;
+  if (M->B) {
+    const BcrsDev& B = *M->B;
+    if (row_ptr) memcpy(row_ptr, M->rp.data(), M->rp.size() * sizeof(int));
+    if (col_ind) memcpy(col_ind, M->ci.data(), M->ci.size() * sizeof(int));
+    if (val && B.nblk)
+      MF_HIP(hipMemcpy(val, B.d_val, B.nblk * BLS * BLS * sizeof(double), hipMemcpyDeviceToHost));
+    if (rhs && B.nb)
+      MF_HIP(hipMemcpy(rhs, B.d_rhs, (size_t)B.nb * BLS * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
   Problem& P = *M->P;
   if (row_ptr) memcpy(row_ptr, P.setup.row_ptr.data(), (P.n + 1) * sizeof(int));
   if (col_ind) memcpy(col_ind, P.setup.col_ind.data(), P.nnz * sizeof(int));

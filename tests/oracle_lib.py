@@ -394,3 +394,64 @@ def _dg_system_sparse(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval
     A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                       shape=(n, n)).tocsr()
     return A, b
+
+
+# --- BCRS storage with 4x4 blocks -------------------------------------------
+
+def bcrs_vector_problem(arrays, seed, diag_shift=6.0):
+    """a 4-component problem on a prism mesh: node blocks, neighbour lists
+    (nodes sharing an element), 24x24 element matrices (column-major,
+    uniform(-1, 1) + diag_shift I) and 24-vectors"""
+    en = arrays["el_nodes"]
+    nn = arrays["coor"].shape[0]
+    nbs = [set() for _ in range(nn)]
+    for e in en:
+        for u in e:
+            nbs[u].update(int(v) for v in e if v != u)
+    nb_lists = [sorted(s) for s in nbs]
+    rng = np.random.default_rng(seed)
+    mats = []
+    for _ in range(en.shape[0]):
+        K = rng.uniform(-1, 1, (24, 24))
+        K += np.eye(24) * diag_shift
+        mats.append((np.ascontiguousarray(K.T.ravel()), rng.uniform(-1, 1, 24)))
+    return en, nn, nb_lists, mats
+
+
+def bcrs_flat(nb_lists):
+    ptr = np.zeros(len(nb_lists) + 1, np.int32)
+    ptr[1:] = np.cumsum([len(x) for x in nb_lists])
+    ind = np.array([v for x in nb_lists for v in x], np.int32)
+    return ptr, ind
+
+
+def bcrs_assemble(nn, nb_lists, en, mats):
+    """the oracle's BCRS pattern and element-by-element assembly"""
+    o = oracle()
+    ptr, ind = bcrs_flat(nb_lists)
+    rp = np.zeros(nn + 1, np.int32)
+    nsm = o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), None)
+    ci = np.zeros(nsm, np.int32)
+    o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), P(ci))
+    val = np.zeros(16 * nsm)
+    rhs = np.zeros(4 * nn)
+    for e, (K, F) in zip(en, mats):
+        l_bl = np.ascontiguousarray(e, np.int32)
+        o.orc_bcrs_assemble(P(rp), P(ci), 6, P(l_bl), P(K), P(np.ascontiguousarray(F)), P(val), P(rhs))
+    return rp, ci, val, rhs
+
+
+def bcrs_diag_lu(nn, rp, ci, val):
+    """diagonal positions and the oracle's LU factors of the diagonal blocks"""
+    o = oracle()
+    diag = np.array([next(s for s in range(rp[r], rp[r + 1]) if ci[s] == r) for r in range(nn)],
+                    np.int32)
+    lu = np.zeros(16 * nn)
+    piv = np.zeros(4 * nn, np.int32)
+    for r in range(nn):
+        blk = np.ascontiguousarray(val[16 * diag[r]:16 * diag[r] + 16])
+        pv = np.zeros(4, np.int32)
+        assert o.orc_lu4(P(blk), P(pv)) == 0
+        lu[16 * r:16 * r + 16] = blk
+        piv[4 * r:4 * r + 4] = pv
+    return diag, lu, piv

@@ -1,0 +1,154 @@
+"""BCRS storage with 4x4 blocks through the storage seam (include/modfem_lar.h)
+on the device, against the oracle (itself bit-exact with the reference's
+lsd_mkb/lad_bcrs/las_bcrs_intf.c for pattern, assembly and residual,
+tests/test_oracle_bcrs.py): pattern, assembly with and without tables,
+residual, diagonal-block LU factors, block Jacobi and block Gauss-Seidel
+sweeps are all bit-exact."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import modfem_amd as mf  # noqa: E402
+from oracle_lib import P, bcrs_assemble, bcrs_diag_lu, bcrs_vector_problem, oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+LAC_STORAGE_BCRS = 3
+BLOCK_JACOBI, BLOCK_GS = 1, 2
+
+
+def allocate(L, nn, nb_lists, storage=LAC_STORAGE_BCRS, block_size=4):
+    arrs = [np.ascontiguousarray(x, np.int32) for x in nb_lists]
+    ptrs = (ctypes.c_void_p * nn)(*[P(x) for x in arrs])
+    nro = np.array([len(x) for x in nb_lists], np.int32)
+    mid = L.lar_allocate_SM_and_LV(storage, 1, 4 * nn, 24, nn, block_size, None, None, P(nro),
+                                   ctypes.cast(ptrs, ctypes.c_void_p))
+    return mid
+
+
+def build(nx, ny, nz, seed, tables, diag_shift=6.0):
+    """allocate and assemble a BCRS matrix through the seam; returns the
+    matrix id and the oracle's arrays"""
+    mf.init()
+    L = mf.lib()
+    a = mf.Mesh.box_prism(nx, ny, nz, seed=seed, jitter=0.2).arrays()
+    en, nn, nb_lists, mats = bcrs_vector_problem(a, seed, diag_shift)
+    mid = allocate(L, nn, nb_lists)
+    assert mid >= 0, L.mf_last_error()
+    assert L.lar_initialize_SM_and_LV(mid, 0) == 0
+    tab = np.zeros(6 * 6 + 6, np.int32)
+    for e, (K, F) in zip(en, mats):
+        l_bl = np.ascontiguousarray(e, np.int32)
+        Fc = np.ascontiguousarray(F)
+        if tables:
+            assert L.lar_fill_assembly_table_int_ent(mid, 6, P(l_bl), None, P(tab)) == 1
+            assert L.lar_assemble_SM_and_LV_with_table(mid, 0, 6, P(tab), P(K), P(Fc), None) == 1
+        else:
+            assert L.lar_assemble_SM_and_LV(mid, 0, 6, P(l_bl), None, P(K), P(Fc), None) == 1
+    rp, ci, val, rhs = bcrs_assemble(nn, nb_lists, en, mats)
+    return L, mid, nn, rp, ci, val, rhs
+
+
+def free(L, mid):
+    assert L.lar_free_preconditioner(mid) == 0
+    assert L.lar_free_SM_and_LV(mid) == 0
+
+
+@pytest.mark.parametrize("tables", [True, False])
+def test_pattern_assembly_and_residual_bitexact(tables):
+    L, mid, nn, rp, ci, val, rhs = build(4, 3, 3, 5, tables)
+    st, bs, nb, ns = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()
+    assert L.mf_lar_get_sizes(mid, ctypes.byref(st), ctypes.byref(bs), ctypes.byref(nb),
+                              ctypes.byref(ns)) == 0
+    assert (st.value, bs.value, nb.value, ns.value) == (LAC_STORAGE_BCRS, 4, nn, len(ci))
+    grp, gci = np.zeros(nn + 1, np.int32), np.zeros(len(ci), np.int32)
+    gval, grhs = np.zeros(16 * len(ci)), np.zeros(4 * nn)
+    assert L.mf_lar_get_system(mid, P(grp), P(gci), P(gval), P(grhs)) == 0
+    assert np.array_equal(grp, rp) and np.array_equal(gci, ci)
+    assert np.array_equal(gval, val) and np.array_equal(grhs, rhs)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, 4 * nn)
+    b = rng.uniform(-1, 1, 4 * nn)
+    o = oracle()
+    for use_rhs, bb, ini_zero in [(1, None, 0), (1, b, 0), (0, None, 0), (1, None, 1)]:
+        v, v0 = np.zeros(4 * nn), np.zeros(4 * nn)
+        L.lar_compute_residual(mid, use_rhs, ini_zero, 4 * nn, P(x), P(bb) if bb is not None else None,
+                               P(v))
+        o.orc_bcrs_residual(nn, P(rp), P(ci), P(val), use_rhs, None if ini_zero else P(x),
+                            P(bb if bb is not None else rhs), P(v0))
+        assert np.array_equal(v, v0), (use_rhs, ini_zero)
+    assert L.lar_free_SM_and_LV(mid) == 0
+
+
+@pytest.mark.parametrize("precon", [BLOCK_JACOBI, BLOCK_GS])
+def test_block_lu_and_sweeps_bitexact(precon):
+    # no diagonal shift: the diagonal blocks need row interchanges
+    L, mid, nn, rp, ci, val, rhs = build(5, 4, 3, 9, True, diag_shift=0.0)
+    assert L.lar_allocate_preconditioner(mid, precon, 0) == nn, L.mf_last_error()
+    assert L.lar_fill_preconditioner(mid) == 0, L.mf_last_error()
+    diag, lu0, piv0 = bcrs_diag_lu(nn, rp, ci, val)
+    lu, piv = np.zeros(16 * nn), np.zeros(4 * nn, np.int32)
+    assert L.mf_lar_get_block_precon(mid, P(lu), P(piv)) == 0
+    assert np.array_equal(piv, piv0) and np.array_equal(lu, lu0)
+    assert (piv != np.tile(np.arange(4), nn)).any()  # row interchanges occur
+    o = oracle()
+    rng = np.random.default_rng(4)
+    v_in = rng.uniform(-1, 1, 4 * nn)
+    b = rng.uniform(-1, 1, 4 * nn)
+    gs = int(precon == BLOCK_GS)
+    for nr_prec, use_rhs, bb in [(1, 1, None), (2, 1, None), (3, 1, b), (4, 0, None), (5, 1, b)]:
+        v, v0 = v_in.copy(), v_in.copy()
+        L.lar_perform_BJ_or_GS_iterations(mid, use_rhs, 0, nr_prec, 4 * nn, P(v),
+                                          P(bb) if bb is not None else None)
+        o.orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu0), P(piv0), gs, use_rhs, nr_prec,
+                         P(bb if bb is not None else rhs), P(v0))
+        assert np.array_equal(v, v0), (nr_prec, use_rhs)
+    free(L, mid)
+
+
+def test_gauss_seidel_many_levels_bitexact_and_converges():
+    """a larger mesh (several hundred levels per sweep) with dominant
+    diagonal blocks: bitwise the oracle, and converging to the solution"""
+    L, mid, nn, rp, ci, val, rhs = build(10, 9, 6, 21, True, diag_shift=40.0)
+    assert L.lar_allocate_preconditioner(mid, BLOCK_GS, 0) == nn
+    assert L.lar_fill_preconditioner(mid) == 0
+    diag, lu0, piv0 = bcrs_diag_lu(nn, rp, ci, val)
+    v, v0 = np.zeros(4 * nn), np.zeros(4 * nn)
+    L.lar_perform_BJ_or_GS_iterations(mid, 1, 1, 40, 4 * nn, P(v), None)
+    oracle().orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu0), P(piv0), 1, 1, 40, P(rhs),
+                            P(v0))
+    assert np.array_equal(v, v0)
+    r = np.zeros(4 * nn)
+    L.lar_compute_residual(mid, 1, 0, 4 * nn, P(v), None, P(r))
+    assert np.abs(r).max() <= 1e-10 * np.abs(rhs).max()
+    free(L, mid)
+
+
+def test_errors():
+    mf.init()
+    L = mf.lib()
+    a = mf.Mesh.box_prism(2, 2, 1, seed=1, jitter=0.0).arrays()
+    en, nn, nb_lists, mats = bcrs_vector_problem(a, 1)
+    # BCRS needs block size 4
+    assert allocate(L, nn, nb_lists, block_size=3) < 0
+    # block size 4 with undefined storage picks BCRS
+    mid = allocate(L, nn, nb_lists, storage=-1)
+    assert mid >= 0
+    st = ctypes.c_int()
+    assert L.mf_lar_get_sizes(mid, ctypes.byref(st), None, None, None) == 0 and st.value == 3
+    # ILU is not offered
+    assert L.lar_allocate_preconditioner(mid, 5, 0) < 0
+    # all-zero matrix: singular diagonal blocks are reported
+    assert L.lar_allocate_preconditioner(mid, BLOCK_JACOBI, 0) == nn
+    assert L.lar_fill_preconditioner(mid) < 0
+    assert b"singular" in L.mf_last_error()
+    # a block pair outside the pattern
+    tab = np.zeros(6, np.int32)
+    far = np.array([0, nn - 1], np.int32)
+    assert nn - 1 not in nb_lists[0]
+    assert L.lar_fill_assembly_table_int_ent(mid, 2, P(far), None, P(tab)) < 0
+    free(L, mid)

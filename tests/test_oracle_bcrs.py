@@ -14,7 +14,7 @@ import pytest
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import modfem_amd as mf  # noqa: E402
-from oracle_lib import P, oracle, ref, ref_available  # noqa: E402
+from oracle_lib import P, bcrs_assemble, bcrs_vector_problem, oracle, ref, ref_available  # noqa: E402
 
 need_bcrs = pytest.mark.skipif(not ref_available("libref_bcrs.so"), reason="oracle/_ref not built")
 
@@ -39,47 +39,8 @@ class BcrsStruct(ctypes.Structure):
 
 
 def vector_problem(nx, ny, nz, seed):
-    """a 4-component problem on a prism mesh: node blocks, neighbour lists
-    (nodes sharing an element), 24x24 element matrices (column-major) and
-    24-vectors"""
     mesh = mf.Mesh.box_prism(nx, ny, nz, seed=seed, jitter=0.2)
-    a = mesh.arrays()
-    en = a["el_nodes"]
-    nn = a["coor"].shape[0]
-    nbs = [set() for _ in range(nn)]
-    for e in en:
-        for u in e:
-            nbs[u].update(int(v) for v in e if v != u)
-    nb_lists = [sorted(s) for s in nbs]
-    rng = np.random.default_rng(seed)
-    mats = []
-    for _ in range(en.shape[0]):
-        K = rng.uniform(-1, 1, (24, 24))
-        K += np.eye(24) * 6.0
-        mats.append((np.ascontiguousarray(K.T.ravel()), rng.uniform(-1, 1, 24)))
-    return mesh, en, nn, nb_lists, mats
-
-
-def flat(nb_lists):
-    ptr = np.zeros(len(nb_lists) + 1, np.int32)
-    ptr[1:] = np.cumsum([len(x) for x in nb_lists])
-    ind = np.array([v for x in nb_lists for v in x], np.int32)
-    return ptr, ind
-
-
-def oracle_assemble(nn, nb_lists, en, mats):
-    o = oracle()
-    ptr, ind = flat(nb_lists)
-    rp = np.zeros(nn + 1, np.int32)
-    nsm = o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), None)
-    ci = np.zeros(nsm, np.int32)
-    o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), P(ci))
-    val = np.zeros(16 * nsm)
-    rhs = np.zeros(4 * nn)
-    for e, (K, F) in zip(en, mats):
-        l_bl = np.ascontiguousarray(e, np.int32)
-        o.orc_bcrs_assemble(P(rp), P(ci), 6, P(l_bl), P(K), P(np.ascontiguousarray(F)), P(val), P(rhs))
-    return rp, ci, val, rhs
+    return (mesh,) + bcrs_vector_problem(mesh.arrays(), seed)
 
 
 def ref_bcrs():
@@ -128,7 +89,7 @@ def test_pattern_assembly_and_residual_bitexact_with_reference():
         L.lar_assemble_SM_and_LV_bcrs(mid, 0, 6, P(l_bl), None, P(K), P(np.ascontiguousarray(F)),
                                       None)
     rp, ci, val, rhs = ref_arrays(L, mid, nn)
-    orp, oci, oval, orhs = oracle_assemble(nn, nb_lists, en, mats)
+    orp, oci, oval, orhs = bcrs_assemble(nn, nb_lists, en, mats)
     assert np.array_equal(rp, orp) and np.array_equal(ci, oci)
     assert np.array_equal(val, oval) and np.array_equal(rhs, orhs)
     # residual V = b - A x
@@ -153,7 +114,7 @@ def test_assembly_with_tables_equals_table_less():
         L.lar_assemble_SM_and_LV_with_table_bcrs(mid, 0, 6, P(tab), P(K), P(np.ascontiguousarray(F)),
                                                  None)
     rp, ci, val, rhs = ref_arrays(L, mid, nn)
-    orp, oci, oval, orhs = oracle_assemble(nn, nb_lists, en, mats)
+    orp, oci, oval, orhs = bcrs_assemble(nn, nb_lists, en, mats)
     assert np.array_equal(val, oval) and np.array_equal(rhs, orhs)
     L.lar_free_SM_and_LV_bcrs(mid)
 
@@ -186,7 +147,7 @@ def test_lu4_factors_and_solves():
 
 def bj_gs_setup(seed):
     mesh, en, nn, nb_lists, mats = vector_problem(3, 2, 2, seed)
-    rp, ci, val, rhs = oracle_assemble(nn, nb_lists, en, mats)
+    rp, ci, val, rhs = bcrs_assemble(nn, nb_lists, en, mats)
     # make the diagonal blocks dominant so that both iterations converge
     diag = np.array([next(s for s in range(rp[r], rp[r + 1]) if ci[s] == r) for r in range(nn)], np.int32)
     for r in range(nn):
