@@ -48,10 +48,14 @@ int lar_assemble_SM_and_LV(int Matrix_id, int Scope, int Nr_dof_bl,
 /* lah_intf.h:184 */
 int lar_free_SM_and_LV(int Matrix_id);
 /* lah_intf.h:192 -- CRS: NO_PRECON (0) and BLOCK_JACOBI (1); BCRS also
-   BLOCK_GS (2).  Returns the number of diagonal blocks */
+   BLOCK_GS (2) and block ILU(0): MULTI_ILU (4) / BLOCK_ILU (5) with
+   Second_arg (k of ILU(k)) = 0 on a structurally symmetric pattern.
+   Returns the number of diagonal blocks */
 int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg);
 /* lah_intf.h:203 -- CRS: 1/a_ii; BCRS: LU with partial pivoting of every
-   diagonal block (LAPACK dgetrf's algorithm); <0 for a singular block */
+   diagonal block (LAPACK dgetrf's algorithm), or the block ILU(0) factors
+   (las_bcrs_intf.c:1175-1246, 2181-2640: L blocks a_ki inv(a_ii), U blocks,
+   inverted diagonal blocks); <0 for a singular block */
 int lar_fill_preconditioner(int Matrix_id);
 /* lah_intf.h:210 */
 int lar_free_preconditioner(int Matrix_id);
@@ -65,8 +69,10 @@ void lar_compute_residual(int Matrix_id, int Use_rhs, int Ini_zero, int Ndof,
    forward sweeps */
 void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero,
                                      int Nr_prec, int Ndof, double* V, double* B);
-/* lah_intf.h:269 -- empty for CRS storage in the reference
-   (las_crs_intf.c:1019-1057); records an error (no ILU is offered) */
+/* lah_intf.h:269 -- BCRS with ILU(0): V = (LU)^-1 B (B NULL: the stored
+   rhs), forward reduction and back substitution in the reference's order
+   (las_bcrs_intf.c:3319-3510), bitwise its result.  Empty for CRS storage in
+   the reference (las_crs_intf.c:1019-1057): records an error there */
 void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B);
 
 /* Inspection (no reference counterpart; lar_get_SM_and_LV_crs_from_crs /
@@ -79,7 +85,8 @@ int mf_lar_get_sizes(int Matrix_id, int* storage, int* block_size,
 int mf_lar_get_system(int Matrix_id, int* row_ptr, int* col_ind, double* val,
                       double* rhs);
 /* BCRS: LU factors (16 per block row, column-major) and 0-based pivots
-   (4 per block row) of the filled diagonal-block preconditioner */
+   (4 per block row) of the filled diagonal-block preconditioner; with ILU(0)
+   the factors of every stored block (16 each) and no pivots */
 int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv);
 
 #ifdef __cplusplus

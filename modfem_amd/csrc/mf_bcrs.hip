@@ -142,6 +142,170 @@ __global__ void k_bcrs_gs_level(int cnt, const int* rows, int dir, const int* rp
   bcrs_row(rows[t], dir, rp, ci, diag, val, lu, piv, b, v, out, out);
 }
 
+// inverse of a 4x4 block from its lu4 factors, the unblocked algorithm of
+// LAPACK dgetri: inv(U) column by column (dtrti2 with dtrmv, which skips zero
+// entries), then inv(A)(:, j) -= inv(A)(:, j+1:) L(j+1:, j) for j = 3..0
+// (dgemv), then the column interchanges in reverse order
+__device__ __forceinline__ void inv4(double* a, const int* ip) {
+  for (int j = 0; j < 4; j++) {
+    a[j + 4 * j] = __ddiv_rn(1.0, a[j + 4 * j]);
+    const double ajj = -a[j + 4 * j];
+    for (int jj = 0; jj < j; jj++)
+      if (a[jj + 4 * j] != 0.0) {
+        const double t = a[jj + 4 * j];
+        for (int ii = 0; ii < jj; ii++) a[ii + 4 * j] = __dadd_rn(a[ii + 4 * j], __dmul_rn(t, a[ii + 4 * jj]));
+        a[jj + 4 * j] = __dmul_rn(a[jj + 4 * j], a[jj + 4 * jj]);
+      }
+    for (int ii = 0; ii < j; ii++) a[ii + 4 * j] = __dmul_rn(ajj, a[ii + 4 * j]);
+  }
+  double work[4];
+  for (int j = 3; j >= 0; j--) {
+    for (int i = j + 1; i < 4; i++) {
+      work[i] = a[i + 4 * j];
+      a[i + 4 * j] = 0.0;
+    }
+    for (int jj = j + 1; jj < 4; jj++) {
+      const double t = -work[jj];
+      for (int i = 0; i < 4; i++) a[i + 4 * j] = __dadd_rn(a[i + 4 * j], __dmul_rn(t, a[i + 4 * jj]));
+    }
+  }
+  for (int j = 2; j >= 0; j--) {
+    const int jp = ip[j];
+    if (jp != j)
+      for (int i = 0; i < 4; i++) {
+        const double t = a[i + 4 * j];
+        a[i + 4 * j] = a[i + 4 * jp];
+        a[i + 4 * jp] = t;
+      }
+  }
+}
+
+// LU with partial pivoting of a 4x4 block in registers (dgetf2); false for a
+// zero pivot
+__device__ __forceinline__ bool lu4(double* a, int* ip) {
+  bool ok = true;
+  for (int j = 0; j < 4; j++) {
+    int jp = j;
+    double amax = fabs(a[j + 4 * j]);
+    for (int i = j + 1; i < 4; i++)
+      if (fabs(a[i + 4 * j]) > amax) { amax = fabs(a[i + 4 * j]); jp = i; }
+    ip[j] = jp;
+    if (a[jp + 4 * j] != 0.0) {
+      if (jp != j)
+        for (int c = 0; c < 4; c++) {
+          const double t = a[j + 4 * c];
+          a[j + 4 * c] = a[jp + 4 * c];
+          a[jp + 4 * c] = t;
+        }
+      const double rec = __ddiv_rn(1.0, a[j + 4 * j]);
+      for (int i = j + 1; i < 4; i++) a[i + 4 * j] = __dmul_rn(a[i + 4 * j], rec);
+    } else {
+      ok = false;
+    }
+    for (int c = j + 1; c < 4; c++) {
+      const double t = -a[j + 4 * c];
+      for (int i = j + 1; i < 4; i++) a[i + 4 * c] = __dadd_rn(a[i + 4 * c], __dmul_rn(a[i + 4 * j], t));
+    }
+  }
+  return ok;
+}
+
+// block ILU(0), the rows of one level (a row's level exceeds those of the
+// rows in its lower part).  Row k in the order of the reference's
+// right-looking variant (las_bcrs_intf.c:2181-2640), which applies to every
+// block of row k the pivots i < k in increasing order: for each lower block
+// a_ki, a_ki = a_ki inv(a_ii) (sums from 0), then a_kj -= a_ki a_ij for the
+// blocks of row i's upper part present in row k (dgemm order: column, inner
+// index, row); finally the diagonal block is replaced by its inverse
+__global__ void k_bcrs_ilu_level(int cnt, const int* rows, const int* rp, const int* ci,
+                                 const int* diag, double* ilu, int* bad) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int k = rows[t];
+  const int dk = diag[k], end = rp[k + 1];
+  for (int s = rp[k]; s < dk; s++) {
+    const int i = ci[s];
+    double* aki = ilu + 16 * (size_t)s;
+    const double* w = ilu + 16 * (size_t)diag[i];
+    double a[16], p[16];
+    for (int q = 0; q < 16; q++) a[q] = aki[q];
+    for (int r = 0; r < 4; r++)
+      for (int c = 0; c < 4; c++) {
+        double d = 0.0;
+        for (int l = 0; l < 4; l++) d = __dadd_rn(d, __dmul_rn(a[r + 4 * l], w[l + 4 * c]));
+        p[r + 4 * c] = d;
+      }
+    for (int q = 0; q < 16; q++) aki[q] = p[q];
+    int skj = s + 1;
+    for (int sij = diag[i] + 1; sij < rp[i + 1]; sij++) {
+      const int j = ci[sij];
+      while (skj < end && ci[skj] < j) skj++;
+      if (skj >= end) break;
+      if (ci[skj] != j) continue;
+      const double* aij = ilu + 16 * (size_t)sij;
+      double* akj = ilu + 16 * (size_t)skj;
+      double c4[16];
+      for (int q = 0; q < 16; q++) c4[q] = akj[q];
+      for (int c = 0; c < 4; c++)
+        for (int l = 0; l < 4; l++) {
+          const double tt = -aij[l + 4 * c];
+          for (int r = 0; r < 4; r++) c4[r + 4 * c] = __dadd_rn(c4[r + 4 * c], __dmul_rn(tt, p[r + 4 * l]));
+        }
+      for (int q = 0; q < 16; q++) akj[q] = c4[q];
+    }
+  }
+  double d[16];
+  int ip[4];
+  double* akk = ilu + 16 * (size_t)dk;
+  for (int q = 0; q < 16; q++) d[q] = akk[q];
+  if (!lu4(d, ip)) {
+    bad[0] = 1;
+    return;
+  }
+  inv4(d, ip);
+  for (int q = 0; q < 16; q++) akk[q] = d[q];
+}
+
+// forward reduction of the ILU solve (lar_perform_rhsub_bcrs, variant 0,
+// las_bcrs_intf.c:3349-3430), the rows of one level: v_i = b_i - L_i. v
+__global__ void k_bcrs_fwd_level(int cnt, const int* rows, const int* rp, const int* ci,
+                                 const int* diag, const double* ilu, const double* b, double* v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  double x[4];
+  for (int r = 0; r < 4; r++) x[r] = b[4 * (size_t)i + r];
+  for (int s = rp[i]; s < diag[i]; s++) {
+    const double* a = ilu + 16 * (size_t)s;
+    const double* y = v + 4 * (size_t)ci[s];
+    for (int c = 0; c < 4; c++)
+      for (int r = 0; r < 4; r++) x[r] = __dsub_rn(x[r], __dmul_rn(a[r + 4 * c], y[c]));
+  }
+  for (int r = 0; r < 4; r++) v[4 * (size_t)i + r] = x[r];
+}
+
+// back substitution (:3432-3510), the rows of one level:
+// v_i = inv(a_ii) (v_i - U_i. v)
+__global__ void k_bcrs_bwd_level(int cnt, const int* rows, const int* rp, const int* ci,
+                                 const int* diag, const double* ilu, double* v) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  const int i = rows[t];
+  double x[4];
+  for (int r = 0; r < 4; r++) x[r] = v[4 * (size_t)i + r];
+  for (int s = diag[i] + 1; s < rp[i + 1]; s++) {
+    const double* a = ilu + 16 * (size_t)s;
+    const double* y = v + 4 * (size_t)ci[s];
+    for (int c = 0; c < 4; c++)
+      for (int r = 0; r < 4; r++) x[r] = __dsub_rn(x[r], __dmul_rn(a[r + 4 * c], y[c]));
+  }
+  const double* d = ilu + 16 * (size_t)diag[i];
+  double o[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int c = 0; c < 4; c++)
+    for (int r = 0; r < 4; r++) o[r] = __dadd_rn(o[r], __dmul_rn(d[r + 4 * c], x[c]));
+  for (int r = 0; r < 4; r++) v[4 * (size_t)i + r] = o[r];
+}
+
 }  // namespace
 
 int bcrs_levels(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
@@ -216,6 +380,49 @@ int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, dou
       MF_HIP(hipMemcpyAsync(v, tmp, bytes, hipMemcpyDeviceToDevice, stream()));
     }
   MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int bcrs_ilu_factor(BcrsDev& M) {
+  if (M.nb == 0) return 0;
+  MF_HIP(hipMemcpyAsync(M.d_ilu, M.d_val, (size_t)M.nblk * 16 * sizeof(double),
+                        hipMemcpyDeviceToDevice, stream()));
+  int* d_bad = nullptr;
+  MF_HIP(hipMalloc(&d_bad, sizeof(int)));
+  MF_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), stream()));
+  const std::vector<int>& lp = M.lev_ptr[0];
+  for (size_t l = 0; l + 1 < lp.size(); l++) {
+    const int cnt = lp[l + 1] - lp[l];
+    hipLaunchKernelGGL(k_bcrs_ilu_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
+                       M.d_lev_rows[0] + lp[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, d_bad);
+  }
+  MF_HIP(hipGetLastError());
+  int bad = 0;
+  MF_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, stream()));
+  MF_HIP(hipStreamSynchronize(stream()));
+  (void)hipFree(d_bad);
+  if (bad) {
+    set_error("lar_fill_preconditioner: a diagonal block of the ILU(0) factors is singular");
+    return -1;
+  }
+  return 0;
+}
+
+int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v) {
+  if (M.nb == 0) return 0;
+  const std::vector<int>& lf = M.lev_ptr[0];
+  for (size_t l = 0; l + 1 < lf.size(); l++) {
+    const int cnt = lf[l + 1] - lf[l];
+    hipLaunchKernelGGL(k_bcrs_fwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
+                       M.d_lev_rows[0] + lf[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, b, v);
+  }
+  const std::vector<int>& lb = M.lev_ptr[1];
+  for (size_t l = 0; l + 1 < lb.size(); l++) {
+    const int cnt = lb[l + 1] - lb[l];
+    hipLaunchKernelGGL(k_bcrs_bwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
+                       M.d_lev_rows[1] + lb[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, v);
+  }
+  MF_HIP(hipGetLastError());
   return 0;
 }
 

@@ -290,6 +290,7 @@ struct BcrsDev {
   long long nblk = 0;             // stored blocks
   int *d_rp = nullptr, *d_ci = nullptr, *d_diag = nullptr;
   double *d_val = nullptr, *d_rhs = nullptr, *d_lu = nullptr;
+  double* d_ilu = nullptr;        // block ILU(0) factors (16 per stored block)
   int* d_piv = nullptr;
   std::vector<int> lev_ptr[2];
   int* d_lev_rows[2] = {nullptr, nullptr};
@@ -305,6 +306,10 @@ int bcrs_fill_lu(BcrsDev& M);
 // block Jacobi (gs = 0) or Gauss-Seidel sweeps with the sweep-count rule of
 // the reference; tmp: 4 nb doubles
 int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, double* tmp);
+// block ILU(0) factors of the current values into d_ilu (levels of both
+// directions built), and v = (LU)^-1 b (v may alias b)
+int bcrs_ilu_factor(BcrsDev& M);
+int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v);
 int bench_copy(long long nbytes, int reps, double* ms);
 int build_tiles(Problem& P);
 int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,

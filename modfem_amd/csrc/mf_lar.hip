@@ -30,6 +30,8 @@ using namespace mf;
 #define NO_PRECON 0
 #define BLOCK_JACOBI 1
 #define BLOCK_GS 2
+#define MULTI_ILU 4
+#define BLOCK_ILU 5
 #define LAC_STORAGE_UNDEFINED -1
 #define LAC_STORAGE_CRS 2
 #define LAC_STORAGE_BCRS 3
@@ -165,12 +167,13 @@ int allocate_bcrs(int nb, const int* Nroffbl, int* const* L_offbl) {
 }
 
 void free_bcrs_precon(BcrsDev& B) {
-  void* ptrs[] = {B.d_diag, B.d_lu, B.d_piv, B.d_lev_rows[0], B.d_lev_rows[1]};
+  void* ptrs[] = {B.d_diag, B.d_lu, B.d_piv, B.d_ilu, B.d_lev_rows[0], B.d_lev_rows[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   B.d_diag = nullptr;
   B.d_lu = nullptr;
   B.d_piv = nullptr;
+  B.d_ilu = nullptr;
   for (int d = 0; d < 2; d++) {
     B.d_lev_rows[d] = nullptr;
     B.lev_ptr[d].clear();
@@ -436,16 +439,22 @@ int lar_free_SM_and_LV(int Matrix_id) {
 }
 
 // lar_allocate_preconditioner (las_crs_intf.c:658-700): Jacobi or none;
-// BCRS (las_bcrs_intf.c:630-711): block Jacobi or block Gauss-Seidel with the
-// diagonal blocks' LU factors, the Gauss-Seidel level schedules built here
+// BCRS (las_bcrs_intf.c:630-1016): block Jacobi or block Gauss-Seidel with
+// the diagonal blocks' LU factors, or block ILU(0) (MULTI_ILU / BLOCK_ILU with
+// Second_arg = k = 0) on the matrix pattern, which must be structurally
+// symmetric; the level schedules of the sweeps are built here
 int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
-  (void)Second_arg;
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
   if (M->B) {
-    if (Precon != NO_PRECON && Precon != BLOCK_JACOBI && Precon != BLOCK_GS) {
-      set_error("lar_allocate_preconditioner: BCRS storage offers NO_PRECON, BLOCK_JACOBI and "
-                "BLOCK_GS");
+    const bool ilu = Precon == MULTI_ILU || Precon == BLOCK_ILU;
+    if (Precon != NO_PRECON && Precon != BLOCK_JACOBI && Precon != BLOCK_GS && !ilu) {
+      set_error("lar_allocate_preconditioner: BCRS storage offers NO_PRECON, BLOCK_JACOBI, "
+                "BLOCK_GS, MULTI_ILU and BLOCK_ILU");
+      return -1;
+    }
+    if (ilu && Second_arg != 0) {
+      set_error("lar_allocate_preconditioner: ILU(k) is available for k = 0 only");
       return -1;
     }
     BcrsDev& B = *M->B;
@@ -463,11 +472,27 @@ int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
         return -1;
       }
     }
+    if (ilu)  // every stored block (k, i) needs its mirror (i, k)
+      for (int k = 0; k < B.nb; k++)
+        for (int s = M->rp[k]; s < M->rp[k + 1]; s++) {
+          const int i = M->ci[s];
+          bool found = false;
+          for (int t = M->rp[i]; t < M->rp[i + 1] && !found; t++) found = M->ci[t] == k;
+          if (!found) {
+            set_error("lar_allocate_preconditioner: block ILU needs a structurally symmetric "
+                      "pattern");
+            return -1;
+          }
+        }
     MF_HIP(hipMalloc(&B.d_diag, std::max(B.nb, 1) * sizeof(int)));
-    MF_HIP(hipMalloc(&B.d_lu, std::max(B.nb, 1) * BLS * BLS * sizeof(double)));
-    MF_HIP(hipMalloc(&B.d_piv, std::max(B.nb, 1) * BLS * sizeof(int)));
     MF_HIP(hipMemcpy(B.d_diag, M->diag.data(), B.nb * sizeof(int), hipMemcpyHostToDevice));
-    if (Precon == BLOCK_GS)
+    if (ilu) {
+      MF_HIP(hipMalloc(&B.d_ilu, std::max<size_t>(B.nblk, 1) * BLS * BLS * sizeof(double)));
+    } else {
+      MF_HIP(hipMalloc(&B.d_lu, std::max(B.nb, 1) * BLS * BLS * sizeof(double)));
+      MF_HIP(hipMalloc(&B.d_piv, std::max(B.nb, 1) * BLS * sizeof(int)));
+    }
+    if (Precon == BLOCK_GS || ilu)
       for (int dir = 0; dir < 2; dir++) {
         std::vector<int> rows;
         bcrs_levels(B.nb, M->rp, M->ci, M->diag, dir, B.lev_ptr[dir], rows);
@@ -497,11 +522,11 @@ int lar_fill_preconditioner(int Matrix_id) {
   if (flush(*M)) return -2;
   if (M->B) {
     if (M->precon == NO_PRECON) return 0;
-    if (M->precon < 0 || !M->B->d_lu) {
+    if (M->precon < 0 || !(M->B->d_lu || M->B->d_ilu)) {
       set_error("lar_fill_preconditioner: preconditioner not allocated");
       return -1;
     }
-    if (bcrs_fill_lu(*M->B)) return -1;
+    if (M->B->d_ilu ? bcrs_ilu_factor(*M->B) : bcrs_fill_lu(*M->B)) return -1;
     M->filled = true;
     return 0;
   }
@@ -612,11 +637,24 @@ void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero, i
   (void)hipFree(tmp);
 }
 
-// lar_perform_rhsub: the CRS body is empty in the reference
-// (las_crs_intf.c:1019-1057); kept as a no-op with a diagnostic
+// lar_perform_rhsub: V = (LU)^-1 B (B null: the stored rhs) with the BCRS
+// block ILU(0) factors (las_bcrs_intf.c:3319-3510, variant 0: forward
+// reduction and back substitution in the reference's row order, bitwise its
+// result).  The CRS body is empty in the reference (las_crs_intf.c:1019-1057):
+// a no-op with a diagnostic there
 void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B) {
-  (void)Matrix_id; (void)Ndof; (void)V; (void)B;
-  set_error("lar_perform_rhsub: no ILU for this storage");
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M || !M->B || !M->B->d_ilu || !M->filled || Ndof != M->ndof()) {
+    set_error("lar_perform_rhsub: needs a BCRS matrix with filled ILU(0) factors");
+    fprintf(stderr, "lar_perform_rhsub: needs a BCRS matrix with filled ILU(0) factors\n");
+    return;
+  }
+  if (flush(*M)) return;
+  Staged sb, sv;
+  if (B && stage_in(B, Ndof, sb, true)) return;
+  if (stage_in(V, Ndof, sv, false)) return;
+  if (bcrs_ilu_solve(*M->B, B ? sb.d : M->B->d_rhs, sv.d)) return;
+  stage_out(V, sv, Ndof);
 }
 
 // storage type (2 CRS, 3 BCRS), block size, rows of blocks and stored blocks
@@ -634,7 +672,9 @@ int mf_lar_get_sizes(int Matrix_id, int* storage, int* block_size, int* nrblocks
 
 // host copies of the BCRS diagonal-block LU factors (16 per block row,
 // column-major, unit lower part below the diagonal) and 0-based pivots (4 per
-// block row: row j was interchanged with row piv[j])
+// block row: row j was interchanged with row piv[j]); for block ILU(0) the
+// factors of every stored block (16 each: L blocks below the diagonal, U
+// blocks above, inverted U diagonal blocks) and no pivots
 int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
@@ -643,6 +683,11 @@ int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv) {
     return -1;
   }
   const BcrsDev& B = *M->B;
+  if (B.d_ilu) {  // ILU(0): the factors of every stored block
+    if (lu && B.nblk)
+      MF_HIP(hipMemcpy(lu, B.d_ilu, B.nblk * BLS * BLS * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+  }
   if (lu) MF_HIP(hipMemcpy(lu, B.d_lu, (size_t)B.nb * BLS * BLS * sizeof(double), hipMemcpyDeviceToHost));
   if (piv) MF_HIP(hipMemcpy(piv, B.d_piv, (size_t)B.nb * BLS * sizeof(int), hipMemcpyDeviceToHost));
   return 0;

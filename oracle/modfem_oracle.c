@@ -1669,3 +1669,133 @@ void orc_bcrs_bj_gs(int nb, const int *row_ptr, const int *col_ind, const int *d
   }
   free(tmp);
 }
+
+/* inverse of a column-major 4x4 block from its orc_lu4 factors, the
+   unblocked algorithm of LAPACK's dgetri: inv(U) column by column (dtrti2:
+   a_jj = 1/a_jj, then the column above it by the inverted leading triangle
+   (dtrmv, skipping zero entries) scaled by -a_jj), then for j = 3..0 the
+   column of L moved out and inv(A)(:, j) -= inv(A)(:, j+1:) L(j+1:, j)
+   (dgemv), then the column interchanges in reverse order */
+void orc_inv4(double *a, const int *ipiv) {
+  for (int j = 0; j < 4; j++) {
+    a[j + 4 * j] = 1.0 / a[j + 4 * j];
+    const double ajj = -a[j + 4 * j];
+    double *x = a + 4 * j;
+    for (int jj = 0; jj < j; jj++)
+      if (x[jj] != 0.0) {
+        const double t = x[jj];
+        for (int ii = 0; ii < jj; ii++) x[ii] = x[ii] + t * a[ii + 4 * jj];
+        x[jj] = x[jj] * a[jj + 4 * jj];
+      }
+    for (int ii = 0; ii < j; ii++) x[ii] = ajj * x[ii];
+  }
+  double work[4];
+  for (int j = 3; j >= 0; j--) {
+    for (int i = j + 1; i < 4; i++) {
+      work[i] = a[i + 4 * j];
+      a[i + 4 * j] = 0.0;
+    }
+    for (int jj = j + 1; jj < 4; jj++) {
+      const double t = -1.0 * work[jj];
+      for (int i = 0; i < 4; i++) a[i + 4 * j] = a[i + 4 * j] + t * a[i + 4 * jj];
+    }
+  }
+  for (int j = 2; j >= 0; j--) {
+    const int jp = ipiv[j];
+    if (jp != j)
+      for (int i = 0; i < 4; i++) {
+        const double t = a[i + 4 * j]; a[i + 4 * j] = a[i + 4 * jp]; a[i + 4 * jp] = t;
+      }
+  }
+}
+
+/* block ILU(0) on the BCRS pattern (lar_fill_preconditioner_bcrs, the
+   active variant 2, las_bcrs_intf.c:2181-2640), one thread: for every pivot
+   row i in order, W = inv(a_ii) (dgetrf + dgetri); for every block a_ik of
+   row i's upper part, the mirrored block a_ki of row k becomes a_ki W
+   (sums from 0 over the inner index), and every a_kj present in row k with j
+   in row i's upper part gets a_kj - a_ki a_ij (dgemm with alpha -1, beta 1:
+   for each column, inner index outer, rows inner); finally a_ii = W.  The
+   pattern must be structurally symmetric.  ilu: in the values, out the
+   factors.  Returns 0, or -1 for a missing mirrored block, or j+1 for a
+   zero pivot in a diagonal block */
+int orc_bcrs_ilu0(int nb, const int *rp, const int *ci, const int *diag, double *ilu) {
+  for (int i = 0; i < nb; i++) {
+    if (rp[i] >= rp[i + 1]) continue;
+    double *aii = ilu + 16 * (size_t)diag[i];
+    double w[16];
+    int piv[4];
+    memcpy(w, aii, sizeof w);
+    const int info = orc_lu4(w, piv);
+    if (info) return info;
+    orc_inv4(w, piv);
+    for (int s = diag[i] + 1; s < rp[i + 1]; s++) {
+      const int k = ci[s];
+      int ski = -1;
+      for (int t = rp[k]; t < rp[k + 1]; t++)
+        if (ci[t] == i) { ski = t; break; }
+      if (ski < 0) return -1;
+      double *aki = ilu + 16 * (size_t)ski;
+      double prod[16];
+      for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+          double d = 0.0;
+          for (int l = 0; l < 4; l++) d += aki[r + 4 * l] * w[l + 4 * c];
+          prod[r + 4 * c] = d;
+        }
+      memcpy(aki, prod, sizeof prod);
+      for (int sij = diag[i] + 1; sij < rp[i + 1]; sij++) {
+        const int j = ci[sij];
+        int skj = -1;
+        for (int t = ski + 1; t < rp[k + 1]; t++)
+          if (ci[t] == j) { skj = t; break; }
+        if (skj < 0) continue;
+        const double *aij = ilu + 16 * (size_t)sij;
+        double *akj = ilu + 16 * (size_t)skj;
+        for (int c = 0; c < 4; c++)
+          for (int l = 0; l < 4; l++) {
+            const double t = -1.0 * aij[l + 4 * c];
+            for (int r = 0; r < 4; r++) akj[r + 4 * c] = akj[r + 4 * c] + t * aki[r + 4 * l];
+          }
+      }
+    }
+    memcpy(aii, w, sizeof w);
+  }
+  return 0;
+}
+
+/* v = (LU)^-1 b with the orc_bcrs_ilu0 factors (lar_perform_rhsub_bcrs,
+   variant 0, las_bcrs_intf.c:3349-3510): forward, rows in order (rows with
+   no stored blocks skipped), v_i = b_i - sum over the lower blocks of
+   L_ij v_j (block by block, column outer, row inner); backward, rows in
+   reverse, t = v_i - sum over the upper blocks of U_ij v_j, then v_i =
+   inv(a_ii) t summed from 0 (column outer) */
+void orc_bcrs_rhsub(int nb, const int *rp, const int *ci, const int *diag, const double *ilu,
+                    const double *b, double *v) {
+  for (int i = 0; i < nb; i++) {
+    if (rp[i] >= rp[i + 1]) continue;
+    double t[4];
+    for (int r = 0; r < 4; r++) t[r] = b[4 * i + r];
+    for (int s = rp[i]; s < diag[i]; s++) {
+      const double *a = ilu + 16 * (size_t)s;
+      const double *x = v + 4 * (size_t)ci[s];
+      for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) t[r] -= a[r + 4 * c] * x[c];
+    }
+    for (int r = 0; r < 4; r++) v[4 * i + r] = t[r];
+  }
+  for (int i = nb - 1; i >= 0; i--) {
+    double t[4];
+    for (int r = 0; r < 4; r++) t[r] = v[4 * i + r];
+    for (int s = diag[i] + 1; s < rp[i + 1]; s++) {
+      const double *a = ilu + 16 * (size_t)s;
+      const double *x = v + 4 * (size_t)ci[s];
+      for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 4; r++) t[r] -= a[r + 4 * c] * x[c];
+    }
+    const double *d = ilu + 16 * (size_t)diag[i];
+    for (int r = 0; r < 4; r++) v[4 * i + r] = 0.0;
+    for (int c = 0; c < 4; c++)
+      for (int r = 0; r < 4; r++) v[4 * i + r] += d[r + 4 * c] * t[c];
+  }
+}

@@ -185,6 +185,10 @@ void orc_lu4_solve(const double *lu, const int *ipiv, double *b);
 void orc_bcrs_bj_gs(int nb, const int *row_ptr, const int *col_ind, const int *diag_ptr,
                     const double *val, const double *lu, const int *ipiv, int gs, int use_rhs,
                     int nr_prec, const double *b, double *v);
+void orc_inv4(double *a, const int *ipiv);
+int orc_bcrs_ilu0(int nb, const int *rp, const int *ci, const int *diag, double *ilu);
+void orc_bcrs_rhsub(int nb, const int *rp, const int *ci, const int *diag, const double *ilu,
+                    const double *b, double *v);
 
 /* BLAS level-1/2 restatements (netlib reference algorithms) */
 void orc_drotg(double *a, double *b, double *c, double *s);

@@ -65,6 +65,9 @@ def oracle():
             "orc_lu4": (I, [V, V]),
             "orc_lu4_solve": (None, [V, V, V]),
             "orc_bcrs_bj_gs": (None, [I, V, V, V, V, V, V, I, I, I, V, V]),
+            "orc_inv4": (None, [V, V]),
+            "orc_bcrs_ilu0": (I, [I, V, V, V, V]),
+            "orc_bcrs_rhsub": (None, [I, V, V, V, V, V, V]),
             "orc_set_threads": (None, [I]),
         }
         for k, (r, a) in sig.items():

@@ -1,9 +1,10 @@
 """BCRS storage with 4x4 blocks through the storage seam (include/modfem_lar.h)
 on the device, against the oracle (itself bit-exact with the reference's
-lsd_mkb/lad_bcrs/las_bcrs_intf.c for pattern, assembly and residual,
+lsd_mkb/lad_bcrs/las_bcrs_intf.c for pattern, assembly and residual, and
+pinned by the ILU(0) pattern identity for the factors,
 tests/test_oracle_bcrs.py): pattern, assembly with and without tables,
 residual, diagonal-block LU factors, block Jacobi and block Gauss-Seidel
-sweeps are all bit-exact."""
+sweeps, block ILU(0) factors and solves are all bit-exact."""
 import ctypes
 import os
 import sys
@@ -18,7 +19,7 @@ from oracle_lib import P, bcrs_assemble, bcrs_diag_lu, bcrs_vector_problem, orac
 pytestmark = pytest.mark.gpu
 
 LAC_STORAGE_BCRS = 3
-BLOCK_JACOBI, BLOCK_GS = 1, 2
+BLOCK_JACOBI, BLOCK_GS, MULTI_ILU, BLOCK_ILU = 1, 2, 4, 5
 
 
 def allocate(L, nn, nb_lists, storage=LAC_STORAGE_BCRS, block_size=4):
@@ -140,8 +141,9 @@ def test_errors():
     assert mid >= 0
     st = ctypes.c_int()
     assert L.mf_lar_get_sizes(mid, ctypes.byref(st), None, None, None) == 0 and st.value == 3
-    # ILU is not offered
-    assert L.lar_allocate_preconditioner(mid, 5, 0) < 0
+    # ILU(k) for k > 0 and additive Schwarz are not offered
+    assert L.lar_allocate_preconditioner(mid, 5, 1) < 0
+    assert L.lar_allocate_preconditioner(mid, 3, 0) < 0
     # all-zero matrix: singular diagonal blocks are reported
     assert L.lar_allocate_preconditioner(mid, BLOCK_JACOBI, 0) == nn
     assert L.lar_fill_preconditioner(mid) < 0
@@ -151,4 +153,31 @@ def test_errors():
     far = np.array([0, nn - 1], np.int32)
     assert nn - 1 not in nb_lists[0]
     assert L.lar_fill_assembly_table_int_ent(mid, 2, P(far), None, P(tab)) < 0
+    free(L, mid)
+
+
+@pytest.mark.parametrize("shape,precon", [((5, 4, 3), MULTI_ILU), ((10, 9, 6), BLOCK_ILU)])
+def test_block_ilu0_factors_and_solves_bitexact(shape, precon):
+    L, mid, nn, rp, ci, val, rhs = build(*shape, 13, True)
+    assert L.lar_allocate_preconditioner(mid, precon, 0) == nn, L.mf_last_error()
+    assert L.lar_fill_preconditioner(mid) == 0, L.mf_last_error()
+    diag = np.array([next(s for s in range(rp[r], rp[r + 1]) if ci[s] == r) for r in range(nn)],
+                    np.int32)
+    ilu0 = val.copy()
+    o = oracle()
+    assert o.orc_bcrs_ilu0(nn, P(rp), P(ci), P(diag), P(ilu0)) == 0
+    ilu = np.zeros(16 * len(ci))
+    assert L.mf_lar_get_block_precon(mid, P(ilu), None) == 0
+    assert np.array_equal(ilu, ilu0)
+    b = np.random.default_rng(3).uniform(-1, 1, 4 * nn)
+    for bb in (b, None):
+        v, v0 = np.full(4 * nn, 7.0), np.zeros(4 * nn)
+        L.lar_perform_rhsub(mid, 4 * nn, P(v), P(bb) if bb is not None else None)
+        o.orc_bcrs_rhsub(nn, P(rp), P(ci), P(diag), P(ilu0), P(bb if bb is not None else rhs), P(v0))
+        assert np.array_equal(v, v0)
+    # V and B may be the same array
+    w = b.copy()
+    L.lar_perform_rhsub(mid, 4 * nn, P(w), P(w))
+    o.orc_bcrs_rhsub(nn, P(rp), P(ci), P(diag), P(ilu0), P(b), P(v0))
+    assert np.array_equal(w, v0)
     free(L, mid)

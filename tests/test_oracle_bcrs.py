@@ -191,3 +191,105 @@ def test_sweep_count_rule():
     for _ in range(2):
         o.orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu), P(piv), 1, 1, 1, P(rhs), P(x11))
     assert np.array_equal(x3, x11)
+
+
+def test_inv4_inverts():
+    """orc_inv4 after orc_lu4 (LAPACK dgetrf + dgetri's algorithm): A inv(A)
+    = I to rounding; exact for permutation and power-of-two diagonal
+    matrices"""
+    o = oracle()
+    rng = np.random.default_rng(8)
+    for t in range(200):
+        A = rng.uniform(-1, 1, (4, 4))
+        if t % 4 == 0:
+            A[0, 0] = 0.0
+        a = np.ascontiguousarray(A.T.ravel())
+        piv = np.zeros(4, np.int32)
+        assert o.orc_lu4(P(a), P(piv)) == 0
+        o.orc_inv4(P(a), P(piv))
+        Ai = a.reshape(4, 4).T
+        cond = np.linalg.cond(A)
+        assert np.abs(A @ Ai - np.eye(4)).max() <= 1e-15 * 40 * cond
+    for perm in ([1, 0, 3, 2], [3, 2, 1, 0], [2, 0, 3, 1]):
+        A = np.eye(4)[perm] * 2.0
+        a = np.ascontiguousarray(A.T.ravel())
+        piv = np.zeros(4, np.int32)
+        o.orc_lu4(P(a), P(piv))
+        o.orc_inv4(P(a), P(piv))
+        assert np.array_equal(a.reshape(4, 4).T, A.T / 4.0)
+    A = np.diag([2.0, -0.5, 8.0, 0.25])
+    a = np.ascontiguousarray(A.T.ravel())
+    piv = np.zeros(4, np.int32)
+    o.orc_lu4(P(a), P(piv))
+    o.orc_inv4(P(a), P(piv))
+    assert np.array_equal(a.reshape(4, 4).T, np.diag(1.0 / np.diag(A)))
+
+
+def ilu_dense(nn, rp, ci, diag, ilu):
+    """dense unit-lower L and upper U from the factors (U's diagonal blocks
+    are the inverses of the stored ones)"""
+    Lm, U = np.eye(4 * nn), np.zeros((4 * nn, 4 * nn))
+    for r in range(nn):
+        for s in range(rp[r], rp[r + 1]):
+            c = ci[s]
+            blk = ilu[16 * s:16 * s + 16].reshape(4, 4).T
+            if c < r:
+                Lm[4 * r:4 * r + 4, 4 * c:4 * c + 4] = blk
+            elif c > r:
+                U[4 * r:4 * r + 4, 4 * c:4 * c + 4] = blk
+            else:
+                U[4 * r:4 * r + 4, 4 * c:4 * c + 4] = np.linalg.inv(blk)
+    return Lm, U
+
+
+def test_ilu0_reproduces_the_matrix_on_its_pattern():
+    """block ILU(0): (L U)_kj = A_kj for every stored block (k, j), within
+    rounding; L U differs from A off the pattern (fill is dropped)"""
+    nn, rp, ci, diag, val, lu, piv, rhs, A = bj_gs_setup(17)
+    ilu = val.copy()
+    assert oracle().orc_bcrs_ilu0(nn, P(rp), P(ci), P(diag), P(ilu)) == 0
+    Lm, U = ilu_dense(nn, rp, ci, diag, ilu)
+    LU = Lm @ U
+    mask = np.zeros((4 * nn, 4 * nn), bool)
+    for r in range(nn):
+        for s in range(rp[r], rp[r + 1]):
+            mask[4 * r:4 * r + 4, 4 * ci[s]:4 * ci[s] + 4] = True
+    assert np.abs((LU - A)[mask]).max() <= 1e-12 * np.abs(A).max()
+    assert np.abs((LU - A)[~mask]).max() > 1e-6  # fill-in exists and is dropped
+    # the solves apply (L U)^-1
+    b = np.random.default_rng(5).uniform(-1, 1, 4 * nn)
+    v = np.zeros(4 * nn)
+    oracle().orc_bcrs_rhsub(nn, P(rp), P(ci), P(diag), P(ilu), P(b), P(v))
+    ref = np.linalg.solve(LU, b)
+    assert np.abs(v - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_ilu0_of_a_block_tridiagonal_matrix_is_exact():
+    """a chain of blocks has no fill: ILU(0) is the exact block LU and one
+    forward/backward solve gives A^-1 b"""
+    nn = 40
+    rng = np.random.default_rng(6)
+    nb_lists = [[j for j in (i - 1, i + 1) if 0 <= j < nn] for i in range(nn)]
+    ptr = np.zeros(nn + 1, np.int32)
+    ptr[1:] = np.cumsum([len(x) for x in nb_lists])
+    ind = np.array([v for x in nb_lists for v in x], np.int32)
+    o = oracle()
+    rp = np.zeros(nn + 1, np.int32)
+    nsm = o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), None)
+    ci = np.zeros(nsm, np.int32)
+    o.orc_bcrs_pattern(nn, P(ptr), P(ind), P(rp), P(ci))
+    val = rng.uniform(-1, 1, 16 * nsm)
+    diag = np.array([next(s for s in range(rp[r], rp[r + 1]) if ci[s] == r) for r in range(nn)], np.int32)
+    for r in range(nn):
+        val[16 * diag[r]:16 * diag[r] + 16] += (np.eye(4) * 5.0).ravel()
+    A = np.zeros((4 * nn, 4 * nn))
+    for r in range(nn):
+        for s in range(rp[r], rp[r + 1]):
+            A[4 * r:4 * r + 4, 4 * ci[s]:4 * ci[s] + 4] = val[16 * s:16 * s + 16].reshape(4, 4).T
+    ilu = val.copy()
+    assert o.orc_bcrs_ilu0(nn, P(rp), P(ci), P(diag), P(ilu)) == 0
+    b = rng.uniform(-1, 1, 4 * nn)
+    v = np.zeros(4 * nn)
+    o.orc_bcrs_rhsub(nn, P(rp), P(ci), P(diag), P(ilu), P(b), P(v))
+    x = np.linalg.solve(A, b)
+    assert np.abs(v - x).max() <= 1e-13 * np.abs(x).max()
