@@ -75,6 +75,19 @@ void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero,
    the reference (las_crs_intf.c:1019-1057): records an error there */
 void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B);
 
+/* Restarted GMRES(krylbas) on a BCRS matrix with the preconditioner
+   allocated and filled on it (lsr_mkb_core_gmres, lsd_mkb/lsd_mkb_core/
+   lss_mkb_core.c:2317, over the storage operations as lsr_mkb_core_compreres
+   applies them, :1915-1990): none, block Jacobi / block Gauss-Seidel with
+   nr_prec sweeps per application (smoothing), MULTI_ILU (one ILU correction
+   as smoothing) or BLOCK_ILU ((LU)^-1 of the residual).  The rhs is the
+   stored one; x (host) is the initial guess unless ini_zero and receives the
+   solution.  tol: relative to the first preconditioned residual norm.
+   Returns 1 on convergence, 0 otherwise, <0 on error */
+int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol,
+                 int nr_prec, int ini_zero, double* x, int* iters,
+                 double* res_rel, double* resid);
+
 /* Inspection (no reference counterpart; lar_get_SM_and_LV_crs_from_crs /
    _from_bcrs, las_crs_intf.c:1063, las_bcrs_intf.c:3812, are the nearest):
    storage type, block size, rows of blocks and stored entries/blocks */

@@ -21,10 +21,9 @@ namespace {
 // V = (b ? b : 0) - A x (lar_compute_residual_bcrs, las_bcrs_intf.c:2787-2930),
 // one thread per unknown
 __global__ void k_bcrs_residual(int nb, const int* rp, const int* ci, const double* val,
-                                const double* x, const double* b, double* v) {
-
+                                const double* x, const double* b, double* v, const int* stop) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 4LL * nb) return;
+  if (i >= 4LL * nb || (stop && *stop)) return;
   const int r = (int)(i >> 2), k = (int)(i & 3);
   double acc = 0.0;
   if (x)
@@ -42,7 +41,6 @@ __global__ void k_bcrs_residual(int nb, const int* rp, const int* ci, const doub
 // LAPACK dgetf2 (lar_fill_preconditioner_bcrs calls dgetrf, :1119-1148)
 __global__ void k_bcrs_lu4(int nb, const int* diag, const double* val, double* lu, int* piv,
                            int* bad) {
-
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nb) return;
   double a[16];
@@ -77,7 +75,6 @@ __global__ void k_bcrs_lu4(int nb, const int* diag, const double* val, double* l
 
 // solve with the factors as LAPACK dgetrs ('N') does
 __device__ __forceinline__ void lu4_solve(const double* lu, const int* ip, double* b) {
-
   for (int j = 0; j < 4; j++)
     if (ip[j] != j) {
       const double t = b[j];
@@ -101,7 +98,6 @@ __device__ __forceinline__ void bcrs_row(int r, int dir, const int* rp, const in
                                          const int* diag, const double* val, const double* lu,
                                          const int* piv, const double* b, const double* v,
                                          const double* passed, double* out) {
-
   double vloc[4] = {0.0, 0.0, 0.0, 0.0};
   const int p0 = dir == 0 ? rp[r] : diag[r] + 1, p1 = dir == 0 ? diag[r] : rp[r + 1];
   const int o0 = dir == 0 ? diag[r] + 1 : rp[r], o1 = dir == 0 ? rp[r + 1] : diag[r];
@@ -126,9 +122,9 @@ __device__ __forceinline__ void bcrs_row(int r, int dir, const int* rp, const in
 // block Jacobi sweep: every row reads the old v
 __global__ void k_bcrs_bj(int nb, int dir, const int* rp, const int* ci, const int* diag,
                           const double* val, const double* lu, const int* piv, const double* b,
-                          const double* v, double* out) {
+                          const double* v, double* out, const int* stop) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nb) return;
+  if (r >= nb || (stop && *stop)) return;
   bcrs_row(r, dir, rp, ci, diag, val, lu, piv, b, v, v, out);
 }
 
@@ -136,9 +132,10 @@ __global__ void k_bcrs_bj(int nb, int dir, const int* rp, const int* ci, const i
 // output (rows of earlier levels are final there)
 __global__ void k_bcrs_gs_level(int cnt, const int* rows, int dir, const int* rp, const int* ci,
                                 const int* diag, const double* val, const double* lu,
-                                const int* piv, const double* b, const double* v, double* out) {
+                                const int* piv, const double* b, const double* v, double* out,
+                                const int* stop) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
+  if (t >= cnt || (stop && *stop)) return;
   bcrs_row(rows[t], dir, rp, ci, diag, val, lu, piv, b, v, out, out);
 }
 
@@ -269,9 +266,10 @@ __global__ void k_bcrs_ilu_level(int cnt, const int* rows, const int* rp, const 
 // forward reduction of the ILU solve (lar_perform_rhsub_bcrs, variant 0,
 // las_bcrs_intf.c:3349-3430), the rows of one level: v_i = b_i - L_i. v
 __global__ void k_bcrs_fwd_level(int cnt, const int* rows, const int* rp, const int* ci,
-                                 const int* diag, const double* ilu, const double* b, double* v) {
+                                 const int* diag, const double* ilu, const double* b, double* v,
+                                 const int* stop) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
+  if (t >= cnt || (stop && *stop)) return;
   const int i = rows[t];
   double x[4];
   for (int r = 0; r < 4; r++) x[r] = b[4 * (size_t)i + r];
@@ -287,9 +285,9 @@ __global__ void k_bcrs_fwd_level(int cnt, const int* rows, const int* rp, const 
 // back substitution (:3432-3510), the rows of one level:
 // v_i = inv(a_ii) (v_i - U_i. v)
 __global__ void k_bcrs_bwd_level(int cnt, const int* rows, const int* rp, const int* ci,
-                                 const int* diag, const double* ilu, double* v) {
+                                 const int* diag, const double* ilu, double* v, const int* stop) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
+  if (t >= cnt || (stop && *stop)) return;
   const int i = rows[t];
   double x[4];
   for (int r = 0; r < 4; r++) x[r] = v[4 * (size_t)i + r];
@@ -304,6 +302,33 @@ __global__ void k_bcrs_bwd_level(int cnt, const int* rows, const int* rp, const 
   for (int c = 0; c < 4; c++)
     for (int r = 0; r < 4; r++) o[r] = __dadd_rn(o[r], __dmul_rn(d[r + 4 * c], x[c]));
   for (int r = 0; r < 4; r++) v[4 * (size_t)i + r] = o[r];
+}
+
+// vector steps of the preconditioned residual (lsr_mkb_core_compreres,
+// lss_mkb_core.c:1915-1990): y = s_j x (s null: copy), y = a - b (negated
+// when neg), y = -y, y += z
+__global__ void k_bcrs_vscale(long long n, const double* x, const double* s, int j, double* y,
+                              const int* stop) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (stop && *stop)) return;
+  y[i] = s ? __dmul_rn(s[j], x[i]) : x[i];
+}
+__global__ void k_bcrs_vsub(long long n, const double* a, const double* b, double* y, int neg,
+                            const int* stop) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (stop && *stop)) return;
+  const double d = __dsub_rn(a[i], b[i]);
+  y[i] = neg ? -d : d;
+}
+__global__ void k_bcrs_vneg(long long n, double* y, const int* stop) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (stop && *stop)) return;
+  y[i] = -y[i];
+}
+__global__ void k_bcrs_vadd(long long n, const double* z, double* y, const int* stop) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (stop && *stop)) return;
+  y[i] = __dadd_rn(y[i], z[i]);
 }
 
 }  // namespace
@@ -330,11 +355,12 @@ int bcrs_levels(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
   return nlev;
 }
 
-int bcrs_residual(const BcrsDev& M, const double* x, const double* b, double* v) {
+int bcrs_residual(const BcrsDev& M, const double* x, const double* b, double* v,
+                  const int* stop) {
   const long long n = 4LL * M.nb;
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_bcrs_residual, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream(),
-                     M.nb, M.d_rp, M.d_ci, M.d_val, x, b, v);
+                     M.nb, M.d_rp, M.d_ci, M.d_val, x, b, v, stop);
   MF_HIP(hipGetLastError());
   return 0;
 }
@@ -357,29 +383,32 @@ int bcrs_fill_lu(BcrsDev& M) {
   return 0;
 }
 
-int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, double* tmp) {
+int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, double* tmp,
+                const int* stop, bool sync) {
   const int loops = nr_prec % 2 == 0 ? nr_prec / 2 : nr_prec / 2 + 1;
   const int dirs = nr_prec % 2 == 0 ? 2 : 1;
-  const size_t bytes = 4 * (size_t)M.nb * sizeof(double);
+  const long long n = 4LL * M.nb;
   if (M.nb == 0) return 0;
+  const unsigned gv = (unsigned)((n + 255) / 256);
   for (int it = 0; it < loops; it++)
     for (int dir = 0; dir < dirs; dir++) {
       if (!gs) {
         hipLaunchKernelGGL(k_bcrs_bj, dim3((M.nb + 255) / 256), dim3(256), 0, stream(), M.nb, dir,
-                           M.d_rp, M.d_ci, M.d_diag, M.d_val, M.d_lu, M.d_piv, b, v, tmp);
+                           M.d_rp, M.d_ci, M.d_diag, M.d_val, M.d_lu, M.d_piv, b, v, tmp, stop);
       } else {
         const std::vector<int>& lp = M.lev_ptr[dir];
         for (size_t l = 0; l + 1 < lp.size(); l++) {
           const int cnt = lp[l + 1] - lp[l];
           hipLaunchKernelGGL(k_bcrs_gs_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
                              M.d_lev_rows[dir] + lp[l], dir, M.d_rp, M.d_ci, M.d_diag, M.d_val,
-                             M.d_lu, M.d_piv, b, v, tmp);
+                             M.d_lu, M.d_piv, b, v, tmp, stop);
         }
       }
+      hipLaunchKernelGGL(k_bcrs_vscale, dim3(gv), dim3(256), 0, stream(), n, tmp, nullptr, 0, v,
+                         stop);
       MF_HIP(hipGetLastError());
-      MF_HIP(hipMemcpyAsync(v, tmp, bytes, hipMemcpyDeviceToDevice, stream()));
     }
-  MF_HIP(hipStreamSynchronize(stream()));
+  if (sync) MF_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
@@ -408,22 +437,93 @@ int bcrs_ilu_factor(BcrsDev& M) {
   return 0;
 }
 
-int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v) {
+int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v, const int* stop) {
   if (M.nb == 0) return 0;
   const std::vector<int>& lf = M.lev_ptr[0];
   for (size_t l = 0; l + 1 < lf.size(); l++) {
     const int cnt = lf[l + 1] - lf[l];
     hipLaunchKernelGGL(k_bcrs_fwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                       M.d_lev_rows[0] + lf[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, b, v);
+                       M.d_lev_rows[0] + lf[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, b, v, stop);
   }
   const std::vector<int>& lb = M.lev_ptr[1];
   for (size_t l = 0; l + 1 < lb.size(); l++) {
     const int cnt = lb[l + 1] - lb[l];
     hipLaunchKernelGGL(k_bcrs_bwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                       M.d_lev_rows[1] + lb[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, v);
+                       M.d_lev_rows[1] + lb[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, v, stop);
   }
   MF_HIP(hipGetLastError());
   return 0;
+}
+
+// The preconditioned operator of GMRES on a BCRS matrix, as
+// lsr_mkb_core_compreres (lss_mkb_core.c:1915-1990) forms it from the
+// storage operations: none, V = b - A x; BLOCK_ILU, V = (LU)^-1 (b - A x);
+// block Jacobi / Gauss-Seidel (nr_prec sweeps) and MULTI_ILU through
+// smoothing, V = smooth(x; b) - x, with smooth one ILU correction
+// x + (LU)^-1 (b - A x) for MULTI_ILU (lsr_mkb_core_smooth, :1621-1645).
+// The Arnoldi product takes b = 0 and negates the result (dscal by -1):
+// w = -V(X), X = s_j V_j.
+struct BcrsOp : ExtOp {
+  BcrsDev* B = nullptr;
+  int kind = 0, nr_prec = 1;
+  double *t1 = nullptr, *t2 = nullptr, *t3 = nullptr, *xs = nullptr;  // 4 nb each
+  ~BcrsOp() override {
+    for (double* p : {t1, t2, t3, xs})
+      if (p) (void)hipFree(p);
+  }
+  long long n() const { return 4LL * B->nb; }
+  unsigned gv() const { return (unsigned)((n() + 255) / 256); }
+  int alloc() {
+    for (double** p : {&t1, &t2, &t3, &xs})
+      if (!*p) MF_HIP(hipMalloc(p, std::max<long long>(n(), 1) * sizeof(double)));
+    return 0;
+  }
+  // out = (neg ? -1 : 1) M^-1 (b - A x), b null: zero
+  int precon_residual(const double* b, const double* x, double* out, const int* stop, int neg) {
+    hipStream_t s = stream();
+    if (kind == 0 || kind == 5) {
+      if (kind == 0) {
+        if (bcrs_residual(*B, x, b, out, stop)) return -2;
+      } else {
+        if (bcrs_residual(*B, x, b, t1, stop)) return -2;
+        if (bcrs_ilu_solve(*B, t1, out, stop)) return -2;
+      }
+      if (neg) hipLaunchKernelGGL(k_bcrs_vneg, dim3(gv()), dim3(256), 0, s, n(), out, stop);
+      MF_HIP(hipGetLastError());
+      return 0;
+    }
+    hipLaunchKernelGGL(k_bcrs_vscale, dim3(gv()), dim3(256), 0, s, n(), x, nullptr, 0, t2, stop);
+    if (kind == 4) {
+      if (bcrs_residual(*B, t2, b, t1, stop)) return -2;
+      if (bcrs_ilu_solve(*B, t1, t3, stop)) return -2;
+      hipLaunchKernelGGL(k_bcrs_vadd, dim3(gv()), dim3(256), 0, s, n(), t3, t2, stop);
+    } else if (bcrs_sweeps(*B, kind == 2, nr_prec, b, t2, t1, stop, false)) {
+      return -2;
+    }
+    hipLaunchKernelGGL(k_bcrs_vsub, dim3(gv()), dim3(256), 0, s, n(), t2, x, out, neg, stop);
+    MF_HIP(hipGetLastError());
+    return 0;
+  }
+  int residual(const double* b, const double* x, double* r) override {
+    return precon_residual(b, x, r, nullptr, 0);
+  }
+  int apply(const double* v, const double* scale, int j, double* w, const int* stop) override {
+    hipLaunchKernelGGL(k_bcrs_vscale, dim3(gv()), dim3(256), 0, stream(), n(), v, scale, j, xs,
+                       stop);
+    return precon_residual(nullptr, xs, w, stop, 1);
+  }
+};
+
+ExtOp* bcrs_make_op(BcrsDev* B, int kind, int nr_prec) {
+  BcrsOp* op = new BcrsOp();
+  op->B = B;
+  op->kind = kind;
+  op->nr_prec = nr_prec;
+  if (op->alloc()) {
+    delete op;
+    return nullptr;
+  }
+  return op;
 }
 
 }  // namespace mf

@@ -299,17 +299,24 @@ struct BcrsDev {
 int bcrs_levels(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
                 const std::vector<int>& diag, int dir, std::vector<int>& lev_ptr,
                 std::vector<int>& rows);
-// v = (b ? b : 0) - A x (x null: zero)
-int bcrs_residual(const BcrsDev& M, const double* x, const double* b, double* v);
+// v = (b ? b : 0) - A x (x null: zero); kernels skip their work once *stop
+// is set (stop may be null)
+int bcrs_residual(const BcrsDev& M, const double* x, const double* b, double* v,
+                  const int* stop = nullptr);
 // LU factors of the diagonal blocks (d_lu, d_piv allocated)
 int bcrs_fill_lu(BcrsDev& M);
 // block Jacobi (gs = 0) or Gauss-Seidel sweeps with the sweep-count rule of
 // the reference; tmp: 4 nb doubles
-int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, double* tmp);
+int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, double* tmp,
+                const int* stop = nullptr, bool sync = true);
 // block ILU(0) factors of the current values into d_ilu (levels of both
 // directions built), and v = (LU)^-1 b (v may alias b)
 int bcrs_ilu_factor(BcrsDev& M);
-int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v);
+int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v, const int* stop = nullptr);
+// GMRES operator on a BCRS matrix with preconditioner kind 0 (none), 1
+// (block Jacobi), 2 (block Gauss-Seidel), 4 (MULTI_ILU), 5 (BLOCK_ILU);
+// owned by the caller
+struct ExtOp* bcrs_make_op(BcrsDev* B, int kind, int nr_prec);
 int bench_copy(long long nbytes, int reps, double* ms);
 int build_tiles(Problem& P);
 int launch_rowop(Problem& P, int mode, int use_rhs, const double* d_x, const double* d_b,

@@ -49,6 +49,10 @@ struct LarMatrix {
   bool filled = false;  // BCRS: diagonal-block factors current
   int precon = -1;
   int ndof() const { return B ? BLS * B->nb : P->n; }
+  // BCRS GMRES: workspace (Krylov basis, solution) and the operator
+  std::unique_ptr<Problem> G;
+  std::unique_ptr<ExtOp> op;
+  int op_kind = -1, op_sweeps = -1;
 };
 
 std::vector<LarMatrix> g_mats;  // LIFO like itv_crs_matrices
@@ -164,6 +168,19 @@ int allocate_bcrs(int nb, const int* Nroffbl, int* const* L_offbl) {
   g_mats.push_back(std::move(M));
   if (flush(g_mats.back())) return -2;
   return (int)g_mats.size() - 1;
+}
+
+void free_gmres(LarMatrix& M) {
+  if (M.G) {
+    Problem& P = *M.G;
+    void* ptrs[] = {P.d_V, P.d_H, P.d_partials, P.d_stat, P.d_x};
+    for (void* q : ptrs)
+      if (q) (void)hipFree(q);
+    if (P.h_stat) (void)hipHostFree(P.h_stat);
+    M.G.reset();
+  }
+  M.op.reset();
+  M.op_kind = M.op_sweeps = -1;
 }
 
 void free_bcrs_precon(BcrsDev& B) {
@@ -423,6 +440,7 @@ int lar_free_SM_and_LV(int Matrix_id) {
   if (!M) return -3;
   if (M->B) {
     BcrsDev& B = *M->B;
+    free_gmres(*M);
     free_bcrs_precon(B);
     void* ptrs[] = {B.d_rp, B.d_ci, B.d_val, B.d_rhs};
     for (void* p : ptrs)
@@ -458,6 +476,8 @@ int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
       return -1;
     }
     BcrsDev& B = *M->B;
+    M->op.reset();
+    M->op_kind = -1;
     free_bcrs_precon(B);
     M->filled = false;
     M->precon = Precon;
@@ -544,6 +564,8 @@ int lar_free_preconditioner(int Matrix_id) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
   if (M->B) {
+    M->op.reset();
+    M->op_kind = -1;
     free_bcrs_precon(*M->B);
     M->filled = false;
     M->precon = -1;
@@ -655,6 +677,45 @@ void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B) {
   if (stage_in(V, Ndof, sv, false)) return;
   if (bcrs_ilu_solve(*M->B, B ? sb.d : M->B->d_rhs, sv.d)) return;
   stage_out(V, sv, Ndof);
+}
+
+// restarted GMRES on a BCRS matrix with its preconditioner (lsr_mkb_core_gmres,
+// lss_mkb_core.c:2317, over the storage operations as compreres applies them,
+// lss_mkb_core.c:1915-1990); b is the stored rhs, x host memory (the initial
+// guess unless ini_zero; the solution on return)
+int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol, int nr_prec, int ini_zero,
+                 double* x, int* iters, double* res_rel, double* resid) {
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M) return -3;
+  if (!M->B) {
+    set_error("mf_lar_gmres: needs a BCRS matrix (CRS problems solve with mf_gmres)");
+    return -1;
+  }
+  const int kind = M->precon < 0 ? NO_PRECON : M->precon;
+  if (kind != NO_PRECON && !M->filled) {
+    set_error("mf_lar_gmres: the preconditioner is not filled");
+    return -1;
+  }
+  if ((kind == BLOCK_JACOBI || kind == BLOCK_GS) && nr_prec < 1) {
+    set_error("mf_lar_gmres: nr_prec must be >= 1");
+    return -1;
+  }
+  if (flush(*M)) return -2;
+  BcrsDev& B = *M->B;
+  if (!M->op || M->op_kind != kind || M->op_sweeps != nr_prec) {
+    M->op.reset(bcrs_make_op(&B, kind, nr_prec));
+    if (!M->op) return -2;
+    M->op_kind = kind;
+    M->op_sweeps = nr_prec;
+  }
+  if (!M->G) {
+    M->G = std::make_unique<Problem>();
+    M->G->n = BLS * B.nb;
+    M->G->el_type = kPrism;
+  }
+  M->G->ext = M->op.get();
+  M->G->d_rhs = B.d_rhs;
+  return gmres_solve(*M->G, 0, krylbas, max_iter, tol, 0.0, ini_zero, x, iters, res_rel, resid);
 }
 
 // storage type (2 CRS, 3 BCRS), block size, rows of blocks and stored blocks

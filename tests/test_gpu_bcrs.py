@@ -181,3 +181,37 @@ def test_block_ilu0_factors_and_solves_bitexact(shape, precon):
     o.orc_bcrs_rhsub(nn, P(rp), P(ci), P(diag), P(ilu0), P(b), P(v0))
     assert np.array_equal(w, v0)
     free(L, mid)
+
+
+@pytest.mark.parametrize("precon,nr_prec", [(0, 1), (BLOCK_JACOBI, 1), (BLOCK_JACOBI, 2),
+                                            (BLOCK_GS, 1), (BLOCK_GS, 2), (MULTI_ILU, 1),
+                                            (BLOCK_ILU, 1)])
+def test_gmres_on_bcrs_solves_the_system(precon, nr_prec):
+    """GMRES(30) over the storage operations reaches the dense solution to
+    the solver tolerance; preconditioning cuts the iteration count"""
+    L, mid, nn, rp, ci, val, rhs = build(6, 5, 4, 3, True, diag_shift=2.0)
+    A = np.zeros((4 * nn, 4 * nn))
+    for r in range(nn):
+        for s in range(rp[r], rp[r + 1]):
+            A[4 * r:4 * r + 4, 4 * ci[s]:4 * ci[s] + 4] = val[16 * s:16 * s + 16].reshape(4, 4).T
+    xs = np.linalg.solve(A, rhs)
+    if precon:
+        assert L.lar_allocate_preconditioner(mid, precon, 0) == nn
+        assert L.lar_fill_preconditioner(mid) == 0
+    x = np.zeros(4 * nn)
+    it, rr, res = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    assert L.mf_lar_gmres(mid, 30, 2000, 1e-10, nr_prec, 1, P(x), ctypes.byref(it), ctypes.byref(rr),
+                          ctypes.byref(res)) == 1, L.mf_last_error()
+    assert rr.value <= 1e-10
+    assert np.abs(x - xs).max() <= 1e-10 * 120 * np.abs(xs).max()  # tol x cond(A)
+    if precon in (BLOCK_GS, MULTI_ILU, BLOCK_ILU):
+        assert it.value < 60, it.value
+    # a given initial guess is used (ini_zero = 0)
+    x = 0.5 * xs
+    assert L.mf_lar_gmres(mid, 30, 2000, 1e-10, nr_prec, 0, P(x), ctypes.byref(it), ctypes.byref(rr),
+                          ctypes.byref(res)) == 1
+    assert np.abs(x - xs).max() <= 1e-10 * 120 * np.abs(xs).max()
+    if precon:
+        free(L, mid)
+    else:
+        assert L.lar_free_SM_and_LV(mid) == 0
