@@ -398,6 +398,190 @@ def bench_dg(args):
     print(json.dumps(out), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# BCRS workload: a 4-component reaction-diffusion system in 4x4 blocks
+# ---------------------------------------------------------------------------
+BCRS_COUPLING = np.array([[1.0, 0.2, 0.0, -0.1], [-0.2, 1.0, 0.1, 0.0],
+                          [0.0, -0.1, 1.0, 0.2], [0.1, 0.0, -0.2, 1.0]])
+
+
+def bcrs_system(mf, n, seed):
+    """element data of the BCRS workload: a jittered n^3 prism box, scalar
+    diffusion + reaction (c = 10) + unit source element matrices from the
+    device integration, coupled over 4 components as kron(K_e, C) with a
+    fixed non-symmetric C; node blocks, neighbour lists flattened"""
+    mesh = mf.Mesh.box_prism(n, n, n, seed=seed, jitter=0.2)
+    a = mesh.arrays()
+    en = np.ascontiguousarray(a["el_nodes"], np.int32)
+    nn = a["coor"].shape[0]
+    prob = mf.Problem.from_mesh(mesh)
+    cc = np.zeros(mf.NCOEF)
+    cc[[0, 4, 8]] = 1.0
+    cc[mf.C_C] = 10.0
+    cc[mf.C_S] = 1.0
+    prob.set_coefficients(mf.coeff_indicator("axx", "ayy", "azz", "c", "s"), mf.COEF_UNIFORM,
+                          coef_const=cc)
+    em = prob.element_matrices()
+    prob.free()
+    iu, iv = np.nonzero(~np.eye(6, dtype=bool))
+    u = en[:, iu].ravel().astype(np.int64)
+    v = en[:, iv].ravel().astype(np.int64)
+    key = np.unique(u * nn + v)
+    nb_ind = (key % nn).astype(np.int32)
+    nb_cnt = np.bincount((key // nn).astype(np.int64), minlength=nn).astype(np.int32)
+    return mesh, en, nn, em, nb_ind, nb_cnt
+
+
+def bcrs_allocate(L, nn, nb_ind, nb_cnt, alloc):
+    """allocate through the lar_ seam (or the reference's own BCRS module)"""
+    import ctypes
+    ptr = np.zeros(nn + 1, np.int64)
+    ptr[1:] = np.cumsum(nb_cnt)
+    base = nb_ind.ctypes.data
+    ptrs = (ctypes.c_void_p * nn)(*(base + 4 * ptr[:-1]).tolist())
+    return alloc(nn, ctypes.cast(ptrs, ctypes.c_void_p))
+
+
+def bcrs_assemble(en, em, assemble_el, chunk=16384):
+    """kron(K_e, C) element by element, column-major 24x24, rhs kron(F_e, w)"""
+    C = BCRS_COUPLING
+    w = np.array([1.0, 0.5, -0.25, 1.0])
+    nel = en.shape[0]
+    for e0 in range(0, nel, chunk):
+        e1 = min(nel, e0 + chunk)
+        Ks = em[e0:e1, :36].reshape(-1, 6, 6).transpose(0, 2, 1)  # [row][col]
+        K = np.einsum("eij,ab->eiajb", Ks, C).reshape(-1, 24, 24)
+        Kc = np.ascontiguousarray(K.transpose(0, 2, 1)).reshape(-1, 576)  # column-major
+        F = np.ascontiguousarray(np.einsum("ei,a->eia", em[e0:e1, 36:], w).reshape(-1, 24))
+        for q in range(e1 - e0):
+            assemble_el(en[e0 + q], Kc[q], F[q])
+
+
+def bench_bcrs(args):
+    """BCRS 4x4-block storage through the lar_ seam: device residual, block
+    Jacobi / Gauss-Seidel sweeps, block ILU(0) factorisation and solves, and
+    GMRES(30) with each preconditioner to 1e-8; the reference's own compiled
+    residual (lar_compute_residual_bcrs) on the host beside it"""
+    import ctypes
+    import modfem_amd as mf
+    mf.init()
+    L = mf.lib()
+    P = lambda v: None if v is None else v.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    t0 = time.perf_counter()
+    mesh, en, nn, em, nb_ind, nb_cnt = bcrs_system(mf, args.bcrs_n, args.seed)
+    mid = bcrs_allocate(L, nn, nb_ind, nb_cnt,
+                        lambda nb, lists: L.lar_allocate_SM_and_LV(3, 1, 4 * nb, 24, nb, 4, None, None,
+                                                                   P(nb_cnt), lists))
+    if mid < 0:
+        raise RuntimeError(L.mf_last_error().decode())
+    tab = np.zeros(42, np.int32)
+
+    def asm(l_bl, K, F):
+        L.lar_fill_assembly_table_int_ent(mid, 6, P(l_bl), None, P(tab))
+        L.lar_assemble_SM_and_LV_with_table(mid, 0, 6, P(tab), P(K), P(F), None)
+    bcrs_assemble(en, em, asm)
+    t_setup = time.perf_counter() - t0
+    nblk = ctypes.c_longlong()
+    L.mf_lar_get_sizes(mid, None, None, None, ctypes.byref(nblk))
+    nblk = nblk.value
+    reps = max(args.steps, 1) * 10
+    ms = ctypes.c_double()
+
+    def bench(what):
+        rc = L.mf_lar_bench(mid, what, reps, ctypes.byref(ms))
+        if rc < 0:
+            raise RuntimeError(L.mf_last_error().decode())
+        return ms.value
+    res_ms = bench(0)
+    res_bytes = 132.0 * nblk + 4.0 * (nn + 1) + 96.0 * nn
+    rec = {"blocks": nn, "stored_blocks": nblk, "dofs": 4 * nn, "setup_s": t_setup,
+           "residual_ms": res_ms, "residual_GBs": res_bytes / (res_ms * 1e-3) / 1e9}
+    x = np.zeros(4 * nn)
+    it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    for name, precon, nr_prec in [("bj", 1, 1), ("gs", 2, 2), ("ilu", 5, 1)]:
+        assert L.lar_allocate_preconditioner(mid, precon, 0) == nn, L.mf_last_error()
+        t0 = time.perf_counter()
+        if L.lar_fill_preconditioner(mid) != 0:
+            raise RuntimeError(L.mf_last_error().decode())
+        rec[name + "_fill_ms"] = (time.perf_counter() - t0) * 1e3
+        if precon == 5:
+            rec["ilu_factor_ms"] = bench(2)
+            rec["ilu_solve_ms"] = bench(3)
+        else:
+            rec[name + "_sweep_ms"] = bench(1)
+        t0 = time.perf_counter()
+        conv = L.mf_lar_gmres(mid, 30, args.bcrs_max_iter, 1e-8, nr_prec, 1, P(x), ctypes.byref(it),
+                              ctypes.byref(rr), ctypes.byref(rs))
+        dt = time.perf_counter() - t0
+        rec[name + "_gmres"] = {"converged": conv == 1, "iters": it.value, "res_rel": rr.value,
+                                "s": dt, "ms_per_iter": dt * 1e3 / max(it.value, 1)}
+        print(json.dumps({"bcrs_" + name: rec[name + "_gmres"]}), file=sys.stderr, flush=True)
+        assert L.lar_free_preconditioner(mid) == 0
+    # the residual vector of the solution, as a check of the last solve
+    r = np.zeros(4 * nn)
+    L.lar_compute_residual(mid, 1, 0, 4 * nn, P(x), None, P(r))
+    rhs = np.zeros(4 * nn)
+    L.mf_lar_get_system(mid, None, None, None, P(rhs))
+    rec["true_res_rel"] = float(np.linalg.norm(r) / np.linalg.norm(rhs))
+    L.lar_free_SM_and_LV(mid)
+    out = {
+        "metric": "BCRS 4x4 residual v = b - A x (lar_compute_residual)", "value": rec["residual_GBs"],
+        "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded jittered box, device element matrices, fixed coupling)",
+        "config": {"workload": "4-component reaction-diffusion in BCRS 4x4 blocks, %d^3 hex box x 2 "
+                               "prisms (%d node blocks, %d stored blocks)" % (args.bcrs_n, nn, nblk),
+                   "levels": rec},
+        "roofline": {"bound": "hbm", "achieved": rec["residual_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": rec["residual_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_bcrs_residual"},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = bcrs_cpu_baseline(en, em, nn, nb_ind, nb_cnt, res_bytes)
+    print(json.dumps(out), flush=True)
+
+
+def bcrs_cpu_baseline(en, em, nn, nb_ind, nb_cnt, res_bytes):
+    """the reference's own lar_compute_residual_bcrs (compiled from its
+    sources into oracle/_ref) on one host thread, on the same system"""
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import ref, ref_available
+    if not ref_available("libref_bcrs.so"):
+        return None
+    R = ref("libref_bcrs.so")
+    gomp = ctypes.CDLL("libgomp.so.1")
+    threads = gomp.omp_get_max_threads()
+    gomp.omp_set_num_threads(1)
+    R.lar_allocate_SM_and_LV_bcrs.restype = ctypes.c_int
+    R.lar_allocate_SM_and_LV_bcrs.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 2
+    R.lar_assemble_SM_and_LV_bcrs.restype = ctypes.c_int
+    R.lar_assemble_SM_and_LV_bcrs.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 5
+    R.lar_compute_residual_bcrs.restype = None
+    R.lar_compute_residual_bcrs.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3
+    R.lar_free_SM_and_LV_bcrs.argtypes = [ctypes.c_int]
+    P = lambda v: v.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    rid = bcrs_allocate(R, nn, nb_ind, nb_cnt,
+                        lambda nb, lists: R.lar_allocate_SM_and_LV_bcrs(4 * nb, 24, 4, P(nb_cnt), lists))
+
+    def asm(l_bl, K, F):
+        R.lar_assemble_SM_and_LV_bcrs(rid, 0, 6, P(l_bl), None, P(K), P(F), None)
+    bcrs_assemble(en, em, asm)
+    x = np.random.default_rng(1).uniform(-1, 1, 4 * nn)
+    v = np.zeros(4 * nn)
+    reps, t = 0, 0.0
+    while t < 10.0 and reps < 200:
+        t0 = time.perf_counter()
+        R.lar_compute_residual_bcrs(rid, 1, 0, 4 * nn, P(x), None, P(v))
+        t += time.perf_counter() - t0
+        reps += 1
+    R.lar_free_SM_and_LV_bcrs(rid)
+    gomp.omp_set_num_threads(threads)
+    return {"value": res_bytes / (t / reps) / 1e9, "unit": "GB/s", "cores": 1, "kind": "reference",
+            "sample": "%d calls of the reference's lar_compute_residual_bcrs (las_bcrs_intf.c, built "
+                      "from its sources, OpenMP at one thread) on the full system" % reps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -418,7 +602,9 @@ def main():
     ap.add_argument("--dg-solve", type=int, nargs="*", default=[101, 202],
                     help="DG levels solved to 1e-8 with block Gauss-Seidel (2 sweeps); p=3 at "
                          "96^3 takes about two minutes, so it is left out by default")
-    ap.add_argument("--workload", choices=["box128", "batch", "serve", "dg"], default="box128",
+    ap.add_argument("--bcrs-n", type=int, default=48, help="BCRS workload: hexes per box edge")
+    ap.add_argument("--bcrs-max-iter", type=int, default=3000)
+    ap.add_argument("--workload", choices=["box128", "batch", "serve", "dg", "bcrs"], default="box128",
                     help="box128: the single large problem (default line); batch: many "
                          "independent problems in one pipeline; serve: a request stream; "
                          "dg: the DG NIPG Poisson case, p = 1, 2, 3")
@@ -436,6 +622,8 @@ def main():
         return bench_batch(args)
     if args.workload == "dg":
         return bench_dg(args)
+    if args.workload == "bcrs":
+        return bench_bcrs(args)
     if args.workload == "serve":
         return bench_serve(args)
     import modfem_amd as mf

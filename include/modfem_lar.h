@@ -88,6 +88,12 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol,
                  int nr_prec, int ini_zero, double* x, int* iters,
                  double* res_rel, double* resid);
 
+/* HIP-event time (ms, averaged over reps) of one BCRS operation on
+   device-resident vectors: what 0 residual, 1 one forward sweep of the block
+   Jacobi / Gauss-Seidel preconditioner, 2 the ILU(0) factorisation, 3 one
+   ILU(0) solve (no reference counterpart; measurement only) */
+int mf_lar_bench(int Matrix_id, int what, int reps, double* ms);
+
 /* Inspection (no reference counterpart; lar_get_SM_and_LV_crs_from_crs /
    _from_bcrs, las_crs_intf.c:1063, las_bcrs_intf.c:3812, are the nearest):
    storage type, block size, rows of blocks and stored entries/blocks */

@@ -158,6 +158,7 @@ def _declare(L):
         "mf_lar_get_sizes": (I, [I, V, V, V, V]),
         "mf_lar_get_block_precon": (I, [I, V, V]),
         "mf_lar_gmres": (I, [I, I, I, D, I, I, V, V, V, V]),
+        "mf_lar_bench": (I, [I, I, I, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

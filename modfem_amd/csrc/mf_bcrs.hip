@@ -304,6 +304,131 @@ __global__ void k_bcrs_bwd_level(int cnt, const int* rows, const int* rp, const 
   for (int r = 0; r < 4; r++) v[4 * (size_t)i + r] = o[r];
 }
 
+// One wave per row for the sweeps of the level schedules, when no row holds
+// more than kWaveBlocks blocks: the wave forms every product a_kl y_l of the
+// row in parallel (coalesced block loads) into LDS, then lane k subtracts the
+// products of component k in the sequential order (blocks ascending, l
+// ascending) -- the same roundings in the same order as one thread per row.
+//   mode 0: Gauss-Seidel row in direction dir (passed side from `passed`,
+//           other side from v, + b, then the diagonal block's LU solve) -> out
+//   mode 1: ILU forward row, y = b - L y (lower blocks from v, in place)
+//   mode 2: ILU backward row, y = inv(a_ii) (y - U y) (upper blocks from v)
+constexpr int kWaveBlocks = 64;
+
+__global__ void __launch_bounds__(64) k_bcrs_rows_wave(
+    int cnt, const int* rows, int mode, int dir, const int* rp, const int* ci, const int* diag,
+    const double* val, const double* lu, const int* piv, const double* b, const double* v,
+    const double* passed, double* out, const int* stop) {
+  __shared__ double prod[kWaveBlocks * 16];
+  __shared__ double acc[4];
+  const int t = blockIdx.x, lane = threadIdx.x;
+  if (t >= cnt || (stop && *stop)) return;  // uniform over the workgroup
+  const int r = rows[t];
+  int p0, p1, o0 = 0, o1 = 0;
+  if (mode == 0) {
+    p0 = dir == 0 ? rp[r] : diag[r] + 1;
+    p1 = dir == 0 ? diag[r] : rp[r + 1];
+    o0 = dir == 0 ? diag[r] + 1 : rp[r];
+    o1 = dir == 0 ? rp[r + 1] : diag[r];
+  } else if (mode == 1) {
+    p0 = rp[r];
+    p1 = diag[r];
+  } else {
+    p0 = diag[r] + 1;
+    p1 = rp[r + 1];
+  }
+  const int np = p1 - p0, nbk = np + (o1 - o0);
+  for (int idx = lane; idx < 16 * nbk; idx += 64) {
+    const int q = idx >> 4, e = idx & 15;
+    const int s = q < np ? p0 + q : o0 + (q - np);
+    const double* src = q < np ? passed : v;
+    prod[idx] = __dmul_rn(val[16 * (size_t)s + e], src[4 * (size_t)ci[s] + (e >> 2)]);
+  }
+  __syncthreads();
+  if (lane < 4) {
+    const int k = lane;
+    double x = mode == 2 ? v[4 * (size_t)r + k] : (mode == 1 ? b[4 * (size_t)r + k] : 0.0);
+    for (int q = 0; q < nbk; q++)
+      for (int l = 0; l < 4; l++) x = __dsub_rn(x, prod[16 * q + k + 4 * l]);
+    if (mode == 0 && b) x = __dadd_rn(x, b[4 * (size_t)r + k]);
+    acc[k] = x;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double x[4] = {acc[0], acc[1], acc[2], acc[3]};
+    if (mode == 0) {
+      lu4_solve(lu + 16 * (size_t)r, piv + 4 * (size_t)r, x);
+      for (int k = 0; k < 4; k++) out[4 * (size_t)r + k] = x[k];
+    } else if (mode == 1) {
+      for (int k = 0; k < 4; k++) out[4 * (size_t)r + k] = x[k];
+    } else {
+      const double* d = lu + 16 * (size_t)diag[r];
+      double o[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int c = 0; c < 4; c++)
+        for (int k = 0; k < 4; k++) o[k] = __dadd_rn(o[k], __dmul_rn(d[k + 4 * c], x[c]));
+      for (int k = 0; k < 4; k++) out[4 * (size_t)r + k] = o[k];
+    }
+  }
+}
+
+// Block ILU(0) of the rows of one level, one wave per row: the row's blocks
+// in LDS; for each lower block a_ki in order, lanes 0..15 form a_ki inv(a_ii)
+// (one entry each, sums from 0), then the wave updates the blocks a_kj of the
+// row present in row i's upper part, one entry per lane (dgemm order: inner
+// index ascending); finally lane 0 inverts the diagonal block.  The same
+// operations in the same order per entry as k_bcrs_ilu_level.
+__global__ void __launch_bounds__(64) k_bcrs_ilu_wave(int cnt, const int* rows, const int* rp,
+                                                       const int* ci, const int* diag, double* ilu,
+                                                       int* bad) {
+  __shared__ double A[kWaveBlocks * 16];
+  __shared__ int C[kWaveBlocks];
+  const int t = blockIdx.x, lane = threadIdx.x;
+  if (t >= cnt) return;
+  const int k = rows[t];
+  const int r0 = rp[k], nbk = rp[k + 1] - r0, dk = diag[k] - r0;
+  double* row = ilu + 16 * (size_t)r0;
+  for (int idx = lane; idx < 16 * nbk; idx += 64) A[idx] = row[idx];
+  for (int q = lane; q < nbk; q += 64) C[q] = ci[r0 + q];
+  __syncthreads();
+  for (int q = 0; q < dk; q++) {
+    const int i = C[q];
+    const double* w = ilu + 16 * (size_t)diag[i];
+    double pv = 0.0;
+    if (lane < 16) {
+      const int rr = lane & 3, cc = lane >> 2;
+      for (int l = 0; l < 4; l++) pv = __dadd_rn(pv, __dmul_rn(A[16 * q + rr + 4 * l], w[l + 4 * cc]));
+    }
+    __syncthreads();
+    if (lane < 16) A[16 * q + lane] = pv;
+    __syncthreads();
+    const int u0 = diag[i] + 1, nu = rp[i + 1] - u0;
+    for (int idx = lane; idx < 16 * nu; idx += 64) {
+      const int jj = idx >> 4, e = idx & 15, rr = e & 3, cc = e >> 2;
+      const int j = ci[u0 + jj];
+      int pos = -1;
+      for (int z = q + 1; z < nbk && C[z] <= j; z++)
+        if (C[z] == j) pos = z;
+      if (pos < 0) continue;
+      const double* aij = ilu + 16 * (size_t)(u0 + jj);
+      double c4 = A[16 * pos + e];
+      for (int l = 0; l < 4; l++)
+        c4 = __dadd_rn(c4, __dmul_rn(-aij[l + 4 * cc], A[16 * q + rr + 4 * l]));
+      A[16 * pos + e] = c4;
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    double d[16];
+    int ip[4];
+    for (int z = 0; z < 16; z++) d[z] = A[16 * dk + z];
+    if (!lu4(d, ip)) bad[0] = 1;
+    inv4(d, ip);
+    for (int z = 0; z < 16; z++) A[16 * dk + z] = d[z];
+  }
+  __syncthreads();
+  for (int idx = lane; idx < 16 * nbk; idx += 64) row[idx] = A[idx];
+}
+
 // vector steps of the preconditioned residual (lsr_mkb_core_compreres,
 // lss_mkb_core.c:1915-1990): y = s_j x (s null: copy), y = a - b (negated
 // when neg), y = -y, y += z
@@ -399,9 +524,14 @@ int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, dou
         const std::vector<int>& lp = M.lev_ptr[dir];
         for (size_t l = 0; l + 1 < lp.size(); l++) {
           const int cnt = lp[l + 1] - lp[l];
-          hipLaunchKernelGGL(k_bcrs_gs_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                             M.d_lev_rows[dir] + lp[l], dir, M.d_rp, M.d_ci, M.d_diag, M.d_val,
-                             M.d_lu, M.d_piv, b, v, tmp, stop);
+          if (M.max_row <= kWaveBlocks)
+            hipLaunchKernelGGL(k_bcrs_rows_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
+                               M.d_lev_rows[dir] + lp[l], 0, dir, M.d_rp, M.d_ci, M.d_diag,
+                               M.d_val, M.d_lu, M.d_piv, b, v, tmp, tmp, stop);
+          else
+            hipLaunchKernelGGL(k_bcrs_gs_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
+                               M.d_lev_rows[dir] + lp[l], dir, M.d_rp, M.d_ci, M.d_diag, M.d_val,
+                               M.d_lu, M.d_piv, b, v, tmp, stop);
         }
       }
       hipLaunchKernelGGL(k_bcrs_vscale, dim3(gv), dim3(256), 0, stream(), n, tmp, nullptr, 0, v,
@@ -422,8 +552,12 @@ int bcrs_ilu_factor(BcrsDev& M) {
   const std::vector<int>& lp = M.lev_ptr[0];
   for (size_t l = 0; l + 1 < lp.size(); l++) {
     const int cnt = lp[l + 1] - lp[l];
-    hipLaunchKernelGGL(k_bcrs_ilu_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                       M.d_lev_rows[0] + lp[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, d_bad);
+    if (M.max_row <= kWaveBlocks)
+      hipLaunchKernelGGL(k_bcrs_ilu_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
+                         M.d_lev_rows[0] + lp[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, d_bad);
+    else
+      hipLaunchKernelGGL(k_bcrs_ilu_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
+                         M.d_lev_rows[0] + lp[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, d_bad);
   }
   MF_HIP(hipGetLastError());
   int bad = 0;
@@ -442,14 +576,24 @@ int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v, const int* stop
   const std::vector<int>& lf = M.lev_ptr[0];
   for (size_t l = 0; l + 1 < lf.size(); l++) {
     const int cnt = lf[l + 1] - lf[l];
-    hipLaunchKernelGGL(k_bcrs_fwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                       M.d_lev_rows[0] + lf[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, b, v, stop);
+    if (M.max_row <= kWaveBlocks)
+      hipLaunchKernelGGL(k_bcrs_rows_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
+                         M.d_lev_rows[0] + lf[l], 1, 0, M.d_rp, M.d_ci, M.d_diag, M.d_ilu,
+                         nullptr, nullptr, b, v, v, v, stop);
+    else
+      hipLaunchKernelGGL(k_bcrs_fwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
+                         M.d_lev_rows[0] + lf[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, b, v, stop);
   }
   const std::vector<int>& lb = M.lev_ptr[1];
   for (size_t l = 0; l + 1 < lb.size(); l++) {
     const int cnt = lb[l + 1] - lb[l];
-    hipLaunchKernelGGL(k_bcrs_bwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                       M.d_lev_rows[1] + lb[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, v, stop);
+    if (M.max_row <= kWaveBlocks)
+      hipLaunchKernelGGL(k_bcrs_rows_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
+                         M.d_lev_rows[1] + lb[l], 2, 0, M.d_rp, M.d_ci, M.d_diag, M.d_ilu,
+                         M.d_ilu, nullptr, nullptr, v, v, v, stop);
+    else
+      hipLaunchKernelGGL(k_bcrs_bwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
+                         M.d_lev_rows[1] + lb[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, v, stop);
   }
   MF_HIP(hipGetLastError());
   return 0;

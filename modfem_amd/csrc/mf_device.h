@@ -288,6 +288,7 @@ void gs_free(Problem& P);
 struct BcrsDev {
   int nb = 0;                     // block rows
   long long nblk = 0;             // stored blocks
+  int max_row = 0;                // most blocks in one row
   int *d_rp = nullptr, *d_ci = nullptr, *d_diag = nullptr;
   double *d_val = nullptr, *d_rhs = nullptr, *d_lu = nullptr;
   double* d_ilu = nullptr;        // block ILU(0) factors (16 per stored block)

@@ -155,6 +155,7 @@ int allocate_bcrs(int nb, const int* Nroffbl, int* const* L_offbl) {
   BcrsDev& B = *M.B;
   B.nb = nb;
   B.nblk = (long long)M.ci.size();
+  for (int r = 0; r < nb; r++) B.max_row = std::max(B.max_row, M.rp[r + 1] - M.rp[r]);
   MF_HIP(hipMalloc(&B.d_rp, M.rp.size() * sizeof(int)));
   MF_HIP(hipMalloc(&B.d_ci, std::max<size_t>(M.ci.size(), 1) * sizeof(int)));
   MF_HIP(hipMalloc(&B.d_val, std::max<size_t>(M.ci.size(), 1) * BLS * BLS * sizeof(double)));
@@ -716,6 +717,63 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol, int nr_pr
   M->G->ext = M->op.get();
   M->G->d_rhs = B.d_rhs;
   return gmres_solve(*M->G, 0, krylbas, max_iter, tol, 0.0, ini_zero, x, iters, res_rel, resid);
+}
+
+// HIP-event times on the library stream of one BCRS operation, averaged over
+// reps, on device-resident vectors: what 0 residual v = b - A x, 1 one
+// forward sweep of the allocated block Jacobi / Gauss-Seidel preconditioner,
+// 2 the ILU(0) factorisation, 3 one ILU(0) solve
+int mf_lar_bench(int Matrix_id, int what, int reps, double* ms) {
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M) return -3;
+  if (!M->B || reps < 1 || !ms || what < 0 || what > 3) {
+    set_error("mf_lar_bench: needs a BCRS matrix, reps >= 1 and what in 0..3");
+    return -1;
+  }
+  BcrsDev& B = *M->B;
+  const bool ilu = B.d_ilu != nullptr;
+  if ((what == 1 && (ilu || !B.d_lu || !M->filled)) || (what >= 2 && !ilu) ||
+      (what == 3 && !M->filled)) {
+    set_error("mf_lar_bench: the operation needs its (filled) preconditioner");
+    return -1;
+  }
+  if (flush(*M)) return -2;
+  const size_t n = (size_t)BLS * B.nb;
+  double *x = nullptr, *v = nullptr, *t = nullptr;
+  MF_HIP(hipMalloc(&x, std::max<size_t>(n, 1) * sizeof(double)));
+  MF_HIP(hipMalloc(&v, std::max<size_t>(n, 1) * sizeof(double)));
+  MF_HIP(hipMalloc(&t, std::max<size_t>(n, 1) * sizeof(double)));
+  hipStream_t s = stream();
+  MF_HIP(hipMemcpyAsync(x, B.d_rhs, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  MF_HIP(hipMemcpyAsync(v, B.d_rhs, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  hipEvent_t e0, e1;
+  MF_HIP(hipEventCreate(&e0));
+  MF_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  auto once = [&]() -> int {
+    switch (what) {
+      case 0: return bcrs_residual(B, x, B.d_rhs, v);
+      case 1: return bcrs_sweeps(B, M->precon == BLOCK_GS, 1, B.d_rhs, v, t, nullptr, false);
+      case 2: return bcrs_ilu_factor(B);
+      default: return bcrs_ilu_solve(B, B.d_rhs, v);
+    }
+  };
+  rc = once();  // warm-up
+  if (rc == 0) {
+    MF_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps && rc == 0; r++) rc = once();
+    MF_HIP(hipEventRecord(e1, s));
+    MF_HIP(hipEventSynchronize(e1));
+    float t_ms = 0;
+    MF_HIP(hipEventElapsedTime(&t_ms, e0, e1));
+    *ms = t_ms / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(x);
+  (void)hipFree(v);
+  (void)hipFree(t);
+  return rc;
 }
 
 // storage type (2 CRS, 3 BCRS), block size, rows of blocks and stored blocks
