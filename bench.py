@@ -498,8 +498,10 @@ def bench_bcrs(args):
            "residual_ms": res_ms, "residual_GBs": res_bytes / (res_ms * 1e-3) / 1e9}
     x = np.zeros(4 * nn)
     it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
-    for name, precon, nr_prec in [("bj", 1, 1), ("gs", 2, 2), ("ilu", 5, 1)]:
+    for name, precon, nr_prec in [("bj", 1, 1), ("gs", 2, 2), ("gs_colour", 2, 2), ("ilu", 5, 1)]:
         assert L.lar_allocate_preconditioner(mid, precon, 0) == nn, L.mf_last_error()
+        if precon == 2:
+            rec[name + "_colours"] = L.mf_lar_set_gs_order(mid, 1 if name == "gs_colour" else 0)
         t0 = time.perf_counter()
         if L.lar_fill_preconditioner(mid) != 0:
             raise RuntimeError(L.mf_last_error().decode())

@@ -88,6 +88,13 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol,
                  int nr_prec, int ini_zero, double* x, int* iters,
                  double* res_rel, double* resid);
 
+/* Order of the BCRS block Gauss-Seidel sweeps (no reference counterpart):
+   0 the reference's row order (default, bitwise the reference), 1 colour
+   classes of a greedy colouring, swept in place one class per launch
+   (classes ascending forward, descending backward; the same iteration
+   count rule).  Returns the number of colours (1 for order 0) */
+int mf_lar_set_gs_order(int Matrix_id, int order);
+
 /* HIP-event time (ms, averaged over reps) of one BCRS operation on
    device-resident vectors: what 0 residual, 1 one forward sweep of the block
    Jacobi / Gauss-Seidel preconditioner, 2 the ILU(0) factorisation, 3 one

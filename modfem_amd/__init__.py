@@ -159,6 +159,7 @@ def _declare(L):
         "mf_lar_get_block_precon": (I, [I, V, V]),
         "mf_lar_gmres": (I, [I, I, I, D, I, I, V, V, V, V]),
         "mf_lar_bench": (I, [I, I, I, V]),
+        "mf_lar_set_gs_order": (I, [I, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -313,6 +313,9 @@ __global__ void k_bcrs_bwd_level(int cnt, const int* rows, const int* rp, const 
 //           other side from v, + b, then the diagonal block's LU solve) -> out
 //   mode 1: ILU forward row, y = b - L y (lower blocks from v, in place)
 //   mode 2: ILU backward row, y = inv(a_ii) (y - U y) (upper blocks from v)
+//   mode 3: Gauss-Seidel row of a colour class, in place: every off-diagonal
+//           block reads v (rows of earlier colours already updated), lower
+//           blocks first, + b, LU solve -> v
 constexpr int kWaveBlocks = 64;
 
 __global__ void __launch_bounds__(64) k_bcrs_rows_wave(
@@ -325,7 +328,12 @@ __global__ void __launch_bounds__(64) k_bcrs_rows_wave(
   if (t >= cnt || (stop && *stop)) return;  // uniform over the workgroup
   const int r = rows[t];
   int p0, p1, o0 = 0, o1 = 0;
-  if (mode == 0) {
+  if (mode == 3) {
+    p0 = rp[r];
+    p1 = diag[r];
+    o0 = diag[r] + 1;
+    o1 = rp[r + 1];
+  } else if (mode == 0) {
     p0 = dir == 0 ? rp[r] : diag[r] + 1;
     p1 = dir == 0 ? diag[r] : rp[r + 1];
     o0 = dir == 0 ? diag[r] + 1 : rp[r];
@@ -350,13 +358,13 @@ __global__ void __launch_bounds__(64) k_bcrs_rows_wave(
     double x = mode == 2 ? v[4 * (size_t)r + k] : (mode == 1 ? b[4 * (size_t)r + k] : 0.0);
     for (int q = 0; q < nbk; q++)
       for (int l = 0; l < 4; l++) x = __dsub_rn(x, prod[16 * q + k + 4 * l]);
-    if (mode == 0 && b) x = __dadd_rn(x, b[4 * (size_t)r + k]);
+    if ((mode == 0 || mode == 3) && b) x = __dadd_rn(x, b[4 * (size_t)r + k]);
     acc[k] = x;
   }
   __syncthreads();
   if (lane == 0) {
     double x[4] = {acc[0], acc[1], acc[2], acc[3]};
-    if (mode == 0) {
+    if (mode == 0 || mode == 3) {
       lu4_solve(lu + 16 * (size_t)r, piv + 4 * (size_t)r, x);
       for (int k = 0; k < 4; k++) out[4 * (size_t)r + k] = x[k];
     } else if (mode == 1) {
@@ -480,6 +488,28 @@ int bcrs_levels(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
   return nlev;
 }
 
+int bcrs_colors(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
+                std::vector<int>& col_ptr, std::vector<int>& rows) {
+  std::vector<int> color(nb, -1), mark;
+  int ncol = 0;
+  for (int r = 0; r < nb; r++) {
+    mark.assign(ncol + 1, 0);
+    for (int s = rp[r]; s < rp[r + 1]; s++)
+      if (color[ci[s]] >= 0) mark[color[ci[s]]] = 1;
+    int c = 0;
+    while (c < ncol && mark[c]) c++;
+    color[r] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  col_ptr.assign(ncol + 1, 0);
+  for (int r = 0; r < nb; r++) col_ptr[color[r] + 1]++;
+  for (int c = 0; c < ncol; c++) col_ptr[c + 1] += col_ptr[c];
+  std::vector<int> pos(col_ptr.begin(), col_ptr.end() - 1);
+  rows.assign(nb, 0);
+  for (int r = 0; r < nb; r++) rows[pos[color[r]]++] = r;
+  return ncol;
+}
+
 int bcrs_residual(const BcrsDev& M, const double* x, const double* b, double* v,
                   const int* stop) {
   const long long n = 4LL * M.nb;
@@ -515,6 +545,21 @@ int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, dou
   const long long n = 4LL * M.nb;
   if (M.nb == 0) return 0;
   const unsigned gv = (unsigned)((n + 255) / 256);
+  if (gs && M.gs_order == 1) {  // colour classes, in place
+    const int ncol = (int)M.col_ptr.size() - 1;
+    for (int it = 0; it < loops; it++)
+      for (int dir = 0; dir < dirs; dir++)
+        for (int q = 0; q < ncol; q++) {
+          const int c = dir == 0 ? q : ncol - 1 - q;
+          const int cnt = M.col_ptr[c + 1] - M.col_ptr[c];
+          hipLaunchKernelGGL(k_bcrs_rows_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
+                             M.d_col_rows + M.col_ptr[c], 3, 0, M.d_rp, M.d_ci, M.d_diag, M.d_val,
+                             M.d_lu, M.d_piv, b, v, v, v, stop);
+        }
+    MF_HIP(hipGetLastError());
+    if (sync) MF_HIP(hipStreamSynchronize(stream()));
+    return 0;
+  }
   for (int it = 0; it < loops; it++)
     for (int dir = 0; dir < dirs; dir++) {
       if (!gs) {

@@ -295,7 +295,15 @@ struct BcrsDev {
   int* d_piv = nullptr;
   std::vector<int> lev_ptr[2];
   int* d_lev_rows[2] = {nullptr, nullptr};
+  // Gauss-Seidel order: 0 the reference's row order (level schedules), 1
+  // colour classes of a greedy colouring (rows of a class share no block)
+  int gs_order = 0;
+  std::vector<int> col_ptr;
+  int* d_col_rows = nullptr;
 };
+// greedy colouring of the block graph in row order; rows grouped by colour
+int bcrs_colors(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
+                std::vector<int>& col_ptr, std::vector<int>& rows);
 // level schedule of a sweep in direction dir; returns the number of levels
 int bcrs_levels(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
                 const std::vector<int>& diag, int dir, std::vector<int>& lev_ptr,

@@ -443,7 +443,7 @@ int lar_free_SM_and_LV(int Matrix_id) {
     BcrsDev& B = *M->B;
     free_gmres(*M);
     free_bcrs_precon(B);
-    void* ptrs[] = {B.d_rp, B.d_ci, B.d_val, B.d_rhs};
+    void* ptrs[] = {B.d_rp, B.d_ci, B.d_val, B.d_rhs, B.d_col_rows};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     g_mats.pop_back();
@@ -774,6 +774,32 @@ int mf_lar_bench(int Matrix_id, int what, int reps, double* ms) {
   (void)hipFree(v);
   (void)hipFree(t);
   return rc;
+}
+
+// order of the BCRS block Gauss-Seidel sweeps: 0 the reference's row order
+// (default; bitwise the reference), 1 colour classes (rows of a class share
+// no block; a sweep is one launch per class, classes ascending forward and
+// descending backward).  Returns the number of colours (1 for order 0)
+int mf_lar_set_gs_order(int Matrix_id, int order) {
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M) return -3;
+  if (!M->B || (order != 0 && order != 1)) {
+    set_error("mf_lar_set_gs_order: needs a BCRS matrix and order 0 or 1");
+    return -1;
+  }
+  BcrsDev& B = *M->B;
+  if (order == 1 && B.max_row > 64) {
+    set_error("mf_lar_set_gs_order: colour-ordered sweeps need rows of at most 64 blocks");
+    return -1;
+  }
+  if (order == 1 && !B.d_col_rows && B.nb > 0) {
+    std::vector<int> rows;
+    bcrs_colors(B.nb, M->rp, M->ci, B.col_ptr, rows);
+    MF_HIP(hipMalloc(&B.d_col_rows, B.nb * sizeof(int)));
+    MF_HIP(hipMemcpy(B.d_col_rows, rows.data(), B.nb * sizeof(int), hipMemcpyHostToDevice));
+  }
+  B.gs_order = order;
+  return order == 1 ? (int)B.col_ptr.size() - 1 : 1;
 }
 
 // storage type (2 CRS, 3 BCRS), block size, rows of blocks and stored blocks

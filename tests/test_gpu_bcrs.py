@@ -215,3 +215,77 @@ def test_gmres_on_bcrs_solves_the_system(precon, nr_prec):
         free(L, mid)
     else:
         assert L.lar_free_SM_and_LV(mid) == 0
+
+
+def greedy_colors(nn, rp, ci):
+    color = -np.ones(nn, int)
+    for r in range(nn):
+        used = {color[c] for c in ci[rp[r]:rp[r + 1]] if color[c] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        color[r] = c
+    return color
+
+
+def test_colour_ordered_gauss_seidel_is_the_permuted_sweep():
+    """colour-ordered block GS equals the oracle's sequential block GS on the
+    matrix renumbered colour by colour (summation order aside)"""
+    L, mid, nn, rp, ci, val, rhs = build(5, 4, 3, 29, True, diag_shift=6.0)
+    assert L.lar_allocate_preconditioner(mid, BLOCK_GS, 0) == nn
+    assert L.lar_fill_preconditioner(mid) == 0
+    color = greedy_colors(nn, rp, ci)
+    assert L.mf_lar_set_gs_order(mid, 1) == color.max() + 1
+    order = np.argsort(color, kind="stable")  # new position -> old row
+    newpos = np.empty(nn, int)
+    newpos[order] = np.arange(nn)
+    # renumbered pattern and blocks
+    prp = np.zeros(nn + 1, np.int32)
+    pci, pval = [], []
+    for rn in range(nn):
+        r = order[rn]
+        ent = sorted((newpos[ci[s]], s) for s in range(rp[r], rp[r + 1]))
+        for c, s in ent:
+            pci.append(c)
+            pval.append(val[16 * s:16 * s + 16])
+        prp[rn + 1] = len(pci)
+    pci = np.array(pci, np.int32)
+    pval = np.concatenate(pval)
+    pdiag, plu, ppiv = bcrs_diag_lu(nn, prp, pci, pval)
+    rng = np.random.default_rng(8)
+    v_in = rng.uniform(-1, 1, 4 * nn)
+    perm4 = (4 * order[:, None] + np.arange(4)).ravel()
+    for nr_prec in (1, 2, 3):
+        v = v_in.copy()
+        L.lar_perform_BJ_or_GS_iterations(mid, 1, 0, nr_prec, 4 * nn, P(v), None)
+        vp = np.ascontiguousarray(v_in[perm4])
+        oracle().orc_bcrs_bj_gs(nn, P(prp), P(pci), P(pdiag), P(pval), P(plu), P(ppiv), 1, 1, nr_prec,
+                                P(np.ascontiguousarray(rhs[perm4])), P(vp))
+        assert np.abs(v[perm4] - vp).max() <= 1e-13 * np.abs(vp).max(), nr_prec
+    # back to the reference order: bitwise the oracle again
+    assert L.mf_lar_set_gs_order(mid, 0) == 1
+    diag, lu0, piv0 = bcrs_diag_lu(nn, rp, ci, val)
+    v, v0 = v_in.copy(), v_in.copy()
+    L.lar_perform_BJ_or_GS_iterations(mid, 1, 0, 2, 4 * nn, P(v), None)
+    oracle().orc_bcrs_bj_gs(nn, P(rp), P(ci), P(diag), P(val), P(lu0), P(piv0), 1, 1, 2, P(rhs), P(v0))
+    assert np.array_equal(v, v0)
+    free(L, mid)
+
+
+def test_gmres_with_colour_ordered_gauss_seidel():
+    L, mid, nn, rp, ci, val, rhs = build(6, 5, 4, 3, True, diag_shift=2.0)
+    A = np.zeros((4 * nn, 4 * nn))
+    for r in range(nn):
+        for s in range(rp[r], rp[r + 1]):
+            A[4 * r:4 * r + 4, 4 * ci[s]:4 * ci[s] + 4] = val[16 * s:16 * s + 16].reshape(4, 4).T
+    xs = np.linalg.solve(A, rhs)
+    assert L.lar_allocate_preconditioner(mid, BLOCK_GS, 0) == nn
+    assert L.lar_fill_preconditioner(mid) == 0
+    assert L.mf_lar_set_gs_order(mid, 1) > 1
+    x = np.zeros(4 * nn)
+    it, rr, res = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+    assert L.mf_lar_gmres(mid, 30, 2000, 1e-10, 2, 1, P(x), ctypes.byref(it), ctypes.byref(rr),
+                          ctypes.byref(res)) == 1
+    assert np.abs(x - xs).max() <= 1e-10 * 120 * np.abs(xs).max()
+    assert it.value < 60
+    free(L, mid)
