@@ -19,20 +19,35 @@ namespace mf {
 namespace {
 
 // V = (b ? b : 0) - A x (lar_compute_residual_bcrs, las_bcrs_intf.c:2787-2930),
-// one thread per unknown
+// one thread per unknown; the blocks of a row are taken two at a time with
+// all loads issued before the (sequential) subtractions
 __global__ void k_bcrs_residual(int nb, const int* rp, const int* ci, const double* val,
                                 const double* x, const double* b, double* v, const int* stop) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 4LL * nb || (stop && *stop)) return;
   const int r = (int)(i >> 2), k = (int)(i & 3);
   double acc = 0.0;
-  if (x)
-    for (int s = rp[r]; s < rp[r + 1]; s++) {
-      const int c = ci[s];
-      const double* a = val + 16 * (size_t)s + k;
-      const double* xc = x + 4 * (size_t)c;
-      for (int l = 0; l < 4; l++) acc = __dsub_rn(acc, __dmul_rn(a[4 * l], xc[l]));
+  if (x) {
+    const int s1 = rp[r + 1];
+    int s = rp[r];
+    for (; s + 2 <= s1; s += 2) {
+      const int c0 = ci[s], c1 = ci[s + 1];
+      double a0[4], a1[4], x0[4], x1[4];
+      for (int l = 0; l < 4; l++) {
+        a0[l] = val[16 * (size_t)s + k + 4 * l];
+        a1[l] = val[16 * (size_t)(s + 1) + k + 4 * l];
+        x0[l] = x[4 * (size_t)c0 + l];
+        x1[l] = x[4 * (size_t)c1 + l];
+      }
+      for (int l = 0; l < 4; l++) acc = __dsub_rn(acc, __dmul_rn(a0[l], x0[l]));
+      for (int l = 0; l < 4; l++) acc = __dsub_rn(acc, __dmul_rn(a1[l], x1[l]));
     }
+    if (s < s1) {
+      const int c = ci[s];
+      for (int l = 0; l < 4; l++)
+        acc = __dsub_rn(acc, __dmul_rn(val[16 * (size_t)s + k + 4 * l], x[4 * (size_t)c + l]));
+    }
+  }
   if (b) acc = __dadd_rn(acc, b[i]);
   v[i] = acc;
 }
@@ -119,13 +134,58 @@ __device__ __forceinline__ void bcrs_row(int r, int dir, const int* rp, const in
   for (int k = 0; k < 4; k++) out[4 * (size_t)r + k] = vloc[k];
 }
 
-// block Jacobi sweep: every row reads the old v
-__global__ void k_bcrs_bj(int nb, int dir, const int* rp, const int* ci, const int* diag,
-                          const double* val, const double* lu, const int* piv, const double* b,
-                          const double* v, double* out, const int* stop) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nb || (stop && *stop)) return;
-  bcrs_row(r, dir, rp, ci, diag, val, lu, piv, b, v, v, out);
+// subtract the products of blocks [s0, s1) of component k in order (two
+// blocks per step, loads first)
+__device__ __forceinline__ double chain_sub(double acc, int s0, int s1, int k, const int* ci,
+                                            const double* val, const double* y) {
+  int s = s0;
+  for (; s + 2 <= s1; s += 2) {
+    const int c0 = ci[s], c1 = ci[s + 1];
+    double a0[4], a1[4], y0[4], y1[4];
+    for (int l = 0; l < 4; l++) {
+      a0[l] = val[16 * (size_t)s + k + 4 * l];
+      a1[l] = val[16 * (size_t)(s + 1) + k + 4 * l];
+      y0[l] = y[4 * (size_t)c0 + l];
+      y1[l] = y[4 * (size_t)c1 + l];
+    }
+    for (int l = 0; l < 4; l++) acc = __dsub_rn(acc, __dmul_rn(a0[l], y0[l]));
+    for (int l = 0; l < 4; l++) acc = __dsub_rn(acc, __dmul_rn(a1[l], y1[l]));
+  }
+  if (s < s1) {
+    const int c = ci[s];
+    for (int l = 0; l < 4; l++)
+      acc = __dsub_rn(acc, __dmul_rn(val[16 * (size_t)s + k + 4 * l], y[4 * (size_t)c + l]));
+  }
+  return acc;
+}
+
+// block Jacobi sweep: every row reads the old v; one thread per unknown, the
+// four threads of a row exchange their sums and each solves with the
+// diagonal block's factors, keeping its own component.  With a row list
+// (rows of one colour class, none sharing a block) the same kernel sweeps
+// the class in place (out = v).
+__global__ void k_bcrs_bj(int nrows, const int* rows, int dir, const int* rp, const int* ci,
+                          const int* diag, const double* val, const double* lu, const int* piv,
+                          const double* b, const double* v, double* out, const int* stop) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4LL * nrows || (stop && *stop)) return;
+  const int r = rows ? rows[t >> 2] : (int)(t >> 2), k = (int)(t & 3);
+  const long long i = 4LL * r + k;
+  const int d = diag[r];
+  double acc = 0.0;
+  if (dir == 0) {
+    acc = chain_sub(acc, rp[r], d, k, ci, val, v);
+    acc = chain_sub(acc, d + 1, rp[r + 1], k, ci, val, v);
+  } else {
+    acc = chain_sub(acc, d + 1, rp[r + 1], k, ci, val, v);
+    acc = chain_sub(acc, rp[r], d, k, ci, val, v);
+  }
+  if (b) acc = __dadd_rn(acc, b[i]);
+  const int base = (int)(threadIdx.x & ~3u);
+  double y[4];
+  for (int q = 0; q < 4; q++) y[q] = __shfl(acc, base + q, 64);
+  lu4_solve(lu + 16 * (size_t)r, piv + 4 * (size_t)r, y);
+  out[i] = y[k];
 }
 
 // block Gauss-Seidel, the rows of one level: passed blocks read the sweep's
@@ -313,9 +373,6 @@ __global__ void k_bcrs_bwd_level(int cnt, const int* rows, const int* rp, const 
 //           other side from v, + b, then the diagonal block's LU solve) -> out
 //   mode 1: ILU forward row, y = b - L y (lower blocks from v, in place)
 //   mode 2: ILU backward row, y = inv(a_ii) (y - U y) (upper blocks from v)
-//   mode 3: Gauss-Seidel row of a colour class, in place: every off-diagonal
-//           block reads v (rows of earlier colours already updated), lower
-//           blocks first, + b, LU solve -> v
 constexpr int kWaveBlocks = 64;
 
 __global__ void __launch_bounds__(64) k_bcrs_rows_wave(
@@ -328,12 +385,7 @@ __global__ void __launch_bounds__(64) k_bcrs_rows_wave(
   if (t >= cnt || (stop && *stop)) return;  // uniform over the workgroup
   const int r = rows[t];
   int p0, p1, o0 = 0, o1 = 0;
-  if (mode == 3) {
-    p0 = rp[r];
-    p1 = diag[r];
-    o0 = diag[r] + 1;
-    o1 = rp[r + 1];
-  } else if (mode == 0) {
+  if (mode == 0) {
     p0 = dir == 0 ? rp[r] : diag[r] + 1;
     p1 = dir == 0 ? diag[r] : rp[r + 1];
     o0 = dir == 0 ? diag[r] + 1 : rp[r];
@@ -358,13 +410,13 @@ __global__ void __launch_bounds__(64) k_bcrs_rows_wave(
     double x = mode == 2 ? v[4 * (size_t)r + k] : (mode == 1 ? b[4 * (size_t)r + k] : 0.0);
     for (int q = 0; q < nbk; q++)
       for (int l = 0; l < 4; l++) x = __dsub_rn(x, prod[16 * q + k + 4 * l]);
-    if ((mode == 0 || mode == 3) && b) x = __dadd_rn(x, b[4 * (size_t)r + k]);
+    if (mode == 0 && b) x = __dadd_rn(x, b[4 * (size_t)r + k]);
     acc[k] = x;
   }
   __syncthreads();
   if (lane == 0) {
     double x[4] = {acc[0], acc[1], acc[2], acc[3]};
-    if (mode == 0 || mode == 3) {
+    if (mode == 0) {
       lu4_solve(lu + 16 * (size_t)r, piv + 4 * (size_t)r, x);
       for (int k = 0; k < 4; k++) out[4 * (size_t)r + k] = x[k];
     } else if (mode == 1) {
@@ -552,9 +604,9 @@ int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, dou
         for (int q = 0; q < ncol; q++) {
           const int c = dir == 0 ? q : ncol - 1 - q;
           const int cnt = M.col_ptr[c + 1] - M.col_ptr[c];
-          hipLaunchKernelGGL(k_bcrs_rows_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
-                             M.d_col_rows + M.col_ptr[c], 3, 0, M.d_rp, M.d_ci, M.d_diag, M.d_val,
-                             M.d_lu, M.d_piv, b, v, v, v, stop);
+          hipLaunchKernelGGL(k_bcrs_bj, dim3((4 * cnt + 255) / 256), dim3(256), 0, stream(), cnt,
+                             M.d_col_rows + M.col_ptr[c], 0, M.d_rp, M.d_ci, M.d_diag, M.d_val,
+                             M.d_lu, M.d_piv, b, v, v, stop);
         }
     MF_HIP(hipGetLastError());
     if (sync) MF_HIP(hipStreamSynchronize(stream()));
@@ -563,7 +615,7 @@ int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, dou
   for (int it = 0; it < loops; it++)
     for (int dir = 0; dir < dirs; dir++) {
       if (!gs) {
-        hipLaunchKernelGGL(k_bcrs_bj, dim3((M.nb + 255) / 256), dim3(256), 0, stream(), M.nb, dir,
+        hipLaunchKernelGGL(k_bcrs_bj, dim3(gv), dim3(256), 0, stream(), M.nb, nullptr, dir,
                            M.d_rp, M.d_ci, M.d_diag, M.d_val, M.d_lu, M.d_piv, b, v, tmp, stop);
       } else {
         const std::vector<int>& lp = M.lev_ptr[dir];

@@ -788,10 +788,6 @@ int mf_lar_set_gs_order(int Matrix_id, int order) {
     return -1;
   }
   BcrsDev& B = *M->B;
-  if (order == 1 && B.max_row > 64) {
-    set_error("mf_lar_set_gs_order: colour-ordered sweeps need rows of at most 64 blocks");
-    return -1;
-  }
   if (order == 1 && !B.d_col_rows && B.nb > 0) {
     std::vector<int> rows;
     bcrs_colors(B.nb, M->rp, M->ci, B.col_ptr, rows);
