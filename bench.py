@@ -494,6 +494,14 @@ def bench_bcrs(args):
         return ms.value
     res_ms = bench(0)
     res_bytes = 132.0 * nblk + 4.0 * (nn + 1) + 96.0 * nn
+    if args.probe:
+        # counter probe: 11 residual launches, then the copy yardstick
+        cms = mf.bench_copy(COPY_BYTES, reps=3)
+        print(json.dumps({"probe": True, "residual_launches": reps + 1, "residual_bytes": res_bytes,
+                          "copy_bytes": COPY_BYTES, "copy_launches": 4,
+                          "copy_GBs": 2 * COPY_BYTES / (cms * 1e-3) / 1e9}))
+        L.lar_free_SM_and_LV(mid)
+        return
     rec = {"blocks": nn, "stored_blocks": nblk, "dofs": 4 * nn, "setup_s": t_setup,
            "residual_ms": res_ms, "residual_GBs": res_bytes / (res_ms * 1e-3) / 1e9}
     x = np.zeros(4 * nn)
@@ -535,12 +543,24 @@ def bench_bcrs(args):
                                "prisms (%d node blocks, %d stored blocks)" % (args.bcrs_n, nn, nblk),
                    "levels": rec},
         "roofline": {"bound": "hbm", "achieved": rec["residual_GBs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": rec["residual_GBs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_bcrs_residual"},
+                     "unit": "GB/s", "frac": rec["residual_GBs"] / HBM_PEAK_GBS,
+                     "traffic": bcrs_traffic(args.bcrs_n), "kernel": "k_bcrs_residual"},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = bcrs_cpu_baseline(en, em, nn, nb_ind, nb_cnt, res_bytes)
     print(json.dumps(out), flush=True)
+
+
+def bcrs_traffic(n):
+    """HBM bytes per launch of k_bcrs_residual from the committed FETCH_SIZE /
+    WRITE_SIZE counter summary of this workload (None if absent)"""
+    path = os.path.join(ROOT, "profiles", "k_bcrs_residual_box48_pmc.txt")
+    if n != 48 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def bcrs_cpu_baseline(en, em, nn, nb_ind, nb_cnt, res_bytes):
@@ -563,6 +583,21 @@ def bcrs_cpu_baseline(en, em, nn, nb_ind, nb_cnt, res_bytes):
     R.lar_compute_residual_bcrs.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3
     R.lar_free_SM_and_LV_bcrs.argtypes = [ctypes.c_int]
     P = lambda v: v.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    # the reference prints progress on stdout: send it to stderr, so that
+    # stdout keeps the one JSON line
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        return bcrs_cpu_time(R, P, en, em, nn, nb_ind, nb_cnt, res_bytes, gomp, threads)
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def bcrs_cpu_time(R, P, en, em, nn, nb_ind, nb_cnt, res_bytes, gomp, threads):
     rid = bcrs_allocate(R, nn, nb_ind, nb_cnt,
                         lambda nb, lists: R.lar_allocate_SM_and_LV_bcrs(4 * nb, 24, 4, P(nb_cnt), lists))
 
