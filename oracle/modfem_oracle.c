@@ -255,6 +255,11 @@ double orc_elem_calc_3d(int el_type, int control, const double *eta,
 /* exact solutions of the DIFF_IN_CUBE / DIFF_IN_BOX test problems
    (pdd_conv_diff/main/pds_conv_diff_util.c, pdr_exact_sol) */
 static void exact_sol(int model, const double *x, double *u, double *lapl) {
+  if (model == ORC_SRC_LINEAR) { /* pinning case only: u = 0.3 + x - 2y + 0.5z */
+    if (u) *u = 0.3 + x[0] - 2.0 * x[1] + 0.5 * x[2];
+    if (lapl) *lapl = 0.0;
+    return;
+  }
   double a1 = (model == ORC_SRC_DIFF_IN_BOX) ? 100.0 : 1.0;
   double X = x[0], Y = x[1], Z = x[2];
   double e = exp((-X * X - Y * Y - Z * Z) / a1);
@@ -1339,30 +1344,41 @@ static int dg_phys(int pdeg, const double *nc, const double *eta, double *phi, d
 }
 
 /* one face's stiffness matrix and load vector for nreq = 1, steady
-   (implicitness 1), as pdr_conv_diff_comp_fa_stiff_mat_app writes them:
-   ndofs_fa = ns (boundary) or 2 ns (interior), column-major
-   stiff[j*ndofs_fa + i] with the first neighbour's functions first.
+   (implicitness 1), as pdr_conv_diff_comp_fa_stiff_mat_app writes them
+   (pds_conv_diff_weakform_dg.c:232-1400): ndofs_fa = ns1 (boundary) or
+   ns1 + ns2 (interior), column-major stiff[j*ndofs_fa + i] with the first
+   neighbour's functions first.  The neighbours may have different degrees:
+   the face rule is the one of the neighbour with more points, the second
+   one's on a tie (:508-514); the caller passes penalty = min over the
+   neighbours of min(px, pz), times the penalty constant (:516-531).
    bc: 0 interior, 1 Dirichlet, 2 Neumann.  Normal fluxes An = a^T n and
    Bn = b . n of each neighbour as pdr_fa_coeff forms them
-   (pds_conv_diff_weakform.c:577-740, Qn = 0); penalty = min(px, pz) times
-   the penalty constant (weakform_dg.c:516-531); hf = sqrt(face area);
+   (pds_conv_diff_weakform.c:577-740, Qn = 0); hf = sqrt(face area);
    Dirichlet data f: the exact solution when src_model selects one
    (pdr_bc_diri_coeff), else bval; Neumann data g: the exact normal
    derivative when src_model selects one, else bval. */
-int orc_dg_face_int(int pdeg, int bc, const double *nc1, const int *g1, int side1,
-                    const double *coef1, const double *nc2, const int *g2, int side2,
-                    const double *coef2, double penalty, double hf, int src_model, double bval,
-                    double *stiff, double *rhs) {
+int orc_dg_face_int2(int pdeg1, int pdeg2, int bc, const double *nc1, const int *g1, int side1,
+                     const double *coef1, const double *nc2, const int *g2, int side2,
+                     const double *coef2, double penalty, double hf, int src_model, double bval,
+                     double *stiff, double *rhs) {
   double xg[2 * 64], wg[64];
-  const int ng = orc_dg_face_quadr(pdeg, side1 >= 2, xg, wg);
+  int ng = orc_dg_face_quadr(pdeg1, side1 >= 2, xg, wg);
+  if (bc == 0) {
+    double xg2[2 * 64], wg2[64];
+    const int ng2 = orc_dg_face_quadr(pdeg2, side1 >= 2, xg2, wg2);
+    if (!(ng > ng2)) {
+      ng = ng2;
+      memcpy(xg, xg2, sizeof(double) * 2 * ng2);
+      memcpy(wg, wg2, sizeof(double) * ng2);
+    }
+  }
   double p1[64], x1[64], y1[64], z1[64], p2[64], x2[64], y2[64], z2[64];
-  int ns = 0, nf = 0;
+  int ns1 = 0, ns2 = 0, nf = 0;
   for (int ki = 0; ki < ng; ki++) {
     double eta[3], eta2[3], n[3], xc[3];
     orc_dg_face_eta(side1, &xg[2 * ki], eta);
     const double area = orc_dg_face_ds(side1, nc1, eta, n, xc) * wg[ki];
-    ns = dg_phys(pdeg, nc1, eta, p1, x1, y1, z1);
-    nf = bc == 0 ? 2 * ns : ns;
+    ns1 = dg_phys(pdeg1, nc1, eta, p1, x1, y1, z1);
     const double *a = coef1 + ORC_C_A, *b = coef1 + ORC_C_B;
     double an1[3], an2[3] = {0, 0, 0};
     for (int c = 0; c < 3; c++) an1[c] = a[c] * n[0] + a[3 + c] * n[1] + a[6 + c] * n[2];
@@ -1370,78 +1386,106 @@ int orc_dg_face_int(int pdeg, int bc, const double *nc1, const int *g1, int side
     double bn2 = 0.0, nv = bn1;
     if (bc == 0) {
       if (orc_dg_map_point(side1, g1, side2, g2, eta, eta2)) return -1;
-      dg_phys(pdeg, nc2, eta2, p2, x2, y2, z2);
+      ns2 = dg_phys(pdeg2, nc2, eta2, p2, x2, y2, z2);
       const double *a2 = coef2 + ORC_C_A, *b2 = coef2 + ORC_C_B;
       for (int c = 0; c < 3; c++) an2[c] = a2[c] * n[0] + a2[3 + c] * n[1] + a2[6 + c] * n[2];
       bn2 = b2[0] * n[0] + b2[1] * n[1] + b2[2] * n[2];
       nv = 0.5 * (bn1 + bn2);
     }
+    nf = bc == 0 ? ns1 + ns2 : ns1;
     if (bc == 0) {
-      for (int j = 0; j < ns; j++)
-        for (int i = 0; i < ns; i++) {
-          const double g1i = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
-          const double g1j = an1[0] * x1[j] + an1[1] * y1[j] + an1[2] * z1[j];
-          const double g2i = an2[0] * x2[i] + an2[1] * y2[i] + an2[2] * z2[i];
-          const double g2j = an2[0] * x2[j] + an2[1] * y2[j] + an2[2] * z2[j];
-          /* (1,1): test 1, trial 1 */
-          stiff[j * nf + i] += ((p1[j] * g1i - p1[i] * g1j) * 0.5 +
+      double gg1[64], gg2[64];
+      for (int i = 0; i < ns1; i++) gg1[i] = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
+      for (int i = 0; i < ns2; i++) gg2[i] = an2[0] * x2[i] + an2[1] * y2[i] + an2[2] * z2[i];
+      /* (1,1): test 1, trial 1 */
+      for (int j = 0; j < ns1; j++)
+        for (int i = 0; i < ns1; i++)
+          stiff[j * nf + i] += ((p1[j] * gg1[i] - p1[i] * gg1[j]) * 0.5 +
                                 penalty * p1[j] * p1[i] / hf) * area;
-          /* (1,2): test 1, trial 2 */
-          stiff[(ns + j) * nf + i] += ((-p2[j] * g1i - p1[i] * g2j) * 0.5 -
-                                       penalty * p2[j] * p1[i] / hf) * area;
-          /* (2,1): test 2, trial 1 */
-          stiff[j * nf + ns + i] += ((p1[j] * g2i + p2[i] * g1j) * 0.5 -
-                                     penalty * p1[j] * p2[i] / hf) * area;
-          /* (2,2) */
-          stiff[(ns + j) * nf + ns + i] += ((-p2[j] * g2i + p2[i] * g2j) * 0.5 +
-                                            penalty * p2[j] * p2[i] / hf) * area;
-        }
+      /* (1,2): test 1, trial 2 */
+      for (int j = 0; j < ns2; j++)
+        for (int i = 0; i < ns1; i++)
+          stiff[(ns1 + j) * nf + i] += ((-p2[j] * gg1[i] - p1[i] * gg2[j]) * 0.5 -
+                                        penalty * p2[j] * p1[i] / hf) * area;
+      /* (2,1): test 2, trial 1 */
+      for (int j = 0; j < ns1; j++)
+        for (int i = 0; i < ns2; i++)
+          stiff[j * nf + ns1 + i] += ((p1[j] * gg2[i] + p2[i] * gg1[j]) * 0.5 -
+                                      penalty * p1[j] * p2[i] / hf) * area;
+      /* (2,2) */
+      for (int j = 0; j < ns2; j++)
+        for (int i = 0; i < ns2; i++)
+          stiff[(ns1 + j) * nf + ns1 + i] += ((-p2[j] * gg2[i] + p2[i] * gg2[j]) * 0.5 +
+                                              penalty * p2[j] * p2[i] / hf) * area;
       /* upwinding on the averaged normal velocity */
-      for (int j = 0; j < ns; j++)
-        for (int i = 0; i < ns; i++) {
-          if (nv > 0) {
-            stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
-            stiff[j * nf + ns + i] -= area * bn2 * p2[i] * p1[j];
-          } else if (nv < 0) {
-            stiff[(ns + j) * nf + i] += area * bn1 * p1[i] * p2[j];
-            stiff[(ns + j) * nf + ns + i] -= area * bn2 * p2[i] * p2[j];
-          }
+      if (nv > 0) {
+        for (int j = 0; j < ns1; j++) {
+          for (int i = 0; i < ns1; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+          for (int i = 0; i < ns2; i++) stiff[j * nf + ns1 + i] -= area * bn2 * p2[i] * p1[j];
         }
+      } else if (nv < 0) {
+        for (int j = 0; j < ns2; j++) {
+          for (int i = 0; i < ns1; i++) stiff[(ns1 + j) * nf + i] += area * bn1 * p1[i] * p2[j];
+          for (int i = 0; i < ns2; i++) stiff[(ns1 + j) * nf + ns1 + i] -= area * bn2 * p2[i] * p2[j];
+        }
+      }
     } else if (bc == 1) {
       double f = bval;
       if (src_model != ORC_SRC_CONST) exact_sol(src_model, xc, &f, NULL);
-      for (int j = 0; j < ns; j++)
-        for (int i = 0; i < ns; i++) {
+      for (int j = 0; j < ns1; j++)
+        for (int i = 0; i < ns1; i++) {
           const double gi = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
           const double gj = an1[0] * x1[j] + an1[1] * y1[j] + an1[2] * z1[j];
           stiff[j * nf + i] += (gi * p1[j] - gj * p1[i] + penalty * p1[j] * p1[i] / hf) * area;
         }
-      for (int i = 0; i < ns; i++) {
+      for (int i = 0; i < ns1; i++) {
         const double gi = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
         rhs[i] += (f * gi + penalty * f * p1[i] / hf) * area;
       }
       if (nv < 0) {
-        for (int i = 0; i < ns; i++) rhs[i] += -bn1 * f * p1[i] * area;
+        for (int i = 0; i < ns1; i++) rhs[i] += -bn1 * f * p1[i] * area;
       } else {
-        for (int j = 0; j < ns; j++)
-          for (int i = 0; i < ns; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+        for (int j = 0; j < ns1; j++)
+          for (int i = 0; i < ns1; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
       }
     } else {
       /* Neumann data (pdr_bc_neum_coeff, pds_conv_diff_weakform.c:1100-1140):
          the exact solution's normal derivative when src_model selects one */
       double g = bval;
-      if (src_model != ORC_SRC_CONST) {
+      if (src_model == ORC_SRC_LINEAR) {
+        g = n[0] - 2.0 * n[1] + 0.5 * n[2];
+      } else if (src_model != ORC_SRC_CONST) {
         const double a1 = src_model == ORC_SRC_DIFF_IN_BOX ? 100.0 : 1.0;
         const double e = exp((-xc[0] * xc[0] - xc[1] * xc[1] - xc[2] * xc[2]) / a1);
         g = (-2.0 * xc[0] * n[0] - 2.0 * xc[1] * n[1] - 2.0 * xc[2] * n[2]) * e / a1;
       }
       if (nv > 0)
-        for (int j = 0; j < ns; j++)
-          for (int i = 0; i < ns; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
-      for (int i = 0; i < ns; i++) rhs[i] += g * p1[i] * area;
+        for (int j = 0; j < ns1; j++)
+          for (int i = 0; i < ns1; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+      for (int i = 0; i < ns1; i++) rhs[i] += g * p1[i] * area;
     }
   }
   return nf;
+}
+
+/* both neighbours of degree pdeg */
+int orc_dg_face_int(int pdeg, int bc, const double *nc1, const int *g1, int side1,
+                    const double *coef1, const double *nc2, const int *g2, int side2,
+                    const double *coef2, double penalty, double hf, int src_model, double bval,
+                    double *stiff, double *rhs) {
+  return orc_dg_face_int2(pdeg, pdeg, bc, nc1, g1, side1, coef1, nc2, g2, side2, coef2, penalty,
+                          hf, src_model, bval, stiff, rhs);
+}
+
+/* value of a DG field at a local point of one element and the point's
+   physical coordinates (test helper) */
+double orc_dg_eval(int pdeg, const double *nc, const double *dofs, const double *eta, double *x) {
+  double phi[128], dx[128], dy[128], dz[128], J[9];
+  const int ns = orc_dg_shape(pdeg, eta, phi, dx, dy, dz);
+  dg_geom(nc, eta, J, x);
+  double u = 0.0;
+  for (int i = 0; i < ns; i++) u += dofs[i] * phi[i];
+  return u;
 }
 
 /* squared L2 and H1-seminorm errors of a DG solution against the exact

@@ -36,6 +36,7 @@ enum {
 #define ORC_SRC_CONST 0        /* s and g constant per element / face      */
 #define ORC_SRC_DIFF_IN_CUBE 1 /* u=exp(-(x2+y2+z2)),     s=-lapl u, g=u   */
 #define ORC_SRC_DIFF_IN_BOX 2  /* u=exp(-(x2+y2+z2)/100), s=-lapl u, g=u   */
+#define ORC_SRC_LINEAR 9       /* u=0.3+x-2y+0.5z, s=0 (oracle pinning only) */
 
 /* per-element PDE coefficients, constant over the element:
    a[9] = axx axy axz ayx ayy ayz azx azy azz, b[3], t[3], c, q[3], s, m */
@@ -161,6 +162,11 @@ double orc_dg_face_ds(int side, const double *node_coor, const double *eta, doub
 int orc_dg_map_point(int side1, const int *g1, int side2, const int *g2, const double *eta,
                      double *eta2);
 double orc_dg_face_area(const double *node_coor, int side);
+double orc_dg_eval(int pdeg, const double *nc, const double *dofs, const double *eta, double *x);
+int orc_dg_face_int2(int pdeg1, int pdeg2, int bc, const double *nc1, const int *g1, int side1,
+                     const double *coef1, const double *nc2, const int *g2, int side2,
+                     const double *coef2, double penalty, double hf, int src_model, double bval,
+                     double *stiff, double *rhs);
 int orc_dg_face_int(int pdeg, int bc, const double *nc1, const int *g1, int side1,
                     const double *coef1, const double *nc2, const int *g2, int side2,
                     const double *coef2, double penalty, double hf, int src_model, double bval,

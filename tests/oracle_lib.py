@@ -57,6 +57,8 @@ def oracle():
             "orc_dg_map_point": (I, [I, V, I, V, V, V]),
             "orc_dg_face_area": (D, [V, I]),
             "orc_dg_face_int": (I, [I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
+            "orc_dg_face_int2": (I, [I, I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
+            "orc_dg_eval": (D, [I, V, V, V, V]),
             "orc_dg_errors": (I, [I, V, V, I, V, V]),
             "orc_dg_assemble_sample": (D, [I, I, V, V, I, V, V, V, D, I]),
             "orc_bcrs_pattern": (I, [I, V, V, V, V]),
@@ -271,31 +273,34 @@ def dg_faces(el_nodes):
     return faces
 
 
-def dg_system_dense(arrays, pdeg, ind, coef_of, penalty_const=1.0, src_model=0, bval=0.0,
-                    sparse=False):
-    """the oracle's DG system: element integrals plus every face's matrices
-    (each neighbour pair once, first neighbour = lower element), boundary
-    faces Dirichlet; dense, or scipy CSR with sparse=True"""
+def dg_degrees(pdeg, nel):
+    """per-element degrees, block sizes and block offsets (pdeg: one degree
+    or one per element)"""
+    pd = np.full(nel, int(pdeg)) if np.isscalar(pdeg) else np.asarray(pdeg, int)
+    px, pz = pd % 100, pd // 100
+    ns = (px + 1) * (px + 2) // 2 * (pz + 1)
+    return pd, ns, np.r_[0, np.cumsum(ns)]
+
+
+def _dg_parts(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval, swap=False):
+    """(index vector, matrix, load) of every element and face integral of the
+    oracle's DG system, element blocks at the offsets of dg_degrees; faces
+    each once with the lower element as first neighbour (swap: the higher)"""
     o = oracle()
-    if sparse:
-        return _dg_system_sparse(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval)
     coor, en = arrays["coor"], arrays["el_nodes"]
     nel = en.shape[0]
     nodes = coor[en]
-    px, pz = pdeg % 100, pdeg // 100
-    ns = (px + 1) * (px + 2) // 2 * (pz + 1)
-    n = nel * ns
-    A = np.zeros((n, n))
-    b = np.zeros(n)
+    pd, ns, off = dg_degrees(pdeg, nel)
+    pmin = np.minimum(pd % 100, pd // 100)
     for e in range(nel):
-        K, F = dg_num_int_el(pdeg, nodes[e], ind, coef_of(e), src_model)
-        A[e * ns:(e + 1) * ns, e * ns:(e + 1) * ns] += K.T
-        b[e * ns:(e + 1) * ns] += F
-    pen = min(px, pz) * penalty_const
+        K, F = dg_num_int_el(int(pd[e]), nodes[e], ind, coef_of(e), src_model)
+        yield np.arange(off[e], off[e + 1]), K.T, F
     bc_of = _dg_bc_types(arrays)
-    st = np.zeros(4 * ns * ns)
-    rhs = np.zeros(2 * ns)
+    st = np.zeros(80 * 80)
+    rhs = np.zeros(80)
     for (e1, s1, e2, s2) in dg_faces(en):
+        if swap and e2 >= 0:
+            e1, s1, e2, s2 = e2, s2, e1, s1
         st[:] = 0.0
         rhs[:] = 0.0
         nc1 = np.ascontiguousarray(nodes[e1].ravel())
@@ -303,22 +308,48 @@ def dg_system_dense(arrays, pdeg, ind, coef_of, penalty_const=1.0, src_model=0, 
         hf = np.sqrt(o.orc_dg_face_area(P(nc1), s1))
         c1 = np.ascontiguousarray(coef_of(e1), np.float64)
         if e2 < 0:
-            nf = o.orc_dg_face_int(pdeg, bc_of(e1, s1), P(nc1), P(g1), s1, P(c1), None, None, 0,
-                                   None, pen, hf, src_model, bval, P(st), P(rhs))
-            assert nf == ns
-            A[e1 * ns:(e1 + 1) * ns, e1 * ns:(e1 + 1) * ns] += st[:ns * ns].reshape(ns, ns).T
-            b[e1 * ns:(e1 + 1) * ns] += rhs[:ns]
-            continue
-        nc2 = np.ascontiguousarray(nodes[e2].ravel())
-        g2 = np.ascontiguousarray(en[e2], np.int32)
-        c2 = np.ascontiguousarray(coef_of(e2), np.float64)
-        nf = o.orc_dg_face_int(pdeg, 0, P(nc1), P(g1), s1, P(c1), P(nc2), P(g2), s2, P(c2), pen,
-                               hf, src_model, bval, P(st), P(rhs))
-        assert nf == 2 * ns
-        M = st[:nf * nf].reshape(nf, nf).T  # M[i, j]: test i, trial j
-        idx = np.r_[e1 * ns:(e1 + 1) * ns, e2 * ns:(e2 + 1) * ns]
+            pen = pmin[e1] * penalty_const
+            nf = o.orc_dg_face_int2(int(pd[e1]), int(pd[e1]), bc_of(e1, s1), P(nc1), P(g1), s1,
+                                    P(c1), None, None, 0, None, pen, hf, src_model, bval, P(st),
+                                    P(rhs))
+            assert nf == ns[e1]
+            idx = np.arange(off[e1], off[e1 + 1])
+        else:
+            nc2 = np.ascontiguousarray(nodes[e2].ravel())
+            g2 = np.ascontiguousarray(en[e2], np.int32)
+            c2 = np.ascontiguousarray(coef_of(e2), np.float64)
+            pen = min(pmin[e1], pmin[e2]) * penalty_const
+            nf = o.orc_dg_face_int2(int(pd[e1]), int(pd[e2]), 0, P(nc1), P(g1), s1, P(c1), P(nc2),
+                                    P(g2), s2, P(c2), pen, hf, src_model, bval, P(st), P(rhs))
+            assert nf == ns[e1] + ns[e2]
+            idx = np.r_[off[e1]:off[e1 + 1], off[e2]:off[e2 + 1]]
+        yield idx, st[:nf * nf].reshape(nf, nf).T.copy(), rhs[:nf].copy()
+
+
+def dg_system_dense(arrays, pdeg, ind, coef_of, penalty_const=1.0, src_model=0, bval=0.0,
+                    sparse=False, swap=False):
+    """the oracle's DG system: element integrals plus every face's matrices
+    (each neighbour pair once, first neighbour = lower element), boundary
+    faces by their flags; dense, or scipy CSR with sparse=True.  pdeg: one
+    degree or one per element (blocks at the offsets of dg_degrees)"""
+    nel = arrays["el_nodes"].shape[0]
+    n = dg_degrees(pdeg, nel)[2][-1]
+    parts = _dg_parts(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval, swap)
+    b = np.zeros(n)
+    if sparse:
+        import scipy.sparse as sp
+        rows, cols, vals = [], [], []
+        for idx, M, F in parts:
+            k = len(idx)
+            rows.append(np.repeat(idx, k)); cols.append(np.tile(idx, k)); vals.append(M.ravel())
+            b[idx] += F
+        A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                          shape=(n, n)).tocsr()
+        return A, b
+    A = np.zeros((n, n))
+    for idx, M, F in parts:
         A[np.ix_(idx, idx)] += M
-        b[idx] += rhs[:nf]
+        b[idx] += F
     return A, b
 
 
@@ -348,55 +379,6 @@ def dg_errors(arrays, pdeg, x, src_model=1):
         l2 += a.value
         h1 += c.value
     return np.sqrt(l2), np.sqrt(h1)
-
-
-def _dg_system_sparse(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval):
-    import scipy.sparse as sp
-    o = oracle()
-    coor, en = arrays["coor"], arrays["el_nodes"]
-    nel = en.shape[0]
-    nodes = coor[en]
-    px, pz = pdeg % 100, pdeg // 100
-    ns = (px + 1) * (px + 2) // 2 * (pz + 1)
-    n = nel * ns
-    rows, cols, vals = [], [], []
-    b = np.zeros(n)
-    loc = np.arange(ns)
-    for e in range(nel):
-        K, F = dg_num_int_el(pdeg, nodes[e], ind, coef_of(e), src_model)
-        idx = e * ns + loc
-        rows.append(np.repeat(idx, ns)); cols.append(np.tile(idx, ns)); vals.append(K.T.ravel())
-        b[idx] += F
-    pen = min(px, pz) * penalty_const
-    bc_of = _dg_bc_types(arrays)
-    st = np.zeros(4 * ns * ns)
-    rhs = np.zeros(2 * ns)
-    for (e1, s1, e2, s2) in dg_faces(en):
-        st[:] = 0.0
-        rhs[:] = 0.0
-        nc1 = np.ascontiguousarray(nodes[e1].ravel())
-        g1 = np.ascontiguousarray(en[e1], np.int32)
-        hf = np.sqrt(o.orc_dg_face_area(P(nc1), s1))
-        c1 = np.ascontiguousarray(coef_of(e1), np.float64)
-        if e2 < 0:
-            o.orc_dg_face_int(pdeg, bc_of(e1, s1), P(nc1), P(g1), s1, P(c1), None, None, 0, None,
-                              pen, hf, src_model, bval, P(st), P(rhs))
-            idx = e1 * ns + loc
-            M = st[:ns * ns].reshape(ns, ns).T
-            nf = ns
-        else:
-            nc2 = np.ascontiguousarray(nodes[e2].ravel())
-            g2 = np.ascontiguousarray(en[e2], np.int32)
-            c2 = np.ascontiguousarray(coef_of(e2), np.float64)
-            nf = o.orc_dg_face_int(pdeg, 0, P(nc1), P(g1), s1, P(c1), P(nc2), P(g2), s2, P(c2),
-                                   pen, hf, src_model, bval, P(st), P(rhs))
-            idx = np.r_[e1 * ns + loc, e2 * ns + loc]
-            M = st[:nf * nf].reshape(nf, nf).T
-        rows.append(np.repeat(idx, nf)); cols.append(np.tile(idx, nf)); vals.append(M.ravel())
-        b[idx] += rhs[:nf]
-    A = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                      shape=(n, n)).tocsr()
-    return A, b
 
 
 # --- BCRS storage with 4x4 blocks -------------------------------------------

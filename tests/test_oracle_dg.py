@@ -351,3 +351,61 @@ def test_reference_error_table_reproduced(n, pdeg):
     x = spl.spsolve(A.tocsc(), b)
     l2, h1 = dg_errors(a, pdeg, x, 1)
     assert abs(l2 - l2_ref) <= 6e-10 and abs(h1 - h1_ref) <= 6e-10, (l2, h1, l2_ref, h1_ref)
+
+
+def mixed_degrees(nel, seed):
+    return np.random.default_rng(seed).choice([101, 202, 303], size=nel)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_mixed_degree_system_reproduces_a_linear_solution(seed):
+    """elements of degree 1, 2 and 3 side by side: a linear exact solution
+    (in every element's space) with its Dirichlet data is reproduced to
+    rounding on an affine prism mesh.  Faces between degrees need the rule
+    of the higher one: with the lower one's rule the degree-3 test functions
+    are integrated inexactly and the solution is off by far more"""
+    import modfem_amd as mf
+    from oracle_lib import dg_degrees, dg_system_dense, oracle
+    a = mf.Mesh.box_prism(3, 2, 2, seed=seed).arrays()  # all sides Dirichlet
+    nel = a["el_nodes"].shape[0]
+    pd = mixed_degrees(nel, seed)
+    assert len(set(pd)) == 3
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    A, b = dg_system_dense(a, pd, mf.IND_DIFF, lambda e: cc, 1.0, 9)
+    x = np.linalg.solve(A, b)
+    _, ns, off = dg_degrees(pd, nel)
+    nodes = a["coor"][a["el_nodes"]]
+    o = oracle()
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for e in range(nel):
+        for _ in range(4):
+            u, v = rng.uniform(0, 0.5, 2)
+            eta = np.array([u, v, rng.uniform(-1, 1)])
+            xp = np.zeros(3)
+            uh = o.orc_dg_eval(int(pd[e]), P(np.ascontiguousarray(nodes[e].ravel())),
+                               P(np.ascontiguousarray(x[off[e]:off[e + 1]])), P(eta), P(xp))
+            worst = max(worst, abs(uh - (0.3 + xp[0] - 2 * xp[1] + 0.5 * xp[2])))
+    assert worst <= 1e-11, worst
+
+
+def test_mixed_degree_system_does_not_depend_on_the_first_neighbour():
+    """the face rule (more points wins) and the penalty (smaller degree) are
+    symmetric in the two neighbours: assembling every interior face with
+    the higher element first gives the same system on planar faces (on
+    warped quad faces the face size sqrt|F| depends on the first
+    neighbour's vertex cycle, as in the reference)"""
+    import modfem_amd as mf
+    from oracle_lib import dg_system_dense
+    a = mf.Mesh.box_prism(3, 2, 2, seed=3).arrays()
+    nel = a["el_nodes"].shape[0]
+    pd = mixed_degrees(nel, 3)
+    rng = np.random.default_rng(3)
+    coefs = [np.r_[np.eye(3).ravel() * rng.uniform(0.5, 2), rng.uniform(-1, 1, 3), np.zeros(9)]
+             for _ in range(nel)]
+    ind = mf.coeff_indicator("axx", "ayy", "azz", "bx", "by", "bz", "s")
+    A1, b1 = dg_system_dense(a, pd, ind, lambda e: coefs[e], 1.5, 1)
+    A2, b2 = dg_system_dense(a, pd, ind, lambda e: coefs[e], 1.5, 1, swap=True)
+    assert np.abs(A1 - A2).max() <= 1e-12 * np.abs(A1).max()
+    assert np.abs(b1 - b2).max() <= 1e-12 * np.abs(b1).max()
