@@ -245,10 +245,25 @@ int mf_batch_free(int bid);
  * tensor-product monomial base of degree pdeg = 100*pz + px, 1 <= px, pz <= 3
  * (apr_shape_fun_3D, apd_dg_prism/aps_dg_util.c:408; 6 / 18 / 40 shape
  * functions for p = 1 / 2 / 3, apr_get_el_pdeg_numshap,
- * aps_dg_prism_intf.c:725), uniform over the mesh.  Element DOF blocks
- * follow the element order of the mesh. */
+ * aps_dg_prism_intf.c:725).  Element DOF blocks follow the element order
+ * of the mesh. */
+/* one degree for every element */
 int mf_dg_create(int mesh, int pdeg);
+/* a degree per element (apr_get_ent_pdeg, as an adapted field has them):
+   101, 202 or 303 each, or any one tensor degree for all.  A face between
+   different degrees integrates with the rule of the neighbour with more
+   points and penalises with the smaller degree
+   (pds_conv_diff_weakform_dg.c:508-531); the coupling blocks are
+   rectangular.  Element blocks sit at the offsets of mf_dg_layout */
+int mf_dg_create_mixed(int mesh, const int* pdeg);
+/* nshape: the block size of a one-degree problem, 0 for a mixed one */
 int mf_dg_sizes(int did, int* nel, int* nshape, int* ngauss);
+/* number of unknowns, offsets of every element's unknowns (nel + 1) and of
+   its six block-row slots (6 nel + 1: own block ns_e x ns_e, then the
+   coupling across each side, ns_e x ns_nb when interior, ns_e x ns_e
+   otherwise; column-major); any pointer may be NULL (no reference
+   counterpart) */
+int mf_dg_layout(int did, long long* ndof, long long* xoff, long long* soff);
 /* coefficients as pdr_select_el_coeff_vect / pdr_el_coeff provide them to
    apr_num_int_el (same ind[23], coefficient slots and storage models as
    mf_problem_set_coefficients); the DG form has no t (tx, ty, tz) terms

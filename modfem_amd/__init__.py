@@ -140,6 +140,8 @@ def _declare(L):
         "mf_dg_gmres": (I, [I, I, I, I, D, V, V, V, V]),
         "mf_dg_get_solution": (I, [I, V]),
         "mf_dg_bench": (I, [I, I, V]),
+        "mf_dg_create_mixed": (I, [I, V]),
+        "mf_dg_layout": (I, [I, V, V, V]),
         # storage seam with the reference's names (lsd_mkb/lah_intf.h)
         "lar_allocate_SM_and_LV": (I, [I, I, I, I, I, I, V, V, V, V]),
         "lar_initialize_SM_and_LV": (I, [I, I]),
@@ -339,15 +341,26 @@ class Mesh:
 
 class DgProblem:
     """A DG prism field of degree pdeg = 100*pz + px on a prism mesh
-    (apd_dg_prism); element DOF blocks in mesh element order."""
+    (apd_dg_prism), one degree for every element or one per element
+    (101, 202 or 303 each); element DOF blocks in mesh element order at the
+    offsets `xoff`."""
 
     def __init__(self, mesh, pdeg):
         init()
-        self.h = check(lib().mf_dg_create(mesh.h, int(pdeg)))
+        if np.isscalar(pdeg):
+            self.h = check(lib().mf_dg_create(mesh.h, int(pdeg)))
+        else:
+            pd = np.ascontiguousarray(pdeg, np.int32)
+            self.h = check(lib().mf_dg_create_mixed(mesh.h, _p(pd)))
         self.pdeg = pdeg
         nel, ns, ng = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         check(lib().mf_dg_sizes(self.h, ctypes.byref(nel), ctypes.byref(ns), ctypes.byref(ng)))
         self.nel, self.ns, self.ng = nel.value, ns.value, ng.value
+        nd = ctypes.c_longlong()
+        self.xoff = np.zeros(self.nel + 1, np.int64)
+        self.soff = np.zeros(6 * self.nel + 1, np.int64)
+        check(lib().mf_dg_layout(self.h, ctypes.byref(nd), _p(self.xoff), _p(self.soff)))
+        self.ndof = nd.value
 
     def set_coefficients(self, ind, coef_model=COEF_UNIFORM, coef_const=None, coef_el=None,
                          src_model=SRC_CONST):
@@ -397,18 +410,27 @@ class DgProblem:
 
     def block_row(self, e):
         """blocks[6, ns, ns] with blocks[slot, j, i] = row i, column j of the
-        block (slot 0 own, 1 + s the neighbour across side s), rhs[ns]"""
-        blk = np.zeros((6, self.ns, self.ns))
-        rhs = np.zeros(self.ns)
-        check(lib().mf_dg_get_block_row(self.h, int(e), _p(blk), _p(rhs)))
-        return blk, rhs
+        block (slot 0 own, 1 + s the neighbour across side s), rhs[ns]; for a
+        mixed problem a list of six column-major blocks (ns_e x ns_e, then
+        ns_e x ns_nb across an interior side, ns_e x ns_e otherwise) as
+        arrays [cols, rows]"""
+        e = int(e)
+        ns = int(self.xoff[e + 1] - self.xoff[e])
+        flat = np.zeros(int(self.soff[6 * e + 6] - self.soff[6 * e]))
+        rhs = np.zeros(ns)
+        check(lib().mf_dg_get_block_row(self.h, e, _p(flat), _p(rhs)))
+        if self.ns:
+            return flat.reshape(6, self.ns, self.ns), rhs
+        cuts = self.soff[6 * e:6 * e + 7] - self.soff[6 * e]
+        return [flat[cuts[k]:cuts[k + 1]].reshape(-1, ns) for k in range(6)], rhs
 
     def fill_precon(self):
         check(lib().mf_dg_fill_precon(self.h))
 
     def precon_block(self, e):
         """inverse of element e's own block (inv[j, i]: row i, column j)"""
-        inv = np.zeros((self.ns, self.ns))
+        ns = int(self.xoff[int(e) + 1] - self.xoff[int(e)])
+        inv = np.zeros((ns, ns))
         check(lib().mf_dg_get_precon_block(self.h, int(e), _p(inv)))
         return inv
 
@@ -419,7 +441,7 @@ class DgProblem:
         return nc.value
 
     def gmres(self, precon=1, krylbas=30, max_iter=1000, tol=1e-8, fetch=True):
-        x = np.zeros(self.nel * self.ns) if fetch else None
+        x = np.zeros(self.ndof) if fetch else None
         it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
         conv = check(lib().mf_dg_gmres(self.h, precon, krylbas, max_iter, tol, _p(x),
                                        ctypes.byref(it), ctypes.byref(rr), ctypes.byref(rs)))
@@ -427,7 +449,7 @@ class DgProblem:
                    "resid": rs.value}
 
     def solution(self):
-        x = np.zeros(self.nel * self.ns)
+        x = np.zeros(self.ndof)
         check(lib().mf_dg_get_solution(self.h, _p(x)))
         return x
 

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <vector>
@@ -449,6 +450,29 @@ struct DgProblem {
   double* d_work = nullptr;   // n, GS work vector
   Problem* P = nullptr;       // GMRES workspace over the block system
   DgOp* op = nullptr;
+  // per-element degrees (degree index d = 3 (pz - 1) + (px - 1)), elements
+  // grouped by degree (grp_ptr[d] .. grp_ptr[d + 1] in d_grp).  A uniform
+  // problem keeps the implicit layout (rows [nel][6][ns^2], vectors
+  // [nel][ns]) and null offset arrays; a mixed one stores the offsets of
+  // every element's unknowns, block-row slots (own block ns_e^2, then per
+  // side ns_e x ns_nb for an interior side, ns_e^2 otherwise) and inverse
+  bool mixed = false;
+  std::vector<int> deg, grp_ptr;
+  int* d_grp = nullptr;
+  int* d_deg = nullptr;
+  std::vector<long long> xoff, soff, voff;
+  long long *d_xoff = nullptr, *d_soff = nullptr, *d_voff = nullptr;
+  long long ndof = 0, blk_len = 0, inv_len = 0;
+  // face rules of all nine degrees, concatenated in d_qt / d_qq
+  int qt_off[9] = {0}, ngt9[9] = {0}, qq_off[9] = {0}, ngq9[9] = {0};
+  int ex9[9][40] = {};
+  // volume rules of the degrees present (mixed problems)
+  double* d_qv9 = nullptr;
+  int qv_off[9] = {0};
+  // interior slots grouped by (own degree, neighbour degree): slot_ptr[81 + 1]
+  std::vector<int> slot_ptr;
+  // Gauss-Seidel colours split by degree: gs_cd_ptr[9 c + d]
+  std::vector<int> gs_cd_ptr;
 };
 
 namespace {
@@ -505,7 +529,8 @@ void free_dg(DgProblem& D) {
   free_dg_solver(D);
   void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs, D.d_fnb, D.d_fcode,
                   D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv, D.d_slots,
-                  D.d_gs_elems, D.d_work};
+                  D.d_gs_elems, D.d_work, D.d_grp, D.d_deg, D.d_xoff, D.d_soff, D.d_voff,
+                  D.d_qv9};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -800,18 +825,43 @@ __device__ __forceinline__ double face_ds(int s, const double* J, double* n) {
 }
 
 struct DgRowArgs {
-  DgElemArgs E;        // element data, coefficients, volume rule
+  DgElemArgs E;        // element data, coefficients, volume rule of the launch's degree
   const int* fnb;
   const int* fcode;
   const double* fhf;
   const double* fval;
-  const double* qt;    // [ngt][3]
-  const double* qq;    // [ngq][3]
-  double penalty;      // min(px, pz) * penalty constant
-  int ex[40];
-  double* blk;         // [nel][6][NS*NS]
-  double* brhs;        // [nel][NS]
+  // face rules of every degree index d = 3 (pz - 1) + (px - 1): triangle
+  // rules [ngt[d]][3] at qt + 3 qt_off[d], quad rules likewise
+  const double* qt;
+  const double* qq;
+  int qt_off[9], ngt[9], qq_off[9], ngq[9];
+  double penalty_const;
+  const int* deg;      // degree index of every element
+  const int* elems;    // elements of the launch (own blocks) or null: all, in order
+  // offsets of every element's unknowns and block-row slots (own block, then
+  // the coupling across each side); null for a uniform problem, whose rows
+  // are [nel][6][NS*NS] and vectors [nel][NS]
+  const long long* xoff;
+  const long long* soff;
+  int ex[40];          // exponents of the launch's (element's) degree
+  int exn[40];         // exponents of the neighbour's degree (coupling launches)
+  double* blk;
+  double* brhs;
 };
+
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int deg_index(int px, int pz) { return 3 * (pz - 1) + (px - 1); }
+__host__ __device__ constexpr int ns_of_deg(int d) { return num_shape(d % 3 + 1, d / 3 + 1); }
+__host__ __device__ constexpr int pmin_of_deg(int d) { return d % 3 < d / 3 ? d % 3 + 1 : d / 3 + 1; }
+
+// the rule of a face between element e (degree de) and its neighbour nb
+// (degree dn): the one with more points, the second neighbour's (the higher
+// element) on a tie (pds_conv_diff_weakform_dg.c:508-514)
+__device__ __forceinline__ int face_rule_deg(const DgRowArgs& R, int tri, int e, int de, int nb,
+                                             int dn) {
+  const int ce = tri ? R.ngt[de] : R.ngq[de], cn = tri ? R.ngt[dn] : R.ngq[dn];
+  return e < nb ? (ce > cn ? de : dn) : (cn > ce ? dn : de);
+}
 
 // own side of a face point: local coordinates, area element times weight,
 // outward normal, global point, Ji a^T n (the flux a grad(.) . n in
@@ -900,17 +950,37 @@ struct LaneShapes {
 template <int PX, int PZ>
 __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   using S = DgShape<PX, PZ>;
-  using Q = DgRowShape<PX, PZ>;
-  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, NPTS = Q::NPTS, REC = 32;
+  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, DE = deg_index(PX, PZ), REC = 32;
   __shared__ double rec[64 * REC];
   __shared__ double sX[18];
   __shared__ int sex[40];
+  __shared__ int fq[5], fst[6];
+  __shared__ double fpen[5];
   const DgElemArgs& A = R.E;
   const int lane = threadIdx.x;
-  const int e = blockIdx.x;
+  const int e = R.elems ? R.elems[blockIdx.x] : blockIdx.x;
   if (lane < 40) sex[lane] = R.ex[lane];
   if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
+  if (lane < 5) {
+    // each face's rule and penalty / h_F (weakform_dg.c:508-531)
+    const int sd = lane, nb = R.fnb[5 * e + sd];
+    int q = DE, pm = pmin_of_deg(DE);
+    if (nb >= 0) {
+      const int dn = R.deg[nb];
+      q = face_rule_deg(R, sd < 2, e, DE, nb, dn);
+      pm = min(pm, pmin_of_deg(dn));
+    }
+    fq[sd] = q;
+    fst[sd + 1] = sd < 2 ? R.ngt[q] : R.ngq[q];
+    fpen[sd] = pm * R.penalty_const / R.fhf[5 * e + sd];
+  }
   __syncthreads();
+  if (lane == 0) {
+    fst[0] = NG;
+    for (int sd = 0; sd < 5; sd++) fst[sd + 1] += fst[sd];
+  }
+  __syncthreads();
+  const int NPTS = fst[5];
   const int p = lane >> 4, c16 = lane & 15;
   LaneShapes<NS, NT> L;
   L.init(sex, c16);
@@ -959,18 +1029,13 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
         for (int kk = 0; kk < 3; kk++)
           M[17 + kk] = vol * (q0 * Ji[3 * kk] + q1 * Ji[3 * kk + 1] + q2 * Ji[3 * kk + 2]);
       } else {
-        int fq = k - NG, sd;
-        const double* rule;
-        if (fq < 2 * Q::NGT) {
-          sd = fq / Q::NGT;
-          rule = R.qt + 3 * (fq - sd * Q::NGT);
-        } else {
-          fq -= 2 * Q::NGT;
-          sd = 2 + fq / Q::NGQ;
-          rule = R.qq + 3 * (fq - (sd - 2) * Q::NGQ);
-        }
+        int sd = 0;
+        while (k >= fst[sd + 1]) sd++;
+        const int q = fq[sd];
+        const double* rule = (sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) +
+                             3 * (k - fst[sd]);
         const int fi = 5 * e + sd, nb = R.fnb[fi];
-        const double pen = R.penalty / R.fhf[fi];
+        const double pen = fpen[sd];
         FacePt F;
         face_point(A, sX, e, sd, rule, F);
         put_powers(F.eta, Rg);
@@ -1036,7 +1101,8 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
     }
     __syncthreads();
   }
-  double* out = R.blk + (size_t)e * 6 * NS * NS;
+  double* out = R.blk + (R.soff ? R.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
+  double* rout = R.brhs + (R.xoff ? R.xoff[e] : (size_t)e * NS);
 #pragma unroll
   for (int jt = 0; jt < NT; jt++)
 #pragma unroll
@@ -1049,7 +1115,7 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   if (p == 0) {
 #pragma unroll
     for (int t = 0; t < NT; t++)
-      if (16 * t + c16 < NS) R.brhs[(size_t)e * NS + 16 * t + c16] = f[t];
+      if (16 * t + c16 < NS) rout[16 * t + c16] = f[t];
   }
 }
 
@@ -1057,27 +1123,35 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
 // element's test functions against the neighbour's trial functions,
 // -0.5 (psi_j g_i + h_j phi_i) + (-pen + upwind) psi_j phi_i over the face
 // (the (1,2) block of pdr_conv_diff_comp_fa_stiff_mat_app seen from either
-// side).  Records hold the powers of both elements' local coordinates.
-template <int PX, int PZ>
+// side), ns_e x ns_nb, column-major.  Records hold the powers of both
+// elements' local coordinates.
+template <int PXE, int PZE, int PXN, int PZN>
 __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots) {
-  using S = DgShape<PX, PZ>;
-  using Q = DgRowShape<PX, PZ>;
-  constexpr int NS = S::NS, NT = S::NT, NGQ = Q::NGQ, REC = 40;
-  __shared__ double rec[NGQ * REC];
+  constexpr int NSE = num_shape(PXE, PZE), NTE = (NSE + 15) / 16;
+  constexpr int NSN = num_shape(PXN, PZN), NTN = (NSN + 15) / 16;
+  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN), REC = 40;
+  constexpr int QE = (PXE + 1) * (PZE + 1), QN = (PXN + 1) * (PZN + 1);
+  constexpr int TE = tri_points(PXE), TN = tri_points(PXN);
+  constexpr int NPMAX = cmax(cmax(QE, QN), cmax(TE, TN));
+  __shared__ double rec[NPMAX * REC];
   __shared__ double sX[36];
-  __shared__ int sex[40];
+  __shared__ int sex[40], sexn[40];
   const DgElemArgs& A = R.E;
   const int lane = threadIdx.x;
   const int slot = slots[blockIdx.x];
   const int e = slot / 5, s = slot % 5;
   const int nb = R.fnb[slot], code = R.fcode[slot];
-  const int npf = s < 2 ? Q::NGT : NGQ;
-  if (lane < 40) sex[lane] = R.ex[lane];
+  const int q = face_rule_deg(R, s < 2, e, DE, nb, DN);
+  const int npf = s < 2 ? R.ngt[q] : R.ngq[q];
+  if (lane < 40) {
+    sex[lane] = R.ex[lane];
+    sexn[lane] = R.exn[lane];
+  }
   if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
   else if (lane < 36) sX[lane] = A.nc[(size_t)(lane - 18) * A.nel + nb];
   __syncthreads();
   if (lane < npf) {
-    const double* rule = (s < 2 ? R.qt : R.qq) + 3 * lane;
+    const double* rule = (s < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * lane;
     FacePt F;
     face_point(A, sX, e, s, rule, F);
     double eta2[3], J2[9], Ji2[9], x2[3];
@@ -1092,7 +1166,8 @@ __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots)
     for (int k = 0; k < 3; k++) hh[k] = Ji2[3 * k] * An[0] + Ji2[3 * k + 1] * An[1] + Ji2[3 * k + 2] * An[2];
     const double bnn = b[0] * F.n[0] + b[1] * F.n[1] + b[2] * F.n[2];
     const double nv = 0.5 * (F.bn + bnn);
-    const double pen = R.penalty / R.fhf[slot];
+    const int pm = min(pmin_of_deg(DE), pmin_of_deg(DN));
+    const double pen = pm * R.penalty_const / R.fhf[slot];
     double* Rg = rec + REC * lane;
     put_powers(F.eta, Rg);
     put_powers(eta2, Rg + 12);
@@ -1108,43 +1183,44 @@ __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots)
   }
   __syncthreads();
   const int p = lane >> 4, c16 = lane & 15;
-  LaneShapes<NS, NT> L;
+  LaneShapes<NSE, NTE> L;
   L.init(sex, c16);
+  LaneShapes<NSN, NTN> LN;
+  LN.init(sexn, c16);
   const double w0 = p == 0 ? 1.0 : 0.0, w1 = p == 1 ? 1.0 : 0.0, w2 = p == 2 ? 1.0 : 0.0,
                w3 = p == 3 ? 1.0 : 0.0;
-  dbl4 acc[NT][NT];
+  dbl4 acc[NTN][NTE];
 #pragma unroll
-  for (int jt = 0; jt < NT; jt++)
+  for (int jt = 0; jt < NTN; jt++)
 #pragma unroll
-    for (int it = 0; it < NT; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int it = 0; it < NTE; it++) acc[jt][it] = dbl4{0.0, 0.0, 0.0, 0.0};
   for (int g = 0; g < npf; g++) {
     const double* Rg = rec + REC * g;
-    double u[4][NT], w[4][NT];
+    double u[4][NTE], w[4][NTN];
     L.eval(Rg, u);
-    L.eval(Rg + 12, w);
+    LN.eval(Rg + 12, w);
     const double* M = Rg + 24 + 4 * p;
     const double m0 = M[0], m1 = M[1], m2 = M[2], m3 = M[3];
-    double av[NT], bv[NT];
+    double av[NTN], bv[NTE];
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-      av[t] = w0 * w[0][t] + w1 * w[1][t] + w2 * w[2][t] + w3 * w[3][t];
-      bv[t] = m0 * u[0][t] + m1 * u[1][t] + m2 * u[2][t] + m3 * u[3][t];
-    }
+    for (int t = 0; t < NTN; t++) av[t] = w0 * w[0][t] + w1 * w[1][t] + w2 * w[2][t] + w3 * w[3][t];
 #pragma unroll
-    for (int jt = 0; jt < NT; jt++)
+    for (int t = 0; t < NTE; t++) bv[t] = m0 * u[0][t] + m1 * u[1][t] + m2 * u[2][t] + m3 * u[3][t];
 #pragma unroll
-      for (int it = 0; it < NT; it++)
+    for (int jt = 0; jt < NTN; jt++)
+#pragma unroll
+      for (int it = 0; it < NTE; it++)
         acc[jt][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[jt], bv[it], acc[jt][it], 0, 0, 0);
   }
-  double* out = R.blk + ((size_t)e * 6 + 1 + s) * NS * NS;
+  double* out = R.blk + (R.soff ? R.soff[6 * (size_t)e + 1 + s] : ((size_t)e * 6 + 1 + s) * NSE * NSE);
 #pragma unroll
-  for (int jt = 0; jt < NT; jt++)
+  for (int jt = 0; jt < NTN; jt++)
 #pragma unroll
-    for (int it = 0; it < NT; it++)
+    for (int it = 0; it < NTE; it++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
-        if (j < NS && i < NS) out[j * NS + i] = acc[jt][it][r];
+        if (j < NSN && i < NSE) out[j * NSE + i] = acc[jt][it][r];
       }
 }
 
@@ -1158,11 +1234,13 @@ __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots)
 // element 0 of the left part.  The pivot row of step c is row c of the
 // inverse.  A zero pivot sets *bad.
 template <int NS>
-__global__ void __launch_bounds__(64) k_dg_inv(int nel, const double* blk, double* dinv, int* bad) {
+__global__ void __launch_bounds__(64) k_dg_inv(const int* elems, const long long* soff,
+                                               const long long* voff, const double* blk,
+                                               double* dinv, int* bad) {
   __shared__ double prow[2 * NS];
-  const int lane = threadIdx.x, e = blockIdx.x;
+  const int lane = threadIdx.x, e = elems ? elems[blockIdx.x] : blockIdx.x;
   const bool row = lane < NS;
-  const double* Ae = blk + (size_t)e * 6 * NS * NS;
+  const double* Ae = blk + (soff ? soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
   double L[NS], Rr[NS];
 #pragma unroll
   for (int j = 0; j < NS; j++) {
@@ -1213,45 +1291,61 @@ __global__ void __launch_bounds__(64) k_dg_inv(int nel, const double* blk, doubl
     __syncthreads();
   }
   if (row) {
-    double* Di = dinv + (size_t)e * NS * NS;
+    double* Di = dinv + (voff ? voff[e] : (size_t)e * NS * NS);
 #pragma unroll
     for (int j = 0; j < NS; j++) Di[(size_t)j * NS + mycol] = Rr[j];
   }
 }
 
+// Layout of a block system for the operator kernels: a uniform problem has
+// rows [nel][6][NS*NS] and vectors [nel][NS] (offset arrays null); a mixed
+// one has per-element offsets of unknowns (xoff), block-row slots (soff) and
+// block inverses (voff), and the degree index of every element
+struct DgLayout {
+  const int* fnb;
+  const int* deg;
+  const long long* xoff;
+  const long long* soff;
+  const long long* voff;
+};
+
 // w = scale * Dinv (A x) for the block rows (mode 0, scale = sc[j] when sc,
-// else 1; Dinv skipped when dinv is null), or r = Dinv (b - A x) (mode 1).
+// else 1; Dinv skipped when dinv is null), or r = Dinv (b - A x) (mode 1),
+// for the cnt elements of one degree (elems, or 0..cnt-1 when null).
 // 64 / NS elements per wave, lane = (element, row).
 template <int NS>
-__global__ void __launch_bounds__(256) k_dg_op(int nel, const double* blk, const int* fnb,
-                                               const double* dinv, const double* x,
-                                               const double* b, const double* sc, int j,
-                                               const int* stop, double* w, int mode) {
+__global__ void __launch_bounds__(256) k_dg_op(int cnt, const int* elems, DgLayout Y,
+                                               const double* blk, const double* dinv,
+                                               const double* x, const double* b, const double* sc,
+                                               int j, const int* stop, double* w, int mode) {
   if (stop && *stop) return;
   constexpr int EPW = 64 / NS;
   __shared__ double T[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int el = lane / NS, i = lane - el * NS;
   const double s = sc ? sc[j] : 1.0;
-  for (int e0 = (blockIdx.x * 4 + wave) * EPW; e0 < nel; e0 += gridDim.x * 4 * EPW) {
-    const int e = e0 + el;
-    const bool act = el < EPW && e < nel;
+  for (int g0 = (blockIdx.x * 4 + wave) * EPW; g0 < cnt; g0 += gridDim.x * 4 * EPW) {
+    const int gi = g0 + el;
+    const bool act = el < EPW && gi < cnt;
+    const int e = act ? (elems ? elems[gi] : gi) : 0;
+    const size_t xe0 = Y.xoff ? Y.xoff[e] : (size_t)e * NS;
     double acc = 0.0;
     if (act) {
-      const double* Ae = blk + (size_t)e * 6 * NS * NS;
-      const double* xe = x + (size_t)e * NS;
+      const double* Ae = blk + (Y.soff ? Y.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
+      const double* xe = x + xe0;
       for (int k = 0; k < NS; k++) acc += Ae[(size_t)k * NS + i] * xe[k];
       for (int sd = 0; sd < 5; sd++) {
-        const int nb = fnb[5 * (size_t)e + sd];
+        const int nb = Y.fnb[5 * (size_t)e + sd];
         if (nb < 0) continue;
-        const double* B = Ae + (size_t)(1 + sd) * NS * NS;
-        const double* xn = x + (size_t)nb * NS;
-        for (int k = 0; k < NS; k++) acc += B[(size_t)k * NS + i] * xn[k];
+        const double* B = Y.soff ? blk + Y.soff[6 * (size_t)e + 1 + sd] : Ae + (size_t)(1 + sd) * NS * NS;
+        const double* xn = x + (Y.xoff ? Y.xoff[nb] : (size_t)nb * NS);
+        const int nsn = Y.deg ? ns_of_deg(Y.deg[nb]) : NS;
+        for (int k = 0; k < nsn; k++) acc += B[(size_t)k * NS + i] * xn[k];
       }
-      if (mode == 1) acc = b[(size_t)e * NS + i] - acc;
+      if (mode == 1) acc = b[xe0 + i] - acc;
     }
     if (!dinv) {
-      if (act) w[(size_t)e * NS + i] = (mode == 1 ? 1.0 : s) * acc;
+      if (act) w[xe0 + i] = (mode == 1 ? 1.0 : s) * acc;
       continue;
     }
     T[wave][lane] = acc;
@@ -1259,10 +1353,10 @@ __global__ void __launch_bounds__(256) k_dg_op(int nel, const double* blk, const
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     double r = 0.0;
     if (act) {
-      const double* Di = dinv + (size_t)e * NS * NS;
+      const double* Di = dinv + (Y.voff ? Y.voff[e] : (size_t)e * NS * NS);
       const double* t = &T[wave][el * NS];
       for (int k = 0; k < NS; k++) r += Di[(size_t)k * NS + i] * t[k];
-      w[(size_t)e * NS + i] = (mode == 1 ? 1.0 : s) * r;
+      w[xe0 + i] = (mode == 1 ? 1.0 : s) * r;
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1271,11 +1365,11 @@ __global__ void __launch_bounds__(256) k_dg_op(int nel, const double* blk, const
 
 // one colour of a block Gauss-Seidel sweep (lar_perform_BJ_or_GS_iterations
 // for block storage, las_block_intf.c:2269-2460, in colour order):
-// x_e = Dinv_e (b_e - sum_s B_{e,s} x_nb(s)) for the colour's elements (b
-// null: zero right-hand side)
+// x_e = Dinv_e (b_e - sum_s B_{e,s} x_nb(s)) for the colour's elements of
+// one degree (b null: zero right-hand side)
 template <int NS>
-__global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, const double* blk,
-                                               const int* fnb, const double* dinv,
+__global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, DgLayout Y,
+                                               const double* blk, const double* dinv,
                                                const double* b, double* x, const int* stop) {
   if (stop && *stop) return;
   constexpr int EPW = 64 / NS;
@@ -1286,16 +1380,18 @@ __global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, const 
     const int gi = g0 + el;
     const bool act = el < EPW && gi < cnt;
     const int e = act ? elems[gi] : 0;
+    const size_t xe0 = Y.xoff ? Y.xoff[e] : (size_t)e * NS;
     double acc = 0.0;
     if (act) {
-      acc = b ? b[(size_t)e * NS + i] : 0.0;
-      const double* Ae = blk + (size_t)e * 6 * NS * NS;
+      acc = b ? b[xe0 + i] : 0.0;
+      const double* Ae = blk + (Y.soff ? Y.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
       for (int sd = 0; sd < 5; sd++) {
-        const int nb = fnb[5 * (size_t)e + sd];
+        const int nb = Y.fnb[5 * (size_t)e + sd];
         if (nb < 0) continue;
-        const double* B = Ae + (size_t)(1 + sd) * NS * NS;
-        const double* xn = x + (size_t)nb * NS;
-        for (int k = 0; k < NS; k++) acc -= B[(size_t)k * NS + i] * xn[k];
+        const double* B = Y.soff ? blk + Y.soff[6 * (size_t)e + 1 + sd] : Ae + (size_t)(1 + sd) * NS * NS;
+        const double* xn = x + (Y.xoff ? Y.xoff[nb] : (size_t)nb * NS);
+        const int nsn = Y.deg ? ns_of_deg(Y.deg[nb]) : NS;
+        for (int k = 0; k < nsn; k++) acc -= B[(size_t)k * NS + i] * xn[k];
       }
     }
     T[wave][lane] = acc;
@@ -1303,13 +1399,13 @@ __global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, const 
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     double r = 0.0;
     if (act) {
-      const double* Di = dinv + (size_t)e * NS * NS;
+      const double* Di = dinv + (Y.voff ? Y.voff[e] : (size_t)e * NS * NS);
       const double* t = &T[wave][el * NS];
       for (int k = 0; k < NS; k++) r += Di[(size_t)k * NS + i] * t[k];
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (act) x[(size_t)e * NS + i] = r;
+    if (act) x[xe0 + i] = r;
   }
 }
 
@@ -1333,49 +1429,91 @@ __global__ void k_dg_sub(long long n, const double* y, double* w, const int* sto
 
 }  // namespace
 
+// call F::template run<NS>() for the block size of degree index d
+template <class F>
+static int with_ns(int d, F& f) {
+  switch (ns_of_deg(d)) {
+    case 6: return f.template run<6>(d);
+    case 9: return f.template run<9>(d);
+    case 10: return f.template run<10>(d);
+    case 12: return f.template run<12>(d);
+    case 18: return f.template run<18>(d);
+    case 20: return f.template run<20>(d);
+    case 24: return f.template run<24>(d);
+    case 30: return f.template run<30>(d);
+    case 40: return f.template run<40>(d);
+  }
+  set_error("DG: unsupported block size");
+  return -1;
+}
+
 // GMRES operator over the DG block system (precon 0: none, 1: block Jacobi,
 // 2: block Gauss-Seidel smoothing with gs_sweeps multicolour sweeps, the
 // smoothing branch of compreres, lss_mkb_core.c:2001-2054)
 struct DgOp : ExtOp {
   DgProblem* D = nullptr;
   int precon = 1;
-  int grid() const {
-    const int epw = 64 / D->ns;
-    const long long waves = (D->nel + epw - 1) / epw;
+  DgLayout layout() const {
+    return DgLayout{D->d_fnb, D->mixed ? D->d_deg : nullptr, D->mixed ? D->d_xoff : nullptr,
+                    D->mixed ? D->d_soff : nullptr, D->mixed ? D->d_voff : nullptr};
+  }
+  int grid(int cnt, int ns) const {
+    const int epw = 64 / ns;
+    const long long waves = (cnt + epw - 1) / epw;
     return (int)std::max<long long>(1, std::min<long long>((waves + 3) / 4, (long long)D->ncu * 16));
   }
+  // arguments of the current operator or sweep launch, read by run<NS>
+  const double *x_ = nullptr, *b_ = nullptr, *sc_ = nullptr;
+  const int* stop_ = nullptr;
+  double* w_ = nullptr;
+  int j_ = 0, mode_ = 0, color_ = -1;
   template <int NS>
-  int launch(const double* x, const double* b, const double* sc, int j, const int* stop,
-             double* w, int mode) {
-    hipLaunchKernelGGL((k_dg_op<NS>), dim3(grid()), dim3(256), 0, stream(), D->nel, D->d_blk,
-                       D->d_fnb, precon ? D->d_dinv : nullptr, x, b, sc, j, stop, w, mode);
+  int run(int d) {
+    if (color_ < 0) {  // operator over the elements of degree d
+      const int c0 = D->grp_ptr[d], cnt = D->grp_ptr[d + 1] - c0;
+      hipLaunchKernelGGL((k_dg_op<NS>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
+                         D->mixed ? (const int*)D->d_grp + c0 : nullptr, layout(), D->d_blk,
+                         precon ? D->d_dinv : nullptr, x_, b_, sc_, j_, stop_, w_, mode_);
+    } else {  // the elements of degree d in colour color_
+      const size_t k = 9 * (size_t)color_ + d;
+      const int c0 = D->gs_cd_ptr[k], cnt = D->gs_cd_ptr[k + 1] - c0;
+      hipLaunchKernelGGL((k_dg_gs<NS>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
+                         (const int*)D->d_gs_elems + c0, layout(), D->d_blk, D->d_dinv, b_, w_,
+                         stop_);
+    }
     MF_HIP(hipGetLastError());
     return 0;
   }
-  template <int NS>
+  int op(const double* x, const double* b, const double* sc, int j, const int* stop, double* w,
+         int mode) {
+    x_ = x; b_ = b; sc_ = sc; j_ = j; stop_ = stop; w_ = w; mode_ = mode; color_ = -1;
+    for (int d = 0; d < 9; d++)
+      if (D->grp_ptr[d + 1] > D->grp_ptr[d] && with_ns(d, *this)) return -2;
+    return 0;
+  }
   int sweeps(const double* b, double* y, const int* stop) {
     const int ncol = (int)D->gs_color_ptr.size() - 1;
+    b_ = b; w_ = y; stop_ = stop;
     for (int k = 0; k < D->gs_sweeps; k++)
-      for (int c = 0; c < ncol; c++) {
-        const int c0 = D->gs_color_ptr[c], cnt = D->gs_color_ptr[c + 1] - c0;
-        const int epw = 64 / NS;
-        const int g = std::max(1, std::min((cnt + 4 * epw - 1) / (4 * epw), D->ncu * 16));
-        hipLaunchKernelGGL((k_dg_gs<NS>), dim3(g), dim3(256), 0, stream(), cnt,
-                           D->d_gs_elems + c0, D->d_blk, D->d_fnb, D->d_dinv, b, y, stop);
-      }
-    MF_HIP(hipGetLastError());
+      for (int c = 0; c < ncol; c++)
+        for (int d = 0; d < 9; d++) {
+          const size_t q = 9 * (size_t)c + d;
+          if (D->gs_cd_ptr[q + 1] == D->gs_cd_ptr[q]) continue;
+          color_ = c;
+          if (with_ns(d, *this)) return -2;
+        }
+    color_ = -1;
     return 0;
   }
-  template <int NS>
-  int run_t(const double* x, const double* b, const double* sc, int j, const int* stop, double* w,
-            int mode) {
-    if (precon != 2) return launch<NS>(x, b, sc, j, stop, w, mode);
-    const long long n = (long long)D->nel * NS;
+  int run_all(const double* x, const double* b, const double* sc, int j, const int* stop,
+              double* w, int mode) {
+    if (precon != 2) return op(x, b, sc, j, stop, w, mode);
+    const long long n = D->ndof;
     const int g = (int)((n + 255) / 256);
     if (mode == 1) {
       // r = GS_k(x; b) - x
       MF_HIP(hipMemcpyAsync(w, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
-      if (sweeps<NS>(b, w, stop)) return -2;
+      if (sweeps(b, w, stop)) return -2;
       hipLaunchKernelGGL(k_dg_sub, dim3(g), dim3(256), 0, stream(), n, x, w, stop);
       MF_HIP(hipGetLastError());
       return 0;
@@ -1383,32 +1521,16 @@ struct DgOp : ExtOp {
     // w = X - GS_k(X; 0), X = s_j v
     hipLaunchKernelGGL(k_dg_scale2, dim3(g), dim3(256), 0, stream(), n, x, sc, j, w, D->d_work,
                        stop);
-    if (sweeps<NS>(nullptr, D->d_work, stop)) return -2;
+    if (sweeps(nullptr, D->d_work, stop)) return -2;
     hipLaunchKernelGGL(k_dg_sub, dim3(g), dim3(256), 0, stream(), n, D->d_work, w, stop);
     MF_HIP(hipGetLastError());
     return 0;
   }
-  int run(const double* x, const double* b, const double* sc, int j, const int* stop, double* w,
-          int mode) {
-    switch (D->ns) {
-      case 6: return run_t<6>(x, b, sc, j, stop, w, mode);
-      case 9: return run_t<9>(x, b, sc, j, stop, w, mode);
-      case 10: return run_t<10>(x, b, sc, j, stop, w, mode);
-      case 12: return run_t<12>(x, b, sc, j, stop, w, mode);
-      case 18: return run_t<18>(x, b, sc, j, stop, w, mode);
-      case 20: return run_t<20>(x, b, sc, j, stop, w, mode);
-      case 24: return run_t<24>(x, b, sc, j, stop, w, mode);
-      case 30: return run_t<30>(x, b, sc, j, stop, w, mode);
-      case 40: return run_t<40>(x, b, sc, j, stop, w, mode);
-    }
-    set_error("DG operator: unsupported block size");
-    return -1;
-  }
   int residual(const double* b, const double* x, double* r) override {
-    return run(x, b, nullptr, 0, nullptr, r, 1);
+    return run_all(x, b, nullptr, 0, nullptr, r, 1);
   }
   int apply(const double* v, const double* scale, int j, double* w, const int* stop) override {
-    return run(v, nullptr, scale, j, stop, w, 0);
+    return run_all(v, nullptr, scale, j, stop, w, 0);
   }
 };
 
@@ -1428,47 +1550,90 @@ void free_dg_solver(DgProblem& D) {
   D.op = nullptr;
 }
 
-template <int PX, int PZ>
-int launch_rows_t(DgProblem& D) {
-  DgRowArgs R;
+// the launch arguments common to the own-block and coupling kernels
+void row_args(const DgProblem& D, int d, DgRowArgs& R) {
   DgElemArgs& A = R.E;
   A.nel = D.nel; A.e0 = 0; A.count = D.nel;
-  A.nc = D.d_nc; A.qv = D.d_qv; A.U = D.d_U;
+  A.nc = D.d_nc; A.U = D.d_U;
+  A.qv = D.mixed ? D.d_qv9 + 4 * (size_t)D.qv_off[d] : D.d_qv;
   A.coef_model = D.coef_model; A.src_model = D.src_model; A.mask = D.mask;
   for (int k = 0; k < NCOEF; k++) A.cc[k] = D.cc[k];
   A.coef = D.d_coef; A.stiff = nullptr; A.rhs = nullptr;
   R.fnb = D.d_fnb; R.fcode = D.d_fcode; R.fhf = D.d_fhf; R.fval = D.d_fval;
   R.qt = D.d_qt; R.qq = D.d_qq;
-  R.penalty = std::min(PX, PZ) * D.penalty_const;
-  for (int k = 0; k < 40; k++) R.ex[k] = D.ex[k];
+  for (int k = 0; k < 9; k++) {
+    R.qt_off[k] = D.qt_off[k]; R.ngt[k] = D.ngt9[k];
+    R.qq_off[k] = D.qq_off[k]; R.ngq[k] = D.ngq9[k];
+  }
+  R.penalty_const = D.penalty_const;
+  R.deg = D.d_deg;
+  R.elems = D.mixed ? D.d_grp + D.grp_ptr[d] : nullptr;
+  R.xoff = D.mixed ? D.d_xoff : nullptr;
+  R.soff = D.mixed ? D.d_soff : nullptr;
+  for (int k = 0; k < 40; k++) R.ex[k] = D.ex9[d][k];
   R.blk = D.d_blk; R.brhs = D.d_brhs;
-  hipLaunchKernelGGL((k_dg_diag<PX, PZ>), dim3(D.nel), dim3(64), 0, stream(), R);
+}
+
+template <int PX, int PZ>
+int launch_diag_t(DgProblem& D) {
+  constexpr int d = deg_index(PX, PZ);
+  const int cnt = D.grp_ptr[d + 1] - D.grp_ptr[d];
+  if (cnt == 0) return 0;
+  DgRowArgs R;
+  row_args(D, d, R);
+  hipLaunchKernelGGL((k_dg_diag<PX, PZ>), dim3(cnt), dim3(64), 0, stream(), R);
   MF_HIP(hipGetLastError());
-  if (D.nslots > 0)
-    hipLaunchKernelGGL((k_dg_couple<PX, PZ>), dim3(D.nslots), dim3(64), 0, stream(), R,
-                       (const int*)D.d_slots);
+  return 0;
+}
+
+template <int PXE, int PZE, int PXN, int PZN>
+int launch_couple_t(DgProblem& D) {
+  constexpr int de = deg_index(PXE, PZE), dn = deg_index(PXN, PZN);
+  const int s0 = D.slot_ptr[9 * de + dn], cnt = D.slot_ptr[9 * de + dn + 1] - s0;
+  if (cnt == 0) return 0;
+  DgRowArgs R;
+  row_args(D, de, R);
+  for (int k = 0; k < 40; k++) R.exn[k] = D.ex9[dn][k];
+  hipLaunchKernelGGL((k_dg_couple<PXE, PZE, PXN, PZN>), dim3(cnt), dim3(64), 0, stream(), R,
+                     (const int*)D.d_slots + s0);
   MF_HIP(hipGetLastError());
   return 0;
 }
 
 int launch_rows(DgProblem& D) {
+  int rc = 0;
 #define MF_DG_CASE(X, Z) \
-  if (D.px == X && D.pz == Z) return launch_rows_t<X, Z>(D);
+  if (rc == 0) rc = launch_diag_t<X, Z>(D);
   MF_DG_CASE(1, 1) MF_DG_CASE(2, 1) MF_DG_CASE(3, 1)
   MF_DG_CASE(1, 2) MF_DG_CASE(2, 2) MF_DG_CASE(3, 2)
   MF_DG_CASE(1, 3) MF_DG_CASE(2, 3) MF_DG_CASE(3, 3)
 #undef MF_DG_CASE
-  set_error("DG degree out of range");
-  return -1;
+  // couplings between equal degrees, and between the isotropic degrees of
+  // mixed problems
+#define MF_DG_PAIR(XE, ZE, XN, ZN) \
+  if (rc == 0) rc = launch_couple_t<XE, ZE, XN, ZN>(D);
+  MF_DG_PAIR(1, 1, 1, 1) MF_DG_PAIR(2, 1, 2, 1) MF_DG_PAIR(3, 1, 3, 1)
+  MF_DG_PAIR(1, 2, 1, 2) MF_DG_PAIR(2, 2, 2, 2) MF_DG_PAIR(3, 2, 3, 2)
+  MF_DG_PAIR(1, 3, 1, 3) MF_DG_PAIR(2, 3, 2, 3) MF_DG_PAIR(3, 3, 3, 3)
+  MF_DG_PAIR(1, 1, 2, 2) MF_DG_PAIR(1, 1, 3, 3) MF_DG_PAIR(2, 2, 1, 1)
+  MF_DG_PAIR(2, 2, 3, 3) MF_DG_PAIR(3, 3, 1, 1) MF_DG_PAIR(3, 3, 2, 2)
+#undef MF_DG_PAIR
+  return rc;
 }
 
 template <int NS>
-int launch_inv_t(DgProblem& D, int* d_bad) {
-  hipLaunchKernelGGL((k_dg_inv<NS>), dim3(D.nel), dim3(64), 0, stream(), D.nel, D.d_blk, D.d_dinv,
+int launch_inv_t(DgProblem& D, int d, int* d_bad) {
+  const int cnt = D.grp_ptr[d + 1] - D.grp_ptr[d];
+  if (cnt == 0) return 0;
+  hipLaunchKernelGGL((k_dg_inv<NS>), dim3(cnt), dim3(64), 0, stream(),
+                     D.mixed ? (const int*)D.d_grp + D.grp_ptr[d] : nullptr,
+                     (const long long*)D.d_soff, (const long long*)D.d_voff, D.d_blk, D.d_dinv,
                      d_bad);
   MF_HIP(hipGetLastError());
   return 0;
 }
+
+
 
 // greedy colouring of the element graph (elements sharing a face get
 // different colours), elements listed colour by colour
@@ -1493,32 +1658,34 @@ int build_gs_colors(DgProblem& D) {
   D.gs_color_ptr.assign(ncol + 1, 0);
   for (int e = 0; e < D.nel; e++) D.gs_color_ptr[col[e] + 1]++;
   for (int c = 0; c < ncol; c++) D.gs_color_ptr[c + 1] += D.gs_color_ptr[c];
-  std::vector<int> pos(D.gs_color_ptr.begin(), D.gs_color_ptr.end() - 1), elems(D.nel);
-  for (int e = 0; e < D.nel; e++) elems[pos[col[e]]++] = e;
+  // within a colour, elements grouped by degree (ascending element order in each group)
+  D.gs_cd_ptr.assign(9 * (size_t)ncol + 1, 0);
+  for (int e = 0; e < D.nel; e++) D.gs_cd_ptr[9 * (size_t)col[e] + D.deg[e] + 1]++;
+  for (size_t k = 0; k + 1 < D.gs_cd_ptr.size(); k++) D.gs_cd_ptr[k + 1] += D.gs_cd_ptr[k];
+  std::vector<int> pos(D.gs_cd_ptr.begin(), D.gs_cd_ptr.end() - 1), elems(D.nel);
+  for (int e = 0; e < D.nel; e++) elems[pos[9 * (size_t)col[e] + D.deg[e]]++] = e;
   MF_HIP(hipMalloc(&D.d_gs_elems, (size_t)D.nel * sizeof(int)));
   MF_HIP(hipMemcpy(D.d_gs_elems, elems.data(), (size_t)D.nel * sizeof(int), hipMemcpyHostToDevice));
-  if (!D.d_work) MF_HIP(hipMalloc(&D.d_work, (size_t)D.nel * D.ns * sizeof(double)));
+  if (!D.d_work) MF_HIP(hipMalloc(&D.d_work, (size_t)D.ndof * sizeof(double)));
   return 0;
 }
 
+struct InvLaunch {
+  DgProblem* D;
+  int* d_bad;
+  template <int NS>
+  int run(int d) { return launch_inv_t<NS>(*D, d, d_bad); }
+};
+
 int fill_block_jacobi(DgProblem& D) {
-  if (!D.d_dinv) MF_HIP(hipMalloc(&D.d_dinv, (size_t)D.nel * D.ns * D.ns * sizeof(double)));
+  if (!D.d_dinv) MF_HIP(hipMalloc(&D.d_dinv, (size_t)D.inv_len * sizeof(double)));
   int* d_bad = nullptr;
   MF_HIP(hipMalloc(&d_bad, sizeof(int)));
   MF_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), stream()));
-  int rc = -1;
-  switch (D.ns) {
-    case 6: rc = launch_inv_t<6>(D, d_bad); break;
-    case 9: rc = launch_inv_t<9>(D, d_bad); break;
-    case 10: rc = launch_inv_t<10>(D, d_bad); break;
-    case 12: rc = launch_inv_t<12>(D, d_bad); break;
-    case 18: rc = launch_inv_t<18>(D, d_bad); break;
-    case 20: rc = launch_inv_t<20>(D, d_bad); break;
-    case 24: rc = launch_inv_t<24>(D, d_bad); break;
-    case 30: rc = launch_inv_t<30>(D, d_bad); break;
-    case 40: rc = launch_inv_t<40>(D, d_bad); break;
-    default: set_error("DG block Jacobi: unsupported block size");
-  }
+  int rc = 0;
+  InvLaunch L{&D, d_bad};
+  for (int d = 0; d < 9 && rc == 0; d++)
+    if (D.grp_ptr[d + 1] > D.grp_ptr[d]) rc = with_ns(d, L);
   int bad = 0;
   if (rc == 0) {
     MF_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, stream()));
@@ -1542,16 +1709,17 @@ using namespace mf;
 
 extern "C" {
 
-int mf_dg_create(int mesh, int pdeg) {
+}  // extern "C"
+
+namespace {
+
+// a DG problem on a prism mesh with the given degree (pdeg = 100 pz + px)
+// of every element; mixed when they differ
+int dg_create(int mesh, const std::vector<int>& pdeg) {
   Mesh* m = mesh_by_id(mesh);
   if (!m) return -3;
   if (!stream()) {
     set_error("mf_init has not been called");
-    return -1;
-  }
-  const int px = pdeg % 100, pz = pdeg / 100;
-  if (px < 1 || px > 3 || pz < 1 || pz > 3) {
-    set_error("mf_dg_create: pdeg must be 100*pz + px with 1 <= px, pz <= 3");
     return -1;
   }
   for (int e = 0; e < m->nel; e++)
@@ -1559,74 +1727,215 @@ int mf_dg_create(int mesh, int pdeg) {
       set_error("mf_dg_create: the DG approximation is defined on prisms only");
       return -1;
     }
+  const size_t nel = m->nel;
+  bool mixed = false;
+  for (size_t e = 1; e < nel; e++) mixed = mixed || pdeg[e] != pdeg[0];
   auto D = std::make_unique<DgProblem>();
-  D->mesh = mesh; D->nel = m->nel; D->px = px; D->pz = pz;
-  D->ns = num_shape(px, pz);
-  D->nsp = (D->ns + 15) / 16 * 16;
-  std::vector<double> qv;
-  D->ng = volume_rule(px, pz, qv);
+  D->mesh = mesh; D->nel = m->nel; D->mixed = mixed;
+  D->deg.resize(nel);
+  for (size_t e = 0; e < nel; e++) D->deg[e] = deg_index(pdeg[e] % 100, pdeg[e] / 100);
+  if (!mixed) {
+    D->px = pdeg[0] % 100; D->pz = pdeg[0] / 100;
+    D->ns = num_shape(D->px, D->pz);
+  } else {
+    D->px = D->pz = 0;  // no single degree: element-matrix entry points are off
+    D->ns = 0;
+  }
   int dev = 0;
   MF_HIP(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   MF_HIP(hipGetDeviceProperties(&prop, dev));
   D->ncu = prop.multiProcessorCount;
-  const size_t nel = m->nel;
   std::vector<double> nc(18 * nel);
   for (size_t e = 0; e < nel; e++)
     for (int i = 0; i < 6; i++)
       for (int r = 0; r < 3; r++)
         nc[(size_t)(3 * i + r) * nel + e] = m->coor[3 * (size_t)m->nodes[6 * e + i] + r];
   MF_HIP(hipMalloc(&D->d_nc, nc.size() * sizeof(double)));
-  MF_HIP(hipMalloc(&D->d_qv, qv.size() * sizeof(double)));
-  MF_HIP(hipMalloc(&D->d_U, (size_t)D->ng * 4 * D->nsp * sizeof(double)));
   MF_HIP(hipMemcpyAsync(D->d_nc, nc.data(), nc.size() * sizeof(double), hipMemcpyHostToDevice,
                         stream()));
-  MF_HIP(hipMemcpyAsync(D->d_qv, qv.data(), qv.size() * sizeof(double), hipMemcpyHostToDevice,
-                        stream()));
-  hipLaunchKernelGGL(k_dg_table, dim3(1), dim3(64), 0, stream(), px, pz, D->ng, D->nsp,
-                     D->d_qv, D->d_U);
-  MF_HIP(hipGetLastError());
-  // faces, face rules, shape-function exponents
+  // degree groups, offsets of unknowns / block-row slots / inverses
+  D->grp_ptr.assign(10, 0);
+  for (size_t e = 0; e < nel; e++) D->grp_ptr[D->deg[e] + 1]++;
+  for (int d = 0; d < 9; d++) D->grp_ptr[d + 1] += D->grp_ptr[d];
+  std::vector<int> grp(nel), pos(D->grp_ptr.begin(), D->grp_ptr.end() - 1);
+  for (size_t e = 0; e < nel; e++) grp[pos[D->deg[e]]++] = (int)e;
+  // faces
   std::vector<int> fnb, fcode;
-  std::vector<double> fhf, qt, qq;
+  std::vector<double> fhf;
   if (build_faces(*m, fnb, fcode, fhf)) {
     free_dg(*D);
     return -1;
   }
-  face_rules(px, pz, qt, qq);
-  D->ngt = (int)qt.size() / 3;
-  D->ngq = (int)qq.size() / 3;
-  shape_exponents(px, pz, D->ex);
   const size_t nf = 5 * nel;
+  if (mixed) {
+    D->xoff.assign(nel + 1, 0);
+    D->voff.assign(nel + 1, 0);
+    D->soff.assign(6 * nel + 1, 0);
+    for (size_t e = 0; e < nel; e++) {
+      const long long ns = ns_of_deg(D->deg[e]);
+      D->xoff[e + 1] = D->xoff[e] + ns;
+      D->voff[e + 1] = D->voff[e] + ns * ns;
+      D->soff[6 * e + 1] = D->soff[6 * e] + ns * ns;
+      for (int sd = 0; sd < 5; sd++) {
+        const int nb = fnb[5 * e + sd];
+        D->soff[6 * e + 2 + sd] = D->soff[6 * e + 1 + sd] + ns * (nb >= 0 ? ns_of_deg(D->deg[nb]) : ns);
+      }
+    }
+    D->ndof = D->xoff[nel];
+    D->blk_len = D->soff[6 * nel];
+    D->inv_len = D->voff[nel];
+    MF_HIP(hipMalloc(&D->d_xoff, D->xoff.size() * sizeof(long long)));
+    MF_HIP(hipMalloc(&D->d_soff, D->soff.size() * sizeof(long long)));
+    MF_HIP(hipMalloc(&D->d_voff, D->voff.size() * sizeof(long long)));
+    MF_HIP(hipMalloc(&D->d_grp, nel * sizeof(int)));
+    MF_HIP(hipMemcpyAsync(D->d_xoff, D->xoff.data(), D->xoff.size() * sizeof(long long),
+                          hipMemcpyHostToDevice, stream()));
+    MF_HIP(hipMemcpyAsync(D->d_soff, D->soff.data(), D->soff.size() * sizeof(long long),
+                          hipMemcpyHostToDevice, stream()));
+    MF_HIP(hipMemcpyAsync(D->d_voff, D->voff.data(), D->voff.size() * sizeof(long long),
+                          hipMemcpyHostToDevice, stream()));
+    MF_HIP(hipMemcpyAsync(D->d_grp, grp.data(), nel * sizeof(int), hipMemcpyHostToDevice,
+                          stream()));
+  } else {
+    const long long ns = D->ns;
+    D->ndof = (long long)nel * ns;
+    D->blk_len = (long long)nel * 6 * ns * ns;
+    D->inv_len = (long long)nel * ns * ns;
+  }
+  MF_HIP(hipMalloc(&D->d_deg, nel * sizeof(int)));
+  MF_HIP(hipMemcpyAsync(D->d_deg, D->deg.data(), nel * sizeof(int), hipMemcpyHostToDevice,
+                        stream()));
+  // volume rules: the table of the single degree, or every degree present
+  if (!mixed) {
+    std::vector<double> qv;
+    D->ng = volume_rule(D->px, D->pz, qv);
+    D->nsp = (D->ns + 15) / 16 * 16;
+    MF_HIP(hipMalloc(&D->d_qv, qv.size() * sizeof(double)));
+    MF_HIP(hipMalloc(&D->d_U, (size_t)D->ng * 4 * D->nsp * sizeof(double)));
+    MF_HIP(hipMemcpyAsync(D->d_qv, qv.data(), qv.size() * sizeof(double), hipMemcpyHostToDevice,
+                          stream()));
+    hipLaunchKernelGGL(k_dg_table, dim3(1), dim3(64), 0, stream(), D->px, D->pz, D->ng, D->nsp,
+                       D->d_qv, D->d_U);
+    MF_HIP(hipGetLastError());
+  } else {
+    std::vector<double> all, qv;
+    for (int d = 0; d < 9; d++) {
+      if (D->grp_ptr[d + 1] == D->grp_ptr[d]) continue;
+      D->qv_off[d] = (int)all.size() / 4;
+      volume_rule(d % 3 + 1, d / 3 + 1, qv);
+      all.insert(all.end(), qv.begin(), qv.end());
+    }
+    MF_HIP(hipMalloc(&D->d_qv9, all.size() * sizeof(double)));
+    MF_HIP(hipMemcpyAsync(D->d_qv9, all.data(), all.size() * sizeof(double),
+                          hipMemcpyHostToDevice, stream()));
+  }
+  // face rules and shape-function exponents of all nine degrees
+  std::vector<double> qt_all, qq_all, qt, qq;
+  for (int d = 0; d < 9; d++) {
+    face_rules(d % 3 + 1, d / 3 + 1, qt, qq);
+    D->qt_off[d] = (int)qt_all.size() / 3;
+    D->ngt9[d] = (int)qt.size() / 3;
+    D->qq_off[d] = (int)qq_all.size() / 3;
+    D->ngq9[d] = (int)qq.size() / 3;
+    qt_all.insert(qt_all.end(), qt.begin(), qt.end());
+    qq_all.insert(qq_all.end(), qq.begin(), qq.end());
+    shape_exponents(d % 3 + 1, d / 3 + 1, D->ex9[d]);
+  }
+  if (!mixed) {
+    const int d = D->deg[0];
+    D->ngt = D->ngt9[d];
+    D->ngq = D->ngq9[d];
+    for (int k = 0; k < 40; k++) D->ex[k] = D->ex9[d][k];
+  }
   MF_HIP(hipMalloc(&D->d_fnb, nf * sizeof(int)));
   MF_HIP(hipMalloc(&D->d_fcode, nf * sizeof(int)));
   MF_HIP(hipMalloc(&D->d_fhf, nf * sizeof(double)));
   MF_HIP(hipMalloc(&D->d_fval, nf * sizeof(double)));
-  MF_HIP(hipMalloc(&D->d_qt, qt.size() * sizeof(double)));
-  MF_HIP(hipMalloc(&D->d_qq, qq.size() * sizeof(double)));
+  MF_HIP(hipMalloc(&D->d_qt, qt_all.size() * sizeof(double)));
+  MF_HIP(hipMalloc(&D->d_qq, qq_all.size() * sizeof(double)));
   MF_HIP(hipMemcpyAsync(D->d_fnb, fnb.data(), nf * sizeof(int), hipMemcpyHostToDevice, stream()));
   MF_HIP(hipMemcpyAsync(D->d_fcode, fcode.data(), nf * sizeof(int), hipMemcpyHostToDevice,
                         stream()));
   MF_HIP(hipMemcpyAsync(D->d_fhf, fhf.data(), nf * sizeof(double), hipMemcpyHostToDevice,
                         stream()));
   MF_HIP(hipMemsetAsync(D->d_fval, 0, nf * sizeof(double), stream()));
-  std::vector<int> slots;
+  MF_HIP(hipMemcpyAsync(D->d_qt, qt_all.data(), qt_all.size() * sizeof(double),
+                        hipMemcpyHostToDevice, stream()));
+  MF_HIP(hipMemcpyAsync(D->d_qq, qq_all.data(), qq_all.size() * sizeof(double),
+                        hipMemcpyHostToDevice, stream()));
+  // interior slots grouped by (own degree, neighbour degree), ascending within a pair
+  D->slot_ptr.assign(82, 0);
   for (size_t k = 0; k < nf; k++)
-    if (fnb[k] >= 0) slots.push_back((int)k);
+    if (fnb[k] >= 0) D->slot_ptr[9 * D->deg[k / 5] + D->deg[fnb[k]] + 1]++;
+  for (int q = 0; q < 81; q++) D->slot_ptr[q + 1] += D->slot_ptr[q];
+  std::vector<int> slots(D->slot_ptr[81]), spos(D->slot_ptr.begin(), D->slot_ptr.end() - 1);
+  for (size_t k = 0; k < nf; k++)
+    if (fnb[k] >= 0) slots[spos[9 * D->deg[k / 5] + D->deg[fnb[k]]]++] = (int)k;
   D->nslots = (int)slots.size();
   if (D->nslots) {
     MF_HIP(hipMalloc(&D->d_slots, slots.size() * sizeof(int)));
     MF_HIP(hipMemcpyAsync(D->d_slots, slots.data(), slots.size() * sizeof(int),
                           hipMemcpyHostToDevice, stream()));
   }
-  MF_HIP(hipMemcpyAsync(D->d_qt, qt.data(), qt.size() * sizeof(double), hipMemcpyHostToDevice,
-                        stream()));
-  MF_HIP(hipMemcpyAsync(D->d_qq, qq.data(), qq.size() * sizeof(double), hipMemcpyHostToDevice,
-                        stream()));
   MF_HIP(hipStreamSynchronize(stream()));
   const int id = g_next_dg++;
   g_dg[id] = std::move(D);
   return id;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mf_dg_create(int mesh, int pdeg) {
+  Mesh* m = mesh_by_id(mesh);
+  if (!m) return -3;
+  const int px = pdeg % 100, pz = pdeg / 100;
+  if (px < 1 || px > 3 || pz < 1 || pz > 3) {
+    set_error("mf_dg_create: pdeg must be 100*pz + px with 1 <= px, pz <= 3");
+    return -1;
+  }
+  return dg_create(mesh, std::vector<int>(m->nel, pdeg));
+}
+
+int mf_dg_create_mixed(int mesh, const int* pdeg) {
+  Mesh* m = mesh_by_id(mesh);
+  if (!m) return -3;
+  if (!pdeg) {
+    set_error("mf_dg_create_mixed: no degrees");
+    return -1;
+  }
+  std::vector<int> v(pdeg, pdeg + m->nel);
+  bool same = true;
+  for (int e = 0; e < m->nel; e++) same = same && v[e] == v[0];
+  for (int e = 0; e < m->nel; e++) {
+    const int px = v[e] % 100, pz = v[e] / 100;
+    const bool ok = same ? (px >= 1 && px <= 3 && pz >= 1 && pz <= 3)
+                         : (px == pz && px >= 1 && px <= 3);
+    if (!ok) {
+      set_error("mf_dg_create_mixed: degrees must be 101, 202 or 303 per element (any one "
+                "tensor degree when all are equal)");
+      return -1;
+    }
+  }
+  return dg_create(mesh, v);
+}
+
+int mf_dg_layout(int did, long long* ndof, long long* xoff, long long* soff) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (ndof) *ndof = D->ndof;
+  const size_t nel = D->nel;
+  if (xoff) {
+    if (D->mixed) memcpy(xoff, D->xoff.data(), (nel + 1) * sizeof(long long));
+    else for (size_t e = 0; e <= nel; e++) xoff[e] = (long long)e * D->ns;
+  }
+  if (soff) {
+    if (D->mixed) memcpy(soff, D->soff.data(), (6 * nel + 1) * sizeof(long long));
+    else for (size_t k = 0; k <= 6 * nel; k++) soff[k] = (long long)k * D->ns * D->ns;
+  }
+  return 0;
 }
 
 int mf_dg_sizes(int did, int* nel, int* ns, int* ng) {
@@ -1683,6 +1992,10 @@ int mf_dg_set_coefficients(int did, const int* ind, int coef_model, const double
 int mf_dg_element_matrices(int did, int first, int count, double* stiff, double* rhs) {
   DgProblem* D = get_dg(did);
   if (!D) return -3;
+  if (D->mixed) {
+    set_error("mf_dg_element_matrices: one degree per problem (mixed degrees: mf_dg_get_block_row)");
+    return -1;
+  }
   if (first < 0 || count < 0 || first + (long long)count > D->nel || !stiff || !rhs) {
     set_error("mf_dg_element_matrices: invalid range or output");
     return -1;
@@ -1714,6 +2027,10 @@ int mf_dg_element_matrices(int did, int first, int count, double* stiff, double*
 int mf_dg_integrate(int did) {
   DgProblem* D = get_dg(did);
   if (!D) return -3;
+  if (D->mixed) {
+    set_error("mf_dg_integrate: one degree per problem (mixed degrees: mf_dg_assemble)");
+    return -1;
+  }
   const size_t ns = D->ns;
   if (!D->d_stiff) {
     MF_HIP(hipMalloc(&D->d_stiff, (size_t)D->nel * ns * ns * sizeof(double)));
@@ -1783,12 +2100,11 @@ int mf_dg_get_faces(int did, int* nb, int* code, double* hf) {
 int mf_dg_assemble(int did) {
   DgProblem* D = get_dg(did);
   if (!D) return -3;
-  const size_t ns = D->ns, nel = D->nel;
   if (!D->d_blk) {
-    MF_HIP(hipMalloc(&D->d_blk, nel * 6 * ns * ns * sizeof(double)));
-    MF_HIP(hipMalloc(&D->d_brhs, nel * ns * sizeof(double)));
+    MF_HIP(hipMalloc(&D->d_blk, (size_t)D->blk_len * sizeof(double)));
+    MF_HIP(hipMalloc(&D->d_brhs, (size_t)D->ndof * sizeof(double)));
     // boundary slots are never written: keep them zero
-    MF_HIP(hipMemsetAsync(D->d_blk, 0, nel * 6 * ns * ns * sizeof(double), stream()));
+    MF_HIP(hipMemsetAsync(D->d_blk, 0, (size_t)D->blk_len * sizeof(double), stream()));
   }
   const int rc = launch_rows(*D);
   if (rc) return rc;
@@ -1805,13 +2121,16 @@ int mf_dg_get_block_row(int did, int e, double* blocks, double* rhs) {
     set_error("mf_dg_get_block_row: not assembled or element out of range");
     return -1;
   }
-  const size_t ns = D->ns;
+  const size_t ns = D->mixed ? ns_of_deg(D->deg[e]) : D->ns;
+  const size_t b0 = D->mixed ? D->soff[6 * (size_t)e] : (size_t)e * 6 * ns * ns;
+  const size_t b1 = D->mixed ? D->soff[6 * (size_t)e + 6] : b0 + 6 * ns * ns;
+  const size_t x0 = D->mixed ? D->xoff[e] : (size_t)e * ns;
   if (blocks)
-    MF_HIP(hipMemcpyAsync(blocks, D->d_blk + (size_t)e * 6 * ns * ns, 6 * ns * ns * sizeof(double),
-                          hipMemcpyDeviceToHost, stream()));
+    MF_HIP(hipMemcpyAsync(blocks, D->d_blk + b0, (b1 - b0) * sizeof(double), hipMemcpyDeviceToHost,
+                          stream()));
   if (rhs)
-    MF_HIP(hipMemcpyAsync(rhs, D->d_brhs + (size_t)e * ns, ns * sizeof(double),
-                          hipMemcpyDeviceToHost, stream()));
+    MF_HIP(hipMemcpyAsync(rhs, D->d_brhs + x0, ns * sizeof(double), hipMemcpyDeviceToHost,
+                          stream()));
   MF_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
@@ -1833,9 +2152,9 @@ int mf_dg_get_precon_block(int did, int e, double* inv) {
     set_error("mf_dg_get_precon_block: no block Jacobi filled or element out of range");
     return -1;
   }
-  const size_t nn = (size_t)D->ns * D->ns;
-  MF_HIP(hipMemcpyAsync(inv, D->d_dinv + (size_t)e * nn, nn * sizeof(double), hipMemcpyDeviceToHost,
-                        stream()));
+  const size_t ns = D->mixed ? ns_of_deg(D->deg[e]) : D->ns, nn = ns * ns;
+  const size_t v0 = D->mixed ? D->voff[e] : (size_t)e * nn;
+  MF_HIP(hipMemcpyAsync(inv, D->d_dinv + v0, nn * sizeof(double), hipMemcpyDeviceToHost, stream()));
   MF_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
@@ -1856,7 +2175,7 @@ int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, doub
   if (precon == 2 && build_gs_colors(*D)) return -2;
   if (!D->P) {
     D->P = new Problem();
-    D->P->n = D->nel * D->ns;
+    D->P->n = (int)D->ndof;
     D->P->el_type = kPrism;
     D->op = new DgOp();
     D->op->D = D;
@@ -1906,7 +2225,7 @@ int mf_dg_bench(int did, int reps, double* out) {
   hipEvent_t ev[4];
   for (auto& e : ev) MF_HIP(hipEventCreate(&e));
   double t_asm = 0, t_op = 0;
-  const size_t n = (size_t)D->nel * D->ns;
+  const size_t n = (size_t)D->ndof;
   double *x = nullptr, *w = nullptr;
   MF_HIP(hipMalloc(&x, n * sizeof(double)));
   MF_HIP(hipMalloc(&w, n * sizeof(double)));
@@ -1920,7 +2239,7 @@ int mf_dg_bench(int did, int reps, double* out) {
     MF_HIP(hipEventRecord(ev[0], s));
     rc = launch_rows(*D);
     MF_HIP(hipEventRecord(ev[1], s));
-    if (rc == 0) rc = op.run(x, nullptr, nullptr, 0, nullptr, w, 0);
+    if (rc == 0) rc = op.run_all(x, nullptr, nullptr, 0, nullptr, w, 0);
     MF_HIP(hipEventRecord(ev[2], s));
     MF_HIP(hipEventSynchronize(ev[2]));
     MF_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));

@@ -242,3 +242,79 @@ def test_block_gauss_seidel_solve(n, pdeg, sweeps):
     _, infoj = dg.gmres(precon=1, krylbas=50, max_iter=20000, tol=1e-13, fetch=False)
     assert info["iters"] < infoj["iters"], (info, infoj)
     dg.free()
+
+
+# ------------------------------------------------ mixed degrees per element
+def mixed_block_of(dg, A):
+    """compare every device block row of a mixed-degree problem with the
+    oracle's assembled matrix (blocks at the offsets of dg.xoff)"""
+    nb, _, _ = dg.faces()
+    xo = dg.xoff
+    worst = 0.0
+    for e in range(dg.nel):
+        blk, rhs = dg.block_row(e)
+        rows = slice(xo[e], xo[e + 1])
+        ref_row = A[rows].toarray()
+        got = np.zeros_like(ref_row)
+        got[:, xo[e]:xo[e + 1]] += blk[0].T
+        for s in range(5):
+            if nb[e, s] >= 0:
+                got[:, xo[nb[e, s]]:xo[nb[e, s] + 1]] += blk[1 + s].T
+        worst = max(worst, np.abs(got - ref_row).max() / np.abs(ref_row).max())
+    return worst
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_mixed_degree_block_system_matches_oracle(seed):
+    """elements of degree 1, 2 and 3 mixed at random: own and rectangular
+    coupling blocks (face rule of the neighbour with more points, penalty of
+    the smaller degree) and the load vector, with per-element anisotropic
+    diffusion, convection, Dirichlet and Neumann data, on a distorted mesh"""
+    from oracle_lib import dg_system_dense
+    mesh = cube_mesh(3, jitter=0.2, seed=seed)
+    a = mesh.arrays()
+    rng = np.random.default_rng(seed)
+    pd = rng.choice([101, 202, 303], size=mesh.nel)
+    ce = random_full(rng, mesh.nel)
+    ind = mf.coeff_indicator("m", "axx", "axy", "axz", "ayx", "ayy", "ayz", "azx", "azy", "azz",
+                             "bx", "by", "bz", "c", "s")
+    dg = mf.DgProblem(mesh, pd)
+    assert dg.ns == 0 and dg.ndof == int(sum({101: 6, 202: 18, 303: 40}[p] for p in pd))
+    dg.set_coefficients(ind, mf.COEF_FULL, coef_el=ce.ravel(), src_model=mf.SRC_DIFF_IN_CUBE)
+    dg.set_penalty(1.5)
+    dg.assemble()
+    A, b = dg_system_dense(a, pd, ind, lambda e: ce[:, e], 1.5, mf.SRC_DIFF_IN_CUBE, sparse=True)
+    assert mixed_block_of(dg, A) <= 1e-12
+    rhs = np.concatenate([dg.block_row(e)[1] for e in range(dg.nel)])
+    assert np.abs(rhs - b).max() <= 1e-12 * np.abs(b).max()
+    # block Jacobi of the variable-size own blocks
+    dg.fill_precon()
+    for e in range(0, dg.nel, 7):
+        own = dg.block_row(e)[0][0].T
+        inv = dg.precon_block(e).T
+        assert np.abs(own @ inv - np.eye(own.shape[0])).max() <= 1e-9
+    dg.free()
+
+
+@pytest.mark.parametrize("precon", [1, 2])
+def test_mixed_degree_gmres_matches_direct_solve(precon):
+    """GMRES with block Jacobi / block Gauss-Seidel on the mixed-degree
+    DIFF_IN_CUBE system reaches the oracle system's direct solution"""
+    import scipy.sparse.linalg as spl
+    from oracle_lib import dg_system_dense
+    mesh = cube_mesh(4)
+    a = mesh.arrays()
+    pd = np.random.default_rng(7).choice([101, 202, 303], size=mesh.nel)
+    cc = np.r_[np.eye(3).ravel(), np.zeros(12)]
+    dg = mf.DgProblem(mesh, pd)
+    dg.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc,
+                        src_model=mf.SRC_DIFF_IN_CUBE)
+    dg.assemble()
+    if precon == 2:
+        assert dg.set_gs(2) > 1
+    x, info = dg.gmres(precon=precon, krylbas=30, max_iter=3000, tol=1e-10)
+    assert info["converged"]
+    A, b = dg_system_dense(a, pd, mf.IND_DIFF, lambda e: cc, 1.0, mf.SRC_DIFF_IN_CUBE, sparse=True)
+    xs = spl.spsolve(A.tocsc(), b)
+    assert np.abs(x - xs).max() <= 1e-7 * np.abs(xs).max()
+    dg.free()
