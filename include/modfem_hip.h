@@ -259,10 +259,11 @@ int mf_dg_create_mixed(int mesh, const int* pdeg);
 /* nshape: the block size of a one-degree problem, 0 for a mixed one */
 int mf_dg_sizes(int did, int* nel, int* nshape, int* ngauss);
 /* number of unknowns, offsets of every element's unknowns (nel + 1) and of
-   its six block-row slots (6 nel + 1: own block ns_e x ns_e, then the
-   coupling across each side, ns_e x ns_nb when interior, ns_e x ns_e
-   otherwise; column-major); any pointer may be NULL (no reference
-   counterpart) */
+   its six block-row slots in the block storage (6 nel, then the total:
+   own block ns_e x ns_e, then the coupling across each side, ns_e x ns_nb
+   when interior, ns_e x ns_e otherwise; column-major; a row's slots are
+   contiguous, rows of a mixed problem stored degree by degree); any pointer
+   may be NULL (no reference counterpart) */
 int mf_dg_layout(int did, long long* ndof, long long* xoff, long long* soff);
 /* coefficients as pdr_select_el_coeff_vect / pdr_el_coeff provide them to
    apr_num_int_el (same ind[23], coefficient slots and storage models as

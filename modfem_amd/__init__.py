@@ -416,13 +416,19 @@ class DgProblem:
         arrays [cols, rows]"""
         e = int(e)
         ns = int(self.xoff[e + 1] - self.xoff[e])
-        flat = np.zeros(int(self.soff[6 * e + 6] - self.soff[6 * e]))
+        if self.ns:
+            blk = np.zeros((6, ns, ns))
+            rhs = np.zeros(ns)
+            check(lib().mf_dg_get_block_row(self.h, e, _p(blk), _p(rhs)))
+            return blk, rhs
+        if not hasattr(self, "_fnb"):
+            self._fnb = self.faces()[0]
+        cols = [ns] + [int(self.xoff[n + 1] - self.xoff[n]) if n >= 0 else ns for n in self._fnb[e]]
+        flat = np.zeros(ns * sum(cols))
         rhs = np.zeros(ns)
         check(lib().mf_dg_get_block_row(self.h, e, _p(flat), _p(rhs)))
-        if self.ns:
-            return flat.reshape(6, self.ns, self.ns), rhs
-        cuts = self.soff[6 * e:6 * e + 7] - self.soff[6 * e]
-        return [flat[cuts[k]:cuts[k + 1]].reshape(-1, ns) for k in range(6)], rhs
+        cuts = np.r_[0, np.cumsum(cols)] * ns
+        return [flat[cuts[k]:cuts[k + 1]].reshape(cols[k], ns) for k in range(6)], rhs
 
     def fill_precon(self):
         check(lib().mf_dg_fill_precon(self.h))

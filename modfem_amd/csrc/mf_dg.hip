@@ -457,7 +457,7 @@ struct DgProblem {
   // every element's unknowns, block-row slots (own block ns_e^2, then per
   // side ns_e x ns_nb for an interior side, ns_e^2 otherwise) and inverse
   bool mixed = false;
-  std::vector<int> deg, grp_ptr;
+  std::vector<int> deg, grp_ptr, h_fnb;
   int* d_grp = nullptr;
   int* d_deg = nullptr;
   std::vector<long long> xoff, soff, voff;
@@ -1297,6 +1297,36 @@ __global__ void __launch_bounds__(64) k_dg_inv(const int* elems, const long long
   }
 }
 
+// acc + (or -) sum_k B[k ld + i] x[k] over N columns, unrolled
+template <int N, int SIGN>
+__device__ __forceinline__ double colsum(double acc, const double* B, int ld, int i,
+                                         const double* x) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    if (SIGN > 0) acc += B[(size_t)k * ld + i] * x[k];
+    else acc -= B[(size_t)k * ld + i] * x[k];
+  }
+  return acc;
+}
+
+// the neighbour columns of a mixed block row: the isotropic block sizes
+// unrolled, any other size by a plain loop
+template <int SIGN>
+__device__ __forceinline__ double nb_colsum(double acc, const double* B, int ld, int i,
+                                            const double* x, int dn) {
+  switch (dn) {
+    case 0: return colsum<6, SIGN>(acc, B, ld, i, x);
+    case 4: return colsum<18, SIGN>(acc, B, ld, i, x);
+    case 8: return colsum<40, SIGN>(acc, B, ld, i, x);
+  }
+  const int n = ns_of_deg(dn);
+  for (int k = 0; k < n; k++) {
+    if (SIGN > 0) acc += B[(size_t)k * ld + i] * x[k];
+    else acc -= B[(size_t)k * ld + i] * x[k];
+  }
+  return acc;
+}
+
 // Layout of a block system for the operator kernels: a uniform problem has
 // rows [nel][6][NS*NS] and vectors [nel][NS] (offset arrays null); a mixed
 // one has per-element offsets of unknowns (xoff), block-row slots (soff) and
@@ -1313,7 +1343,7 @@ struct DgLayout {
 // else 1; Dinv skipped when dinv is null), or r = Dinv (b - A x) (mode 1),
 // for the cnt elements of one degree (elems, or 0..cnt-1 when null).
 // 64 / NS elements per wave, lane = (element, row).
-template <int NS>
+template <int NS, bool MIXED>
 __global__ void __launch_bounds__(256) k_dg_op(int cnt, const int* elems, DgLayout Y,
                                                const double* blk, const double* dinv,
                                                const double* x, const double* b, const double* sc,
@@ -1327,20 +1357,24 @@ __global__ void __launch_bounds__(256) k_dg_op(int cnt, const int* elems, DgLayo
   for (int g0 = (blockIdx.x * 4 + wave) * EPW; g0 < cnt; g0 += gridDim.x * 4 * EPW) {
     const int gi = g0 + el;
     const bool act = el < EPW && gi < cnt;
-    const int e = act ? (elems ? elems[gi] : gi) : 0;
-    const size_t xe0 = Y.xoff ? Y.xoff[e] : (size_t)e * NS;
+    const int e = act ? (MIXED ? elems[gi] : gi) : 0;
+    const size_t xe0 = MIXED ? Y.xoff[e] : (size_t)e * NS;
     double acc = 0.0;
     if (act) {
-      const double* Ae = blk + (Y.soff ? Y.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
+      const double* Ae = blk + (MIXED ? Y.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
       const double* xe = x + xe0;
       for (int k = 0; k < NS; k++) acc += Ae[(size_t)k * NS + i] * xe[k];
       for (int sd = 0; sd < 5; sd++) {
         const int nb = Y.fnb[5 * (size_t)e + sd];
         if (nb < 0) continue;
-        const double* B = Y.soff ? blk + Y.soff[6 * (size_t)e + 1 + sd] : Ae + (size_t)(1 + sd) * NS * NS;
-        const double* xn = x + (Y.xoff ? Y.xoff[nb] : (size_t)nb * NS);
-        const int nsn = Y.deg ? ns_of_deg(Y.deg[nb]) : NS;
-        for (int k = 0; k < nsn; k++) acc += B[(size_t)k * NS + i] * xn[k];
+        if (MIXED) {
+          acc = nb_colsum<1>(acc, blk + Y.soff[6 * (size_t)e + 1 + sd], NS, i, x + Y.xoff[nb],
+                             Y.deg[nb]);
+        } else {
+          const double* B = Ae + (size_t)(1 + sd) * NS * NS;
+          const double* xn = x + (size_t)nb * NS;
+          for (int k = 0; k < NS; k++) acc += B[(size_t)k * NS + i] * xn[k];
+        }
       }
       if (mode == 1) acc = b[xe0 + i] - acc;
     }
@@ -1353,7 +1387,7 @@ __global__ void __launch_bounds__(256) k_dg_op(int cnt, const int* elems, DgLayo
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     double r = 0.0;
     if (act) {
-      const double* Di = dinv + (Y.voff ? Y.voff[e] : (size_t)e * NS * NS);
+      const double* Di = dinv + (MIXED ? Y.voff[e] : (size_t)e * NS * NS);
       const double* t = &T[wave][el * NS];
       for (int k = 0; k < NS; k++) r += Di[(size_t)k * NS + i] * t[k];
       w[xe0 + i] = (mode == 1 ? 1.0 : s) * r;
@@ -1367,7 +1401,7 @@ __global__ void __launch_bounds__(256) k_dg_op(int cnt, const int* elems, DgLayo
 // for block storage, las_block_intf.c:2269-2460, in colour order):
 // x_e = Dinv_e (b_e - sum_s B_{e,s} x_nb(s)) for the colour's elements of
 // one degree (b null: zero right-hand side)
-template <int NS>
+template <int NS, bool MIXED>
 __global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, DgLayout Y,
                                                const double* blk, const double* dinv,
                                                const double* b, double* x, const int* stop) {
@@ -1380,18 +1414,22 @@ __global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, DgLayo
     const int gi = g0 + el;
     const bool act = el < EPW && gi < cnt;
     const int e = act ? elems[gi] : 0;
-    const size_t xe0 = Y.xoff ? Y.xoff[e] : (size_t)e * NS;
+    const size_t xe0 = MIXED ? Y.xoff[e] : (size_t)e * NS;
     double acc = 0.0;
     if (act) {
       acc = b ? b[xe0 + i] : 0.0;
-      const double* Ae = blk + (Y.soff ? Y.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
+      const double* Ae = blk + (MIXED ? Y.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
       for (int sd = 0; sd < 5; sd++) {
         const int nb = Y.fnb[5 * (size_t)e + sd];
         if (nb < 0) continue;
-        const double* B = Y.soff ? blk + Y.soff[6 * (size_t)e + 1 + sd] : Ae + (size_t)(1 + sd) * NS * NS;
-        const double* xn = x + (Y.xoff ? Y.xoff[nb] : (size_t)nb * NS);
-        const int nsn = Y.deg ? ns_of_deg(Y.deg[nb]) : NS;
-        for (int k = 0; k < nsn; k++) acc -= B[(size_t)k * NS + i] * xn[k];
+        if (MIXED) {
+          acc = nb_colsum<-1>(acc, blk + Y.soff[6 * (size_t)e + 1 + sd], NS, i, x + Y.xoff[nb],
+                              Y.deg[nb]);
+        } else {
+          const double* B = Ae + (size_t)(1 + sd) * NS * NS;
+          const double* xn = x + (size_t)nb * NS;
+          for (int k = 0; k < NS; k++) acc -= B[(size_t)k * NS + i] * xn[k];
+        }
       }
     }
     T[wave][lane] = acc;
@@ -1399,7 +1437,7 @@ __global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, DgLayo
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     double r = 0.0;
     if (act) {
-      const double* Di = dinv + (Y.voff ? Y.voff[e] : (size_t)e * NS * NS);
+      const double* Di = dinv + (MIXED ? Y.voff[e] : (size_t)e * NS * NS);
       const double* t = &T[wave][el * NS];
       for (int k = 0; k < NS; k++) r += Di[(size_t)k * NS + i] * t[k];
     }
@@ -1471,15 +1509,25 @@ struct DgOp : ExtOp {
   int run(int d) {
     if (color_ < 0) {  // operator over the elements of degree d
       const int c0 = D->grp_ptr[d], cnt = D->grp_ptr[d + 1] - c0;
-      hipLaunchKernelGGL((k_dg_op<NS>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
-                         D->mixed ? (const int*)D->d_grp + c0 : nullptr, layout(), D->d_blk,
-                         precon ? D->d_dinv : nullptr, x_, b_, sc_, j_, stop_, w_, mode_);
+      if (D->mixed)
+        hipLaunchKernelGGL((k_dg_op<NS, true>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
+                           (const int*)D->d_grp + c0, layout(), D->d_blk,
+                           precon ? D->d_dinv : nullptr, x_, b_, sc_, j_, stop_, w_, mode_);
+      else
+        hipLaunchKernelGGL((k_dg_op<NS, false>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
+                           nullptr, layout(), D->d_blk, precon ? D->d_dinv : nullptr, x_, b_, sc_,
+                           j_, stop_, w_, mode_);
     } else {  // the elements of degree d in colour color_
       const size_t k = 9 * (size_t)color_ + d;
       const int c0 = D->gs_cd_ptr[k], cnt = D->gs_cd_ptr[k + 1] - c0;
-      hipLaunchKernelGGL((k_dg_gs<NS>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
-                         (const int*)D->d_gs_elems + c0, layout(), D->d_blk, D->d_dinv, b_, w_,
-                         stop_);
+      if (D->mixed)
+        hipLaunchKernelGGL((k_dg_gs<NS, true>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
+                           (const int*)D->d_gs_elems + c0, layout(), D->d_blk, D->d_dinv, b_, w_,
+                           stop_);
+      else
+        hipLaunchKernelGGL((k_dg_gs<NS, false>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
+                           (const int*)D->d_gs_elems + c0, layout(), D->d_blk, D->d_dinv, b_, w_,
+                           stop_);
     }
     MF_HIP(hipGetLastError());
     return 0;
@@ -1768,23 +1816,33 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     return -1;
   }
   const size_t nf = 5 * nel;
+  D->h_fnb = fnb;
   if (mixed) {
+    // unknowns in element order; block rows and inverses stored degree
+    // group by group, so that a launch over one degree streams its blocks
     D->xoff.assign(nel + 1, 0);
     D->voff.assign(nel + 1, 0);
     D->soff.assign(6 * nel + 1, 0);
-    for (size_t e = 0; e < nel; e++) {
+    for (size_t e = 0; e < nel; e++) D->xoff[e + 1] = D->xoff[e] + ns_of_deg(D->deg[e]);
+    long long bpos = 0, vpos = 0;
+    for (size_t g = 0; g < nel; g++) {
+      const int e = grp[g];
       const long long ns = ns_of_deg(D->deg[e]);
-      D->xoff[e + 1] = D->xoff[e] + ns;
-      D->voff[e + 1] = D->voff[e] + ns * ns;
-      D->soff[6 * e + 1] = D->soff[6 * e] + ns * ns;
+      D->voff[e] = vpos;
+      vpos += ns * ns;
+      D->soff[6 * (size_t)e] = bpos;
+      bpos += ns * ns;
       for (int sd = 0; sd < 5; sd++) {
-        const int nb = fnb[5 * e + sd];
-        D->soff[6 * e + 2 + sd] = D->soff[6 * e + 1 + sd] + ns * (nb >= 0 ? ns_of_deg(D->deg[nb]) : ns);
+        const int nb = fnb[5 * (size_t)e + sd];
+        D->soff[6 * (size_t)e + 1 + sd] = bpos;
+        bpos += ns * (nb >= 0 ? ns_of_deg(D->deg[nb]) : ns);
       }
     }
     D->ndof = D->xoff[nel];
-    D->blk_len = D->soff[6 * nel];
-    D->inv_len = D->voff[nel];
+    D->blk_len = bpos;
+    D->inv_len = vpos;
+    D->soff[6 * nel] = bpos;
+    D->voff[nel] = vpos;
     MF_HIP(hipMalloc(&D->d_xoff, D->xoff.size() * sizeof(long long)));
     MF_HIP(hipMalloc(&D->d_soff, D->soff.size() * sizeof(long long)));
     MF_HIP(hipMalloc(&D->d_voff, D->voff.size() * sizeof(long long)));
@@ -2123,7 +2181,11 @@ int mf_dg_get_block_row(int did, int e, double* blocks, double* rhs) {
   }
   const size_t ns = D->mixed ? ns_of_deg(D->deg[e]) : D->ns;
   const size_t b0 = D->mixed ? D->soff[6 * (size_t)e] : (size_t)e * 6 * ns * ns;
-  const size_t b1 = D->mixed ? D->soff[6 * (size_t)e + 6] : b0 + 6 * ns * ns;
+  size_t b1 = b0 + 6 * ns * ns;
+  if (D->mixed) {  // the row's six slots are contiguous; the last one ends the row
+    const int nb = D->h_fnb[5 * (size_t)e + 4];
+    b1 = D->soff[6 * (size_t)e + 5] + ns * (nb >= 0 ? ns_of_deg(D->deg[nb]) : ns);
+  }
   const size_t x0 = D->mixed ? D->xoff[e] : (size_t)e * ns;
   if (blocks)
     MF_HIP(hipMemcpyAsync(blocks, D->d_blk + b0, (b1 - b0) * sizeof(double), hipMemcpyDeviceToHost,
