@@ -947,9 +947,10 @@ struct LaneShapes {
 // (phi, d/deta phi) of trial and test function, the load-vector row) to
 // LDS, and the wave runs the chunk through v_mfma_f64_16x16x4, one K-step
 // per point.
-template <int PX, int PZ>
+template <int PX, int PZ, bool MIXED>
 __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   using S = DgShape<PX, PZ>;
+  using Q = DgRowShape<PX, PZ>;
   constexpr int NS = S::NS, NG = S::NG, NT = S::NT, DE = deg_index(PX, PZ), REC = 32;
   __shared__ double rec[64 * REC];
   __shared__ double sX[18];
@@ -961,7 +962,7 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   const int e = R.elems ? R.elems[blockIdx.x] : blockIdx.x;
   if (lane < 40) sex[lane] = R.ex[lane];
   if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
-  if (lane < 5) {
+  if (MIXED && lane < 5) {
     // each face's rule and penalty / h_F (weakform_dg.c:508-531)
     const int sd = lane, nb = R.fnb[5 * e + sd];
     int q = DE, pm = pmin_of_deg(DE);
@@ -975,12 +976,12 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
     fpen[sd] = pm * R.penalty_const / R.fhf[5 * e + sd];
   }
   __syncthreads();
-  if (lane == 0) {
+  if (MIXED && lane == 0) {
     fst[0] = NG;
     for (int sd = 0; sd < 5; sd++) fst[sd + 1] += fst[sd];
   }
   __syncthreads();
-  const int NPTS = fst[5];
+  const int NPTS = MIXED ? fst[5] : Q::NPTS;
   const int p = lane >> 4, c16 = lane & 15;
   LaneShapes<NS, NT> L;
   L.init(sex, c16);
@@ -1030,12 +1031,24 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
           M[17 + kk] = vol * (q0 * Ji[3 * kk] + q1 * Ji[3 * kk + 1] + q2 * Ji[3 * kk + 2]);
       } else {
         int sd = 0;
-        while (k >= fst[sd + 1]) sd++;
-        const int q = fq[sd];
-        const double* rule = (sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) +
-                             3 * (k - fst[sd]);
+        const double* rule;
+        if (MIXED) {
+          while (k >= fst[sd + 1]) sd++;
+          const int q = fq[sd];
+          rule = (sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * (k - fst[sd]);
+        } else {
+          int fq0 = k - NG;
+          if (fq0 < 2 * Q::NGT) {
+            sd = fq0 / Q::NGT;
+            rule = R.qt + 3 * (R.qt_off[DE] + fq0 - sd * Q::NGT);
+          } else {
+            fq0 -= 2 * Q::NGT;
+            sd = 2 + fq0 / Q::NGQ;
+            rule = R.qq + 3 * (R.qq_off[DE] + fq0 - (sd - 2) * Q::NGQ);
+          }
+        }
         const int fi = 5 * e + sd, nb = R.fnb[fi];
-        const double pen = fpen[sd];
+        const double pen = MIXED ? fpen[sd] : pmin_of_deg(DE) * R.penalty_const / R.fhf[fi];
         FacePt F;
         face_point(A, sX, e, sd, rule, F);
         put_powers(F.eta, Rg);
@@ -1629,7 +1642,10 @@ int launch_diag_t(DgProblem& D) {
   if (cnt == 0) return 0;
   DgRowArgs R;
   row_args(D, d, R);
-  hipLaunchKernelGGL((k_dg_diag<PX, PZ>), dim3(cnt), dim3(64), 0, stream(), R);
+  if (D.mixed)
+    hipLaunchKernelGGL((k_dg_diag<PX, PZ, true>), dim3(cnt), dim3(64), 0, stream(), R);
+  else
+    hipLaunchKernelGGL((k_dg_diag<PX, PZ, false>), dim3(cnt), dim3(64), 0, stream(), R);
   MF_HIP(hipGetLastError());
   return 0;
 }
