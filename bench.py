@@ -135,6 +135,15 @@ def bench_batch(args):
         loop_iters.append(info["iters"])
     t_loop = time.perf_counter() - t0
     x_loop = [p.solution() for p in probs]
+    # the tile tuner's trial cycles for this batch shape (cached for the
+    # process; trial cycles are undone, so the solves below start afresh)
+    b = mf.Batch(m)
+    for p in probs:
+        b.add(p, 1, mi, tol)
+    t0 = time.perf_counter()
+    b.run(max_cycles=1, use_graph=True)
+    t_tune = time.perf_counter() - t0
+    b.free()
     # one batch
     b = mf.Batch(m)
     for p in probs:
@@ -144,7 +153,18 @@ def bench_batch(args):
     b.run(use_graph=True)
     t_batch = time.perf_counter() - t0
     st = [b.status(p) for p in probs]
+    tiles = b.tiles()
     diff = max(float(np.linalg.norm(p.solution() - x) / np.linalg.norm(x)) for p, x in zip(probs, x_loop))
+    b.free()
+    # the same batch with the former fixed tile (8 slices per block)
+    b = mf.Batch(m)
+    b.set_tiles(8)
+    for p in probs:
+        b.add(p, 1, mi, tol)
+    t0 = time.perf_counter()
+    b.integrate_assemble(64)
+    b.run(use_graph=True)
+    t_fixed8 = time.perf_counter() - t0
     b.free()
     out = {
         "metric": "solves/s of a batch of independent problems at 1e-8 rel residual",
@@ -161,6 +181,9 @@ def bench_batch(args):
                   "max": max(x["iters"] for x in st)},
         "max_iter_diff_vs_loop": max(abs(x["iters"] - i) for x, i in zip(st, loop_iters)),
         "max_rel_diff_vs_loop_solution": diff, "setup_s": t_setup,
+        "tiles": {"slices_per_block": tiles["spb"], "tuned": tiles["tuned"],
+                  "trial_cycle_ms": tiles["trials"], "tuning_s": t_tune,
+                  "batch_s_with_fixed_tile_8": t_fixed8},
     }
     for p in probs:
         p.free()
@@ -198,15 +221,8 @@ def bench_serve(args):
     budget = args.hbm_gb * 1e9
     if max(footprint(p, m) for p in probs) > budget:
         raise SystemExit("a request is larger than the admission budget")
-    for lam in args.rates:
-        arr = np.cumsum(rng.exponential(1.0 / lam, len(probs)))
-        # sequential FIFO server
-        fin, seq_lat = 0.0, []
-        for a, sv in zip(arr, service):
-            fin = max(fin, a) + sv
-            seq_lat.append(fin - a)
-        seq_span = fin - arr[0]
-        # continuous batching server
+    def serve(arr):
+        """continuous batching server over arrival times arr: latencies and span"""
         b = mf.Batch(m)
         nxt, live, used, lat = 0, {}, 0.0, []
         t0 = time.perf_counter()
@@ -236,6 +252,22 @@ def bench_serve(args):
                     del live[k]
         span = time.perf_counter() - t0 - arr[0]
         b.free()
+        return lat, span
+
+    # a running server has tuned the batch tiles of the shapes its traffic
+    # makes: one untimed pass at the highest rate fills the tile cache
+    t0 = time.perf_counter()
+    serve(np.cumsum(np.random.default_rng(args.seed + 1).exponential(1.0 / max(args.rates), len(probs))))
+    t_warm = time.perf_counter() - t0
+    for lam in args.rates:
+        arr = np.cumsum(rng.exponential(1.0 / lam, len(probs)))
+        # sequential FIFO server
+        fin, seq_lat = 0.0, []
+        for a, sv in zip(arr, service):
+            fin = max(fin, a) + sv
+            seq_lat.append(fin - a)
+        seq_span = fin - arr[0]
+        lat, span = serve(arr)
         results.append({"rate_per_s": lam, "solves_per_s": len(probs) / span,
                         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                                        "p99": float(np.percentile(lat, 99) * 1e3)},
@@ -253,6 +285,7 @@ def bench_serve(args):
                                "budget" % (args.count, args.el_min, args.el_max, args.slots, args.hbm_gb)},
         "rates": results,
         "sequential_mean_service_ms": float(np.mean(service) * 1e3),
+        "tile_warmup_pass_s": t_warm,
     }
     for p in probs:
         p.free()

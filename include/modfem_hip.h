@@ -238,6 +238,24 @@ int mf_batch_integrate_assemble(int bid, int precision);
 int mf_batch_run(int bid, int max_cycles, int use_graph, int* active);
 int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters,
                     double* res_rel);
+/* Batch tiles (no reference counterpart).  A tile is the number of 64-row
+ * slices one block of a member streams in every batched kernel.  spb in
+ * [1,64] fixes it; spb = 0 (the default) lets the library tune it: when the
+ * membership has changed, the next mf_batch_run times one restart cycle per
+ * candidate (mf_batch_tile_candidates) and keeps the fastest.  Trial cycles
+ * are undone (x and the GMRES state are restored), so tuning changes no
+ * iterate.  The choice is cached for the process per batch shape (total
+ * rows in half-octaves, member count in octaves, graph or eager launches);
+ * a later batch of that shape, or within one step of it on either count,
+ * takes the cached choice without trials. */
+int mf_batch_set_tiles(int bid, int spb);
+/* tile in use, whether it was tuned, and the trial cycle times in ms of the
+   candidates (trial_ms: room for the returned count); returns the count */
+int mf_batch_tiles(int bid, int* spb, int* tuned, double* trial_ms);
+/* candidate tiles in trial order (spb: room for the returned count) */
+int mf_batch_tile_candidates(int* spb);
+/* forget every cached tile choice */
+int mf_batch_tile_cache_clear(void);
 int mf_batch_free(int bid);
 
 /* ------------------------------------------------------------ DG prisms

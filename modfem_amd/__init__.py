@@ -112,6 +112,10 @@ def _declare(L):
         "mf_batch_run": (I, [I, I, I, V]),
         "mf_batch_status": (I, [I, I, V, V, V, V]),
         "mf_batch_free": (I, [I]),
+        "mf_batch_set_tiles": (I, [I, I]),
+        "mf_batch_tiles": (I, [I, V, V, V]),
+        "mf_batch_tile_candidates": (I, [V]),
+        "mf_batch_tile_cache_clear": (I, []),
         "mf_gmres_mixed": (I, [I, I, I, I, ctypes.c_double, V, V, V, V, V]),
         "mf_mkb_read": (I, [ctypes.c_char_p, I, I, ctypes.c_double, I, V, V]),
         "mf_solve_mkb": (I, [I, V, ctypes.c_double, I, V, V, V, V]),
@@ -258,6 +262,26 @@ class Batch:
         check(lib().mf_batch_status(self.h, problem.h, ctypes.byref(d), ctypes.byref(c),
                                     ctypes.byref(it), ctypes.byref(rr)))
         return dict(done=bool(d.value), converged=bool(c.value), iters=it.value, res_rel=rr.value)
+
+    def set_tiles(self, spb):
+        """slices of 64 rows per block of every member; 0 lets the library
+        tune it on the next run"""
+        check(lib().mf_batch_set_tiles(self.h, spb))
+
+    def tiles(self):
+        """tile in use, whether it was tuned, and the trial cycle times (ms)
+        of the candidates"""
+        spb, tuned = ctypes.c_int(), ctypes.c_int()
+        cand = np.zeros(16, dtype=np.int32)
+        n = check(lib().mf_batch_tile_candidates(cand.ctypes.data))
+        ms = np.zeros(n)
+        check(lib().mf_batch_tiles(self.h, ctypes.byref(spb), ctypes.byref(tuned), ms.ctypes.data))
+        return dict(spb=spb.value, tuned=bool(tuned.value),
+                    trials=dict(zip(cand[:n].tolist(), ms.tolist())))
+
+    @staticmethod
+    def clear_tile_cache():
+        check(lib().mf_batch_tile_cache_clear())
 
     def free(self):
         if self.h:

@@ -1106,6 +1106,33 @@ int mf_batch_run(int bid, int max_cycles, int use_graph, int* active) {
   return batch_run(*B, max_cycles, use_graph, active);
 }
 
+int mf_batch_set_tiles(int bid, int spb) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  return batch_set_tiles(*B, spb);
+}
+
+int mf_batch_tiles(int bid, int* spb, int* tuned, double* trial_ms) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  if (spb) *spb = B->spb_used;
+  if (tuned) *tuned = B->spb_tuned ? 1 : 0;
+  if (trial_ms)
+    for (int k = 0; k < kBatchTileNum; k++) trial_ms[k] = B->spb_tuned ? B->tile_ms[k] : 0.0;
+  return kBatchTileNum;
+}
+
+int mf_batch_tile_candidates(int* spb) {
+  if (spb)
+    for (int k = 0; k < kBatchTileNum; k++) spb[k] = kBatchTileCands[k];
+  return kBatchTileNum;
+}
+
+int mf_batch_tile_cache_clear(void) {
+  batch_tile_cache_clear();
+  return 0;
+}
+
 int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters, double* res_rel) {
   Batch* B = get_batch(bid);
   if (!B) return -3;

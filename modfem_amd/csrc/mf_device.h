@@ -239,7 +239,16 @@ struct Batch {
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   int cycles = 0;
+  // batch tiles: slices of 64 rows per block of every member (0 = tuned)
+  int spb = 0, spb_used = 8;
+  bool spb_tuned = false;          // spb_used came from timed trial cycles
+  float tile_ms[8] = {};           // trial cycle times of the tile candidates
 };
+// tile candidates the tuner times (slices per block)
+constexpr int kBatchTileCands[] = {2, 4, 8, 16, 32};
+constexpr int kBatchTileNum = 5;
+int batch_set_tiles(Batch& B, int spb);
+void batch_tile_cache_clear();
 int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double tol);
 int batch_remove(Batch& B, int pid);
 int batch_run(Batch& B, int max_cycles, int use_graph, int* active);

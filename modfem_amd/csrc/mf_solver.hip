@@ -33,6 +33,8 @@
 #include <cstdio>
 #include <cstddef>
 #include <cstdlib>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "mf_device.h"
@@ -1132,8 +1134,6 @@ struct BatchArgs {
   const int* nblk;
 };
 
-constexpr int BATCH_SPB = 8;  // target slices per block
-
 __global__ void __launch_bounds__(SB) kb_residual(BatchArgs B) {
   const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
   const SolveArgs A = B.args[p];
@@ -1378,7 +1378,7 @@ static int batch_build(Batch& B) {
     A.stat = B.d_stat + k;
     A.hstat = nullptr;
     const int nsl = (P.n + 63) / 64;
-    const int G = std::max(1, std::min(MAX_BLOCKS, (nsl + BATCH_SPB - 1) / BATCH_SPB));
+    const int G = std::max(1, std::min(MAX_BLOCKS, (nsl + B.spb_used - 1) / B.spb_used));
     nblk[k] = G;
     max_sl = std::max(max_sl, (nsl + G - 1) / G + 1);
     for (int r = 0; r < G; r++) {
@@ -1430,15 +1430,130 @@ static void batch_enqueue_cycle(Batch& B, hipStream_t s) {
   hipLaunchKernelGGL(kb_advance, dim3((np + 255) / 256), dim3(256), 0, s, a, np);
 }
 
+// ---------------------------------------------------------------------
+// Batch tile tuner.  The tile of a batch is the number of 64-row slices one
+// block of a member streams: small tiles give many blocks (more parallel
+// dot-product partials, more work per finishing pass), large tiles fewer,
+// fuller blocks.  Which is fastest depends on how the rows are spread over
+// the members, so the tuner times one restart cycle per candidate on the
+// batch itself and keeps the fastest.  A trial cycle is undone: every
+// member's x is restored from a copy and the GMRES state from the host copy
+// taken before the trial, so tuning does not change any iterate.  Choices
+// are cached per shape: (rows in half-octaves, members in octaves, graph or
+// eager launches), and a shape next to a cached one takes its choice.
+// ---------------------------------------------------------------------
+struct TileChoice {
+  int spb;
+  float ms[kBatchTileNum];
+};
+static std::map<std::tuple<int, int, int>, TileChoice> g_tile_cache;
+
+void batch_tile_cache_clear() { g_tile_cache.clear(); }
+
+int batch_set_tiles(Batch& B, int spb) {
+  if (spb < 0 || spb > 64) {
+    set_error("batch tiles: slices per block must be in [1,64], or 0 to tune");
+    return -1;
+  }
+  B.spb = spb;
+  B.dirty = true;
+  return 0;
+}
+
+static std::tuple<int, int, int> tile_key(const Batch& B, int use_graph) {
+  long long rows = 0;
+  for (const auto& M : B.mem) rows += M.P->n;
+  const int half_oct = (int)std::lround(2.0 * std::log2((double)std::max(1LL, rows)));
+  const int oct = (int)std::lround(std::log2((double)std::max<size_t>(1, B.mem.size())));
+  return std::make_tuple(half_oct, oct, use_graph ? 1 : 0);
+}
+
+// a continuously batched server changes shape every few cycles; a choice
+// cached within one half-octave of rows and one octave of members (same
+// launch mode) is taken instead of new trials
+static std::map<std::tuple<int, int, int>, TileChoice>::iterator tile_near(
+    const std::tuple<int, int, int>& key) {
+  auto best = g_tile_cache.end();
+  int bd = 1 << 30;
+  for (auto it = g_tile_cache.begin(); it != g_tile_cache.end(); ++it) {
+    const auto& k = it->first;
+    if (std::get<2>(k) != std::get<2>(key)) continue;
+    const int dh = std::abs(std::get<0>(k) - std::get<0>(key));
+    const int dm = std::abs(std::get<1>(k) - std::get<1>(key));
+    if (dh > 1 || dm > 1) continue;
+    if (dh + dm < bd) {
+      bd = dh + dm;
+      best = it;
+    }
+  }
+  return best;
+}
+
+static int batch_capture(Batch& B, hipStream_t s) {
+  MF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  batch_enqueue_cycle(B, s);
+  MF_HIP(hipStreamEndCapture(s, &B.graph));
+  MF_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+static int batch_tune(Batch& B, int use_graph, TileChoice& out) {
+  hipStream_t s = stream();
+  const int np = (int)B.mem.size();
+  std::vector<long long> off(np + 1, 0);
+  for (int k = 0; k < np; k++) off[k + 1] = off[k] + B.mem[k].P->n;
+  double* snap = nullptr;
+  MF_HIP(hipMalloc(&snap, (size_t)std::max(1LL, off[np]) * sizeof(double)));
+  for (int k = 0; k < np; k++)
+    MF_HIP(hipMemcpyAsync(snap + off[k], B.mem[k].P->d_x, (size_t)B.mem[k].P->n * sizeof(double),
+                          hipMemcpyDeviceToDevice, s));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  MF_HIP(hipEventCreate(&e0));
+  MF_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  float best = 0.0f;
+  out.spb = kBatchTileCands[0];
+  // the first candidate runs twice; its first (cold) cycle is not counted
+  for (int t = -1; t < kBatchTileNum && rc == 0; t++) {
+    B.spb_used = kBatchTileCands[t < 0 ? 0 : t];
+    rc = batch_build(B);  // also uploads the pre-trial GMRES state
+    if (rc) break;
+    if (use_graph) {
+      if (!B.exec && (rc = batch_capture(B, s))) break;
+      MF_HIP(hipGraphUpload(B.exec, s));
+    }
+    MF_HIP(hipEventRecord(e0, s));
+    if (use_graph)
+      MF_HIP(hipGraphLaunch(B.exec, s));
+    else
+      batch_enqueue_cycle(B, s);
+    MF_HIP(hipEventRecord(e1, s));
+    MF_HIP(hipGetLastError());
+    for (int k = 0; k < np; k++)
+      MF_HIP(hipMemcpyAsync(B.mem[k].P->d_x, snap + off[k], (size_t)B.mem[k].P->n * sizeof(double),
+                            hipMemcpyDeviceToDevice, s));
+    MF_HIP(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    MF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (t < 0) continue;
+    out.ms[t] = ms;
+    if (t == 0 || ms < best) {
+      best = ms;
+      out.spb = B.spb_used;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(snap);
+  B.dirty = true;
+  return rc;
+}
+
 int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
   const int np = (int)B.mem.size();
   if (B.m < 1 || B.m >= MAX_NV) {
     set_error("batch: number of Krylov vectors must be in [1,63]");
     return -1;
-  }
-  if (B.dirty) {
-    const int rc = batch_build(B);
-    if (rc) return rc;
   }
   hipStream_t s = stream();
   // members re-assembled since they joined get their Jacobi refilled in place
@@ -1446,14 +1561,30 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
     if (M.precon && !M.P->dinv_valid && launch_fill_jacobi(*M.P)) return -2;
   int act = 0;
   for (int k = 0; k < np; k++) act += !B.h_stat[k].done;
+  if (B.dirty) {
+    B.spb_tuned = false;
+    if (B.spb > 0) {
+      B.spb_used = B.spb;
+    } else if (act > 0 && max_cycles > 0) {
+      const auto key = tile_key(B, use_graph);
+      auto it = g_tile_cache.find(key);
+      if (it == g_tile_cache.end()) it = tile_near(key);
+      if (it == g_tile_cache.end()) {
+        TileChoice tc{};
+        const int rc = batch_tune(B, use_graph, tc);
+        if (rc) return rc;
+        it = g_tile_cache.emplace(key, tc).first;
+      }
+      B.spb_used = it->second.spb;
+      B.spb_tuned = true;
+      std::copy(it->second.ms, it->second.ms + kBatchTileNum, B.tile_ms);
+    }
+    const int rc = batch_build(B);
+    if (rc) return rc;
+  }
   for (int c = 0; c < max_cycles && act > 0; c++) {
     if (use_graph) {
-      if (!B.exec) {
-        MF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        batch_enqueue_cycle(B, s);
-        MF_HIP(hipStreamEndCapture(s, &B.graph));
-        MF_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
-      }
+      if (!B.exec && batch_capture(B, s)) return -2;
       MF_HIP(hipGraphLaunch(B.exec, s));
     } else {
       batch_enqueue_cycle(B, s);

@@ -129,3 +129,71 @@ def test_member_cannot_be_freed():
     b.remove(p)
     b.free()
     p.free()
+
+
+@pytest.mark.parametrize("spb", [1, 3, 8, 32, 64])
+def test_every_fixed_tile_matches_single_solves(spb):
+    probs = make_problems(SIZES[:4], seed=11)
+    ref = [alone(p) for _, p in probs]
+    b = mf.Batch(krylbas=30)
+    b.set_tiles(spb)
+    for _, p in probs:
+        b.add(p, precon=1, max_iter=4000, tol=1e-8)
+    assert b.run(use_graph=True) == 0
+    t = b.tiles()
+    assert t["spb"] == spb and not t["tuned"]
+    for (x0, i0), (_, p) in zip(ref, probs):
+        st = b.status(p)
+        assert st["done"] and st["converged"], st
+        assert abs(st["iters"] - i0["iters"]) <= max(2, i0["iters"] // 50), (st, i0)
+        assert rel_err(p.solution(), x0) <= 1e-6
+    b.free()
+    for _, p in probs:
+        p.free()
+
+
+def test_tile_bounds_are_checked():
+    b = mf.Batch(krylbas=10)
+    for bad in (-1, 65):
+        with pytest.raises(RuntimeError):
+            b.set_tiles(bad)
+    b.free()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_tuned_tile_changes_no_iterate(use_graph):
+    """trial cycles of the tuner are undone: a tuned batch gives bitwise the
+    iterates of a batch run with the chosen tile fixed from the start; a
+    second batch of the same shape takes the cached choice without trials"""
+    probs = make_problems(SIZES, seed=5)
+    mf.Batch.clear_tile_cache()
+    b = mf.Batch(krylbas=20)
+    for _, p in probs:
+        b.add(p, precon=1, max_iter=2000, tol=1e-9)
+    b.run(max_cycles=1, use_graph=use_graph)
+    t = b.tiles()
+    assert t["tuned"]
+    trials = t["trials"]
+    assert all(ms > 0 for ms in trials.values()), trials
+    assert t["spb"] == min(trials, key=trials.get)
+    for _, p in probs:
+        assert b.status(p)["iters"] <= 20  # one real cycle, none from the trials
+    b.run(use_graph=use_graph)
+    tuned = [(b.status(p)["iters"], p.solution()) for _, p in probs]
+    b.free()
+    f = mf.Batch(krylbas=20)
+    f.set_tiles(t["spb"])
+    for _, p in probs:
+        f.add(p, precon=1, max_iter=2000, tol=1e-9)
+    f.run(use_graph=use_graph)
+    for (i0, x0), (_, p) in zip(tuned, probs):
+        assert f.status(p)["iters"] == i0 and np.array_equal(p.solution(), x0)
+    f.free()
+    c = mf.Batch(krylbas=20)
+    for _, p in probs:
+        c.add(p, precon=1, max_iter=2000, tol=1e-9)
+    c.run(max_cycles=1, use_graph=use_graph)
+    assert c.tiles() == t  # cached: same tile and the same recorded trials
+    c.free()
+    for _, p in probs:
+        p.free()
