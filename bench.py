@@ -397,19 +397,26 @@ def bench_dg(args):
                "op_ms": ms_op, "op_GBs": op_bytes / (ms_op * 1e-3) / 1e9}
         if name in args.dg_solve:
             # block Gauss-Seidel smoothing diverges on mixed-degree systems
-            # (spectral radius 1.5-1.8 of the sweep): block Jacobi there
-            precon = 1 if name == "mixed" else 2
-            if precon == 2:
-                rec["gs_colours"] = dg.set_gs(2)
-            t0 = time.perf_counter()
-            x, info = dg.gmres(precon=precon, krylbas=30, max_iter=args.dg_max_iter, tol=1e-8,
-                               fetch=False)
-            solve_s = time.perf_counter() - t0
-            rec.update({"gmres_precon": "block Jacobi" if precon == 1 else
-                        "block Gauss-Seidel, 2 multicolour sweeps",
-                        "gmres_iters": info["iters"], "gmres_converged": info["converged"],
-                        "gmres_s": solve_s, "gmres_res_rel": info["res_rel"],
-                        "gmres_ms_per_iter": solve_s * 1e3 / max(info["iters"], 1)})
+            # (spectral radius 1.5-1.8 of the sweep): block Jacobi there;
+            # block ILU(0) in element order for every case
+            solves = {}
+            for precon in ((1, 3) if name == "mixed" else (2, 3)):
+                label = {1: "block Jacobi", 2: "block Gauss-Seidel, 2 multicolour sweeps",
+                         3: "block ILU(0), element order"}[precon]
+                if precon == 2:
+                    rec["gs_colours"] = dg.set_gs(2)
+                if precon == 3:
+                    t0 = time.perf_counter()
+                    rec["ilu_levels"] = dg.fill_ilu()
+                    rec["ilu_factor_s"] = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                x, info = dg.gmres(precon=precon, krylbas=30, max_iter=args.dg_max_iter, tol=1e-8,
+                                   fetch=False)
+                solve_s = time.perf_counter() - t0
+                solves[label] = {"gmres_iters": info["iters"], "gmres_converged": info["converged"],
+                                 "gmres_s": solve_s, "gmres_res_rel": info["res_rel"],
+                                 "gmres_ms_per_iter": solve_s * 1e3 / max(info["iters"], 1)}
+            rec["solves"] = solves
         if not args.no_cpu and name != "mixed":
             sample = min(nel, {101: 20000, 202: 4000}.get(int(name), 800))
             rate, _ = dg_cpu_sample(a, int(name), sample)
@@ -686,8 +693,8 @@ def main():
     ap.add_argument("--dg-pdegs", type=int, nargs="+", default=[101, 202, 303])
     ap.add_argument("--dg-max-iter", type=int, default=6000)
     ap.add_argument("--dg-solve", nargs="*", default=[],
-                    help="DG cases solved to 1e-8: mixed (block Jacobi) and/or the uniform degrees "
-                         "101 202 303 (block Gauss-Seidel, 2 sweeps)")
+                    help="DG cases solved to 1e-8: mixed (block Jacobi, block ILU(0)) and/or the "
+                         "uniform degrees 101 202 303 (block Gauss-Seidel with 2 sweeps, block ILU(0))")
     ap.add_argument("--bcrs-n", type=int, default=48, help="BCRS workload: hexes per box edge")
     ap.add_argument("--bcrs-max-iter", type=int, default=3000)
     ap.add_argument("--workload", choices=["box128", "batch", "serve", "dg", "bcrs"], default="box128",

@@ -329,9 +329,27 @@ int mf_dg_get_precon_block(int did, int e, double* inv);
    per application, elements updated colour by colour (a greedy colouring of
    the element graph; the reference sweeps the blocks in their order) */
 int mf_dg_set_gs(int did, int nsweeps, int* ncolors);
+/* block ILU(0) in element order (lar_fill_preconditioner for block storage,
+   MULTI_ILU / BLOCK_ILU, las_block_intf.c:1409-2143): for every element i
+   and its lower face neighbours k in ascending order a_ik <- a_ik inv(D_k),
+   D_i <- D_i - a_ik a_ki, a_ij <- a_ij - a_ik a_kj for neighbours j > k of
+   both; then D_i is inverted (LU with partial pivoting, inverse from the
+   factors).  Rows run level by level of the dependency schedule;
+   nlevels[0..1] (may be null) receive the forward and backward level
+   counts.  Needs a copy of the block rows beside the matrix. */
+int mf_dg_fill_ilu(int did, int* nlevels);
+/* the factored block row of element e in the layout of mf_dg_get_block_row:
+   slot 0 the inverse of U's diagonal block, the other slots the L (lower
+   neighbour) or U (upper neighbour) blocks */
+int mf_dg_get_ilu_row(int did, int e, double* blocks);
+/* v = (LU)^-1 b (lar_perform_rhsub_block, las_block_intf.c:3085-3476),
+   host vectors of mf_dg_layout's ndof entries */
+int mf_dg_ilu_apply(int did, const double* b, double* v);
 /* GMRES (lsr_mkb_core_gmres, lss_mkb_core.c:2317) on the block system with
-   precon 0 (none), 1 (block Jacobi, left) or 2 (block Gauss-Seidel
-   smoothing, the smoothing branch of compreres, lss_mkb_core.c:2001-2054),
+   precon 0 (none), 1 (block Jacobi, left), 2 (block Gauss-Seidel
+   smoothing, the smoothing branch of compreres, lss_mkb_core.c:2001-2054)
+   or 3 (block ILU(0), the BLOCK_ILU branch: (LU)^-1 (b - A x), filled on
+   first use), 
    zero initial guess, stopping on the preconditioned residual relative to its
    initial value; x may be null (the solution stays on the device,
    mf_dg_get_solution) */

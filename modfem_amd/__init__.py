@@ -145,6 +145,9 @@ def _declare(L):
         "mf_dg_get_solution": (I, [I, V]),
         "mf_dg_bench": (I, [I, I, V]),
         "mf_dg_create_mixed": (I, [I, V]),
+        "mf_dg_fill_ilu": (I, [I, V]),
+        "mf_dg_get_ilu_row": (I, [I, I, V]),
+        "mf_dg_ilu_apply": (I, [I, V, V]),
         "mf_dg_layout": (I, [I, V, V, V]),
         # storage seam with the reference's names (lsd_mkb/lah_intf.h)
         "lar_allocate_SM_and_LV": (I, [I, I, I, I, I, I, V, V, V, V]),
@@ -463,6 +466,37 @@ class DgProblem:
         inv = np.zeros((ns, ns))
         check(lib().mf_dg_get_precon_block(self.h, int(e), _p(inv)))
         return inv
+
+    def fill_ilu(self):
+        """block ILU(0) factors on the device; returns the forward and
+        backward level counts of its schedule"""
+        lev = np.zeros(2, np.int32)
+        check(lib().mf_dg_fill_ilu(self.h, _p(lev)))
+        return int(lev[0]), int(lev[1])
+
+    def ilu_row(self, e):
+        """factored block row of element e in the layout of block_row (slot
+        0: inverse of U's diagonal block)"""
+        e = int(e)
+        ns = int(self.xoff[e + 1] - self.xoff[e])
+        if self.ns:
+            blk = np.zeros((6, ns, ns))
+            check(lib().mf_dg_get_ilu_row(self.h, e, _p(blk)))
+            return blk
+        if not hasattr(self, "_fnb"):
+            self._fnb = self.faces()[0]
+        cols = [ns] + [int(self.xoff[n + 1] - self.xoff[n]) if n >= 0 else ns for n in self._fnb[e]]
+        flat = np.zeros(ns * sum(cols))
+        check(lib().mf_dg_get_ilu_row(self.h, e, _p(flat)))
+        cuts = np.r_[0, np.cumsum(cols)] * ns
+        return [flat[cuts[k]:cuts[k + 1]].reshape(cols[k], ns) for k in range(6)]
+
+    def ilu_apply(self, b):
+        """(LU)^-1 b with the device factors"""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        v = np.zeros(self.ndof)
+        check(lib().mf_dg_ilu_apply(self.h, _p(b), _p(v)))
+        return v
 
     def set_gs(self, nsweeps):
         """block Gauss-Seidel sweeps for precon 2; returns the number of colours"""

@@ -473,6 +473,19 @@ struct DgProblem {
   std::vector<int> slot_ptr;
   // Gauss-Seidel colours split by degree: gs_cd_ptr[9 c + d]
   std::vector<int> gs_cd_ptr;
+  // block ILU(0) in element order: factors on the block-row layout (d_ilu),
+  // inverses of U's diagonal blocks (d_ilu_inv, inverse layout), rows by
+  // level of the forward (factor, L solve) and backward (U solve) schedules
+  bool ilu_valid = false;
+  double* d_ilu = nullptr;
+  double* d_ilu_inv = nullptr;
+  std::vector<int> ilu_fptr, ilu_bptr;
+  int* d_ilu_frows = nullptr;
+  int* d_ilu_brows = nullptr;
+  // wave tasks of the solves per level (ilu_ftask[l] .. ilu_ftask[l + 1])
+  std::vector<int> ilu_ftask, ilu_btask;
+  int2* d_ilu_ftasks = nullptr;
+  int2* d_ilu_btasks = nullptr;
 };
 
 namespace {
@@ -524,13 +537,15 @@ int launch_elem(DgProblem& D, int e0, int count, double* d_stiff, double* d_rhs)
 }
 
 void free_dg_solver(DgProblem& D);
+int dg_ilu_solve(DgProblem& D, double* v, const int* stop);
 
 void free_dg(DgProblem& D) {
   free_dg_solver(D);
   void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs, D.d_fnb, D.d_fcode,
                   D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv, D.d_slots,
                   D.d_gs_elems, D.d_work, D.d_grp, D.d_deg, D.d_xoff, D.d_soff, D.d_voff,
-                  D.d_qv9};
+                  D.d_qv9, D.d_ilu, D.d_ilu_inv, D.d_ilu_frows, D.d_ilu_brows,
+                  D.d_ilu_ftasks, D.d_ilu_btasks};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -1460,6 +1475,252 @@ __global__ void __launch_bounds__(256) k_dg_gs(int cnt, const int* elems, DgLayo
   }
 }
 
+
+// ---------------------------------------------------------------------
+// Block ILU(0) of the block system in element order (lar_fill_preconditioner
+// and lar_perform_rhsub for block storage, MULTI_ILU / BLOCK_ILU:
+// lad_block/las_block_intf.c:1409-2143 and 3085-3476).  Row i, for its
+// lower neighbours k in ascending order: a_ik <- a_ik inv(D_k),
+// D_i <- D_i - a_ik a_ki, a_ij <- a_ij - a_ik a_kj for every neighbour
+// j > k of i that is also a neighbour of k; then D_i is inverted (LU with
+// partial pivoting and the inverse from the factors, LAPACK's dgetf2 and
+// dgetri order, as R calls dgetrf + dgetri).  Rows of one level of the
+// schedule depend only on rows of earlier levels.
+// ---------------------------------------------------------------------
+struct DgIluLay {
+  const int* fnb;
+  const int* fcode;
+  const int* deg;
+  int ns;
+  const long long* xoff;
+  const long long* soff;
+  const long long* voff;
+  __device__ int nse(int e) const { return deg ? ns_of_deg(deg[e]) : ns; }
+  __device__ long long xo(int e) const { return xoff ? xoff[e] : (long long)e * ns; }
+  __device__ long long so(int e, int s) const {
+    return soff ? soff[6 * (size_t)e + s] : ((long long)e * 6 + s) * ns * ns;
+  }
+  __device__ long long vo(int e) const { return voff ? voff[e] : (long long)e * ns * ns; }
+};
+
+// sides of element i towards neighbours below (lower) or above i, ordered by
+// neighbour index
+__device__ __forceinline__ int dg_sorted_sides(const int* fnb, int i, bool lower, int* side) {
+  int n = 0;
+  for (int s = 0; s < 5; s++) {
+    const int nb = fnb[5 * (size_t)i + s];
+    if (nb < 0 || (lower ? nb >= i : nb <= i)) continue;
+    int t = n++;
+    while (t > 0 && nb < fnb[5 * (size_t)i + side[t - 1]]) {
+      side[t] = side[t - 1];
+      t--;
+    }
+    side[t] = s;
+  }
+  return n;
+}
+
+constexpr int kIluMax = 40;  // largest block (p = 3 in both directions)
+
+// one row of the factorisation per 64-lane block
+__global__ void __launch_bounds__(64) k_dg_ilu_row(int cnt, const int* rows, DgIluLay Y,
+                                                   double* F, double* Finv, int* bad) {
+  __shared__ double D[kIluMax * kIluMax];
+  __shared__ double T[kIluMax * kIluMax];
+  __shared__ int piv[kIluMax];
+  const int lane = threadIdx.x;
+  const int i = rows[blockIdx.x];
+  const int ni = Y.nse(i);
+  const double* Ai = F + Y.so(i, 0);
+  for (int q = lane; q < ni * ni; q += 64) D[q] = Ai[q];
+  int side[5];
+  const int nl = dg_sorted_sides(Y.fnb, i, true, side);
+  __syncthreads();
+  for (int t = 0; t < nl; t++) {
+    const int s = side[t], k = Y.fnb[5 * (size_t)i + s], nk = Y.nse(k);
+    double* Aik = F + Y.so(i, 1 + s);
+    const double* Dk = Finv + Y.vo(k);
+    // a_ik inv(D_k), sums from 0 over the inner index
+    for (int q = lane; q < ni * nk; q += 64) {
+      const int r = q % ni, c = q / ni;
+      double d = 0.0;
+      for (int l = 0; l < nk; l++) d += Aik[r + (size_t)ni * l] * Dk[l + (size_t)nk * c];
+      T[q] = d;
+    }
+    __syncthreads();
+    for (int q = lane; q < ni * nk; q += 64) Aik[q] = T[q];
+    // D_i -= a_ik a_ki (a_ki: row k's block across the side facing i)
+    const int sk = Y.fcode[5 * (size_t)i + s] & 15;
+    const double* Aki = F + Y.so(k, 1 + sk);
+    for (int q = lane; q < ni * ni; q += 64) {
+      const int r = q % ni, c = q / ni;
+      double d = 0.0;
+      for (int l = 0; l < nk; l++) d += T[r + ni * l] * Aki[l + (size_t)nk * c];
+      D[q] -= d;
+    }
+    // a_ij -= a_ik a_kj for neighbours j > k of both
+    for (int s2 = 0; s2 < 5; s2++) {
+      const int j = Y.fnb[5 * (size_t)i + s2];
+      if (j <= k) continue;
+      int t2 = -1;
+      for (int u = 0; u < 5; u++)
+        if (Y.fnb[5 * (size_t)k + u] == j) t2 = u;
+      if (t2 < 0) continue;
+      const int nj = Y.nse(j);
+      double* Aij = F + Y.so(i, 1 + s2);
+      const double* Akj = F + Y.so(k, 1 + t2);
+      for (int q = lane; q < ni * nj; q += 64) {
+        const int r = q % ni, c = q / ni;
+        double d = 0.0;
+        for (int l = 0; l < nk; l++) d += T[r + ni * l] * Akj[l + (size_t)nk * c];
+        Aij[q] -= d;
+      }
+    }
+    __syncthreads();
+  }
+  // LU of D with partial pivoting (dgetf2): lane = row
+  for (int j = 0; j < ni; j++) {
+    double v = (lane >= j && lane < ni) ? fabs(D[lane + ni * j]) : -1.0;
+    int r = lane;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double v2 = __shfl_xor(v, o);
+      const int r2 = __shfl_xor(r, o);
+      if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
+    }
+    if (!(v > 0.0)) {
+      if (lane == 0) bad[0] = 1;
+      return;
+    }
+    if (lane == 0) piv[j] = r;
+    if (r != j && lane < ni) {  // swap rows j and r, lane = column
+      const double a = D[j + ni * lane];
+      D[j + ni * lane] = D[r + ni * lane];
+      D[r + ni * lane] = a;
+    }
+    __syncthreads();
+    const double rp = 1.0 / D[j + ni * j];
+    if (lane > j && lane < ni) D[lane + ni * j] *= rp;
+    __syncthreads();
+    if (lane > j && lane < ni) {
+      const double lij = D[lane + ni * j];
+      for (int c = j + 1; c < ni; c++) D[lane + ni * c] += lij * -D[j + ni * c];
+    }
+    __syncthreads();
+  }
+  // inv(U) column by column (dtrti2), lane = row
+  for (int j = 0; j < ni; j++) {
+    double x = lane < j ? D[lane + ni * j] : 0.0;
+    const double ujj = 1.0 / D[j + ni * j];
+    for (int jj = 0; jj < j; jj++) {
+      const double t = __shfl(x, jj);
+      if (t != 0.0) {
+        if (lane < jj) x = x + t * D[lane + ni * jj];
+        else if (lane == jj) x = x * D[jj + ni * jj];
+      }
+    }
+    __syncthreads();
+    if (lane < j) D[lane + ni * j] = -ujj * x;
+    if (lane == 0) D[j + ni * j] = ujj;
+    __syncthreads();
+  }
+  // inv(A) = inv(U) inv(L) (dgetri), columns from the last, lane = row
+  for (int j = ni - 1; j >= 0; j--) {
+    const double w = (lane > j && lane < ni) ? D[lane + ni * j] : 0.0;
+    double y = (lane <= j && lane < ni) ? D[lane + ni * j] : 0.0;
+    for (int jj = j + 1; jj < ni; jj++) {
+      const double t = -1.0 * __shfl(w, jj);
+      if (lane < ni) y = y + t * D[lane + ni * jj];
+    }
+    __syncthreads();
+    if (lane < ni) D[lane + ni * j] = y;
+    __syncthreads();
+  }
+  for (int j = ni - 2; j >= 0; j--) {  // column interchanges in reverse order
+    const int jp = piv[j];
+    if (jp != j && lane < ni) {
+      const double a = D[lane + ni * j];
+      D[lane + ni * j] = D[lane + ni * jp];
+      D[lane + ni * jp] = a;
+    }
+    __syncthreads();
+  }
+  double* Di = Finv + Y.vo(i);
+  for (int q = lane; q < ni * ni; q += 64) Di[q] = D[q];
+}
+
+// The solves run wave tasks: a task is up to 64 / ns rows of one level with
+// the same block size ns (rows[first .. first + count)), lane = (row of the
+// task, entry of the row).  task.y = count | ns << 8.
+
+// forward (L) solve of one level, in place: t = v_i less the lower
+// neighbours' terms in ascending neighbour order, one term at a time
+__global__ void __launch_bounds__(256) k_dg_ilu_fwd(int ntask, const int2* tasks, const int* rows,
+                                                    DgIluLay Y, const double* F, double* v,
+                                                    const int* stop) {
+  if (stop && *stop) return;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (g >= ntask) return;
+  const int2 tk = tasks[g];
+  const int ni = tk.y >> 8, el = lane / ni, r = lane - el * ni;
+  if (el >= (tk.y & 255)) return;
+  const int i = rows[tk.x + el];
+  int side[5];
+  const int nl = dg_sorted_sides(Y.fnb, i, true, side);
+  const long long xi = Y.xo(i);
+  double t = v[xi + r];
+  for (int q = 0; q < nl; q++) {
+    const int s = side[q], k = Y.fnb[5 * (size_t)i + s], nk = Y.nse(k);
+    const double* A = F + Y.so(i, 1 + s) + r;
+    const double* xk = v + Y.xo(k);
+#pragma unroll 2
+    for (int c = 0; c < nk; c++) t -= A[(size_t)ni * c] * xk[c];
+  }
+  v[xi + r] = t;
+}
+
+// backward (U) solve of one level, in place: t = v_i less the upper
+// neighbours' terms, then v_i = inv(D_i) t summed from 0
+__global__ void __launch_bounds__(256) k_dg_ilu_bwd(int ntask, const int2* tasks, const int* rows,
+                                                    DgIluLay Y, const double* F,
+                                                    const double* Finv, double* v,
+                                                    const int* stop) {
+  if (stop && *stop) return;
+  __shared__ double Tw[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + wave;
+  if (g >= ntask) return;
+  const int2 tk = tasks[g];
+  const int ni = tk.y >> 8, el = lane / ni, r = lane - el * ni;
+  const bool act = el < (tk.y & 255);
+  const int i = act ? rows[tk.x + el] : 0;
+  const long long xi = act ? Y.xo(i) : 0;
+  double t = 0.0;
+  if (act) {
+    int side[5];
+    const int nu = dg_sorted_sides(Y.fnb, i, false, side);
+    t = v[xi + r];
+    for (int q = 0; q < nu; q++) {
+      const int s = side[q], j = Y.fnb[5 * (size_t)i + s], nj = Y.nse(j);
+      const double* A = F + Y.so(i, 1 + s) + r;
+      const double* xj = v + Y.xo(j);
+#pragma unroll 2
+      for (int c = 0; c < nj; c++) t -= A[(size_t)ni * c] * xj[c];
+    }
+  }
+  Tw[wave][lane] = t;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  if (act) {
+    const double* Di = Finv + Y.vo(i) + r;
+    const double* tw = &Tw[wave][el * ni];
+    double y = 0.0;
+#pragma unroll 2
+    for (int c = 0; c < ni; c++) y += Di[(size_t)ni * c] * tw[c];
+    v[xi + r] = y;
+  }
+}
+
 // w = s_j v and y = w (start of a GS-smoothed operator application)
 __global__ void k_dg_scale2(long long n, const double* v, const double* sc, int j, double* w,
                             double* y, const int* stop) {
@@ -1525,11 +1786,11 @@ struct DgOp : ExtOp {
       if (D->mixed)
         hipLaunchKernelGGL((k_dg_op<NS, true>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
                            (const int*)D->d_grp + c0, layout(), D->d_blk,
-                           precon ? D->d_dinv : nullptr, x_, b_, sc_, j_, stop_, w_, mode_);
+                           precon == 1 ? D->d_dinv : nullptr, x_, b_, sc_, j_, stop_, w_, mode_);
       else
         hipLaunchKernelGGL((k_dg_op<NS, false>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
-                           nullptr, layout(), D->d_blk, precon ? D->d_dinv : nullptr, x_, b_, sc_,
-                           j_, stop_, w_, mode_);
+                           nullptr, layout(), D->d_blk, precon == 1 ? D->d_dinv : nullptr, x_, b_,
+                           sc_, j_, stop_, w_, mode_);
     } else {  // the elements of degree d in colour color_
       const size_t k = 9 * (size_t)color_ + d;
       const int c0 = D->gs_cd_ptr[k], cnt = D->gs_cd_ptr[k + 1] - c0;
@@ -1568,6 +1829,11 @@ struct DgOp : ExtOp {
   }
   int run_all(const double* x, const double* b, const double* sc, int j, const int* stop,
               double* w, int mode) {
+    if (precon == 3) {
+      // BLOCK_ILU branch of compreres: w = (LU)^-1 (b - A x), or (LU)^-1 s_j A v
+      if (op(x, b, sc, j, stop, w, mode)) return -2;
+      return dg_ilu_solve(*D, w, stop);
+    }
     if (precon != 2) return op(x, b, sc, j, stop, w, mode);
     const long long n = D->ndof;
     const int g = (int)((n + 255) / 256);
@@ -1762,6 +2028,130 @@ int fill_block_jacobi(DgProblem& D) {
     return -1;
   }
   D.dinv_valid = true;
+  return 0;
+}
+
+DgIluLay ilu_layout(const DgProblem& D) {
+  return DgIluLay{D.d_fnb, D.d_fcode, D.mixed ? D.d_deg : nullptr, D.mixed ? 0 : D.ns,
+                  D.mixed ? D.d_xoff : nullptr, D.mixed ? D.d_soff : nullptr,
+                  D.mixed ? D.d_voff : nullptr};
+}
+
+// level schedules of the ILU: the forward level of element i is one more than
+// the largest level of its lower neighbours (0 without any), the backward
+// level likewise over its upper neighbours; rows listed level by level
+int build_ilu_levels(DgProblem& D) {
+  if (D.d_ilu_frows) return 0;
+  const int nel = D.nel;
+  std::vector<int> lf(nel, 0), lb(nel, 0);
+  int nlf = 0, nlb = 0;
+  for (int e = 0; e < nel; e++) {
+    int l = 0;
+    for (int s = 0; s < 5; s++) {
+      const int nb = D.h_fnb[5 * (size_t)e + s];
+      if (nb >= 0 && nb < e) l = std::max(l, lf[nb] + 1);
+    }
+    lf[e] = l;
+    nlf = std::max(nlf, l + 1);
+  }
+  for (int e = nel - 1; e >= 0; e--) {
+    int l = 0;
+    for (int s = 0; s < 5; s++) {
+      const int nb = D.h_fnb[5 * (size_t)e + s];
+      if (nb > e) l = std::max(l, lb[nb] + 1);
+    }
+    lb[e] = l;
+    nlb = std::max(nlb, l + 1);
+  }
+  // rows level by level, by block size within a level; wave tasks of up to
+  // 64 / ns rows of one size
+  auto nse = [&D](int e) { return D.mixed ? ns_of_deg(D.deg[e]) : D.ns; };
+  auto group = [&](const std::vector<int>& lev, int nl, std::vector<int>& ptr,
+                   std::vector<int>& rows, std::vector<int>& tptr, std::vector<int2>& tasks) {
+    ptr.assign(nl + 1, 0);
+    for (int e = 0; e < nel; e++) ptr[lev[e] + 1]++;
+    for (int l = 0; l < nl; l++) ptr[l + 1] += ptr[l];
+    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
+    rows.resize(nel);
+    for (int e = 0; e < nel; e++) rows[pos[lev[e]]++] = e;
+    tptr.assign(1, 0);
+    tasks.clear();
+    for (int l = 0; l < nl; l++) {
+      std::stable_sort(rows.begin() + ptr[l], rows.begin() + ptr[l + 1],
+                       [&](int a, int b) { return nse(a) < nse(b); });
+      int q = ptr[l];
+      while (q < ptr[l + 1]) {
+        const int ns = nse(rows[q]), epw = 64 / ns;
+        int c = 0;
+        while (c < epw && q + c < ptr[l + 1] && nse(rows[q + c]) == ns) c++;
+        tasks.push_back(make_int2(q, c | ns << 8));
+        q += c;
+      }
+      tptr.push_back((int)tasks.size());
+    }
+  };
+  std::vector<int> fr, br;
+  std::vector<int2> ft, bt;
+  group(lf, nlf, D.ilu_fptr, fr, D.ilu_ftask, ft);
+  group(lb, nlb, D.ilu_bptr, br, D.ilu_btask, bt);
+  MF_HIP(hipMalloc(&D.d_ilu_frows, (size_t)nel * sizeof(int)));
+  MF_HIP(hipMalloc(&D.d_ilu_brows, (size_t)nel * sizeof(int)));
+  MF_HIP(hipMalloc(&D.d_ilu_ftasks, ft.size() * sizeof(int2)));
+  MF_HIP(hipMalloc(&D.d_ilu_btasks, bt.size() * sizeof(int2)));
+  MF_HIP(hipMemcpy(D.d_ilu_frows, fr.data(), (size_t)nel * sizeof(int), hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(D.d_ilu_brows, br.data(), (size_t)nel * sizeof(int), hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(D.d_ilu_ftasks, ft.data(), ft.size() * sizeof(int2), hipMemcpyHostToDevice));
+  MF_HIP(hipMemcpy(D.d_ilu_btasks, bt.data(), bt.size() * sizeof(int2), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int fill_block_ilu(DgProblem& D) {
+  if (build_ilu_levels(D)) return -2;
+  if (!D.d_ilu) MF_HIP(hipMalloc(&D.d_ilu, (size_t)D.blk_len * sizeof(double)));
+  if (!D.d_ilu_inv) MF_HIP(hipMalloc(&D.d_ilu_inv, (size_t)D.inv_len * sizeof(double)));
+  hipStream_t s = stream();
+  MF_HIP(hipMemcpyAsync(D.d_ilu, D.d_blk, (size_t)D.blk_len * sizeof(double),
+                        hipMemcpyDeviceToDevice, s));
+  int* d_bad = nullptr;
+  MF_HIP(hipMalloc(&d_bad, sizeof(int)));
+  MF_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+  const DgIluLay Y = ilu_layout(D);
+  for (size_t l = 0; l + 1 < D.ilu_fptr.size(); l++) {
+    const int c0 = D.ilu_fptr[l], cnt = D.ilu_fptr[l + 1] - c0;
+    hipLaunchKernelGGL(k_dg_ilu_row, dim3(cnt), dim3(64), 0, s, cnt,
+                       (const int*)D.d_ilu_frows + c0, Y, D.d_ilu, D.d_ilu_inv, d_bad);
+  }
+  int bad = 0;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d_bad);
+  MF_HIP(e);
+  if (bad) {
+    set_error("DG block ILU: a diagonal block is singular");
+    return -1;
+  }
+  D.ilu_valid = true;
+  return 0;
+}
+
+// v = (LU)^-1 v in place: the forward levels, then the backward levels
+int dg_ilu_solve(DgProblem& D, double* v, const int* stop) {
+  hipStream_t s = stream();
+  const DgIluLay Y = ilu_layout(D);
+  for (size_t l = 0; l + 1 < D.ilu_ftask.size(); l++) {
+    const int t0 = D.ilu_ftask[l], nt = D.ilu_ftask[l + 1] - t0;
+    hipLaunchKernelGGL(k_dg_ilu_fwd, dim3((nt + 3) / 4), dim3(256), 0, s, nt,
+                       (const int2*)D.d_ilu_ftasks + t0, (const int*)D.d_ilu_frows, Y,
+                       (const double*)D.d_ilu, v, stop);
+  }
+  for (size_t l = 0; l + 1 < D.ilu_btask.size(); l++) {
+    const int t0 = D.ilu_btask[l], nt = D.ilu_btask[l + 1] - t0;
+    hipLaunchKernelGGL(k_dg_ilu_bwd, dim3((nt + 3) / 4), dim3(256), 0, s, nt,
+                       (const int2*)D.d_ilu_btasks + t0, (const int*)D.d_ilu_brows, Y,
+                       (const double*)D.d_ilu, (const double*)D.d_ilu_inv, v, stop);
+  }
+  MF_HIP(hipGetLastError());
   return 0;
 }
 
@@ -2042,7 +2432,7 @@ int mf_dg_set_coefficients(int did, const int* ind, int coef_model, const double
   M.q = ind[21] ? 2 : (ind[19] ? 1 : 0);
   M.s = ind[22] != 0;
   D->mask = M;
-  D->assembled = D->dinv_valid = false;
+  D->assembled = D->dinv_valid = D->ilu_valid = false;
   for (int k = 0; k < NCOEF; k++) D->cc[k] = coef_const ? coef_const[k] : 0.0;
   D->coef_model = coef_model;
   D->src_model = src_model;
@@ -2142,7 +2532,7 @@ int mf_dg_set_penalty(int did, double penalty_const) {
     return -1;
   }
   D->penalty_const = penalty_const;
-  D->assembled = D->dinv_valid = false;
+  D->assembled = D->dinv_valid = D->ilu_valid = false;
   return 0;
 }
 
@@ -2155,7 +2545,7 @@ int mf_dg_set_boundary_values(int did, const double* val) {
   else
     MF_HIP(hipMemsetAsync(D->d_fval, 0, nf * sizeof(double), stream()));
   MF_HIP(hipStreamSynchronize(stream()));
-  D->assembled = D->dinv_valid = false;
+  D->assembled = D->dinv_valid = D->ilu_valid = false;
   return 0;
 }
 
@@ -2184,7 +2574,7 @@ int mf_dg_assemble(int did) {
   if (rc) return rc;
   MF_HIP(hipStreamSynchronize(stream()));
   D->assembled = true;
-  D->dinv_valid = false;
+  D->dinv_valid = D->ilu_valid = false;
   return 0;
 }
 
@@ -2237,6 +2627,64 @@ int mf_dg_get_precon_block(int did, int e, double* inv) {
   return 0;
 }
 
+int mf_dg_fill_ilu(int did, int* nlevels) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->assembled) {
+    set_error("mf_dg_fill_ilu: the system is not assembled");
+    return -1;
+  }
+  const int rc = fill_block_ilu(*D);
+  if (rc) return rc;
+  if (nlevels) {
+    nlevels[0] = (int)D->ilu_fptr.size() - 1;
+    nlevels[1] = (int)D->ilu_bptr.size() - 1;
+  }
+  return 0;
+}
+
+int mf_dg_get_ilu_row(int did, int e, double* blocks) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->ilu_valid || e < 0 || e >= D->nel || !blocks) {
+    set_error("mf_dg_get_ilu_row: no block ILU filled or element out of range");
+    return -1;
+  }
+  const size_t ns = D->mixed ? ns_of_deg(D->deg[e]) : D->ns, nn = ns * ns;
+  const size_t b0 = D->mixed ? D->soff[6 * (size_t)e] : (size_t)e * 6 * nn;
+  size_t b1 = b0 + 6 * nn;
+  if (D->mixed) {
+    const int nb = D->h_fnb[5 * (size_t)e + 4];
+    b1 = D->soff[6 * (size_t)e + 5] + ns * (nb >= 0 ? ns_of_deg(D->deg[nb]) : ns);
+  }
+  const size_t v0 = D->mixed ? D->voff[e] : (size_t)e * nn;
+  MF_HIP(hipMemcpyAsync(blocks, D->d_ilu_inv + v0, nn * sizeof(double), hipMemcpyDeviceToHost,
+                        stream()));
+  MF_HIP(hipMemcpyAsync(blocks + nn, D->d_ilu + b0 + nn, (b1 - b0 - nn) * sizeof(double),
+                        hipMemcpyDeviceToHost, stream()));
+  MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_dg_ilu_apply(int did, const double* b, double* v) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->ilu_valid || !b || !v) {
+    set_error("mf_dg_ilu_apply: no block ILU filled");
+    return -1;
+  }
+  double* d_v = nullptr;
+  const size_t bytes = (size_t)D->ndof * sizeof(double);
+  MF_HIP(hipMalloc(&d_v, bytes));
+  hipError_t e = hipMemcpyAsync(d_v, b, bytes, hipMemcpyHostToDevice, stream());
+  int rc = e == hipSuccess ? dg_ilu_solve(*D, d_v, nullptr) : 0;
+  if (e == hipSuccess && rc == 0) e = hipMemcpyAsync(v, d_v, bytes, hipMemcpyDeviceToHost, stream());
+  if (e == hipSuccess && rc == 0) e = hipStreamSynchronize(stream());
+  (void)hipFree(d_v);
+  MF_HIP(e);
+  return rc;
+}
+
 int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, double* x,
                 int* iters, double* res_rel, double* resid) {
   DgProblem* D = get_dg(did);
@@ -2245,12 +2693,17 @@ int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, doub
     set_error("mf_dg_gmres: the system is not assembled");
     return -1;
   }
-  if (precon < 0 || precon > 2) {
-    set_error("mf_dg_gmres: precon must be 0 (none), 1 (block Jacobi) or 2 (block Gauss-Seidel)");
+  if (precon < 0 || precon > 3) {
+    set_error("mf_dg_gmres: precon must be 0 (none), 1 (block Jacobi), 2 (block Gauss-Seidel) "
+              "or 3 (block ILU(0))");
     return -1;
   }
-  if (precon >= 1 && !D->dinv_valid && fill_block_jacobi(*D)) return -2;
+  if ((precon == 1 || precon == 2) && !D->dinv_valid && fill_block_jacobi(*D)) return -2;
   if (precon == 2 && build_gs_colors(*D)) return -2;
+  if (precon == 3 && !D->ilu_valid) {
+    const int rc = fill_block_ilu(*D);
+    if (rc) return rc;
+  }
   if (!D->P) {
     D->P = new Problem();
     D->P->n = (int)D->ndof;

@@ -1621,36 +1621,40 @@ void orc_bcrs_residual(int nb, const int *row_ptr, const int *col_ind, const dou
     for (int i = 0; i < 4 * nb; i++) v[i] += b[i];
 }
 
-/* LU factorisation of a column-major 4x4 block with partial pivoting, the
+/* LU factorisation of a column-major n x n block with partial pivoting, the
    unblocked right-looking algorithm of LAPACK's dgetf2 (pivot = first
    largest |a| in the column, rows swapped whole, column scaled by the
    reciprocal pivot, rank-1 update); ipiv 0-based.  Returns 0, or j+1 for a
-   zero pivot in column j */
-int orc_lu4(double *a, int *ipiv) {
+   zero pivot in column j.  dgetrf runs this unblocked path for blocks
+   smaller than LAPACK's block size (64), which covers the 4x4 BCRS blocks
+   and DG blocks of up to 40 shape functions */
+int orc_lun(int n, double *a, int *ipiv) {
   int info = 0;
-  for (int j = 0; j < 4; j++) {
+  for (int j = 0; j < n; j++) {
     int jp = j;
-    double amax = fabs(a[j + 4 * j]);
-    for (int i = j + 1; i < 4; i++)
-      if (fabs(a[i + 4 * j]) > amax) { amax = fabs(a[i + 4 * j]); jp = i; }
+    double amax = fabs(a[j + n * j]);
+    for (int i = j + 1; i < n; i++)
+      if (fabs(a[i + n * j]) > amax) { amax = fabs(a[i + n * j]); jp = i; }
     ipiv[j] = jp;
-    if (a[jp + 4 * j] != 0.0) {
+    if (a[jp + n * j] != 0.0) {
       if (jp != j)
-        for (int c = 0; c < 4; c++) {
-          const double t = a[j + 4 * c]; a[j + 4 * c] = a[jp + 4 * c]; a[jp + 4 * c] = t;
+        for (int c = 0; c < n; c++) {
+          const double t = a[j + n * c]; a[j + n * c] = a[jp + n * c]; a[jp + n * c] = t;
         }
-      const double r = 1.0 / a[j + 4 * j];
-      for (int i = j + 1; i < 4; i++) a[i + 4 * j] *= r;
+      const double r = 1.0 / a[j + n * j];
+      for (int i = j + 1; i < n; i++) a[i + n * j] *= r;
     } else if (!info) {
       info = j + 1;
     }
-    for (int c = j + 1; c < 4; c++) {
-      const double t = -a[j + 4 * c];
-      for (int i = j + 1; i < 4; i++) a[i + 4 * c] += a[i + 4 * j] * t;
+    for (int c = j + 1; c < n; c++) {
+      const double t = -a[j + n * c];
+      for (int i = j + 1; i < n; i++) a[i + n * c] += a[i + n * j] * t;
     }
   }
   return info;
 }
+
+int orc_lu4(double *a, int *ipiv) { return orc_lun(4, a, ipiv); }
 
 /* solve with the factors of orc_lu4 as LAPACK's dgetrs ('N'): row
    interchanges, unit-lower forward and upper backward column sweeps
@@ -1714,44 +1718,46 @@ void orc_bcrs_bj_gs(int nb, const int *row_ptr, const int *col_ind, const int *d
   free(tmp);
 }
 
-/* inverse of a column-major 4x4 block from its orc_lu4 factors, the
+/* inverse of a column-major n x n block from its orc_lun factors, the
    unblocked algorithm of LAPACK's dgetri: inv(U) column by column (dtrti2:
    a_jj = 1/a_jj, then the column above it by the inverted leading triangle
-   (dtrmv, skipping zero entries) scaled by -a_jj), then for j = 3..0 the
+   (dtrmv, skipping zero entries) scaled by -a_jj), then for j = n-1..0 the
    column of L moved out and inv(A)(:, j) -= inv(A)(:, j+1:) L(j+1:, j)
-   (dgemv), then the column interchanges in reverse order */
-void orc_inv4(double *a, const int *ipiv) {
-  for (int j = 0; j < 4; j++) {
-    a[j + 4 * j] = 1.0 / a[j + 4 * j];
-    const double ajj = -a[j + 4 * j];
-    double *x = a + 4 * j;
+   (dgemv), then the column interchanges in reverse order.  n <= 64 */
+void orc_invn(int n, double *a, const int *ipiv) {
+  for (int j = 0; j < n; j++) {
+    a[j + n * j] = 1.0 / a[j + n * j];
+    const double ajj = -a[j + n * j];
+    double *x = a + n * j;
     for (int jj = 0; jj < j; jj++)
       if (x[jj] != 0.0) {
         const double t = x[jj];
-        for (int ii = 0; ii < jj; ii++) x[ii] = x[ii] + t * a[ii + 4 * jj];
-        x[jj] = x[jj] * a[jj + 4 * jj];
+        for (int ii = 0; ii < jj; ii++) x[ii] = x[ii] + t * a[ii + n * jj];
+        x[jj] = x[jj] * a[jj + n * jj];
       }
     for (int ii = 0; ii < j; ii++) x[ii] = ajj * x[ii];
   }
-  double work[4];
-  for (int j = 3; j >= 0; j--) {
-    for (int i = j + 1; i < 4; i++) {
-      work[i] = a[i + 4 * j];
-      a[i + 4 * j] = 0.0;
+  double work[64];
+  for (int j = n - 1; j >= 0; j--) {
+    for (int i = j + 1; i < n; i++) {
+      work[i] = a[i + n * j];
+      a[i + n * j] = 0.0;
     }
-    for (int jj = j + 1; jj < 4; jj++) {
+    for (int jj = j + 1; jj < n; jj++) {
       const double t = -1.0 * work[jj];
-      for (int i = 0; i < 4; i++) a[i + 4 * j] = a[i + 4 * j] + t * a[i + 4 * jj];
+      for (int i = 0; i < n; i++) a[i + n * j] = a[i + n * j] + t * a[i + n * jj];
     }
   }
-  for (int j = 2; j >= 0; j--) {
+  for (int j = n - 2; j >= 0; j--) {
     const int jp = ipiv[j];
     if (jp != j)
-      for (int i = 0; i < 4; i++) {
-        const double t = a[i + 4 * j]; a[i + 4 * j] = a[i + 4 * jp]; a[i + 4 * jp] = t;
+      for (int i = 0; i < n; i++) {
+        const double t = a[i + n * j]; a[i + n * j] = a[i + n * jp]; a[i + n * jp] = t;
       }
   }
 }
+
+void orc_inv4(double *a, const int *ipiv) { orc_invn(4, a, ipiv); }
 
 /* block ILU(0) on the BCRS pattern (lar_fill_preconditioner_bcrs, the
    active variant 2, las_bcrs_intf.c:2181-2640), one thread: for every pivot
@@ -1841,5 +1847,157 @@ void orc_bcrs_rhsub(int nb, const int *rp, const int *ci, const int *diag, const
     for (int r = 0; r < 4; r++) v[4 * i + r] = 0.0;
     for (int c = 0; c < 4; c++)
       for (int r = 0; r < 4; r++) v[4 * i + r] += d[r + 4 * c] * t[c];
+  }
+}
+
+/* ------------------------------------------------------------------
+   Block ILU(0) of a matrix of variable-size blocks (the DG block system),
+   lar_fill_preconditioner for block storage, MULTI_ILU / BLOCK_ILU branch
+   (lad_block/las_block_intf.c:1409-2143), one thread.  Row i has its
+   diagonal block at val + doff[i] (ns[i] x ns[i], column-major) and, for
+   p in nptr[i]..nptr[i+1], the block a_ij, j = nbr[p], at val + ooff[p]
+   (ns[i] x ns[j], column-major with leading dimension ns[i]).  For every
+   row i in order, its lower neighbours k in ascending order (R sorts
+   Llowerneig, :1490-1560):
+     a_ik <- a_ik D_k, D_k the already inverted diagonal block (sums from 0
+             over the inner index, :1880-1900);
+     D_i  <- D_i - a_ik a_ki (:1930-1962);
+     a_ij <- a_ij - a_ik a_kj for every neighbour j > k of i that is also a
+             neighbour of k (:1975-2040);
+   then D_i <- inv(D_i): 1/d for a 1x1 block, else dgetrf + dgetri (:2085-
+   2110).  On exit the lower blocks hold L (without its unit diagonal), the
+   upper blocks U and the diagonal blocks the inverses of U's diagonal.
+   Returns 0, -1 when a_ki is missing (the pattern must be symmetric), or
+   i+1 for a singular diagonal block in row i.  Blocks of at most 64. */
+static int nbr_pos(const int *nptr, const int *nbr, int row, int col) {
+  for (int p = nptr[row]; p < nptr[row + 1]; p++)
+    if (nbr[p] == col) return p;
+  return -1;
+}
+
+static int lower_sorted(const int *nptr, const int *nbr, int i, int *kk, int *pp) {
+  int nl = 0;
+  for (int p = nptr[i]; p < nptr[i + 1]; p++) {
+    const int k = nbr[p];
+    if (k >= i) continue;
+    int t = nl++;
+    while (t > 0 && k < kk[t - 1]) { kk[t] = kk[t - 1]; pp[t] = pp[t - 1]; t--; }
+    kk[t] = k; pp[t] = p;
+  }
+  return nl;
+}
+
+static int upper_sorted(const int *nptr, const int *nbr, int i, int *jj, int *pp) {
+  int nu = 0;
+  for (int p = nptr[i]; p < nptr[i + 1]; p++) {
+    const int j = nbr[p];
+    if (j <= i) continue;
+    int t = nu++;
+    while (t > 0 && j < jj[t - 1]) { jj[t] = jj[t - 1]; pp[t] = pp[t - 1]; t--; }
+    jj[t] = j; pp[t] = p;
+  }
+  return nu;
+}
+
+int orc_block_ilu0(int nb, const int *ns, const int *nptr, const int *nbr,
+                   const long long *doff, const long long *ooff, double *val) {
+  double work[64 * 64];
+  int piv[64];
+  for (int i = 0; i < nb; i++) {
+    const int ni = ns[i], deg = nptr[i + 1] - nptr[i];
+    double *di = val + doff[i];
+    int *kk = (int *)malloc(sizeof(int) * (deg + 1)), *pp = (int *)malloc(sizeof(int) * (deg + 1));
+    const int nl = lower_sorted(nptr, nbr, i, kk, pp);
+    for (int t = 0; t < nl; t++) {
+      const int k = kk[t], nk = ns[k];
+      double *aik = val + ooff[pp[t]];
+      const double *dk = val + doff[k];
+      for (int r = 0; r < ni; r++)
+        for (int c = 0; c < nk; c++) {
+          double d = 0.0;
+          for (int l = 0; l < nk; l++) d += aik[r + ni * l] * dk[l + nk * c];
+          work[r + ni * c] = d;
+        }
+      memcpy(aik, work, sizeof(double) * ni * nk);
+      const int q = nbr_pos(nptr, nbr, k, i);
+      if (q < 0) { free(kk); free(pp); return -1; }
+      const double *aki = val + ooff[q];
+      for (int r = 0; r < ni; r++)
+        for (int c = 0; c < ni; c++) {
+          double d = 0.0;
+          for (int l = 0; l < nk; l++) d += aik[r + ni * l] * aki[l + nk * c];
+          di[r + ni * c] -= d;
+        }
+      for (int p2 = nptr[i]; p2 < nptr[i + 1]; p2++) {
+        const int j = nbr[p2];
+        if (j <= k) continue;
+        const int q2 = nbr_pos(nptr, nbr, k, j);
+        if (q2 < 0) continue;
+        const int nj = ns[j];
+        double *aij = val + ooff[p2];
+        const double *akj = val + ooff[q2];
+        for (int r = 0; r < ni; r++)
+          for (int c = 0; c < nj; c++) {
+            double d = 0.0;
+            for (int l = 0; l < nk; l++) d += aik[r + ni * l] * akj[l + nk * c];
+            aij[r + ni * c] -= d;
+          }
+      }
+    }
+    free(kk); free(pp);
+    if (ni == 1) {
+      if (di[0] == 0.0) return i + 1;
+      di[0] = 1.0 / di[0];
+    } else {
+      if (orc_lun(ni, di, piv)) return i + 1;
+      orc_invn(ni, di, piv);
+    }
+  }
+  return 0;
+}
+
+/* v = (LU)^-1 b with the orc_block_ilu0 factors (lar_perform_rhsub_block,
+   las_block_intf.c:3085-3476): forward, rows in order, t = b_i, then for
+   the lower neighbours in ascending order t_r -= L_ik(r, c) v_k(c) one term
+   at a time (row outer, column inner), v_i = t; backward, rows in reverse,
+   t = v_i less the upper neighbours' terms in the same way, then v_i =
+   inv(D_i) t summed from 0 (column inner).  xoff[i]: first unknown of row
+   i.  b and v may be the same array. */
+void orc_block_ilu_solve(int nb, const int *ns, const long long *xoff, const int *nptr,
+                         const int *nbr, const long long *doff, const long long *ooff,
+                         const double *val, const double *b, double *v) {
+  double t[64];
+  for (int i = 0; i < nb; i++) {
+    const int ni = ns[i], deg = nptr[i + 1] - nptr[i];
+    int *kk = (int *)malloc(sizeof(int) * (deg + 1)), *pp = (int *)malloc(sizeof(int) * (deg + 1));
+    const int nl = lower_sorted(nptr, nbr, i, kk, pp);
+    for (int r = 0; r < ni; r++) t[r] = b[xoff[i] + r];
+    for (int q = 0; q < nl; q++) {
+      const int k = kk[q], nk = ns[k];
+      const double *a = val + ooff[pp[q]];
+      for (int r = 0; r < ni; r++)
+        for (int c = 0; c < nk; c++) t[r] -= a[r + ni * c] * v[xoff[k] + c];
+    }
+    for (int r = 0; r < ni; r++) v[xoff[i] + r] = t[r];
+    free(kk); free(pp);
+  }
+  for (int i = nb - 1; i >= 0; i--) {
+    const int ni = ns[i], deg = nptr[i + 1] - nptr[i];
+    int *jj = (int *)malloc(sizeof(int) * (deg + 1)), *pp = (int *)malloc(sizeof(int) * (deg + 1));
+    const int nu = upper_sorted(nptr, nbr, i, jj, pp);
+    for (int r = 0; r < ni; r++) t[r] = v[xoff[i] + r];
+    for (int q = 0; q < nu; q++) {
+      const int j = jj[q], nj = ns[j];
+      const double *a = val + ooff[pp[q]];
+      for (int r = 0; r < ni; r++)
+        for (int c = 0; c < nj; c++) t[r] -= a[r + ni * c] * v[xoff[j] + c];
+    }
+    const double *d = val + doff[i];
+    for (int r = 0; r < ni; r++) {
+      double y = 0.0;
+      for (int c = 0; c < ni; c++) y += d[r + ni * c] * t[c];
+      v[xoff[i] + r] = y;
+    }
+    free(jj); free(pp);
   }
 }

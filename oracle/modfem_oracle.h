@@ -192,6 +192,13 @@ void orc_bcrs_bj_gs(int nb, const int *row_ptr, const int *col_ind, const int *d
                     const double *val, const double *lu, const int *ipiv, int gs, int use_rhs,
                     int nr_prec, const double *b, double *v);
 void orc_inv4(double *a, const int *ipiv);
+int orc_lun(int n, double *a, int *ipiv);
+void orc_invn(int n, double *a, const int *ipiv);
+int orc_block_ilu0(int nb, const int *ns, const int *nptr, const int *nbr,
+                   const long long *doff, const long long *ooff, double *val);
+void orc_block_ilu_solve(int nb, const int *ns, const long long *xoff, const int *nptr,
+                         const int *nbr, const long long *doff, const long long *ooff,
+                         const double *val, const double *b, double *v);
 int orc_bcrs_ilu0(int nb, const int *rp, const int *ci, const int *diag, double *ilu);
 void orc_bcrs_rhsub(int nb, const int *rp, const int *ci, const int *diag, const double *ilu,
                     const double *b, double *v);

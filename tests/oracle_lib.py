@@ -71,6 +71,10 @@ def oracle():
             "orc_bcrs_ilu0": (I, [I, V, V, V, V]),
             "orc_bcrs_rhsub": (None, [I, V, V, V, V, V, V]),
             "orc_set_threads": (None, [I]),
+            "orc_lun": (I, [I, V, V]),
+            "orc_invn": (None, [I, V, V]),
+            "orc_block_ilu0": (I, [I, V, V, V, V, V, V]),
+            "orc_block_ilu_solve": (None, [I, V, V, V, V, V, V, V, V, V]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -440,3 +444,82 @@ def bcrs_diag_lu(nn, rp, ci, val):
         lu[16 * r:16 * r + 16] = blk
         piv[4 * r:4 * r + 4] = pv
     return diag, lu, piv
+
+
+class BlockSystem:
+    """a matrix of variable-size blocks packed as the oracle's block ILU
+    reads it: row i has its diagonal block (ns[i] x ns[i]) and blocks a_ij
+    for the neighbours j in nbrs[i] (listed in the given order), every block
+    column-major"""
+
+    def __init__(self, ns, nbrs, diag, off):
+        """diag[i]: array [ns_i, ns_i] (row, column); off[i][q]: array
+        [ns_i, ns_j] for j = nbrs[i][q]"""
+        self.nb = len(ns)
+        self.ns = np.asarray(ns, np.int32)
+        self.xoff = np.r_[0, np.cumsum(self.ns)].astype(np.int64)
+        self.nptr = np.r_[0, np.cumsum([len(x) for x in nbrs])].astype(np.int32)
+        self.nbr = np.array([j for x in nbrs for j in x], np.int32)
+        parts, pos = [], 0
+        self.doff = np.zeros(self.nb, np.int64)
+        self.ooff = np.zeros(max(1, len(self.nbr)), np.int64)
+        for i in range(self.nb):
+            self.doff[i] = pos
+            parts.append(np.asarray(diag[i], float).T.ravel())
+            pos += self.ns[i] ** 2
+            for q, j in enumerate(nbrs[i]):
+                self.ooff[self.nptr[i] + q] = pos
+                parts.append(np.asarray(off[i][q], float).T.ravel())
+                pos += self.ns[i] * self.ns[j]
+        self.val = np.ascontiguousarray(np.concatenate(parts))
+
+    def block(self, val, i, j):
+        """block (i, j) of a packed value array as [rows, cols]"""
+        ni = self.ns[i]
+        if i == j:
+            o = self.doff[i]
+            return val[o:o + ni * ni].reshape(ni, ni).T
+        p = next(p for p in range(self.nptr[i], self.nptr[i + 1]) if self.nbr[p] == j)
+        nj = self.ns[j]
+        o = self.ooff[p]
+        return val[o:o + ni * nj].reshape(nj, ni).T
+
+    def dense(self, val=None):
+        val = self.val if val is None else val
+        n = int(self.xoff[-1])
+        A = np.zeros((n, n))
+        for i in range(self.nb):
+            ri = slice(self.xoff[i], self.xoff[i + 1])
+            A[ri, ri] = self.block(val, i, i)
+            for p in range(self.nptr[i], self.nptr[i + 1]):
+                j = self.nbr[p]
+                A[ri, self.xoff[j]:self.xoff[j + 1]] = self.block(val, i, j)
+        return A
+
+    def ilu0(self):
+        """the oracle's block ILU(0) factors (packed like val) and its return code"""
+        f = self.val.copy()
+        rc = oracle().orc_block_ilu0(self.nb, P(self.ns), P(self.nptr), P(self.nbr), P(self.doff),
+                                     P(self.ooff), P(f))
+        return f, rc
+
+    def ilu_solve(self, f, b):
+        v = np.zeros(int(self.xoff[-1]))
+        oracle().orc_block_ilu_solve(self.nb, P(self.ns), P(self.xoff), P(self.nptr), P(self.nbr),
+                                     P(self.doff), P(self.ooff), P(f),
+                                     P(np.ascontiguousarray(b, float)), P(v))
+        return v
+
+    def factors_dense(self, f):
+        """dense L (unit lower) and U (upper, diagonal blocks inv(D_i)^-1) of
+        packed factors"""
+        n = int(self.xoff[-1])
+        L, U = np.eye(n), np.zeros((n, n))
+        for i in range(self.nb):
+            ri = slice(self.xoff[i], self.xoff[i + 1])
+            U[ri, ri] = np.linalg.inv(self.block(f, i, i))
+            for p in range(self.nptr[i], self.nptr[i + 1]):
+                j = self.nbr[p]
+                rj = slice(self.xoff[j], self.xoff[j + 1])
+                (L if j < i else U)[ri, rj] = self.block(f, i, j)
+        return L, U
