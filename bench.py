@@ -562,17 +562,31 @@ def bench_bcrs(args):
            "residual_ms": res_ms, "residual_GBs": res_bytes / (res_ms * 1e-3) / 1e9}
     x = np.zeros(4 * nn)
     it, rr, rs = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
-    for name, precon, nr_prec in [("bj", 1, 1), ("gs", 2, 2), ("gs_colour", 2, 2), ("ilu", 5, 1)]:
-        assert L.lar_allocate_preconditioner(mid, precon, 0) == nn, L.mf_last_error()
+    cases = [("bj", 1, 1, 0), ("gs", 2, 2, 0), ("gs_colour", 2, 2, 0), ("ilu", 5, 1, 0)]
+    if args.bcrs_iluk > 0:
+        cases.append(("ilu%d" % args.bcrs_iluk, 5, 1, args.bcrs_iluk))
+    for name, precon, nr_prec, iluk in cases:
+        t0 = time.perf_counter()
+        assert L.lar_allocate_preconditioner(mid, precon, iluk) == nn, L.mf_last_error()
+        if iluk:
+            ns = ctypes.c_longlong()
+            L.mf_lar_get_ilu_pattern(mid, None, None, ctypes.byref(ns))
+            rec[name + "_stored_blocks"] = ns.value
+            rec[name + "_allocate_s"] = time.perf_counter() - t0
+            print(json.dumps({"bcrs_" + name + "_pattern": {"stored_blocks": ns.value,
+                                                           "s": rec[name + "_allocate_s"]}}),
+                  file=sys.stderr, flush=True)
         if precon == 2:
             rec[name + "_colours"] = L.mf_lar_set_gs_order(mid, 1 if name == "gs_colour" else 0)
         t0 = time.perf_counter()
         if L.lar_fill_preconditioner(mid) != 0:
             raise RuntimeError(L.mf_last_error().decode())
         rec[name + "_fill_ms"] = (time.perf_counter() - t0) * 1e3
+        print(json.dumps({"bcrs_" + name + "_fill_ms": rec[name + "_fill_ms"]}), file=sys.stderr,
+              flush=True)
         if precon == 5:
-            rec["ilu_factor_ms"] = bench(2)
-            rec["ilu_solve_ms"] = bench(3)
+            rec[name + "_factor_ms"] = bench(2)
+            rec[name + "_solve_ms"] = bench(3)
         else:
             rec[name + "_sweep_ms"] = bench(1)
         t0 = time.perf_counter()
@@ -697,6 +711,8 @@ def main():
                          "uniform degrees 101 202 303 (block Gauss-Seidel with 2 sweeps, block ILU(0))")
     ap.add_argument("--bcrs-n", type=int, default=48, help="BCRS workload: hexes per box edge")
     ap.add_argument("--bcrs-max-iter", type=int, default=3000)
+    ap.add_argument("--bcrs-iluk", type=int, default=0,
+                    help="BCRS workload: also solve with block ILU(k) for this k > 0")
     ap.add_argument("--workload", choices=["box128", "batch", "serve", "dg", "bcrs"], default="box128",
                     help="box128: the single large problem (default line); batch: many "
                          "independent problems in one pipeline; serve: a request stream; "

@@ -48,12 +48,15 @@ int lar_assemble_SM_and_LV(int Matrix_id, int Scope, int Nr_dof_bl,
 /* lah_intf.h:184 */
 int lar_free_SM_and_LV(int Matrix_id);
 /* lah_intf.h:192 -- CRS: NO_PRECON (0) and BLOCK_JACOBI (1); BCRS also
-   BLOCK_GS (2) and block ILU(0): MULTI_ILU (4) / BLOCK_ILU (5) with
-   Second_arg (k of ILU(k)) = 0 on a structurally symmetric pattern.
+   BLOCK_GS (2) and block ILU(k): MULTI_ILU (4) / BLOCK_ILU (5) with
+   Second_arg = k on a structurally symmetric pattern; k > 0 grows the
+   pattern by k rings of neighbours (las_bcrs_intf.c:751-1016), with the
+   reference's list limit of (int)(0.5 (k+2)^3 max_nrneig) blocks per row
+   (an error beyond it, where the reference stops).
    Returns the number of diagonal blocks */
 int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg);
 /* lah_intf.h:203 -- CRS: 1/a_ii; BCRS: LU with partial pivoting of every
-   diagonal block (LAPACK dgetrf's algorithm), or the block ILU(0) factors
+   diagonal block (LAPACK dgetrf's algorithm), or the block ILU(k) factors
    (las_bcrs_intf.c:1175-1246, 2181-2640: L blocks a_ki inv(a_ii), U blocks,
    inverted diagonal blocks); <0 for a singular block */
 int lar_fill_preconditioner(int Matrix_id);
@@ -69,7 +72,7 @@ void lar_compute_residual(int Matrix_id, int Use_rhs, int Ini_zero, int Ndof,
    forward sweeps */
 void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero,
                                      int Nr_prec, int Ndof, double* V, double* B);
-/* lah_intf.h:269 -- BCRS with ILU(0): V = (LU)^-1 B (B NULL: the stored
+/* lah_intf.h:269 -- BCRS with ILU(k): V = (LU)^-1 B (B NULL: the stored
    rhs), forward reduction and back substitution in the reference's order
    (las_bcrs_intf.c:3319-3510), bitwise its result.  Empty for CRS storage in
    the reference (las_crs_intf.c:1019-1057): records an error there */
@@ -97,8 +100,8 @@ int mf_lar_set_gs_order(int Matrix_id, int order);
 
 /* HIP-event time (ms, averaged over reps) of one BCRS operation on
    device-resident vectors: what 0 residual, 1 one forward sweep of the block
-   Jacobi / Gauss-Seidel preconditioner, 2 the ILU(0) factorisation, 3 one
-   ILU(0) solve (no reference counterpart; measurement only) */
+   Jacobi / Gauss-Seidel preconditioner, 2 the ILU(k) factorisation, 3 one
+   ILU(k) solve (no reference counterpart; measurement only) */
 int mf_lar_bench(int Matrix_id, int what, int reps, double* ms);
 
 /* Inspection (no reference counterpart; lar_get_SM_and_LV_crs_from_crs /
@@ -111,9 +114,14 @@ int mf_lar_get_sizes(int Matrix_id, int* storage, int* block_size,
 int mf_lar_get_system(int Matrix_id, int* row_ptr, int* col_ind, double* val,
                       double* rhs);
 /* BCRS: LU factors (16 per block row, column-major) and 0-based pivots
-   (4 per block row) of the filled diagonal-block preconditioner; with ILU(0)
-   the factors of every stored block (16 each) and no pivots */
+   (4 per block row) of the filled diagonal-block preconditioner; with ILU(k)
+   the factors of every block of its pattern (16 each, mf_lar_get_ilu_pattern)
+   and no pivots */
 int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv);
+/* BCRS block ILU(k): the block pattern its factors live on (k > 0: the grown
+   pattern; k = 0: the matrix's), row_ptr [nrblocks+1] and col_ind
+   [*nr_stored], either may be NULL; returns k */
+int mf_lar_get_ilu_pattern(int Matrix_id, int* row_ptr, int* col_ind, long long* nr_stored);
 
 #ifdef __cplusplus
 }

@@ -166,6 +166,7 @@ def _declare(L):
         "mf_lar_get_system": (I, [I, V, V, V, V]),
         "mf_lar_get_sizes": (I, [I, V, V, V, V]),
         "mf_lar_get_block_precon": (I, [I, V, V]),
+        "mf_lar_get_ilu_pattern": (I, [I, V, V, V]),
         "mf_lar_gmres": (I, [I, I, I, D, I, I, V, V, V, V]),
         "mf_lar_bench": (I, [I, I, I, V]),
         "mf_lar_set_gs_order": (I, [I, I]),

@@ -373,7 +373,7 @@ __global__ void k_bcrs_bwd_level(int cnt, const int* rows, const int* rp, const 
 //           other side from v, + b, then the diagonal block's LU solve) -> out
 //   mode 1: ILU forward row, y = b - L y (lower blocks from v, in place)
 //   mode 2: ILU backward row, y = inv(a_ii) (y - U y) (upper blocks from v)
-constexpr int kWaveBlocks = 64;
+constexpr int kWaveBlocks = 128;
 
 __global__ void __launch_bounds__(64) k_bcrs_rows_wave(
     int cnt, const int* rows, int mode, int dir, const int* rp, const int* ci, const int* diag,
@@ -442,6 +442,7 @@ __global__ void __launch_bounds__(64) k_bcrs_ilu_wave(int cnt, const int* rows, 
                                                        int* bad) {
   __shared__ double A[kWaveBlocks * 16];
   __shared__ int C[kWaveBlocks];
+  __shared__ int Pos[kWaveBlocks];
   const int t = blockIdx.x, lane = threadIdx.x;
   if (t >= cnt) return;
   const int k = rows[t];
@@ -462,12 +463,22 @@ __global__ void __launch_bounds__(64) k_bcrs_ilu_wave(int cnt, const int* rows, 
     if (lane < 16) A[16 * q + lane] = pv;
     __syncthreads();
     const int u0 = diag[i] + 1, nu = rp[i + 1] - u0;
+    // position in row k of every block of row i's upper part (-1: absent),
+    // by bisection over the sorted columns after q
+    for (int jj = lane; jj < nu; jj += 64) {
+      const int j = ci[u0 + jj];
+      int lo = q + 1, hi = nbk;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (C[mid] < j) lo = mid + 1;
+        else hi = mid;
+      }
+      Pos[jj] = (lo < nbk && C[lo] == j) ? lo : -1;
+    }
+    __syncthreads();
     for (int idx = lane; idx < 16 * nu; idx += 64) {
       const int jj = idx >> 4, e = idx & 15, rr = e & 3, cc = e >> 2;
-      const int j = ci[u0 + jj];
-      int pos = -1;
-      for (int z = q + 1; z < nbk && C[z] <= j; z++)
-        if (C[z] == j) pos = z;
+      const int pos = Pos[jj];
       if (pos < 0) continue;
       const double* aij = ilu + 16 * (size_t)(u0 + jj);
       double c4 = A[16 * pos + e];
@@ -639,22 +650,93 @@ int bcrs_sweeps(BcrsDev& M, int gs, int nr_prec, const double* b, double* v, dou
   return 0;
 }
 
+// ILU(k) pattern of lar_allocate_preconditioner_bcrs (las_bcrs_intf.c:751-
+// 1016): row i starts from its stored blocks; each of k rings appends, for
+// every column the previous ring added (in list order), the columns of that
+// column's row not yet listed; the list is then sorted.  A row holds at most
+// cap columns, as the reference's fixed-length list does (it stops the
+// program when a row needs more; here the row is reported)
+int bcrs_iluk_pattern(int nb, const std::vector<int>& rp, const std::vector<int>& ci, int k,
+                      int cap, std::vector<int>& prp, std::vector<int>& pci, int* bad_row) {
+  std::vector<int> seen(nb, -1), list;
+  prp.assign(nb + 1, 0);
+  pci.clear();
+  for (int i = 0; i < nb; i++) {
+    list.clear();
+    auto put = [&](int c) {
+      if (seen[c] == i) return true;
+      if ((int)list.size() == cap) return false;
+      seen[c] = i;
+      list.push_back(c);
+      return true;
+    };
+    bool ok = true;
+    for (int s = rp[i]; s < rp[i + 1] && ok; s++) ok = put(ci[s]);
+    size_t first = 0, last = list.size();
+    for (int ring = 0; ring < k && ok; ring++) {
+      for (size_t q = first; q < last && ok; q++) {
+        const int r = list[q];
+        for (int s = rp[r]; s < rp[r + 1] && ok; s++) ok = put(ci[s]);
+      }
+      first = last;
+      last = list.size();
+    }
+    if (!ok) {
+      if (bad_row) *bad_row = i;
+      return -1;
+    }
+    std::sort(list.begin(), list.end());
+    pci.insert(pci.end(), list.begin(), list.end());
+    prp[i + 1] = (int)pci.size();
+  }
+  return 0;
+}
+
+// ILU values on the grown pattern: the matrix block, or zero for fill
+__global__ void k_bcrs_ilu_gather(long long n, const int* src, const double* val, double* ilu) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int s = src[q >> 4];
+  ilu[q] = s >= 0 ? val[16 * (long long)s + (q & 15)] : 0.0;
+}
+
+// the pattern the factors live on: the matrix's for ILU(0), the grown one
+// for ILU(k > 0)
+struct IluPattern {
+  const int *rp, *ci, *diag;
+  int max_row;
+  const std::vector<int>* lev_ptr;
+  int* const* lev_rows;
+};
+static IluPattern ilu_pattern(const BcrsDev& M) {
+  if (M.ilu_k > 0)
+    return {M.d_ilu_rp, M.d_ilu_ci, M.d_ilu_diag, M.ilu_max_row, M.ilu_lev_ptr, M.d_ilu_lev_rows};
+  return {M.d_rp, M.d_ci, M.d_diag, M.max_row, M.lev_ptr, M.d_lev_rows};
+}
+
 int bcrs_ilu_factor(BcrsDev& M) {
   if (M.nb == 0) return 0;
-  MF_HIP(hipMemcpyAsync(M.d_ilu, M.d_val, (size_t)M.nblk * 16 * sizeof(double),
-                        hipMemcpyDeviceToDevice, stream()));
+  if (M.ilu_k > 0) {
+    const long long n = M.ilu_nblk * 16;
+    hipLaunchKernelGGL(k_bcrs_ilu_gather, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       stream(), n, (const int*)M.d_ilu_src, (const double*)M.d_val, M.d_ilu);
+  } else {
+    MF_HIP(hipMemcpyAsync(M.d_ilu, M.d_val, (size_t)M.nblk * 16 * sizeof(double),
+                          hipMemcpyDeviceToDevice, stream()));
+  }
+  const IluPattern Q = ilu_pattern(M);
   int* d_bad = nullptr;
   MF_HIP(hipMalloc(&d_bad, sizeof(int)));
   MF_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), stream()));
-  const std::vector<int>& lp = M.lev_ptr[0];
+  const std::vector<int>& lp = Q.lev_ptr[0];
   for (size_t l = 0; l + 1 < lp.size(); l++) {
     const int cnt = lp[l + 1] - lp[l];
-    if (M.max_row <= kWaveBlocks)
+    if (Q.max_row <= kWaveBlocks)
       hipLaunchKernelGGL(k_bcrs_ilu_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
-                         M.d_lev_rows[0] + lp[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, d_bad);
+                         Q.lev_rows[0] + lp[l], Q.rp, Q.ci, Q.diag, M.d_ilu, d_bad);
     else
       hipLaunchKernelGGL(k_bcrs_ilu_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                         M.d_lev_rows[0] + lp[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, d_bad);
+                         Q.lev_rows[0] + lp[l], Q.rp, Q.ci, Q.diag, M.d_ilu, d_bad);
   }
   MF_HIP(hipGetLastError());
   int bad = 0;
@@ -670,27 +752,28 @@ int bcrs_ilu_factor(BcrsDev& M) {
 
 int bcrs_ilu_solve(const BcrsDev& M, const double* b, double* v, const int* stop) {
   if (M.nb == 0) return 0;
-  const std::vector<int>& lf = M.lev_ptr[0];
+  const IluPattern Q = ilu_pattern(M);
+  const std::vector<int>& lf = Q.lev_ptr[0];
   for (size_t l = 0; l + 1 < lf.size(); l++) {
     const int cnt = lf[l + 1] - lf[l];
-    if (M.max_row <= kWaveBlocks)
+    if (Q.max_row <= kWaveBlocks)
       hipLaunchKernelGGL(k_bcrs_rows_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
-                         M.d_lev_rows[0] + lf[l], 1, 0, M.d_rp, M.d_ci, M.d_diag, M.d_ilu,
+                         Q.lev_rows[0] + lf[l], 1, 0, Q.rp, Q.ci, Q.diag, M.d_ilu,
                          nullptr, nullptr, b, v, v, v, stop);
     else
       hipLaunchKernelGGL(k_bcrs_fwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                         M.d_lev_rows[0] + lf[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, b, v, stop);
+                         Q.lev_rows[0] + lf[l], Q.rp, Q.ci, Q.diag, M.d_ilu, b, v, stop);
   }
-  const std::vector<int>& lb = M.lev_ptr[1];
+  const std::vector<int>& lb = Q.lev_ptr[1];
   for (size_t l = 0; l + 1 < lb.size(); l++) {
     const int cnt = lb[l + 1] - lb[l];
-    if (M.max_row <= kWaveBlocks)
+    if (Q.max_row <= kWaveBlocks)
       hipLaunchKernelGGL(k_bcrs_rows_wave, dim3(cnt), dim3(64), 0, stream(), cnt,
-                         M.d_lev_rows[1] + lb[l], 2, 0, M.d_rp, M.d_ci, M.d_diag, M.d_ilu,
+                         Q.lev_rows[1] + lb[l], 2, 0, Q.rp, Q.ci, Q.diag, M.d_ilu,
                          M.d_ilu, nullptr, nullptr, v, v, v, stop);
     else
       hipLaunchKernelGGL(k_bcrs_bwd_level, dim3((cnt + 63) / 64), dim3(64), 0, stream(), cnt,
-                         M.d_lev_rows[1] + lb[l], M.d_rp, M.d_ci, M.d_diag, M.d_ilu, v, stop);
+                         Q.lev_rows[1] + lb[l], Q.rp, Q.ci, Q.diag, M.d_ilu, v, stop);
   }
   MF_HIP(hipGetLastError());
   return 0;

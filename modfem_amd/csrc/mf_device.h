@@ -309,7 +309,20 @@ struct BcrsDev {
   int gs_order = 0;
   std::vector<int> col_ptr;
   int* d_col_rows = nullptr;
+  // block ILU(k), k > 0: the grown pattern (ilu_k = 0 factors on the
+  // matrix's own pattern), the matrix block behind every ILU block (-1 for
+  // fill) and the level schedules of that pattern
+  int ilu_k = 0;
+  long long ilu_nblk = 0;
+  int ilu_max_row = 0;
+  int *d_ilu_rp = nullptr, *d_ilu_ci = nullptr, *d_ilu_diag = nullptr, *d_ilu_src = nullptr;
+  std::vector<int> ilu_lev_ptr[2];
+  int* d_ilu_lev_rows[2] = {nullptr, nullptr};
 };
+// ILU(k) pattern growth (host, lar_allocate_preconditioner_bcrs); returns 0
+// or the row (>= 0) whose list outgrew cap
+int bcrs_iluk_pattern(int nb, const std::vector<int>& rp, const std::vector<int>& ci, int k,
+                      int cap, std::vector<int>& prp, std::vector<int>& pci, int* bad_row);
 // greedy colouring of the block graph in row order; rows grouped by colour
 int bcrs_colors(int nb, const std::vector<int>& rp, const std::vector<int>& ci,
                 std::vector<int>& col_ptr, std::vector<int>& rows);

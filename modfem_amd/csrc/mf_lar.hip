@@ -44,6 +44,7 @@ struct LarMatrix {
   std::unique_ptr<Problem> P;   // CRS, block size 1
   std::unique_ptr<BcrsDev> B;   // BCRS, 4x4 blocks
   std::vector<int> rp, ci, diag;  // BCRS block pattern on the host
+  std::vector<int> ilu_rp, ilu_ci;  // BCRS ILU(k > 0) pattern on the host
   std::vector<double> h_val, h_rhs;  // host mirror for per-element assembly
   bool dirty = false;
   bool filled = false;  // BCRS: diagonal-block factors current
@@ -185,16 +186,24 @@ void free_gmres(LarMatrix& M) {
 }
 
 void free_bcrs_precon(BcrsDev& B) {
-  void* ptrs[] = {B.d_diag, B.d_lu, B.d_piv, B.d_ilu, B.d_lev_rows[0], B.d_lev_rows[1]};
+  void* ptrs[] = {B.d_diag, B.d_lu, B.d_piv, B.d_ilu, B.d_lev_rows[0], B.d_lev_rows[1],
+                  B.d_ilu_rp, B.d_ilu_ci, B.d_ilu_diag, B.d_ilu_src, B.d_ilu_lev_rows[0],
+                  B.d_ilu_lev_rows[1]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   B.d_diag = nullptr;
   B.d_lu = nullptr;
   B.d_piv = nullptr;
   B.d_ilu = nullptr;
+  B.d_ilu_rp = B.d_ilu_ci = B.d_ilu_diag = B.d_ilu_src = nullptr;
+  B.ilu_k = 0;
+  B.ilu_nblk = 0;
+  B.ilu_max_row = 0;
   for (int d = 0; d < 2; d++) {
     B.d_lev_rows[d] = nullptr;
     B.lev_ptr[d].clear();
+    B.d_ilu_lev_rows[d] = nullptr;
+    B.ilu_lev_ptr[d].clear();
   }
 }
 
@@ -459,9 +468,10 @@ int lar_free_SM_and_LV(int Matrix_id) {
 
 // lar_allocate_preconditioner (las_crs_intf.c:658-700): Jacobi or none;
 // BCRS (las_bcrs_intf.c:630-1016): block Jacobi or block Gauss-Seidel with
-// the diagonal blocks' LU factors, or block ILU(0) (MULTI_ILU / BLOCK_ILU with
-// Second_arg = k = 0) on the matrix pattern, which must be structurally
-// symmetric; the level schedules of the sweeps are built here
+// the diagonal blocks' LU factors, or block ILU(k) (MULTI_ILU / BLOCK_ILU with
+// Second_arg = k): k = 0 on the matrix pattern, k > 0 on the pattern grown
+// by k rings of neighbours; the matrix pattern must be structurally
+// symmetric.  The level schedules of the sweeps are built here
 int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
@@ -472,8 +482,8 @@ int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
                 "BLOCK_GS, MULTI_ILU and BLOCK_ILU");
       return -1;
     }
-    if (ilu && Second_arg != 0) {
-      set_error("lar_allocate_preconditioner: ILU(k) is available for k = 0 only");
+    if (ilu && Second_arg < 0) {
+      set_error("lar_allocate_preconditioner: ILU(k) needs k >= 0");
       return -1;
     }
     BcrsDev& B = *M->B;
@@ -507,7 +517,57 @@ int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
         }
     MF_HIP(hipMalloc(&B.d_diag, std::max(B.nb, 1) * sizeof(int)));
     MF_HIP(hipMemcpy(B.d_diag, M->diag.data(), B.nb * sizeof(int), hipMemcpyHostToDevice));
-    if (ilu) {
+    M->ilu_rp.clear();
+    M->ilu_ci.clear();
+    if (ilu && Second_arg > 0) {
+      // the reference's list holds (int)(0.5 (k+2)^3 max_nrneig) columns,
+      // max_nrneig the most off-diagonal blocks of a row
+      const int k = Second_arg;
+      int max_off = 0;
+      for (int r = 0; r < B.nb; r++) max_off = std::max(max_off, M->rp[r + 1] - M->rp[r] - 1);
+      const int cap = (int)(0.5 * (k + 2) * (k + 2) * (k + 2) * max_off);
+      int bad_row = -1;
+      if (bcrs_iluk_pattern(B.nb, M->rp, M->ci, k, cap, M->ilu_rp, M->ilu_ci, &bad_row)) {
+        set_error("lar_allocate_preconditioner: the ILU(" + std::to_string(k) + ") pattern of block row " +
+                  std::to_string(bad_row) + " exceeds the reference's list of " +
+                  std::to_string(cap) + " neighbours");
+        free_bcrs_precon(B);
+        M->precon = -1;
+        return -1;
+      }
+      const std::vector<int>& prp = M->ilu_rp;
+      const std::vector<int>& pci = M->ilu_ci;
+      std::vector<int> pdiag(B.nb), src(pci.size());
+      int max_row = 0;
+      for (int r = 0; r < B.nb; r++) {
+        max_row = std::max(max_row, prp[r + 1] - prp[r]);
+        int s = M->rp[r];
+        for (int s2 = prp[r]; s2 < prp[r + 1]; s2++) {  // both rows sorted
+          while (s < M->rp[r + 1] && M->ci[s] < pci[s2]) s++;
+          src[s2] = (s < M->rp[r + 1] && M->ci[s] == pci[s2]) ? s : -1;
+          if (pci[s2] == r) pdiag[r] = s2;
+        }
+      }
+      B.ilu_k = k;
+      B.ilu_nblk = (long long)pci.size();
+      B.ilu_max_row = max_row;
+      MF_HIP(hipMalloc(&B.d_ilu_rp, prp.size() * sizeof(int)));
+      MF_HIP(hipMalloc(&B.d_ilu_ci, std::max<size_t>(pci.size(), 1) * sizeof(int)));
+      MF_HIP(hipMalloc(&B.d_ilu_diag, std::max(B.nb, 1) * sizeof(int)));
+      MF_HIP(hipMalloc(&B.d_ilu_src, std::max<size_t>(src.size(), 1) * sizeof(int)));
+      MF_HIP(hipMemcpy(B.d_ilu_rp, prp.data(), prp.size() * sizeof(int), hipMemcpyHostToDevice));
+      MF_HIP(hipMemcpy(B.d_ilu_ci, pci.data(), pci.size() * sizeof(int), hipMemcpyHostToDevice));
+      MF_HIP(hipMemcpy(B.d_ilu_diag, pdiag.data(), B.nb * sizeof(int), hipMemcpyHostToDevice));
+      MF_HIP(hipMemcpy(B.d_ilu_src, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice));
+      for (int dir = 0; dir < 2; dir++) {
+        std::vector<int> rows;
+        bcrs_levels(B.nb, prp, pci, pdiag, dir, B.ilu_lev_ptr[dir], rows);
+        MF_HIP(hipMalloc(&B.d_ilu_lev_rows[dir], std::max(B.nb, 1) * sizeof(int)));
+        MF_HIP(hipMemcpy(B.d_ilu_lev_rows[dir], rows.data(), B.nb * sizeof(int),
+                         hipMemcpyHostToDevice));
+      }
+      MF_HIP(hipMalloc(&B.d_ilu, std::max<size_t>(B.ilu_nblk, 1) * BLS * BLS * sizeof(double)));
+    } else if (ilu) {
       MF_HIP(hipMalloc(&B.d_ilu, std::max<size_t>(B.nblk, 1) * BLS * BLS * sizeof(double)));
     } else {
       MF_HIP(hipMalloc(&B.d_lu, std::max(B.nb, 1) * BLS * BLS * sizeof(double)));
@@ -661,15 +721,15 @@ void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero, i
 }
 
 // lar_perform_rhsub: V = (LU)^-1 B (B null: the stored rhs) with the BCRS
-// block ILU(0) factors (las_bcrs_intf.c:3319-3510, variant 0: forward
+// block ILU(k) factors (las_bcrs_intf.c:3319-3510, variant 0: forward
 // reduction and back substitution in the reference's row order, bitwise its
 // result).  The CRS body is empty in the reference (las_crs_intf.c:1019-1057):
 // a no-op with a diagnostic there
 void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M || !M->B || !M->B->d_ilu || !M->filled || Ndof != M->ndof()) {
-    set_error("lar_perform_rhsub: needs a BCRS matrix with filled ILU(0) factors");
-    fprintf(stderr, "lar_perform_rhsub: needs a BCRS matrix with filled ILU(0) factors\n");
+    set_error("lar_perform_rhsub: needs a BCRS matrix with filled ILU factors");
+    fprintf(stderr, "lar_perform_rhsub: needs a BCRS matrix with filled ILU factors\n");
     return;
   }
   if (flush(*M)) return;
@@ -722,7 +782,7 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol, int nr_pr
 // HIP-event times on the library stream of one BCRS operation, averaged over
 // reps, on device-resident vectors: what 0 residual v = b - A x, 1 one
 // forward sweep of the allocated block Jacobi / Gauss-Seidel preconditioner,
-// 2 the ILU(0) factorisation, 3 one ILU(0) solve
+// 2 the ILU(k) factorisation, 3 one ILU(k) solve
 int mf_lar_bench(int Matrix_id, int what, int reps, double* ms) {
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
@@ -813,7 +873,7 @@ int mf_lar_get_sizes(int Matrix_id, int* storage, int* block_size, int* nrblocks
 
 // host copies of the BCRS diagonal-block LU factors (16 per block row,
 // column-major, unit lower part below the diagonal) and 0-based pivots (4 per
-// block row: row j was interchanged with row piv[j]); for block ILU(0) the
+// block row: row j was interchanged with row piv[j]); for block ILU(k) the
 // factors of every stored block (16 each: L blocks below the diagonal, U
 // blocks above, inverted U diagonal blocks) and no pivots
 int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv) {
@@ -824,14 +884,34 @@ int mf_lar_get_block_precon(int Matrix_id, double* lu, int* piv) {
     return -1;
   }
   const BcrsDev& B = *M->B;
-  if (B.d_ilu) {  // ILU(0): the factors of every stored block
-    if (lu && B.nblk)
-      MF_HIP(hipMemcpy(lu, B.d_ilu, B.nblk * BLS * BLS * sizeof(double), hipMemcpyDeviceToHost));
+  if (B.d_ilu) {  // ILU(k): the factors of every block of its pattern
+    const long long nf = B.ilu_k > 0 ? B.ilu_nblk : B.nblk;
+    if (lu && nf)
+      MF_HIP(hipMemcpy(lu, B.d_ilu, nf * BLS * BLS * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
   }
   if (lu) MF_HIP(hipMemcpy(lu, B.d_lu, (size_t)B.nb * BLS * BLS * sizeof(double), hipMemcpyDeviceToHost));
   if (piv) MF_HIP(hipMemcpy(piv, B.d_piv, (size_t)B.nb * BLS * sizeof(int), hipMemcpyDeviceToHost));
   return 0;
+}
+
+// the block pattern of the allocated BCRS ILU(k) factors: for k > 0 the
+// grown pattern, for k = 0 the matrix's own; row_ptr [nrblocks+1], col_ind
+// [*nr_stored] (either may be null); returns k
+int mf_lar_get_ilu_pattern(int Matrix_id, int* row_ptr, int* col_ind, long long* nr_stored) {
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M) return -3;
+  if (!M->B || !M->B->d_ilu) {
+    set_error("mf_lar_get_ilu_pattern: needs a BCRS matrix with block ILU allocated");
+    return -1;
+  }
+  const bool grown = M->B->ilu_k > 0;
+  const std::vector<int>& rp = grown ? M->ilu_rp : M->rp;
+  const std::vector<int>& ci = grown ? M->ilu_ci : M->ci;
+  if (row_ptr) memcpy(row_ptr, rp.data(), rp.size() * sizeof(int));
+  if (col_ind) memcpy(col_ind, ci.data(), ci.size() * sizeof(int));
+  if (nr_stored) *nr_stored = (long long)ci.size();
+  return M->B->ilu_k;
 }
 
 // host copies of the stored system (lar_get_SM_and_LV_crs_from_crs,

@@ -1814,6 +1814,55 @@ int orc_bcrs_ilu0(int nb, const int *rp, const int *ci, const int *diag, double 
   return 0;
 }
 
+/* pattern of block ILU(k), k > 0 (lar_allocate_preconditioner_bcrs,
+   las_bcrs_intf.c:751-1016): row i starts from its own stored blocks; each of
+   k rings appends, for every column added by the previous ring (in list
+   order), the columns of that column's original row not yet listed; the
+   row's list is then sorted.  A row holds at most cap columns (R's list
+   length, (int)(0.5 (k+2)^3 max_nrneig)); R stops when a row needs more, and
+   this returns -1 - row then.  rp2 [nb+1] is always filled; ci2 (may be
+   NULL) receives the columns.  Returns the number of blocks. */
+long orc_bcrs_iluk_pattern(int nb, const int *rp, const int *ci, int k, int cap, int *rp2,
+                           int *ci2) {
+  int *list = (int *)malloc(sizeof(int) * (cap > 0 ? cap : 1));
+  long total = 0;
+  rp2[0] = 0;
+  for (int i = 0; i < nb; i++) {
+    int n = 0;
+    for (int s = rp[i]; s < rp[i + 1]; s++) {
+      if (n == cap) { free(list); return -1 - i; }
+      list[n++] = ci[s];
+    }
+    int first = 0, last = n;
+    for (int ring = 0; ring < k; ring++) {
+      for (int q = first; q < last; q++) {
+        const int r = list[q];
+        for (int s = rp[r]; s < rp[r + 1]; s++) {
+          const int c = ci[s];
+          int found = 0;
+          for (int t = 0; t < n && !found; t++) found = list[t] == c;
+          if (found) continue;
+          if (n == cap) { free(list); return -1 - i; }
+          list[n++] = c;
+        }
+      }
+      first = last;
+      last = n;
+    }
+    for (int a = 1; a < n; a++) {  /* insertion sort */
+      const int v = list[a];
+      int b = a;
+      while (b > 0 && list[b - 1] > v) { list[b] = list[b - 1]; b--; }
+      list[b] = v;
+    }
+    if (ci2) memcpy(ci2 + total, list, sizeof(int) * n);
+    total += n;
+    rp2[i + 1] = (int)total;
+  }
+  free(list);
+  return total;
+}
+
 /* v = (LU)^-1 b with the orc_bcrs_ilu0 factors (lar_perform_rhsub_bcrs,
    variant 0, las_bcrs_intf.c:3349-3510): forward, rows in order (rows with
    no stored blocks skipped), v_i = b_i - sum over the lower blocks of

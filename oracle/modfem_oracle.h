@@ -200,6 +200,8 @@ void orc_block_ilu_solve(int nb, const int *ns, const long long *xoff, const int
                          const int *nbr, const long long *doff, const long long *ooff,
                          const double *val, const double *b, double *v);
 int orc_bcrs_ilu0(int nb, const int *rp, const int *ci, const int *diag, double *ilu);
+long orc_bcrs_iluk_pattern(int nb, const int *rp, const int *ci, int k, int cap, int *rp2,
+                           int *ci2);
 void orc_bcrs_rhsub(int nb, const int *rp, const int *ci, const int *diag, const double *ilu,
                     const double *b, double *v);
 

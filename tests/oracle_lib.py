@@ -72,6 +72,7 @@ def oracle():
             "orc_bcrs_rhsub": (None, [I, V, V, V, V, V, V]),
             "orc_set_threads": (None, [I]),
             "orc_lun": (I, [I, V, V]),
+            "orc_bcrs_iluk_pattern": (ctypes.c_long, [I, V, V, I, I, V, V]),
             "orc_invn": (None, [I, V, V]),
             "orc_block_ilu0": (I, [I, V, V, V, V, V, V]),
             "orc_block_ilu_solve": (None, [I, V, V, V, V, V, V, V, V, V]),
@@ -523,3 +524,30 @@ class BlockSystem:
                 rj = slice(self.xoff[j], self.xoff[j + 1])
                 (L if j < i else U)[ri, rj] = self.block(f, i, j)
         return L, U
+
+
+def bcrs_iluk(nn, rp, ci, val, k):
+    """the oracle's block ILU(k) on BCRS storage: the grown pattern (R's list
+    capacity from the most off-diagonal blocks of a row), values copied onto
+    it (zero for fill), the diagonal positions and the factors"""
+    o = oracle()
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    max_off = int(max(rp[r + 1] - rp[r] - 1 for r in range(nn)))
+    cap = int(0.5 * (k + 2) ** 3 * max_off)
+    rp2 = np.zeros(nn + 1, np.int32)
+    n2 = o.orc_bcrs_iluk_pattern(nn, P(rp), P(ci), k, cap, P(rp2), None)
+    assert n2 > 0
+    ci2 = np.zeros(n2, np.int32)
+    o.orc_bcrs_iluk_pattern(nn, P(rp), P(ci), k, cap, P(rp2), P(ci2))
+    ilu = np.zeros(16 * n2)
+    for r in range(nn):
+        src = {int(ci[s]): s for s in range(rp[r], rp[r + 1])}
+        for s2 in range(rp2[r], rp2[r + 1]):
+            s = src.get(int(ci2[s2]))
+            if s is not None:
+                ilu[16 * s2:16 * s2 + 16] = val[16 * s:16 * s + 16]
+    diag = np.array([next(s for s in range(rp2[r], rp2[r + 1]) if ci2[s] == r) for r in range(nn)],
+                    np.int32)
+    assert o.orc_bcrs_ilu0(nn, P(rp2), P(ci2), P(diag), P(ilu)) == 0
+    return rp2, ci2, diag, ilu

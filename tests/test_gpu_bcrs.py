@@ -4,7 +4,9 @@ lsd_mkb/lad_bcrs/las_bcrs_intf.c for pattern, assembly and residual, and
 pinned by the ILU(0) pattern identity for the factors,
 tests/test_oracle_bcrs.py): pattern, assembly with and without tables,
 residual, diagonal-block LU factors, block Jacobi and block Gauss-Seidel
-sweeps, block ILU(0) factors and solves are all bit-exact."""
+sweeps, block ILU(0) and ILU(k) factors and solves are all bit-exact (the
+ILU(k) pattern also against the reference's own pattern growth, through the
+oracle)."""
 import ctypes
 import os
 import sys
@@ -141,8 +143,8 @@ def test_errors():
     assert mid >= 0
     st = ctypes.c_int()
     assert L.mf_lar_get_sizes(mid, ctypes.byref(st), None, None, None) == 0 and st.value == 3
-    # ILU(k) for k > 0 and additive Schwarz are not offered
-    assert L.lar_allocate_preconditioner(mid, 5, 1) < 0
+    # ILU(k) with k < 0 and additive Schwarz are not offered
+    assert L.lar_allocate_preconditioner(mid, 5, -1) < 0
     assert L.lar_allocate_preconditioner(mid, 3, 0) < 0
     # all-zero matrix: singular diagonal blocks are reported
     assert L.lar_allocate_preconditioner(mid, BLOCK_JACOBI, 0) == nn
@@ -288,4 +290,64 @@ def test_gmres_with_colour_ordered_gauss_seidel():
                           ctypes.byref(res)) == 1
     assert np.abs(x - xs).max() <= 1e-10 * 120 * np.abs(xs).max()
     assert it.value < 60
+    free(L, mid)
+
+
+@pytest.mark.parametrize("k,precon", [(1, BLOCK_ILU), (2, MULTI_ILU)])
+def test_block_iluk_pattern_factors_and_solves_bitexact(k, precon):
+    """block ILU(k), k > 0: the grown pattern equals the oracle's (itself
+    bit-exact with the reference's lar_allocate_preconditioner_bcrs), the
+    factors on it and the solves are bit-exact"""
+    from oracle_lib import bcrs_iluk
+    L, mid, nn, rp, ci, val, rhs = build(5, 4, 3, 21, True)
+    assert L.lar_allocate_preconditioner(mid, precon, k) == nn, L.mf_last_error()
+    assert L.lar_fill_preconditioner(mid) == 0, L.mf_last_error()
+    orp, oci, odiag, oilu = bcrs_iluk(nn, rp, ci, val, k)
+    prp = np.zeros(nn + 1, np.int32)
+    ns = ctypes.c_longlong()
+    assert L.mf_lar_get_ilu_pattern(mid, P(prp), None, ctypes.byref(ns)) == k
+    pci = np.zeros(ns.value, np.int32)
+    assert L.mf_lar_get_ilu_pattern(mid, None, P(pci), None) == k
+    assert np.array_equal(prp, orp) and np.array_equal(pci, oci)
+    assert len(pci) > len(ci)
+    ilu = np.zeros(16 * len(pci))
+    assert L.mf_lar_get_block_precon(mid, P(ilu), None) == 0
+    assert np.array_equal(ilu, oilu)
+    o = oracle()
+    b = np.random.default_rng(4).uniform(-1, 1, 4 * nn)
+    v, v0 = np.full(4 * nn, 3.0), np.zeros(4 * nn)
+    L.lar_perform_rhsub(mid, 4 * nn, P(v), P(b))
+    o.orc_bcrs_rhsub(nn, P(orp), P(oci), P(odiag), P(oilu), P(b), P(v0))
+    assert np.array_equal(v, v0)
+    # a second fill gives the same factors; k = 0 goes back to the matrix pattern
+    assert L.lar_fill_preconditioner(mid) == 0
+    ilu2 = np.zeros_like(ilu)
+    assert L.mf_lar_get_block_precon(mid, P(ilu2), None) == 0
+    assert np.array_equal(ilu2, oilu)
+    assert L.lar_allocate_preconditioner(mid, precon, 0) == nn
+    assert L.mf_lar_get_ilu_pattern(mid, P(prp), None, ctypes.byref(ns)) == 0
+    assert ns.value == len(ci)
+    free(L, mid)
+
+
+def test_gmres_with_block_iluk():
+    """GMRES(30) with block ILU(1) and ILU(2) reaches the dense solution in
+    fewer iterations than with ILU(0); a too-small reference list is an error"""
+    L, mid, nn, rp, ci, val, rhs = build(6, 5, 4, 3, True, diag_shift=2.0)
+    A = np.zeros((4 * nn, 4 * nn))
+    for r in range(nn):
+        for s in range(rp[r], rp[r + 1]):
+            A[4 * r:4 * r + 4, 4 * ci[s]:4 * ci[s] + 4] = val[16 * s:16 * s + 16].reshape(4, 4).T
+    xs = np.linalg.solve(A, rhs)
+    its = []
+    for k in (0, 1, 2):
+        assert L.lar_allocate_preconditioner(mid, BLOCK_ILU, k) == nn, L.mf_last_error()
+        assert L.lar_fill_preconditioner(mid) == 0
+        x = np.zeros(4 * nn)
+        it, rr, res = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        assert L.mf_lar_gmres(mid, 30, 2000, 1e-10, 1, 1, P(x), ctypes.byref(it), ctypes.byref(rr),
+                              ctypes.byref(res)) == 1, L.mf_last_error()
+        assert np.abs(x - xs).max() <= 1e-10 * 120 * np.abs(xs).max()
+        its.append(it.value)
+    assert its[2] <= its[1] < its[0], its
     free(L, mid)

@@ -293,3 +293,86 @@ def test_ilu0_of_a_block_tridiagonal_matrix_is_exact():
     o.orc_bcrs_rhsub(nn, P(rp), P(ci), P(diag), P(ilu), P(b), P(v))
     x = np.linalg.solve(A, b)
     assert np.abs(v - x).max() <= 1e-13 * np.abs(x).max()
+
+
+# ------------------------------------------------------------ block ILU(k)
+def graph_within(nn, rp, ci, hops):
+    """rows/columns within the given number of hops in the block graph"""
+    G = np.zeros((nn, nn), bool)
+    for r in range(nn):
+        G[r, ci[rp[r]:rp[r + 1]]] = True
+    R = np.eye(nn, dtype=bool)
+    for _ in range(hops):
+        R = (R.astype(np.int64) @ G.astype(np.int64)) > 0
+    return R
+
+
+@need_bcrs
+@pytest.mark.parametrize("k", [1, 2])
+def test_iluk_pattern_bitexact_with_reference(k):
+    """the grown ILU(k) pattern equals the one the reference's own
+    lar_allocate_preconditioner_bcrs builds (compiled from its sources; the
+    allocation calls no LAPACK), and it is the set of blocks within k+1 hops
+    of the block graph"""
+    from oracle_lib import bcrs_iluk
+    L = ref_bcrs()
+    L.lar_allocate_preconditioner_bcrs.restype = ctypes.c_int
+    L.lar_allocate_preconditioner_bcrs.argtypes = [ctypes.c_int] * 3
+    mesh, en, nn, nb_lists, mats = vector_problem(3, 3, 2, 5)
+    mid, keep = ref_allocate(L, nn, nb_lists)
+    assert L.lar_allocate_preconditioner_bcrs(mid, 5, k) >= 0
+    st = (BcrsStruct * 20).in_dll(L, "itv_bcrs_matrices")[mid]
+    rrp = np.ctypeslib.as_array(st.ilu_row_ptr, (nn + 1,)).copy()
+    rci = np.ctypeslib.as_array(st.ilu_col_ind, (st.ILU_nr_sm_blocks,)).copy()
+    rp, ci, val, rhs = bcrs_assemble(nn, nb_lists, en, mats)
+    orp, oci, diag, ilu = bcrs_iluk(nn, rp, ci, val, k)
+    assert np.array_equal(orp, rrp) and np.array_equal(oci, rci)
+    R = graph_within(nn, rp, ci, k + 1)
+    for r in range(nn):
+        assert np.array_equal(oci[orp[r]:orp[r + 1]], np.flatnonzero(R[r]))
+    assert len(oci) > len(ci)
+    mesh.free()
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_iluk_reproduces_the_matrix_on_its_pattern(k):
+    """block ILU(k): (L U)_ij = A_ij on every block of the grown pattern
+    (A_ij = 0 at fill); more fill brings L U closer to A off the pattern"""
+    from oracle_lib import bcrs_iluk
+    nn, rp, ci, diag, val, lu, piv, rhs, A = bj_gs_setup(17)
+    orp, oci, odiag, ilu = bcrs_iluk(nn, rp, ci, val, k)
+    Lm, U = ilu_dense(nn, orp, oci, odiag, ilu)
+    LU = Lm @ U
+    mask = np.zeros((4 * nn, 4 * nn), bool)
+    for r in range(nn):
+        for s in range(orp[r], orp[r + 1]):
+            mask[4 * r:4 * r + 4, 4 * oci[s]:4 * oci[s] + 4] = True
+    assert np.abs((LU - A)[mask]).max() <= 1e-12 * np.abs(A).max()
+    ilu0 = val.copy()
+    assert oracle().orc_bcrs_ilu0(nn, P(rp), P(ci), P(diag), P(ilu0)) == 0
+    L0, U0 = ilu_dense(nn, rp, ci, diag, ilu0)
+    assert np.abs(LU - A).max() < np.abs(L0 @ U0 - A).max()
+
+
+def test_iluk_with_a_full_pattern_is_exact():
+    """when k+1 hops reach every block the ILU(k) factors are the exact block
+    LU, and one solve gives A^-1 b"""
+    from oracle_lib import bcrs_iluk
+    nn, rp, ci, diag, val, lu, piv, rhs, A = bj_gs_setup(19)
+    k = 12
+    orp, oci, odiag, ilu = bcrs_iluk(nn, rp, ci, val, k)
+    assert orp[-1] == nn * nn
+    b = np.random.default_rng(8).uniform(-1, 1, 4 * nn)
+    v = np.zeros(4 * nn)
+    oracle().orc_bcrs_rhsub(nn, P(orp), P(oci), P(odiag), P(ilu), P(b), P(v))
+    x = np.linalg.solve(A, b)
+    assert np.abs(v - x).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_iluk_list_capacity_is_enforced():
+    """a row needing more columns than R's list holds is an error (R exits)"""
+    nn, rp, ci, *_ = bj_gs_setup(17)
+    rp2 = np.zeros(nn + 1, np.int32)
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    assert oracle().orc_bcrs_iluk_pattern(nn, P(rp), P(ci), 1, 8, P(rp2), None) == -1
