@@ -76,6 +76,15 @@ int mf_mesh_setup_table(int mesh, int int_ent, int* table);
    boundary): device row i holds DOF row_perm[i], device element slot e holds
    element el_perm[e] */
 int mf_mesh_setup_order(int mesh, int* row_perm, int* el_perm);
+/* field dumps in R's standard format (host only, no GPU needed).
+   mf_field_write: apr_write_field (apd_std_lin/aps_std_lin_intf.c:532-650)
+   for one equation and one solution vector -- "1 1", the number of DOF
+   entities, then "id value " per node with a DOF in ascending node id,
+   %.12lg, |value| < accuracy written as 0; x in DOF order.
+   mf_field_read: the APC_READ branch of apr_create_field (:300-420); the
+   first solution vector into x (DOF order); returns the entity count. */
+int mf_field_write(int mesh, const double* x, double accuracy, const char* filename);
+int mf_field_read(int mesh, const char* filename, double* x);
 
 /* --------------------------------------------------------------- problem
  * mf_problem_create replaces sir_create's integer set-up for a std-linear

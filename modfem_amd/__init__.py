@@ -81,6 +81,8 @@ def _declare(L):
         "mf_mesh_free": (I, [I]),
         "mf_mesh_setup": (I, [I, V, V]),
         "mf_mesh_setup_get": (I, [I, V, V, V]),
+        "mf_field_write": (I, [I, V, D, ctypes.c_char_p]),
+        "mf_field_read": (I, [I, ctypes.c_char_p, V]),
         "mf_mesh_setup_table": (I, [I, I, V]),
         "mf_mesh_setup_order": (I, [I, V, V]),
         "mf_problem_create": (I, [I]),
@@ -313,6 +315,22 @@ class Mesh:
     @classmethod
     def read_jk(cls, path):
         return cls(check(lib().mf_mesh_read_jk(path.encode())))
+
+    def ndof(self):
+        n, nnz = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().mf_mesh_setup(self.h, ctypes.byref(n), ctypes.byref(nnz)))
+        return n.value
+
+    def write_field(self, x, path, accuracy=0.0):
+        """dump x (DOF order) in the reference's field format"""
+        x = np.ascontiguousarray(x, np.float64)
+        check(lib().mf_field_write(self.h, _p(x), float(accuracy), str(path).encode()))
+
+    def read_field(self, path):
+        """the first solution vector of a field dump, in DOF order"""
+        x = np.zeros(self.ndof())
+        check(lib().mf_field_read(self.h, str(path).encode(), _p(x)))
+        return x
 
     @classmethod
     def from_arrays(cls, coor, el_type, el_nodes, bf_elem, bf_side, bf_bc):

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -290,6 +291,98 @@ int mf_mesh_setup_get(int mesh, int* dof_node, int* row_ptr, int* col_ind) {
   if (row_ptr) std::memcpy(row_ptr, s.row_ptr.data(), s.row_ptr.size() * sizeof(int));
   if (col_ind) std::memcpy(col_ind, s.col_ind.data(), s.col_ind.size() * sizeof(int));
   return 0;
+}
+
+// the DOF-to-node map of a mesh (set up on first use)
+static const Setup* field_setup(int mesh) {
+  auto it = g_setups.find(mesh);
+  if (it == g_setups.end()) {
+    if (mf_mesh_setup(mesh, nullptr, nullptr) < 0) return nullptr;
+    it = g_setups.find(mesh);
+  }
+  return it->second.get();
+}
+
+// field dump in R's standard format (apr_write_field, apd_std_lin/
+// aps_std_lin_intf.c:532-650, one equation, one solution vector): "1 1",
+// the number of DOF entities, then per node holding a DOF, in ascending node
+// id (1-based), "id value " with %.12lg and values below accuracy as 0
+int mf_field_write(int mesh, const double* x, double accuracy, const char* filename) {
+  const Setup* S = field_setup(mesh);
+  if (!S) return -3;
+  if (!x || !filename) {
+    set_error("mf_field_write: x and filename required");
+    return -1;
+  }
+  Mesh* m = get_mesh(mesh);
+  std::vector<int> dof_of(m->nno, -1);
+  for (size_t d = 0; d < S->dof_node.size(); d++) dof_of[S->dof_node[d]] = (int)d;
+  FILE* fp = std::fopen(filename, "w");
+  if (!fp) {
+    set_error(std::string("mf_field_write: cannot open ") + filename);
+    return -1;
+  }
+  std::fprintf(fp, "%d %d\n", 1, 1);
+  std::fprintf(fp, "%d\n", (int)S->dof_node.size());
+  for (int v = 0; v < m->nno; v++) {
+    if (dof_of[v] < 0) continue;
+    const double val = x[dof_of[v]];
+    std::fprintf(fp, "%d ", v + 1);
+    std::fprintf(fp, "%.12lg ", std::fabs(val) < accuracy ? 0.0 : val);
+    std::fprintf(fp, "\n");
+  }
+  const bool ok = std::fclose(fp) == 0;
+  if (!ok) {
+    set_error("mf_field_write: write failed");
+    return -1;
+  }
+  return 1;
+}
+
+// read a field dump (apr_create_field with APC_READ, aps_std_lin_intf.c:
+// 300-420): "nreq nr_sol", the number of DOF entities, then per entity its
+// node id and nr_sol vectors of nreq values; the first vector is returned
+// in DOF order.  One equation; every DOF must be present.
+int mf_field_read(int mesh, const char* filename, double* x) {
+  const Setup* S = field_setup(mesh);
+  if (!S) return -3;
+  Mesh* m = get_mesh(mesh);
+  FILE* fp = filename ? std::fopen(filename, "r") : nullptr;
+  if (!fp || !x) {
+    if (fp) std::fclose(fp);
+    set_error(std::string("mf_field_read: cannot open ") + (filename ? filename : "(null)"));
+    return -1;
+  }
+  std::vector<int> dof_of(m->nno, -1);
+  for (size_t d = 0; d < S->dof_node.size(); d++) dof_of[S->dof_node[d]] = (int)d;
+  std::vector<char> seen(S->dof_node.size(), 0);
+  int nreq = 0, nsol = 0, nent = 0, rc = 0;
+  if (std::fscanf(fp, "%d %d", &nreq, &nsol) != 2 || std::fscanf(fp, "%d", &nent) != 1 ||
+      nreq != 1 || nsol < 1 || nsol > 3 || nent < 0 || nent > m->nno) {
+    set_error("mf_field_read: bad header (one equation, 1..3 solution vectors expected)");
+    rc = -1;
+  }
+  for (int k = 0; k < nent && rc == 0; k++) {
+    int id = 0;
+    double v[3] = {0, 0, 0};
+    bool ok = std::fscanf(fp, "%d", &id) == 1;
+    for (int q = 0; q < nsol && ok; q++) ok = std::fscanf(fp, "%lg", &v[q]) == 1;
+    if (!ok || id < 1 || id > m->nno || dof_of[id - 1] < 0) {
+      set_error("mf_field_read: bad entry " + std::to_string(k) + " (node " + std::to_string(id) + ")");
+      rc = -1;
+      break;
+    }
+    x[dof_of[id - 1]] = v[0];
+    seen[dof_of[id - 1]] = 1;
+  }
+  std::fclose(fp);
+  if (rc == 0)
+    for (size_t d = 0; d < seen.size(); d++)
+      if (!seen[d]) {
+        set_error("mf_field_read: no value for the DOF of node " + std::to_string(S->dof_node[d] + 1));
+        return -1;
+      }
+  return rc == 0 ? nent : rc;
 }
 
 int mf_mesh_setup_order(int mesh, int* row_perm, int* el_perm) {

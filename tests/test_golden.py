@@ -9,6 +9,7 @@ import os
 import sys
 
 import numpy as np
+import pytest
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import modfem_amd as mf  # noqa: E402
@@ -38,3 +39,31 @@ def test_oracle_reproduces_reference_field_dump():
     ids, ref = load_dump(os.path.join(GOLD, "laplace_box_field.dmp"))
     assert len(ids) == m.nno == 1702
     assert np.abs(u[ids - 1] - ref).max() <= 1e-9
+
+
+def test_field_dump_read_write_reemits_the_reference_file(tmp_path):
+    """the reference's own dump read into DOF order and written back is
+    byte-identical (apr_write_field's format: header, entity count, node id
+    order, %.12lg values with a trailing space); values below accuracy are
+    written as 0; a dump missing a DOF is rejected"""
+    m = mf.Mesh.read_jk(os.path.join(GOLD, "laplace_box_mesh_jk.dat"))
+    src = os.path.join(GOLD, "laplace_box_field.dmp")
+    x = m.read_field(src)
+    ids, ref = load_dump(src)
+    dof_node = np.zeros(len(x), np.int32)
+    mf.check(mf.lib().mf_mesh_setup_get(m.h, dof_node.ctypes.data, None, None))
+    u = np.zeros(m.nno)
+    u[dof_node] = x
+    assert np.array_equal(u[ids - 1], ref)
+    out = tmp_path / "field.dmp"
+    m.write_field(x, out)
+    assert open(out).read() == open(src).read()
+    m.write_field(x, out, accuracy=1.0)
+    lines = open(out).read().split("\n")
+    small = [l for l in lines[2:] if l and float(l.split()[1]) == 0.0]
+    assert len(small) == int(np.sum(np.abs(ref) < 1.0)) and all(l.endswith(" 0 ") for l in small)
+    trunc = tmp_path / "short.dmp"
+    trunc.write_text("\n".join(open(src).read().split("\n")[:-3]) + "\n")
+    with pytest.raises(RuntimeError):
+        m.read_field(trunc)
+    m.free()
