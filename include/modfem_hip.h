@@ -125,8 +125,10 @@ int mf_problem_set_coefficients(int pid, const int* coeff_ind, int coef_model,
 int mf_integrate_assemble(int pid, int precision);
 /* per boundary face (face-ID order of the mesh): type 1 Dirichlet penalty
    K += 1e7 phi_j phi_i dA, F += 1e7 g phi_i dA; type 2 Neumann F += g phi_i dA
-   with constant flux g (pds_conv_diff_weakform_std.c:480-522; the type is
-   what pdr_get_bc_type returns for the face's bc flag) */
+   with constant flux g, or for the exact-solution source models the exact
+   normal flux grad u . n at every face point (pdr_bc_neum_coeff)
+   (pds_conv_diff_weakform_std.c:480-522; the type is what pdr_get_bc_type
+   returns for the face's bc flag) */
 int mf_problem_set_face_bc(int pid, const int* bc_type, const double* g);
 /* element / face matrices without assembly, column-major stiff[u*k+w] then
    rhs[k], 42 doubles per entity (apr_num_int_el output layout,

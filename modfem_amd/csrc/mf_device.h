@@ -52,6 +52,21 @@ __host__ __device__ inline void exact_sol(int model, const double* x, double* u,
           ((1 - 2 * X * X / a1) + (1 - 2 * Y * Y / a1) + (1 - 2 * Z * Z / a1)) * e;
 }
 
+// exact normal flux grad u . ds / |ds| of the DIFF_IN_CUBE / DIFF_IN_BOX
+// solution, the Neumann data of those problems (pdr_bc_neum_coeff,
+// pdd_conv_diff/weak_formulation/pds_conv_diff_weakform.c; gradient as
+// pdr_exact_sol, pds_conv_diff_util.c:1195-1260); ds is the face's area
+// vector of apr_elem_calc_3D
+__host__ __device__ inline double exact_flux(int model, const double* x, const double* ds) {
+  double a1 = (model == SRC_DIFF_IN_BOX) ? 100.0 : 1.0;
+  double X = x[0], Y = x[1], Z = x[2];
+  double fx = -2.0 * X * exp((-X * X - Y * Y - Z * Z) / a1) / a1;
+  double fy = -2.0 * Y * exp((-X * X - Y * Y - Z * Z) / a1) / a1;
+  double fz = -2.0 * Z * exp((-X * X - Y * Y - Z * Z) / a1) / a1;
+  double len = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  return fx * (ds[0] / len) + fy * (ds[1] / len) + fz * (ds[2] / len);
+}
+
 // selected-term bits (reference coefficient indicator slots 1..22)
 struct TermMask {
   int m, diff_full, diff_diag, conv, tconv, reac, q, s;

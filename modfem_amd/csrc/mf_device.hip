@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(256) k_faces(FaceArgs A) {
       }
     } else {
       const double vv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-      const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};  // outward (v1 x v2)
       const double *p0 = vv[fn[side][0]], *p1 = vv[fn[side][1]], *p2 = vv[fn[side][2]];
       for (int d = 0; d < 3; d++) eta[d] = p0[d] + c0 * (p1[d] - p0[d]) + c1 * (p2[d] - p0[d]);
     }
@@ -477,7 +477,7 @@ __global__ void __launch_bounds__(256) k_faces(FaceArgs A) {
           ds[2] = J[2] * J[4] - J[5] * J[1]; break;
       }
     } else {
-      const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};  // outward (v1 x v2)
       const double *p0 = xn[fn[side][0]], *p1 = xn[fn[side][1]], *p2 = xn[fn[side][2]];
       double v1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
       double v2[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
@@ -488,6 +488,14 @@ __global__ void __launch_bounds__(256) k_faces(FaceArgs A) {
     double area = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]) * wgt;
     double gv = gconst;
     if (neu) {  // flux only (pds_conv_diff_weakform_std.c:513-522)
+      if (A.src_model != SRC_CONST) {  // the exact normal flux
+        double xc[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          xc[0] += xn[i][0] * phi[i]; xc[1] += xn[i][1] * phi[i]; xc[2] += xn[i][2] * phi[i];
+        }
+        gv = exact_flux(A.src_model, xc, ds);
+      }
 #pragma unroll
       for (int i = 0; i < NS; i++) F[i] += gv * phi[i] * area;
       continue;
@@ -784,7 +792,7 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
       }
     } else {
       const double vv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-      const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};  // outward (v1 x v2)
       const double *p0 = vv[fn[side][0]], *p1 = vv[fn[side][1]], *p2 = vv[fn[side][2]];
       for (int d = 0; d < 3; d++) eta[d] = p0[d] + c0 * (p1[d] - p0[d]) + c1 * (p2[d] - p0[d]);
     }
@@ -812,7 +820,7 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
           ds[2] = J[2] * J[4] - J[5] * J[1]; break;
       }
     } else {
-      const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+      const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};  // outward (v1 x v2)
       const double *p0 = xn[fn[side][0]], *p1 = xn[fn[side][1]], *p2 = xn[fn[side][2]];
       double v1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
       double v2[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
@@ -836,6 +844,14 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
     for (int i = 0; i < NS; i++)
       if (i == w) phw = phi[i];
     if (neu) {
+      if (A.I.src_model != SRC_CONST) {  // the exact normal flux
+        double xc[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          xc[0] += xn[i][0] * phi[i]; xc[1] += xn[i][1] * phi[i]; xc[2] += xn[i][2] * phi[i];
+        }
+        gv = exact_flux(A.I.src_model, xc, ds);
+      }
       Fw += gv * phw * area;
       continue;
     }

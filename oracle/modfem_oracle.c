@@ -207,7 +207,9 @@ double orc_elem_calc_3d(int el_type, int control, const double *eta,
   if (control == 2) return fabs(det);
   double ds[3];
   if (el_type == ORC_TETRA) {
-    static const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    /* MMC_FACE_NODES_FOR_TETRA of the hybrid mesh module
+       (mmd_t4_prism/mms_prism_t4_intf.cpp:218): v1 x v2 points outward */
+    static const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
     int f = control - 3;
     const double *p0 = nc + 3 * fn[f][0], *p1 = nc + 3 * fn[f][1],
                  *p2 = nc + 3 * fn[f][2];
@@ -267,6 +269,20 @@ static void exact_sol(int model, const double *x, double *u, double *lapl) {
   if (lapl)
     *lapl = -(2.0 / a1) *
             ((1 - 2 * X * X / a1) + (1 - 2 * Y * Y / a1) + (1 - 2 * Z * Z / a1)) * e;
+}
+
+/* gradient of the exact solution (pdr_exact_sol, pds_conv_diff_util.c:
+   1195-1260: u_x = -2 x exp(-|x|^2 / a1) / a1, likewise y, z) */
+static void exact_grad(int model, const double *x, double *gr) {
+  if (model == ORC_SRC_LINEAR) {
+    gr[0] = 1.0; gr[1] = -2.0; gr[2] = 0.5;
+    return;
+  }
+  double a1 = (model == ORC_SRC_DIFF_IN_BOX) ? 100.0 : 1.0;
+  double X = x[0], Y = x[1], Z = x[2];
+  gr[0] = -2.0 * X * exp((-X * X - Y * Y - Z * Z) / a1) / a1;
+  gr[1] = -2.0 * Y * exp((-X * X - Y * Y - Z * Z) / a1) / a1;
+  gr[2] = -2.0 * Z * exp((-X * X - Y * Y - Z * Z) / a1) / a1;
 }
 
 /* apr_num_int_el, nreq==1 (aps_std_lin_intf.c:1537-2870).  Term order and
@@ -371,7 +387,7 @@ int orc_num_int_el(int el_type, const double *nc, const int *ind,
 static void face_to_elem(int el_type, int side, const double *c, double *eta) {
   if (el_type == ORC_TETRA) {
     static const double v[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    static const int fn[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+    static const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
     const double *p0 = v[fn[side][0]], *p1 = v[fn[side][1]], *p2 = v[fn[side][2]];
     for (int d = 0; d < 3; d++)
       eta[d] = p0[d] + c[0] * (p1[d] - p0[d]) + c[1] * (p2[d] - p0[d]);
@@ -427,6 +443,29 @@ int orc_face_neumann(int el_type, int side, const double *nc, double g, double *
     face_to_elem(el_type, side, &xg[2 * ki], eta);
     double determ = orc_elem_calc_3d(el_type, 3 + side, eta, nc, phi, dx, dy,
                                      dz, xcoor, vn);
+    double area = determ * wg[ki];
+    for (int i = 0; i < ns; i++) rhs[i] += g * phi[i] * area;
+  }
+  return ns;
+}
+
+/* Neumann face of an exact-solution problem (DIFF_IN_CUBE / DIFF_IN_BOX):
+   at every face point the data is the exact normal flux g = grad u . n with
+   the face's unit normal from apr_elem_calc_3D (pdr_bc_neum_coeff,
+   pds_conv_diff_weakform.c; face loop pds_conv_diff_weakform_std.c:
+   420-522), rhs_i += g phi_i dA */
+int orc_face_neumann_exact(int el_type, int side, const double *nc, int src_model,
+                           double *rhs) {
+  int quad = (el_type == ORC_PRISM && side >= 2);
+  double xg[8], wg[4], eta[3], phi[6], dx[6], dy[6], dz[6], xcoor[3], vn[3], gr[3];
+  int ng = orc_quadr_2d(quad, xg, wg);
+  int ns = (el_type == ORC_TETRA) ? 4 : 6;
+  for (int ki = 0; ki < ng; ki++) {
+    face_to_elem(el_type, side, &xg[2 * ki], eta);
+    double determ = orc_elem_calc_3d(el_type, 3 + side, eta, nc, phi, dx, dy,
+                                     dz, xcoor, vn);
+    exact_grad(src_model, xcoor, gr);
+    double g = gr[0] * vn[0] + gr[1] * vn[1] + gr[2] * vn[2];
     double area = determ * wg[ki];
     for (int i = 0; i < ns; i++) rhs[i] += g * phi[i] * area;
   }

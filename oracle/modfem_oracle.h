@@ -72,6 +72,8 @@ int orc_face_penalty(int el_type, int side, const double *node_coor,
 /* Neumann face with constant flux g (pds_conv_diff_weakform_std.c:513-522) */
 int orc_face_neumann(int el_type, int side, const double *node_coor, double g,
                      double *rhs);
+int orc_face_neumann_exact(int el_type, int side, const double *nc, int src_model,
+                           double *rhs);
 
 /* lists and DOF structures (uts_ls_intf.c:102-398, sis_mkb_intf.c:396-1142)
    el_nodes: nel x 6 (tets use the first 4).  Returns number of DOFs.

@@ -34,6 +34,7 @@ def oracle():
             "orc_num_int_el": (I, [I, V, V, V, I, V, V]),
             "orc_face_penalty": (I, [I, I, V, D, I, V, V]),
             "orc_face_neumann": (I, [I, I, V, D, V]),
+            "orc_face_neumann_exact": (I, [I, I, V, I, V]),
             "orc_dof_list": (I, [I, V, V, I, V, V]),
             "orc_crs_pattern": (ctypes.c_long, [I, V, V, V, I, V, V]),
             "orc_assembly_tables": (ctypes.c_long, [I, V, V, I, V, V, V, V, V, V]),
@@ -173,12 +174,17 @@ class OracleProblem:
         o.orc_face_penalty(int(self.et[e]), int(self.bfs[f]), P(nc), g, src_model, P(sm), P(rhs))
         return sm[: k * k].reshape(k, k), rhs[:k]
 
-    def face_neumann(self, f, g):
+    def face_neumann(self, f, g, src_model=0):
+        """Neumann face rhs: constant flux g, or for an exact-solution source
+        model the exact normal flux at every face point"""
         e = self.bfe[f]
         k = 4 if self.et[e] == TETRA else 6
         nc = np.ascontiguousarray(self.coor[self.en[e, :k]].reshape(-1))
         rhs = np.zeros(6)
-        oracle().orc_face_neumann(int(self.et[e]), int(self.bfs[f]), P(nc), g, P(rhs))
+        if src_model:
+            oracle().orc_face_neumann_exact(int(self.et[e]), int(self.bfs[f]), P(nc), src_model, P(rhs))
+        else:
+            oracle().orc_face_neumann(int(self.et[e]), int(self.bfs[f]), P(nc), g, P(rhs))
         return rhs[:k]
 
     def gmres(self, val, rhs, precon=1, krylbas=30, max_iter=1000, tol=1e-8, tol_abs=0.0):

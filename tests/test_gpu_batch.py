@@ -70,6 +70,7 @@ def test_graph_replays_match_eager_launches():
     out = []
     for use_graph in (False, True):
         b = mf.Batch(krylbas=20)
+        b.set_tiles(8)  # same blocks in both modes (tuned tiles may differ by mode)
         for _, p in probs:
             b.add(p, precon=1, max_iter=2000, tol=1e-9)
         while b.run(max_cycles=1, use_graph=use_graph) > 0:

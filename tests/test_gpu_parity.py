@@ -424,3 +424,47 @@ def test_dirichlet_and_neumann_faces(mode):
             assert np.abs(Fm[f, :36]).max() == 0
             assert np.abs(Fm[f, 36:42] - op.face_neumann(f, float(g[f]))).max() <= 1e-14
     p.free()
+
+
+@pytest.mark.parametrize("kind,mode", [("prism", 0), ("prism", 1), ("tetra", 0), ("tetra", 2)])
+def test_neumann_faces_of_an_exact_solution_problem(kind, mode):
+    """DIFF_IN_CUBE with Neumann faces: the face data is the exact normal
+    flux grad u . n at every face point (pdr_bc_neum_coeff), not a constant;
+    face right-hand sides and the assembled system against the oracle"""
+    mesh = (mf.Mesh.box_prism(4, 3, 3, seed=11, jitter=0.2) if kind == "prism"
+            else mf.Mesh.box_tetra(3, 3, 2, seed=11, jitter=0.2))
+    op = OracleProblem(mesh.arrays())
+    rng = np.random.default_rng(12)
+    types = rng.choice([1, 2], op.nbf).astype(np.int32)
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    p = mf.Problem.from_mesh(mesh)
+    p.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc, src_model=mf.SRC_DIFF_IN_CUBE)
+    p.set_face_bc(types, np.zeros(op.nbf))
+    p.set_assembly_mode(mode)
+    p.integrate_assemble(64)
+    Fm = p.face_matrices()
+    scale = 0.0
+    for f in np.nonzero(types == 2)[0]:
+        ref = op.face_neumann(int(f), 0.0, src_model=mf.SRC_DIFF_IN_CUBE)
+        k = len(ref)
+        scale = max(scale, np.abs(ref).max())
+        assert np.abs(Fm[f, k * k:k * k + k] - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max())
+    assert scale > 1e-3  # the flux is not zero on these faces
+    # the assembled system: penalty everywhere in the oracle, then the
+    # Neumann faces' penalty swapped for their exact flux
+    coef = np.tile(cc, (op.nel, 1))
+    v0, r0 = op.assemble(mf.IND_DIFF, coef, src_model=mf.SRC_DIFF_IN_CUBE)
+    v0, r0 = v0.copy(), r0.copy()
+    val, rhs = p.system()
+    for f in np.nonzero(types == 2)[0]:
+        K, F = op.face(int(f), g=0.0, src_model=mf.SRC_DIFF_IN_CUBE)
+        k = K.shape[0]
+        t = op.table[op.table_ptr[op.nel + f]:op.table_ptr[op.nel + f + 1]]
+        flux = op.face_neumann(int(f), 0.0, src_model=mf.SRC_DIFF_IN_CUBE)
+        for w in range(k):
+            for u in range(k):
+                v0[t[w + u * k]] -= K[u][w]
+            r0[t[k * k + w]] += flux[w] - F[w]
+    assert rel_err(val, v0) <= 1e-12 and rel_err(rhs, r0) <= 1e-12
+    p.free()

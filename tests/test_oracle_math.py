@@ -321,3 +321,40 @@ def test_gmres_gs_dense_solution():
     oracle().orc_set_gs(None, 1)
     assert info["converged"]
     assert np.abs(x - np.linalg.solve(A, b)).max() <= 1e-9
+
+
+@pytest.mark.parametrize("kind", ["prism", "tetra"])
+def test_exact_flux_neumann_is_the_outward_flux(kind):
+    """Neumann data of an exact-solution problem is grad u . n at every face
+    point.  For a linear u (constant gradient c) the face total sum_i F_i is
+    c . S, S the face's outward vector area (exact under the face rules, also
+    for the bilinear quads of jittered prisms), and the totals over an
+    element's closed surface cancel: a wrong or inward normal fails this"""
+    m = (mf.Mesh.box_prism(2, 2, 2, seed=4, jitter=0.3) if kind == "prism"
+         else mf.Mesh.box_tetra(2, 2, 2, seed=4, jitter=0.3))
+    a = m.arrays()
+    coor = a["coor"].reshape(-1, 3)
+    en = a["el_nodes"].reshape(-1, 6)
+    c = np.array([1.0, -2.0, 0.5])
+    sides = ({0: [0, 1, 2], 1: [3, 4, 5], 2: [0, 1, 4, 3], 3: [1, 2, 5, 4], 4: [2, 0, 3, 5]}
+             if kind == "prism" else {0: [0, 1, 2], 1: [0, 1, 3], 2: [0, 2, 3], 3: [1, 2, 3]})
+    k = 6 if kind == "prism" else 4
+    et = PRISM if kind == "prism" else TETRA
+    for e in range(a["el_type"].size):
+        X = coor[en[e, :k]]
+        nc = np.ascontiguousarray(X.reshape(-1))
+        total = 0.0
+        for s, nodes in sides.items():
+            rhs = np.zeros(6)
+            oracle().orc_face_neumann_exact(et, s, P(nc), 9, P(rhs))
+            Q = X[nodes]
+            S = (0.5 * np.cross(Q[1] - Q[0], Q[2] - Q[0]) if len(nodes) == 3
+                 else 0.5 * np.cross(Q[2] - Q[0], Q[3] - Q[1]))
+            if S @ (Q.mean(0) - X.mean(0)) < 0:
+                S = -S
+            assert abs(rhs.sum() - c @ S) <= 1e-13, (e, s, rhs.sum(), c @ S)
+            off = [i for i in range(k) if i not in nodes]
+            assert np.abs(rhs[off]).max() <= 1e-15
+            total += rhs.sum()
+        assert abs(total) <= 1e-13
+    m.free()
