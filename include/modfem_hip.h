@@ -56,6 +56,12 @@ int mf_mesh_create(int nr_nodes, const double* node_coor, int nr_elems,
 /* jk ("gradmesh") half-structured prism format: mmr_import_mesh_grad
    (mmd_prism/mms_prism_input_gradmesh.c:29) */
 int mf_mesh_read_jk(const char* filename);
+/* NASTRAN tetrahedral mesh, free field format (utr_io_read_nas,
+   utd_util/uts_io_intf.cpp:812-1020): GRID nodes, CTETRA elements in file
+   order (vertices sorted ascending, then oriented positively), CTRIAX /
+   CTRIA3 triangles with bc > 0 as boundary faces in file order; prisms and
+   fixed-field files are rejected */
+int mf_mesh_read_nas(const char* filename);
 int mf_mesh_write_jk(int mesh, const char* filename);
 /* seeded synthetic boxes (no reference counterpart: workload generators) */
 int mf_mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,

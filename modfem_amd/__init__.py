@@ -73,6 +73,7 @@ def _declare(L):
         "mf_version": (I, []),
         "mf_mesh_create": (I, [I, V, I, V, V, I, V, V, V]),
         "mf_mesh_read_jk": (I, [S]),
+        "mf_mesh_read_nas": (I, [ctypes.c_char_p]),
         "mf_mesh_write_jk": (I, [I, S]),
         "mf_mesh_box_prism": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
         "mf_mesh_box_tetra": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
@@ -315,6 +316,10 @@ class Mesh:
     @classmethod
     def read_jk(cls, path):
         return cls(check(lib().mf_mesh_read_jk(path.encode())))
+
+    @classmethod
+    def read_nas(cls, path):
+        return cls(check(lib().mf_mesh_read_nas(str(path).encode())))
 
     def ndof(self):
         n, nnz = ctypes.c_int(), ctypes.c_longlong()

@@ -1,3 +1,6 @@
+#include <sstream>
+#include <fstream>
+#include <array>
 // Mesh construction: the jk ("gradmesh") half-structured prism format and
 // seeded box generators.  The 3D mesh built from a jk structure follows the
 // reference importer mmr_import_mesh_grad (mmd_prism/
@@ -246,6 +249,113 @@ int mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
 // diagonal (Kuhn split, conforming).  Node and element IDs are permuted by the
 // seed; boundary faces are listed element by element, local face order
 // (mmd_prism/mms_prism_intf.c:97 face-node table {0,1,2},{0,1,3},{0,2,3},{1,2,3}).
+// NASTRAN bulk data with tetrahedra (utr_io_read_nas, utd_util/
+// uts_io_intf.cpp:812-1020), free field format: "GRID id x y z [cd]" nodes
+// in id order 1..n, "CTRIAX|CTRIA3 id bc n1 n2 n3" boundary triangles (bc > 0
+// kept as boundary faces in file order, bc <= 0 interior), "CTETRA id group
+// n1 n2 n3 n4" elements in file order.  As the reference, element vertices
+// are sorted ascending; here they are then oriented positively (nodes 2 and
+// 3 swapped for a negative volume), the element-local order the face table
+// of the hybrid mesh module expects.  Prisms (CPENTA, CPRIZM), quads and the
+// fixed-field formats are not read.
+int mesh_read_nas(const char* fname, Mesh& m) {
+  std::ifstream in(fname);
+  if (!in) {
+    set_error(std::string("cannot open ") + fname);
+    return -1;
+  }
+  m = Mesh();
+  std::vector<std::array<int, 4>> tets;
+  std::vector<std::array<int, 4>> tris;  // n1 n2 n3 bc
+  std::string line;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::string key;
+    if (!(ls >> key)) continue;
+    if (key == "GRID") {
+      long id;
+      double x[3];
+      if (!(ls >> id >> x[0] >> x[1] >> x[2]) || id != m.nno + 1) {
+        set_error("NASTRAN: free-format GRID lines with ids 1..n in order expected");
+        return -1;
+      }
+      m.coor.insert(m.coor.end(), x, x + 3);
+      m.nno++;
+    } else if (key == "CTRIAX" || key == "CTRIA3") {
+      long id;
+      int bc, v[3];
+      if (!(ls >> id >> bc >> v[0] >> v[1] >> v[2])) {
+        set_error("NASTRAN: bad triangle line");
+        return -1;
+      }
+      if (bc > 0) tris.push_back({v[0], v[1], v[2], bc});
+    } else if (key == "CTETRA") {
+      long id;
+      int grp;
+      std::array<int, 4> v;
+      if (!(ls >> id >> grp >> v[0] >> v[1] >> v[2] >> v[3])) {
+        set_error("NASTRAN: bad CTETRA line");
+        return -1;
+      }
+      tets.push_back(v);
+    } else if (key == "CPENTA" || key == "CPRIZM" || key == "CQUADX" || key == "CQUAD4") {
+      set_error("NASTRAN: only tetrahedra and triangles are read");
+      return -1;
+    }
+  }
+  if (m.nno == 0 || tets.empty()) {
+    set_error("NASTRAN: no nodes or no tetrahedra");
+    return -1;
+  }
+  m.nel = (int)tets.size();
+  m.type.assign(m.nel, kTetra);
+  m.nodes.assign(6 * (size_t)m.nel, -1);
+  for (int e = 0; e < m.nel; e++) {
+    std::array<int, 4> v = tets[e];
+    std::sort(v.begin(), v.end());
+    for (int a = 0; a < 4; a++)
+      if (v[a] < 1 || v[a] > m.nno) {
+        set_error("NASTRAN: element vertex out of range");
+        return -1;
+      }
+    const double* p[4];
+    for (int a = 0; a < 4; a++) p[a] = &m.coor[3 * (size_t)(v[a] - 1)];
+    double a1[3], a2[3], a3[3];
+    for (int d = 0; d < 3; d++) {
+      a1[d] = p[1][d] - p[0][d]; a2[d] = p[2][d] - p[0][d]; a3[d] = p[3][d] - p[0][d];
+    }
+    const double det = a1[0] * (a2[1] * a3[2] - a2[2] * a3[1]) -
+                       a1[1] * (a2[0] * a3[2] - a2[2] * a3[0]) +
+                       a1[2] * (a2[0] * a3[1] - a2[1] * a3[0]);
+    if (det < 0) std::swap(v[2], v[3]);
+    for (int a = 0; a < 4; a++) m.nodes[6 * (size_t)e + a] = v[a] - 1;
+  }
+  // boundary triangles -> (element, side) by vertex set
+  static const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
+  std::map<std::array<int, 3>, std::pair<int, int>> face_of;
+  for (int e = 0; e < m.nel; e++)
+    for (int f = 0; f < 4; f++) {
+      std::array<int, 3> k = {m.nodes[6 * (size_t)e + fn[f][0]], m.nodes[6 * (size_t)e + fn[f][1]],
+                              m.nodes[6 * (size_t)e + fn[f][2]]};
+      std::sort(k.begin(), k.end());
+      face_of.emplace(k, std::make_pair(e, f));
+    }
+  for (const auto& t : tris) {
+    std::array<int, 3> k = {t[0] - 1, t[1] - 1, t[2] - 1};
+    std::sort(k.begin(), k.end());
+    auto it = face_of.find(k);
+    if (it == face_of.end()) {
+      set_error("NASTRAN: a boundary triangle is not a face of any tetrahedron");
+      return -1;
+    }
+    m.bf_elem.push_back(it->second.first);
+    m.bf_side.push_back(it->second.second);
+    m.bf_bc.push_back(t[3]);
+  }
+  m.nbf = (int)m.bf_elem.size();
+  return 0;
+}
+
 int mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
                    uint64_t seed, double jitter, int bc, Mesh& m) {
   if (nx < 1 || ny < 1 || nz < 1) {

@@ -67,3 +67,32 @@ def test_field_dump_read_write_reemits_the_reference_file(tmp_path):
     with pytest.raises(RuntimeError):
         m.read_field(trunc)
     m.free()
+
+
+def test_nastran_tetra_mesh_reader():
+    """the reference's NASTRAN test mesh (ctest Laplace_std_in_cube_tetra,
+    copied as data): 8 GRID nodes, 6 CTETRA elements in file order with
+    their vertices sorted and positively oriented, the 12 triangles with
+    bc > 0 as boundary faces (bc 11 top and bottom, bc 1 sides) mapped to the
+    element side holding the same vertices; interior triangles (bc -1) skipped"""
+    m = mf.Mesh.read_nas(os.path.join(GOLD, "tetra_cube", "mesh.nas"))
+    a = m.arrays()
+    assert (m.nno, m.nel, m.nbf) == (8, 6, 12)
+    coor = a["coor"].reshape(-1, 3)
+    en = a["el_nodes"].reshape(-1, 6)[:, :4]
+    assert sorted(en[0]) == [0, 1, 2, 4]  # CTETRA 1: 3 1 2 5
+    for e in range(m.nel):
+        X = coor[en[e]]
+        assert np.linalg.det(np.array([X[1] - X[0], X[2] - X[0], X[3] - X[0]])) > 0
+    sides = {0: [0, 2, 1], 1: [0, 1, 3], 2: [0, 3, 2], 3: [1, 2, 3]}
+    tris = [([1, 2, 3], 11), ([1, 3, 4], 11), ([5, 6, 7], 11), ([5, 7, 8], 11), ([1, 2, 5], 1),
+            ([2, 5, 6], 1), ([2, 3, 7], 1), ([2, 6, 7], 1), ([3, 4, 8], 1), ([3, 7, 8], 1),
+            ([1, 4, 8], 1), ([1, 5, 8], 1)]
+    for f, (nodes, bc) in enumerate(tris):
+        e, s = a["bf_elem"][f], a["bf_side"][f]
+        assert sorted(en[e][sides[s]] + 1) == nodes and a["bf_bc"][f] == bc
+        # the side's normal points out of the unit cube at that face
+        Q = coor[en[e][sides[s]]]
+        n = np.cross(Q[1] - Q[0], Q[2] - Q[1])
+        assert n @ (Q.mean(0) - np.array([0.5, 0.5, 0.5])) > 0
+    m.free()

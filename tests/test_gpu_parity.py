@@ -468,3 +468,4 @@ def test_neumann_faces_of_an_exact_solution_problem(kind, mode):
             r0[t[k * k + w]] += flux[w] - F[w]
     assert rel_err(val, v0) <= 1e-12 and rel_err(rhs, r0) <= 1e-12
     p.free()
+
