@@ -62,6 +62,11 @@ int mf_mesh_read_jk(const char* filename);
    CTRIA3 triangles with bc > 0 as boundary faces in file order; prisms and
    fixed-field files are rejected */
 int mf_mesh_read_nas(const char* filename);
+/* ModFEM prism mesh dump (".dat" / ".dmp"): mmr_read_mesh (mmd_prism/
+   mms_prism_io_dump.c:26-270) with the element-local node order of
+   mmr_el_node_coor (mmd_prism/mms_prism_datstr.c:365-510); meshes of active
+   prisms only (no refinement history) */
+int mf_mesh_read_dat(const char* filename);
 int mf_mesh_write_jk(int mesh, const char* filename);
 /* seeded synthetic boxes (no reference counterpart: workload generators) */
 int mf_mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,

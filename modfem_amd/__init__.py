@@ -74,6 +74,7 @@ def _declare(L):
         "mf_mesh_create": (I, [I, V, I, V, V, I, V, V, V]),
         "mf_mesh_read_jk": (I, [S]),
         "mf_mesh_read_nas": (I, [ctypes.c_char_p]),
+        "mf_mesh_read_dat": (I, [ctypes.c_char_p]),
         "mf_mesh_write_jk": (I, [I, S]),
         "mf_mesh_box_prism": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
         "mf_mesh_box_tetra": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
@@ -320,6 +321,11 @@ class Mesh:
     @classmethod
     def read_nas(cls, path):
         return cls(check(lib().mf_mesh_read_nas(str(path).encode())))
+
+    @classmethod
+    def read_dat(cls, path):
+        """the reference's prism mesh dump (.dat / .dmp)"""
+        return cls(check(lib().mf_mesh_read_dat(str(path).encode())))
 
     def ndof(self):
         n, nnz = ctypes.c_int(), ctypes.c_longlong()

@@ -222,6 +222,12 @@ int mf_mesh_read_nas(const char* filename) {
   return add_mesh(std::move(m));
 }
 
+int mf_mesh_read_dat(const char* filename) {
+  auto m = std::make_unique<Mesh>();
+  if (mesh_read_dat(filename, *m) < 0) return -1;
+  return add_mesh(std::move(m));
+}
+
 int mf_mesh_write_jk(int mesh, const char* filename) {
   Mesh* m = get_mesh(mesh);
   if (!m) return -3;

@@ -378,6 +378,26 @@ __global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
   unsafeAtomicAdd(&A.rhs[dw], (double)F);
 }
 
+// Face point (c0, c1) of the reference triangle -> tetrahedron coordinates,
+// as the hybrid mesh module maps it (mmr_fa_elem_coor, mmd_t4_prism/
+// mms_prism_t4_intf.cpp:2004-2037): sides 0-2 lie in the coordinate planes;
+// side 3 (vertices 1, 2, 3) takes the face coordinates scaled by 0.75 first,
+// which moves the points toward vertex 1 on that face.  The tetrahedral
+// results of the reference (its L2/H1 error figures on the NASTRAN cube)
+// are made with this map, so it is kept as the reference has it.
+__device__ __forceinline__ void tet_face_eta(int side, double c0, double c1, double* eta) {
+  switch (side) {
+    case 0: eta[0] = c1; eta[1] = c0; eta[2] = 0.0; break;
+    case 1: eta[0] = c0; eta[1] = 0.0; eta[2] = c1; break;
+    case 2: eta[0] = 0.0; eta[1] = c1; eta[2] = c0; break;
+    default:
+      c0 *= 0.75;
+      c1 *= 0.75;
+      eta[0] = 1.0 - c0 - c1; eta[1] = c0; eta[2] = c1;
+      break;
+  }
+}
+
 // ---------------------------------------------------------------------
 // Dirichlet penalty faces: K += 1e7 phi_j phi_i dA, F += 1e7 g phi_i dA
 // on the owning element's DOFs.
@@ -448,10 +468,7 @@ __global__ void __launch_bounds__(256) k_faces(FaceArgs A) {
         default: eta[0] = 0.0; eta[1] = 0.5 * (-c0 + 1.0); eta[2] = c1; break;
       }
     } else {
-      const double vv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-      const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};  // outward (v1 x v2)
-      const double *p0 = vv[fn[side][0]], *p1 = vv[fn[side][1]], *p2 = vv[fn[side][2]];
-      for (int d = 0; d < 3; d++) eta[d] = p0[d] + c0 * (p1[d] - p0[d]) + c1 * (p2[d] - p0[d]);
+      tet_face_eta(side, c0, c1, eta);
     }
     double phi[NS], dx[NS], dy[NS], dz[NS], J[9], Ji[9];
     ref_shape<NS>(eta[0], eta[1], eta[2], phi, dx, dy, dz);
@@ -791,10 +808,7 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
         default: eta[0] = 0.0; eta[1] = 0.5 * (-c0 + 1.0); eta[2] = c1; break;
       }
     } else {
-      const double vv[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-      const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};  // outward (v1 x v2)
-      const double *p0 = vv[fn[side][0]], *p1 = vv[fn[side][1]], *p2 = vv[fn[side][2]];
-      for (int d = 0; d < 3; d++) eta[d] = p0[d] + c0 * (p1[d] - p0[d]) + c1 * (p2[d] - p0[d]);
+      tet_face_eta(side, c0, c1, eta);
     }
     double phi[NS], dx[NS], dy[NS], dz[NS], J[9], Ji[9];
     ref_shape<NS>(eta[0], eta[1], eta[2], phi, dx, dy, dz);

@@ -42,6 +42,7 @@ struct Mesh {
 int mesh_from_jk_struct(Mesh& m);  // builds 3D mesh from the jk_* fields
 int mesh_read_jk(const char* fname, Mesh& m);
 int mesh_read_nas(const char* fname, Mesh& m);
+int mesh_read_dat(const char* fname, Mesh& m);
 int mesh_write_jk(const Mesh& m, const char* fname);
 int mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
                    uint64_t seed, double jitter, int bc, Mesh& m);

@@ -254,9 +254,10 @@ int mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
 // in id order 1..n, "CTRIAX|CTRIA3 id bc n1 n2 n3" boundary triangles (bc > 0
 // kept as boundary faces in file order, bc <= 0 interior), "CTETRA id group
 // n1 n2 n3 n4" elements in file order.  As the reference, element vertices
-// are sorted ascending; here they are then oriented positively (nodes 2 and
-// 3 swapped for a negative volume), the element-local order the face table
-// of the hybrid mesh module expects.  Prisms (CPENTA, CPRIZM), quads and the
+// are sorted ascending and then oriented positively by swapping local
+// vertices 1 and 2 for a negative volume (ElemT4::init, mmd_t4_prism/
+// MeshModule/ElemT4.cpp:24-26), the element-local order the face table of
+// the hybrid mesh module expects.  Prisms (CPENTA, CPRIZM), quads and the
 // fixed-field formats are not read.
 int mesh_read_nas(const char* fname, Mesh& m) {
   std::ifstream in(fname);
@@ -327,7 +328,7 @@ int mesh_read_nas(const char* fname, Mesh& m) {
     const double det = a1[0] * (a2[1] * a3[2] - a2[2] * a3[1]) -
                        a1[1] * (a2[0] * a3[2] - a2[2] * a3[0]) +
                        a1[2] * (a2[0] * a3[1] - a2[1] * a3[0]);
-    if (det < 0) std::swap(v[2], v[3]);
+    if (det < 0) std::swap(v[1], v[2]);
     for (int a = 0; a < 4; a++) m.nodes[6 * (size_t)e + a] = v[a] - 1;
   }
   // boundary triangles -> (element, side) by vertex set
@@ -351,6 +352,140 @@ int mesh_read_nas(const char* fname, Mesh& m) {
     m.bf_elem.push_back(it->second.first);
     m.bf_side.push_back(it->second.second);
     m.bf_bc.push_back(t[3]);
+  }
+  m.nbf = (int)m.bf_elem.size();
+  return 0;
+}
+
+// ModFEM prism mesh dump (".dat" / ".dmp", the "prismatic mesh in standard
+// format" of the problem files): mmr_read_mesh (mmd_prism/
+// mms_prism_io_dump.c:26-270).  Sections in order: array sizes; nodes
+// (nmno, first free, then x y z per node); edges (type n0 n1); faces (type
+// bc neig0 neig1, then 3 or 4 signed edge IDs); elements (type mate fath
+// refi, then 5 signed face IDs for a prism).  Only meshes of active
+// entities are read (no refinement history, no free slots).  Element-local
+// node order follows mmr_el_node_coor (mms_prism_datstr.c:365-510): the
+// bottom face's nodes from its edges (mmr_fa_node_coor, :1488-1530, an edge
+// entered with a negative sign contributes its second node), taken as
+// (0, 2, 1) for a face of the element's orientation, else rotated by the
+// shift stored as -bc of an interior face; the top face likewise as
+// (0, 1, 2) or rotated and reflected.  Boundary faces are the faces with
+// bc > 0 in face-ID order, owned by neighbour 0 on the side where the
+// element lists the face (mmr_fa_eq_neig, :1140-1212).
+int mesh_read_dat(const char* fname, Mesh& m) {
+  FILE* fp = fopen(fname, "r");
+  if (!fp) {
+    set_error(std::string("cannot open mesh dump ") + fname);
+    return -1;
+  }
+  auto fail = [&](const char* what) {
+    fclose(fp);
+    set_error(std::string("mesh dump ") + fname + ": " + what);
+    return -1;
+  };
+  int mx[4], nmno, pf;
+  if (fscanf(fp, "%d %d %d %d", &mx[0], &mx[1], &mx[2], &mx[3]) != 4) return fail("header");
+  if (fscanf(fp, "%d %d", &nmno, &pf) != 2 || nmno < 1) return fail("node count");
+  m = Mesh();
+  m.nno = nmno;
+  m.coor.assign(3 * (size_t)nmno, 0.0);
+  for (int i = 0; i < nmno; i++) {
+    double* c = &m.coor[3 * (size_t)i];
+    if (fscanf(fp, "%lg", &c[0]) != 1) return fail("node record");
+    if (c[0] < -1e10) return fail("free node slots are not read");
+    if (fscanf(fp, "%lg %lg", &c[1], &c[2]) != 2) return fail("node record");
+  }
+  int nmed;
+  if (fscanf(fp, "%d %d", &nmed, &pf) != 2 || nmed < 1) return fail("edge count");
+  std::vector<int> ed(2 * (size_t)(nmed + 1), 0);
+  for (int i = 1; i <= nmed; i++) {
+    int t;
+    if (fscanf(fp, "%d %d %d", &t, &ed[2 * i], &ed[2 * i + 1]) != 3) return fail("edge record");
+    if (t <= 0) return fail("inactive or free edges are not read");
+    if (ed[2 * i] < 1 || ed[2 * i] > nmno || ed[2 * i + 1] < 1 || ed[2 * i + 1] > nmno)
+      return fail("edge node out of range");
+  }
+  int nmfa;
+  if (fscanf(fp, "%d %d", &nmfa, &pf) != 2 || nmfa < 1) return fail("face count");
+  std::vector<int> ftype(nmfa + 1), fbc(nmfa + 1), fneig(2 * (size_t)(nmfa + 1));
+  std::vector<int> fedge(4 * (size_t)(nmfa + 1), 0);
+  for (int i = 1; i <= nmfa; i++) {
+    if (fscanf(fp, "%d %d %d %d", &ftype[i], &fbc[i], &fneig[2 * i], &fneig[2 * i + 1]) != 4)
+      return fail("face record");
+    const int ne = ftype[i] == 3 ? 3 : ftype[i] == 4 ? 4 : 0;
+    if (ne == 0) return fail("inactive, free or unknown faces are not read");
+    for (int k = 0; k < ne; k++) {
+      if (fscanf(fp, "%d", &fedge[4 * i + k]) != 1) return fail("face edges");
+      const int a = std::abs(fedge[4 * i + k]);
+      if (a < 1 || a > nmed) return fail("face edge out of range");
+    }
+  }
+  int nmel;
+  if (fscanf(fp, "%d %d", &nmel, &pf) != 2 || nmel < 1) return fail("element count");
+  std::vector<int> efa(5 * (size_t)nmel);
+  for (int e = 0; e < nmel; e++) {
+    int t, mate, fath, refi;
+    if (fscanf(fp, "%d %d %d %d", &t, &mate, &fath, &refi) != 4) return fail("element record");
+    if (t != kPrism) return fail("only active prisms are read");
+    for (int k = 0; k < 5; k++) {
+      if (fscanf(fp, "%d", &efa[5 * (size_t)e + k]) != 1) return fail("element faces");
+      const int a = std::abs(efa[5 * (size_t)e + k]);
+      if (a < 1 || a > nmfa) return fail("element face out of range");
+    }
+  }
+  fclose(fp);
+  // face node k: first node of edge k, or its second node when the edge is
+  // entered with a negative sign
+  auto fa_node = [&](int f, int k) {
+    const int ied = fedge[4 * f + k];
+    return ied > 0 ? ed[2 * ied] : ed[2 * (-ied) + 1];
+  };
+  m.nel = nmel;
+  m.type.assign(nmel, kPrism);
+  m.nodes.assign(6 * (size_t)nmel, -1);
+  for (int e = 0; e < nmel; e++) {
+    int* nd = &m.nodes[6 * (size_t)e];
+    for (int b = 0; b < 2; b++) {
+      const int sf = efa[5 * (size_t)e + b], f = std::abs(sf);
+      if (ftype[f] != 3) {
+        set_error("mesh dump: a prism base is not a triangle");
+        return -1;
+      }
+      int v[3];
+      if (sf > 0) {
+        v[0] = fa_node(f, 0);
+        v[1] = b == 0 ? fa_node(f, 2) : fa_node(f, 1);
+        v[2] = b == 0 ? fa_node(f, 1) : fa_node(f, 2);
+      } else {
+        const int sh = -fbc[f];
+        if (sh < 0 || sh > 2) {
+          set_error("mesh dump: a base entered against its orientation has no node shift");
+          return -1;
+        }
+        v[0] = fa_node(f, sh);
+        v[b == 0 ? 1 : 2] = fa_node(f, (sh + 1) % 3);
+        v[b == 0 ? 2 : 1] = fa_node(f, (sh + 2) % 3);
+      }
+      for (int k = 0; k < 3; k++) nd[3 * b + k] = v[k] - 1;
+    }
+  }
+  for (int f = 1; f <= nmfa; f++) {
+    if (fbc[f] <= 0) continue;
+    const int owner = fneig[2 * f];
+    if (owner < 1 || owner > nmel) {
+      set_error("mesh dump: boundary face without an owning element");
+      return -1;
+    }
+    int side = -1;
+    for (int k = 0; k < 5 && side < 0; k++)
+      if (std::abs(efa[5 * (size_t)(owner - 1) + k]) == f) side = k;
+    if (side < 0) {
+      set_error("mesh dump: boundary face not listed by its owner");
+      return -1;
+    }
+    m.bf_elem.push_back(owner - 1);
+    m.bf_side.push_back(side);
+    m.bf_bc.push_back(fbc[f]);
   }
   m.nbf = (int)m.bf_elem.size();
   return 0;

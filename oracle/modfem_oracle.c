@@ -382,15 +382,29 @@ int orc_num_int_el(int el_type, const double *nc, const int *ind,
 }
 
 /* face-local -> element-local coordinates for the first (owning) neighbour,
-   mmr_fa_elem_coor (mmd_prism/mms_prism_datstr.c:1562-1700); tetra faces
-   are parametrised over their MMC_FACE_NODES_FOR_TETRA vertices. */
+   mmr_fa_elem_coor (prisms: mmd_prism/mms_prism_datstr.c:1562-1700;
+   tetrahedra: the hybrid mesh module, mmd_t4_prism/mms_prism_t4_intf.cpp:
+   2004-2037, whose side 3 scales the face coordinates by 0.75 before
+   mapping them onto the face opposite vertex 0). */
+static double tet_side3_scale = 0.75;
+
+/* test hook: plants another side-3 scale (1.0 is the undistorted map) so the
+   tests can show which map the reference's figures need */
+void orc_set_tet_side3_scale(double s) { tet_side3_scale = s; }
+
 static void face_to_elem(int el_type, int side, const double *c, double *eta) {
   if (el_type == ORC_TETRA) {
-    static const double v[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    static const int fn[4][3] = {{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
-    const double *p0 = v[fn[side][0]], *p1 = v[fn[side][1]], *p2 = v[fn[side][2]];
-    for (int d = 0; d < 3; d++)
-      eta[d] = p0[d] + c[0] * (p1[d] - p0[d]) + c[1] * (p2[d] - p0[d]);
+    double c0 = c[0], c1 = c[1];
+    switch (side) {
+      case 0: eta[0] = c1; eta[1] = c0; eta[2] = 0.0; break;
+      case 1: eta[0] = c0; eta[1] = 0.0; eta[2] = c1; break;
+      case 2: eta[0] = 0.0; eta[1] = c1; eta[2] = c0; break;
+      default:
+        c0 *= tet_side3_scale;
+        c1 *= tet_side3_scale;
+        eta[0] = 1.0 - c0 - c1; eta[1] = c0; eta[2] = c1;
+        break;
+    }
     return;
   }
   switch (side) {
@@ -584,6 +598,49 @@ long orc_assembly_tables(int nel, const int *el_type, const int *el_nodes,
 static void gather_coor(const double *all, const int *nodes, int k, double *nc) {
   for (int i = 0; i < k; i++)
     for (int d = 0; d < 3; d++) nc[3 * i + d] = all[3 * nodes[i] + d];
+}
+
+/* pdr_error_test_exact (pdd_conv_diff/main/pds_conv_diff_util.c:749-960)
+   for the std-linear approximation: over the elements in ID order, at the
+   element's volume rule (apr_set_quadr_3D), u_h and grad u_h from the nodal
+   values (apr_elem_calc_3D control 2) against the exact solution of the
+   source model; err_L2^2 += (u - u_h)^2 |det| w, err_H1^2 += |grad u -
+   grad u_h|^2 |det| w, per element first, then summed.  u holds one value
+   per mesh node; returns the square roots in l2[0], l2[1]. */
+int orc_std_errors(int nel, const int *el_type, const int *el_nodes,
+                   const double *coor, const double *u, int src_model,
+                   double *l2h1) {
+  double l2 = 0.0, h1 = 0.0;
+  for (int e = 0; e < nel; e++) {
+    int k = nodes_of(el_type[e]);
+    double nc[18], xg[3 * 6], wg[6], phi[6], dx[6], dy[6], dz[6], xc[3];
+    gather_coor(coor, el_nodes + 6 * e, k, nc);
+    int ng = orc_quadr_3d(el_type[e], xg, wg);
+    double el2 = 0.0, eh1 = 0.0;
+    for (int ki = 0; ki < ng; ki++) {
+      double det = orc_elem_calc_3d(el_type[e], 2, &xg[3 * ki], nc, phi, dx, dy, dz, xc, NULL);
+      double vol = det * wg[ki];
+      double uh = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+      for (int i = 0; i < k; i++) {
+        double d = u[el_nodes[6 * e + i]];
+        uh += d * phi[i];
+        ux += d * dx[i];
+        uy += d * dy[i];
+        uz += d * dz[i];
+      }
+      double f, g[3];
+      exact_sol(src_model, xc, &f, NULL);
+      exact_grad(src_model, xc, g);
+      el2 += (f - uh) * (f - uh) * vol;
+      eh1 += ((g[0] - ux) * (g[0] - ux) + (g[1] - uy) * (g[1] - uy) +
+              (g[2] - uz) * (g[2] - uz)) * vol;
+    }
+    l2 += el2;
+    h1 += eh1;
+  }
+  l2h1[0] = sqrt(l2);
+  l2h1[1] = sqrt(h1);
+  return nel;
 }
 
 /* sequential create+assemble (sis_mkb_intf.c:1843-1866): element int-ents in
@@ -1382,6 +1439,11 @@ static int dg_phys(int pdeg, const double *nc, const double *eta, double *phi, d
   return ns;
 }
 
+static int dg_plant_downwind = 0;
+
+/* test hook: 1 plants the downwind neighbour in the interior upwind terms */
+void orc_dg_set_plant_downwind(int on) { dg_plant_downwind = on; }
+
 /* one face's stiffness matrix and load vector for nreq = 1, steady
    (implicitness 1), as pdr_conv_diff_comp_fa_stiff_mat_app writes them
    (pds_conv_diff_weakform_dg.c:232-1400): ndofs_fa = ns1 (boundary) or
@@ -1456,13 +1518,15 @@ int orc_dg_face_int2(int pdeg1, int pdeg2, int bc, const double *nc1, const int 
         for (int i = 0; i < ns2; i++)
           stiff[(ns1 + j) * nf + ns1 + i] += ((-p2[j] * gg2[i] + p2[i] * gg2[j]) * 0.5 +
                                               penalty * p2[j] * p2[i] / hf) * area;
-      /* upwinding on the averaged normal velocity */
-      if (nv > 0) {
+      /* upwinding on the averaged normal velocity (a test hook can plant
+         the downwind side instead) */
+      const double nvs = dg_plant_downwind ? -nv : nv;
+      if (nvs > 0) {
         for (int j = 0; j < ns1; j++) {
           for (int i = 0; i < ns1; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
           for (int i = 0; i < ns2; i++) stiff[j * nf + ns1 + i] -= area * bn2 * p2[i] * p1[j];
         }
-      } else if (nv < 0) {
+      } else if (nvs < 0) {
         for (int j = 0; j < ns2; j++) {
           for (int i = 0; i < ns1; i++) stiff[(ns1 + j) * nf + i] += area * bn1 * p1[i] * p2[j];
           for (int i = 0; i < ns2; i++) stiff[(ns1 + j) * nf + ns1 + i] -= area * bn2 * p2[i] * p2[j];

@@ -93,6 +93,20 @@ long orc_assembly_tables(int nel, const int *el_type, const int *el_nodes,
                          const int *row_ptr, const int *col_ind,
                          long *table_ptr, int *table);
 
+/* test hook: the factor the hybrid mesh module applies to face coordinates
+   on tetrahedron side 3 (0.75 as the reference; 1.0 plants the undistorted
+   map) */
+void orc_set_tet_side3_scale(double s);
+
+/* test hook: 1 plants the downwind neighbour in the DG interior upwind terms
+   (0 restores the reference's upwind side) */
+void orc_dg_set_plant_downwind(int on);
+
+/* L2 norm and H1 seminorm of u - u_exact (pdr_error_test_exact) */
+int orc_std_errors(int nel, const int *el_type, const int *el_nodes,
+                   const double *coor, const double *u, int src_model,
+                   double *l2h1);
+
 /* integrate + assemble sequentially in int-ent order (sis_mkb_intf.c:1843-1866,
    las_crs_intf.c:440-513).  val[nnz], rhs[n] zeroed inside. */
 int orc_integrate_assemble(int nel, const int *el_type, const int *el_nodes,

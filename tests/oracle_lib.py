@@ -77,6 +77,9 @@ def oracle():
             "orc_invn": (None, [I, V, V]),
             "orc_block_ilu0": (I, [I, V, V, V, V, V, V]),
             "orc_block_ilu_solve": (None, [I, V, V, V, V, V, V, V, V, V]),
+            "orc_std_errors": (I, [I, V, V, V, V, I, V]),
+            "orc_set_tet_side3_scale": (None, [D]),
+            "orc_dg_set_plant_downwind": (None, [I]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -195,6 +198,37 @@ class OracleProblem:
                                krylbas, max_iter, tol, tol_abs, 1, P(x), ctypes.byref(it),
                                ctypes.byref(rr), ctypes.byref(rs))
         return x, dict(converged=bool(conv), iters=it.value, res_rel=rr.value, resid=rs.value)
+
+
+def system_with_face_types(op, ind, coef, src_model, neumann):
+    """the oracle's assembled system with the boundary faces flagged in
+    `neumann` taking the exact (or zero, for a constant-source model) normal
+    flux instead of the Dirichlet penalty: the assembly with penalty on every
+    face, then each Neumann face's penalty block and load replaced"""
+    v0, r0 = op.assemble(ind, coef, src_model=src_model)
+    v0, r0 = v0.copy(), r0.copy()
+    for f in np.nonzero(neumann)[0]:
+        K, F = op.face(int(f), g=0.0, src_model=src_model)
+        k = K.shape[0]
+        t = op.table[op.table_ptr[op.nel + f]:op.table_ptr[op.nel + f + 1]]
+        flux = op.face_neumann(int(f), 0.0, src_model=src_model)
+        for w in range(k):
+            for u in range(k):
+                v0[t[w + u * k]] -= K[u][w]
+            r0[t[k * k + w]] += flux[w] - F[w]
+    return v0, r0
+
+
+def std_errors(arrays, u_node, src_model):
+    """(L2 norm, H1 seminorm) of u_h - u_exact as pdr_error_test_exact
+    computes them; u_node holds one value per mesh node"""
+    et = np.ascontiguousarray(arrays["el_type"], np.int32)
+    en = np.ascontiguousarray(arrays["el_nodes"], np.int32)
+    coor = np.ascontiguousarray(arrays["coor"], np.float64)
+    u = np.ascontiguousarray(u_node, np.float64)
+    out = np.zeros(2)
+    oracle().orc_std_errors(len(et), P(et), P(en), P(coor), P(u), int(src_model), P(out))
+    return out[0], out[1]
 
 
 def oracle_gmres_crs(row_ptr, col_ind, val, rhs, precon=1, krylbas=30, max_iter=1000, tol=1e-8):

@@ -409,3 +409,63 @@ def test_mixed_degree_system_does_not_depend_on_the_first_neighbour():
     A2, b2 = dg_system_dense(a, pd, ind, lambda e: coefs[e], 1.5, 1, swap=True)
     assert np.abs(A1 - A2).max() <= 1e-12 * np.abs(A1).max()
     assert np.abs(b1 - b2).max() <= 1e-12 * np.abs(b1).max()
+
+
+def _face_geometry(coor, nodes, side):
+    """area and unit normal (pointing out of the element) of a planar face"""
+    from oracle_lib import DG_FACE_V
+    X = coor[nodes[DG_FACE_V[side]]]
+    if len(X) == 3:
+        nv = 0.5 * np.cross(X[1] - X[0], X[2] - X[0])
+    else:
+        nv = 0.5 * np.cross(X[2] - X[0], X[3] - X[1])
+    area = np.linalg.norm(nv)
+    n = nv / area
+    if n @ (X.mean(0) - coor[nodes].mean(0)) < 0:
+        n = -n
+    return area, n
+
+
+@pytest.mark.parametrize("pdeg", [101, 202, 303])
+def test_upwind_convection_energy_identity(pdeg):
+    """the convection part of the DG form (volume term plus the upwind face
+    terms, pds_conv_diff_weakform_dg.c:819-881, and the Dirichlet inflow /
+    outflow faces, :1114-1160) with constant b obeys the upwind energy
+    identity B(w, w) = 1/2 sum_interior |b.n| int [w]^2 + 1/2 int_boundary
+    |b.n| w^2.  For w constant per element it is a closed form in the face
+    areas and normals of an affine mesh.  The downwind side, planted in the
+    oracle, flips the sign of the interior part and fails"""
+    import modfem_amd as mf
+    from oracle_lib import dg_faces, dg_shape, dg_system_dense, oracle
+    m = mf.Mesh.box_prism(3, 3, 2, lx=1.0, ly=1.3, lz=0.7, seed=4, jitter=0.0)
+    a = m.arrays()
+    coor, en = a["coor"], a["el_nodes"]
+    nel = en.shape[0]
+    rng = np.random.default_rng(pdeg)
+    b = rng.uniform(-1, 1, 3)
+    coef = np.r_[np.zeros(9), b, np.zeros(9)]
+    ind = mf.coeff_indicator("bx", "by", "bz")
+    # the constant function in the element base
+    pts = np.random.default_rng(0).uniform(0, 0.45, (60, 3))  # inside the reference prism
+    pts[:, 2] = 4 * pts[:, 2] - 0.9
+    Phi = np.array([dg_shape(pdeg, p)[0] for p in pts])
+    assert np.linalg.matrix_rank(Phi) == Phi.shape[1]
+    one = np.linalg.lstsq(Phi, np.ones(len(pts)), rcond=None)[0]
+    assert np.abs(Phi @ one - 1).max() <= 1e-13
+    c = rng.uniform(-1, 1, nel)
+    x = np.concatenate([ci * one for ci in c])
+    expect = 0.0
+    for (e1, s1, e2, s2) in dg_faces(en):
+        area, n = _face_geometry(coor, en[e1], s1)
+        jump2 = (c[e1] - c[e2]) ** 2 if e2 >= 0 else c[e1] ** 2
+        expect += 0.5 * abs(b @ n) * area * jump2
+    A, _ = dg_system_dense(a, pdeg, ind, lambda e: coef, penalty_const=0.0)
+    got = x @ A @ x
+    assert abs(got - expect) <= 1e-12 * abs(expect), (got, expect)
+    try:
+        oracle().orc_dg_set_plant_downwind(1)
+        Ad, _ = dg_system_dense(a, pdeg, ind, lambda e: coef, penalty_const=0.0)
+    finally:
+        oracle().orc_dg_set_plant_downwind(0)
+    assert abs(x @ Ad @ x - expect) > 0.1 * abs(expect)
+    m.free()

@@ -87,6 +87,56 @@ def test_convection_term_orientation():
     assert np.abs(K - Kc).max() <= 1e-13 * np.abs(Kc).max()
 
 
+def test_t_and_q_terms_closed_form():
+    """the t term K[u][w] += (t . grad phi_w) phi_u dV (aps_std_lin_intf.c:
+    2478-2492) and the q load F[w] += q . grad phi_w dV (:2710-2716).  On an
+    affine tet the gradients g_w are constant and int phi_u = vol/4, so
+    K[u][w] = vol/4 (t . g_w) (every row u equal) and F[w] = vol (q . g_w).
+    A transposed t term gives vol/4 (t . g_u) (every column equal), a sign
+    slip the negatives: both fail."""
+    rng = np.random.default_rng(4)
+    ind = mf.coeff_indicator("tx", "ty", "tz", "qx", "qy", "qz")
+    for _ in range(20):
+        v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) + rng.uniform(-.3, .3, (4, 3))
+        t = rng.uniform(-1, 1, 3)
+        q = rng.uniform(-1, 1, 3)
+        cf = coef_vec(a=np.zeros((3, 3)))
+        cf[12:15] = t
+        cf[16:19] = q
+        K, F = elem(TETRA, v, ind, cf)
+        vol, G = tet_closed_form(v)
+        Kc = np.outer(np.ones(4), G @ t) * vol / 4
+        assert np.abs(K - Kc).max() <= 1e-13 * np.abs(Kc).max()
+        assert np.abs(F - vol * (G @ q)).max() <= 1e-13 * np.abs(vol * (G @ q)).max()
+        assert np.abs(K - Kc.T).max() > 1e-3 * np.abs(Kc).max()  # the transposed form differs
+
+
+def test_t_and_q_terms_on_prisms_by_divergence():
+    """the same terms on distorted prisms, by identities the divergence
+    theorem fixes for constant t, q: sum over w of K[u][w] = int phi_u t .
+    grad(sum phi_w) = 0 (partition of unity); sum over u of K[u][w] = int t .
+    grad phi_w = F[w] of the q load with q = t; and sum over w of x_w F[w] =
+    int grad(x) q dV = vol q (the nodal coordinates interpolate x exactly, so
+    sum_w x_w grad phi_w = I)"""
+    rng = np.random.default_rng(5)
+    ind_t = mf.coeff_indicator("tx", "ty", "tz")
+    ind_q = mf.coeff_indicator("qx", "qy", "qz")
+    for _ in range(20):
+        base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]], float)
+        v = base + rng.uniform(-.15, .15, (6, 3))
+        t = rng.uniform(-1, 1, 3)
+        cf = coef_vec(a=np.zeros((3, 3)))
+        cf[12:15] = t
+        K, _ = elem(PRISM, v, ind_t, cf)
+        cq = coef_vec(a=np.zeros((3, 3)))
+        cq[16:19] = t
+        _, F = elem(PRISM, v, ind_q, cq)
+        vol = elem(PRISM, v, mf.coeff_indicator("s"), coef_vec(a=np.zeros((3, 3)), s=1.0))[1].sum()
+        assert np.abs(K.sum(1)).max() <= 1e-14
+        assert np.abs(K.sum(0) - F).max() <= 1e-14
+        assert np.abs(v.T @ F - vol * t).max() <= 1e-13
+
+
 def test_prism_row_sums_symmetry_volume():
     rng = np.random.default_rng(4)
     base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1]], float)

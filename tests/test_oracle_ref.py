@@ -108,6 +108,39 @@ def test_elem_calc_bitexact(el_type, base):
                 assert np.array_equal(a, b), (trial, control)
 
 
+@need_apstd
+def test_tet_face_area_and_normal_bitexact_with_hybrid_vertex_order():
+    """tetrahedron faces (controls 3-6): the oracle uses the hybrid mesh
+    module's face table {0,2,1},{0,1,3},{0,3,2},{1,2,3} (mmd_t4_prism/
+    mms_prism_t4_intf.cpp:218); the compiled reference is linked with the
+    prism module's {0,1,2},{0,1,3},{0,2,3},{1,2,3}.  Handing the reference
+    the element with vertices 1,2 (face 0) or 2,3 (face 2) exchanged makes
+    it read the hybrid table's vertices: area element and unit normal are
+    then bit-exact"""
+    L = apstd()
+    o = oracle()
+    rng = np.random.default_rng(9)
+    pd = np.array([1], np.int32)
+    perm = {3: [0, 2, 1, 3], 4: [0, 1, 2, 3], 5: [0, 1, 3, 2], 6: [0, 1, 2, 3]}
+    for trial in range(100):
+        X = random_tet(rng)
+        eta = rng.uniform(0, 0.33, 3)
+        for control in (3, 4, 5, 6):
+            nc_o = np.ascontiguousarray(X.reshape(-1))
+            nc_r = np.ascontiguousarray(X[perm[control]].reshape(-1))
+            vr, vo = np.zeros(3), np.zeros(3)
+            w = [np.zeros(4) for _ in range(4)] + [np.zeros(3)]
+            det_r = L.apr_elem_calc_3D(control, 1, P(pd), APC_BASE_TETRA_STD, P(eta), P(nc_r),
+                                       None, P(w[0]), P(w[1]), P(w[2]), P(w[3]), P(w[4]),
+                                       None, None, None, None, P(vr))
+            det_o = o.orc_elem_calc_3d(TETRA, control, P(eta), P(nc_o), P(w[0]), P(w[1]),
+                                       P(w[2]), P(w[3]), P(w[4]), P(vo))
+            assert det_r == det_o and np.array_equal(vr, vo), (trial, control)
+            # outward: the normal points away from the opposite vertex
+            opp = {3: 3, 4: 2, 5: 1, 6: 0}[control]
+            assert vo @ (X[opp] - X[[i for i in range(4) if i != opp]].mean(0)) < 0
+
+
 def write_box_jk(path, nx, ny, nz, seed):
     m = mf.Mesh.box_prism(nx, ny, nz, lx=2.0, ly=1.0, lz=0.5, seed=seed)
     m.write_jk(path)
@@ -173,6 +206,60 @@ def test_reference_golden_mesh_imports_identically():
         assert [nodes[i + 1] - 1 for i in range(6)] == list(a["el_nodes"][e, :6])
         e += 1
     assert e == m.nel == 1528
+
+
+def mesh_dump_paths(tmp_path):
+    import zipfile
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    zipfile.ZipFile(os.path.join(gold, "prism_cube", "laplace_std_mesh_prism.dat.zip")).extractall(tmp_path)
+    return [os.path.join(gold, "prism_cube", "mesh_prism.dat"),
+            os.path.join(gold, "dg_cube", "mesh_prism.dmp"),
+            os.path.join(str(tmp_path), "mesh_prism.dat")]
+
+
+@need_mesh
+def test_mesh_dump_reader_matches_reference_mesh_module(tmp_path):
+    """the reference's prism mesh dumps (ctest Laplace_std_in_cube
+    mesh_prism.dat, Laplace_dg mesh_prism.dmp, Laplace_std mesh_prism.dat
+    with 1702 nodes) through our reader and through the reference's
+    mmr_read_mesh + mmr_el_node_coor + mmr_fa_neig / mmr_fa_bc: element
+    order, element-local node order, coordinates and the boundary int-ent
+    list (owner, side, bc) identical"""
+    L = ref("libref_mesh.so")
+    sizes = []
+    for path in mesh_dump_paths(tmp_path):
+        m = mf.Mesh.read_dat(path)
+        a = m.arrays()
+        mid = L.mmr_init_mesh(1, path.encode(), None)  # MMC_MOD_FEM_PRISM_DATA
+        assert mid > 0
+        nodes, coor = (I * 10)(), (D * 30)()
+        el, e = 0, 0
+        while True:
+            el = L.mmr_get_next_elem_all(mid, el)
+            if el == 0:
+                break
+            L.mmr_el_node_coor(mid, el, nodes, coor)
+            assert el == e + 1
+            assert [nodes[i + 1] - 1 for i in range(6)] == list(a["el_nodes"][e, :6]), (path, el)
+            assert np.array_equal(np.array(coor[:18]).reshape(6, 3), a["coor"][a["el_nodes"][e, :6]])
+            e += 1
+        assert e == m.nel
+        fn, fs, shift = (I * 2)(), (I * 2)(), I()
+        ac, bc = (D * 4)(), (D * 2)()
+        fa, f = 0, 0
+        while True:
+            fa = L.mmr_get_next_face_all(mid, fa)
+            if fa == 0:
+                break
+            b = L.mmr_fa_bc(mid, fa)
+            if b > 0:
+                L.mmr_fa_neig(mid, fa, fn, fs, ctypes.byref(shift), None, ac, bc)
+                assert (fn[0] - 1, fs[0], b) == (a["bf_elem"][f], a["bf_side"][f], a["bf_bc"][f])
+                f += 1
+        assert f == m.nbf
+        sizes.append((m.nno, m.nel, m.nbf))
+        m.free()
+    assert sizes[0] == sizes[1] == (8, 2, 8) and sizes[2][0] == 1702
 
 
 def ref_crs():
