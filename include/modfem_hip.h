@@ -150,7 +150,9 @@ int mf_element_matrices(int pid, int first, int count, int precision, double* ou
    the reference's summation order; 1 one lane group per element with FP64
    atomics (order-dependent rounding); 2 two passes: element rows written
    once to a device buffer, then gathered per matrix row in ascending element
-   order (deterministic) */
+   order (deterministic).  Modes 0 and 2 hold a row's sums in LDS for rows of
+   up to 32 entries; a mesh with longer rows (the reference allows up to 200
+   neighbours, sih_intf.h:57-63) is assembled by mode 1 */
 int mf_set_assembly_mode(int pid, int mode);
 int mf_face_matrices(int pid, int first, int count, double* out);
 int mf_get_system(int pid, double* val, double* rhs);

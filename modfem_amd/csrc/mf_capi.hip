@@ -65,7 +65,7 @@ void free_problem_device(Problem& P) {
   void* ptrs[] = {P.d_coor, P.d_el_dof, P.d_el_off, P.d_bf_elem, P.d_bf_side, P.d_bf_off,
                   P.d_bf_g, P.d_row_ptr, P.d_col, P.d_val, P.d_rhs, P.d_val32,
                   P.d_coef, P.d_dinv, P.d_diag_ptr, P.d_dinv32, P.d_tile_row, P.d_V, P.d_H,
-                  P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work, P.d_sl_ptr,
+                  P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work, P.d_work2, P.d_sl_ptr,
                   P.d_scol, P.d_diag_k, P.d_inc_ptr, P.d_inc_el, P.d_inc_loc,
                   P.d_finc_ptr, P.d_finc_f, P.d_finc_loc, P.d_brows, P.d_el_offT,
                   P.d_erow, P.d_bf_type};
@@ -129,11 +129,19 @@ void to_reference_order(const Setup& S, const double* d, double* x) {
   for (int i = 0; i < S.n; i++) x[S.perm[i]] = d[i];
 }
 
+// the assembly mode a problem runs: the row-wise and two-pass modes keep a
+// row's sums in LDS for rows of up to 32 entries; a mesh with longer rows
+// (vertices of high valence) is assembled by element atomics instead
+static int effective_mode(const Problem& P) {
+  return (P.assembly_mode != 1 && P.setup.max_row > 32) ? 1 : P.assembly_mode;
+}
+
 // integrate + assemble in the problem's assembly mode
 int assemble_problem(Problem& P, int precision) {
-  if (P.assembly_mode == 0) {
+  const int mode = effective_mode(P);
+  if (mode == 0) {
     if (launch_assemble_rows(P, precision)) return -2;
-  } else if (P.assembly_mode == 2) {
+  } else if (mode == 2) {
     if (launch_assemble_two_pass(P, precision)) return -2;
   } else {
     if (launch_zero(P.d_val, P.nnz_sell) || launch_zero(P.d_rhs, P.n)) return -2;
@@ -1064,10 +1072,11 @@ int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems, doubl
   MF_HIP(hipEventCreate(&c));
   float te = 0, tf = 0;
   for (int r = 0; r < reps; r++) {
-    if (P->assembly_mode == 0 || P->assembly_mode == 2) {
+    const int mode = effective_mode(*P);
+    if (mode == 0 || mode == 2) {
       MF_HIP(hipEventRecord(a, stream()));
-      if (P->assembly_mode == 0 ? launch_assemble_rows(*P, precision)
-                                : launch_assemble_two_pass(*P, precision))
+      if (mode == 0 ? launch_assemble_rows(*P, precision)
+                    : launch_assemble_two_pass(*P, precision))
         return -2;
       MF_HIP(hipEventRecord(b, stream()));
       MF_HIP(hipEventRecord(c, stream()));

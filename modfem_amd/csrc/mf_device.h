@@ -211,6 +211,7 @@ struct Problem {
   GmresStat* h_stat = nullptr; // pinned
   double* d_x = nullptr;
   double* d_work = nullptr;    // n
+  double* d_work2 = nullptr;   // n + 1: mixed-precision correction and a reduction slot
   // last measured times (ms)
   double t_integrate_ms = 0, t_faces_ms = 0, t_solve_ms = 0;
   // operator of another discretisation (not owned); replaces the CRS/SELL

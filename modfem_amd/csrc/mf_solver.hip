@@ -1052,9 +1052,8 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   hipLaunchKernelGGL(k_round_f32, dim3((unsigned)((P.nnz_sell + 255) / 256)), dim3(256), 0, s,
                      P.nnz_sell, P.d_val, P.d_val32);
   if (!P.d_work) MF_HIP(hipMalloc(&P.d_work, (size_t)n * sizeof(double)));
-  double *d_r = P.d_work, *d_d = nullptr, *d_sum = nullptr;
-  MF_HIP(hipMalloc(&d_d, (size_t)n * sizeof(double)));
-  MF_HIP(hipMalloc(&d_sum, sizeof(double)));
+  if (!P.d_work2) MF_HIP(hipMalloc(&P.d_work2, (size_t)(n + 1) * sizeof(double)));
+  double *d_r = P.d_work, *d_d = P.d_work2, *d_sum = P.d_work2 + n;
   MF_HIP(hipMemsetAsync(P.d_x, 0, (size_t)n * sizeof(double), s));
   SolveArgs A;
   fill_solve_args(P, A, ilu ? 0 : precon, krylbas);
@@ -1107,8 +1106,6 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   P.t_solve_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipFree(d_d);
-  (void)hipFree(d_sum);
   if (rc < 0) return rc;
   if (h_x) MF_HIP(hipMemcpy(h_x, P.d_x, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   if (iters) *iters = total;

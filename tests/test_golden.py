@@ -219,3 +219,26 @@ def test_std_errors_closed_forms():
         l2, h1 = std_errors(a, lin, 9)
         assert l2 <= 1e-14 and h1 <= 1e-13
         m.free()
+
+
+def load_dg_dump(path):
+    """apr_write_field's DG dump: header, -pdeg, one line of element DOFs
+    per element"""
+    lines = [l for l in open(path).read().split("\n") if l.strip()]
+    return int(lines[1]), np.array([[float(v) for v in l.split()] for l in lines[2:]])
+
+
+def test_oracle_reproduces_reference_dg_cube_dump():
+    """the reference's DG known answer (ctest Laplace_dg: the 2-prism cube
+    mesh_prism.dmp, DG tensor degree 101, NIPG with penalty constant 1,
+    DIFF_IN_CUBE, bc 1 Dirichlet, bc 11 Neumann): every element DOF of
+    comparison/field.dmp to 1e-11 by a direct solve of the oracle's system"""
+    from oracle_lib import dg_system_dense
+    m = mf.Mesh.read_dat(os.path.join(GOLD, "dg_cube", "mesh_prism.dmp"))
+    pdeg, ref = load_dg_dump(os.path.join(GOLD, "dg_cube", "field.dmp"))
+    assert pdeg == -101
+    coef = np.r_[np.eye(3).ravel(), np.zeros(12)]
+    A, b = dg_system_dense(m.arrays(), 101, mf.IND_DIFF, lambda e: coef, 1.0, mf.SRC_DIFF_IN_CUBE)
+    x = np.linalg.solve(A, b).reshape(ref.shape)
+    assert np.abs(x - ref).max() <= 1e-11
+    m.free()

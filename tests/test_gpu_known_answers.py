@@ -109,3 +109,23 @@ def test_gpu_ldc_tetra_mesh_against_oracle_and_gates(mode, tmp_path):
     assert l2 < 0.000062 and h1 < 0.0066, (l2, h1)
     p.free()
     m.free()
+
+
+def test_gpu_reproduces_reference_dg_cube_dump():
+    """ctest Laplace_dg on the device: the 2-prism cube mesh dump, DG degree
+    101, penalty constant 1, DIFF_IN_CUBE; GMRES with block Jacobi to 1e-13:
+    the 12 element DOFs of comparison/field.dmp to 1e-10"""
+    from test_golden import load_dg_dump
+    m = mf.Mesh.read_dat(os.path.join(GOLD, "dg_cube", "mesh_prism.dmp"))
+    _, ref = load_dg_dump(os.path.join(GOLD, "dg_cube", "field.dmp"))
+    dg = mf.DgProblem(m, 101)
+    dg.set_penalty(1.0)
+    dg.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=np.r_[np.eye(3).ravel(), np.zeros(12)],
+                        src_model=mf.SRC_DIFF_IN_CUBE)
+    dg.assemble()
+    x, info = dg.gmres(precon=1, krylbas=30, max_iter=1000, tol=1e-13)
+    assert info["converged"], info
+    print("DG cube: max |x - dump|", np.abs(x.reshape(ref.shape) - ref).max())
+    assert np.abs(x.reshape(ref.shape) - ref).max() <= 1e-10
+    dg.free()
+    m.free()

@@ -469,3 +469,45 @@ def test_neumann_faces_of_an_exact_solution_problem(kind, mode):
     assert rel_err(val, v0) <= 1e-12 and rel_err(rhs, r0) <= 1e-12
     p.free()
 
+
+
+def star_mesh(k=40):
+    """tetrahedra around one vertex of valence k + 2: a ring of k points,
+    an apex above and below; the centre's matrix row has k + 3 entries"""
+    ang = 2 * np.pi * np.arange(k) / k
+    ring = np.c_[np.cos(ang), np.sin(ang), np.zeros(k)]
+    coor = np.vstack([[0.0, 0.0, 0.0], ring, [0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])
+    top, bot = k + 1, k + 2
+    el, bf_elem, bf_side = [], [], []
+    for i in range(k):
+        a, b = 1 + i, 1 + (i + 1) % k
+        el.append([0, a, b, top, -1, -1])   # positive volume
+        bf_elem.append(2 * i); bf_side.append(3)       # (a, b, top)
+        el.append([0, b, a, bot, -1, -1])
+        bf_elem.append(2 * i + 1); bf_side.append(3)   # (b, a, bot)
+    el = np.array(el, np.int32)
+    return mf.Mesh.from_arrays(coor, np.full(len(el), mf.TETRA, np.int32), el,
+                               np.array(bf_elem, np.int32), np.array(bf_side, np.int32),
+                               np.ones(len(bf_elem), np.int32))
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_rows_longer_than_32_entries_assemble_through_the_atomic_path(mode):
+    """a vertex of valence 42 (row of 43 entries): the row-wise and
+    two-pass modes hand the mesh to the element-atomic path, whose system
+    matches the oracle to 1e-12"""
+    m = star_mesh(40)
+    op = OracleProblem(m.arrays())
+    assert np.diff(op.row_ptr).max() == 43
+    p = mf.Problem.from_mesh(m)
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    cc[19] = 1.0
+    p.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc)
+    p.set_assembly_mode(mode)
+    p.integrate_assemble(64)
+    val, rhs = p.system()
+    v0, r0 = op.assemble(mf.IND_DIFF, np.tile(cc, (op.nel, 1)))
+    assert rel_err(val, v0) <= 1e-12 and rel_err(rhs, r0) <= 1e-12
+    p.free()
+    m.free()
