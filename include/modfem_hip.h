@@ -219,6 +219,19 @@ int mf_solve_mkb(int pid, const int* params, double conv_meas, int ini_zero, dou
    solution on the device). */
 int mf_gmres_mixed(int pid, int precon, int krylbas, int max_iter, double tol, double* x,
                    int* iters, double* res_rel, double* resid, int* refinements);
+/* mf_gmres with precon 0 (none) or 1 (Jacobi) that allocates no device
+   memory: no counterpart in the reference; it is for callers that cannot
+   allocate (code under graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  Every work array of the solve (start and solution
+   vectors, Jacobi inverses, Krylov basis, Hessenberg data, partial sums,
+   status) lives in a fixed 128 MiB device array per GPU, set up by mf_init;
+   a solve that needs more (about (krylbas + 4) * 8 * n bytes plus 2 MiB)
+   returns an error.  The problem's matrix and load are its inputs, x its
+   start (unless ini_zero) and output, in the DOF order of
+   mf_problem_get_dofs.  The array keeps the last call's contents; calls that
+   overlap on one GPU race.  Iterates equal mf_gmres's. */
+int mf_gmres_noalloc(int pid, int precon, int krylbas, int max_iter, double tol, int ini_zero,
+                     double* x, int* iters, double* res_rel, double* resid);
 /* host copy of the last mf_gmres solution (DOF order of mf_problem_get_dofs) */
 int mf_get_solution(int pid, double* x);
 /* true relative residual ||b - A x|| / ||b|| in FP64 on the device */

@@ -102,6 +102,7 @@ def _declare(L):
         "mf_spmv": (I, [I, V, V]),
         "mf_fill_precon": (I, [I, I]),
         "mf_gmres": (I, [I, I, I, I, D, I, V, V, V, V]),
+        "mf_gmres_noalloc": (I, [I, I, I, I, D, I, V, V, V, V]),
         "mf_true_residual": (I, [I, V, V]),
         "mf_get_solution": (I, [I, V]),
         "mf_problem_set_face_bc": (I, [I, V, V]),

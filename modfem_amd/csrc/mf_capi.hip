@@ -167,6 +167,7 @@ extern "C" {
 
 int mf_init(int device) {
   int rc = device_init(device);
+  if (rc == 0) rc = noalloc_scratch_init(device);
   if (rc == 0) g_inited = true;
   return rc;
 }
@@ -890,6 +891,13 @@ int mf_gmres(int pid, int precon, int krylbas, int max_iter, double tol, int ini
   // rc: 1 converged, 0 not converged, <0 error
   if (rc >= 0 && x) to_reference_order(P->setup, xd.data(), x);
   return rc;
+}
+
+int mf_gmres_noalloc(int pid, int precon, int krylbas, int max_iter, double tol, int ini_zero,
+                     double* x, int* iters, double* res_rel, double* resid) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  return gmres_noalloc(*P, precon, krylbas, max_iter, tol, ini_zero, x, iters, res_rel, resid);
 }
 
 int mf_mkb_read(const char* filename, int max_iter, int error_type, double error_tol,

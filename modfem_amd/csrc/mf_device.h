@@ -288,6 +288,12 @@ int launch_fill_jacobi(Problem& P);
 int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol,
                 double tol_abs, int ini_zero, double* h_x, int* iters, double* res_rel,
                 double* resid);
+// GMRES(m) with precon 0/1 whose work arrays live in a fixed device array
+// (no device allocation); noalloc_scratch_init looks the array up for the
+// current device (called by mf_init)
+int noalloc_scratch_init(int device);
+int gmres_noalloc(Problem& P, int precon, int krylbas, int max_iter, double tol, int ini_zero,
+                  double* h_x, int* iters, double* res_rel, double* resid);
 // FP64 refinement around GMRES with an FP32-valued operator
 int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, double* h_x,
                 int* iters, double* res_rel, double* resid, int* refinements);

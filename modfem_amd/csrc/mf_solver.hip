@@ -812,9 +812,13 @@ static void launch_arnoldi(int G_op, int G_up, SolveArgs& A, int j) {
 // d_b / d_x: right-hand side and iterate in device memory; op32: the
 // operator reads the FP32 copy of the values (Gram-Schmidt, Givens and the
 // iterate stay FP64)
+// noalloc: no events are created and the status is read back by a plain
+// copy (the pinned mirror is absent); d_x already holds the start vector
+// unless ini_zero
 static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double tol,
                       double tol_abs, int ini_zero, double* h_x, const double* d_b, double* d_x,
-                      int op32, int* iters, double* res_rel_out, double* resid_out) {
+                      int op32, int* iters, double* res_rel_out, double* resid_out,
+                      bool noalloc = false) {
   const int m = krylbas;
   if (m < 1 || m >= MAX_NV) {
     set_error("GMRES: number of Krylov vectors must be in [1,63]");
@@ -862,15 +866,27 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
   const double* d_scale = P.d_H + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
   if (!P.sell) A.sl_ptr = nullptr;
   hipStream_t s = stream();
-  hipEvent_t e0, e1;
-  MF_HIP(hipEventCreate(&e0));
-  MF_HIP(hipEventCreate(&e1));
-  MF_HIP(hipEventRecord(e0, s));
-  if (ini_zero || !h_x) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!noalloc) {
+    MF_HIP(hipEventCreate(&e0));
+    MF_HIP(hipEventCreate(&e1));
+    MF_HIP(hipEventRecord(e0, s));
+  }
+  if (ini_zero || (!h_x && !noalloc)) {
     MF_HIP(hipMemsetAsync(d_x, 0, (size_t)P.n * sizeof(double), s));
-  } else {
+  } else if (h_x) {
     MF_HIP(hipMemcpyAsync(d_x, h_x, (size_t)P.n * sizeof(double), hipMemcpyHostToDevice, s));
   }
+  // status written by the finishing kernels: pinned mirror, or a copy
+  auto read_stat = [&](GmresStat& out) -> int {
+    if (P.h_stat) {
+      out = *P.h_stat;
+      return 0;
+    }
+    MF_HIP(hipMemcpyAsync(&out, P.d_stat, sizeof(GmresStat), hipMemcpyDeviceToHost, s));
+    MF_HIP(hipStreamSynchronize(s));
+    return 0;
+  };
   const double SMALL = 1.0e-15, MAX_CONVRATE = 0.999;
   // both tolerances floored at 1e-12 (lss_mkb_core.c:2377-2378)
   const double tol_rel = tol < 1e-12 ? 1e-12 : tol;
@@ -880,7 +896,10 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
   st.max_iter = max_iter;
   int restrt = max_iter / m + 1, restit = 0, gmiter = 0, conv = 0;
   double beta0 = 0, beta1 = 0, convrate = 0, faux = 0, resid = 0, res_rel = 0;
-  std::vector<double> H((size_t)(m + 1) * (m + 1) + 4 * (m + 1));
+  // Hessenberg columns, Givens pairs, scales and update coefficients (host
+  // copy of the device array), sized for the largest basis on the stack
+  double H[(MAX_NV + 1) * (MAX_NV + 1) + 4 * (MAX_NV + 1)];
+  const size_t Hn = (size_t)(m + 1) * (m + 1) + 4 * (m + 1);
   const double op_bytes_base =
       (op32 ? 8.0 : 12.0) * (double)P.nnz + 4.0 * (P.n + 1) + 24.0 * (double)P.n;
   do {
@@ -909,7 +928,9 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(SB), 0, s, A, G_op);
     MF_HIP(hipGetLastError());
     MF_HIP(hipStreamSynchronize(s));
-    double rnrm2 = P.h_stat->rnrm;
+    GmresStat rs{};
+    if (read_stat(rs)) return -2;
+    double rnrm2 = rs.rnrm;
     if (restit == 0) {
       convrate = 0.0; faux = 0.0; beta0 = rnrm2; beta1 = beta0;
     } else if (rnrm2 < SMALL * SMALL || gmiter == 0 || beta0 < SMALL * SMALL ||
@@ -924,8 +945,8 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     res_rel = beta0 != 0.0 ? rnrm2 / beta0 : 0.0;
     if (!(rnrm2 == rnrm2)) {
       set_error("GMRES: residual is not a number (singular matrix or zero diagonal?)");
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
       return -4;
     }
     if (rnrm2 < st.tol_res || res_rel < tol_rel || rnrm2 < SMALL * SMALL) {
@@ -933,13 +954,13 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
       break;
     }
     if (convrate > MAX_CONVRATE || faux > MAX_CONVRATE) break;
-    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(H, H + Hn, 0.0);
     H[0] = rnrm2;
-    double* scale = H.data() + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
+    double* scale = H + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
     scale[0] = 1.0 / rnrm2;
     st.beta0 = beta0;
     st.rnrm = rnrm2;
-    MF_HIP(hipMemcpyAsync(P.d_H, H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(P.d_H, H, Hn * sizeof(double), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(P.d_stat, &st, sizeof(GmresStat), hipMemcpyHostToDevice, s));
     if (ilu || gs || ext) {
       for (int j = 0; j < m; j++) {
@@ -963,9 +984,10 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
       for (int j = 0; j < m; j++) launch_arnoldi(G_op, G_up, A, j);
     }
     MF_HIP(hipGetLastError());
-    MF_HIP(hipMemcpyAsync(H.data(), P.d_H, H.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    MF_HIP(hipMemcpyAsync(H, P.d_H, Hn * sizeof(double), hipMemcpyDeviceToHost, s));
     MF_HIP(hipStreamSynchronize(s));
-    GmresStat hs = *P.h_stat;
+    GmresStat hs{};
+    if (read_stat(hs)) return -2;
     if (g_ks.on) {
       for (size_t q = 0; q < g_ks.kind.size(); q++)
         if (g_ks.kind[q] == 0) g_ks.bytes[q] = op_bytes_base + 8.0 * P.n * g_ks.step[q];
@@ -976,18 +998,18 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     resid = hs.resid;
     res_rel = hs.res_rel;
     // y = R^-1 g (dtrsv "U","N","N"), then x += V diag(s) y (dgemv)
-    std::vector<double> y(ik);
+    double y[MAX_NV];
     for (int i = ik - 1; i >= 0; i--) {
       double sum = H[i];
       for (int c = i + 1; c < ik; c++) sum -= H[(size_t)(c + 1) * (m + 1) + i] * y[c];
       y[i] = sum / H[(size_t)(i + 1) * (m + 1) + i];
     }
-    const double* sc = H.data() + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
-    std::vector<double> coef(std::max(ik, 1));
+    const double* sc = H + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
+    double coef[MAX_NV];
     for (int c = 0; c < ik; c++) coef[c] = y[c] * sc[c];
     double* d_coef = P.d_H + (size_t)(m + 1) * (m + 1) + 3 * (m + 1);
     if (ik > 0) {
-      MF_HIP(hipMemcpyAsync(d_coef, coef.data(), ik * sizeof(double), hipMemcpyHostToDevice, s));
+      MF_HIP(hipMemcpyAsync(d_coef, coef, ik * sizeof(double), hipMemcpyHostToDevice, s));
       int gx = (int)std::min<long long>((P.n + SB - 1) / SB, 4096);
       hipLaunchKernelGGL(k_xupdate, dim3(gx), dim3(SB), 0, s, A, ik);
       MF_HIP(hipGetLastError());
@@ -1000,13 +1022,15 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     if (gmiter == max_iter) break;
     restit++;
   } while (restit <= restrt);
-  MF_HIP(hipEventRecord(e1, s));
-  MF_HIP(hipEventSynchronize(e1));
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  P.t_solve_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (!noalloc) {
+    MF_HIP(hipEventRecord(e1, s));
+    MF_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    P.t_solve_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   if (h_x) MF_HIP(hipMemcpy(h_x, d_x, (size_t)P.n * sizeof(double), hipMemcpyDeviceToHost));
   if (iters) *iters = gmiter;
   if (res_rel_out) *res_rel_out = res_rel;
@@ -1115,6 +1139,150 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   return conv;
 }
 
+
+// ---------------------------------------------------------------------
+// GMRES without device allocation (mf_gmres_noalloc), for callers that may
+// not allocate: every work array of the solve (start and solution vectors,
+// Jacobi inverses, Krylov basis, Hessenberg data, dot-product partials,
+// status) lives in one file-scope device array of fixed size.  Each device
+// the library runs on holds its own copy from module load to process end;
+// its address is looked up once per device at mf_init.  The array keeps the
+// last call's contents, so everything a solve reads before writing is set
+// by the call itself; calls that overlap on one device race.
+// ---------------------------------------------------------------------
+constexpr size_t kNoallocScratchBytes = 128ull << 20;  // 128 MiB per device
+constexpr int kNoallocMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoallocScratchBytes / sizeof(double)];
+static double* g_scratch_ptr[kNoallocMaxDevices] = {};
+
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kNoallocMaxDevices) {
+    set_error("mf_init: device index beyond the no-allocation table");
+    return -1;
+  }
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error(std::string("mf_init: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    return -2;
+  }
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return 0;
+}
+
+__global__ void k_gather_perm(int n, const int* __restrict__ perm, const double* __restrict__ src,
+                              double* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+__global__ void k_scatter_perm(int n, const int* __restrict__ perm, const double* __restrict__ src,
+                               double* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+
+// doubles of scratch the solve of P with m Krylov vectors needs (each
+// region starts on a 256-byte boundary)
+static size_t noalloc_doubles(const Problem& P, int m) {
+  auto up = [](size_t d) { return (d + 31) / 32 * 32; };
+  const size_t ldv = (size_t)ldv_of(P.n);
+  const size_t G = std::max<size_t>(MAX_BLOCKS, (size_t)grid_slices(P));
+  return up(ldv) * 3                                   // reference x, device x, Jacobi
+         + up(((size_t)P.n + 1) / 2)                   // row permutation (int)
+         + up(G * MAX_NV)                              // dot-product partials
+         + up((sizeof(GmresStat) + 7) / 8)             // status
+         + up((size_t)(m + 1) * (m + 1) + 4 * (m + 1)) // Hessenberg data
+         + (size_t)(m + 1) * ldv;                      // Krylov basis
+}
+
+int gmres_noalloc(Problem& P, int precon, int krylbas, int max_iter, double tol, int ini_zero,
+                  double* h_x, int* iters, double* res_rel, double* resid) {
+  const int m = krylbas;
+  if (precon != 0 && precon != 1) {
+    set_error("mf_gmres_noalloc: precon must be 0 (none) or 1 (Jacobi)");
+    return -1;
+  }
+  if (m < 1 || m >= MAX_NV || max_iter < 1 || tol < 0) {
+    set_error("mf_gmres_noalloc: krylbas in [1,63], max_iter >= 1 and tol >= 0 required");
+    return -1;
+  }
+  if (!P.sell || P.ext) {
+    set_error("mf_gmres_noalloc: needs a problem assembled in SELL storage");
+    return -1;
+  }
+  int dev = -1;
+  MF_HIP(hipGetDevice(&dev));
+  double* base = (dev >= 0 && dev < kNoallocMaxDevices) ? g_scratch_ptr[dev] : nullptr;
+  if (!base) {
+    set_error("mf_gmres_noalloc: call mf_init on this device first");
+    return -1;
+  }
+  const size_t need = noalloc_doubles(P, m);
+  if (need * sizeof(double) > kNoallocScratchBytes) {
+    set_error("mf_gmres_noalloc: the solve needs " + std::to_string(need * sizeof(double) >> 20) +
+              " MiB of work space, more than the 128 MiB the no-allocation path holds");
+    return -1;
+  }
+  auto up = [](size_t d) { return (d + 31) / 32 * 32; };
+  const size_t ldv = (size_t)ldv_of(P.n);
+  const size_t G = std::max<size_t>(MAX_BLOCKS, (size_t)grid_slices(P));
+  double* q = base;
+  double* xr = q; q += up(ldv);
+  double* xd = q; q += up(ldv);
+  double* dinv = q; q += up(ldv);
+  int* perm = reinterpret_cast<int*>(q); q += up(((size_t)P.n + 1) / 2);
+  double* part = q; q += up(G * MAX_NV);
+  GmresStat* stat = reinterpret_cast<GmresStat*>(q); q += up((sizeof(GmresStat) + 7) / 8);
+  double* Hd = q; q += up((size_t)(m + 1) * (m + 1) + 4 * (m + 1));
+  double* V = q;
+  hipStream_t s = stream();
+  const int gn = (P.n + 255) / 256;
+  MF_HIP(hipMemsetAsync(V, 0, (size_t)(m + 1) * ldv * sizeof(double), s));
+  MF_HIP(hipMemsetAsync(stat, 0, sizeof(GmresStat), s));
+  if (!ini_zero && h_x) {
+    // start vector in reference order -> device order
+    MF_HIP(hipMemcpyAsync(xr, h_x, (size_t)P.n * sizeof(double), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(perm, P.setup.perm.data(), (size_t)P.n * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_gather_perm, dim3(gn), dim3(256), 0, s, P.n, perm, xr, xd);
+  } else {
+    MF_HIP(hipMemsetAsync(xd, 0, (size_t)P.n * sizeof(double), s));
+  }
+  // the solve's work arrays point into the scratch array for this call
+  // only; the problem's own pointers come back on every exit path
+  struct Swap {
+    Problem& P;
+    double *V, *H, *partials, *x, *dinv;
+    GmresStat *stat, *hstat;
+    int kryl;
+    bool dinv_valid, ks_on;
+    explicit Swap(Problem& p)
+        : P(p), V(p.d_V), H(p.d_H), partials(p.d_partials), x(p.d_x), dinv(p.d_dinv),
+          stat(p.d_stat), hstat(p.h_stat), kryl(p.kryl_alloc), dinv_valid(p.dinv_valid),
+          ks_on(g_ks.on) {}
+    ~Swap() {
+      P.d_V = V; P.d_H = H; P.d_partials = partials; P.d_x = x; P.d_dinv = dinv;
+      P.d_stat = stat; P.h_stat = hstat; P.kryl_alloc = kryl; P.dinv_valid = dinv_valid;
+      g_ks.on = ks_on;
+    }
+  } swap(P);
+  P.d_V = V; P.d_H = Hd; P.d_partials = part; P.d_x = xd; P.d_dinv = dinv;
+  P.d_stat = stat; P.h_stat = nullptr; P.kryl_alloc = m; P.dinv_valid = false;
+  g_ks.on = false;
+  // ini_zero 0 with no host vector: the iterate already sits in xd
+  const int rc = gmres_core(P, precon, m, max_iter, tol, 0.0, 0, nullptr, P.d_rhs, xd, 0, iters,
+                            res_rel, resid, true);
+  if (rc >= 0 && h_x) {
+    MF_HIP(hipMemcpyAsync(perm, P.setup.perm.data(), (size_t)P.n * sizeof(int),
+                          hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_scatter_perm, dim3(gn), dim3(256), 0, s, P.n, perm, xd, xr);
+    MF_HIP(hipMemcpyAsync(h_x, xr, (size_t)P.n * sizeof(double), hipMemcpyDeviceToHost, s));
+    MF_HIP(hipStreamSynchronize(s));
+  }
+  return rc;
+}
 
 // ---------------------------------------------------------------------
 // Batched GMRES.  All members advance one restart cycle per round of
