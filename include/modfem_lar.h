@@ -24,7 +24,10 @@ extern "C" {
    the matrix id (<0 on error).  LAC_STORAGE_CRS (2) with Block_size 1,
    LAC_STORAGE_BCRS (3) with Block_size 4 and Nrdof_glob = 4 Nrblocks, or
    LAC_STORAGE_UNDEFINED (-1), which picks by Block_size (las_intf.c:90-97);
-   other storage types are refused */
+   LAC_STORAGE_BLOCK (1) with a constant block size of 1 or 4 is held as CRS
+   or BCRS (same operations, sums in column order rather than lad_block's
+   diagonal-first order); variable block sizes and other storage types are
+   refused */
 int lar_allocate_SM_and_LV(int Storage_type, int solver_type, int Nrdof_glob,
                            int Max_SM_size, int Nrblocks, int Block_size,
                            int* Nrdofbl, int* Posglob, int* Nroffbl,

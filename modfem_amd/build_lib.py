@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 
 HOST_SRCS = ["mf_mesh.cpp", "mf_setup.cpp", "mf_mkb.cpp"]
 HIP_SRCS = ["mf_device.hip", "mf_solver.hip", "mf_capi.hip", "mf_lar.hip", "mf_ilu.hip",
-            "mf_gs.hip", "mf_dg.hip", "mf_bcrs.hip"]
+            "mf_gs.hip", "mf_dg.hip", "mf_bcrs.hip", "mf_accel.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -55,6 +55,21 @@ def build(verbose=False):
     if verbose:
         print("built", LIB)
     return LIB
+
+
+def build_test_drivers(verbose=False):
+    """the compiled C drivers of the test suite (tests/c): plain C programs
+    that reach the library only through the reference's seam headers"""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "c", "seam_driver.c")
+    out = os.path.join(root, "tests", "c", "seam_driver")
+    _run(["gcc", "-O2", "-std=c99", "-Wall", "-I" + os.path.join(root, "include"), src,
+          "-L" + HERE, "-lmodfem_hip", "-Wl,-rpath," + "$ORIGIN/../../modfem_amd", "-lm",
+          "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":

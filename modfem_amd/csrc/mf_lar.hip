@@ -14,10 +14,12 @@
 // is mf_integrate_assemble (include/modfem_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "mf_device.h"
@@ -33,6 +35,7 @@ using namespace mf;
 #define MULTI_ILU 4
 #define BLOCK_ILU 5
 #define LAC_STORAGE_UNDEFINED -1
+#define LAC_STORAGE_BLOCK 1
 #define LAC_STORAGE_CRS 2
 #define LAC_STORAGE_BCRS 3
 #define LAC_MAX_MATRICES 20
@@ -57,6 +60,10 @@ struct LarMatrix {
 };
 
 std::vector<LarMatrix> g_mats;  // LIFO like itv_crs_matrices
+
+int lar_bcrs_gmres(LarMatrix& L, int krylbas, int max_iter, double tol, double tol_abs,
+                   int nr_prec, int ini_zero, double* x, int* iters, double* res_rel,
+                   double* resid);
 
 LarMatrix* lar_get(int id) {
   if (id < 0 || id >= (int)g_mats.size() || !(g_mats[id].P || g_mats[id].B)) {
@@ -221,9 +228,15 @@ int lar_allocate_SM_and_LV(int Storage_type, int solver_type, int Nrdof_glob, in
   int storage = Storage_type;
   if (storage == LAC_STORAGE_UNDEFINED)
     storage = Block_size == 1 ? LAC_STORAGE_CRS : Block_size > 1 ? LAC_STORAGE_BCRS : -1;
+  // block storage (lad_block, the reference's choice for block-type > 0
+  // control files) with a constant block size is held as CRS (1x1 blocks)
+  // or BCRS (4x4 blocks): the same matrix and operations, whose sums run in
+  // column order instead of lad_block's diagonal-block-first order
+  if (storage == LAC_STORAGE_BLOCK && (Block_size == 1 || Block_size == BLS))
+    storage = Block_size == 1 ? LAC_STORAGE_CRS : LAC_STORAGE_BCRS;
   if (storage != LAC_STORAGE_CRS && storage != LAC_STORAGE_BCRS) {
-    set_error("lar_allocate_SM_and_LV: CRS (block size 1) and BCRS (block size 4) storage are "
-              "available");
+    set_error("lar_allocate_SM_and_LV: CRS (block size 1) and BCRS (block size 4) storage, and "
+              "block storage of constant size 1 or 4, are available");
     return -1;
   }
   if (storage == LAC_STORAGE_CRS && (Block_size != 1 || Nrblocks != Nrdof_glob)) {
@@ -459,9 +472,11 @@ int lar_free_SM_and_LV(int Matrix_id) {
     return 0;
   }
   Problem& P = *M->P;
-  void* ptrs[] = {P.d_row_ptr, P.d_col, P.d_val, P.d_rhs, P.d_dinv, P.d_diag_ptr, P.d_tile_row};
+  void* ptrs[] = {P.d_row_ptr, P.d_col, P.d_val, P.d_rhs, P.d_dinv, P.d_diag_ptr, P.d_tile_row,
+                  P.d_V, P.d_H, P.d_partials, P.d_stat, P.d_x};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (P.h_stat) (void)hipHostFree(P.h_stat);
   g_mats.pop_back();
   return 0;
 }
@@ -752,6 +767,19 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol, int nr_pr
     set_error("mf_lar_gmres: needs a BCRS matrix (CRS problems solve with mf_gmres)");
     return -1;
   }
+  return lar_bcrs_gmres(*M, krylbas, max_iter, tol, 0.0, nr_prec, ini_zero, x, iters, res_rel,
+                        resid);
+}
+
+}  // extern "C"
+
+namespace {
+// GMRES over a BCRS matrix's storage operations, tolerances relative to the
+// first preconditioned residual and absolute (each floored at 1e-12)
+int lar_bcrs_gmres(LarMatrix& L, int krylbas, int max_iter, double tol, double tol_abs,
+                   int nr_prec, int ini_zero, double* x, int* iters, double* res_rel,
+                   double* resid) {
+  LarMatrix* M = &L;
   const int kind = M->precon < 0 ? NO_PRECON : M->precon;
   if (kind != NO_PRECON && !M->filled) {
     set_error("mf_lar_gmres: the preconditioner is not filled");
@@ -776,8 +804,12 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol, int nr_pr
   }
   M->G->ext = M->op.get();
   M->G->d_rhs = B.d_rhs;
-  return gmres_solve(*M->G, 0, krylbas, max_iter, tol, 0.0, ini_zero, x, iters, res_rel, resid);
+  return gmres_solve(*M->G, 0, krylbas, max_iter, tol, tol_abs, ini_zero, x, iters, res_rel,
+                     resid);
 }
+}  // namespace
+
+extern "C" {
 
 // HIP-event times on the library stream of one BCRS operation, averaged over
 // reps, on device-resident vectors: what 0 residual v = b - A x, 1 one
@@ -937,6 +969,248 @@ int mf_lar_get_system(int Matrix_id, int* row_ptr, int* col_ind, double* val, do
   if (val) MF_HIP(hipMemcpy(val, P.d_val, P.nnz * sizeof(double), hipMemcpyDeviceToHost));
   if (rhs) MF_HIP(hipMemcpy(rhs, P.d_rhs, P.n * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Solver-module seam with the reference's own names and signatures
+// (lsd_mkb/lsh_mkb_intf.h:165-338, implemented in lsd_mkb/lss_mkb_intf.c and
+// lsd_mkb/lsd_mkb_core/lss_mkb_core.c): a solver instance reads its control
+// file (lsr_mkb_core_init, single level), owns one matrix of the storage seam
+// above, and solves with restarted GMRES on the device (lsr_mkb_core_solve /
+// lsr_mkb_core_gmres).  Solver ids count from 0 in creation order and are
+// destroyed in reverse order, as lsv_mkb_solver is (lss_mkb_intf.c:104-110).
+// ---------------------------------------------------------------------
+namespace {
+#define LSC_SOLVE 0
+#define LSC_RESOLVE 1
+#define LSC_MAX_NUM_SOLV 10
+#define REL_RES_INI 0
+#define ABS_RES 1
+
+struct LsrSolver {
+  bool alive = false;
+  int solver_type = 0;
+  MkbParams prm;
+  int matrix_id = -1;
+  int nrdofgl = 0;
+};
+std::vector<LsrSolver> g_lsr;
+
+LsrSolver* lsr_get(int id, int level = 0) {
+  if (id < 0 || id >= (int)g_lsr.size() || !g_lsr[id].alive) {
+    set_error("lsr_mkb_: unknown solver id " + std::to_string(id));
+    return nullptr;
+  }
+  if (level != 0) {
+    set_error("lsr_mkb_: single-level solvers only (level 0)");
+    return nullptr;
+  }
+  return &g_lsr[id];
+}
+}  // namespace
+
+extern "C" {
+
+// lsr_mkb_init (lss_mkb_intf.c:88-175): control parameters from Filename
+// (arguments >= 0 override them), single level; returns the solver id
+int lsr_mkb_init(int Solver_type, int Parallel, int* Max_num_levels_p, int* Storage_type,
+                 char* Filename, int Max_iter, int Error_type, double Error_tolerance,
+                 int Monitoring_level) {
+  if (Parallel != 0) {
+    set_error("lsr_mkb_init: the sequential solver only (one subdomain per device)");
+    return -1;
+  }
+  if (Solver_type <= 0 || Solver_type >= 100) {
+    set_error("lsr_mkb_init: the MKB iterative solvers (types 1..99) only");
+    return -1;
+  }
+  if ((int)g_lsr.size() >= LSC_MAX_NUM_SOLV) {
+    set_error("lsr_mkb_init: too many solvers");
+    return -1;
+  }
+  LsrSolver S;
+  if (mkb_read(Filename, Max_iter, Error_type, Error_tolerance, Monitoring_level, S.prm) < 0)
+    return -1;
+  if (S.prm.solver != 1 && !(Max_num_levels_p && *Max_num_levels_p == 1)) {
+    set_error("lsr_mkb_init: single-level GMRES only");
+    return -1;
+  }
+  S.alive = true;
+  S.solver_type = Solver_type;
+  if (Max_num_levels_p) *Max_num_levels_p = 1;
+  if (Storage_type) *Storage_type = S.prm.storage_type;
+  g_lsr.push_back(S);
+  return (int)g_lsr.size() - 1;
+}
+
+// lsr_mkb_destroy (lss_mkb_intf.c:355-410): the last created solver first
+int lsr_mkb_destroy(int Solver_id) {
+  if (!lsr_get(Solver_id)) return -3;
+  if (Solver_id != (int)g_lsr.size() - 1) {
+    set_error("lsr_mkb_destroy: solvers are destroyed in reverse order of creation");
+    return -1;
+  }
+  if (g_lsr.back().matrix_id >= 0) {
+    set_error("lsr_mkb_destroy: free the solver's matrix first (lsr_mkb_free_matrix)");
+    return -1;
+  }
+  g_lsr.pop_back();
+  return 1;
+}
+
+// lsr_mkb_create_matrix (lss_mkb_intf.c:417-520): the storage type enforced
+// by the caller, else the control file's, through lar_allocate_SM_and_LV
+int lsr_mkb_create_matrix(int Solver_id, int Level_id, int Storage_type, int Nrblocks,
+                          int Nrdof_glob, int Block_size, int Max_sm_size, int* Nrdofbl,
+                          int* Posglob, int* Nroffbl, int** L_offbl) {
+  LsrSolver* S = lsr_get(Solver_id, Level_id);
+  if (!S) return -3;
+  if (S->matrix_id >= 0) {
+    set_error("lsr_mkb_create_matrix: the solver already has a matrix");
+    return -1;
+  }
+  const int storage = Storage_type != LAC_STORAGE_UNDEFINED ? Storage_type : S->prm.storage_type;
+  const int id = lar_allocate_SM_and_LV(storage, S->solver_type, Nrdof_glob, Max_sm_size,
+                                        Nrblocks, Block_size, Nrdofbl, Posglob, Nroffbl, L_offbl);
+  if (id < 0) return id;
+  S->matrix_id = id;
+  S->nrdofgl = Nrdof_glob;
+  return 0;
+}
+
+// lsr_mkb_clear_matrix (lss_mkb_intf.c:528-600): LSC_SOLVE clears values and
+// rhs, LSC_RESOLVE the rhs
+int lsr_mkb_clear_matrix(int Solver_id, int Level_id, int Comp_type) {
+  LsrSolver* S = lsr_get(Solver_id, Level_id);
+  if (!S) return -3;
+  return lar_initialize_SM_and_LV(S->matrix_id,
+                                  Comp_type == LSC_RESOLVE ? LAC_SCOPE_LV : LAC_SCOPE_SM_AND_LV);
+}
+
+// lsr_mkb_fill_assembly_table_int_ent (lss_mkb_intf.c:605-635)
+int lsr_mkb_fill_assembly_table_int_ent(int Solver_id, int Level_id, int Nr_dof_bl, int* L_bl_id,
+                                        int* L_bl_nrdof, int* Assembly_table_int_ent) {
+  LsrSolver* S = lsr_get(Solver_id, Level_id);
+  if (!S) return -3;
+  return lar_fill_assembly_table_int_ent(S->matrix_id, Nr_dof_bl, L_bl_id, L_bl_nrdof,
+                                         Assembly_table_int_ent);
+}
+
+// lsr_mkb_assemble_local_stiff_mat_with_table (lss_mkb_intf.c:641-700)
+int lsr_mkb_assemble_local_stiff_mat_with_table(int Solver_id, int Level_id, int Comp_type,
+                                                int Nr_dof_bl, int* Assembly_table_int_ent,
+                                                double* Stiff_mat, double* Rhs_vect,
+                                                char* Rewr_dofs) {
+  LsrSolver* S = lsr_get(Solver_id, Level_id);
+  if (!S) return -3;
+  return lar_assemble_SM_and_LV_with_table(
+      S->matrix_id, Comp_type == LSC_RESOLVE ? LAC_SCOPE_LV : LAC_SCOPE_SM_AND_LV, Nr_dof_bl,
+      Assembly_table_int_ent, Stiff_mat, Rhs_vect, Rewr_dofs);
+}
+
+// lsr_mkb_assemble_local_sm (lss_mkb_intf.c:705-785)
+int lsr_mkb_assemble_local_sm(int Solver_id, int Level_id, int Comp_type, int Nr_dof_bl,
+                              int* L_bl_id, int* L_bl_nrdof, double* Stiff_mat, double* Rhs_vect,
+                              char* Rewr_dofs) {
+  LsrSolver* S = lsr_get(Solver_id, Level_id);
+  if (!S) return -3;
+  return lar_assemble_SM_and_LV(S->matrix_id,
+                                Comp_type == LSC_RESOLVE ? LAC_SCOPE_LV : LAC_SCOPE_SM_AND_LV,
+                                Nr_dof_bl, L_bl_id, L_bl_nrdof, Stiff_mat, Rhs_vect, Rewr_dofs);
+}
+
+// lsr_mkb_free_matrix (lss_mkb_intf.c:791-838): preconditioner and matrix
+int lsr_mkb_free_matrix(int Solver_id) {
+  LsrSolver* S = lsr_get(Solver_id);
+  if (!S) return -3;
+  if (S->matrix_id < 0) return 0;
+  (void)lar_free_preconditioner(S->matrix_id);
+  const int rc = lar_free_SM_and_LV(S->matrix_id);
+  if (rc < 0) return rc;
+  S->matrix_id = -1;
+  return 1;
+}
+
+// lsr_mkb_create_precon (lss_mkb_intf.c:844-880 -> lsr_mkb_core_create_precon,
+// lss_mkb_core.c:964-1020): the control file's preconditioner, with its
+// block type as the second argument, or ILU's k for non-block storage;
+// returns the number of diagonal blocks
+int lsr_mkb_create_precon(int Solver_id, int Level_id) {
+  LsrSolver* S = lsr_get(Solver_id, Level_id);
+  if (!S) return -3;
+  const int precon = S->prm.precon;
+  if (precon == NO_PRECON) return 1;
+  const int second = (precon == MULTI_ILU || precon == BLOCK_ILU) ? S->prm.ilu_k
+                                                                   : S->prm.block_type;
+  return lar_allocate_preconditioner(S->matrix_id, precon, second);
+}
+
+// lsr_mkb_fill_precon (lss_mkb_intf.c:888-940)
+int lsr_mkb_fill_precon(int Solver_id, int Level_id) {
+  LsrSolver* S = lsr_get(Solver_id, Level_id);
+  if (!S) return -3;
+  if (S->prm.precon == NO_PRECON) return 1;
+  return lar_fill_preconditioner(S->matrix_id);
+}
+
+// lsr_mkb_solve (lss_mkb_intf.c:184-350 -> lsr_mkb_core_solve, lss_mkb_core.c:
+// 712-925): *Nr_iter > 0 and *Toler > 0 override the control file; the
+// tolerance is relative to the first preconditioned residual (REL_RES_INI),
+// absolute (ABS_RES), or -- as the reference resets it -- none for
+// REL_RES_RHS, each floored at 1e-12 (lss_mkb_core.c:2377-2384).  B != NULL
+// replaces the stored rhs for this solve.  Returns 1 on convergence, 0
+// otherwise; *Nr_iter, *Toler (the final measure of the chosen kind) and
+// *Conv_rate (the mean reduction per iteration) come back
+int lsr_mkb_solve(int Solver_id, int Comp_type, int Ini_zero, double* X, double* B,
+                  int* Nr_iter, double* Toler, int Monitor, double* Conv_rate) {
+  (void)Comp_type; (void)Monitor;
+  LsrSolver* S = lsr_get(Solver_id);
+  if (!S) return -3;
+  LarMatrix* M = lar_get(S->matrix_id);
+  if (!M) return -3;
+  const MkbParams& p = S->prm;
+  const int max_iter = (Nr_iter && *Nr_iter > 0) ? *Nr_iter : p.max_iter;
+  const double meas = (Toler && *Toler > 0) ? *Toler : p.conv_meas;
+  const double tol_rel = p.conv_type == REL_RES_INI ? meas : 0.0;
+  const double tol_abs = p.conv_type == ABS_RES ? meas : 0.0;
+  if (flush(*M)) return -2;
+  const int n = M->ndof();
+  // the rhs of this solve: the stored one, or B staged in its place
+  double* d_rhs_saved = M->B ? M->B->d_rhs : M->P->d_rhs;
+  Staged sb;
+  if (B) {
+    if (stage_in(B, n, sb, true)) return -2;
+    if (M->B) M->B->d_rhs = sb.d;
+    else M->P->d_rhs = sb.d;
+  }
+  int it = 0, rc = 0;
+  double rr = 0.0, rs = 0.0;
+  if (M->B) {
+    rc = lar_bcrs_gmres(*M, p.krylbas, max_iter, tol_rel, tol_abs, p.nr_prec, Ini_zero, X, &it,
+                        &rr, &rs);
+  } else {
+    int precon = 0;
+    if (p.precon == BLOCK_JACOBI && M->precon == BLOCK_JACOBI && p.nr_prec == 1) precon = 1;
+    else if (p.precon != NO_PRECON) {
+      set_error("lsr_mkb_solve: CRS storage solves with no preconditioner or one block-Jacobi "
+                "sweep");
+      rc = -1;
+    }
+    if (rc == 0)
+      rc = gmres_solve(*M->P, precon, p.krylbas, max_iter, tol_rel, tol_abs, Ini_zero, X, &it,
+                       &rr, &rs);
+  }
+  if (B) {
+    if (M->B) M->B->d_rhs = d_rhs_saved;
+    else M->P->d_rhs = d_rhs_saved;
+  }
+  if (rc < 0) return rc;
+  if (Nr_iter) *Nr_iter = it;
+  if (Toler) *Toler = p.conv_type == ABS_RES ? rs : rr;
+  if (Conv_rate) *Conv_rate = (it > 0 && rr > 0.0) ? std::exp(std::log(rr) / it) : 0.0;
+  return rc;
 }
 
 }  // extern "C"
