@@ -43,32 +43,146 @@ def make_workload(n, seed):
     return mesh, kappa, b, cc, coef_el
 
 
-def cpu_baseline(mesh, kappa, b, gpu_iters, krylbas, threads, sample_iters):
-    """oracle timed on the host: full integrate+assemble of the workload plus a
-    bounded GMRES sample, extrapolated to the GPU iteration count"""
+def _median_time(fn, reps, warmup=1):
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def _ref_crs_residual_GBs(op, val, rhs, reps):
+    """the reference's own lar_compute_residual_crs and one block-Jacobi sweep
+    (lar_perform_BJ_or_GS_iterations_crs), compiled from las_crs_intf.c into
+    oracle/_ref, on the oracle's assembled system; algorithmic bytes of the
+    residual over its median time (None when oracle/_ref is absent)"""
+    import ctypes
+    from oracle_lib import ref, ref_available
+    if not ref_available("libref_crs.so"):
+        return None
+    L = ref("libref_crs.so")
+    I = ctypes.c_int
+    P = lambda v: v.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    lists = []
+    for r in range(op.n):
+        cols = op.col_ind[op.row_ptr[r]: op.row_ptr[r + 1]]
+        lists.append(np.ascontiguousarray(cols[cols != r], np.int32))
+    nro = np.array([len(x) for x in lists], np.int32)
+    ptrs = (ctypes.POINTER(I) * op.n)(*[x.ctypes.data_as(ctypes.POINTER(I)) for x in lists])
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)  # the reference prints progress on stdout
+    try:
+        mid = L.lar_allocate_SM_and_LV_crs(op.n, 36, P(nro), ptrs)
+        rp, ci, va, rh = (ctypes.POINTER(I)(), ctypes.POINTER(I)(), ctypes.POINTER(ctypes.c_double)(),
+                          ctypes.POINTER(ctypes.c_double)())
+        L.lar_get_SM_and_LV_crs_from_crs.argtypes = [I, I] + [ctypes.c_void_p] * 4
+        L.lar_get_SM_and_LV_crs_from_crs(mid, 0, ctypes.byref(rp), ctypes.byref(ci),
+                                         ctypes.byref(va), ctypes.byref(rh))
+        np.ctypeslib.as_array(va, (op.nnz,))[:] = val
+        np.ctypeslib.as_array(rh, (op.n,))[:] = rhs
+        x = np.random.default_rng(1).uniform(-1, 1, op.n)
+        v = np.zeros(op.n)
+        L.lar_compute_residual_crs.restype = None
+        L.lar_compute_residual_crs.argtypes = [I, I, I, I] + [ctypes.c_void_p] * 3
+        t_res = _median_time(lambda: L.lar_compute_residual_crs(mid, 1, 0, op.n, P(x), None, P(v)),
+                             reps)
+        L.lar_allocate_preconditioner_crs(mid, 1, 0)
+        L.lar_fill_preconditioner_crs(mid)
+        L.lar_perform_BJ_or_GS_iterations_crs.restype = None
+        L.lar_perform_BJ_or_GS_iterations_crs.argtypes = [I, I, I, I, I, ctypes.c_void_p,
+                                                          ctypes.c_void_p]
+        t_bj = _median_time(lambda: L.lar_perform_BJ_or_GS_iterations_crs(mid, 1, 0, 1, op.n, P(v),
+                                                                          None), reps)
+        L.lar_free_SM_and_LV_crs(mid)
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+    b_res = 12.0 * op.nnz + 4.0 * (op.n + 1) + 16.0 * op.n
+    return {"residual_GBs": b_res / t_res / 1e9, "residual_ms": t_res * 1e3,
+            "jacobi_sweep_ms": t_bj * 1e3}
+
+
+def cpu_baseline(mesh, kappa, b, gpu_iters, krylbas, sample_iters, seed):
+    """the host-side baseline of the same workload: the oracle (the C
+    restatement of the reference path, OpenMP) at the job's host threads and
+    at one core, phases timed separately -- set-up (DOF list, pattern, tables),
+    integrate+assemble, Jacobi fill, GMRES per iteration -- with one warm-up
+    and the median of 5 (3 at one core); a step is integrate+assemble, fill
+    and the GPU's iteration count of GMRES iterations.  The one-core
+    integrate+assemble and the reference's own compiled CRS residual and
+    Jacobi sweep are timed on a 32^3 box of the same generator"""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import modfem_amd as mf
-    from oracle_lib import OracleProblem, oracle
-    oracle().orc_set_threads(threads)
+    from oracle_lib import OracleProblem, oracle, P
+    nproc = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+
+    def coef_of(kap, bb):
+        c = np.zeros((len(kap), 21))
+        c[:, 0] = c[:, 4] = c[:, 8] = kap
+        c[:, 9:12] = bb
+        c[:, 19] = 1.0
+        return c
+
     arr = mesh.arrays()
+    t0 = time.perf_counter()
     op = OracleProblem(arr)
-    coef = np.zeros((mesh.nel, 21))
-    coef[:, 0] = coef[:, 4] = coef[:, 8] = kappa
-    coef[:, 9:12] = b
-    coef[:, 19] = 1.0
-    t0 = time.perf_counter()
-    val, rhs = op.assemble(mf.IND_CONV_DIFF, coef)
-    t_ia = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    _, info = op.gmres(val, rhs, precon=1, krylbas=krylbas, max_iter=sample_iters, tol=1e-8)
-    t_it = (time.perf_counter() - t0) / max(info["iters"], 1)
-    t_step = t_ia + gpu_iters * t_it
+    t_setup = time.perf_counter() - t0
+    coef = coef_of(kappa, b)
+    res = {"threads": threads, "nproc": nproc, "setup_s": t_setup}
+    o = oracle()
+    for nt, reps in ((threads, 5), (1, 3)):
+        o.orc_set_threads(nt)
+        key = "threads_%d" % nt
+        rec = {}
+        if nt == threads:
+            rec["integrate_assemble_s"] = _median_time(lambda: op.assemble(mf.IND_CONV_DIFF, coef), reps)
+            val, rhs = op.val.copy(), op.rhs.copy()
+        else:
+            # one core: a 32^3 box of the same generator, per element
+            sm, kap_s, b_s, _, _ = make_workload(32, seed)
+            ops = OracleProblem(sm.arrays())
+            cs = coef_of(kap_s, b_s)
+            t = _median_time(lambda: ops.assemble(mf.IND_CONV_DIFF, cs), reps)
+            rec["integrate_assemble_s"] = t * mesh.nel / sm.nel
+            rec["integrate_assemble_sample"] = "32^3 box (%d prisms), scaled per element" % sm.nel
+        dinv, dptr = np.zeros(op.n), np.zeros(op.n, np.int32)
+        rec["jacobi_fill_ms"] = _median_time(
+            lambda: o.orc_fill_jacobi(op.n, P(op.row_ptr), P(op.col_ind), P(val), P(dptr), P(dinv)),
+            reps) * 1e3
+        its = sample_iters if nt == threads else max(3, sample_iters // 10)
+        t0 = time.perf_counter()
+        _, info = op.gmres(val, rhs, precon=1, krylbas=krylbas, max_iter=its, tol=1e-8)
+        rec["gmres_ms_per_iter"] = (time.perf_counter() - t0) / max(info["iters"], 1) * 1e3
+        rec["gmres_sample_iters"] = info["iters"]
+        rec["step_s"] = (rec["integrate_assemble_s"] + rec["jacobi_fill_ms"] * 1e-3 +
+                         gpu_iters * rec["gmres_ms_per_iter"] * 1e-3)
+        rec["elements_per_s"] = mesh.nel / rec["step_s"]
+        res[key] = rec
+    o.orc_set_threads(threads)
+    # a 64^3 box (275k rows, 5.7 M non-zeros: larger than the host caches)
+    sm, kap_s, b_s, _, _ = make_workload(64, seed)
+    ops = OracleProblem(sm.arrays())
+    v_s, r_s = ops.assemble(mf.IND_CONV_DIFF, coef_of(kap_s, b_s))
+    res["reference_crs_64cubed"] = _ref_crs_residual_GBs(ops, v_s.copy(), r_s.copy(), 5)
+    full = res["threads_%d" % threads]
     return {
-        "value": mesh.nel / t_step, "unit": "elements/s", "cores": threads, "kind": "port",
-        "sample": (f"oracle (C, OpenMP) on the same mesh: full integrate+assemble "
-                   f"({t_ia:.2f} s) + {info['iters']} GMRES({krylbas}) iterations "
-                   f"({t_it * 1e3:.1f} ms/it) extrapolated to the GPU's {gpu_iters} iterations"),
-        "integrate_assemble_s": t_ia, "gmres_ms_per_iter": t_it * 1e3,
+        "value": full["elements_per_s"], "unit": "elements/s", "cores": threads, "kind": "port",
+        "sample": ("oracle (C restatement, OpenMP, %d threads of %d host cores) on the same mesh: "
+                   "set-up %.2f s; integrate+assemble %.2f s, Jacobi fill %.1f ms, GMRES(%d) "
+                   "%.1f ms/it over %d iterations (medians of 5 after a warm-up), extrapolated "
+                   "to the GPU's %d iterations; one-core figures and the reference's compiled "
+                   "CRS residual beside it" % (threads, nproc, t_setup, full["integrate_assemble_s"],
+                                               full["jacobi_fill_ms"], krylbas,
+                                               full["gmres_ms_per_iter"], full["gmres_sample_iters"],
+                                               gpu_iters)),
+        "phases": res,
     }
 
 
@@ -187,7 +301,7 @@ def bench_batch(args):
     }
     for p in probs:
         p.free()
-    print(json.dumps(out))
+    return out
 
 
 def footprint(p, m):
@@ -289,7 +403,7 @@ def bench_serve(args):
     }
     for p in probs:
         p.free()
-    print(json.dumps(out))
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -451,7 +565,7 @@ def bench_dg(args):
                                    "sample": "oracle (C, one core) element + face integrals of the first "
                                              "20000/4000/800 elements at p=1/2/3 and their faces, "
                                              "combined for equal shares of the three degrees"}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -689,6 +803,34 @@ def bcrs_cpu_time(R, P, en, em, nn, nb_ind, nb_cnt, res_bytes, gomp, threads):
                       "from its sources, OpenMP at one thread) on the full system" % reps}
 
 
+def other_workloads(args):
+    """the batch (256 problems), served-stream and mixed-degree DG
+    configurations, compact, measured in the same run"""
+    out = {}
+    a = argparse.Namespace(**vars(args))
+    a.no_cpu = True
+    a.count, a.el_min, a.el_max = 256, 2000, 200000
+    print("workload: batch of 256 problems", file=sys.stderr, flush=True)
+    r = bench_batch(a)
+    out["batch_256"] = {k: r[k] for k in ("value", "unit", "config", "batch_s", "loop_s",
+                                          "loop_solves_per_s", "speedup_vs_loop",
+                                          "all_converged", "iters", "max_rel_diff_vs_loop_solution",
+                                          "tiles")}
+    a.count, a.el_min, a.el_max, a.rates = 128, 5000, 500000, [1000.0, 2000.0]
+    print("workload: served stream", file=sys.stderr, flush=True)
+    r = bench_serve(a)
+    out["serve"] = {k: r[k] for k in ("value", "unit", "config", "rates")}
+    a.dg_n, a.dg_pdegs, a.dg_solve, a.steps, a.warmup = 96, [], [], 3, 1
+    print("workload: DG mixed degrees 96^3", file=sys.stderr, flush=True)
+    r = bench_dg(a)
+    out["dg_mixed_96"] = {k: r[k] for k in ("value", "unit", "config", "roofline", "roofline_op")}
+    a.dg_n, a.dg_solve = 48, ["mixed"]
+    print("workload: DG mixed degrees 48^3 solve", file=sys.stderr, flush=True)
+    r = bench_dg(a)
+    out["dg_mixed_48_solve"] = r["config"]["levels"]["mixed"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -698,7 +840,8 @@ def main():
     ap.add_argument("--krylbas", type=int, default=50)
     ap.add_argument("--seed", type=int, default=2017)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--quick", action="store_true",
+                    help="the default workload only (no batch, serve and DG figures)")
     ap.add_argument("--cpu-iters", type=int, default=50)
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: one integrate+assemble, one "
@@ -728,13 +871,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
     if args.workload == "batch":
-        return bench_batch(args)
+        print(json.dumps(bench_batch(args)), flush=True)
+        return
     if args.workload == "dg":
-        return bench_dg(args)
+        print(json.dumps(bench_dg(args)), flush=True)
+        return
     if args.workload == "bcrs":
         return bench_bcrs(args)
     if args.workload == "serve":
-        return bench_serve(args)
+        print(json.dumps(bench_serve(args)), flush=True)
+        return
     import modfem_amd as mf
     t_setup = time.perf_counter()
     mesh, kappa, b, cc, coef_el = make_workload(args.n, args.seed)
@@ -863,10 +1009,11 @@ def main():
         "ilu0_multicolour": ilu,
         "setup_s": t_setup,
     }
+    if not args.quick:
+        out["workloads"] = other_workloads(args)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(mesh, kappa, b, iters, args.krylbas, threads,
-                                           args.cpu_iters)
+        out["cpu_baseline"] = cpu_baseline(mesh, kappa, b, iters, args.krylbas, args.cpu_iters,
+                                           args.seed)
     print(json.dumps(out))
 
 
