@@ -265,9 +265,20 @@ int mf_problem_free(int pid);
  * stay in device memory (mf_get_solution).  A member must not be freed, and
  * should not be re-assembled, while it is in a batch. */
 int mf_batch_create(int krylbas);
-/* start a member: zero initial guess, GMRES(krylbas) with precon 0/1 to the
-   REL_RES_INI tolerance tol within max_iter iterations */
+/* start a member: zero initial guess, GMRES(krylbas) with precon 0/1/2 to
+   the REL_RES_INI tolerance tol within max_iter iterations.  precon 2 is
+   ILU(0) in the multicolour elimination order (mf_set_ilu_order(pid, 1)
+   first); its substitutions run one launch per colour level for all ILU
+   members together. */
 int mf_batch_add(int bid, int pid, int precon, int max_iter, double tol);
+/* mf_batch_add with the inner precision: 64 as mf_batch_add; 32 makes a
+   mixed-precision member (as mf_gmres_mixed): FP64 refinement steps
+   between cycles around inner GMRES solves whose matrix values, Krylov
+   basis and Jacobi inverses / ILU(0) factors are FP32 (sums, Hessenberg
+   data and the solution FP64).  The member ends when ||D^-1 (b - A x)||,
+   evaluated in FP64 (D the Jacobi diagonal; the plain norm for precon 0),
+   is at most tol times its value at x = 0. */
+int mf_batch_add_ex(int bid, int pid, int precon, int max_iter, double tol, int inner_precision);
 int mf_batch_remove(int bid, int pid);
 /* integrate + assemble every member (mf_integrate_assemble of each) */
 int mf_batch_integrate_assemble(int bid, int precision);
@@ -275,6 +286,11 @@ int mf_batch_integrate_assemble(int bid, int precision);
 int mf_batch_run(int bid, int max_cycles, int use_graph, int* active);
 int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters,
                     double* res_rel);
+/* mf_batch_status with the refinement steps of a mixed-precision member
+   (0 otherwise); for such a member iters counts its inner iterations and
+   res_rel is the FP64 measure of the last refinement step */
+int mf_batch_status_ex(int bid, int pid, int* done, int* converged, int* iters,
+                       double* res_rel, int* refinements);
 /* Batch tiles (no reference counterpart).  A tile is the number of 64-row
  * slices one block of a member streams in every batched kernel.  spb in
  * [1,64] fixes it; spb = 0 (the default) lets the library tune it: when the

@@ -112,6 +112,8 @@ def _declare(L):
         "mf_ilu_get": (I, [I, V, V, V]),
         "mf_batch_create": (I, [I]),
         "mf_batch_add": (I, [I, I, I, I, ctypes.c_double]),
+        "mf_batch_add_ex": (I, [I, I, I, I, ctypes.c_double, I]),
+        "mf_batch_status_ex": (I, [I, I, V, V, V, V, V]),
         "mf_batch_remove": (I, [I, I]),
         "mf_batch_integrate_assemble": (I, [I, I]),
         "mf_batch_run": (I, [I, I, I, V]),
@@ -251,8 +253,12 @@ class Batch:
         init()
         self.h = check(lib().mf_batch_create(krylbas))
 
-    def add(self, problem, precon=1, max_iter=1000, tol=1e-8):
-        return check(lib().mf_batch_add(self.h, problem.h, precon, max_iter, tol))
+    def add(self, problem, precon=1, max_iter=1000, tol=1e-8, inner_precision=64):
+        """start a member; inner_precision=32 makes it a mixed-precision
+        member (FP64 refinement around FP32 inner solves); precon 2 (ILU(0))
+        needs problem.set_ilu_order(1)"""
+        return check(lib().mf_batch_add_ex(self.h, problem.h, precon, max_iter, tol,
+                                           inner_precision))
 
     def remove(self, problem):
         check(lib().mf_batch_remove(self.h, problem.h))
@@ -267,10 +273,12 @@ class Batch:
         return a.value
 
     def status(self, problem):
-        d, c, it, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        check(lib().mf_batch_status(self.h, problem.h, ctypes.byref(d), ctypes.byref(c),
-                                    ctypes.byref(it), ctypes.byref(rr)))
-        return dict(done=bool(d.value), converged=bool(c.value), iters=it.value, res_rel=rr.value)
+        d, c, it, rr, nr = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double(),
+                            ctypes.c_int())
+        check(lib().mf_batch_status_ex(self.h, problem.h, ctypes.byref(d), ctypes.byref(c),
+                                       ctypes.byref(it), ctypes.byref(rr), ctypes.byref(nr)))
+        return dict(done=bool(d.value), converged=bool(c.value), iters=it.value, res_rel=rr.value,
+                    refinements=nr.value)
 
     def set_tiles(self, spb):
         """slices of 64 rows per block of every member; 0 lets the library

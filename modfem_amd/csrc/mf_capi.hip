@@ -149,6 +149,7 @@ int assemble_problem(Problem& P, int precision) {
     if (launch_faces(P, 0, nullptr, 0, P.nbf)) return -2;
   }
   P.dinv_valid = false;
+  P.val32_valid = false;
   if (P.ilu) P.ilu->valid = false;
   if (P.gs) P.gs->valid = false;
   return 0;
@@ -822,6 +823,7 @@ int mf_set_system(int pid, const double* val, const double* rhs) {
     MF_HIP(hipMemcpy(P->d_rhs, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   P->dinv_valid = false;  // refilled before the next preconditioned solve
+  P->val32_valid = false;
   if (P->ilu) P->ilu->valid = false;
   if (P->gs) P->gs->valid = false;
   return 0;
@@ -1012,6 +1014,10 @@ int mf_set_ilu_order(int pid, int order) {
     set_error("mf_set_ilu_order: 0 (reference DOF order) or 1 (multicolour)");
     return -1;
   }
+  if (P->batch_refs > 0) {
+    set_error("mf_set_ilu_order: the problem is a member of a batch");
+    return -1;
+  }
   P->ilu_order = order;
   ilu_free(*P);
   return 0;
@@ -1106,6 +1112,11 @@ int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems, doubl
   hipEventDestroy(a);
   hipEventDestroy(b);
   hipEventDestroy(c);
+  // the system was re-assembled: derived data is remade before the next use
+  P->dinv_valid = false;
+  P->val32_valid = false;
+  if (P->ilu) P->ilu->valid = false;
+  if (P->gs) P->gs->valid = false;
   if (ms_elems) *ms_elems = te / reps;
   if (ms_faces) *ms_faces = tf / reps;
   return 0;
@@ -1199,6 +1210,17 @@ int mf_batch_add(int bid, int pid, int precon, int max_iter, double tol) {
   return batch_add(*B, P, pid, precon, max_iter, tol);
 }
 
+int mf_batch_add_ex(int bid, int pid, int precon, int max_iter, double tol, int inner_precision) {
+  Batch* B = get_batch(bid);
+  Problem* P = get_problem(pid);
+  if (!B || !P) return -3;
+  if (inner_precision != 64 && inner_precision != 32) {
+    set_error("mf_batch_add_ex: inner_precision must be 64 or 32");
+    return -1;
+  }
+  return batch_add(*B, P, pid, precon, max_iter, tol, inner_precision == 32);
+}
+
 int mf_batch_remove(int bid, int pid) {
   Batch* B = get_batch(bid);
   if (!B) return -3;
@@ -1255,20 +1277,36 @@ int mf_batch_tile_cache_clear(void) {
   return 0;
 }
 
-int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters, double* res_rel) {
+int mf_batch_status_ex(int bid, int pid, int* done, int* converged, int* iters, double* res_rel,
+                       int* refinements) {
   Batch* B = get_batch(bid);
   if (!B) return -3;
   for (size_t k = 0; k < B->mem.size(); k++)
     if (B->mem[k].pid == pid) {
       const GmresStat& st = B->h_stat[k];
+      const BatchMember& M = B->mem[k];
+      if (M.mixed) {
+        // refinement state: FP64 measure rho / rho_0, inner iterations so far
+        if (done) *done = M.fin ? 1 : 0;
+        if (converged) *converged = M.conv ? 1 : 0;
+        if (iters) *iters = M.total + (M.phase == 0 && !M.fin ? st.gmiter : 0);
+        if (res_rel) *res_rel = M.rho0 > 0 ? M.rho / M.rho0 : 0.0;
+        if (refinements) *refinements = M.outer;
+        return M.nan ? -4 : 0;
+      }
       if (done) *done = st.done;
       if (converged) *converged = st.converged;
       if (iters) *iters = st.gmiter;
       if (res_rel) *res_rel = st.res_rel;
+      if (refinements) *refinements = 0;
       return st.nan ? -4 : 0;
     }
   set_error("mf_batch_status: problem is not a member");
   return -1;
+}
+
+int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters, double* res_rel) {
+  return mf_batch_status_ex(bid, pid, done, converged, iters, res_rel, nullptr);
 }
 
 int mf_batch_free(int bid) {

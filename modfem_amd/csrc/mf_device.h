@@ -102,10 +102,10 @@ struct IluData {
   std::vector<int> lev_ptr_l, lev_ptr_u;   // host: level offsets
   std::vector<long long> ref_pos;          // host: reference CRS position of each entry
   std::vector<int> pos_of_dev;             // host: elimination position of a device row
-  int *d_rp = nullptr, *d_ci = nullptr, *d_dg = nullptr, *d_ord = nullptr;
+  int *d_rp = nullptr, *d_ci = nullptr, *d_dg = nullptr;
   int *d_lev_l = nullptr, *d_lev_u = nullptr;
   long long* d_src = nullptr;              // SELL position of every entry
-  double *d_lu = nullptr, *d_t = nullptr;
+  double* d_lu = nullptr;
   bool valid = false;                      // factors match the current values
   // substitution copies of the factors, SELL-64 packed per level: forward
   // (strict L) and backward (strict U, 1/u_ii); slice s of a sweep holds 64
@@ -113,16 +113,19 @@ struct IluData {
   struct Sweep {
     std::vector<int> lev_sl;               // host: first slice of every level
     long long* d_ptr = nullptr;            // slices + 1
-    int* d_col = nullptr;                  // elimination positions
+    int* d_col = nullptr;                  // device row of the column
     int* d_src = nullptr;                  // factor entry, -1 padding
     double* d_val = nullptr;
-    int* d_row = nullptr;                  // elimination position, -1 padding
+    int* d_row = nullptr;                  // device row, -1 padding
     int* d_len = nullptr;
     double* d_diag = nullptr;              // backward: 1/u_ii per packed row
     int* d_dpos = nullptr;                 // backward: factor entry of u_ii
+    float* d_val32 = nullptr;              // FP32 copies for inner mixed-precision solves
+    float* d_diag32 = nullptr;
     long long nent = 0;
     int nrows = 0;
   } fwd, bwd;
+  bool valid32 = false;                    // FP32 copies match the factors
 };
 
 // Multicolour Gauss-Seidel (mf_gs.hip): off-diagonal rows of every colour
@@ -197,6 +200,7 @@ struct Problem {
   int batch_refs = 0;          // batches this problem is a member of
   int* d_diag_ptr = nullptr;
   float* d_dinv32 = nullptr;
+  bool val32_valid = false;    // d_val32 (and d_dinv32) are the current values rounded
   // solver tiles
   std::vector<int> tile_row;   // ntiles+1
   int* d_tile_row = nullptr;
@@ -237,6 +241,15 @@ struct BatchMember {
   int pid = -1;
   int precon = 1, max_iter = 1000;
   double tol = 1e-8;
+  // mixed-precision member: FP64 refinement steps (host-driven, between
+  // restart cycles) around inner solves with FP32 values, basis and
+  // preconditioner data; the inner solve runs on (r, d) = (d_work, d_work2)
+  bool mixed = false;
+  int phase = 0;        // 0: inner solve running; 1: first refinement step due;
+                        // 2: refinement step due after an inner solve
+  bool fin = false, conv = false, nan = false;
+  int total = 0, outer = 0;  // inner iterations and refinement steps so far
+  double rho0 = 0.0, rho = 0.0;
 };
 struct Batch {
   int m = 30;
@@ -259,13 +272,25 @@ struct Batch {
   int spb = 0, spb_used = 8;
   bool spb_tuned = false;          // spb_used came from timed trial cycles
   float tile_ms[8] = {};           // trial cycle times of the tile candidates
+  // ILU(0) members: per-member factor pointers and, per substitution level,
+  // the blocks (member, slice range) of every member that has the level
+  bool has_ilu = false;
+  void* d_ilu = nullptr;           // IluMember per member
+  int* d_ilu_mem = nullptr;
+  int* d_ilu_s0 = nullptr;
+  int* d_ilu_s1 = nullptr;
+  std::vector<int> ilu_off_f, ilu_off_b;  // level l's blocks: [off[l], off[l+1])
+  // refinement steps of mixed-precision members
+  double* d_rho2 = nullptr;
+  double* h_rho2 = nullptr;        // pinned
 };
 // tile candidates the tuner times (slices per block)
 constexpr int kBatchTileCands[] = {2, 4, 8, 16, 32};
 constexpr int kBatchTileNum = 5;
 int batch_set_tiles(Batch& B, int spb);
 void batch_tile_cache_clear();
-int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double tol);
+int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double tol,
+              bool mixed = false);
 int batch_remove(Batch& B, int pid);
 int batch_run(Batch& B, int max_cycles, int use_graph, int* active);
 void batch_free(Batch& B);
@@ -304,6 +329,10 @@ int launch_spmv(Problem& P, const double* d_x, double* d_y);
 int ilu_build(Problem& P, int order);
 int ilu_factor(Problem& P);
 int ilu_apply(Problem& P, const double* in, double* out, const int* stop);
+// FP32 copies of the packed factors (mixed-precision inner solves), and
+// the substitution on FP32 vectors with them
+int ilu_round_f32(Problem& P);
+int ilu_apply32(Problem& P, const float* in, float* out, const int* stop);
 void ilu_free(Problem& P);
 int greedy_colors(const Setup& S, std::vector<int>& color);
 // multicolour Gauss-Seidel (mf_gs.hip): structure, value refresh, and

@@ -10,7 +10,10 @@
 // depend only on rows of earlier levels, so each level is one launch with a
 // lane per row; the factorization uses the forward levels, the backward
 // substitution its own levels.  The substitutions read copies of the
-// factors packed per level in SELL-64 slices, so their loads coalesce.
+// factors packed per level in SELL-64 slices, so their loads coalesce, and
+// run in place on the vector in device-row numbering: a row's gathers then
+// touch its mesh neighbours, which the device order keeps close in memory
+// (rows of one colour are spread over the whole mesh).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,6 +21,7 @@
 #include <vector>
 
 #include "mf_device.h"
+#include "mf_solver_dev.h"
 
 namespace mf {
 
@@ -55,60 +59,17 @@ __global__ void k_ilu_factor_level(const int* rows, int count, const int* rp, co
 }
 
 // substitution over SELL-packed levels: a wavefront owns a slice of 64
-// rows of the level, lane = row; the factor entries of the slice are read
-// column-major (coalesced)
-// sum_k val_k t[col_k] over the L entries of one packed row, loads issued
-// in groups of 8 so that a group costs about two memory round trips
-__device__ __forceinline__ double packed_row_sum(const int* col, const double* val,
-                                                 long long base, int L, const double* t) {
-  constexpr int CH = 8;
-  double s = 0.0;
-  for (int k0 = 0; k0 < L; k0 += CH) {
-    int c[CH];
-    double v[CH], x[CH];
-#pragma unroll
-    for (int k = 0; k < CH; k++) {
-      const bool ok = k0 + k < L;
-      const long long q = base + 64LL * (ok ? k0 + k : k0);
-      c[k] = col[q];
-      v[k] = ok ? val[q] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < CH; k++) x[k] = t[c[k]];
-#pragma unroll
-    for (int k = 0; k < CH; k++) s += v[k] * x[k];
-  }
-  return s;
-}
-
-__global__ void __launch_bounds__(256) k_ilu_fwd_sell(int s0, int s1, const long long* ptr,
-                                                      const int* col, const double* val,
-                                                      const int* row, const int* len,
-                                                      const int* ord, const double* in,
-                                                      double* t, const int* stop) {
+// rows of the level, lane = row (ilu_fwd_row / ilu_bwd_row)
+template <bool BWD, typename FT, typename VT>
+__global__ void __launch_bounds__(256) k_ilu_level_sell(int s0, int s1, IluSweepArgs<FT> W, VT* v,
+                                                        const int* stop) {
   if (stop && *stop) return;
-  const int sl = s0 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int sl = s0 + blockIdx.x * 4 + (threadIdx.x >> 6);
   if (sl >= s1) return;
-  const int pr = sl * 64 + lane;
-  const int i = row[pr];
-  if (i < 0) return;
-  t[i] = in[ord[i]] - packed_row_sum(col, val, ptr[sl] + lane, len[pr], t);
-}
-
-__global__ void __launch_bounds__(256) k_ilu_bwd_sell(int s0, int s1, const long long* ptr,
-                                                      const int* col, const double* val,
-                                                      const int* row, const int* len,
-                                                      const double* diag, const int* ord,
-                                                      double* t, double* out, const int* stop) {
-  if (stop && *stop) return;
-  const int sl = s0 + blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (sl >= s1) return;
-  const int pr = sl * 64 + lane;
-  const int i = row[pr];
-  if (i < 0) return;
-  const double z = (t[i] - packed_row_sum(col, val, ptr[sl] + lane, len[pr], t)) * diag[pr];
-  t[i] = z;
-  out[ord[i]] = z;
+  if (BWD)
+    ilu_bwd_row(W, sl, threadIdx.x & 63, v);
+  else
+    ilu_fwd_row(W, sl, threadIdx.x & 63, v);
 }
 
 __global__ void k_ilu_pack(long long nent, const int* src, const double* lu, double* val) {
@@ -122,7 +83,8 @@ __global__ void k_ilu_pack_diag(int nrows, const int* dpos, const double* lu, do
 }
 
 static void free_sweep(IluData::Sweep& W) {
-  void* ptrs[] = {W.d_ptr, W.d_col, W.d_src, W.d_val, W.d_row, W.d_len, W.d_diag, W.d_dpos};
+  void* ptrs[] = {W.d_ptr, W.d_col, W.d_src, W.d_val, W.d_row, W.d_len, W.d_diag, W.d_dpos,
+                  W.d_val32, W.d_diag32};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   W = IluData::Sweep();
@@ -131,7 +93,7 @@ static void free_sweep(IluData::Sweep& W) {
 void ilu_free(Problem& P) {
   if (!P.ilu) return;
   IluData& L = *P.ilu;
-  void* ptrs[] = {L.d_rp, L.d_ci, L.d_dg, L.d_ord, L.d_lev_l, L.d_lev_u, L.d_src, L.d_lu, L.d_t};
+  void* ptrs[] = {L.d_rp, L.d_ci, L.d_dg, L.d_lev_l, L.d_lev_u, L.d_src, L.d_lu};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   free_sweep(L.fwd);
@@ -180,12 +142,13 @@ int greedy_colors(const Setup& S, std::vector<int>& color) {
 }
 
 // pack the rows of every level into SELL-64 slices: lower (fwd) or upper
-// (bwd) entries of each row, in ascending column order
-// Rows and columns are elimination positions; the work vector is in
-// elimination order (rows of one colour are contiguous there).
+// (bwd) entries of each row, in ascending elimination order of the columns.
+// The packed row ids and column indices are device rows (ord maps an
+// elimination position to its device row).
 static int pack_sweep(IluData::Sweep& W, int nlev, const std::vector<int>& lev_ptr,
                       const std::vector<int>& rows, const std::vector<int>& rp,
-                      const std::vector<int>& ci, const std::vector<int>& dg, bool upper) {
+                      const std::vector<int>& ci, const std::vector<int>& dg,
+                      const std::vector<int>& ord, bool upper) {
   std::vector<long long> ptr(1, 0);
   std::vector<int> prow, plen, pdiag;
   W.lev_sl.assign(nlev + 1, 0);
@@ -220,6 +183,9 @@ static int pack_sweep(IluData::Sweep& W, int nlev, const std::vector<int>& lev_p
     }
   W.nent = nent;
   W.nrows = (int)prow.size();
+  for (auto& c : col) c = ord[c];
+  for (auto& k : prow)
+    if (k >= 0) k = ord[k];
   if (up(&W.d_ptr, ptr) || up(&W.d_col, col) || up(&W.d_src, src) || up(&W.d_row, prow) ||
       up(&W.d_len, plen) || (upper && up(&W.d_dpos, pdiag)))
     return -2;
@@ -308,13 +274,12 @@ int ilu_build(Problem& P, int order) {
   L.pos_of_dev = pos;
   group_levels(levl, nl, L.lev_ptr_l, rows_l);
   group_levels(levu, nu, L.lev_ptr_u, rows_u);
-  if (up(&L.d_rp, rp) || up(&L.d_ci, ci) || up(&L.d_dg, dg) || up(&L.d_ord, ord) ||
-      up(&L.d_lev_l, rows_l) || up(&L.d_lev_u, rows_u) || up(&L.d_src, src))
+  if (up(&L.d_rp, rp) || up(&L.d_ci, ci) || up(&L.d_dg, dg) || up(&L.d_lev_l, rows_l) ||
+      up(&L.d_lev_u, rows_u) || up(&L.d_src, src))
     return -2;
   MF_HIP(hipMalloc(&L.d_lu, std::max<long long>(S.nnz, 1) * sizeof(double)));
-  MF_HIP(hipMalloc(&L.d_t, std::max(n, 1) * sizeof(double)));
-  if (pack_sweep(L.fwd, nl, L.lev_ptr_l, rows_l, rp, ci, dg, false) ||
-      pack_sweep(L.bwd, nu, L.lev_ptr_u, rows_u, rp, ci, dg, true))
+  if (pack_sweep(L.fwd, nl, L.lev_ptr_l, rows_l, rp, ci, dg, ord, false) ||
+      pack_sweep(L.bwd, nu, L.lev_ptr_u, rows_u, rp, ci, dg, ord, true))
     return -2;
   L.valid = false;
   return 0;
@@ -342,26 +307,78 @@ int ilu_factor(Problem& P) {
                      L.bwd.nrows, L.bwd.d_dpos, L.d_lu, L.bwd.d_diag);
   MF_HIP(hipGetLastError());
   L.valid = true;
+  L.valid32 = false;
+  return 0;
+}
+
+__global__ void k_ilu_to_f32(long long n, const double* a, float* b) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+int ilu_round_f32(Problem& P) {
+  if (!P.ilu || !P.ilu->valid) {
+    set_error("ILU(0): no factors to round");
+    return -1;
+  }
+  IluData& L = *P.ilu;
+  hipStream_t s = stream();
+  for (IluData::Sweep* W : {&L.fwd, &L.bwd}) {
+    if (!W->d_val32) MF_HIP(hipMalloc(&W->d_val32, std::max<long long>(W->nent, 1) * sizeof(float)));
+    if (W->nent > 0)
+      hipLaunchKernelGGL(k_ilu_to_f32, dim3((unsigned)((W->nent + 255) / 256)), dim3(256), 0, s,
+                         W->nent, W->d_val, W->d_val32);
+  }
+  if (!L.bwd.d_diag32) MF_HIP(hipMalloc(&L.bwd.d_diag32, std::max(L.bwd.nrows, 1) * sizeof(float)));
+  hipLaunchKernelGGL(k_ilu_to_f32, dim3((L.bwd.nrows + 255) / 256), dim3(256), 0, s,
+                     (long long)L.bwd.nrows, L.bwd.d_diag, L.bwd.d_diag32);
+  MF_HIP(hipGetLastError());
+  L.valid32 = true;
+  return 0;
+}
+
+IluSweepArgs<double> sweep_args(const IluData::Sweep& W) {
+  return IluSweepArgs<double>{W.d_ptr, W.d_col, W.d_val, W.d_row, W.d_len, W.d_diag};
+}
+IluSweepArgs<float> sweep_args32(const IluData::Sweep& W) {
+  return IluSweepArgs<float>{W.d_ptr, W.d_col, W.d_val32, W.d_row, W.d_len, W.d_diag32};
+}
+
+// out = (LU)^-1 in: in is copied to out (unless they are the same vector),
+// then both substitutions run in place on out
+template <typename FT, typename VT>
+static int ilu_sweeps(const Problem& P, IluSweepArgs<FT> F, IluSweepArgs<FT> U, const VT* in,
+                      VT* out, const int* stop) {
+  const IluData& L = *P.ilu;
+  hipStream_t s = stream();
+  if (in != out)
+    MF_HIP(hipMemcpyAsync(out, in, (size_t)P.n * sizeof(VT), hipMemcpyDeviceToDevice, s));
+  for (int l = 0; l < L.nlev_l; l++) {
+    const int s0 = L.fwd.lev_sl[l], s1 = L.fwd.lev_sl[l + 1];
+    hipLaunchKernelGGL((k_ilu_level_sell<false, FT, VT>), dim3((s1 - s0 + 3) / 4), dim3(256), 0, s,
+                       s0, s1, F, out, stop);
+  }
+  for (int l = 0; l < L.nlev_u; l++) {
+    const int s0 = L.bwd.lev_sl[l], s1 = L.bwd.lev_sl[l + 1];
+    hipLaunchKernelGGL((k_ilu_level_sell<true, FT, VT>), dim3((s1 - s0 + 3) / 4), dim3(256), 0, s,
+                       s0, s1, U, out, stop);
+  }
+  MF_HIP(hipGetLastError());
   return 0;
 }
 
 int ilu_apply(Problem& P, const double* in, double* out, const int* stop) {
   IluData& L = *P.ilu;
-  hipStream_t s = stream();
-  for (int l = 0; l < L.nlev_l; l++) {
-    const int s0 = L.fwd.lev_sl[l], s1 = L.fwd.lev_sl[l + 1];
-    hipLaunchKernelGGL(k_ilu_fwd_sell, dim3((s1 - s0 + 3) / 4), dim3(256), 0, s, s0, s1,
-                       L.fwd.d_ptr, L.fwd.d_col, L.fwd.d_val, L.fwd.d_row, L.fwd.d_len, L.d_ord,
-                       in, L.d_t, stop);
+  return ilu_sweeps(P, sweep_args(L.fwd), sweep_args(L.bwd), in, out, stop);
+}
+
+int ilu_apply32(Problem& P, const float* in, float* out, const int* stop) {
+  IluData& L = *P.ilu;
+  if (!L.valid32) {
+    set_error("ILU(0): FP32 factors are not current");
+    return -1;
   }
-  for (int l = 0; l < L.nlev_u; l++) {
-    const int s0 = L.bwd.lev_sl[l], s1 = L.bwd.lev_sl[l + 1];
-    hipLaunchKernelGGL(k_ilu_bwd_sell, dim3((s1 - s0 + 3) / 4), dim3(256), 0, s, s0, s1,
-                       L.bwd.d_ptr, L.bwd.d_col, L.bwd.d_val, L.bwd.d_row, L.bwd.d_len,
-                       L.bwd.d_diag, L.d_ord, L.d_t, out, stop);
-  }
-  MF_HIP(hipGetLastError());
-  return 0;
+  return ilu_sweeps(P, sweep_args32(L.fwd), sweep_args32(L.bwd), in, out, stop);
 }
 
 }  // namespace mf

@@ -110,22 +110,31 @@ static void ks_collect(int steps_done) {
 }
 
 // V_0 = D^-1 (b - A x) over slices [s0, s1); partial ||V_0||^2 to slot pb
+// (of the stored, possibly FP32-rounded, V_0)
+template <typename VT>
 __device__ __forceinline__ void residual_body(const SolveArgs& A, int s0, int s1, int pb) {
   __shared__ double red[1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  VT* V = basis<VT>(A);
   double acc = 0.0;
   for (int sl = s0 + wv; sl < s1; sl += SW) {
     const double s = sell_row_op(A, A.x, sl, lane);
     const int r = sl * 64 + lane;
     if (r < A.n) {
-      double v = A.b[r] - s;
-      if (A.dinv) v *= A.dinv[r];
-      A.V[r] = v;
-      acc += v * v;
+      const VT v = (VT)jacobi(A, r, A.b[r] - s);
+      V[r] = v;
+      acc += (double)v * (double)v;
     }
   }
   block_sum_multi<1>(&acc, 1, red);
   if (threadIdx.x == 0) A.partials[pb] = red[0];
+}
+
+__device__ __forceinline__ void residual_any(const SolveArgs& A, int s0, int s1, int pb) {
+  if (A.v32)
+    residual_body<float>(A, s0, s1, pb);
+  else
+    residual_body<double>(A, s0, s1, pb);
 }
 
 __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
@@ -133,7 +142,7 @@ __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   int s0, s1;
   slice_range(nsl, lb, s0, s1);
-  residual_body(A, s0, s1, lb);
+  residual_any(A, s0, s1, lb);
 }
 
 // Arnoldi step j, part 1 (SELL-64).  Phase 1: w = s_j D^-1 A V_j for the
@@ -141,12 +150,14 @@ __global__ void __launch_bounds__(SB) k_residual_sell(SolveArgs A) {
 // forms V_i . w over the block's rows for i = q, q+SW, ..., streaming V_i with
 // four independent accumulators and one wave-wide sum per i, so the number of
 // basis vectors costs neither registers nor occupancy.
+template <typename VT>
 __device__ __forceinline__ void op_dots_body(const SolveArgs& A, int j, const SliceSet& S, int pb,
                                              int G, double* ws) {
   const int ns = S.count();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const double* x = A.V + (size_t)j * A.ldv;
-  double* w = A.V + (size_t)(j + 1) * A.ldv;
+  VT* V = basis<VT>(A);
+  const VT* x = V + (size_t)j * A.ldv;
+  VT* w = V + (size_t)(j + 1) * A.ldv;
   const double sj = Scale(A)[j];
   for (int t = wv; t < ns; t += SW) {
     const int sl = S.slice(t);
@@ -154,80 +165,105 @@ __device__ __forceinline__ void op_dots_body(const SolveArgs& A, int j, const Sl
     const int r = sl * 64 + lane;
     double wr = 0.0;
     if (r < A.n) {
-      wr = sj * s;
-      if (A.dinv) wr *= A.dinv[r];
-      w[r] = wr;
+      const VT wq = (VT)jacobi(A, r, sj * s);
+      w[r] = wq;
+      wr = (double)wq;
     }
     ws[t * 64 + lane] = wr;
   }
   __syncthreads();
   for (int i = wv; i <= j; i += SW) {
-    const double* v = A.V + (size_t)i * A.ldv + lane;
+    const VT* v = V + (size_t)i * A.ldv + lane;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int t = 0;
     // rows past n read zero padding of V (ldv covers whole slices) times w = 0
     for (; t + 3 < ns; t += 4) {
-      a0 += v[(size_t)S.slice(t) * 64] * ws[t * 64 + lane];
-      a1 += v[(size_t)S.slice(t + 1) * 64] * ws[(t + 1) * 64 + lane];
-      a2 += v[(size_t)S.slice(t + 2) * 64] * ws[(t + 2) * 64 + lane];
-      a3 += v[(size_t)S.slice(t + 3) * 64] * ws[(t + 3) * 64 + lane];
+      a0 += (double)v[(size_t)S.slice(t) * 64] * ws[t * 64 + lane];
+      a1 += (double)v[(size_t)S.slice(t + 1) * 64] * ws[(t + 1) * 64 + lane];
+      a2 += (double)v[(size_t)S.slice(t + 2) * 64] * ws[(t + 2) * 64 + lane];
+      a3 += (double)v[(size_t)S.slice(t + 3) * 64] * ws[(t + 3) * 64 + lane];
     }
-    for (; t < ns; t++) a0 += v[(size_t)S.slice(t) * 64] * ws[t * 64 + lane];
+    for (; t < ns; t++) a0 += (double)v[(size_t)S.slice(t) * 64] * ws[t * 64 + lane];
     const double d = wave_sum((a0 + a1) + (a2 + a3));
     if (lane == 0) A.partials[(size_t)i * G + pb] = d;
   }
+}
+
+__device__ __forceinline__ void op_dots_any(const SolveArgs& A, int j, const SliceSet& S, int pb,
+                                            int G, double* ws) {
+  if (A.v32)
+    op_dots_body<float>(A, j, S, pb, G, ws);
+  else
+    op_dots_body<double>(A, j, S, pb, G, ws);
 }
 
 __global__ void __launch_bounds__(SB) k_op_dots_sell(SolveArgs A, int j) {
   if (A.stat->stop) return;
   extern __shared__ double ws[];  // w of the block's slices, 64 per slice
   const int nsl = (A.n + 63) >> 6;
-  op_dots_body(A, j, block_slices(nsl), blockIdx.x, gridDim.x, ws);
+  op_dots_any(A, j, block_slices(nsl), blockIdx.x, gridDim.x, ws);
 }
 
 // ILU-preconditioned Arnoldi step (precon 2), part 1a: w = s_j A V_j into
 // V_{j+1}; the ILU solve then runs in place on w, and k_dots_sell forms the
 // CGS dots with the preconditioned w
-__global__ void __launch_bounds__(SB) k_op_sell(SolveArgs A, int j) {
-  if (A.stat->stop) return;
-  const int nsl = (A.n + 63) >> 6;
-  const SliceSet S = block_slices(nsl);
+template <typename VT>
+__device__ __forceinline__ void op_body(const SolveArgs& A, int j, const SliceSet& S) {
   const int ns = S.count();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const double* x = A.V + (size_t)j * A.ldv;
-  double* w = A.V + (size_t)(j + 1) * A.ldv;
+  VT* V = basis<VT>(A);
+  const VT* x = V + (size_t)j * A.ldv;
+  VT* w = V + (size_t)(j + 1) * A.ldv;
   const double sj = Scale(A)[j];
   for (int t = wv; t < ns; t += SW) {
     const int sl = S.slice(t);
     const double sum = sell_row_op(A, x, sl, lane);
     const int r = sl * 64 + lane;
-    if (r < A.n) w[r] = sj * sum;
+    if (r < A.n) w[r] = (VT)(sj * sum);
+  }
+}
+
+__global__ void __launch_bounds__(SB) k_op_sell(SolveArgs A, int j) {
+  if (A.stat->stop) return;
+  const SliceSet S = block_slices((A.n + 63) >> 6);
+  if (A.v32)
+    op_body<float>(A, j, S);
+  else
+    op_body<double>(A, j, S);
+}
+
+template <typename VT>
+__device__ __forceinline__ void dots_body(const SolveArgs& A, int j, const SliceSet& S, int pb,
+                                          int G) {
+  const int ns = S.count();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const VT* V = basis<VT>(A);
+  const VT* w = V + (size_t)(j + 1) * A.ldv + lane;
+  for (int i = wv; i <= j; i += SW) {
+    const VT* v = V + (size_t)i * A.ldv + lane;
+    double a0 = 0.0, a1 = 0.0;
+    int t = 0;
+    for (; t + 1 < ns; t += 2) {
+      const size_t o0 = (size_t)S.slice(t) * 64, o1 = (size_t)S.slice(t + 1) * 64;
+      a0 += (double)v[o0] * (double)w[o0];
+      a1 += (double)v[o1] * (double)w[o1];
+    }
+    for (; t < ns; t++) {
+      const size_t o = (size_t)S.slice(t) * 64;
+      a0 += (double)v[o] * (double)w[o];
+    }
+    const double d = wave_sum(a0 + a1);
+    if (lane == 0) A.partials[(size_t)i * G + pb] = d;
   }
 }
 
 __global__ void __launch_bounds__(SB) k_dots_sell(SolveArgs A, int j) {
   if (A.stat->stop) return;
-  const int nsl = (A.n + 63) >> 6;
-  const SliceSet S = block_slices(nsl);
-  const int ns = S.count();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const double* w = A.V + (size_t)(j + 1) * A.ldv + lane;
-  for (int i = wv; i <= j; i += SW) {
-    const double* v = A.V + (size_t)i * A.ldv + lane;
-    double a0 = 0.0, a1 = 0.0;
-    int t = 0;
-    for (; t + 1 < ns; t += 2) {
-      const size_t o0 = (size_t)S.slice(t) * 64, o1 = (size_t)S.slice(t + 1) * 64;
-      a0 += v[o0] * w[o0];
-      a1 += v[o1] * w[o1];
-    }
-    for (; t < ns; t++) {
-      const size_t o = (size_t)S.slice(t) * 64;
-      a0 += v[o] * w[o];
-    }
-    const double d = wave_sum(a0 + a1);
-    if (lane == 0) A.partials[(size_t)i * gridDim.x + blockIdx.x] = d;
-  }
+  const SliceSet S = block_slices((A.n + 63) >> 6);
+  if (A.v32)
+    dots_body<float>(A, j, S, blockIdx.x, gridDim.x);
+  else
+    dots_body<double>(A, j, S, blockIdx.x, gridDim.x);
 }
 
 // GS-smoothed operator (precon 3, lss_mkb_core.c:2001-2054): with
@@ -253,18 +289,27 @@ __global__ void k_gs_restart_finish(SolveArgs A) {
   if (i < A.n) A.V[i] += -1.0 * A.x[i];
 }
 
-// per-block partial ||V_0||^2 over contiguous slice ranges (ILU restart)
-__global__ void __launch_bounds__(SB) k_sumsq_v0(SolveArgs A) {
+// partial ||V_0||^2 over slices [s0, s1) to slot pb (ILU restart)
+template <typename VT>
+__device__ __forceinline__ void sumsq_body(const SolveArgs& A, int s0, int s1, int pb) {
   __shared__ double red[1];
+  const VT* V = basis<VT>(A);
+  double acc = 0.0;
+  for (long long r = (long long)s0 * 64 + threadIdx.x; r < (long long)s1 * 64 && r < A.n; r += SB)
+    acc += (double)V[r] * (double)V[r];
+  block_sum_multi<1>(&acc, 1, red);
+  if (threadIdx.x == 0) A.partials[pb] = red[0];
+}
+
+__global__ void __launch_bounds__(SB) k_sumsq_v0(SolveArgs A) {
   const int nsl = (A.n + 63) >> 6;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   int s0, s1;
   slice_range(nsl, lb, s0, s1);
-  double acc = 0.0;
-  for (long long r = (long long)s0 * 64 + threadIdx.x; r < (long long)s1 * 64 && r < A.n; r += SB)
-    acc += A.V[r] * A.V[r];
-  block_sum_multi<1>(&acc, 1, red);
-  if (threadIdx.x == 0) A.partials[lb] = red[0];
+  if (A.v32)
+    sumsq_body<float>(A, s0, s1, lb);
+  else
+    sumsq_body<double>(A, s0, s1, lb);
 }
 
 __global__ void __launch_bounds__(SB) k_spmv_sell(SolveArgs A, const double* x, double* y) {
@@ -422,47 +467,65 @@ __global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
 // Arnoldi step j, part 2: w -= sum_i c_i V_i over row pairs (16-byte loads),
 // per-block partial ||w||^2.  The vectors are read in groups of eight
 // independent loads; the subtraction order is i = 0, 1, ..., j.
+template <typename VT> struct Pair;
+template <> struct Pair<double> { using T = double2; };
+template <> struct Pair<float> { using T = float2; };
+
+template <typename VT>
 __device__ __forceinline__ void update_body(const SolveArgs& A, int j, long long p0, long long p1,
                                             int pb) {
+  using P2 = typename Pair<VT>::T;
   __shared__ double red[1];
   __shared__ double cs[MAX_NV];
   if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
   __syncthreads();
   const long long ldp = A.ldv / 2;
-  const double2* V2 = reinterpret_cast<const double2*>(A.V);
-  double2* w = reinterpret_cast<double2*>(A.V + (size_t)(j + 1) * A.ldv);
+  const P2* V2 = reinterpret_cast<const P2*>(basis<VT>(A));
+  P2* w = reinterpret_cast<P2*>(basis<VT>(A) + (size_t)(j + 1) * A.ldv);
   double acc = 0.0;
   for (long long p = p0 + threadIdx.x; p < p1; p += SB) {
-    double2 wr = w[p];
+    const P2 w0 = w[p];
+    double wx = (double)w0.x, wy = (double)w0.y;
     int i = 0;
     for (; i + 7 <= j; i += 8) {
-      double2 v[8];
+      P2 v[8];
 #pragma unroll
       for (int q = 0; q < 8; q++) v[q] = V2[(size_t)(i + q) * ldp + p];
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        wr.x -= cs[i + q] * v[q].x;
-        wr.y -= cs[i + q] * v[q].y;
+        wx -= cs[i + q] * (double)v[q].x;
+        wy -= cs[i + q] * (double)v[q].y;
       }
     }
     for (; i <= j; i++) {
-      const double2 v = V2[(size_t)i * ldp + p];
-      wr.x -= cs[i] * v.x;
-      wr.y -= cs[i] * v.y;
+      const P2 v = V2[(size_t)i * ldp + p];
+      wx -= cs[i] * (double)v.x;
+      wy -= cs[i] * (double)v.y;
     }
-    if (2 * p + 1 >= A.n) wr.y = 0.0;  // padding lane of an odd n
+    if (2 * p + 1 >= A.n) wy = 0.0;  // padding lane of an odd n
+    P2 wr;
+    wr.x = (VT)wx;
+    wr.y = (VT)wy;
     w[p] = wr;
-    acc += wr.x * wr.x + wr.y * wr.y;
+    acc += (double)wr.x * (double)wr.x + (double)wr.y * (double)wr.y;
   }
   block_sum_multi<1>(&acc, 1, red);
   if (threadIdx.x == 0) A.partials[pb] = red[0];
+}
+
+__device__ __forceinline__ void update_any(const SolveArgs& A, int j, long long p0, long long p1,
+                                           int pb) {
+  if (A.v32)
+    update_body<float>(A, j, p0, p1, pb);
+  else
+    update_body<double>(A, j, p0, p1, pb);
 }
 
 __global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
   if (A.stat->stop) return;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const long long np = (A.n + 1) / 2;  // row pairs (ldv is even)
-  update_body(A, j, lb * np / gridDim.x, (lb + 1) * np / gridDim.x, lb);
+  update_any(A, j, lb * np / gridDim.x, (lb + 1) * np / gridDim.x, lb);
 }
 
 // h_{j+1,j} = ||w||, Givens rotations (drot/drotg as lss_mkb_core.c:2539-2560),
@@ -515,17 +578,27 @@ __global__ void __launch_bounds__(SB) k_fin_update(SolveArgs A, int j, int G) {
   fin_update_body(A, j, G);
 }
 
-// x += sum_c (y_c s_c) V_c
+// x += sum_c (y_c s_c) V_c over rows [r0, r1), strided by `step`
+template <typename VT>
+__device__ __forceinline__ void xupdate_body(const SolveArgs& A, int ik, const double* cs,
+                                             long long r0, long long r1, long long step) {
+  const VT* V = basis<VT>(A);
+  for (long long r = r0; r < r1; r += step) {
+    double xr = A.x[r];
+    for (int c = 0; c < ik; c++) xr += (double)V[(size_t)c * A.ldv + r] * cs[c];
+    A.x[r] = xr;
+  }
+}
+
 __global__ void __launch_bounds__(SB) k_xupdate(SolveArgs A, int ik) {
   __shared__ double cs[MAX_NV];
   if ((int)threadIdx.x < ik) cs[threadIdx.x] = Coef(A)[threadIdx.x];
   __syncthreads();
-  const size_t n = A.n;
-  for (size_t r = (size_t)blockIdx.x * SB + threadIdx.x; r < n; r += (size_t)gridDim.x * SB) {
-    double xr = A.x[r];
-    for (int c = 0; c < ik; c++) xr += A.V[(size_t)c * A.ldv + r] * cs[c];
-    A.x[r] = xr;
-  }
+  const long long r0 = (long long)blockIdx.x * SB + threadIdx.x, step = (long long)gridDim.x * SB;
+  if (A.v32)
+    xupdate_body<float>(A, ik, cs, r0, A.n, step);
+  else
+    xupdate_body<double>(A, ik, cs, r0, A.n, step);
 }
 
 __global__ void k_fill_jacobi(int n, const int* rp, const int* col, const double* val,
@@ -612,6 +685,7 @@ static void fill_solve_args(Problem& P, SolveArgs& A, int precon, int m) {
   A.dinv = precon ? P.d_dinv : nullptr;
   A.b = P.d_rhs; A.x = P.d_x; A.V = P.d_V; A.H = P.d_H; A.partials = P.d_partials;
   A.stat = P.d_stat; A.hstat = P.h_stat;
+  A.Vf = reinterpret_cast<float*>(P.d_V); A.dinv32 = nullptr; A.v32 = 0; A.ilu = 0;
 }
 
 // grid sizes: one resident round over the CUs (LDS limits a tile kernel to 5
@@ -759,6 +833,30 @@ static int ensure_workspace(Problem& P, int m) {
   return 0;
 }
 
+// The operator+dots kernels read the rows [n, ldv) of every basis vector
+// (times w = 0) and need them zero.  The FP32 basis shares the storage of
+// the FP64 one, so those rows of one view are data of the other: a solve
+// clears them in the view it uses before it starts.
+__global__ void k_zero_padding(long long ldv, int n, int nvec, int v32, double* V) {
+  const long long pad = ldv - n;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= pad * nvec) return;
+  const long long at = (q / pad) * ldv + n + q % pad;
+  if (v32)
+    reinterpret_cast<float*>(V)[at] = 0.0f;
+  else
+    V[at] = 0.0;
+}
+
+static int zero_padding(const Problem& P, int m, int v32) {
+  const long long ldv = ldv_of(P.n), tot = (ldv - P.n) * (m + 1);
+  if (tot == 0) return 0;
+  hipLaunchKernelGGL(k_zero_padding, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream(),
+                     ldv, P.n, m + 1, v32, P.d_V);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
 template <int NV>
 static void launch_step(int G_op, int G_up, SolveArgs& A, int j) {
   hipStream_t s = stream();
@@ -862,6 +960,26 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
     set_error("GMRES: the FP32 operator needs SELL storage and its FP32 values");
     return -1;
   }
+  if (op32) {
+    // inner solve of the mixed-precision refinement: FP32 basis and FP32
+    // preconditioner data
+    if (gs) {
+      set_error("GMRES: no FP32 inner solve with Gauss-Seidel");
+      return -1;
+    }
+    if (ilu && !P.ilu->valid32) {
+      set_error("GMRES: the FP32 ILU(0) factors are not current");
+      return -1;
+    }
+    if (!ilu && precon && !P.d_dinv32) {
+      set_error("GMRES: the FP32 Jacobi inverses are missing");
+      return -1;
+    }
+    A.v32 = 1;
+    A.dinv32 = (!ilu && precon) ? P.d_dinv32 : nullptr;
+    A.dinv = nullptr;
+  }
+  if (P.sell && zero_padding(P, m, A.v32)) return -2;
   const int G_op = (P.sell || ext) ? grid_slices(P) : grid_tiles(P), G_up = grid_rows(P);
   const double* d_scale = P.d_H + (size_t)(m + 1) * (m + 1) + 2 * (m + 1);
   if (!P.sell) A.sl_ptr = nullptr;
@@ -922,7 +1040,7 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
       hipLaunchKernelGGL(k_residual, dim3(G_op), dim3(SB), 0, s, A);
     if (ilu) {
       // V_0 = (LU)^-1 (b - A x), lss_mkb_core.c:1953-1977
-      if (ilu_apply(P, A.V, A.V, nullptr)) return -2;
+      if (A.v32 ? ilu_apply32(P, A.Vf, A.Vf, nullptr) : ilu_apply(P, A.V, A.V, nullptr)) return -2;
       hipLaunchKernelGGL(k_sumsq_v0, dim3(G_op), dim3(SB), 0, s, A);
     }
     hipLaunchKernelGGL(k_fin_residual, dim3(1), dim3(SB), 0, s, A, G_op);
@@ -973,7 +1091,8 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
           hipLaunchKernelGGL(k_gs_op_finish, dim3(gn), dim3(256), 0, s, A, j);
         } else {
           hipLaunchKernelGGL(k_op_sell, dim3(G_op), dim3(SB), 0, s, A, j);
-          if (ilu_apply(P, w, w, d_stop)) return -2;
+          float* wf = A.Vf + (size_t)(j + 1) * A.ldv;
+          if (A.v32 ? ilu_apply32(P, wf, wf, d_stop) : ilu_apply(P, w, w, d_stop)) return -2;
         }
         hipLaunchKernelGGL(k_dots_sell, dim3(G_op), dim3(SB), 0, s, A, j);
         hipLaunchKernelGGL(k_fin_dots, dim3(j + 1), dim3(SB), 0, s, A, j, G_op);
@@ -1045,6 +1164,36 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, d
                     iters, res_rel_out, resid_out);
 }
 
+// FP32 copies for mixed-precision inner solves: matrix values, and the
+// Jacobi inverses (precon 1) or the ILU(0) factors (precon 2), refreshed
+// when the matrix changed since they were made
+static int round_mixed(Problem& P, int precon) {
+  hipStream_t s = stream();
+  if (precon == 1 && (!P.d_dinv || !P.dinv_valid)) {
+    if (launch_fill_jacobi(P)) return -2;
+    P.val32_valid = false;
+  }
+  if (!P.val32_valid) {
+    if (!P.d_val32) MF_HIP(hipMalloc(&P.d_val32, (size_t)P.nnz_sell * sizeof(float)));
+    hipLaunchKernelGGL(k_round_f32, dim3((unsigned)((P.nnz_sell + 255) / 256)), dim3(256), 0, s,
+                       P.nnz_sell, P.d_val, P.d_val32);
+    if (P.d_dinv && P.dinv_valid) {
+      if (!P.d_dinv32) MF_HIP(hipMalloc(&P.d_dinv32, (size_t)P.n * sizeof(float)));
+      hipLaunchKernelGGL(k_round_f32, dim3((P.n + 255) / 256), dim3(256), 0, s, (long long)P.n,
+                         P.d_dinv, P.d_dinv32);
+    }
+    MF_HIP(hipGetLastError());
+    P.val32_valid = true;
+  }
+  if (precon == 1 && !P.d_dinv32) {
+    MF_HIP(hipMalloc(&P.d_dinv32, (size_t)P.n * sizeof(float)));
+    hipLaunchKernelGGL(k_round_f32, dim3((P.n + 255) / 256), dim3(256), 0, s, (long long)P.n,
+                       P.d_dinv, P.d_dinv32);
+  }
+  if (precon == 2 && !P.ilu->valid32 && ilu_round_f32(P)) return -2;
+  return 0;
+}
+
 // Mixed-precision solve: FP64 iterative refinement around GMRES(m) whose
 // operator reads FP32 matrix values (Gram-Schmidt, Givens rotations, Krylov
 // basis, corrections and residuals FP64).  Step k: r_k = b - A x_k and
@@ -1072,9 +1221,7 @@ int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, d
   }
   hipStream_t s = stream();
   const long long n = P.n;
-  if (!P.d_val32) MF_HIP(hipMalloc(&P.d_val32, (size_t)P.nnz_sell * sizeof(float)));
-  hipLaunchKernelGGL(k_round_f32, dim3((unsigned)((P.nnz_sell + 255) / 256)), dim3(256), 0, s,
-                     P.nnz_sell, P.d_val, P.d_val32);
+  if (round_mixed(P, precon)) return -2;
   if (!P.d_work) MF_HIP(hipMalloc(&P.d_work, (size_t)n * sizeof(double)));
   if (!P.d_work2) MF_HIP(hipMalloc(&P.d_work2, (size_t)(n + 1) * sizeof(double)));
   double *d_r = P.d_work, *d_d = P.d_work2, *d_sum = P.d_work2 + n;
@@ -1291,6 +1438,18 @@ int gmres_noalloc(Problem& P, int precon, int krylbas, int max_iter, double tol,
 // restart logic of gmres_core runs on the device (kb_restart, kb_advance),
 // so a cycle needs no host decision and can be replayed as one hipGraph.
 // Members that have stopped leave every later launch at once.
+//
+// Members preconditioned by multicolour ILU(0) add, after the residual and
+// after each operator application, one launch per substitution level that
+// covers the rows of that level in every such member (the number of levels
+// is the number of colours, not the number of members), followed by the
+// norm or the Gram-Schmidt dots of the preconditioned vector.
+//
+// Mixed-precision members run FP32-valued inner solves (FP32 values, basis,
+// Jacobi inverses or ILU factors; FP64 sums and Hessenberg data) on the
+// correction equation A d = r; between cycles the host performs their FP64
+// refinement steps x += d, r = b - A x, rho = ||D^-1 r|| and restarts the
+// inner solve to max(tol rho_0 / rho, 1e-5) until rho <= tol rho_0.
 // ---------------------------------------------------------------------
 struct BatchArgs {
   const SolveArgs* args;
@@ -1304,7 +1463,7 @@ __global__ void __launch_bounds__(SB) kb_residual(BatchArgs B) {
   const SolveArgs A = B.args[p];
   if (A.stat->done) return;
   const int nsl = (A.n + 63) >> 6;
-  residual_body(A, (int)((long long)r * nsl / G), (int)((long long)(r + 1) * nsl / G), r);
+  residual_any(A, (int)((long long)r * nsl / G), (int)((long long)(r + 1) * nsl / G), r);
 }
 
 // restart test of lsr_mkb_core_gmres (lss_mkb_core.c:2404-2470) per member
@@ -1365,17 +1524,78 @@ __global__ void __launch_bounds__(SB) kb_restart(BatchArgs B) {
   }
 }
 
+__device__ __forceinline__ SliceSet member_slices(int n, int r, int G) {
+  const int nsl = (n + 63) >> 6;
+  SliceSet S;
+  S.base = (int)((long long)r * nsl / G);
+  S.end = (int)((long long)(r + 1) * nsl / G);
+  S.step = SW;  // contiguous
+  return S;
+}
+
+// ILU members: partial ||(LU)^-1 r||^2 of the preconditioned V_0
+__global__ void __launch_bounds__(SB) kb_sumsq(BatchArgs B) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs A = B.args[p];
+  if (!A.ilu || A.stat->done) return;
+  const SliceSet S = member_slices(A.n, r, G);
+  if (A.v32)
+    sumsq_body<float>(A, S.base, S.end, r);
+  else
+    sumsq_body<double>(A, S.base, S.end, r);
+}
+
+// one substitution level of every ILU member: block b covers up to SW
+// slices [s0[b], s1[b]) of member mem[b]; j < 0 preconditions V_0 (restart),
+// j >= 0 the new vector V_{j+1} of Arnoldi step j
+template <bool BWD>
+__global__ void __launch_bounds__(SB) kb_ilu_level(BatchArgs B, const IluMember* L, const int* mem,
+                                                   const int* s0, const int* s1, int off, int j) {
+  const int b = off + blockIdx.x;
+  const int k = mem[b];
+  const SolveArgs& A = B.args[k];
+  if (j < 0 ? A.stat->done : A.stat->stop) return;
+  const int sl = s0[b] + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (sl >= s1[b]) return;
+  const IluMember& M = L[k];
+  const size_t v = (size_t)(j < 0 ? 0 : j + 1) * A.ldv;
+  if (A.v32) {
+    if (BWD)
+      ilu_bwd_row(M.b32, sl, lane, A.Vf + v);
+    else
+      ilu_fwd_row(M.f32, sl, lane, A.Vf + v);
+  } else {
+    if (BWD)
+      ilu_bwd_row(M.b, sl, lane, A.V + v);
+    else
+      ilu_fwd_row(M.f, sl, lane, A.V + v);
+  }
+}
+
 __global__ void __launch_bounds__(SB) kb_op_dots(BatchArgs B, int j) {
   const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
   const SolveArgs A = B.args[p];
   if (A.stat->stop) return;
   extern __shared__ double ws[];
-  const int nsl = (A.n + 63) >> 6;
-  SliceSet S;
-  S.base = (int)((long long)r * nsl / G);
-  S.end = (int)((long long)(r + 1) * nsl / G);
-  S.step = SW;  // contiguous
-  op_dots_body(A, j, S, r, G, ws);
+  const SliceSet S = member_slices(A.n, r, G);
+  if (!A.ilu)
+    op_dots_any(A, j, S, r, G, ws);
+  else if (A.v32)
+    op_body<float>(A, j, S);  // dots after the substitution (kb_dots)
+  else
+    op_body<double>(A, j, S);
+}
+
+// ILU members: Gram-Schmidt dots with the preconditioned w
+__global__ void __launch_bounds__(SB) kb_dots(BatchArgs B, int j) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs A = B.args[p];
+  if (!A.ilu || A.stat->stop) return;
+  const SliceSet S = member_slices(A.n, r, G);
+  if (A.v32)
+    dots_body<float>(A, j, S, r, G);
+  else
+    dots_body<double>(A, j, S, r, G);
 }
 
 __global__ void __launch_bounds__(SB) kb_fin_dots(BatchArgs B, int j) {
@@ -1389,7 +1609,7 @@ __global__ void __launch_bounds__(SB) kb_update(BatchArgs B, int j) {
   const SolveArgs A = B.args[p];
   if (A.stat->stop) return;
   const long long np = (A.n + 1) / 2;
-  update_body(A, j, r * np / G, (r + 1) * np / G, r);
+  update_any(A, j, r * np / G, (r + 1) * np / G, r);
 }
 
 __global__ void __launch_bounds__(SB) kb_fin_update(BatchArgs B, int j) {
@@ -1424,11 +1644,10 @@ __global__ void __launch_bounds__(SB) kb_xupdate(BatchArgs B) {
   if ((int)threadIdx.x < ik) cs[threadIdx.x] = Coef(A)[threadIdx.x];
   __syncthreads();
   const long long r0 = (long long)r * A.n / G, r1 = (long long)(r + 1) * A.n / G;
-  for (long long i = r0 + threadIdx.x; i < r1; i += SB) {
-    double xr = A.x[i];
-    for (int c = 0; c < ik; c++) xr += A.V[(size_t)c * A.ldv + i] * cs[c];
-    A.x[i] = xr;
-  }
+  if (A.v32)
+    xupdate_body<float>(A, ik, cs, r0 + threadIdx.x, r1, SB);
+  else
+    xupdate_body<double>(A, ik, cs, r0 + threadIdx.x, r1, SB);
 }
 
 // end of a restart (lss_mkb_core.c:2606-2668): converged, out of
@@ -1457,14 +1676,26 @@ static void batch_drop_graph(Batch& B) {
   B.graph = nullptr;
 }
 
+static void batch_drop_ilu_tables(Batch& B) {
+  void* ptrs[] = {B.d_ilu_mem, B.d_ilu_s0, B.d_ilu_s1};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  B.d_ilu_mem = B.d_ilu_s0 = B.d_ilu_s1 = nullptr;
+  B.ilu_off_f.clear();
+  B.ilu_off_b.clear();
+}
+
 static void batch_drop_tables(Batch& B) {
   batch_drop_graph(B);
-  void* ptrs[] = {B.d_args, B.d_blk_prob, B.d_blk_rank, B.d_nblk, B.d_stat};
+  batch_drop_ilu_tables(B);
+  void* ptrs[] = {B.d_args, B.d_blk_prob, B.d_blk_rank, B.d_nblk, B.d_stat, B.d_ilu, B.d_rho2};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (B.h_pinned) (void)hipHostFree(B.h_pinned);
+  if (B.h_rho2) (void)hipHostFree(B.h_rho2);
   B.d_args = nullptr; B.d_blk_prob = B.d_blk_rank = B.d_nblk = nullptr;
   B.d_stat = nullptr; B.h_pinned = nullptr;
+  B.d_ilu = nullptr; B.d_rho2 = nullptr; B.h_rho2 = nullptr;
   B.cap = 0;
 }
 
@@ -1479,13 +1710,36 @@ static GmresStat member_init(const BatchMember& M, int m) {
   return st;
 }
 
-int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double tol) {
+// the data a member's solve reads, made current: Jacobi inverses (FP64;
+// also the refinement measure of mixed members), ILU(0) factors, FP32 copies
+static int prepare_member(const BatchMember& M) {
+  Problem& P = *M.P;
+  if ((M.precon == 1 || M.mixed) && (!P.d_dinv || !P.dinv_valid)) {
+    if (launch_fill_jacobi(P)) return -2;
+    P.val32_valid = false;  // the FP32 inverses follow the FP64 ones
+  }
+  if (M.precon == 2 && !P.ilu->valid) {
+    if (ilu_factor(P)) return -2;
+  }
+  if (M.mixed && round_mixed(P, M.precon)) return -2;
+  return 0;
+}
+
+int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double tol, bool mixed) {
   if (!P->sell) {
     set_error("batch: member needs SELL storage");
     return -1;
   }
-  if (max_iter < 1 || tol < 0 || (precon != 0 && precon != 1)) {
-    set_error("batch: precon 0/1, max_iter >= 1 and tol >= 0 required");
+  if (max_iter < 1 || tol < 0 || precon < 0 || precon > 2) {
+    set_error("batch: precon 0/1/2, max_iter >= 1 and tol >= 0 required");
+    return -1;
+  }
+  if (mixed && tol <= 0) {
+    set_error("batch: a mixed-precision member needs tol > 0");
+    return -1;
+  }
+  if (precon == 2 && P->ilu_order != 1) {
+    set_error("batch: ILU(0) members need the multicolour elimination order (mf_set_ilu_order 1)");
     return -1;
   }
   for (auto& M : B.mem)
@@ -1494,14 +1748,26 @@ int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double to
       return -1;
     }
   if (ensure_workspace(*P, B.m)) return -2;
-  if (precon && (!P->d_dinv || !P->dinv_valid) && launch_fill_jacobi(*P)) return -2;
-  if (!P->d_x) MF_HIP(hipMalloc(&P->d_x, (size_t)P->n * sizeof(double)));
-  MF_HIP(hipMemsetAsync(P->d_x, 0, (size_t)P->n * sizeof(double), stream()));
+  if (zero_padding(*P, B.m, mixed ? 1 : 0)) return -2;
+  if (precon == 2 && !P->ilu && ilu_build(*P, 1)) return -2;
   BatchMember M;
   M.P = P; M.pid = pid; M.precon = precon; M.max_iter = max_iter; M.tol = tol;
+  M.mixed = mixed;
+  if (prepare_member(M)) return -2;
+  hipStream_t s = stream();
+  if (!P->d_x) MF_HIP(hipMalloc(&P->d_x, (size_t)P->n * sizeof(double)));
+  MF_HIP(hipMemsetAsync(P->d_x, 0, (size_t)P->n * sizeof(double), s));
+  GmresStat st = member_init(M, B.m);
+  if (mixed) {
+    if (!P->d_work) MF_HIP(hipMalloc(&P->d_work, (size_t)P->n * sizeof(double)));
+    if (!P->d_work2) MF_HIP(hipMalloc(&P->d_work2, (size_t)(P->n + 1) * sizeof(double)));
+    MF_HIP(hipMemsetAsync(P->d_work2, 0, (size_t)P->n * sizeof(double), s));
+    M.phase = 1;  // the first refinement step sets rho_0 and starts the inner solve
+    st.done = st.stop = 1;
+  }
   P->batch_refs++;
   B.mem.push_back(M);
-  B.h_stat.push_back(member_init(M, B.m));
+  B.h_stat.push_back(st);
   B.dirty = true;
   return (int)B.mem.size() - 1;
 }
@@ -1519,29 +1785,103 @@ int batch_remove(Batch& B, int pid) {
   return -1;
 }
 
+static int batch_alloc(Batch& B, int np) {
+  if (np <= B.cap) return 0;
+  batch_drop_tables(B);
+  const int cap = std::max(np, 16);
+  MF_HIP(hipMalloc(&B.d_args, (size_t)cap * sizeof(SolveArgs)));
+  MF_HIP(hipMalloc(&B.d_nblk, (size_t)cap * sizeof(int)));
+  MF_HIP(hipMalloc(&B.d_stat, (size_t)cap * sizeof(GmresStat)));
+  MF_HIP(hipMalloc(&B.d_ilu, (size_t)cap * sizeof(IluMember)));
+  MF_HIP(hipMalloc(&B.d_rho2, (size_t)cap * sizeof(double)));
+  MF_HIP(hipHostMalloc(&B.h_pinned, (size_t)cap * sizeof(GmresStat), hipHostMallocDefault));
+  MF_HIP(hipHostMalloc(&B.h_rho2, (size_t)cap * sizeof(double), hipHostMallocDefault));
+  B.cap = cap;
+  B.dirty = true;
+  return 0;
+}
+
+// substitution level tables of the ILU members: level l of the forward
+// (backward) sweep of every member that has it, SW slices per block
+static int batch_build_ilu(Batch& B) {
+  batch_drop_ilu_tables(B);
+  const int np = (int)B.mem.size();
+  std::vector<int> bm, b0, b1;
+  for (int sweep = 0; sweep < 2; sweep++) {
+    std::vector<int>& off = sweep ? B.ilu_off_b : B.ilu_off_f;
+    int nlev = 0;
+    for (const auto& M : B.mem)
+      if (M.precon == 2) nlev = std::max(nlev, sweep ? M.P->ilu->nlev_u : M.P->ilu->nlev_l);
+    off.assign(1, (int)bm.size());
+    for (int l = 0; l < nlev; l++) {
+      for (int k = 0; k < np; k++) {
+        const BatchMember& M = B.mem[k];
+        if (M.precon != 2) continue;
+        const IluData& L = *M.P->ilu;
+        if (l >= (sweep ? L.nlev_u : L.nlev_l)) continue;
+        const std::vector<int>& ls = sweep ? L.bwd.lev_sl : L.fwd.lev_sl;
+        for (int q = ls[l]; q < ls[l + 1]; q += SW) {
+          bm.push_back(k);
+          b0.push_back(q);
+          b1.push_back(std::min(q + SW, ls[l + 1]));
+        }
+      }
+      off.push_back((int)bm.size());
+    }
+  }
+  if (bm.empty()) return 0;
+  hipStream_t s = stream();
+  MF_HIP(hipMalloc(&B.d_ilu_mem, bm.size() * sizeof(int)));
+  MF_HIP(hipMalloc(&B.d_ilu_s0, b0.size() * sizeof(int)));
+  MF_HIP(hipMalloc(&B.d_ilu_s1, b1.size() * sizeof(int)));
+  MF_HIP(hipMemcpyAsync(B.d_ilu_mem, bm.data(), bm.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  MF_HIP(hipMemcpyAsync(B.d_ilu_s0, b0.data(), b0.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  MF_HIP(hipMemcpyAsync(B.d_ilu_s1, b1.data(), b1.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  MF_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// the vectors a member's inner iteration reads and updates
+static void member_args(const BatchMember& M, SolveArgs& A, int m) {
+  Problem& P = *M.P;
+  fill_solve_args(P, A, (M.precon == 1 && !M.mixed) ? 1 : 0, m);
+  A.ilu = M.precon == 2;
+  if (M.mixed) {
+    A.op32 = 1;
+    A.v32 = 1;
+    A.dinv32 = M.precon == 1 ? P.d_dinv32 : nullptr;
+    A.b = P.d_work;   // r of the current refinement step
+    A.x = P.d_work2;  // its correction d
+  } else {
+    A.b = P.d_rhs;
+    A.x = P.d_x;
+  }
+}
+
 static int batch_build(Batch& B) {
   const int np = (int)B.mem.size();
   batch_drop_graph(B);
-  if (np > B.cap) {
-    batch_drop_tables(B);
-    const int cap = std::max(np, 16);
-    MF_HIP(hipMalloc(&B.d_args, (size_t)cap * sizeof(SolveArgs)));
-    MF_HIP(hipMalloc(&B.d_nblk, (size_t)cap * sizeof(int)));
-    MF_HIP(hipMalloc(&B.d_stat, (size_t)cap * sizeof(GmresStat)));
-    MF_HIP(hipHostMalloc(&B.h_pinned, (size_t)cap * sizeof(GmresStat), hipHostMallocDefault));
-    B.cap = cap;
-  }
+  if (batch_alloc(B, np)) return -2;
   std::vector<SolveArgs> args(np);
+  std::vector<IluMember> ilus(np);
   std::vector<int> nblk(np), bp, br;
   int max_sl = 1;
+  B.has_ilu = false;
   for (int k = 0; k < np; k++) {
     Problem& P = *B.mem[k].P;
     SolveArgs& A = args[k];
-    fill_solve_args(P, A, B.mem[k].precon, B.m);
-    A.b = P.d_rhs;
-    A.x = P.d_x;
+    member_args(B.mem[k], A, B.m);
     A.stat = B.d_stat + k;
     A.hstat = nullptr;
+    IluMember& L = ilus[k];
+    L = IluMember{};
+    if (B.mem[k].precon == 2) {
+      B.has_ilu = true;
+      L.f = sweep_args(P.ilu->fwd);
+      L.b = sweep_args(P.ilu->bwd);
+      L.f32 = sweep_args32(P.ilu->fwd);
+      L.b32 = sweep_args32(P.ilu->bwd);
+    }
     const int nsl = (P.n + 63) / 64;
     const int G = std::max(1, std::min(MAX_BLOCKS, (nsl + B.spb_used - 1) / B.spb_used));
     nblk[k] = G;
@@ -1569,10 +1909,28 @@ static int batch_build(Batch& B) {
     MF_HIP(hipMemcpyAsync(B.d_args, args.data(), np * sizeof(SolveArgs), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(B.d_nblk, nblk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(B.d_stat, B.h_stat.data(), np * sizeof(GmresStat), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(B.d_ilu, ilus.data(), np * sizeof(IluMember), hipMemcpyHostToDevice, s));
     MF_HIP(hipStreamSynchronize(s));
   }
+  if (batch_build_ilu(B)) return -2;
   B.dirty = false;
   return 0;
+}
+
+static void batch_ilu_levels(Batch& B, const BatchArgs& a, int j, hipStream_t s) {
+  const IluMember* L = reinterpret_cast<const IluMember*>(B.d_ilu);
+  for (size_t l = 0; l + 1 < B.ilu_off_f.size(); l++) {
+    const int nb = B.ilu_off_f[l + 1] - B.ilu_off_f[l];
+    if (nb > 0)
+      hipLaunchKernelGGL(kb_ilu_level<false>, dim3(nb), dim3(SB), 0, s, a, L, B.d_ilu_mem,
+                         B.d_ilu_s0, B.d_ilu_s1, B.ilu_off_f[l], j);
+  }
+  for (size_t l = 0; l + 1 < B.ilu_off_b.size(); l++) {
+    const int nb = B.ilu_off_b[l + 1] - B.ilu_off_b[l];
+    if (nb > 0)
+      hipLaunchKernelGGL(kb_ilu_level<true>, dim3(nb), dim3(SB), 0, s, a, L, B.d_ilu_mem,
+                         B.d_ilu_s0, B.d_ilu_s1, B.ilu_off_b[l], j);
+  }
 }
 
 static void batch_enqueue_cycle(Batch& B, hipStream_t s) {
@@ -1583,9 +1941,17 @@ static void batch_enqueue_cycle(Batch& B, hipStream_t s) {
   a.nblk = B.d_nblk;
   const int np = (int)B.mem.size();
   hipLaunchKernelGGL(kb_residual, dim3(B.nblk), dim3(SB), 0, s, a);
+  if (B.has_ilu) {
+    batch_ilu_levels(B, a, -1, s);
+    hipLaunchKernelGGL(kb_sumsq, dim3(B.nblk), dim3(SB), 0, s, a);
+  }
   hipLaunchKernelGGL(kb_restart, dim3(np), dim3(SB), 0, s, a);
   for (int j = 0; j < B.m; j++) {
     hipLaunchKernelGGL(kb_op_dots, dim3(B.nblk), dim3(SB), B.lds, s, a, j);
+    if (B.has_ilu) {
+      batch_ilu_levels(B, a, j, s);
+      hipLaunchKernelGGL(kb_dots, dim3(B.nblk), dim3(SB), 0, s, a, j);
+    }
     hipLaunchKernelGGL(kb_fin_dots, dim3(j + 1, np), dim3(SB), 0, s, a, j);
     hipLaunchKernelGGL(kb_update, dim3(B.nblk), dim3(SB), 0, s, a, j);
     hipLaunchKernelGGL(kb_fin_update, dim3(np), dim3(SB), 0, s, a, j);
@@ -1662,6 +2028,9 @@ static int batch_capture(Batch& B, hipStream_t s) {
   return 0;
 }
 
+// the iterate a member's cycles update: x, or a mixed member's correction
+static double* member_iterate(const BatchMember& M) { return M.mixed ? M.P->d_work2 : M.P->d_x; }
+
 static int batch_tune(Batch& B, int use_graph, TileChoice& out) {
   hipStream_t s = stream();
   const int np = (int)B.mem.size();
@@ -1670,8 +2039,8 @@ static int batch_tune(Batch& B, int use_graph, TileChoice& out) {
   double* snap = nullptr;
   MF_HIP(hipMalloc(&snap, (size_t)std::max(1LL, off[np]) * sizeof(double)));
   for (int k = 0; k < np; k++)
-    MF_HIP(hipMemcpyAsync(snap + off[k], B.mem[k].P->d_x, (size_t)B.mem[k].P->n * sizeof(double),
-                          hipMemcpyDeviceToDevice, s));
+    MF_HIP(hipMemcpyAsync(snap + off[k], member_iterate(B.mem[k]),
+                          (size_t)B.mem[k].P->n * sizeof(double), hipMemcpyDeviceToDevice, s));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   MF_HIP(hipEventCreate(&e0));
   MF_HIP(hipEventCreate(&e1));
@@ -1695,8 +2064,8 @@ static int batch_tune(Batch& B, int use_graph, TileChoice& out) {
     MF_HIP(hipEventRecord(e1, s));
     MF_HIP(hipGetLastError());
     for (int k = 0; k < np; k++)
-      MF_HIP(hipMemcpyAsync(B.mem[k].P->d_x, snap + off[k], (size_t)B.mem[k].P->n * sizeof(double),
-                            hipMemcpyDeviceToDevice, s));
+      MF_HIP(hipMemcpyAsync(member_iterate(B.mem[k]), snap + off[k],
+                            (size_t)B.mem[k].P->n * sizeof(double), hipMemcpyDeviceToDevice, s));
     MF_HIP(hipStreamSynchronize(s));
     float ms = 0.0f;
     MF_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1714,6 +2083,82 @@ static int batch_tune(Batch& B, int use_graph, TileChoice& out) {
   return rc;
 }
 
+// Refinement steps of the mixed-precision members whose step is due: after
+// an inner solve x += d; then d = 0, r = b - A x and rho^2 = ||D^-1 r||^2
+// in FP64 (D: the Jacobi diagonal; none for precon 0), for all of them
+// before one read-back.  Each member then either ends (rho <= tol rho_0,
+// out of iterations or refinement steps, or not a number) or restarts its
+// inner solve from d = 0 with the relative tolerance max(tol rho_0/rho,
+// 1e-5).  The GMRES state of every member is uploaded afterwards.
+constexpr double kInnerTolFloor = 1e-5;
+constexpr int kMaxRefinements = 30;
+
+static GmresStat inner_init(int m, int max_iter, double tol_rel) {
+  GmresStat st{};
+  st.tol_rel = tol_rel < 1e-12 ? 1e-12 : tol_rel;
+  st.tol_res = 1e-12;
+  st.max_iter = max_iter;
+  st.m = m;
+  st.restrt = max_iter / m + 1;
+  return st;
+}
+
+static int batch_refine(Batch& B) {
+  const int np = (int)B.mem.size();
+  std::vector<int> due;
+  for (int k = 0; k < np; k++)
+    if (B.mem[k].mixed && B.mem[k].phase != 0 && !B.mem[k].fin) due.push_back(k);
+  if (due.empty()) return 0;
+  if (batch_alloc(B, np)) return -2;
+  hipStream_t s = stream();
+  for (int k : due) {
+    const BatchMember& M = B.mem[k];
+    Problem& P = *M.P;
+    const int gn = (P.n + 255) / 256;
+    if (M.phase == 2)
+      hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, s, (long long)P.n, P.d_work2, P.d_x);
+    if (launch_zero(P.d_work2, P.n)) return -2;
+    SolveArgs A;
+    fill_solve_args(P, A, M.precon > 0 ? 1 : 0, B.m);  // FP64 values and Jacobi measure
+    const int G = grid_slices(P);
+    hipLaunchKernelGGL(k_refine_residual, dim3(G), dim3(SB), 0, s, A, P.d_rhs, P.d_x, P.d_work);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(SB), 0, s, P.d_partials, G, B.d_rho2 + k);
+  }
+  MF_HIP(hipGetLastError());
+  MF_HIP(hipMemcpyAsync(B.h_rho2, B.d_rho2, np * sizeof(double), hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  for (int k : due) {
+    BatchMember& M = B.mem[k];
+    const double rho = std::sqrt(B.h_rho2[k]);
+    if (M.phase == 1) M.rho0 = rho;
+    M.rho = rho;
+    M.phase = 0;
+    GmresStat& st = B.h_stat[k];
+    if (!(rho == rho) || st.nan) {
+      M.fin = M.nan = true;
+    } else if (rho <= M.tol * M.rho0 || rho < 1e-300) {
+      M.fin = M.conv = true;
+    } else if (M.total >= M.max_iter || M.outer >= kMaxRefinements) {
+      M.fin = true;
+    }
+    if (M.fin) {
+      st.done = st.stop = 1;
+      continue;
+    }
+    st = inner_init(B.m, M.max_iter - M.total, std::max(M.tol * M.rho0 / rho, kInnerTolFloor));
+    M.outer++;
+  }
+  MF_HIP(hipMemcpyAsync(B.d_stat, B.h_stat.data(), np * sizeof(GmresStat), hipMemcpyHostToDevice, s));
+  MF_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// a member still being solved: an FP64 member whose GMRES has not ended, or
+// a mixed member that has not met its refinement test
+static bool member_active(const BatchMember& M, const GmresStat& st) {
+  return M.mixed ? !M.fin : !st.done;
+}
+
 int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
   const int np = (int)B.mem.size();
   if (B.m < 1 || B.m >= MAX_NV) {
@@ -1721,11 +2166,13 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
     return -1;
   }
   hipStream_t s = stream();
-  // members re-assembled since they joined get their Jacobi refilled in place
+  // members re-assembled since they joined get their preconditioner data
+  // remade in place
   for (auto& M : B.mem)
-    if (M.precon && !M.P->dinv_valid && launch_fill_jacobi(*M.P)) return -2;
+    if (prepare_member(M)) return -2;
+  if (batch_refine(B)) return -2;
   int act = 0;
-  for (int k = 0; k < np; k++) act += !B.h_stat[k].done;
+  for (int k = 0; k < np; k++) act += member_active(B.mem[k], B.h_stat[k]);
   if (B.dirty) {
     B.spb_tuned = false;
     if (B.spb > 0) {
@@ -1759,8 +2206,16 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
     MF_HIP(hipStreamSynchronize(s));
     std::copy(B.h_pinned, B.h_pinned + np, B.h_stat.begin());
     B.cycles++;
+    for (int k = 0; k < np; k++) {
+      BatchMember& M = B.mem[k];
+      if (M.mixed && M.phase == 0 && !M.fin && B.h_stat[k].done) {
+        M.total += B.h_stat[k].gmiter;
+        M.phase = 2;
+      }
+    }
+    if (batch_refine(B)) return -2;
     act = 0;
-    for (int k = 0; k < np; k++) act += !B.h_stat[k].done;
+    for (int k = 0; k < np; k++) act += member_active(B.mem[k], B.h_stat[k]);
   }
   if (active) *active = act;
   return 0;

@@ -73,7 +73,25 @@ struct SolveArgs {
   double* partials;
   GmresStat* stat;
   GmresStat* hstat;     // pinned host mirror
+  // mixed-precision inner solves: the Krylov basis and the Jacobi inverses
+  // are FP32 (Vf is the storage of V read with a stride of ldv floats);
+  // sums, dot products, Hessenberg data and the iterate stay FP64
+  float* Vf;
+  const float* dinv32;
+  int v32;
+  int ilu;              // batched solves: the member applies ILU(0) factors
 };
+
+template <typename VT> __device__ __forceinline__ VT* basis(const SolveArgs& A);
+template <> __device__ __forceinline__ double* basis<double>(const SolveArgs& A) { return A.V; }
+template <> __device__ __forceinline__ float* basis<float>(const SolveArgs& A) { return A.Vf; }
+
+// v scaled by the Jacobi inverse of row r, when there is one
+__device__ __forceinline__ double jacobi(const SolveArgs& A, long long r, double v) {
+  if (A.dinv) return v * A.dinv[r];
+  if (A.dinv32) return v * (double)A.dinv32[r];
+  return v;
+}
 
 __device__ __forceinline__ double* Hcol(const SolveArgs& A, int c) { return A.H + (size_t)(A.m + 1) * c; }
 __device__ __forceinline__ double* Giv(const SolveArgs& A) { return A.H + (size_t)(A.m + 1) * (A.m + 1); }
@@ -148,8 +166,8 @@ __device__ __forceinline__ SliceSet block_slices(int nsl) {
 // are loaded first (up to 24 entries in flight per lane), then the vector
 // gathers are issued together, so a slice costs about two memory round trips
 // instead of one per entry
-template <typename VT>
-__device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, const double* x,
+template <typename VT, typename XT>
+__device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, const XT* x,
                                            int sl, int lane) {
   constexpr int CH = 12;
   const long long base = A.sl_ptr[sl] + lane;
@@ -167,7 +185,7 @@ __device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, co
     }
     double xv[CH];
 #pragma unroll
-    for (int k = 0; k < CH; k++) xv[k] = x[c[k]];
+    for (int k = 0; k < CH; k++) xv[k] = (double)x[c[k]];
 #pragma unroll
     for (int k = 0; k < CH; k++) s += v[k] * xv[k];
   }
@@ -175,9 +193,80 @@ __device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, co
 }
 
 // row sum with the FP64 or the FP32 copy of the values (accumulated in FP64)
-__device__ __forceinline__ double sell_row_op(const SolveArgs& A, const double* x, int sl, int lane) {
+template <typename XT>
+__device__ __forceinline__ double sell_row_op(const SolveArgs& A, const XT* x, int sl, int lane) {
   return A.op32 ? sell_row(A, A.val32, x, sl, lane) : sell_row(A, A.val, x, sl, lane);
 }
+
+// ILU(0) substitution over SELL-64 packed levels: a wavefront owns a slice
+// of 64 rows of one level, lane = row, and the factor entries of the slice
+// are read column-major (coalesced).  Row ids and columns are device rows;
+// the substitutions run in place on the vector.  FT: factor type, VT:
+// vector type; the sums are FP64.
+template <typename FT>
+struct IluSweepArgs {
+  const long long* ptr;  // slice start in the packed entries
+  const int* col;        // device row of each entry's column
+  const FT* val;
+  const int* row;        // device row of each packed row (-1: padding)
+  const int* len;        // entries of each packed row
+  const FT* diag;        // backward sweep: 1/u_ii per packed row
+};
+
+// host: the packed factors of a sweep, FP64 or their FP32 copies
+IluSweepArgs<double> sweep_args(const IluData::Sweep& W);
+IluSweepArgs<float> sweep_args32(const IluData::Sweep& W);
+
+// sum_k val_k v[col_k] over the L entries of one packed row, loads issued
+// in groups of 8 so that a group costs about two memory round trips
+template <typename FT, typename VT>
+__device__ __forceinline__ double packed_row_sum(const int* col, const FT* val, long long base,
+                                                 int L, const VT* v) {
+  constexpr int CH = 8;
+  double s = 0.0;
+  for (int k0 = 0; k0 < L; k0 += CH) {
+    int c[CH];
+    double w[CH], x[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+      const bool ok = k0 + k < L;
+      const long long q = base + 64LL * (ok ? k0 + k : k0);
+      c[k] = col[q];
+      w[k] = ok ? (double)val[q] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++) x[k] = (double)v[c[k]];
+#pragma unroll
+    for (int k = 0; k < CH; k++) s += w[k] * x[k];
+  }
+  return s;
+}
+
+// forward: v_i <- v_i - sum_{k before i} l_ik v_k (the v_k are already
+// substituted: they belong to earlier levels)
+template <typename FT, typename VT>
+__device__ __forceinline__ void ilu_fwd_row(const IluSweepArgs<FT>& W, int sl, int lane, VT* v) {
+  const int pr = sl * 64 + lane;
+  const int i = W.row[pr];
+  if (i < 0) return;
+  v[i] = (VT)((double)v[i] - packed_row_sum(W.col, W.val, W.ptr[sl] + lane, W.len[pr], v));
+}
+
+// backward: v_i <- (v_i - sum_{k after i} u_ik v_k) / u_ii
+template <typename FT, typename VT>
+__device__ __forceinline__ void ilu_bwd_row(const IluSweepArgs<FT>& W, int sl, int lane, VT* v) {
+  const int pr = sl * 64 + lane;
+  const int i = W.row[pr];
+  if (i < 0) return;
+  v[i] = (VT)(((double)v[i] - packed_row_sum(W.col, W.val, W.ptr[sl] + lane, W.len[pr], v)) *
+              (double)W.diag[pr]);
+}
+
+// factors of one batch member, FP64 and FP32 copies
+struct IluMember {
+  IluSweepArgs<double> f, b;
+  IluSweepArgs<float> f32, b32;
+};
 
 __device__ inline void drotg_dev(double* a, double* b, double* c, double* s) {
   double roe = *b;
