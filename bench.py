@@ -216,11 +216,25 @@ def make_box_problem(mf, n_el, rng, seed):
     return p
 
 
+def fp64_measure(p, precon=1):
+    """||D^-1 (b - A x)|| / ||D^-1 b|| of a problem's solution, in FP64 on the
+    host with the matrix read back from the device"""
+    import scipy.sparse as sp
+    val, rhs = p.system()
+    rp, ci = p.pattern()
+    A = sp.csr_matrix((val, ci, rp), shape=(p.n, p.n))
+    d = 1.0 / A.diagonal() if precon > 0 else np.ones(p.n)
+    x = p.solution()
+    return float(np.linalg.norm(d * (rhs - A @ x)) / np.linalg.norm(d * rhs))
+
+
 def bench_batch(args):
     """independent std-linear Poisson problems, element counts log-uniform in
     [el_min, el_max]: integrate+assemble+GMRES(30)+Jacobi to 1e-8 for all of
     them as one batch (device restart logic, hipGraph cycles) versus a loop of
-    plain solves of the same problems"""
+    plain solves of the same problems; the batch in FP64 and with
+    mixed-precision members (FP32 inner solves, FP64 refinement to the same
+    1e-8 Jacobi-scaled FP64 measure, checked on the host)"""
     import modfem_amd as mf
     rng = np.random.default_rng(args.seed)
     t0 = time.perf_counter()
@@ -280,16 +294,34 @@ def bench_batch(args):
     b.run(use_graph=True)
     t_fixed8 = time.perf_counter() - t0
     b.free()
+    # mixed-precision members (the tile cache already holds this shape)
+    b = mf.Batch(m)
+    for p in probs:
+        b.add(p, 1, mi, tol, inner_precision=32)
+    t0 = time.perf_counter()
+    b.integrate_assemble(64)
+    b.run(use_graph=True)
+    t_mixed = time.perf_counter() - t0
+    stm = [b.status(p) for p in probs]
+    b.free()
+    meas = max(fp64_measure(p) for p in probs)
     out = {
         "metric": "solves/s of a batch of independent problems at 1e-8 rel residual",
-        "value": len(probs) / t_batch, "unit": "solves/s", "n_gpus": 1,
-        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "value": len(probs) / t_mixed, "unit": "solves/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32 inner solves, f64 refinement", "data": "synthetic",
         "config": {"workload": "%d std-linear prism Poisson problems, %d-%d prisms log-uniform, "
                                "aspect 1-4, kappa lognormal(0,0.5), source U(-1,1), "
                                "GMRES(30)+Jacobi 1e-8" % (args.count, args.el_min, args.el_max),
                    "elements": nel, "dofs": ndof},
-        "batch_s": t_batch, "loop_s": t_loop, "loop_solves_per_s": len(probs) / t_loop,
-        "speedup_vs_loop": t_loop / t_batch, "elements_per_s": nel / t_batch,
+        "mixed_batch_s": t_mixed, "speedup_vs_loop": t_loop / t_mixed,
+        "mixed_all_converged": all(x["converged"] for x in stm),
+        "mixed_max_fp64_measure": meas,
+        "mixed_iters_max": max(x["iters"] for x in stm),
+        "mixed_refinements_max": max(x["refinements"] for x in stm),
+        "fp64_batch_s": t_batch, "fp64_solves_per_s": len(probs) / t_batch,
+        "fp64_speedup_vs_loop": t_loop / t_batch,
+        "loop_s": t_loop, "loop_solves_per_s": len(probs) / t_loop,
+        "elements_per_s": nel / t_mixed,
         "all_converged": all(x["converged"] for x in st),
         "iters": {"min": min(x["iters"] for x in st), "median": float(np.median([x["iters"] for x in st])),
                   "max": max(x["iters"] for x in st)},
@@ -304,10 +336,12 @@ def bench_batch(args):
     return out
 
 
-def footprint(p, m):
+def footprint(p, m, mixed=False):
     """device bytes a resident problem holds: SELL values+columns, element
-    tables, vectors and the Krylov basis"""
-    return 12.0 * p.nnz * 1.04 + 8.0 * (m + 8) * p.n + 120.0 * p.nel
+    tables, vectors and the Krylov basis (and the FP32 values, inverses and
+    refinement vectors of a mixed-precision member)"""
+    extra = 4.0 * p.nnz * 1.04 + 20.0 * p.n if mixed else 0.0
+    return 12.0 * p.nnz * 1.04 + 8.0 * (m + 8) * p.n + 120.0 * p.nel + extra
 
 
 def bench_serve(args):
@@ -315,7 +349,9 @@ def bench_serve(args):
     log-uniform sizes): admission by device-memory footprint and slot count,
     continuous batching between restart cycles, latency p50/p99 and solves/s;
     versus serving the same stream one plain FP64 solve at a time (FIFO,
-    with each request's measured solve time)"""
+    with each request's measured solve time).  The server runs FP64 members
+    and mixed-precision members (FP32 inner solves, FP64 refinement to the
+    same FP64 measure, checked on the host after the run)"""
     import modfem_amd as mf
     rng = np.random.default_rng(args.seed)
     m, tol, mi = 30, 1e-8, 20000
@@ -333,22 +369,23 @@ def bench_serve(args):
         service.append(time.perf_counter() - t0)
     results = []
     budget = args.hbm_gb * 1e9
-    if max(footprint(p, m) for p in probs) > budget:
+    if max(footprint(p, m, True) for p in probs) > budget:
         raise SystemExit("a request is larger than the admission budget")
-    def serve(arr):
+    def serve(arr, mixed):
         """continuous batching server over arrival times arr: latencies and span"""
+        prec = 32 if mixed else 64
         b = mf.Batch(m)
         nxt, live, used, lat = 0, {}, 0.0, []
         t0 = time.perf_counter()
         while len(lat) < len(probs):
             now = time.perf_counter() - t0
             while (nxt < len(probs) and arr[nxt] <= now and len(live) < args.slots and
-                   used + footprint(probs[nxt], m) <= budget):
+                   used + footprint(probs[nxt], m, mixed) <= budget):
                 p = probs[nxt]
-                p.integrate_assemble(64)
-                b.add(p, 1, mi, tol)
+                p.integrate_assemble(64, wait=False)  # ordered before the next cycle
+                b.add(p, 1, mi, tol, inner_precision=prec)
                 live[nxt] = p
-                used += footprint(p, m)
+                used += footprint(p, m, mixed)
                 nxt += 1
             if not live:
                 time.sleep(max(0.0, min(arr[nxt] - now, 0.01)))
@@ -361,7 +398,7 @@ def bench_serve(args):
                     if not st["converged"]:
                         raise RuntimeError("request %d did not converge" % k)
                     lat.append(now - arr[k])
-                    used -= footprint(live[k], m)
+                    used -= footprint(live[k], m, mixed)
                     b.remove(live[k])
                     del live[k]
         span = time.perf_counter() - t0 - arr[0]
@@ -371,8 +408,17 @@ def bench_serve(args):
     # a running server has tuned the batch tiles of the shapes its traffic
     # makes: one untimed pass at the highest rate fills the tile cache
     t0 = time.perf_counter()
-    serve(np.cumsum(np.random.default_rng(args.seed + 1).exponential(1.0 / max(args.rates), len(probs))))
+    warm = np.cumsum(np.random.default_rng(args.seed + 1).exponential(1.0 / max(args.rates), len(probs)))
+    serve(warm, True)
     t_warm = time.perf_counter() - t0
+    serve(warm, False)
+
+    def served(lat, span, seq_span):
+        return {"solves_per_s": len(probs) / span,
+                "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                               "p99": float(np.percentile(lat, 99) * 1e3)},
+                "speedup_solves_per_s": seq_span / span}
+    meas = 0.0
     for lam in args.rates:
         arr = np.cumsum(rng.exponential(1.0 / lam, len(probs)))
         # sequential FIFO server
@@ -381,23 +427,28 @@ def bench_serve(args):
             fin = max(fin, a) + sv
             seq_lat.append(fin - a)
         seq_span = fin - arr[0]
-        lat, span = serve(arr)
-        results.append({"rate_per_s": lam, "solves_per_s": len(probs) / span,
-                        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
-                                       "p99": float(np.percentile(lat, 99) * 1e3)},
-                        "sequential_solves_per_s": len(probs) / seq_span,
-                        "sequential_latency_ms": {"p50": float(np.percentile(seq_lat, 50) * 1e3),
-                                                  "p99": float(np.percentile(seq_lat, 99) * 1e3)},
-                        "speedup_solves_per_s": seq_span / span})
+        lat64, span64 = serve(arr, False)
+        lat, span = serve(arr, True)
+        meas = max([meas] + [fp64_measure(p) for p in probs])
+        r = {"rate_per_s": lam}
+        r.update(served(lat, span, seq_span))
+        r.update({"fp64_members": served(lat64, span64, seq_span),
+                  "sequential_solves_per_s": len(probs) / seq_span,
+                  "sequential_latency_ms": {"p50": float(np.percentile(seq_lat, 50) * 1e3),
+                                            "p99": float(np.percentile(seq_lat, 99) * 1e3)}})
+        results.append(r)
     best = max(results, key=lambda r: r["solves_per_s"])
     out = {
         "metric": "served solves/s and p50/p99 latency at 1e-8 rel residual",
         "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f32 inner solves, f64 refinement", "data": "synthetic",
         "config": {"workload": "Poisson arrivals of %d std-linear prism Poisson requests, %d-%d prisms "
-                               "log-uniform, GMRES(30)+Jacobi 1e-8, <= %d slots, %.0f GB admission "
-                               "budget" % (args.count, args.el_min, args.el_max, args.slots, args.hbm_gb)},
+                               "log-uniform, GMRES(30)+Jacobi 1e-8 (mixed-precision members: FP32 "
+                               "inner solves, FP64 refinement to the same FP64 measure), <= %d slots, "
+                               "%.0f GB admission budget; sequential baseline: plain FP64 solves"
+                               % (args.count, args.el_min, args.el_max, args.slots, args.hbm_gb)},
         "rates": results,
+        "max_fp64_measure_of_served_solutions": meas,
         "sequential_mean_service_ms": float(np.mean(service) * 1e3),
         "tile_warmup_pass_s": t_warm,
     }
@@ -812,14 +863,16 @@ def other_workloads(args):
     a.count, a.el_min, a.el_max = 256, 2000, 200000
     print("workload: batch of 256 problems", file=sys.stderr, flush=True)
     r = bench_batch(a)
-    out["batch_256"] = {k: r[k] for k in ("value", "unit", "config", "batch_s", "loop_s",
-                                          "loop_solves_per_s", "speedup_vs_loop",
-                                          "all_converged", "iters", "max_rel_diff_vs_loop_solution",
-                                          "tiles")}
+    out["batch_256"] = {k: r[k] for k in ("value", "unit", "config", "mixed_batch_s", "speedup_vs_loop",
+                                          "mixed_all_converged", "mixed_max_fp64_measure",
+                                          "fp64_batch_s", "fp64_speedup_vs_loop", "loop_s",
+                                          "loop_solves_per_s", "all_converged", "iters",
+                                          "max_rel_diff_vs_loop_solution", "tiles")}
     a.count, a.el_min, a.el_max, a.rates = 128, 5000, 500000, [1000.0, 2000.0]
     print("workload: served stream", file=sys.stderr, flush=True)
     r = bench_serve(a)
-    out["serve"] = {k: r[k] for k in ("value", "unit", "config", "rates")}
+    out["serve"] = {k: r[k] for k in ("value", "unit", "config", "rates",
+                                      "max_fp64_measure_of_served_solutions")}
     a.dg_n, a.dg_pdegs, a.dg_solve, a.steps, a.warmup = 96, [], [], 3, 1
     print("workload: DG mixed degrees 96^3", file=sys.stderr, flush=True)
     r = bench_dg(a)

@@ -134,6 +134,10 @@ int mf_problem_set_coefficients(int pid, const int* coeff_ind, int coef_model,
    (utd_util/uts_accel_intf.c:70-103).  precision: 64 or 32 (FP32
    shape-function products with FP64 geometry and FP64 global sums). */
 int mf_integrate_assemble(int pid, int precision);
+/* mf_integrate_assemble without waiting for the device: the work is
+   enqueued on the library stream, so every later library call on the
+   problem (a solve, a batch cycle, a read-back) sees the new system */
+int mf_integrate_assemble_async(int pid, int precision);
 /* per boundary face (face-ID order of the mesh): type 1 Dirichlet penalty
    K += 1e7 phi_j phi_i dA, F += 1e7 g phi_i dA; type 2 Neumann F += g phi_i dA
    with constant flux g, or for the exact-solution source models the exact
@@ -242,6 +246,10 @@ int mf_true_residual(int pid, const double* x, double* rel);
 int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems,
                        double* ms_faces);
 int mf_bench_spmv(int pid, int reps, double* ms);
+/* ms per SELL-64 operator application y = A x with prec 0 (FP64 values and
+   vectors), 1 (FP32 values, FP64 vectors) or 2 (FP32 values and vectors),
+   ch in {8, 12, 16, 24} matrix entries per row in flight per lane */
+int mf_bench_spmv_ex(int pid, int prec, int ch, int reps, double* ms);
 int mf_last_solve_ms(int pid, double* ms);
 /* plain device-to-device stream copy of nbytes (rounded down to 16 bytes):
    the achievable-bandwidth yardstick and the known-size kernel that scales

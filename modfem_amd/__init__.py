@@ -95,6 +95,7 @@ def _declare(L):
         "mf_problem_get_table": (I, [I, I, V]),
         "mf_problem_set_coefficients": (I, [I, V, I, V, V, I, V]),
         "mf_integrate_assemble": (I, [I, I]),
+        "mf_integrate_assemble_async": (I, [I, I]),
         "mf_element_matrices": (I, [I, I, I, I, V]),
         "mf_face_matrices": (I, [I, I, I, V]),
         "mf_get_system": (I, [I, V, V]),
@@ -128,6 +129,7 @@ def _declare(L):
         "mf_solve_mkb": (I, [I, V, ctypes.c_double, I, V, V, V, V]),
         "mf_bench_integrate": (I, [I, I, I, V, V]),
         "mf_bench_spmv": (I, [I, I, V]),
+        "mf_bench_spmv_ex": (I, [I, I, I, I, V]),
         "mf_last_solve_ms": (I, [I, V]),
         "mf_bench_copy": (I, [ctypes.c_longlong, I, V]),
         "mf_gmres_kernel_stats": (I, [I, V]),
@@ -622,8 +624,13 @@ class Problem:
     def set_assembly_mode(self, mode):
         check(lib().mf_set_assembly_mode(self.h, mode))
 
-    def integrate_assemble(self, precision=64):
-        check(lib().mf_integrate_assemble(self.h, precision))
+    def integrate_assemble(self, precision=64, wait=True):
+        """wait=False only enqueues the work on the library stream (later
+        library calls see the new system)"""
+        if wait:
+            check(lib().mf_integrate_assemble(self.h, precision))
+        else:
+            check(lib().mf_integrate_assemble_async(self.h, precision))
 
     def element_matrices(self, first=0, count=None, precision=64):
         count = self.nel - first if count is None else count
@@ -732,9 +739,14 @@ class Problem:
         check(lib().mf_bench_integrate(self.h, precision, reps, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    def bench_spmv(self, reps=20):
+    def bench_spmv(self, reps=20, prec=None, ch=12):
+        """ms per operator application; prec 0/1/2 selects FP64, FP32 values,
+        FP32 values and vectors (None: the solver's FP64 operator)"""
         a = ctypes.c_double()
-        check(lib().mf_bench_spmv(self.h, reps, ctypes.byref(a)))
+        if prec is None:
+            check(lib().mf_bench_spmv(self.h, reps, ctypes.byref(a)))
+        else:
+            check(lib().mf_bench_spmv_ex(self.h, prec, ch, reps, ctypes.byref(a)))
         return a.value
 
     def last_solve_ms(self):

@@ -735,6 +735,20 @@ int mf_integrate_assemble(int pid, int precision) {
   return 0;
 }
 
+int mf_integrate_assemble_async(int pid, int precision) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (precision != 64 && precision != 32) {
+    set_error("precision must be 64 or 32");
+    return -1;
+  }
+  if (P->nel == 0) {
+    set_error("problem has no mesh");
+    return -1;
+  }
+  return assemble_problem(*P, precision) ? -2 : 0;
+}
+
 int mf_element_matrices(int pid, int first, int count, int precision, double* out) {
   Problem* P = get_problem(pid);
   if (!P) return -3;
@@ -1148,6 +1162,16 @@ int mf_bench_spmv(int pid, int reps, double* ms) {
   return 0;
 }
 
+int mf_bench_spmv_ex(int pid, int prec, int ch, int reps, double* ms) {
+  Problem* P = get_problem(pid);
+  if (!P) return -3;
+  if (!ms) {
+    set_error("mf_bench_spmv_ex: ms is null");
+    return -1;
+  }
+  return bench_spmv_variant(*P, prec, ch, reps, ms);
+}
+
 int mf_bench_copy(long long nbytes, int reps, double* ms) {
   if (need_init()) return -1;
   if (!ms) {
@@ -1284,7 +1308,7 @@ int mf_batch_status_ex(int bid, int pid, int* done, int* converged, int* iters, 
   for (size_t k = 0; k < B->mem.size(); k++)
     if (B->mem[k].pid == pid) {
       const GmresStat& st = B->h_stat[k];
-      const BatchMember& M = B->mem[k];
+      const MixedState& M = B->h_mix[k];
       if (M.mixed) {
         // refinement state: FP64 measure rho / rho_0, inner iterations so far
         if (done) *done = M.fin ? 1 : 0;

@@ -241,21 +241,27 @@ struct BatchMember {
   int pid = -1;
   int precon = 1, max_iter = 1000;
   double tol = 1e-8;
-  // mixed-precision member: FP64 refinement steps (host-driven, between
-  // restart cycles) around inner solves with FP32 values, basis and
-  // preconditioner data; the inner solve runs on (r, d) = (d_work, d_work2)
+  // mixed-precision member: FP64 refinement steps around inner solves with
+  // FP32 values, basis and preconditioner data; the inner solve runs on
+  // (r, d) = (d_work, d_work2)
   bool mixed = false;
-  int phase = 0;        // 0: inner solve running; 1: first refinement step due;
-                        // 2: refinement step due after an inner solve
-  bool fin = false, conv = false, nan = false;
-  int total = 0, outer = 0;  // inner iterations and refinement steps so far
-  double rho0 = 0.0, rho = 0.0;
+};
+// refinement state of a batch member, kept on the device (host copy after
+// every cycle)
+struct MixedState {
+  int mixed;
+  int phase;            // 1: the first refinement step is due
+  int fin, conv, nan;   // ended; met the test; not a number
+  int total, outer;     // inner iterations and refinement steps so far
+  int max_iter;
+  double tol, rho0, rho;
 };
 struct Batch {
   int m = 30;
   std::vector<BatchMember> mem;
   std::vector<GmresStat> h_stat;   // host copy, current after every cycle
   bool dirty = true;               // tables must be rebuilt
+  bool retile = true;              // membership changed: choose the tile again
   // device tables (valid when !dirty)
   void* d_args = nullptr;          // SolveArgs per member
   int* d_blk_prob = nullptr;
@@ -263,7 +269,7 @@ struct Batch {
   int* d_nblk = nullptr;
   GmresStat* d_stat = nullptr;
   GmresStat* h_pinned = nullptr;
-  int nblk = 0, cap = 0;
+  int nblk = 0, cap = 0, blk_cap = 0;
   size_t lds = 0;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -281,8 +287,11 @@ struct Batch {
   int* d_ilu_s1 = nullptr;
   std::vector<int> ilu_off_f, ilu_off_b;  // level l's blocks: [off[l], off[l+1])
   // refinement steps of mixed-precision members
-  double* d_rho2 = nullptr;
-  double* h_rho2 = nullptr;        // pinned
+  bool has_mixed = false;
+  std::vector<MixedState> h_mix;   // host copy, current after every cycle
+  MixedState* d_mix = nullptr;
+  MixedState* h_mix_pinned = nullptr;
+  void* d_ref = nullptr;           // RefineArgs per member
 };
 // tile candidates the tuner times (slices per block)
 constexpr int kBatchTileCands[] = {2, 4, 8, 16, 32};
@@ -323,6 +332,8 @@ int gmres_noalloc(Problem& P, int precon, int krylbas, int max_iter, double tol,
 int gmres_mixed(Problem& P, int precon, int krylbas, int max_iter, double tol, double* h_x,
                 int* iters, double* res_rel, double* resid, int* refinements);
 int launch_spmv(Problem& P, const double* d_x, double* d_y);
+// ms per operator application of a precision variant (mf_bench_spmv_ex)
+int bench_spmv_variant(Problem& P, int prec, int ch, int reps, double* ms);
 // ILU(0) (mf_ilu.hip): structure for an elimination order, factorization
 // from the current values, and out = (LU)^-1 in (device-order vectors; out
 // may alias in); stop: skip when *stop is set (GMRES stop flag) or null

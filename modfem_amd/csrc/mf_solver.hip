@@ -312,6 +312,22 @@ __global__ void __launch_bounds__(SB) k_sumsq_v0(SolveArgs A) {
     sumsq_body<double>(A, s0, s1, lb);
 }
 
+// y = A x for the operator benchmark: values FP64 or FP32 (VT), vector FP64
+// or FP32 (XT), CH entries of a row in flight per lane
+template <typename VT, typename XT, int CH>
+__global__ void __launch_bounds__(SB) k_spmv_bench(SolveArgs A, const VT* val, const XT* x, XT* y) {
+  const int nsl = (A.n + 63) >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int s0, s1;
+  slice_range(nsl, lb, s0, s1);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int sl = s0 + wv; sl < s1; sl += SW) {
+    const double s = sell_row<VT, XT, CH>(A, val, x, sl, lane);
+    const int r = sl * 64 + lane;
+    if (r < A.n) y[r] = (XT)s;
+  }
+}
+
 __global__ void __launch_bounds__(SB) k_spmv_sell(SolveArgs A, const double* x, double* y) {
   const int nsl = (A.n + 63) >> 6;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -467,65 +483,71 @@ __global__ void __launch_bounds__(SB) k_fin_dots(SolveArgs A, int j, int G) {
 // Arnoldi step j, part 2: w -= sum_i c_i V_i over row pairs (16-byte loads),
 // per-block partial ||w||^2.  The vectors are read in groups of eight
 // independent loads; the subtraction order is i = 0, 1, ..., j.
-template <typename VT> struct Pair;
-template <> struct Pair<double> { using T = double2; };
-template <> struct Pair<float> { using T = float2; };
+// rows are updated in groups of W consecutive rows per lane (16-byte
+// loads): W = 2 for the FP64 basis, 4 for the FP32 one
+template <typename VT> struct Grp;
+template <> struct Grp<double> { using T = double2; static constexpr int W = 2; };
+template <> struct Grp<float> { using T = float4; static constexpr int W = 4; };
 
+// block r of G of a member updates the row groups [r g / G, (r+1) g / G)
 template <typename VT>
-__device__ __forceinline__ void update_body(const SolveArgs& A, int j, long long p0, long long p1,
-                                            int pb) {
-  using P2 = typename Pair<VT>::T;
+__device__ __forceinline__ void update_body(const SolveArgs& A, int j, int r, int G, int pb) {
+  using GT = typename Grp<VT>::T;
+  constexpr int W = Grp<VT>::W;
   __shared__ double red[1];
   __shared__ double cs[MAX_NV];
   if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
   __syncthreads();
-  const long long ldp = A.ldv / 2;
-  const P2* V2 = reinterpret_cast<const P2*>(basis<VT>(A));
-  P2* w = reinterpret_cast<P2*>(basis<VT>(A) + (size_t)(j + 1) * A.ldv);
+  const long long ng = (A.n + W - 1) / W;  // ldv is a multiple of W: whole groups stay inside
+  const long long p0 = r * ng / G, p1 = (r + 1) * ng / G;
+  const long long ldg = A.ldv / W;
+  const GT* VG = reinterpret_cast<const GT*>(basis<VT>(A));
+  GT* w = reinterpret_cast<GT*>(basis<VT>(A) + (size_t)(j + 1) * A.ldv);
   double acc = 0.0;
   for (long long p = p0 + threadIdx.x; p < p1; p += SB) {
-    const P2 w0 = w[p];
-    double wx = (double)w0.x, wy = (double)w0.y;
+    const GT w0 = w[p];
+    double wv[W];
+#pragma unroll
+    for (int e = 0; e < W; e++) wv[e] = (double)reinterpret_cast<const VT*>(&w0)[e];
     int i = 0;
     for (; i + 7 <= j; i += 8) {
-      P2 v[8];
+      GT v[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) v[q] = V2[(size_t)(i + q) * ldp + p];
+      for (int q = 0; q < 8; q++) v[q] = VG[(size_t)(i + q) * ldg + p];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        wx -= cs[i + q] * (double)v[q].x;
-        wy -= cs[i + q] * (double)v[q].y;
-      }
+      for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int e = 0; e < W; e++) wv[e] -= cs[i + q] * (double)reinterpret_cast<const VT*>(&v[q])[e];
     }
     for (; i <= j; i++) {
-      const P2 v = V2[(size_t)i * ldp + p];
-      wx -= cs[i] * (double)v.x;
-      wy -= cs[i] * (double)v.y;
+      const GT v = VG[(size_t)i * ldg + p];
+#pragma unroll
+      for (int e = 0; e < W; e++) wv[e] -= cs[i] * (double)reinterpret_cast<const VT*>(&v)[e];
     }
-    if (2 * p + 1 >= A.n) wy = 0.0;  // padding lane of an odd n
-    P2 wr;
-    wr.x = (VT)wx;
-    wr.y = (VT)wy;
+    GT wr;
+#pragma unroll
+    for (int e = 0; e < W; e++) {
+      const VT q = W * p + e < A.n ? (VT)wv[e] : (VT)0;  // padding rows stay zero
+      reinterpret_cast<VT*>(&wr)[e] = q;
+      acc += (double)q * (double)q;
+    }
     w[p] = wr;
-    acc += (double)wr.x * (double)wr.x + (double)wr.y * (double)wr.y;
   }
   block_sum_multi<1>(&acc, 1, red);
   if (threadIdx.x == 0) A.partials[pb] = red[0];
 }
 
-__device__ __forceinline__ void update_any(const SolveArgs& A, int j, long long p0, long long p1,
-                                           int pb) {
+__device__ __forceinline__ void update_any(const SolveArgs& A, int j, int r, int G, int pb) {
   if (A.v32)
-    update_body<float>(A, j, p0, p1, pb);
+    update_body<float>(A, j, r, G, pb);
   else
-    update_body<double>(A, j, p0, p1, pb);
+    update_body<double>(A, j, r, G, pb);
 }
 
 __global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
   if (A.stat->stop) return;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const long long np = (A.n + 1) / 2;  // row pairs (ldv is even)
-  update_any(A, j, lb * np / gridDim.x, (lb + 1) * np / gridDim.x, lb);
+  update_any(A, j, lb, gridDim.x, lb);
 }
 
 // h_{j+1,j} = ||w||, Givens rotations (drot/drotg as lss_mkb_core.c:2539-2560),
@@ -811,6 +833,66 @@ int launch_spmv(Problem& P, const double* d_x, double* d_y) {
   else
     hipLaunchKernelGGL(k_spmv, dim3(grid_tiles(P)), dim3(SB), 0, stream(), A, d_x, d_y);
   MF_HIP(hipGetLastError());
+  return 0;
+}
+
+static int round_mixed(Problem& P, int precon);
+
+// operator benchmark variants: prec 0 FP64 values and vector, 1 FP32
+// values and FP64 vector, 2 FP32 values and vector; ch entries in flight
+template <typename VT, typename XT>
+static void launch_spmv_bench(const Problem& P, const SolveArgs& A, const VT* val, const void* x,
+                              void* y, int ch) {
+  const dim3 g(grid_slices(P)), b(SB);
+  const XT* xx = static_cast<const XT*>(x);
+  XT* yy = static_cast<XT*>(y);
+  hipStream_t s = stream();
+  if (ch == 8)
+    hipLaunchKernelGGL((k_spmv_bench<VT, XT, 8>), g, b, 0, s, A, val, xx, yy);
+  else if (ch == 16)
+    hipLaunchKernelGGL((k_spmv_bench<VT, XT, 16>), g, b, 0, s, A, val, xx, yy);
+  else if (ch == 24)
+    hipLaunchKernelGGL((k_spmv_bench<VT, XT, 24>), g, b, 0, s, A, val, xx, yy);
+  else
+    hipLaunchKernelGGL((k_spmv_bench<VT, XT, 12>), g, b, 0, s, A, val, xx, yy);
+}
+
+int bench_spmv_variant(Problem& P, int prec, int ch, int reps, double* ms) {
+  if (!P.sell || prec < 0 || prec > 2 || reps < 1) {
+    set_error("mf_bench_spmv_ex: SELL problem, prec 0/1/2 and reps >= 1 required");
+    return -1;
+  }
+  if (prec > 0 && round_mixed(P, 0)) return -2;
+  SolveArgs A;
+  fill_solve_args(P, A, 0, 0);
+  void *x = nullptr, *y = nullptr;
+  MF_HIP(hipMalloc(&x, (size_t)P.n * sizeof(double)));
+  MF_HIP(hipMalloc(&y, (size_t)P.n * sizeof(double)));
+  MF_HIP(hipMemsetAsync(x, 0, (size_t)P.n * sizeof(double), stream()));
+  auto run = [&]() {
+    if (prec == 0)
+      launch_spmv_bench<double, double>(P, A, P.d_val, x, y, ch);
+    else if (prec == 1)
+      launch_spmv_bench<float, double>(P, A, P.d_val32, x, y, ch);
+    else
+      launch_spmv_bench<float, float>(P, A, P.d_val32, x, y, ch);
+  };
+  hipEvent_t e0, e1;
+  MF_HIP(hipEventCreate(&e0));
+  MF_HIP(hipEventCreate(&e1));
+  run();
+  MF_HIP(hipEventRecord(e0, stream()));
+  for (int r = 0; r < reps; r++) run();
+  MF_HIP(hipEventRecord(e1, stream()));
+  MF_HIP(hipEventSynchronize(e1));
+  MF_HIP(hipGetLastError());
+  float t = 0;
+  (void)hipEventElapsedTime(&t, e0, e1);
+  *ms = t / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(x);
+  (void)hipFree(y);
   return 0;
 }
 
@@ -1447,9 +1529,10 @@ int gmres_noalloc(Problem& P, int precon, int krylbas, int max_iter, double tol,
 //
 // Mixed-precision members run FP32-valued inner solves (FP32 values, basis,
 // Jacobi inverses or ILU factors; FP64 sums and Hessenberg data) on the
-// correction equation A d = r; between cycles the host performs their FP64
-// refinement steps x += d, r = b - A x, rho = ||D^-1 r|| and restarts the
-// inner solve to max(tol rho_0 / rho, 1e-5) until rho <= tol rho_0.
+// correction equation A d = r.  Their FP64 refinement steps x += d,
+// r = b - A x, rho = ||D^-1 r|| run on the device before and after every
+// cycle (three launches each, no host decision), restarting the inner solve
+// to max(tol rho_0 / rho, 1e-5) until rho <= tol rho_0.
 // ---------------------------------------------------------------------
 struct BatchArgs {
   const SolveArgs* args;
@@ -1608,8 +1691,7 @@ __global__ void __launch_bounds__(SB) kb_update(BatchArgs B, int j) {
   const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
   const SolveArgs A = B.args[p];
   if (A.stat->stop) return;
-  const long long np = (A.n + 1) / 2;
-  update_any(A, j, r * np / G, (r + 1) * np / G, r);
+  update_any(A, j, r, G, r);
 }
 
 __global__ void __launch_bounds__(SB) kb_fin_update(BatchArgs B, int j) {
@@ -1669,6 +1751,114 @@ __global__ void kb_advance(BatchArgs B, int nprob) {
   if (st->done) st->stop = 1;
 }
 
+// Refinement steps of mixed-precision members, on the device.  A step is
+// due for a member that has not ended when its first step has not been
+// taken (phase 1) or its inner solve has ended: x += d (not on the first
+// step), d = 0, r = b - A x and rho = ||D^-1 r|| in FP64 (D the Jacobi
+// diagonal; the plain norm for precon 0).  The member then ends (rho <= tol
+// rho_0, out of iterations or refinement steps, or not a number) or starts
+// a new inner solve to the relative tolerance max(tol rho_0 / rho, 1e-5).
+constexpr double kInnerTolFloor = 1e-5;
+constexpr int kMaxRefinements = 30;
+
+__device__ __forceinline__ bool refine_due(const MixedState& M, const GmresStat& st) {
+  return M.mixed && !M.fin && (M.phase == 1 || st.done);
+}
+
+__global__ void __launch_bounds__(SB) kb_refine_axpy(BatchArgs B, const RefineArgs* R,
+                                                     const MixedState* mix) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs& A = B.args[p];
+  if (!refine_due(mix[p], *A.stat)) return;
+  const bool add = mix[p].phase != 1;
+  double* x = R[p].x;
+  const long long r0 = (long long)r * A.n / G, r1 = (long long)(r + 1) * A.n / G;
+  for (long long i = r0 + threadIdx.x; i < r1; i += SB) {
+    if (add) x[i] += A.x[i];
+    A.x[i] = 0.0;
+  }
+}
+
+// r = b - A x with the FP64 values, partial ||D^-1 r||^2 per block
+__global__ void __launch_bounds__(SB) kb_refine_resid(BatchArgs B, const RefineArgs* R,
+                                                      const MixedState* mix) {
+  const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
+  const SolveArgs& A = B.args[p];
+  if (!refine_due(mix[p], *A.stat)) return;
+  __shared__ double red[1];
+  const RefineArgs& Q = R[p];
+  double* res = const_cast<double*>(A.b);
+  const int nsl = (A.n + 63) >> 6;
+  const int s0 = (int)((long long)r * nsl / G), s1 = (int)((long long)(r + 1) * nsl / G);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double acc = 0.0;
+  for (int sl = s0 + wv; sl < s1; sl += SW) {
+    const double sum = sell_row(A, A.val, Q.x, sl, lane);
+    const int i = sl * 64 + lane;
+    if (i < A.n) {
+      const double v = Q.rhs[i] - sum;
+      res[i] = v;
+      const double pv = Q.dinv ? v * Q.dinv[i] : v;
+      acc += pv * pv;
+    }
+  }
+  block_sum_multi<1>(&acc, 1, red);
+  if (threadIdx.x == 0) A.partials[r] = red[0];
+}
+
+// one block per member: rho, the test, and the next inner solve
+__global__ void __launch_bounds__(SB) kb_refine_fin(BatchArgs B, MixedState* mix, int m) {
+  const int p = blockIdx.x;
+  const SolveArgs& A = B.args[p];
+  if (!refine_due(mix[p], *A.stat)) return;
+  __shared__ double red[1];
+  const int G = B.nblk[p];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < G; b += SB) s += A.partials[b];
+  block_sum_multi<1>(&s, 1, red);
+  if (threadIdx.x != 0) return;
+  MixedState& M = mix[p];
+  GmresStat* st = A.stat;
+  const double rho = sqrt(red[0]);
+  if (M.phase == 1)
+    M.rho0 = rho;
+  else
+    M.total += st->gmiter;
+  M.rho = rho;
+  M.phase = 0;
+  if (!(rho == rho) || st->nan) {
+    M.fin = M.nan = 1;
+  } else if (rho <= M.tol * M.rho0 || rho < 1e-300) {
+    M.fin = M.conv = 1;
+  } else if (M.total >= M.max_iter || M.outer >= kMaxRefinements) {
+    M.fin = 1;
+  }
+  if (M.fin) {
+    st->done = st->stop = 1;
+    return;
+  }
+  // a new inner solve from d = 0 (tolerances floored as lss_mkb_core.c:2377-2378)
+  const int mi = M.max_iter - M.total;
+  const double tr = fmax(M.tol * M.rho0 / rho, kInnerTolFloor);
+  GmresStat z = {};
+  z.tol_rel = tr < 1e-12 ? 1e-12 : tr;
+  z.tol_res = 1e-12;
+  z.max_iter = mi;
+  z.m = m;
+  z.restrt = mi / m + 1;
+  *st = z;
+  M.outer++;
+}
+
+static BatchArgs batch_args(const Batch& B) {
+  BatchArgs a;
+  a.args = reinterpret_cast<const SolveArgs*>(B.d_args);
+  a.blk_prob = B.d_blk_prob;
+  a.blk_rank = B.d_blk_rank;
+  a.nblk = B.d_nblk;
+  return a;
+}
+
 static void batch_drop_graph(Batch& B) {
   if (B.exec) (void)hipGraphExecDestroy(B.exec);
   if (B.graph) (void)hipGraphDestroy(B.graph);
@@ -1688,15 +1878,19 @@ static void batch_drop_ilu_tables(Batch& B) {
 static void batch_drop_tables(Batch& B) {
   batch_drop_graph(B);
   batch_drop_ilu_tables(B);
-  void* ptrs[] = {B.d_args, B.d_blk_prob, B.d_blk_rank, B.d_nblk, B.d_stat, B.d_ilu, B.d_rho2};
+  void* ptrs[] = {B.d_args, B.d_blk_prob, B.d_blk_rank, B.d_nblk, B.d_stat, B.d_ilu, B.d_mix,
+                  B.d_ref};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  if (B.h_pinned) (void)hipHostFree(B.h_pinned);
-  if (B.h_rho2) (void)hipHostFree(B.h_rho2);
+  void* hptrs[] = {B.h_pinned, B.h_mix_pinned};
+  for (void* q : hptrs)
+    if (q) (void)hipHostFree(q);
   B.d_args = nullptr; B.d_blk_prob = B.d_blk_rank = B.d_nblk = nullptr;
   B.d_stat = nullptr; B.h_pinned = nullptr;
-  B.d_ilu = nullptr; B.d_rho2 = nullptr; B.h_rho2 = nullptr;
+  B.d_ilu = nullptr; B.d_mix = nullptr; B.h_mix_pinned = nullptr;
+  B.d_ref = nullptr;
   B.cap = 0;
+  B.blk_cap = 0;
 }
 
 // initial state of a member: tolerances floored as lss_mkb_core.c:2377-2378
@@ -1762,13 +1956,18 @@ int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double to
     if (!P->d_work) MF_HIP(hipMalloc(&P->d_work, (size_t)P->n * sizeof(double)));
     if (!P->d_work2) MF_HIP(hipMalloc(&P->d_work2, (size_t)(P->n + 1) * sizeof(double)));
     MF_HIP(hipMemsetAsync(P->d_work2, 0, (size_t)P->n * sizeof(double), s));
-    M.phase = 1;  // the first refinement step sets rho_0 and starts the inner solve
-    st.done = st.stop = 1;
+    st.done = st.stop = 1;  // until the first refinement step starts the inner solve
   }
+  MixedState ms{};
+  ms.mixed = mixed ? 1 : 0;
+  ms.phase = mixed ? 1 : 0;
+  ms.max_iter = max_iter;
+  ms.tol = tol;
   P->batch_refs++;
   B.mem.push_back(M);
   B.h_stat.push_back(st);
-  B.dirty = true;
+  B.h_mix.push_back(ms);
+  B.dirty = B.retile = true;
   return (int)B.mem.size() - 1;
 }
 
@@ -1778,7 +1977,8 @@ int batch_remove(Batch& B, int pid) {
       B.mem[k].P->batch_refs--;
       B.mem.erase(B.mem.begin() + k);
       B.h_stat.erase(B.h_stat.begin() + k);
-      B.dirty = true;
+      B.h_mix.erase(B.h_mix.begin() + k);
+      B.dirty = B.retile = true;
       return 0;
     }
   set_error("batch: problem is not a member");
@@ -1793,9 +1993,10 @@ static int batch_alloc(Batch& B, int np) {
   MF_HIP(hipMalloc(&B.d_nblk, (size_t)cap * sizeof(int)));
   MF_HIP(hipMalloc(&B.d_stat, (size_t)cap * sizeof(GmresStat)));
   MF_HIP(hipMalloc(&B.d_ilu, (size_t)cap * sizeof(IluMember)));
-  MF_HIP(hipMalloc(&B.d_rho2, (size_t)cap * sizeof(double)));
+  MF_HIP(hipMalloc(&B.d_mix, (size_t)cap * sizeof(MixedState)));
   MF_HIP(hipHostMalloc(&B.h_pinned, (size_t)cap * sizeof(GmresStat), hipHostMallocDefault));
-  MF_HIP(hipHostMalloc(&B.h_rho2, (size_t)cap * sizeof(double), hipHostMallocDefault));
+  MF_HIP(hipHostMalloc(&B.h_mix_pinned, (size_t)cap * sizeof(MixedState), hipHostMallocDefault));
+  MF_HIP(hipMalloc(&B.d_ref, (size_t)cap * sizeof(RefineArgs)));
   B.cap = cap;
   B.dirty = true;
   return 0;
@@ -1864,15 +2065,19 @@ static int batch_build(Batch& B) {
   if (batch_alloc(B, np)) return -2;
   std::vector<SolveArgs> args(np);
   std::vector<IluMember> ilus(np);
+  std::vector<RefineArgs> refs(np);
   std::vector<int> nblk(np), bp, br;
   int max_sl = 1;
   B.has_ilu = false;
+  B.has_mixed = false;
   for (int k = 0; k < np; k++) {
     Problem& P = *B.mem[k].P;
     SolveArgs& A = args[k];
     member_args(B.mem[k], A, B.m);
     A.stat = B.d_stat + k;
     A.hstat = nullptr;
+    refs[k] = RefineArgs{P.d_x, P.d_rhs, B.mem[k].precon > 0 ? P.d_dinv : nullptr};
+    B.has_mixed = B.has_mixed || B.mem[k].mixed;
     IluMember& L = ilus[k];
     L = IluMember{};
     if (B.mem[k].precon == 2) {
@@ -1897,12 +2102,17 @@ static int batch_build(Batch& B) {
     set_error("batch: a member is too large for the batched operator (> 33M rows)");
     return -1;
   }
-  if (B.d_blk_prob) (void)hipFree(B.d_blk_prob);
-  if (B.d_blk_rank) (void)hipFree(B.d_blk_rank);
-  B.d_blk_prob = B.d_blk_rank = nullptr;
+  if ((int)bp.size() > B.blk_cap) {
+    if (B.d_blk_prob) (void)hipFree(B.d_blk_prob);
+    if (B.d_blk_rank) (void)hipFree(B.d_blk_rank);
+    B.d_blk_prob = B.d_blk_rank = nullptr;
+    B.blk_cap = 0;
+    const int cap = (int)bp.size() * 2;
+    MF_HIP(hipMalloc(&B.d_blk_prob, (size_t)cap * sizeof(int)));
+    MF_HIP(hipMalloc(&B.d_blk_rank, (size_t)cap * sizeof(int)));
+    B.blk_cap = cap;
+  }
   if (np > 0) {
-    MF_HIP(hipMalloc(&B.d_blk_prob, bp.size() * sizeof(int)));
-    MF_HIP(hipMalloc(&B.d_blk_rank, br.size() * sizeof(int)));
     hipStream_t s = stream();
     MF_HIP(hipMemcpyAsync(B.d_blk_prob, bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(B.d_blk_rank, br.data(), br.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -1910,6 +2120,8 @@ static int batch_build(Batch& B) {
     MF_HIP(hipMemcpyAsync(B.d_nblk, nblk.data(), np * sizeof(int), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(B.d_stat, B.h_stat.data(), np * sizeof(GmresStat), hipMemcpyHostToDevice, s));
     MF_HIP(hipMemcpyAsync(B.d_ilu, ilus.data(), np * sizeof(IluMember), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(B.d_ref, refs.data(), np * sizeof(RefineArgs), hipMemcpyHostToDevice, s));
+    MF_HIP(hipMemcpyAsync(B.d_mix, B.h_mix.data(), np * sizeof(MixedState), hipMemcpyHostToDevice, s));
     MF_HIP(hipStreamSynchronize(s));
   }
   if (batch_build_ilu(B)) return -2;
@@ -1933,13 +2145,20 @@ static void batch_ilu_levels(Batch& B, const BatchArgs& a, int j, hipStream_t s)
   }
 }
 
-static void batch_enqueue_cycle(Batch& B, hipStream_t s) {
-  BatchArgs a;
-  a.args = reinterpret_cast<const SolveArgs*>(B.d_args);
-  a.blk_prob = B.d_blk_prob;
-  a.blk_rank = B.d_blk_rank;
-  a.nblk = B.d_nblk;
+static void batch_refine_steps(Batch& B, hipStream_t s) {
+  const BatchArgs a = batch_args(B);
+  const RefineArgs* R = reinterpret_cast<const RefineArgs*>(B.d_ref);
+  hipLaunchKernelGGL(kb_refine_axpy, dim3(B.nblk), dim3(SB), 0, s, a, R, B.d_mix);
+  hipLaunchKernelGGL(kb_refine_resid, dim3(B.nblk), dim3(SB), 0, s, a, R, B.d_mix);
+  hipLaunchKernelGGL(kb_refine_fin, dim3((int)B.mem.size()), dim3(SB), 0, s, a, B.d_mix, B.m);
+}
+
+// one restart cycle; with refine, the refinement steps of mixed-precision
+// members due before it (first steps) and after it (ended inner solves)
+static void batch_enqueue_cycle(Batch& B, hipStream_t s, bool refine = true) {
+  const BatchArgs a = batch_args(B);
   const int np = (int)B.mem.size();
+  if (refine && B.has_mixed) batch_refine_steps(B, s);
   hipLaunchKernelGGL(kb_residual, dim3(B.nblk), dim3(SB), 0, s, a);
   if (B.has_ilu) {
     batch_ilu_levels(B, a, -1, s);
@@ -1959,6 +2178,7 @@ static void batch_enqueue_cycle(Batch& B, hipStream_t s) {
   hipLaunchKernelGGL(kb_ysolve, dim3(np), dim3(64), 0, s, a);
   hipLaunchKernelGGL(kb_xupdate, dim3(B.nblk), dim3(SB), 0, s, a);
   hipLaunchKernelGGL(kb_advance, dim3((np + 255) / 256), dim3(256), 0, s, a, np);
+  if (refine && B.has_mixed) batch_refine_steps(B, s);
 }
 
 // ---------------------------------------------------------------------
@@ -1987,7 +2207,7 @@ int batch_set_tiles(Batch& B, int spb) {
     return -1;
   }
   B.spb = spb;
-  B.dirty = true;
+  B.dirty = B.retile = true;
   return 0;
 }
 
@@ -2020,9 +2240,9 @@ static std::map<std::tuple<int, int, int>, TileChoice>::iterator tile_near(
   return best;
 }
 
-static int batch_capture(Batch& B, hipStream_t s) {
+static int batch_capture(Batch& B, hipStream_t s, bool refine = true) {
   MF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  batch_enqueue_cycle(B, s);
+  batch_enqueue_cycle(B, s, refine);
   MF_HIP(hipStreamEndCapture(s, &B.graph));
   MF_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
   return 0;
@@ -2052,15 +2272,16 @@ static int batch_tune(Batch& B, int use_graph, TileChoice& out) {
     B.spb_used = kBatchTileCands[t < 0 ? 0 : t];
     rc = batch_build(B);  // also uploads the pre-trial GMRES state
     if (rc) break;
+    // trial cycles run the inner solves only (no refinement steps)
     if (use_graph) {
-      if (!B.exec && (rc = batch_capture(B, s))) break;
+      if (!B.exec && (rc = batch_capture(B, s, false))) break;
       MF_HIP(hipGraphUpload(B.exec, s));
     }
     MF_HIP(hipEventRecord(e0, s));
     if (use_graph)
       MF_HIP(hipGraphLaunch(B.exec, s));
     else
-      batch_enqueue_cycle(B, s);
+      batch_enqueue_cycle(B, s, false);
     MF_HIP(hipEventRecord(e1, s));
     MF_HIP(hipGetLastError());
     for (int k = 0; k < np; k++)
@@ -2083,80 +2304,23 @@ static int batch_tune(Batch& B, int use_graph, TileChoice& out) {
   return rc;
 }
 
-// Refinement steps of the mixed-precision members whose step is due: after
-// an inner solve x += d; then d = 0, r = b - A x and rho^2 = ||D^-1 r||^2
-// in FP64 (D: the Jacobi diagonal; none for precon 0), for all of them
-// before one read-back.  Each member then either ends (rho <= tol rho_0,
-// out of iterations or refinement steps, or not a number) or restarts its
-// inner solve from d = 0 with the relative tolerance max(tol rho_0/rho,
-// 1e-5).  The GMRES state of every member is uploaded afterwards.
-constexpr double kInnerTolFloor = 1e-5;
-constexpr int kMaxRefinements = 30;
-
-static GmresStat inner_init(int m, int max_iter, double tol_rel) {
-  GmresStat st{};
-  st.tol_rel = tol_rel < 1e-12 ? 1e-12 : tol_rel;
-  st.tol_res = 1e-12;
-  st.max_iter = max_iter;
-  st.m = m;
-  st.restrt = max_iter / m + 1;
-  return st;
-}
-
-static int batch_refine(Batch& B) {
-  const int np = (int)B.mem.size();
-  std::vector<int> due;
-  for (int k = 0; k < np; k++)
-    if (B.mem[k].mixed && B.mem[k].phase != 0 && !B.mem[k].fin) due.push_back(k);
-  if (due.empty()) return 0;
-  if (batch_alloc(B, np)) return -2;
-  hipStream_t s = stream();
-  for (int k : due) {
-    const BatchMember& M = B.mem[k];
-    Problem& P = *M.P;
-    const int gn = (P.n + 255) / 256;
-    if (M.phase == 2)
-      hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, s, (long long)P.n, P.d_work2, P.d_x);
-    if (launch_zero(P.d_work2, P.n)) return -2;
-    SolveArgs A;
-    fill_solve_args(P, A, M.precon > 0 ? 1 : 0, B.m);  // FP64 values and Jacobi measure
-    const int G = grid_slices(P);
-    hipLaunchKernelGGL(k_refine_residual, dim3(G), dim3(SB), 0, s, A, P.d_rhs, P.d_x, P.d_work);
-    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(SB), 0, s, P.d_partials, G, B.d_rho2 + k);
-  }
-  MF_HIP(hipGetLastError());
-  MF_HIP(hipMemcpyAsync(B.h_rho2, B.d_rho2, np * sizeof(double), hipMemcpyDeviceToHost, s));
-  MF_HIP(hipStreamSynchronize(s));
-  for (int k : due) {
-    BatchMember& M = B.mem[k];
-    const double rho = std::sqrt(B.h_rho2[k]);
-    if (M.phase == 1) M.rho0 = rho;
-    M.rho = rho;
-    M.phase = 0;
-    GmresStat& st = B.h_stat[k];
-    if (!(rho == rho) || st.nan) {
-      M.fin = M.nan = true;
-    } else if (rho <= M.tol * M.rho0 || rho < 1e-300) {
-      M.fin = M.conv = true;
-    } else if (M.total >= M.max_iter || M.outer >= kMaxRefinements) {
-      M.fin = true;
-    }
-    if (M.fin) {
-      st.done = st.stop = 1;
-      continue;
-    }
-    st = inner_init(B.m, M.max_iter - M.total, std::max(M.tol * M.rho0 / rho, kInnerTolFloor));
-    M.outer++;
-  }
-  MF_HIP(hipMemcpyAsync(B.d_stat, B.h_stat.data(), np * sizeof(GmresStat), hipMemcpyHostToDevice, s));
-  MF_HIP(hipStreamSynchronize(s));
-  return 0;
-}
 
 // a member still being solved: an FP64 member whose GMRES has not ended, or
 // a mixed member that has not met its refinement test
-static bool member_active(const BatchMember& M, const GmresStat& st) {
+static bool member_active(const MixedState& M, const GmresStat& st) {
   return M.mixed ? !M.fin : !st.done;
+}
+
+// device state to the host copies
+static int batch_read_state(Batch& B, hipStream_t s) {
+  const int np = (int)B.mem.size();
+  MF_HIP(hipMemcpyAsync(B.h_pinned, B.d_stat, np * sizeof(GmresStat), hipMemcpyDeviceToHost, s));
+  if (B.has_mixed)
+    MF_HIP(hipMemcpyAsync(B.h_mix_pinned, B.d_mix, np * sizeof(MixedState), hipMemcpyDeviceToHost, s));
+  MF_HIP(hipStreamSynchronize(s));
+  std::copy(B.h_pinned, B.h_pinned + np, B.h_stat.begin());
+  if (B.has_mixed) std::copy(B.h_mix_pinned, B.h_mix_pinned + np, B.h_mix.begin());
+  return 0;
 }
 
 int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
@@ -2170,11 +2334,12 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
   // remade in place
   for (auto& M : B.mem)
     if (prepare_member(M)) return -2;
-  if (batch_refine(B)) return -2;
   int act = 0;
-  for (int k = 0; k < np; k++) act += member_active(B.mem[k], B.h_stat[k]);
-  if (B.dirty) {
+  for (int k = 0; k < np; k++) act += member_active(B.h_mix[k], B.h_stat[k]);
+  if (B.retile) {
+    B.retile = false;
     B.spb_tuned = false;
+    const int prev = B.spb_used;
     if (B.spb > 0) {
       B.spb_used = B.spb;
     } else if (act > 0 && max_cycles > 0) {
@@ -2182,6 +2347,16 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
       auto it = g_tile_cache.find(key);
       if (it == g_tile_cache.end()) it = tile_near(key);
       if (it == g_tile_cache.end()) {
+        // the trial cycles time the inner solves: members whose first
+        // refinement step is due take it before
+        bool fresh = false;
+        for (const auto& ms : B.h_mix) fresh = fresh || (ms.mixed && ms.phase == 1);
+        if (fresh) {
+          if (B.dirty && batch_build(B)) return -2;
+          batch_refine_steps(B, s);
+          MF_HIP(hipGetLastError());
+          if (batch_read_state(B, s)) return -2;
+        }
         TileChoice tc{};
         const int rc = batch_tune(B, use_graph, tc);
         if (rc) return rc;
@@ -2191,6 +2366,9 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
       B.spb_tuned = true;
       std::copy(it->second.ms, it->second.ms + kBatchTileNum, B.tile_ms);
     }
+    if (B.spb_used != prev) B.dirty = true;
+  }
+  if (B.dirty) {
     const int rc = batch_build(B);
     if (rc) return rc;
   }
@@ -2202,20 +2380,10 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
       batch_enqueue_cycle(B, s);
     }
     MF_HIP(hipGetLastError());
-    MF_HIP(hipMemcpyAsync(B.h_pinned, B.d_stat, np * sizeof(GmresStat), hipMemcpyDeviceToHost, s));
-    MF_HIP(hipStreamSynchronize(s));
-    std::copy(B.h_pinned, B.h_pinned + np, B.h_stat.begin());
+    if (batch_read_state(B, s)) return -2;
     B.cycles++;
-    for (int k = 0; k < np; k++) {
-      BatchMember& M = B.mem[k];
-      if (M.mixed && M.phase == 0 && !M.fin && B.h_stat[k].done) {
-        M.total += B.h_stat[k].gmiter;
-        M.phase = 2;
-      }
-    }
-    if (batch_refine(B)) return -2;
     act = 0;
-    for (int k = 0; k < np; k++) act += member_active(B.mem[k], B.h_stat[k]);
+    for (int k = 0; k < np; k++) act += member_active(B.h_mix[k], B.h_stat[k]);
   }
   if (active) *active = act;
   return 0;
@@ -2230,6 +2398,7 @@ void batch_free(Batch& B) {
   for (auto& M : B.mem) M.P->batch_refs--;
   B.mem.clear();
   B.h_stat.clear();
+  B.h_mix.clear();
 }
 
 }  // namespace mf

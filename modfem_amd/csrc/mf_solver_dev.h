@@ -163,31 +163,34 @@ __device__ __forceinline__ SliceSet block_slices(int nsl) {
 }
 
 // row sum of the calling lane's row: the slice's column indices and values
-// are loaded first (up to 24 entries in flight per lane), then the vector
-// gathers are issued together, so a slice costs about two memory round trips
-// instead of one per entry
-template <typename VT, typename XT>
+// are loaded first (CH entries in flight per lane), then the vector gathers
+// are issued together, so a chunk costs about two memory round trips instead
+// of one per entry.  Every load is unconditional (indices past the slice's
+// length are clamped to its last entry and their products dropped by a
+// select), so the compiler keeps the loads of a chunk in flight together.
+template <typename VT, typename XT, int CH = 12>
 __device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, const XT* x,
                                            int sl, int lane) {
-  constexpr int CH = 12;
   const long long base = A.sl_ptr[sl] + lane;
   const int len = (int)((A.sl_ptr[sl + 1] - A.sl_ptr[sl]) >> 6);
   double s = 0.0;
   for (int k0 = 0; k0 < len; k0 += CH) {
     int c[CH];
-    double v[CH];
+    VT v[CH];
 #pragma unroll
     for (int k = 0; k < CH; k++) {
-      const bool ok = k0 + k < len;
-      const long long q = base + 64LL * (ok ? k0 + k : k0);
+      const long long q = base + 64LL * min(k0 + k, len - 1);
       c[k] = A.scol[q];
-      v[k] = ok ? (double)val[q] : 0.0;
+      v[k] = val[q];
     }
-    double xv[CH];
+    XT xv[CH];
 #pragma unroll
-    for (int k = 0; k < CH; k++) xv[k] = (double)x[c[k]];
+    for (int k = 0; k < CH; k++) xv[k] = x[c[k]];
 #pragma unroll
-    for (int k = 0; k < CH; k++) s += v[k] * xv[k];
+    for (int k = 0; k < CH; k++) {
+      const double t = (double)v[k] * (double)xv[k];
+      s += k0 + k < len ? t : 0.0;
+    }
   }
   return s;
 }
@@ -261,6 +264,13 @@ __device__ __forceinline__ void ilu_bwd_row(const IluSweepArgs<FT>& W, int sl, i
   v[i] = (VT)(((double)v[i] - packed_row_sum(W.col, W.val, W.ptr[sl] + lane, W.len[pr], v)) *
               (double)W.diag[pr]);
 }
+
+// FP64 data of a mixed-precision batch member's refinement steps
+struct RefineArgs {
+  double* x;            // the solution
+  const double* rhs;
+  const double* dinv;   // Jacobi inverses of the measure (null: plain norm)
+};
 
 // factors of one batch member, FP64 and FP32 copies
 struct IluMember {
