@@ -248,7 +248,9 @@ int mf_bench_integrate(int pid, int precision, int reps, double* ms_elems,
 int mf_bench_spmv(int pid, int reps, double* ms);
 /* ms per SELL-64 operator application y = A x with prec 0 (FP64 values and
    vectors), 1 (FP32 values, FP64 vectors) or 2 (FP32 values and vectors),
-   ch in {8, 12, 16, 24} matrix entries per row in flight per lane */
+   16-bit columns in the slices spanning fewer than 65536 rows (as the
+   solvers use them); prec + 10: 32-bit columns everywhere; ch in
+   {8, 12, 16, 24} matrix entries per row in flight per lane */
 int mf_bench_spmv_ex(int pid, int prec, int ch, int reps, double* ms);
 int mf_last_solve_ms(int pid, double* ms);
 /* plain device-to-device stream copy of nbytes (rounded down to 16 bytes):
@@ -292,6 +294,15 @@ int mf_batch_remove(int bid, int pid);
 int mf_batch_integrate_assemble(int bid, int precision);
 /* run up to max_cycles restart cycles; *active = members still iterating */
 int mf_batch_run(int bid, int max_cycles, int use_graph, int* active);
+/* give the batch a stream of its own (default priority): its cycles then
+   run there, after the work enqueued on the library stream before each
+   run, so that several batches can run at once */
+int mf_batch_set_stream(int bid);
+/* enqueue one restart cycle and return; mf_batch_wait completes it
+   (status, membership and tile calls are refused while a cycle is in
+   flight, and the members must not be touched until then) */
+int mf_batch_run_async(int bid, int use_graph);
+int mf_batch_wait(int bid, int* active);
 int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters,
                     double* res_rel);
 /* mf_batch_status with the refinement steps of a mixed-precision member
@@ -307,7 +318,7 @@ int mf_batch_status_ex(int bid, int pid, int* done, int* converged, int* iters,
  * are undone (x and the GMRES state are restored), so tuning changes no
  * iterate.  The choice is cached for the process per batch shape (total
  * rows in half-octaves, member count in octaves, graph or eager launches);
- * a later batch of that shape, or within one step of it on either count,
+ * a later batch of that shape, or within two steps of it on either count,
  * takes the cached choice without trials. */
 int mf_batch_set_tiles(int bid, int spb);
 /* tile in use, whether it was tuned, and the trial cycle times in ms of the

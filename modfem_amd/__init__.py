@@ -114,6 +114,9 @@ def _declare(L):
         "mf_batch_create": (I, [I]),
         "mf_batch_add": (I, [I, I, I, I, ctypes.c_double]),
         "mf_batch_add_ex": (I, [I, I, I, I, ctypes.c_double, I]),
+        "mf_batch_set_stream": (I, [I]),
+        "mf_batch_run_async": (I, [I, I]),
+        "mf_batch_wait": (I, [I, V]),
         "mf_batch_status_ex": (I, [I, I, V, V, V, V, V]),
         "mf_batch_remove": (I, [I, I]),
         "mf_batch_integrate_assemble": (I, [I, I]),
@@ -251,9 +254,13 @@ def bench_copy(nbytes, reps=5):
 class Batch:
     """Batched GMRES over resident problems (continuous batching)."""
 
-    def __init__(self, krylbas=30):
+    def __init__(self, krylbas=30, own_stream=False):
+        """own_stream: cycles run on a stream of the batch's own, so that
+        several batches can run at once (run_async / wait)"""
         init()
         self.h = check(lib().mf_batch_create(krylbas))
+        if own_stream:
+            check(lib().mf_batch_set_stream(self.h))
 
     def add(self, problem, precon=1, max_iter=1000, tol=1e-8, inner_precision=64):
         """start a member; inner_precision=32 makes it a mixed-precision
@@ -272,6 +279,16 @@ class Batch:
         """run restart cycles; returns the number of members still iterating"""
         a = ctypes.c_int()
         check(lib().mf_batch_run(self.h, max_cycles, int(use_graph), ctypes.byref(a)))
+        return a.value
+
+    def run_async(self, use_graph=False):
+        """enqueue one restart cycle; wait() completes it"""
+        check(lib().mf_batch_run_async(self.h, int(use_graph)))
+
+    def wait(self):
+        """complete the cycle in flight; returns the members still iterating"""
+        a = ctypes.c_int()
+        check(lib().mf_batch_wait(self.h, ctypes.byref(a)))
         return a.value
 
     def status(self, problem):

@@ -67,7 +67,7 @@ void free_problem_device(Problem& P) {
                   P.d_coef, P.d_dinv, P.d_diag_ptr, P.d_dinv32, P.d_tile_row, P.d_V, P.d_H,
                   P.d_partials, P.d_counter, P.d_stat, P.d_x, P.d_work, P.d_work2, P.d_sl_ptr,
                   P.d_scol, P.d_diag_k, P.d_inc_ptr, P.d_inc_el, P.d_inc_loc,
-                  P.d_finc_ptr, P.d_finc_f, P.d_finc_loc, P.d_brows, P.d_el_offT,
+                  P.d_finc_ptr, P.d_finc_f, P.d_finc_loc, P.d_el_offT, P.d_scol16, P.d_sbase,
                   P.d_erow, P.d_bf_type};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -517,11 +517,6 @@ int mf_problem_create(int mesh) {
     if (upload(&P->d_finc_ptr, fp.data(), fp.size())) return -2;
     if (upload(&P->d_finc_f, ff.data(), ff.size())) return -2;
     if (upload(&P->d_finc_loc, fl.data(), fl.size())) return -2;
-    std::vector<int> brows;
-    for (int i = 0; i < n; i++)
-      if (fp[i + 1] > fp[i]) brows.push_back(i);
-    P->nbrows = (int)brows.size();
-    if (upload(&P->d_brows, brows.data(), brows.size())) return -2;
   }
   // default coefficients: -Laplace u = 1 (DIFF_IN_CUBE-style selection)
   P->mask = TermMask{0, 1, 0, 0, 0, 0, 0, 1};
@@ -1227,10 +1222,16 @@ static Batch* get_batch(int bid) {
   return it->second.get();
 }
 
+static bool batch_idle(const Batch* B) {
+  if (B->pending) set_error("batch: a cycle is in flight (mf_batch_wait first)");
+  return !B->pending;
+}
+
 int mf_batch_add(int bid, int pid, int precon, int max_iter, double tol) {
   Batch* B = get_batch(bid);
   Problem* P = get_problem(pid);
   if (!B || !P) return -3;
+  if (!batch_idle(B)) return -1;
   return batch_add(*B, P, pid, precon, max_iter, tol);
 }
 
@@ -1242,18 +1243,21 @@ int mf_batch_add_ex(int bid, int pid, int precon, int max_iter, double tol, int 
     set_error("mf_batch_add_ex: inner_precision must be 64 or 32");
     return -1;
   }
+  if (!batch_idle(B)) return -1;
   return batch_add(*B, P, pid, precon, max_iter, tol, inner_precision == 32);
 }
 
 int mf_batch_remove(int bid, int pid) {
   Batch* B = get_batch(bid);
   if (!B) return -3;
+  if (!batch_idle(B)) return -1;
   return batch_remove(*B, pid);
 }
 
 int mf_batch_integrate_assemble(int bid, int precision) {
   Batch* B = get_batch(bid);
   if (!B) return -3;
+  if (!batch_idle(B)) return -1;
   if (precision != 64 && precision != 32) {
     set_error("precision must be 64 or 32");
     return -1;
@@ -1274,9 +1278,29 @@ int mf_batch_run(int bid, int max_cycles, int use_graph, int* active) {
   return batch_run(*B, max_cycles, use_graph, active);
 }
 
+int mf_batch_set_stream(int bid) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  if (!batch_idle(B)) return -1;
+  return batch_own_stream(*B);
+}
+
+int mf_batch_run_async(int bid, int use_graph) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  return batch_run_async(*B, use_graph);
+}
+
+int mf_batch_wait(int bid, int* active) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  return batch_wait(*B, active);
+}
+
 int mf_batch_set_tiles(int bid, int spb) {
   Batch* B = get_batch(bid);
   if (!B) return -3;
+  if (!batch_idle(B)) return -1;
   return batch_set_tiles(*B, spb);
 }
 
@@ -1305,6 +1329,7 @@ int mf_batch_status_ex(int bid, int pid, int* done, int* converged, int* iters, 
                        int* refinements) {
   Batch* B = get_batch(bid);
   if (!B) return -3;
+  if (!batch_idle(B)) return -1;
   for (size_t k = 0; k < B->mem.size(); k++)
     if (B->mem[k].pid == pid) {
       const GmresStat& st = B->h_stat[k];

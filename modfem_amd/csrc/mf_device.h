@@ -177,8 +177,6 @@ struct Problem {
   int* d_finc_ptr = nullptr;
   int* d_finc_f = nullptr;
   uint8_t* d_finc_loc = nullptr;
-  int* d_brows = nullptr;      // rows touched by boundary faces
-  int nbrows = 0;
   uint8_t* d_el_offT = nullptr; // [nel][36] offsets (row-major per element)
   void* d_erow = nullptr;       // element-row buffers of the two-pass assembly
   size_t erow_bytes = 0;
@@ -200,6 +198,8 @@ struct Problem {
   int batch_refs = 0;          // batches this problem is a member of
   int* d_diag_ptr = nullptr;
   float* d_dinv32 = nullptr;
+  uint16_t* d_scol16 = nullptr;  // FP32 operator: 16-bit columns of narrow slices
+  int* d_sbase = nullptr;        // per slice: their base column, or -1
   bool val32_valid = false;    // d_val32 (and d_dinv32) are the current values rounded
   // solver tiles
   std::vector<int> tile_row;   // ntiles+1
@@ -292,6 +292,10 @@ struct Batch {
   MixedState* d_mix = nullptr;
   MixedState* h_mix_pinned = nullptr;
   void* d_ref = nullptr;           // RefineArgs per member
+  // own stream (optional): cycles run there after the library stream's work
+  hipStream_t own = nullptr;
+  hipEvent_t ev_ready = nullptr, ev_done = nullptr;
+  bool pending = false;            // an asynchronous cycle has not been waited for
 };
 // tile candidates the tuner times (slices per block)
 constexpr int kBatchTileCands[] = {2, 4, 8, 16, 32};
@@ -302,6 +306,9 @@ int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double to
               bool mixed = false);
 int batch_remove(Batch& B, int pid);
 int batch_run(Batch& B, int max_cycles, int use_graph, int* active);
+int batch_run_async(Batch& B, int use_graph);
+int batch_wait(Batch& B, int* active);
+int batch_own_stream(Batch& B);
 void batch_free(Batch& B);
 
 int device_init(int device);

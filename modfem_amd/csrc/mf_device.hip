@@ -878,8 +878,27 @@ __device__ __forceinline__ void face_row(const RowArgs& A, int f, int w, double*
 
 constexpr int ROW_WAVES = 4;  // wavefronts (SELL slices) per block
 
-// elements: one lane per row, accumulators in LDS (k-major per wave, so the
-// 64 lanes of a wave touch 64 consecutive words)
+// boundary faces of row r, added after all elements as the reference
+// assembles them (face rows of the owning elements, ascending face order)
+template <int NS>
+__device__ __forceinline__ double face_rows_into(const RowArgs& A, int r, double (*acc)[64],
+                                                 int lane) {
+  const size_t N = A.I.nel;
+  double F = 0.0;
+  for (int p = A.finc_ptr[r]; p < A.finc_ptr[r + 1]; p++) {
+    const int f = A.finc_f[p], w = A.finc_loc[p];
+    const int e = A.bf_elem[f];
+    double Kd[NS], Fd;
+    face_row<NS>(A, f, w, Kd, Fd);
+#pragma unroll
+    for (int u = 0; u < NS; u++) acc[A.I.el_off[(size_t)(w * 6 + u) * N + e]][lane] += Kd[u];
+    F += Fd;
+  }
+  return F;
+}
+
+// elements, then boundary faces: one lane per row, accumulators in LDS
+// (k-major per wave, so the 64 lanes of a wave touch 64 consecutive words)
 template <int NS, class Q, int TS, int RMAX, typename R>
 __global__ void __launch_bounds__(64 * ROW_WAVES) k_assemble_rows(RowArgs A) {
   __shared__ double s_acc[ROW_WAVES][RMAX][64];
@@ -906,36 +925,12 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_assemble_rows(RowArgs A) {
       }
       F += (double)Fr;
     }
+    F += face_rows_into<NS>(A, r, acc, lane);
   }
   const long long base = A.I.sl_ptr[sl];
   const int len = (int)((A.I.sl_ptr[sl + 1] - base) >> 6);
   for (int k = 0; k < len; k++) A.val[base + 64LL * k + lane] = acc[k][lane];
   if (r < A.n) A.rhs[r] = F;
-}
-
-// boundary faces, added afterwards to the rows they touch (the reference
-// assembles faces after all elements); one lane per such row
-template <int NS>
-__global__ void __launch_bounds__(256) k_assemble_face_rows(RowArgs A, const int* rows, int nrows) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int r = rows[t];
-  const long long base = A.I.sl_ptr[r >> 6] + (r & 63);
-  const size_t N = A.I.nel;
-  double F = 0.0;
-  for (int p = A.finc_ptr[r]; p < A.finc_ptr[r + 1]; p++) {
-    const int f = A.finc_f[p], w = A.finc_loc[p];
-    const int e = A.bf_elem[f];
-    double Kd[NS], Fd;
-    face_row<NS>(A, f, w, Kd, Fd);
-#pragma unroll
-    for (int u = 0; u < NS; u++) {
-      const int o = A.I.el_off[(size_t)(w * 6 + u) * N + e];
-      A.val[base + 64LL * o] += Kd[u];
-    }
-    F += Fd;
-  }
-  A.rhs[r] += F;
 }
 
 template <int NS, class Q, int RMAX, typename R>
@@ -983,26 +978,18 @@ int launch_assemble_rows(Problem& P, int precision) {
     }
   }
   MF_HIP(hipGetLastError());
-  if (P.nbrows > 0) {
-    dim3 gf((P.nbrows + 255) / 256);
-    if (P.el_type == kTetra)
-      hipLaunchKernelGGL(k_assemble_face_rows<4>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
-    else
-      hipLaunchKernelGGL(k_assemble_face_rows<6>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
-    MF_HIP(hipGetLastError());
-  }
   return 0;
 }
 
 // Two-pass assembly, pass 2: one lane per matrix row gathers the rows its
 // DOF owns in the element-row buffers (ascending element order), adds them
 // into LDS accumulators through the per-element offset table and writes the
-// SELL slice once.  T is the buffer type (FP64, or FP32 for the
-// mixed-precision form; sums are FP64 either way).
-template <int RMAX, typename T>
+// SELL slice once, after the boundary faces of the row.  T is the buffer
+// type (FP64, or FP32 for the mixed-precision form; sums are FP64 either way).
+template <int NS, int RMAX, typename T>
 __global__ void __launch_bounds__(64 * ROW_WAVES) k_gather_rows(RowArgs A, const T* erow,
-                                                               const T* ef, const uint8_t* offT,
-                                                               int ns) {
+                                                               const T* ef, const uint8_t* offT) {
+  constexpr int ns = NS;
   __shared__ double s_acc[ROW_WAVES][RMAX][64];
   const int nsl = (A.n + 63) >> 6;
   const int wv = threadIdx.x >> 6;
@@ -1020,6 +1007,7 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_gather_rows(RowArgs A, const
       for (int u = 0; u < ns; u++) acc[offT[q + u]][lane] += (double)erow[q + u];
       F += (double)ef[(size_t)A.inc_el[p] * 6 + A.inc_loc[p]];
     }
+    F += face_rows_into<NS>(A, r, acc, lane);
   }
   const long long base = A.I.sl_ptr[sl];
   const int len = (int)((A.I.sl_ptr[sl + 1] - base) >> 6);
@@ -1062,31 +1050,32 @@ int launch_assemble_two_pass(Problem& P, int precision) {
   A.val = P.d_val; A.rhs = P.d_rhs;
   const int nsl = (P.n + 63) / 64;
   dim3 g((nsl + ROW_WAVES - 1) / ROW_WAVES), b(64 * ROW_WAVES);
-  const int mr = P.setup.max_row, ns = nodes_of(P.el_type);
+  const int mr = P.setup.max_row;
   if (mr > 32) {
     set_error("two-pass assembly supports rows of at most 32 entries; use the atomic path");
     return -1;
   }
+  const bool tet = P.el_type == kTetra;
   if (precision == 32) {
     const float* er = (const float*)P.d_erow;
     const float* ef = er + (size_t)P.nel * 36;
-    if (mr <= 24) hipLaunchKernelGGL((k_gather_rows<24, float>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
-    else hipLaunchKernelGGL((k_gather_rows<32, float>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
+    if (tet)
+      hipLaunchKernelGGL((k_gather_rows<4, 32, float>), g, b, 0, g_stream, A, er, ef, P.d_el_offT);
+    else if (mr <= 24)
+      hipLaunchKernelGGL((k_gather_rows<6, 24, float>), g, b, 0, g_stream, A, er, ef, P.d_el_offT);
+    else
+      hipLaunchKernelGGL((k_gather_rows<6, 32, float>), g, b, 0, g_stream, A, er, ef, P.d_el_offT);
   } else {
     const double* er = (const double*)P.d_erow;
     const double* ef = er + (size_t)P.nel * 36;
-    if (mr <= 24) hipLaunchKernelGGL((k_gather_rows<24, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
-    else hipLaunchKernelGGL((k_gather_rows<32, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT, ns);
+    if (tet)
+      hipLaunchKernelGGL((k_gather_rows<4, 32, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT);
+    else if (mr <= 24)
+      hipLaunchKernelGGL((k_gather_rows<6, 24, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT);
+    else
+      hipLaunchKernelGGL((k_gather_rows<6, 32, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT);
   }
   MF_HIP(hipGetLastError());
-  if (P.nbrows > 0) {
-    dim3 gf((P.nbrows + 255) / 256);
-    if (P.el_type == kTetra)
-      hipLaunchKernelGGL(k_assemble_face_rows<4>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
-    else
-      hipLaunchKernelGGL(k_assemble_face_rows<6>, gf, dim3(256), 0, g_stream, A, P.d_brows, P.nbrows);
-    MF_HIP(hipGetLastError());
-  }
   return 0;
 }
 

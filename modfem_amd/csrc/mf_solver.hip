@@ -490,14 +490,13 @@ template <> struct Grp<double> { using T = double2; static constexpr int W = 2; 
 template <> struct Grp<float> { using T = float4; static constexpr int W = 4; };
 
 // block r of G of a member updates the row groups [r g / G, (r+1) g / G)
+// cs: the coefficients c_i, i <= j, in shared memory (visible to the block)
 template <typename VT>
-__device__ __forceinline__ void update_body(const SolveArgs& A, int j, int r, int G, int pb) {
+__device__ __forceinline__ void update_body(const SolveArgs& A, int j, int r, int G, int pb,
+                                            const double* cs) {
   using GT = typename Grp<VT>::T;
   constexpr int W = Grp<VT>::W;
   __shared__ double red[1];
-  __shared__ double cs[MAX_NV];
-  if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
-  __syncthreads();
   const long long ng = (A.n + W - 1) / W;  // ldv is a multiple of W: whole groups stay inside
   const long long p0 = r * ng / G, p1 = (r + 1) * ng / G;
   const long long ldg = A.ldv / W;
@@ -537,17 +536,21 @@ __device__ __forceinline__ void update_body(const SolveArgs& A, int j, int r, in
   if (threadIdx.x == 0) A.partials[pb] = red[0];
 }
 
-__device__ __forceinline__ void update_any(const SolveArgs& A, int j, int r, int G, int pb) {
+__device__ __forceinline__ void update_any(const SolveArgs& A, int j, int r, int G, int pb,
+                                           const double* cs) {
   if (A.v32)
-    update_body<float>(A, j, r, G, pb);
+    update_body<float>(A, j, r, G, pb, cs);
   else
-    update_body<double>(A, j, r, G, pb);
+    update_body<double>(A, j, r, G, pb, cs);
 }
 
 __global__ void __launch_bounds__(SB) k_update(SolveArgs A, int j) {
   if (A.stat->stop) return;
+  __shared__ double cs[MAX_NV];
+  if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
+  __syncthreads();
   const int lb = xcd_block(blockIdx.x, gridDim.x);
-  update_any(A, j, lb, gridDim.x, lb);
+  update_any(A, j, lb, gridDim.x, lb, cs);
 }
 
 // h_{j+1,j} = ||w||, Givens rotations (drot/drotg as lss_mkb_core.c:2539-2560),
@@ -708,6 +711,7 @@ static void fill_solve_args(Problem& P, SolveArgs& A, int precon, int m) {
   A.b = P.d_rhs; A.x = P.d_x; A.V = P.d_V; A.H = P.d_H; A.partials = P.d_partials;
   A.stat = P.d_stat; A.hstat = P.h_stat;
   A.Vf = reinterpret_cast<float*>(P.d_V); A.dinv32 = nullptr; A.v32 = 0; A.ilu = 0;
+  A.scol16 = P.d_scol16; A.sbase = P.d_sbase;
 }
 
 // grid sizes: one resident round over the CUs (LDS limits a tile kernel to 5
@@ -838,6 +842,42 @@ int launch_spmv(Problem& P, const double* d_x, double* d_y) {
 
 static int round_mixed(Problem& P, int precon);
 
+// 16-bit columns of the SELL operator: one wavefront per slice finds the
+// smallest and largest column of the slice; a slice spanning fewer than
+// 65536 rows stores offsets from its smallest column
+__global__ void k_narrow_columns(int nsl, const long long* sl_ptr, const int* scol,
+                                 uint16_t* scol16, int* sbase) {
+  const int sl = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (sl >= nsl) return;
+  const int lane = threadIdx.x & 63;
+  const long long b = sl_ptr[sl], e = sl_ptr[sl + 1];
+  int lo = 0x7fffffff, hi = -1;
+  for (long long q = b + lane; q < e; q += 64) {
+    lo = min(lo, scol[q]);
+    hi = max(hi, scol[q]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  const bool narrow = hi >= lo && hi - lo < 65536;
+  if (lane == 0) sbase[sl] = narrow ? lo : -1;
+  if (narrow)
+    for (long long q = b + lane; q < e; q += 64) scol16[q] = (uint16_t)(scol[q] - lo);
+}
+
+static int build_narrow_columns(Problem& P) {
+  if (P.d_sbase) return 0;
+  const int nsl = (P.n + 63) / 64;
+  MF_HIP(hipMalloc(&P.d_scol16, (size_t)std::max<long long>(P.nnz_sell, 1) * sizeof(uint16_t)));
+  MF_HIP(hipMalloc(&P.d_sbase, (size_t)std::max(nsl, 1) * sizeof(int)));
+  hipLaunchKernelGGL(k_narrow_columns, dim3((nsl + 3) / 4), dim3(256), 0, stream(), nsl, P.d_sl_ptr,
+                     P.d_scol, P.d_scol16, P.d_sbase);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+
 // operator benchmark variants: prec 0 FP64 values and vector, 1 FP32
 // values and FP64 vector, 2 FP32 values and vector; ch entries in flight
 template <typename VT, typename XT>
@@ -858,13 +898,20 @@ static void launch_spmv_bench(const Problem& P, const SolveArgs& A, const VT* va
 }
 
 int bench_spmv_variant(Problem& P, int prec, int ch, int reps, double* ms) {
+  const bool wide = prec >= 10;  // the 32-bit columns everywhere
+  if (wide) prec -= 10;
   if (!P.sell || prec < 0 || prec > 2 || reps < 1) {
-    set_error("mf_bench_spmv_ex: SELL problem, prec 0/1/2 and reps >= 1 required");
+    set_error("mf_bench_spmv_ex: SELL problem, prec 0/1/2 (+10) and reps >= 1 required");
     return -1;
   }
+  if (build_narrow_columns(P)) return -2;
   if (prec > 0 && round_mixed(P, 0)) return -2;
   SolveArgs A;
   fill_solve_args(P, A, 0, 0);
+  if (wide) {
+    A.scol16 = nullptr;
+    A.sbase = nullptr;
+  }
   void *x = nullptr, *y = nullptr;
   MF_HIP(hipMalloc(&x, (size_t)P.n * sizeof(double)));
   MF_HIP(hipMalloc(&y, (size_t)P.n * sizeof(double)));
@@ -1030,6 +1077,9 @@ static int gmres_core(Problem& P, int precon, int krylbas, int max_iter, double 
   } else if (precon && (!P.d_dinv || !P.dinv_valid)) {
     if (launch_fill_jacobi(P)) return -2;
   }
+  // 16-bit columns of the operator (made once; a no-allocation solve uses
+  // them only if an earlier solve made them)
+  if (P.sell && !noalloc && build_narrow_columns(P)) return -2;
   SolveArgs A;
   fill_solve_args(P, A, (ilu || gs || ext) ? 0 : precon, m);
   A.b = d_b;
@@ -1248,9 +1298,10 @@ int gmres_solve(Problem& P, int precon, int krylbas, int max_iter, double tol, d
 
 // FP32 copies for mixed-precision inner solves: matrix values, and the
 // Jacobi inverses (precon 1) or the ILU(0) factors (precon 2), refreshed
-// when the matrix changed since they were made
+// when the matrix changed since they were made; the 16-bit columns once
 static int round_mixed(Problem& P, int precon) {
   hipStream_t s = stream();
+  if (build_narrow_columns(P)) return -2;
   if (precon == 1 && (!P.d_dinv || !P.dinv_valid)) {
     if (launch_fill_jacobi(P)) return -2;
     P.val32_valid = false;
@@ -1691,7 +1742,10 @@ __global__ void __launch_bounds__(SB) kb_update(BatchArgs B, int j) {
   const int p = B.blk_prob[blockIdx.x], r = B.blk_rank[blockIdx.x], G = B.nblk[p];
   const SolveArgs A = B.args[p];
   if (A.stat->stop) return;
-  update_any(A, j, r, G, r);
+  __shared__ double cs[MAX_NV];
+  if ((int)threadIdx.x <= j) cs[threadIdx.x] = Coef(A)[threadIdx.x];
+  __syncthreads();
+  update_any(A, j, r, G, r, cs);
 }
 
 __global__ void __launch_bounds__(SB) kb_fin_update(BatchArgs B, int j) {
@@ -1942,6 +1996,7 @@ int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double to
       return -1;
     }
   if (ensure_workspace(*P, B.m)) return -2;
+  if (build_narrow_columns(*P)) return -2;
   if (zero_padding(*P, B.m, mixed ? 1 : 0)) return -2;
   if (precon == 2 && !P->ilu && ilu_build(*P, 1)) return -2;
   BatchMember M;
@@ -2220,7 +2275,7 @@ static std::tuple<int, int, int> tile_key(const Batch& B, int use_graph) {
 }
 
 // a continuously batched server changes shape every few cycles; a choice
-// cached within one half-octave of rows and one octave of members (same
+// cached within two half-octaves of rows and two octaves of members (same
 // launch mode) is taken instead of new trials
 static std::map<std::tuple<int, int, int>, TileChoice>::iterator tile_near(
     const std::tuple<int, int, int>& key) {
@@ -2231,7 +2286,7 @@ static std::map<std::tuple<int, int, int>, TileChoice>::iterator tile_near(
     if (std::get<2>(k) != std::get<2>(key)) continue;
     const int dh = std::abs(std::get<0>(k) - std::get<0>(key));
     const int dm = std::abs(std::get<1>(k) - std::get<1>(key));
-    if (dh > 1 || dm > 1) continue;
+    if (dh > 2 || dm > 2) continue;
     if (dh + dm < bd) {
       bd = dh + dm;
       best = it;
@@ -2323,18 +2378,21 @@ static int batch_read_state(Batch& B, hipStream_t s) {
   return 0;
 }
 
-int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
+static hipStream_t batch_stream(const Batch& B) { return B.own ? B.own : stream(); }
+
+// Before cycles: preconditioner data of members re-assembled since they
+// joined, the tile choice, the tables; then the batch's own stream (if it
+// has one) waits for the work enqueued on the library stream (assembly of
+// new members, their preconditioner data).  act: members still active.
+static int batch_prologue(Batch& B, int max_cycles, int use_graph, int& act) {
   const int np = (int)B.mem.size();
   if (B.m < 1 || B.m >= MAX_NV) {
     set_error("batch: number of Krylov vectors must be in [1,63]");
     return -1;
   }
-  hipStream_t s = stream();
-  // members re-assembled since they joined get their preconditioner data
-  // remade in place
   for (auto& M : B.mem)
     if (prepare_member(M)) return -2;
-  int act = 0;
+  act = 0;
   for (int k = 0; k < np; k++) act += member_active(B.h_mix[k], B.h_stat[k]);
   if (B.retile) {
     B.retile = false;
@@ -2353,9 +2411,9 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
         for (const auto& ms : B.h_mix) fresh = fresh || (ms.mixed && ms.phase == 1);
         if (fresh) {
           if (B.dirty && batch_build(B)) return -2;
-          batch_refine_steps(B, s);
+          batch_refine_steps(B, stream());
           MF_HIP(hipGetLastError());
-          if (batch_read_state(B, s)) return -2;
+          if (batch_read_state(B, stream())) return -2;
         }
         TileChoice tc{};
         const int rc = batch_tune(B, use_graph, tc);
@@ -2372,14 +2430,36 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
     const int rc = batch_build(B);
     if (rc) return rc;
   }
+  if (B.own) {
+    MF_HIP(hipEventRecord(B.ev_ready, stream()));
+    MF_HIP(hipStreamWaitEvent(B.own, B.ev_ready, 0));
+  }
+  return 0;
+}
+
+static int batch_launch_cycle(Batch& B, int use_graph, hipStream_t s) {
+  if (use_graph) {
+    if (!B.exec && batch_capture(B, s)) return -2;
+    MF_HIP(hipGraphLaunch(B.exec, s));
+  } else {
+    batch_enqueue_cycle(B, s);
+  }
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
+  if (B.pending) {
+    set_error("batch: a cycle is in flight (mf_batch_wait first)");
+    return -1;
+  }
+  const int np = (int)B.mem.size();
+  int act = 0;
+  int rc = batch_prologue(B, max_cycles, use_graph, act);
+  if (rc) return rc;
+  const hipStream_t s = batch_stream(B);
   for (int c = 0; c < max_cycles && act > 0; c++) {
-    if (use_graph) {
-      if (!B.exec && batch_capture(B, s)) return -2;
-      MF_HIP(hipGraphLaunch(B.exec, s));
-    } else {
-      batch_enqueue_cycle(B, s);
-    }
-    MF_HIP(hipGetLastError());
+    if ((rc = batch_launch_cycle(B, use_graph, s))) return rc;
     if (batch_read_state(B, s)) return -2;
     B.cycles++;
     act = 0;
@@ -2389,8 +2469,54 @@ int batch_run(Batch& B, int max_cycles, int use_graph, int* active) {
   return 0;
 }
 
+// one cycle enqueued with the copy of the state to the host; batch_wait
+// completes it
+int batch_run_async(Batch& B, int use_graph) {
+  if (B.pending) {
+    set_error("batch: a cycle is in flight (mf_batch_wait first)");
+    return -1;
+  }
+  const int np = (int)B.mem.size();
+  int act = 0;
+  int rc = batch_prologue(B, 1, use_graph, act);
+  if (rc || act == 0) return rc;
+  const hipStream_t s = batch_stream(B);
+  if ((rc = batch_launch_cycle(B, use_graph, s))) return rc;
+  MF_HIP(hipMemcpyAsync(B.h_pinned, B.d_stat, np * sizeof(GmresStat), hipMemcpyDeviceToHost, s));
+  if (B.has_mixed)
+    MF_HIP(hipMemcpyAsync(B.h_mix_pinned, B.d_mix, np * sizeof(MixedState), hipMemcpyDeviceToHost, s));
+  MF_HIP(hipEventRecord(B.ev_done, s));
+  B.pending = true;
+  return 0;
+}
+
+int batch_wait(Batch& B, int* active) {
+  const int np = (int)B.mem.size();
+  if (B.pending) {
+    MF_HIP(hipEventSynchronize(B.ev_done));
+    std::copy(B.h_pinned, B.h_pinned + np, B.h_stat.begin());
+    if (B.has_mixed) std::copy(B.h_mix_pinned, B.h_mix_pinned + np, B.h_mix.begin());
+    B.cycles++;
+    B.pending = false;
+  }
+  int act = 0;
+  for (int k = 0; k < np; k++) act += member_active(B.h_mix[k], B.h_stat[k]);
+  if (active) *active = act;
+  return 0;
+}
+
+int batch_own_stream(Batch& B) {
+  if (B.own) return 0;
+  MF_HIP(hipStreamCreateWithFlags(&B.own, hipStreamNonBlocking));
+  MF_HIP(hipEventCreateWithFlags(&B.ev_ready, hipEventDisableTiming));
+  MF_HIP(hipEventCreateWithFlags(&B.ev_done, hipEventDisableTiming));
+  return 0;
+}
+
 void batch_free(Batch& B) {
+  if (B.own) (void)hipStreamSynchronize(B.own);
   (void)hipStreamSynchronize(stream());
+  B.pending = false;
   batch_drop_tables(B);
   if (B.d_blk_prob) (void)hipFree(B.d_blk_prob);
   if (B.d_blk_rank) (void)hipFree(B.d_blk_rank);
@@ -2399,6 +2525,11 @@ void batch_free(Batch& B) {
   B.mem.clear();
   B.h_stat.clear();
   B.h_mix.clear();
+  if (B.ev_ready) (void)hipEventDestroy(B.ev_ready);
+  if (B.ev_done) (void)hipEventDestroy(B.ev_done);
+  if (B.own) (void)hipStreamDestroy(B.own);
+  B.ev_ready = B.ev_done = nullptr;
+  B.own = nullptr;
 }
 
 }  // namespace mf

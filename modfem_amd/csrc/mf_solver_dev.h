@@ -63,6 +63,10 @@ struct SolveArgs {
   const double* val;    // CRS values (tile path) or SELL-64 values
   const float* val32;   // SELL-64 values rounded to FP32 (mixed-precision inner solves)
   int op32;             // 1: the operator kernels read val32
+  // 16-bit columns relative to a per-slice base where the slice's columns
+  // span less than 65536 rows (null, or sbase[sl] < 0: use scol)
+  const uint16_t* scol16;
+  const int* sbase;
   const long long* sl_ptr;  // SELL-64 path
   const int* scol;
   const double* dinv;   // null: no preconditioner
@@ -173,15 +177,26 @@ __device__ __forceinline__ double sell_row(const SolveArgs& A, const VT* val, co
                                            int sl, int lane) {
   const long long base = A.sl_ptr[sl] + lane;
   const int len = (int)((A.sl_ptr[sl + 1] - A.sl_ptr[sl]) >> 6);
+  // 16-bit columns where the slice has them (a wave-uniform test)
+  const int cb = A.sbase ? A.sbase[sl] : -1;
   double s = 0.0;
   for (int k0 = 0; k0 < len; k0 += CH) {
     int c[CH];
     VT v[CH];
+    if (cb >= 0) {
 #pragma unroll
-    for (int k = 0; k < CH; k++) {
-      const long long q = base + 64LL * min(k0 + k, len - 1);
-      c[k] = A.scol[q];
-      v[k] = val[q];
+      for (int k = 0; k < CH; k++) {
+        const long long q = base + 64LL * min(k0 + k, len - 1);
+        c[k] = cb + (int)A.scol16[q];
+        v[k] = val[q];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < CH; k++) {
+        const long long q = base + 64LL * min(k0 + k, len - 1);
+        c[k] = A.scol[q];
+        v[k] = val[q];
+      }
     }
     XT xv[CH];
 #pragma unroll
