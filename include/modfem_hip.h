@@ -290,6 +290,16 @@ int mf_batch_add(int bid, int pid, int precon, int max_iter, double tol);
    is at most tol times its value at x = 0. */
 int mf_batch_add_ex(int bid, int pid, int precon, int max_iter, double tol, int inner_precision);
 int mf_batch_remove(int bid, int pid);
+/* a DG problem (one degree for every element, assembled) as a batch
+   member: GMRES on the block-Jacobi-preconditioned system Dinv A x =
+   Dinv b (Dinv: inverses of the diagonal blocks), which is the reference's
+   left-preconditioned block-Jacobi GMRES of DG systems, held as a SELL-64
+   matrix formed on the device; inner_precision 64 or 32 as mf_batch_add_ex.
+   The solution is read with mf_dg_get_solution. */
+int mf_batch_add_dg(int bid, int did, int max_iter, double tol, int inner_precision);
+int mf_batch_remove_dg(int bid, int did);
+int mf_batch_status_dg(int bid, int did, int* done, int* converged, int* iters,
+                       double* res_rel, int* refinements);
 /* integrate + assemble every member (mf_integrate_assemble of each) */
 int mf_batch_integrate_assemble(int bid, int precision);
 /* run up to max_cycles restart cycles; *active = members still iterating */

@@ -115,6 +115,9 @@ def _declare(L):
         "mf_batch_add": (I, [I, I, I, I, ctypes.c_double]),
         "mf_batch_add_ex": (I, [I, I, I, I, ctypes.c_double, I]),
         "mf_batch_set_stream": (I, [I]),
+        "mf_batch_add_dg": (I, [I, I, I, ctypes.c_double, I]),
+        "mf_batch_remove_dg": (I, [I, I]),
+        "mf_batch_status_dg": (I, [I, I, V, V, V, V, V]),
         "mf_batch_run_async": (I, [I, I]),
         "mf_batch_wait": (I, [I, V]),
         "mf_batch_status_ex": (I, [I, I, V, V, V, V, V]),
@@ -265,12 +268,18 @@ class Batch:
     def add(self, problem, precon=1, max_iter=1000, tol=1e-8, inner_precision=64):
         """start a member; inner_precision=32 makes it a mixed-precision
         member (FP64 refinement around FP32 inner solves); precon 2 (ILU(0))
-        needs problem.set_ilu_order(1)"""
+        needs problem.set_ilu_order(1).  A DgProblem joins preconditioned by
+        block Jacobi (precon is ignored)."""
+        if isinstance(problem, DgProblem):
+            return check(lib().mf_batch_add_dg(self.h, problem.h, max_iter, tol, inner_precision))
         return check(lib().mf_batch_add_ex(self.h, problem.h, precon, max_iter, tol,
                                            inner_precision))
 
     def remove(self, problem):
-        check(lib().mf_batch_remove(self.h, problem.h))
+        if isinstance(problem, DgProblem):
+            check(lib().mf_batch_remove_dg(self.h, problem.h))
+        else:
+            check(lib().mf_batch_remove(self.h, problem.h))
 
     def integrate_assemble(self, precision=64):
         check(lib().mf_batch_integrate_assemble(self.h, precision))
@@ -294,8 +303,9 @@ class Batch:
     def status(self, problem):
         d, c, it, rr, nr = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double(),
                             ctypes.c_int())
-        check(lib().mf_batch_status_ex(self.h, problem.h, ctypes.byref(d), ctypes.byref(c),
-                                       ctypes.byref(it), ctypes.byref(rr), ctypes.byref(nr)))
+        fn = lib().mf_batch_status_dg if isinstance(problem, DgProblem) else lib().mf_batch_status_ex
+        check(fn(self.h, problem.h, ctypes.byref(d), ctypes.byref(c), ctypes.byref(it),
+                 ctypes.byref(rr), ctypes.byref(nr)))
         return dict(done=bool(d.value), converged=bool(c.value), iters=it.value, res_rel=rr.value,
                     refinements=nr.value)
 

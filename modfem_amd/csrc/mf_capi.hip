@@ -1247,6 +1247,22 @@ int mf_batch_add_ex(int bid, int pid, int precon, int max_iter, double tol, int 
   return batch_add(*B, P, pid, precon, max_iter, tol, inner_precision == 32);
 }
 
+int mf_batch_add_dg(int bid, int did, int max_iter, double tol, int inner_precision) {
+  Batch* B = get_batch(bid);
+  if (!B) return -3;
+  if (!batch_idle(B)) return -1;
+  if (inner_precision != 64 && inner_precision != 32) {
+    set_error("mf_batch_add_dg: inner_precision must be 64 or 32");
+    return -1;
+  }
+  Problem* P = dg_batch_problem(did);
+  if (!P) return -2;
+  // members are told apart by handle: a DG problem joins as -did
+  return batch_add(*B, P, -did, 0, max_iter, tol, inner_precision == 32);
+}
+
+int mf_batch_remove_dg(int bid, int did) { return mf_batch_remove(bid, -did); }
+
 int mf_batch_remove(int bid, int pid) {
   Batch* B = get_batch(bid);
   if (!B) return -3;
@@ -1356,6 +1372,11 @@ int mf_batch_status_ex(int bid, int pid, int* done, int* converged, int* iters, 
 
 int mf_batch_status(int bid, int pid, int* done, int* converged, int* iters, double* res_rel) {
   return mf_batch_status_ex(bid, pid, done, converged, iters, res_rel, nullptr);
+}
+
+int mf_batch_status_dg(int bid, int did, int* done, int* converged, int* iters, double* res_rel,
+                       int* refinements) {
+  return mf_batch_status_ex(bid, -did, done, converged, iters, res_rel, refinements);
 }
 
 int mf_batch_free(int bid) {

@@ -307,6 +307,9 @@ int batch_add(Batch& B, Problem* P, int pid, int precon, int max_iter, double to
 int batch_remove(Batch& B, int pid);
 int batch_run(Batch& B, int max_cycles, int use_graph, int* active);
 int batch_run_async(Batch& B, int use_graph);
+// the batch form of a DG problem (block-Jacobi-preconditioned SELL system),
+// made current; null and an error if the problem cannot join a batch
+Problem* dg_batch_problem(int did);
 int batch_wait(Batch& B, int* active);
 int batch_own_stream(Batch& B);
 void batch_free(Batch& B);

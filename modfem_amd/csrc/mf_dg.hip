@@ -486,6 +486,11 @@ struct DgProblem {
   std::vector<int> ilu_ftask, ilu_btask;
   int2* d_ilu_ftasks = nullptr;
   int2* d_ilu_btasks = nullptr;
+  // batch member (uniform degree): the block-Jacobi-preconditioned system
+  // M = Dinv A, b' = Dinv b as a SELL-64 problem, rows in DOF order
+  Problem* bp = nullptr;
+  bool bp_valid = false;
+  bool sol_batch = false;     // the last solution came from a batch
 };
 
 namespace {
@@ -539,8 +544,11 @@ int launch_elem(DgProblem& D, int e0, int count, double* d_stiff, double* d_rhs)
 void free_dg_solver(DgProblem& D);
 int dg_ilu_solve(DgProblem& D, double* v, const int* stop);
 
+void free_dg_batch(DgProblem& D);
+
 void free_dg(DgProblem& D) {
   free_dg_solver(D);
+  free_dg_batch(D);
   void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs, D.d_fnb, D.d_fcode,
                   D.d_fhf, D.d_fval, D.d_qt, D.d_qq, D.d_blk, D.d_brhs, D.d_dinv, D.d_slots,
                   D.d_gs_elems, D.d_work, D.d_grp, D.d_deg, D.d_xoff, D.d_soff, D.d_voff,
@@ -2028,6 +2036,7 @@ int fill_block_jacobi(DgProblem& D) {
     return -1;
   }
   D.dinv_valid = true;
+  D.bp_valid = false;  // the batch form reads the inverses
   return 0;
 }
 
@@ -2350,6 +2359,151 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
 
 }  // namespace
 
+namespace mf {
+namespace {
+
+// ---------------------------------------------------------------------
+// DG problems as batch members.  Left-preconditioned GMRES with block
+// Jacobi (the reference's precon of DG systems, lss_mkb_core.c:1861-2097 for
+// block storage) is plain GMRES on M x = b' with M = Dinv A and b' = Dinv b:
+// the block rows of the system scaled by the inverses of their diagonal
+// blocks.  M is stored as a scalar SELL-64 matrix in DOF order (row
+// (e, i): the own block's columns, then those of the neighbour across each
+// interior side in side order), so a DG problem joins a batch like any
+// other SELL problem, preconditioner folded in; its measure ||b' - M x|| is
+// the block-Jacobi-preconditioned residual of the DG solve.
+// ---------------------------------------------------------------------
+// M's row (e, i) and b'_(e, i), one thread per row (uniform degree)
+__global__ void k_dg_to_sell(long long n, int ns, const int* fnb, const double* blk,
+                             const double* dinv, const double* b, const long long* sl_ptr,
+                             double* val, double* bp) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const long long e = r / ns;
+  const int i = (int)(r - e * ns);
+  const size_t nn = (size_t)ns * ns;
+  const double* Di = dinv + e * nn;  // column-major: (i, k) at k ns + i
+  const long long base = sl_ptr[r >> 6] + (r & 63);
+  int q = 0;
+  for (int slot = 0; slot < 6; slot++) {
+    if (slot > 0 && fnb[5 * e + slot - 1] < 0) continue;
+    const double* Bk = blk + (e * 6 + slot) * nn;
+    for (int j = 0; j < ns; j++, q++) {
+      double m = 0.0;
+      for (int k = 0; k < ns; k++) m += Di[(size_t)k * ns + i] * Bk[(size_t)j * ns + k];
+      val[base + 64LL * q] = m;
+    }
+  }
+  double t = 0.0;
+  for (int k = 0; k < ns; k++) t += Di[(size_t)k * ns + i] * b[e * ns + k];
+  bp[r] = t;
+}
+
+// structure of M (host): slice lengths and columns; padding entries point
+// at the own row with value zero
+int dg_batch_structure(DgProblem& D) {
+  const long long n = D.ndof;
+  const int ns = D.ns;
+  const int nsl = (int)((n + 63) / 64);
+  std::vector<long long> sl_ptr(nsl + 1, 0);
+  auto row_len = [&](long long r) {
+    const long long e = r / ns;
+    int k = 1;
+    for (int sd = 0; sd < 5; sd++) k += D.h_fnb[5 * e + sd] >= 0;
+    return k * ns;
+  };
+  long long nnz = 0;
+  for (int sl = 0; sl < nsl; sl++) {
+    int len = 0;
+    for (long long r = 64LL * sl; r < std::min(n, 64LL * sl + 64); r++) {
+      len = std::max(len, row_len(r));
+      nnz += row_len(r);
+    }
+    sl_ptr[sl + 1] = sl_ptr[sl] + 64LL * len;
+  }
+  std::vector<int> col(sl_ptr[nsl]);
+  for (int sl = 0; sl < nsl; sl++) {
+    const int len = (int)((sl_ptr[sl + 1] - sl_ptr[sl]) / 64);
+    for (int lane = 0; lane < 64; lane++) {
+      const long long r = 64LL * sl + lane;
+      const long long own = r < n ? r : n - 1;
+      for (int q = 0; q < len; q++) col[sl_ptr[sl] + 64LL * q + lane] = (int)own;
+      if (r >= n) continue;
+      const long long e = r / ns;
+      int q = 0;
+      for (int slot = 0; slot < 6; slot++) {
+        const long long nb = slot == 0 ? e : D.h_fnb[5 * e + slot - 1];
+        if (nb < 0) continue;
+        for (int j = 0; j < ns; j++, q++) col[sl_ptr[sl] + 64LL * q + lane] = (int)(nb * ns + j);
+      }
+    }
+  }
+  Problem* P = new Problem();
+  D.bp = P;
+  P->n = (int)n;
+  P->el_type = kPrism;
+  P->sell = true;
+  P->nnz = nnz;
+  P->nnz_sell = sl_ptr[nsl];
+  MF_HIP(hipMalloc(&P->d_sl_ptr, sl_ptr.size() * sizeof(long long)));
+  MF_HIP(hipMemcpy(P->d_sl_ptr, sl_ptr.data(), sl_ptr.size() * sizeof(long long),
+                   hipMemcpyHostToDevice));
+  MF_HIP(hipMalloc(&P->d_scol, col.size() * sizeof(int)));
+  MF_HIP(hipMemcpy(P->d_scol, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice));
+  MF_HIP(hipMalloc(&P->d_val, (size_t)P->nnz_sell * sizeof(double)));
+  MF_HIP(hipMemset(P->d_val, 0, (size_t)P->nnz_sell * sizeof(double)));  // padding stays zero
+  MF_HIP(hipMalloc(&P->d_rhs, (size_t)n * sizeof(double)));
+  return 0;
+}
+
+void free_dg_batch(DgProblem& D) {
+  if (!D.bp) return;
+  Problem& P = *D.bp;
+  void* ptrs[] = {P.d_sl_ptr, P.d_scol, P.d_val, P.d_rhs, P.d_val32, P.d_dinv, P.d_dinv32,
+                  P.d_scol16, P.d_sbase, P.d_V, P.d_H, P.d_partials, P.d_stat, P.d_x,
+                  P.d_work, P.d_work2};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (P.h_stat) (void)hipHostFree(P.h_stat);
+  delete D.bp;
+  D.bp = nullptr;
+}
+
+}  // namespace
+
+Problem* dg_batch_problem(int did) {
+  DgProblem* D = get_dg(did);
+  if (!D) return nullptr;
+  if (D->mixed) {
+    set_error("batch: DG members need one degree for all elements");
+    return nullptr;
+  }
+  if (!D->assembled) {
+    set_error("batch: the DG system is not assembled");
+    return nullptr;
+  }
+  if (!D->dinv_valid && fill_block_jacobi(*D)) return nullptr;
+  if (!D->bp && dg_batch_structure(*D)) return nullptr;
+  if (!D->bp_valid) {
+    Problem& P = *D->bp;
+    const long long n = D->ndof;
+    hipLaunchKernelGGL(k_dg_to_sell, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream(), n,
+                       D->ns, D->d_fnb, D->d_blk, D->d_dinv, D->d_brhs, P.d_sl_ptr, P.d_val,
+                       P.d_rhs);
+    if (hipGetLastError() != hipSuccess) {
+      set_error("batch: the DG member's matrix could not be formed");
+      return nullptr;
+    }
+    P.dinv_valid = false;
+    P.val32_valid = false;
+    D->bp_valid = true;
+  }
+  D->sol_batch = true;
+  return D->bp;
+}
+
+}  // namespace mf
+
 extern "C" {
 
 int mf_dg_create(int mesh, int pdeg) {
@@ -2432,7 +2586,7 @@ int mf_dg_set_coefficients(int did, const int* ind, int coef_model, const double
   M.q = ind[21] ? 2 : (ind[19] ? 1 : 0);
   M.s = ind[22] != 0;
   D->mask = M;
-  D->assembled = D->dinv_valid = D->ilu_valid = false;
+  D->assembled = D->dinv_valid = D->ilu_valid = D->bp_valid = false;
   for (int k = 0; k < NCOEF; k++) D->cc[k] = coef_const ? coef_const[k] : 0.0;
   D->coef_model = coef_model;
   D->src_model = src_model;
@@ -2532,7 +2686,7 @@ int mf_dg_set_penalty(int did, double penalty_const) {
     return -1;
   }
   D->penalty_const = penalty_const;
-  D->assembled = D->dinv_valid = D->ilu_valid = false;
+  D->assembled = D->dinv_valid = D->ilu_valid = D->bp_valid = false;
   return 0;
 }
 
@@ -2545,7 +2699,7 @@ int mf_dg_set_boundary_values(int did, const double* val) {
   else
     MF_HIP(hipMemsetAsync(D->d_fval, 0, nf * sizeof(double), stream()));
   MF_HIP(hipStreamSynchronize(stream()));
-  D->assembled = D->dinv_valid = D->ilu_valid = false;
+  D->assembled = D->dinv_valid = D->ilu_valid = D->bp_valid = false;
   return 0;
 }
 
@@ -2574,7 +2728,7 @@ int mf_dg_assemble(int did) {
   if (rc) return rc;
   MF_HIP(hipStreamSynchronize(stream()));
   D->assembled = true;
-  D->dinv_valid = D->ilu_valid = false;
+  D->dinv_valid = D->ilu_valid = D->bp_valid = false;
   return 0;
 }
 
@@ -2714,6 +2868,7 @@ int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, doub
   }
   D->op->precon = precon;
   D->P->d_rhs = D->d_brhs;
+  D->sol_batch = false;
   if (!D->P->d_x) MF_HIP(hipMalloc(&D->P->d_x, (size_t)D->P->n * sizeof(double)));
   return gmres_solve(*D->P, precon, krylbas, max_iter, tol, 0.0, 1, x, iters, res_rel, resid);
 }
@@ -2734,11 +2889,12 @@ int mf_dg_set_gs(int did, int nsweeps, int* ncolors) {
 int mf_dg_get_solution(int did, double* x) {
   DgProblem* D = get_dg(did);
   if (!D) return -3;
-  if (!D->P || !D->P->d_x) {
+  const Problem* S = D->sol_batch ? D->bp : D->P;  // the last solve's workspace
+  if (!S || !S->d_x) {
     set_error("mf_dg_get_solution: no solve has run");
     return -1;
   }
-  MF_HIP(hipMemcpyAsync(x, D->P->d_x, (size_t)D->P->n * sizeof(double), hipMemcpyDeviceToHost,
+  MF_HIP(hipMemcpyAsync(x, S->d_x, (size_t)S->n * sizeof(double), hipMemcpyDeviceToHost,
                         stream()));
   MF_HIP(hipStreamSynchronize(stream()));
   return 0;
@@ -2790,6 +2946,10 @@ int mf_dg_bench(int did, int reps, double* out) {
 int mf_dg_free(int did) {
   DgProblem* D = get_dg(did);
   if (!D) return -3;
+  if (D->bp && D->bp->batch_refs > 0) {
+    set_error("mf_dg_free: the problem is a member of a batch (mf_batch_remove_dg first)");
+    return -1;
+  }
   free_dg(*D);
   g_dg.erase(did);
   return 0;

@@ -762,6 +762,10 @@ int build_tiles(Problem& P) {
 }
 
 int launch_fill_jacobi(Problem& P) {
+  if (P.sell && !P.d_diag_k) {
+    set_error("Jacobi: this system has no diagonal positions (no point Jacobi)");
+    return -1;
+  }
   if (!P.d_dinv) MF_HIP(hipMalloc(&P.d_dinv, (size_t)P.n * sizeof(double)));
   P.dinv_valid = true;
   if (P.sell)
@@ -1962,7 +1966,10 @@ static GmresStat member_init(const BatchMember& M, int m) {
 // also the refinement measure of mixed members), ILU(0) factors, FP32 copies
 static int prepare_member(const BatchMember& M) {
   Problem& P = *M.P;
-  if ((M.precon == 1 || M.mixed) && (!P.d_dinv || !P.dinv_valid)) {
+  // the Jacobi inverses: the preconditioner of precon 1, and the refinement
+  // measure of mixed-precision ILU(0) members (precon 0 measures the plain
+  // residual and needs none)
+  if ((M.precon == 1 || (M.mixed && M.precon == 2)) && (!P.d_dinv || !P.dinv_valid)) {
     if (launch_fill_jacobi(P)) return -2;
     P.val32_valid = false;  // the FP32 inverses follow the FP64 ones
   }
