@@ -201,9 +201,9 @@ def measured_traffic(n, krylbas):
 # many-problem workloads: a batch of independent problems (one pipeline versus
 # a loop of plain solves) and a served request stream (continuous batching)
 # ---------------------------------------------------------------------------
-def make_box_problem(mf, n_el, rng, seed):
+def make_box_problem(mf, n_el, rng, seed, sigma=0.5):
     """std-linear prism Poisson on a box of about n_el prisms with random
-    aspect ratios in [1,4], kappa lognormal(0, 0.5) and source U(-1,1) per
+    aspect ratios in [1,4], kappa lognormal(0, sigma) and source U(-1,1) per
     element, Dirichlet penalty on the boundary; two-pass deterministic
     assembly (element rows, then row gathers in element order: the fastest
     measured below a few million elements)"""
@@ -212,7 +212,7 @@ def make_box_problem(mf, n_el, rng, seed):
     nx, ny, nz = (max(1, int(round(a * sc))) for a in asp)
     mesh = mf.Mesh.box_prism(nx, ny, nz, lx=asp[0], ly=asp[1], lz=asp[2], seed=seed, jitter=0.2)
     p = mf.Problem.from_mesh(mesh)
-    ce = np.concatenate([np.exp(rng.normal(0.0, 0.5, mesh.nel)), rng.uniform(-1.0, 1.0, mesh.nel)])
+    ce = np.concatenate([np.exp(rng.normal(0.0, sigma, mesh.nel)), rng.uniform(-1.0, 1.0, mesh.nel)])
     p.set_coefficients(mf.IND_DIFF, mf.COEF_KAPPA_S, coef_el=ce)
     p.set_assembly_mode(2)
     mesh.free()
@@ -339,36 +339,77 @@ def bench_batch(args):
     return out
 
 
+def make_dg_request(mf, n_el, pdeg, rng, seed):
+    """DG prism Poisson of degree pdeg on a box of about n_el prisms: random
+    aspect ratios in [1,4], kappa lognormal(0, 1) and source U(-1,1) per
+    element, interior-penalty flux, Dirichlet boundary"""
+    asp = rng.uniform(1.0, 4.0, 3)
+    sc = (n_el / 2.0 / asp.prod()) ** (1.0 / 3.0)
+    nx, ny, nz = (max(1, int(round(a * sc))) for a in asp)
+    mesh = mf.Mesh.box_prism(nx, ny, nz, lx=asp[0], ly=asp[1], lz=asp[2], seed=seed, jitter=0.2)
+    dg = mf.DgProblem(mesh, pdeg)
+    ce = np.concatenate([np.exp(rng.normal(0.0, 1.0, mesh.nel)), rng.uniform(-1.0, 1.0, mesh.nel)])
+    dg.set_coefficients(mf.IND_DIFF, mf.COEF_KAPPA_S, coef_el=ce)
+    mesh.free()
+    return dg
+
+
+def is_dg(p):
+    return type(p).__name__ == "DgProblem"
+
+
 def footprint(p, m, mixed=False):
     """device bytes a resident problem holds: SELL values+columns, element
     tables, vectors and the Krylov basis (and the FP32 values, inverses and
     refinement vectors of a mixed-precision member)"""
+    if is_dg(p):  # blocks, their inverses, the SELL form (+ FP32 copy), vectors
+        nd, nn = p.nel * p.ns, p.ns * p.ns
+        sell = 6.0 * nn * p.nel
+        return (8.0 * 7 * nn * p.nel + (14.0 if not mixed else 20.0) * sell
+                + 8.0 * (m + 10) * nd)
     extra = 4.0 * p.nnz * 1.04 + 20.0 * p.n if mixed else 0.0
     return 12.0 * p.nnz * 1.04 + 8.0 * (m + 8) * p.n + 120.0 * p.nel + extra
 
 
 def bench_serve(args):
-    """Poisson-arrival stream of std-linear Poisson requests (prisms,
-    log-uniform sizes): admission by device-memory footprint and slot count,
-    continuous batching between restart cycles, latency p50/p99 and solves/s;
-    versus serving the same stream one plain FP64 solve at a time (FIFO,
-    with each request's measured solve time).  The server runs FP64 members
+    """Poisson-arrival stream of std-linear and DG p=1/2 Poisson requests
+    (prisms, log-uniform sizes): admission by device-memory footprint and
+    slot count, continuous batching between restart cycles, latency p50/p99
+    and solves/s; versus serving the same stream one plain FP64 solve at a
+    time (FIFO, with each request's measured solve time: std GMRES(30) with
+    Jacobi, DG GMRES(30) with block Jacobi).  The server runs FP64 members
     and mixed-precision members (FP32 inner solves, FP64 refinement to the
-    same FP64 measure, checked on the host after the run)"""
+    same FP64 measure; the std solutions checked on the host after the run)"""
     import modfem_amd as mf
     rng = np.random.default_rng(args.seed)
+    drng = np.random.default_rng(args.seed + 7)
     m, tol, mi = 30, 1e-8, 20000
-    probs = [make_box_problem(mf, float(np.exp(rng.uniform(np.log(args.el_min), np.log(args.el_max)))),
-                              rng, args.seed * 1000 + k) for k in range(args.count)]
-    # sequential plain solves: service time of every request alone
-    for p in probs[:3]:
-        p.integrate_assemble(64)
+    probs = []
+    for k in range(args.count):
+        if drng.random() < args.dg_frac:
+            n_el = float(np.exp(drng.uniform(np.log(args.dg_el_min), np.log(args.dg_el_max))))
+            probs.append(make_dg_request(mf, n_el, 101 if k % 2 else 202, drng, args.seed * 1000 + k))
+        else:
+            probs.append(make_box_problem(mf, float(np.exp(rng.uniform(np.log(args.el_min),
+                                                                       np.log(args.el_max)))),
+                                          rng, args.seed * 1000 + k, sigma=1.0))
+
+    def assemble(p, wait=True):
+        if is_dg(p):
+            p.assemble()
+        else:
+            p.integrate_assemble(64, wait=wait)
+
+    def plain_solve(p):
+        assemble(p)
         p.gmres(precon=1, krylbas=m, max_iter=mi, tol=tol, fetch=False)
+    # sequential plain solves: service time of every request alone
+    for p in probs[:3] + [q for q in probs if is_dg(q)][:2]:
+        plain_solve(p)
     service = []
     for p in probs:
         t0 = time.perf_counter()
-        p.integrate_assemble(64)
-        p.gmres(precon=1, krylbas=m, max_iter=mi, tol=tol, fetch=False)
+        plain_solve(p)
         service.append(time.perf_counter() - t0)
     results = []
     budget = args.hbm_gb * 1e9
@@ -385,7 +426,7 @@ def bench_serve(args):
             while (nxt < len(probs) and arr[nxt] <= now and len(live) < args.slots and
                    used + footprint(probs[nxt], m, mixed) <= budget):
                 p = probs[nxt]
-                p.integrate_assemble(64, wait=False)  # ordered before the next cycle
+                assemble(p, wait=False)  # ordered before the next cycle
                 b.add(p, 1, mi, tol, inner_precision=prec)
                 live[nxt] = p
                 used += footprint(p, m, mixed)
@@ -435,7 +476,7 @@ def bench_serve(args):
         seq_span = fin - arr[0]
         lat64, span64 = serve(arr, False)
         lat, span = serve(arr, True)
-        meas = max([meas] + [fp64_measure(p) for p in probs])
+        meas = max([meas] + [fp64_measure(p) for p in probs if not is_dg(p)])
         r = {"rate_per_s": lam}
         r.update(served(lat, span, seq_span))
         r.update({"fp64_members": served(lat64, span64, seq_span),
@@ -448,11 +489,15 @@ def bench_serve(args):
         "metric": "served solves/s and p50/p99 latency at 1e-8 rel residual",
         "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32 inner solves, f64 refinement", "data": "synthetic",
-        "config": {"workload": "Poisson arrivals of %d std-linear prism Poisson requests, %d-%d prisms "
-                               "log-uniform, GMRES(30)+Jacobi 1e-8 (mixed-precision members: FP32 "
-                               "inner solves, FP64 refinement to the same FP64 measure), <= %d slots, "
-                               "%.0f GB admission budget; sequential baseline: plain FP64 solves"
-                               % (args.count, args.el_min, args.el_max, args.slots, args.hbm_gb)},
+        "config": {"workload": "Poisson arrivals of %d prism Poisson requests: std-linear (%d-%d "
+                               "prisms) and, with probability %.2f, DG p=1 or p=2 (%d-%d prisms), "
+                               "log-uniform sizes, lognormal kappa, GMRES(30)+(block) Jacobi 1e-8 "
+                               "(mixed-precision members: FP32 inner solves, FP64 refinement to "
+                               "the same FP64 measure), <= %d slots, %.0f GB admission budget; "
+                               "sequential baseline: plain FP64 solves"
+                               % (args.count, args.el_min, args.el_max, args.dg_frac,
+                                  args.dg_el_min, args.dg_el_max, args.slots, args.hbm_gb),
+                   "dg_requests": int(sum(is_dg(p) for p in probs))},
         "rates": results,
         "max_fp64_measure_of_served_solutions": meas,
         "sequential_mean_service_ms": float(np.mean(service) * 1e3),
@@ -874,11 +919,17 @@ def other_workloads(args):
                                           "fp64_batch_s", "fp64_speedup_vs_loop", "loop_s",
                                           "loop_solves_per_s", "all_converged", "iters",
                                           "max_rel_diff_vs_loop_solution", "tiles")}
-    a.count, a.el_min, a.el_max, a.rates = 128, 5000, 500000, [1000.0, 2000.0]
+    a.count, a.el_min, a.el_max, a.rates = 128, 5000, 500000, [1000.0, 2000.0, 4000.0]
     print("workload: served stream", file=sys.stderr, flush=True)
+    a.dg_frac = 0.0
     r = bench_serve(a)
     out["serve"] = {k: r[k] for k in ("value", "unit", "config", "rates",
                                       "max_fp64_measure_of_served_solutions")}
+    print("workload: served stream with DG requests", file=sys.stderr, flush=True)
+    a.dg_frac, a.dg_el_min, a.dg_el_max, a.rates = 0.25, 2000, 10000, [2000.0]
+    r = bench_serve(a)
+    out["serve_with_dg"] = {k: r[k] for k in ("value", "unit", "config", "rates",
+                                              "max_fp64_measure_of_served_solutions")}
     a.dg_n, a.dg_pdegs, a.dg_solve, a.steps, a.warmup = 96, [], [], 3, 1
     print("workload: DG mixed degrees 96^3", file=sys.stderr, flush=True)
     r = bench_dg(a)
@@ -925,6 +976,9 @@ def main():
     ap.add_argument("--rates", type=float, nargs="+", default=[10.0, 100.0, 1000.0, 2000.0])
     ap.add_argument("--slots", type=int, default=256)
     ap.add_argument("--hbm-gb", type=float, default=64.0)
+    ap.add_argument("--dg-frac", type=float, default=0.0, help="serve: share of DG requests")
+    ap.add_argument("--dg-el-min", type=int, default=2000)
+    ap.add_argument("--dg-el-max", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
