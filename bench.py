@@ -616,7 +616,8 @@ def bench_dg(args):
             # (spectral radius 1.5-1.8 of the sweep): block Jacobi there;
             # block ILU(0) in element order for every case
             solves = {}
-            for precon in ((1, 3) if name == "mixed" else (2, 3)):
+            precons = args.dg_precons or ((1, 3) if name == "mixed" else (2, 3))
+            for precon in precons:
                 label = {1: "block Jacobi", 2: "block Gauss-Seidel, 2 multicolour sweeps",
                          3: "block ILU(0), element order"}[precon]
                 if precon == 2:
@@ -957,8 +958,12 @@ def main():
                     help="short run for hardware-counter passes: one integrate+assemble, one "
                          "GMRES restart and a 1 GiB stream copy (the counters' yardstick)")
     ap.add_argument("--dg-n", type=int, default=96, help="DG workload: hexes per box edge")
-    ap.add_argument("--dg-pdegs", type=int, nargs="+", default=[101, 202, 303])
+    ap.add_argument("--dg-pdegs", type=int, nargs="*", default=[101, 202, 303],
+                    help="DG workload: uniform degrees run beside the mixed-degree case")
     ap.add_argument("--dg-max-iter", type=int, default=6000)
+    ap.add_argument("--dg-precons", type=int, nargs="*", default=[],
+                    help="DG solves: preconditioners to run (1 block Jacobi, 2 block GS, 3 block "
+                         "ILU(0)); default 1 3 for mixed degrees, 2 3 for uniform ones")
     ap.add_argument("--dg-solve", nargs="*", default=[],
                     help="DG cases solved to 1e-8: mixed (block Jacobi, block ILU(0)) and/or the "
                          "uniform degrees 101 202 303 (block Gauss-Seidel with 2 sweeps, block ILU(0))")
