@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E, GB/s
 FP64_PEAK_TFS = 78.6       # MI355X dense FP64, TFLOP/s
+DG_VALU_NS = 6             # the library's default for mf_dg_set_valu_blocks
 ASSEMBLY_MODE = 0          # row-wise deterministic assembly (fastest measured)
 COPY_BYTES = 1 << 30        # stream-copy yardstick (read + write of 1 GiB)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_op_dots_sell_box128_pmc.txt")
@@ -597,6 +598,8 @@ def bench_dg(args):
     for name, pd in cases:
         dg = mf.DgProblem(mesh, pd if name == "mixed" else int(pd[0]))
         dg.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc, src_model=mf.SRC_DIFF_IN_CUBE)
+        if args.dg_valu_ns is not None:
+            dg.set_valu_blocks(args.dg_valu_ns)
         t0 = time.perf_counter()
         dg.assemble()
         t_first = time.perf_counter() - t0
@@ -606,10 +609,16 @@ def bench_dg(args):
         dg.fill_precon()
         t_prec = time.perf_counter() - t0
         ms_asm, ms_op = dg.bench(max(args.steps, 1))
+        # the same assembly with every block on the MFMA kernels
+        dg.set_valu_blocks(0)
+        ms_mfma, _ = dg.bench(max(args.steps, 1))
+        dg.set_valu_blocks(DG_VALU_NS if args.dg_valu_ns is None else args.dg_valu_ns)
+        dg.assemble()
         flops, op_bytes = dg_work(dg, pd)
         rec = {"pdeg": name, "dofs": int(dg.ndof), "assemble_ms": ms_asm,
                "elements_per_s": nel / (ms_asm * 1e-3), "assemble_tflops": flops / (ms_asm * 1e-3) / 1e12,
                "first_assemble_s": t_first, "block_jacobi_ms": t_prec * 1e3,
+               "assemble_ms_all_mfma": ms_mfma,
                "op_ms": ms_op, "op_GBs": op_bytes / (ms_op * 1e-3) / 1e9}
         if name in args.dg_solve:
             # block Gauss-Seidel smoothing diverges on mixed-degree systems
@@ -960,6 +969,9 @@ def main():
     ap.add_argument("--dg-pdegs", type=int, nargs="*", default=[101, 202, 303],
                     help="DG workload: uniform degrees run beside the mixed-degree case")
     ap.add_argument("--dg-max-iter", type=int, default=6000)
+    ap.add_argument("--dg-valu-ns", type=int, default=None,
+                    help="DG blocks of at most this many shape functions assembled on the vector "
+                         "ALUs (library default: %d)" % DG_VALU_NS)
     ap.add_argument("--dg-precons", type=int, nargs="*", default=[],
                     help="DG solves: preconditioners to run (1 block Jacobi, 2 block GS, 3 block "
                          "ILU(0)); default 1 3 for mixed degrees, 2 3 for uniform ones")

@@ -105,6 +105,13 @@ int mf_field_read(int mesh, const char* filename, double* x);
  * (lsd_mkb/lad_crs/las_crs_intf.c:72) and the assembly tables
  * (las_crs_intf.c:354).  The matrix lives in device memory. */
 int mf_problem_create(int mesh);
+/* the same set-up computed on the device (the same replaced routines; the
+   reference has no device set-up): DOF numbering, CRS pattern, assembly
+   tables, incidence lists and the device ordering come from radix sorts on
+   the GPU and are bitwise those of mf_problem_create; only the maps that
+   host-side calls read are copied back.  Meshes of one element type with
+   at most 2^31 element-matrix entries. */
+int mf_problem_create_device(int mesh);
 /* solver-only problem from a given CRS matrix (strictly ascending columns
    with the diagonal present), e.g. for a caller that assembles itself */
 int mf_problem_create_crs(int n, const int* row_ptr, const int* col_ind,
@@ -390,6 +397,13 @@ int mf_dg_get_element_matrices(int did, int first, int count, double* stiff, dou
    (its normal derivative) when a source model selects one, else the value
    given per (element, side) by mf_dg_set_boundary_values (default 0) */
 int mf_dg_set_penalty(int did, double penalty_const);
+/* kernel choice for the block rows (no reference counterpart; the reference
+   integrates DG elements on the CPU, aps_dg_prism_intf.c:1239): own and
+   coupling blocks of at most max_ns shape functions (default 6, p = 1) are
+   assembled by vector-ALU kernels holding several elements per wave, larger
+   ones by FP64 MFMA kernels, one element (side) per wave; 0 puts every
+   block on MFMA.  Both forms sum the same terms to rounding. */
+int mf_dg_set_valu_blocks(int did, int max_ns);
 int mf_dg_set_boundary_values(int did, const double* val);
 /* element-centric face table, 5 entries per element: neighbour element or
    -1 (Dirichlet) / -2 (Neumann), orientation code, sqrt(face area) */

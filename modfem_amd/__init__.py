@@ -88,6 +88,7 @@ def _declare(L):
         "mf_mesh_setup_table": (I, [I, I, V]),
         "mf_mesh_setup_order": (I, [I, V, V]),
         "mf_problem_create": (I, [I]),
+        "mf_problem_create_device": (I, [I]),
         "mf_problem_create_crs": (I, [I, V, V, V, V]),
         "mf_problem_sizes": (I, [I, V, V, V, V, V]),
         "mf_problem_get_dofs": (I, [I, V]),
@@ -149,6 +150,7 @@ def _declare(L):
         "mf_dg_get_element_matrices": (I, [I, I, I, V, V]),
         "mf_dg_free": (I, [I]),
         "mf_dg_set_penalty": (I, [I, D]),
+        "mf_dg_set_valu_blocks": (I, [I, I]),
         "mf_dg_set_boundary_values": (I, [I, V]),
         "mf_dg_get_faces": (I, [I, V, V, V]),
         "mf_dg_assemble": (I, [I]),
@@ -485,6 +487,11 @@ class DgProblem:
     def set_penalty(self, penalty_const):
         check(lib().mf_dg_set_penalty(self.h, float(penalty_const)))
 
+    def set_valu_blocks(self, max_ns):
+        """blocks of at most max_ns shape functions assembled on the vector
+        ALUs, larger ones on MFMA (0: all on MFMA)"""
+        check(lib().mf_dg_set_valu_blocks(self.h, int(max_ns)))
+
     def set_boundary_values(self, val=None):
         """per (element, side) boundary data used when no exact solution is selected"""
         v = None if val is None else np.ascontiguousarray(val, np.float64).reshape(-1)
@@ -610,8 +617,12 @@ class Problem:
                                                                nbf.value, mr.value)
 
     @classmethod
-    def from_mesh(cls, mesh):
+    def from_mesh(cls, mesh, device_setup=False):
+        """device_setup: build the DOF numbering, pattern and tables on the
+        GPU (bitwise the host set-up's)"""
         init()
+        if device_setup:
+            return cls(check(lib().mf_problem_create_device(mesh.h)))
         return cls(check(lib().mf_problem_create(mesh.h)))
 
     @classmethod

@@ -136,6 +136,14 @@ static int effective_mode(const Problem& P) {
   return (P.assembly_mode != 1 && P.setup.max_row > 32) ? 1 : P.assembly_mode;
 }
 
+// a new system invalidates everything derived from the old one
+void assembled(Problem& P) {
+  P.dinv_valid = false;
+  P.val32_valid = false;
+  if (P.ilu) P.ilu->valid = false;
+  if (P.gs) P.gs->valid = false;
+}
+
 // integrate + assemble in the problem's assembly mode
 int assemble_problem(Problem& P, int precision) {
   const int mode = effective_mode(P);
@@ -148,10 +156,7 @@ int assemble_problem(Problem& P, int precision) {
     if (launch_integrate(P, precision, 0, nullptr, 0, P.nel)) return -2;
     if (launch_faces(P, 0, nullptr, 0, P.nbf)) return -2;
   }
-  P.dinv_valid = false;
-  P.val32_valid = false;
-  if (P.ilu) P.ilu->valid = false;
-  if (P.gs) P.gs->valid = false;
+  assembled(P);
   return 0;
 }
 
@@ -523,6 +528,36 @@ int mf_problem_create(int mesh) {
   P->coef_const[C_A + 0] = P->coef_const[C_A + 4] = P->coef_const[C_A + 8] = 1.0;
   P->coef_const[C_S] = 1.0;
   MF_HIP(hipStreamSynchronize(stream()));
+  int id = g_next_problem++;
+  g_problems[id] = std::move(P);
+  return id;
+}
+
+int mf_problem_create_device(int mesh) {
+  if (need_init()) return -1;
+  Mesh* m = get_mesh(mesh);
+  if (!m) return -3;
+  auto P = std::make_unique<Problem>();
+  P->mesh = *m;
+  const Mesh& M = P->mesh;
+  for (int e = 0; e < M.nel; e++)
+    if (M.type[e] != M.type[0]) {
+      set_error("mf_problem_create_device: mixed prism/tetra std meshes are not supported");
+      return -1;
+    }
+  const long long K = nodes_of(M.type[0]);
+  if ((long long)M.nel * K * K > 2147483647LL || 6LL * M.nel > 4294967295LL) {
+    set_error("mf_problem_create_device: more element-matrix entries than 2^31");
+    return -1;
+  }
+  P->nel = M.nel;
+  P->nbf = M.nbf;
+  P->el_type = M.type[0];
+  const int rc = problem_setup_device(M, *P);
+  if (rc) return rc;
+  P->mask = TermMask{0, 1, 0, 0, 0, 0, 0, 1};
+  P->coef_const[C_A + 0] = P->coef_const[C_A + 4] = P->coef_const[C_A + 8] = 1.0;
+  P->coef_const[C_S] = 1.0;
   int id = g_next_problem++;
   g_problems[id] = std::move(P);
   return id;
@@ -1278,8 +1313,19 @@ int mf_batch_integrate_assemble(int bid, int precision) {
     set_error("precision must be 64 or 32");
     return -1;
   }
-  for (auto& M : B->mem)
-    if (M.P->nel > 0 && assemble_problem(*M.P, precision)) return -2;
+  // row-wise and two-pass members: one launch per kernel for all of them
+  std::vector<Problem*> many;
+  for (auto& M : B->mem) {
+    if (M.P->nel <= 0) continue;
+    const int mode = effective_mode(*M.P);
+    if (mode == 0 || mode == 2) {
+      many.push_back(M.P);
+    } else if (assemble_problem(*M.P, precision)) {
+      return -2;
+    }
+  }
+  if (launch_assemble_many(many, precision)) return -2;
+  for (Problem* P : many) assembled(*P);
   MF_HIP(hipStreamSynchronize(stream()));
   return 0;
 }

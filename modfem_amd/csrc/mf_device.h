@@ -326,6 +326,13 @@ int launch_integrate(Problem& P, int precision, int write_elements,
                      double* d_emat, int e0, int ecount, const int* d_elist = nullptr);
 int launch_assemble_rows(Problem& P, int precision);
 int launch_assemble_two_pass(Problem& P, int precision);
+// row-wise (mode 0) and two-pass (mode 2) problems with rows of at most 32
+// entries, one launch per kernel and problem group; on the library stream
+int launch_assemble_many(const std::vector<Problem*>& ps, int precision);
+// the set-up maps of a std-linear problem built on the device, bitwise the
+// host set-up's; fills the problem's device arrays and the host maps its
+// later calls read (mf_setup_dev.hip)
+int problem_setup_device(const Mesh& M, Problem& P);
 int launch_faces(Problem& P, int write_faces, double* d_fmat, int f0, int fcount);
 int launch_fill_jacobi(Problem& P);
 // tol: relative to the initial preconditioned residual; tol_abs: absolute

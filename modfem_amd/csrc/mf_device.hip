@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -223,7 +224,7 @@ __device__ __forceinline__ double ref_phi(int i, double e0, double e1, double e2
 constexpr int INT_BLOCK = 128;
 
 template <int NS, class Q, int TS, typename R>
-__global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
+__device__ __forceinline__ void integrate_block(const IntArgs& A, int bid) {
   constexpr int NG = Q::NG;
   constexpr int EPW = 64 / NS;                 // elements per wavefront
   constexpr int EPB = EPW * (INT_BLOCK / 64);  // elements per block
@@ -236,7 +237,7 @@ __global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
   const int le = wave * EPW + lane / NS;       // element slot in block
   const int sub = lane % NS;
   const bool act_lane = (lane / NS) < EPW;
-  const int t = blockIdx.x * EPB + le;
+  const int t = bid * EPB + le;
   const bool active = act_lane && t < A.count;
   const int e = active ? (A.elist ? A.elist[t] : A.e0 + t) : 0;
   const size_t N = A.nel;
@@ -376,6 +377,11 @@ __global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
     unsafeAtomicAdd(&A.val[base + 64LL * off], (double)K[u]);
   }
   unsafeAtomicAdd(&A.rhs[dw], (double)F);
+}
+
+template <int NS, class Q, int TS, typename R>
+__global__ void __launch_bounds__(INT_BLOCK) k_integrate(IntArgs A) {
+  integrate_block<NS, Q, TS, R>(A, blockIdx.x);
 }
 
 // Face point (c0, c1) of the reference triangle -> tetrahedron coordinates,
@@ -900,11 +906,11 @@ __device__ __forceinline__ double face_rows_into(const RowArgs& A, int r, double
 // elements, then boundary faces: one lane per row, accumulators in LDS
 // (k-major per wave, so the 64 lanes of a wave touch 64 consecutive words)
 template <int NS, class Q, int TS, int RMAX, typename R>
-__global__ void __launch_bounds__(64 * ROW_WAVES) k_assemble_rows(RowArgs A) {
+__device__ __forceinline__ void assemble_rows_block(const RowArgs& A, int bid) {
   __shared__ double s_acc[ROW_WAVES][RMAX][64];
   const int nsl = (A.n + 63) >> 6;
   const int wv = threadIdx.x >> 6;
-  const int sl = blockIdx.x * ROW_WAVES + wv;
+  const int sl = bid * ROW_WAVES + wv;
   if (sl >= nsl) return;  // wave-uniform, no block barriers below
   const int lane = threadIdx.x & 63;
   const int r = sl * 64 + lane;
@@ -931,6 +937,11 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_assemble_rows(RowArgs A) {
   const int len = (int)((A.I.sl_ptr[sl + 1] - base) >> 6);
   for (int k = 0; k < len; k++) A.val[base + 64LL * k + lane] = acc[k][lane];
   if (r < A.n) A.rhs[r] = F;
+}
+
+template <int NS, class Q, int TS, int RMAX, typename R>
+__global__ void __launch_bounds__(64 * ROW_WAVES) k_assemble_rows(RowArgs A) {
+  assemble_rows_block<NS, Q, TS, RMAX, R>(A, blockIdx.x);
 }
 
 template <int NS, class Q, int RMAX, typename R>
@@ -987,13 +998,13 @@ int launch_assemble_rows(Problem& P, int precision) {
 // SELL slice once, after the boundary faces of the row.  T is the buffer
 // type (FP64, or FP32 for the mixed-precision form; sums are FP64 either way).
 template <int NS, int RMAX, typename T>
-__global__ void __launch_bounds__(64 * ROW_WAVES) k_gather_rows(RowArgs A, const T* erow,
-                                                               const T* ef, const uint8_t* offT) {
+__device__ __forceinline__ void gather_rows_block(const RowArgs& A, int bid, const T* erow,
+                                                  const T* ef, const uint8_t* offT) {
   constexpr int ns = NS;
   __shared__ double s_acc[ROW_WAVES][RMAX][64];
   const int nsl = (A.n + 63) >> 6;
   const int wv = threadIdx.x >> 6;
-  const int sl = blockIdx.x * ROW_WAVES + wv;
+  const int sl = bid * ROW_WAVES + wv;
   if (sl >= nsl) return;
   const int lane = threadIdx.x & 63;
   const int r = sl * 64 + lane;
@@ -1013,6 +1024,12 @@ __global__ void __launch_bounds__(64 * ROW_WAVES) k_gather_rows(RowArgs A, const
   const int len = (int)((A.I.sl_ptr[sl + 1] - base) >> 6);
   for (int k = 0; k < len; k++) A.val[base + 64LL * k + lane] = acc[k][lane];
   if (r < A.n) A.rhs[r] = F;
+}
+
+template <int NS, int RMAX, typename T>
+__global__ void __launch_bounds__(64 * ROW_WAVES) k_gather_rows(RowArgs A, const T* erow,
+                                                               const T* ef, const uint8_t* offT) {
+  gather_rows_block<NS, RMAX, T>(A, blockIdx.x, erow, ef, offT);
 }
 
 int launch_assemble_two_pass(Problem& P, int precision) {
@@ -1075,6 +1092,218 @@ int launch_assemble_two_pass(Problem& P, int precision) {
     else
       hipLaunchKernelGGL((k_gather_rows<6, 32, double>), g, b, 0, g_stream, A, er, ef, P.d_el_offT);
   }
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------
+// Assembly of many problems with one launch per kernel (no reference
+// counterpart: the reference assembles one system at a time).  Problems of
+// one element type, term set and row bound share a launch: a block table
+// maps every block to (problem, block of that problem) and the problems'
+// launch arguments sit in a device array that the blocks read.  Each block
+// runs exactly the code of the single-problem kernels, so every problem's
+// matrix is bitwise the one its own launches give.
+// ---------------------------------------------------------------------
+struct GatherArgs {
+  RowArgs A;
+  const void* erow;
+  const void* ef;
+  const uint8_t* offT;
+};
+
+template <int NS, class Q, int TS, typename R>
+__global__ void __launch_bounds__(INT_BLOCK) kb_integrate(const IntArgs* args, const int2* tab) {
+  const int2 tb = tab[blockIdx.x];
+  integrate_block<NS, Q, TS, R>(args[tb.x], tb.y);
+}
+
+template <int NS, int RMAX, typename T>
+__global__ void __launch_bounds__(64 * ROW_WAVES) kb_gather_rows(const GatherArgs* args,
+                                                                const int2* tab) {
+  const int2 tb = tab[blockIdx.x];
+  const GatherArgs& G = args[tb.x];
+  gather_rows_block<NS, RMAX, T>(G.A, tb.y, (const T*)G.erow, (const T*)G.ef, G.offT);
+}
+
+template <int NS, class Q, int TS, int RMAX, typename R>
+__global__ void __launch_bounds__(64 * ROW_WAVES) kb_assemble_rows(const RowArgs* args,
+                                                                  const int2* tab) {
+  const int2 tb = tab[blockIdx.x];
+  assemble_rows_block<NS, Q, TS, RMAX, R>(args[tb.x], tb.y);
+}
+
+namespace {
+
+void fill_row_args(Problem& P, RowArgs& A) {
+  fill_int_args(P, A.I);
+  A.n = P.n;
+  A.inc_ptr = P.d_inc_ptr; A.inc_el = P.d_inc_el; A.inc_loc = P.d_inc_loc;
+  A.finc_ptr = P.d_finc_ptr; A.finc_f = P.d_finc_f; A.finc_loc = P.d_finc_loc;
+  A.bf_elem = P.d_bf_elem; A.bf_side = P.d_bf_side; A.bf_g = P.d_bf_g; A.bf_type = P.d_bf_type;
+  A.val = P.d_val; A.rhs = P.d_rhs;
+}
+
+// one launch of a batched kernel: its template key, the argument records
+// (as bytes) and the block table
+struct ManyLaunch {
+  int kind;   // 0 row-wise, 1 element rows (pass 1), 2 row gathers (pass 2)
+  int ns, ts, rmax;
+  std::vector<char> args;
+  std::vector<int2> tab;
+  size_t args_off = 0, tab_off = 0;
+};
+
+// device copy of every launch's records and tables; the host image stays
+// alive until the next call, which first waits for the stream
+struct ManyBuf {
+  void* d = nullptr;
+  size_t cap = 0;
+  std::vector<char> h;
+};
+ManyBuf g_many;
+
+template <class T>
+void add_record(ManyLaunch& L, const T& rec, int blocks) {
+  const int idx = (int)(L.args.size() / sizeof(T));
+  L.args.insert(L.args.end(), (const char*)&rec, (const char*)&rec + sizeof(T));
+  for (int b = 0; b < blocks; b++) L.tab.push_back(make_int2(idx, b));
+}
+
+ManyLaunch& launch_for(std::vector<ManyLaunch>& ls, int kind, int ns, int ts, int rmax) {
+  for (auto& L : ls)
+    if (L.kind == kind && L.ns == ns && L.ts == ts && L.rmax == rmax) return L;
+  ls.push_back(ManyLaunch{kind, ns, ts, rmax, {}, {}});
+  return ls.back();
+}
+
+template <int NS, class Q, typename R>
+void kb_int_ts(int ts, dim3 g, const IntArgs* a, const int2* t) {
+  if (ts == TS_DIFF)
+    hipLaunchKernelGGL((kb_integrate<NS, Q, TS_DIFF, R>), g, dim3(INT_BLOCK), 0, g_stream, a, t);
+  else if (ts == TS_DIFF_CONV)
+    hipLaunchKernelGGL((kb_integrate<NS, Q, TS_DIFF_CONV, R>), g, dim3(INT_BLOCK), 0, g_stream, a, t);
+  else
+    hipLaunchKernelGGL((kb_integrate<NS, Q, TS_ALL, R>), g, dim3(INT_BLOCK), 0, g_stream, a, t);
+}
+
+template <int NS, class Q, int RMAX, typename R>
+void kb_rows_ts(int ts, dim3 g, const RowArgs* a, const int2* t) {
+  const dim3 b(64 * ROW_WAVES);
+  if (ts == TS_DIFF)
+    hipLaunchKernelGGL((kb_assemble_rows<NS, Q, TS_DIFF, RMAX, R>), g, b, 0, g_stream, a, t);
+  else if (ts == TS_DIFF_CONV)
+    hipLaunchKernelGGL((kb_assemble_rows<NS, Q, TS_DIFF_CONV, RMAX, R>), g, b, 0, g_stream, a, t);
+  else
+    hipLaunchKernelGGL((kb_assemble_rows<NS, Q, TS_ALL, RMAX, R>), g, b, 0, g_stream, a, t);
+}
+
+template <typename R>
+void run_many(const ManyLaunch& L, const char* dbase) {
+  const dim3 g((unsigned)L.tab.size());
+  const int2* t = (const int2*)(dbase + L.tab_off);
+  const dim3 b(64 * ROW_WAVES);
+  if (L.kind == 1) {
+    const IntArgs* a = (const IntArgs*)(dbase + L.args_off);
+    if (L.ns == 4) kb_int_ts<4, TetQ, R>(L.ts, g, a, t);
+    else kb_int_ts<6, PrismQ, R>(L.ts, g, a, t);
+  } else if (L.kind == 2) {
+    const GatherArgs* a = (const GatherArgs*)(dbase + L.args_off);
+    if (L.ns == 4) hipLaunchKernelGGL((kb_gather_rows<4, 32, R>), g, b, 0, g_stream, a, t);
+    else if (L.rmax == 24) hipLaunchKernelGGL((kb_gather_rows<6, 24, R>), g, b, 0, g_stream, a, t);
+    else hipLaunchKernelGGL((kb_gather_rows<6, 32, R>), g, b, 0, g_stream, a, t);
+  } else {
+    const RowArgs* a = (const RowArgs*)(dbase + L.args_off);
+    if (L.ns == 4) {
+      if (L.rmax == 16) kb_rows_ts<4, TetQ, 16, R>(L.ts, g, a, t);
+      else kb_rows_ts<4, TetQ, 32, R>(L.ts, g, a, t);
+    } else {
+      if (L.rmax == 24) kb_rows_ts<6, PrismQ, 24, R>(L.ts, g, a, t);
+      else kb_rows_ts<6, PrismQ, 32, R>(L.ts, g, a, t);
+    }
+  }
+}
+
+}  // namespace
+
+int launch_assemble_many(const std::vector<Problem*>& ps, int precision) {
+  MF_HIP(hipStreamSynchronize(g_stream));
+  const size_t esz = precision == 32 ? sizeof(float) : sizeof(double);
+  std::vector<ManyLaunch> ls;
+  ls.reserve(8);
+  for (Problem* P : ps) {
+    const int mr = P->setup.max_row;
+    const bool tet = P->el_type == kTetra;
+    const int ns = tet ? 4 : 6;
+    if (P->nel <= 0) continue;
+    if (mr > 32 || (P->assembly_mode != 0 && P->assembly_mode != 2)) {
+      set_error("launch_assemble_many: row-wise or two-pass problems with rows of at most 32 entries");
+      return -1;
+    }
+    const int nsl = (P->n + 63) / 64;
+    const int row_blocks = (nsl + ROW_WAVES - 1) / ROW_WAVES;
+    if (P->assembly_mode == 0) {
+      const int rmax = tet ? (mr <= 16 ? 16 : 32) : (mr <= 24 ? 24 : 32);
+      RowArgs A;
+      fill_row_args(*P, A);
+      add_record(launch_for(ls, 0, ns, P->term_set, rmax), A, row_blocks);
+      continue;
+    }
+    if (!P->d_erow || P->erow_bytes < (size_t)P->nel * 42 * esz) {
+      if (P->d_erow) (void)hipFree(P->d_erow);
+      P->d_erow = nullptr;
+      MF_HIP(hipMalloc(&P->d_erow, (size_t)P->nel * 42 * esz));
+      P->erow_bytes = (size_t)P->nel * 42 * esz;
+    }
+    IntArgs I;
+    fill_int_args(*P, I);
+    I.write_elements = 2;
+    I.erow = P->d_erow;
+    I.ef = (char*)P->d_erow + (size_t)P->nel * 36 * esz;
+    I.emat = nullptr;
+    I.e0 = 0;
+    I.count = P->nel;
+    constexpr int EPB6 = (64 / 6) * (INT_BLOCK / 64), EPB4 = (64 / 4) * (INT_BLOCK / 64);
+    const int epb = tet ? EPB4 : EPB6;
+    add_record(launch_for(ls, 1, ns, P->term_set, 0), I, (P->nel + epb - 1) / epb);
+    GatherArgs G;
+    fill_row_args(*P, G.A);
+    G.A.I = I;
+    G.erow = I.erow;
+    G.ef = I.ef;
+    G.offT = P->d_el_offT;
+    add_record(launch_for(ls, 2, ns, 0, tet ? 32 : (mr <= 24 ? 24 : 32)), G, row_blocks);
+  }
+  if (ls.empty()) return 0;
+  // one host image: records (256-byte aligned) then tables
+  size_t off = 0;
+  for (auto& L : ls) {
+    L.args_off = off;
+    off = (off + L.args.size() + 255) & ~(size_t)255;
+    L.tab_off = off;
+    off = (off + L.tab.size() * sizeof(int2) + 255) & ~(size_t)255;
+  }
+  g_many.h.assign(off, 0);
+  for (auto& L : ls) {
+    std::memcpy(g_many.h.data() + L.args_off, L.args.data(), L.args.size());
+    std::memcpy(g_many.h.data() + L.tab_off, L.tab.data(), L.tab.size() * sizeof(int2));
+  }
+  if (off > g_many.cap) {
+    if (g_many.d) (void)hipFree(g_many.d);
+    g_many.d = nullptr;
+    g_many.cap = 0;
+    MF_HIP(hipMalloc(&g_many.d, off));
+    g_many.cap = off;
+  }
+  MF_HIP(hipMemcpyAsync(g_many.d, g_many.h.data(), off, hipMemcpyHostToDevice, g_stream));
+  const char* dbase = (const char*)g_many.d;
+  // element rows before their gathers
+  for (int kind : {0, 1, 2})
+    for (auto& L : ls)
+      if (L.kind == kind) {
+        if (precision == 32) run_many<float>(L, dbase);
+        else run_many<double>(L, dbase);
+      }
   MF_HIP(hipGetLastError());
   return 0;
 }

@@ -424,6 +424,9 @@ struct DgProblem {
   double* d_stiff = nullptr;  // device-resident element matrices (mf_dg_integrate)
   double* d_rhs = nullptr;
   int ncu = 0;
+  // own and coupling blocks of at most this many shape functions are
+  // assembled by the vector-ALU kernels, larger ones on MFMA
+  int valu_ns = 6;
   // faces, element-centric: for element e and side s (index 5 e + s)
   int* d_fnb = nullptr;       // neighbour element, or -1 Dirichlet / -2 Neumann
   int* d_fcode = nullptr;     // neighbour side | orientation << 4
@@ -963,6 +966,112 @@ struct LaneShapes {
   }
 };
 
+// Record of point k of element e for its own block (volume points first,
+// then the points of sides 0..4 under each side's rule): the powers of the
+// local coordinates (Rg[0..11]), the 4x4 matrix M between (phi, d/deta phi)
+// of trial function (row) and test function (column) (Rg[12..27]) and the
+// load-vector row (Rg[28..31]).  Mixed problems pass each side's rule,
+// first point and penalty / h_F in fq, fst and fpen.
+template <int PX, int PZ, bool MIXED>
+__device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, const double* sX,
+                                            const int* fq, const int* fst, const double* fpen,
+                                            double* Rg) {
+  using S = DgShape<PX, PZ>;
+  using Q = DgRowShape<PX, PZ>;
+  constexpr int NG = S::NG, DE = deg_index(PX, PZ);
+  const DgElemArgs& A = R.E;
+  double* M = Rg + 12;
+#pragma unroll
+  for (int t = 0; t < 20; t++) M[t] = 0.0;
+  if (k < NG) {
+    const double* qg = A.qv + 4 * k;
+    double a[9], b[3], mc, q0, q1, q2, sv, J[9], Ji[9], x[3];
+    dg_coefs(A, e, a, b, mc, q0, q1, q2, sv);
+    const double vol = fabs(prism_jac(sX, qg, J, Ji, x)) * qg[3];
+    if (A.src_model != SRC_CONST && A.mask.s) {
+      double uu, lapl;
+      exact_sol(A.src_model, x, &uu, &lapl);
+      sv = -lapl;
+    }
+    put_powers(qg, Rg);
+    M[0] = vol * mc;
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++) {
+      M[1 + kk] = vol * (b[0] * Ji[3 * kk] + b[1] * Ji[3 * kk + 1] + b[2] * Ji[3 * kk + 2]);
+      double t[3];
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        t[r] = Ji[3 * kk] * a[3 * r] + Ji[3 * kk + 1] * a[3 * r + 1] + Ji[3 * kk + 2] * a[3 * r + 2];
+#pragma unroll
+      for (int l = 0; l < 3; l++)
+        M[4 * (1 + kk) + 1 + l] = vol * (t[0] * Ji[3 * l] + t[1] * Ji[3 * l + 1] + t[2] * Ji[3 * l + 2]);
+    }
+    M[16] = vol * sv;
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++)
+      M[17 + kk] = vol * (q0 * Ji[3 * kk] + q1 * Ji[3 * kk + 1] + q2 * Ji[3 * kk + 2]);
+  } else {
+    int sd = 0;
+    const double* rule;
+    if (MIXED) {
+      while (k >= fst[sd + 1]) sd++;
+      const int q = fq[sd];
+      rule = (sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * (k - fst[sd]);
+    } else {
+      int fq0 = k - NG;
+      if (fq0 < 2 * Q::NGT) {
+        sd = fq0 / Q::NGT;
+        rule = R.qt + 3 * (R.qt_off[DE] + fq0 - sd * Q::NGT);
+      } else {
+        fq0 -= 2 * Q::NGT;
+        sd = 2 + fq0 / Q::NGQ;
+        rule = R.qq + 3 * (R.qq_off[DE] + fq0 - (sd - 2) * Q::NGQ);
+      }
+    }
+    const int fi = 5 * e + sd, nb = R.fnb[fi];
+    const double pen = MIXED ? fpen[sd] : pmin_of_deg(DE) * R.penalty_const / R.fhf[fi];
+    FacePt F;
+    face_point(A, sX, e, sd, rule, F);
+    put_powers(F.eta, Rg);
+    if (nb >= 0) {
+      // interior: 0.5 (phi_j g_i - g_j phi_i) + (pen + upwind) phi_j phi_i
+      const double nv = 0.5 * (F.bn + neighbour_bn(A, nb, F.n));
+      M[0] = F.area * (pen + (nv > 0 ? F.bn : 0.0));
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++) {
+        M[1 + kk] = 0.5 * F.area * F.gh[kk];
+        M[4 * (1 + kk)] = -0.5 * F.area * F.gh[kk];
+      }
+    } else if (nb == -1) {
+      // Dirichlet (weakform_dg.c:1036-1120)
+      double fd = R.fval[fi];
+      if (A.src_model != SRC_CONST) {
+        double lapl;
+        exact_sol(A.src_model, F.x, &fd, &lapl);
+      }
+      M[0] = F.area * (pen + (F.bn >= 0 ? F.bn : 0.0));
+#pragma unroll
+      for (int kk = 0; kk < 3; kk++) {
+        M[1 + kk] = F.area * F.gh[kk];
+        M[4 * (1 + kk)] = -F.area * F.gh[kk];
+        M[17 + kk] = F.area * fd * F.gh[kk];
+      }
+      M[16] = F.area * (pen * fd - (F.bn < 0 ? F.bn * fd : 0.0));
+    } else {
+      // Neumann (weakform_dg.c:1148-1185): the exact normal derivative
+      double gd = R.fval[fi];
+      if (A.src_model != SRC_CONST) {
+        const double a1 = A.src_model == SRC_DIFF_IN_BOX ? 100.0 : 1.0;
+        const double* x = F.x;
+        const double ex = exp((-x[0] * x[0] - x[1] * x[1] - x[2] * x[2]) / a1);
+        gd = (-2.0 * x[0] * F.n[0] - 2.0 * x[1] * F.n[1] - 2.0 * x[2] * F.n[2]) * ex / a1;
+      }
+      M[0] = F.area * (F.bn > 0 ? F.bn : 0.0);
+      M[16] = F.area * gd;
+    }
+  }
+}
+
 // Own blocks and load vectors.  One wave per element.  Points come in
 // chunks of up to 64, one lane each: the volume points, then the points of
 // the five faces with the element's own terms of each face; a lane writes its
@@ -974,7 +1083,9 @@ template <int PX, int PZ, bool MIXED>
 __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   using S = DgShape<PX, PZ>;
   using Q = DgRowShape<PX, PZ>;
-  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, DE = deg_index(PX, PZ), REC = 32;
+  // records of 32 doubles at an odd stride: the lanes writing their own
+  // records fall on different LDS banks
+  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, DE = deg_index(PX, PZ), REC = 33;
   __shared__ double rec[64 * REC];
   __shared__ double sX[18];
   __shared__ int sex[40];
@@ -1021,97 +1132,7 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   for (int c0 = 0; c0 < NPTS; c0 += 64) {
     const int k = c0 + lane;
     if (k < NPTS) {
-      double* Rg = rec + REC * lane;
-      double* M = Rg + 12;
-#pragma unroll
-      for (int t = 0; t < 20; t++) M[t] = 0.0;
-      if (k < NG) {
-        const double* qg = A.qv + 4 * k;
-        double a[9], b[3], mc, q0, q1, q2, sv, J[9], Ji[9], x[3];
-        dg_coefs(A, e, a, b, mc, q0, q1, q2, sv);
-        const double vol = fabs(prism_jac(sX, qg, J, Ji, x)) * qg[3];
-        if (A.src_model != SRC_CONST && A.mask.s) {
-          double uu, lapl;
-          exact_sol(A.src_model, x, &uu, &lapl);
-          sv = -lapl;
-        }
-        put_powers(qg, Rg);
-        M[0] = vol * mc;
-#pragma unroll
-        for (int kk = 0; kk < 3; kk++) {
-          M[1 + kk] = vol * (b[0] * Ji[3 * kk] + b[1] * Ji[3 * kk + 1] + b[2] * Ji[3 * kk + 2]);
-          double t[3];
-#pragma unroll
-          for (int r = 0; r < 3; r++)
-            t[r] = Ji[3 * kk] * a[3 * r] + Ji[3 * kk + 1] * a[3 * r + 1] + Ji[3 * kk + 2] * a[3 * r + 2];
-#pragma unroll
-          for (int l = 0; l < 3; l++)
-            M[4 * (1 + kk) + 1 + l] = vol * (t[0] * Ji[3 * l] + t[1] * Ji[3 * l + 1] + t[2] * Ji[3 * l + 2]);
-        }
-        M[16] = vol * sv;
-#pragma unroll
-        for (int kk = 0; kk < 3; kk++)
-          M[17 + kk] = vol * (q0 * Ji[3 * kk] + q1 * Ji[3 * kk + 1] + q2 * Ji[3 * kk + 2]);
-      } else {
-        int sd = 0;
-        const double* rule;
-        if (MIXED) {
-          while (k >= fst[sd + 1]) sd++;
-          const int q = fq[sd];
-          rule = (sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * (k - fst[sd]);
-        } else {
-          int fq0 = k - NG;
-          if (fq0 < 2 * Q::NGT) {
-            sd = fq0 / Q::NGT;
-            rule = R.qt + 3 * (R.qt_off[DE] + fq0 - sd * Q::NGT);
-          } else {
-            fq0 -= 2 * Q::NGT;
-            sd = 2 + fq0 / Q::NGQ;
-            rule = R.qq + 3 * (R.qq_off[DE] + fq0 - (sd - 2) * Q::NGQ);
-          }
-        }
-        const int fi = 5 * e + sd, nb = R.fnb[fi];
-        const double pen = MIXED ? fpen[sd] : pmin_of_deg(DE) * R.penalty_const / R.fhf[fi];
-        FacePt F;
-        face_point(A, sX, e, sd, rule, F);
-        put_powers(F.eta, Rg);
-        if (nb >= 0) {
-          // interior: 0.5 (phi_j g_i - g_j phi_i) + (pen + upwind) phi_j phi_i
-          const double nv = 0.5 * (F.bn + neighbour_bn(A, nb, F.n));
-          M[0] = F.area * (pen + (nv > 0 ? F.bn : 0.0));
-#pragma unroll
-          for (int kk = 0; kk < 3; kk++) {
-            M[1 + kk] = 0.5 * F.area * F.gh[kk];
-            M[4 * (1 + kk)] = -0.5 * F.area * F.gh[kk];
-          }
-        } else if (nb == -1) {
-          // Dirichlet (weakform_dg.c:1036-1120)
-          double fd = R.fval[fi];
-          if (A.src_model != SRC_CONST) {
-            double lapl;
-            exact_sol(A.src_model, F.x, &fd, &lapl);
-          }
-          M[0] = F.area * (pen + (F.bn >= 0 ? F.bn : 0.0));
-#pragma unroll
-          for (int kk = 0; kk < 3; kk++) {
-            M[1 + kk] = F.area * F.gh[kk];
-            M[4 * (1 + kk)] = -F.area * F.gh[kk];
-            M[17 + kk] = F.area * fd * F.gh[kk];
-          }
-          M[16] = F.area * (pen * fd - (F.bn < 0 ? F.bn * fd : 0.0));
-        } else {
-          // Neumann (weakform_dg.c:1148-1185): the exact normal derivative
-          double gd = R.fval[fi];
-          if (A.src_model != SRC_CONST) {
-            const double a1 = A.src_model == SRC_DIFF_IN_BOX ? 100.0 : 1.0;
-            const double* x = F.x;
-            const double ex = exp((-x[0] * x[0] - x[1] * x[1] - x[2] * x[2]) / a1);
-            gd = (-2.0 * x[0] * F.n[0] - 2.0 * x[1] * F.n[1] - 2.0 * x[2] * F.n[2]) * ex / a1;
-          }
-          M[0] = F.area * (F.bn > 0 ? F.bn : 0.0);
-          M[16] = F.area * gd;
-        }
-      }
+      diag_record<PX, PZ, MIXED>(R, e, k, sX, fq, fst, fpen, rec + REC * lane);
     }
     __syncthreads();
     const int nk = NPTS - c0 < 64 ? NPTS - c0 : 64;
@@ -1155,6 +1176,48 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   }
 }
 
+// Record of face point k of the interior side s of element e (neighbour nb
+// across it, orientation code, the face's rule q): the powers of both
+// elements' local coordinates (Rg[0..11] own, Rg[12..23] neighbour) and the
+// 4x4 matrix M between (psi, d/deta psi) of the neighbour's trial function
+// (row) and (phi, d/deta phi) of the element's test function (column)
+// (Rg[24..39]).  sX holds both elements' node coordinates.
+template <int PXE, int PZE, int PXN, int PZN>
+__device__ __forceinline__ void couple_record(const DgRowArgs& R, int e, int s, int nb, int code,
+                                              int q, int kp, const double* sX, double* Rg) {
+  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN);
+  const DgElemArgs& A = R.E;
+  const int slot = 5 * e + s;
+  const double* rule = (s < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * kp;
+  FacePt F;
+  face_point(A, sX, e, s, rule, F);
+  double eta2[3], J2[9], Ji2[9], x2[3];
+  neighbour_eta(s, code, F.eta, eta2);
+  prism_jac(sX + 18, eta2, J2, Ji2, x2);
+  double a[9], b[3], mc, q0, q1, q2, sv;
+  dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
+  double An[3], hh[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) An[c] = a[c] * F.n[0] + a[3 + c] * F.n[1] + a[6 + c] * F.n[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++) hh[k] = Ji2[3 * k] * An[0] + Ji2[3 * k + 1] * An[1] + Ji2[3 * k + 2] * An[2];
+  const double bnn = b[0] * F.n[0] + b[1] * F.n[1] + b[2] * F.n[2];
+  const double nv = 0.5 * (F.bn + bnn);
+  const int pm = min(pmin_of_deg(DE), pmin_of_deg(DN));
+  const double pen = pm * R.penalty_const / R.fhf[slot];
+  put_powers(F.eta, Rg);
+  put_powers(eta2, Rg + 12);
+  double* M = Rg + 24;
+#pragma unroll
+  for (int t = 0; t < 16; t++) M[t] = 0.0;
+  M[0] = F.area * (-pen + (nv < 0 ? F.bn : 0.0));
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    M[1 + k] = -0.5 * F.area * F.gh[k];
+    M[4 * (1 + k)] = -0.5 * F.area * hh[k];
+  }
+}
+
 // Coupling blocks: one wave per interior (element, side): the block of the
 // element's test functions against the neighbour's trial functions,
 // -0.5 (psi_j g_i + h_j phi_i) + (-pen + upwind) psi_j phi_i over the face
@@ -1165,7 +1228,7 @@ template <int PXE, int PZE, int PXN, int PZN>
 __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots) {
   constexpr int NSE = num_shape(PXE, PZE), NTE = (NSE + 15) / 16;
   constexpr int NSN = num_shape(PXN, PZN), NTN = (NSN + 15) / 16;
-  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN), REC = 40;
+  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN), REC = 41;  // 40 + bank pad
   constexpr int QE = (PXE + 1) * (PZE + 1), QN = (PXN + 1) * (PZN + 1);
   constexpr int TE = tri_points(PXE), TN = tri_points(PXN);
   constexpr int NPMAX = cmax(cmax(QE, QN), cmax(TE, TN));
@@ -1186,37 +1249,7 @@ __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots)
   if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
   else if (lane < 36) sX[lane] = A.nc[(size_t)(lane - 18) * A.nel + nb];
   __syncthreads();
-  if (lane < npf) {
-    const double* rule = (s < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * lane;
-    FacePt F;
-    face_point(A, sX, e, s, rule, F);
-    double eta2[3], J2[9], Ji2[9], x2[3];
-    neighbour_eta(s, code, F.eta, eta2);
-    prism_jac(sX + 18, eta2, J2, Ji2, x2);
-    double a[9], b[3], mc, q0, q1, q2, sv;
-    dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
-    double An[3], hh[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) An[c] = a[c] * F.n[0] + a[3 + c] * F.n[1] + a[6 + c] * F.n[2];
-#pragma unroll
-    for (int k = 0; k < 3; k++) hh[k] = Ji2[3 * k] * An[0] + Ji2[3 * k + 1] * An[1] + Ji2[3 * k + 2] * An[2];
-    const double bnn = b[0] * F.n[0] + b[1] * F.n[1] + b[2] * F.n[2];
-    const double nv = 0.5 * (F.bn + bnn);
-    const int pm = min(pmin_of_deg(DE), pmin_of_deg(DN));
-    const double pen = pm * R.penalty_const / R.fhf[slot];
-    double* Rg = rec + REC * lane;
-    put_powers(F.eta, Rg);
-    put_powers(eta2, Rg + 12);
-    double* M = Rg + 24;
-#pragma unroll
-    for (int t = 0; t < 16; t++) M[t] = 0.0;
-    M[0] = F.area * (-pen + (nv < 0 ? F.bn : 0.0));
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      M[1 + k] = -0.5 * F.area * F.gh[k];
-      M[4 * (1 + k)] = -0.5 * F.area * hh[k];
-    }
-  }
+  if (lane < npf) couple_record<PXE, PZE, PXN, PZN>(R, e, s, nb, code, q, lane, sX, rec + REC * lane);
   __syncthreads();
   const int p = lane >> 4, c16 = lane & 15;
   LaneShapes<NSE, NTE> L;
@@ -1258,6 +1291,195 @@ __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots)
         const int j = 16 * jt + 4 * r + p, i = 16 * it + c16;
         if (j < NSN && i < NSE) out[j * NSE + i] = acc[jt][it][r];
       }
+}
+
+// Small blocks on the vector ALUs.  With few shape functions a 16x16 MFMA
+// tile holds mostly padding and a wave per element leaves most lanes idle
+// while the records are built, so for these degrees a wave holds 64/NS
+// elements (or interior sides) and lane (element, i) owns test function i,
+// i.e. column i of the block (rows j over the trial functions) in registers.
+// Points go in rounds: the element's lanes write one record each (the same
+// records as the MFMA kernels), then the values (phi, d/deta phi) of their
+// own function at every point of the round to LDS, and every lane contracts
+// M with its test function and the trial functions' values.
+
+// largest block size (shape functions) the vector-ALU kernels are built for
+constexpr int kValuMaxNs = 18;
+
+// values of the shape function with exponent code `code` (and its local
+// derivatives) from a record's power table
+__device__ __forceinline__ void shape_from_powers(const double* P, int code, double* u) {
+  const int a = code & 3, b = (code >> 2) & 3, c = (code >> 4) & 3;
+  const double pa = P[a], pb = P[4 + b], pc = P[8 + c];
+  const double qa = a * P[a > 0 ? a - 1 : 0], qb = b * P[4 + (b > 0 ? b - 1 : 0)],
+               qc = c * P[8 + (c > 0 ? c - 1 : 0)];
+  u[0] = pa * pb * pc;
+  u[1] = qa * pb * pc;
+  u[2] = pa * qb * pc;
+  u[3] = pa * pb * qc;
+}
+
+template <int PX, int PZ, bool MIXED>
+__global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
+  using S = DgShape<PX, PZ>;
+  using Q = DgRowShape<PX, PZ>;
+  constexpr int NS = S::NS, NG = S::NG, DE = deg_index(PX, PZ), REC = 33;  // 32 + bank pad
+  constexpr int EPW = 64 / NS;
+  // values of the element's functions at the round's points, element-major
+  // with an odd stride between elements (bank spread of the broadcast reads)
+  constexpr int UVS = NS * NS * 4 + 1;
+  __shared__ double rec[EPW][NS][REC];
+  __shared__ double uv[EPW * UVS];
+  __shared__ double sX[EPW][18];
+  __shared__ int sex[NS];
+  __shared__ int fq[EPW][5], fst[EPW][6];
+  __shared__ double fpen[EPW][5];
+  __shared__ int snp[EPW];
+  const DgElemArgs& A = R.E;
+  const int lane = threadIdx.x, le = lane / NS, i = lane % NS;
+  const int t = blockIdx.x * EPW + le;
+  const bool act = le < EPW && t < cnt;
+  const int e = act ? (R.elems ? R.elems[t] : t) : 0;
+  if (lane < NS) sex[lane] = R.ex[lane];
+  if (act)
+    for (int c = i; c < 18; c += NS) sX[le][c] = A.nc[(size_t)c * A.nel + e];
+  if (MIXED && act && i < 5) {
+    // each face's rule and penalty / h_F (weakform_dg.c:508-531)
+    const int sd = i, nb = R.fnb[5 * e + sd];
+    int q = DE, pm = pmin_of_deg(DE);
+    if (nb >= 0) {
+      const int dn = R.deg[nb];
+      q = face_rule_deg(R, sd < 2, e, DE, nb, dn);
+      pm = min(pm, pmin_of_deg(dn));
+    }
+    fq[le][sd] = q;
+    fst[le][sd + 1] = sd < 2 ? R.ngt[q] : R.ngq[q];
+    fpen[le][sd] = pm * R.penalty_const / R.fhf[5 * e + sd];
+  }
+  __syncthreads();
+  if (lane < EPW) {
+    int np = 0;
+    if (blockIdx.x * EPW + lane < cnt) {
+      if (MIXED) {
+        fst[lane][0] = NG;
+        for (int sd = 0; sd < 5; sd++) fst[lane][sd + 1] += fst[lane][sd];
+        np = fst[lane][5];
+      } else {
+        np = Q::NPTS;
+      }
+    }
+    snp[lane] = np;
+  }
+  __syncthreads();
+  int nmax = 0;
+#pragma unroll
+  for (int k = 0; k < EPW; k++) nmax = max(nmax, snp[k]);
+  const int npts = act ? snp[le] : 0;
+  const int code = sex[i];
+  double K[NS];
+#pragma unroll
+  for (int j = 0; j < NS; j++) K[j] = 0.0;
+  double f = 0.0;
+  for (int c0 = 0; c0 < nmax; c0 += NS) {
+    const int k = c0 + i;
+    if (act && k < npts)
+      diag_record<PX, PZ, MIXED>(R, e, k, sX[le], fq[le], fst[le], fpen[le], rec[le][i]);
+    __syncthreads();
+    const int nk = min(NS, npts - c0);
+    double* uve = uv + le * UVS;
+#pragma unroll 1
+    for (int g = 0; g < nk; g++) shape_from_powers(rec[le][g], code, uve + (g * NS + i) * 4);
+    __syncthreads();
+#pragma unroll 1
+    for (int g = 0; g < nk; g++) {
+      const double* M = rec[le][g] + 12;
+      const double* u = uve + (g * NS + i) * 4;
+      const double u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+      double tv[4];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+        tv[a] = M[4 * a] * u0 + M[4 * a + 1] * u1 + M[4 * a + 2] * u2 + M[4 * a + 3] * u3;
+      f += M[16] * u0 + M[17] * u1 + M[18] * u2 + M[19] * u3;
+#pragma unroll
+      for (int j = 0; j < NS; j++) {
+        const double* w = uve + (g * NS + j) * 4;
+        K[j] += w[0] * tv[0] + w[1] * tv[1] + w[2] * tv[2] + w[3] * tv[3];
+      }
+    }
+    __syncthreads();
+  }
+  if (!act) return;
+  double* out = R.blk + (R.soff ? R.soff[6 * (size_t)e] : (size_t)e * 6 * NS * NS);
+#pragma unroll
+  for (int j = 0; j < NS; j++) out[j * NS + i] = K[j];
+  R.brhs[(R.xoff ? R.xoff[e] : (size_t)e * NS) + i] = f;
+}
+
+template <int PXE, int PZE, int PXN, int PZN>
+__global__ void __launch_bounds__(64) k_dg_couple_v(DgRowArgs R, const int* slots, int cnt) {
+  constexpr int NSE = num_shape(PXE, PZE), NSN = num_shape(PXN, PZN);
+  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN), REC = 41;  // 40 + bank pad
+  constexpr int QE = (PXE + 1) * (PZE + 1), QN = (PXN + 1) * (PZN + 1);
+  constexpr int TE = tri_points(PXE), TN = tri_points(PXN);
+  constexpr int NPMAX = cmax(cmax(QE, QN), cmax(TE, TN));
+  constexpr int FPW = 64 / NSE;  // interior sides per wave
+  constexpr int WVS = NPMAX * NSN * 4 + 1;  // the neighbour's trial functions, per side
+  __shared__ double rec[FPW][NPMAX][REC];
+  __shared__ double wv[FPW * WVS];
+  __shared__ double sX[FPW][36];
+  __shared__ int sex[NSE], sexn[NSN];
+  const DgElemArgs& A = R.E;
+  const int lane = threadIdx.x, lf = lane / NSE, i = lane % NSE;
+  const int t = blockIdx.x * FPW + lf;
+  const bool act = lf < FPW && t < cnt;
+  const int slot = act ? slots[t] : 0;
+  const int e = slot / 5, s = slot % 5;
+  const int nb = act ? R.fnb[slot] : 0, code = act ? R.fcode[slot] : 0;
+  const int q = act ? face_rule_deg(R, s < 2, e, DE, nb, DN) : 0;
+  const int npf = act ? (s < 2 ? R.ngt[q] : R.ngq[q]) : 0;
+  if (lane < NSE) sex[lane] = R.ex[lane];
+  if (lane < NSN) sexn[lane] = R.exn[lane];
+  if (act) {
+    for (int c = i; c < 18; c += NSE) sX[lf][c] = A.nc[(size_t)c * A.nel + e];
+    for (int c = i; c < 18; c += NSE) sX[lf][18 + c] = A.nc[(size_t)c * A.nel + nb];
+  }
+  __syncthreads();
+  // the side's points over its lanes, in a fixed number of rounds
+  constexpr int RR = (NPMAX + NSE - 1) / NSE;
+#pragma unroll
+  for (int r = 0; r < RR; r++) {
+    const int k = i + r * NSE;
+    if (k < npf) couple_record<PXE, PZE, PXN, PZN>(R, e, s, nb, code, q, k, sX[lf], rec[lf][k]);
+  }
+  __syncthreads();
+  double* wvf = wv + lf * WVS;
+#pragma unroll 1
+  for (int k = 0; k < npf; k++)
+    for (int j = i; j < NSN; j += NSE) shape_from_powers(rec[lf][k] + 12, sexn[j], wvf + (k * NSN + j) * 4);
+  __syncthreads();
+  if (!act) return;
+  const int cd = sex[i];
+  double K[NSN];
+#pragma unroll
+  for (int j = 0; j < NSN; j++) K[j] = 0.0;
+#pragma unroll 1
+  for (int k = 0; k < npf; k++) {
+    double u[4];
+    shape_from_powers(rec[lf][k], cd, u);
+    const double* M = rec[lf][k] + 24;
+    double tv[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+      tv[a] = M[4 * a] * u[0] + M[4 * a + 1] * u[1] + M[4 * a + 2] * u[2] + M[4 * a + 3] * u[3];
+#pragma unroll
+    for (int j = 0; j < NSN; j++) {
+      const double* w = wvf + (k * NSN + j) * 4;
+      K[j] += w[0] * tv[0] + w[1] * tv[1] + w[2] * tv[2] + w[3] * tv[3];
+    }
+  }
+  double* out = R.blk + (R.soff ? R.soff[6 * (size_t)e + 1 + s] : ((size_t)e * 6 + 1 + s) * NSE * NSE);
+#pragma unroll
+  for (int j = 0; j < NSN; j++) out[j * NSE + i] = K[j];
 }
 
 // inverse of each element's own block by Gauss-Jordan elimination with
@@ -1916,6 +2138,18 @@ int launch_diag_t(DgProblem& D) {
   if (cnt == 0) return 0;
   DgRowArgs R;
   row_args(D, d, R);
+  constexpr int NS = num_shape(PX, PZ);
+  if constexpr (NS <= kValuMaxNs) {
+    if (NS <= D.valu_ns) {
+      const dim3 g((cnt + 64 / NS - 1) / (64 / NS));
+      if (D.mixed)
+        hipLaunchKernelGGL((k_dg_diag_v<PX, PZ, true>), g, dim3(64), 0, stream(), R, cnt);
+      else
+        hipLaunchKernelGGL((k_dg_diag_v<PX, PZ, false>), g, dim3(64), 0, stream(), R, cnt);
+      MF_HIP(hipGetLastError());
+      return 0;
+    }
+  }
   if (D.mixed)
     hipLaunchKernelGGL((k_dg_diag<PX, PZ, true>), dim3(cnt), dim3(64), 0, stream(), R);
   else
@@ -1932,6 +2166,16 @@ int launch_couple_t(DgProblem& D) {
   DgRowArgs R;
   row_args(D, de, R);
   for (int k = 0; k < 40; k++) R.exn[k] = D.ex9[dn][k];
+  constexpr int NSE = num_shape(PXE, PZE), NSN = num_shape(PXN, PZN);
+  if constexpr (NSE <= kValuMaxNs && NSN <= kValuMaxNs) {
+    if (NSE <= D.valu_ns && NSN <= D.valu_ns) {
+      const dim3 g((cnt + 64 / NSE - 1) / (64 / NSE));
+      hipLaunchKernelGGL((k_dg_couple_v<PXE, PZE, PXN, PZN>), g, dim3(64), 0, stream(), R,
+                         (const int*)D.d_slots + s0, cnt);
+      MF_HIP(hipGetLastError());
+      return 0;
+    }
+  }
   hipLaunchKernelGGL((k_dg_couple<PXE, PZE, PXN, PZN>), dim3(cnt), dim3(64), 0, stream(), R,
                      (const int*)D.d_slots + s0);
   MF_HIP(hipGetLastError());
@@ -2675,6 +2919,17 @@ int mf_dg_get_element_matrices(int did, int first, int count, double* stiff, dou
     MF_HIP(hipMemcpyAsync(rhs, D->d_rhs + first * ns, count * ns * sizeof(double),
                           hipMemcpyDeviceToHost, stream()));
   MF_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+int mf_dg_set_valu_blocks(int did, int max_ns) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (max_ns < 0) {
+    set_error("mf_dg_set_valu_blocks: max_ns >= 0 required");
+    return -1;
+  }
+  D->valu_ns = max_ns;
   return 0;
 }
 

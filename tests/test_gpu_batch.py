@@ -198,3 +198,60 @@ def test_tuned_tile_changes_no_iterate(use_graph):
     c.free()
     for _, p in probs:
         p.free()
+
+
+def member_case(k, rng):
+    """problem k of the mixed-assembly batch: prisms and tetrahedra, three
+    term sets, row-wise, two-pass and atomic assembly modes"""
+    kind = k % 6
+    shape = (int(rng.integers(3, 9)), int(rng.integers(3, 9)), int(rng.integers(2, 7)))
+    mesh = (mf.Mesh.box_tetra(*shape, seed=300 + k, jitter=0.2) if kind in (1, 4)
+            else mf.Mesh.box_prism(*shape, seed=300 + k, jitter=0.2))
+    p = mf.Problem.from_mesh(mesh)
+    nel = mesh.nel
+    kappa = np.exp(rng.normal(0.0, 0.5, nel))
+    cc = np.zeros(mf.NCOEF)
+    cc[mf.C_S] = 1.0
+    if kind in (0, 1):
+        p.set_coefficients(mf.IND_DIFF, mf.COEF_KAPPA, coef_const=cc, coef_el=kappa)
+    elif kind in (2, 4):
+        b = rng.uniform(-1, 1, (nel, 3))
+        p.set_coefficients(mf.IND_CONV_DIFF, mf.COEF_KAPPA_B, coef_const=cc,
+                           coef_el=np.concatenate([kappa, b[:, 0], b[:, 1], b[:, 2]]))
+    else:
+        full = rng.uniform(-0.3, 0.3, (21, nel))
+        full[0] += 2.0
+        full[4] += 2.0
+        full[8] += 2.0
+        p.set_coefficients(mf.IND_ALL, mf.COEF_FULL, coef_el=full.ravel())
+    p.set_assembly_mode([0, 2, 2, 0, 2, 1][kind] if k < 6 else [2, 0, 0, 2, 0, 1][kind])
+    return mesh, p
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_batch_assembly_matches_each_problems_own(precision):
+    """one launch per kernel for every row-wise and two-pass member (block
+    table over the members), the atomic member on its own: every member's
+    assembled system is bitwise the one its own assembly gives"""
+    rng = np.random.default_rng(11)
+    made = [member_case(k, rng) for k in range(12)]
+    ref = []
+    for _, p in made:
+        p.integrate_assemble(precision)
+        ref.append(p.system())
+    b = mf.Batch(krylbas=10)
+    for _, p in made:
+        p.set_system(np.zeros(p.nnz), np.zeros(p.n))
+        b.add(p, precon=1, max_iter=100, tol=1e-8)
+    b.integrate_assemble(precision)
+    for k, ((_, p), (v0, r0)) in enumerate(zip(made, ref)):
+        v, r = p.system()
+        if k % 6 == 5:  # element-parallel atomics: reproducible to rounding only
+            assert np.abs(v - v0).max() <= 1e-12 * np.abs(v0).max(), k
+            assert np.abs(r - r0).max() <= 1e-12 * np.abs(r0).max(), k
+        else:
+            assert np.array_equal(v, v0) and np.array_equal(r, r0), k
+    b.free()
+    for mesh, p in made:
+        p.free()
+        mesh.free()

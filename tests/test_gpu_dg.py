@@ -136,11 +136,14 @@ def block_of(dg, A):
     return worst
 
 
+@pytest.mark.parametrize("valu", [6, 0, 18])
 @pytest.mark.parametrize("pdeg", [101, 202, 303, 102, 201])
-def test_block_system_matches_oracle(pdeg):
+def test_block_system_matches_oracle(pdeg, valu):
     """volume and NIPG face terms with per-element anisotropic diffusion and
     convection (upwinding on both sides of interior faces), Dirichlet and
-    Neumann data from the exact solution, on a distorted mesh"""
+    Neumann data from the exact solution, on a distorted mesh; blocks of up
+    to `valu` shape functions from the vector-ALU kernels, the rest from the
+    MFMA kernels"""
     from oracle_lib import dg_system_dense
     mesh = cube_mesh(3, jitter=0.2, seed=pdeg)
     a = mesh.arrays()
@@ -151,6 +154,7 @@ def test_block_system_matches_oracle(pdeg):
     dg = mf.DgProblem(mesh, pdeg)
     dg.set_coefficients(ind, mf.COEF_FULL, coef_el=ce.ravel(), src_model=mf.SRC_DIFF_IN_CUBE)
     dg.set_penalty(1.5)
+    dg.set_valu_blocks(valu)
     dg.assemble()
     A, b = dg_system_dense(a, pdeg, ind, lambda e: ce[:, e], 1.5, mf.SRC_DIFF_IN_CUBE,
                            sparse=True)
@@ -264,8 +268,9 @@ def mixed_block_of(dg, A):
     return worst
 
 
+@pytest.mark.parametrize("valu", [6, 0, 18])
 @pytest.mark.parametrize("seed", [1, 2])
-def test_mixed_degree_block_system_matches_oracle(seed):
+def test_mixed_degree_block_system_matches_oracle(seed, valu):
     """elements of degree 1, 2 and 3 mixed at random: own and rectangular
     coupling blocks (face rule of the neighbour with more points, penalty of
     the smaller degree) and the load vector, with per-element anisotropic
@@ -282,6 +287,7 @@ def test_mixed_degree_block_system_matches_oracle(seed):
     assert dg.ns == 0 and dg.ndof == int(sum({101: 6, 202: 18, 303: 40}[p] for p in pd))
     dg.set_coefficients(ind, mf.COEF_FULL, coef_el=ce.ravel(), src_model=mf.SRC_DIFF_IN_CUBE)
     dg.set_penalty(1.5)
+    dg.set_valu_blocks(valu)
     dg.assemble()
     A, b = dg_system_dense(a, pd, ind, lambda e: ce[:, e], 1.5, mf.SRC_DIFF_IN_CUBE, sparse=True)
     assert mixed_block_of(dg, A) <= 1e-12
