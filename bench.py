@@ -915,6 +915,20 @@ def bcrs_cpu_time(R, P, en, em, nn, nb_ind, nb_cnt, res_bytes, gomp, threads):
                       "from its sources, OpenMP at one thread) on the full system" % reps}
 
 
+def device_setup_figures(mf, mesh, p, t_host):
+    """the problem set-up (DOF numbering, pattern, tables, device order)
+    built on the host and on the device; the device maps checked against
+    the host's (DOFs and pattern here, everything in the GPU tests)"""
+    t0 = time.perf_counter()
+    pd = mf.Problem.from_mesh(mesh, device_setup=True)
+    t_dev = time.perf_counter() - t0
+    same = bool(np.array_equal(pd.dofs(), p.dofs()) and
+                all(np.array_equal(a, b) for a, b in zip(pd.pattern(), p.pattern())))
+    pd.free()
+    return {"host_s": t_host, "device_s": t_dev, "speedup": t_host / t_dev,
+            "device_maps_equal_host": same}
+
+
 def other_workloads(args):
     """the batch (256 problems), served-stream and mixed-degree DG
     configurations, compact, measured in the same run"""
@@ -1013,7 +1027,9 @@ def main():
     import modfem_amd as mf
     t_setup = time.perf_counter()
     mesh, kappa, b, cc, coef_el = make_workload(args.n, args.seed)
+    t_host_setup = time.perf_counter()
     p = mf.Problem.from_mesh(mesh)
+    t_host_setup = time.perf_counter() - t_host_setup
     p.set_coefficients(mf.IND_CONV_DIFF, mf.COEF_KAPPA_B, coef_const=cc, coef_el=coef_el)
     p.set_assembly_mode(ASSEMBLY_MODE)
     t_setup = time.perf_counter() - t_setup
@@ -1137,6 +1153,7 @@ def main():
         "gmres_mixed": mixed,
         "ilu0_multicolour": ilu,
         "setup_s": t_setup,
+        "problem_setup": device_setup_figures(mf, mesh, p, t_host_setup),
     }
     if not args.quick:
         out["workloads"] = other_workloads(args)

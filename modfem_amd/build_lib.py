@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 
 HOST_SRCS = ["mf_mesh.cpp", "mf_setup.cpp", "mf_mkb.cpp"]
 HIP_SRCS = ["mf_device.hip", "mf_solver.hip", "mf_capi.hip", "mf_lar.hip", "mf_ilu.hip",
-            "mf_gs.hip", "mf_dg.hip", "mf_bcrs.hip", "mf_accel.hip"]
+            "mf_gs.hip", "mf_dg.hip", "mf_bcrs.hip", "mf_accel.hip", "mf_setup_dev.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -27,7 +27,7 @@ HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp"]
 # translation units whose kernels repeat the reference's rounding step by step:
 # no multiply-add contraction anywhere in them (hipcc contracts by default and
 # then ignores "#pragma clang fp contract")
-FILE_FLAGS = {"mf_bcrs.hip": ["-ffp-contract=off"]}
+FILE_FLAGS = {"mf_bcrs.hip": ["-ffp-contract=off"], "mf_setup_dev.hip": ["-ffp-contract=off"]}
 
 
 def _run(cmd):
