@@ -73,6 +73,11 @@ int mf_mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
                       unsigned long long seed, double jitter, int bc);
 int mf_mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
                       unsigned long long seed, double jitter, int bc);
+/* z-layers of hex cells split into 2 prisms or 6 tetrahedra each (a layer
+   is tetrahedral with probability tet_frac), conforming on the triangles
+   where the layers meet: the mixed prism/tet meshes of DG problems */
+int mf_mesh_box_hybrid(int nx, int ny, int nz, double lx, double ly, double lz,
+                       unsigned long long seed, double jitter, int bc, double tet_frac);
 int mf_mesh_sizes(int mesh, int* nr_nodes, int* nr_elems, int* nr_bfaces);
 int mf_mesh_get(int mesh, double* node_coor, int* el_type, int* el_nodes,
                 int* bf_elem, int* bf_side, int* bf_bc);

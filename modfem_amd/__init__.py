@@ -78,6 +78,7 @@ def _declare(L):
         "mf_mesh_write_jk": (I, [I, S]),
         "mf_mesh_box_prism": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
         "mf_mesh_box_tetra": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I]),
+        "mf_mesh_box_hybrid": (I, [I, I, I, D, D, D, ctypes.c_ulonglong, D, I, D]),
         "mf_mesh_sizes": (I, [I, V, V, V]),
         "mf_mesh_get": (I, [I, V, V, V, V, V, V]),
         "mf_mesh_free": (I, [I]),
@@ -353,6 +354,13 @@ class Mesh:
     @classmethod
     def box_tetra(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, seed=0, jitter=0.0, bc=1):
         return cls(check(lib().mf_mesh_box_tetra(nx, ny, nz, lx, ly, lz, seed, jitter, bc)))
+
+    @classmethod
+    def box_hybrid(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0, seed=0, jitter=0.0, bc=1, tet_frac=0.4):
+        """z-layers of prisms and of tetrahedra (a layer is tetrahedral with
+        probability tet_frac), conforming: for DG problems"""
+        return cls(check(lib().mf_mesh_box_hybrid(nx, ny, nz, lx, ly, lz, seed, jitter, bc,
+                                                  float(tet_frac))))
 
     @classmethod
     def read_jk(cls, path):

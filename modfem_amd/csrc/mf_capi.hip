@@ -263,6 +263,13 @@ int mf_mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
   return add_mesh(std::move(m));
 }
 
+int mf_mesh_box_hybrid(int nx, int ny, int nz, double lx, double ly, double lz,
+                       unsigned long long seed, double jitter, int bc, double tet_frac) {
+  auto m = std::make_unique<Mesh>();
+  if (mesh_box_hybrid(nx, ny, nz, lx, ly, lz, seed, jitter, bc, tet_frac, *m) < 0) return -1;
+  return add_mesh(std::move(m));
+}
+
 int mf_mesh_sizes(int mesh, int* nno, int* nel, int* nbf) {
   Mesh* m = get_mesh(mesh);
   if (!m) return -3;

@@ -48,6 +48,8 @@ int mesh_box_prism(int nx, int ny, int nz, double lx, double ly, double lz,
                    uint64_t seed, double jitter, int bc, Mesh& m);
 int mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
                    uint64_t seed, double jitter, int bc, Mesh& m);
+int mesh_box_hybrid(int nx, int ny, int nz, double lx, double ly, double lz,
+                    uint64_t seed, double jitter, int bc, double tet_frac, Mesh& m);
 
 // ----------------------------------------------------------------------
 // Setup (integer, bit-exact with the reference):

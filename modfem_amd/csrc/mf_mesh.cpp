@@ -571,4 +571,116 @@ int mesh_box_tetra(int nx, int ny, int nz, double lx, double ly, double lz,
   return 0;
 }
 
+
+// Hybrid box: every z-layer of hex cells is split either into 2 prisms per
+// cell (along the (0,0)-(1,1) xy-diagonal, as the prism box) or into 6
+// tetrahedra per cell (Kuhn split, as the tetrahedral box).  Both splits cut
+// the horizontal faces along that same diagonal, so prism and tet layers
+// meet on matching triangles and the mesh is conforming; tets touch prisms
+// only through those triangles.  A layer is tetrahedral with probability
+// tet_frac (seeded draws, layer by layer).  Node and element IDs are
+// permuted by the seed; boundary faces are listed element by element in
+// local face order (prisms: bottom, top, quads {0,1,4,3}, {1,2,5,4},
+// {2,0,3,5}; tets: {0,1,2}, {0,1,3}, {0,2,3}, {1,2,3}).
+int mesh_box_hybrid(int nx, int ny, int nz, double lx, double ly, double lz, uint64_t seed,
+                    double jitter, int bc, double tet_frac, Mesh& m) {
+  if (nx < 1 || ny < 1 || nz < 1 || tet_frac < 0 || tet_frac > 1) {
+    set_error("hybrid box mesh: nx, ny, nz >= 1 and 0 <= tet_frac <= 1 required");
+    return -1;
+  }
+  std::mt19937_64 rng(seed);
+  std::vector<int> tet_layer(nz);
+  long long ne = 0;
+  for (int k = 0; k < nz; k++) {
+    tet_layer[k] = unit_draw(rng) < tet_frac ? 1 : 0;
+    ne += (long long)nx * ny * (tet_layer[k] ? 6 : 2);
+  }
+  if (ne > 2147483647LL / 6) {
+    set_error("hybrid box mesh: too many elements");
+    return -1;
+  }
+  const int NN = (nx + 1) * (ny + 1) * (nz + 1), NE = (int)ne;
+  std::vector<int> pn = seeded_perm(NN, rng);
+  std::vector<int> pe = seeded_perm(NE, rng);
+  m = Mesh();
+  m.nno = NN;
+  m.coor.assign(3 * (size_t)NN, 0.0);
+  auto gid = [&](int i, int j, int k) { return i + (nx + 1) * (j + (ny + 1) * k); };
+  for (int k = 0; k <= nz; k++)
+    for (int j = 0; j <= ny; j++)
+      for (int i = 0; i <= nx; i++) {
+        const size_t v = pn[gid(i, j, k)];
+        m.coor[3 * v] = lx * i / nx;
+        m.coor[3 * v + 1] = ly * j / ny;
+        m.coor[3 * v + 2] = lz * k / nz;
+      }
+  m.nel = NE;
+  m.type.assign(NE, kPrism);
+  m.nodes.assign(6 * (size_t)NE, -1);
+  std::vector<int> grid(6 * (size_t)NE, -1);  // grid node ids, element order
+  static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2},
+                                  {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  int t = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        if (!tet_layer[k]) {
+          const int a = gid(i, j, k), b = gid(i + 1, j, k), c = gid(i + 1, j + 1, k),
+                    d = gid(i, j + 1, k), up = (nx + 1) * (ny + 1);
+          const int pr[2][3] = {{a, b, c}, {a, c, d}};
+          for (int q = 0; q < 2; q++, t++) {
+            const size_t e = pe[t];
+            for (int r = 0; r < 3; r++) {
+              grid[6 * e + r] = pr[q][r];
+              grid[6 * e + 3 + r] = pr[q][r] + up;
+            }
+          }
+        } else {
+          for (int q = 0; q < 6; q++, t++) {
+            int c[3] = {i, j, k}, vv[4];
+            vv[0] = gid(c[0], c[1], c[2]);
+            for (int s = 0; s < 3; s++) {
+              c[perms[q][s]]++;
+              vv[s + 1] = gid(c[0], c[1], c[2]);
+            }
+            // the Kuhn path i -> j -> k order is positive for even permutations
+            if (q == 1 || q == 2 || q == 5) std::swap(vv[2], vv[3]);
+            const size_t e = pe[t];
+            m.type[e] = kTetra;
+            for (int r = 0; r < 4; r++) grid[6 * e + r] = vv[r];
+          }
+        }
+      }
+  for (size_t q = 0; q < grid.size(); q++)
+    if (grid[q] >= 0) m.nodes[q] = pn[grid[q]];
+  // boundary faces: every vertex of the face on one side of the box
+  static const int pf[5][4] = {{0, 1, 2, -1}, {3, 4, 5, -1}, {0, 1, 4, 3}, {1, 2, 5, 4},
+                               {2, 0, 3, 5}};
+  static const int tf[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  const int hi[3] = {nx, ny, nz};
+  for (int e = 0; e < NE; e++) {
+    const bool tet = m.type[e] == kTetra;
+    for (int s = 0; s < (tet ? 4 : 5); s++) {
+      const int nv = tet || s < 2 ? 3 : 4;
+      bool on = false;
+      for (int d = 0; d < 3 && !on; d++) {
+        bool lo = true, up = true;
+        for (int r = 0; r < nv; r++) {
+          const int g = grid[6 * (size_t)e + (tet ? tf[s][r] : pf[s][r])];
+          const int ci[3] = {g % (nx + 1), (g / (nx + 1)) % (ny + 1), g / ((nx + 1) * (ny + 1))};
+          if (ci[d] != 0) lo = false;
+          if (ci[d] != hi[d]) up = false;
+        }
+        on = lo || up;
+      }
+      if (on) {
+        m.bf_elem.push_back(e); m.bf_side.push_back(s); m.bf_bc.push_back(bc);
+      }
+    }
+  }
+  m.nbf = (int)m.bf_elem.size();
+  jitter_interior(m, lx, ly, lz, nx, ny, nz, jitter, seed);
+  return 0;
+}
+
 }  // namespace mf

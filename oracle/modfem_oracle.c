@@ -2153,3 +2153,362 @@ void orc_block_ilu_solve(int nb, const int *ns, const long long *xoff, const int
     free(jj); free(pp);
   }
 }
+
+/* ---------------------------------------------------------------------
+ * DG on tetrahedra.  No reference counterpart: the reference's DG module is
+ * prism-only (apd_dg_prism/aph_dg_prism.h:1-3).  Built from the mathematics
+ * and the prism module's weak form:
+ *  - base: the complete polynomials of degree p in the reference
+ *    coordinates, xi^a eta^b zeta^c with a + b + c <= p, by total degree,
+ *    then a descending, then b descending; ns = (p+1)(p+2)(p+3)/6;
+ *  - reference vertices (0,0,0), (1,0,0), (0,1,0), (0,0,1) and the affine
+ *    map x = x0 + J xi, J = [x1 - x0, x2 - x0, x3 - x0];
+ *  - volume rule: collapsed Gauss-Legendre (xi = u, eta = v (1 - u),
+ *    zeta = w (1 - u)(1 - v), weight (1 - u)^2 (1 - v)), p + 2 points in u
+ *    and p + 1 in v and w: exact for every monomial of degree <= 2 p;
+ *  - sides s = 0..3 with the mesh module's vertex sets {0,1,2}, {0,1,3},
+ *    {0,2,3}, {1,2,3}; face point (c0, c1) of the reference triangle at
+ *    X_a + c0 (X_b - X_a) + c1 (X_c - X_a); the prism module's triangle rule
+ *    of degree 2 p; the normal points away from the opposite vertex;
+ *  - the prism module's NIPG face terms, penalty and upwinding
+ *    (pds_conv_diff_weakform_dg.c:232-1400) with the tet's p in place of
+ *    min(px, pz).  Faces between a tet and a prism are triangles; the
+ *    neighbour's local coordinates of a face point follow from the shared
+ *    vertices, as orc_dg_map_point does for prisms.
+ * ------------------------------------------------------------------- */
+static const int DGT_FACE_V[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+static const double DGT_X[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+/* reference vertices of the prism (aps_dg_prism: triangle x [-1, 1]) */
+static const double DGP_X[6][3] = {{0, 0, -1}, {1, 0, -1}, {0, 1, -1},
+                                   {0, 0, 1},  {1, 0, 1},  {0, 1, 1}};
+
+int orc_dgt_ns(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+
+int orc_dgt_shape(int p, const double *eta, double *phi, double *dx, double *dy, double *dz) {
+  if (p < 0 || p > 3) return -1;
+  double P[3][4], D[3][4];
+  for (int d = 0; d < 3; d++) {
+    const double x = eta[d], x2 = x * x;
+    P[d][0] = 1.0; P[d][1] = x; P[d][2] = x2; P[d][3] = x2 * x;
+    D[d][0] = 0.0; D[d][1] = 1.0; D[d][2] = 2.0 * x; D[d][3] = 3.0 * x2;
+  }
+  int n = 0;
+  for (int t = 0; t <= p; t++)
+    for (int a = t; a >= 0; a--)
+      for (int b = t - a; b >= 0; b--) {
+        const int c = t - a - b;
+        phi[n] = P[0][a] * P[1][b] * P[2][c];
+        dx[n] = D[0][a] * P[1][b] * P[2][c];
+        dy[n] = P[0][a] * D[1][b] * P[2][c];
+        dz[n] = P[0][a] * P[1][b] * D[2][c];
+        n++;
+      }
+  return n;
+}
+
+/* exponents a | b << 2 | c << 4 of the base, in base order */
+int orc_dgt_exponents(int p, int *ex) {
+  int n = 0;
+  for (int t = 0; t <= p; t++)
+    for (int a = t; a >= 0; a--)
+      for (int b = t - a; b >= 0; b--) ex[n++] = a | b << 2 | (t - a - b) << 4;
+  return n;
+}
+
+int orc_dgt_quadr(int p, double *xg, double *wg) {
+  if (p < 1 || p > 3) return -1;
+  double tu[8], wu[8], tv[8], wv[8];
+  const int nu = p + 2, nv = p + 1;
+  gauss_legendre(nu, tu, wu);
+  gauss_legendre(nv, tv, wv);
+  int g = 0;
+  for (int iu = 0; iu < nu; iu++)
+    for (int iv = 0; iv < nv; iv++)
+      for (int iw = 0; iw < nv; iw++) {
+        const double u = 0.5 * (tu[iu] + 1.0), v = 0.5 * (tv[iv] + 1.0), w = 0.5 * (tv[iw] + 1.0);
+        xg[3 * g] = u;
+        xg[3 * g + 1] = v * (1.0 - u);
+        xg[3 * g + 2] = w * (1.0 - u) * (1.0 - v);
+        wg[g] = 0.125 * wu[iu] * wv[iv] * wv[iw] * (1.0 - u) * (1.0 - u) * (1.0 - v);
+        g++;
+      }
+  return g;
+}
+
+/* affine map of a tet: J[3r+c] = dx_r/dxi_c, point x at eta; returns det */
+static double dgt_geom(const double *nc, const double *eta, double *J, double *x) {
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) J[3 * r + c] = nc[3 * (c + 1) + r] - nc[r];
+  for (int r = 0; r < 3; r++)
+    x[r] = nc[r] + J[3 * r] * eta[0] + J[3 * r + 1] * eta[1] + J[3 * r + 2] * eta[2];
+  return J[0] * (J[4] * J[8] - J[7] * J[5]) + J[3] * (J[7] * J[2] - J[1] * J[8]) +
+         J[6] * (J[1] * J[5] - J[4] * J[2]);
+}
+
+static int dgt_phys(int p, const double *nc, const double *eta, double *phi, double *dx,
+                    double *dy, double *dz) {
+  const int ns = orc_dgt_shape(p, eta, phi, dx, dy, dz);
+  double J[9], Ji[9], x[3];
+  dgt_geom(nc, eta, J, x);
+  orc_mat3_inv(J, Ji);
+  for (int i = 0; i < ns; i++) {
+    const double ax = dx[i] * Ji[0] + dy[i] * Ji[3] + dz[i] * Ji[6];
+    const double ay = dx[i] * Ji[1] + dy[i] * Ji[4] + dz[i] * Ji[7];
+    const double az = dx[i] * Ji[2] + dy[i] * Ji[5] + dz[i] * Ji[8];
+    dx[i] = ax; dy[i] = ay; dz[i] = az;
+  }
+  return ns;
+}
+
+/* element integral of a tet, the terms and column-major layout of
+   orc_dg_num_int_el */
+int orc_dgt_num_int_el(int p, const double *nc, const int *ind, const double *coef,
+                       int src_model, double *stiff, double *rhs) {
+  double xg[3 * 128], wg[128];
+  const int ng = orc_dgt_quadr(p, xg, wg);
+  if (ng < 0) return -1;
+  double phi[32], dx[32], dy[32], dz[32];
+  int ns = 0;
+  const double *a = coef + ORC_C_A, *b = coef + ORC_C_B, *q = coef + ORC_C_Q;
+  for (int ki = 0; ki < ng; ki++) {
+    const double *eta = &xg[3 * ki];
+    ns = orc_dgt_shape(p, eta, phi, dx, dy, dz);
+    double J[9], Ji[9], xc[3];
+    dgt_geom(nc, eta, J, xc);
+    const double det = orc_mat3_inv(J, Ji);
+    for (int i = 0; i < ns; i++) {
+      const double ax = dx[i] * Ji[0] + dy[i] * Ji[3] + dz[i] * Ji[6];
+      const double ay = dx[i] * Ji[1] + dy[i] * Ji[4] + dz[i] * Ji[7];
+      const double az = dx[i] * Ji[2] + dy[i] * Ji[5] + dz[i] * Ji[8];
+      dx[i] = ax; dy[i] = ay; dz[i] = az;
+    }
+    const double vol = fabs(det) * wg[ki];
+    double s = coef[ORC_C_S];
+    if (src_model != ORC_SRC_CONST) {
+      double u, lapl;
+      exact_sol(src_model, xc, &u, &lapl);
+      s = -lapl;
+    }
+    for (int j = 0; j < ns; j++)
+      for (int i = 0; i < ns; i++) {
+        double k = 0.0;
+        if (ind[ORC_IND_M]) k += coef[ORC_C_M] * phi[j] * phi[i] * vol;
+        if (ind[ORC_IND_AZZ])
+          k += (a[0] * dx[j] * dx[i] + a[1] * dy[j] * dx[i] + a[2] * dz[j] * dx[i] +
+                a[3] * dx[j] * dy[i] + a[4] * dy[j] * dy[i] + a[5] * dz[j] * dy[i] +
+                a[6] * dx[j] * dz[i] + a[7] * dy[j] * dz[i] + a[8] * dz[j] * dz[i]) * vol;
+        else if (ind[ORC_IND_AXX])
+          k += (a[0] * dx[j] * dx[i] + a[1] * dy[j] * dx[i] + a[3] * dx[j] * dy[i] +
+                a[4] * dy[j] * dy[i]) * vol;
+        if (ind[ORC_IND_BZ])
+          k += (b[0] * phi[j] * dx[i] + b[1] * phi[j] * dy[i] + b[2] * phi[j] * dz[i]) * vol;
+        else if (ind[ORC_IND_BX])
+          k += (b[0] * phi[j] * dx[i] + b[1] * phi[j] * dy[i]) * vol;
+        if (ind[ORC_IND_C]) k += coef[ORC_C_C] * phi[j] * phi[i] * vol;
+        stiff[j * ns + i] += k;
+      }
+    for (int i = 0; i < ns; i++) {
+      double f = 0.0;
+      if (ind[ORC_IND_QZ]) f += (q[0] * dx[i] + q[1] * dy[i] + q[2] * dz[i]) * vol;
+      else if (ind[ORC_IND_QX]) f += (q[0] * dx[i] + q[1] * dy[i]) * vol;
+      if (ind[ORC_IND_S]) f += s * phi[i] * vol;
+      rhs[i] += f;
+    }
+  }
+  return ns;
+}
+
+void orc_dgt_face_eta(int side, const double *xf, double *eta) {
+  const double *A = DGT_X[DGT_FACE_V[side][0]], *B = DGT_X[DGT_FACE_V[side][1]],
+               *C = DGT_X[DGT_FACE_V[side][2]];
+  for (int d = 0; d < 3; d++) eta[d] = A[d] + xf[0] * (B[d] - A[d]) + xf[1] * (C[d] - A[d]);
+}
+
+/* area element |(x_b - x_a) x (x_c - x_a)| (twice the face area; the face
+   rule's weights sum to 1/2) and the normal pointing away from the vertex
+   opposite the side; x = the point at eta */
+double orc_dgt_face_ds(int side, const double *nc, const double *eta, double *n, double *x) {
+  double J[9];
+  dgt_geom(nc, eta, J, x);
+  const int a = DGT_FACE_V[side][0], b = DGT_FACE_V[side][1], c = DGT_FACE_V[side][2];
+  const int o = 6 - a - b - c;
+  double u[3], v[3], ds[3], w[3];
+  for (int d = 0; d < 3; d++) {
+    u[d] = nc[3 * b + d] - nc[3 * a + d];
+    v[d] = nc[3 * c + d] - nc[3 * a + d];
+    w[d] = nc[3 * o + d] - nc[3 * a + d];
+  }
+  ds[0] = u[1] * v[2] - u[2] * v[1];
+  ds[1] = u[2] * v[0] - u[0] * v[2];
+  ds[2] = u[0] * v[1] - u[1] * v[0];
+  if (ds[0] * w[0] + ds[1] * w[1] + ds[2] * w[2] > 0) {
+    ds[0] = -ds[0]; ds[1] = -ds[1]; ds[2] = -ds[2];
+  }
+  const double det = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  n[0] = ds[0] / det; n[1] = ds[1] / det; n[2] = ds[2] / det;
+  return det;
+}
+
+double orc_dgt_face_area(const double *nc, int side) {
+  double n[3], x[3], eta[3];
+  const double xf[2] = {0.0, 0.0};
+  orc_dgt_face_eta(side, xf, eta);
+  return 0.5 * orc_dgt_face_ds(side, nc, eta, n, x);
+}
+
+/* local coordinates in the neighbour (type t2: 0 prism, 1 tet, side s2,
+   global node numbers g2) of a triangle-face point given by the local
+   coordinates eta of element (t1, s1, g1): the point's barycentric
+   coordinates on the face carried over through the shared vertices */
+int orc_dgx_map_point(int t1, int s1, const int *g1, const double *eta, int t2, int s2,
+                      const int *g2, double *eta2) {
+  const int *f1 = t1 ? DGT_FACE_V[s1] : DG_FACE_V[s1];
+  const int *f2 = t2 ? DGT_FACE_V[s2] : DG_FACE_V[s2];
+  double lam[3];
+  if (t1) {
+    const double bary[4] = {1.0 - eta[0] - eta[1] - eta[2], eta[0], eta[1], eta[2]};
+    for (int k = 0; k < 3; k++) lam[k] = bary[f1[k]];
+  } else {
+    if (s1 >= 2) return -1;
+    lam[0] = 1.0 - eta[0] - eta[1]; lam[1] = eta[0]; lam[2] = eta[1];
+  }
+  eta2[0] = eta2[1] = eta2[2] = 0.0;
+  for (int k = 0; k < 3; k++) {
+    int v2 = -1;
+    for (int m = 0; m < 3; m++)
+      if (g2[f2[m]] == g1[f1[k]]) v2 = f2[m];
+    if (v2 < 0) return -1;
+    const double *X = t2 ? DGT_X[v2] : DGP_X[v2];
+    for (int d = 0; d < 3; d++) eta2[d] += lam[k] * X[d];
+  }
+  if (!t2) {
+    if (s2 >= 2) return -1;
+    eta2[2] = s2 == 0 ? -1.0 : 1.0;
+  }
+  return 0;
+}
+
+/* one face with at least one tet (a triangle), the terms and layout of
+   orc_dg_face_int2: t = 0 prism (pdeg as there), 1 tet (pdeg = p) */
+int orc_dgx_face_int(int t1, int pdeg1, int t2, int pdeg2, int bc, const double *nc1,
+                     const int *g1, int side1, const double *coef1, const double *nc2,
+                     const int *g2, int side2, const double *coef2, double penalty, double hf,
+                     int src_model, double bval, double *stiff, double *rhs) {
+  if (!t1 && (bc != 0 ? 1 : !t2))
+    return orc_dg_face_int2(pdeg1, pdeg2, bc, nc1, g1, side1, coef1, nc2, g2, side2, coef2,
+                            penalty, hf, src_model, bval, stiff, rhs);
+  double xg[2 * 64], wg[64];
+  int ng = orc_dg_face_quadr(t1 ? pdeg1 : pdeg1 % 100, 0, xg, wg);
+  if (bc == 0) {
+    double xg2[2 * 64], wg2[64];
+    const int ng2 = orc_dg_face_quadr(t2 ? pdeg2 : pdeg2 % 100, 0, xg2, wg2);
+    if (!(ng > ng2)) {
+      ng = ng2;
+      memcpy(xg, xg2, sizeof(double) * 2 * ng2);
+      memcpy(wg, wg2, sizeof(double) * ng2);
+    }
+  }
+  double p1[64], x1[64], y1[64], z1[64], p2[64], x2[64], y2[64], z2[64];
+  int ns1 = 0, ns2 = 0, nf = 0;
+  for (int ki = 0; ki < ng; ki++) {
+    double eta[3], eta2[3], n[3], xc[3], area;
+    if (t1) {
+      orc_dgt_face_eta(side1, &xg[2 * ki], eta);
+      area = orc_dgt_face_ds(side1, nc1, eta, n, xc) * wg[ki];
+      ns1 = dgt_phys(pdeg1, nc1, eta, p1, x1, y1, z1);
+    } else {
+      orc_dg_face_eta(side1, &xg[2 * ki], eta);
+      area = orc_dg_face_ds(side1, nc1, eta, n, xc) * wg[ki];
+      ns1 = dg_phys(pdeg1, nc1, eta, p1, x1, y1, z1);
+    }
+    const double *a = coef1 + ORC_C_A, *b = coef1 + ORC_C_B;
+    double an1[3], an2[3] = {0, 0, 0};
+    for (int c = 0; c < 3; c++) an1[c] = a[c] * n[0] + a[3 + c] * n[1] + a[6 + c] * n[2];
+    const double bn1 = b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+    double bn2 = 0.0, nv = bn1;
+    if (bc == 0) {
+      if (orc_dgx_map_point(t1, side1, g1, eta, t2, side2, g2, eta2)) return -1;
+      ns2 = t2 ? dgt_phys(pdeg2, nc2, eta2, p2, x2, y2, z2) : dg_phys(pdeg2, nc2, eta2, p2, x2, y2, z2);
+      const double *a2 = coef2 + ORC_C_A, *b2 = coef2 + ORC_C_B;
+      for (int c = 0; c < 3; c++) an2[c] = a2[c] * n[0] + a2[3 + c] * n[1] + a2[6 + c] * n[2];
+      bn2 = b2[0] * n[0] + b2[1] * n[1] + b2[2] * n[2];
+      nv = 0.5 * (bn1 + bn2);
+    }
+    nf = bc == 0 ? ns1 + ns2 : ns1;
+    if (bc == 0) {
+      double gg1[64], gg2[64];
+      for (int i = 0; i < ns1; i++) gg1[i] = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
+      for (int i = 0; i < ns2; i++) gg2[i] = an2[0] * x2[i] + an2[1] * y2[i] + an2[2] * z2[i];
+      for (int j = 0; j < ns1; j++)
+        for (int i = 0; i < ns1; i++)
+          stiff[j * nf + i] += ((p1[j] * gg1[i] - p1[i] * gg1[j]) * 0.5 +
+                                penalty * p1[j] * p1[i] / hf) * area;
+      for (int j = 0; j < ns2; j++)
+        for (int i = 0; i < ns1; i++)
+          stiff[(ns1 + j) * nf + i] += ((-p2[j] * gg1[i] - p1[i] * gg2[j]) * 0.5 -
+                                        penalty * p2[j] * p1[i] / hf) * area;
+      for (int j = 0; j < ns1; j++)
+        for (int i = 0; i < ns2; i++)
+          stiff[j * nf + ns1 + i] += ((p1[j] * gg2[i] + p2[i] * gg1[j]) * 0.5 -
+                                      penalty * p1[j] * p2[i] / hf) * area;
+      for (int j = 0; j < ns2; j++)
+        for (int i = 0; i < ns2; i++)
+          stiff[(ns1 + j) * nf + ns1 + i] += ((-p2[j] * gg2[i] + p2[i] * gg2[j]) * 0.5 +
+                                              penalty * p2[j] * p2[i] / hf) * area;
+      if (nv > 0) {
+        for (int j = 0; j < ns1; j++) {
+          for (int i = 0; i < ns1; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+          for (int i = 0; i < ns2; i++) stiff[j * nf + ns1 + i] -= area * bn2 * p2[i] * p1[j];
+        }
+      } else if (nv < 0) {
+        for (int j = 0; j < ns2; j++) {
+          for (int i = 0; i < ns1; i++) stiff[(ns1 + j) * nf + i] += area * bn1 * p1[i] * p2[j];
+          for (int i = 0; i < ns2; i++) stiff[(ns1 + j) * nf + ns1 + i] -= area * bn2 * p2[i] * p2[j];
+        }
+      }
+    } else if (bc == 1) {
+      double f = bval;
+      if (src_model != ORC_SRC_CONST) exact_sol(src_model, xc, &f, NULL);
+      for (int j = 0; j < ns1; j++)
+        for (int i = 0; i < ns1; i++) {
+          const double gi = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
+          const double gj = an1[0] * x1[j] + an1[1] * y1[j] + an1[2] * z1[j];
+          stiff[j * nf + i] += (gi * p1[j] - gj * p1[i] + penalty * p1[j] * p1[i] / hf) * area;
+        }
+      for (int i = 0; i < ns1; i++) {
+        const double gi = an1[0] * x1[i] + an1[1] * y1[i] + an1[2] * z1[i];
+        rhs[i] += (f * gi + penalty * f * p1[i] / hf) * area;
+      }
+      if (nv < 0) {
+        for (int i = 0; i < ns1; i++) rhs[i] += -bn1 * f * p1[i] * area;
+      } else {
+        for (int j = 0; j < ns1; j++)
+          for (int i = 0; i < ns1; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+      }
+    } else {
+      double g = bval;
+      if (src_model == ORC_SRC_LINEAR) {
+        g = n[0] - 2.0 * n[1] + 0.5 * n[2];
+      } else if (src_model != ORC_SRC_CONST) {
+        const double a1 = src_model == ORC_SRC_DIFF_IN_BOX ? 100.0 : 1.0;
+        const double e = exp((-xc[0] * xc[0] - xc[1] * xc[1] - xc[2] * xc[2]) / a1);
+        g = (-2.0 * xc[0] * n[0] - 2.0 * xc[1] * n[1] - 2.0 * xc[2] * n[2]) * e / a1;
+      }
+      if (nv > 0)
+        for (int j = 0; j < ns1; j++)
+          for (int i = 0; i < ns1; i++) stiff[j * nf + i] += area * bn1 * p1[i] * p1[j];
+      for (int i = 0; i < ns1; i++) rhs[i] += g * p1[i] * area;
+    }
+  }
+  return nf;
+}
+
+/* value of a tet DG field at a local point and the point's coordinates */
+double orc_dgt_eval(int p, const double *nc, const double *dofs, const double *eta, double *x) {
+  double phi[32], dx[32], dy[32], dz[32], J[9];
+  const int ns = orc_dgt_shape(p, eta, phi, dx, dy, dz);
+  dgt_geom(nc, eta, J, x);
+  double u = 0.0;
+  for (int i = 0; i < ns; i++) u += dofs[i] * phi[i];
+  return u;
+}

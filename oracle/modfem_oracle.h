@@ -230,4 +230,23 @@ void orc_set_threads(int nthreads);
 #ifdef __cplusplus
 }
 #endif
+/* DG on tetrahedra (complete base of degree p; no reference counterpart) */
+int orc_dgt_ns(int p);
+int orc_dgt_shape(int p, const double *eta, double *phi, double *dx, double *dy, double *dz);
+int orc_dgt_exponents(int p, int *ex);
+int orc_dgt_quadr(int p, double *xg, double *wg);
+int orc_dgt_num_int_el(int p, const double *node_coor, const int *ind, const double *coef,
+                       int src_model, double *stiff, double *rhs);
+void orc_dgt_face_eta(int side, const double *xf, double *eta);
+double orc_dgt_face_ds(int side, const double *node_coor, const double *eta, double *normal,
+                       double *x);
+double orc_dgt_face_area(const double *node_coor, int side);
+int orc_dgx_map_point(int t1, int s1, const int *g1, const double *eta, int t2, int s2,
+                      const int *g2, double *eta2);
+int orc_dgx_face_int(int t1, int pdeg1, int t2, int pdeg2, int bc, const double *nc1,
+                     const int *g1, int side1, const double *coef1, const double *nc2,
+                     const int *g2, int side2, const double *coef2, double penalty, double hf,
+                     int src_model, double bval, double *stiff, double *rhs);
+double orc_dgt_eval(int p, const double *nc, const double *dofs, const double *eta, double *x);
+
 #endif

@@ -60,6 +60,17 @@ def oracle():
             "orc_dg_face_int": (I, [I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
             "orc_dg_face_int2": (I, [I, I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
             "orc_dg_eval": (D, [I, V, V, V, V]),
+            "orc_dgt_ns": (I, [I]),
+            "orc_dgt_shape": (I, [I, V, V, V, V, V]),
+            "orc_dgt_exponents": (I, [I, V]),
+            "orc_dgt_quadr": (I, [I, V, V]),
+            "orc_dgt_num_int_el": (I, [I, V, V, V, I, V, V]),
+            "orc_dgt_face_eta": (None, [I, V, V]),
+            "orc_dgt_face_ds": (D, [I, V, V, V, V]),
+            "orc_dgt_face_area": (D, [V, I]),
+            "orc_dgx_map_point": (I, [I, I, V, V, I, I, V, V]),
+            "orc_dgx_face_int": (I, [I, I, I, I, I, V, V, I, V, V, V, I, V, D, D, I, D, V, V]),
+            "orc_dgt_eval": (D, [I, V, V, V, V]),
             "orc_dg_errors": (I, [I, V, V, I, V, V]),
             "orc_dg_assemble_sample": (D, [I, I, V, V, I, V, V, V, D, I]),
             "orc_bcrs_pattern": (I, [I, V, V, V, V]),
@@ -284,30 +295,41 @@ def dg_quadr(pdeg):
     return xg[:3 * ng].reshape(ng, 3), wg[:ng]
 
 
-def dg_num_int_el(pdeg, nc, ind, coef, src_model=0):
+def dg_num_int_el(pdeg, nc, ind, coef, src_model=0, tet=False):
     """oracle DG element integral: (stiff[ns, ns] column-major as written,
-    i.e. stiff[j, i] = entry for trial j, test i; rhs[ns])"""
+    i.e. stiff[j, i] = entry for trial j, test i; rhs[ns]); a tet takes the
+    complete base of degree pdeg % 100 and its first four nodes"""
     st, rhs = np.zeros(128 * 128), np.zeros(128)
-    ns = oracle().orc_dg_num_int_el(pdeg, P(np.ascontiguousarray(nc, np.float64)),
-                                    P(np.ascontiguousarray(ind, np.int32)),
-                                    P(np.ascontiguousarray(coef, np.float64)), src_model,
-                                    P(st), P(rhs))
+    o = oracle()
+    if tet:
+        ns = o.orc_dgt_num_int_el(int(pdeg) % 100, P(np.ascontiguousarray(np.asarray(nc).reshape(-1)[:12], np.float64)),
+                                  P(np.ascontiguousarray(ind, np.int32)),
+                                  P(np.ascontiguousarray(coef, np.float64)), src_model, P(st), P(rhs))
+    else:
+        ns = o.orc_dg_num_int_el(pdeg, P(np.ascontiguousarray(nc, np.float64)),
+                                 P(np.ascontiguousarray(ind, np.int32)),
+                                 P(np.ascontiguousarray(coef, np.float64)), src_model,
+                                 P(st), P(rhs))
     assert ns > 0
     return st[:ns * ns].reshape(ns, ns), rhs[:ns]
 
 
 # prism face vertices (mmd_prism/mms_prism_intf.c:98), quads cyclic
 DG_FACE_V = [[0, 1, 2], [3, 4, 5], [0, 1, 4, 3], [1, 2, 5, 4], [2, 0, 3, 5]]
+# tetrahedron sides (the mesh module's face-node table)
+DGT_FACE_V = [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]
+TETRA = 7
 
 
-def dg_faces(el_nodes):
-    """faces of a prism mesh, each once: (e1, s1, e2, s2) with e1 < e2, or
-    (e, s, -1, -1) for a boundary face"""
+def dg_faces(el_nodes, el_type=None):
+    """faces of a prism / tet mesh, each once: (e1, s1, e2, s2) with e1 < e2,
+    or (e, s, -1, -1) for a boundary face"""
     seen = {}
     faces = []
     for e, nodes in enumerate(el_nodes):
-        for s in range(5):
-            key = tuple(sorted(int(nodes[v]) for v in DG_FACE_V[s]))
+        tet = el_type is not None and el_type[e] == TETRA
+        for s, fv in enumerate(DGT_FACE_V if tet else DG_FACE_V):
+            key = tuple(sorted(int(nodes[v]) for v in fv))
             if key in seen:
                 e1, s1 = seen.pop(key)
                 faces.append((e1, s1, e, s))
@@ -318,12 +340,15 @@ def dg_faces(el_nodes):
     return faces
 
 
-def dg_degrees(pdeg, nel):
+def dg_degrees(pdeg, nel, el_type=None):
     """per-element degrees, block sizes and block offsets (pdeg: one degree
-    or one per element)"""
+    or one per element; tets take the complete base of degree pdeg % 100)"""
     pd = np.full(nel, int(pdeg)) if np.isscalar(pdeg) else np.asarray(pdeg, int)
     px, pz = pd % 100, pd // 100
     ns = (px + 1) * (px + 2) // 2 * (pz + 1)
+    if el_type is not None:
+        tet = np.asarray(el_type) == TETRA
+        ns = np.where(tet, (px + 1) * (px + 2) * (px + 3) // 6, ns)
     return pd, ns, np.r_[0, np.cumsum(ns)]
 
 
@@ -333,30 +358,34 @@ def _dg_parts(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval, swap=F
     each once with the lower element as first neighbour (swap: the higher)"""
     o = oracle()
     coor, en = arrays["coor"], arrays["el_nodes"]
+    et = arrays.get("el_type")
     nel = en.shape[0]
     nodes = coor[en]
-    pd, ns, off = dg_degrees(pdeg, nel)
-    pmin = np.minimum(pd % 100, pd // 100)
+    pd, ns, off = dg_degrees(pdeg, nel, et)
+    tet = np.zeros(nel, bool) if et is None else np.asarray(et) == TETRA
+    pmin = np.where(tet, pd % 100, np.minimum(pd % 100, pd // 100))
+    # tets pass their complete degree p, prisms their pdeg
+    pk = np.where(tet, pd % 100, pd)
     for e in range(nel):
-        K, F = dg_num_int_el(int(pd[e]), nodes[e], ind, coef_of(e), src_model)
+        K, F = dg_num_int_el(int(pd[e]), nodes[e], ind, coef_of(e), src_model, tet=bool(tet[e]))
         yield np.arange(off[e], off[e + 1]), K.T, F
     bc_of = _dg_bc_types(arrays)
     st = np.zeros(80 * 80)
     rhs = np.zeros(80)
-    for (e1, s1, e2, s2) in dg_faces(en):
+    for (e1, s1, e2, s2) in dg_faces(en, et):
         if swap and e2 >= 0:
             e1, s1, e2, s2 = e2, s2, e1, s1
         st[:] = 0.0
         rhs[:] = 0.0
         nc1 = np.ascontiguousarray(nodes[e1].ravel())
         g1 = np.ascontiguousarray(en[e1], np.int32)
-        hf = np.sqrt(o.orc_dg_face_area(P(nc1), s1))
+        hf = np.sqrt(o.orc_dgt_face_area(P(nc1), s1) if tet[e1] else o.orc_dg_face_area(P(nc1), s1))
         c1 = np.ascontiguousarray(coef_of(e1), np.float64)
         if e2 < 0:
             pen = pmin[e1] * penalty_const
-            nf = o.orc_dg_face_int2(int(pd[e1]), int(pd[e1]), bc_of(e1, s1), P(nc1), P(g1), s1,
-                                    P(c1), None, None, 0, None, pen, hf, src_model, bval, P(st),
-                                    P(rhs))
+            nf = o.orc_dgx_face_int(int(tet[e1]), int(pk[e1]), int(tet[e1]), int(pk[e1]),
+                                    bc_of(e1, s1), P(nc1), P(g1), s1, P(c1), None, None, 0, None,
+                                    pen, hf, src_model, bval, P(st), P(rhs))
             assert nf == ns[e1]
             idx = np.arange(off[e1], off[e1 + 1])
         else:
@@ -364,8 +393,9 @@ def _dg_parts(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval, swap=F
             g2 = np.ascontiguousarray(en[e2], np.int32)
             c2 = np.ascontiguousarray(coef_of(e2), np.float64)
             pen = min(pmin[e1], pmin[e2]) * penalty_const
-            nf = o.orc_dg_face_int2(int(pd[e1]), int(pd[e2]), 0, P(nc1), P(g1), s1, P(c1), P(nc2),
-                                    P(g2), s2, P(c2), pen, hf, src_model, bval, P(st), P(rhs))
+            nf = o.orc_dgx_face_int(int(tet[e1]), int(pk[e1]), int(tet[e2]), int(pk[e2]), 0,
+                                    P(nc1), P(g1), s1, P(c1), P(nc2), P(g2), s2, P(c2), pen, hf,
+                                    src_model, bval, P(st), P(rhs))
             assert nf == ns[e1] + ns[e2]
             idx = np.r_[off[e1]:off[e1 + 1], off[e2]:off[e2 + 1]]
         yield idx, st[:nf * nf].reshape(nf, nf).T.copy(), rhs[:nf].copy()
@@ -378,7 +408,7 @@ def dg_system_dense(arrays, pdeg, ind, coef_of, penalty_const=1.0, src_model=0, 
     faces by their flags; dense, or scipy CSR with sparse=True.  pdeg: one
     degree or one per element (blocks at the offsets of dg_degrees)"""
     nel = arrays["el_nodes"].shape[0]
-    n = dg_degrees(pdeg, nel)[2][-1]
+    n = dg_degrees(pdeg, nel, arrays.get("el_type"))[2][-1]
     parts = _dg_parts(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval, swap)
     b = np.zeros(n)
     if sparse:
