@@ -580,6 +580,86 @@ def dg_work(dg, pd):
     return float(flops), float(op_bytes)
 
 
+def dg_work_hybrid(dg, pd, et):
+    """dg_work for prism / tet meshes: tets have (p+1)(p+2)(p+3)/6 functions,
+    (p+2)(p+1)^2 volume points and four triangle sides"""
+    nb, _, _ = dg.faces()
+    p = pd % 100
+    tet = et == 7
+    ns = np.where(tet, (p + 1) * (p + 2) * (p + 3) // 6, (p + 1) * (p + 2) // 2 * (p + 1))
+    tri = np.array([0, 3, 6, 12])
+    ngv = np.where(tet, (p + 2) * (p + 1) ** 2, tri[p] * (p + 1))
+    inner = nb >= 0
+    nbs = np.where(inner, nb, 0)
+    pn = np.where(inner, p[nbs], p[:, None])
+    q = np.maximum(p[:, None], pn)
+    side = np.arange(5)[None, :]
+    exists = ~(tet[:, None] & (side == 4))
+    is_tri = tet[:, None] | (side < 2)
+    pts = np.where(is_tri, tri[q], (q + 1) ** 2) * exists
+    nsn = np.where(inner, ns[nbs], 0)
+    flops = ((ngv + pts.sum(1)) * (8.0 * ns * ns + 32.0 * ns)).sum()
+    flops += (pts * inner * (8.0 * ns[:, None] * nsn + 32.0 * ns[:, None])).sum()
+    op_bytes = 8.0 * (2 * ns * ns + (ns[:, None] * nsn).sum(1) + 2 * ns + nsn.sum(1)).sum()
+    return float(flops), float(op_bytes)
+
+
+def bench_dg_hybrid(mf, args, cc):
+    """config 3's element mix: z-layers of the n^3 hex box split into prisms
+    or into tetrahedra (a layer is tetrahedral with probability tet_frac:
+    0.18 gives about 40% tets by count), degrees drawn from {1, 2, 3} per
+    element; bottom/top Neumann, sides Dirichlet; assembly, block-Jacobi
+    fill and operator, and (with --dg-solve hybrid) GMRES(30) + block ILU(0)
+    to 1e-8"""
+    m0 = mf.Mesh.box_hybrid(args.dg_n, args.dg_n, args.dg_n, seed=args.seed, jitter=0.2,
+                            tet_frac=args.dg_tet_frac)
+    a = m0.arrays()
+    m0.free()
+    X, en, et = a["coor"], a["el_nodes"], a["el_type"]
+    bc = a["bf_bc"].copy()
+    zmax = X[:, 2].max()
+    for k, (e, sd) in enumerate(zip(a["bf_elem"], a["bf_side"])):
+        fv = ([[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]][sd] if et[e] == 7 else
+              [[0, 1, 2], [3, 4, 5], [0, 1, 4, 3], [1, 2, 5, 4], [2, 0, 3, 5]][sd])
+        z = X[en[e][fv], 2]
+        if np.all(np.abs(z) < 1e-12) or np.all(np.abs(z - zmax) < 1e-12):
+            bc[k] = 11
+    mesh = mf.Mesh.from_arrays(X, et, en, a["bf_elem"], a["bf_side"], bc)
+    nel = mesh.nel
+    pd = np.random.default_rng(args.seed + 1).choice([101, 202, 303], size=nel)
+    dg = mf.DgProblem(mesh, pd)
+    dg.set_coefficients(mf.IND_DIFF, mf.COEF_UNIFORM, coef_const=cc, src_model=mf.SRC_DIFF_IN_CUBE)
+    t0 = time.perf_counter()
+    dg.assemble()
+    t_first = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        dg.assemble()
+    t0 = time.perf_counter()
+    dg.fill_precon()
+    t_prec = time.perf_counter() - t0
+    ms_asm, ms_op = dg.bench(max(args.steps, 1))
+    flops, op_bytes = dg_work_hybrid(dg, pd, et)
+    rec = {"elements": int(nel), "tets": int((et == 7).sum()), "prisms": int((et == 5).sum()),
+           "dofs": int(dg.ndof), "assemble_ms": ms_asm, "elements_per_s": nel / (ms_asm * 1e-3),
+           "assemble_tflops": flops / (ms_asm * 1e-3) / 1e12, "first_assemble_s": t_first,
+           "block_jacobi_ms": t_prec * 1e3, "op_ms": ms_op,
+           "op_GBs": op_bytes / (ms_op * 1e-3) / 1e9}
+    if "hybrid" in args.dg_solve:
+        t0 = time.perf_counter()
+        rec["ilu_levels"] = dg.fill_ilu()
+        rec["ilu_factor_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        x, info = dg.gmres(precon=3, krylbas=30, max_iter=args.dg_max_iter, tol=1e-8, fetch=False)
+        solve_s = time.perf_counter() - t0
+        rec["solve"] = {"preconditioner": "block ILU(0), element order", "gmres_iters": info["iters"],
+                        "gmres_converged": info["converged"], "gmres_s": solve_s,
+                        "gmres_res_rel": info["res_rel"]}
+    dg.free()
+    mesh.free()
+    print(json.dumps({"dg_hybrid": rec}), file=sys.stderr, flush=True)
+    return rec
+
+
 def bench_dg(args):
     """NIPG DG Poisson (DIFF_IN_CUBE: unit diffusion, manufactured source,
     Dirichlet sides, Neumann top/bottom, penalty constant 1) on a jittered
@@ -650,6 +730,8 @@ def bench_dg(args):
         res[name] = rec
         dg.free()
         print(json.dumps({"dg_level": rec}), file=sys.stderr, flush=True)
+    if args.dg_tet_frac > 0:
+        res["hybrid"] = bench_dg_hybrid(mf, args, cc)
     r = res["mixed"]
     out = {
         "metric": METRIC, "value": r["elements_per_s"], "unit": "elements integrated+assembled/s",
@@ -957,8 +1039,10 @@ def other_workloads(args):
     # config 3 at its size: assembly timings, then GMRES(30) with block
     # ILU(0) to 1e-8 (block Gauss-Seidel diverges on mixed degrees and block
     # Jacobi needs thousands of iterations)
-    a.dg_n, a.dg_pdegs, a.dg_solve, a.dg_precons, a.steps, a.warmup = 96, [], ["mixed"], [3], 3, 1
-    print("workload: DG mixed degrees 96^3, assembly and block-ILU(0) solve", file=sys.stderr, flush=True)
+    a.dg_n, a.dg_pdegs, a.dg_solve, a.dg_precons, a.steps, a.warmup = 96, [], ["mixed", "hybrid"], [3], 3, 1
+    a.dg_tet_frac = 0.18
+    print("workload: DG mixed degrees 96^3 (prisms, and prism/tet layers), assembly and "
+          "block-ILU(0) solves", file=sys.stderr, flush=True)
     r = bench_dg(a)
     out["dg_mixed_96"] = {k: r[k] for k in ("value", "unit", "config", "roofline", "roofline_op")}
     return out
@@ -983,6 +1067,9 @@ def main():
     ap.add_argument("--dg-pdegs", type=int, nargs="*", default=[101, 202, 303],
                     help="DG workload: uniform degrees run beside the mixed-degree case")
     ap.add_argument("--dg-max-iter", type=int, default=6000)
+    ap.add_argument("--dg-tet-frac", type=float, default=0.18,
+                    help="DG workload: share of tetrahedral z-layers in the hybrid prism/tet case "
+                         "(0.18: about 40%% tets by count); 0 skips the hybrid case")
     ap.add_argument("--dg-valu-ns", type=int, default=None,
                     help="DG blocks of at most this many shape functions assembled on the vector "
                          "ALUs (library default: %d)" % DG_VALU_NS)
@@ -990,8 +1077,9 @@ def main():
                     help="DG solves: preconditioners to run (1 block Jacobi, 2 block GS, 3 block "
                          "ILU(0)); default 1 3 for mixed degrees, 2 3 for uniform ones")
     ap.add_argument("--dg-solve", nargs="*", default=[],
-                    help="DG cases solved to 1e-8: mixed (block Jacobi, block ILU(0)) and/or the "
-                         "uniform degrees 101 202 303 (block Gauss-Seidel with 2 sweeps, block ILU(0))")
+                    help="DG cases solved to 1e-8: mixed (block Jacobi, block ILU(0)), hybrid "
+                         "(block ILU(0)) and/or the uniform degrees 101 202 303 (block "
+                         "Gauss-Seidel with 2 sweeps, block ILU(0))")
     ap.add_argument("--bcrs-n", type=int, default=48, help="BCRS workload: hexes per box edge")
     ap.add_argument("--bcrs-max-iter", type=int, default=3000)
     ap.add_argument("--bcrs-iluk", type=int, default=0,

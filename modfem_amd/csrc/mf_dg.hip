@@ -110,6 +110,48 @@ int volume_rule(int px, int pz, std::vector<double>& q) {
 constexpr int tri_points(int px) { return px == 1 ? 3 : px == 2 ? 6 : 12; }
 constexpr int num_shape(int px, int pz) { return (px + 1) * (px + 2) / 2 * (pz + 1); }
 
+// Tetrahedra (no reference counterpart: the reference's DG module is
+// prism-only).  Complete polynomial base of degree p in the reference
+// coordinates, xi^a eta^b zeta^c with a + b + c <= p, by total degree, then
+// a descending, then b descending; reference vertices (0,0,0), (1,0,0),
+// (0,1,0), (0,0,1); sides with the mesh module's vertex sets {0,1,2},
+// {0,1,3}, {0,2,3}, {1,2,3}; the prism module's triangle rule of degree 2p
+// on the sides.  Degree indices: 0..8 the prisms' (px, pz), 9..11 tets of
+// degree 1..3.
+constexpr int kNDeg = 12;
+constexpr int tet_ns(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+constexpr int tet_points(int p) { return (p + 2) * (p + 1) * (p + 1); }
+constexpr int kTetFaceV[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+
+// collapsed Gauss-Legendre rule of the tet, [ng][4]: xi = u, eta = v (1 - u),
+// zeta = w (1 - u)(1 - v), weight (1 - u)^2 (1 - v); p + 2 points in u and
+// p + 1 in v and w, exact for every monomial of degree <= 2p
+int tet_volume_rule(int p, std::vector<double>& q) {
+  double tu[8], wu[8], tv[8], wv[8];
+  const int nu = p + 2, nv = p + 1;
+  gauss_legendre(nu, tu, wu);
+  gauss_legendre(nv, tv, wv);
+  q.clear();
+  for (int iu = 0; iu < nu; iu++)
+    for (int iv = 0; iv < nv; iv++)
+      for (int iw = 0; iw < nv; iw++) {
+        const double u = 0.5 * (tu[iu] + 1.0), v = 0.5 * (tv[iv] + 1.0), w = 0.5 * (tv[iw] + 1.0);
+        q.push_back(u);
+        q.push_back(v * (1.0 - u));
+        q.push_back(w * (1.0 - u) * (1.0 - v));
+        q.push_back(0.125 * wu[iu] * wv[iv] * wv[iw] * (1.0 - u) * (1.0 - u) * (1.0 - v));
+      }
+  return nu * nv * nv;
+}
+
+// exponents a | b << 2 | c << 4 of the tet base in base order
+void tet_exponents(int p, int* ex) {
+  int n = 0;
+  for (int t = 0; t <= p; t++)
+    for (int a = t; a >= 0; a--)
+      for (int b = t - a; b >= 0; b--) ex[n++] = a | b << 2 | (t - a - b) << 4;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -429,7 +471,7 @@ struct DgProblem {
   int valu_ns = 6;
   // faces, element-centric: for element e and side s (index 5 e + s)
   int* d_fnb = nullptr;       // neighbour element, or -1 Dirichlet / -2 Neumann
-  int* d_fcode = nullptr;     // neighbour side | orientation << 4
+  int* d_fcode = nullptr;     // neighbour side (bits 0-2) | orientation (see build_faces)
   double* d_fhf = nullptr;    // sqrt(face area)
   double* d_fval = nullptr;   // boundary data when no exact solution is selected
   int* d_slots = nullptr;     // interior (element, side) slots 5 e + s
@@ -467,14 +509,14 @@ struct DgProblem {
   long long *d_xoff = nullptr, *d_soff = nullptr, *d_voff = nullptr;
   long long ndof = 0, blk_len = 0, inv_len = 0;
   // face rules of all nine degrees, concatenated in d_qt / d_qq
-  int qt_off[9] = {0}, ngt9[9] = {0}, qq_off[9] = {0}, ngq9[9] = {0};
-  int ex9[9][40] = {};
+  int qt_off[kNDeg] = {0}, ngt9[kNDeg] = {0}, qq_off[kNDeg] = {0}, ngq9[kNDeg] = {0};
+  int ex9[kNDeg][40] = {};
   // volume rules of the degrees present (mixed problems)
   double* d_qv9 = nullptr;
-  int qv_off[9] = {0};
-  // interior slots grouped by (own degree, neighbour degree): slot_ptr[81 + 1]
+  int qv_off[kNDeg] = {0};
+  // interior slots grouped by (own degree, neighbour degree): slot_ptr[kNDeg^2 + 1]
   std::vector<int> slot_ptr;
-  // Gauss-Seidel colours split by degree: gs_cd_ptr[9 c + d]
+  // Gauss-Seidel colours split by degree: gs_cd_ptr[kNDeg c + d]
   std::vector<int> gs_cd_ptr;
   // block ILU(0) in element order: factors on the block-row layout (d_ilu),
   // inverses of U's diagonal blocks (d_ilu_inv, inverse layout), rows by
@@ -611,10 +653,12 @@ void face_rules(int px, int pz, std::vector<double>& qt, std::vector<double>& qq
 }
 
 double face_area(const Mesh& m, int e, int s) {
-  const int nv = s < 2 ? 3 : 4;
+  const bool tet = m.type[e] == kTetra;
+  const int nv = tet || s < 2 ? 3 : 4;
   double p[4][3], d[4][3];
   for (int k = 0; k < nv; k++)
-    for (int r = 0; r < 3; r++) p[k][r] = m.coor[3 * (size_t)m.nodes[6 * (size_t)e + kFaceV[s][k]] + r];
+    for (int r = 0; r < 3; r++)
+      p[k][r] = m.coor[3 * (size_t)m.nodes[6 * (size_t)e + (tet ? kTetFaceV[s][k] : kFaceV[s][k])] + r];
   for (int k = 0; k < nv; k++)
     for (int r = 0; r < 3; r++) d[k][r] = p[(k + 1) % nv][r] - p[k][r];
   auto cross_len = [](const double* a, const double* b) {
@@ -628,9 +672,11 @@ double face_area(const Mesh& m, int e, int s) {
 
 // element-centric face table.  Pairs are matched by their sorted vertex
 // numbers; the orientation code lets an element carry a point of its own
-// face into the neighbour: triangles store, for each neighbour face vertex,
-// the matching vertex of the own face (2 bits each), quads whether the
-// bottom edge runs the other way.  The face size comes from the
+// face into the neighbour: between prisms, triangles store, for each
+// neighbour face vertex, the matching vertex of the own face (2 bits each),
+// quads whether the bottom edge runs the other way; a face with a tet on
+// either side sets bit 3 and stores, for each own face vertex, the
+// neighbour's local vertex (3 bits each).  A tet's fifth side is -3.  The face size comes from the
 // lower-numbered element's vertex cycle (both sides use the same value).
 int build_faces(const Mesh& m, std::vector<int>& fnb, std::vector<int>& fcode,
                 std::vector<double>& fhf) {
@@ -642,15 +688,23 @@ int build_faces(const Mesh& m, std::vector<int>& fnb, std::vector<int>& fcode,
     int v[4];
     int es;
   };
-  std::vector<Key> keys(nf);
+  auto is_tet = [&m](size_t e) { return m.type[e] == kTetra; };
+  // the element's local vertex of side s, position k
+  auto fv = [&](size_t e, int s, int k) { return is_tet(e) ? kTetFaceV[s][k] : kFaceV[s][k]; };
+  auto tri = [&](size_t e, int s) { return is_tet(e) || s < 2; };
+  std::vector<Key> keys;
+  keys.reserve(nf);
   for (size_t e = 0; e < nel; e++)
-    for (int s = 0; s < 5; s++) {
-      Key& k = keys[5 * e + s];
-      const int nv = s < 2 ? 3 : 4;
-      for (int t = 0; t < 4; t++) k.v[t] = t < nv ? m.nodes[6 * e + kFaceV[s][t]] : -1;
+    for (int s = 0; s < (is_tet(e) ? 4 : 5); s++) {
+      Key k;
+      const int nv = tri(e, s) ? 3 : 4;
+      for (int t = 0; t < 4; t++) k.v[t] = t < nv ? m.nodes[6 * e + fv(e, s, t)] : -1;
       std::sort(k.v, k.v + nv);
       k.es = (int)(5 * e + s);
+      keys.push_back(k);
     }
+  for (size_t e = 0; e < nel; e++)
+    if (is_tet(e)) fnb[5 * e + 4] = -3;  // a tet has four sides
   std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
     for (int t = 0; t < 4; t++)
       if (a.v[t] != b.v[t]) return a.v[t] < b.v[t];
@@ -658,10 +712,11 @@ int build_faces(const Mesh& m, std::vector<int>& fnb, std::vector<int>& fcode,
   });
   std::map<std::pair<int, int>, int> bc;
   for (int f = 0; f < m.nbf; f++) bc[{m.bf_elem[f], m.bf_side[f]}] = m.bf_bc[f];
+  const size_t nk = keys.size();
   size_t i = 0;
-  while (i < nf) {
+  while (i < nk) {
     size_t j = i + 1;
-    while (j < nf && std::equal(keys[j].v, keys[j].v + 4, keys[i].v)) j++;
+    while (j < nk && std::equal(keys[j].v, keys[j].v + 4, keys[i].v)) j++;
     if (j - i > 2) {
       set_error("DG faces: a face is shared by more than two elements");
       return -1;
@@ -678,18 +733,32 @@ int build_faces(const Mesh& m, std::vector<int>& fnb, std::vector<int>& fcode,
       fhf[keys[i].es] = std::sqrt(face_area(m, e1, s1));
     } else {
       const int e2 = keys[i + 1].es / 5, s2 = keys[i + 1].es % 5;
-      if ((s1 < 2) != (s2 < 2)) {
+      if (tri(e1, s1) != tri(e2, s2)) {
         set_error("DG faces: a triangle face meets a quad face");
         return -1;
       }
       const double hf = std::sqrt(face_area(m, std::min(e1, e2), e1 < e2 ? s1 : s2));
+      const bool generic = is_tet(e1) || is_tet(e2);
       for (int side = 0; side < 2; side++) {
         const int ea = side ? e2 : e1, sa = side ? s2 : s1, eb = side ? e1 : e2,
                   sb = side ? s1 : s2;
         const int* ga = &m.nodes[6 * (size_t)ea];
         const int* gb = &m.nodes[6 * (size_t)eb];
         int code = sb;
-        if (sa < 2) {
+        if (generic) {
+          // for each own face vertex k the neighbour's local vertex (3 bits)
+          code |= 8;
+          for (int k = 0; k < 3; k++) {
+            int v2 = -1;
+            for (int t = 0; t < 3; t++)
+              if (gb[fv(eb, sb, t)] == ga[fv(ea, sa, k)]) v2 = fv(eb, sb, t);
+            if (v2 < 0) {
+              set_error("DG faces: inconsistent triangle face");
+              return -1;
+            }
+            code |= v2 << (4 + 3 * k);
+          }
+        } else if (sa < 2) {
           for (int kb = 0; kb < 3; kb++) {
             int match = -1;
             for (int ka = 0; ka < 3; ka++)
@@ -771,7 +840,7 @@ __device__ __forceinline__ void face_eta(int s, const double* xf, double* eta) {
 
 // the same point in the neighbour across side s (code from build_faces)
 __device__ __forceinline__ void neighbour_eta(int s, int code, const double* eta, double* eta2) {
-  const int s2 = code & 15;
+  const int s2 = code & 7;
   if (s < 2) {
     const double lam[3] = {1.0 - eta[0] - eta[1], eta[0], eta[1]};
     const int m1 = (code >> 6) & 3, m2 = (code >> 8) & 3;
@@ -860,7 +929,7 @@ struct DgRowArgs {
   // rules [ngt[d]][3] at qt + 3 qt_off[d], quad rules likewise
   const double* qt;
   const double* qq;
-  int qt_off[9], ngt[9], qq_off[9], ngq[9];
+  int qt_off[kNDeg], ngt[kNDeg], qq_off[kNDeg], ngq[kNDeg];
   double penalty_const;
   const int* deg;      // degree index of every element
   const int* elems;    // elements of the launch (own blocks) or null: all, in order
@@ -877,8 +946,17 @@ struct DgRowArgs {
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 __host__ __device__ constexpr int deg_index(int px, int pz) { return 3 * (pz - 1) + (px - 1); }
-__host__ __device__ constexpr int ns_of_deg(int d) { return num_shape(d % 3 + 1, d / 3 + 1); }
-__host__ __device__ constexpr int pmin_of_deg(int d) { return d % 3 < d / 3 ? d % 3 + 1 : d / 3 + 1; }
+__host__ __device__ constexpr int tet_deg_index(int p) { return 8 + p; }
+// degree index of a launch's elements: a tet of degree PX, or a prism (PX, PZ)
+__host__ __device__ constexpr int deg_of(int px, int pz, bool tet) {
+  return tet ? tet_deg_index(px) : deg_index(px, pz);
+}
+__host__ __device__ constexpr int ns_of_deg(int d) {
+  return d >= 9 ? tet_ns(d - 8) : num_shape(d % 3 + 1, d / 3 + 1);
+}
+__host__ __device__ constexpr int pmin_of_deg(int d) {
+  return d >= 9 ? d - 8 : d % 3 < d / 3 ? d % 3 + 1 : d / 3 + 1;
+}
 
 // the rule of a face between element e (degree de) and its neighbour nb
 // (degree dn): the one with more points, the second neighbour's (the higher
@@ -916,6 +994,123 @@ __device__ __forceinline__ double neighbour_bn(const DgElemArgs& A, int nb, cons
   double a[9], b[3], mc, q0, q1, q2, sv;
   dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
   return b[0] * n[0] + b[1] * n[1] + b[2] * n[2];
+}
+
+// reference vertex coordinates: tets (0,0,0), (1,0,0), (0,1,0), (0,0,1);
+// prisms the triangle times [-1, 1]
+__device__ __forceinline__ void tet_vertex(int v, double* X) {
+  X[0] = v == 1 ? 1.0 : 0.0;
+  X[1] = v == 2 ? 1.0 : 0.0;
+  X[2] = v == 3 ? 1.0 : 0.0;
+}
+__device__ __forceinline__ void prism_vertex(int v, double* X) {
+  const int t = v % 3;
+  X[0] = t == 1 ? 1.0 : 0.0;
+  X[1] = t == 2 ? 1.0 : 0.0;
+  X[2] = v < 3 ? -1.0 : 1.0;
+}
+// local vertex k of tet side s ({0,1,2}, {0,1,3}, {0,2,3}, {1,2,3})
+__device__ __forceinline__ int tet_side_vertex(int s, int k) {
+  return k == 0 ? (s == 3 ? 1 : 0) : k == 1 ? (s == 0 || s == 1 ? 1 : s == 2 ? 2 : 2)
+                                            : (s == 0 ? 2 : 3);
+}
+
+// affine tet map at eta: J[3r + c] = x_{c+1, r} - x_{0, r}, its inverse and
+// the point; returns det J
+__device__ __forceinline__ double tet_jac(const double* xn, const double* eta, double* J,
+                                          double* Ji, double* x) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) J[3 * r + c] = xn[3 * (c + 1) + r] - xn[r];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+    x[r] = xn[r] + J[3 * r] * eta[0] + J[3 * r + 1] * eta[1] + J[3 * r + 2] * eta[2];
+  const double* m = J;
+  const double det = m[0] * (m[4] * m[8] - m[7] * m[5]) + m[3] * (m[7] * m[2] - m[1] * m[8]) +
+                     m[6] * (m[1] * m[5] - m[4] * m[2]);
+  const double r = 1.0 / det;
+  Ji[0] = (m[4] * m[8] - m[7] * m[5]) * r;
+  Ji[3] = (-m[3] * m[8] + m[6] * m[5]) * r;
+  Ji[6] = (m[3] * m[7] - m[6] * m[4]) * r;
+  Ji[1] = (m[7] * m[2] - m[1] * m[8]) * r;
+  Ji[4] = (m[0] * m[8] - m[6] * m[2]) * r;
+  Ji[7] = (-m[0] * m[7] + m[6] * m[1]) * r;
+  Ji[2] = (m[1] * m[5] - m[4] * m[2]) * r;
+  Ji[5] = (-m[0] * m[5] + m[3] * m[2]) * r;
+  Ji[8] = (m[0] * m[4] - m[3] * m[1]) * r;
+  return det;
+}
+
+// own side of a face point of a tet: side s's point (r0, r1) of the
+// reference triangle at X_a + r0 (X_b - X_a) + r1 (X_c - X_a); area element
+// |(x_b - x_a) x (x_c - x_a)| times the weight, normal away from the vertex
+// opposite the side
+__device__ __forceinline__ void face_point_tet(const DgElemArgs& A, const double* xe, int e,
+                                               int s, const double* rule, FacePt& F) {
+  const int a = tet_side_vertex(s, 0), b = tet_side_vertex(s, 1), c = tet_side_vertex(s, 2);
+  const int o = 6 - a - b - c;
+  double Xa[3], Xb[3], Xc[3], J[9], Ji[9];
+  tet_vertex(a, Xa);
+  tet_vertex(b, Xb);
+  tet_vertex(c, Xc);
+#pragma unroll
+  for (int d = 0; d < 3; d++) F.eta[d] = Xa[d] + rule[0] * (Xb[d] - Xa[d]) + rule[1] * (Xc[d] - Xa[d]);
+  tet_jac(xe, F.eta, J, Ji, F.x);
+  double u[3], v[3], w[3], ds[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    u[d] = xe[3 * b + d] - xe[3 * a + d];
+    v[d] = xe[3 * c + d] - xe[3 * a + d];
+    w[d] = xe[3 * o + d] - xe[3 * a + d];
+  }
+  ds[0] = u[1] * v[2] - u[2] * v[1];
+  ds[1] = u[2] * v[0] - u[0] * v[2];
+  ds[2] = u[0] * v[1] - u[1] * v[0];
+  if (ds[0] * w[0] + ds[1] * w[1] + ds[2] * w[2] > 0) {
+    ds[0] = -ds[0]; ds[1] = -ds[1]; ds[2] = -ds[2];
+  }
+  const double det = sqrt(ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2]);
+  F.n[0] = ds[0] / det; F.n[1] = ds[1] / det; F.n[2] = ds[2] / det;
+  F.area = det * rule[2];
+  double aa[9], b3[3], mc, q0, q1, q2, sv;
+  dg_coefs(A, e, aa, b3, mc, q0, q1, q2, sv);
+  double An[3];
+#pragma unroll
+  for (int cc = 0; cc < 3; cc++) An[cc] = aa[cc] * F.n[0] + aa[3 + cc] * F.n[1] + aa[6 + cc] * F.n[2];
+#pragma unroll
+  for (int k = 0; k < 3; k++) F.gh[k] = Ji[3 * k] * An[0] + Ji[3 * k + 1] * An[1] + Ji[3 * k + 2] * An[2];
+  F.bn = b3[0] * F.n[0] + b3[1] * F.n[1] + b3[2] * F.n[2];
+}
+
+// the neighbour's local coordinates of a face point: prism pairs by their
+// orientation code, faces with a tet through the shared vertices (code bit
+// 3: the neighbour's local vertex of each own face vertex in 3-bit fields)
+template <bool TE, bool TN>
+__device__ __forceinline__ void neighbour_eta_x(int s, int code, const double* eta, double* eta2) {
+  if (!TE && !TN) {
+    neighbour_eta(s, code, eta, eta2);
+    return;
+  }
+  double lam[3];
+  if (TE) {
+    const double bary[4] = {1.0 - eta[0] - eta[1] - eta[2], eta[0], eta[1], eta[2]};
+#pragma unroll
+    for (int k = 0; k < 3; k++) lam[k] = bary[tet_side_vertex(s, k)];
+  } else {
+    lam[0] = 1.0 - eta[0] - eta[1]; lam[1] = eta[0]; lam[2] = eta[1];
+  }
+  eta2[0] = eta2[1] = eta2[2] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int v2 = (code >> (4 + 3 * k)) & 7;
+    double X[3];
+    if (TN) tet_vertex(v2, X);
+    else prism_vertex(v2, X);
+#pragma unroll
+    for (int d = 0; d < 3; d++) eta2[d] += lam[k] * X[d];
+  }
+  if (!TN) eta2[2] = (code & 7) == 0 ? -1.0 : 1.0;
 }
 
 template <int PX, int PZ>
@@ -972,13 +1167,22 @@ struct LaneShapes {
 // of trial function (row) and test function (column) (Rg[12..27]) and the
 // load-vector row (Rg[28..31]).  Mixed problems pass each side's rule,
 // first point and penalty / h_F in fq, fst and fpen.
-template <int PX, int PZ, bool MIXED>
+// sizes of a launch's elements: a tet of degree PX (TET) or a prism (PX, PZ)
+template <int PX, int PZ, bool TET>
+struct DgEl {
+  static constexpr int NS = TET ? tet_ns(PX) : num_shape(PX, PZ);
+  static constexpr int NG = TET ? tet_points(PX) : tri_points(PX) * (PZ + 1);
+  static constexpr int NT = (NS + 15) / 16;
+  static constexpr int NSIDES = TET ? 4 : 5;
+  static constexpr int DE = deg_of(PX, PZ, TET);
+};
+
+template <int PX, int PZ, bool MIXED, bool TET = false>
 __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, const double* sX,
                                             const int* fq, const int* fst, const double* fpen,
                                             double* Rg) {
-  using S = DgShape<PX, PZ>;
   using Q = DgRowShape<PX, PZ>;
-  constexpr int NG = S::NG, DE = deg_index(PX, PZ);
+  constexpr int NG = DgEl<PX, PZ, TET>::NG, DE = DgEl<PX, PZ, TET>::DE;
   const DgElemArgs& A = R.E;
   double* M = Rg + 12;
 #pragma unroll
@@ -987,7 +1191,7 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
     const double* qg = A.qv + 4 * k;
     double a[9], b[3], mc, q0, q1, q2, sv, J[9], Ji[9], x[3];
     dg_coefs(A, e, a, b, mc, q0, q1, q2, sv);
-    const double vol = fabs(prism_jac(sX, qg, J, Ji, x)) * qg[3];
+    const double vol = fabs(TET ? tet_jac(sX, qg, J, Ji, x) : prism_jac(sX, qg, J, Ji, x)) * qg[3];
     if (A.src_model != SRC_CONST && A.mask.s) {
       double uu, lapl;
       exact_sol(A.src_model, x, &uu, &lapl);
@@ -1016,7 +1220,7 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
     if (MIXED) {
       while (k >= fst[sd + 1]) sd++;
       const int q = fq[sd];
-      rule = (sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * (k - fst[sd]);
+      rule = (TET || sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * (k - fst[sd]);
     } else {
       int fq0 = k - NG;
       if (fq0 < 2 * Q::NGT) {
@@ -1031,7 +1235,8 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
     const int fi = 5 * e + sd, nb = R.fnb[fi];
     const double pen = MIXED ? fpen[sd] : pmin_of_deg(DE) * R.penalty_const / R.fhf[fi];
     FacePt F;
-    face_point(A, sX, e, sd, rule, F);
+    if constexpr (TET) face_point_tet(A, sX, e, sd, rule, F);
+    else face_point(A, sX, e, sd, rule, F);
     put_powers(F.eta, Rg);
     if (nb >= 0) {
       // interior: 0.5 (phi_j g_i - g_j phi_i) + (pen + upwind) phi_j phi_i
@@ -1079,13 +1284,13 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
 // (phi, d/deta phi) of trial and test function, the load-vector row) to
 // LDS, and the wave runs the chunk through v_mfma_f64_16x16x4, one K-step
 // per point.
-template <int PX, int PZ, bool MIXED>
+template <int PX, int PZ, bool MIXED, bool TET = false>
 __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
-  using S = DgShape<PX, PZ>;
+  using S = DgEl<PX, PZ, TET>;
   using Q = DgRowShape<PX, PZ>;
   // records of 32 doubles at an odd stride: the lanes writing their own
   // records fall on different LDS banks
-  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, DE = deg_index(PX, PZ), REC = 33;
+  constexpr int NS = S::NS, NG = S::NG, NT = S::NT, DE = S::DE, REC = 33, NSD = S::NSIDES;
   __shared__ double rec[64 * REC];
   __shared__ double sX[18];
   __shared__ int sex[40];
@@ -1096,26 +1301,27 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   const int e = R.elems ? R.elems[blockIdx.x] : blockIdx.x;
   if (lane < 40) sex[lane] = R.ex[lane];
   if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
-  if (MIXED && lane < 5) {
+  if (MIXED && lane < NSD) {
     // each face's rule and penalty / h_F (weakform_dg.c:508-531)
     const int sd = lane, nb = R.fnb[5 * e + sd];
+    const bool tri = TET || sd < 2;
     int q = DE, pm = pmin_of_deg(DE);
     if (nb >= 0) {
       const int dn = R.deg[nb];
-      q = face_rule_deg(R, sd < 2, e, DE, nb, dn);
+      q = face_rule_deg(R, tri, e, DE, nb, dn);
       pm = min(pm, pmin_of_deg(dn));
     }
     fq[sd] = q;
-    fst[sd + 1] = sd < 2 ? R.ngt[q] : R.ngq[q];
+    fst[sd + 1] = tri ? R.ngt[q] : R.ngq[q];
     fpen[sd] = pm * R.penalty_const / R.fhf[5 * e + sd];
   }
   __syncthreads();
   if (MIXED && lane == 0) {
     fst[0] = NG;
-    for (int sd = 0; sd < 5; sd++) fst[sd + 1] += fst[sd];
+    for (int sd = 0; sd < NSD; sd++) fst[sd + 1] += fst[sd];
   }
   __syncthreads();
-  const int NPTS = MIXED ? fst[5] : Q::NPTS;
+  const int NPTS = MIXED ? fst[NSD] : Q::NPTS;
   const int p = lane >> 4, c16 = lane & 15;
   LaneShapes<NS, NT> L;
   L.init(sex, c16);
@@ -1132,7 +1338,7 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   for (int c0 = 0; c0 < NPTS; c0 += 64) {
     const int k = c0 + lane;
     if (k < NPTS) {
-      diag_record<PX, PZ, MIXED>(R, e, k, sX, fq, fst, fpen, rec + REC * lane);
+      diag_record<PX, PZ, MIXED, TET>(R, e, k, sX, fq, fst, fpen, rec + REC * lane);
     }
     __syncthreads();
     const int nk = NPTS - c0 < 64 ? NPTS - c0 : 64;
@@ -1182,18 +1388,20 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
 // 4x4 matrix M between (psi, d/deta psi) of the neighbour's trial function
 // (row) and (phi, d/deta phi) of the element's test function (column)
 // (Rg[24..39]).  sX holds both elements' node coordinates.
-template <int PXE, int PZE, int PXN, int PZN>
+template <int PXE, int PZE, int PXN, int PZN, bool TE = false, bool TN = false>
 __device__ __forceinline__ void couple_record(const DgRowArgs& R, int e, int s, int nb, int code,
                                               int q, int kp, const double* sX, double* Rg) {
-  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN);
+  constexpr int DE = deg_of(PXE, PZE, TE), DN = deg_of(PXN, PZN, TN);
   const DgElemArgs& A = R.E;
   const int slot = 5 * e + s;
-  const double* rule = (s < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * kp;
+  const double* rule = (TE || TN || s < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * kp;
   FacePt F;
-  face_point(A, sX, e, s, rule, F);
+  if constexpr (TE) face_point_tet(A, sX, e, s, rule, F);
+  else face_point(A, sX, e, s, rule, F);
   double eta2[3], J2[9], Ji2[9], x2[3];
-  neighbour_eta(s, code, F.eta, eta2);
-  prism_jac(sX + 18, eta2, J2, Ji2, x2);
+  neighbour_eta_x<TE, TN>(s, code, F.eta, eta2);
+  if constexpr (TN) tet_jac(sX + 18, eta2, J2, Ji2, x2);
+  else prism_jac(sX + 18, eta2, J2, Ji2, x2);
   double a[9], b[3], mc, q0, q1, q2, sv;
   dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
   double An[3], hh[3];
@@ -1224,12 +1432,13 @@ __device__ __forceinline__ void couple_record(const DgRowArgs& R, int e, int s, 
 // (the (1,2) block of pdr_conv_diff_comp_fa_stiff_mat_app seen from either
 // side), ns_e x ns_nb, column-major.  Records hold the powers of both
 // elements' local coordinates.
-template <int PXE, int PZE, int PXN, int PZN>
+template <int PXE, int PZE, int PXN, int PZN, bool TET_E = false, bool TET_N = false>
 __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots) {
-  constexpr int NSE = num_shape(PXE, PZE), NTE = (NSE + 15) / 16;
-  constexpr int NSN = num_shape(PXN, PZN), NTN = (NSN + 15) / 16;
-  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN), REC = 41;  // 40 + bank pad
-  constexpr int QE = (PXE + 1) * (PZE + 1), QN = (PXN + 1) * (PZN + 1);
+  constexpr int NSE = DgEl<PXE, PZE, TET_E>::NS, NTE = (NSE + 15) / 16;
+  constexpr int NSN = DgEl<PXN, PZN, TET_N>::NS, NTN = (NSN + 15) / 16;
+  constexpr int DE = deg_of(PXE, PZE, TET_E), DN = deg_of(PXN, PZN, TET_N), REC = 41;  // 40 + bank pad
+  constexpr bool TRI = TET_E || TET_N;  // a face with a tet is a triangle
+  constexpr int QE = TRI ? 0 : (PXE + 1) * (PZE + 1), QN = TRI ? 0 : (PXN + 1) * (PZN + 1);
   constexpr int TE = tri_points(PXE), TN = tri_points(PXN);
   constexpr int NPMAX = cmax(cmax(QE, QN), cmax(TE, TN));
   __shared__ double rec[NPMAX * REC];
@@ -1240,8 +1449,8 @@ __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots)
   const int slot = slots[blockIdx.x];
   const int e = slot / 5, s = slot % 5;
   const int nb = R.fnb[slot], code = R.fcode[slot];
-  const int q = face_rule_deg(R, s < 2, e, DE, nb, DN);
-  const int npf = s < 2 ? R.ngt[q] : R.ngq[q];
+  const int q = face_rule_deg(R, TRI || s < 2, e, DE, nb, DN);
+  const int npf = TRI || s < 2 ? R.ngt[q] : R.ngq[q];
   if (lane < 40) {
     sex[lane] = R.ex[lane];
     sexn[lane] = R.exn[lane];
@@ -1249,7 +1458,8 @@ __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots)
   if (lane < 18) sX[lane] = A.nc[(size_t)lane * A.nel + e];
   else if (lane < 36) sX[lane] = A.nc[(size_t)(lane - 18) * A.nel + nb];
   __syncthreads();
-  if (lane < npf) couple_record<PXE, PZE, PXN, PZN>(R, e, s, nb, code, q, lane, sX, rec + REC * lane);
+  if (lane < npf)
+    couple_record<PXE, PZE, PXN, PZN, TET_E, TET_N>(R, e, s, nb, code, q, lane, sX, rec + REC * lane);
   __syncthreads();
   const int p = lane >> 4, c16 = lane & 15;
   LaneShapes<NSE, NTE> L;
@@ -1319,11 +1529,11 @@ __device__ __forceinline__ void shape_from_powers(const double* P, int code, dou
   u[3] = pa * pb * qc;
 }
 
-template <int PX, int PZ, bool MIXED>
+template <int PX, int PZ, bool MIXED, bool TET = false>
 __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
-  using S = DgShape<PX, PZ>;
+  using S = DgEl<PX, PZ, TET>;
   using Q = DgRowShape<PX, PZ>;
-  constexpr int NS = S::NS, NG = S::NG, DE = deg_index(PX, PZ), REC = 33;  // 32 + bank pad
+  constexpr int NS = S::NS, NG = S::NG, DE = S::DE, REC = 33, NSD = S::NSIDES;  // 32 + bank pad
   constexpr int EPW = 64 / NS;
   // values of the element's functions at the round's points, element-major
   // with an odd stride between elements (bank spread of the broadcast reads)
@@ -1343,17 +1553,18 @@ __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
   if (lane < NS) sex[lane] = R.ex[lane];
   if (act)
     for (int c = i; c < 18; c += NS) sX[le][c] = A.nc[(size_t)c * A.nel + e];
-  if (MIXED && act && i < 5) {
+  if (MIXED && act && i < NSD) {
     // each face's rule and penalty / h_F (weakform_dg.c:508-531)
     const int sd = i, nb = R.fnb[5 * e + sd];
+    const bool tri = TET || sd < 2;
     int q = DE, pm = pmin_of_deg(DE);
     if (nb >= 0) {
       const int dn = R.deg[nb];
-      q = face_rule_deg(R, sd < 2, e, DE, nb, dn);
+      q = face_rule_deg(R, tri, e, DE, nb, dn);
       pm = min(pm, pmin_of_deg(dn));
     }
     fq[le][sd] = q;
-    fst[le][sd + 1] = sd < 2 ? R.ngt[q] : R.ngq[q];
+    fst[le][sd + 1] = tri ? R.ngt[q] : R.ngq[q];
     fpen[le][sd] = pm * R.penalty_const / R.fhf[5 * e + sd];
   }
   __syncthreads();
@@ -1362,8 +1573,8 @@ __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
     if (blockIdx.x * EPW + lane < cnt) {
       if (MIXED) {
         fst[lane][0] = NG;
-        for (int sd = 0; sd < 5; sd++) fst[lane][sd + 1] += fst[lane][sd];
-        np = fst[lane][5];
+        for (int sd = 0; sd < NSD; sd++) fst[lane][sd + 1] += fst[lane][sd];
+        np = fst[lane][NSD];
       } else {
         np = Q::NPTS;
       }
@@ -1383,7 +1594,7 @@ __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
   for (int c0 = 0; c0 < nmax; c0 += NS) {
     const int k = c0 + i;
     if (act && k < npts)
-      diag_record<PX, PZ, MIXED>(R, e, k, sX[le], fq[le], fst[le], fpen[le], rec[le][i]);
+      diag_record<PX, PZ, MIXED, TET>(R, e, k, sX[le], fq[le], fst[le], fpen[le], rec[le][i]);
     __syncthreads();
     const int nk = min(NS, npts - c0);
     double* uve = uv + le * UVS;
@@ -1415,11 +1626,12 @@ __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
   R.brhs[(R.xoff ? R.xoff[e] : (size_t)e * NS) + i] = f;
 }
 
-template <int PXE, int PZE, int PXN, int PZN>
+template <int PXE, int PZE, int PXN, int PZN, bool TET_E = false, bool TET_N = false>
 __global__ void __launch_bounds__(64) k_dg_couple_v(DgRowArgs R, const int* slots, int cnt) {
-  constexpr int NSE = num_shape(PXE, PZE), NSN = num_shape(PXN, PZN);
-  constexpr int DE = deg_index(PXE, PZE), DN = deg_index(PXN, PZN), REC = 41;  // 40 + bank pad
-  constexpr int QE = (PXE + 1) * (PZE + 1), QN = (PXN + 1) * (PZN + 1);
+  constexpr int NSE = DgEl<PXE, PZE, TET_E>::NS, NSN = DgEl<PXN, PZN, TET_N>::NS;
+  constexpr int DE = deg_of(PXE, PZE, TET_E), DN = deg_of(PXN, PZN, TET_N), REC = 41;  // 40 + bank pad
+  constexpr bool TRI = TET_E || TET_N;  // a face with a tet is a triangle
+  constexpr int QE = TRI ? 0 : (PXE + 1) * (PZE + 1), QN = TRI ? 0 : (PXN + 1) * (PZN + 1);
   constexpr int TE = tri_points(PXE), TN = tri_points(PXN);
   constexpr int NPMAX = cmax(cmax(QE, QN), cmax(TE, TN));
   constexpr int FPW = 64 / NSE;  // interior sides per wave
@@ -1435,8 +1647,8 @@ __global__ void __launch_bounds__(64) k_dg_couple_v(DgRowArgs R, const int* slot
   const int slot = act ? slots[t] : 0;
   const int e = slot / 5, s = slot % 5;
   const int nb = act ? R.fnb[slot] : 0, code = act ? R.fcode[slot] : 0;
-  const int q = act ? face_rule_deg(R, s < 2, e, DE, nb, DN) : 0;
-  const int npf = act ? (s < 2 ? R.ngt[q] : R.ngq[q]) : 0;
+  const int q = act ? face_rule_deg(R, TRI || s < 2, e, DE, nb, DN) : 0;
+  const int npf = act ? (TRI || s < 2 ? R.ngt[q] : R.ngq[q]) : 0;
   if (lane < NSE) sex[lane] = R.ex[lane];
   if (lane < NSN) sexn[lane] = R.exn[lane];
   if (act) {
@@ -1449,7 +1661,8 @@ __global__ void __launch_bounds__(64) k_dg_couple_v(DgRowArgs R, const int* slot
 #pragma unroll
   for (int r = 0; r < RR; r++) {
     const int k = i + r * NSE;
-    if (k < npf) couple_record<PXE, PZE, PXN, PZN>(R, e, s, nb, code, q, k, sX[lf], rec[lf][k]);
+    if (k < npf)
+      couple_record<PXE, PZE, PXN, PZN, TET_E, TET_N>(R, e, s, nb, code, q, k, sX[lf], rec[lf][k]);
   }
   __syncthreads();
   double* wvf = wv + lf * WVS;
@@ -1576,6 +1789,9 @@ __device__ __forceinline__ double nb_colsum(double acc, const double* B, int ld,
     case 0: return colsum<6, SIGN>(acc, B, ld, i, x);
     case 4: return colsum<18, SIGN>(acc, B, ld, i, x);
     case 8: return colsum<40, SIGN>(acc, B, ld, i, x);
+    case 9: return colsum<4, SIGN>(acc, B, ld, i, x);
+    case 10: return colsum<10, SIGN>(acc, B, ld, i, x);
+    case 11: return colsum<20, SIGN>(acc, B, ld, i, x);
   }
   const int n = ns_of_deg(dn);
   for (int k = 0; k < n; k++) {
@@ -1780,7 +1996,7 @@ __global__ void __launch_bounds__(64) k_dg_ilu_row(int cnt, const int* rows, DgI
     __syncthreads();
     for (int q = lane; q < ni * nk; q += 64) Aik[q] = T[q];
     // D_i -= a_ik a_ki (a_ki: row k's block across the side facing i)
-    const int sk = Y.fcode[5 * (size_t)i + s] & 15;
+    const int sk = Y.fcode[5 * (size_t)i + s] & 7;  // bits 0-2: the neighbour's side
     const double* Aki = F + Y.so(k, 1 + sk);
     for (int q = lane; q < ni * ni; q += 64) {
       const int r = q % ni, c = q / ni;
@@ -1975,6 +2191,7 @@ __global__ void k_dg_sub(long long n, const double* y, double* w, const int* sto
 template <class F>
 static int with_ns(int d, F& f) {
   switch (ns_of_deg(d)) {
+    case 4: return f.template run<4>(d);
     case 6: return f.template run<6>(d);
     case 9: return f.template run<9>(d);
     case 10: return f.template run<10>(d);
@@ -2022,7 +2239,7 @@ struct DgOp : ExtOp {
                            nullptr, layout(), D->d_blk, precon == 1 ? D->d_dinv : nullptr, x_, b_,
                            sc_, j_, stop_, w_, mode_);
     } else {  // the elements of degree d in colour color_
-      const size_t k = 9 * (size_t)color_ + d;
+      const size_t k = kNDeg * (size_t)color_ + d;
       const int c0 = D->gs_cd_ptr[k], cnt = D->gs_cd_ptr[k + 1] - c0;
       if (D->mixed)
         hipLaunchKernelGGL((k_dg_gs<NS, true>), dim3(grid(cnt, NS)), dim3(256), 0, stream(), cnt,
@@ -2039,7 +2256,7 @@ struct DgOp : ExtOp {
   int op(const double* x, const double* b, const double* sc, int j, const int* stop, double* w,
          int mode) {
     x_ = x; b_ = b; sc_ = sc; j_ = j; stop_ = stop; w_ = w; mode_ = mode; color_ = -1;
-    for (int d = 0; d < 9; d++)
+    for (int d = 0; d < kNDeg; d++)
       if (D->grp_ptr[d + 1] > D->grp_ptr[d] && with_ns(d, *this)) return -2;
     return 0;
   }
@@ -2048,8 +2265,8 @@ struct DgOp : ExtOp {
     b_ = b; w_ = y; stop_ = stop;
     for (int k = 0; k < D->gs_sweeps; k++)
       for (int c = 0; c < ncol; c++)
-        for (int d = 0; d < 9; d++) {
-          const size_t q = 9 * (size_t)c + d;
+        for (int d = 0; d < kNDeg; d++) {
+          const size_t q = kNDeg * (size_t)c + d;
           if (D->gs_cd_ptr[q + 1] == D->gs_cd_ptr[q]) continue;
           color_ = c;
           if (with_ns(d, *this)) return -2;
@@ -2118,7 +2335,7 @@ void row_args(const DgProblem& D, int d, DgRowArgs& R) {
   A.coef = D.d_coef; A.stiff = nullptr; A.rhs = nullptr;
   R.fnb = D.d_fnb; R.fcode = D.d_fcode; R.fhf = D.d_fhf; R.fval = D.d_fval;
   R.qt = D.d_qt; R.qq = D.d_qq;
-  for (int k = 0; k < 9; k++) {
+  for (int k = 0; k < kNDeg; k++) {
     R.qt_off[k] = D.qt_off[k]; R.ngt[k] = D.ngt9[k];
     R.qq_off[k] = D.qq_off[k]; R.ngq[k] = D.ngq9[k];
   }
@@ -2131,19 +2348,19 @@ void row_args(const DgProblem& D, int d, DgRowArgs& R) {
   R.blk = D.d_blk; R.brhs = D.d_brhs;
 }
 
-template <int PX, int PZ>
+template <int PX, int PZ, bool TET = false>
 int launch_diag_t(DgProblem& D) {
-  constexpr int d = deg_index(PX, PZ);
+  constexpr int d = deg_of(PX, PZ, TET);
   const int cnt = D.grp_ptr[d + 1] - D.grp_ptr[d];
   if (cnt == 0) return 0;
   DgRowArgs R;
   row_args(D, d, R);
-  constexpr int NS = num_shape(PX, PZ);
+  constexpr int NS = DgEl<PX, PZ, TET>::NS;
   if constexpr (NS <= kValuMaxNs) {
     if (NS <= D.valu_ns) {
       const dim3 g((cnt + 64 / NS - 1) / (64 / NS));
       if (D.mixed)
-        hipLaunchKernelGGL((k_dg_diag_v<PX, PZ, true>), g, dim3(64), 0, stream(), R, cnt);
+        hipLaunchKernelGGL((k_dg_diag_v<PX, PZ, true, TET>), g, dim3(64), 0, stream(), R, cnt);
       else
         hipLaunchKernelGGL((k_dg_diag_v<PX, PZ, false>), g, dim3(64), 0, stream(), R, cnt);
       MF_HIP(hipGetLastError());
@@ -2151,33 +2368,33 @@ int launch_diag_t(DgProblem& D) {
     }
   }
   if (D.mixed)
-    hipLaunchKernelGGL((k_dg_diag<PX, PZ, true>), dim3(cnt), dim3(64), 0, stream(), R);
+    hipLaunchKernelGGL((k_dg_diag<PX, PZ, true, TET>), dim3(cnt), dim3(64), 0, stream(), R);
   else
     hipLaunchKernelGGL((k_dg_diag<PX, PZ, false>), dim3(cnt), dim3(64), 0, stream(), R);
   MF_HIP(hipGetLastError());
   return 0;
 }
 
-template <int PXE, int PZE, int PXN, int PZN>
+template <int PXE, int PZE, int PXN, int PZN, bool TET_E = false, bool TET_N = false>
 int launch_couple_t(DgProblem& D) {
-  constexpr int de = deg_index(PXE, PZE), dn = deg_index(PXN, PZN);
-  const int s0 = D.slot_ptr[9 * de + dn], cnt = D.slot_ptr[9 * de + dn + 1] - s0;
+  constexpr int de = deg_of(PXE, PZE, TET_E), dn = deg_of(PXN, PZN, TET_N);
+  const int s0 = D.slot_ptr[kNDeg * de + dn], cnt = D.slot_ptr[kNDeg * de + dn + 1] - s0;
   if (cnt == 0) return 0;
   DgRowArgs R;
   row_args(D, de, R);
   for (int k = 0; k < 40; k++) R.exn[k] = D.ex9[dn][k];
-  constexpr int NSE = num_shape(PXE, PZE), NSN = num_shape(PXN, PZN);
+  constexpr int NSE = DgEl<PXE, PZE, TET_E>::NS, NSN = DgEl<PXN, PZN, TET_N>::NS;
   if constexpr (NSE <= kValuMaxNs && NSN <= kValuMaxNs) {
     if (NSE <= D.valu_ns && NSN <= D.valu_ns) {
       const dim3 g((cnt + 64 / NSE - 1) / (64 / NSE));
-      hipLaunchKernelGGL((k_dg_couple_v<PXE, PZE, PXN, PZN>), g, dim3(64), 0, stream(), R,
-                         (const int*)D.d_slots + s0, cnt);
+      hipLaunchKernelGGL((k_dg_couple_v<PXE, PZE, PXN, PZN, TET_E, TET_N>), g, dim3(64), 0,
+                         stream(), R, (const int*)D.d_slots + s0, cnt);
       MF_HIP(hipGetLastError());
       return 0;
     }
   }
-  hipLaunchKernelGGL((k_dg_couple<PXE, PZE, PXN, PZN>), dim3(cnt), dim3(64), 0, stream(), R,
-                     (const int*)D.d_slots + s0);
+  hipLaunchKernelGGL((k_dg_couple<PXE, PZE, PXN, PZN, TET_E, TET_N>), dim3(cnt), dim3(64), 0,
+                     stream(), R, (const int*)D.d_slots + s0);
   MF_HIP(hipGetLastError());
   return 0;
 }
@@ -2190,6 +2407,10 @@ int launch_rows(DgProblem& D) {
   MF_DG_CASE(1, 2) MF_DG_CASE(2, 2) MF_DG_CASE(3, 2)
   MF_DG_CASE(1, 3) MF_DG_CASE(2, 3) MF_DG_CASE(3, 3)
 #undef MF_DG_CASE
+  // tets of degree 1..3
+  if (rc == 0) rc = launch_diag_t<1, 1, true>(D);
+  if (rc == 0) rc = launch_diag_t<2, 2, true>(D);
+  if (rc == 0) rc = launch_diag_t<3, 3, true>(D);
   // couplings between equal degrees, and between the isotropic degrees of
   // mixed problems
 #define MF_DG_PAIR(XE, ZE, XN, ZN) \
@@ -2200,6 +2421,16 @@ int launch_rows(DgProblem& D) {
   MF_DG_PAIR(1, 1, 2, 2) MF_DG_PAIR(1, 1, 3, 3) MF_DG_PAIR(2, 2, 1, 1)
   MF_DG_PAIR(2, 2, 3, 3) MF_DG_PAIR(3, 3, 1, 1) MF_DG_PAIR(3, 3, 2, 2)
 #undef MF_DG_PAIR
+  // faces with a tet (triangles): tet-tet, tet-prism and prism-tet pairs of
+  // degrees 1..3 (isotropic prisms)
+#define MF_DG_TPAIR(PE, PN)                                                      \
+  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, true, true>(D);              \
+  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, true, false>(D);             \
+  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, false, true>(D);
+  MF_DG_TPAIR(1, 1) MF_DG_TPAIR(1, 2) MF_DG_TPAIR(1, 3)
+  MF_DG_TPAIR(2, 1) MF_DG_TPAIR(2, 2) MF_DG_TPAIR(2, 3)
+  MF_DG_TPAIR(3, 1) MF_DG_TPAIR(3, 2) MF_DG_TPAIR(3, 3)
+#undef MF_DG_TPAIR
   return rc;
 }
 
@@ -2241,11 +2472,11 @@ int build_gs_colors(DgProblem& D) {
   for (int e = 0; e < D.nel; e++) D.gs_color_ptr[col[e] + 1]++;
   for (int c = 0; c < ncol; c++) D.gs_color_ptr[c + 1] += D.gs_color_ptr[c];
   // within a colour, elements grouped by degree (ascending element order in each group)
-  D.gs_cd_ptr.assign(9 * (size_t)ncol + 1, 0);
-  for (int e = 0; e < D.nel; e++) D.gs_cd_ptr[9 * (size_t)col[e] + D.deg[e] + 1]++;
+  D.gs_cd_ptr.assign(kNDeg * (size_t)ncol + 1, 0);
+  for (int e = 0; e < D.nel; e++) D.gs_cd_ptr[kNDeg * (size_t)col[e] + D.deg[e] + 1]++;
   for (size_t k = 0; k + 1 < D.gs_cd_ptr.size(); k++) D.gs_cd_ptr[k + 1] += D.gs_cd_ptr[k];
   std::vector<int> pos(D.gs_cd_ptr.begin(), D.gs_cd_ptr.end() - 1), elems(D.nel);
-  for (int e = 0; e < D.nel; e++) elems[pos[9 * (size_t)col[e] + D.deg[e]]++] = e;
+  for (int e = 0; e < D.nel; e++) elems[pos[kNDeg * (size_t)col[e] + D.deg[e]]++] = e;
   MF_HIP(hipMalloc(&D.d_gs_elems, (size_t)D.nel * sizeof(int)));
   MF_HIP(hipMemcpy(D.d_gs_elems, elems.data(), (size_t)D.nel * sizeof(int), hipMemcpyHostToDevice));
   if (!D.d_work) MF_HIP(hipMalloc(&D.d_work, (size_t)D.ndof * sizeof(double)));
@@ -2266,7 +2497,7 @@ int fill_block_jacobi(DgProblem& D) {
   MF_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), stream()));
   int rc = 0;
   InvLaunch L{&D, d_bad};
-  for (int d = 0; d < 9 && rc == 0; d++)
+  for (int d = 0; d < kNDeg && rc == 0; d++)
     if (D.grp_ptr[d + 1] > D.grp_ptr[d]) rc = with_ns(d, L);
   int bad = 0;
   if (rc == 0) {
@@ -2420,8 +2651,9 @@ extern "C" {
 
 namespace {
 
-// a DG problem on a prism mesh with the given degree (pdeg = 100 pz + px)
-// of every element; mixed when they differ
+// a DG problem on a prism, tet or hybrid mesh with the given degree (pdeg =
+// 100 pz + px; a tet takes the complete base of degree px = pz) of every
+// element; mixed (variable block sizes) when they differ or tets are present
 int dg_create(int mesh, const std::vector<int>& pdeg) {
   Mesh* m = mesh_by_id(mesh);
   if (!m) return -3;
@@ -2429,18 +2661,28 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     set_error("mf_init has not been called");
     return -1;
   }
-  for (int e = 0; e < m->nel; e++)
-    if (m->type[e] != kPrism) {
-      set_error("mf_dg_create: the DG approximation is defined on prisms only");
+  bool any_tet = false;
+  for (int e = 0; e < m->nel; e++) {
+    if (m->type[e] == kTetra) {
+      any_tet = true;
+      if (pdeg[e] % 100 != pdeg[e] / 100) {
+        set_error("mf_dg_create: a tetrahedron takes an isotropic degree (101, 202 or 303)");
+        return -1;
+      }
+    } else if (m->type[e] != kPrism) {
+      set_error("mf_dg_create: DG elements are prisms or tetrahedra");
       return -1;
     }
+  }
   const size_t nel = m->nel;
-  bool mixed = false;
+  bool mixed = any_tet;
   for (size_t e = 1; e < nel; e++) mixed = mixed || pdeg[e] != pdeg[0];
   auto D = std::make_unique<DgProblem>();
   D->mesh = mesh; D->nel = m->nel; D->mixed = mixed;
   D->deg.resize(nel);
-  for (size_t e = 0; e < nel; e++) D->deg[e] = deg_index(pdeg[e] % 100, pdeg[e] / 100);
+  for (size_t e = 0; e < nel; e++)
+    D->deg[e] = m->type[e] == kTetra ? tet_deg_index(pdeg[e] % 100)
+                                     : deg_index(pdeg[e] % 100, pdeg[e] / 100);
   if (!mixed) {
     D->px = pdeg[0] % 100; D->pz = pdeg[0] / 100;
     D->ns = num_shape(D->px, D->pz);
@@ -2453,18 +2695,18 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
   hipDeviceProp_t prop;
   MF_HIP(hipGetDeviceProperties(&prop, dev));
   D->ncu = prop.multiProcessorCount;
-  std::vector<double> nc(18 * nel);
+  std::vector<double> nc(18 * nel, 0.0);
   for (size_t e = 0; e < nel; e++)
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < (m->type[e] == kTetra ? 4 : 6); i++)
       for (int r = 0; r < 3; r++)
         nc[(size_t)(3 * i + r) * nel + e] = m->coor[3 * (size_t)m->nodes[6 * e + i] + r];
   MF_HIP(hipMalloc(&D->d_nc, nc.size() * sizeof(double)));
   MF_HIP(hipMemcpyAsync(D->d_nc, nc.data(), nc.size() * sizeof(double), hipMemcpyHostToDevice,
                         stream()));
   // degree groups, offsets of unknowns / block-row slots / inverses
-  D->grp_ptr.assign(10, 0);
+  D->grp_ptr.assign(kNDeg + 1, 0);
   for (size_t e = 0; e < nel; e++) D->grp_ptr[D->deg[e] + 1]++;
-  for (int d = 0; d < 9; d++) D->grp_ptr[d + 1] += D->grp_ptr[d];
+  for (int d = 0; d < kNDeg; d++) D->grp_ptr[d + 1] += D->grp_ptr[d];
   std::vector<int> grp(nel), pos(D->grp_ptr.begin(), D->grp_ptr.end() - 1);
   for (size_t e = 0; e < nel; e++) grp[pos[D->deg[e]]++] = (int)e;
   // faces
@@ -2537,10 +2779,11 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     MF_HIP(hipGetLastError());
   } else {
     std::vector<double> all, qv;
-    for (int d = 0; d < 9; d++) {
+    for (int d = 0; d < kNDeg; d++) {
       if (D->grp_ptr[d + 1] == D->grp_ptr[d]) continue;
       D->qv_off[d] = (int)all.size() / 4;
-      volume_rule(d % 3 + 1, d / 3 + 1, qv);
+      if (d >= 9) tet_volume_rule(d - 8, qv);
+      else volume_rule(d % 3 + 1, d / 3 + 1, qv);
       all.insert(all.end(), qv.begin(), qv.end());
     }
     MF_HIP(hipMalloc(&D->d_qv9, all.size() * sizeof(double)));
@@ -2558,6 +2801,15 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     qt_all.insert(qt_all.end(), qt.begin(), qt.end());
     qq_all.insert(qq_all.end(), qq.begin(), qq.end());
     shape_exponents(d % 3 + 1, d / 3 + 1, D->ex9[d]);
+  }
+  // tets: the triangle rule of the isotropic prism degree, no quads
+  for (int p = 1; p <= 3; p++) {
+    const int d = tet_deg_index(p), dp = deg_index(p, p);
+    D->qt_off[d] = D->qt_off[dp];
+    D->ngt9[d] = D->ngt9[dp];
+    D->qq_off[d] = 0;
+    D->ngq9[d] = 0;
+    tet_exponents(p, D->ex9[d]);
   }
   if (!mixed) {
     const int d = D->deg[0];
@@ -2582,13 +2834,14 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
   MF_HIP(hipMemcpyAsync(D->d_qq, qq_all.data(), qq_all.size() * sizeof(double),
                         hipMemcpyHostToDevice, stream()));
   // interior slots grouped by (own degree, neighbour degree), ascending within a pair
-  D->slot_ptr.assign(82, 0);
+  constexpr int NP = kNDeg * kNDeg;
+  D->slot_ptr.assign(NP + 1, 0);
   for (size_t k = 0; k < nf; k++)
-    if (fnb[k] >= 0) D->slot_ptr[9 * D->deg[k / 5] + D->deg[fnb[k]] + 1]++;
-  for (int q = 0; q < 81; q++) D->slot_ptr[q + 1] += D->slot_ptr[q];
-  std::vector<int> slots(D->slot_ptr[81]), spos(D->slot_ptr.begin(), D->slot_ptr.end() - 1);
+    if (fnb[k] >= 0) D->slot_ptr[kNDeg * D->deg[k / 5] + D->deg[fnb[k]] + 1]++;
+  for (int q = 0; q < NP; q++) D->slot_ptr[q + 1] += D->slot_ptr[q];
+  std::vector<int> slots(D->slot_ptr[NP]), spos(D->slot_ptr.begin(), D->slot_ptr.end() - 1);
   for (size_t k = 0; k < nf; k++)
-    if (fnb[k] >= 0) slots[spos[9 * D->deg[k / 5] + D->deg[fnb[k]]]++] = (int)k;
+    if (fnb[k] >= 0) slots[spos[kNDeg * D->deg[k / 5] + D->deg[fnb[k]]]++] = (int)k;
   D->nslots = (int)slots.size();
   if (D->nslots) {
     MF_HIP(hipMalloc(&D->d_slots, slots.size() * sizeof(int)));

@@ -102,8 +102,9 @@ def test_dg_rejects_bad_input():
     with pytest.raises(mf.MFError):
         mf.DgProblem(mesh, 3)
     tet = mf.Mesh.box_tetra(2, 2, 1, seed=1)
-    with pytest.raises(mf.MFError):
-        mf.DgProblem(tet, 101)
+    with pytest.raises(mf.MFError):  # a tetrahedron takes an isotropic degree
+        mf.DgProblem(tet, 102)
+    mf.DgProblem(tet, 101).free()
 
 
 # ------------------------------------------------ block system and solve
