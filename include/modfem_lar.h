@@ -50,8 +50,8 @@ int lar_assemble_SM_and_LV(int Matrix_id, int Scope, int Nr_dof_bl,
                            double* Rhs_vect, char* Rewr_dofs);
 /* lah_intf.h:184 */
 int lar_free_SM_and_LV(int Matrix_id);
-/* lah_intf.h:192 -- CRS: NO_PRECON (0) and BLOCK_JACOBI (1); BCRS also
-   BLOCK_GS (2) and block ILU(k): MULTI_ILU (4) / BLOCK_ILU (5) with
+/* lah_intf.h:192 -- CRS: NO_PRECON (0), BLOCK_JACOBI (1) and BLOCK_GS (2);
+   BCRS also block ILU(k): MULTI_ILU (4) / BLOCK_ILU (5) with
    Second_arg = k on a structurally symmetric pattern; k > 0 grows the
    pattern by k rings of neighbours (las_bcrs_intf.c:751-1016), with the
    reference's list limit of (int)(0.5 (k+2)^3 max_nrneig) blocks per row
@@ -69,7 +69,10 @@ int lar_free_preconditioner(int Matrix_id);
 void lar_compute_residual(int Matrix_id, int Use_rhs, int Ini_zero, int Ndof,
                           double* X, double* B, double* V);
 /* lah_intf.h:252 -- CRS BLOCK_JACOBI: V = D^-1 (B - (A-D) V), Nr_prec
-   sweeps.  BCRS (las_bcrs_intf.c:2938-3300): block Jacobi or block
+   sweeps; CRS BLOCK_GS: Nr_prec forward Gauss-Seidel sweeps in the row
+   chunks of n/T + 1 rows that the reference's OpenMP loop gives T threads
+   (las_crs_intf.c:905-998), bitwise its result on T threads (T from
+   mf_lar_set_gs_threads, default 1).  BCRS (las_bcrs_intf.c:2938-3300): block Jacobi or block
    Gauss-Seidel in the reference's row order, bitwise its single-thread
    result; even Nr_prec: Nr_prec/2 forward+backward pairs, odd: Nr_prec/2+1
    forward sweeps */
@@ -81,7 +84,8 @@ void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero,
    the reference (las_crs_intf.c:1019-1057): records an error there */
 void lar_perform_rhsub(int Matrix_id, int Ndof, double* V, double* B);
 
-/* Restarted GMRES(krylbas) on a BCRS matrix with the preconditioner
+/* Restarted GMRES(krylbas) on a BCRS matrix, or a CRS matrix with BLOCK_GS,
+   with the preconditioner
    allocated and filled on it (lsr_mkb_core_gmres, lsd_mkb/lsd_mkb_core/
    lss_mkb_core.c:2317, over the storage operations as lsr_mkb_core_compreres
    applies them, :1915-1990): none, block Jacobi / block Gauss-Seidel with
@@ -100,6 +104,13 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol,
    (classes ascending forward, descending backward; the same iteration
    count rule).  Returns the number of colours (1 for order 0) */
 int mf_lar_set_gs_order(int Matrix_id, int order);
+
+/* Thread count T whose row chunks the CRS BLOCK_GS sweeps reproduce (no
+   reference counterpart: the reference takes it from its OpenMP runtime,
+   las_crs_intf.c:910-911); 1 <= T <= 2^20, default 1 (forward Gauss-Seidel in
+   DOF order).  One wave walks each chunk's rows in order, so small T is a
+   compatibility mode, not a fast smoother */
+int mf_lar_set_gs_threads(int Matrix_id, int Nthreads);
 
 /* HIP-event time (ms, averaged over reps) of one BCRS operation on
    device-resident vectors: what 0 residual, 1 one forward sweep of the block

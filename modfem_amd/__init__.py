@@ -188,6 +188,7 @@ def _declare(L):
         "mf_lar_gmres": (I, [I, I, I, D, I, I, V, V, V, V]),
         "mf_lar_bench": (I, [I, I, I, V]),
         "mf_lar_set_gs_order": (I, [I, I]),
+        "mf_lar_set_gs_threads": (I, [I, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

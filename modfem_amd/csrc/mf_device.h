@@ -369,6 +369,14 @@ int gs_build(Problem& P);
 int gs_refresh(Problem& P);
 int gs_sweeps(Problem& P, int nsweeps, const double* b, double* v, const int* stop);
 void gs_free(Problem& P);
+// CRS Gauss-Seidel as the reference computes it on nthreads OpenMP threads
+// (row chunks of n/nthreads + 1, mf_gs.hip) on a CRS problem (row_ptr, col,
+// val, diag_ptr, dinv): nr_prec sweeps on v with rhs b (null: none), tmp n
+// doubles; and the GMRES operator preconditioned by it
+constexpr int kMaxGsChunks = 1 << 20;
+int crs_gs_chunked(Problem& P, int nthreads, int nr_prec, const double* b, double* v, double* tmp,
+                   const int* stop);
+struct ExtOp* crs_gs_make_op(Problem* P, int nthreads, int nr_prec);
 // BCRS storage with 4x4 column-major blocks (mf_bcrs.hip): block rows in
 // CRS order with the diagonal in place, rhs, and for the diagonal-block
 // preconditioners the LU factors and pivots of every diagonal block and the

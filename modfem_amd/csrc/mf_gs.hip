@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "mf_device.h"
@@ -163,6 +164,165 @@ int gs_sweeps(Problem& P, int nsweeps, const double* b, double* v, const int* st
     }
   MF_HIP(hipGetLastError());
   return 0;
+}
+
+// ---------------------------------------------------------------------
+// BLOCK_GS of the CRS storage as the reference computes it on T OpenMP
+// threads (lar_perform_BJ_or_GS_iterations_crs, las_crs_intf.c:905-998).
+// The rows are cut into chunks of portion = n/T + 1; within a chunk the rows
+// are updated in order, and an entry left of the diagonal (CRS position
+// k < diag_ptr) whose column lies in [chunk start, row) takes the chunk's new
+// value, every other entry the sweep's start value.  The result depends on T
+// and equals the reference's run on T threads bit for bit (T = 1: forward
+// Gauss-Seidel in DOF order).
+// One wave per chunk walks its rows: the lanes form the products a_k x_k, the
+// row sum is taken in CRS position order exactly as "vloc -= a*x" runs, then
+// "+= b", "*= d".  The chunk's newest kGsRing values are kept in LDS; older
+// ones are read back from vnew (written by this workgroup before a barrier).
+namespace {
+
+constexpr int kGsRing = 4096;
+
+__global__ void __launch_bounds__(64) k_crs_gs_chunks(int n, int portion, const int* __restrict__ rp,
+                                                      const int* __restrict__ col,
+                                                      const double* __restrict__ val,
+                                                      const int* __restrict__ dp,
+                                                      const double* __restrict__ dinv,
+                                                      const double* b, const double* vold,
+                                                      double* vnew, const int* stop) {
+  if (stop && *stop) return;
+  __shared__ double ring[kGsRing];
+  __shared__ double prod[64];
+  const int lane = threadIdx.x;
+  const long long f = (long long)blockIdx.x * portion;
+  if (f >= n) return;
+  const int first = (int)f;
+  const int last = (int)std::min<long long>(n, f + portion);
+  for (int r = first; r < last; r++) {
+    const int k0 = rp[r], k1 = rp[r + 1], dk = dp[r];
+    double s = 0.0;
+    for (int base = k0; base < k1; base += 64) {
+      const int k = base + lane;
+      double p = 0.0;
+      if (k < k1 && k != dk) {
+        const int c = col[k];
+        double x;
+        if (k < dk && c >= first && c < r)
+          x = (r - c < kGsRing) ? ring[c & (kGsRing - 1)] : vnew[c];
+        else
+          x = vold[c];
+        p = __dmul_rn(val[k], x);
+      }
+      prod[lane] = p;
+      __syncthreads();
+      const int cnt = std::min(64, k1 - base);
+      for (int j = 0; j < cnt; j++)
+        if (base + j != dk) s = __dsub_rn(s, prod[j]);
+      __syncthreads();
+    }
+    if (b) s = __dadd_rn(s, b[r]);
+    s = __dmul_rn(s, dinv[r]);
+    if (lane == 0) {
+      vnew[r] = s;
+      ring[r & (kGsRing - 1)] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_gs_vcopy(long long n, const double* x, const double* s, int j, double* y,
+                           const int* stop) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (stop && *stop)) return;
+  y[i] = s ? __dmul_rn(s[j], x[i]) : x[i];
+}
+
+// y = (neg ? -1 : 1) (a - c)
+__global__ void k_gs_vsub(long long n, const double* a, const double* c, double* y, int neg,
+                          const int* stop) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (stop && *stop)) return;
+  const double d = __dsub_rn(a[i], c[i]);
+  y[i] = neg ? -d : d;
+}
+
+}  // namespace
+
+// nr_prec sweeps over v (device), b null: no rhs; tmp: n doubles
+int crs_gs_chunked(Problem& P, int nthreads, int nr_prec, const double* b, double* v, double* tmp,
+                   const int* stop) {
+  if (P.n == 0 || nr_prec < 1) return 0;
+  if (nthreads < 1 || nthreads > kMaxGsChunks) {
+    set_error("CRS Gauss-Seidel: the thread count must be in [1, " +
+              std::to_string(kMaxGsChunks) + "]");
+    return -1;
+  }
+  if (!P.d_dinv || !P.d_diag_ptr) {
+    set_error("CRS Gauss-Seidel: the preconditioner is not filled");
+    return -1;
+  }
+  const int portion = P.n / nthreads + 1;
+  const int grid = (int)std::min<long long>(nthreads, ((long long)P.n + portion - 1) / portion);
+  const unsigned gv = (unsigned)((P.n + 255) / 256);
+  for (int it = 0; it < nr_prec; it++) {
+    hipLaunchKernelGGL(k_crs_gs_chunks, dim3(grid), dim3(64), 0, stream(), P.n, portion,
+                       P.d_row_ptr, P.d_col, P.d_val, P.d_diag_ptr, P.d_dinv, b, v, tmp, stop);
+    hipLaunchKernelGGL(k_gs_vcopy, dim3(gv), dim3(256), 0, stream(), (long long)P.n, tmp, nullptr,
+                       0, v, stop);
+  }
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+
+// GMRES operator of a CRS matrix preconditioned by chunked Gauss-Seidel, as
+// lsr_mkb_core_compreres applies a smoother (lss_mkb_core.c:1861-2097):
+// V = x; nr_prec sweeps on V with (Control = 1) or without the rhs; V -= x;
+// V = -V when Control = 0
+struct CrsGsOp : ExtOp {
+  Problem* P = nullptr;
+  int nthreads = 1, nr_prec = 1;
+  double *t1 = nullptr, *t2 = nullptr, *xs = nullptr;
+  ~CrsGsOp() override {
+    for (double* p : {t1, t2, xs})
+      if (p) (void)hipFree(p);
+  }
+  unsigned gv() const { return (unsigned)((P->n + 255) / 256); }
+  int precon_residual(const double* b, const double* x, double* out, const int* stop, int neg) {
+    hipLaunchKernelGGL(k_gs_vcopy, dim3(gv()), dim3(256), 0, stream(), (long long)P->n, x,
+                       nullptr, 0, t2, stop);
+    if (crs_gs_chunked(*P, nthreads, nr_prec, b, t2, t1, stop)) return -2;
+    hipLaunchKernelGGL(k_gs_vsub, dim3(gv()), dim3(256), 0, stream(), (long long)P->n, t2, x, out,
+                       neg, stop);
+    MF_HIP(hipGetLastError());
+    return 0;
+  }
+  int residual(const double* b, const double* x, double* r) override {
+    return precon_residual(b, x, r, nullptr, 0);
+  }
+  int apply(const double* v, const double* scale, int j, double* w, const int* stop) override {
+    hipLaunchKernelGGL(k_gs_vcopy, dim3(gv()), dim3(256), 0, stream(), (long long)P->n, v, scale,
+                       j, xs, stop);
+    return precon_residual(nullptr, xs, w, stop, 1);
+  }
+};
+
+}  // namespace
+
+ExtOp* crs_gs_make_op(Problem* P, int nthreads, int nr_prec) {
+  CrsGsOp* op = new CrsGsOp();
+  op->P = P;
+  op->nthreads = nthreads;
+  op->nr_prec = nr_prec;
+  const size_t bytes = (size_t)std::max(P->n, 1) * sizeof(double);
+  if (hipMalloc(&op->t1, bytes) != hipSuccess || hipMalloc(&op->t2, bytes) != hipSuccess ||
+      hipMalloc(&op->xs, bytes) != hipSuccess) {
+    set_error("CRS Gauss-Seidel operator: out of device memory");
+    delete op;
+    return nullptr;
+  }
+  return op;
 }
 
 }  // namespace mf

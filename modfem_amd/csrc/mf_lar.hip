@@ -50,8 +50,9 @@ struct LarMatrix {
   std::vector<int> ilu_rp, ilu_ci;  // BCRS ILU(k > 0) pattern on the host
   std::vector<double> h_val, h_rhs;  // host mirror for per-element assembly
   bool dirty = false;
-  bool filled = false;  // BCRS: diagonal-block factors current
+  bool filled = false;  // diagonal-block factors / diagonal inverses current
   int precon = -1;
+  int gs_threads = 1;   // CRS BLOCK_GS: the reference's OpenMP thread count
   int ndof() const { return B ? BLS * B->nb : P->n; }
   // BCRS GMRES: workspace (Krylov basis, solution) and the operator
   std::unique_ptr<Problem> G;
@@ -64,6 +65,9 @@ std::vector<LarMatrix> g_mats;  // LIFO like itv_crs_matrices
 int lar_bcrs_gmres(LarMatrix& L, int krylbas, int max_iter, double tol, double tol_abs,
                    int nr_prec, int ini_zero, double* x, int* iters, double* res_rel,
                    double* resid);
+int lar_crs_gs_gmres(LarMatrix& L, int krylbas, int max_iter, double tol, double tol_abs,
+                     int nr_prec, int ini_zero, double* x, int* iters, double* res_rel,
+                     double* resid);
 
 LarMatrix* lar_get(int id) {
   if (id < 0 || id >= (int)g_mats.size() || !(g_mats[id].P || g_mats[id].B)) {
@@ -598,11 +602,15 @@ int lar_allocate_preconditioner(int Matrix_id, int Precon, int Second_arg) {
       }
     return B.nb;
   }
-  if (Precon != NO_PRECON && Precon != BLOCK_JACOBI) {
-    set_error("lar_allocate_preconditioner: CRS storage offers NO_PRECON and BLOCK_JACOBI");
+  if (Precon != NO_PRECON && Precon != BLOCK_JACOBI && Precon != BLOCK_GS) {
+    set_error("lar_allocate_preconditioner: CRS storage offers NO_PRECON, BLOCK_JACOBI and "
+              "BLOCK_GS");
     return -1;
   }
   M->precon = Precon;
+  M->filled = false;
+  M->op.reset();
+  M->op_kind = -1;
   Problem& P = *M->P;
   if (!P.d_dinv) MF_HIP(hipMalloc(&P.d_dinv, (size_t)P.n * sizeof(double)));
   if (!P.d_diag_ptr) MF_HIP(hipMalloc(&P.d_diag_ptr, (size_t)P.n * sizeof(int)));
@@ -633,6 +641,7 @@ int lar_fill_preconditioner(int Matrix_id) {
   }
   if (launch_diag_ptr(P, P.d_diag_ptr) || launch_fill_jacobi(P)) return -2;
   MF_HIP(hipStreamSynchronize(stream()));
+  M->filled = true;
   return 0;
 }
 
@@ -648,6 +657,10 @@ int lar_free_preconditioner(int Matrix_id) {
     return 0;
   }
   Problem& P = *M->P;
+  M->op.reset();
+  M->op_kind = -1;
+  M->filled = false;
+  M->precon = -1;
   if (P.d_dinv) (void)hipFree(P.d_dinv);
   if (P.d_diag_ptr) (void)hipFree(P.d_diag_ptr);
   P.d_dinv = nullptr;
@@ -686,7 +699,9 @@ void lar_compute_residual(int Matrix_id, int Use_rhs, int Ini_zero, int Ndof, do
 }
 
 // lar_perform_BJ_or_GS_iterations (las_crs_intf.c:857-1013), BLOCK_JACOBI:
-// V <- D^-1 (B - (A - D) V), Nr_prec sweeps.  BCRS (las_bcrs_intf.c:
+// V <- D^-1 (B - (A - D) V), Nr_prec sweeps; BLOCK_GS: Nr_prec forward
+// sweeps in row chunks as the reference runs them on gs_threads OpenMP
+// threads (las_crs_intf.c:905-998, mf_lar_set_gs_threads).  BCRS (las_bcrs_intf.c:
 // 2938-3300): block Jacobi or block Gauss-Seidel in the reference's row order;
 // even Nr_prec gives Nr_prec/2 forward + backward sweep pairs, odd gives
 // Nr_prec/2 + 1 forward sweeps
@@ -713,8 +728,10 @@ void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero, i
     (void)hipFree(tmp);
     return;
   }
-  if (!M || Ndof != M->P->n || M->precon != BLOCK_JACOBI || !M->P->d_dinv) {
-    fprintf(stderr, "lar_perform_BJ_or_GS_iterations: needs a filled BLOCK_JACOBI preconditioner\n");
+  if (!M || Ndof != M->P->n || (M->precon != BLOCK_JACOBI && M->precon != BLOCK_GS) ||
+      !M->filled) {
+    fprintf(stderr, "lar_perform_BJ_or_GS_iterations: needs a filled BLOCK_JACOBI or BLOCK_GS "
+                    "preconditioner\n");
     return;
   }
   if (flush(*M)) return;
@@ -725,6 +742,14 @@ void lar_perform_BJ_or_GS_iterations(int Matrix_id, int Use_rhs, int Ini_zero, i
   double* tmp = nullptr;
   if (hipMalloc(&tmp, (size_t)Ndof * sizeof(double)) != hipSuccess) return;
   const double* db = (Use_rhs == 1) ? (B ? sb.d : P.d_rhs) : nullptr;
+  if (M->precon == BLOCK_GS) {
+    if (crs_gs_chunked(P, M->gs_threads, Nr_prec, db, sv.d, tmp, nullptr) == 0)
+      stage_out(V, sv, Ndof);
+    else
+      fprintf(stderr, "lar_perform_BJ_or_GS_iterations: %s\n", last_error());
+    (void)hipFree(tmp);
+    return;
+  }
   for (int it = 0; it < Nr_prec; it++) {
     if (launch_rowop(P, 1, Use_rhs == 1, sv.d, db, P.d_diag_ptr, tmp)) break;
     if (hipMemcpyAsync(sv.d, tmp, (size_t)Ndof * sizeof(double), hipMemcpyDeviceToDevice,
@@ -764,8 +789,13 @@ int mf_lar_gmres(int Matrix_id, int krylbas, int max_iter, double tol, int nr_pr
   LarMatrix* M = lar_get(Matrix_id);
   if (!M) return -3;
   if (!M->B) {
-    set_error("mf_lar_gmres: needs a BCRS matrix (CRS problems solve with mf_gmres)");
-    return -1;
+    if (M->precon != BLOCK_GS) {
+      set_error("mf_lar_gmres: needs a BCRS matrix, or a CRS matrix with BLOCK_GS (other CRS "
+                "problems solve with mf_gmres)");
+      return -1;
+    }
+    return lar_crs_gs_gmres(*M, krylbas, max_iter, tol, 0.0, nr_prec, ini_zero, x, iters,
+                            res_rel, resid);
   }
   return lar_bcrs_gmres(*M, krylbas, max_iter, tol, 0.0, nr_prec, ini_zero, x, iters, res_rel,
                         resid);
@@ -804,6 +834,40 @@ int lar_bcrs_gmres(LarMatrix& L, int krylbas, int max_iter, double tol, double t
   }
   M->G->ext = M->op.get();
   M->G->d_rhs = B.d_rhs;
+  return gmres_solve(*M->G, 0, krylbas, max_iter, tol, tol_abs, ini_zero, x, iters, res_rel,
+                     resid);
+}
+
+// GMRES over a CRS matrix preconditioned by nr_prec chunked Gauss-Seidel
+// sweeps per application (smoothing, as lsr_mkb_core_compreres applies
+// BLOCK_GS), tolerances as lar_bcrs_gmres
+int lar_crs_gs_gmres(LarMatrix& L, int krylbas, int max_iter, double tol, double tol_abs,
+                     int nr_prec, int ini_zero, double* x, int* iters, double* res_rel,
+                     double* resid) {
+  LarMatrix* M = &L;
+  if (M->precon != BLOCK_GS || !M->filled) {
+    set_error("mf_lar_gmres: the BLOCK_GS preconditioner is not filled");
+    return -1;
+  }
+  if (nr_prec < 1) {
+    set_error("mf_lar_gmres: nr_prec must be >= 1");
+    return -1;
+  }
+  if (flush(*M)) return -2;
+  Problem& P = *M->P;
+  if (!M->op || M->op_kind != (BLOCK_GS << 24 | M->gs_threads) || M->op_sweeps != nr_prec) {
+    M->op.reset(crs_gs_make_op(&P, M->gs_threads, nr_prec));
+    if (!M->op) return -2;
+    M->op_kind = BLOCK_GS << 24 | M->gs_threads;
+    M->op_sweeps = nr_prec;
+  }
+  if (!M->G) {
+    M->G = std::make_unique<Problem>();
+    M->G->n = P.n;
+    M->G->el_type = kPrism;
+  }
+  M->G->ext = M->op.get();
+  M->G->d_rhs = P.d_rhs;
   return gmres_solve(*M->G, 0, krylbas, max_iter, tol, tol_abs, ini_zero, x, iters, res_rel,
                      resid);
 }
@@ -888,6 +952,20 @@ int mf_lar_set_gs_order(int Matrix_id, int order) {
   }
   B.gs_order = order;
   return order == 1 ? (int)B.col_ptr.size() - 1 : 1;
+}
+
+// thread count whose row chunks the CRS BLOCK_GS sweeps reproduce
+// (las_crs_intf.c:905-998 runs one chunk of n/T + 1 rows per OpenMP thread)
+int mf_lar_set_gs_threads(int Matrix_id, int Nthreads) {
+  LarMatrix* M = lar_get(Matrix_id);
+  if (!M) return -3;
+  if (M->B || Nthreads < 1 || Nthreads > kMaxGsChunks) {
+    set_error("mf_lar_set_gs_threads: needs a CRS matrix and 1 <= Nthreads <= " +
+              std::to_string(kMaxGsChunks));
+    return -1;
+  }
+  M->gs_threads = Nthreads;
+  return 0;
 }
 
 // storage type (2 CRS, 3 BCRS), block size, rows of blocks and stored blocks
@@ -1192,13 +1270,17 @@ int lsr_mkb_solve(int Solver_id, int Comp_type, int Ini_zero, double* X, double*
                         &rr, &rs);
   } else {
     int precon = 0;
+    const bool gs = p.precon == BLOCK_GS && M->precon == BLOCK_GS;
     if (p.precon == BLOCK_JACOBI && M->precon == BLOCK_JACOBI && p.nr_prec == 1) precon = 1;
-    else if (p.precon != NO_PRECON) {
-      set_error("lsr_mkb_solve: CRS storage solves with no preconditioner or one block-Jacobi "
-                "sweep");
+    else if (p.precon != NO_PRECON && !gs) {
+      set_error("lsr_mkb_solve: CRS storage solves with no preconditioner, one block-Jacobi "
+                "sweep or block Gauss-Seidel");
       rc = -1;
     }
-    if (rc == 0)
+    if (rc == 0 && gs)
+      rc = lar_crs_gs_gmres(*M, p.krylbas, max_iter, tol_rel, tol_abs, p.nr_prec, Ini_zero, X,
+                            &it, &rr, &rs);
+    else if (rc == 0)
       rc = gmres_solve(*M->P, precon, p.krylbas, max_iter, tol_rel, tol_abs, Ini_zero, X, &it,
                        &rr, &rs);
   }

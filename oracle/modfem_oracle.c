@@ -740,6 +740,46 @@ void orc_bj_iteration(int n, const int *row_ptr, const int *col_ind,
   free(vt);
 }
 
+/* lar_perform_BJ_or_GS_iterations_crs, BLOCK_GS, as it runs on nthreads
+   OpenMP threads (las_crs_intf.c:905-998): the rows are cut into chunks of
+   portion = n/nthreads + 1, chunk t holding rows [t*portion, (t+1)*portion);
+   within a chunk the rows are updated in order and an entry left of the
+   diagonal (CRS position, k < diag_ptr) whose column lies in
+   [chunk start, row) takes the chunk's new value, every other entry the value
+   of the sweep's start; the chunks' new values replace v after the sweep.
+   Columns that R's per-thread copy does not cover (more than half bandwidth
+   + 10 below the chunk) read the sweep's start value here; R leaves them
+   uninitialised, so matrices whose DOF half bandwidth exceeds R's block half
+   bandwidth differ there.  nthreads = 1 is textbook forward Gauss-Seidel. */
+void orc_crs_gs_chunked(int n, const int *row_ptr, const int *col_ind,
+                        const double *val, const int *diag_ptr,
+                        const double *diag_precon, int use_rhs, const double *b,
+                        int nthreads, int nr_prec, double *v) {
+  double *vold = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  const int portion = n / nthreads + 1;
+  for (int it = 0; it < nr_prec; it++) {
+    memcpy(vold, v, (size_t)n * sizeof(double));
+    for (int t = 0; t < nthreads; t++) {
+      const int first = t * portion;
+      int last = (t + 1) * portion;
+      if (last > n) last = n;
+      for (int r = first; r < last; r++) {
+        double vloc = 0.0;
+        for (int k = row_ptr[r]; k < diag_ptr[r]; k++) {
+          const int c = col_ind[k];
+          vloc -= val[k] * ((c >= first && c < r) ? v[c] : vold[c]);
+        }
+        for (int k = diag_ptr[r] + 1; k < row_ptr[r + 1]; k++)
+          vloc -= val[k] * vold[col_ind[k]];
+        if (use_rhs) vloc += b[r];
+        vloc *= diag_precon[r];
+        v[r] = vloc;
+      }
+    }
+  }
+  free(vold);
+}
+
 /* ---------------------------------------------------------------------
  * BLAS restatements (netlib reference algorithms)
  * ------------------------------------------------------------------- */

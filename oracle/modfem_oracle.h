@@ -126,6 +126,12 @@ void orc_bj_iteration(int n, const int *row_ptr, const int *col_ind,
                       const double *val, const int *diag_ptr,
                       const double *diag_precon, int use_rhs,
                       const double *b, double *v);
+/* BLOCK_GS sweeps as R runs them on nthreads OpenMP threads
+   (las_crs_intf.c:905-998) */
+void orc_crs_gs_chunked(int n, const int *row_ptr, const int *col_ind,
+                        const double *val, const int *diag_ptr,
+                        const double *diag_precon, int use_rhs, const double *b,
+                        int nthreads, int nr_prec, double *v);
 
 /* restarted left-preconditioned GMRES with CGS and Givens
    (lss_mkb_core.c:2317-2668), Jacobi preconditioner applied as one BJ sweep

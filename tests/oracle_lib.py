@@ -42,6 +42,7 @@ def oracle():
             "orc_fill_jacobi": (None, [I, V, V, V, V, V]),
             "orc_compute_residual": (None, [I, V, V, V, I, V, V, V]),
             "orc_bj_iteration": (None, [I, V, V, V, V, V, I, V, V]),
+            "orc_crs_gs_chunked": (None, [I, V, V, V, V, V, I, V, I, I, V]),
             "orc_gmres": (I, [I, V, V, V, V, I, I, I, D, I, V, V, V, V]),
             "orc_gmres_tol": (I, [I, V, V, V, V, I, I, I, D, D, I, V, V, V, V]),
             "orc_ilu0": (I, [I, V, V, V, V]),

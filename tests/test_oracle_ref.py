@@ -345,3 +345,93 @@ def test_crs_storage_bitexact_with_reference():
     assert np.array_equal(w_r, w_o)
     L.lar_free_preconditioner_crs(mid)
     L.lar_free_SM_and_LV_crs(mid)
+
+
+def ref_crs_matrix(L, op, val, rhs):
+    """a CRS matrix in R's lad_crs (R's own allocation from neighbour lists)
+    holding the oracle's values and rhs, written through R's accessor"""
+    lists = neighbour_lists(op)
+    nro = np.array([len(x) for x in lists], np.int32)
+    ptrs = (ctypes.POINTER(I) * op.n)(*[x.ctypes.data_as(ctypes.POINTER(I)) for x in lists])
+    mid = L.lar_allocate_SM_and_LV_crs(op.n, 36, P(nro), ptrs)
+    assert mid >= 0
+    L.lar_initialize_SM_and_LV_crs(mid, 0)
+    rp, ci, va, rh = (ctypes.POINTER(I)(), ctypes.POINTER(I)(), ctypes.POINTER(D)(),
+                      ctypes.POINTER(D)())
+    L.lar_get_SM_and_LV_crs_from_crs(mid, 0, ctypes.byref(rp), ctypes.byref(ci), ctypes.byref(va),
+                                     ctypes.byref(rh))
+    np.ctypeslib.as_array(va, (len(val),))[:] = val
+    np.ctypeslib.as_array(rh, (op.n,))[:] = rhs
+    return mid
+
+
+@need_crs
+@pytest.mark.parametrize("nthreads", [1, 2, 3, 4])
+def test_chunked_gs_bitexact_with_reference_threads(nthreads):
+    """R's BLOCK_GS sweep depends on its OpenMP thread count through the row
+    chunks; the oracle's chunked restatement reproduces it bit for bit for
+    1..4 threads, two sweeps with the stored rhs and one without"""
+    m = mf.Mesh.box_prism(7, 6, 5, seed=23, jitter=0.2)
+    op = OracleProblem(m.arrays())
+    rng = np.random.default_rng(5 + nthreads)
+    kappa = np.exp(rng.normal(size=op.nel))
+    coef = np.zeros((op.nel, 21))
+    coef[:, 0] = coef[:, 4] = coef[:, 8] = kappa
+    coef[:, 1] = coef[:, 3] = 0.1 * kappa
+    coef[:, 19] = 1.0
+    val, rhs = op.assemble(mf.IND_DIFF, coef)
+    L = ref_crs()
+    gomp = ctypes.CDLL("libgomp.so.1")
+    mid = ref_crs_matrix(L, op, val, rhs)
+    assert L.lar_allocate_preconditioner_crs(mid, 2, 0) > 0
+    L.lar_fill_preconditioner_crs(mid)
+    o = oracle()
+    dptr, dinv = np.zeros(op.n, np.int32), np.zeros(op.n)
+    o.orc_fill_jacobi(op.n, P(op.row_ptr), P(op.col_ind), P(val), P(dptr), P(dinv))
+    x = rng.uniform(-1, 1, op.n)
+    gomp.omp_set_num_threads(nthreads)
+    try:
+        for use_rhs, sweeps in ((1, 2), (0, 1)):
+            w_r, w_o = x.copy(), x.copy()
+            L.lar_perform_BJ_or_GS_iterations_crs(mid, use_rhs, 0, sweeps, op.n, P(w_r), None)
+            o.orc_crs_gs_chunked(op.n, P(op.row_ptr), P(op.col_ind), P(val), P(dptr), P(dinv),
+                                 use_rhs, P(rhs), nthreads, sweeps, P(w_o))
+            assert np.array_equal(w_r, w_o), (nthreads, use_rhs)
+    finally:
+        gomp.omp_set_num_threads(os.cpu_count() or 1)
+        L.lar_free_preconditioner_crs(mid)
+        L.lar_free_SM_and_LV_crs(mid)
+
+
+@pytest.mark.parametrize("nthreads", [1, 5])
+def test_chunked_gs_is_gauss_seidel_per_chunk(nthreads):
+    """independent of R: one chunked sweep solves (D + L_c) v = b - (A - D - L_c) v0,
+    L_c the strictly lower entries whose column lies in the row's own chunk"""
+    m = mf.Mesh.box_prism(5, 4, 4, seed=2, jitter=0.2)
+    op = OracleProblem(m.arrays())
+    rng = np.random.default_rng(11)
+    coef = np.zeros((op.nel, 21))
+    coef[:, 0] = coef[:, 4] = coef[:, 8] = np.exp(rng.normal(size=op.nel))
+    coef[:, 19] = 1.0
+    val, rhs = op.assemble(mf.IND_DIFF, coef)
+    o = oracle()
+    dptr, dinv = np.zeros(op.n, np.int32), np.zeros(op.n)
+    o.orc_fill_jacobi(op.n, P(op.row_ptr), P(op.col_ind), P(val), P(dptr), P(dinv))
+    x = rng.uniform(-1, 1, op.n)
+    w = x.copy()
+    o.orc_crs_gs_chunked(op.n, P(op.row_ptr), P(op.col_ind), P(val), P(dptr), P(dinv), 1, P(rhs),
+                         nthreads, 1, P(w))
+    A = np.zeros((op.n, op.n))
+    for r in range(op.n):
+        A[r, op.col_ind[op.row_ptr[r]:op.row_ptr[r + 1]]] = val[op.row_ptr[r]:op.row_ptr[r + 1]]
+    chunk = np.arange(op.n) // (op.n // nthreads + 1)
+    Lc = np.tril(A, -1) * (chunk[:, None] == chunk[None, :])
+    D = np.diag(np.diag(A))
+    lhs = (D + Lc) @ w + (A - D - Lc) @ x
+    scale = np.abs(A) @ (np.abs(w) + np.abs(x)) + np.abs(rhs)
+    assert (np.abs(lhs - rhs) <= 1e-13 * scale).all()
+    if nthreads > 1:  # the chunks matter: plain GS differs
+        w1 = x.copy()
+        o.orc_crs_gs_chunked(op.n, P(op.row_ptr), P(op.col_ind), P(val), P(dptr), P(dinv), 1,
+                             P(rhs), 1, 1, P(w1))
+        assert np.abs(w1 - w).max() > 1e-6
