@@ -357,12 +357,19 @@ int mf_batch_free(int bid);
  * tensor-product monomial base of degree pdeg = 100*pz + px, 1 <= px, pz <= 3
  * (apr_shape_fun_3D, apd_dg_prism/aps_dg_util.c:408; 6 / 18 / 40 shape
  * functions for p = 1 / 2 / 3, apr_get_el_pdeg_numshap,
- * aps_dg_prism_intf.c:725).  Element DOF blocks follow the element order
- * of the mesh. */
+ * aps_dg_prism_intf.c:725), R's pz = 0 degrees pdeg 1 and 2 (3 / 6 functions
+ * constant in z, one Gauss point in z, penalty degree min(px, pz) = 0,
+ * aps_dg_util.c:495-600, pds_conv_diff_weakform_dg.c:516-527), or the
+ * complete base APC_BASE_COMPLETE_DG of degree p, given as MF_DG_COMPLETE + p
+ * (every monomial of total degree <= p: 4 / 10 / 20 functions, the rules of
+ * the tensor degree (p, p), penalty degree p, aps_dg_util.c:566-870,
+ * 1446-1451).  Element DOF blocks follow the element order of the mesh. */
+#define MF_DG_COMPLETE 1000
 /* one degree for every element */
 int mf_dg_create(int mesh, int pdeg);
-/* a degree per element (apr_get_ent_pdeg, as an adapted field has them):
-   101, 202 or 303 each, or any one tensor degree for all.  A face between
+/* a degree per element (apr_get_ent_pdeg, as an adapted field has them)
+   from one family: 101, 202 or 303 each; 1 or 2 each; or MF_DG_COMPLETE + p
+   each (any one degree for all).  A face between
    different degrees integrates with the rule of the neighbour with more
    points and penalises with the smaller degree
    (pds_conv_diff_weakform_dg.c:508-531); the coupling blocks are

@@ -46,6 +46,8 @@ def coeff_indicator(*names):
 
 # DIFF_IN_CUBE / DIFF_IN_BOX selection (pds_conv_diff_weakform.c:103-113)
 IND_DIFF = coeff_indicator("axx", "ayy", "azz", "s")
+# DG degree code of the complete prism base of degree p: DG_COMPLETE + p
+DG_COMPLETE = 1000
 # generic conv-diff selection used for convection-diffusion workloads
 IND_CONV_DIFF = coeff_indicator("axx", "axy", "axz", "ayx", "ayy", "ayz",
                                 "azx", "azy", "azz", "bx", "by", "bz", "s")

@@ -116,9 +116,15 @@ constexpr int num_shape(int px, int pz) { return (px + 1) * (px + 2) / 2 * (pz +
 // a descending, then b descending; reference vertices (0,0,0), (1,0,0),
 // (0,1,0), (0,0,1); sides with the mesh module's vertex sets {0,1,2},
 // {0,1,3}, {0,2,3}, {1,2,3}; the prism module's triangle rule of degree 2p
-// on the sides.  Degree indices: 0..8 the prisms' (px, pz), 9..11 tets of
-// degree 1..3.
-constexpr int kNDeg = 12;
+// on the sides.  Degree indices: 0..8 the prisms' tensor (px, pz), 9..11
+// tets of degree 1..3, 12..14 prisms with the complete base of degree 1..3
+// (APC_BASE_COMPLETE_DG: every monomial of total degree <= p, the volume and
+// face rules of the tensor degree (p, p), aps_dg_util.c:1446-1451), 15..16
+// the tensor degrees px = 1, 2 with pz = 0 (pdeg 1, 2: constant in z, one
+// Gauss point in z, penalty degree min(px, pz) = 0).  Element kinds of the
+// kernels' templates: 0 tensor prism, 1 tet, 2 complete-base prism.
+constexpr int kNDeg = 17;
+constexpr int kDgComplete = 1000;  // degree code of the complete base: kDgComplete + p
 constexpr int tet_ns(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
 constexpr int tet_points(int p) { return (p + 2) * (p + 1) * (p + 1); }
 constexpr int kTetFaceV[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
@@ -634,6 +640,39 @@ void shape_exponents(int px, int pz, int* ex) {
     for (size_t i = 0; i < b2.size(); i++) ex[k++] = b2[i].first | b2[i].second << 2 | j << 4;
 }
 
+// exponents of the complete base of degree p in apr_shape_fun_3D's order
+// (aps_dg_util.c:566-870): 1, e0, e1, z, z e0, z e1, then e0 e1, e0^2, e1^2,
+// z^2 (p = 2) or e0 e1, e0 e1 z and the loops over the 2D functions
+// e0^i e1^j times z^k (p >= 3)
+void complete_exponents(int p, int* ex) {
+  int n = 0;
+  auto put = [&](int a, int b, int c) { ex[n++] = a | b << 2 | c << 4; };
+  put(0, 0, 0); put(1, 0, 0); put(0, 1, 0); put(0, 0, 1);
+  if (p == 1) return;
+  put(1, 0, 1); put(0, 1, 1); put(1, 1, 0);
+  if (p == 2) {
+    put(2, 0, 0); put(0, 2, 0); put(0, 0, 2);
+    return;
+  }
+  put(1, 1, 1);
+  for (int j = 0; j < 2; j++)
+    for (int i = 0; i < 2; i++)
+      for (int k = 2; k <= p - i - j; k++) put(i, j, k);
+  for (int j = 0; j < 2; j++)
+    for (int i = 2; i <= p - j; i++)
+      for (int k = 0; k <= p - i - j; k++) put(i, j, k);
+  for (int j = 2; j <= p; j++)
+    for (int i = 0; i <= p - j; i++)
+      for (int k = 0; k <= p - i - j; k++) put(i, j, k);
+}
+
+// exponents of the pz = 0 tensor degrees (aps_dg_util.c:495-600): 1, e0, e1
+// (, e0^2, e0 e1, e1^2)
+void flat_exponents(int px, int* ex) {
+  const int e[6][2] = {{0, 0}, {1, 0}, {0, 1}, {2, 0}, {1, 1}, {0, 2}};
+  for (int k = 0; k < (px == 1 ? 3 : 6); k++) ex[k] = e[k][0] | e[k][1] << 2;
+}
+
 // face rules (apr_set_quadr_2D, aps_dg_util.c:1499-1555): [n][3] = (x0, x1, w)
 void face_rules(int px, int pz, std::vector<double>& qt, std::vector<double>& qq) {
   std::vector<double> l, w;
@@ -945,17 +984,33 @@ struct DgRowArgs {
 };
 
 __host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
-__host__ __device__ constexpr int deg_index(int px, int pz) { return 3 * (pz - 1) + (px - 1); }
+__host__ __device__ constexpr int deg_index(int px, int pz) {
+  return pz == 0 ? 14 + px : 3 * (pz - 1) + (px - 1);
+}
 __host__ __device__ constexpr int tet_deg_index(int p) { return 8 + p; }
-// degree index of a launch's elements: a tet of degree PX, or a prism (PX, PZ)
-__host__ __device__ constexpr int deg_of(int px, int pz, bool tet) {
-  return tet ? tet_deg_index(px) : deg_index(px, pz);
+__host__ __device__ constexpr int complete_deg_index(int p) { return 11 + p; }
+// degree index of a launch's elements of kind 0 (prism (PX, PZ)), 1 (tet of
+// degree PX) or 2 (complete-base prism of degree PX)
+__host__ __device__ constexpr int deg_of(int px, int pz, int kind) {
+  return kind == 1 ? tet_deg_index(px) : kind == 2 ? complete_deg_index(px) : deg_index(px, pz);
 }
 __host__ __device__ constexpr int ns_of_deg(int d) {
-  return d >= 9 ? tet_ns(d - 8) : num_shape(d % 3 + 1, d / 3 + 1);
+  return d >= 15 ? num_shape(d - 14, 0)
+         : d >= 12 ? tet_ns(d - 11)
+         : d >= 9 ? tet_ns(d - 8) : num_shape(d % 3 + 1, d / 3 + 1);
 }
 __host__ __device__ constexpr int pmin_of_deg(int d) {
-  return d >= 9 ? d - 8 : d % 3 < d / 3 ? d % 3 + 1 : d / 3 + 1;
+  return d >= 15 ? 0
+         : d >= 12 ? d - 11
+         : d >= 9 ? d - 8 : d % 3 < d / 3 ? d % 3 + 1 : d / 3 + 1;
+}
+// the tensor degree (px, pz) whose volume and face rules degree index d uses
+// (tets: the isotropic prism degree's triangle rule)
+__host__ __device__ constexpr int rule_px(int d) {
+  return d >= 15 ? d - 14 : d >= 12 ? d - 11 : d >= 9 ? d - 8 : d % 3 + 1;
+}
+__host__ __device__ constexpr int rule_pz(int d) {
+  return d >= 15 ? 0 : d >= 12 ? d - 11 : d >= 9 ? d - 8 : d / 3 + 1;
 }
 
 // the rule of a face between element e (degree de) and its neighbour nb
@@ -1168,16 +1223,16 @@ struct LaneShapes {
 // load-vector row (Rg[28..31]).  Mixed problems pass each side's rule,
 // first point and penalty / h_F in fq, fst and fpen.
 // sizes of a launch's elements: a tet of degree PX (TET) or a prism (PX, PZ)
-template <int PX, int PZ, bool TET>
+template <int PX, int PZ, int TET>
 struct DgEl {
-  static constexpr int NS = TET ? tet_ns(PX) : num_shape(PX, PZ);
-  static constexpr int NG = TET ? tet_points(PX) : tri_points(PX) * (PZ + 1);
+  static constexpr int NS = TET ? tet_ns(PX) : num_shape(PX, PZ);  // complete: tet_ns too
+  static constexpr int NG = TET == 1 ? tet_points(PX) : tri_points(PX) * (PZ + 1);
   static constexpr int NT = (NS + 15) / 16;
-  static constexpr int NSIDES = TET ? 4 : 5;
+  static constexpr int NSIDES = TET == 1 ? 4 : 5;
   static constexpr int DE = deg_of(PX, PZ, TET);
 };
 
-template <int PX, int PZ, bool MIXED, bool TET = false>
+template <int PX, int PZ, bool MIXED, int TET = 0>
 __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, const double* sX,
                                             const int* fq, const int* fst, const double* fpen,
                                             double* Rg) {
@@ -1191,7 +1246,7 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
     const double* qg = A.qv + 4 * k;
     double a[9], b[3], mc, q0, q1, q2, sv, J[9], Ji[9], x[3];
     dg_coefs(A, e, a, b, mc, q0, q1, q2, sv);
-    const double vol = fabs(TET ? tet_jac(sX, qg, J, Ji, x) : prism_jac(sX, qg, J, Ji, x)) * qg[3];
+    const double vol = fabs(TET == 1 ? tet_jac(sX, qg, J, Ji, x) : prism_jac(sX, qg, J, Ji, x)) * qg[3];
     if (A.src_model != SRC_CONST && A.mask.s) {
       double uu, lapl;
       exact_sol(A.src_model, x, &uu, &lapl);
@@ -1220,7 +1275,7 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
     if (MIXED) {
       while (k >= fst[sd + 1]) sd++;
       const int q = fq[sd];
-      rule = (TET || sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * (k - fst[sd]);
+      rule = (TET == 1 || sd < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * (k - fst[sd]);
     } else {
       int fq0 = k - NG;
       if (fq0 < 2 * Q::NGT) {
@@ -1235,7 +1290,7 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
     const int fi = 5 * e + sd, nb = R.fnb[fi];
     const double pen = MIXED ? fpen[sd] : pmin_of_deg(DE) * R.penalty_const / R.fhf[fi];
     FacePt F;
-    if constexpr (TET) face_point_tet(A, sX, e, sd, rule, F);
+    if constexpr (TET == 1) face_point_tet(A, sX, e, sd, rule, F);
     else face_point(A, sX, e, sd, rule, F);
     put_powers(F.eta, Rg);
     if (nb >= 0) {
@@ -1284,7 +1339,7 @@ __device__ __forceinline__ void diag_record(const DgRowArgs& R, int e, int k, co
 // (phi, d/deta phi) of trial and test function, the load-vector row) to
 // LDS, and the wave runs the chunk through v_mfma_f64_16x16x4, one K-step
 // per point.
-template <int PX, int PZ, bool MIXED, bool TET = false>
+template <int PX, int PZ, bool MIXED, int TET = 0>
 __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   using S = DgEl<PX, PZ, TET>;
   using Q = DgRowShape<PX, PZ>;
@@ -1304,7 +1359,7 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
   if (MIXED && lane < NSD) {
     // each face's rule and penalty / h_F (weakform_dg.c:508-531)
     const int sd = lane, nb = R.fnb[5 * e + sd];
-    const bool tri = TET || sd < 2;
+    const bool tri = TET == 1 || sd < 2;
     int q = DE, pm = pmin_of_deg(DE);
     if (nb >= 0) {
       const int dn = R.deg[nb];
@@ -1388,19 +1443,19 @@ __global__ void __launch_bounds__(64) k_dg_diag(DgRowArgs R) {
 // 4x4 matrix M between (psi, d/deta psi) of the neighbour's trial function
 // (row) and (phi, d/deta phi) of the element's test function (column)
 // (Rg[24..39]).  sX holds both elements' node coordinates.
-template <int PXE, int PZE, int PXN, int PZN, bool TE = false, bool TN = false>
+template <int PXE, int PZE, int PXN, int PZN, int TE = 0, int TN = 0>
 __device__ __forceinline__ void couple_record(const DgRowArgs& R, int e, int s, int nb, int code,
                                               int q, int kp, const double* sX, double* Rg) {
   constexpr int DE = deg_of(PXE, PZE, TE), DN = deg_of(PXN, PZN, TN);
   const DgElemArgs& A = R.E;
   const int slot = 5 * e + s;
-  const double* rule = (TE || TN || s < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * kp;
+  const double* rule = (TE == 1 || TN == 1 || s < 2 ? R.qt + 3 * R.qt_off[q] : R.qq + 3 * R.qq_off[q]) + 3 * kp;
   FacePt F;
-  if constexpr (TE) face_point_tet(A, sX, e, s, rule, F);
+  if constexpr (TE == 1) face_point_tet(A, sX, e, s, rule, F);
   else face_point(A, sX, e, s, rule, F);
   double eta2[3], J2[9], Ji2[9], x2[3];
-  neighbour_eta_x<TE, TN>(s, code, F.eta, eta2);
-  if constexpr (TN) tet_jac(sX + 18, eta2, J2, Ji2, x2);
+  neighbour_eta_x<TE == 1, TN == 1>(s, code, F.eta, eta2);
+  if constexpr (TN == 1) tet_jac(sX + 18, eta2, J2, Ji2, x2);
   else prism_jac(sX + 18, eta2, J2, Ji2, x2);
   double a[9], b[3], mc, q0, q1, q2, sv;
   dg_coefs(A, nb, a, b, mc, q0, q1, q2, sv);
@@ -1432,12 +1487,12 @@ __device__ __forceinline__ void couple_record(const DgRowArgs& R, int e, int s, 
 // (the (1,2) block of pdr_conv_diff_comp_fa_stiff_mat_app seen from either
 // side), ns_e x ns_nb, column-major.  Records hold the powers of both
 // elements' local coordinates.
-template <int PXE, int PZE, int PXN, int PZN, bool TET_E = false, bool TET_N = false>
+template <int PXE, int PZE, int PXN, int PZN, int TET_E = 0, int TET_N = 0>
 __global__ void __launch_bounds__(64) k_dg_couple(DgRowArgs R, const int* slots) {
   constexpr int NSE = DgEl<PXE, PZE, TET_E>::NS, NTE = (NSE + 15) / 16;
   constexpr int NSN = DgEl<PXN, PZN, TET_N>::NS, NTN = (NSN + 15) / 16;
   constexpr int DE = deg_of(PXE, PZE, TET_E), DN = deg_of(PXN, PZN, TET_N), REC = 41;  // 40 + bank pad
-  constexpr bool TRI = TET_E || TET_N;  // a face with a tet is a triangle
+  constexpr bool TRI = TET_E == 1 || TET_N == 1;  // a face with a tet is a triangle
   constexpr int QE = TRI ? 0 : (PXE + 1) * (PZE + 1), QN = TRI ? 0 : (PXN + 1) * (PZN + 1);
   constexpr int TE = tri_points(PXE), TN = tri_points(PXN);
   constexpr int NPMAX = cmax(cmax(QE, QN), cmax(TE, TN));
@@ -1529,7 +1584,7 @@ __device__ __forceinline__ void shape_from_powers(const double* P, int code, dou
   u[3] = pa * pb * qc;
 }
 
-template <int PX, int PZ, bool MIXED, bool TET = false>
+template <int PX, int PZ, bool MIXED, int TET = 0>
 __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
   using S = DgEl<PX, PZ, TET>;
   using Q = DgRowShape<PX, PZ>;
@@ -1553,10 +1608,11 @@ __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
   if (lane < NS) sex[lane] = R.ex[lane];
   if (act)
     for (int c = i; c < 18; c += NS) sX[le][c] = A.nc[(size_t)c * A.nel + e];
-  if (MIXED && act && i < NSD) {
-    // each face's rule and penalty / h_F (weakform_dg.c:508-531)
-    const int sd = i, nb = R.fnb[5 * e + sd];
-    const bool tri = TET || sd < 2;
+  // each face's rule and penalty / h_F (weakform_dg.c:508-531); an element
+  // may have fewer lanes (NS) than sides
+  for (int sd = i; MIXED && act && sd < NSD; sd += NS) {
+    const int nb = R.fnb[5 * e + sd];
+    const bool tri = TET == 1 || sd < 2;
     int q = DE, pm = pmin_of_deg(DE);
     if (nb >= 0) {
       const int dn = R.deg[nb];
@@ -1626,11 +1682,11 @@ __global__ void __launch_bounds__(64) k_dg_diag_v(DgRowArgs R, int cnt) {
   R.brhs[(R.xoff ? R.xoff[e] : (size_t)e * NS) + i] = f;
 }
 
-template <int PXE, int PZE, int PXN, int PZN, bool TET_E = false, bool TET_N = false>
+template <int PXE, int PZE, int PXN, int PZN, int TET_E = 0, int TET_N = 0>
 __global__ void __launch_bounds__(64) k_dg_couple_v(DgRowArgs R, const int* slots, int cnt) {
   constexpr int NSE = DgEl<PXE, PZE, TET_E>::NS, NSN = DgEl<PXN, PZN, TET_N>::NS;
   constexpr int DE = deg_of(PXE, PZE, TET_E), DN = deg_of(PXN, PZN, TET_N), REC = 41;  // 40 + bank pad
-  constexpr bool TRI = TET_E || TET_N;  // a face with a tet is a triangle
+  constexpr bool TRI = TET_E == 1 || TET_N == 1;  // a face with a tet is a triangle
   constexpr int QE = TRI ? 0 : (PXE + 1) * (PZE + 1), QN = TRI ? 0 : (PXN + 1) * (PZN + 1);
   constexpr int TE = tri_points(PXE), TN = tri_points(PXN);
   constexpr int NPMAX = cmax(cmax(QE, QN), cmax(TE, TN));
@@ -2191,6 +2247,7 @@ __global__ void k_dg_sub(long long n, const double* y, double* w, const int* sto
 template <class F>
 static int with_ns(int d, F& f) {
   switch (ns_of_deg(d)) {
+    case 3: return f.template run<3>(d);
     case 4: return f.template run<4>(d);
     case 6: return f.template run<6>(d);
     case 9: return f.template run<9>(d);
@@ -2348,7 +2405,7 @@ void row_args(const DgProblem& D, int d, DgRowArgs& R) {
   R.blk = D.d_blk; R.brhs = D.d_brhs;
 }
 
-template <int PX, int PZ, bool TET = false>
+template <int PX, int PZ, int TET = 0>
 int launch_diag_t(DgProblem& D) {
   constexpr int d = deg_of(PX, PZ, TET);
   const int cnt = D.grp_ptr[d + 1] - D.grp_ptr[d];
@@ -2375,7 +2432,7 @@ int launch_diag_t(DgProblem& D) {
   return 0;
 }
 
-template <int PXE, int PZE, int PXN, int PZN, bool TET_E = false, bool TET_N = false>
+template <int PXE, int PZE, int PXN, int PZN, int TET_E = 0, int TET_N = 0>
 int launch_couple_t(DgProblem& D) {
   constexpr int de = deg_of(PXE, PZE, TET_E), dn = deg_of(PXN, PZN, TET_N);
   const int s0 = D.slot_ptr[kNDeg * de + dn], cnt = D.slot_ptr[kNDeg * de + dn + 1] - s0;
@@ -2407,10 +2464,15 @@ int launch_rows(DgProblem& D) {
   MF_DG_CASE(1, 2) MF_DG_CASE(2, 2) MF_DG_CASE(3, 2)
   MF_DG_CASE(1, 3) MF_DG_CASE(2, 3) MF_DG_CASE(3, 3)
 #undef MF_DG_CASE
-  // tets of degree 1..3
-  if (rc == 0) rc = launch_diag_t<1, 1, true>(D);
-  if (rc == 0) rc = launch_diag_t<2, 2, true>(D);
-  if (rc == 0) rc = launch_diag_t<3, 3, true>(D);
+  // tets of degree 1..3, complete-base prisms of degree 1..3, pz = 0 prisms
+  if (rc == 0) rc = launch_diag_t<1, 1, 1>(D);
+  if (rc == 0) rc = launch_diag_t<2, 2, 1>(D);
+  if (rc == 0) rc = launch_diag_t<3, 3, 1>(D);
+  if (rc == 0) rc = launch_diag_t<1, 1, 2>(D);
+  if (rc == 0) rc = launch_diag_t<2, 2, 2>(D);
+  if (rc == 0) rc = launch_diag_t<3, 3, 2>(D);
+  if (rc == 0) rc = launch_diag_t<1, 0>(D);
+  if (rc == 0) rc = launch_diag_t<2, 0>(D);
   // couplings between equal degrees, and between the isotropic degrees of
   // mixed problems
 #define MF_DG_PAIR(XE, ZE, XN, ZN) \
@@ -2424,13 +2486,19 @@ int launch_rows(DgProblem& D) {
   // faces with a tet (triangles): tet-tet, tet-prism and prism-tet pairs of
   // degrees 1..3 (isotropic prisms)
 #define MF_DG_TPAIR(PE, PN)                                                      \
-  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, true, true>(D);              \
-  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, true, false>(D);             \
-  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, false, true>(D);
+  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, 1, 1>(D);                    \
+  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, 1, 0>(D);                    \
+  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, 0, 1>(D);                    \
+  if (rc == 0) rc = launch_couple_t<PE, PE, PN, PN, 2, 2>(D);
   MF_DG_TPAIR(1, 1) MF_DG_TPAIR(1, 2) MF_DG_TPAIR(1, 3)
   MF_DG_TPAIR(2, 1) MF_DG_TPAIR(2, 2) MF_DG_TPAIR(2, 3)
   MF_DG_TPAIR(3, 1) MF_DG_TPAIR(3, 2) MF_DG_TPAIR(3, 3)
 #undef MF_DG_TPAIR
+  // pz = 0 prisms among themselves
+  if (rc == 0) rc = launch_couple_t<1, 0, 1, 0>(D);
+  if (rc == 0) rc = launch_couple_t<1, 0, 2, 0>(D);
+  if (rc == 0) rc = launch_couple_t<2, 0, 1, 0>(D);
+  if (rc == 0) rc = launch_couple_t<2, 0, 2, 0>(D);
   return rc;
 }
 
@@ -2661,11 +2729,12 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     set_error("mf_init has not been called");
     return -1;
   }
-  bool any_tet = false;
+  bool any_tet = false, any_new = false;
   for (int e = 0; e < m->nel; e++) {
+    any_new = any_new || pdeg[e] > kDgComplete || pdeg[e] < 100;
     if (m->type[e] == kTetra) {
       any_tet = true;
-      if (pdeg[e] % 100 != pdeg[e] / 100) {
+      if (pdeg[e] > kDgComplete || pdeg[e] % 100 != pdeg[e] / 100) {
         set_error("mf_dg_create: a tetrahedron takes an isotropic degree (101, 202 or 303)");
         return -1;
       }
@@ -2675,14 +2744,17 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     }
   }
   const size_t nel = m->nel;
-  bool mixed = any_tet;
+  // the complete base and the pz = 0 degrees always take the per-degree
+  // (mixed) layout and rules
+  bool mixed = any_tet || any_new;
   for (size_t e = 1; e < nel; e++) mixed = mixed || pdeg[e] != pdeg[0];
   auto D = std::make_unique<DgProblem>();
   D->mesh = mesh; D->nel = m->nel; D->mixed = mixed;
   D->deg.resize(nel);
   for (size_t e = 0; e < nel; e++)
-    D->deg[e] = m->type[e] == kTetra ? tet_deg_index(pdeg[e] % 100)
-                                     : deg_index(pdeg[e] % 100, pdeg[e] / 100);
+    D->deg[e] = m->type[e] == kTetra     ? tet_deg_index(pdeg[e] % 100)
+                : pdeg[e] > kDgComplete ? complete_deg_index(pdeg[e] - kDgComplete)
+                                        : deg_index(pdeg[e] % 100, pdeg[e] / 100);
   if (!mixed) {
     D->px = pdeg[0] % 100; D->pz = pdeg[0] / 100;
     D->ns = num_shape(D->px, D->pz);
@@ -2782,8 +2854,8 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     for (int d = 0; d < kNDeg; d++) {
       if (D->grp_ptr[d + 1] == D->grp_ptr[d]) continue;
       D->qv_off[d] = (int)all.size() / 4;
-      if (d >= 9) tet_volume_rule(d - 8, qv);
-      else volume_rule(d % 3 + 1, d / 3 + 1, qv);
+      if (d >= 9 && d < 12) tet_volume_rule(d - 8, qv);
+      else volume_rule(rule_px(d), rule_pz(d), qv);
       all.insert(all.end(), qv.begin(), qv.end());
     }
     MF_HIP(hipMalloc(&D->d_qv9, all.size() * sizeof(double)));
@@ -2810,6 +2882,27 @@ int dg_create(int mesh, const std::vector<int>& pdeg) {
     D->qq_off[d] = 0;
     D->ngq9[d] = 0;
     tet_exponents(p, D->ex9[d]);
+  }
+  // complete-base prisms: the rules of the tensor degree (p, p)
+  for (int p = 1; p <= 3; p++) {
+    const int d = complete_deg_index(p), dp = deg_index(p, p);
+    D->qt_off[d] = D->qt_off[dp];
+    D->ngt9[d] = D->ngt9[dp];
+    D->qq_off[d] = D->qq_off[dp];
+    D->ngq9[d] = D->ngq9[dp];
+    complete_exponents(p, D->ex9[d]);
+  }
+  // pz = 0: one Gauss point in z on the quads
+  for (int px = 1; px <= 2; px++) {
+    const int d = deg_index(px, 0);
+    face_rules(px, 0, qt, qq);
+    D->qt_off[d] = (int)qt_all.size() / 3;
+    D->ngt9[d] = (int)qt.size() / 3;
+    D->qq_off[d] = (int)qq_all.size() / 3;
+    D->ngq9[d] = (int)qq.size() / 3;
+    qt_all.insert(qt_all.end(), qt.begin(), qt.end());
+    qq_all.insert(qq_all.end(), qq.begin(), qq.end());
+    flat_exponents(px, D->ex9[d]);
   }
   if (!mixed) {
     const int d = D->deg[0];
@@ -3001,14 +3094,24 @@ Problem* dg_batch_problem(int did) {
 
 }  // namespace mf
 
+namespace {
+// a degree code of the C ABI: tensor 100 pz + px (px, pz 1..3, or pz = 0 with
+// px 1, 2), or the complete base kDgComplete + p (p 1..3)
+bool dg_degree_ok(int pdeg) {
+  if (pdeg > kDgComplete) return pdeg - kDgComplete >= 1 && pdeg - kDgComplete <= 3;
+  const int px = pdeg % 100, pz = pdeg / 100;
+  return (px >= 1 && px <= 3 && pz >= 1 && pz <= 3) || pdeg == 1 || pdeg == 2;
+}
+}  // namespace
+
 extern "C" {
 
 int mf_dg_create(int mesh, int pdeg) {
   Mesh* m = mesh_by_id(mesh);
   if (!m) return -3;
-  const int px = pdeg % 100, pz = pdeg / 100;
-  if (px < 1 || px > 3 || pz < 1 || pz > 3) {
-    set_error("mf_dg_create: pdeg must be 100*pz + px with 1 <= px, pz <= 3");
+  if (!dg_degree_ok(pdeg)) {
+    set_error("mf_dg_create: pdeg must be 100*pz + px with 1 <= px, pz <= 3, 1 or 2 (pz = 0), "
+              "or MF_DG_COMPLETE + p with 1 <= p <= 3");
     return -1;
   }
   return dg_create(mesh, std::vector<int>(m->nel, pdeg));
@@ -3024,13 +3127,19 @@ int mf_dg_create_mixed(int mesh, const int* pdeg) {
   std::vector<int> v(pdeg, pdeg + m->nel);
   bool same = true;
   for (int e = 0; e < m->nel; e++) same = same && v[e] == v[0];
+  // one family per mesh: isotropic tensor degrees, pz = 0 degrees, or the
+  // complete base
+  auto family = [](int c) {
+    if (c > kDgComplete) return dg_degree_ok(c) ? 2 : -1;
+    if (c == 1 || c == 2) return 1;
+    return (c % 100 == c / 100 && c % 100 >= 1 && c % 100 <= 3) ? 0 : -1;
+  };
   for (int e = 0; e < m->nel; e++) {
-    const int px = v[e] % 100, pz = v[e] / 100;
-    const bool ok = same ? (px >= 1 && px <= 3 && pz >= 1 && pz <= 3)
-                         : (px == pz && px >= 1 && px <= 3);
+    const bool ok = same ? dg_degree_ok(v[e]) : family(v[e]) >= 0 && family(v[e]) == family(v[0]);
     if (!ok) {
-      set_error("mf_dg_create_mixed: degrees must be 101, 202 or 303 per element (any one "
-                "tensor degree when all are equal)");
+      set_error("mf_dg_create_mixed: degrees per element from one family: 101, 202, 303; or "
+                "1, 2 (pz = 0); or MF_DG_COMPLETE + p, 1 <= p <= 3 (any one degree when all are "
+                "equal)");
       return -1;
     }
   }

@@ -1101,22 +1101,90 @@ static int dg_base2d(int p, const double *eta, double *f, double *fx, double *fy
   return k;
 }
 
-/* tensor-product prism base for pdeg = 100*pz + px, px, pz >= 1
-   (apr_shape_fun_3D, aps_dg_util.c:408-992): 1, e0, e1, z, z e0, z e1,
-   then the 2D functions i >= 3 times z^0 and z^1, then all 2D functions
-   times z^j, j >= 2 */
+/* prism bases of the DG module (apr_shape_fun_3D, aps_dg_util.c:408-992).
+   Tensor base, pdeg = 100*pz + px:
+     pz = 0 (pdeg 1, 2; :495-600): 1, e0, e1 (, e0^2, e0 e1, e1^2), constant
+     in z;
+     px, pz >= 1: 1, e0, e1, z, z e0, z e1, then the 2D functions i >= 3
+     times z^0 and z^1, then all 2D functions times z^j, j >= 2.
+   Complete base of degree p, encoded ORC_DG_COMPLETE + p (:566-870):
+     p = 1: 1, e0, e1, z; p = 2: 1, e0, e1, z, z e0, z e1, e0 e1, e0^2, e1^2,
+     z^2; p >= 3: 1, e0, e1, z, z e0, z e1, e0 e1, e0 e1 z, then the 2D
+     complete functions times z^k in the loops of :822-870 (total degree
+     <= p). */
 int orc_dg_shape(int pdeg, const double *eta, double *phi, double *dx, double *dy,
                  double *dz) {
-  const int px = pdeg % 100, pz = pdeg / 100;
-  if (px < 1 || pz < 1) return -1;
+  const int complete = pdeg > ORC_DG_COMPLETE;
+  const int px = complete ? pdeg - ORC_DG_COMPLETE : pdeg % 100;
+  const int pz = complete ? px : pdeg / 100;
+  if (px < 1 || px > 3 || pz < 0 || (pz == 0 && px > 2)) return -1;
+  if (pz == 0) {
+    phi[0] = 1.0; phi[1] = eta[0]; phi[2] = eta[1];
+    dx[0] = 0.0; dx[1] = 1.0; dx[2] = 0.0;
+    dy[0] = 0.0; dy[1] = 0.0; dy[2] = 1.0;
+    dz[0] = 0.0; dz[1] = 0.0; dz[2] = 0.0;
+    if (px == 1) return 3;
+    phi[3] = eta[0] * eta[0]; phi[4] = eta[0] * eta[1]; phi[5] = eta[1] * eta[1];
+    dx[3] = 2 * eta[0]; dx[4] = eta[1]; dx[5] = 0.0;
+    dy[3] = 0.0; dy[4] = eta[0]; dy[5] = 2 * eta[1];
+    dz[3] = 0.0; dz[4] = 0.0; dz[5] = 0.0;
+    return 6;
+  }
+  phi[0] = 1.0; phi[1] = eta[0]; phi[2] = eta[1]; phi[3] = eta[2];
+  dx[0] = 0.0; dx[1] = 1.0; dx[2] = 0.0; dx[3] = 0.0;
+  dy[0] = 0.0; dy[1] = 0.0; dy[2] = 1.0; dy[3] = 0.0;
+  dz[0] = 0.0; dz[1] = 0.0; dz[2] = 0.0; dz[3] = 1.0;
+  if (complete && px == 1) return 4;
+  phi[4] = eta[2] * eta[0]; phi[5] = eta[2] * eta[1];
+  dx[4] = eta[2]; dx[5] = 0.0;
+  dy[4] = 0.0; dy[5] = eta[2];
+  dz[4] = eta[0]; dz[5] = eta[1];
+  if (!complete && px == 1 && pz == 1) return 6;
   double f[64], fx[64], fy[64], u[16], du[16];
+  if (complete) {
+    phi[6] = eta[1] * eta[0];
+    dx[6] = eta[1]; dy[6] = eta[0]; dz[6] = 0.0;
+    if (px == 2) {
+      phi[7] = eta[0] * eta[0]; phi[8] = eta[1] * eta[1]; phi[9] = eta[2] * eta[2];
+      dx[7] = 2.0 * eta[0]; dx[8] = 0.0; dx[9] = 0.0;
+      dy[7] = 0.0; dy[8] = 2.0 * eta[1]; dy[9] = 0.0;
+      dz[7] = 0.0; dz[8] = 0.0; dz[9] = 2.0 * eta[2];
+      return 10;
+    }
+    phi[7] = eta[2] * eta[1] * eta[0];
+    dx[7] = eta[2] * eta[1]; dy[7] = eta[2] * eta[0]; dz[7] = eta[1] * eta[0];
+    dg_base2d(px, eta, f, fx, fy);
+    dg_mono1d(px, eta[2], u, du);
+    const int p = px;
+    int kk = 8, ii = 0;
+    for (int j = 0; j < 2; j++)
+      for (int i = 0; i < 2; i++) {
+        for (int k = 2; k <= p - i - j; k++) {
+          phi[kk] = f[ii] * u[k]; dx[kk] = fx[ii] * u[k]; dy[kk] = fy[ii] * u[k];
+          dz[kk] = f[ii] * du[k]; kk++;
+        }
+        ii++;
+      }
+    for (int j = 0; j < 2; j++)
+      for (int i = 2; i <= p - j; i++) {
+        for (int k = 0; k <= p - i - j; k++) {
+          phi[kk] = f[ii] * u[k]; dx[kk] = fx[ii] * u[k]; dy[kk] = fy[ii] * u[k];
+          dz[kk] = f[ii] * du[k]; kk++;
+        }
+        ii++;
+      }
+    for (int j = 2; j <= p; j++)
+      for (int i = 0; i <= p - j; i++) {
+        for (int k = 0; k <= p - i - j; k++) {
+          phi[kk] = f[ii] * u[k]; dx[kk] = fx[ii] * u[k]; dy[kk] = fy[ii] * u[k];
+          dz[kk] = f[ii] * du[k]; kk++;
+        }
+        ii++;
+      }
+    return kk;
+  }
   int n2 = dg_base2d(px, eta, f, fx, fy);
   dg_mono1d(pz, eta[2], u, du);
-  phi[0] = 1.0; phi[1] = eta[0]; phi[2] = eta[1]; phi[3] = eta[2];
-  phi[4] = eta[2] * eta[0]; phi[5] = eta[2] * eta[1];
-  dx[0] = 0.0; dx[1] = 1.0; dx[2] = 0.0; dx[3] = 0.0; dx[4] = eta[2]; dx[5] = 0.0;
-  dy[0] = 0.0; dy[1] = 0.0; dy[2] = 1.0; dy[3] = 0.0; dy[4] = 0.0; dy[5] = eta[2];
-  dz[0] = 0.0; dz[1] = 0.0; dz[2] = 0.0; dz[3] = 1.0; dz[4] = eta[0]; dz[5] = eta[1];
   int k = 6;
   for (int j = 0; j < 2; j++)
     for (int i = 3; i < n2; i++) {
@@ -1129,6 +1197,41 @@ int orc_dg_shape(int pdeg, const double *eta, double *phi, double *dx, double *d
       k++;
     }
   return k;
+}
+
+/* number of shape functions (apr_get_el_pdeg_numshap,
+   aps_dg_prism_intf.c:725-780) */
+int orc_dg_ns(int pdeg) {
+  if (pdeg > ORC_DG_COMPLETE) {
+    const int p = pdeg - ORC_DG_COMPLETE;
+    return (p + 1) * (p + 2) * (p + 3) / 6;
+  }
+  const int px = pdeg % 100, pz = pdeg / 100;
+  return (pz + 1) * (px + 1) * (px + 2) / 2;
+}
+
+/* penalty degree of a face (pds_conv_diff_weakform_dg.c:516-527): complete
+   base min(p1, p2) (boundary: p1); tensor base min over the neighbours of
+   min(px, pz), which is zero for the pz = 0 degrees; pdeg2 <= 0: boundary */
+int orc_dg_penalty_degree(int pdeg1, int pdeg2) {
+  if (pdeg1 > ORC_DG_COMPLETE) {
+    const int p1 = pdeg1 - ORC_DG_COMPLETE;
+    if (pdeg2 <= 0) return p1;
+    const int p2 = pdeg2 - ORC_DG_COMPLETE;
+    return p1 < p2 ? p1 : p2;
+  }
+  int d = pdeg1 / 100 < pdeg1 % 100 ? pdeg1 / 100 : pdeg1 % 100;
+  if (pdeg2 > 0) {
+    const int e = pdeg2 / 100 < pdeg2 % 100 ? pdeg2 / 100 : pdeg2 % 100;
+    if (e < d) d = e;
+  }
+  return d;
+}
+
+/* the tensor degree whose volume and face rules a complete-base degree
+   uses (aps_dg_util.c:1446-1451, 1531-1536: orders 2p in x and z) */
+static int dg_rule_pdeg(int pdeg) {
+  return pdeg > ORC_DG_COMPLETE ? 101 * (pdeg - ORC_DG_COMPLETE) : pdeg;
 }
 
 /* Gauss-Legendre points and weights on [-1,1] (Newton on P_n) */
@@ -1200,6 +1303,7 @@ static int dunavant(int order, double *l, double *w) {
    1415-1496): degree 2 px on the triangle x degree 2 pz on the line; point
    (l2, l3, z), weight 1/2 w_tri w_line */
 int orc_dg_quadr(int pdeg, double *xg, double *wg) {
+  pdeg = dg_rule_pdeg(pdeg);
   int ox = 2 * (pdeg % 100), oz = 2 * (pdeg / 100);
   if (ox == 0) ox = 1;
   if (oz == 0) oz = 1;
@@ -1380,6 +1484,7 @@ double orc_dg_face_ds(int side, const double *nc, const double *eta, double *n, 
    Gauss-Legendre of degree 2 px (first coordinate) x 2 pz on [-1,1]^2 with
    weights 1/2 wx wz, triangles the degree-2 px rule with weights 1/2 w */
 int orc_dg_face_quadr(int pdeg, int quad, double *xg, double *wg) {
+  pdeg = dg_rule_pdeg(pdeg);
   const int px = pdeg % 100, pz = pdeg / 100;
   if (quad) {
     double x1[8], w1[8], x2[8], w2[8];

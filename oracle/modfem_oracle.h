@@ -163,11 +163,17 @@ void orc_gs_sweeps(int n, const int *row_ptr, const int *col_ind, const double *
                    int nsweeps, int use_rhs, const double *b, double *v);
 void orc_set_gs(const int *order, int nsweeps);
 
-/* DG prism approximation, tensor base pdeg = 100 pz + px (apd_dg_prism):
-   shape functions, prism quadrature, element integral (nreq = 1; column-
-   major stiff[j*ns + i], i = test function; arrays sized for ns <= 128) */
+/* DG prism approximation (apd_dg_prism): tensor base pdeg = 100 pz + px
+   (px 1..3, pz 1..3, or pz = 0 with px 1, 2), complete base of degree p as
+   ORC_DG_COMPLETE + p (p 1..3): shape functions, number of them, the
+   penalty degree of a face (pdeg2 <= 0: boundary), prism quadrature,
+   element integral (nreq = 1; column-major stiff[j*ns + i], i = test
+   function; arrays sized for ns <= 128) */
+#define ORC_DG_COMPLETE 1000
 int orc_dg_shape(int pdeg, const double *eta, double *phi, double *dx, double *dy,
                  double *dz);
+int orc_dg_ns(int pdeg);
+int orc_dg_penalty_degree(int pdeg1, int pdeg2);
 int orc_dg_quadr(int pdeg, double *xg, double *wg);
 int orc_dg_num_int_el(int pdeg, const double *node_coor, const int *ind, const double *coef,
                       int src_model, double *stiff, double *rhs);

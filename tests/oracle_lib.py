@@ -52,6 +52,8 @@ def oracle():
             "orc_drotg": (None, [V, V, V, V]),
             "orc_dg_shape": (I, [I, V, V, V, V, V]),
             "orc_dg_quadr": (I, [I, V, V]),
+            "orc_dg_ns": (I, [I]),
+            "orc_dg_penalty_degree": (I, [I, I]),
             "orc_dg_num_int_el": (I, [I, V, V, V, I, V, V]),
             "orc_dg_face_quadr": (I, [I, I, V, V]),
             "orc_dg_face_eta": (None, [I, V, V]),
@@ -320,6 +322,8 @@ DG_FACE_V = [[0, 1, 2], [3, 4, 5], [0, 1, 4, 3], [1, 2, 5, 4], [2, 0, 3, 5]]
 # tetrahedron sides (the mesh module's face-node table)
 DGT_FACE_V = [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]]
 TETRA = 7
+# complete prism base of degree p: pdeg DG_COMPLETE + p (the oracle's encoding)
+DG_COMPLETE = 1000
 
 
 def dg_faces(el_nodes, el_type=None):
@@ -345,8 +349,10 @@ def dg_degrees(pdeg, nel, el_type=None):
     """per-element degrees, block sizes and block offsets (pdeg: one degree
     or one per element; tets take the complete base of degree pdeg % 100)"""
     pd = np.full(nel, int(pdeg)) if np.isscalar(pdeg) else np.asarray(pdeg, int)
-    px, pz = pd % 100, pd // 100
-    ns = (px + 1) * (px + 2) // 2 * (pz + 1)
+    comp = pd > DG_COMPLETE
+    px = np.where(comp, pd - DG_COMPLETE, pd % 100)
+    pz = pd // 100
+    ns = np.where(comp, (px + 1) * (px + 2) * (px + 3) // 6, (px + 1) * (px + 2) // 2 * (pz + 1))
     if el_type is not None:
         tet = np.asarray(el_type) == TETRA
         ns = np.where(tet, (px + 1) * (px + 2) * (px + 3) // 6, ns)
@@ -364,7 +370,8 @@ def _dg_parts(arrays, pdeg, ind, coef_of, penalty_const, src_model, bval, swap=F
     nodes = coor[en]
     pd, ns, off = dg_degrees(pdeg, nel, et)
     tet = np.zeros(nel, bool) if et is None else np.asarray(et) == TETRA
-    pmin = np.where(tet, pd % 100, np.minimum(pd % 100, pd // 100))
+    comp = pd > DG_COMPLETE
+    pmin = np.where(tet, pd % 100, np.where(comp, pd - DG_COMPLETE, np.minimum(pd % 100, pd // 100)))
     # tets pass their complete degree p, prisms their pdeg
     pk = np.where(tet, pd % 100, pd)
     for e in range(nel):

@@ -21,9 +21,23 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 from oracle_lib import P, dg_num_int_el, dg_quadr, dg_shape, ref, ref_available  # noqa: E402
 
 PDEGS = [101, 102, 103, 201, 202, 203, 301, 302, 303]
+# R's pz = 0 tensor degrees (pdeg 1, 2) and the complete base of degree
+# 1..3 (the oracle's code DG_COMPLETE + p)
+EXTRA = [1, 2, 1001, 1002, 1003]
 REF_NODES = np.array([[0, 0, -1], [1, 0, -1], [0, 1, -1], [0, 0, 1], [1, 0, 1], [0, 1, 1]], float)
 need_dg = pytest.mark.skipif(not ref_available("libref_dg.so"), reason="oracle/_ref not built")
 TENSOR = 1  # APC_BASE_TENSOR_DG, include/aph_intf.h:122
+COMPLETE = 2  # APC_BASE_COMPLETE_DG, include/aph_intf.h:123
+
+
+def rbase(pdeg):
+    """(base type, pdeg) of a degree code as the reference takes them"""
+    return (COMPLETE, pdeg - 1000) if pdeg > 1000 else (TENSOR, pdeg)
+
+
+def rule_degrees(pdeg):
+    """(px, pz) of the tensor rule a degree code integrates with"""
+    return (pdeg - 1000, pdeg - 1000) if pdeg > 1000 else (pdeg % 100, pdeg // 100)
 
 # slots of the coefficient-presence vector and of the coefficient vector
 IND_M, IND_AXX, IND_AZZ, IND_BX, IND_BZ, IND_C, IND_QX, IND_QZ, IND_S = 1, 2, 10, 11, 13, 17, 19, 21, 22
@@ -47,7 +61,7 @@ def ref_dg():
 def ref_shape(L, pdeg, eta):
     out = [np.zeros(200) for _ in range(4)]
     e = np.ascontiguousarray(eta, float)
-    ns = L.apr_shape_fun_3D(TENSOR, pdeg, P(e), *[P(a) for a in out])
+    ns = L.apr_shape_fun_3D(*rbase(pdeg), P(e), *[P(a) for a in out])
     return [a[:ns] for a in out]
 
 
@@ -78,7 +92,7 @@ def mono_int(a, b, c):
 
 
 @need_dg
-@pytest.mark.parametrize("pdeg", PDEGS)
+@pytest.mark.parametrize("pdeg", PDEGS + EXTRA)
 def test_shape_functions_match_reference(pdeg):
     L = ref_dg()
     rng = np.random.default_rng(pdeg)
@@ -88,18 +102,40 @@ def test_shape_functions_match_reference(pdeg):
         assert len(mine[0]) == len(theirs[0])
         for m, t in zip(mine, theirs):
             assert np.array_equal(m, t)
-    px, pz = pdeg % 100, pdeg // 100
-    assert len(mine[0]) == (px + 1) * (px + 2) // 2 * (pz + 1)
+    from oracle_lib import oracle
+    assert len(mine[0]) == oracle().orc_dg_ns(pdeg)
+    if pdeg > 1000:
+        p = pdeg - 1000
+        assert len(mine[0]) == (p + 1) * (p + 2) * (p + 3) // 6
+    else:
+        px, pz = pdeg % 100, pdeg // 100
+        assert len(mine[0]) == (px + 1) * (px + 2) // 2 * (pz + 1)
+
+
+@pytest.mark.parametrize("pdeg", EXTRA)
+def test_extra_bases_are_the_expected_monomials(pdeg):
+    """pz = 0: the 2D monomials of degree <= px, constant in z; complete:
+    every monomial of total degree <= p, each once"""
+    ex = exponents(pdeg)
+    assert len(set(ex)) == len(ex)
+    if pdeg > 1000:
+        p = pdeg - 1000
+        want = {(a, b, c) for a in range(p + 1) for b in range(p + 1) for c in range(p + 1)
+                if a + b + c <= p}
+    else:
+        want = {(a, b, 0) for a in range(pdeg + 1) for b in range(pdeg + 1) if a + b <= pdeg}
+    assert set(ex) == want
 
 
 @need_dg
-@pytest.mark.parametrize("pdeg", PDEGS)
+@pytest.mark.parametrize("pdeg", PDEGS + EXTRA)
 def test_quadrature_matches_reference(pdeg):
     L = ref_dg()
     xg, wg = np.zeros(3000), np.zeros(1000)
     ng = ctypes.c_int(0)
-    pv = np.array([pdeg], np.int32)
-    L.apr_set_quadr_3D(TENSOR, P(pv), ctypes.byref(ng), P(xg), P(wg))
+    base, rp = rbase(pdeg)
+    pv = np.array([rp], np.int32)
+    L.apr_set_quadr_3D(base, P(pv), ctypes.byref(ng), P(xg), P(wg))
     x, w = dg_quadr(pdeg)
     assert len(w) == ng.value
     # 15-digit literals in the reference, Newton-computed Gauss-Legendre here
@@ -107,13 +143,13 @@ def test_quadrature_matches_reference(pdeg):
     assert np.abs(w - wg[:ng.value]).max() <= 5e-15
 
 
-@pytest.mark.parametrize("pdeg", PDEGS)
+@pytest.mark.parametrize("pdeg", PDEGS + EXTRA)
 def test_quadrature_is_exact_to_its_degree(pdeg):
     x, w = dg_quadr(pdeg)
-    px, pz = pdeg % 100, pdeg // 100
+    px, pz = rule_degrees(pdeg)
     for a in range(2 * px + 1):
         for b in range(2 * px + 1 - a):
-            for c in range(2 * pz + 1):
+            for c in range(max(2 * pz, 1) + 1):
                 q = np.sum(w * x[:, 0] ** a * x[:, 1] ** b * x[:, 2] ** c)
                 assert abs(q - mono_int(a, b, c)) <= 1e-13, (a, b, c)
 
@@ -131,7 +167,7 @@ def all_terms(rng, full=True):
     return ind, coef
 
 
-@pytest.mark.parametrize("pdeg", PDEGS)
+@pytest.mark.parametrize("pdeg", PDEGS + EXTRA)
 @pytest.mark.parametrize("full", [True, False])
 def test_affine_prism_matches_closed_form(pdeg, full):
     rng = np.random.default_rng(pdeg + 7 * full)
@@ -192,7 +228,7 @@ def test_affine_prism_matches_closed_form(pdeg, full):
 
 
 @need_dg
-@pytest.mark.parametrize("pdeg", [101, 202, 303, 102])
+@pytest.mark.parametrize("pdeg", [101, 202, 303, 102, 2, 1003])
 def test_distorted_prism_geometry_matches_reference(pdeg):
     """on a prism whose quads are not planar the Jacobian varies; the
     oracle's matrix equals the one built from the reference's own
@@ -204,8 +240,9 @@ def test_distorted_prism_geometry_matches_reference(pdeg):
     K, F = dg_num_int_el(pdeg, nc, ind, coef)
     xg, wg = np.zeros(3000), np.zeros(1000)
     ng = ctypes.c_int(0)
-    pv = np.array([pdeg], np.int32)
-    L.apr_set_quadr_3D(TENSOR, P(pv), ctypes.byref(ng), P(xg), P(wg))
+    base, rp = rbase(pdeg)
+    pv = np.array([rp], np.int32)
+    L.apr_set_quadr_3D(base, P(pv), ctypes.byref(ng), P(xg), P(wg))
     ncc = np.ascontiguousarray(nc.ravel())
     ns = K.shape[0]
     Kr, Fr = np.zeros((ns, ns)), np.zeros(ns)
@@ -213,7 +250,7 @@ def test_distorted_prism_geometry_matches_reference(pdeg):
     for g in range(ng.value):
         phi, dx, dy, dz, xc = [np.zeros(200) for _ in range(4)] + [np.zeros(3)]
         eta = np.ascontiguousarray(xg[3 * g:3 * g + 3])
-        det = L.apr_elem_calc_3D(2, 1, P(pv), TENSOR, P(eta), P(ncc), None, P(phi), P(dx), P(dy),
+        det = L.apr_elem_calc_3D(2, 1, P(pv), base, P(eta), P(ncc), None, P(phi), P(dx), P(dy),
                                  P(dz), P(xc), None, None, None, None, None)
         vol = det * wg[g]
         grad = np.stack([dx[:ns], dy[:ns], dz[:ns]])  # grad[c, j]
@@ -291,7 +328,7 @@ def test_face_points_coincide_across_neighbours():
 
 
 @need_dg
-@pytest.mark.parametrize("pdeg", PDEGS)
+@pytest.mark.parametrize("pdeg", PDEGS + EXTRA)
 def test_face_quadrature_matches_reference(pdeg):
     L = ref_dg()
     L.apr_set_quadr_2D.restype = ctypes.c_int
@@ -300,7 +337,8 @@ def test_face_quadrature_matches_reference(pdeg):
     for fa_type, quad in ((3, 0), (4, 1)):  # MMC_TRIA, MMC_QUAD (include/mmh_intf.h:224)
         xg, wg = np.zeros(450), np.zeros(150)
         ng = ctypes.c_int(0)
-        L.apr_set_quadr_2D(fa_type, TENSOR, P(np.array([pdeg], np.int32)), ctypes.byref(ng),
+        base, rp = rbase(pdeg)
+        L.apr_set_quadr_2D(fa_type, base, P(np.array([rp], np.int32)), ctypes.byref(ng),
                            P(xg), P(wg))
         mx, mw = np.zeros(128), np.zeros(64)
         n = oracle().orc_dg_face_quadr(pdeg, quad, P(mx), P(mw))
@@ -469,3 +507,96 @@ def test_upwind_convection_energy_identity(pdeg):
         oracle().orc_dg_set_plant_downwind(0)
     assert abs(x @ Ad @ x - expect) > 0.1 * abs(expect)
     m.free()
+
+
+# ---------------------------------------------- complete base, pz = 0 degrees
+def index_map(sub, full):
+    """positions in base `full` of the functions of base `sub` (by exponents)"""
+    ef = exponents(full)
+    return np.array([ef.index(t) for t in exponents(sub)])
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_complete_system_is_the_tensor_system_on_its_functions(p):
+    """the complete base of degree p is a subset of the tensor base (p, p)
+    and takes the same volume and face rules and penalty degree, so its
+    assembled system (every element and face term, distorted mesh, Dirichlet
+    and Neumann faces) equals the tensor system's rows and columns of those
+    functions; a wrong rule, penalty or function would show here"""
+    import modfem_amd as mf
+    from oracle_lib import dg_system_dense, oracle
+    a = cube_arrays(2, jitter=0.15, seed=p)
+    nel = a["el_nodes"].shape[0]
+    rng = np.random.default_rng(p)
+    ind, _ = all_terms(rng, True)
+    ce = [all_terms(rng, True)[1] for _ in range(nel)]
+    Ac, bc = dg_system_dense(a, 1000 + p, ind, lambda e: ce[e], 1.5, 1)
+    At, bt = dg_system_dense(a, 101 * p, ind, lambda e: ce[e], 1.5, 1)
+    o = oracle()
+    nc, nt = o.orc_dg_ns(1000 + p), o.orc_dg_ns(101 * p)
+    m = index_map(1000 + p, 101 * p)
+    idx = (np.arange(nel)[:, None] * nt + m[None, :]).ravel()
+    assert len(bc) == nel * nc
+    sub = At[np.ix_(idx, idx)]
+    assert np.abs(Ac - sub).max() <= 1e-13 * np.abs(sub).max()
+    assert np.abs(bc - bt[idx]).max() <= 1e-13 * np.abs(bt).max()
+    assert o.orc_dg_penalty_degree(1000 + p, -1) == p
+    del mf
+
+
+@pytest.mark.parametrize("px", [1, 2])
+def test_flat_degree_system_is_the_tensor_system_on_its_functions(px):
+    """pz = 0 (pdeg 1, 2): on a mesh extruded straight up (every Jacobian
+    constant in z) the one-point rule in z integrates the z-constant products
+    exactly, so the system equals the tensor (px, 1) system's rows and columns
+    of the z-constant functions once both use the penalty degree R gives the
+    pz = 0 degrees, min(px, pz) = 0 (constant source and boundary data)"""
+    import modfem_amd as mf
+    from oracle_lib import dg_system_dense, oracle
+    a = mf.Mesh.box_prism(2, 2, 2, seed=px).arrays()
+    nel = a["el_nodes"].shape[0]
+    rng = np.random.default_rng(10 + px)
+    ind, _ = all_terms(rng, True)
+    ce = [all_terms(rng, True)[1] for _ in range(nel)]
+    o = oracle()
+    assert o.orc_dg_penalty_degree(px, -1) == 0 and o.orc_dg_penalty_degree(px, 101) == 0
+    # constant source and boundary data: the load integrands are z-constant too
+    Af, bf = dg_system_dense(a, px, ind, lambda e: ce[e], 1.5, 0, 0.7)
+    At, bt = dg_system_dense(a, 100 + px, ind, lambda e: ce[e], 0.0, 0, 0.7)
+    nf, nt = o.orc_dg_ns(px), o.orc_dg_ns(100 + px)
+    m = index_map(px, 100 + px)
+    idx = (np.arange(nel)[:, None] * nt + m[None, :]).ravel()
+    sub = At[np.ix_(idx, idx)]
+    assert len(bf) == nel * nf
+    assert np.abs(Af - sub).max() <= 1e-12 * np.abs(sub).max()
+    assert np.abs(bf - bt[idx]).max() <= 1e-12 * np.abs(bt).max()
+
+
+@pytest.mark.parametrize("pd", [1001, 1002, 1003, "mixed"])
+def test_complete_base_reproduces_a_linear_solution(pd):
+    """every complete degree holds the linear functions: the linear exact
+    solution is reproduced to rounding, uniform and with degrees mixed"""
+    import modfem_amd as mf
+    from oracle_lib import dg_degrees, dg_system_dense, oracle
+    a = mf.Mesh.box_prism(3, 2, 2, seed=4).arrays()
+    nel = a["el_nodes"].shape[0]
+    pdv = (np.random.default_rng(5).choice([1001, 1002, 1003], size=nel) if pd == "mixed"
+           else np.full(nel, pd))
+    cc = np.zeros(21)
+    cc[0] = cc[4] = cc[8] = 1.0
+    A, b = dg_system_dense(a, pdv, mf.IND_DIFF, lambda e: cc, 1.0, 9)
+    x = np.linalg.solve(A, b)
+    _, ns, off = dg_degrees(pdv, nel)
+    nodes = a["coor"][a["el_nodes"]]
+    o = oracle()
+    rng = np.random.default_rng(6)
+    worst = 0.0
+    for e in range(nel):
+        for _ in range(4):
+            u, v = rng.uniform(0, 0.5, 2)
+            eta = np.array([u, v, rng.uniform(-1, 1)])
+            xp = np.zeros(3)
+            uh = o.orc_dg_eval(int(pdv[e]), P(np.ascontiguousarray(nodes[e].ravel())),
+                               P(np.ascontiguousarray(x[off[e]:off[e + 1]])), P(eta), P(xp))
+            worst = max(worst, abs(uh - (0.3 + xp[0] - 2 * xp[1] + 0.5 * xp[2])))
+    assert worst <= 1e-11, worst
