@@ -437,6 +437,18 @@ int mf_dg_get_precon_block(int did, int e, double* inv);
    per application, elements updated colour by colour (a greedy colouring of
    the element graph; the reference sweeps the blocks in their order) */
 int mf_dg_set_gs(int did, int nsweeps, int* ncolors);
+/* p-multigrid V-cycle as precon 4 of mf_dg_gmres (the V-cycle of
+   lsr_mkb_core_vcycle, lsd_mkb_core/lss_mkb_core.c:1288-1470, over levels of
+   decreasing isotropic degree on the same mesh where the reference walks
+   mesh levels): for a uniform degree 202 or 303, levels of degree p-1 .. 1
+   with the Galerkin operators P^T A P (sub-blocks of the finer blocks: the
+   tensor bases are nested, P injects coefficients), nu1 / nu2 multicolour
+   block Gauss-Seidel sweeps before / after the coarse correction, `coarse`
+   block-ILU(0) corrections on degree 1.  Returns the number of levels */
+int mf_dg_set_multigrid(int did, int nu1, int nu2, int coarse);
+/* handle of level l (0: degree 1) below the problem's own, with its Galerkin
+   blocks current (read them with mf_dg_get_block_row) */
+int mf_dg_multigrid_level(int did, int l);
 /* block ILU(0) in element order (lar_fill_preconditioner for block storage,
    MULTI_ILU / BLOCK_ILU, las_block_intf.c:1409-2143): for every element i
    and its lower face neighbours k in ascending order a_ik <- a_ik inv(D_k),
@@ -456,9 +468,9 @@ int mf_dg_ilu_apply(int did, const double* b, double* v);
 /* GMRES (lsr_mkb_core_gmres, lss_mkb_core.c:2317) on the block system with
    precon 0 (none), 1 (block Jacobi, left), 2 (block Gauss-Seidel
    smoothing, the smoothing branch of compreres, lss_mkb_core.c:2001-2054)
-   or 3 (block ILU(0), the BLOCK_ILU branch: (LU)^-1 (b - A x), filled on
-   first use), 
-   zero initial guess, stopping on the preconditioned residual relative to its
+   3 (block ILU(0), the BLOCK_ILU branch: (LU)^-1 (b - A x), filled on
+   first use) or 4 (one p-multigrid V-cycle as smoothing, after
+   mf_dg_set_multigrid); zero initial guess, stopping on the preconditioned residual relative to its
    initial value; x may be null (the solution stays on the device,
    mf_dg_get_solution) */
 int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, double* x,

@@ -161,6 +161,8 @@ def _declare(L):
         "mf_dg_fill_precon": (I, [I]),
         "mf_dg_get_precon_block": (I, [I, I, V]),
         "mf_dg_set_gs": (I, [I, I, V]),
+        "mf_dg_set_multigrid": (I, [I, I, I, I]),
+        "mf_dg_multigrid_level": (I, [I, I]),
         "mf_dg_gmres": (I, [I, I, I, I, D, V, V, V, V]),
         "mf_dg_get_solution": (I, [I, V]),
         "mf_dg_bench": (I, [I, I, V]),
@@ -552,6 +554,19 @@ class DgProblem:
         inv = np.zeros((ns, ns))
         check(lib().mf_dg_get_precon_block(self.h, int(e), _p(inv)))
         return inv
+
+    def set_multigrid(self, nu1=2, nu2=2, coarse=2):
+        """p-multigrid V-cycle levels for gmres(precon=4); returns the level count"""
+        return check(lib().mf_dg_set_multigrid(self.h, int(nu1), int(nu2), int(coarse)))
+
+    def multigrid_level_row(self, level, e):
+        """(blocks[6, ns, ns], rhs) of element e on p-multigrid level `level`
+        (0: degree 1), its Galerkin blocks current"""
+        h = check(lib().mf_dg_multigrid_level(self.h, int(level)))
+        ns = (level + 2) * (level + 3) // 2 * (level + 2)
+        blk, rhs = np.zeros((6, ns, ns)), np.zeros(ns)
+        check(lib().mf_dg_get_block_row(h, int(e), _p(blk), _p(rhs)))
+        return blk, rhs
 
     def fill_ilu(self):
         """block ILU(0) factors on the device; returns the forward and

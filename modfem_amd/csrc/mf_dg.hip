@@ -460,6 +460,7 @@ __global__ void __launch_bounds__(dg_block_threads(PX, PZ)) k_dg_elem(DgElemArgs
 // ---------------------------------------------------------------------
 struct DgOp;
 
+struct DgMg;
 struct DgProblem {
   int mesh = 0, nel = 0, px = 1, pz = 1, ns = 0, ng = 0, nsp = 0;
   double* d_nc = nullptr;    // [18][nel]
@@ -542,6 +543,9 @@ struct DgProblem {
   Problem* bp = nullptr;
   bool bp_valid = false;
   bool sol_batch = false;     // the last solution came from a batch
+  // p-multigrid levels below this problem's degree (mf_dg_set_multigrid)
+  DgMg* mg = nullptr;
+  bool mg_valid = false;
 };
 
 namespace {
@@ -596,8 +600,10 @@ void free_dg_solver(DgProblem& D);
 int dg_ilu_solve(DgProblem& D, double* v, const int* stop);
 
 void free_dg_batch(DgProblem& D);
+void free_dg_mg(DgProblem& D);
 
 void free_dg(DgProblem& D) {
+  free_dg_mg(D);
   free_dg_solver(D);
   free_dg_batch(D);
   void* ptrs[] = {D.d_nc, D.d_qv, D.d_U, D.d_coef, D.d_stiff, D.d_rhs, D.d_fnb, D.d_fcode,
@@ -2331,8 +2337,11 @@ struct DgOp : ExtOp {
     color_ = -1;
     return 0;
   }
+  int run_mg(const double* x, const double* b, const double* sc, int j, const int* stop,
+             double* w, int mode);
   int run_all(const double* x, const double* b, const double* sc, int j, const int* stop,
               double* w, int mode) {
+    if (precon == 4) return run_mg(x, b, sc, j, stop, w, mode);
     if (precon == 3) {
       // BLOCK_ILU branch of compreres: w = (LU)^-1 (b - A x), or (LU)^-1 s_j A v
       if (op(x, b, sc, j, stop, w, mode)) return -2;
@@ -2707,6 +2716,166 @@ int dg_ilu_solve(DgProblem& D, double* v, const int* stop) {
   return 0;
 }
 
+}  // namespace
+
+// ---------------------------------------------------------------------
+// p-multigrid V-cycle (the V-cycle of lsr_mkb_core_vcycle,
+// lss_mkb_core.c:1288-1470, over levels of decreasing isotropic degree on
+// the same mesh instead of the reference's mesh levels): pre-smoothing by
+// block Gauss-Seidel sweeps, residual, restriction to the next lower
+// degree, the V-cycle there for the correction from zero, prolongation and
+// addition, post-smoothing; on degree 1 block-ILU(0) corrections
+// (lsr_mkb_core_solve_coarse's role).  The tensor bases are nested (the
+// functions of (p-1, p-1) are functions of (p, p)), so prolongation is the
+// injection of coefficients, restriction its transpose (the coarse test
+// functions' residual entries) and the coarse operators the Galerkin
+// products P^T A P: sub-blocks of the finer level's blocks.
+struct DgMg {
+  std::vector<int> ids;          // handles of levels 0 (degree 1) .. L-2
+  std::vector<DgProblem*> lev;   // levels 0 .. L-1 (L-1: the fine problem)
+  std::vector<DgOp> ops;         // residual / smoothing of every level
+  std::vector<int*> d_map;       // level l's functions in level l+1's base
+  std::vector<double*> b, e, t;  // per level: rhs, correction, residual
+  double* zero = nullptr;        // a zero rhs of the fine size
+  int nu1 = 2, nu2 = 2, coarse = 2;
+  ~DgMg() {
+    for (int* p : d_map) if (p) (void)hipFree(p);
+    for (auto* v : {&b, &e, &t})
+      for (double* p : *v) if (p) (void)hipFree(p);
+    if (zero) (void)hipFree(zero);
+  }
+};
+
+namespace {
+
+__global__ void k_mg_gather_blocks(long long cnt, int nc, int nf, const int* map, const double* bf,
+                                   double* bc) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  const long long nn = (long long)nc * nc, blk = k / nn;
+  const int r = (int)(k % nn), j = r / nc, i = r % nc;
+  bc[k] = bf[blk * nf * nf + (long long)map[j] * nf + map[i]];
+}
+__global__ void k_mg_restrict(long long cnt, int nc, int nf, const int* map, const double* rf,
+                              double* bc, double* ec, const int* stop) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt || (stop && *stop)) return;
+  bc[k] = rf[(k / nc) * nf + map[k % nc]];
+  ec[k] = 0.0;
+}
+__global__ void k_mg_prolong_add(long long cnt, int nc, int nf, const int* map, const double* ec,
+                                 double* yf, const int* stop) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt || (stop && *stop)) return;
+  yf[(k / nc) * nf + map[k % nc]] += ec[k];
+}
+__global__ void k_mg_add(long long n, const double* t, double* y, const int* stop) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || (stop && *stop)) return;
+  y[k] += t[k];
+}
+
+unsigned blocks_of(long long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// the Galerkin blocks, block inverses, colourings and coarse ILU factors of
+// every level below the fine one
+int mg_refresh(DgProblem& D) {
+  DgMg& M = *D.mg;
+  const int L = (int)M.lev.size();
+  for (int l = L - 2; l >= 0; l--) {
+    DgProblem& C = *M.lev[l];
+    const DgProblem& F = *M.lev[l + 1];
+    const long long cnt = (long long)C.nel * 6 * C.ns * C.ns;
+    if (!C.d_blk) MF_HIP(hipMalloc(&C.d_blk, cnt * sizeof(double)));
+    if (!C.d_brhs) MF_HIP(hipMalloc(&C.d_brhs, (size_t)C.ndof * sizeof(double)));
+    hipLaunchKernelGGL(k_mg_gather_blocks, dim3(blocks_of(cnt)), dim3(256), 0, stream(), cnt, C.ns,
+                       F.ns, M.d_map[l], F.d_blk, C.d_blk);
+    MF_HIP(hipGetLastError());
+    C.assembled = true;
+    C.dinv_valid = C.ilu_valid = false;
+  }
+  for (int l = 0; l < L; l++) {
+    DgProblem& C = *M.lev[l];
+    if (l == 0) {
+      if (!C.ilu_valid && fill_block_ilu(C)) return -2;
+    } else {
+      if (!C.dinv_valid && fill_block_jacobi(C)) return -2;
+      if (build_gs_colors(C)) return -2;
+    }
+  }
+  D.mg_valid = true;
+  return 0;
+}
+
+// one V-cycle from level l on y (in/out) with rhs b (null: zero)
+int mg_vcycle(DgMg& M, int l, const double* b, double* y, const int* stop) {
+  DgProblem& C = *M.lev[l];
+  DgOp& O = M.ops[l];
+  const long long n = C.ndof;
+  if (!b) b = M.zero;
+  if (l == 0) {
+    for (int k = 0; k < M.coarse; k++) {
+      if (O.op(y, b, nullptr, 0, stop, M.t[0], 1)) return -2;
+      if (dg_ilu_solve(C, M.t[0], stop)) return -2;
+      hipLaunchKernelGGL(k_mg_add, dim3(blocks_of(n)), dim3(256), 0, stream(), n, M.t[0], y, stop);
+    }
+    MF_HIP(hipGetLastError());
+    return 0;
+  }
+  C.gs_sweeps = M.nu1;
+  if (M.nu1 > 0 && O.sweeps(b, y, stop)) return -2;
+  if (O.op(y, b, nullptr, 0, stop, M.t[l], 1)) return -2;
+  DgProblem& Q = *M.lev[l - 1];
+  const long long cnt = (long long)Q.nel * Q.ns;
+  hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_of(cnt)), dim3(256), 0, stream(), cnt, Q.ns, C.ns,
+                     M.d_map[l - 1], M.t[l], M.b[l - 1], M.e[l - 1], stop);
+  MF_HIP(hipGetLastError());
+  if (mg_vcycle(M, l - 1, M.b[l - 1], M.e[l - 1], stop)) return -2;
+  hipLaunchKernelGGL(k_mg_prolong_add, dim3(blocks_of(cnt)), dim3(256), 0, stream(), cnt, Q.ns,
+                     C.ns, M.d_map[l - 1], M.e[l - 1], y, stop);
+  MF_HIP(hipGetLastError());
+  C.gs_sweeps = M.nu2;
+  if (M.nu2 > 0 && O.sweeps(b, y, stop)) return -2;
+  return 0;
+}
+
+int DgOp::run_mg(const double* x, const double* b, const double* sc, int j, const int* stop,
+                 double* w, int mode) {
+  DgMg& M = *D->mg;
+  const int L = (int)M.lev.size();
+  const long long n = D->ndof;
+  const int g = (int)((n + 255) / 256);
+  if (mode == 1) {
+    // r = MG(x; b) - x
+    MF_HIP(hipMemcpyAsync(w, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream()));
+    if (mg_vcycle(M, L - 1, b, w, stop)) return -2;
+    hipLaunchKernelGGL(k_dg_sub, dim3(g), dim3(256), 0, stream(), n, x, w, stop);
+    MF_HIP(hipGetLastError());
+    return 0;
+  }
+  // w = X - MG(X; 0), X = s_j v
+  hipLaunchKernelGGL(k_dg_scale2, dim3(g), dim3(256), 0, stream(), n, x, sc, j, w, D->d_work, stop);
+  if (mg_vcycle(M, L - 1, nullptr, D->d_work, stop)) return -2;
+  hipLaunchKernelGGL(k_dg_sub, dim3(g), dim3(256), 0, stream(), n, D->d_work, w, stop);
+  MF_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+void free_dg_mg(DgProblem& D) {
+  if (!D.mg) return;
+  for (int id : D.mg->ids) {
+    auto it = g_dg.find(id);
+    if (it == g_dg.end()) continue;
+    free_dg(*it->second);
+    g_dg.erase(it);
+  }
+  delete D.mg;
+  D.mg = nullptr;
+  D.mg_valid = false;
+}
 }  // namespace
 
 }  // namespace mf
@@ -3345,7 +3514,7 @@ int mf_dg_assemble(int did) {
   if (rc) return rc;
   MF_HIP(hipStreamSynchronize(stream()));
   D->assembled = true;
-  D->dinv_valid = D->ilu_valid = D->bp_valid = false;
+  D->dinv_valid = D->ilu_valid = D->bp_valid = D->mg_valid = false;
   return 0;
 }
 
@@ -3464,10 +3633,17 @@ int mf_dg_gmres(int did, int precon, int krylbas, int max_iter, double tol, doub
     set_error("mf_dg_gmres: the system is not assembled");
     return -1;
   }
-  if (precon < 0 || precon > 3) {
-    set_error("mf_dg_gmres: precon must be 0 (none), 1 (block Jacobi), 2 (block Gauss-Seidel) "
-              "or 3 (block ILU(0))");
+  if (precon < 0 || precon > 4) {
+    set_error("mf_dg_gmres: precon must be 0 (none), 1 (block Jacobi), 2 (block Gauss-Seidel), "
+              "3 (block ILU(0)) or 4 (p-multigrid V-cycle)");
     return -1;
+  }
+  if (precon == 4) {
+    if (!D->mg) {
+      set_error("mf_dg_gmres: precon 4 needs mf_dg_set_multigrid first");
+      return -1;
+    }
+    if (!D->mg_valid && mg_refresh(*D)) return -2;
   }
   if ((precon == 1 || precon == 2) && !D->dinv_valid && fill_block_jacobi(*D)) return -2;
   if (precon == 2 && build_gs_colors(*D)) return -2;
@@ -3501,6 +3677,94 @@ int mf_dg_set_gs(int did, int nsweeps, int* ncolors) {
   if (build_gs_colors(*D)) return -2;
   if (ncolors) *ncolors = (int)D->gs_color_ptr.size() - 1;
   return 0;
+}
+
+// p-multigrid V-cycle as the GMRES preconditioner (precon 4) of a uniform
+// isotropic tensor degree p >= 2: levels p-1 .. 1 on the same mesh, nu1 / nu2
+// block Gauss-Seidel sweeps before / after the coarse correction, `coarse`
+// block-ILU(0) corrections on degree 1.  Returns the number of levels
+int mf_dg_set_multigrid(int did, int nu1, int nu2, int coarse) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (D->mixed || D->px != D->pz || D->px < 2 || nu1 < 0 || nu2 < 0 || nu1 + nu2 < 1 ||
+      coarse < 1) {
+    set_error("mf_dg_set_multigrid: needs a uniform degree 202 or 303, nu1, nu2 >= 0 with "
+              "nu1 + nu2 >= 1, and coarse >= 1");
+    return -1;
+  }
+  if (!D->mg) {
+    auto M = std::make_unique<DgMg>();
+    const int p = D->px;
+    for (int q = 1; q < p; q++) {
+      const int id = dg_create(D->mesh, std::vector<int>(D->nel, 101 * q));
+      if (id < 0) {
+        for (int h : M->ids) {
+          auto it = g_dg.find(h);
+          if (it != g_dg.end()) {
+            free_dg(*it->second);
+            g_dg.erase(it);
+          }
+        }
+        return id;
+      }
+      M->ids.push_back(id);
+      M->lev.push_back(get_dg(id));
+    }
+    M->lev.push_back(D);
+    const int L = (int)M->lev.size();
+    M->ops.resize(L);
+    for (int l = 0; l < L; l++) {
+      M->ops[l].D = M->lev[l];
+      M->ops[l].precon = 0;
+    }
+    M->d_map.assign(L - 1, nullptr);
+    M->b.assign(L, nullptr);
+    M->e.assign(L, nullptr);
+    M->t.assign(L, nullptr);
+    D->mg = M.release();  // owned from here: freed with the problem
+    DgMg& G = *D->mg;
+    for (int l = 0; l + 1 < L; l++) {
+      int exc[40], exf[40];
+      shape_exponents(l + 1, l + 1, exc);
+      shape_exponents(l + 2, l + 2, exf);
+      const int nc = num_shape(l + 1, l + 1), nf = num_shape(l + 2, l + 2);
+      std::vector<int> map(nc, -1);
+      for (int i = 0; i < nc; i++)
+        for (int j = 0; j < nf; j++)
+          if (exf[j] == exc[i]) map[i] = j;
+      MF_HIP(hipMalloc(&G.d_map[l], nc * sizeof(int)));
+      MF_HIP(hipMemcpy(G.d_map[l], map.data(), nc * sizeof(int), hipMemcpyHostToDevice));
+    }
+    for (int l = 0; l < L; l++) {
+      const size_t bytes = std::max<long long>(G.lev[l]->ndof, 1) * sizeof(double);
+      MF_HIP(hipMalloc(&G.t[l], bytes));
+      if (l + 1 < L) {
+        MF_HIP(hipMalloc(&G.b[l], bytes));
+        MF_HIP(hipMalloc(&G.e[l], bytes));
+      }
+    }
+    MF_HIP(hipMalloc(&G.zero, std::max<long long>(D->ndof, 1) * sizeof(double)));
+    MF_HIP(hipMemset(G.zero, 0, std::max<long long>(D->ndof, 1) * sizeof(double)));
+  }
+  D->mg->nu1 = nu1;
+  D->mg->nu2 = nu2;
+  D->mg->coarse = coarse;
+  D->mg_valid = false;
+  return (int)D->mg->lev.size();
+}
+
+// handle of p-multigrid level l (0: degree 1) below the problem's own,
+// its Galerkin blocks current (inspection)
+int mf_dg_multigrid_level(int did, int l) {
+  DgProblem* D = get_dg(did);
+  if (!D) return -3;
+  if (!D->mg || l < 0 || l >= (int)D->mg->ids.size() || !D->assembled) {
+    set_error("mf_dg_multigrid_level: needs an assembled problem with multigrid levels and "
+              "0 <= l < levels - 1");
+    return -1;
+  }
+  if (!D->mg_valid && mg_refresh(*D)) return -2;
+  return D->mg->ids[l];
 }
 
 int mf_dg_get_solution(int did, double* x) {
