@@ -708,7 +708,14 @@ def bench_dg(args):
             precons = args.dg_precons or ((1, 3) if name == "mixed" else (2, 3))
             for precon in precons:
                 label = {1: "block Jacobi", 2: "block Gauss-Seidel, 2 multicolour sweeps",
-                         3: "block ILU(0), element order"}[precon]
+                         3: "block ILU(0), element order",
+                         4: "p-multigrid V-cycle (2+2 block GS sweeps, 2 ILU(0) on p=1)"}[precon]
+                if precon == 4:
+                    if name == "mixed" or int(name) not in (202, 303):
+                        continue
+                    t0 = time.perf_counter()
+                    rec["mg_levels"] = dg.set_multigrid(2, 2, 2)
+                    rec["mg_setup_s"] = time.perf_counter() - t0
                 if precon == 2:
                     rec["gs_colours"] = dg.set_gs(2)
                 if precon == 3:
@@ -1075,7 +1082,8 @@ def main():
                          "ALUs (library default: %d)" % DG_VALU_NS)
     ap.add_argument("--dg-precons", type=int, nargs="*", default=[],
                     help="DG solves: preconditioners to run (1 block Jacobi, 2 block GS, 3 block "
-                         "ILU(0)); default 1 3 for mixed degrees, 2 3 for uniform ones")
+                         "ILU(0), 4 p-multigrid V-cycle for 202 / 303); default 1 3 for mixed "
+                         "degrees, 2 3 for uniform ones")
     ap.add_argument("--dg-solve", nargs="*", default=[],
                     help="DG cases solved to 1e-8: mixed (block Jacobi, block ILU(0)), hybrid "
                          "(block ILU(0)) and/or the uniform degrees 101 202 303 (block "
