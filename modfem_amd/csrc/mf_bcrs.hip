@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "mf_device.h"
+#include "mf_solver_dev.h"
 
 namespace mf {
 
@@ -23,7 +24,9 @@ namespace {
 // all loads issued before the (sequential) subtractions
 __global__ void k_bcrs_residual(int nb, const int* rp, const int* ci, const double* val,
                                 const double* x, const double* b, double* v, const int* stop) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // contiguous block rows per XCD: neighbouring rows share their x blocks in
+  // that XCD's L2
+  const long long i = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= 4LL * nb || (stop && *stop)) return;
   const int r = (int)(i >> 2), k = (int)(i & 3);
   double acc = 0.0;
